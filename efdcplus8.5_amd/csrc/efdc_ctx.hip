@@ -1,0 +1,697 @@
+// efdc_ctx.hip -- C-ABI entry points: context lifetime, host<->device
+// copies in the var_global layout, open-boundary / source / constituent
+// tables.  The compute entry points are in efdc_puv.hip and efdc_tran.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <set>
+
+#include "efdc_internal.h"
+
+namespace {
+
+template <typename T>
+__global__ void k_fill(T *a, T v, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+template <typename D, typename S>
+__global__ void k_convert(D *dst, const S *src, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (D)src[i];
+}
+
+// first-call initialisation of CALPUV2C: RCX=RCY=1 on wet cells, 0 on the
+// ghosts, CC=1 (EFDC/calpuv2c.f90:88-108)
+__global__ void k_init_puv(double *rcx, double *rcy, double *cc, int lcm,
+                           int lc0) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= lcm) return;
+  const bool ghost = (i == 0 || i == lc0);
+  rcx[i] = ghost ? 0.0 : 1.0;
+  rcy[i] = ghost ? 0.0 : 1.0;
+  cc[i] = 1.0;
+}
+
+int blocks_for(int64_t n, int bs = EFDC_BLOCK) { return (int)((n + bs - 1) / bs); }
+
+template <typename T>
+int dmalloc(efdc_ctx *c, T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) return EFDC_OK;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    c->err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
+    *p = nullptr;
+    return EFDC_ERR_NOMEM;
+  }
+  e = hipMemsetAsync(*p, 0, n * sizeof(T), c->stream);
+  if (e != hipSuccess) {
+    c->err = hipGetErrorString(e);
+    return EFDC_ERR_HIP;
+  }
+  return EFDC_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+#define TRY(x)                 \
+  do {                         \
+    int s_ = (x);              \
+    if (s_ != EFDC_OK) return s_; \
+  } while (0)
+
+int upload(efdc_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return EFDC_OK;
+  HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return EFDC_OK;
+}
+
+template <typename T>
+int upload_new(efdc_ctx *c, T **dst, const T *src, size_t n) {
+  dfree(*dst);
+  if (n == 0) return EFDC_OK;
+  TRY(dmalloc(c, dst, n));
+  return upload(c, *dst, src, n * sizeof(T));
+}
+
+bool valid_id(int id) {
+  return (id >= 0 && id < EFDC_NUM_H) ||
+         (id >= EFDC_HPK && id < EFDC_NUM_K_END) ||
+         (id >= EFDC_W2 && id < EFDC_NUM_W_END);
+}
+
+}  // namespace
+
+int efdc_grid_blocks(int n) { return blocks_for(n); }
+
+extern "C" {
+
+int efdc_abi_version(void) { return EFDC_ABI_VERSION; }
+
+void efdc_params_default(efdc_params_t *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->G = 9.81;
+  p->RITB1 = 1.0;
+  p->SNLT = 0.0;
+  p->HDRY = 0.2;
+  p->TCON_TBEGIN = 0.0;
+  p->ITERM = 200;
+  p->RSQM = 1e-9;
+  p->cg_rel_tol = 0.0;
+  p->cg_mode = EFDC_CG_FP64;
+  p->store = EFDC_STORE_FP64;
+  p->mixed_inner_rtol = 1e-5;
+  p->mixed_max_outer = 30;
+}
+
+const char *efdc_last_error(const efdc_ctx_t *ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
+                    efdc_ctx_t **out) {
+  if (!out) return EFDC_ERR_ARG;
+  *out = nullptr;
+  if (!g || !prm) return EFDC_ERR_ARG;
+  if (g->LA < 2 || g->LCM < g->LA + 1 || g->KC < 1 || g->KCM < g->KC ||
+      g->KC > 64 || !g->LWC || !g->LEC || !g->LSC || !g->LNC)
+    return EFDC_ERR_ARG;
+  if (prm->store != EFDC_STORE_FP64 && prm->store != EFDC_STORE_FP32)
+    return EFDC_ERR_ARG;
+  if (prm->cg_mode != EFDC_CG_FP64 && prm->cg_mode != EFDC_CG_MIXED)
+    return EFDC_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return EFDC_ERR_NODEVICE;
+  efdc_ctx *c = new efdc_ctx();
+  (void)hipGetDevice(&c->device);
+  c->LA = g->LA;
+  c->LCM = g->LCM;
+  c->KC = g->KC;
+  c->KCM = g->KCM;
+  c->LC = g->LA + 1;
+  c->pitch = ((int64_t)g->LCM + 63) / 64 * 64;
+  c->prm = *prm;
+  c->store = prm->store;
+  c->tsz = prm->store == EFDC_STORE_FP32 ? 4 : 8;
+  auto fail = [&](int s) {
+    efdc_ctx_destroy(c);
+    return s;
+  };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(EFDC_ERR_HIP);
+
+  // maps: validate on the host, store 0-based
+  const int LA = g->LA, LC = c->LC, LCM = g->LCM;
+  std::vector<int> lw(LCM), le(LCM), ls(LCM), ln(LCM), lnw(LCM), lse(LCM);
+  bool rowmajor = true;
+  for (int L = 1; L <= LCM; ++L) {
+    const int i = L - 1;
+    auto chk = [&](int v) { return v >= 1 && v <= LCM; };
+    int a = g->LWC[i], b = g->LEC[i], s = g->LSC[i], n = g->LNC[i];
+    if (L >= 2 && L <= LA) {
+      if (!chk(a) || !chk(b) || !chk(s) || !chk(n)) {
+        c->err = "neighbour map value out of range at L=" + std::to_string(L);
+        return fail(EFDC_ERR_ARG);
+      }
+      if (!(a == L - 1 || a == LC) || !(b == L + 1 || b == LC)) rowmajor = false;
+      if (a != LC && a >= 2 && a <= LA && g->LEC[a - 1] != L) rowmajor = false;
+      if (s != LC && s >= 2 && s <= LA && g->LNC[s - 1] != L) rowmajor = false;
+    }
+    lw[i] = chk(a) ? a - 1 : LC - 1;
+    le[i] = chk(b) ? b - 1 : LC - 1;
+    ls[i] = chk(s) ? s - 1 : LC - 1;
+    ln[i] = chk(n) ? n - 1 : LC - 1;
+    lnw[i] = (g->LNWC && chk(g->LNWC[i])) ? g->LNWC[i] - 1 : LC - 1;
+    lse[i] = (g->LSEC && chk(g->LSEC[i])) ? g->LSEC[i] - 1 : LC - 1;
+  }
+  c->rowmajor = rowmajor;
+  int s;
+  if ((s = upload_new(c, &c->dLW, lw.data(), LCM)) ||
+      (s = upload_new(c, &c->dLE, le.data(), LCM)) ||
+      (s = upload_new(c, &c->dLS, ls.data(), LCM)) ||
+      (s = upload_new(c, &c->dLN, ln.data(), LCM)) ||
+      (s = upload_new(c, &c->dLNW, lnw.data(), LCM)) ||
+      (s = upload_new(c, &c->dLSE, lse.data(), LCM)))
+    return fail(s);
+
+  for (int id = 0; id < EFDC_NUM_H; ++id)
+    if ((s = dmalloc(c, &c->dH[id], (size_t)LCM))) return fail(s);
+  const size_t kel = (size_t)c->pitch * (size_t)c->KC;
+  const size_t wel = (size_t)c->pitch * (size_t)(c->KC + 1);
+  for (int j = 0; j < EFDC_NK; ++j) {
+    char *p = nullptr;
+    if ((s = dmalloc(c, &p, kel * c->tsz))) return fail(s);
+    c->dK[j] = p;
+  }
+  for (int j = 0; j < EFDC_NW; ++j) {
+    char *p = nullptr;
+    if ((s = dmalloc(c, &p, wel * c->tsz))) return fail(s);
+    c->dW[j] = p;
+  }
+  c->staging_elems = wel;
+  if ((s = dmalloc(c, &c->staging, wel))) return fail(s);
+
+  // CG scratch
+  c->max_blocks = blocks_for(LCM) + 8;
+  if ((s = dmalloc(c, &c->r, LCM)) || (s = dmalloc(c, &c->pb[0], LCM)) ||
+      (s = dmalloc(c, &c->pb[1], LCM)) || (s = dmalloc(c, &c->ap, LCM)) ||
+      (s = dmalloc(c, &c->part, (size_t)8 * c->max_blocks)) ||
+      (s = dmalloc(c, &c->st, 1)))
+    return fail(s);
+  if (prm->cg_mode == EFDC_CG_MIXED) {
+    if ((s = dmalloc(c, &c->dis, LCM)) || (s = dmalloc(c, &c->as32, LCM)) ||
+        (s = dmalloc(c, &c->aw32, LCM)) || (s = dmalloc(c, &c->r32, LCM)) ||
+        (s = dmalloc(c, &c->y32, LCM)) || (s = dmalloc(c, &c->p32[0], LCM)) ||
+        (s = dmalloc(c, &c->p32[1], LCM)) || (s = dmalloc(c, &c->ap32, LCM)))
+      return fail(s);
+  }
+  if (hipHostMalloc((void **)&c->st_host, sizeof(CgState)) != hipSuccess)
+    return fail(EFDC_ERR_NOMEM);
+  std::memset(c->st_host, 0, sizeof(CgState));
+  if ((s = dmalloc(c, &c->lopenbcdry, LCM)) ||
+      (s = dmalloc(c, &c->pbcflag, LCM)) || (s = dmalloc(c, &c->bcrec, LCM)) ||
+      (s = dmalloc(c, &c->srcrec, LCM)) || (s = dmalloc(c, &c->cdzkk_k, 64)) ||
+      (s = dmalloc(c, &c->cdzkmk_k, 64)) || (s = dmalloc(c, &c->scal, 64)) ||
+      (s = dmalloc(c, &c->nman, LCM)))
+    return fail(s);
+  const int nb = blocks_for(LCM);
+  k_fill<int><<<nb, EFDC_BLOCK, 0, c->stream>>>(c->bcrec, -1, LCM);
+  k_fill<int><<<nb, EFDC_BLOCK, 0, c->stream>>>(c->srcrec, -1, LCM);
+  k_init_puv<<<nb, EFDC_BLOCK, 0, c->stream>>>(c->dH[EFDC_RCX], c->dH[EFDC_RCY],
+                                               c->dH[EFDC_CC], LCM, LC - 1);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(EFDC_ERR_HIP);
+  *out = c;
+  return EFDC_OK;
+}
+
+int efdc_ctx_destroy(efdc_ctx_t *c) {
+  if (!c) return EFDC_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  dfree(c->dLW); dfree(c->dLE); dfree(c->dLS); dfree(c->dLN);
+  dfree(c->dLNW); dfree(c->dLSE);
+  for (auto &p : c->dH) dfree(p);
+  for (auto &p : c->dK) dfree(p);
+  for (auto &p : c->dW) dfree(p);
+  dfree(c->staging);
+  dfree(c->r); dfree(c->pb[0]); dfree(c->pb[1]); dfree(c->ap);
+  dfree(c->dis); dfree(c->as32); dfree(c->aw32); dfree(c->r32); dfree(c->y32);
+  dfree(c->p32[0]); dfree(c->p32[1]); dfree(c->ap32);
+  dfree(c->part); dfree(c->st);
+  if (c->st_host) (void)hipHostFree(c->st_host);
+  if (c->tide_host) (void)hipHostFree(c->tide_host);
+  for (auto &b : c->pbc) {
+    dfree(b.L); dfree(b.ispb); dfree(b.pser); dfree(b.pcb); dfree(b.psb);
+  }
+  delete[] c->tcp_host;
+  dfree(c->tide_dev); dfree(c->lopenbcdry); dfree(c->pbcflag);
+  dfree(c->lobcs); dfree(c->lobcs2);
+  for (auto &b : c->cbc) {
+    dfree(b.L); dfree(b.cb); dfree(b.csert); dfree(b.ntscr); dfree(b.clo);
+    dfree(b.nlo);
+  }
+  dfree(c->wtci); dfree(c->bcrec); dfree(c->bcinfo); dfree(c->bc_con1);
+  dfree(c->lqs); dfree(c->srcrec); dfree(c->srcnext); dfree(c->qss);
+  dfree(c->qsercell); dfree(c->cqs); dfree(c->csertq);
+  for (int j = 0; j < EFDC_MAXC; ++j) {
+    char *a = (char *)c->con[j].buf[0], *b = (char *)c->con[j].buf[1];
+    dfree(a); dfree(b);
+  }
+  { char *t = (char *)c->tran_tmp; dfree(t); }
+  for (auto &p : c->adac) { char *q = (char *)p; dfree(q); p = nullptr; }
+  dfree(c->cdzkk_k); dfree(c->cdzkmk_k); dfree(c->scal); dfree(c->nman);
+  dfree(c->bccell);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return EFDC_OK;
+}
+
+int efdc_set_params(efdc_ctx_t *c, const efdc_params_t *p) {
+  if (!c || !p) return EFDC_ERR_ARG;
+  if (p->store != c->store) {
+    c->err = "the storage precision is fixed at context creation";
+    return EFDC_ERR_ARG;
+  }
+  if (p->cg_mode == EFDC_CG_MIXED && !c->dis) {
+    c->err = "EFDC_CG_MIXED needs a context created in that mode";
+    return EFDC_ERR_ARG;
+  }
+  c->prm = *p;
+  return EFDC_OK;
+}
+
+int efdc_set_array(efdc_ctx_t *c, int32_t id, const double *host) {
+  if (!c || !host || !valid_id(id)) return EFDC_ERR_ARG;
+  if (id < EFDC_NUM_H) {
+    HIP_TRY(c, hipMemcpyAsync(c->dH[id], host, sizeof(double) * c->LCM,
+                              hipMemcpyHostToDevice, c->stream));
+  } else {
+    const bool isw = id >= EFDC_W2;
+    const int planes = isw ? c->KC + 1 : c->KC;
+    void *dst = isw ? c->dW[id - EFDC_W2] : c->dK[id - EFDC_HPK];
+    double *d64 = c->store == EFDC_STORE_FP64 ? (double *)dst : c->staging;
+    HIP_TRY(c, hipMemcpy2DAsync(d64, sizeof(double) * c->pitch, host,
+                                sizeof(double) * c->LCM, sizeof(double) * c->LCM,
+                                planes, hipMemcpyHostToDevice, c->stream));
+    if (c->store == EFDC_STORE_FP32) {
+      const int64_t n = c->pitch * planes;
+      k_convert<float, double><<<blocks_for(n), EFDC_BLOCK, 0, c->stream>>>(
+          (float *)dst, c->staging, n);
+    }
+    if (id == EFDC_CDZKK || id == EFDC_CDZKMK) {
+      // sigma grids carry the same layer constants in every column; keep
+      // one FP64 value per K then (EFDC/aaefdc.f90:1226-1232)
+      bool uni = true;
+      std::vector<double> perk(64, 0.0);
+      for (int k = 0; k < c->KC && uni; ++k) {
+        const double *row = host + (size_t)k * c->LCM;
+        perk[k] = row[1];
+        for (int L = 2; L <= c->LA; ++L)
+          if (row[L - 1] != perk[k]) { uni = false; break; }
+      }
+      if (uni)
+        HIP_TRY(c, hipMemcpyAsync(id == EFDC_CDZKK ? c->cdzkk_k : c->cdzkmk_k,
+                                  perk.data(), sizeof(double) * 64,
+                                  hipMemcpyHostToDevice, c->stream));
+      (id == EFDC_CDZKK ? c->dz_uni_kk : c->dz_uni_kmk) = uni;
+    }
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
+int efdc_get_array(efdc_ctx_t *c, int32_t id, double *host) {
+  if (!c || !host || !valid_id(id)) return EFDC_ERR_ARG;
+  if (id < EFDC_NUM_H) {
+    HIP_TRY(c, hipMemcpyAsync(host, c->dH[id], sizeof(double) * c->LCM,
+                              hipMemcpyDeviceToHost, c->stream));
+  } else {
+    const bool isw = id >= EFDC_W2;
+    const int planes = isw ? c->KC + 1 : c->KC;
+    void *src = isw ? c->dW[id - EFDC_W2] : c->dK[id - EFDC_HPK];
+    const double *s64 = (const double *)src;
+    if (c->store == EFDC_STORE_FP32) {
+      const int64_t n = c->pitch * planes;
+      k_convert<double, float><<<blocks_for(n), EFDC_BLOCK, 0, c->stream>>>(
+          c->staging, (const float *)src, n);
+      s64 = c->staging;
+    }
+    HIP_TRY(c, hipMemcpy2DAsync(host, sizeof(double) * c->LCM, s64,
+                                sizeof(double) * c->pitch, sizeof(double) * c->LCM,
+                                planes, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
+int efdc_device_ptr(efdc_ctx_t *c, int32_t id, void **ptr, int64_t *pitch) {
+  if (!c || !ptr || !valid_id(id)) return EFDC_ERR_ARG;
+  if (id < EFDC_NUM_H) *ptr = c->dH[id];
+  else if (id >= EFDC_W2) *ptr = c->dW[id - EFDC_W2];
+  else *ptr = c->dK[id - EFDC_HPK];
+  if (pitch) *pitch = c->pitch;
+  return EFDC_OK;
+}
+
+int efdc_set_open_bc(efdc_ctx_t *c, const efdc_open_bc_t *bc) {
+  if (!c || !bc || bc->mtide < 0 || (bc->mtide > 0 && !bc->tcp))
+    return EFDC_ERR_ARG;
+  const int LA = c->LA, LC = c->LC;
+  std::vector<unsigned char> flag(c->LCM, 0);
+  std::vector<int> lob, lob2;
+  // LOBCS / LOBCS2 in the order S, W, E, N (EFDC/setbcs.f90:884-938)
+  static const int order[4] = {2, 0, 1, 3};
+  for (int side = 0; side < 4; ++side) {
+    const efdc_pbc_side_t &s = bc->side[side];
+    if (s.n < 0 || (s.n > 0 && (!s.L || !s.ispb || !s.pser ||
+                                (bc->mtide > 0 && (!s.pcb || !s.psb)))))
+      return EFDC_ERR_ARG;
+    for (int LL = 0; LL < s.n; ++LL)
+      if (s.L[LL] < 2 || s.L[LL] > LA) {
+        c->err = "open-boundary cell out of range";
+        return EFDC_ERR_ARG;
+      }
+  }
+  c->npbc_total = 0;
+  for (int side = 0; side < 4; ++side) {
+    const efdc_pbc_side_t &s = bc->side[side];
+    PbcDev &d = c->pbc[side];
+    std::vector<int> L0(s.n);
+    std::set<int> own, nbs;
+    bool serial = false;
+    std::vector<int> nbv;
+    for (int LL = 0; LL < s.n; ++LL) {
+      L0[LL] = s.L[LL] - 1;
+      own.insert(L0[LL]);
+      flag[L0[LL]] = 1;
+    }
+    std::vector<int> mapv(c->LCM);
+    int *dmap = side == 0 ? c->dLE : side == 1 ? c->dLW : side == 2 ? c->dLN : c->dLS;
+    HIP_TRY(c, hipMemcpy(mapv.data(), dmap, sizeof(int) * c->LCM,
+                         hipMemcpyDeviceToHost));
+    for (int LL = 0; LL < s.n; ++LL) {
+      const int nb = mapv[L0[LL]];
+      if (nb != LC - 1 && (own.count(nb) || !nbs.insert(nb).second)) serial = true;
+    }
+    d.n = s.n;
+    d.serial = serial ? 1 : 0;
+    int st;
+    std::vector<double> pc((size_t)s.n * std::max(bc->mtide, 1), 0.0),
+        ps((size_t)s.n * std::max(bc->mtide, 1), 0.0);
+    if (bc->mtide > 0 && s.n > 0) {
+      std::memcpy(pc.data(), s.pcb, sizeof(double) * s.n * bc->mtide);
+      std::memcpy(ps.data(), s.psb, sizeof(double) * s.n * bc->mtide);
+    }
+    if ((st = upload_new(c, &d.L, L0.data(), s.n)) ||
+        (st = upload_new(c, &d.ispb, s.ispb, s.n)) ||
+        (st = upload_new(c, &d.pser, s.pser, s.n)) ||
+        (st = upload_new(c, &d.pcb, pc.data(), pc.size())) ||
+        (st = upload_new(c, &d.psb, ps.data(), ps.size())))
+      return st;
+    c->npbc_total += s.n;
+  }
+  for (int q = 0; q < 4; ++q) {
+    const efdc_pbc_side_t &s = bc->side[order[q]];
+    for (int LL = 0; LL < s.n; ++LL) {
+      const int L = s.L[LL];
+      lob.push_back(L - 1);
+      if (s.ispb[LL] <= 1) {
+        const int side = order[q];
+        std::vector<int> one(1);
+        int *dmap = side == 0 ? c->dLE : side == 1 ? c->dLW : side == 2 ? c->dLN : c->dLS;
+        HIP_TRY(c, hipMemcpy(one.data(), dmap + (L - 1), sizeof(int),
+                             hipMemcpyDeviceToHost));
+        lob2.push_back(one[0]);
+      }
+    }
+  }
+  int st;
+  c->nlobcs = (int)lob.size();
+  c->nlobcs2 = (int)lob2.size();
+  if ((st = upload_new(c, &c->lobcs, lob.data(), lob.size())) ||
+      (st = upload_new(c, &c->lobcs2, lob2.data(), lob2.size())))
+    return st;
+  HIP_TRY(c, hipMemcpyAsync(c->pbcflag, flag.data(), c->LCM,
+                            hipMemcpyHostToDevice, c->stream));
+  c->mtide = bc->mtide;
+  delete[] c->tcp_host;
+  c->tcp_host = new double[std::max(bc->mtide, 1)];
+  for (int m = 0; m < bc->mtide; ++m) c->tcp_host[m] = bc->tcp[m];
+  dfree(c->tide_dev);
+  if (c->tide_host) (void)hipHostFree(c->tide_host);
+  c->tide_host = nullptr;
+  const int nt = 2 * std::max(bc->mtide, 1);
+  if ((st = dmalloc(c, &c->tide_dev, nt))) return st;
+  HIP_TRY(c, hipHostMalloc((void **)&c->tide_host, sizeof(double) * nt));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
+int efdc_add_constituent(efdc_ctx_t *c, const efdc_constituent_t *d,
+                         int32_t *slot) {
+  if (!c || !d || !slot) return EFDC_ERR_ARG;
+  if (c->nconst >= EFDC_MAXC) {
+    c->err = "too many constituents";
+    return EFDC_ERR_ARG;
+  }
+  if (d->isadac > 0 && !c->rowmajor) {
+    // the anti-diffusion stencil uses the LNWC/LSEC diagonals; they are
+    // always uploaded (LC when absent), nothing else is required.
+  }
+  ConstDev &k = c->con[c->nconst];
+  k.desc = *d;
+  k.cur = 0;
+  const size_t n = (size_t)c->pitch * c->KC * c->tsz;
+  for (int j = 0; j < 2; ++j) {
+    char *p = nullptr;
+    int s = dmalloc(c, &p, n);
+    if (s) return s;
+    k.buf[j] = p;
+  }
+  *slot = c->nconst++;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
+static int put_layers(efdc_ctx *c, void *dst, const double *host) {
+  double *d64 = c->store == EFDC_STORE_FP64 ? (double *)dst : c->staging;
+  HIP_TRY(c, hipMemcpy2DAsync(d64, sizeof(double) * c->pitch, host,
+                              sizeof(double) * c->LCM, sizeof(double) * c->LCM,
+                              c->KC, hipMemcpyHostToDevice, c->stream));
+  if (c->store == EFDC_STORE_FP32) {
+    const int64_t n = c->pitch * c->KC;
+    k_convert<float, double><<<blocks_for(n), EFDC_BLOCK, 0, c->stream>>>(
+        (float *)dst, c->staging, n);
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
+static int get_layers(efdc_ctx *c, const void *src, double *host) {
+  const double *s64 = (const double *)src;
+  if (c->store == EFDC_STORE_FP32) {
+    const int64_t n = c->pitch * c->KC;
+    k_convert<double, float><<<blocks_for(n), EFDC_BLOCK, 0, c->stream>>>(
+        c->staging, (const float *)src, n);
+    s64 = c->staging;
+  }
+  HIP_TRY(c, hipMemcpy2DAsync(host, sizeof(double) * c->LCM, s64,
+                              sizeof(double) * c->pitch, sizeof(double) * c->LCM,
+                              c->KC, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
+int efdc_set_constituent(efdc_ctx_t *c, int32_t slot, const double *con,
+                         const double *con1) {
+  if (!c || !con || slot < 0 || slot >= c->nconst) return EFDC_ERR_ARG;
+  ConstDev &k = c->con[slot];
+  int s = put_layers(c, k.buf[k.cur], con);
+  if (s) return s;
+  if (!con1) return put_layers(c, k.buf[k.cur ^ 1], con);
+  // CALTRAN starts with CON1=CON over the whole array, ghosts included
+  // (EFDC/caltran.f90:59-65): keep the ghost entries of both buffers equal.
+  std::vector<double> tmp((size_t)c->LCM * c->KC);
+  for (int kk = 0; kk < c->KC; ++kk) {
+    const double *a = con + (size_t)kk * c->LCM;
+    const double *b = con1 + (size_t)kk * c->LCM;
+    double *t = tmp.data() + (size_t)kk * c->LCM;
+    std::memcpy(t, b, sizeof(double) * c->LCM);
+    t[0] = a[0];
+    for (int i = c->LA; i < c->LCM; ++i) t[i] = a[i];
+  }
+  return put_layers(c, k.buf[k.cur ^ 1], tmp.data());
+}
+
+int efdc_get_constituent(efdc_ctx_t *c, int32_t slot, double *con,
+                         double *con1) {
+  if (!c || slot < 0 || slot >= c->nconst) return EFDC_ERR_ARG;
+  ConstDev &k = c->con[slot];
+  if (con) {
+    int s = get_layers(c, k.buf[k.cur], con);
+    if (s) return s;
+  }
+  if (con1) return get_layers(c, k.buf[k.cur ^ 1], con1);
+  return EFDC_OK;
+}
+
+int efdc_set_conc_bc(efdc_ctx_t *c, const efdc_conc_bc_t *bc) {
+  if (!c || !bc) return EFDC_ERR_ARG;
+  const int nc = c->nconst, KC = c->KC, LA = c->LA;
+  if (nc < 1) {
+    c->err = "register the constituents before their boundary tables";
+    return EFDC_ERR_STATE;
+  }
+  std::vector<int> rec(c->LCM, -1);
+  std::vector<int4> info;
+  int total = 0;
+  for (int side = 0; side < 4; ++side) {
+    const efdc_cbc_side_t &s = bc->side[side];
+    if (s.n < 0 || (s.n > 0 && (!s.L || !s.cb || !s.csert || !s.ntscr)))
+      return EFDC_ERR_ARG;
+    total += s.n;
+  }
+  if (total > 0 && !bc->wtci) return EFDC_ERR_ARG;
+  for (int side = 0; side < 4; ++side) {
+    const efdc_cbc_side_t &s = bc->side[side];
+    CbcDev &d = c->cbc[side];
+    const bool same = (d.n == s.n);
+    std::vector<int> L0(s.n);
+    for (int LL = 0; LL < s.n; ++LL) {
+      if (s.L[LL] < 2 || s.L[LL] > LA) return EFDC_ERR_ARG;
+      L0[LL] = s.L[LL] - 1;
+      int &r = rec[L0[LL]];
+      if (r < 0) {
+        r = (int)info.size();
+        info.push_back(make_int4(-1, -1, -1, -1));
+      }
+      int4 &q = info[r];
+      if (side == 0) q.x = LL;
+      else if (side == 1) q.y = LL;
+      else if (side == 2) q.z = LL;
+      else q.w = LL;
+    }
+    int st;
+    if ((st = upload_new(c, &d.L, L0.data(), s.n)) ||
+        (st = upload_new(c, &d.cb, s.cb, (size_t)s.n * 2 * nc)) ||
+        (st = upload_new(c, &d.csert, s.csert, (size_t)s.n * KC * nc)) ||
+        (st = upload_new(c, &d.ntscr, s.ntscr, s.n)))
+      return st;
+    if (!same || !d.clo) {  // recovery memory survives table refreshes
+      dfree(d.clo);
+      dfree(d.nlo);
+      if ((st = dmalloc(c, &d.clo, (size_t)s.n * KC * nc)) ||
+          (st = dmalloc(c, &d.nlo, (size_t)s.n * KC * nc)))
+        return st;
+    }
+    d.n = s.n;
+  }
+  int st;
+  c->nbcrec = (int)info.size();
+  c->cbc_nc = nc;
+  if ((st = upload_new(c, &c->wtci, bc->wtci, (size_t)KC * 2)) ||
+      (st = upload_new(c, &c->bcinfo, info.data(), info.size())))
+    return st;
+  dfree(c->bc_con1);
+  if (c->nbcrec > 0 &&
+      (st = dmalloc(c, &c->bc_con1, (size_t)c->nbcrec * KC * nc)))
+    return st;
+  {
+    std::vector<int> cell(std::max(c->nbcrec, 1), 0);
+    for (int i = 0; i < c->LCM; ++i)
+      if (rec[i] >= 0) cell[rec[i]] = i;
+    if ((st = upload_new(c, &c->bccell, cell.data(), (size_t)c->nbcrec))) return st;
+  }
+  HIP_TRY(c, hipMemcpyAsync(c->bcrec, rec.data(), sizeof(int) * c->LCM,
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
+int efdc_set_sources(efdc_ctx_t *c, const efdc_sources_t *src) {
+  if (!c || !src || src->n < 0) return EFDC_ERR_ARG;
+  const int n = src->n, KC = c->KC, nc = std::max(c->nconst, 1);
+  if (n > 0 && (!src->LQS || !src->qss || !src->qsercell || !src->cqs ||
+                !src->csert))
+    return EFDC_ERR_ARG;
+  std::vector<int> rec(c->LCM, -1), next(std::max(n, 1), -1), l0(n);
+  // chain sources of one cell in increasing NS (accumulation order of
+  // EFDC/calfqc.f90:211-224)
+  std::vector<int> last(c->LCM, -1);
+  for (int s = 0; s < n; ++s) {
+    const int L = src->LQS[s];
+    if (L < 2 || L > c->LA) return EFDC_ERR_ARG;
+    l0[s] = L - 1;
+    if (rec[L - 1] < 0) rec[L - 1] = s;
+    else next[last[L - 1]] = s;
+    last[L - 1] = s;
+  }
+  int st;
+  c->nqsij = n;
+  c->src_nc = nc;
+  if ((st = upload_new(c, &c->lqs, l0.data(), n)) ||
+      (st = upload_new(c, &c->srcnext, next.data(), next.size())) ||
+      (st = upload_new(c, &c->qss, src->qss, (size_t)n * KC)) ||
+      (st = upload_new(c, &c->qsercell, src->qsercell, (size_t)n * KC)) ||
+      (st = upload_new(c, &c->cqs, src->cqs, (size_t)n * KC * nc)) ||
+      (st = upload_new(c, &c->csertq, src->csert, (size_t)n * KC * nc)))
+    return st;
+  HIP_TRY(c, hipMemcpyAsync(c->srcrec, rec.data(), sizeof(int) * c->LCM,
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
+int efdc_synchronize(efdc_ctx_t *c) {
+  if (!c) return EFDC_ERR_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
+int efdc_closure_set_manning(efdc_ctx_t *c, const double *nman) {
+  if (!c || !nman) return EFDC_ERR_ARG;
+  HIP_TRY(c, hipMemcpyAsync(c->nman, nman, sizeof(double) * c->LCM,
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
+int efdc_closure_pre(efdc_ctx_t *c) {
+  if (!c) return EFDC_ERR_ARG;
+  return closure_pre_run(c, c->nman);
+}
+
+int efdc_closure_post(efdc_ctx_t *c) {
+  if (!c) return EFDC_ERR_ARG;
+  return closure_post_run(c);
+}
+
+int efdc_congrad(efdc_ctx_t *c, efdc_cg_stats_t *stats) {
+  if (!c) return EFDC_ERR_ARG;
+  return cg_run(c, true, stats);
+}
+
+int efdc_calpuv2c(efdc_ctx_t *c, const efdc_step_t *step,
+                  efdc_cg_stats_t *stats) {
+  if (!c || !step || !(step->dt > 0.0)) return EFDC_ERR_ARG;
+  return puv_run(c, step, stats);
+}
+
+int efdc_calconc(efdc_ctx_t *c, const efdc_step_t *step) {
+  if (!c || !step || !(step->dt > 0.0)) return EFDC_ERR_ARG;
+  return tran_run(c, step);
+}
+
+int efdc_balance(efdc_ctx_t *c, efdc_balance_t *out) {
+  if (!c || !out) return EFDC_ERR_ARG;
+  return balance_run(c, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,278 @@
+// efdc_internal.h -- device context and shared device helpers of the
+// gfx950 implementation.  Layout: device index i = L-1 (L is the Fortran
+// cell index), wet cells are i = 1 .. LA-1, ghosts are i = 0 (L=1) and
+// i = LA (L=LC).  Horizontal arrays have LCM entries; layer arrays are
+// planes of `pitch` entries per K (K = 1..KC at plane K-1); W-class arrays
+// (LCM,0:KCM) have plane K at offset K*pitch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/efdc_hip.h"
+
+#define EFDC_MAXC 16
+#define EFDC_NK (EFDC_NUM_K_END - EFDC_HPK)
+#define EFDC_NW (EFDC_NUM_W_END - EFDC_W2)
+#define EFDC_BLOCK 256
+
+// ------------------------------------------------------------ error glue
+#define HIP_TRY(ctx, expr)                                                   \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);        \
+      return EFDC_ERR_HIP;                                                   \
+    }                                                                        \
+  } while (0)
+
+// ------------------------------------------------------- device scalars
+// State of one CG solve / one CALPUV2C call, kept on the device so that
+// no host round trip is needed between iterations.
+struct CgState {
+  double rpcg, papcg, alpha, beta, rsq, rsqm, bnorm2, ccmnm;
+  double rsq64, bnorm2_in, rz_in;  // mixed mode: outer residual, inner rhs
+  int iter, done, status, symok;   // symok: coefficients verified symmetric
+  int negdep, outer, inner_iter, pad;
+  unsigned ticket[8];
+};
+
+struct PbcDev {           // one pressure open-BC side on the device
+  int n;
+  int serial;             // 1: process the side with one thread
+  int *L;                 // 0-based device cell index
+  int *ispb;
+  double *pser, *pcb, *psb;
+};
+
+struct CbcDev {           // one concentration open-BC side on the device
+  int n;
+  int *L;
+  double *cb, *csert;     // [(LL*2+j)*nc+c], [(LL*KC+K-1)*nc+c]
+  int *ntscr;
+  double *clo;            // CLO*(LL,K,M)
+  long long *nlo;         // NLO*(LL,K,M)
+};
+
+struct ConstDev {
+  efdc_constituent_t desc;
+  void *buf[2];           // CON and CON1 alternate between the two buffers
+  int cur;                // buf[cur] = CON (current), buf[cur^1] = CON1
+};
+
+struct efdc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int LA = 0, LCM = 0, KC = 0, KCM = 0, LC = 0;
+  int64_t pitch = 0;      // device plane pitch of layer arrays (elements)
+  efdc_params_t prm{};
+  std::string err;
+  int store = EFDC_STORE_FP64;
+  size_t tsz = 8;         // bytes per stored layer/constituent value
+
+  // maps, 0-based device indices
+  int *dLW = nullptr, *dLE = nullptr, *dLS = nullptr, *dLN = nullptr;
+  int *dLNW = nullptr, *dLSE = nullptr;
+  bool rowmajor = false;  // LWC in {L-1,LC}, LEC in {L+1,LC}, maps mutual
+
+  double *dH[EFDC_NUM_H] = {};
+  void *dK[EFDC_NK] = {};
+  void *dW[EFDC_NW] = {};
+  double *staging = nullptr;   // FP64 staging of one layer array
+  size_t staging_elems = 0;
+
+  // CG scratch
+  double *r = nullptr, *pb[2] = {nullptr, nullptr}, *ap = nullptr;
+  double *dis = nullptr;       // 1/sqrt(CCC) (mixed mode)
+  float *as32 = nullptr, *aw32 = nullptr, *r32 = nullptr, *y32 = nullptr;
+  float *p32[2] = {nullptr, nullptr}, *ap32 = nullptr;
+  double *part = nullptr;      // reduction partials, 4 x max_blocks
+  int max_blocks = 0;
+  CgState *st = nullptr;       // device
+  CgState *st_host = nullptr;  // pinned mirror
+  int last_iters = 0;
+
+  // open boundaries
+  PbcDev pbc[4] = {};
+  int npbc_total = 0;
+  int mtide = 0;
+  double *tcp_host = nullptr;  // host copy of TCP
+  double *tide_dev = nullptr;  // [cos(M)..., sin(M)...] for the current call
+  double *tide_host = nullptr; // pinned
+  int *lopenbcdry = nullptr;   // (LCM)
+  unsigned char *pbcflag = nullptr;   // (LCM) 1 on pressure-BC cells
+  int *lobcs = nullptr;        // LOBCS order S, W, E, N (0-based)
+  int *lobcs2 = nullptr;
+  int nlobcs = 0, nlobcs2 = 0;
+
+  CbcDev cbc[4] = {};
+  double *wtci = nullptr;
+  int *bcrec = nullptr;        // (LCM) -1 or record index into bcinfo
+  int4 *bcinfo = nullptr;      // per BC column: LL per side (-1 if none)
+  int nbcrec = 0;
+  double *bc_con1 = nullptr;   // BC values of CON1 written after the step
+  int *bccell = nullptr;       // column index of every BC record
+
+  int cbc_nc = 0;              // constituent count the BC tables were built for
+  // sources
+  int src_nc = 0;              // constituent count the source tables were built for
+  int nqsij = 0;
+  int *lqs = nullptr;          // 0-based
+  int *srcrec = nullptr;       // (LCM) -1 or first source index at the cell
+  int *srcnext = nullptr;      // next source index at the same cell or -1
+  double *qss = nullptr, *qsercell = nullptr, *cqs = nullptr, *csertq = nullptr;
+
+  // constituents
+  int nconst = 0;
+  ConstDev con[EFDC_MAXC] = {};
+  void *tran_tmp = nullptr;    // forward-sweep storage when LDS is too small
+  // anti-diffusion scratch (allocated on first use)
+  void *adac[16] = {};
+  bool adac_alloc = false;
+  // uniform layer constants (sigma grid): CDZKK/CDZKMK per K
+  bool dz_uni_kk = true, dz_uni_kmk = true;
+  bool dz_uniform() const { return dz_uni_kk && dz_uni_kmk; }
+  double *cdzkk_k = nullptr, *cdzkmk_k = nullptr;  // device, KC entries
+  double *scal = nullptr;      // small result buffer (balance totals)
+  double *nman = nullptr;      // per-cell Manning n of the closure
+
+  int n_wet() const { return LA - 1; }
+};
+
+// ---------------------------------------------------------- device utils
+// Workgroups are dispatched round-robin over the 8 XCDs of the MI355X.
+// Remapping the hardware block id so that every XCD owns one contiguous
+// range of tiles keeps the N/S neighbour rows of a tile in the same XCD's
+// L2 and keeps the tiles of one XCD in that L2 between dependent kernels.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int x = b & 7, q = b >> 3;
+  const int per = nb >> 3, rem = nb & 7;
+  return x * per + (x < rem ? x : rem) + q;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Sum NV values over the block; result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  __shared__ double sh[NV][EFDC_BLOCK / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] = wave_sum(v[j]);
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) sh[j][w] = v[j];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      double s = 0.0;
+      for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += sh[j][q];
+      v[j] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Deterministic grid reduction: every block stores its partials, the last
+// block to arrive (atomic ticket) sums the partials in block order with a
+// fixed tree, so the result does not depend on scheduling.  Returns true
+// in every thread of the last block; the sums are then in out[0..NV-1]
+// (valid in all threads of that block).  The ticket is reset for reuse.
+template <int NV>
+__device__ __forceinline__ bool grid_sum(double (&v)[NV], double *part,
+                                         unsigned *ticket, int nblocks,
+                                         int bid, double (&out)[NV]) {
+  __shared__ bool am_last;
+  block_sum<NV>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) part[(size_t)j * nblocks + bid] = v[j];
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    am_last = (t == (unsigned)nblocks - 1u);
+  }
+  __syncthreads();
+  if (!am_last) return false;
+  __threadfence();
+  double acc[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    double s = 0.0;
+    for (int q = threadIdx.x; q < nblocks; q += blockDim.x)
+      s += __builtin_nontemporal_load(&part[(size_t)j * nblocks + q]);
+    acc[j] = s;
+  }
+  block_sum<NV>(acc);
+  __shared__ double res[NV];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) res[j] = acc[j];
+    *ticket = 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NV; ++j) out[j] = res[j];
+  return true;
+}
+
+// Same pattern for a minimum.
+__device__ __forceinline__ bool grid_min(double v, double *part,
+                                         unsigned *ticket, int nblocks,
+                                         int bid, double &out) {
+  __shared__ double sh[EFDC_BLOCK / 64];
+  __shared__ bool am_last;
+  __shared__ double res;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  v = wave_min(v);
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = sh[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmin(m, sh[q]);
+    part[bid] = m;
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    am_last = (t == (unsigned)nblocks - 1u);
+  }
+  __syncthreads();
+  if (!am_last) return false;
+  __threadfence();
+  double m = 1.0e300;
+  for (int q = threadIdx.x; q < nblocks; q += blockDim.x)
+    m = fmin(m, __builtin_nontemporal_load(&part[q]));
+  m = wave_min(m);
+  if (lane == 0) sh[w] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = sh[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) mm = fmin(mm, sh[q]);
+    res = mm;
+    *ticket = 0u;
+  }
+  __syncthreads();
+  out = res;
+  return true;
+}
+
+// ------------------------------------------------------------ host glue
+int efdc_grid_blocks(int n);
+int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats);
+int cg_run(efdc_ctx *c, bool check_sym, efdc_cg_stats_t *stats);
+int tran_run(efdc_ctx *c, const efdc_step_t *step);
+int balance_run(efdc_ctx *c, efdc_balance_t *out);
+int closure_pre_run(efdc_ctx *c, const double *nman_dev);
+int closure_post_run(efdc_ctx *c);

@@ -1,0 +1,869 @@
+// efdc_puv.hip -- CALPUV2C (2TL, ISDRY=0) and CONGRAD on gfx950.
+//
+// One thread per wet cell, 256-thread workgroups, XCD-aware tile order.
+// Every kernel here is a streaming pass over L-ordered (LCM) FP64 arrays;
+// the roofline is HBM (or MALL/L2 when the working set is resident).
+//
+// CONGRAD (EFDC/congrad.f90:54-187) runs as two kernels per iteration:
+//   k_cg_a: p = CCCI*r + beta*p (recomputed for the 4 neighbours) fused
+//           with Ap = A p and the partial sums of p.Ap;
+//   k_cg_b: x += alpha p, r -= alpha Ap, partial sums of r.z and r.r.
+// The alpha/beta scalars never leave the device: the last workgroup of a
+// kernel (atomic ticket) finishes the FP64 reduction in a fixed order and
+// writes the scalars for the next kernel.  The next launches read a device
+// `done` flag and return immediately once the stop test (sum r^2 <= RSQM)
+// has been met, so the host only polls every few iterations.
+#include <cmath>
+#include <cstring>
+
+#include "efdc_internal.h"
+
+namespace {
+
+struct HPtr {
+  double *a[EFDC_NUM_H];
+};
+
+HPtr hptr(efdc_ctx *c) {
+  HPtr h;
+  for (int j = 0; j < EFDC_NUM_H; ++j) h.a[j] = c->dH[j];
+  return h;
+}
+
+#define HA(id) h.a[EFDC_##id]
+
+__device__ __forceinline__ int cell_index(int nwet, int &tile) {
+  tile = xcd_tile(blockIdx.x, gridDim.x);
+  return 1 + tile * EFDC_BLOCK + (int)threadIdx.x;  // wet iff <= nwet
+}
+
+// ---------------------------------------------------------------------
+// Explicit stage and time-level shifts: EFDC/calpuv2c.f90:214-220,256-290
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_puv_explicit(
+    HPtr h, const int *__restrict__ lw, const int *__restrict__ ls,
+    const int *__restrict__ le, const int *__restrict__ ln, const T *hpk,
+    T *h1pk, int64_t pitch, int KC, int nwet, double dt, double G,
+    double RITB1, double SNLT) {
+  int tile;
+  const int i = cell_index(nwet, tile);
+  if (i > nwet) return;
+  const double DELT = dt, DELTD2 = 0.5 * dt, DELTI = 1. / DELT;
+  const int w = lw[i], s = ls[i], e = le[i], n = ln[i];
+  const double *P = HA(P);
+  HA(FUHDYE)[i] =
+      HA(UHDYE)[i] - DELTD2 * HA(SUB)[i] * HA(HRUO)[i] * HA(HU)[i] * (P[i] - P[w]) +
+      HA(SUB)[i] * DELT * HA(DXIU)[i] *
+          (HA(DXYU)[i] * (HA(TSX)[i] - RITB1 * HA(TBX)[i]) + HA(FCAXE)[i] +
+           HA(FPGXE)[i] - SNLT * HA(FXE)[i]);
+  HA(FVHDXE)[i] =
+      HA(VHDXE)[i] - DELTD2 * HA(SVB)[i] * HA(HRVO)[i] * HA(HV)[i] * (P[i] - P[s]) +
+      HA(SVB)[i] * DELT * HA(DYIV)[i] *
+          (HA(DXYV)[i] * (HA(TSY)[i] - RITB1 * HA(TBY)[i]) - HA(FCAYE)[i] +
+           HA(FPGYE)[i] - SNLT * HA(FYE)[i]);
+  HA(UHDY1E)[i] = HA(UHDYE)[i];
+  HA(VHDX1E)[i] = HA(VHDXE)[i];
+  HA(P1)[i] = P[i];
+  HA(H1U)[i] = HA(HU)[i];
+  HA(H1V)[i] = HA(HV)[i];
+  HA(H1UI)[i] = HA(HUI)[i];
+  HA(H1VI)[i] = HA(HVI)[i];
+  HA(H2P)[i] = HA(H1P)[i];
+  HA(H1P)[i] = HA(HP)[i];
+  for (int k = 0; k < KC; ++k) h1pk[i + k * pitch] = hpk[i + k * pitch];
+  const double C1 = 0.5 * G;
+  HA(FP1)[i] = DELTI * HA(DXYP)[i] * P[i] -
+               C1 * (HA(UHDYE)[e] - HA(UHDYE)[i] + HA(VHDXE)[n] - HA(VHDXE)[i]);
+}
+
+// ---------------------------------------------------------------------
+// Assembly: EFDC/calpuv2c.f90:315-342, with the min-diagonal reduction of
+// :360-376 over the cells that are not pressure open boundaries (those
+// get their final CC in k_puv_bc, which completes the minimum).
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(EFDC_BLOCK) void k_puv_assemble(
+    HPtr h, const int *__restrict__ le, const int *__restrict__ ln,
+    const unsigned char *__restrict__ pbcflag, int nwet, double dt, double G,
+    double *part, CgState *st) {
+  int tile;
+  const int i = cell_index(nwet, tile);
+  double ccv = 1.E+18;
+  if (i <= nwet) {
+    const double DELTD2 = 0.5 * dt, DELTI = 1. / dt;
+    const int e = le[i], n = ln[i];
+    const double *SUB = HA(SUB), *SVB = HA(SVB);
+    const double C1 = 0.5 * G;
+    HA(FP)[i] = HA(FP1)[i] - C1 * (SUB[e] * HA(FUHDYE)[e] - SUB[i] * HA(FUHDYE)[i] +
+                                   SVB[n] * HA(FVHDXE)[n] - SVB[i] * HA(FVHDXE)[i] -
+                                   2.0 * HA(QSUME)[i]);
+    const double C2 = -0.5 * DELTD2 * G;
+    const double cs = C2 * SVB[i] * HA(HRVO)[i] * HA(RCY)[i] * HA(HV)[i];
+    const double cw = C2 * SUB[i] * HA(HRUO)[i] * HA(RCX)[i] * HA(HU)[i];
+    const double ce = C2 * SUB[e] * HA(HRUO)[e] * HA(RCX)[e] * HA(HU)[e];
+    const double cn = C2 * SVB[n] * HA(HRVO)[n] * HA(RCY)[n] * HA(HV)[n];
+    HA(CS)[i] = cs;
+    HA(CW)[i] = cw;
+    HA(CE)[i] = ce;
+    HA(CN)[i] = cn;
+    const double cc = DELTI * HA(DXYP)[i] - cs - cw - ce - cn;
+    HA(CC)[i] = cc;
+    if (!pbcflag[i]) ccv = cc;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->symok = 0;  // reset flags
+  double m;
+  if (grid_min(ccv, part, &st->ticket[0], gridDim.x, tile, m) &&
+      threadIdx.x == 0)
+    st->ccmnm = m;
+}
+
+// ---------------------------------------------------------------------
+// SETOPENBC (EFDC/setopenbc.f90:17-206), the open-boundary part of the
+// min-diagonal reduction, and SETOPENBC2 (EFDC/setopenbc2.f90:17-45).
+// One workgroup; the sides are applied in the reference order W, E, S, N.
+// ---------------------------------------------------------------------
+struct PbcArgs {
+  PbcDev side[4];
+  const double *tide;  // cos(M) at [M], sin(M) at [mtide+M]
+  int mtide;
+  int *lopenbcdry;
+};
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_puv_bc(
+    HPtr h, PbcArgs b, const int *__restrict__ lw, const int *__restrict__ le,
+    const int *__restrict__ ls, const int *__restrict__ ln, double dt,
+    double G, double HDRY, CgState *st) {
+  const double D_LTD2 = 0.5 * dt, DELTI = 1. / dt, DE_TI = DELTI;
+  const double HDRY2 = 0.2 * HDRY;
+  double *CC = HA(CC), *CS = HA(CS), *CW = HA(CW), *CE = HA(CE), *CN = HA(CN),
+         *FP = HA(FP), *DXYP = HA(DXYP), *BELV = HA(BELV), *HRUO = HA(HRUO),
+         *HRVO = HA(HRVO), *RCX = HA(RCX), *RCY = HA(RCY), *HUT = HA(HU),
+         *HVT = HA(HV), *DXIU = HA(DXIU), *DYIU = HA(DYIU), *DXIV = HA(DXIV),
+         *DYIV = HA(DYIV), *FUHDYE = HA(FUHDYE), *FVHDXE = HA(FVHDXE);
+  for (int side = 0; side < 4; ++side) {
+    const PbcDev &s = b.side[side];
+    const int stride = s.serial ? 1 : blockDim.x;
+    if (s.serial && threadIdx.x != 0) {
+      __syncthreads();
+      continue;
+    }
+    for (int LL = threadIdx.x; LL < s.n; LL += stride) {
+      const int i = s.L[LL];
+      CC[i] = DELTI * DXYP[i];
+      CS[i] = 0.;
+      CW[i] = 0.;
+      CE[i] = 0.;
+      CN[i] = 0.;
+      if (b.lopenbcdry[i]) continue;
+      double fp = s.pser[LL];
+      for (int M = 0; M < b.mtide; ++M)
+        fp = fp + s.pcb[LL * b.mtide + M] * b.tide[M] +
+             s.psb[LL * b.mtide + M] * b.tide[b.mtide + M];
+      FP[i] = fp;
+      const bool rad = (s.ispb[LL] == 1 || s.ispb[LL] == 2);
+      if (side == 0) {
+        const int e = le[i];
+        if (rad) {
+          const double CET = 0.5 * D_LTD2 * G * HRUO[e] * RCX[e] * HUT[e];
+          const double TMP = D_LTD2 * sqrt(G * HUT[e]) * DXIU[e];
+          CC[i] = CET * (1. + TMP) / TMP;
+          CE[i] = -CET;
+          FP[i] = CET * (2. * FP[i] - sqrt(G * HUT[e]) * FUHDYE[e] * DYIU[e] / HUT[e]) / TMP;
+        } else {
+          double CET;
+          const double FP1G = FP[i] / G - HDRY2;
+          if (FP1G < BELV[i] || FP1G < BELV[e]) {
+            FP[i] = (BELV[i] + HDRY2) * G;
+            CET = 0.;
+            b.lopenbcdry[i] = 1;
+          } else {
+            CET = 0.5 * D_LTD2 * G * HRUO[e] * RCX[e] * HUT[e];
+          }
+          FP[e] = FP[e] + CET * FP[i];
+          FP[i] = DE_TI * DXYP[i] * FP[i];
+        }
+      } else if (side == 1) {
+        const int w = lw[i];
+        if (rad) {
+          const double CWT = 0.5 * D_LTD2 * G * HRUO[i] * RCX[i] * HUT[i];
+          const double TMP = D_LTD2 * sqrt(G * HUT[i]) * DXIU[i];
+          CC[i] = CWT * (1. + TMP) / TMP;
+          CW[i] = -CWT;
+          FP[i] = CWT * (2. * FP[i] + sqrt(G * HUT[i]) * FUHDYE[i] * DYIU[i] / HUT[i]) / TMP;
+        } else {
+          double CWT;
+          const double FP1G = FP[i] / G - HDRY2;
+          if (FP1G < BELV[i] || FP1G < BELV[w]) {
+            FP[i] = (BELV[i] + HDRY2) * G;
+            CWT = 0.;
+            b.lopenbcdry[i] = 1;
+          } else {
+            CWT = 0.5 * D_LTD2 * G * HRUO[i] * RCX[i] * HUT[i];
+          }
+          FP[w] = FP[w] + CWT * FP[i];
+          FP[i] = DE_TI * DXYP[i] * FP[i];
+        }
+      } else if (side == 2) {
+        const int n = ln[i];
+        if (rad) {
+          const double CNT = 0.5 * D_LTD2 * G * HRVO[n] * RCY[n] * HVT[n];
+          const double TMP = D_LTD2 * sqrt(G * HVT[n]) * DYIV[n];
+          CC[i] = CNT * (1. + TMP) / TMP;
+          CN[i] = -CNT;
+          FP[i] = CNT * (2. * FP[i] - sqrt(G * HVT[n]) * FVHDXE[n] * DXIV[n] / HVT[n]) / TMP;
+        } else {
+          double CNT;
+          const double FP1G = FP[i] / G - HDRY2;
+          if (FP1G < BELV[i] || FP1G < BELV[n]) {
+            FP[i] = (BELV[i] + HDRY2) * G;
+            CNT = 0.;
+            b.lopenbcdry[i] = 1;
+          } else {
+            CNT = 0.5 * D_LTD2 * G * HRVO[n] * RCY[n] * HVT[n];
+          }
+          FP[n] = FP[n] + CNT * FP[i];
+          FP[i] = DE_TI * DXYP[i] * FP[i];
+        }
+      } else {
+        const int s2 = ls[i];
+        if (rad) {
+          const double CST = 0.5 * D_LTD2 * G * HRVO[i] * RCY[i] * HVT[i];
+          const double TMP = D_LTD2 * sqrt(G * HVT[i]) * DYIV[i];
+          CC[i] = CST * (1. + TMP) / TMP;
+          CS[i] = -CST;
+          FP[i] = CST * (2. * FP[i] + sqrt(G * HVT[i]) * FVHDXE[i] * DXIV[i] / HVT[i]) / TMP;
+        } else {
+          double CST;
+          const double FP1G = FP[i] / G - HDRY2;
+          if (FP1G < BELV[i] || FP1G < BELV[s2]) {
+            FP[i] = (BELV[i] + HDRY2) * G;
+            CST = 0.;
+            b.lopenbcdry[i] = 1;
+          } else {
+            CST = 0.5 * D_LTD2 * G * HRVO[i] * RCY[i] * HVT[i];
+          }
+          FP[s2] = FP[s2] + CST * FP[i];
+          FP[i] = DE_TI * DXYP[i] * FP[i];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // minimum of CC over the open-boundary cells (:365-375)
+  double m = 1.E+18;
+  for (int side = 0; side < 4; ++side)
+    for (int LL = threadIdx.x; LL < b.side[side].n; LL += blockDim.x)
+      m = fmin(m, CC[b.side[side].L[LL]]);
+  m = wave_min(m);
+  __shared__ double sh[EFDC_BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = st->ccmnm;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) mm = fmin(mm, sh[q]);
+    st->ccmnm = mm;
+  }
+  __syncthreads();
+  // SETOPENBC2
+  for (int side = 0; side < 4; ++side) {
+    const PbcDev &s = b.side[side];
+    for (int LL = threadIdx.x; LL < s.n; LL += blockDim.x) {
+      if (s.ispb[LL] != 0) continue;
+      const int i = s.L[LL];
+      if (side == 0) CW[le[i]] = 0.;
+      else if (side == 1) CE[lw[i]] = 0.;
+      else if (side == 2) CS[ln[i]] = 0.;
+      else CN[ls[i]] = 0.;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------
+// Scaling by the minimum diagonal: EFDC/calpuv2c.f90:384-398, plus the
+// check that the system is symmetric and E/W neighbours are contiguous,
+// which selects the compact CG operator (CCN(L)=CCS(LN), CCE(L)=CCW(L+1)).
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(EFDC_BLOCK) void k_puv_scale(
+    HPtr h, const int *__restrict__ lw, const int *__restrict__ le,
+    const int *__restrict__ ln, int nwet, int lc0, CgState *st) {
+  int tile;
+  const int i = cell_index(nwet, tile);
+  if (i > nwet) return;
+  const double CCMNMI = 1. / st->ccmnm;
+  const double cs = HA(CS)[i], cw = HA(CW)[i], ce = HA(CE)[i], cn = HA(CN)[i];
+  HA(CCS)[i] = cs * CCMNMI;
+  const double ccw = cw * CCMNMI;
+  HA(CCW)[i] = ccw;
+  HA(CCE)[i] = ce * CCMNMI;
+  HA(CCN)[i] = cn * CCMNMI;
+  const double ccc = HA(CC)[i] * CCMNMI;
+  HA(CCC)[i] = ccc;
+  HA(FPTMP)[i] = HA(FP)[i] * CCMNMI;
+  HA(CCCI)[i] = 1. / ccc;
+  const int n = ln[i], e = le[i], w = lw[i];
+  bool bad = false;
+  if (n != lc0 && cn != HA(CS)[n]) bad = true;
+  if (e != lc0 && ce != HA(CW)[e]) bad = true;
+  if (w == lc0 && ccw != 0.0) bad = true;
+  if (bad) atomicOr(&st->symok, 1);
+}
+
+// Same check on an already scaled system (standalone CONGRAD calls).
+__global__ __launch_bounds__(EFDC_BLOCK) void k_cg_check(
+    const double *ccs, const double *ccw, const double *cce, const double *ccn,
+    const int *__restrict__ lw, const int *__restrict__ le,
+    const int *__restrict__ ln, int nwet, int lc0, CgState *st) {
+  int tile;
+  const int i = cell_index(nwet, tile);
+  if (blockIdx.x == 0 && threadIdx.x == 0) (void)0;
+  if (i > nwet) return;
+  const int n = ln[i], e = le[i], w = lw[i];
+  bool bad = false;
+  if (n != lc0 && ccn[i] != ccs[n]) bad = true;
+  if (e != lc0 && cce[i] != ccw[e]) bad = true;
+  if (w == lc0 && ccw[i] != 0.0) bad = true;
+  if (bad) atomicOr(&st->symok, 1);
+}
+
+__global__ void k_reset_flags(CgState *st) {
+  st->symok = 0;
+  st->negdep = 0;
+}
+
+// ---------------------------------------------------------------------
+// CONGRAD, FP64 (EFDC/congrad.f90)
+// ---------------------------------------------------------------------
+struct CgArgs {
+  const double *ccc, *ccs, *ccw, *ccn, *cce, *ccci, *fptmp;
+  double *x, *r, *ap;
+  const int *lw, *le, *ls, *ln;
+  int nwet, lc0;
+  int rowmajor;
+  CgState *st;
+  double *part;
+};
+
+// Off-diagonal coefficients of row i in the reference's sense, with
+// land neighbours contributing zero (the reference multiplies by the
+// zero ghost entry of PCG).
+struct Row {
+  double cs, cn, cw, ce;
+  int js, jn, jw, je;
+};
+
+__device__ __forceinline__ Row row_of(const CgArgs &a, int i, bool fast) {
+  Row q;
+  q.js = a.ls[i];
+  q.jn = a.ln[i];
+  q.cs = a.ccs[i];
+  q.cw = a.ccw[i];
+  if (fast) {
+    q.jw = i - 1;
+    q.je = i + 1;
+    q.cn = (q.jn != a.lc0) ? a.ccs[q.jn] : 0.0;
+    q.ce = (q.je != a.lc0) ? a.ccw[q.je] : 0.0;
+  } else {
+    q.jw = a.lw[i];
+    q.je = a.le[i];
+    q.cn = (q.jn != a.lc0) ? a.ccn[i] : 0.0;
+    q.ce = (q.je != a.lc0) ? a.cce[i] : 0.0;
+    if (q.jw == a.lc0) q.cw = 0.0;
+  }
+  if (q.js == a.lc0) q.cs = 0.0;
+  return q;
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_cg_init(CgArgs a, double rsqm,
+                                                        double rel_tol) {
+  int tile;
+  const int i = cell_index(a.nwet, tile);
+  const bool fast = a.rowmajor && (a.st->symok == 0);
+  double v[2] = {0.0, 0.0};
+  if (i <= a.nwet) {
+    const Row q = row_of(a, i, fast);
+    const double *P = a.x;
+    // RCG = FPTMP - CCC*P - CCN*PN - CCS*PS - CCW*PW - CCE*PE (:73)
+    const double r = a.fptmp[i] - a.ccc[i] * P[i] - q.cn * P[q.jn] -
+                     q.cs * P[q.js] - q.cw * P[q.jw] - q.ce * P[q.je];
+    a.r[i] = r;
+    const double z = r * a.ccci[i];  // PCG = RCG*CCCI (:80)
+    v[0] = r * z;
+    v[1] = a.fptmp[i] * a.fptmp[i];
+  }
+  double s[2];
+  if (grid_sum<2>(v, a.part, &a.st->ticket[1], gridDim.x, tile, s) &&
+      threadIdx.x == 0) {
+    CgState *st = a.st;
+    st->rpcg = s[0];
+    st->bnorm2 = s[1];
+    st->rsqm = rel_tol > 0.0 ? rel_tol * rel_tol * s[1] : rsqm;
+    st->beta = 0.0;
+    st->alpha = 0.0;
+    st->iter = 0;
+    st->rsq = 0.0;
+    st->status = 0;
+    st->done = (s[0] == 0.0) ? 1 : 0;  // :88
+  }
+}
+
+__device__ __forceinline__ double pnew_at(const CgArgs &a, const double *pold,
+                                          double beta, int j) {
+  // PCG = TMPCG + BETA*PCG with TMPCG = CCCI*RCG (:134,158); ghosts hold 0
+  return (j > 0 && j != a.lc0) ? a.ccci[j] * a.r[j] + beta * pold[j] : 0.0;
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_cg_a(CgArgs a,
+                                                     const double *pold,
+                                                     double *pnew) {
+  if (a.st->done) return;
+  int tile;
+  const int i = cell_index(a.nwet, tile);
+  const bool fast = a.rowmajor && (a.st->symok == 0);
+  const double beta = a.st->beta;
+  double v[1] = {0.0};
+  if (i <= a.nwet) {
+    const Row q = row_of(a, i, fast);
+    const double pc = a.ccci[i] * a.r[i] + beta * pold[i];
+    const double ps = pnew_at(a, pold, beta, q.js);
+    const double pn = pnew_at(a, pold, beta, q.jn);
+    const double pw = pnew_at(a, pold, beta, q.jw);
+    const double pe = pnew_at(a, pold, beta, q.je);
+    // APCG = CCC*PCG + CCS*PS + CCN*PN + CCW*PW + CCE*PE (:113)
+    const double apc = a.ccc[i] * pc + q.cs * ps + q.cn * pn + q.cw * pw + q.ce * pe;
+    pnew[i] = pc;
+    a.ap[i] = apc;
+    v[0] = apc * pc;
+  }
+  double s[1];
+  if (grid_sum<1>(v, a.part, &a.st->ticket[2], gridDim.x, tile, s) &&
+      threadIdx.x == 0) {
+    a.st->papcg = s[0];
+    a.st->alpha = a.st->rpcg / s[0];  // :124
+  }
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_cg_b(CgArgs a,
+                                                     const double *pnew,
+                                                     int iterm) {
+  if (a.st->done) return;
+  int tile;
+  const int i = cell_index(a.nwet, tile);
+  const double alpha = a.st->alpha;
+  double v[2] = {0.0, 0.0};
+  if (i <= a.nwet) {
+    a.x[i] = a.x[i] + alpha * pnew[i];       // :132
+    const double r = a.r[i] - alpha * a.ap[i];  // :133
+    a.r[i] = r;
+    const double z = a.ccci[i] * r;          // :134
+    v[0] = r * z;
+    v[1] = r * r;
+  }
+  double s[2];
+  if (grid_sum<2>(v, a.part, &a.st->ticket[3], gridDim.x, tile, s) &&
+      threadIdx.x == 0) {
+    CgState *st = a.st;
+    const double beta = s[0] / st->rpcg;  // :147-148
+    st->beta = beta;
+    st->rpcg = s[0];
+    st->rsq = s[1];
+    st->iter += 1;
+    if (s[1] <= st->rsqm) st->done = 1;             // :165
+    else if (st->iter >= iterm) { st->done = 1; st->status = 1; }
+  }
+}
+
+// ---------------------------------------------------------------------
+// Mixed precision: FP32 inner CG on the Jacobi-scaled operator
+// A' = D^-1/2 A D^-1/2 (unit diagonal), FP64 dot products, FP64
+// residual-based refinement of P.  Uses the compact symmetric operator.
+// ---------------------------------------------------------------------
+struct MxArgs {
+  CgArgs g;
+  double *dis;       // 1/sqrt(CCC)
+  float *as, *aw;    // scaled off-diagonals (south, west) in FP32
+  float *r32, *y32, *ap32;
+};
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_mx_prep(MxArgs m, double rsqm,
+                                                        double rel_tol) {
+  int tile;
+  const CgArgs &a = m.g;
+  const int i = cell_index(a.nwet, tile);
+  double v[1] = {0.0};
+  if (i <= a.nwet) {
+    const double di = 1.0 / sqrt(a.ccc[i]);
+    m.dis[i] = di;
+    const int js = a.ls[i];
+    const double ds = (js != a.lc0) ? 1.0 / sqrt(a.ccc[js]) : 0.0;
+    const double dw = (i - 1 > 0) ? 1.0 / sqrt(a.ccc[i - 1]) : 0.0;
+    const double cs = (js != a.lc0) ? a.ccs[i] : 0.0;
+    m.as[i] = (float)(cs * di * ds);
+    m.aw[i] = (float)(a.ccw[i] * di * dw);
+    v[0] = a.fptmp[i] * a.fptmp[i];
+  }
+  double s[1];
+  if (grid_sum<1>(v, a.part, &a.st->ticket[4], gridDim.x, tile, s) &&
+      threadIdx.x == 0) {
+    a.st->bnorm2 = s[0];
+    a.st->rsqm = rel_tol > 0.0 ? rel_tol * rel_tol * s[0] : rsqm;
+    a.st->outer = 0;
+    a.st->iter = 0;
+    a.st->status = 0;
+  }
+}
+
+// r64 = FPTMP - A P (FP64, compact operator), r' = D^-1/2 r64 in FP32,
+// y = 0; starts one inner solve.
+__global__ __launch_bounds__(EFDC_BLOCK) void k_mx_resid(MxArgs m) {
+  int tile;
+  const CgArgs &a = m.g;
+  const int i = cell_index(a.nwet, tile);
+  double v[2] = {0.0, 0.0};
+  if (i <= a.nwet) {
+    const Row q = row_of(a, i, true);
+    const double *P = a.x;
+    const double r = a.fptmp[i] - a.ccc[i] * P[i] - q.cn * P[q.jn] -
+                     q.cs * P[q.js] - q.cw * P[q.jw] - q.ce * P[q.je];
+    const float rs = (float)(m.dis[i] * r);
+    m.r32[i] = rs;
+    m.y32[i] = 0.0f;
+    v[0] = r * r;
+    v[1] = (double)rs * (double)rs;
+  }
+  double s[2];
+  if (grid_sum<2>(v, a.part, &a.st->ticket[5], gridDim.x, tile, s) &&
+      threadIdx.x == 0) {
+    CgState *st = a.st;
+    st->rsq64 = s[0];
+    st->rsq = s[0];
+    st->bnorm2_in = s[1];
+    st->rz_in = s[1];
+    st->beta = 0.0;
+    st->inner_iter = 0;
+    st->done = (s[0] <= st->rsqm || s[1] == 0.0) ? 1 : 0;
+  }
+}
+
+__device__ __forceinline__ float p32_at(const MxArgs &m, const float *pold,
+                                        double beta, int j) {
+  return (j > 0 && j != m.g.lc0)
+             ? (float)((double)m.r32[j] + beta * (double)pold[j])
+             : 0.0f;
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_mx_a(MxArgs m, const float *pold,
+                                                     float *pnew) {
+  const CgArgs &a = m.g;
+  if (a.st->done) return;
+  int tile;
+  const int i = cell_index(a.nwet, tile);
+  const double beta = a.st->beta;
+  double v[1] = {0.0};
+  if (i <= a.nwet) {
+    const int js = a.ls[i], jn = a.ln[i];
+    const float pc = p32_at(m, pold, beta, i);
+    const float ps = p32_at(m, pold, beta, js);
+    const float pn = p32_at(m, pold, beta, jn);
+    const float pw = p32_at(m, pold, beta, i - 1);
+    const float pe = p32_at(m, pold, beta, i + 1);
+    const float an = (jn != a.lc0) ? m.as[jn] : 0.0f;
+    const float ae = (i + 1 != a.lc0) ? m.aw[i + 1] : 0.0f;
+    const float apc = pc + m.as[i] * ps + an * pn + m.aw[i] * pw + ae * pe;
+    pnew[i] = pc;
+    m.ap32[i] = apc;
+    v[0] = (double)apc * (double)pc;
+  }
+  double s[1];
+  if (grid_sum<1>(v, a.part, &a.st->ticket[6], gridDim.x, tile, s) &&
+      threadIdx.x == 0)
+    a.st->alpha = a.st->rz_in / s[0];
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_mx_b(MxArgs m, const float *pnew,
+                                                     double rtol2, int cap) {
+  const CgArgs &a = m.g;
+  if (a.st->done) return;
+  int tile;
+  const int i = cell_index(a.nwet, tile);
+  const double alpha = a.st->alpha;
+  double v[1] = {0.0};
+  if (i <= a.nwet) {
+    m.y32[i] = (float)((double)m.y32[i] + alpha * (double)pnew[i]);
+    const float r = (float)((double)m.r32[i] - alpha * (double)m.ap32[i]);
+    m.r32[i] = r;
+    v[0] = (double)r * (double)r;
+  }
+  double s[1];
+  if (grid_sum<1>(v, a.part, &a.st->ticket[7], gridDim.x, tile, s) &&
+      threadIdx.x == 0) {
+    CgState *st = a.st;
+    st->beta = s[0] / st->rz_in;
+    st->rz_in = s[0];
+    st->inner_iter += 1;
+    st->iter += 1;
+    if (s[0] <= rtol2 * st->bnorm2_in || st->inner_iter >= cap) st->done = 1;
+  }
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_mx_apply(MxArgs m) {
+  int tile;
+  const CgArgs &a = m.g;
+  const int i = cell_index(a.nwet, tile);
+  if (i <= a.nwet) a.x[i] = a.x[i] + m.dis[i] * (double)m.y32[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.st->outer += 1;
+}
+
+// ---------------------------------------------------------------------
+// Post-solve: EFDC/calpuv2c.f90:423-434 (face transports), :472-477
+// (depths), :490-492 (open boundaries), :896-914 (P, HU, HV, inverses),
+// EFDC/negdep.f90:34-45 (negative depth flag).
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(EFDC_BLOCK) void k_puv_post1(
+    HPtr h, const int *__restrict__ lw, const int *__restrict__ ls, int nwet,
+    double dt) {
+  int tile;
+  const int i = cell_index(nwet, tile);
+  if (i > nwet) return;
+  const double DELTD2 = 0.5 * dt;
+  const double *P = HA(P);
+  const int w = lw[i], s = ls[i];
+  const double u = HA(SUB)[i] * (HA(FUHDYE)[i] - DELTD2 * HA(HRUO)[i] * HA(RCX)[i] *
+                                                    HA(HU)[i] * (P[i] - P[w]));
+  const double v = HA(SVB)[i] * (HA(FVHDXE)[i] - DELTD2 * HA(HRVO)[i] * HA(RCY)[i] *
+                                                    HA(HV)[i] * (P[i] - P[s]));
+  HA(UHDYE)[i] = u;
+  HA(VHDXE)[i] = v;
+  HA(UHE)[i] = u * HA(DYIU)[i];
+  HA(VHE)[i] = v * HA(DXIV)[i];
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_puv_post2(
+    HPtr h, const int *__restrict__ le, const int *__restrict__ ln,
+    const unsigned char *__restrict__ pbcflag, int nwet, double dt, double G) {
+  int tile;
+  const int i = cell_index(nwet, tile);
+  if (i > nwet) return;
+  const double DELTD2 = 0.5 * dt, GI = 1. / G;
+  const int e = le[i], n = ln[i];
+  const double *U = HA(UHDYE), *U1 = HA(UHDY1E), *V = HA(VHDXE), *V1 = HA(VHDX1E);
+  double hp = HA(H1P)[i] + DELTD2 * HA(DXYIP)[i] *
+                               (2. * HA(QSUME)[i] -
+                                (U[e] + U1[e] - U[i] - U1[i] + V[n] + V1[n] - V[i] - V1[i]));
+  if (pbcflag[i]) hp = GI * HA(P)[i] - HA(BELV)[i];
+  HA(HP)[i] = hp;
+  HA(P)[i] = G * (hp + HA(BELV)[i]);
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_puv_post3(
+    HPtr h, const int *__restrict__ lw, const int *__restrict__ ls, int nwet,
+    CgState *st) {
+  int tile;
+  const int i = cell_index(nwet, tile);
+  if (i > nwet) return;
+  const int w = lw[i], s = ls[i];
+  const double *HP = HA(HP), *DXYP = HA(DXYP);
+  const double hu = (DXYP[i] * HP[i] + DXYP[w] * HP[w]) * (0.5 / HA(DXYU)[i]);
+  const double hv = (DXYP[i] * HP[i] + DXYP[s] * HP[s]) * (0.5 / HA(DXYV)[i]);
+  HA(HU)[i] = hu;
+  HA(HV)[i] = hv;
+  HA(HPI)[i] = 1. / HP[i];
+  HA(HUI)[i] = 1. / hu;
+  HA(HVI)[i] = 1. / hv;
+  if (HP[i] <= 0.) st->negdep = 1;
+}
+
+CgArgs cg_args(efdc_ctx *c) {
+  CgArgs a;
+  a.ccc = c->dH[EFDC_CCC];
+  a.ccs = c->dH[EFDC_CCS];
+  a.ccw = c->dH[EFDC_CCW];
+  a.ccn = c->dH[EFDC_CCN];
+  a.cce = c->dH[EFDC_CCE];
+  a.ccci = c->dH[EFDC_CCCI];
+  a.fptmp = c->dH[EFDC_FPTMP];
+  a.x = c->dH[EFDC_P];
+  a.r = c->r;
+  a.ap = c->ap;
+  a.lw = c->dLW;
+  a.le = c->dLE;
+  a.ls = c->dLS;
+  a.ln = c->dLN;
+  a.nwet = c->n_wet();
+  a.lc0 = c->LC - 1;
+  a.rowmajor = c->rowmajor ? 1 : 0;
+  a.st = c->st;
+  a.part = c->part;
+  return a;
+}
+
+int read_state(efdc_ctx *c) {
+  HIP_TRY(c, hipMemcpyAsync(c->st_host, c->st, sizeof(CgState),
+                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
+int cg_fp64(efdc_ctx *c) {
+  const int nb = efdc_grid_blocks(c->n_wet());
+  CgArgs a = cg_args(c);
+  const int iterm = c->prm.ITERM;
+  k_cg_init<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->prm.RSQM, c->prm.cg_rel_tol);
+  int it = 0;
+  int chunk = c->last_iters > 0 ? c->last_iters : 8;
+  for (;;) {
+    for (int j = 0; j < chunk && it < iterm; ++j, ++it) {
+      k_cg_a<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->pb[it & 1], c->pb[(it + 1) & 1]);
+      k_cg_b<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->pb[(it + 1) & 1], iterm);
+    }
+    int s = read_state(c);
+    if (s) return s;
+    if (c->st_host->done || it >= iterm) break;
+    chunk = 4;
+  }
+  c->last_iters = c->st_host->iter;
+  return EFDC_OK;
+}
+
+int cg_mixed(efdc_ctx *c) {
+  const int nb = efdc_grid_blocks(c->n_wet());
+  MxArgs m;
+  m.g = cg_args(c);
+  m.dis = c->dis;
+  m.as = c->as32;
+  m.aw = c->aw32;
+  m.r32 = c->r32;
+  m.y32 = c->y32;
+  m.ap32 = c->ap32;
+  const double rtol = c->prm.mixed_inner_rtol > 0 ? c->prm.mixed_inner_rtol : 1e-5;
+  const int cap = c->prm.ITERM;
+  k_mx_prep<<<nb, EFDC_BLOCK, 0, c->stream>>>(m, c->prm.RSQM, c->prm.cg_rel_tol);
+  int outer = 0, total = 0;
+  for (;;) {
+    k_mx_resid<<<nb, EFDC_BLOCK, 0, c->stream>>>(m);
+    int s = read_state(c);
+    if (s) return s;
+    if (c->st_host->rsq64 <= c->st_host->rsqm) break;
+    if (outer >= c->prm.mixed_max_outer) {
+      // report through the status word like the FP64 path
+      c->st_host->status = 1;
+      break;
+    }
+    int it = 0;
+    int chunk = c->last_iters > 0 ? c->last_iters : 8;
+    for (;;) {
+      for (int j = 0; j < chunk && it < cap; ++j, ++it) {
+        k_mx_a<<<nb, EFDC_BLOCK, 0, c->stream>>>(m, c->p32[it & 1], c->p32[(it + 1) & 1]);
+        k_mx_b<<<nb, EFDC_BLOCK, 0, c->stream>>>(m, c->p32[(it + 1) & 1], rtol * rtol, cap);
+      }
+      s = read_state(c);
+      if (s) return s;
+      if (c->st_host->done || it >= cap) break;
+      chunk = 4;
+    }
+    c->last_iters = c->st_host->inner_iter;
+    total += c->st_host->inner_iter;
+    k_mx_apply<<<nb, EFDC_BLOCK, 0, c->stream>>>(m);
+    ++outer;
+  }
+  c->st_host->iter = total;
+  c->st_host->outer = outer;
+  return EFDC_OK;
+}
+
+int cg_dispatch(efdc_ctx *c, efdc_cg_stats_t *stats) {
+  int s;
+  bool mixed = (c->prm.cg_mode == EFDC_CG_MIXED);
+  if (mixed) {
+    // the mixed operator is the compact symmetric one
+    if ((s = read_state(c))) return s;
+    if (!c->rowmajor || c->st_host->symok != 0) mixed = false;
+  }
+  s = mixed ? cg_mixed(c) : cg_fp64(c);
+  if (s) return s;
+  const CgState &h = *c->st_host;
+  if (stats) {
+    stats->iters = h.iter;
+    stats->outer = mixed ? h.outer : 1;
+    stats->rsq = mixed ? h.rsq64 : h.rsq;
+    stats->rsqm_used = h.rsqm;
+    stats->ccmnm = h.ccmnm;
+  }
+  if (h.status != 0) {
+    c->err = "CONGRAD: maximum iterations exceeded in external solution";
+    return EFDC_ERR_CG_MAXITER;
+  }
+  return EFDC_OK;
+}
+
+}  // namespace
+
+int cg_run(efdc_ctx *c, bool check_sym, efdc_cg_stats_t *stats) {
+  const int nb = efdc_grid_blocks(c->n_wet());
+  if (check_sym) {
+    k_reset_flags<<<1, 1, 0, c->stream>>>(c->st);
+    k_cg_check<<<nb, EFDC_BLOCK, 0, c->stream>>>(
+        c->dH[EFDC_CCS], c->dH[EFDC_CCW], c->dH[EFDC_CCE], c->dH[EFDC_CCN],
+        c->dLW, c->dLE, c->dLN, c->n_wet(), c->LC - 1, c->st);
+  }
+  return cg_dispatch(c, stats);
+}
+
+int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
+  const int nb = efdc_grid_blocks(c->n_wet());
+  const int nwet = c->n_wet(), lc0 = c->LC - 1;
+  const double dt = step->dt, G = c->prm.G;
+  HPtr h = hptr(c);
+  k_reset_flags<<<1, 1, 0, c->stream>>>(c->st);
+  if (c->store == EFDC_STORE_FP64)
+    k_puv_explicit<double><<<nb, EFDC_BLOCK, 0, c->stream>>>(
+        h, c->dLW, c->dLS, c->dLE, c->dLN, (const double *)c->dK[EFDC_HPK - EFDC_HPK],
+        (double *)c->dK[EFDC_H1PK - EFDC_HPK], c->pitch, c->KC, nwet, dt, G,
+        c->prm.RITB1, c->prm.SNLT);
+  else
+    k_puv_explicit<float><<<nb, EFDC_BLOCK, 0, c->stream>>>(
+        h, c->dLW, c->dLS, c->dLE, c->dLN, (const float *)c->dK[EFDC_HPK - EFDC_HPK],
+        (float *)c->dK[EFDC_H1PK - EFDC_HPK], c->pitch, c->KC, nwet, dt, G,
+        c->prm.RITB1, c->prm.SNLT);
+  k_puv_assemble<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->pbcflag,
+                                                   nwet, dt, G, c->part, c->st);
+  if (c->npbc_total > 0) {
+    // tidal phase on the host in FP64 like SETOPENBC (:22-32)
+    const double PI2 = 2. * 3.1415926535898;  // EFDC/aaefdc.f90:650-651
+    const double TN = dt * (double)step->n + c->prm.TCON_TBEGIN;
+    for (int M = 0; M < c->mtide; ++M) {
+      double TM = std::fmod(TN, c->tcp_host[M]);
+      TM = PI2 * TM / c->tcp_host[M];
+      c->tide_host[M] = std::cos(TM);
+      c->tide_host[c->mtide + M] = std::sin(TM);
+    }
+    if (c->mtide > 0)
+      HIP_TRY(c, hipMemcpyAsync(c->tide_dev, c->tide_host,
+                                sizeof(double) * 2 * c->mtide,
+                                hipMemcpyHostToDevice, c->stream));
+    PbcArgs b;
+    for (int s = 0; s < 4; ++s) b.side[s] = c->pbc[s];
+    b.tide = c->tide_dev;
+    b.mtide = c->mtide;
+    b.lopenbcdry = c->lopenbcdry;
+    k_puv_bc<<<1, EFDC_BLOCK, 0, c->stream>>>(h, b, c->dLW, c->dLE, c->dLS, c->dLN,
+                                              dt, G, c->prm.HDRY, c->st);
+  }
+  k_puv_scale<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLE, c->dLN, nwet,
+                                                lc0, c->st);
+  int s = cg_dispatch(c, stats);
+  if (s != EFDC_OK && s != EFDC_ERR_CG_MAXITER) return s;
+  const int cgs = s;
+  k_puv_post1<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, dt);
+  k_puv_post2<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->pbcflag,
+                                                nwet, dt, G);
+  k_puv_post3<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, c->st);
+  if ((s = read_state(c))) return s;
+  if (stats) stats->ccmnm = c->st_host->ccmnm;
+  if (cgs != EFDC_OK) return cgs;
+  if (c->st_host->negdep) {
+    c->err = "NEGDEP: negative depth after the external solution";
+    return EFDC_ERR_NEGDEPTH;
+  }
+  return EFDC_OK;
+}

@@ -1,0 +1,484 @@
+"""Synthetic EFDC+ cases in the var_global_mod array layout.
+
+Every array is a numpy FP64 (or int32 for maps) array in the layout the
+Fortran host hands over through the C-ABI:
+
+* horizontal arrays have length LCM, element L (1-based) at index L-1;
+* layer arrays are numpy (KCM, LCM) C-order == Fortran (LCM,KCM), so
+  element (L,K) is ``a[K-1, L-1]``;
+* W2 / DZIG are (KCM+1, LCM) == Fortran (LCM,0:KCM), element (L,K) is
+  ``a[K, L-1]``.
+
+Cell numbering follows CELLMAP (EFDC/cellmap.f90:12-26): wet cells are
+numbered L=2..LA row by row (J outer, I inner); L=1 and LC=LA+1 are ghost
+cells and land neighbours map to LC (EFDC/cellmap.f90:83-148).  As in
+SCANINP, LCM = LC+1 and KCM = KC+1 (EFDC/scaninpmod.f90:71,80).
+
+Geometry follows EFDC/aaefdc.f90:2172-2185 (uniform dx=dy here) and the
+layer constants EFDC/aaefdc.f90:1212-1232 (DZIC, DZIG, CDZKK, CDZKMK).
+
+The closure that stands in for the internal mode (which produces the
+transport inputs UHDY2/VHDX2/W2/HPK upstream of CALCONC and lies outside
+the accelerated path) is the depth-uniform one: UHDY2 = 0.5*(UHDYE+UHDY1E)
+*DZC, W2 = 0, HPK = HP*DZC, and Manning bottom drag
+TBX = g n^2 u|u| / HU^(1/3) on the external velocity.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Dict, List, Optional
+
+import numpy as np
+
+G = 9.81
+H_KEYS = (
+    "DXYP DXYIP DXYU DXYV DXIU DYIU DXIV DYIV HRUO HRVO SUB SVB BELV "
+    "P P1 HP H1P H2P HU HV HUI HVI HPI H1U H1V H1UI H1VI "
+    "UHDYE VHDXE UHDY1E VHDX1E UHE VHE "
+    "TSX TSY TBX TBY FCAXE FCAYE FPGXE FPGYE FXE FYE QSUME RCX RCY "
+    "FUHDYE FVHDXE FP FP1 CC CS CW CE CN CCC CCS CCW CCE CCN CCCI FPTMP"
+).split()
+K_KEYS = ("HPK H1PK HPKI UHDY2 VHDX2 CDZKK CDZKMK AB QSUM U2 V2 SUB3D SVB3D "
+          "DZC").split()
+W_KEYS = ("W2", "DZIG")
+
+
+@dataclasses.dataclass
+class Grid:
+    """Cell numbering and neighbour maps of a masked rectangular grid."""
+
+    IC: int
+    JC: int
+    LA: int
+    KC: int
+    IL: np.ndarray
+    JL: np.ndarray
+    LIJ: np.ndarray
+    maps: Dict[str, np.ndarray]
+
+    @property
+    def LC(self) -> int:
+        return self.LA + 1
+
+    @property
+    def LCM(self) -> int:
+        return self.LA + 2
+
+    @property
+    def KCM(self) -> int:
+        return self.KC + 1
+
+    @property
+    def nwet(self) -> int:
+        return self.LA - 1
+
+
+def build_grid(wet: np.ndarray, kc: int) -> Grid:
+    """Number wet cells like CELLMAP and build LWC/LEC/LSC/LNC/LNWC/LSEC.
+
+    ``wet`` is a (JC, IC) boolean array (J rows, I columns, 0-based); its
+    outer ring must be land.  Follows EFDC/cellmap.f90:12-26 (ordering) and
+    EFDC/cellmap.f90:83-148 (maps; LC for land, ghosts map to LC / 1).
+    """
+    wet = np.asarray(wet, dtype=bool)
+    JC, IC = wet.shape
+    if wet[0].any() or wet[-1].any() or wet[:, 0].any() or wet[:, -1].any():
+        raise ValueError("the outer ring of the mask must be land")
+    jj, ii = np.nonzero(wet)  # row-major: J outer, I inner
+    nwet = len(jj)
+    LA = nwet + 1
+    LC = LA + 1
+    LCM = LC + 1
+    LIJ = np.zeros((JC, IC), dtype=np.int32)
+    LIJ[jj, ii] = np.arange(2, LA + 1, dtype=np.int32)
+    IL = np.zeros(LCM, dtype=np.int32)
+    JL = np.zeros(LCM, dtype=np.int32)
+    IL[1:LA] = ii + 1  # 1-based I, J
+    JL[1:LA] = jj + 1
+
+    def nb(dj, di):
+        v = LIJ[jj + dj, ii + di]
+        return np.where(v > 0, v, LC).astype(np.int32)
+
+    maps = {}
+    for name, (dj, di) in dict(LNC=(1, 0), LSC=(-1, 0), LEC=(0, 1), LWC=(0, -1),
+                               LNWC=(1, -1), LSEC=(-1, 1), LNEC=(1, 1),
+                               LSWC=(-1, -1)).items():
+        m = np.zeros(LCM, dtype=np.int32)
+        m[0] = LC       # L=1 -> LC
+        m[1:LA] = nb(dj, di)
+        m[LA] = 1       # L=LC -> 1
+        m[LC] = LC      # padding element of LCM
+        maps[name] = m
+    return Grid(IC=IC, JC=JC, LA=LA, KC=kc, IL=IL, JL=JL, LIJ=LIJ, maps=maps)
+
+
+@dataclasses.dataclass
+class Case:
+    """One EFDC+ case: grid, var_global arrays, boundary tables, scalars."""
+
+    name: str
+    grid: Grid
+    h: Dict[str, np.ndarray]
+    k: Dict[str, np.ndarray]
+    w: Dict[str, np.ndarray]
+    consts: List[dict]                 # transported constituents
+    open_bc: Optional[dict] = None     # pressure BCs
+    conc_bc: Optional[dict] = None     # concentration BCs
+    sources: Optional[dict] = None     # flow sources
+    dt: float = 5.0
+    manning_n: Optional[np.ndarray] = None  # per-cell Manning n (closure)
+    meta: dict = dataclasses.field(default_factory=dict)
+
+    def copy(self) -> "Case":
+        def cp(d):
+            return {a: b.copy() for a, b in d.items()}
+        c = dataclasses.replace(self, h=cp(self.h), k=cp(self.k), w=cp(self.w))
+        c.consts = [dict(x, con=x["con"].copy(), con1=x["con1"].copy())
+                    for x in self.consts]
+        c.meta = dict(self.meta)
+        return c
+
+
+# --------------------------------------------------------------- helpers
+
+def _smooth_noise(rng, shape, sigma):
+    from scipy.ndimage import gaussian_filter
+    z = rng.uniform(0.0, 1.0, size=shape)
+    return gaussian_filter(z, sigma=sigma, mode="reflect")
+
+
+def _rescale(z, lo, hi):
+    zmin, zmax = float(z.min()), float(z.max())
+    if zmax - zmin < 1e-300:
+        return np.full_like(z, 0.5 * (lo + hi))
+    return lo + (hi - lo) * (z - zmin) / (zmax - zmin)
+
+
+def _empty_arrays(g: Grid):
+    LCM, KCM = g.LCM, g.KCM
+    h = {key: np.zeros(LCM) for key in H_KEYS}
+    k = {key: np.zeros((KCM, LCM)) for key in K_KEYS}
+    w = {key: np.zeros((KCM + 1, LCM)) for key in W_KEYS}
+    return h, k, w
+
+
+def _face_masks(g: Grid):
+    """SUB/SVB: 1 on faces between two wet cells (EFDC SUBO/SVBO)."""
+    LA, LC = g.LA, g.LC
+    sub = np.zeros(g.LCM)
+    svb = np.zeros(g.LCM)
+    L = np.arange(2, LA + 1)
+    sub[L - 1] = (g.maps["LWC"][L - 1] != LC).astype(float)
+    svb[L - 1] = (g.maps["LSC"][L - 1] != LC).astype(float)
+    return sub, svb
+
+
+def fill_geometry(g: Grid, h, k, w, dx: float, dzck: np.ndarray):
+    """Metrics (EFDC/aaefdc.f90:2172-2185) and layer constants
+    (EFDC/aaefdc.f90:1212-1232) for a uniform dx=dy grid with sigma layers
+    DZC(L,K)=DZCK(K)/sum(DZCK)."""
+    LA, KC = g.LA, g.KC
+    L = np.arange(2, LA + 1) - 1
+    sub, svb = _face_masks(g)
+    h["SUB"][:] = sub
+    h["SVB"][:] = svb
+    for key, val in dict(DXYP=dx * dx, DXYIP=1.0 / (dx * dx), DXYU=dx * dx,
+                         DXYV=dx * dx, DXIU=1.0 / dx, DYIU=1.0 / dx,
+                         DXIV=1.0 / dx, DYIV=1.0 / dx).items():
+        h[key][L] = val
+    h["HRUO"][L] = sub[L] * dx * (1.0 / dx)
+    h["HRVO"][L] = svb[L] * dx * (1.0 / dx)
+    h["RCX"][:] = 1.0
+    h["RCY"][:] = 1.0
+    h["RCX"][0] = h["RCY"][0] = 0.0
+    h["RCX"][g.LC - 1] = h["RCY"][g.LC - 1] = 0.0
+    dzc = np.asarray(dzck, float) / float(np.sum(dzck))
+    for K in range(1, KC + 1):
+        k["DZC"][K - 1, L] = dzc[K - 1]
+    dzic = 1.0 / dzc
+    dzig = np.zeros(KC + 1)
+    for K in range(1, KC):
+        dzig[K] = 1.0 / (0.5 * (dzc[K - 1] + dzc[K]))
+    for K in range(0, KC + 1):
+        w["DZIG"][K, L] = dzig[K]
+    for K in range(2, KC + 1):
+        k["CDZKMK"][K - 1, L] = dzig[K - 1] * dzic[K - 1]
+    for K in range(1, KC):
+        k["CDZKK"][K - 1, L] = dzic[K - 1] * dzig[K]
+    # 3D face masks (sigma grid: every layer open where the 2D face is)
+    for K in range(1, KC + 1):
+        k["SUB3D"][K - 1, :] = sub
+        k["SVB3D"][K - 1, :] = svb
+
+
+def update_depth_arrays(g: Grid, h):
+    """HU/HV and inverses from HP (EFDC/calpuv2c.f90:901-913)."""
+    LA = g.LA
+    L = np.arange(2, LA + 1) - 1
+    LW = g.maps["LWC"][L] - 1
+    LS = g.maps["LSC"][L] - 1
+    dxyp, hp = h["DXYP"], h["HP"]
+    h["HU"][L] = (dxyp[L] * hp[L] + dxyp[LW] * hp[LW]) * (0.5 / h["DXYU"][L])
+    h["HV"][L] = (dxyp[L] * hp[L] + dxyp[LS] * hp[LS]) * (0.5 / h["DXYV"][L])
+    h["HPI"][L] = 1.0 / hp[L]
+    h["HUI"][L] = 1.0 / h["HU"][L]
+    h["HVI"][L] = 1.0 / h["HV"][L]
+
+
+def layer_depths(g: Grid, h, k):
+    """HPK = HP*DZC, HPKI = 1/HPK on wet cells (EFDC/caluvw.f90:1163-1172
+    form for a sigma grid)."""
+    L = np.arange(2, g.LA + 1) - 1
+    for K in range(g.KC):
+        k["HPK"][K, L] = h["HP"][L] * k["DZC"][K, L]
+        k["HPKI"][K, L] = 1.0 / k["HPK"][K, L]
+
+
+def largest_component(mask: np.ndarray) -> np.ndarray:
+    from scipy.ndimage import label
+    lab, n = label(mask)  # 4-connectivity
+    if n == 0:
+        return mask
+    counts = np.bincount(lab.ravel())
+    counts[0] = 0
+    return lab == int(np.argmax(counts))
+
+
+def _pad_ring(a: np.ndarray, value=False):
+    return np.pad(a, 1, mode="constant", constant_values=value)
+
+
+# ------------------------------------------------------------ constructors
+
+def lake_mask(nx: int, ny: int, seed: int, wet_fraction=0.67, sigma=16.0):
+    """Irregular lake: thresholded smoothed noise, largest 4-connected
+    component, padded with a land ring.  Returns (JC, IC) bool."""
+    rng = np.random.default_rng(seed)
+    z = _smooth_noise(rng, (ny, nx), sigma)
+    thr = np.quantile(z, 1.0 - wet_fraction)
+    m = z > thr
+    m[0, :] = m[-1, :] = m[:, 0] = m[:, -1] = False
+    m = largest_component(m)
+    return _pad_ring(m)
+
+
+def external_case(wet: np.ndarray, *, seed: int, dx=100.0, dt=10.0, kc=1,
+                  depth=(2.0, 40.0), depth_sigma=4.0, eta_sigma=0.05,
+                  eta_smooth=2.0, name="external") -> Case:
+    """Config-1 style external-mode case: irregular lake, random depth, a
+    random free-surface perturbation, zero transports and forcings.  One
+    CALPUV2C call assembles the SPD system exactly as
+    EFDC/calpuv2c.f90:164-398 does and solves it."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    g = build_grid(wet, kc)
+    h, k, w = _empty_arrays(g)
+    fill_geometry(g, h, k, w, dx, np.ones(kc))
+    JC, IC = wet.shape
+    dep = _rescale(_smooth_noise(rng, (JC, IC), depth_sigma), *depth)
+    eta = gaussian_filter(rng.normal(0.0, 1.0, size=(JC, IC)), eta_smooth)
+    eta *= eta_sigma / max(float(eta[wet].std()), 1e-30)
+    L = np.arange(2, g.LA + 1) - 1
+    jj, ii = g.JL[L] - 1, g.IL[L] - 1
+    h["BELV"][L] = -dep[jj, ii]
+    h["HP"][L] = dep[jj, ii] + eta[jj, ii]
+    h["H1P"][:] = h["HP"]
+    h["P"][L] = G * (h["HP"][L] + h["BELV"][L])
+    update_depth_arrays(g, h)
+    layer_depths(g, h, k)
+    k["H1PK"][:] = k["HPK"]
+    return Case(name=name, grid=g, h=h, k=k, w=w, consts=[], dt=dt,
+                meta=dict(seed=seed, dx=dx))
+
+
+def config1(seed=1001, n=1024, wet_fraction=0.67) -> Case:
+    """External-mode solve at scale: n x n lattice, irregular lake mask
+    (~0.67 n^2 wet cells), depths 2-40 m, dt=10 s, eta' sigma=0.05 m."""
+    wet = lake_mask(n, n, seed, wet_fraction=wet_fraction)
+    return external_case(wet, seed=seed, dt=10.0, name=f"cfg1-{n}")
+
+
+def _constituent(g: Grid, mo: int, con: np.ndarray, **kw) -> dict:
+    d = dict(mo=mo, isadac=0, isfct=0, rkdye=0.0, is_dye=0,
+             con=con, con1=con.copy())
+    d.update(kw)
+    return d
+
+
+def tidal_basin(nx=128, ny=64, kc=5, *, seed=1000, dx=100.0, dt=5.0,
+                amp=0.5, period=44712.0, manning=0.025, ab=1e-4,
+                dye_load=1.0, depth=(5.0, 20.0), depth_sigma=8.0,
+                ntscr=12, isadac=0, isfct=0, name=None) -> Case:
+    """Config-0 style case: rectangular basin nx*ny, west column is a
+    clamped open boundary (ISPBW=0) with an M2-like tide, other sides
+    closed.  Salinity 0->30 psu W->E, a dye point source (QSS=1e-3 m^3/s
+    split over layers by DZC, CQS = dye_load/1e-3) at the centre cell."""
+    rng = np.random.default_rng(seed)
+    wet = _pad_ring(np.ones((ny, nx), dtype=bool))
+    g = build_grid(wet, kc)
+    h, k, w = _empty_arrays(g)
+    fill_geometry(g, h, k, w, dx, np.ones(kc))
+    dep_i = _rescale(_smooth_noise(rng, (ny, nx), depth_sigma), *depth)
+    dep = _pad_ring(dep_i, 0.0)
+    L = np.arange(2, g.LA + 1) - 1
+    jj, ii = g.JL[L] - 1, g.IL[L] - 1
+    h["BELV"][L] = -dep[jj, ii]
+    h["HP"][L] = dep[jj, ii]
+    h["H1P"][:] = h["HP"]
+    h["H2P"][:] = h["HP"]
+    update_depth_arrays(g, h)
+    layer_depths(g, h, k)
+    k["H1PK"][:] = k["HPK"]
+    k["AB"][:kc - 1, L] = ab if kc > 1 else 0.0
+    # west open boundary: first wet column (I index 1 in 0-based padded grid)
+    west = np.nonzero(ii == 1)[0]
+    lpbw = (L[west] + 1).astype(np.int32)
+    nb = len(lpbw)
+    open_bc = dict(side=[dict(L=lpbw, ispb=np.zeros(nb, np.int32),
+                              pser=np.zeros(nb), pcb=np.full(nb, G * amp),
+                              psb=np.zeros(nb))] + [None] * 3,
+                   mtide=1, tcp=np.array([period]))
+    # constituents: salinity gradient, dye zero
+    sal = np.zeros((g.KCM, g.LCM))
+    for K in range(kc):
+        sal[K, L] = 30.0 * (ii - 1) / max(nx - 1, 1)
+    dye = np.zeros((g.KCM, g.LCM))
+    consts = [_constituent(g, 1, sal, isadac=isadac, isfct=isfct),
+              _constituent(g, 3, dye, is_dye=1, isadac=isadac, isfct=isfct)]
+    nc = len(consts)
+    # concentration BC on the same west cells: inflow value = west salinity
+    cb = np.zeros((nb, 2, nc))
+    wtci = np.zeros((kc, 2))
+    for K in range(kc):
+        f = (K) / max(kc - 1, 1)
+        wtci[K] = (1.0 - f, f)
+    conc_bc = dict(side=[None, dict(L=lpbw, cb=cb, csert=np.zeros((nb, kc, nc)),
+                                    ntscr=np.full(nb, ntscr, np.int32)),
+                         None, None], wtci=wtci)
+    # dye point source at the centre cell (I=nx/2, J=ny/2, 1-based interior)
+    lsrc = int(g.LIJ[ny // 2, nx // 2])
+    q = 1e-3
+    qss = np.zeros((1, kc))
+    qss[0] = q * k["DZC"][:kc, lsrc - 1]
+    cqs = np.zeros((1, kc, nc))
+    cqs[0, :, 1] = dye_load / q
+    cqs[0, :, 0] = sal[:kc, lsrc - 1]
+    sources = dict(LQS=np.array([lsrc], np.int32), qss=qss,
+                   qsercell=np.zeros((1, kc)), cqs=cqs,
+                   csert=np.zeros((1, kc, nc)))
+    h["QSUME"][lsrc - 1] = q
+    for K in range(kc):
+        k["QSUM"][K, lsrc - 1] = qss[0, K]
+    nman = np.full(g.LCM, manning)
+    return Case(name=name or f"basin-{nx}x{ny}x{kc}", grid=g, h=h, k=k, w=w,
+                consts=consts, open_bc=open_bc, conc_bc=conc_bc,
+                sources=sources, dt=dt, manning_n=nman,
+                meta=dict(seed=seed, dx=dx))
+
+
+def config0(**kw) -> Case:
+    return tidal_basin(128, 64, 5, seed=kw.pop("seed", 1000), name="cfg0", **kw)
+
+
+def transport_case(nx=2048, ny=1024, kc=20, *, seed=1002, dx=100.0, dt=5.0,
+                   depth=10.0, umax=1.0, isadac=0, isfct=0,
+                   name=None, with_w=True) -> Case:
+    """Config-2 style transport case: all cells wet, uniform depth, a
+    divergence-free layer flow from a streamfunction plus a vertically
+    alternating potential flow whose column sum vanishes (non-zero W2 from
+    continuity, EFDC/caluvw.f90:668-680), log-uniform AB, 4 constituents."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    wet = _pad_ring(np.ones((ny, nx), dtype=bool))
+    g = build_grid(wet, kc)
+    h, k, w = _empty_arrays(g)
+    fill_geometry(g, h, k, w, dx, np.ones(kc))
+    L = np.arange(2, g.LA + 1) - 1
+    jj, ii = g.JL[L] - 1, g.IL[L] - 1
+    h["BELV"][L] = -depth
+    h["HP"][L] = depth
+    h["H1P"][:] = h["HP"]
+    update_depth_arrays(g, h)
+    layer_depths(g, h, k)
+    k["H1PK"][:] = k["HPK"]
+    # streamfunction on cell corners (ny+1, nx+1), zero on the boundary
+    psi = gaussian_filter(rng.normal(size=(ny + 1, nx + 1)), 24.0)
+    taper = np.outer(np.sin(np.linspace(0, np.pi, ny + 1)),
+                     np.sin(np.linspace(0, np.pi, nx + 1)))
+    psi *= taper
+    # U face of cell (j,i): psi(j+1,i) - psi(j,i) (corner (i-1/2, j+-1/2))
+    uf = psi[1:, :-1] - psi[:-1, :-1]      # (ny, nx) west faces
+    vf = -(psi[:-1, 1:] - psi[:-1, :-1])   # (ny, nx) south faces
+    uf[:, 0] = 0.0
+    vf[0, :] = 0.0
+    phi = gaussian_filter(rng.normal(size=(ny, nx)), 16.0)
+    gu = np.zeros((ny, nx))
+    gv = np.zeros((ny, nx))
+    gu[:, 1:] = phi[:, 1:] - phi[:, :-1]
+    gv[1:, :] = phi[1:, :] - phi[:-1, :]
+    dzc = 1.0 / kc
+    hu = depth
+    smax = max(np.abs(uf).max(), np.abs(vf).max(), 1e-30)
+    scale = umax * dx * hu * dzc / smax    # max layer velocity ~ umax
+    gmax = max(np.abs(gu).max(), np.abs(gv).max(), 1e-30)
+    gscale = 0.5 * umax * dx * hu * dzc / gmax if with_w else 0.0
+    ipad = ii - 1
+    jpad = jj - 1
+    for K in range(kc):
+        b = math.cos(math.pi * (K + 0.5) / kc)
+        k["UHDY2"][K, L] = scale * uf[jpad, ipad] + gscale * b * gu[jpad, ipad]
+        k["VHDX2"][K, L] = scale * vf[jpad, ipad] + gscale * b * gv[jpad, ipad]
+    # W2 from layer continuity, W2(L,0)=0, W2(L,KC)=0
+    LE = g.maps["LEC"][L] - 1
+    LN = g.maps["LNC"][L] - 1
+    for K in range(1, kc):
+        uk, vk = k["UHDY2"][K - 1], k["VHDX2"][K - 1]
+        div = (uk[LE] - uk[L] + vk[LN] - vk[L]) * h["DXYIP"][L]
+        w["W2"][K, L] = w["W2"][K - 1, L] - div
+    w["W2"][kc, L] = 0.0
+    lo, hi = math.log(1e-5), math.log(1e-2)
+    k["AB"][:kc - 1, L] = np.exp(rng.uniform(lo, hi, size=(kc - 1, len(L))))
+    k["U2"][:kc, L] = k["UHDY2"][:kc, L] / (dx * hu * dzc)
+    k["V2"][:kc, L] = k["VHDX2"][:kc, L] / (dx * hu * dzc)
+
+    def field(lo_, hi_, sig):
+        z = _rescale(_smooth_noise(rng, (ny, nx), sig), lo_, hi_)
+        a = np.zeros((g.KCM, g.LCM))
+        for K in range(kc):
+            a[K, L] = z[jpad, ipad] * (1.0 + 0.01 * K)
+        return a
+
+    sal = field(0.0, 35.0, 32.0)
+    tem = field(4.0, 25.0, 32.0)
+    dye = np.zeros((g.KCM, g.LCM))
+    sed = np.zeros((g.KCM, g.LCM))
+    sed[:kc, L] = 0.1
+    consts = [_constituent(g, 1, sal, isadac=isadac, isfct=isfct),
+              _constituent(g, 2, tem, isadac=isadac, isfct=isfct),
+              _constituent(g, 3, dye, is_dye=1, isadac=isadac, isfct=isfct),
+              _constituent(g, 4, sed, isadac=isadac, isfct=isfct)]
+    nc = len(consts)
+    lsrc = int(g.LIJ[ny // 2 + 1, nx // 2 + 1])
+    qss = np.zeros((1, kc))
+    cqs = np.zeros((1, kc, nc))
+    qsercell = np.zeros((1, kc))
+    cqs[0, :, 2] = 1000.0
+    qss[0] = 1e-3 / kc
+    for c in (0, 1, 3):
+        cqs[0, :, c] = consts[c]["con"][:kc, lsrc - 1]
+    sources = dict(LQS=np.array([lsrc], np.int32), qss=qss, qsercell=qsercell,
+                   cqs=cqs, csert=np.zeros((1, kc, nc)))
+    for K in range(kc):
+        k["QSUM"][K, lsrc - 1] = qss[0, K]
+    h["QSUME"][lsrc - 1] = qss.sum()
+    # sources add volume: keep HPK consistent only where no source acts;
+    # the source is tiny (1e-3 m^3/s) and the tests account for it.
+    return Case(name=name or f"transport-{nx}x{ny}x{kc}", grid=g, h=h, k=k,
+                w=w, consts=consts, sources=sources, dt=dt,
+                meta=dict(seed=seed, dx=dx))
+
+
+def config2(**kw) -> Case:
+    return transport_case(2048, 1024, 20, name="cfg2", **kw)

@@ -1,0 +1,362 @@
+"""ctypes binding of libefdc_hip.so -- the Python side of the C-ABI.
+
+Mirrors the reference's hot-path calls: ``Context.calpuv2c`` (CALPUV2C,
+EFDC/calpuv2c.f90), ``Context.congrad`` (CONGRAD, EFDC/congrad.f90) and
+``Context.calconc`` (CALCONC/CALTRAN, EFDC/calconc.f90, EFDC/caltran.f90).
+There is no CPU fallback: without the built library or without a GPU the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from .abi import ARRAY_IDS, STATUS_NAMES, shape_class
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libefdc_hip.so")
+
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int32)
+
+CG_FP64, CG_MIXED = 0, 1
+STORE_FP64, STORE_FP32 = 0, 1
+
+
+class EfdcError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Grid(C.Structure):
+    _fields_ = [("LA", C.c_int32), ("LCM", C.c_int32), ("KC", C.c_int32),
+                ("KCM", C.c_int32), ("LWC", PI), ("LEC", PI), ("LSC", PI),
+                ("LNC", PI), ("LNWC", PI), ("LSEC", PI)]
+
+
+class Params(C.Structure):
+    _fields_ = [("G", C.c_double), ("RITB1", C.c_double), ("SNLT", C.c_double),
+                ("HDRY", C.c_double), ("TCON_TBEGIN", C.c_double),
+                ("ITERM", C.c_int32), ("RSQM", C.c_double),
+                ("cg_rel_tol", C.c_double), ("cg_mode", C.c_int32),
+                ("store", C.c_int32), ("mixed_inner_rtol", C.c_double),
+                ("mixed_max_outer", C.c_int32)]
+
+
+class PbcSide(C.Structure):
+    _fields_ = [("n", C.c_int32), ("L", PI), ("ispb", PI), ("pser", PD),
+                ("pcb", PD), ("psb", PD)]
+
+
+class OpenBC(C.Structure):
+    _fields_ = [("side", PbcSide * 4), ("mtide", C.c_int32), ("tcp", PD)]
+
+
+class CbcSide(C.Structure):
+    _fields_ = [("n", C.c_int32), ("L", PI), ("cb", PD), ("csert", PD),
+                ("ntscr", PI)]
+
+
+class ConcBC(C.Structure):
+    _fields_ = [("side", CbcSide * 4), ("wtci", PD)]
+
+
+class Sources(C.Structure):
+    _fields_ = [("n", C.c_int32), ("LQS", PI), ("qss", PD), ("qsercell", PD),
+                ("cqs", PD), ("csert", PD)]
+
+
+class ConstDesc(C.Structure):
+    _fields_ = [("mo", C.c_int32), ("isadac", C.c_int32), ("isfct", C.c_int32),
+                ("rkdye", C.c_double), ("is_dye", C.c_int32)]
+
+
+class Step(C.Structure):
+    _fields_ = [("dt", C.c_double), ("n", C.c_int64), ("niter", C.c_int64)]
+
+
+class CgStats(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("outer", C.c_int32), ("rsq", C.c_double),
+                ("rsqm_used", C.c_double), ("ccmnm", C.c_double)]
+
+
+class Balance(C.Structure):
+    _fields_ = [("volume", C.c_double), ("mass", C.c_double * 16),
+                ("nconst", C.c_int32)]
+
+
+_lib = None
+
+
+def load(path: Optional[str] = None):
+    """Load the library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it with __graft_entry__.build() "
+            "(there is no CPU fallback)")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    sig = {
+        "efdc_abi_version": ([], C.c_int),
+        "efdc_params_default": ([C.POINTER(Params)], None),
+        "efdc_ctx_create": ([C.POINTER(Grid), C.POINTER(Params), C.POINTER(P)], C.c_int),
+        "efdc_ctx_destroy": ([P], C.c_int),
+        "efdc_last_error": ([P], C.c_char_p),
+        "efdc_set_params": ([P, C.POINTER(Params)], C.c_int),
+        "efdc_set_array": ([P, C.c_int32, PD], C.c_int),
+        "efdc_get_array": ([P, C.c_int32, PD], C.c_int),
+        "efdc_set_open_bc": ([P, C.POINTER(OpenBC)], C.c_int),
+        "efdc_set_conc_bc": ([P, C.POINTER(ConcBC)], C.c_int),
+        "efdc_set_sources": ([P, C.POINTER(Sources)], C.c_int),
+        "efdc_add_constituent": ([P, C.POINTER(ConstDesc), PI], C.c_int),
+        "efdc_set_constituent": ([P, C.c_int32, PD, PD], C.c_int),
+        "efdc_get_constituent": ([P, C.c_int32, PD, PD], C.c_int),
+        "efdc_congrad": ([P, C.POINTER(CgStats)], C.c_int),
+        "efdc_calpuv2c": ([P, C.POINTER(Step), C.POINTER(CgStats)], C.c_int),
+        "efdc_calconc": ([P, C.POINTER(Step)], C.c_int),
+        "efdc_balance": ([P, C.POINTER(Balance)], C.c_int),
+        "efdc_synchronize": ([P], C.c_int),
+        "efdc_device_ptr": ([P, C.c_int32, C.POINTER(P), C.POINTER(C.c_int64)], C.c_int),
+        "efdc_closure_set_manning": ([P, PD], C.c_int),
+        "efdc_closure_pre": ([P], C.c_int),
+        "efdc_closure_post": ([P], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = lib
+    return lib
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    load().efdc_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _pd(a: np.ndarray):
+    return a.ctypes.data_as(PD)
+
+
+def _pi(a: np.ndarray):
+    return a.ctypes.data_as(PI)
+
+
+class Context:
+    """One device context holding a case's arrays in HBM."""
+
+    def __init__(self, grid, params: Optional[Params] = None, **kw):
+        self.lib = load()
+        self.grid = grid
+        self._keep = []
+        g = Grid()
+        g.LA, g.LCM, g.KC, g.KCM = grid.LA, grid.LCM, grid.KC, grid.KCM
+        for m in ("LWC", "LEC", "LSC", "LNC", "LNWC", "LSEC"):
+            a = np.ascontiguousarray(grid.maps[m], np.int32)
+            self._keep.append(a)
+            setattr(g, m, _pi(a))
+        self.params = params if params is not None else default_params(**kw)
+        h = C.c_void_p()
+        self._check(self.lib.efdc_ctx_create(C.byref(g), C.byref(self.params),
+                                             C.byref(h)), ctx=None)
+        self.h = h
+        self.nconst = 0
+        self.has_manning = False
+
+    # ----------------------------------------------------------- errors
+    def _check(self, st: int, ctx="self"):
+        if st != 0:
+            msg = ""
+            if ctx == "self" and getattr(self, "h", None):
+                msg = (self.lib.efdc_last_error(self.h) or b"").decode()
+            raise EfdcError(st, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.efdc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ----------------------------------------------------------- arrays
+    def set(self, name: str, arr: np.ndarray):
+        a = np.ascontiguousarray(arr, np.float64)
+        self._check(self.lib.efdc_set_array(self.h, ARRAY_IDS[name], _pd(a)))
+
+    def get(self, name: str) -> np.ndarray:
+        g = self.grid
+        cls = shape_class(name)
+        shape = {"h": (g.LCM,), "k": (g.KCM, g.LCM), "w": (g.KCM + 1, g.LCM)}[cls]
+        out = np.zeros(shape)
+        self._check(self.lib.efdc_get_array(self.h, ARRAY_IDS[name], _pd(out)))
+        return out
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        self._check(self.lib.efdc_set_params(self.h, C.byref(self.params)))
+
+    # ----------------------------------------------------------- tables
+    def set_open_bc(self, obc: dict):
+        o = OpenBC()
+        mt = int(obc["mtide"])
+        o.mtide = mt
+        tcp = np.ascontiguousarray(obc["tcp"], np.float64)
+        self._keep.append(tcp)
+        o.tcp = _pd(tcp)
+        for s, side in enumerate(obc["side"]):
+            if side is None:
+                continue
+            arrs = [np.ascontiguousarray(side["L"], np.int32),
+                    np.ascontiguousarray(side["ispb"], np.int32),
+                    np.ascontiguousarray(side["pser"], np.float64),
+                    np.ascontiguousarray(side["pcb"], np.float64).reshape(-1),
+                    np.ascontiguousarray(side["psb"], np.float64).reshape(-1)]
+            self._keep.append(arrs)
+            o.side[s].n = len(arrs[0])
+            o.side[s].L, o.side[s].ispb = _pi(arrs[0]), _pi(arrs[1])
+            o.side[s].pser, o.side[s].pcb, o.side[s].psb = (
+                _pd(arrs[2]), _pd(arrs[3]), _pd(arrs[4]))
+        self._check(self.lib.efdc_set_open_bc(self.h, C.byref(o)))
+
+    def set_conc_bc(self, cbc: dict):
+        o = ConcBC()
+        w = np.ascontiguousarray(cbc["wtci"], np.float64).reshape(-1)
+        self._keep.append(w)
+        o.wtci = _pd(w)
+        for s, side in enumerate(cbc["side"]):
+            if side is None:
+                continue
+            arrs = [np.ascontiguousarray(side["L"], np.int32),
+                    np.ascontiguousarray(side["cb"], np.float64).reshape(-1),
+                    np.ascontiguousarray(side["csert"], np.float64).reshape(-1),
+                    np.ascontiguousarray(side["ntscr"], np.int32)]
+            self._keep.append(arrs)
+            o.side[s].n = len(arrs[0])
+            o.side[s].L, o.side[s].cb = _pi(arrs[0]), _pd(arrs[1])
+            o.side[s].csert, o.side[s].ntscr = _pd(arrs[2]), _pi(arrs[3])
+        self._check(self.lib.efdc_set_conc_bc(self.h, C.byref(o)))
+
+    def set_sources(self, src: dict):
+        o = Sources()
+        arrs = [np.ascontiguousarray(src["LQS"], np.int32),
+                np.ascontiguousarray(src["qss"], np.float64).reshape(-1),
+                np.ascontiguousarray(src["qsercell"], np.float64).reshape(-1),
+                np.ascontiguousarray(src["cqs"], np.float64).reshape(-1),
+                np.ascontiguousarray(src["csert"], np.float64).reshape(-1)]
+        self._keep.append(arrs)
+        o.n = len(arrs[0])
+        o.LQS, o.qss, o.qsercell = _pi(arrs[0]), _pd(arrs[1]), _pd(arrs[2])
+        o.cqs, o.csert = _pd(arrs[3]), _pd(arrs[4])
+        self._check(self.lib.efdc_set_sources(self.h, C.byref(o)))
+
+    def add_constituent(self, mo, isadac=0, isfct=0, rkdye=0.0, is_dye=0) -> int:
+        d = ConstDesc(mo=mo, isadac=isadac, isfct=isfct, rkdye=rkdye, is_dye=is_dye)
+        slot = C.c_int32(-1)
+        self._check(self.lib.efdc_add_constituent(self.h, C.byref(d), C.byref(slot)))
+        self.nconst += 1
+        return slot.value
+
+    def set_constituent(self, slot, con, con1=None):
+        a = np.ascontiguousarray(con, np.float64)
+        b = None if con1 is None else np.ascontiguousarray(con1, np.float64)
+        self._check(self.lib.efdc_set_constituent(
+            self.h, slot, _pd(a), None if b is None else _pd(b)))
+
+    def get_constituent(self, slot):
+        g = self.grid
+        con = np.zeros((g.KCM, g.LCM))
+        con1 = np.zeros((g.KCM, g.LCM))
+        self._check(self.lib.efdc_get_constituent(self.h, slot, _pd(con), _pd(con1)))
+        return con, con1
+
+    # ------------------------------------------------------------ calls
+    def congrad(self) -> dict:
+        s = CgStats()
+        self._check(self.lib.efdc_congrad(self.h, C.byref(s)))
+        return dict(iters=s.iters, outer=s.outer, rsq=s.rsq, rsqm=s.rsqm_used)
+
+    def calpuv2c(self, dt: float, n: int, niter: Optional[int] = None,
+                 check: bool = True) -> dict:
+        st = Step(dt=dt, n=n, niter=n if niter is None else niter)
+        s = CgStats()
+        code = self.lib.efdc_calpuv2c(self.h, C.byref(st), C.byref(s))
+        if check:
+            self._check(code)
+        return dict(iters=s.iters, outer=s.outer, rsq=s.rsq, rsqm=s.rsqm_used,
+                    ccmnm=s.ccmnm, status=code)
+
+    def calconc(self, dt: float, n: int, niter: Optional[int] = None):
+        st = Step(dt=dt, n=n, niter=n if niter is None else niter)
+        self._check(self.lib.efdc_calconc(self.h, C.byref(st)))
+
+    def balance(self):
+        b = Balance()
+        self._check(self.lib.efdc_balance(self.h, C.byref(b)))
+        return b.volume, np.array(b.mass[:b.nconst])
+
+    def synchronize(self):
+        self._check(self.lib.efdc_synchronize(self.h))
+
+    def closure_pre(self):
+        self._check(self.lib.efdc_closure_pre(self.h))
+
+    def closure_post(self):
+        self._check(self.lib.efdc_closure_post(self.h))
+
+    def set_manning(self, n):
+        a = np.ascontiguousarray(n, np.float64)
+        self._check(self.lib.efdc_closure_set_manning(self.h, _pd(a)))
+        self.has_manning = True
+
+    # ------------------------------------------------------- whole case
+    def upload_case(self, case):
+        """Copy every array and table of a grids.Case to the device."""
+        for dct in (case.h, case.k, case.w):
+            for name, arr in dct.items():
+                self.set(name, arr)
+        if case.open_bc is not None:
+            self.set_open_bc(case.open_bc)
+        for c in case.consts:
+            slot = self.add_constituent(c["mo"], c["isadac"], c["isfct"],
+                                        c["rkdye"], c["is_dye"])
+            self.set_constituent(slot, c["con"], c["con1"])
+        if case.conc_bc is not None and case.consts:
+            self.set_conc_bc(case.conc_bc)
+        if case.sources is not None:
+            self.set_sources(case.sources)
+        if case.manning_n is not None:
+            self.set_manning(case.manning_n)
+        return self
+
+    def step(self, dt: float, n: int) -> dict:
+        """One hot-path step in the order of EFDC/hdmt2t.f90:494-562 with
+        the depth-uniform stand-in for the internal mode."""
+        if self.has_manning:
+            self.closure_pre()
+        st = self.calpuv2c(dt, n)
+        self.closure_post()
+        if self.nconst:
+            self.calconc(dt, n)
+        return st

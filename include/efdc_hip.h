@@ -1,0 +1,300 @@
+/*
+ * efdc_hip.h -- C-ABI of the MI355X (gfx950) implementation of EFDC+'s
+ * per-timestep external-mode solve and constituent transport.
+ *
+ * Host arrays follow the var_global_mod layout of EFDC+ 8.5:
+ *   - horizontal arrays are (LCM), element L (1-based) at offset L-1;
+ *   - layer arrays are (LCM,KCM) column-major, element (L,K) at
+ *     (L-1) + (K-1)*LCM;
+ *   - W2 and DZIG are (LCM,0:KCM), element (L,K) at (L-1) + K*LCM;
+ *   - wet cells are L = 2..LA, ghosts are L = 1 and L = LC = LA+1;
+ *   - neighbour maps hold 1-based L values and point to LC for land.
+ *   (EFDC/var_global_mod.f90:1083-1114, EFDC/varalloc.f90:54,839,1993-2002,
+ *    EFDC/cellmap.f90:12-148)
+ *
+ * All host arrays are FP64 (the ReleaseDP64 build: /real_size:64,
+ * EFDC/EFDCPlus_085_OMP.vfproj:48).  The library keeps its own device copy
+ * of every array in HBM; efdc_set_* / efdc_get_* move data across the
+ * boundary, every compute entry point works on the device copy only.
+ *
+ * Every entry point returns an efdc_status (0 = success).  The text of the
+ * last error of a context is available from efdc_last_error().  A context
+ * owns one HIP stream; calls on one context must not overlap.
+ */
+#ifndef EFDC_HIP_H
+#define EFDC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EFDC_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- status */
+typedef enum {
+  EFDC_OK = 0,
+  EFDC_ERR_ARG = 1,        /* invalid argument or shape                       */
+  EFDC_ERR_HIP = 2,        /* a HIP runtime call failed                        */
+  EFDC_ERR_CG_MAXITER = 3, /* CONGRAD: RSQ > RSQM after ITERM iterations
+                              (EFDC/congrad.f90:171-184)                       */
+  EFDC_ERR_NEGDEPTH = 4,   /* NEGDEP: HP <= 0 after the solve
+                              (EFDC/negdep.f90:34-45)                          */
+  EFDC_ERR_STATE = 5,      /* call out of order (e.g. no open-BC table yet)    */
+  EFDC_ERR_NOMEM = 6,      /* device allocation failed                         */
+  EFDC_ERR_NODEVICE = 7    /* no HIP device visible                            */
+} efdc_status;
+
+/* ------------------------------------------------------------- precision */
+typedef enum {
+  /* Plain FP64 Jacobi-PCG, the method of EFDC/congrad.f90:1-191. */
+  EFDC_CG_FP64 = 0,
+  /* FP32 coefficients and SpMV, FP64 dot products and alpha/beta, wrapped
+     in an FP64 residual-based iterative refinement of P.  No reference
+     counterpart (EFDC/congrad.f90:20-21 runs the whole CG in the build's
+     REAL kind). */
+  EFDC_CG_MIXED = 1
+} efdc_cg_mode;
+
+typedef enum {
+  EFDC_STORE_FP64 = 0, /* constituents stored as FP64 on the device          */
+  EFDC_STORE_FP32 = 1  /* constituents stored as FP32, tridiagonal in FP64   */
+} efdc_store;
+
+/* ------------------------------------------------------------------ grid */
+typedef struct {
+  int32_t LA;   /* last wet cell index; wet cells are L = 2..LA             */
+  int32_t LCM;  /* leading dimension of host (LCM) arrays, >= LA+1          */
+  int32_t KC;   /* number of sigma layers (K = 1 bottom .. KC top)          */
+  int32_t KCM;  /* second dimension of host (LCM,KCM) arrays, >= KC         */
+  const int32_t *LWC, *LEC, *LSC, *LNC; /* (LCM) neighbour maps             */
+  const int32_t *LNWC, *LSEC;           /* (LCM) diagonal maps; may be NULL
+                                           when no constituent uses
+                                           anti-diffusion (ISADAC=0)          */
+} efdc_grid_t;
+
+/* ---------------------------------------------------------------- params */
+typedef struct {
+  double G;        /* gravity (EFDC GLOBAL G)                               */
+  double RITB1;    /* explicit bottom-stress weight (EFDC/caltbxy.f90:94-106)*/
+  double SNLT;     /* nonlinear-term switch multiplying FXE/FYE              */
+  double HDRY;     /* drying depth (only the open-BC inactivation test)     */
+  double TCON_TBEGIN; /* TCON*TBEGIN, offset of the tidal clock             */
+  int32_t ITERM;   /* CG iteration cap                                      */
+  double RSQM;     /* CG stop: sum r^2 <= RSQM on the scaled system          */
+  double cg_rel_tol; /* if > 0: stop when ||r||_2 <= cg_rel_tol*||FPTMP||_2   */
+  int32_t cg_mode;   /* efdc_cg_mode                                        */
+  int32_t store;     /* efdc_store, transport storage precision             */
+  double mixed_inner_rtol; /* EFDC_CG_MIXED: inner FP32 relative reduction  */
+  int32_t mixed_max_outer; /* EFDC_CG_MIXED: refinement step cap            */
+} efdc_params_t;
+
+/* Fill *p with the values EFDC uses by default: G=9.81, RITB1=1, SNLT=0,
+   HDRY=0.2, TCON_TBEGIN=0, ITERM=200, RSQM=1e-9 (the sample decks,
+   SampleModels/Lake_Washington/efdc.inp:92), cg_rel_tol=0, FP64 CG, FP64
+   storage, mixed_inner_rtol=1e-5, mixed_max_outer=30. */
+void efdc_params_default(efdc_params_t *p);
+
+/* ------------------------------------------------------------- array ids */
+typedef enum {
+  /* (LCM) arrays: geometry (EFDC/aaefdc.f90:2172-2185) */
+  EFDC_DXYP = 0, EFDC_DXYIP, EFDC_DXYU, EFDC_DXYV,
+  EFDC_DXIU, EFDC_DYIU, EFDC_DXIV, EFDC_DYIV,
+  EFDC_HRUO, EFDC_HRVO, EFDC_SUB, EFDC_SVB, EFDC_BELV,
+  /* (LCM) external-mode state (EFDC/calpuv2c.f90) */
+  EFDC_P, EFDC_P1, EFDC_HP, EFDC_H1P, EFDC_H2P,
+  EFDC_HU, EFDC_HV, EFDC_HUI, EFDC_HVI, EFDC_HPI,
+  EFDC_H1U, EFDC_H1V, EFDC_H1UI, EFDC_H1VI,
+  EFDC_UHDYE, EFDC_VHDXE, EFDC_UHDY1E, EFDC_VHDX1E, EFDC_UHE, EFDC_VHE,
+  /* (LCM) forcings produced upstream of CALPUV2C */
+  EFDC_TSX, EFDC_TSY, EFDC_TBX, EFDC_TBY, EFDC_FCAXE, EFDC_FCAYE,
+  EFDC_FPGXE, EFDC_FPGYE, EFDC_FXE, EFDC_FYE, EFDC_QSUME, EFDC_RCX, EFDC_RCY,
+  /* (LCM) linear system (EFDC/calpuv2c.f90:214-398) */
+  EFDC_FUHDYE, EFDC_FVHDXE, EFDC_FP, EFDC_FP1,
+  EFDC_CC, EFDC_CS, EFDC_CW, EFDC_CE, EFDC_CN,
+  EFDC_CCC, EFDC_CCS, EFDC_CCW, EFDC_CCE, EFDC_CCN, EFDC_CCCI, EFDC_FPTMP,
+  EFDC_NUM_H,
+  /* (LCM,KCM) arrays (EFDC/varalloc.f90:54,118-120,611,773,803,1959-1996) */
+  EFDC_HPK = 64, EFDC_H1PK, EFDC_HPKI, EFDC_UHDY2, EFDC_VHDX2,
+  EFDC_CDZKK, EFDC_CDZKMK, EFDC_AB, EFDC_QSUM, EFDC_U2, EFDC_V2,
+  EFDC_SUB3D, EFDC_SVB3D, EFDC_DZC,
+  EFDC_NUM_K_END,
+  /* (LCM,0:KCM) arrays (EFDC/varalloc.f90:199,839) */
+  EFDC_W2 = 96, EFDC_DZIG,
+  EFDC_NUM_W_END
+} efdc_array_id;
+
+/* ------------------------------------------------------------ open BCs  */
+/* Pressure open boundaries, one table per side in the order W, E, S, N
+   (LPBW/LPBE/LPBS/LPBN, EFDC/setopenbc.f90:35-206).  For cell LL of a side:
+     ispb[LL]        ISPB* (0 = clamped; 1,2 = radiation)
+     pser[LL]        PSERT+0.5*PSERZDF+PSERST+0.5*PSERZDS, already blended
+                     with the second series when NPFORT>=1
+                     (EFDC/setopenbc.f90:44-48); the time-series part of the
+                     boundary value, in P units (m^2/s^2)
+     pcb[LL*mtide+M], psb[...]  PCB*(LL,M), PSB*(LL,M) (P units)            */
+typedef struct {
+  int32_t n;
+  const int32_t *L;
+  const int32_t *ispb;
+  const double *pser;
+  const double *pcb;
+  const double *psb;
+} efdc_pbc_side_t;
+
+typedef struct {
+  efdc_pbc_side_t side[4]; /* 0=W 1=E 2=S 3=N */
+  int32_t mtide;           /* MTIDE */
+  const double *tcp;       /* TCP(M), tidal periods in seconds */
+} efdc_open_bc_t;
+
+/* Concentration open boundaries for transport (LCBS/LCBW/LCBE/LCBN,
+   EFDC/caltran.f90:316-471), sides in the order S, W, E, N as the
+   reference applies them.  For each side and each transported constituent
+   c (0..nconst-1, as registered by efdc_add_constituent):
+     cb[(LL*2 + j)*nconst + c]          CB*(LL,j+1,M), j = 0,1
+     csert[(LL*KC + (K-1))*nconst + c]  CSERT(K,NCSER*(LL,M),M) this step
+     ntscr[LL]                          NTSCR*(LL)                          */
+typedef struct {
+  int32_t n;
+  const int32_t *L;
+  const double *cb;
+  const double *csert;
+  const int32_t *ntscr;
+} efdc_cbc_side_t;
+
+typedef struct {
+  efdc_cbc_side_t side[4]; /* 0=S 1=W 2=E 3=N */
+  const double *wtci;      /* WTCI(K,2): wtci[(K-1)*2 + j] */
+} efdc_conc_bc_t;
+
+/* Flow sources (CALFQC, EFDC/calfqc.f90:209-225): for source NS
+   (0-based s), at cell LQS:
+     qss[s*KC + K-1], qsercell[s*KC + K-1]          QSS(K,NS), QSERCELL(K,NS)
+     cqs[(s*KC + K-1)*nconst + c], csert[...]      CQS(K,NS,M),
+                                                   CSERT(K,NCSERQ(NS,M),M)  */
+typedef struct {
+  int32_t n;
+  const int32_t *LQS;
+  const double *qss;
+  const double *qsercell;
+  const double *cqs;
+  const double *csert;
+} efdc_sources_t;
+
+/* ---------------------------------------------------------- constituents */
+typedef struct {
+  int32_t mo;      /* reference index MO: 1=SAL, 2=TEM, 3=DYE, >=4 others
+                      (only MO==1 gets the bottom-value outflow clamp,
+                      EFDC/caltran.f90:336-342)                             */
+  int32_t isadac;  /* ISADAC(MVAR): 0 off, 1 anti-diffusion                  */
+  int32_t isfct;   /* ISFCT(MVAR):  0 off, 1 flux-corrected limiter         */
+  double rkdye;    /* dye decay RKDYE (0 none, 1000 age of water), only
+                      applied when is_dye != 0 (EFDC/calconc.f90:581-616)    */
+  int32_t is_dye;
+} efdc_constituent_t;
+
+/* ------------------------------------------------------------------ step */
+typedef struct {
+  double dt;      /* DT (2TL, ISDYNSTP=0)                                   */
+  int64_t n;      /* step counter N (tidal phase TN = DT*N + TCON*TBEGIN)   */
+  int64_t niter;  /* NITER, the clock of the open-BC recovery ramps         */
+} efdc_step_t;
+
+typedef struct {
+  int32_t iters;        /* CG iterations of the last solve (IDRYTBP)        */
+  int32_t outer;        /* refinement steps (EFDC_CG_MIXED), else 1         */
+  double rsq;           /* final sum r^2 on the scaled system (FP64)        */
+  double rsqm_used;     /* the threshold that was applied                   */
+  double ccmnm;         /* min diagonal CCMNM used for scaling              */
+} efdc_cg_stats_t;
+
+typedef struct {
+  double volume;          /* sum DXYP*HP over L=2..LA (BAL2T1 VOLBEG2T form,
+                             EFDC/bal2t.f90:90)                              */
+  double mass[16];        /* per constituent: sum DXYP*HP*C*DZC
+                             (EFDC/bal2t.f90:103-104)                        */
+  int32_t nconst;
+} efdc_balance_t;
+
+typedef struct efdc_ctx efdc_ctx_t;
+
+/* --------------------------------------------------------------- entries */
+int efdc_abi_version(void);
+
+/* Create a context on the current HIP device; copies and validates the
+   maps (replaces the SAVE allocation blocks of EFDC/calpuv2c.f90:59-126 and
+   EFDC/congrad.f90:34-49). */
+int efdc_ctx_create(const efdc_grid_t *grid, const efdc_params_t *params,
+                    efdc_ctx_t **out);
+int efdc_ctx_destroy(efdc_ctx_t *ctx);
+const char *efdc_last_error(const efdc_ctx_t *ctx);
+int efdc_set_params(efdc_ctx_t *ctx, const efdc_params_t *params);
+
+/* Host <-> device copies of one var_global array (full extent, ghosts
+   included: L = 1..LA+1, K = 1..KC or 0..KC). */
+int efdc_set_array(efdc_ctx_t *ctx, int32_t id, const double *host);
+int efdc_get_array(efdc_ctx_t *ctx, int32_t id, double *host);
+
+int efdc_set_open_bc(efdc_ctx_t *ctx, const efdc_open_bc_t *bc);
+int efdc_set_conc_bc(efdc_ctx_t *ctx, const efdc_conc_bc_t *bc);
+int efdc_set_sources(efdc_ctx_t *ctx, const efdc_sources_t *src);
+
+/* Register a transported constituent (the CALTRAN calls of
+   EFDC/calconc.f90:159-228); returns its slot in *slot. */
+int efdc_add_constituent(efdc_ctx_t *ctx, const efdc_constituent_t *desc,
+                         int32_t *slot);
+/* CON and CON1 of one constituent, (LCM,KCM) host layout; con1 may be NULL */
+int efdc_set_constituent(efdc_ctx_t *ctx, int32_t slot, const double *con,
+                         const double *con1);
+int efdc_get_constituent(efdc_ctx_t *ctx, int32_t slot, double *con,
+                         double *con1);
+
+/* CONGRAD (EFDC/congrad.f90:1-191) on the scaled system already held in
+   CCC, CCS, CCW, CCE, CCN, CCCI, FPTMP, starting from and updating P. */
+int efdc_congrad(efdc_ctx_t *ctx, efdc_cg_stats_t *stats);
+
+/* CALPUV2C for the 2TL scheme with ISDRY=0 (EFDC/calpuv2c.f90:128-571,
+   896-914): explicit stage, assembly, open boundaries, min-diagonal
+   scaling, CONGRAD, face transports, depths, P, HU/HV and inverses. */
+int efdc_calpuv2c(efdc_ctx_t *ctx, const efdc_step_t *step,
+                  efdc_cg_stats_t *stats);
+
+/* CALCONC(ISTL=2, IS2TL=1) for every registered constituent
+   (EFDC/calconc.f90:65-473,581-616, EFDC/caltran.f90:1-813,
+   EFDC/calfqc.f90:1-225): upwind advection with sources and open
+   boundaries, optional anti-diffusion with FCT, implicit vertical diffusion
+   and dye decay. */
+int efdc_calconc(efdc_ctx_t *ctx, const efdc_step_t *step);
+
+/* FP64 volume and constituent-mass totals of the device state. */
+int efdc_balance(efdc_ctx_t *ctx, efdc_balance_t *out);
+
+/* Wait for all work queued on the context's stream. */
+int efdc_synchronize(efdc_ctx_t *ctx);
+
+/* Device address of an array of the context (FP64 device copies have the
+   device pitch returned in *pitch elements for (LCM,KCM) arrays). */
+int efdc_device_ptr(efdc_ctx_t *ctx, int32_t id, void **ptr, int64_t *pitch);
+
+/* ------------------------------------------ internal-mode stand-in
+   The momentum closure and the layer transports that EFDC+ computes
+   between CALPUV2C and CALCONC (CALTBXY, CALUVW: EFDC/caltbxy.f90,
+   EFDC/caluvw.f90:668-680,770-790,1163-1172) are outside this library.
+   For device-resident time stepping without them, a depth-uniform
+   stand-in is provided:
+     pre : TBX = G n^2 u|u| / HU^(1/3), u = UHE/HU (likewise TBY);
+     post: UHDY2 = 0.5*(UHDYE+UHDY1E)*DZC, VHDX2 likewise,
+           U2 = 0.5*(UHDYE+UHDY1E)*DYIU/HU, V2 likewise,
+           HPK = HP*DZC, HPKI = 1/HPK (W2 untouched).
+   These have no reference counterpart. */
+int efdc_closure_set_manning(efdc_ctx_t *ctx, const double *n_lcm);
+int efdc_closure_pre(efdc_ctx_t *ctx);
+int efdc_closure_post(efdc_ctx_t *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EFDC_HIP_H */

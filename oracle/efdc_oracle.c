@@ -1,0 +1,1021 @@
+/*
+ * efdc_oracle.c -- CPU restatement of the EFDC+ 8.5 external-mode solve and
+ * constituent transport, FP64, plain C + OpenMP.
+ *
+ * THIS IS TEST INFRASTRUCTURE.  It is the checker of the HIP library in
+ * tests/, the check of __graft_entry__.smoke() and the CPU baseline leg of
+ * bench.py.  The product never links or calls it.
+ *
+ * Each function follows the reference subroutine it names; statement
+ * order and the left-to-right evaluation order of the reference's
+ * expressions are kept so that the restatement can be compared bit for bit
+ * with the reference units that build here (oracle/_ref) and with the FP64
+ * HIP kernels that do not reduce.  The OpenMP structure follows the
+ * reference: CALPUV2C/CONGRAD loops over at most 8 static L-domains
+ * (EFDC/calpuv2c.f90:61-62) with serial dot products
+ * (EFDC/congrad.f90:118-125,139-149), CALTRAN one constituent per thread
+ * (EFDC/calconc.f90:159-229), vertical diffusion column-parallel
+ * (EFDC/calconc.f90:259).
+ */
+#include "efdc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define A(id) (o->a[id])
+#define HX(L) ((size_t)(L) - 1)
+#define KX(L, K) ((size_t)(L) - 1 + (size_t)((K) - 1) * (size_t)o->LCM)
+#define WX(L, K) ((size_t)(L) - 1 + (size_t)(K) * (size_t)o->LCM)
+
+static const double PI_EFDC = 3.1415926535898; /* EFDC/aaefdc.f90:650-651 */
+
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+
+/* number of CALPUV threads: NOPTIMAL = MIN(NTHREADS,8,MAX(LA/100,1))
+   (EFDC/calpuv2c.f90:61) */
+static int noptimal(const orc_ctx *o) {
+  int nt = o->nthreads > 0 ? o->nthreads : 1;
+  int m = o->LA / 100;
+  if (m < 1) m = 1;
+  if (nt > 8) nt = 8;
+  return nt < m ? nt : m;
+}
+
+/* ===================================================================== */
+/* CONGRAD: EFDC/congrad.f90:54-187                                        */
+/* ===================================================================== */
+int orc_congrad(orc_ctx *o, orc_cg_out *out) {
+  const int LA = o->LA, LCM = o->LCM;
+  double *P = A(EFDC_P), *CCC = A(EFDC_CCC), *CCS = A(EFDC_CCS),
+         *CCW = A(EFDC_CCW), *CCE = A(EFDC_CCE), *CCN = A(EFDC_CCN),
+         *CCCI = A(EFDC_CCCI), *FPTMP = A(EFDC_FPTMP);
+  const int32_t *LNC = o->LNC, *LSC = o->LSC, *LEC = o->LEC, *LWC = o->LWC;
+  double *PCG = calloc(LCM, sizeof(double)), *PN = calloc(LCM, sizeof(double)),
+         *PE = calloc(LCM, sizeof(double)), *PW = calloc(LCM, sizeof(double)),
+         *PS = calloc(LCM, sizeof(double)), *RCG = calloc(LCM, sizeof(double)),
+         *TMPCG = calloc(LCM, sizeof(double)),
+         *APCG = calloc(LCM, sizeof(double));
+  const int nopt = noptimal(o);
+  int iter = 0;
+  double rsq = 0.0;
+  out->status = EFDC_OK;
+
+  /* initial residual, :54-77 */
+#pragma omp parallel for num_threads(nopt) schedule(static)
+  for (int L = 2; L <= LA; ++L) {
+    PN[HX(L)] = P[HX(LNC[HX(L)])];
+    PE[HX(L)] = P[HX(LEC[HX(L)])];
+    PW[HX(L)] = P[HX(LWC[HX(L)])];
+    PS[HX(L)] = P[HX(LSC[HX(L)])];
+  }
+#pragma omp parallel for num_threads(nopt) schedule(static)
+  for (int L = 2; L <= LA; ++L) {
+    const size_t i = HX(L);
+    RCG[i] = FPTMP[i] - CCC[i] * P[i] - CCN[i] * PN[i] - CCS[i] * PS[i] -
+             CCW[i] * PW[i] - CCE[i] * PE[i];
+  }
+  /* :79-86 */
+  for (int L = 2; L <= LA; ++L) PCG[HX(L)] = RCG[HX(L)] * CCCI[HX(L)];
+  double rpcg = 0.0;
+  for (int L = 2; L <= LA; ++L) rpcg = rpcg + RCG[HX(L)] * PCG[HX(L)];
+  if (rpcg == 0.0) { /* :88 */
+    out->iters = 0;
+    out->rsq = 0.0;
+    goto done;
+  }
+  for (iter = 1; iter <= o->ITERM; ++iter) { /* :91-169 */
+#pragma omp parallel for num_threads(nopt) schedule(static)
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L);
+      PN[i] = PCG[HX(LNC[i])];
+      PE[i] = PCG[HX(LEC[i])];
+      PW[i] = PCG[HX(LWC[i])];
+      PS[i] = PCG[HX(LSC[i])];
+    }
+#pragma omp parallel for num_threads(nopt) schedule(static)
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L);
+      APCG[i] = CCC[i] * PCG[i] + CCS[i] * PS[i] + CCN[i] * PN[i] +
+                CCW[i] * PW[i] + CCE[i] * PE[i];
+    }
+    double papcg = 0.0; /* :119-124, serial */
+    for (int L = 2; L <= LA; ++L) papcg = papcg + APCG[HX(L)] * PCG[HX(L)];
+    const double alpha = rpcg / papcg;
+#pragma omp parallel for num_threads(nopt) schedule(static)
+    for (int L = 2; L <= LA; ++L) { /* :127-137 */
+      const size_t i = HX(L);
+      P[i] = P[i] + alpha * PCG[i];
+      RCG[i] = RCG[i] - alpha * APCG[i];
+      TMPCG[i] = CCCI[i] * RCG[i];
+    }
+    double rpcgn = 0.0; /* :140-149, serial */
+    rsq = 0.0;
+    for (int L = 2; L <= LA; ++L) {
+      rpcgn = rpcgn + RCG[HX(L)] * TMPCG[HX(L)];
+      rsq = rsq + RCG[HX(L)] * RCG[HX(L)];
+    }
+    const double beta = rpcgn / rpcg;
+    rpcg = rpcgn;
+    if (rsq > o->RSQM) { /* :151-162 */
+#pragma omp parallel for num_threads(nopt) schedule(static)
+      for (int L = 2; L <= LA; ++L)
+        PCG[HX(L)] = TMPCG[HX(L)] + beta * PCG[HX(L)];
+    }
+    if (rsq <= o->RSQM) break; /* :165-167 */
+  }
+  out->iters = iter; /* IDRYTBP = ITER, :187 */
+  out->rsq = rsq;
+  if (rsq > o->RSQM) out->status = EFDC_ERR_CG_MAXITER; /* :171-184 */
+done:
+  free(PCG); free(PN); free(PE); free(PW); free(PS); free(RCG); free(TMPCG);
+  free(APCG);
+  return out->status;
+}
+
+/* ===================================================================== */
+/* SETOPENBC: EFDC/setopenbc.f90:1-209 (HUT=HU, HVT=HV)                    */
+/* ===================================================================== */
+static void orc_setopenbc(orc_ctx *o, double dt, int64_t n, double D_LTD2,
+                          double DE_TI) {
+  const double G = o->G, HDRY2 = 0.2 * o->HDRY, DELTI = DE_TI;
+  double *CC = A(EFDC_CC), *CS = A(EFDC_CS), *CW = A(EFDC_CW),
+         *CE = A(EFDC_CE), *CN = A(EFDC_CN), *FP = A(EFDC_FP),
+         *DXYP = A(EFDC_DXYP), *BELV = A(EFDC_BELV), *HRUO = A(EFDC_HRUO),
+         *HRVO = A(EFDC_HRVO), *RCX = A(EFDC_RCX), *RCY = A(EFDC_RCY),
+         *HUT = A(EFDC_HU), *HVT = A(EFDC_HV), *DXIU = A(EFDC_DXIU),
+         *DYIU = A(EFDC_DYIU), *DXIV = A(EFDC_DXIV), *DYIV = A(EFDC_DYIV),
+         *FUHDYE = A(EFDC_FUHDYE), *FVHDXE = A(EFDC_FVHDXE);
+  double cccos[64], sssin[64];
+  const double PI2 = 2. * PI_EFDC;
+  const double TN = dt * (double)n + o->TCON_TBEGIN; /* :22-26 */
+  for (int M = 0; M < o->mtide && M < 64; ++M) {     /* :27-32 */
+    double TM = fmod(TN, o->tcp[M]);
+    TM = PI2 * TM / o->tcp[M];
+    cccos[M] = cos(TM);
+    sssin[M] = sin(TM);
+  }
+  for (int side = 0; side < 4; ++side) {
+    const orc_pbc_side *s = &o->pbc[side];
+    for (int LL = 0; LL < s->n; ++LL) {
+      const int L = s->L[LL];
+      const size_t i = HX(L);
+      CC[i] = DELTI * DXYP[i];
+      CS[i] = 0.; CW[i] = 0.; CE[i] = 0.; CN[i] = 0.;
+      if (o->lopenbcdry[i]) continue;
+      FP[i] = s->pser[LL];
+      for (int M = 0; M < o->mtide; ++M)
+        FP[i] = FP[i] + s->pcb[LL * o->mtide + M] * cccos[M] +
+                s->psb[LL * o->mtide + M] * sssin[M];
+      const int rad = (s->ispb[LL] == 1 || s->ispb[LL] == 2);
+      if (side == 0) { /* WEST :55-74 */
+        const size_t e = HX(o->LEC[i]);
+        if (rad) {
+          const double CET = 0.5 * D_LTD2 * G * HRUO[e] * RCX[e] * HUT[e];
+          const double TMP = D_LTD2 * sqrt(G * HUT[e]) * DXIU[e];
+          CC[i] = CET * (1. + TMP) / TMP;
+          CE[i] = -CET;
+          FP[i] = CET * (2. * FP[i] - sqrt(G * HUT[e]) * FUHDYE[e] * DYIU[e] / HUT[e]) / TMP;
+        } else {
+          double CET;
+          const double FP1G = FP[i] / G - HDRY2;
+          if (FP1G < BELV[i] || FP1G < BELV[e]) {
+            FP[i] = (BELV[i] + HDRY2) * G;
+            CET = 0.;
+            o->lopenbcdry[i] = 1;
+          } else {
+            CET = 0.5 * D_LTD2 * G * HRUO[e] * RCX[e] * HUT[e];
+          }
+          FP[e] = FP[e] + CET * FP[i];
+          FP[i] = DE_TI * DXYP[i] * FP[i];
+        }
+      } else if (side == 1) { /* EAST :98-117 */
+        const size_t w = HX(o->LWC[i]);
+        if (rad) {
+          const double CWT = 0.5 * D_LTD2 * G * HRUO[i] * RCX[i] * HUT[i];
+          const double TMP = D_LTD2 * sqrt(G * HUT[i]) * DXIU[i];
+          CC[i] = CWT * (1. + TMP) / TMP;
+          CW[i] = -CWT;
+          FP[i] = CWT * (2. * FP[i] + sqrt(G * HUT[i]) * FUHDYE[i] * DYIU[i] / HUT[i]) / TMP;
+        } else {
+          double CWT;
+          const double FP1G = FP[i] / G - HDRY2;
+          if (FP1G < BELV[i] || FP1G < BELV[w]) {
+            FP[i] = (BELV[i] + HDRY2) * G;
+            CWT = 0.;
+            o->lopenbcdry[i] = 1;
+          } else {
+            CWT = 0.5 * D_LTD2 * G * HRUO[i] * RCX[i] * HUT[i];
+          }
+          FP[w] = FP[w] + CWT * FP[i];
+          FP[i] = DE_TI * DXYP[i] * FP[i];
+        }
+      } else if (side == 2) { /* SOUTH :142-161 */
+        const size_t nn = HX(o->LNC[i]);
+        if (rad) {
+          const double CNT = 0.5 * D_LTD2 * G * HRVO[nn] * RCY[nn] * HVT[nn];
+          const double TMP = D_LTD2 * sqrt(G * HVT[nn]) * DYIV[nn];
+          CC[i] = CNT * (1. + TMP) / TMP;
+          CN[i] = -CNT;
+          FP[i] = CNT * (2. * FP[i] - sqrt(G * HVT[nn]) * FVHDXE[nn] * DXIV[nn] / HVT[nn]) / TMP;
+        } else {
+          double CNT;
+          const double FP1G = FP[i] / G - HDRY2;
+          if (FP1G < BELV[i] || FP1G < BELV[nn]) {
+            FP[i] = (BELV[i] + HDRY2) * G;
+            CNT = 0.;
+            o->lopenbcdry[i] = 1;
+          } else {
+            CNT = 0.5 * D_LTD2 * G * HRVO[nn] * RCY[nn] * HVT[nn];
+          }
+          FP[nn] = FP[nn] + CNT * FP[i];
+          FP[i] = DE_TI * DXYP[i] * FP[i];
+        }
+      } else { /* NORTH :186-205 */
+        const size_t ss = HX(o->LSC[i]);
+        if (rad) {
+          const double CST = 0.5 * D_LTD2 * G * HRVO[i] * RCY[i] * HVT[i];
+          const double TMP = D_LTD2 * sqrt(G * HVT[i]) * DYIV[i];
+          CC[i] = CST * (1. + TMP) / TMP;
+          CS[i] = -CST;
+          FP[i] = CST * (2. * FP[i] + sqrt(G * HVT[i]) * FVHDXE[i] * DXIV[i] / HVT[i]) / TMP;
+        } else {
+          double CST;
+          const double FP1G = FP[i] / G - HDRY2;
+          if (FP1G < BELV[i] || FP1G < BELV[ss]) {
+            FP[i] = (BELV[i] + HDRY2) * G;
+            CST = 0.;
+            o->lopenbcdry[i] = 1;
+          } else {
+            CST = 0.5 * D_LTD2 * G * HRVO[i] * RCY[i] * HVT[i];
+          }
+          FP[ss] = FP[ss] + CST * FP[i];
+          FP[i] = DE_TI * DXYP[i] * FP[i];
+        }
+      }
+    }
+  }
+}
+
+/* SETOPENBC2: EFDC/setopenbc2.f90:17-45 */
+static void orc_setopenbc2(orc_ctx *o) {
+  for (int LL = 0; LL < o->pbc[0].n; ++LL)
+    if (o->pbc[0].ispb[LL] == 0)
+      A(EFDC_CW)[HX(o->LEC[HX(o->pbc[0].L[LL])])] = 0.;
+  for (int LL = 0; LL < o->pbc[1].n; ++LL)
+    if (o->pbc[1].ispb[LL] == 0)
+      A(EFDC_CE)[HX(o->LWC[HX(o->pbc[1].L[LL])])] = 0.;
+  for (int LL = 0; LL < o->pbc[2].n; ++LL)
+    if (o->pbc[2].ispb[LL] == 0)
+      A(EFDC_CS)[HX(o->LNC[HX(o->pbc[2].L[LL])])] = 0.;
+  for (int LL = 0; LL < o->pbc[3].n; ++LL)
+    if (o->pbc[3].ispb[LL] == 0)
+      A(EFDC_CN)[HX(o->LSC[HX(o->pbc[3].L[LL])])] = 0.;
+}
+
+/* ===================================================================== */
+/* CALPUV2C assembly: EFDC/calpuv2c.f90:128-136,164-398 (ISDRY=0,         */
+/* BSC=0, ISITB=0, ISVEG=0, MDCHH=0, ISGWIE=0)                             */
+/* ===================================================================== */
+int orc_assemble(orc_ctx *o, double dt, int64_t n, double *ccmnm_out) {
+  const int LA = o->LA, KC = o->KC;
+  const double G = o->G;
+  const double DELT = dt, DELTD2 = 0.5 * dt, DELTI = 1. / DELT; /* :128-131 */
+  const int nopt = noptimal(o);
+  double *UHDYE = A(EFDC_UHDYE), *VHDXE = A(EFDC_VHDXE), *P = A(EFDC_P),
+         *SUB = A(EFDC_SUB), *SVB = A(EFDC_SVB), *HRUO = A(EFDC_HRUO),
+         *HRVO = A(EFDC_HRVO), *HU = A(EFDC_HU), *HV = A(EFDC_HV),
+         *DXIU = A(EFDC_DXIU), *DYIV = A(EFDC_DYIV), *DXYU = A(EFDC_DXYU),
+         *DXYV = A(EFDC_DXYV), *TSX = A(EFDC_TSX), *TSY = A(EFDC_TSY),
+         *TBX = A(EFDC_TBX), *TBY = A(EFDC_TBY), *FCAXE = A(EFDC_FCAXE),
+         *FCAYE = A(EFDC_FCAYE), *FPGXE = A(EFDC_FPGXE),
+         *FPGYE = A(EFDC_FPGYE), *FXE = A(EFDC_FXE), *FYE = A(EFDC_FYE),
+         *FUHDYE = A(EFDC_FUHDYE), *FVHDXE = A(EFDC_FVHDXE),
+         *DXYP = A(EFDC_DXYP), *FP1 = A(EFDC_FP1), *FP = A(EFDC_FP),
+         *QSUME = A(EFDC_QSUME), *RCX = A(EFDC_RCX), *RCY = A(EFDC_RCY),
+         *CC = A(EFDC_CC), *CS = A(EFDC_CS), *CW = A(EFDC_CW),
+         *CE = A(EFDC_CE), *CN = A(EFDC_CN);
+  const double RITB1 = o->RITB1, SNLT = o->SNLT;
+
+  /* explicit stage, :214-220 */
+#pragma omp parallel for num_threads(nopt) schedule(static)
+  for (int L = 2; L <= LA; ++L) {
+    const size_t i = HX(L), s = HX(o->LSC[i]), w = HX(o->LWC[i]);
+    FUHDYE[i] = UHDYE[i] - DELTD2 * SUB[i] * HRUO[i] * HU[i] * (P[i] - P[w]) +
+                SUB[i] * DELT * DXIU[i] *
+                    (DXYU[i] * (TSX[i] - RITB1 * TBX[i]) + FCAXE[i] + FPGXE[i] -
+                     SNLT * FXE[i]);
+    FVHDXE[i] = VHDXE[i] - DELTD2 * SVB[i] * HRVO[i] * HV[i] * (P[i] - P[s]) +
+                SVB[i] * DELT * DYIV[i] *
+                    (DXYV[i] * (TSY[i] - RITB1 * TBY[i]) - FCAYE[i] + FPGYE[i] -
+                     SNLT * FYE[i]);
+  }
+  /* time-level shifts, :256-274 */
+  {
+    double *UHDY1E = A(EFDC_UHDY1E), *VHDX1E = A(EFDC_VHDX1E),
+           *P1 = A(EFDC_P1), *H1U = A(EFDC_H1U), *H1V = A(EFDC_H1V),
+           *H1UI = A(EFDC_H1UI), *H1VI = A(EFDC_H1VI), *HUI = A(EFDC_HUI),
+           *HVI = A(EFDC_HVI), *H2P = A(EFDC_H2P), *H1P = A(EFDC_H1P),
+           *HP = A(EFDC_HP), *H1PK = A(EFDC_H1PK), *HPK = A(EFDC_HPK);
+#pragma omp parallel for num_threads(nopt) schedule(static)
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L);
+      UHDY1E[i] = UHDYE[i];
+      VHDX1E[i] = VHDXE[i];
+      P1[i] = P[i];
+      H1U[i] = HU[i];
+      H1V[i] = HV[i];
+      H1UI[i] = HUI[i];
+      H1VI[i] = HVI[i];
+      H2P[i] = H1P[i];
+      H1P[i] = HP[i];
+      for (int K = 1; K <= KC; ++K) H1PK[KX(L, K)] = HPK[KX(L, K)];
+    }
+  }
+  /* FP1, :285-290 */
+  {
+    const double C1 = 0.5 * G;
+#pragma omp parallel for num_threads(nopt) schedule(static)
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L), e = HX(o->LEC[i]), nn = HX(o->LNC[i]);
+      FP1[i] = DELTI * DXYP[i] * P[i] -
+               C1 * (UHDYE[e] - UHDYE[i] + VHDXE[nn] - VHDXE[i]);
+    }
+  }
+  /* FP, CS, CW, CE, CN, CC: :315-342 */
+#pragma omp parallel for num_threads(nopt) schedule(static)
+  for (int L = 2; L <= LA; ++L) {
+    const size_t i = HX(L), e = HX(o->LEC[i]), nn = HX(o->LNC[i]);
+    const double C1 = 0.5 * G;
+    FP[i] = FP1[i] - C1 * (SUB[e] * FUHDYE[e] - SUB[i] * FUHDYE[i] +
+                           SVB[nn] * FVHDXE[nn] - SVB[i] * FVHDXE[i] -
+                           2.0 * QSUME[i]);
+  }
+#pragma omp parallel for num_threads(nopt) schedule(static)
+  for (int L = 2; L <= LA; ++L) {
+    const size_t i = HX(L), e = HX(o->LEC[i]), nn = HX(o->LNC[i]);
+    const double C1 = -0.5 * DELTD2 * G;
+    CS[i] = C1 * SVB[i] * HRVO[i] * RCY[i] * HV[i];
+    CW[i] = C1 * SUB[i] * HRUO[i] * RCX[i] * HU[i];
+    CE[i] = C1 * SUB[e] * HRUO[e] * RCX[e] * HU[e];
+    CN[i] = C1 * SVB[nn] * HRVO[nn] * RCY[nn] * HV[nn];
+  }
+#pragma omp parallel for num_threads(nopt) schedule(static)
+  for (int L = 2; L <= LA; ++L) {
+    const size_t i = HX(L);
+    CC[i] = DELTI * DXYP[i] - CS[i] - CW[i] - CE[i] - CN[i];
+  }
+  /* open boundaries, :349 */
+  orc_setopenbc(o, dt, n, DELTD2, DELTI);
+  /* min diagonal, :360-376 */
+  double ccmnm = 1.E+18;
+  double *FPTMP = A(EFDC_FPTMP);
+  for (int L = 2; L <= LA; ++L) {
+    ccmnm = dmin(ccmnm, CC[HX(L)]);
+    FPTMP[HX(L)] = FP[HX(L)];
+  }
+  const double CCMNMI = 1. / ccmnm;
+  orc_setopenbc2(o); /* :379 */
+  /* scaling, :384-398 */
+  {
+    double *CCS = A(EFDC_CCS), *CCW = A(EFDC_CCW), *CCE = A(EFDC_CCE),
+           *CCN = A(EFDC_CCN), *CCC = A(EFDC_CCC), *CCCI = A(EFDC_CCCI);
+#pragma omp parallel for num_threads(nopt) schedule(static)
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L);
+      CCS[i] = CS[i] * CCMNMI;
+      CCW[i] = CW[i] * CCMNMI;
+      CCE[i] = CE[i] * CCMNMI;
+      CCN[i] = CN[i] * CCMNMI;
+      CCC[i] = CC[i] * CCMNMI;
+      FPTMP[i] = FPTMP[i] * CCMNMI;
+      CCCI[i] = 1. / CCC[i];
+    }
+  }
+  if (ccmnm_out) *ccmnm_out = ccmnm;
+  return EFDC_OK;
+}
+
+/* ===================================================================== */
+/* CALPUV2C: assembly + CONGRAD + post-solve updates                       */
+/* ===================================================================== */
+int orc_calpuv2c(orc_ctx *o, double dt, int64_t n, orc_cg_out *out) {
+  const int LA = o->LA;
+  const double G = o->G, GI = 1. / o->G; /* EFDC/aaefdc.f90:648 */
+  const double DELTD2 = 0.5 * dt;
+  const int nopt = noptimal(o);
+  double ccmnm = 0.;
+  orc_assemble(o, dt, n, &ccmnm);
+  int st = orc_congrad(o, out); /* :409 */
+  out->ccmnm = ccmnm;
+  double *UHDYE = A(EFDC_UHDYE), *VHDXE = A(EFDC_VHDXE), *P = A(EFDC_P),
+         *SUB = A(EFDC_SUB), *SVB = A(EFDC_SVB), *HRUO = A(EFDC_HRUO),
+         *HRVO = A(EFDC_HRVO), *HU = A(EFDC_HU), *HV = A(EFDC_HV),
+         *RCX = A(EFDC_RCX), *RCY = A(EFDC_RCY), *FUHDYE = A(EFDC_FUHDYE),
+         *FVHDXE = A(EFDC_FVHDXE), *UHE = A(EFDC_UHE), *VHE = A(EFDC_VHE),
+         *DYIU = A(EFDC_DYIU), *DXIV = A(EFDC_DXIV), *HP = A(EFDC_HP),
+         *H1P = A(EFDC_H1P), *DXYIP = A(EFDC_DXYIP), *QSUME = A(EFDC_QSUME),
+         *UHDY1E = A(EFDC_UHDY1E), *VHDX1E = A(EFDC_VHDX1E),
+         *BELV = A(EFDC_BELV), *DXYP = A(EFDC_DXYP), *DXYU = A(EFDC_DXYU),
+         *DXYV = A(EFDC_DXYV), *HPI = A(EFDC_HPI), *HUI = A(EFDC_HUI),
+         *HVI = A(EFDC_HVI);
+  /* face transports, :423-434 */
+#pragma omp parallel for num_threads(nopt) schedule(static)
+  for (int L = 2; L <= LA; ++L) {
+    const size_t i = HX(L), s = HX(o->LSC[i]), w = HX(o->LWC[i]);
+    UHDYE[i] = SUB[i] * (FUHDYE[i] - DELTD2 * HRUO[i] * RCX[i] * HU[i] * (P[i] - P[w]));
+    VHDXE[i] = SVB[i] * (FVHDXE[i] - DELTD2 * HRVO[i] * RCY[i] * HV[i] * (P[i] - P[s]));
+    UHE[i] = UHDYE[i] * DYIU[i];
+    VHE[i] = VHDXE[i] * DXIV[i];
+  }
+  /* depths, :472-477 */
+#pragma omp parallel for num_threads(nopt) schedule(static)
+  for (int L = 2; L <= LA; ++L) {
+    const size_t i = HX(L), e = HX(o->LEC[i]), nn = HX(o->LNC[i]);
+    HP[i] = H1P[i] + DELTD2 * DXYIP[i] *
+                         (2. * QSUME[i] -
+                          (UHDYE[e] + UHDY1E[e] - UHDYE[i] - UHDY1E[i] +
+                           VHDXE[nn] + VHDX1E[nn] - VHDXE[i] - VHDX1E[i]));
+  }
+  /* open boundary depths, :490-492 (LOBCS order S, W, E, N,
+     EFDC/setbcs.f90:884-938) */
+  {
+    static const int order[4] = {2, 0, 1, 3};
+    for (int q = 0; q < 4; ++q) {
+      const orc_pbc_side *s = &o->pbc[order[q]];
+      for (int LL = 0; LL < s->n; ++LL) {
+        const size_t i = HX(s->L[LL]);
+        HP[i] = GI * P[i] - BELV[i];
+      }
+    }
+  }
+  /* final updates, :896-914 (FSGZUDXYPI = 0.5/DXYU, :110-117) */
+#pragma omp parallel for num_threads(nopt) schedule(static)
+  for (int L = 2; L <= LA; ++L) {
+    const size_t i = HX(L);
+    P[i] = G * (HP[i] + BELV[i]);
+  }
+#pragma omp parallel for num_threads(nopt) schedule(static)
+  for (int L = 2; L <= LA; ++L) {
+    const size_t i = HX(L), s = HX(o->LSC[i]), w = HX(o->LWC[i]);
+    HU[i] = (DXYP[i] * HP[i] + DXYP[w] * HP[w]) * (0.5 / DXYU[i]);
+    HV[i] = (DXYP[i] * HP[i] + DXYP[s] * HP[s]) * (0.5 / DXYV[i]);
+  }
+#pragma omp parallel for num_threads(nopt) schedule(static)
+  for (int L = 2; L <= LA; ++L) {
+    const size_t i = HX(L);
+    HPI[i] = 1. / HP[i];
+    HUI[i] = 1. / HU[i];
+    HVI[i] = 1. / HV[i];
+  }
+  /* NEGDEP, EFDC/negdep.f90:34-45 */
+  if (st == EFDC_OK) {
+    for (int L = 2; L <= LA; ++L)
+      if (HP[HX(L)] <= 0.) {
+        st = EFDC_ERR_NEGDEPTH;
+        break;
+      }
+  }
+  out->status = st;
+  return st;
+}
+
+/* ===================================================================== */
+/* CALTRAN (2TL: ISTL=2, IS2TL=1, ISCDCA=0) with CALFQC                     */
+/* ===================================================================== */
+typedef struct {
+  double *FUHUD, *FVHUD, *FWUU, *FQC, *CD, *POS, *UUUU, *VVVV, *WWWW,
+      *FUHVD, *FVHVD, *FWVV, *DUU, *DVV, *CMAX, *CMIN, *CONTMX, *CONTMN,
+      *CON2, *WQBCCON;
+} tran_scratch;
+
+/* The first five arrays are always needed; the anti-diffusion/FCT set
+   only when ISADAC>0.  Zero-initialised like the reference's allocations,
+   so ghost entries (L=1, L=LC) read as 0. */
+static void scratch_alloc(const orc_ctx *o, tran_scratch *s, int adac) {
+  const size_t nk = (size_t)o->LCM * (size_t)(o->KC + 2);
+  double **p = (double **)s;
+  const size_t nall = sizeof(tran_scratch) / sizeof(double *);
+  for (size_t j = 0; j < nall; ++j)
+    p[j] = (j < 5 || adac) ? calloc(nk, sizeof(double)) : NULL;
+}
+static void scratch_free(tran_scratch *s) {
+  double **p = (double **)s;
+  for (size_t j = 0; j < sizeof(tran_scratch) / sizeof(double *); ++j)
+    free(p[j]);
+}
+
+/* arrays with a K=0 plane (FWUU, FWVV, WWWW): index (L,K), K in 0..KC+1 */
+#define ZX(L, K) ((size_t)(L) - 1 + (size_t)(K) * (size_t)o->LCM)
+
+static void orc_caltran(orc_ctx *o, int c, double dt, int64_t niter,
+                        tran_scratch *s) {
+  const int LA = o->LA, KC = o->KC, KS = KC - 1, LC = LA + 1;
+  orc_const *cc = &o->c[c];
+  double *CON = cc->con, *CON1 = cc->con1;
+  const int M = cc->mo, nc = o->nconst;
+  const double DDELT = dt, DDELTA = dt, BSMALL = 1.0E-6; /* :36-50 */
+  double *UHDY2 = A(EFDC_UHDY2), *VHDX2 = A(EFDC_VHDX2), *W2 = A(EFDC_W2),
+         *H1PK = A(EFDC_H1PK), *HPKI = A(EFDC_HPKI), *DXYIP = A(EFDC_DXYIP),
+         *HPK = A(EFDC_HPK);
+  const int32_t *LWC = o->LWC, *LEC = o->LEC, *LSC = o->LSC, *LNC = o->LNC;
+  (void)LC;
+
+  /* CON1 = CON, :59-65 */
+  for (int K = 1; K <= KC; ++K)
+    for (int L = 1; L <= o->LCM; ++L) CON1[KX(L, K)] = CON[KX(L, K)];
+
+  /* CALFQC, EFDC/calfqc.f90:42-225 (2TL, MVAR!=8, no GW, no structures):
+     zero FQC at the flow cells, CONQ = 0.5*(3*CON-CON1), accumulate. */
+  for (int ns = 0; ns < o->nqsij; ++ns)
+    for (int K = 1; K <= KC; ++K) s->FQC[KX(o->lqs[ns], K)] = 0.;
+  for (int ns = 0; ns < o->nqsij; ++ns) {
+    const int L = o->lqs[ns];
+    for (int K = 1; K <= KC; ++K) {
+      const double conq = 0.5 * (3. * CON[KX(L, K)] - CON1[KX(L, K)]);
+      const double q = o->qss[ns * KC + K - 1];
+      const double qs = o->qsercell[ns * KC + K - 1];
+      const double cq = o->cqs[(ns * KC + K - 1) * nc + c];
+      const double cs = o->csert_q[(ns * KC + K - 1) * nc + c];
+      s->FQC[KX(L, K)] = s->FQC[KX(L, K)] + dmax(q, 0.) * cq +
+                         dmin(q, 0.) * conq + dmax(qs, 0.) * cs +
+                         dmin(qs, 0.) * conq;
+    }
+  }
+
+  /* upwind fluxes with the donor maps of CALCONC, EFDC/calconc.f90:89-121,
+     EFDC/caltran.f90:95-111 */
+  for (int K = 1; K <= KC; ++K)
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L);
+      const int LUPU = UHDY2[KX(L, K)] >= 0.0 ? LWC[i] : L;
+      const int LUPV = VHDX2[KX(L, K)] >= 0.0 ? LSC[i] : L;
+      s->FUHUD[KX(L, K)] = UHDY2[KX(L, K)] * CON1[KX(LUPU, K)];
+      s->FVHUD[KX(L, K)] = VHDX2[KX(L, K)] * CON1[KX(LUPV, K)];
+    }
+  if (KC > 1)
+    for (int K = 1; K <= KS; ++K)
+      for (int L = 2; L <= LA; ++L) {
+        const int KUPW = W2[WX(L, K)] >= 0. ? K : K + 1;
+        s->FWUU[ZX(L, K)] = W2[WX(L, K)] * CON1[KX(L, KUPW)];
+      }
+
+  /* 2TL update, :190-199 */
+  for (int K = 1; K <= KC; ++K)
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L);
+      s->CD[KX(L, K)] =
+          CON1[KX(L, K)] * H1PK[KX(L, K)] +
+          DDELT * ((s->FQC[KX(L, K)] + s->FUHUD[KX(L, K)] -
+                    s->FUHUD[KX(LEC[i], K)] + s->FVHUD[KX(L, K)] -
+                    s->FVHUD[KX(LNC[i], K)]) *
+                       DXYIP[i] +
+                   (s->FWUU[ZX(L, K - 1)] - s->FWUU[ZX(L, K)]));
+    }
+  if (cc->isfct >= 1 && cc->isadac > 0) /* :200-207 */
+    for (int K = 1; K <= KC; ++K)
+      for (int L = 2; L <= LA; ++L)
+        s->CON2[KX(L, K)] = dmax(CON1[KX(L, K)], 0.0);
+  for (int K = 1; K <= KC; ++K) /* :240-245 */
+    for (int L = 2; L <= LA; ++L)
+      CON[KX(L, K)] = s->CD[KX(L, K)] * HPKI[KX(L, K)];
+
+  /* open boundary concentrations S, W, E, N: :316-471 (ISTL=2, ISUD=1) */
+  for (int side = 0; side < 4; ++side) {
+    orc_cbc_side *b = &o->cbc[side];
+    for (int K = 1; K <= KC; ++K)
+      for (int LL = 0; LL < b->n; ++LL) {
+        const int L = b->L[LL];
+        const size_t i = HX(L);
+        const size_t m = ((size_t)LL * KC + K - 1) * nc + c;
+        const double cb1 = b->cb[((size_t)LL * 2 + 0) * nc + c];
+        const double cb2 = b->cb[((size_t)LL * 2 + 1) * nc + c];
+        const double cs1 = b->csert[((size_t)LL * KC + 0) * nc + c];
+        const double csk = b->csert[m];
+        double flow, flux;
+        int outflow;
+        if (side == 0) { /* south: face LN */
+          const int LN = LNC[i];
+          flow = VHDX2[KX(LN, K)];
+          flux = s->FVHUD[KX(LN, K)];
+          outflow = flow <= 0.;
+        } else if (side == 1) { /* west: face LEC */
+          const int LE = LEC[i];
+          flow = UHDY2[KX(LE, K)];
+          flux = s->FUHUD[KX(LE, K)];
+          outflow = flow <= 0.;
+        } else if (side == 2) { /* east: own face */
+          flow = UHDY2[KX(L, K)];
+          flux = s->FUHUD[KX(L, K)];
+          outflow = flow >= 0.;
+        } else { /* north: own face */
+          flow = VHDX2[KX(L, K)];
+          flux = s->FVHUD[KX(L, K)];
+          outflow = flow >= 0.;
+        }
+        if (outflow) {
+          double CTMP;
+          if (side <= 1)
+            CTMP = CON1[KX(L, K)] + DDELT * (flow * CON1[KX(L, K)] - flux) *
+                                        DXYIP[i] * HPKI[KX(L, K)];
+          else
+            CTMP = CON1[KX(L, K)] + DDELT * (flux - flow * CON1[KX(L, K)]) *
+                                        DXYIP[i] * HPKI[KX(L, K)];
+          CON[KX(L, K)] = dmax(CTMP, 0.);
+          if (M == 1) {
+            const double cbtmp = cb1 + cs1;
+            if (CON[KX(L, K)] > cbtmp) CON[KX(L, K)] = cbtmp;
+          }
+          b->clo[m] = CON[KX(L, K)];
+          b->nlo[m] = niter;
+        } else {
+          const double CBT = o->wtci[(K - 1) * 2 + 0] * cb1 +
+                             o->wtci[(K - 1) * 2 + 1] * cb2 + csk;
+          const int64_t nmnlod = niter - b->nlo[m];
+          if (nmnlod >= b->ntscr[LL]) {
+            CON[KX(L, K)] = CBT;
+          } else {
+            const double cbtmp =
+                b->clo[m] + (CBT - b->clo[m]) * (double)nmnlod / (double)b->ntscr[LL];
+            CON[KX(L, K)] = dmax(cbtmp, 0.);
+          }
+        }
+        CON1[KX(L, K)] = CON[KX(L, K)]; /* ISUD=1 */
+      }
+  }
+
+  /* anti-diffusion, :475-786 */
+  if (cc->isadac == 0) return;
+  double *U2 = A(EFDC_U2), *V2 = A(EFDC_V2), *DXIU = A(EFDC_DXIU),
+         *DYIV = A(EFDC_DYIV), *HPI = A(EFDC_HPI), *DZIG = A(EFDC_DZIG),
+         *QSUM = A(EFDC_QSUM), *SUB3D = A(EFDC_SUB3D), *SVB3D = A(EFDC_SVB3D);
+  const int32_t *LNWC = o->LNWC, *LSEC = o->LSEC;
+  /* open-BC cell list LOBCS in the order S, W, E, N (EFDC/setbcs.f90) */
+  int nob = 0, nob2 = 0;
+  int *lob = malloc(sizeof(int) * (size_t)(o->pbc[0].n + o->pbc[1].n + o->pbc[2].n + o->pbc[3].n + 1));
+  int *lob2 = malloc(sizeof(int) * (size_t)(o->pbc[0].n + o->pbc[1].n + o->pbc[2].n + o->pbc[3].n + 1));
+  {
+    static const int order[4] = {2, 0, 1, 3};
+    for (int q = 0; q < 4; ++q) {
+      const orc_pbc_side *ps = &o->pbc[order[q]];
+      for (int LL = 0; LL < ps->n; ++LL) {
+        const int L = ps->L[LL];
+        lob[nob++] = L;
+        if (ps->ispb[LL] <= 1) {
+          int nb = order[q] == 2 ? LNC[HX(L)] : order[q] == 0 ? LEC[HX(L)]
+                 : order[q] == 1 ? LWC[HX(L)] : LSC[HX(L)];
+          lob2[nob2++] = nb;
+        }
+      }
+    }
+  }
+  for (int q = 0; q < nob; ++q) /* :482-487 */
+    for (int K = 1; K <= KC; ++K)
+      s->WQBCCON[(size_t)q * KC + K - 1] = CON[KX(lob[q], K)];
+  for (int K = 1; K <= KC; ++K) /* :490-495 */
+    for (int L = 2; L <= LA; ++L) s->POS[KX(L, K)] = dmax(CON[KX(L, K)], 0.);
+  for (int K = 1; K <= KC; ++K) /* :497-503 */
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L);
+      s->UUUU[KX(L, K)] = U2[KX(L, K)] * (s->POS[KX(L, K)] - s->POS[KX(LWC[i], K)]) * DXIU[i];
+      s->VVVV[KX(L, K)] = V2[KX(L, K)] * (s->POS[KX(L, K)] - s->POS[KX(LSC[i], K)]) * DYIV[i];
+    }
+  if (KC > 1) /* :505-511 */
+    for (int K = 1; K <= KS; ++K)
+      for (int L = 2; L <= LA; ++L)
+        s->WWWW[ZX(L, K)] = W2[WX(L, K)] * (s->POS[KX(L, K + 1)] - s->POS[KX(L, K)]) *
+                            HPI[HX(L)] * DZIG[WX(L, K)];
+  for (int K = 1; K <= KC; ++K) /* :522-558 */
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L);
+      const int LE = LEC[i], LW = LWC[i], LN = LNC[i], LS = LSC[i];
+      const int LNW = LNWC[i], LSE = LSEC[i];
+      const double uh = UHDY2[KX(L, K)];
+      double UTERM = fabs(uh) * (s->POS[KX(L, K)] - s->POS[KX(LW, K)]);
+      if (uh >= 0.0)
+        UTERM = UTERM - 0.5 * DDELTA * uh *
+                            (s->VVVV[KX(LNW, K)] + s->VVVV[KX(LW, K)] +
+                             s->WWWW[ZX(LW, K)] + s->WWWW[ZX(LW, K - 1)] +
+                             s->UUUU[KX(L, K)] + s->UUUU[KX(LW, K)]);
+      else
+        UTERM = UTERM - 0.5 * DDELTA * uh *
+                            (s->VVVV[KX(LN, K)] + s->VVVV[KX(L, K)] +
+                             s->WWWW[ZX(L, K)] + s->WWWW[ZX(L, K - 1)] +
+                             s->UUUU[KX(L, K)] + s->UUUU[KX(LE, K)]);
+      const double UHU = UTERM / (s->POS[KX(L, K)] + s->POS[KX(LW, K)] + BSMALL);
+      s->FUHUD[KX(L, K)] = dmax(UHU, 0.) * s->POS[KX(LW, K)] + dmin(UHU, 0.) * s->POS[KX(L, K)];
+      const double vh = VHDX2[KX(L, K)];
+      double VTERM = fabs(vh) * (s->POS[KX(L, K)] - s->POS[KX(LS, K)]);
+      if (vh >= 0.0)
+        VTERM = VTERM - 0.5 * DDELTA * vh *
+                            (s->UUUU[KX(LS, K)] + s->UUUU[KX(LSE, K)] +
+                             s->WWWW[ZX(LS, K)] + s->WWWW[ZX(LS, K - 1)] +
+                             s->VVVV[KX(LS, K)] + s->VVVV[KX(L, K)]);
+      else
+        VTERM = VTERM - 0.5 * DDELTA * vh *
+                            (s->UUUU[KX(L, K)] + s->UUUU[KX(LE, K)] +
+                             s->WWWW[ZX(L, K)] + s->WWWW[ZX(L, K - 1)] +
+                             s->VVVV[KX(LN, K)] + s->VVVV[KX(L, K)]);
+      const double VHV = VTERM / (s->POS[KX(L, K)] + s->POS[KX(LS, K)] + BSMALL);
+      s->FVHUD[KX(L, K)] = dmax(VHV, 0.) * s->POS[KX(LS, K)] + dmin(VHV, 0.) * s->POS[KX(L, K)];
+    }
+  if (KC > 1) /* :560-577 */
+    for (int K = 1; K <= KS; ++K)
+      for (int L = 2; L <= LA; ++L) {
+        const size_t i = HX(L);
+        const int LE = LEC[i], LN = LNC[i];
+        const double ww = W2[WX(L, K)];
+        double WTERM = fabs(ww) * (s->POS[KX(L, K + 1)] - s->POS[KX(L, K)]);
+        if (ww >= 0.0)
+          WTERM = WTERM - 0.5 * DDELTA * ww *
+                              (s->UUUU[KX(L, K)] + s->UUUU[KX(LE, K)] +
+                               s->VVVV[KX(L, K)] + s->VVVV[KX(LN, K)] +
+                               s->WWWW[ZX(L, K)] + s->WWWW[ZX(L, K - 1)]);
+        else
+          WTERM = WTERM - 0.5 * DDELTA * ww *
+                              (s->UUUU[KX(L, K + 1)] + s->UUUU[KX(LE, K + 1)] +
+                               s->VVVV[KX(L, K + 1)] + s->VVVV[KX(LN, K + 1)] +
+                               s->WWWW[ZX(L, K)] + s->WWWW[ZX(L, K + 1)]);
+        const double WW = WTERM / (s->POS[KX(L, K + 1)] + s->POS[KX(L, K)] + BSMALL);
+        s->FWUU[ZX(L, K)] = dmax(WW, 0.) * s->POS[KX(L, K)] + dmin(WW, 0.) * s->POS[KX(L, K + 1)];
+      }
+  if (cc->isadac == 1) /* :583-599 */
+    for (int K = 1; K <= KC; ++K)
+      for (int L = 2; L <= LA; ++L)
+        if (fabs(QSUM[KX(L, K)]) > 1.E-8) {
+          const size_t i = HX(L);
+          s->FUHUD[KX(L, K)] = 0.;
+          s->FUHUD[KX(LEC[i], K)] = 0.;
+          s->FVHUD[KX(L, K)] = 0.;
+          s->FVHUD[KX(LNC[i], K)] = 0.;
+          s->FWUU[ZX(L, K)] = 0.;
+          s->FWVV[ZX(L, K - 1)] = 0.;
+        }
+  for (int K = 1; K <= KC; ++K) { /* :602-620 */
+    for (int LL = 0; LL < o->cbc[0].n; ++LL)
+      s->FVHUD[KX(LNC[HX(o->cbc[0].L[LL])], K)] = 0.0;
+    for (int LL = 0; LL < o->cbc[1].n; ++LL)
+      s->FUHUD[KX(LEC[HX(o->cbc[1].L[LL])], K)] = 0.0;
+    for (int LL = 0; LL < o->cbc[2].n; ++LL)
+      s->FUHUD[KX(o->cbc[2].L[LL], K)] = 0.0;
+    for (int LL = 0; LL < o->cbc[3].n; ++LL)
+      s->FVHUD[KX(o->cbc[3].L[LL], K)] = 0.0;
+  }
+  if (cc->isfct != 0) { /* :624-768 */
+    for (int K = 1; K <= KC; ++K)
+      for (int L = 2; L <= LA; ++L) {
+        s->CONTMX[KX(L, K)] = dmax(s->POS[KX(L, K)], fabs(s->CON2[KX(L, K)]));
+        s->CONTMN[KX(L, K)] = dmin(s->POS[KX(L, K)], fabs(s->CON2[KX(L, K)]));
+      }
+    for (int L = 2; L <= LA; ++L) { /* KSZ = 1 (sigma grid) */
+      s->CMAX[KX(L, 1)] = dmax(s->CONTMX[KX(L, 1)], s->CONTMX[KX(L, 2)]);
+      s->CMAX[KX(L, KC)] = dmax(s->CONTMX[KX(L, KS)], s->CONTMX[KX(L, KC)]);
+      s->CMIN[KX(L, 1)] = dmin(s->CONTMN[KX(L, 1)], s->CONTMN[KX(L, 2)]);
+      s->CMIN[KX(L, KC)] = dmin(s->CONTMN[KX(L, KS)], s->CONTMN[KX(L, KC)]);
+    }
+    for (int K = 2; K <= KS; ++K)
+      for (int L = 2; L <= LA; ++L) {
+        const double CMAXT = dmax(s->CONTMX[KX(L, K - 1)], s->CONTMX[KX(L, K + 1)]);
+        s->CMAX[KX(L, K)] = dmax(s->CONTMX[KX(L, K)], CMAXT);
+        const double CMINT = dmin(s->CONTMN[KX(L, K - 1)], s->CONTMN[KX(L, K + 1)]);
+        s->CMIN[KX(L, K)] = dmin(s->CONTMN[KX(L, K)], CMINT);
+      }
+    for (int K = 1; K <= KC; ++K)
+      for (int L = 2; L <= LA; ++L) {
+        const size_t i = HX(L);
+        const int LE = LEC[i], LS = LSC[i], LN = LNC[i], LW = LWC[i];
+        const double CWMAX = SUB3D[KX(L, K)] * s->CONTMX[KX(LW, K)];
+        const double CEMAX = SUB3D[KX(LE, K)] * s->CONTMX[KX(LE, K)];
+        const double CSMAX = SVB3D[KX(L, K)] * s->CONTMX[KX(LS, K)];
+        const double CNMAX = SVB3D[KX(LN, K)] * s->CONTMX[KX(LN, K)];
+        double CMAXT = dmax(CNMAX, CEMAX);
+        CMAXT = dmax(CMAXT, CSMAX);
+        CMAXT = dmax(CMAXT, CWMAX);
+        s->CMAX[KX(L, K)] = dmax(s->CMAX[KX(L, K)], CMAXT);
+        const double CWMIN = SUB3D[KX(L, K)] * s->CONTMN[KX(LW, K)] + 1.E+6 * (1. - SUB3D[KX(L, K)]);
+        const double CEMIN = SUB3D[KX(LE, K)] * s->CONTMN[KX(LE, K)] + 1.E+6 * (1. - SUB3D[KX(LE, K)]);
+        const double CSMIN = SVB3D[KX(L, K)] * s->CONTMN[KX(LS, K)] + 1.E+6 * (1. - SVB3D[KX(L, K)]);
+        const double CNMIN = SVB3D[KX(LN, K)] * s->CONTMN[KX(LN, K)] + 1.E+6 * (1. - SVB3D[KX(LN, K)]);
+        double CMINT = dmin(CNMIN, CEMIN);
+        CMINT = dmin(CMINT, CSMIN);
+        CMINT = dmin(CMINT, CWMIN);
+        s->CMIN[KX(L, K)] = dmin(s->CMIN[KX(L, K)], CMINT);
+      }
+    for (int K = 1; K <= KC; ++K) /* :683-691 */
+      for (int L = 2; L <= LA; ++L) {
+        s->FUHVD[KX(L, K)] = dmin(s->FUHUD[KX(L, K)], 0.);
+        s->FUHUD[KX(L, K)] = dmax(s->FUHUD[KX(L, K)], 0.);
+        s->FVHVD[KX(L, K)] = dmin(s->FVHUD[KX(L, K)], 0.);
+        s->FVHUD[KX(L, K)] = dmax(s->FVHUD[KX(L, K)], 0.);
+      }
+    if (KC > 1) /* :693-701 */
+      for (int K = 1; K <= KS; ++K)
+        for (int L = 2; L <= LA; ++L) {
+          s->FWVV[ZX(L, K)] = dmin(s->FWUU[ZX(L, K)], 0.);
+          s->FWUU[ZX(L, K)] = dmax(s->FWUU[ZX(L, K)], 0.);
+        }
+    for (int K = 1; K <= KC; ++K) /* :705-714 */
+      for (int L = 2; L <= LA; ++L) {
+        const size_t i = HX(L);
+        const int LE = LEC[i], LN = LNC[i];
+        s->DUU[KX(L, K)] = DDELT *
+            (DXYIP[i] * (s->FUHUD[KX(L, K)] - s->FUHVD[KX(LE, K)] + s->FVHUD[KX(L, K)] - s->FVHVD[KX(LN, K)]) +
+             (s->FWUU[ZX(L, K - 1)] - s->FWVV[ZX(L, K)])) * HPKI[KX(L, K)];
+        s->DVV[KX(L, K)] = DDELT *
+            (DXYIP[i] * (s->FUHUD[KX(LE, K)] - s->FUHVD[KX(L, K)] + s->FVHUD[KX(LN, K)] - s->FVHVD[KX(L, K)]) +
+             (s->FWUU[ZX(L, K)] - s->FWVV[ZX(L, K - 1)])) * HPKI[KX(L, K)];
+      }
+    for (int K = 1; K <= KC; ++K) /* :717-723 */
+      for (int q = 0; q < nob; ++q) {
+        s->DUU[KX(lob[q], K)] = 0.;
+        s->DVV[KX(lob[q], K)] = 0.;
+      }
+    for (int K = 1; K <= KC; ++K) /* :726-734 */
+      for (int L = 2; L <= LA; ++L) {
+        const size_t j = KX(L, K);
+        if (s->DUU[j] > 0.) s->DUU[j] = (s->CMAX[j] - s->POS[j]) / (s->DUU[j] + BSMALL);
+        s->DUU[j] = dmin(s->DUU[j], 1.);
+        if (s->DVV[j] > 0.) s->DVV[j] = (s->POS[j] - s->CMIN[j]) / (s->DVV[j] + BSMALL);
+        s->DVV[j] = dmin(s->DVV[j], 1.);
+      }
+    for (int K = 1; K <= KC; ++K) /* :737-745 */
+      for (int L = 2; L <= LA; ++L) {
+        const size_t i = HX(L);
+        const int LS = LSC[i], LW = LWC[i];
+        s->FUHUD[KX(L, K)] = dmin(s->DVV[KX(LW, K)], s->DUU[KX(L, K)]) * s->FUHUD[KX(L, K)] +
+                             dmin(s->DUU[KX(LW, K)], s->DVV[KX(L, K)]) * s->FUHVD[KX(L, K)];
+        s->FVHUD[KX(L, K)] = dmin(s->DVV[KX(LS, K)], s->DUU[KX(L, K)]) * s->FVHUD[KX(L, K)] +
+                             dmin(s->DUU[KX(LS, K)], s->DVV[KX(L, K)]) * s->FVHVD[KX(L, K)];
+      }
+    for (int K = 1; K <= KS; ++K) /* :747-752 */
+      for (int L = 2; L <= LA; ++L)
+        s->FWUU[ZX(L, K)] = dmin(s->DVV[KX(L, K)], s->DUU[KX(L, K + 1)]) * s->FWUU[ZX(L, K)] +
+                            dmin(s->DUU[KX(L, K)], s->DVV[KX(L, K + 1)]) * s->FWVV[ZX(L, K)];
+    for (int q = 0; q < nob; ++q) /* :755-760 */
+      for (int K = 1; K <= KS; ++K) s->FWUU[ZX(lob[q], K)] = 0.0;
+    for (int q = 0; q < nob2; ++q) /* :761-766 */
+      for (int K = 1; K <= KS; ++K) s->FWUU[ZX(lob2[q], K)] = 0.0;
+  }
+  for (int K = 1; K <= KC; ++K) /* :771-778 */
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L);
+      s->CD[KX(L, K)] = CON[KX(L, K)] * HPK[KX(L, K)] +
+          DDELT * ((s->FUHUD[KX(L, K)] - s->FUHUD[KX(LEC[i], K)] + s->FVHUD[KX(L, K)] -
+                    s->FVHUD[KX(LNC[i], K)]) * DXYIP[i] +
+                   (s->FWUU[ZX(L, K - 1)] - s->FWUU[ZX(L, K)]));
+      CON[KX(L, K)] = s->CD[KX(L, K)] * HPKI[KX(L, K)];
+    }
+  for (int K = 1; K <= KC; ++K) /* :781-786 */
+    for (int q = 0; q < nob; ++q)
+      CON[KX(lob[q], K)] = s->WQBCCON[(size_t)q * KC + K - 1];
+  free(lob);
+  free(lob2);
+}
+
+/* ===================================================================== */
+/* CALCONC vertical diffusion: EFDC/calconc.f90:239-473                    */
+/* ===================================================================== */
+int orc_vertical_diffusion(orc_ctx *o, double dt) {
+  const int LA = o->LA, KC = o->KC, KS = KC - 1;
+  if (KC == 1) return EFDC_OK; /* :239 */
+  const double DELTD2 = dt;    /* :65-73 */
+  double *CDZKK = A(EFDC_CDZKK), *CDZKMK = A(EFDC_CDZKMK), *AB = A(EFDC_AB),
+         *HPI = A(EFDC_HPI);
+  const int nc = o->nconst;
+  const int nt = o->nthreads > 0 ? o->nthreads : 1;
+#pragma omp parallel num_threads(nt)
+  {
+    double *VCU = malloc(sizeof(double) * (size_t)(KC + 1));
+#pragma omp for schedule(dynamic, 256)
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L);
+      double EEB[64], CCLB[64];
+      /* bottom layer, :265-272 (KSZ=1) */
+      {
+        const double RCDZKK = -DELTD2 * CDZKK[KX(L, 1)];
+        const double CCUBTMP = RCDZKK * HPI[i] * AB[KX(L, 1)];
+        const double CCMBTMP = 1.0 - CCUBTMP;
+        EEB[1] = 1.0 / CCMBTMP;
+        VCU[1] = CCUBTMP * EEB[1];
+      }
+      /* middle layers, :317-327 */
+      for (int K = 2; K <= KS; ++K) {
+        const double RCDZKMK = -DELTD2 * CDZKMK[KX(L, K)];
+        const double RCDZKK = -DELTD2 * CDZKK[KX(L, K)];
+        CCLB[K] = RCDZKMK * HPI[i] * AB[KX(L, K - 1)];
+        const double CCUBTMP = RCDZKK * HPI[i] * AB[KX(L, K)];
+        const double CCMBTMP = 1.0 - CCLB[K] - CCUBTMP;
+        EEB[K] = 1.0 / (CCMBTMP - CCLB[K] * VCU[K - 1]);
+        VCU[K] = CCUBTMP * EEB[K];
+      }
+      /* top layer, :373-380 */
+      {
+        const int K = KC;
+        const double RCDZKMK = -DELTD2 * CDZKMK[KX(L, K)];
+        CCLB[K] = RCDZKMK * HPI[i] * AB[KX(L, K - 1)];
+        const double CCMBTMP = 1.0 - CCLB[K];
+        EEB[K] = 1.0 / (CCMBTMP - CCLB[K] * VCU[K - 1]);
+      }
+      for (int c = 0; c < nc; ++c) {
+        double *C = o->c[c].con;
+        C[KX(L, 1)] = C[KX(L, 1)] * EEB[1];                       /* :276 */
+        for (int K = 2; K <= KC; ++K)                             /* :331,384 */
+          C[KX(L, K)] = (C[KX(L, K)] - CCLB[K] * C[KX(L, K - 1)]) * EEB[K];
+        for (int K = KS; K >= 1; --K)                             /* :425-468 */
+          C[KX(L, K)] = C[KX(L, K)] - VCU[K] * C[KX(L, K + 1)];
+      }
+    }
+    free(VCU);
+  }
+  return EFDC_OK;
+}
+
+int orc_calconc(orc_ctx *o, double dt, int64_t niter) {
+  const int nc = o->nconst;
+  const int nt = o->nthreads > 0 ? o->nthreads : 1;
+  /* one constituent per thread, EFDC/calconc.f90:159-229 */
+#pragma omp parallel for num_threads(nt < nc ? nt : (nc > 0 ? nc : 1)) schedule(static, 1)
+  for (int c = 0; c < nc; ++c) {
+    tran_scratch s;
+    scratch_alloc(o, &s, o->c[c].isadac > 0);
+    orc_caltran(o, c, dt, niter, &s);
+    scratch_free(&s);
+  }
+  orc_vertical_diffusion(o, dt);
+  /* dye decay / age, EFDC/calconc.f90:581-616 */
+  for (int c = 0; c < nc; ++c) {
+    if (!o->c[c].is_dye) continue;
+    const double rk = o->c[c].rkdye;
+    double *C = o->c[c].con;
+    if (rk == 1000.0) {
+      const double DAGE = dt / 86400.;
+      for (int K = 1; K <= o->KC; ++K)
+        for (int L = 2; L <= o->LA; ++L) C[KX(L, K)] = C[KX(L, K)] + DAGE;
+    } else if (rk != 0.0) {
+      const double f = rk < 0.0 ? exp(-rk * dt) : 1. / (1. + dt * rk);
+      for (int K = 1; K <= o->KC; ++K)
+        for (int L = 2; L <= o->LA; ++L) C[KX(L, K)] = f * C[KX(L, K)];
+    }
+  }
+  return EFDC_OK;
+}
+
+/* FP64 totals, EFDC/bal2t.f90:90 (volume) and :103-104 (mass) with unit
+   masks */
+int orc_balance(orc_ctx *o, double *volume, double *mass) {
+  double v = 0.;
+  for (int L = 2; L <= o->LA; ++L) v = v + A(EFDC_DXYP)[HX(L)] * A(EFDC_HP)[HX(L)];
+  *volume = v;
+  for (int c = 0; c < o->nconst; ++c) {
+    double m = 0.;
+    for (int K = 1; K <= o->KC; ++K)
+      for (int L = 2; L <= o->LA; ++L)
+        m = m + A(EFDC_DXYP)[HX(L)] * A(EFDC_HP)[HX(L)] *
+                    o->c[c].con[KX(L, K)] * A(EFDC_DZC)[KX(L, K)];
+    mass[c] = m;
+  }
+  return EFDC_OK;
+}
+
+/* Depth-uniform closure standing in for the internal mode: Manning drag
+   on the external velocity before CALPUV2C; layer transports and layer
+   depths after it. */
+int orc_closure_pre(orc_ctx *o, const double *nman) {
+  const double G = o->G;
+  double *TBX = A(EFDC_TBX), *TBY = A(EFDC_TBY), *UHE = A(EFDC_UHE),
+         *VHE = A(EFDC_VHE), *HU = A(EFDC_HU), *HV = A(EFDC_HV);
+  for (int L = 2; L <= o->LA; ++L) {
+    const size_t i = HX(L);
+    const double n2 = nman[i] * nman[i];
+    const double u = UHE[i] / HU[i], v = VHE[i] / HV[i];
+    TBX[i] = G * n2 * u * fabs(u) / cbrt(HU[i]);
+    TBY[i] = G * n2 * v * fabs(v) / cbrt(HV[i]);
+  }
+  return EFDC_OK;
+}
+
+int orc_closure_post(orc_ctx *o) {
+  double *UHDYE = A(EFDC_UHDYE), *UHDY1E = A(EFDC_UHDY1E),
+         *VHDXE = A(EFDC_VHDXE), *VHDX1E = A(EFDC_VHDX1E), *HP = A(EFDC_HP),
+         *DZC = A(EFDC_DZC), *UHDY2 = A(EFDC_UHDY2), *VHDX2 = A(EFDC_VHDX2),
+         *HPK = A(EFDC_HPK), *HPKI = A(EFDC_HPKI), *U2 = A(EFDC_U2),
+         *V2 = A(EFDC_V2), *HU = A(EFDC_HU), *HV = A(EFDC_HV),
+         *DYIU = A(EFDC_DYIU), *DXIV = A(EFDC_DXIV);
+  for (int K = 1; K <= o->KC; ++K)
+    for (int L = 2; L <= o->LA; ++L) {
+      const size_t i = HX(L), j = KX(L, K);
+      const double ub = 0.5 * (UHDYE[i] + UHDY1E[i]);
+      const double vb = 0.5 * (VHDXE[i] + VHDX1E[i]);
+      UHDY2[j] = ub * DZC[j];
+      VHDX2[j] = vb * DZC[j];
+      U2[j] = ub * DYIU[i] / HU[i];
+      V2[j] = vb * DXIV[i] / HV[i];
+      HPK[j] = HP[i] * DZC[j];
+      HPKI[j] = 1. / HPK[j];
+    }
+  return EFDC_OK;
+}

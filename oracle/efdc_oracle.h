@@ -1,0 +1,88 @@
+/*
+ * efdc_oracle.h -- CPU restatement (FP64, plain C + OpenMP) of the EFDC+
+ * 8.5 hot path, used ONLY as the checker in tests/, as the smoke check and
+ * as the CPU baseline of bench.py.  Nothing in the product links it.
+ *
+ * Arrays are the var_global_mod arrays in Fortran layout (see
+ * include/efdc_hip.h); they are addressed through the same efdc_array_id
+ * values as the C-ABI.
+ */
+#ifndef EFDC_ORACLE_H
+#define EFDC_ORACLE_H
+#include <stdint.h>
+#include "../include/efdc_hip.h"
+
+#define ORC_NARR 128
+#define ORC_MAXC 16
+
+typedef struct {
+  int32_t n;
+  const int32_t *L, *ispb;
+  const double *pser, *pcb, *psb;
+} orc_pbc_side;
+
+typedef struct {
+  int32_t n;
+  const int32_t *L;
+  const double *cb;      /* [(LL*2+j)*nc + c] */
+  const double *csert;   /* [(LL*KC+K-1)*nc + c] */
+  const int32_t *ntscr;
+  double *clo;           /* CLO*(LL,K,M) memory: [(LL*KC+K-1)*nc + c] */
+  int64_t *nlo;          /* NLO*(LL,K,M) memory, same indexing           */
+} orc_cbc_side;
+
+typedef struct {
+  int32_t mo, isadac, isfct, is_dye;
+  double rkdye;
+  double *con, *con1;    /* (LCM,KCM) */
+} orc_const;
+
+typedef struct {
+  int32_t LA, LCM, KC, KCM;
+  const int32_t *LWC, *LEC, *LSC, *LNC, *LNWC, *LSEC;
+  double *a[ORC_NARR];   /* indexed by efdc_array_id */
+  /* params */
+  double G, RITB1, SNLT, HDRY, TCON_TBEGIN, RSQM;
+  int32_t ITERM;
+  int32_t nthreads;      /* OpenMP threads (reference threading structure) */
+  /* pressure open BCs: W, E, S, N */
+  orc_pbc_side pbc[4];
+  int32_t mtide;
+  const double *tcp;
+  int32_t *lopenbcdry;   /* (LCM) LOPENBCDRY */
+  /* concentration open BCs: S, W, E, N */
+  orc_cbc_side cbc[4];
+  const double *wtci;    /* [(K-1)*2 + j] */
+  /* sources */
+  int32_t nqsij;
+  const int32_t *lqs;
+  const double *qss, *qsercell, *cqs, *csert_q;
+  /* constituents */
+  int32_t nconst;
+  orc_const c[ORC_MAXC];
+} orc_ctx;
+
+typedef struct {
+  int32_t iters;
+  double rsq;
+  double ccmnm;
+  int32_t status;
+} orc_cg_out;
+
+/* CONGRAD on the scaled system held in a[]: EFDC/congrad.f90:54-187 */
+int orc_congrad(orc_ctx *o, orc_cg_out *out);
+/* CALPUV2C, 2TL, ISDRY=0: EFDC/calpuv2c.f90:128-571,896-914 */
+int orc_calpuv2c(orc_ctx *o, double dt, int64_t n, orc_cg_out *out);
+/* Only the assembly part (explicit stage .. scaling), for unit tests */
+int orc_assemble(orc_ctx *o, double dt, int64_t n, double *ccmnm);
+/* CALCONC(2,1): EFDC/calconc.f90:65-473,581-616 with CALTRAN/CALFQC */
+int orc_calconc(orc_ctx *o, double dt, int64_t niter);
+/* only the vertical diffusion pass, EFDC/calconc.f90:239-473 */
+int orc_vertical_diffusion(orc_ctx *o, double dt);
+/* balance totals: EFDC/bal2t.f90:90,103-104 */
+int orc_balance(orc_ctx *o, double *volume, double *mass);
+/* depth-uniform closure standing in for the internal mode */
+int orc_closure_pre(orc_ctx *o, const double *manning_n);
+int orc_closure_post(orc_ctx *o);
+
+#endif

@@ -1,0 +1,244 @@
+"""ctypes front end of the C oracle (oracle/efdc_oracle.c).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, as the checker.  It works in place on the
+numpy arrays of an efdc_amd.grids.Case.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "efdcplus8.5_amd"))
+from efdc_amd.abi import ARRAY_IDS  # noqa: E402
+
+LIB = os.path.join(HERE, "liboracle.so")
+NARR = 128
+MAXC = 16
+
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int32)
+PL = C.POINTER(C.c_int64)
+
+
+class PbcSide(C.Structure):
+    _fields_ = [("n", C.c_int32), ("L", PI), ("ispb", PI), ("pser", PD),
+                ("pcb", PD), ("psb", PD)]
+
+
+class CbcSide(C.Structure):
+    _fields_ = [("n", C.c_int32), ("L", PI), ("cb", PD), ("csert", PD),
+                ("ntscr", PI), ("clo", PD), ("nlo", PL)]
+
+
+class Const(C.Structure):
+    _fields_ = [("mo", C.c_int32), ("isadac", C.c_int32), ("isfct", C.c_int32),
+                ("is_dye", C.c_int32), ("rkdye", C.c_double), ("con", PD),
+                ("con1", PD)]
+
+
+class OrcCtx(C.Structure):
+    _fields_ = [("LA", C.c_int32), ("LCM", C.c_int32), ("KC", C.c_int32),
+                ("KCM", C.c_int32),
+                ("LWC", PI), ("LEC", PI), ("LSC", PI), ("LNC", PI),
+                ("LNWC", PI), ("LSEC", PI),
+                ("a", PD * NARR),
+                ("G", C.c_double), ("RITB1", C.c_double), ("SNLT", C.c_double),
+                ("HDRY", C.c_double), ("TCON_TBEGIN", C.c_double),
+                ("RSQM", C.c_double), ("ITERM", C.c_int32),
+                ("nthreads", C.c_int32),
+                ("pbc", PbcSide * 4), ("mtide", C.c_int32), ("tcp", PD),
+                ("lopenbcdry", PI),
+                ("cbc", CbcSide * 4), ("wtci", PD),
+                ("nqsij", C.c_int32), ("lqs", PI), ("qss", PD),
+                ("qsercell", PD), ("cqs", PD), ("csert_q", PD),
+                ("nconst", C.c_int32), ("c", Const * MAXC)]
+
+
+class CgOut(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("rsq", C.c_double),
+                ("ccmnm", C.c_double), ("status", C.c_int32)]
+
+
+def build(force: bool = False) -> str:
+    src = os.path.join(HERE, "efdc_oracle.c")
+    if force or not os.path.exists(LIB) or \
+            os.path.getmtime(LIB) < os.path.getmtime(src):
+        subprocess.check_call(
+            ["gcc", "-O2", "-fopenmp", "-fPIC", "-shared",
+             "-ffp-contract=off", "-Wall", src, "-o", LIB, "-lm"])
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+        for fn, args in dict(
+                orc_congrad=[C.POINTER(OrcCtx), C.POINTER(CgOut)],
+                orc_calpuv2c=[C.POINTER(OrcCtx), C.c_double, C.c_int64,
+                              C.POINTER(CgOut)],
+                orc_assemble=[C.POINTER(OrcCtx), C.c_double, C.c_int64, PD],
+                orc_calconc=[C.POINTER(OrcCtx), C.c_double, C.c_int64],
+                orc_vertical_diffusion=[C.POINTER(OrcCtx), C.c_double],
+                orc_balance=[C.POINTER(OrcCtx), PD, PD],
+                orc_closure_pre=[C.POINTER(OrcCtx), PD],
+                orc_closure_post=[C.POINTER(OrcCtx)]).items():
+            f = getattr(_lib, fn)
+            f.argtypes = args
+            f.restype = C.c_int
+    return _lib
+
+
+def _pd(a):
+    return a.ctypes.data_as(PD)
+
+
+def _pi(a):
+    return a.ctypes.data_as(PI)
+
+
+class Oracle:
+    """Binds a Case to the C oracle; arrays are shared, not copied."""
+
+    def __init__(self, case, *, rsqm=1e-9, iterm=200, nthreads=1, g=9.81,
+                 ritb1=1.0, snlt=0.0, hdry=0.2, tcon_tbegin=0.0):
+        self.case = case
+        g_ = case.grid
+        o = OrcCtx()
+        self._keep = []
+        o.LA, o.LCM, o.KC, o.KCM = g_.LA, g_.LCM, g_.KC, g_.KCM
+        for m in ("LWC", "LEC", "LSC", "LNC", "LNWC", "LSEC"):
+            arr = np.ascontiguousarray(g_.maps[m], dtype=np.int32)
+            self._keep.append(arr)
+            setattr(o, m, _pi(arr))
+        for dct in (case.h, case.k, case.w):
+            for name, arr in dct.items():
+                assert arr.dtype == np.float64 and arr.flags.c_contiguous
+                o.a[ARRAY_IDS[name]] = _pd(arr)
+        o.G, o.RITB1, o.SNLT, o.HDRY = g, ritb1, snlt, hdry
+        o.TCON_TBEGIN, o.RSQM, o.ITERM, o.nthreads = tcon_tbegin, rsqm, iterm, nthreads
+        self.lopenbcdry = np.zeros(g_.LCM, np.int32)
+        o.lopenbcdry = _pi(self.lopenbcdry)
+        obc = case.open_bc
+        if obc is not None:
+            mt = int(obc["mtide"])
+            o.mtide = mt
+            tcp = np.ascontiguousarray(obc["tcp"], dtype=np.float64)
+            self._keep.append(tcp)
+            o.tcp = _pd(tcp)
+            for s, side in enumerate(obc["side"]):
+                if side is None:
+                    continue
+                arrs = dict(L=np.ascontiguousarray(side["L"], np.int32),
+                            ispb=np.ascontiguousarray(side["ispb"], np.int32),
+                            pser=np.ascontiguousarray(side["pser"], np.float64),
+                            pcb=np.ascontiguousarray(side["pcb"], np.float64).reshape(-1),
+                            psb=np.ascontiguousarray(side["psb"], np.float64).reshape(-1))
+                self._keep.append(arrs)
+                o.pbc[s].n = len(arrs["L"])
+                o.pbc[s].L, o.pbc[s].ispb = _pi(arrs["L"]), _pi(arrs["ispb"])
+                o.pbc[s].pser, o.pbc[s].pcb, o.pbc[s].psb = (
+                    _pd(arrs["pser"]), _pd(arrs["pcb"]), _pd(arrs["psb"]))
+        nc = len(case.consts)
+        cbc = case.conc_bc
+        self.cbc_mem = []
+        if cbc is not None:
+            wtci = np.ascontiguousarray(cbc["wtci"], np.float64).reshape(-1)
+            self._keep.append(wtci)
+            o.wtci = _pd(wtci)
+            for s, side in enumerate(cbc["side"]):
+                if side is None:
+                    continue
+                n = len(side["L"])
+                arrs = dict(L=np.ascontiguousarray(side["L"], np.int32),
+                            cb=np.ascontiguousarray(side["cb"], np.float64).reshape(-1),
+                            csert=np.ascontiguousarray(side["csert"], np.float64).reshape(-1),
+                            ntscr=np.ascontiguousarray(side["ntscr"], np.int32),
+                            clo=np.zeros(n * g_.KC * max(nc, 1)),
+                            nlo=np.zeros(n * g_.KC * max(nc, 1), np.int64))
+                self._keep.append(arrs)
+                self.cbc_mem.append(arrs)
+                b = o.cbc[s]
+                b.n = n
+                b.L, b.cb, b.csert = _pi(arrs["L"]), _pd(arrs["cb"]), _pd(arrs["csert"])
+                b.ntscr = _pi(arrs["ntscr"])
+                b.clo = _pd(arrs["clo"])
+                b.nlo = arrs["nlo"].ctypes.data_as(PL)
+        src = case.sources
+        if src is not None:
+            arrs = dict(LQS=np.ascontiguousarray(src["LQS"], np.int32),
+                        qss=np.ascontiguousarray(src["qss"], np.float64).reshape(-1),
+                        qsercell=np.ascontiguousarray(src["qsercell"], np.float64).reshape(-1),
+                        cqs=np.ascontiguousarray(src["cqs"], np.float64).reshape(-1),
+                        csert=np.ascontiguousarray(src["csert"], np.float64).reshape(-1))
+            self._keep.append(arrs)
+            o.nqsij = len(arrs["LQS"])
+            o.lqs = _pi(arrs["LQS"])
+            o.qss, o.qsercell = _pd(arrs["qss"]), _pd(arrs["qsercell"])
+            o.cqs, o.csert_q = _pd(arrs["cqs"]), _pd(arrs["csert"])
+        o.nconst = nc
+        for i, c in enumerate(case.consts):
+            o.c[i].mo, o.c[i].isadac, o.c[i].isfct = c["mo"], c["isadac"], c["isfct"]
+            o.c[i].is_dye, o.c[i].rkdye = c["is_dye"], c["rkdye"]
+            o.c[i].con, o.c[i].con1 = _pd(c["con"]), _pd(c["con1"])
+        self.o = o
+
+    # ------------------------------------------------------------ calls
+    def congrad(self):
+        out = CgOut()
+        lib().orc_congrad(C.byref(self.o), C.byref(out))
+        return dict(iters=out.iters, rsq=out.rsq, status=out.status)
+
+    def assemble(self, n=1, dt=None):
+        cm = C.c_double(0.0)
+        lib().orc_assemble(C.byref(self.o), dt or self.case.dt, n, C.byref(cm))
+        return cm.value
+
+    def calpuv2c(self, n=1, dt=None):
+        out = CgOut()
+        lib().orc_calpuv2c(C.byref(self.o), dt or self.case.dt, n, C.byref(out))
+        return dict(iters=out.iters, rsq=out.rsq, ccmnm=out.ccmnm,
+                    status=out.status)
+
+    def calconc(self, niter=1, dt=None):
+        return lib().orc_calconc(C.byref(self.o), dt or self.case.dt, niter)
+
+    def vertical_diffusion(self, dt=None):
+        return lib().orc_vertical_diffusion(C.byref(self.o), dt or self.case.dt)
+
+    def balance(self):
+        v = C.c_double(0.0)
+        m = np.zeros(MAXC)
+        lib().orc_balance(C.byref(self.o), C.byref(v), _pd(m))
+        return v.value, m[:len(self.case.consts)].copy()
+
+    def closure_pre(self):
+        nman = np.ascontiguousarray(self.case.manning_n, np.float64)
+        lib().orc_closure_pre(C.byref(self.o), _pd(nman))
+
+    def closure_post(self):
+        lib().orc_closure_post(C.byref(self.o))
+
+    def step(self, n):
+        """One 2TL hot-path step of the harness sequence (HDMT2T order,
+        EFDC/hdmt2t.f90:494-562): closure -> CALPUV2C -> closure ->
+        CALCONC."""
+        if self.case.manning_n is not None:
+            self.closure_pre()
+        st = self.calpuv2c(n=n)
+        self.closure_post()
+        if self.case.consts:
+            self.calconc(niter=n)
+        return st

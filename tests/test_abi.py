@@ -1,0 +1,66 @@
+"""The C-ABI library loads and exports every entry point declared in
+include/efdc_hip.h (no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "efdc_hip.h")
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\**\s*(efdc_\w+)\s*\(",
+                                 text, flags=re.M)))
+
+
+def test_header_declares_entry_points():
+    fns = declared_functions()
+    for need in ("efdc_ctx_create", "efdc_congrad", "efdc_calpuv2c",
+                 "efdc_calconc", "efdc_balance", "efdc_set_array"):
+        assert need in fns
+
+
+def test_library_exports_every_declared_symbol():
+    from efdc_amd.lib import LIB_PATH
+    assert os.path.exists(LIB_PATH), "build the library first"
+    lib = ctypes.CDLL(LIB_PATH)
+    missing = [f for f in declared_functions() if not hasattr(lib, f)]
+    assert not missing, missing
+
+
+def test_abi_version_and_defaults():
+    from efdc_amd import lib
+    L = lib.load()
+    assert L.efdc_abi_version() == 1
+    p = lib.default_params()
+    assert p.G == 9.81 and p.ITERM == 200 and p.RSQM == 1e-9
+    assert p.cg_mode == lib.CG_FP64 and p.store == lib.STORE_FP64
+
+
+def test_enum_values_match_layout_classes():
+    from efdc_amd.abi import ARRAY_IDS, shape_class
+    from efdc_amd import grids
+    for k in grids.H_KEYS:
+        assert shape_class(k) == "h", k
+    for k in grids.K_KEYS:
+        assert shape_class(k) == "k", k
+    for k in grids.W_KEYS:
+        assert shape_class(k) == "w", k
+    assert len(set(ARRAY_IDS.values())) == len(ARRAY_IDS)
+
+
+def test_context_without_device_fails_loudly():
+    """No CPU fallback: on a machine without a GPU, creating a context
+    reports EFDC_ERR_NODEVICE instead of computing anything."""
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    from efdc_amd import grids, lib
+    from efdc_amd.abi import STATUS
+    case = grids.tidal_basin(8, 4, 2)
+    with pytest.raises(lib.EfdcError) as e:
+        lib.Context(case.grid)
+    assert e.value.status == STATUS["EFDC_ERR_NODEVICE"]
