@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Benchmark of the external-mode solve at scale (CALPUV2C + CONGRAD).
+
+Workload (one "step" = one CALPUV2C call: explicit stage, assembly, open
+boundaries, min-diagonal scaling, CONGRAD to a relative residual of 1e-10,
+face transports, depths, HU/HV): an irregular lake on a 1024x1024 lattice
+(~696k wet cells, depths 2-40 m, dt = 10 s, a random free-surface
+perturbation with sigma = 0.05 m), time-stepped so that every solve starts
+from the previous step's P like the model does.
+
+The headline method is the FP32-refined CG (FP32 coefficients and SpMV,
+FP64 dot products, alpha/beta and residual refinement); the plain FP64
+Jacobi-PCG is timed beside it.  Inputs are resident in HBM before the
+timed region.  Accuracy: after the same number of steps both GPU paths
+are compared with the FP64 CPU oracle, which is also the CPU baseline
+(timed on a bounded sample of the same steps).
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "efdcplus8.5_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# Algorithmic bytes per wet cell moved by one launch of each kernel
+# (every array element the kernel must read or write counted once;
+# neighbour values are reuses of data already counted).
+BYTES_PER_CELL = {
+    # FP64 Jacobi-PCG, compact symmetric operator
+    "cg_spmv_pupdate": 72,   # r,CCCI,p_old,CCC,CCS,CCW (6x8) + LS,LN (2x4) + p,Ap (2x8)
+    "cg_axpy_dots": 56,      # P,p,r,Ap,CCCI (5x8) + P,r (2x8)
+    "cg_init": 64,           # FPTMP,CCC,CCS,CCW,CCCI,P (6x8) + LS,LN + r
+    # FP32 inner CG on the unit-diagonal operator
+    "mx_spmv_pupdate32": 32,  # r,p_old,as,aw (4x4) + LS,LN (2x4) + p,Ap (2x4)
+    "mx_axpy_dots32": 24,     # y,p,r,Ap (4x4) + y,r (2x4)
+    "mx_residual64": 64,      # FPTMP,CCC,CCS,CCW,P,D^-1/2 (6x8) + LS,LN + r',y (2x4)
+    "mx_apply": 28,           # P, D^-1/2 (2x8), y (4), P (8)
+    "mx_prep": 52,            # CCC,CCS,CCW,FPTMP (4x8) + LS + D^-1/2 (8) + as,aw (2x4)
+}
+CG_KERNELS_FP64 = ("cg_init", "cg_spmv_pupdate", "cg_axpy_dots")
+CG_KERNELS_MIXED = ("mx_prep", "mx_residual64", "mx_spmv_pupdate32",
+                    "mx_axpy_dots32", "mx_apply")
+
+
+def make_ctx(case, mixed):
+    from efdc_amd import lib
+    ctx = lib.Context(case.grid, cg_mode=lib.CG_MIXED if mixed else lib.CG_FP64,
+                      cg_rel_tol=1e-10, RSQM=0.0, ITERM=1000)
+    ctx.upload_case(case)
+    return ctx
+
+
+def run_steps(ctx, case, n0, k):
+    iters = []
+    for n in range(n0, n0 + k):
+        st = ctx.calpuv2c(case.dt, n)
+        iters.append(st["iters"])
+    return iters
+
+
+def timed(ctx, case, n0, k):
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    iters = run_steps(ctx, case, n0, k)
+    ctx.synchronize()
+    return time.perf_counter() - t0, iters
+
+
+def roofline(ctx, case, n0, k, kernels, nwet):
+    ctx.timers(True)
+    run_steps(ctx, case, n0, k)
+    t = ctx.read_timers()
+    ctx.timers(False)
+    per = {}
+    for name, (cnt, ms) in t.items():
+        per[name] = dict(launches=cnt, avg_us=1e3 * ms / cnt, total_ms=ms)
+    cg = {k_: per[k_] for k_ in kernels if k_ in per}
+    dom = max(cg, key=lambda k_: cg[k_]["total_ms"])
+    avg_s = cg[dom]["avg_us"] * 1e-6
+    bytes_launch = BYTES_PER_CELL[dom] * nwet
+    ach = bytes_launch / avg_s / 1e9
+    cg_bytes = sum(BYTES_PER_CELL[k_] * nwet * v["launches"] for k_, v in cg.items())
+    cg_ms = sum(v["total_ms"] for v in cg.values())
+    return dict(dominant=dom, achieved=ach, bytes_per_launch=bytes_launch,
+                avg_us=cg[dom]["avg_us"], cg_gbs=cg_bytes / (cg_ms * 1e-3) / 1e9,
+                cg_ms=cg_ms, kernels=per)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1024, help="lattice size")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(collectives and sharding are out of scope); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from efdc_amd import grids
+
+    case = grids.config1(n=args.n)
+    nwet = case.grid.nwet
+    ref_case = case.copy() if not args.no_cpu else None
+    ctx_m = make_ctx(case, mixed=True)
+    ctx_d = make_ctx(case, mixed=False)
+    i = np.arange(2, case.grid.LA + 1) - 1
+
+    # ---- accuracy + CPU baseline on a bounded sample of the same steps
+    cpu = None
+    ncmp = 2
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        o = orc.Oracle(ref_case, rsqm=0.0, iterm=1000, nthreads=args.cpu_threads,
+                       rel_tol=1e-10)
+        t_cpu = 0.0
+        done = 0
+        while done < 1 or (t_cpu < args.cpu_seconds and done < 50):
+            t0 = time.perf_counter()
+            st = o.calpuv2c(n=done + 1)
+            t_cpu += time.perf_counter() - t0
+            done += 1
+            if st["status"] != 0:
+                raise RuntimeError(f"oracle status {st}")
+        ncmp = done
+        cpu = dict(value=nwet * done / t_cpu, unit="cell-steps/s",
+                   cores=min(args.cpu_threads, 8), kind="port",
+                   sample=f"{done} CALPUV2C steps of the same {args.n}x{args.n} "
+                          f"lake by the FP64 C restatement (OpenMP, <=8-way "
+                          f"L-domains, serial dot products), {t_cpu:.1f} s")
+    run_steps(ctx_m, case, 1, ncmp)
+    run_steps(ctx_d, case, 1, ncmp)
+    err_m = err_d = None
+    if ref_case is not None:
+        pr = ref_case.h["P"][i]
+        err_m = float(np.max(np.abs(ctx_m.get("P")[i] - pr))) / 9.81
+        err_d = float(np.max(np.abs(ctx_d.get("P")[i] - pr))) / 9.81
+
+    # ---- timed region: FP32-refined, then FP64
+    n0 = ncmp + 1
+    run_steps(ctx_m, case, n0, args.warmup)
+    t_m, it_m = timed(ctx_m, case, n0 + args.warmup, args.steps)
+    run_steps(ctx_d, case, n0, args.warmup)
+    t_d, it_d = timed(ctx_d, case, n0 + args.warmup, args.steps)
+    n1 = n0 + args.warmup + args.steps
+    rf_m = roofline(ctx_m, case, n1, args.steps, CG_KERNELS_MIXED, nwet)
+    rf_d = roofline(ctx_d, case, n1, args.steps, CG_KERNELS_FP64, nwet)
+
+    value = nwet * args.steps / t_m
+    out = {
+        "metric": "cell-steps/s + CG HBM GB/s (% peak), FP32-refined vs FP64 at <=1e-6 m error",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_m / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"external-mode solve at scale: {args.n}x{args.n} irregular "
+                        f"lake, {nwet} wet cells, dt=10 s, CALPUV2C per step, "
+                        "CONGRAD to relative residual 1e-10",
+            "wet_cells": nwet,
+            "cg": "FP32 SpMV + FP64 dots + FP64 refinement",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": rf_m["achieved"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": rf_m["achieved"] / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": rf_m["dominant"],
+            "avg_us": rf_m["avg_us"],
+            "bytes_per_launch": rf_m["bytes_per_launch"],
+        },
+        "cpu_baseline": cpu,
+        "detail": {
+            "cg_gbs_mixed": rf_m["cg_gbs"],
+            "cg_gbs_fp64": rf_d["cg_gbs"],
+            "fp64_value": nwet * args.steps / t_d,
+            "fp64_ms_per_step": 1e3 * t_d / args.steps,
+            "fp64_dominant": rf_d["dominant"],
+            "fp64_dominant_gbs": rf_d["achieved"],
+            "cg_iters_per_step_mixed": float(np.mean(it_m)),
+            "cg_iters_per_step_fp64": float(np.mean(it_d)),
+            "eta_err_vs_oracle_m": {"mixed": err_m, "fp64": err_d,
+                                    "after_steps": ncmp},
+            "kernels_mixed": rf_m["kernels"],
+            "kernels_fp64": rf_d["kernels"],
+        },
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
 #include <set>
 
 #include "efdc_internal.h"
@@ -89,6 +90,24 @@ bool valid_id(int id) {
 }  // namespace
 
 int efdc_grid_blocks(int n) { return blocks_for(n); }
+
+const char *const efdc_timer_names[TK_COUNT] = {
+    "puv_explicit", "puv_assemble", "puv_openbc", "puv_scale", "cg_init",
+    "cg_spmv_pupdate", "cg_axpy_dots", "mx_prep", "mx_residual64",
+    "mx_spmv_pupdate32", "mx_axpy_dots32", "mx_apply", "puv_transports",
+    "puv_depths", "puv_hu_hv", "tran_upwind_vdiff", "tran_bc_con1",
+    "closure_pre", "closure_post", "balance"};
+
+hipEvent_t timer_event(efdc_ctx *c) {
+  if (c->ev_pool.empty()) {
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  hipEvent_t e = c->ev_pool.back();
+  c->ev_pool.pop_back();
+  return e;
+}
 
 extern "C" {
 
@@ -204,7 +223,8 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
   if ((s = dmalloc(c, &c->r, LCM)) || (s = dmalloc(c, &c->pb[0], LCM)) ||
       (s = dmalloc(c, &c->pb[1], LCM)) || (s = dmalloc(c, &c->ap, LCM)) ||
       (s = dmalloc(c, &c->part, (size_t)8 * c->max_blocks)) ||
-      (s = dmalloc(c, &c->st, 1)))
+      (s = dmalloc(c, &c->st, 1)) ||
+      (s = dmalloc(c, &c->tickets, (size_t)TICKET_SITES * TICKET_SITE)))
     return fail(s);
   if (prm->cg_mode == EFDC_CG_MIXED) {
     if ((s = dmalloc(c, &c->dis, LCM)) || (s = dmalloc(c, &c->as32, LCM)) ||
@@ -244,7 +264,7 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   dfree(c->r); dfree(c->pb[0]); dfree(c->pb[1]); dfree(c->ap);
   dfree(c->dis); dfree(c->as32); dfree(c->aw32); dfree(c->r32); dfree(c->y32);
   dfree(c->p32[0]); dfree(c->p32[1]); dfree(c->ap32);
-  dfree(c->part); dfree(c->st);
+  dfree(c->part); dfree(c->st); dfree(c->tickets);
   if (c->st_host) (void)hipHostFree(c->st_host);
   if (c->tide_host) (void)hipHostFree(c->tide_host);
   for (auto &b : c->pbc) {
@@ -268,6 +288,8 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   for (auto &p : c->adac) { char *q = (char *)p; dfree(q); p = nullptr; }
   dfree(c->cdzkk_k); dfree(c->cdzkmk_k); dfree(c->scal); dfree(c->nman);
   dfree(c->bccell);
+  for (auto &r : c->trec) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto &e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return EFDC_OK;
@@ -646,6 +668,37 @@ int efdc_set_sources(efdc_ctx_t *c, const efdc_sources_t *src) {
   HIP_TRY(c, hipMemcpyAsync(c->srcrec, rec.data(), sizeof(int) * c->LCM,
                             hipMemcpyHostToDevice, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
+int efdc_timers_enable(efdc_ctx_t *c, int32_t on) {
+  if (!c) return EFDC_ERR_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (auto &r : c->trec) {
+    c->ev_pool.push_back(r.a);
+    c->ev_pool.push_back(r.b);
+  }
+  c->trec.clear();
+  c->timing = on != 0;
+  return EFDC_OK;
+}
+
+int efdc_timers_read(efdc_ctx_t *c, efdc_timers_t *out) {
+  if (!c || !out) return EFDC_ERR_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  std::memset(out, 0, sizeof(*out));
+  out->n = TK_COUNT;
+  for (int k = 0; k < TK_COUNT; ++k)
+    std::snprintf(out->k[k].name, sizeof(out->k[k].name), "%s", efdc_timer_names[k]);
+  for (auto &r : c->trec) {
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, r.a, r.b));
+    out->k[r.id].count += 1;
+    out->k[r.id].total_ms += ms;
+    c->ev_pool.push_back(r.a);
+    c->ev_pool.push_back(r.b);
+  }
+  c->trec.clear();
   return EFDC_OK;
 }
 

@@ -36,7 +36,17 @@ struct CgState {
   double rsq64, bnorm2_in, rz_in;  // mixed mode: outer residual, inner rhs
   int iter, done, status, symok;   // symok: coefficients verified symmetric
   int negdep, outer, inner_iter, pad;
-  unsigned ticket[8];
+};
+
+// Arrival counters of the grid reductions: one site per reduction kernel,
+// each with 8 per-XCD counters and one global counter, 128 bytes apart so
+// that no two counters share a cache line.
+#define TICKET_PAD 32
+#define TICKET_SITE (9 * TICKET_PAD)
+#define TICKET_SITES 16
+enum {
+  TS_PUV_MIN = 0, TS_CG_INIT, TS_CG_A, TS_CG_B, TS_MX_PREP, TS_MX_RESID,
+  TS_MX_A, TS_MX_B, TS_BALANCE
 };
 
 struct PbcDev {           // one pressure open-BC side on the device
@@ -91,6 +101,7 @@ struct efdc_ctx {
   double *part = nullptr;      // reduction partials, 4 x max_blocks
   int max_blocks = 0;
   CgState *st = nullptr;       // device
+  unsigned *tickets = nullptr; // TICKET_SITES x TICKET_SITE counters
   CgState *st_host = nullptr;  // pinned mirror
   int last_iters = 0;
 
@@ -138,8 +149,38 @@ struct efdc_ctx {
   double *scal = nullptr;      // small result buffer (balance totals)
   double *nman = nullptr;      // per-cell Manning n of the closure
 
+  // kernel timers (efdc_timers_*)
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  struct TimerRec { int id; hipEvent_t a, b; };
+  std::vector<TimerRec> trec;
+
   int n_wet() const { return LA - 1; }
 };
+
+// kernel ids of the timers
+enum {
+  TK_PUV_EXPLICIT = 0, TK_PUV_ASSEMBLE, TK_PUV_BC, TK_PUV_SCALE,
+  TK_CG_INIT, TK_CG_A, TK_CG_B, TK_MX_PREP, TK_MX_RESID, TK_MX_A, TK_MX_B,
+  TK_MX_APPLY, TK_PUV_POST1, TK_PUV_POST2, TK_PUV_POST3, TK_TRAN,
+  TK_TRAN_BC, TK_CLOSURE_PRE, TK_CLOSURE_POST, TK_BALANCE, TK_COUNT
+};
+extern const char *const efdc_timer_names[TK_COUNT];
+
+hipEvent_t timer_event(efdc_ctx *c);
+// bracket one launch: TIMED(c, id, kernel<<<...>>>(...));
+#define TIMED(c, id, ...)                                     \
+  do {                                                        \
+    if ((c)->timing) {                                        \
+      efdc_ctx::TimerRec r_{(id), timer_event(c), timer_event(c)}; \
+      (void)hipEventRecord(r_.a, (c)->stream);                \
+      __VA_ARGS__;                                            \
+      (void)hipEventRecord(r_.b, (c)->stream);                \
+      (c)->trec.push_back(r_);                                \
+    } else {                                                  \
+      __VA_ARGS__;                                            \
+    }                                                         \
+  } while (0)
 
 // ---------------------------------------------------------- device utils
 // Workgroups are dispatched round-robin over the 8 XCDs of the MI355X.
@@ -187,11 +228,50 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
   __syncthreads();
 }
 
-// Deterministic grid reduction: every block stores its partials, the last
-// block to arrive (atomic ticket) sums the partials in block order with a
-// fixed tree, so the result does not depend on scheduling.  Returns true
-// in every thread of the last block; the sums are then in out[0..NV-1]
-// (valid in all threads of that block).  The ticket is reset for reuse.
+// Cross-workgroup publication without L2 write-backs.  On the multi-XCD
+// gfx950 every XCD has its own L2; an agent-scope release fence would write
+// back the whole L2 of the issuing XCD, once per workgroup.  Instead the
+// partial results are published with agent-scope atomic stores (written
+// through to the coherent memory side), the issuing wave waits for their
+// completion (s_waitcnt), and only then takes the ticket with an
+// agent-scope atomic add.  The last workgroup reads the partials with
+// agent-scope atomic loads, which are served coherently.
+__device__ __forceinline__ void publish(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double fetch(const double *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned atomic_inc(unsigned *t) {
+  return __hip_atomic_fetch_add(t, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void reset_ticket(unsigned *t) {
+  __hip_atomic_store(t, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Two-level arrival: a block counts in on the counter of its XCD (blocks
+// are dispatched round-robin, so XCD x runs the hardware blocks b with
+// b % 8 == x); the last block of each XCD counts in on the global
+// counter.  Returns true in the one block that arrives last overall.
+// Spreading the same-address atomics over 9 cache lines removes their
+// serialisation.
+__device__ __forceinline__ bool last_arrival(unsigned *site, int nblocks) {
+  __builtin_amdgcn_s_waitcnt(0);  // the published partials have completed
+  const int x = blockIdx.x & 7;
+  const int nx = (nblocks >> 3) + (x < (nblocks & 7) ? 1 : 0);
+  const int ngroups = nblocks < 8 ? nblocks : 8;
+  unsigned *gx = site + x * TICKET_PAD, *gg = site + 8 * TICKET_PAD;
+  if (atomic_inc(gx) != (unsigned)nx - 1u) return false;
+  reset_ticket(gx);
+  __builtin_amdgcn_s_waitcnt(0);
+  return atomic_inc(gg) == (unsigned)ngroups - 1u;
+}
+
+// Deterministic grid reduction: every block publishes its partials, the
+// last block to arrive (atomic ticket) sums the partials in block order
+// with a fixed tree, so the result does not depend on scheduling.
+// Returns true in every thread of the last block; the sums are then in
+// out[0..NV-1] (valid in all threads of that block).  The ticket is reset
+// for reuse by the next launch.
 template <int NV>
 __device__ __forceinline__ bool grid_sum(double (&v)[NV], double *part,
                                          unsigned *ticket, int nblocks,
@@ -200,20 +280,28 @@ __device__ __forceinline__ bool grid_sum(double (&v)[NV], double *part,
   block_sum<NV>(v);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int j = 0; j < NV; ++j) part[(size_t)j * nblocks + bid] = v[j];
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    am_last = (t == (unsigned)nblocks - 1u);
+    for (int j = 0; j < NV; ++j) publish(&part[(size_t)j * nblocks + bid], v[j]);
+    am_last = last_arrival(ticket, nblocks);
   }
   __syncthreads();
   if (!am_last) return false;
-  __threadfence();
+  // each thread sums part[tid], part[tid+B], part[tid+2B], ... in that
+  // order; the loads of 8 consecutive terms are issued together
   double acc[NV];
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
     double s = 0.0;
-    for (int q = threadIdx.x; q < nblocks; q += blockDim.x)
-      s += __builtin_nontemporal_load(&part[(size_t)j * nblocks + q]);
+    const double *pj = part + (size_t)j * nblocks;
+    for (int q0 = threadIdx.x; q0 < nblocks; q0 += 8 * (int)blockDim.x) {
+      double a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int q = q0 + u * (int)blockDim.x;
+        a[u] = q < nblocks ? fetch(&pj[q]) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += a[u];
+    }
     acc[j] = s;
   }
   block_sum<NV>(acc);
@@ -221,7 +309,7 @@ __device__ __forceinline__ bool grid_sum(double (&v)[NV], double *part,
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int j = 0; j < NV; ++j) res[j] = acc[j];
-    *ticket = 0u;
+    reset_ticket(ticket + 8 * TICKET_PAD);
   }
   __syncthreads();
 #pragma unroll
@@ -243,17 +331,22 @@ __device__ __forceinline__ bool grid_min(double v, double *part,
   if (threadIdx.x == 0) {
     double m = sh[0];
     for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmin(m, sh[q]);
-    part[bid] = m;
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    am_last = (t == (unsigned)nblocks - 1u);
+    publish(&part[bid], m);
+    am_last = last_arrival(ticket, nblocks);
   }
   __syncthreads();
   if (!am_last) return false;
-  __threadfence();
-  double m = 1.0e300;
-  for (int q = threadIdx.x; q < nblocks; q += blockDim.x)
-    m = fmin(m, __builtin_nontemporal_load(&part[q]));
+  double m = 1.0e18;
+  for (int q0 = threadIdx.x; q0 < nblocks; q0 += 8 * (int)blockDim.x) {
+    double a[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int q = q0 + u * (int)blockDim.x;
+      a[u] = q < nblocks ? fetch(&part[q]) : 1.0e18;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m = fmin(m, a[u]);
+  }
   m = wave_min(m);
   if (lane == 0) sh[w] = m;
   __syncthreads();
@@ -261,7 +354,7 @@ __device__ __forceinline__ bool grid_min(double v, double *part,
     double mm = sh[0];
     for (int q = 1; q < (int)(blockDim.x >> 6); ++q) mm = fmin(mm, sh[q]);
     res = mm;
-    *ticket = 0u;
+    reset_ticket(ticket + 8 * TICKET_PAD);
   }
   __syncthreads();
   out = res;

@@ -32,6 +32,12 @@ HPtr hptr(efdc_ctx *c) {
 
 #define HA(id) h.a[EFDC_##id]
 
+// Cells per thread of the CG kernels: each workgroup owns a contiguous run
+// of CG_CPT*256 cells, which halves the number of workgroups (and of
+// reduction arrivals) per launch.
+#define CG_CPT 2
+int cg_blocks(int nwet) { return (nwet + CG_CPT * EFDC_BLOCK - 1) / (CG_CPT * EFDC_BLOCK); }
+
 __device__ __forceinline__ int cell_index(int nwet, int &tile) {
   tile = xcd_tile(blockIdx.x, gridDim.x);
   return 1 + tile * EFDC_BLOCK + (int)threadIdx.x;  // wet iff <= nwet
@@ -85,7 +91,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_explicit(
 __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_assemble(
     HPtr h, const int *__restrict__ le, const int *__restrict__ ln,
     const unsigned char *__restrict__ pbcflag, int nwet, double dt, double G,
-    double *part, CgState *st) {
+    double *part, CgState *st, unsigned *st_tk) {
   int tile;
   const int i = cell_index(nwet, tile);
   double ccv = 1.E+18;
@@ -112,7 +118,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_assemble(
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) st->symok = 0;  // reset flags
   double m;
-  if (grid_min(ccv, part, &st->ticket[0], gridDim.x, tile, m) &&
+  if (grid_min(ccv, part, st_tk + TS_PUV_MIN * TICKET_SITE, gridDim.x, tile, m) &&
       threadIdx.x == 0)
     st->ccmnm = m;
 }
@@ -342,6 +348,7 @@ struct CgArgs {
   int rowmajor;
   CgState *st;
   double *part;
+  unsigned *tk;
 };
 
 // Off-diagonal coefficients of row i in the reference's sense, with
@@ -376,23 +383,26 @@ __device__ __forceinline__ Row row_of(const CgArgs &a, int i, bool fast) {
 
 __global__ __launch_bounds__(EFDC_BLOCK) void k_cg_init(CgArgs a, double rsqm,
                                                         double rel_tol) {
-  int tile;
-  const int i = cell_index(a.nwet, tile);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const bool fast = a.rowmajor && (a.st->symok == 0);
   double v[2] = {0.0, 0.0};
-  if (i <= a.nwet) {
-    const Row q = row_of(a, i, fast);
-    const double *P = a.x;
-    // RCG = FPTMP - CCC*P - CCN*PN - CCS*PS - CCW*PW - CCE*PE (:73)
-    const double r = a.fptmp[i] - a.ccc[i] * P[i] - q.cn * P[q.jn] -
-                     q.cs * P[q.js] - q.cw * P[q.jw] - q.ce * P[q.je];
-    a.r[i] = r;
-    const double z = r * a.ccci[i];  // PCG = RCG*CCCI (:80)
-    v[0] = r * z;
-    v[1] = a.fptmp[i] * a.fptmp[i];
+  #pragma unroll
+  for (int u = 0; u < CG_CPT; ++u) {
+    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+    if (i <= a.nwet) {
+      const Row q = row_of(a, i, fast);
+      const double *P = a.x;
+      // RCG = FPTMP - CCC*P - CCN*PN - CCS*PS - CCW*PW - CCE*PE (:73)
+      const double r = a.fptmp[i] - a.ccc[i] * P[i] - q.cn * P[q.jn] -
+                       q.cs * P[q.js] - q.cw * P[q.jw] - q.ce * P[q.je];
+      a.r[i] = r;
+      const double z = r * a.ccci[i];  // PCG = RCG*CCCI (:80)
+      v[0] += r * z;
+      v[1] += a.fptmp[i] * a.fptmp[i];
+    }
   }
   double s[2];
-  if (grid_sum<2>(v, a.part, &a.st->ticket[1], gridDim.x, tile, s) &&
+  if (grid_sum<2>(v, a.part, a.tk + TS_CG_INIT * TICKET_SITE, gridDim.x, tile, s) &&
       threadIdx.x == 0) {
     CgState *st = a.st;
     st->rpcg = s[0];
@@ -417,26 +427,29 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_cg_a(CgArgs a,
                                                      const double *pold,
                                                      double *pnew) {
   if (a.st->done) return;
-  int tile;
-  const int i = cell_index(a.nwet, tile);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const bool fast = a.rowmajor && (a.st->symok == 0);
   const double beta = a.st->beta;
   double v[1] = {0.0};
-  if (i <= a.nwet) {
-    const Row q = row_of(a, i, fast);
-    const double pc = a.ccci[i] * a.r[i] + beta * pold[i];
-    const double ps = pnew_at(a, pold, beta, q.js);
-    const double pn = pnew_at(a, pold, beta, q.jn);
-    const double pw = pnew_at(a, pold, beta, q.jw);
-    const double pe = pnew_at(a, pold, beta, q.je);
-    // APCG = CCC*PCG + CCS*PS + CCN*PN + CCW*PW + CCE*PE (:113)
-    const double apc = a.ccc[i] * pc + q.cs * ps + q.cn * pn + q.cw * pw + q.ce * pe;
-    pnew[i] = pc;
-    a.ap[i] = apc;
-    v[0] = apc * pc;
+  #pragma unroll
+  for (int u = 0; u < CG_CPT; ++u) {
+    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+    if (i <= a.nwet) {
+      const Row q = row_of(a, i, fast);
+      const double pc = a.ccci[i] * a.r[i] + beta * pold[i];
+      const double ps = pnew_at(a, pold, beta, q.js);
+      const double pn = pnew_at(a, pold, beta, q.jn);
+      const double pw = pnew_at(a, pold, beta, q.jw);
+      const double pe = pnew_at(a, pold, beta, q.je);
+      // APCG = CCC*PCG + CCS*PS + CCN*PN + CCW*PW + CCE*PE (:113)
+      const double apc = a.ccc[i] * pc + q.cs * ps + q.cn * pn + q.cw * pw + q.ce * pe;
+      pnew[i] = pc;
+      a.ap[i] = apc;
+      v[0] += apc * pc;
+    }
   }
   double s[1];
-  if (grid_sum<1>(v, a.part, &a.st->ticket[2], gridDim.x, tile, s) &&
+  if (grid_sum<1>(v, a.part, a.tk + TS_CG_A * TICKET_SITE, gridDim.x, tile, s) &&
       threadIdx.x == 0) {
     a.st->papcg = s[0];
     a.st->alpha = a.st->rpcg / s[0];  // :124
@@ -447,20 +460,23 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_cg_b(CgArgs a,
                                                      const double *pnew,
                                                      int iterm) {
   if (a.st->done) return;
-  int tile;
-  const int i = cell_index(a.nwet, tile);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const double alpha = a.st->alpha;
   double v[2] = {0.0, 0.0};
-  if (i <= a.nwet) {
-    a.x[i] = a.x[i] + alpha * pnew[i];       // :132
-    const double r = a.r[i] - alpha * a.ap[i];  // :133
-    a.r[i] = r;
-    const double z = a.ccci[i] * r;          // :134
-    v[0] = r * z;
-    v[1] = r * r;
+  #pragma unroll
+  for (int u = 0; u < CG_CPT; ++u) {
+    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+    if (i <= a.nwet) {
+      a.x[i] = a.x[i] + alpha * pnew[i];       // :132
+      const double r = a.r[i] - alpha * a.ap[i];  // :133
+      a.r[i] = r;
+      const double z = a.ccci[i] * r;          // :134
+      v[0] += r * z;
+      v[1] += r * r;
+    }
   }
   double s[2];
-  if (grid_sum<2>(v, a.part, &a.st->ticket[3], gridDim.x, tile, s) &&
+  if (grid_sum<2>(v, a.part, a.tk + TS_CG_B * TICKET_SITE, gridDim.x, tile, s) &&
       threadIdx.x == 0) {
     CgState *st = a.st;
     const double beta = s[0] / st->rpcg;  // :147-148
@@ -487,23 +503,26 @@ struct MxArgs {
 
 __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_prep(MxArgs m, double rsqm,
                                                         double rel_tol) {
-  int tile;
   const CgArgs &a = m.g;
-  const int i = cell_index(a.nwet, tile);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   double v[1] = {0.0};
-  if (i <= a.nwet) {
-    const double di = 1.0 / sqrt(a.ccc[i]);
-    m.dis[i] = di;
-    const int js = a.ls[i];
-    const double ds = (js != a.lc0) ? 1.0 / sqrt(a.ccc[js]) : 0.0;
-    const double dw = (i - 1 > 0) ? 1.0 / sqrt(a.ccc[i - 1]) : 0.0;
-    const double cs = (js != a.lc0) ? a.ccs[i] : 0.0;
-    m.as[i] = (float)(cs * di * ds);
-    m.aw[i] = (float)(a.ccw[i] * di * dw);
-    v[0] = a.fptmp[i] * a.fptmp[i];
+  #pragma unroll
+  for (int u = 0; u < CG_CPT; ++u) {
+    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+    if (i <= a.nwet) {
+      const double di = 1.0 / sqrt(a.ccc[i]);
+      m.dis[i] = di;
+      const int js = a.ls[i];
+      const double ds = (js != a.lc0) ? 1.0 / sqrt(a.ccc[js]) : 0.0;
+      const double dw = (i - 1 > 0) ? 1.0 / sqrt(a.ccc[i - 1]) : 0.0;
+      const double cs = (js != a.lc0) ? a.ccs[i] : 0.0;
+      m.as[i] = (float)(cs * di * ds);
+      m.aw[i] = (float)(a.ccw[i] * di * dw);
+      v[0] += a.fptmp[i] * a.fptmp[i];
+    }
   }
   double s[1];
-  if (grid_sum<1>(v, a.part, &a.st->ticket[4], gridDim.x, tile, s) &&
+  if (grid_sum<1>(v, a.part, a.tk + TS_MX_PREP * TICKET_SITE, gridDim.x, tile, s) &&
       threadIdx.x == 0) {
     a.st->bnorm2 = s[0];
     a.st->rsqm = rel_tol > 0.0 ? rel_tol * rel_tol * s[0] : rsqm;
@@ -516,23 +535,26 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_prep(MxArgs m, double rsqm,
 // r64 = FPTMP - A P (FP64, compact operator), r' = D^-1/2 r64 in FP32,
 // y = 0; starts one inner solve.
 __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_resid(MxArgs m) {
-  int tile;
   const CgArgs &a = m.g;
-  const int i = cell_index(a.nwet, tile);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   double v[2] = {0.0, 0.0};
-  if (i <= a.nwet) {
-    const Row q = row_of(a, i, true);
-    const double *P = a.x;
-    const double r = a.fptmp[i] - a.ccc[i] * P[i] - q.cn * P[q.jn] -
-                     q.cs * P[q.js] - q.cw * P[q.jw] - q.ce * P[q.je];
-    const float rs = (float)(m.dis[i] * r);
-    m.r32[i] = rs;
-    m.y32[i] = 0.0f;
-    v[0] = r * r;
-    v[1] = (double)rs * (double)rs;
+  #pragma unroll
+  for (int u = 0; u < CG_CPT; ++u) {
+    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+    if (i <= a.nwet) {
+      const Row q = row_of(a, i, true);
+      const double *P = a.x;
+      const double r = a.fptmp[i] - a.ccc[i] * P[i] - q.cn * P[q.jn] -
+                       q.cs * P[q.js] - q.cw * P[q.jw] - q.ce * P[q.je];
+      const float rs = (float)(m.dis[i] * r);
+      m.r32[i] = rs;
+      m.y32[i] = 0.0f;
+      v[0] += r * r;
+      v[1] += (double)rs * (double)rs;
+    }
   }
   double s[2];
-  if (grid_sum<2>(v, a.part, &a.st->ticket[5], gridDim.x, tile, s) &&
+  if (grid_sum<2>(v, a.part, a.tk + TS_MX_RESID * TICKET_SITE, gridDim.x, tile, s) &&
       threadIdx.x == 0) {
     CgState *st = a.st;
     st->rsq64 = s[0];
@@ -556,26 +578,29 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_a(MxArgs m, const float *pold
                                                      float *pnew) {
   const CgArgs &a = m.g;
   if (a.st->done) return;
-  int tile;
-  const int i = cell_index(a.nwet, tile);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const double beta = a.st->beta;
   double v[1] = {0.0};
-  if (i <= a.nwet) {
-    const int js = a.ls[i], jn = a.ln[i];
-    const float pc = p32_at(m, pold, beta, i);
-    const float ps = p32_at(m, pold, beta, js);
-    const float pn = p32_at(m, pold, beta, jn);
-    const float pw = p32_at(m, pold, beta, i - 1);
-    const float pe = p32_at(m, pold, beta, i + 1);
-    const float an = (jn != a.lc0) ? m.as[jn] : 0.0f;
-    const float ae = (i + 1 != a.lc0) ? m.aw[i + 1] : 0.0f;
-    const float apc = pc + m.as[i] * ps + an * pn + m.aw[i] * pw + ae * pe;
-    pnew[i] = pc;
-    m.ap32[i] = apc;
-    v[0] = (double)apc * (double)pc;
+  #pragma unroll
+  for (int u = 0; u < CG_CPT; ++u) {
+    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+    if (i <= a.nwet) {
+      const int js = a.ls[i], jn = a.ln[i];
+      const float pc = p32_at(m, pold, beta, i);
+      const float ps = p32_at(m, pold, beta, js);
+      const float pn = p32_at(m, pold, beta, jn);
+      const float pw = p32_at(m, pold, beta, i - 1);
+      const float pe = p32_at(m, pold, beta, i + 1);
+      const float an = (jn != a.lc0) ? m.as[jn] : 0.0f;
+      const float ae = (i + 1 != a.lc0) ? m.aw[i + 1] : 0.0f;
+      const float apc = pc + m.as[i] * ps + an * pn + m.aw[i] * pw + ae * pe;
+      pnew[i] = pc;
+      m.ap32[i] = apc;
+      v[0] += (double)apc * (double)pc;
+    }
   }
   double s[1];
-  if (grid_sum<1>(v, a.part, &a.st->ticket[6], gridDim.x, tile, s) &&
+  if (grid_sum<1>(v, a.part, a.tk + TS_MX_A * TICKET_SITE, gridDim.x, tile, s) &&
       threadIdx.x == 0)
     a.st->alpha = a.st->rz_in / s[0];
 }
@@ -584,18 +609,21 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_b(MxArgs m, const float *pnew
                                                      double rtol2, int cap) {
   const CgArgs &a = m.g;
   if (a.st->done) return;
-  int tile;
-  const int i = cell_index(a.nwet, tile);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const double alpha = a.st->alpha;
   double v[1] = {0.0};
-  if (i <= a.nwet) {
-    m.y32[i] = (float)((double)m.y32[i] + alpha * (double)pnew[i]);
-    const float r = (float)((double)m.r32[i] - alpha * (double)m.ap32[i]);
-    m.r32[i] = r;
-    v[0] = (double)r * (double)r;
+  #pragma unroll
+  for (int u = 0; u < CG_CPT; ++u) {
+    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+    if (i <= a.nwet) {
+      m.y32[i] = (float)((double)m.y32[i] + alpha * (double)pnew[i]);
+      const float r = (float)((double)m.r32[i] - alpha * (double)m.ap32[i]);
+      m.r32[i] = r;
+      v[0] += (double)r * (double)r;
+    }
   }
   double s[1];
-  if (grid_sum<1>(v, a.part, &a.st->ticket[7], gridDim.x, tile, s) &&
+  if (grid_sum<1>(v, a.part, a.tk + TS_MX_B * TICKET_SITE, gridDim.x, tile, s) &&
       threadIdx.x == 0) {
     CgState *st = a.st;
     st->beta = s[0] / st->rz_in;
@@ -607,10 +635,13 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_b(MxArgs m, const float *pnew
 }
 
 __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_apply(MxArgs m) {
-  int tile;
   const CgArgs &a = m.g;
-  const int i = cell_index(a.nwet, tile);
-  if (i <= a.nwet) a.x[i] = a.x[i] + m.dis[i] * (double)m.y32[i];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+#pragma unroll
+  for (int u = 0; u < CG_CPT; ++u) {
+    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+    if (i <= a.nwet) a.x[i] = a.x[i] + m.dis[i] * (double)m.y32[i];
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) a.st->outer += 1;
 }
 
@@ -694,6 +725,7 @@ CgArgs cg_args(efdc_ctx *c) {
   a.rowmajor = c->rowmajor ? 1 : 0;
   a.st = c->st;
   a.part = c->part;
+  a.tk = c->tickets;
   return a;
 }
 
@@ -705,16 +737,16 @@ int read_state(efdc_ctx *c) {
 }
 
 int cg_fp64(efdc_ctx *c) {
-  const int nb = efdc_grid_blocks(c->n_wet());
+  const int nb = cg_blocks(c->n_wet());
   CgArgs a = cg_args(c);
   const int iterm = c->prm.ITERM;
-  k_cg_init<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->prm.RSQM, c->prm.cg_rel_tol);
+  TIMED(c, TK_CG_INIT, k_cg_init<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->prm.RSQM, c->prm.cg_rel_tol));
   int it = 0;
   int chunk = c->last_iters > 0 ? c->last_iters : 8;
   for (;;) {
     for (int j = 0; j < chunk && it < iterm; ++j, ++it) {
-      k_cg_a<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->pb[it & 1], c->pb[(it + 1) & 1]);
-      k_cg_b<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->pb[(it + 1) & 1], iterm);
+      TIMED(c, TK_CG_A, k_cg_a<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->pb[it & 1], c->pb[(it + 1) & 1]));
+      TIMED(c, TK_CG_B, k_cg_b<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->pb[(it + 1) & 1], iterm));
     }
     int s = read_state(c);
     if (s) return s;
@@ -726,7 +758,7 @@ int cg_fp64(efdc_ctx *c) {
 }
 
 int cg_mixed(efdc_ctx *c) {
-  const int nb = efdc_grid_blocks(c->n_wet());
+  const int nb = cg_blocks(c->n_wet());
   MxArgs m;
   m.g = cg_args(c);
   m.dis = c->dis;
@@ -737,10 +769,10 @@ int cg_mixed(efdc_ctx *c) {
   m.ap32 = c->ap32;
   const double rtol = c->prm.mixed_inner_rtol > 0 ? c->prm.mixed_inner_rtol : 1e-5;
   const int cap = c->prm.ITERM;
-  k_mx_prep<<<nb, EFDC_BLOCK, 0, c->stream>>>(m, c->prm.RSQM, c->prm.cg_rel_tol);
+  TIMED(c, TK_MX_PREP, k_mx_prep<<<nb, EFDC_BLOCK, 0, c->stream>>>(m, c->prm.RSQM, c->prm.cg_rel_tol));
   int outer = 0, total = 0;
   for (;;) {
-    k_mx_resid<<<nb, EFDC_BLOCK, 0, c->stream>>>(m);
+    TIMED(c, TK_MX_RESID, k_mx_resid<<<nb, EFDC_BLOCK, 0, c->stream>>>(m));
     int s = read_state(c);
     if (s) return s;
     if (c->st_host->rsq64 <= c->st_host->rsqm) break;
@@ -753,8 +785,8 @@ int cg_mixed(efdc_ctx *c) {
     int chunk = c->last_iters > 0 ? c->last_iters : 8;
     for (;;) {
       for (int j = 0; j < chunk && it < cap; ++j, ++it) {
-        k_mx_a<<<nb, EFDC_BLOCK, 0, c->stream>>>(m, c->p32[it & 1], c->p32[(it + 1) & 1]);
-        k_mx_b<<<nb, EFDC_BLOCK, 0, c->stream>>>(m, c->p32[(it + 1) & 1], rtol * rtol, cap);
+        TIMED(c, TK_MX_A, k_mx_a<<<nb, EFDC_BLOCK, 0, c->stream>>>(m, c->p32[it & 1], c->p32[(it + 1) & 1]));
+        TIMED(c, TK_MX_B, k_mx_b<<<nb, EFDC_BLOCK, 0, c->stream>>>(m, c->p32[(it + 1) & 1], rtol * rtol, cap));
       }
       s = read_state(c);
       if (s) return s;
@@ -763,7 +795,7 @@ int cg_mixed(efdc_ctx *c) {
     }
     c->last_iters = c->st_host->inner_iter;
     total += c->st_host->inner_iter;
-    k_mx_apply<<<nb, EFDC_BLOCK, 0, c->stream>>>(m);
+    TIMED(c, TK_MX_APPLY, k_mx_apply<<<nb, EFDC_BLOCK, 0, c->stream>>>(m));
     ++outer;
   }
   c->st_host->iter = total;
@@ -816,17 +848,18 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
   HPtr h = hptr(c);
   k_reset_flags<<<1, 1, 0, c->stream>>>(c->st);
   if (c->store == EFDC_STORE_FP64)
-    k_puv_explicit<double><<<nb, EFDC_BLOCK, 0, c->stream>>>(
+    TIMED(c, TK_PUV_EXPLICIT, k_puv_explicit<double><<<nb, EFDC_BLOCK, 0, c->stream>>>(
         h, c->dLW, c->dLS, c->dLE, c->dLN, (const double *)c->dK[EFDC_HPK - EFDC_HPK],
         (double *)c->dK[EFDC_H1PK - EFDC_HPK], c->pitch, c->KC, nwet, dt, G,
-        c->prm.RITB1, c->prm.SNLT);
+        c->prm.RITB1, c->prm.SNLT));
   else
-    k_puv_explicit<float><<<nb, EFDC_BLOCK, 0, c->stream>>>(
+    TIMED(c, TK_PUV_EXPLICIT, k_puv_explicit<float><<<nb, EFDC_BLOCK, 0, c->stream>>>(
         h, c->dLW, c->dLS, c->dLE, c->dLN, (const float *)c->dK[EFDC_HPK - EFDC_HPK],
         (float *)c->dK[EFDC_H1PK - EFDC_HPK], c->pitch, c->KC, nwet, dt, G,
-        c->prm.RITB1, c->prm.SNLT);
-  k_puv_assemble<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->pbcflag,
-                                                   nwet, dt, G, c->part, c->st);
+        c->prm.RITB1, c->prm.SNLT));
+  TIMED(c, TK_PUV_ASSEMBLE, k_puv_assemble<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->pbcflag,
+                                                   nwet, dt, G, c->part, c->st,
+                                                   c->tickets));
   if (c->npbc_total > 0) {
     // tidal phase on the host in FP64 like SETOPENBC (:22-32)
     const double PI2 = 2. * 3.1415926535898;  // EFDC/aaefdc.f90:650-651
@@ -846,18 +879,18 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
     b.tide = c->tide_dev;
     b.mtide = c->mtide;
     b.lopenbcdry = c->lopenbcdry;
-    k_puv_bc<<<1, EFDC_BLOCK, 0, c->stream>>>(h, b, c->dLW, c->dLE, c->dLS, c->dLN,
-                                              dt, G, c->prm.HDRY, c->st);
+    TIMED(c, TK_PUV_BC, k_puv_bc<<<1, EFDC_BLOCK, 0, c->stream>>>(h, b, c->dLW, c->dLE, c->dLS, c->dLN,
+                                              dt, G, c->prm.HDRY, c->st));
   }
-  k_puv_scale<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLE, c->dLN, nwet,
-                                                lc0, c->st);
+  TIMED(c, TK_PUV_SCALE, k_puv_scale<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLE, c->dLN, nwet,
+                                                lc0, c->st));
   int s = cg_dispatch(c, stats);
   if (s != EFDC_OK && s != EFDC_ERR_CG_MAXITER) return s;
   const int cgs = s;
-  k_puv_post1<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, dt);
-  k_puv_post2<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->pbcflag,
-                                                nwet, dt, G);
-  k_puv_post3<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, c->st);
+  TIMED(c, TK_PUV_POST1, k_puv_post1<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, dt));
+  TIMED(c, TK_PUV_POST2, k_puv_post2<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->pbcflag,
+                                                nwet, dt, G));
+  TIMED(c, TK_PUV_POST3, k_puv_post3<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, c->st));
   if ((s = read_state(c))) return s;
   if (stats) stats->ccmnm = c->st_host->ccmnm;
   if (cgs != EFDC_OK) return cgs;

@@ -385,11 +385,11 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
     return EFDC_ERR_ARG;
   }
   const int nb = (a.nwet + tb - 1) / tb;
-  k_tran_upwind_vdiff<T><<<nb, tb, lds_for(tb), c->stream>>>(a);
+  TIMED(c, TK_TRAN, k_tran_upwind_vdiff<T><<<nb, tb, lds_for(tb), c->stream>>>(a));
   if (c->nbcrec > 0) {
     const int tot = c->nbcrec * KC;
-    k_tran_bc_con1<T><<<(tot + 255) / 256, 256, 0, c->stream>>>(a, c->bccell,
-                                                                c->nbcrec);
+    TIMED(c, TK_TRAN_BC, k_tran_bc_con1<T><<<(tot + 255) / 256, 256, 0, c->stream>>>(a, c->bccell,
+                                                                c->nbcrec));
   }
   for (int j = 0; j < nc; ++j) c->con[j].cur ^= 1;
   return EFDC_OK;
@@ -411,17 +411,17 @@ int balance_run(efdc_ctx *c, efdc_balance_t *out) {
   double *res = c->scal;
   for (int j = -1; j < c->nconst; ++j) {
     if (c->store == EFDC_STORE_FP64)
-      k_balance<double><<<nb, EFDC_BLOCK, 0, c->stream>>>(
+      TIMED(c, TK_BALANCE, k_balance<double><<<nb, EFDC_BLOCK, 0, c->stream>>>(
           c->dH[EFDC_DXYP], c->dH[EFDC_HP],
           j < 0 ? nullptr : (const double *)c->con[j].buf[c->con[j].cur],
           (const double *)c->dK[EFDC_DZC - EFDC_HPK], c->pitch, c->KC,
-          c->n_wet(), c->part, &c->st->ticket[1], res + (j + 1));
+          c->n_wet(), c->part, c->tickets + TS_BALANCE * TICKET_SITE, res + (j + 1)));
     else
-      k_balance<float><<<nb, EFDC_BLOCK, 0, c->stream>>>(
+      TIMED(c, TK_BALANCE, k_balance<float><<<nb, EFDC_BLOCK, 0, c->stream>>>(
           c->dH[EFDC_DXYP], c->dH[EFDC_HP],
           j < 0 ? nullptr : (const float *)c->con[j].buf[c->con[j].cur],
           (const float *)c->dK[EFDC_DZC - EFDC_HPK], c->pitch, c->KC,
-          c->n_wet(), c->part, &c->st->ticket[1], res + (j + 1));
+          c->n_wet(), c->part, c->tickets + TS_BALANCE * TICKET_SITE, res + (j + 1)));
   }
   std::vector<double> h(c->nconst + 1);
   HIP_TRY(c, hipMemcpyAsync(h.data(), res, sizeof(double) * (c->nconst + 1),
@@ -435,9 +435,9 @@ int balance_run(efdc_ctx *c, efdc_balance_t *out) {
 
 int closure_pre_run(efdc_ctx *c, const double *nman) {
   const int nwet = c->n_wet();
-  k_closure_pre<<<(nwet + 255) / 256, 256, 0, c->stream>>>(
+  TIMED(c, TK_CLOSURE_PRE, k_closure_pre<<<(nwet + 255) / 256, 256, 0, c->stream>>>(
       c->dH[EFDC_TBX], c->dH[EFDC_TBY], c->dH[EFDC_UHE], c->dH[EFDC_VHE],
-      c->dH[EFDC_HU], c->dH[EFDC_HV], nman, c->prm.G, nwet);
+      c->dH[EFDC_HU], c->dH[EFDC_HV], nman, c->prm.G, nwet));
   HIP_TRY(c, hipGetLastError());
   return EFDC_OK;
 }
@@ -446,21 +446,21 @@ int closure_post_run(efdc_ctx *c) {
   const int nwet = c->n_wet();
   const int nb = (nwet + 255) / 256;
   if (c->store == EFDC_STORE_FP64)
-    k_closure_post<double><<<nb, 256, 0, c->stream>>>(
+    TIMED(c, TK_CLOSURE_POST, k_closure_post<double><<<nb, 256, 0, c->stream>>>(
         c->dH[EFDC_UHDYE], c->dH[EFDC_UHDY1E], c->dH[EFDC_VHDXE],
         c->dH[EFDC_VHDX1E], c->dH[EFDC_HP], c->dH[EFDC_HU], c->dH[EFDC_HV],
         c->dH[EFDC_DYIU], c->dH[EFDC_DXIV], kp<double>(c, EFDC_DZC),
         kp<double>(c, EFDC_UHDY2), kp<double>(c, EFDC_VHDX2),
         kp<double>(c, EFDC_U2), kp<double>(c, EFDC_V2), kp<double>(c, EFDC_HPK),
-        kp<double>(c, EFDC_HPKI), c->pitch, c->KC, nwet);
+        kp<double>(c, EFDC_HPKI), c->pitch, c->KC, nwet));
   else
-    k_closure_post<float><<<nb, 256, 0, c->stream>>>(
+    TIMED(c, TK_CLOSURE_POST, k_closure_post<float><<<nb, 256, 0, c->stream>>>(
         c->dH[EFDC_UHDYE], c->dH[EFDC_UHDY1E], c->dH[EFDC_VHDXE],
         c->dH[EFDC_VHDX1E], c->dH[EFDC_HP], c->dH[EFDC_HU], c->dH[EFDC_HV],
         c->dH[EFDC_DYIU], c->dH[EFDC_DXIV], kp<float>(c, EFDC_DZC),
         kp<float>(c, EFDC_UHDY2), kp<float>(c, EFDC_VHDX2), kp<float>(c, EFDC_U2),
         kp<float>(c, EFDC_V2), kp<float>(c, EFDC_HPK), kp<float>(c, EFDC_HPKI),
-        c->pitch, c->KC, nwet);
+        c->pitch, c->KC, nwet));
   HIP_TRY(c, hipGetLastError());
   return EFDC_OK;
 }

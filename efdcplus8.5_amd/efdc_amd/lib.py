@@ -84,6 +84,15 @@ class CgStats(C.Structure):
                 ("rsqm_used", C.c_double), ("ccmnm", C.c_double)]
 
 
+class TimerEntry(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("count", C.c_int64),
+                ("total_ms", C.c_double)]
+
+
+class Timers(C.Structure):
+    _fields_ = [("n", C.c_int32), ("k", TimerEntry * 32)]
+
+
 class Balance(C.Structure):
     _fields_ = [("volume", C.c_double), ("mass", C.c_double * 16),
                 ("nconst", C.c_int32)]
@@ -128,6 +137,8 @@ def load(path: Optional[str] = None):
         "efdc_closure_set_manning": ([P, PD], C.c_int),
         "efdc_closure_pre": ([P], C.c_int),
         "efdc_closure_post": ([P], C.c_int),
+        "efdc_timers_enable": ([P, C.c_int32], C.c_int),
+        "efdc_timers_read": ([P, C.POINTER(Timers)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -318,6 +329,16 @@ class Context:
 
     def synchronize(self):
         self._check(self.lib.efdc_synchronize(self.h))
+
+    def timers(self, on: bool):
+        self._check(self.lib.efdc_timers_enable(self.h, 1 if on else 0))
+
+    def read_timers(self) -> dict:
+        """{kernel name: (launches, total ms)} since the last read."""
+        t = Timers()
+        self._check(self.lib.efdc_timers_read(self.h, C.byref(t)))
+        return {t.k[j].name.decode(): (t.k[j].count, t.k[j].total_ms)
+                for j in range(t.n) if t.k[j].count > 0}
 
     def closure_pre(self):
         self._check(self.lib.efdc_closure_pre(self.h))

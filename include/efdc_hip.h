@@ -275,6 +275,24 @@ int efdc_balance(efdc_ctx_t *ctx, efdc_balance_t *out);
 /* Wait for all work queued on the context's stream. */
 int efdc_synchronize(efdc_ctx_t *ctx);
 
+/* Per-kernel timers, the device-side counterpart of the reference's phase
+   timers TPUV/TCONG/TSADV/TVDIF (DSTIME, EFDC/dstime.f90:1-56,
+   EFDC/congrad.f90:52,186, EFDC/calconc.f90:85,230,240,473).  When
+   enabled, every kernel launch of the context is bracketed by HIP events
+   on the context's stream; efdc_timers_read() waits for the stream,
+   returns the count and total duration per kernel and clears the record. */
+#define EFDC_TIMER_MAX 32
+typedef struct {
+  int32_t n;
+  struct {
+    char name[32];
+    int64_t count;
+    double total_ms;
+  } k[EFDC_TIMER_MAX];
+} efdc_timers_t;
+int efdc_timers_enable(efdc_ctx_t *ctx, int32_t on);
+int efdc_timers_read(efdc_ctx_t *ctx, efdc_timers_t *out);
+
 /* Device address of an array of the context (FP64 device copies have the
    device pitch returned in *pitch elements for (LCM,KCM) arrays). */
 int efdc_device_ptr(efdc_ctx_t *ctx, int32_t id, void **ptr, int64_t *pitch);

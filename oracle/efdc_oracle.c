@@ -63,7 +63,13 @@ int orc_congrad(orc_ctx *o, orc_cg_out *out) {
   const int nopt = noptimal(o);
   int iter = 0;
   double rsq = 0.0;
+  double rsqm = o->RSQM;
   out->status = EFDC_OK;
+  if (o->CG_REL_TOL > 0.0) { /* relative stop (the GPU contexts' cg_rel_tol) */
+    double b2 = 0.0;
+    for (int L = 2; L <= LA; ++L) b2 = b2 + FPTMP[HX(L)] * FPTMP[HX(L)];
+    rsqm = o->CG_REL_TOL * o->CG_REL_TOL * b2;
+  }
 
   /* initial residual, :54-77 */
 #pragma omp parallel for num_threads(nopt) schedule(static)
@@ -121,16 +127,16 @@ int orc_congrad(orc_ctx *o, orc_cg_out *out) {
     }
     const double beta = rpcgn / rpcg;
     rpcg = rpcgn;
-    if (rsq > o->RSQM) { /* :151-162 */
+    if (rsq > rsqm) { /* :151-162 */
 #pragma omp parallel for num_threads(nopt) schedule(static)
       for (int L = 2; L <= LA; ++L)
         PCG[HX(L)] = TMPCG[HX(L)] + beta * PCG[HX(L)];
     }
-    if (rsq <= o->RSQM) break; /* :165-167 */
+    if (rsq <= rsqm) break; /* :165-167 */
   }
   out->iters = iter; /* IDRYTBP = ITER, :187 */
   out->rsq = rsq;
-  if (rsq > o->RSQM) out->status = EFDC_ERR_CG_MAXITER; /* :171-184 */
+  if (rsq > rsqm) out->status = EFDC_ERR_CG_MAXITER; /* :171-184 */
 done:
   free(PCG); free(PN); free(PE); free(PW); free(PS); free(RCG); free(TMPCG);
   free(APCG);

@@ -43,6 +43,7 @@ typedef struct {
   double *a[ORC_NARR];   /* indexed by efdc_array_id */
   /* params */
   double G, RITB1, SNLT, HDRY, TCON_TBEGIN, RSQM;
+  double CG_REL_TOL;     /* > 0: RSQM = CG_REL_TOL^2 * sum FPTMP^2 */
   int32_t ITERM;
   int32_t nthreads;      /* OpenMP threads (reference threading structure) */
   /* pressure open BCs: W, E, S, N */

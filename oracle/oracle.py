@@ -51,7 +51,8 @@ class OrcCtx(C.Structure):
                 ("a", PD * NARR),
                 ("G", C.c_double), ("RITB1", C.c_double), ("SNLT", C.c_double),
                 ("HDRY", C.c_double), ("TCON_TBEGIN", C.c_double),
-                ("RSQM", C.c_double), ("ITERM", C.c_int32),
+                ("RSQM", C.c_double), ("CG_REL_TOL", C.c_double),
+                ("ITERM", C.c_int32),
                 ("nthreads", C.c_int32),
                 ("pbc", PbcSide * 4), ("mtide", C.c_int32), ("tcp", PD),
                 ("lopenbcdry", PI),
@@ -113,7 +114,7 @@ class Oracle:
     """Binds a Case to the C oracle; arrays are shared, not copied."""
 
     def __init__(self, case, *, rsqm=1e-9, iterm=200, nthreads=1, g=9.81,
-                 ritb1=1.0, snlt=0.0, hdry=0.2, tcon_tbegin=0.0):
+                 ritb1=1.0, snlt=0.0, hdry=0.2, tcon_tbegin=0.0, rel_tol=0.0):
         self.case = case
         g_ = case.grid
         o = OrcCtx()
@@ -129,6 +130,7 @@ class Oracle:
                 o.a[ARRAY_IDS[name]] = _pd(arr)
         o.G, o.RITB1, o.SNLT, o.HDRY = g, ritb1, snlt, hdry
         o.TCON_TBEGIN, o.RSQM, o.ITERM, o.nthreads = tcon_tbegin, rsqm, iterm, nthreads
+        o.CG_REL_TOL = rel_tol
         self.lopenbcdry = np.zeros(g_.LCM, np.int32)
         o.lopenbcdry = _pi(self.lopenbcdry)
         obc = case.open_bc
