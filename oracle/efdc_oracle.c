@@ -987,6 +987,23 @@ int orc_balance(orc_ctx *o, double *volume, double *mass) {
   return EFDC_OK;
 }
 
+/* SETOPENBC followed by SETOPENBC2 on the arrays as they are (no
+   min-diagonal reduction); used to compare with the reference units. */
+int orc_setopenbc_apply(orc_ctx *o, double dt, int64_t n) {
+  orc_setopenbc(o, dt, n, 0.5 * dt, 1. / dt);
+  orc_setopenbc2(o);
+  return EFDC_OK;
+}
+
+/* CALTRAN of one constituent alone (with CALFQC), no vertical diffusion */
+int orc_caltran_only(orc_ctx *o, int c, double dt, int64_t niter) {
+  tran_scratch s;
+  scratch_alloc(o, &s, o->c[c].isadac > 0);
+  orc_caltran(o, c, dt, niter, &s);
+  scratch_free(&s);
+  return EFDC_OK;
+}
+
 /* Depth-uniform closure standing in for the internal mode: Manning drag
    on the external velocity before CALPUV2C; layer transports and layer
    depths after it. */

@@ -82,6 +82,10 @@ int orc_calconc(orc_ctx *o, double dt, int64_t niter);
 int orc_vertical_diffusion(orc_ctx *o, double dt);
 /* balance totals: EFDC/bal2t.f90:90,103-104 */
 int orc_balance(orc_ctx *o, double *volume, double *mass);
+/* SETOPENBC + SETOPENBC2 alone, and CALTRAN (+CALFQC) of one constituent
+   alone: the units compared with the reference build in oracle/_ref */
+int orc_setopenbc_apply(orc_ctx *o, double dt, int64_t n);
+int orc_caltran_only(orc_ctx *o, int c, double dt, int64_t niter);
 /* depth-uniform closure standing in for the internal mode */
 int orc_closure_pre(orc_ctx *o, const double *manning_n);
 int orc_closure_post(orc_ctx *o);

@@ -95,7 +95,10 @@ def lib():
                 orc_vertical_diffusion=[C.POINTER(OrcCtx), C.c_double],
                 orc_balance=[C.POINTER(OrcCtx), PD, PD],
                 orc_closure_pre=[C.POINTER(OrcCtx), PD],
-                orc_closure_post=[C.POINTER(OrcCtx)]).items():
+                orc_closure_post=[C.POINTER(OrcCtx)],
+                orc_setopenbc_apply=[C.POINTER(OrcCtx), C.c_double, C.c_int64],
+                orc_caltran_only=[C.POINTER(OrcCtx), C.c_int, C.c_double,
+                                  C.c_int64]).items():
             f = getattr(_lib, fn)
             f.argtypes = args
             f.restype = C.c_int
@@ -229,6 +232,12 @@ class Oracle:
     def closure_pre(self):
         nman = np.ascontiguousarray(self.case.manning_n, np.float64)
         lib().orc_closure_pre(C.byref(self.o), _pd(nman))
+
+    def setopenbc_apply(self, n=1, dt=None):
+        lib().orc_setopenbc_apply(C.byref(self.o), dt or self.case.dt, n)
+
+    def caltran_only(self, c, niter=1, dt=None):
+        lib().orc_caltran_only(C.byref(self.o), c, dt or self.case.dt, niter)
 
     def closure_post(self):
         lib().orc_closure_post(C.byref(self.o))
