@@ -96,7 +96,8 @@ const char *const efdc_timer_names[TK_COUNT] = {
     "cg_spmv_pupdate", "cg_axpy_dots", "mx_prep", "mx_residual64",
     "mx_spmv_pupdate32", "mx_axpy_dots32", "mx_apply", "puv_transports",
     "puv_depths", "puv_hu_hv", "tran_upwind_vdiff", "tran_bc_con1",
-    "closure_pre", "closure_post", "balance"};
+    "closure_pre", "closure_post", "balance", "adac_gradients",
+    "adac_fluxes", "fct_limiters", "tran_finish_vdiff"};
 
 hipEvent_t timer_event(efdc_ctx *c) {
   if (c->ev_pool.empty()) {
@@ -170,7 +171,7 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
   // maps: validate on the host, store 0-based
   const int LA = g->LA, LC = c->LC, LCM = g->LCM;
   std::vector<int> lw(LCM), le(LCM), ls(LCM), ln(LCM), lnw(LCM), lse(LCM);
-  bool rowmajor = true;
+  bool rowmajor = true, mutual = true;
   for (int L = 1; L <= LCM; ++L) {
     const int i = L - 1;
     auto chk = [&](int v) { return v >= 1 && v <= LCM; };
@@ -181,8 +182,8 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
         return fail(EFDC_ERR_ARG);
       }
       if (!(a == L - 1 || a == LC) || !(b == L + 1 || b == LC)) rowmajor = false;
-      if (a != LC && a >= 2 && a <= LA && g->LEC[a - 1] != L) rowmajor = false;
-      if (s != LC && s >= 2 && s <= LA && g->LNC[s - 1] != L) rowmajor = false;
+      if (a != LC && a >= 2 && a <= LA && g->LEC[a - 1] != L) mutual = false;
+      if (s != LC && s >= 2 && s <= LA && g->LNC[s - 1] != L) mutual = false;
     }
     lw[i] = chk(a) ? a - 1 : LC - 1;
     le[i] = chk(b) ? b - 1 : LC - 1;
@@ -191,7 +192,8 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
     lnw[i] = (g->LNWC && chk(g->LNWC[i])) ? g->LNWC[i] - 1 : LC - 1;
     lse[i] = (g->LSEC && chk(g->LSEC[i])) ? g->LSEC[i] - 1 : LC - 1;
   }
-  c->rowmajor = rowmajor;
+  c->rowmajor = rowmajor && mutual;
+  c->mutual = mutual;
   int s;
   if ((s = upload_new(c, &c->dLW, lw.data(), LCM)) ||
       (s = upload_new(c, &c->dLE, le.data(), LCM)) ||
@@ -237,7 +239,8 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
     return fail(EFDC_ERR_NOMEM);
   std::memset(c->st_host, 0, sizeof(CgState));
   if ((s = dmalloc(c, &c->lopenbcdry, LCM)) ||
-      (s = dmalloc(c, &c->pbcflag, LCM)) || (s = dmalloc(c, &c->bcrec, LCM)) ||
+      (s = dmalloc(c, &c->pbcflag, LCM)) || (s = dmalloc(c, &c->lobflag, LCM)) ||
+      (s = dmalloc(c, &c->bcrec, LCM)) ||
       (s = dmalloc(c, &c->srcrec, LCM)) || (s = dmalloc(c, &c->cdzkk_k, 64)) ||
       (s = dmalloc(c, &c->cdzkmk_k, 64)) || (s = dmalloc(c, &c->scal, 64)) ||
       (s = dmalloc(c, &c->nman, LCM)))
@@ -271,7 +274,7 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
     dfree(b.L); dfree(b.ispb); dfree(b.pser); dfree(b.pcb); dfree(b.psb);
   }
   delete[] c->tcp_host;
-  dfree(c->tide_dev); dfree(c->lopenbcdry); dfree(c->pbcflag);
+  dfree(c->tide_dev); dfree(c->lopenbcdry); dfree(c->pbcflag); dfree(c->lobflag);
   dfree(c->lobcs); dfree(c->lobcs2);
   for (auto &b : c->cbc) {
     dfree(b.L); dfree(b.cb); dfree(b.csert); dfree(b.ntscr); dfree(b.clo);
@@ -462,6 +465,12 @@ int efdc_set_open_bc(efdc_ctx_t *c, const efdc_open_bc_t *bc) {
     return st;
   HIP_TRY(c, hipMemcpyAsync(c->pbcflag, flag.data(), c->LCM,
                             hipMemcpyHostToDevice, c->stream));
+  {
+    std::vector<unsigned char> lf(c->LCM, 0);
+    for (int j : lob) lf[j] |= 1;
+    for (int j : lob2) if (j >= 0 && j < c->LCM) lf[j] |= 2;
+    HIP_TRY(c, hipMemcpy(c->lobflag, lf.data(), c->LCM, hipMemcpyHostToDevice));
+  }
   c->mtide = bc->mtide;
   delete[] c->tcp_host;
   c->tcp_host = new double[std::max(bc->mtide, 1)];
@@ -483,9 +492,11 @@ int efdc_add_constituent(efdc_ctx_t *c, const efdc_constituent_t *d,
     c->err = "too many constituents";
     return EFDC_ERR_ARG;
   }
-  if (d->isadac > 0 && !c->rowmajor) {
-    // the anti-diffusion stencil uses the LNWC/LSEC diagonals; they are
-    // always uploaded (LC when absent), nothing else is required.
+  if (d->isadac > 0 && !c->mutual) {
+    // the source / open-boundary zeroing of the antidiffusive fluxes is
+    // applied on the face shared with the neighbour (LEC(LWC(L)) = L)
+    c->err = "anti-diffusion needs mutual neighbour maps";
+    return EFDC_ERR_ARG;
   }
   ConstDev &k = c->con[c->nconst];
   k.desc = *d;

@@ -86,6 +86,7 @@ struct efdc_ctx {
   int *dLW = nullptr, *dLE = nullptr, *dLS = nullptr, *dLN = nullptr;
   int *dLNW = nullptr, *dLSE = nullptr;
   bool rowmajor = false;  // LWC in {L-1,LC}, LEC in {L+1,LC}, maps mutual
+  bool mutual = false;    // LEC(LWC(L)) = L and LNC(LSC(L)) = L for wet L
 
   double *dH[EFDC_NUM_H] = {};
   void *dK[EFDC_NK] = {};
@@ -114,6 +115,7 @@ struct efdc_ctx {
   double *tide_host = nullptr; // pinned
   int *lopenbcdry = nullptr;   // (LCM)
   unsigned char *pbcflag = nullptr;   // (LCM) 1 on pressure-BC cells
+  unsigned char *lobflag = nullptr;   // (LCM) bit 0 LOBCS, bit 1 LOBCS2
   int *lobcs = nullptr;        // LOBCS order S, W, E, N (0-based)
   int *lobcs2 = nullptr;
   int nlobcs = 0, nlobcs2 = 0;
@@ -163,7 +165,8 @@ enum {
   TK_PUV_EXPLICIT = 0, TK_PUV_ASSEMBLE, TK_PUV_BC, TK_PUV_SCALE,
   TK_CG_INIT, TK_CG_A, TK_CG_B, TK_MX_PREP, TK_MX_RESID, TK_MX_A, TK_MX_B,
   TK_MX_APPLY, TK_PUV_POST1, TK_PUV_POST2, TK_PUV_POST3, TK_TRAN,
-  TK_TRAN_BC, TK_CLOSURE_PRE, TK_CLOSURE_POST, TK_BALANCE, TK_COUNT
+  TK_TRAN_BC, TK_CLOSURE_PRE, TK_CLOSURE_POST, TK_BALANCE, TK_ADAC_GRAD,
+  TK_ADAC_FLUX, TK_FCT, TK_COL_FINISH, TK_COUNT
 };
 extern const char *const efdc_timer_names[TK_COUNT];
 

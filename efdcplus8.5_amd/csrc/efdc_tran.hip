@@ -49,12 +49,18 @@ struct TranArgs {
   CbcDev cbc[4];
   const double *wtci;
   double *bc_con1;
+  int vd;  // 0: stop after the open boundaries (anti-diffusion follows)
 };
 
 template <typename T>
 __device__ __forceinline__ double ld(const T *p, int64_t j) {
   return (double)p[j];
 }
+
+// MAX/MIN with the semantics of the restatement the kernels are compared
+// with bit for bit
+__device__ __forceinline__ double dmx(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmn(double a, double b) { return a < b ? a : b; }
 
 template <typename T>
 __global__ __launch_bounds__(TBLOCK) void k_tran_upwind_vdiff(TranArgs<T> a) {
@@ -145,8 +151,8 @@ __global__ __launch_bounds__(TBLOCK) void k_tran_upwind_vdiff(TranArgs<T> a) {
         const double conq = 0.5 * (3. * c1 - c1);
         const size_t j = (size_t)q * KC + k;
         const double qv = a.qss[j], qs = a.qsercell[j];
-        fqc = fqc + fmax(qv, 0.) * a.cqs[j * nc + c] + fmin(qv, 0.) * conq +
-              fmax(qs, 0.) * a.csertq[j * nc + c] + fmin(qs, 0.) * conq;
+        fqc = fqc + dmx(qv, 0.) * a.cqs[j * nc + c] + dmn(qv, 0.) * conq +
+              dmx(qs, 0.) * a.csertq[j * nc + c] + dmn(qs, 0.) * conq;
       }
       // EFDC/caltran.f90:194-196,243
       const double cd = c1 * h1pk + DDELT * ((fqc + fu_i - fu_e + fv_i - fv_n) * dxyip +
@@ -174,7 +180,7 @@ __global__ __launch_bounds__(TBLOCK) void k_tran_upwind_vdiff(TranArgs<T> a) {
             double CTMP;
             if (side <= 1) CTMP = cv1 + DDELT * (flow * cv1 - flux) * dxyip * hpki;
             else CTMP = cv1 + DDELT * (flux - flow * cv1) * dxyip * hpki;
-            con = fmax(CTMP, 0.);
+            con = dmx(CTMP, 0.);
             if (a.mo[c] == 1) {
               const double cbtmp = cb1 + b.csert[((size_t)LL * KC + 0) * nc + c];
               if (con > cbtmp) con = cbtmp;
@@ -189,12 +195,16 @@ __global__ __launch_bounds__(TBLOCK) void k_tran_upwind_vdiff(TranArgs<T> a) {
             } else {
               const double cbtmp =
                   b.clo[m] + (CBT - b.clo[m]) * (double)nmnlod / (double)b.ntscr[LL];
-              con = fmax(cbtmp, 0.);
+              con = dmx(cbtmp, 0.);
             }
           }
           cv1 = con;
         }
         a.bc_con1[((size_t)rec * KC + k) * nc + c] = con;
+      }
+      if (!a.vd) {
+        a.nw[c][i + o] = (T)con;
+        continue;
       }
       // forward sweep of the tridiagonal solve (FP64)
       double y;
@@ -205,6 +215,7 @@ __global__ __launch_bounds__(TBLOCK) void k_tran_upwind_vdiff(TranArgs<T> a) {
     }
     vcu_prev = vcu_k;
   }
+  if (!a.vd) return;
   // back substitution and dye decay, write CON
   for (int c = 0; c < nc; ++c) {
     T *C = a.nw[c];
@@ -298,9 +309,420 @@ __global__ void k_closure_post(const double *uhdye, const double *uhdy1e,
   }
 }
 
+
+// ---------------------------------------------------------------------
+// Anti-diffusion and flux-corrected transport, EFDC/caltran.f90:473-786.
+// The fields of one constituent go through three (L,K)-parallel passes
+// because every pass needs neighbour values of the previous one:
+//   k_adac_grad : POS = max(CON,0); UUUU, VVVV, WWWW (:490-511)
+//   k_adac_flux : antidiffusive fluxes (:522-577) with the zeroing at flow
+//                 sources (:583-599) and open boundaries (:602-620)
+//   k_fct_beta  : CMAX/CMIN (:627-679), DUU/DVV and the beta limiters
+//                 (:705-734)
+// and k_col_finish applies the (limited) fluxes per column (:737-786),
+// then the vertical diffusion and decay of CALCONC.
+// Ghost cells (L=1, LC) are zero in every field, as the never-written
+// entries of the reference's zero-initialised arrays are.
+// ---------------------------------------------------------------------
+template <typename T>
+struct AdacArgs {
+  const T *con, *con1;  // CON after the open boundaries, CON1 before them
+  const T *uhdy2, *vhdx2, *w2, *u2, *v2, *dzig, *qsum, *sub3d, *svb3d, *hpk, *hpki;
+  const double *dxiu, *dyiv, *hpi, *dxyip;
+  const int *lw, *le, *ls, *ln, *lnw, *lse;
+  const int *bcrec;
+  const int4 *bcinfo;
+  const unsigned char *lobflag;  // bit 0: LOBCS, bit 1: LOBCS2
+  int nwet, lc0, KC, isadac, isfct;
+  int64_t pitch;
+  double dt;
+  T *uuuu, *vvvv, *wwww, *fu, *fv, *fw, *duu, *dvv;
+};
+
+template <typename T>
+__device__ __forceinline__ double posv(const AdacArgs<T> &a, int j, int64_t o) {
+  return (j == 0 || j == a.lc0) ? 0.0 : dmx((double)a.con[j + o], 0.);
+}
+
+template <typename T>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_adac_grad(AdacArgs<T> a) {
+  const int i = 1 + xcd_tile(blockIdx.x, gridDim.x) * EFDC_BLOCK + threadIdx.x;
+  if (i > a.nwet) return;
+  const int k = blockIdx.y;
+  const int64_t o = (int64_t)k * a.pitch, p = a.pitch;
+  const double pc = posv(a, i, o);
+  a.uuuu[i + o] = (T)((double)a.u2[i + o] * (pc - posv(a, a.lw[i], o)) * a.dxiu[i]);
+  a.vvvv[i + o] = (T)((double)a.v2[i + o] * (pc - posv(a, a.ls[i], o)) * a.dyiv[i]);
+  if (k < a.KC - 1) {  // WWWW(L,K), K = k+1 <= KS, plane K
+    const int64_t oK = (int64_t)(k + 1) * p;
+    a.wwww[i + oK] = (T)((double)a.w2[i + oK] * (posv(a, i, o + p) - pc) * a.hpi[i] *
+                         (double)a.dzig[i + oK]);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ bool bc_side(const AdacArgs<T> &a, int j, int side) {
+  if (!a.bcrec || j == 0 || j == a.lc0) return false;
+  const int r = a.bcrec[j];
+  if (r < 0) return false;
+  const int4 q = a.bcinfo[r];
+  return (side == 0 ? q.x : side == 1 ? q.y : side == 2 ? q.z : q.w) >= 0;
+}
+
+template <typename T>
+__device__ __forceinline__ bool src_at(const AdacArgs<T> &a, int j, int64_t o) {
+  return j != 0 && j != a.lc0 && fabs((double)a.qsum[j + o]) > 1.E-8;
+}
+
+template <typename T>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_adac_flux(AdacArgs<T> a) {
+  const int i = 1 + xcd_tile(blockIdx.x, gridDim.x) * EFDC_BLOCK + threadIdx.x;
+  if (i > a.nwet) return;
+  const int k = blockIdx.y, K = k + 1;
+  const int64_t o = (int64_t)k * a.pitch, p = a.pitch, oK = (int64_t)K * p;
+  const double DDELTA = a.dt, BSMALL = 1.0E-6;
+  const int LE = a.le[i], LW = a.lw[i], LN = a.ln[i], LS = a.ls[i];
+  const int LNW = a.lnw[i], LSE = a.lse[i];
+  const T *U = a.uuuu, *V = a.vvvv, *Wz = a.wwww;
+  auto u = [&](int j, int64_t off) { return (double)U[j + off]; };
+  auto v = [&](int j, int64_t off) { return (double)V[j + off]; };
+  auto w = [&](int j, int KK) { return (double)Wz[j + (int64_t)KK * p]; };
+  const double pc = posv(a, i, o), pw = posv(a, LW, o), ps = posv(a, LS, o);
+  // U components (:533-543)
+  const double uh = (double)a.uhdy2[i + o];
+  double UTERM = fabs(uh) * (pc - pw);
+  if (uh >= 0.0)
+    UTERM = UTERM - 0.5 * DDELTA * uh *
+                        (v(LNW, o) + v(LW, o) + w(LW, K) + w(LW, K - 1) + u(i, o) + u(LW, o));
+  else
+    UTERM = UTERM - 0.5 * DDELTA * uh *
+                        (v(LN, o) + v(i, o) + w(i, K) + w(i, K - 1) + u(i, o) + u(LE, o));
+  const double UHU = UTERM / (pc + pw + BSMALL);
+  double fu = dmx(UHU, 0.) * pw + dmn(UHU, 0.) * pc;
+  // V components (:546-556)
+  const double vh = (double)a.vhdx2[i + o];
+  double VTERM = fabs(vh) * (pc - ps);
+  if (vh >= 0.0)
+    VTERM = VTERM - 0.5 * DDELTA * vh *
+                        (u(LS, o) + u(LSE, o) + w(LS, K) + w(LS, K - 1) + v(LS, o) + v(i, o));
+  else
+    VTERM = VTERM - 0.5 * DDELTA * vh *
+                        (u(i, o) + u(LE, o) + w(i, K) + w(i, K - 1) + v(LN, o) + v(i, o));
+  const double VHV = VTERM / (pc + ps + BSMALL);
+  double fv = dmx(VHV, 0.) * ps + dmn(VHV, 0.) * pc;
+  // W component (:560-577), FWUU(L,K) for K <= KS
+  double fw = 0.0;
+  if (K < a.KC) {
+    const double pu = posv(a, i, o + p);
+    const double ww = (double)a.w2[i + oK];
+    double WTERM = fabs(ww) * (pu - pc);
+    if (ww >= 0.0)
+      WTERM = WTERM - 0.5 * DDELTA * ww *
+                          (u(i, o) + u(LE, o) + v(i, o) + v(LN, o) + w(i, K) + w(i, K - 1));
+    else
+      WTERM = WTERM - 0.5 * DDELTA * ww *
+                          (u(i, o + p) + u(LE, o + p) + v(i, o + p) + v(LN, o + p) + w(i, K) +
+                           w(i, K + 1));
+    const double WW = WTERM / (pu + pc + BSMALL);
+    fw = dmx(WW, 0.) * pc + dmn(WW, 0.) * pu;
+  }
+  // zero at flow sources (ISADAC=1) and at concentration open boundaries
+  if (a.isadac == 1) {
+    if (src_at(a, i, o) || src_at(a, LW, o)) fu = 0.;
+    if (src_at(a, i, o) || src_at(a, LS, o)) fv = 0.;
+    if (src_at(a, i, o)) fw = 0.;
+  }
+  if (bc_side(a, LS, 0) || bc_side(a, i, 3)) fv = 0.0;
+  if (bc_side(a, LW, 1) || bc_side(a, i, 2)) fu = 0.0;
+  a.fu[i + o] = (T)fu;
+  a.fv[i + o] = (T)fv;
+  if (K < a.KC) a.fw[i + oK] = (T)fw;
+}
+
+template <typename T>
+__device__ __forceinline__ void cont_mx_mn(const AdacArgs<T> &a, int j, int64_t o,
+                                           double &mx, double &mn) {
+  if (j == 0 || j == a.lc0) {
+    mx = 0.0;
+    mn = 0.0;
+    return;
+  }
+  const double pp = dmx((double)a.con[j + o], 0.);
+  const double c2 = fabs(dmx((double)a.con1[j + o], 0.));  // |CON2|
+  mx = dmx(pp, c2);
+  mn = dmn(pp, c2);
+}
+
+template <typename T>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_fct_beta(AdacArgs<T> a) {
+  const int i = 1 + xcd_tile(blockIdx.x, gridDim.x) * EFDC_BLOCK + threadIdx.x;
+  if (i > a.nwet) return;
+  const int k = blockIdx.y, K = k + 1, KC = a.KC;
+  const int64_t o = (int64_t)k * a.pitch, p = a.pitch;
+  const double DDELT = a.dt, BSMALL = 1.0E-6;
+  const int LE = a.le[i], LW = a.lw[i], LN = a.ln[i], LS = a.ls[i];
+  double mx0, mn0, mxa, mna, mxb, mnb;
+  cont_mx_mn(a, i, o, mx0, mn0);
+  double cmax, cmin;
+  if (K == 1) {  // KSZ = 1
+    cont_mx_mn(a, i, o + p, mxa, mna);
+    cmax = dmx(mx0, mxa);
+    cmin = dmn(mn0, mna);
+  } else if (K == KC) {
+    cont_mx_mn(a, i, o - p, mxa, mna);
+    cmax = dmx(mxa, mx0);
+    cmin = dmn(mna, mn0);
+  } else {
+    cont_mx_mn(a, i, o - p, mxa, mna);
+    cont_mx_mn(a, i, o + p, mxb, mnb);
+    cmax = dmx(mx0, dmx(mxa, mxb));
+    cmin = dmn(mn0, dmn(mna, mnb));
+  }
+  double mxW, mnW, mxE, mnE, mxS, mnS, mxN, mnN;
+  cont_mx_mn(a, LW, o, mxW, mnW);
+  cont_mx_mn(a, LE, o, mxE, mnE);
+  cont_mx_mn(a, LS, o, mxS, mnS);
+  cont_mx_mn(a, LN, o, mxN, mnN);
+  const double sbi = (double)a.sub3d[i + o], sbe = (double)a.sub3d[LE + o];
+  const double svi = (double)a.svb3d[i + o], svn = (double)a.svb3d[LN + o];
+  const double CWMAX = sbi * mxW, CEMAX = sbe * mxE, CSMAX = svi * mxS, CNMAX = svn * mxN;
+  double CMAXT = dmx(CNMAX, CEMAX);
+  CMAXT = dmx(CMAXT, CSMAX);
+  CMAXT = dmx(CMAXT, CWMAX);
+  cmax = dmx(cmax, CMAXT);
+  const double CWMIN = sbi * mnW + 1.E+6 * (1. - sbi);
+  const double CEMIN = sbe * mnE + 1.E+6 * (1. - sbe);
+  const double CSMIN = svi * mnS + 1.E+6 * (1. - svi);
+  const double CNMIN = svn * mnN + 1.E+6 * (1. - svn);
+  double CMINT = dmn(CNMIN, CEMIN);
+  CMINT = dmn(CMINT, CSMIN);
+  CMINT = dmn(CMINT, CWMIN);
+  cmin = dmn(cmin, CMINT);
+  // influx / outflux in concentration units (:705-714)
+  const double fui = (double)a.fu[i + o], fue = (double)a.fu[LE + o];
+  const double fvi = (double)a.fv[i + o], fvn = (double)a.fv[LN + o];
+  const double fwb = (double)a.fw[i + (int64_t)(K - 1) * p], fwt = (double)a.fw[i + (int64_t)K * p];
+  const double dxyip = a.dxyip[i], hpki = (double)a.hpki[i + o];
+  double duu = DDELT * (dxyip * (dmx(fui, 0.) - dmn(fue, 0.) + dmx(fvi, 0.) - dmn(fvn, 0.)) +
+                        (dmx(fwb, 0.) - dmn(fwt, 0.))) * hpki;
+  double dvv = DDELT * (dxyip * (dmx(fue, 0.) - dmn(fui, 0.) + dmx(fvn, 0.) - dmn(fvi, 0.)) +
+                        (dmx(fwt, 0.) - dmn(fwb, 0.))) * hpki;
+  if (a.lobflag[i] & 1) {  // :717-723
+    duu = 0.;
+    dvv = 0.;
+  }
+  const double pc = dmx((double)a.con[i + o], 0.);
+  if (duu > 0.) duu = (cmax - pc) / (duu + BSMALL);  // :729-732
+  duu = dmn(duu, 1.);
+  if (dvv > 0.) dvv = (pc - cmin) / (dvv + BSMALL);
+  dvv = dmn(dvv, 1.);
+  a.duu[i + o] = (T)duu;
+  a.dvv[i + o] = (T)dvv;
+}
+
+// Limited (or raw) antidiffusive flux across the face whose downstream
+// cell is `d` and upstream neighbour `up` (the W/S neighbour of d).
+template <typename T>
+__device__ __forceinline__ double lim_h(const AdacArgs<T> &a, const T *f, int d, int up,
+                                        int64_t o) {
+  const double fl = (double)f[d + o];
+  if (!a.isfct) return fl;
+  const double duu_d = (double)a.duu[d + o], dvv_d = (double)a.dvv[d + o];
+  const double duu_u = (double)a.duu[up + o], dvv_u = (double)a.dvv[up + o];
+  return dmn(dvv_u, duu_d) * dmx(fl, 0.) + dmn(duu_u, dvv_d) * dmn(fl, 0.);
+}
+
+template <typename T>
+__device__ __forceinline__ double lim_w(const AdacArgs<T> &a, int i, int K) {
+  // FWUU(L,K), K = 1..KS, limited (:747-752), zero at LOBCS/LOBCS2 (:755-766)
+  if (K <= 0 || K >= a.KC) return 0.0;
+  const int64_t p = a.pitch;
+  const double fl = (double)a.fw[i + (int64_t)K * p];
+  if (!a.isfct) return fl;
+  if (a.lobflag[i] & 3) return 0.0;
+  const int64_t o = (int64_t)(K - 1) * p;  // layer K
+  const double duuK = (double)a.duu[i + o], dvvK = (double)a.dvv[i + o];
+  const double duuK1 = (double)a.duu[i + o + p], dvvK1 = (double)a.dvv[i + o + p];
+  return dmn(dvvK, duuK1) * dmx(fl, 0.) + dmn(duuK, dvvK1) * dmn(fl, 0.);
+}
+
+template <typename T>
+struct FinishArgs {
+  AdacArgs<T> ad;
+  T *con;          // in: CON after the open boundaries; out: final CON
+  const T *ab, *cdzkk, *cdzkmk;
+  const double *cdzkk_k, *cdzkmk_k;
+  int dz_uniform;
+  const unsigned char *pbcflag;
+  int adac;        // apply the antidiffusive correction
+  int dye_mode;
+  double dye_f;
+};
+
+template <typename T>
+__global__ __launch_bounds__(TBLOCK) void k_col_finish(FinishArgs<T> f) {
+  extern __shared__ unsigned char smem[];
+  const AdacArgs<T> &a = f.ad;
+  const int TB = blockDim.x;
+  const int i = 1 + xcd_tile(blockIdx.x, gridDim.x) * TB + (int)threadIdx.x;
+  if (i > a.nwet) return;
+  const int tid = threadIdx.x, KC = a.KC;
+  const int64_t p = a.pitch;
+  double *vcu = (double *)smem;
+  T *ys = (T *)(vcu + (size_t)KC * TB);
+  const double DDELT = a.dt, DELTD2 = a.dt;
+  const double dxyip = a.dxyip[i], hpi = a.hpi[i];
+  const int LE = a.le[i], LN = a.ln[i], LW = a.lw[i], LS = a.ls[i];
+  const bool eg = (LE == a.lc0), ng = (LN == a.lc0);
+  double vcu_prev = 0.0, yprev = 0.0;
+  for (int k = 0; k < KC; ++k) {
+    const int K = k + 1;
+    const int64_t o = (int64_t)k * p;
+    double con = (double)f.con[i + o];
+    if (f.adac && !(f.pbcflag[i])) {
+      // EFDC/caltran.f90:771-778 (LOBCS cells keep CON, :781-786)
+      const double fui = lim_h(a, a.fu, i, LW, o);
+      const double fue = eg ? 0.0 : lim_h(a, a.fu, LE, a.lw[LE], o);
+      const double fvi = lim_h(a, a.fv, i, LS, o);
+      const double fvn = ng ? 0.0 : lim_h(a, a.fv, LN, a.ls[LN], o);
+      const double cd = con * (double)a.hpk[i + o] +
+                        DDELT * ((fui - fue + fvi - fvn) * dxyip + (lim_w(a, i, K - 1) - lim_w(a, i, K)));
+      con = cd * (double)a.hpki[i + o];
+    }
+    // vertical diffusion, forward sweep (EFDC/calconc.f90:264-422)
+    double y = con;
+    if (KC > 1) {
+      const double ckk = f.dz_uniform ? f.cdzkk_k[k] : (double)f.cdzkk[i + o];
+      const double ckm = f.dz_uniform ? f.cdzkmk_k[k] : (double)f.cdzkmk[i + o];
+      double vcu_k = 0.0;
+      if (K == 1) {
+        const double CCUBTMP = (-DELTD2 * ckk) * hpi * (double)f.ab[i + o];
+        const double eeb = 1.0 / (1.0 - CCUBTMP);
+        vcu_k = CCUBTMP * eeb;
+        y = con * eeb;
+      } else if (K < KC) {
+        const double cclb = (-DELTD2 * ckm) * hpi * (double)f.ab[i + o - p];
+        const double CCUBTMP = (-DELTD2 * ckk) * hpi * (double)f.ab[i + o];
+        const double CCMBTMP = 1.0 - cclb - CCUBTMP;
+        const double eeb = 1.0 / (CCMBTMP - cclb * vcu_prev);
+        vcu_k = CCUBTMP * eeb;
+        y = (con - cclb * yprev) * eeb;
+      } else {
+        const double cclb = (-DELTD2 * ckm) * hpi * (double)f.ab[i + o - p];
+        const double eeb = 1.0 / ((1.0 - cclb) - cclb * vcu_prev);
+        y = (con - cclb * yprev) * eeb;
+      }
+      vcu[(size_t)k * TB + tid] = vcu_k;
+      vcu_prev = vcu_k;
+    }
+    ys[(size_t)k * TB + tid] = (T)y;
+    yprev = (double)ys[(size_t)k * TB + tid];
+  }
+  auto fin = [&](double v) {
+    return f.dye_mode == 1 ? f.dye_f * v : (f.dye_mode == 2 ? v + f.dye_f : v);
+  };
+  double up = (double)ys[(size_t)(KC - 1) * TB + tid];
+  f.con[i + (int64_t)(KC - 1) * p] = (T)fin(up);
+  for (int k = KC - 2; k >= 0; --k) {
+    const double v = (double)ys[(size_t)k * TB + tid] - vcu[(size_t)k * TB + tid] * up;
+    f.con[i + (int64_t)k * p] = (T)fin(v);
+    up = v;
+  }
+}
+
 template <typename T>
 T *kp(efdc_ctx *c, int id) {
   return (T *)c->dK[id - EFDC_HPK];
+}
+
+
+// Anti-diffusion/FCT for each constituent that asks for it, then the
+// vertical diffusion and decay of every constituent.
+template <typename T>
+int adac_pipeline(efdc_ctx *c, TranArgs<T> &ta, const efdc_step_t *step) {
+  const int KC = c->KC, nwet = c->n_wet();
+  const size_t plane = (size_t)c->pitch * (KC + 1);
+  if (!c->adac_alloc) {
+    for (int j = 0; j < 8; ++j) {
+      char *q = nullptr;
+      HIP_TRY(c, hipMalloc((void **)&q, plane * sizeof(T)));
+      HIP_TRY(c, hipMemsetAsync(q, 0, plane * sizeof(T), c->stream));
+      c->adac[j] = q;
+    }
+    c->adac_alloc = true;
+  }
+  AdacArgs<T> a;
+  std::memset(&a, 0, sizeof(a));
+  a.uhdy2 = ta.uhdy2;
+  a.vhdx2 = ta.vhdx2;
+  a.w2 = ta.w2;
+  a.u2 = kp<T>(c, EFDC_U2);
+  a.v2 = kp<T>(c, EFDC_V2);
+  a.dzig = (const T *)c->dW[EFDC_DZIG - EFDC_W2];
+  a.qsum = kp<T>(c, EFDC_QSUM);
+  a.sub3d = kp<T>(c, EFDC_SUB3D);
+  a.svb3d = kp<T>(c, EFDC_SVB3D);
+  a.hpk = kp<T>(c, EFDC_HPK);
+  a.hpki = ta.hpki;
+  a.dxiu = c->dH[EFDC_DXIU];
+  a.dyiv = c->dH[EFDC_DYIV];
+  a.hpi = c->dH[EFDC_HPI];
+  a.dxyip = c->dH[EFDC_DXYIP];
+  a.lw = c->dLW;
+  a.le = c->dLE;
+  a.ls = c->dLS;
+  a.ln = c->dLN;
+  a.lnw = c->dLNW;
+  a.lse = c->dLSE;
+  a.bcrec = ta.bcrec;
+  a.bcinfo = c->bcinfo;
+  a.lobflag = c->lobflag;
+  a.nwet = nwet;
+  a.lc0 = c->LC - 1;
+  a.KC = KC;
+  a.pitch = c->pitch;
+  a.dt = step->dt;
+  a.uuuu = (T *)c->adac[0];
+  a.vvvv = (T *)c->adac[1];
+  a.wwww = (T *)c->adac[2];
+  a.fu = (T *)c->adac[3];
+  a.fv = (T *)c->adac[4];
+  a.fw = (T *)c->adac[5];
+  a.duu = (T *)c->adac[6];
+  a.dvv = (T *)c->adac[7];
+  const dim3 grid((nwet + EFDC_BLOCK - 1) / EFDC_BLOCK, KC);
+  auto lds_for = [&](int tb) { return (sizeof(double) + sizeof(T)) * (size_t)KC * tb; };
+  int tb = TBLOCK;
+  while (tb > 32 && lds_for(tb) > 64 * 1024) tb /= 2;
+  const int nbc = (nwet + tb - 1) / tb;
+  for (int j = 0; j < c->nconst; ++j) {
+    const efdc_constituent_t &d = c->con[j].desc;
+    a.con = ta.nw[j];
+    a.con1 = ta.old[j];
+    a.isadac = d.isadac;
+    a.isfct = d.isfct;
+    if (d.isadac > 0) {
+      TIMED(c, TK_ADAC_GRAD, k_adac_grad<T><<<grid, EFDC_BLOCK, 0, c->stream>>>(a));
+      TIMED(c, TK_ADAC_FLUX, k_adac_flux<T><<<grid, EFDC_BLOCK, 0, c->stream>>>(a));
+      if (d.isfct != 0)
+        TIMED(c, TK_FCT, k_fct_beta<T><<<grid, EFDC_BLOCK, 0, c->stream>>>(a));
+    }
+    FinishArgs<T> f;
+    f.ad = a;
+    f.con = ta.nw[j];
+    f.ab = ta.ab;
+    f.cdzkk = ta.cdzkk;
+    f.cdzkmk = ta.cdzkmk;
+    f.cdzkk_k = ta.cdzkk_k;
+    f.cdzkmk_k = ta.cdzkmk_k;
+    f.dz_uniform = ta.dz_uniform;
+    f.pbcflag = c->pbcflag;
+    f.adac = d.isadac > 0 ? 1 : 0;
+    f.dye_mode = ta.dye_mode[j];
+    f.dye_f = ta.dye_f[j];
+    TIMED(c, TK_COL_FINISH, k_col_finish<T><<<nbc, tb, lds_for(tb), c->stream>>>(f));
+  }
+  return EFDC_OK;
 }
 
 template <typename T>
@@ -311,11 +733,14 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
     c->err = "source / boundary tables were set before all constituents were added";
     return EFDC_ERR_STATE;
   }
-  for (int j = 0; j < nc; ++j)
-    if (c->con[j].desc.isadac > 0) {
-      c->err = "anti-diffusion (ISADAC>0) is not available in this build";
+  bool any_adac = false;
+  for (int j = 0; j < nc; ++j) {
+    if (c->con[j].desc.isadac > 0) any_adac = true;
+    if (c->con[j].desc.isadac > 0 && c->con[j].desc.isfct != 0 && KC < 2) {
+      c->err = "FCT needs KC >= 2 (EFDC/caltran.f90:637-640 reads layer 2)";
       return EFDC_ERR_ARG;
     }
+  }
   TranArgs<T> a;
   std::memset(&a, 0, sizeof(a));
   a.uhdy2 = kp<T>(c, EFDC_UHDY2);
@@ -372,11 +797,13 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
   for (int s = 0; s < 4; ++s) a.cbc[s] = c->cbc[s];
   a.wtci = c->wtci;
   a.bc_con1 = c->bc_con1;
+  a.vd = any_adac ? 0 : 1;
 
   // LDS per block: VCU and the FWUU carry (FP64) plus the forward values
   // (storage type); 128 columns per block when that fits in 64 KiB
   auto lds_for = [&](int tb) {
-    return sizeof(double) * (size_t)(KC + nc) * tb + sizeof(T) * (size_t)KC * nc * tb;
+    return a.vd ? sizeof(double) * (size_t)(KC + nc) * tb + sizeof(T) * (size_t)KC * nc * tb
+                : sizeof(double) * (size_t)(KC + nc) * tb;
   };
   int tb = TBLOCK;
   while (tb > 32 && lds_for(tb) > 64 * 1024) tb /= 2;
@@ -386,6 +813,10 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
   }
   const int nb = (a.nwet + tb - 1) / tb;
   TIMED(c, TK_TRAN, k_tran_upwind_vdiff<T><<<nb, tb, lds_for(tb), c->stream>>>(a));
+  if (any_adac) {
+    int s = adac_pipeline<T>(c, a, step);
+    if (s) return s;
+  }
   if (c->nbcrec > 0) {
     const int tot = c->nbcrec * KC;
     TIMED(c, TK_TRAN_BC, k_tran_bc_con1<T><<<(tot + 255) / 256, 256, 0, c->stream>>>(a, c->bccell,

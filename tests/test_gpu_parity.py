@@ -207,3 +207,64 @@ def test_relative_tolerance_stop_and_residual_at_scale():
     print("cfg1 wet", g.nwet, "iters", out["iters"], "rel residual", rel)
     assert out["iters"] > 0
     assert rel <= 1.5e-10
+
+
+@pytest.mark.parametrize("isadac,isfct", [(1, 0), (1, 1)])
+def test_antidiffusion_fct_fp64_matches_oracle(oracle_mod, isadac, isfct):
+    """CALTRAN's anti-diffusion and FCT (EFDC/caltran.f90:473-786) on the
+    GPU against the oracle (itself equal bit for bit to the reference's
+    CALTRAN): the tidal basin with the open boundary, the dye source and
+    the tide, full steps."""
+    case = _basin(32, 16, 5, seed=4, isadac=isadac, isfct=isfct)
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref, rsqm=TIGHT, iterm=2000)
+    ctx = _ctx(case)
+    kc = case.grid.KC
+    for n in range(1, 11):
+        o.step(n)
+        ctx.step(case.dt, n)
+    for s, c in enumerate(ref.consts):
+        con, con1 = ctx.get_constituent(s)
+        scale = max(1e-30, np.max(np.abs(c["con"][:kc])))
+        rel = np.max(np.abs(con[:kc] - c["con"][:kc])) / scale
+        rel1 = np.max(np.abs(con1[:kc] - c["con1"][:kc])) / scale
+        print(f"adac={isadac} fct={isfct} const {s}: rel {rel:.3e} con1 {rel1:.3e}")
+        assert rel <= 1e-9 and rel1 <= 1e-9
+
+
+@pytest.mark.parametrize("isfct", [0, 1])
+def test_antidiffusion_single_step_bitexact(oracle_mod, isfct):
+    """One CALCONC call on identical inputs (random flows, sources, no
+    reductions involved): the FP64 kernels equal the oracle bit for bit."""
+    from efdc_amd import grids
+    case = grids.transport_case(40, 24, 6, seed=31, isadac=1, isfct=isfct)
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref)
+    ctx = _ctx(case)
+    o.calconc(niter=1)
+    ctx.calconc(case.dt, 1)
+    kc = case.grid.KC
+    for s, c in enumerate(ref.consts):
+        con, _ = ctx.get_constituent(s)
+        err = np.max(np.abs(con[:kc] - c["con"][:kc]))
+        print(f"fct={isfct} const {s}: max|dC| {err:.3e}")
+        assert err <= 1e-12 * max(1.0, np.max(np.abs(c["con"][:kc])))
+
+
+def test_fp32_antidiffusion_fct_within_1e5(oracle_mod):
+    from efdc_amd import grids
+    from efdc_amd.lib import STORE_FP32
+    case = grids.transport_case(48, 32, 6, seed=17, isadac=1, isfct=1)
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref)
+    ctx = _ctx(case, store=STORE_FP32)
+    for n in range(1, 6):
+        o.calconc(niter=n)
+        ctx.calconc(case.dt, n)
+    kc = case.grid.KC
+    for s, c in enumerate(ref.consts):
+        con, _ = ctx.get_constituent(s)
+        scale = max(1e-30, np.max(np.abs(c["con"][:kc])))
+        rel = np.max(np.abs(con[:kc] - c["con"][:kc])) / scale
+        print("fp32 adac const", s, "rel", rel)
+        assert rel <= 1e-5
