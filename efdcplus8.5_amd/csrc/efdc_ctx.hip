@@ -131,25 +131,34 @@ void efdc_params_default(efdc_params_t *p) {
   p->mixed_max_outer = 30;
 }
 
+// Reason of the last failed efdc_ctx_create on this thread (there is no
+// context to hold it); efdc_last_error(NULL) returns it.
+static thread_local std::string g_create_err = "no error";
+
 const char *efdc_last_error(const efdc_ctx_t *ctx) {
-  return ctx ? ctx->err.c_str() : "null context";
+  return ctx ? ctx->err.c_str() : g_create_err.c_str();
 }
 
 int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
                     efdc_ctx_t **out) {
-  if (!out) return EFDC_ERR_ARG;
+  auto reject = [](int st, const char *why) {
+    g_create_err = why;
+    return st;
+  };
+  if (!out) return reject(EFDC_ERR_ARG, "null output pointer");
   *out = nullptr;
-  if (!g || !prm) return EFDC_ERR_ARG;
+  if (!g || !prm) return reject(EFDC_ERR_ARG, "null grid or params");
   if (g->LA < 2 || g->LCM < g->LA + 1 || g->KC < 1 || g->KCM < g->KC ||
       g->KC > 64 || !g->LWC || !g->LEC || !g->LSC || !g->LNC)
-    return EFDC_ERR_ARG;
+    return reject(EFDC_ERR_ARG, "grid sizes or maps invalid (need LA>=2, "
+                                "LCM>=LA+1, 1<=KC<=64, KCM>=KC, four maps)");
   if (prm->store != EFDC_STORE_FP64 && prm->store != EFDC_STORE_FP32)
-    return EFDC_ERR_ARG;
+    return reject(EFDC_ERR_ARG, "unknown storage precision");
   if (prm->cg_mode != EFDC_CG_FP64 && prm->cg_mode != EFDC_CG_MIXED)
-    return EFDC_ERR_ARG;
+    return reject(EFDC_ERR_ARG, "unknown CG mode");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
-    return EFDC_ERR_NODEVICE;
+    return reject(EFDC_ERR_NODEVICE, "no HIP device visible");
   efdc_ctx *c = new efdc_ctx();
   (void)hipGetDevice(&c->device);
   c->LA = g->LA;
@@ -162,6 +171,7 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
   c->store = prm->store;
   c->tsz = prm->store == EFDC_STORE_FP32 ? 4 : 8;
   auto fail = [&](int s) {
+    g_create_err = c->err.empty() ? "device set-up failed" : c->err;
     efdc_ctx_destroy(c);
     return s;
   };

@@ -62,5 +62,39 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     return LIB
 
 
+FORTRAN = os.path.join(PKG, "fortran")
+FBIN = os.path.join(FORTRAN, "_bin")
+FC = os.environ.get("EFDC_FC", "/opt/rocm/llvm/bin/flang")
+
+
+def build_fortran(global_mod_dir: str | None = None) -> str:
+    """Compile the ISO_C_BINDING module efdc_hip and link the Fortran
+    check program tests/fortran/congrad_check.f90 against the library.
+
+    With ``global_mod_dir`` (a directory holding the host's global.mod),
+    the drop-in CALPUV2C/CONGRAD/CALCONC are compiled too, as the host's
+    build would compile them.  Returns the path of the check program.
+    """
+    os.makedirs(FBIN, exist_ok=True)
+    mod = os.path.join(FBIN, "mod")
+    os.makedirs(mod, exist_ok=True)
+    iface = os.path.join(FORTRAN, "efdc_hip.f90")
+    subprocess.check_call([FC, "-O2", "-c", "-module-dir", mod, iface,
+                           "-o", os.path.join(mod, "efdc_hip.o")])
+    if global_mod_dir:
+        subprocess.check_call([FC, "-O2", "-c", "-fdefault-real-8", "-cpp",
+                               "-module-dir", mod, "-I", global_mod_dir,
+                               os.path.join(FORTRAN, "efdc_hip_dropin.f90"),
+                               "-o", os.path.join(mod, "efdc_hip_dropin.o")])
+    exe = os.path.join(FBIN, "congrad_check")
+    subprocess.check_call([FC, "-O2", "-I", mod,
+                           os.path.join(ROOT, "tests", "fortran", "congrad_check.f90"),
+                           os.path.join(mod, "efdc_hip.o"),
+                           "-L" + HERE, "-lefdc_hip",
+                           "-Wl,-rpath,$ORIGIN/../../efdc_amd", "-o", exe])
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran())

@@ -1,0 +1,239 @@
+! efdc_hip.f90 -- ISO_C_BINDING interface to the C-ABI of include/efdc_hip.h.
+!
+! The Fortran host (EFDC+ 8.5) calls the gfx950 library through this
+! module.  It does not depend on MODULE GLOBAL; the drop-in replacements of
+! CALPUV2C, CONGRAD and CALCONC that copy the GLOBAL arrays across the
+! boundary are in efdc_hip_dropin.f90.
+module efdc_hip
+  use iso_c_binding
+  implicit none
+
+  ! efdc_status
+  integer(c_int), parameter :: EFDC_OK = 0, EFDC_ERR_ARG = 1, EFDC_ERR_HIP = 2, &
+      EFDC_ERR_CG_MAXITER = 3, EFDC_ERR_NEGDEPTH = 4, EFDC_ERR_STATE = 5, &
+      EFDC_ERR_NOMEM = 6, EFDC_ERR_NODEVICE = 7
+  ! efdc_cg_mode, efdc_store
+  integer(c_int32_t), parameter :: EFDC_CG_FP64 = 0, EFDC_CG_MIXED = 1
+  integer(c_int32_t), parameter :: EFDC_STORE_FP64 = 0, EFDC_STORE_FP32 = 1
+
+  ! efdc_array_id: (LCM) arrays
+  integer(c_int32_t), parameter :: EFDC_DXYP = 0, EFDC_DXYIP = 1, EFDC_DXYU = 2, &
+      EFDC_DXYV = 3, EFDC_DXIU = 4, EFDC_DYIU = 5, EFDC_DXIV = 6, EFDC_DYIV = 7, &
+      EFDC_HRUO = 8, EFDC_HRVO = 9, EFDC_SUB = 10, EFDC_SVB = 11, EFDC_BELV = 12, &
+      EFDC_P = 13, EFDC_P1 = 14, EFDC_HP = 15, EFDC_H1P = 16, EFDC_H2P = 17, &
+      EFDC_HU = 18, EFDC_HV = 19, EFDC_HUI = 20, EFDC_HVI = 21, EFDC_HPI = 22, &
+      EFDC_H1U = 23, EFDC_H1V = 24, EFDC_H1UI = 25, EFDC_H1VI = 26, &
+      EFDC_UHDYE = 27, EFDC_VHDXE = 28, EFDC_UHDY1E = 29, EFDC_VHDX1E = 30, &
+      EFDC_UHE = 31, EFDC_VHE = 32, EFDC_TSX = 33, EFDC_TSY = 34, EFDC_TBX = 35, &
+      EFDC_TBY = 36, EFDC_FCAXE = 37, EFDC_FCAYE = 38, EFDC_FPGXE = 39, &
+      EFDC_FPGYE = 40, EFDC_FXE = 41, EFDC_FYE = 42, EFDC_QSUME = 43, &
+      EFDC_RCX = 44, EFDC_RCY = 45, EFDC_FUHDYE = 46, EFDC_FVHDXE = 47, &
+      EFDC_FP = 48, EFDC_FP1 = 49, EFDC_CC = 50, EFDC_CS = 51, EFDC_CW = 52, &
+      EFDC_CE = 53, EFDC_CN = 54, EFDC_CCC = 55, EFDC_CCS = 56, EFDC_CCW = 57, &
+      EFDC_CCE = 58, EFDC_CCN = 59, EFDC_CCCI = 60, EFDC_FPTMP = 61
+  ! (LCM,KCM) arrays
+  integer(c_int32_t), parameter :: EFDC_HPK = 64, EFDC_H1PK = 65, EFDC_HPKI = 66, &
+      EFDC_UHDY2 = 67, EFDC_VHDX2 = 68, EFDC_CDZKK = 69, EFDC_CDZKMK = 70, &
+      EFDC_AB = 71, EFDC_QSUM = 72, EFDC_U2 = 73, EFDC_V2 = 74, EFDC_SUB3D = 75, &
+      EFDC_SVB3D = 76, EFDC_DZC = 77
+  ! (LCM,0:KCM) arrays
+  integer(c_int32_t), parameter :: EFDC_W2 = 96, EFDC_DZIG = 97
+
+  type, bind(C) :: efdc_grid_t
+    integer(c_int32_t) :: LA, LCM, KC, KCM
+    type(c_ptr) :: LWC, LEC, LSC, LNC, LNWC, LSEC
+  end type
+
+  type, bind(C) :: efdc_params_t
+    real(c_double) :: G, RITB1, SNLT, HDRY, TCON_TBEGIN
+    integer(c_int32_t) :: ITERM
+    real(c_double) :: RSQM, cg_rel_tol
+    integer(c_int32_t) :: cg_mode, store
+    real(c_double) :: mixed_inner_rtol
+    integer(c_int32_t) :: mixed_max_outer
+  end type
+
+  type, bind(C) :: efdc_pbc_side_t
+    integer(c_int32_t) :: n
+    type(c_ptr) :: L, ispb, pser, pcb, psb
+  end type
+
+  type, bind(C) :: efdc_open_bc_t
+    type(efdc_pbc_side_t) :: side(4)     ! W, E, S, N
+    integer(c_int32_t) :: mtide
+    type(c_ptr) :: tcp
+  end type
+
+  type, bind(C) :: efdc_cbc_side_t
+    integer(c_int32_t) :: n
+    type(c_ptr) :: L, cb, csert, ntscr
+  end type
+
+  type, bind(C) :: efdc_conc_bc_t
+    type(efdc_cbc_side_t) :: side(4)     ! S, W, E, N
+    type(c_ptr) :: wtci
+  end type
+
+  type, bind(C) :: efdc_sources_t
+    integer(c_int32_t) :: n
+    type(c_ptr) :: LQS, qss, qsercell, cqs, csert
+  end type
+
+  type, bind(C) :: efdc_constituent_t
+    integer(c_int32_t) :: mo, isadac, isfct
+    real(c_double) :: rkdye
+    integer(c_int32_t) :: is_dye
+  end type
+
+  type, bind(C) :: efdc_step_t
+    real(c_double) :: dt
+    integer(c_int64_t) :: n, niter
+  end type
+
+  type, bind(C) :: efdc_cg_stats_t
+    integer(c_int32_t) :: iters, outer
+    real(c_double) :: rsq, rsqm_used, ccmnm
+  end type
+
+  type, bind(C) :: efdc_balance_t
+    real(c_double) :: volume
+    real(c_double) :: mass(16)
+    integer(c_int32_t) :: nconst
+  end type
+
+  interface
+    integer(c_int) function efdc_abi_version() bind(C, name="efdc_abi_version")
+      import :: c_int
+    end function
+
+    subroutine efdc_params_default(p) bind(C, name="efdc_params_default")
+      import :: efdc_params_t
+      type(efdc_params_t), intent(out) :: p
+    end subroutine
+
+    integer(c_int) function efdc_ctx_create(grid, params, ctx) bind(C, name="efdc_ctx_create")
+      import :: c_int, c_ptr, efdc_grid_t, efdc_params_t
+      type(efdc_grid_t), intent(in) :: grid
+      type(efdc_params_t), intent(in) :: params
+      type(c_ptr), intent(out) :: ctx
+    end function
+
+    integer(c_int) function efdc_ctx_destroy(ctx) bind(C, name="efdc_ctx_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    type(c_ptr) function efdc_last_error(ctx) bind(C, name="efdc_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function efdc_set_params(ctx, params) bind(C, name="efdc_set_params")
+      import :: c_int, c_ptr, efdc_params_t
+      type(c_ptr), value :: ctx
+      type(efdc_params_t), intent(in) :: params
+    end function
+
+    integer(c_int) function efdc_set_array(ctx, id, host) bind(C, name="efdc_set_array")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: id
+      real(c_double), intent(in) :: host(*)
+    end function
+
+    integer(c_int) function efdc_get_array(ctx, id, host) bind(C, name="efdc_get_array")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: id
+      real(c_double), intent(inout) :: host(*)
+    end function
+
+    integer(c_int) function efdc_set_open_bc(ctx, bc) bind(C, name="efdc_set_open_bc")
+      import :: c_int, c_ptr, efdc_open_bc_t
+      type(c_ptr), value :: ctx
+      type(efdc_open_bc_t), intent(in) :: bc
+    end function
+
+    integer(c_int) function efdc_set_conc_bc(ctx, bc) bind(C, name="efdc_set_conc_bc")
+      import :: c_int, c_ptr, efdc_conc_bc_t
+      type(c_ptr), value :: ctx
+      type(efdc_conc_bc_t), intent(in) :: bc
+    end function
+
+    integer(c_int) function efdc_set_sources(ctx, src) bind(C, name="efdc_set_sources")
+      import :: c_int, c_ptr, efdc_sources_t
+      type(c_ptr), value :: ctx
+      type(efdc_sources_t), intent(in) :: src
+    end function
+
+    integer(c_int) function efdc_add_constituent(ctx, desc, slot) bind(C, name="efdc_add_constituent")
+      import :: c_int, c_int32_t, c_ptr, efdc_constituent_t
+      type(c_ptr), value :: ctx
+      type(efdc_constituent_t), intent(in) :: desc
+      integer(c_int32_t), intent(out) :: slot
+    end function
+
+    integer(c_int) function efdc_set_constituent(ctx, slot, con, con1) bind(C, name="efdc_set_constituent")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: slot
+      real(c_double), intent(in) :: con(*), con1(*)
+    end function
+
+    integer(c_int) function efdc_get_constituent(ctx, slot, con, con1) bind(C, name="efdc_get_constituent")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: slot
+      real(c_double), intent(inout) :: con(*), con1(*)
+    end function
+
+    integer(c_int) function efdc_congrad(ctx, stats) bind(C, name="efdc_congrad")
+      import :: c_int, c_ptr, efdc_cg_stats_t
+      type(c_ptr), value :: ctx
+      type(efdc_cg_stats_t), intent(out) :: stats
+    end function
+
+    integer(c_int) function efdc_calpuv2c(ctx, step, stats) bind(C, name="efdc_calpuv2c")
+      import :: c_int, c_ptr, efdc_step_t, efdc_cg_stats_t
+      type(c_ptr), value :: ctx
+      type(efdc_step_t), intent(in) :: step
+      type(efdc_cg_stats_t), intent(out) :: stats
+    end function
+
+    integer(c_int) function efdc_calconc(ctx, step) bind(C, name="efdc_calconc")
+      import :: c_int, c_ptr, efdc_step_t
+      type(c_ptr), value :: ctx
+      type(efdc_step_t), intent(in) :: step
+    end function
+
+    integer(c_int) function efdc_balance(ctx, out) bind(C, name="efdc_balance")
+      import :: c_int, c_ptr, efdc_balance_t
+      type(c_ptr), value :: ctx
+      type(efdc_balance_t), intent(out) :: out
+    end function
+
+    integer(c_int) function efdc_synchronize(ctx) bind(C, name="efdc_synchronize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+  end interface
+
+contains
+
+  ! Text of the last error of a context as a Fortran string.
+  function efdc_error_text(ctx) result(msg)
+    type(c_ptr), intent(in) :: ctx
+    character(len=256) :: msg
+    character(kind=c_char), pointer :: s(:)
+    type(c_ptr) :: p
+    integer :: j
+    msg = ' '
+    p = efdc_last_error(ctx)
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, s, [256])
+    do j = 1, 256
+      if (s(j) == c_null_char) exit
+      msg(j:j) = s(j)
+    end do
+  end function
+end module efdc_hip

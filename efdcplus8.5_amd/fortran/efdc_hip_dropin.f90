@@ -1,0 +1,323 @@
+! efdc_hip_dropin.f90 -- drop-in replacements of CALPUV2C, CONGRAD and
+! CALCONC for the EFDC+ 8.5 Fortran host.  Link this file (with
+! efdc_hip.f90 and -lefdc_hip) in place of calpuv2c.f90, congrad.f90 and
+! calconc.f90; the callers (HDMT2T: EFDC/hdmt2t.f90:509,562) are unchanged.
+!
+! Each routine copies the MODULE GLOBAL inputs it needs to the device,
+! runs the hot path there and copies back every output the host reads
+! afterwards.  Scope: the 2TL scheme with ISDRY=0, IGRIDV=0, MDCHH=0,
+! ISGWIE=0, ISDYNSTP=0 and sigma layers; other settings stop at the first
+! call with a message, like the reference's own unsupported-option stops.
+module efdc_hip_state
+  use iso_c_binding
+  implicit none
+  type(c_ptr), save :: HCTX = c_null_ptr
+  integer, save :: NSLOT = 0
+  integer, save :: SLOT_KIND(16) = 0     ! 1 SAL, 2 TEM, 3 DYE, 6 SED
+  integer, save :: SLOT_NS(16) = 0       ! sediment class of a SED slot
+  integer(c_int32_t), allocatable, target, save :: MW(:), ME(:), MS(:), MN(:), MNW(:), MSE(:)
+  real(c_double), allocatable, target, save :: TCPH(:)
+end module efdc_hip_state
+
+subroutine EFDC_HIP_CHECK(HXERR, WHERE)
+  use efdc_hip
+  use efdc_hip_state
+  implicit none
+  integer(c_int), intent(in) :: HXERR
+  character(*), intent(in) :: WHERE
+  if (HXERR /= EFDC_OK) then
+    write(6,'(A,A,A,I3,A,A)') ' EFDC_HIP: ', WHERE, ' failed, status ', HXERR, ': ', &
+         trim(efdc_error_text(HCTX))
+    stop 'EFDC_HIP ERROR'
+  end if
+end subroutine
+
+subroutine EFDC_HIP_INIT
+  use GLOBAL
+  use efdc_hip
+  use efdc_hip_state
+  implicit none
+  type(efdc_grid_t) :: HXGRID
+  type(efdc_params_t) :: HXPRM
+  type(efdc_constituent_t) :: HXD
+  integer(c_int) :: HXERR
+  integer(c_int32_t) :: HXSLOT
+  integer :: NS
+
+  if (ISDRY /= 0 .or. IGRIDV /= 0 .or. MDCHH /= 0 .or. ISGWIE /= 0 .or. ISDYNSTP /= 0) then
+    stop 'EFDC_HIP: needs ISDRY=0, IGRIDV=0, MDCHH=0, ISGWIE=0, ISDYNSTP=0'
+  end if
+  allocate(MW(LCM), ME(LCM), MS(LCM), MN(LCM), MNW(LCM), MSE(LCM))
+  MW = LWC(1:LCM); ME = LEC(1:LCM); MS = LSC(1:LCM); MN = LNC(1:LCM)
+  MNW = LNWC(1:LCM); MSE = LSEC(1:LCM)
+  HXGRID%LA = LA; HXGRID%LCM = LCM; HXGRID%KC = KC; HXGRID%KCM = KCM
+  HXGRID%LWC = c_loc(MW); HXGRID%LEC = c_loc(ME); HXGRID%LSC = c_loc(MS); HXGRID%LNC = c_loc(MN)
+  HXGRID%LNWC = c_loc(MNW); HXGRID%LSEC = c_loc(MSE)
+  call efdc_params_default(HXPRM)
+  HXPRM%G = G; HXPRM%RITB1 = RITB1; HXPRM%SNLT = SNLT; HXPRM%HDRY = HDRY
+  HXPRM%TCON_TBEGIN = TCON*TBEGIN; HXPRM%ITERM = ITERM; HXPRM%RSQM = RSQM
+  HXERR = efdc_ctx_create(HXGRID, HXPRM, HCTX)
+  call EFDC_HIP_CHECK(HXERR, 'efdc_ctx_create')
+  ! geometry and layer constants (EFDC/aaefdc.f90:1212-1232,2172-2185)
+  HXERR = efdc_set_array(HCTX, EFDC_DXYP, DXYP);   call EFDC_HIP_CHECK(HXERR, 'DXYP')
+  HXERR = efdc_set_array(HCTX, EFDC_DXYIP, DXYIP); call EFDC_HIP_CHECK(HXERR, 'DXYIP')
+  HXERR = efdc_set_array(HCTX, EFDC_DXYU, DXYU);   call EFDC_HIP_CHECK(HXERR, 'DXYU')
+  HXERR = efdc_set_array(HCTX, EFDC_DXYV, DXYV);   call EFDC_HIP_CHECK(HXERR, 'DXYV')
+  HXERR = efdc_set_array(HCTX, EFDC_DXIU, DXIU);   call EFDC_HIP_CHECK(HXERR, 'DXIU')
+  HXERR = efdc_set_array(HCTX, EFDC_DYIU, DYIU);   call EFDC_HIP_CHECK(HXERR, 'DYIU')
+  HXERR = efdc_set_array(HCTX, EFDC_DXIV, DXIV);   call EFDC_HIP_CHECK(HXERR, 'DXIV')
+  HXERR = efdc_set_array(HCTX, EFDC_DYIV, DYIV);   call EFDC_HIP_CHECK(HXERR, 'DYIV')
+  HXERR = efdc_set_array(HCTX, EFDC_HRUO, HRUO);   call EFDC_HIP_CHECK(HXERR, 'HRUO')
+  HXERR = efdc_set_array(HCTX, EFDC_HRVO, HRVO);   call EFDC_HIP_CHECK(HXERR, 'HRVO')
+  HXERR = efdc_set_array(HCTX, EFDC_SUB, SUB);     call EFDC_HIP_CHECK(HXERR, 'SUB')
+  HXERR = efdc_set_array(HCTX, EFDC_SVB, SVB);     call EFDC_HIP_CHECK(HXERR, 'SVB')
+  HXERR = efdc_set_array(HCTX, EFDC_BELV, BELV);   call EFDC_HIP_CHECK(HXERR, 'BELV')
+  HXERR = efdc_set_array(HCTX, EFDC_DZC, DZC);     call EFDC_HIP_CHECK(HXERR, 'DZC')
+  HXERR = efdc_set_array(HCTX, EFDC_CDZKK, CDZKK); call EFDC_HIP_CHECK(HXERR, 'CDZKK')
+  HXERR = efdc_set_array(HCTX, EFDC_CDZKMK, CDZKMK); call EFDC_HIP_CHECK(HXERR, 'CDZKMK')
+  HXERR = efdc_set_array(HCTX, EFDC_DZIG, DZIG);   call EFDC_HIP_CHECK(HXERR, 'DZIG')
+  HXERR = efdc_set_array(HCTX, EFDC_SUB3D, SUB3D); call EFDC_HIP_CHECK(HXERR, 'SUB3D')
+  HXERR = efdc_set_array(HCTX, EFDC_SVB3D, SVB3D); call EFDC_HIP_CHECK(HXERR, 'SVB3D')
+  ! transported constituents in the order CALCONC calls CALTRAN
+  ! (EFDC/calconc.f90:159-228)
+  NSLOT = 0
+  if (ISTRAN(1) == 1) call ADD(1, 1, 0)
+  if (ISTRAN(2) == 1) call ADD(2, 2, 0)
+  if (ISTRAN(3) == 1) call ADD(3, 3, 0)
+  if (ISTRAN(6) == 1) then
+    do NS = 1, NSED
+      call ADD(6, MSVSED(NS), NS)
+    end do
+  end if
+contains
+  subroutine ADD(MVAR, MO, NS_)
+    integer, intent(in) :: MVAR, MO, NS_
+    HXD%mo = MO; HXD%isadac = ISADAC(MVAR); HXD%isfct = ISFCT(MVAR)
+    HXD%rkdye = 0.0; HXD%is_dye = 0
+    if (MVAR == 3) then
+      HXD%rkdye = RKDYE; HXD%is_dye = 1
+    end if
+    HXERR = efdc_add_constituent(HCTX, HXD, HXSLOT)
+    call EFDC_HIP_CHECK(HXERR, 'efdc_add_constituent')
+    NSLOT = NSLOT + 1
+    SLOT_KIND(NSLOT) = MVAR
+    SLOT_NS(NSLOT) = NS_
+  end subroutine
+end subroutine EFDC_HIP_INIT
+
+! Pressure open boundaries of this step (the time-series part of
+! EFDC/setopenbc.f90:44-48 is formed on the host; the tide is added on the
+! device).
+subroutine EFDC_HIP_OPENBC
+  use GLOBAL
+  use efdc_hip
+  use efdc_hip_state
+  implicit none
+  type(efdc_open_bc_t) :: HXBC
+  integer(c_int32_t), allocatable, target, save :: LW_(:), LE_(:), LS_(:), LN_(:)
+  integer(c_int32_t), allocatable, target, save :: IW_(:), IE_(:), IS_(:), IN_(:)
+  real(c_double), allocatable, target, save :: SW_(:), SE_(:), SS_(:), SN_(:)
+  real(c_double), allocatable, target, save :: CW_(:,:), CE_(:,:), CS_(:,:), CN_(:,:)
+  real(c_double), allocatable, target, save :: QW_(:,:), QE_(:,:), QS_(:,:), QN_(:,:)
+  integer(c_int) :: HXERR
+  integer :: LL, M
+
+  if (.not. allocated(LW_)) then
+    allocate(LW_(max(NPBW,1)), LE_(max(NPBE,1)), LS_(max(NPBS,1)), LN_(max(NPBN,1)))
+    allocate(IW_(max(NPBW,1)), IE_(max(NPBE,1)), IS_(max(NPBS,1)), IN_(max(NPBN,1)))
+    allocate(SW_(max(NPBW,1)), SE_(max(NPBE,1)), SS_(max(NPBS,1)), SN_(max(NPBN,1)))
+    allocate(CW_(max(MTIDE,1),max(NPBW,1)), CE_(max(MTIDE,1),max(NPBE,1)))
+    allocate(CS_(max(MTIDE,1),max(NPBS,1)), CN_(max(MTIDE,1),max(NPBN,1)))
+    allocate(QW_(max(MTIDE,1),max(NPBW,1)), QE_(max(MTIDE,1),max(NPBE,1)))
+    allocate(QS_(max(MTIDE,1),max(NPBS,1)), QN_(max(MTIDE,1),max(NPBN,1)))
+    allocate(TCPH(max(MTIDE,1)))
+    TCPH = 0.0
+    if (MTIDE > 0) TCPH(1:MTIDE) = TCP(1:MTIDE)
+  end if
+  do LL = 1, NPBW
+    LW_(LL) = LPBW(LL); IW_(LL) = ISPBW(LL)
+    SW_(LL) = PSERT(NPSERW(LL)) + 0.5*PSERZDF(NPSERW(LL)) + PSERST(NPSERW(LL)) + 0.5*PSERZDS(NPSERW(LL))
+    do M = 1, MTIDE
+      CW_(M,LL) = PCBW(LL,M); QW_(M,LL) = PSBW(LL,M)
+    end do
+  end do
+  do LL = 1, NPBE
+    LE_(LL) = LPBE(LL); IE_(LL) = ISPBE(LL)
+    SE_(LL) = PSERT(NPSERE(LL)) + 0.5*PSERZDF(NPSERE(LL)) + PSERST(NPSERE(LL)) + 0.5*PSERZDS(NPSERE(LL))
+    do M = 1, MTIDE
+      CE_(M,LL) = PCBE(LL,M); QE_(M,LL) = PSBE(LL,M)
+    end do
+  end do
+  do LL = 1, NPBS
+    LS_(LL) = LPBS(LL); IS_(LL) = ISPBS(LL)
+    SS_(LL) = PSERT(NPSERS(LL)) + 0.5*PSERZDF(NPSERS(LL)) + PSERST(NPSERS(LL)) + 0.5*PSERZDS(NPSERS(LL))
+    do M = 1, MTIDE
+      CS_(M,LL) = PCBS(LL,M); QS_(M,LL) = PSBS(LL,M)
+    end do
+  end do
+  do LL = 1, NPBN
+    LN_(LL) = LPBN(LL); IN_(LL) = ISPBN(LL)
+    SN_(LL) = PSERT(NPSERN(LL)) + 0.5*PSERZDF(NPSERN(LL)) + PSERST(NPSERN(LL)) + 0.5*PSERZDS(NPSERN(LL))
+    do M = 1, MTIDE
+      CN_(M,LL) = PCBN(LL,M); QN_(M,LL) = PSBN(LL,M)
+    end do
+  end do
+  if (NPFORT >= 1) stop 'EFDC_HIP: NPFORT>=1 open-boundary blending is formed by the caller'
+  HXBC%side(1) = efdc_pbc_side_t(NPBW, c_loc(LW_), c_loc(IW_), c_loc(SW_), c_loc(CW_), c_loc(QW_))
+  HXBC%side(2) = efdc_pbc_side_t(NPBE, c_loc(LE_), c_loc(IE_), c_loc(SE_), c_loc(CE_), c_loc(QE_))
+  HXBC%side(3) = efdc_pbc_side_t(NPBS, c_loc(LS_), c_loc(IS_), c_loc(SS_), c_loc(CS_), c_loc(QS_))
+  HXBC%side(4) = efdc_pbc_side_t(NPBN, c_loc(LN_), c_loc(IN_), c_loc(SN_), c_loc(CN_), c_loc(QN_))
+  HXBC%mtide = MTIDE
+  HXBC%tcp = c_loc(TCPH)
+  HXERR = efdc_set_open_bc(HCTX, HXBC)
+  call EFDC_HIP_CHECK(HXERR, 'efdc_set_open_bc')
+end subroutine EFDC_HIP_OPENBC
+
+SUBROUTINE CALPUV2C
+  use GLOBAL
+  use efdc_hip
+  use efdc_hip_state
+  implicit none
+  type(efdc_step_t) :: HXSTEP
+  type(efdc_cg_stats_t) :: HXST
+  integer(c_int) :: HXERR
+  integer :: L
+
+  if (.not. c_associated(HCTX)) call EFDC_HIP_INIT
+  if (NBCSOP > 0) call EFDC_HIP_OPENBC
+  DELT = DT; DELTI = 1./DELT; ISTL = 2                ! EFDC/calpuv2c.f90:128-137
+  ! inputs produced by the host since the last call
+  call PUSH(EFDC_UHDYE, UHDYE); call PUSH(EFDC_VHDXE, VHDXE); call PUSH(EFDC_P, P)
+  call PUSH(EFDC_HU, HU); call PUSH(EFDC_HV, HV); call PUSH(EFDC_HUI, HUI)
+  call PUSH(EFDC_HVI, HVI); call PUSH(EFDC_HP, HP); call PUSH(EFDC_H1P, H1P)
+  call PUSH(EFDC_TSX, TSX); call PUSH(EFDC_TSY, TSY); call PUSH(EFDC_TBX, TBX)
+  call PUSH(EFDC_TBY, TBY); call PUSH(EFDC_FCAXE, FCAXE); call PUSH(EFDC_FCAYE, FCAYE)
+  call PUSH(EFDC_FPGXE, FPGXE); call PUSH(EFDC_FPGYE, FPGYE); call PUSH(EFDC_FXE, FXE)
+  call PUSH(EFDC_FYE, FYE); call PUSH(EFDC_QSUME, QSUME); call PUSH(EFDC_RCX, RCX)
+  call PUSH(EFDC_RCY, RCY); call PUSH(EFDC_HPK, HPK)
+  HXSTEP%dt = DT; HXSTEP%n = N; HXSTEP%niter = NITER
+  HXERR = efdc_calpuv2c(HCTX, HXSTEP, HXST)
+  IDRYTBP = HXST%iters                                ! EFDC/congrad.f90:187
+  if (HXERR == EFDC_ERR_CG_MAXITER) then
+    ! the reference's failure path (EFDC/congrad.f90:171-184)
+    call PULL(EFDC_CCS, CCS); call PULL(EFDC_CCW, CCW); call PULL(EFDC_CCC, CCC)
+    call PULL(EFDC_CCE, CCE); call PULL(EFDC_CCN, CCN); call PULL(EFDC_FPTMP, FPTMP)
+    open(8, FILE=OUTDIR//'EFDCLOG.OUT', POSITION='APPEND')
+    write(6,'(A)') '  MAXIMUM ITERATIONS EXCEEDED IN EXTERNAL SOLUTION'
+    write(8,'(A)') '   I    J     CCS     CCW      CCC      CCE      CCN     FPTMP'
+    do L = 2, LA
+      write(8,'(2I6,6E13.4)') IL(L), JL(L), CCS(L), CCW(L), CCC(L), CCE(L), CCN(L), FPTMP(L)
+    end do
+    close(8)
+    stop
+  end if
+  if (HXERR /= EFDC_ERR_NEGDEPTH) call EFDC_HIP_CHECK(HXERR, 'efdc_calpuv2c')
+  ! outputs read by the host afterwards
+  call PULL(EFDC_P, P); call PULL(EFDC_P1, P1); call PULL(EFDC_HP, HP)
+  call PULL(EFDC_H1P, H1P); call PULL(EFDC_H2P, H2P); call PULL(EFDC_HU, HU)
+  call PULL(EFDC_HV, HV); call PULL(EFDC_HUI, HUI); call PULL(EFDC_HVI, HVI)
+  call PULL(EFDC_HPI, HPI); call PULL(EFDC_H1U, H1U); call PULL(EFDC_H1V, H1V)
+  call PULL(EFDC_H1UI, H1UI); call PULL(EFDC_H1VI, H1VI); call PULL(EFDC_UHDYE, UHDYE)
+  call PULL(EFDC_VHDXE, VHDXE); call PULL(EFDC_UHDY1E, UHDY1E); call PULL(EFDC_VHDX1E, VHDX1E)
+  call PULL(EFDC_UHE, UHE); call PULL(EFDC_VHE, VHE); call PULL(EFDC_FUHDYE, FUHDYE)
+  call PULL(EFDC_FVHDXE, FVHDXE); call PULL(EFDC_FP, FP); call PULL(EFDC_FP1, FP1)
+  call PULL(EFDC_CCC, CCC); call PULL(EFDC_CCCI, CCCI); call PULL(EFDC_FPTMP, FPTMP)
+  call PULL(EFDC_H1PK, H1PK)
+  if (HXERR == EFDC_ERR_NEGDEPTH) then
+    write(6,'(A,F12.4)') ' EFDC_HIP: NEGATIVE DEPTH AFTER THE EXTERNAL SOLUTION AT TIMEDAY ', TIMEDAY
+    stop 'NEGDEP'
+  end if
+  LAWET = LA - 1                                      ! EFDC/calpuv2c.f90:1061-1063
+  LADRY = 0
+contains
+  subroutine PUSH(ID, A)
+    integer(c_int32_t), intent(in) :: ID
+    real(c_double), intent(in) :: A(*)
+    call EFDC_HIP_CHECK(efdc_set_array(HCTX, ID, A), 'efdc_set_array')
+  end subroutine
+  subroutine PULL(ID, A)
+    integer(c_int32_t), intent(in) :: ID
+    real(c_double), intent(inout) :: A(*)
+    call EFDC_HIP_CHECK(efdc_get_array(HCTX, ID, A), 'efdc_get_array')
+  end subroutine
+END SUBROUTINE CALPUV2C
+
+SUBROUTINE CONGRAD(NOPTIMAL, LDMOPT)
+  use GLOBAL
+  use efdc_hip
+  use efdc_hip_state
+  implicit none
+  integer, intent(in) :: NOPTIMAL, LDMOPT
+  type(efdc_cg_stats_t) :: HXST
+  integer(c_int) :: HXERR
+
+  if (.not. c_associated(HCTX)) call EFDC_HIP_INIT
+  call EFDC_HIP_CHECK(efdc_set_array(HCTX, EFDC_CCC, CCC), 'CCC')
+  call EFDC_HIP_CHECK(efdc_set_array(HCTX, EFDC_CCS, CCS), 'CCS')
+  call EFDC_HIP_CHECK(efdc_set_array(HCTX, EFDC_CCW, CCW), 'CCW')
+  call EFDC_HIP_CHECK(efdc_set_array(HCTX, EFDC_CCE, CCE), 'CCE')
+  call EFDC_HIP_CHECK(efdc_set_array(HCTX, EFDC_CCN, CCN), 'CCN')
+  call EFDC_HIP_CHECK(efdc_set_array(HCTX, EFDC_CCCI, CCCI), 'CCCI')
+  call EFDC_HIP_CHECK(efdc_set_array(HCTX, EFDC_FPTMP, FPTMP), 'FPTMP')
+  call EFDC_HIP_CHECK(efdc_set_array(HCTX, EFDC_P, P), 'P')
+  HXERR = efdc_congrad(HCTX, HXST)
+  call EFDC_HIP_CHECK(efdc_get_array(HCTX, EFDC_P, P), 'P')
+  IDRYTBP = HXST%iters
+  if (HXERR == EFDC_ERR_CG_MAXITER) then
+    write(6,'(A)') '  MAXIMUM ITERATIONS EXCEEDED IN EXTERNAL SOLUTION'
+    stop
+  end if
+  call EFDC_HIP_CHECK(HXERR, 'efdc_congrad')
+END SUBROUTINE CONGRAD
+
+SUBROUTINE CALCONC(ISTL_, IS2TL_)
+  use GLOBAL
+  use efdc_hip
+  use efdc_hip_state
+  implicit none
+  integer :: ISTL_, IS2TL_
+  type(efdc_step_t) :: HXSTEP
+  integer :: J
+
+  if (ISTL_ /= 2 .or. IS2TL_ /= 1) stop 'EFDC_HIP: CALCONC needs ISTL=2, IS2TL=1'
+  if (.not. c_associated(HCTX)) call EFDC_HIP_INIT
+  call PUSH(EFDC_UHDY2, UHDY2); call PUSH(EFDC_VHDX2, VHDX2); call PUSH(EFDC_W2, W2)
+  call PUSH(EFDC_H1PK, H1PK); call PUSH(EFDC_HPK, HPK); call PUSH(EFDC_HPKI, HPKI)
+  call PUSH(EFDC_AB, AB); call PUSH(EFDC_HPI, HPI); call PUSH(EFDC_QSUM, QSUM)
+  call PUSH(EFDC_U2, U2); call PUSH(EFDC_V2, V2)
+  do J = 1, NSLOT
+    select case (SLOT_KIND(J))
+    case (1)
+      call EFDC_HIP_CHECK(efdc_set_constituent(HCTX, J-1, SAL, SAL1), 'SAL')
+    case (2)
+      call EFDC_HIP_CHECK(efdc_set_constituent(HCTX, J-1, TEM, TEM1), 'TEM')
+    case (3)
+      call EFDC_HIP_CHECK(efdc_set_constituent(HCTX, J-1, DYE, DYE1), 'DYE')
+    case (6)
+      call EFDC_HIP_CHECK(efdc_set_constituent(HCTX, J-1, SED(:,:,SLOT_NS(J)), &
+                                              SED1(:,:,SLOT_NS(J))), 'SED')
+    end select
+  end do
+  DELT = DT
+  HXSTEP%dt = DT; HXSTEP%n = N; HXSTEP%niter = NITER
+  call EFDC_HIP_CHECK(efdc_calconc(HCTX, HXSTEP), 'efdc_calconc')
+  do J = 1, NSLOT
+    select case (SLOT_KIND(J))
+    case (1)
+      call EFDC_HIP_CHECK(efdc_get_constituent(HCTX, J-1, SAL, SAL1), 'SAL')
+    case (2)
+      call EFDC_HIP_CHECK(efdc_get_constituent(HCTX, J-1, TEM, TEM1), 'TEM')
+    case (3)
+      call EFDC_HIP_CHECK(efdc_get_constituent(HCTX, J-1, DYE, DYE1), 'DYE')
+    case (6)
+      call EFDC_HIP_CHECK(efdc_get_constituent(HCTX, J-1, SED(:,:,SLOT_NS(J)), &
+                                              SED1(:,:,SLOT_NS(J))), 'SED')
+    end select
+  end do
+contains
+  subroutine PUSH(ID, A)
+    integer(c_int32_t), intent(in) :: ID
+    real(c_double), intent(in) :: A(*)
+    call EFDC_HIP_CHECK(efdc_set_array(HCTX, ID, A), 'efdc_set_array')
+  end subroutine
+END SUBROUTINE CALCONC

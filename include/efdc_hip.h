@@ -230,6 +230,8 @@ int efdc_abi_version(void);
 int efdc_ctx_create(const efdc_grid_t *grid, const efdc_params_t *params,
                     efdc_ctx_t **out);
 int efdc_ctx_destroy(efdc_ctx_t *ctx);
+/* Text of the last error of a context; with ctx == NULL, the reason the
+   last efdc_ctx_create on the calling thread failed. */
 const char *efdc_last_error(const efdc_ctx_t *ctx);
 int efdc_set_params(efdc_ctx_t *ctx, const efdc_params_t *params);
 
