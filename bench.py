@@ -46,24 +46,40 @@ BYTES_PER_CELL = {
     "mx_apply": 28,           # P, D^-1/2 (2x8), y (4), P (8)
     "mx_prep": 52,            # CCC,CCS,CCW,FPTMP (4x8) + LS + D^-1/2 (8) + as,aw (2x4)
 }
-CG_KERNELS_FP64 = ("cg_init", "cg_spmv_pupdate", "cg_axpy_dots")
+CG_KERNELS_FP64 = ("cg_init", "cg_spmv_pupdate", "cg_axpy_dots",
+                   "cg_persistent64")
 CG_KERNELS_MIXED = ("mx_prep", "mx_residual64", "mx_spmv_pupdate32",
-                    "mx_axpy_dots32", "mx_apply")
+                    "mx_axpy_dots32", "mx_apply", "mx_persistent")
 
 
-def make_ctx(case, mixed):
+def persistent_bytes(name, iters, outer, nwet):
+    """Algorithmic bytes of one persistent solve: the same per-cell traffic
+    as the multi-kernel formulation of the iterations it performed (set-up,
+    every iteration, and for the mixed solve every refinement step)."""
+    b = BYTES_PER_CELL
+    if name == "cg_persistent64":
+        per = b["cg_init"] + iters * (b["cg_spmv_pupdate"] + b["cg_axpy_dots"])
+    else:
+        per = (b["mx_prep"] + (outer + 1) * b["mx_residual64"] + outer * b["mx_apply"] +
+               iters * (b["mx_spmv_pupdate32"] + b["mx_axpy_dots32"]))
+    return per * nwet
+
+
+def make_ctx(case, mixed, cg_kernel=0):
     from efdc_amd import lib
     ctx = lib.Context(case.grid, cg_mode=lib.CG_MIXED if mixed else lib.CG_FP64,
-                      cg_rel_tol=1e-10, RSQM=0.0, ITERM=1000)
+                      cg_rel_tol=1e-10, RSQM=0.0, ITERM=1000, cg_kernel=cg_kernel)
     ctx.upload_case(case)
     return ctx
 
 
-def run_steps(ctx, case, n0, k):
+def run_steps(ctx, case, n0, k, stats=None):
     iters = []
     for n in range(n0, n0 + k):
         st = ctx.calpuv2c(case.dt, n)
         iters.append(st["iters"])
+        if stats is not None:
+            stats.append(st)
     return iters
 
 
@@ -77,7 +93,8 @@ def timed(ctx, case, n0, k):
 
 def roofline(ctx, case, n0, k, kernels, nwet):
     ctx.timers(True)
-    run_steps(ctx, case, n0, k)
+    stats = []
+    run_steps(ctx, case, n0, k, stats)
     t = ctx.read_timers()
     ctx.timers(False)
     per = {}
@@ -86,13 +103,34 @@ def roofline(ctx, case, n0, k, kernels, nwet):
     cg = {k_: per[k_] for k_ in kernels if k_ in per}
     dom = max(cg, key=lambda k_: cg[k_]["total_ms"])
     avg_s = cg[dom]["avg_us"] * 1e-6
-    bytes_launch = BYTES_PER_CELL[dom] * nwet
+
+    def launch_bytes(name):
+        if name in ("cg_persistent64", "mx_persistent"):
+            tot = sum(persistent_bytes(name, st["iters"], st["outer"], nwet)
+                      for st in stats)
+            return tot / max(len(stats), 1), tot
+        return BYTES_PER_CELL[name] * nwet, BYTES_PER_CELL[name] * nwet * per[name]["launches"]
+
+    bytes_launch = launch_bytes(dom)[0]
     ach = bytes_launch / avg_s / 1e9
-    cg_bytes = sum(BYTES_PER_CELL[k_] * nwet * v["launches"] for k_, v in cg.items())
+    cg_bytes = sum(launch_bytes(k_)[1] for k_ in cg)
     cg_ms = sum(v["total_ms"] for v in cg.values())
     return dict(dominant=dom, achieved=ach, bytes_per_launch=bytes_launch,
                 avg_us=cg[dom]["avg_us"], cg_gbs=cg_bytes / (cg_ms * 1e-3) / 1e9,
                 cg_ms=cg_ms, kernels=per)
+
+
+def copy_probe(mib):
+    """One device-to-device copy of exactly ``mib`` MiB (read once, written
+    once): the known-size kernel that scales the HBM byte counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4}))
 
 
 def main():
@@ -104,7 +142,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cg-kernel", choices=("auto", "multi", "persistent"),
+                    default="auto", help="launch strategy of the CG solve")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only launch one plain device copy of MIB MiB and "
+                         "exit (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(collectives and sharding are out of scope); run with --gpus 1",
@@ -117,8 +163,9 @@ def main():
     case = grids.config1(n=args.n)
     nwet = case.grid.nwet
     ref_case = case.copy() if not args.no_cpu else None
-    ctx_m = make_ctx(case, mixed=True)
-    ctx_d = make_ctx(case, mixed=False)
+    ck = {"auto": 0, "multi": 1, "persistent": 2}[args.cg_kernel]
+    ctx_m = make_ctx(case, mixed=True, cg_kernel=ck)
+    ctx_d = make_ctx(case, mixed=False, cg_kernel=ck)
     i = np.arange(2, case.grid.LA + 1) - 1
 
     # ---- accuracy + CPU baseline on a bounded sample of the same steps
@@ -181,6 +228,7 @@ def main():
                         "CONGRAD to relative residual 1e-10",
             "wet_cells": nwet,
             "cg": "FP32 SpMV + FP64 dots + FP64 refinement",
+            "cg_kernel": args.cg_kernel,
         },
         "roofline": {
             "bound": "hbm",

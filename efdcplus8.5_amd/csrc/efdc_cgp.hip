@@ -1,0 +1,831 @@
+// efdc_cgp.hip -- CONGRAD as one persistent cooperative launch per solve.
+//
+// The multi-kernel solve (efdc_puv.hip) pays two launches and two grid
+// reductions per iteration; at the sizes of a real lake (10^4 .. 10^6 wet
+// cells, a working set that lives in the 256 MB MALL) that is latency, not
+// bandwidth.  Here every workgroup owns one contiguous range of wet cells
+// (L order) and keeps the rows of its cells -- coefficients, neighbour
+// indices and all CG vectors -- in registers for the whole solve.  Each
+// iteration exchanges only
+//   * the workgroup's own vector values through LDS, and
+//   * the few values that the neighbouring workgroups need (cells with a
+//     neighbour outside the range), written through to the coherent memory
+//     side and recomputed by the reader from the previous iterate,
+// and ends in a grid barrier that also carries the dot-product partials.
+//
+// FP64 (the reference's Jacobi-PCG, EFDC/congrad.f90:52-187): the same
+// recurrences and stop test as the reference, two barriers per iteration
+// (alpha needs p.Ap, beta needs r.z).
+//
+// Mixed: the inner FP32 CG on the Jacobi-scaled unit-diagonal operator is
+// written in the Chronopoulos-Gear form (the single reduction per
+// iteration carries both (r,r) and (Ar,r)), so it needs one barrier per
+// iteration; FP64 dot products, FP64 alpha/beta, FP64 residual refinement
+// of P inside the same launch.
+//
+// Barriers: every block counts in on the counter of its XCD group, the
+// last block of a group counts in on the global counter, every block
+// polls the global counter.  Polling is bounded; a block that gives up
+// raises the abort flag and leaves, the others give up too, and the host
+// reports the failure.  All blocks take the same branch at every barrier
+// (the reductions are computed identically in every block), so the
+// barrier sequence is the same in all blocks.
+#include "efdc_internal.h"
+
+namespace {
+
+constexpr int PB = 512;           // threads per workgroup
+constexpr int PW = PB / 64;       // waves per workgroup
+constexpr unsigned SPIN_MAX = 1u << 21;
+
+struct Bar {
+  unsigned *ctr;                  // 9 counters, TICKET_PAD apart
+  int *abort;
+};
+
+// Grid barrier number `gen` (1, 2, ...). Returns false if the barrier was
+// abandoned (time-out or abort raised by another block).
+__device__ bool grid_barrier(const Bar &b, unsigned gen) {
+  __shared__ int s_ok;
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's published values completed
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nb = (int)gridDim.x;
+    const int x = blockIdx.x & 7;
+    const unsigned nx = (unsigned)((nb >> 3) + (x < (nb & 7) ? 1 : 0));
+    const unsigned ng = nb < 8 ? (unsigned)nb : 8u;
+    unsigned *gx = b.ctr + x * TICKET_PAD, *gg = b.ctr + 8 * TICKET_PAD;
+    if (atomic_inc(gx) == gen * nx - 1u) atomic_inc(gg);
+    const unsigned target = gen * ng;
+    int ok = 1;
+    unsigned spins = 0;
+    while (__hip_atomic_load(gg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (++spins > SPIN_MAX ||
+          ((spins & 255u) == 0u &&
+           __hip_atomic_load(b.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        __hip_atomic_store(b.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    s_ok = ok;
+  }
+  __syncthreads();
+  return s_ok != 0;
+}
+
+// Block sum of NV values, published to part[j*stride + blockIdx.x].
+template <int NV>
+__device__ void publish_partials(double (&v)[NV], double *part, int stride) {
+  __shared__ double sh[NV][PW];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] = wave_sum(v[j]);
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) sh[j][w] = v[j];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      double s = 0.0;
+      for (int q = 0; q < PW; ++q) s += sh[j][q];
+      publish(&part[(size_t)j * stride + blockIdx.x], s);
+    }
+  }
+}
+
+// Sum of all blocks' partials, in the same order in every block (thread q
+// takes block q's partial, then a fixed tree), so every block obtains the
+// same bits.  Requires gridDim.x <= PB.
+template <int NV>
+__device__ void gather_sums(const double *part, int stride, double (&out)[NV]) {
+  __shared__ double sh[NV][PW];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double v[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j)
+    v[j] = (int)threadIdx.x < (int)gridDim.x ? fetch(&part[(size_t)j * stride + threadIdx.x]) : 0.0;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] = wave_sum(v[j]);
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) sh[j][w] = v[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    double s = 0.0;
+    for (int q = 0; q < PW; ++q) s += sh[j][q];
+    out[j] = s;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void publish_f(float *p, float v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float fetch_f(const float *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// neighbour slots of a row
+enum { NS = 0, NN = 1, NW = 2, NE = 3 };
+
+// An index the compiler must treat as unknown at this point: keeps it from
+// hoisting the 64-bit addresses of every neighbour access out of the
+// iteration loop (which would pin two VGPRs per address for the whole
+// solve and spill); the address is recomputed where it is used.
+__device__ __forceinline__ int opaque(int v) {
+  __asm__ volatile("" : "+v"(v));
+  return v;
+}
+
+// Every workgroup keeps a window of its range in LDS: the values of its own
+// cells plus up to WIN cells on either side (the cells just outside the
+// range that its rows reference; the host only launches when every
+// neighbour lies within WIN of its row).  Slot of cell j: j - b0 + WIN.
+constexpr int WIN = 4096;
+
+// Compact list of the outside cells a workgroup's rows reference
+// (sorted by index).  `mark` holds 2*WIN flags on entry and the list on
+// return; returns the list length.
+__device__ int compact_halo(int *mark, int b0, int b1) {
+  __shared__ int wtot[PW];
+  __shared__ int total;
+  constexpr int PER = 2 * WIN / PB;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int f[PER];
+  int cnt = 0;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    f[u] = mark[t * PER + u];
+    cnt += f[u] ? 1 : 0;
+  }
+  // exclusive scan of cnt over the workgroup
+  int inc = cnt;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += v;
+  }
+  if (lane == 63) wtot[w] = inc;
+  __syncthreads();
+  int base = 0;
+  for (int q = 0; q < w; ++q) base += wtot[q];
+  if (t == PB - 1) total = base + inc;
+  int pos = base + inc - cnt;
+  __syncthreads();   // every flag has been read
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    if (f[u]) {
+      const int g = t * PER + u;
+      mark[pos++] = g < WIN ? b0 - WIN + g : b1 + (g - WIN);
+    }
+  }
+  __syncthreads();
+  return total;
+}
+
+__device__ __forceinline__ int ghost_of(int j, int b0, int b1) {
+  return j < b0 ? WIN - (b0 - j) : WIN + (j - b1);
+}
+
+// ---------------------------------------------------------------------
+// FP64 Jacobi-PCG (EFDC/congrad.f90)
+// ---------------------------------------------------------------------
+struct P64Args {
+  const double *ccc, *ccs, *ccw, *ccn, *cce, *ccci, *fptmp;
+  double *x;                 // P, in/out
+  const int *lw, *le, *ls, *ln;
+  int nwet, lc0, rowmajor, mutual;
+  double *hr;                // published r (one buffer)
+  double *hp[2];             // published p, by iteration parity
+  double *part;              // [2 parities][3][stride]
+  int stride;
+  Bar bar;
+  CgState *st;
+  double rsqm, rel_tol;
+  int iterm;
+};
+
+template <int CPT>
+__global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
+  __shared__ double lds[2 * WIN + CPT * PB];
+  __shared__ int hl[2 * WIN];
+  const int b0 = 1 + (int)blockIdx.x * CPT * PB;
+  const int b1 = min(b0 + CPT * PB, a.nwet + 1);
+  const bool fast = a.rowmajor && (a.st->symok == 0);
+  const int lc0 = a.lc0;
+
+  double ccc[CPT], cs[CPT], cn[CPT], cw[CPT], ce[CPT], cci[CPT];
+  double x[CPT], r[CPT], p[CPT], ap[CPT];
+  int nb[CPT][4];
+  unsigned expo = 0u;        // bit k: another workgroup reads cell k
+
+  auto inside = [&](int j) { return j >= b0 && j < b1; };
+  auto real = [&](int j) { return j > 0 && j != lc0; };
+  auto slot = [&](int j) { return j - b0 + WIN; };
+
+  for (int g = threadIdx.x; g < 2 * WIN; g += PB) hl[g] = 0;
+  __syncthreads();
+  double v3[2] = {0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int i = b0 + k * PB + (int)threadIdx.x;
+    x[k] = r[k] = p[k] = ap[k] = 0.0;
+    ccc[k] = 1.0; cs[k] = cn[k] = cw[k] = ce[k] = cci[k] = 0.0;
+    nb[k][NS] = nb[k][NN] = nb[k][NW] = nb[k][NE] = lc0;
+    if (i < b1) {
+      // the row as in the multi-kernel solve (row_of)
+      const int js = a.ls[i], jn = a.ln[i];
+      int jw, je;
+      double s_ = a.ccs[i], w_ = a.ccw[i], n_, e_;
+      if (fast) {
+        jw = i - 1;
+        je = i + 1;
+        n_ = (jn != lc0) ? a.ccs[jn] : 0.0;
+        e_ = (je != lc0) ? a.ccw[je] : 0.0;
+      } else {
+        jw = a.lw[i];
+        je = a.le[i];
+        n_ = (jn != lc0) ? a.ccn[i] : 0.0;
+        e_ = (je != lc0) ? a.cce[i] : 0.0;
+        if (jw == lc0) w_ = 0.0;
+      }
+      if (js == lc0) s_ = 0.0;
+      nb[k][NS] = js; nb[k][NN] = jn; nb[k][NW] = jw; nb[k][NE] = je;
+      ccc[k] = a.ccc[i]; cs[k] = s_; cn[k] = n_; cw[k] = w_; ce[k] = e_;
+      cci[k] = a.ccci[i];
+      bool out = false;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = nb[k][q];
+        if (real(j) && !inside(j)) {
+          out = true;
+          hl[ghost_of(j, b0, b1)] = 1;
+        }
+      }
+      if (out || !a.mutual) expo |= 1u << k;
+      const double *P = a.x;
+      x[k] = P[i];
+      // RCG = FPTMP - CCC*P - CCN*PN - CCS*PS - CCW*PW - CCE*PE (:73)
+      const double rr = a.fptmp[i] - ccc[k] * x[k] - n_ * P[jn] - s_ * P[js] -
+                        w_ * P[jw] - e_ * P[je];
+      r[k] = rr;
+      v3[0] += rr * (rr * cci[k]);   // RPCG = sum RCG*PCG (:80-85)
+      v3[1] += a.fptmp[i] * a.fptmp[i];
+      if (expo & (1u << k)) publish(&a.hr[i], rr);
+    }
+  }
+  __syncthreads();
+  const int nhalo = compact_halo(hl, b0, b1);
+
+  unsigned gen = 0;
+  double rpcg, rsqm, beta = 0.0, alpha = 0.0, rsq = 0.0, papcg = 0.0, bnorm2;
+  {
+    publish_partials<2>(v3, a.part + (size_t)(gen & 1) * 3 * a.stride, a.stride);
+    if (!grid_barrier(a.bar, ++gen)) return;
+    double s2[2];
+    gather_sums<2>(a.part + (size_t)((gen - 1) & 1) * 3 * a.stride, a.stride, s2);
+    rpcg = s2[0];
+    bnorm2 = s2[1];
+  }
+  rsqm = a.rel_tol > 0.0 ? a.rel_tol * a.rel_tol * bnorm2 : a.rsqm;
+  int iter = 0, status = 0;
+  bool done = (rpcg == 0.0);  // :88
+
+  while (!done) {
+    // p = CCCI*r + beta*p (:134,158) on the own rows ...
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int i = b0 + k * PB + (int)threadIdx.x;
+      p[k] = cci[k] * r[k] + beta * p[k];
+      if (i < b1) lds[slot(i)] = p[k];
+    }
+    // ... and on the outside cells, from their published r and old p
+    {
+      const double *pold = a.hp[(iter + 1) & 1];
+      for (int t = threadIdx.x; t < nhalo; t += PB) {
+        const int j = hl[t];
+        const double po = iter > 0 ? fetch(&pold[j]) : 0.0;
+        lds[slot(j)] = a.ccci[j] * fetch(&a.hr[j]) + beta * po;
+      }
+    }
+    __syncthreads();
+    double v1[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int i = b0 + k * PB + (int)threadIdx.x;
+      if (i < b1) {
+        double pq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int j = nb[k][q];
+          pq[q] = real(j) ? lds[slot(j)] : 0.0;
+        }
+        // APCG = CCC*PCG + CCS*PS + CCN*PN + CCW*PW + CCE*PE (:113)
+        ap[k] = ccc[k] * p[k] + cs[k] * pq[NS] + cn[k] * pq[NN] + cw[k] * pq[NW] +
+                ce[k] * pq[NE];
+        v1[0] += ap[k] * p[k];
+        if (expo & (1u << k)) publish(&a.hp[iter & 1][opaque(i)], p[k]);
+      }
+    }
+    publish_partials<1>(v1, a.part + (size_t)(gen & 1) * 3 * a.stride, a.stride);
+    if (!grid_barrier(a.bar, ++gen)) return;
+    {
+      double s1[1];
+      gather_sums<1>(a.part + (size_t)((gen - 1) & 1) * 3 * a.stride, a.stride, s1);
+      papcg = s1[0];
+      alpha = rpcg / papcg;  // :124
+    }
+    double v2[2] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int i = b0 + k * PB + (int)threadIdx.x;
+      if (i < b1) {
+        x[k] = x[k] + alpha * p[k];        // :132
+        const double rr = r[k] - alpha * ap[k];  // :133
+        r[k] = rr;
+        const double z = cci[k] * rr;      // :134
+        v2[0] += rr * z;
+        v2[1] += rr * rr;
+        if (expo & (1u << k)) publish(&a.hr[opaque(i)], rr);
+      }
+    }
+    publish_partials<2>(v2, a.part + (size_t)(gen & 1) * 3 * a.stride, a.stride);
+    if (!grid_barrier(a.bar, ++gen)) return;
+    double s2[2];
+    gather_sums<2>(a.part + (size_t)((gen - 1) & 1) * 3 * a.stride, a.stride, s2);
+    beta = s2[0] / rpcg;  // :147-148
+    rpcg = s2[0];
+    rsq = s2[1];
+    iter += 1;
+    if (rsq <= rsqm) done = true;  // :165
+    else if (iter >= a.iterm) { done = true; status = 1; }
+  }
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int i = b0 + k * PB + (int)threadIdx.x;
+    if (i < b1) a.x[opaque(i)] = x[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CgState *st = a.st;
+    st->rpcg = rpcg;
+    st->papcg = papcg;
+    st->alpha = alpha;
+    st->beta = beta;
+    st->rsq = rsq;
+    st->rsqm = rsqm;
+    st->bnorm2 = bnorm2;
+    st->iter = iter;
+    st->done = 1;
+    st->status = status;
+  }
+}
+
+// ---------------------------------------------------------------------
+// Mixed: FP32 Chronopoulos-Gear CG on A' = D^-1/2 A D^-1/2, FP64
+// refinement (same operator and scaling as the multi-kernel mixed solve)
+// ---------------------------------------------------------------------
+struct PMxArgs {
+  const double *ccc, *ccs, *ccw, *fptmp;
+  double *x;                 // P, in/out
+  const int *ls, *ln;
+  int nwet, lc0;
+  float *hr[2], *hw[2], *hs[2];   // published r, w = A r, s = A p
+  double *part;              // [2 parities][3][stride]
+  int stride;
+  Bar bar;
+  CgState *st;
+  double rsqm, rel_tol, rtol2;
+  int iterm, max_outer;
+};
+
+template <int CPT>
+__global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
+  __shared__ float lds[2 * WIN + CPT * PB];
+  __shared__ int hl[2 * WIN];
+  const int b0 = 1 + (int)blockIdx.x * CPT * PB;
+  const int b1 = min(b0 + CPT * PB, a.nwet + 1);
+  const int lc0 = a.lc0;
+
+  float as_[CPT], an_[CPT], aw_[CPT], ae_[CPT];
+  float r[CPT], w[CPT], s[CPT], p[CPT], y[CPT];
+  int js[CPT], jn[CPT];
+  unsigned expo = 0u;
+
+  auto inside = [&](int j) { return j >= b0 && j < b1; };
+  auto real = [&](int j) { return j > 0 && j != lc0; };
+  auto slot = [&](int j) { return j - b0 + WIN; };
+  auto dinv = [&](int j) { return 1.0 / sqrt(a.ccc[j]); };
+  auto rval = [&](int j) { return real(j) ? lds[slot(j)] : 0.0f; };
+
+  for (int g = threadIdx.x; g < 2 * WIN; g += PB) hl[g] = 0;
+  __syncthreads();
+  double fp2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int i = b0 + k * PB + (int)threadIdx.x;
+    as_[k] = an_[k] = aw_[k] = ae_[k] = 0.0f;
+    r[k] = w[k] = s[k] = p[k] = y[k] = 0.0f;
+    js[k] = jn[k] = lc0;
+    if (i < b1) {
+      const int s_ = a.ls[i], n_ = a.ln[i];
+      js[k] = s_;
+      jn[k] = n_;
+      const double di = dinv(i);
+      // A'(i,S) = CCS(i) d(i) d(S), A'(i,W) = CCW(i) d(i) d(i-1); the north
+      // and east entries are the south / west entries of the neighbour,
+      // evaluated with the same expression as the neighbour's own
+      const double ds = (s_ != lc0) ? dinv(s_) : 0.0;
+      const double dw = (i - 1 > 0) ? dinv(i - 1) : 0.0;
+      const double cs = (s_ != lc0) ? a.ccs[i] : 0.0;
+      as_[k] = (float)(cs * di * ds);
+      aw_[k] = (float)(a.ccw[i] * di * dw);
+      if (n_ != lc0) {
+        const double dn = dinv(n_);
+        const int ss = a.ls[n_];
+        const double dss = (ss != lc0) ? dinv(ss) : 0.0;
+        const double csn = (ss != lc0) ? a.ccs[n_] : 0.0;
+        an_[k] = (float)(csn * dn * dss);
+      }
+      if (i + 1 != lc0) {
+        const double de = dinv(i + 1);
+        const double dwe = (i > 0) ? di : 0.0;
+        ae_[k] = (float)(a.ccw[i + 1] * de * dwe);
+      }
+      const int nbq[4] = {s_, n_, i - 1, i + 1};
+      bool out = false;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = nbq[q];
+        if (real(j) && !inside(j)) {
+          out = true;
+          hl[ghost_of(j, b0, b1)] = 1;
+        }
+      }
+      if (out) expo |= 1u << k;
+      fp2 += a.fptmp[i] * a.fptmp[i];
+    }
+  }
+  __syncthreads();
+  const int nhalo = compact_halo(hl, b0, b1);
+
+  unsigned gen = 0;
+  double rsqm = a.rsqm, bnorm2 = 0.0, rsq64 = 0.0, bn_in = 0.0;
+  int total = 0, outer = 0, status = 0, inner = 0;
+
+  for (;;) {
+    // ---- FP64 residual r = FPTMP - A P, scaled to r' = D^-1/2 r (FP32)
+    double v3[3] = {0.0, 0.0, fp2};
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int i = b0 + k * PB + (int)threadIdx.x;
+      if (i < b1) {
+        const int s_ = opaque(js[k]), n_ = opaque(jn[k]);
+        const int io = opaque(i);
+        const double cs = (s_ != lc0) ? a.ccs[io] : 0.0;
+        const double cn = (n_ != lc0) ? a.ccs[n_] : 0.0;
+        const double cw = a.ccw[io];
+        const double ce = (io + 1 != lc0) ? a.ccw[io + 1] : 0.0;
+        const double *P = a.x;
+        const double rr = a.fptmp[io] - a.ccc[io] * fetch(&P[io]) - cn * fetch(&P[n_]) -
+                          cs * fetch(&P[s_]) - cw * fetch(&P[io - 1]) -
+                          ce * fetch(&P[io + 1]);
+        const float rs = (float)(dinv(io) * rr);
+        r[k] = rs;
+        y[k] = 0.0f;
+        v3[0] += rr * rr;
+        v3[1] += (double)rs * (double)rs;
+        if (expo & (1u << k)) publish_f(&a.hr[0][io], rs);
+      }
+    }
+    fp2 = 0.0;
+    publish_partials<3>(v3, a.part + (size_t)(gen & 1) * 3 * a.stride, a.stride);
+    if (!grid_barrier(a.bar, ++gen)) return;
+    {
+      double s3[3];
+      gather_sums<3>(a.part + (size_t)((gen - 1) & 1) * 3 * a.stride, a.stride, s3);
+      if (outer == 0) {
+        bnorm2 = s3[2];
+        rsqm = a.rel_tol > 0.0 ? a.rel_tol * a.rel_tol * bnorm2 : a.rsqm;
+      }
+      rsq64 = s3[0];
+      bn_in = s3[1];
+    }
+    if (rsq64 <= rsqm) break;
+    if (outer >= a.max_outer || bn_in == 0.0) { status = 1; break; }
+
+    // ---- w0 = A' r0; gamma0 = (r,r), delta0 = (w,r); s_-1 = p_-1 = 0
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int i = b0 + k * PB + (int)threadIdx.x;
+      if (i < b1) lds[slot(i)] = r[k];
+      s[k] = 0.0f;
+      p[k] = 0.0f;
+    }
+    for (int t = threadIdx.x; t < nhalo; t += PB) {
+      const int j = hl[t];
+      lds[slot(j)] = fetch_f(&a.hr[0][j]);
+    }
+    __syncthreads();
+    double v2[2] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int i = b0 + k * PB + (int)threadIdx.x;
+      if (i < b1) {
+        w[k] = r[k] + as_[k] * rval(js[k]) + an_[k] * rval(jn[k]) +
+               aw_[k] * rval(i - 1) + ae_[k] * rval(i + 1);
+        v2[0] += (double)r[k] * (double)r[k];
+        v2[1] += (double)w[k] * (double)r[k];
+        if (expo & (1u << k)) {
+          const int io = opaque(i);
+          publish_f(&a.hw[0][io], w[k]);
+          publish_f(&a.hs[0][io], 0.0f);
+        }
+      }
+    }
+    publish_partials<2>(v2, a.part + (size_t)(gen & 1) * 3 * a.stride, a.stride);
+    if (!grid_barrier(a.bar, ++gen)) return;
+    double gam, del;
+    {
+      double s2[2];
+      gather_sums<2>(a.part + (size_t)((gen - 1) & 1) * 3 * a.stride, a.stride, s2);
+      gam = s2[0];
+      del = s2[1];
+    }
+    double gam_prev = 0.0, alpha_prev = 1.0;
+    inner = 0;
+    for (;;) {
+      const int par = inner & 1;
+      double beta, alpha;
+      if (inner == 0) {
+        beta = 0.0;
+        alpha = gam / del;
+      } else {
+        beta = gam / gam_prev;
+        alpha = gam / (del - beta * gam / alpha_prev);
+      }
+      // own rows: p = r + beta p, s = w + beta s, y += alpha p, r -= alpha s
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) {
+        const int i = b0 + k * PB + (int)threadIdx.x;
+        p[k] = (float)((double)r[k] + beta * (double)p[k]);
+        s[k] = (float)((double)w[k] + beta * (double)s[k]);
+        y[k] = (float)((double)y[k] + alpha * (double)p[k]);
+        r[k] = (float)((double)r[k] - alpha * (double)s[k]);
+        if (i < b1) lds[slot(i)] = r[k];
+      }
+      // the outside cells' update, from their published r, w, s
+      for (int t = threadIdx.x; t < nhalo; t += PB) {
+        const int j = hl[t];
+        const float sj = (float)((double)fetch_f(&a.hw[par][j]) +
+                                 beta * (double)fetch_f(&a.hs[par][j]));
+        lds[slot(j)] = (float)((double)fetch_f(&a.hr[par][j]) - alpha * (double)sj);
+      }
+      __syncthreads();
+      double v[2] = {0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) {
+        const int i = b0 + k * PB + (int)threadIdx.x;
+        if (i < b1) {
+          w[k] = r[k] + as_[k] * rval(js[k]) + an_[k] * rval(jn[k]) +
+                 aw_[k] * rval(i - 1) + ae_[k] * rval(i + 1);
+          v[0] += (double)r[k] * (double)r[k];
+          v[1] += (double)w[k] * (double)r[k];
+          if (expo & (1u << k)) {
+            const int io = opaque(i);
+            publish_f(&a.hr[par ^ 1][io], r[k]);
+            publish_f(&a.hw[par ^ 1][io], w[k]);
+            publish_f(&a.hs[par ^ 1][io], s[k]);
+          }
+        }
+      }
+      publish_partials<2>(v, a.part + (size_t)(gen & 1) * 3 * a.stride, a.stride);
+      if (!grid_barrier(a.bar, ++gen)) return;
+      double s2[2];
+      gather_sums<2>(a.part + (size_t)((gen - 1) & 1) * 3 * a.stride, a.stride, s2);
+      inner += 1;
+      gam_prev = gam;
+      alpha_prev = alpha;
+      gam = s2[0];
+      del = s2[1];
+      if (gam <= a.rtol2 * bn_in || inner >= a.iterm) break;
+    }
+    total += inner;
+    // ---- P += D^-1/2 y (FP64), published for the next residual
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int i = b0 + k * PB + (int)threadIdx.x;
+      if (i < b1) {
+        const int io = opaque(i);
+        double *P = a.x;
+        publish(&P[io], fetch(&P[io]) + dinv(io) * (double)y[k]);
+      }
+    }
+    ++outer;
+    if (!grid_barrier(a.bar, ++gen)) return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CgState *st = a.st;
+    st->rsq64 = rsq64;
+    st->rsq = rsq64;
+    st->rsqm = rsqm;
+    st->bnorm2 = bnorm2;
+    st->bnorm2_in = bn_in;
+    st->iter = total;
+    st->inner_iter = inner;
+    st->outer = outer;
+    st->done = 1;
+    st->status = status;
+  }
+}
+
+__global__ void k_pcg_reset(unsigned *ctr, int *abort) {
+  const int t = threadIdx.x;
+  if (t < 9 * TICKET_PAD) ctr[t] = 0u;
+  if (t == 0) *abort = 0;
+}
+
+template <typename K>
+int occupancy(K kern) {
+  int nbpc = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbpc, kern, PB, 0) != hipSuccess)
+    return 0;
+  return nbpc;
+}
+
+}  // namespace
+
+// Largest cells-per-thread instantiation of each solve.
+constexpr int PCG64_CPT_MAX = 6;
+constexpr int PCGMX_CPT_MAX = 8;
+
+int pcg_prepare(efdc_ctx *c) {
+  if (c->pcg_ready) return EFDC_OK;
+  int dev = c->device, ncu = 0;
+  HIP_TRY(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  int coop = 0;
+  HIP_TRY(c, hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+  c->pcg_ncu = coop ? ncu : 0;
+  const int occ64[PCG64_CPT_MAX + 1] = {
+      0, occupancy(k_pcg64<1>), occupancy(k_pcg64<2>), occupancy(k_pcg64<3>),
+      occupancy(k_pcg64<4>), occupancy(k_pcg64<5>), occupancy(k_pcg64<6>)};
+  const int occmx[PCGMX_CPT_MAX + 1] = {
+      0, occupancy(k_pcgmx<1>), occupancy(k_pcgmx<2>), occupancy(k_pcgmx<3>),
+      occupancy(k_pcgmx<4>), occupancy(k_pcgmx<5>), occupancy(k_pcgmx<6>),
+      occupancy(k_pcgmx<7>), occupancy(k_pcgmx<8>)};
+  for (int k = 0; k <= PCG64_CPT_MAX; ++k) c->pcg_occ64[k] = occ64[k];
+  for (int k = 0; k <= PCGMX_CPT_MAX; ++k) c->pcg_occmx[k] = occmx[k];
+  const int n = c->n_wet() + 2;
+  int s;
+  if ((s = dev_alloc(c, (void **)&c->pcg_hr64, sizeof(double) * n))) return s;
+  for (int q = 0; q < 2; ++q) {
+    if ((s = dev_alloc(c, (void **)&c->pcg_hp64[q], sizeof(double) * n))) return s;
+    if ((s = dev_alloc(c, (void **)&c->pcg_h32[q][0], sizeof(float) * n))) return s;
+    if ((s = dev_alloc(c, (void **)&c->pcg_h32[q][1], sizeof(float) * n))) return s;
+    if ((s = dev_alloc(c, (void **)&c->pcg_h32[q][2], sizeof(float) * n))) return s;
+  }
+  if ((s = dev_alloc(c, (void **)&c->pcg_part, sizeof(double) * 2 * 3 * PB))) return s;
+  if ((s = dev_alloc(c, (void **)&c->pcg_ctr, sizeof(unsigned) * 9 * TICKET_PAD))) return s;
+  if ((s = dev_alloc(c, (void **)&c->pcg_abort, sizeof(int)))) return s;
+  c->pcg_ready = true;
+  return EFDC_OK;
+}
+
+// Cells per thread and workgroup count of a persistent solve, or 0 when
+// the problem does not fit in one co-resident grid.
+static int pcg_shape(const efdc_ctx *c, const int *occ, int cpt_max, int &nb) {
+  const int n = c->n_wet();
+  for (int cpt = 1; cpt <= cpt_max; ++cpt) {
+    const int need = (n + cpt * PB - 1) / (cpt * PB);
+    const int cap = occ[cpt] * c->pcg_ncu;
+    if (occ[cpt] > 0 && need <= cap && need <= PB) {
+      nb = need;
+      return cpt;
+    }
+  }
+  return 0;
+}
+
+bool pcg_applicable(efdc_ctx *c, bool mixed) {
+  if (c->prm.cg_kernel == EFDC_CG_KERNEL_MULTI) return false;
+  if (c->band > WIN) return false;  // a neighbour beyond the LDS window
+  if (pcg_prepare(c) != EFDC_OK || c->pcg_ncu == 0) return false;
+  int nb = 0;
+  return mixed ? pcg_shape(c, c->pcg_occmx, PCGMX_CPT_MAX, nb) > 0
+               : pcg_shape(c, c->pcg_occ64, PCG64_CPT_MAX, nb) > 0;
+}
+
+template <typename Args>
+static int coop_launch(efdc_ctx *c, const void *fn, int nb, Args &args) {
+  void *kargs[] = {&args};
+  HIP_TRY(c, hipLaunchCooperativeKernel(fn, dim3(nb), dim3(PB), kargs, 0, c->stream));
+  return EFDC_OK;
+}
+
+int pcg_solve(efdc_ctx *c, bool mixed) {
+  int s;
+  if ((s = pcg_prepare(c))) return s;
+  k_pcg_reset<<<1, 9 * TICKET_PAD, 0, c->stream>>>(c->pcg_ctr, c->pcg_abort);
+  HIP_TRY(c, hipGetLastError());
+  int nb = 0;
+  Bar bar{c->pcg_ctr, c->pcg_abort};
+  if (!mixed) {
+    const int cpt = pcg_shape(c, c->pcg_occ64, PCG64_CPT_MAX, nb);
+    if (!cpt) { c->err = "persistent CG: problem too large"; return EFDC_ERR_ARG; }
+    P64Args a;
+    a.ccc = c->dH[EFDC_CCC];
+    a.ccs = c->dH[EFDC_CCS];
+    a.ccw = c->dH[EFDC_CCW];
+    a.ccn = c->dH[EFDC_CCN];
+    a.cce = c->dH[EFDC_CCE];
+    a.ccci = c->dH[EFDC_CCCI];
+    a.fptmp = c->dH[EFDC_FPTMP];
+    a.x = c->dH[EFDC_P];
+    a.lw = c->dLW;
+    a.le = c->dLE;
+    a.ls = c->dLS;
+    a.ln = c->dLN;
+    a.nwet = c->n_wet();
+    a.lc0 = c->LC - 1;
+    a.rowmajor = c->rowmajor ? 1 : 0;
+    a.mutual = c->mutual ? 1 : 0;
+    a.hr = c->pcg_hr64;
+    a.hp[0] = c->pcg_hp64[0];
+    a.hp[1] = c->pcg_hp64[1];
+    a.part = c->pcg_part;
+    a.stride = PB;
+    a.bar = bar;
+    a.st = c->st;
+    a.rsqm = c->prm.RSQM;
+    a.rel_tol = c->prm.cg_rel_tol;
+    a.iterm = c->prm.ITERM;
+    const void *fn = nullptr;
+    switch (cpt) {
+      case 1: fn = (const void *)k_pcg64<1>; break;
+      case 2: fn = (const void *)k_pcg64<2>; break;
+      case 3: fn = (const void *)k_pcg64<3>; break;
+      case 4: fn = (const void *)k_pcg64<4>; break;
+      case 5: fn = (const void *)k_pcg64<5>; break;
+      default: fn = (const void *)k_pcg64<6>; break;
+    }
+    TIMED(c, TK_PCG64, s = coop_launch(c, fn, nb, a));
+    if (s) return s;
+  } else {
+    const int cpt = pcg_shape(c, c->pcg_occmx, PCGMX_CPT_MAX, nb);
+    if (!cpt) { c->err = "persistent CG: problem too large"; return EFDC_ERR_ARG; }
+    PMxArgs a;
+    a.ccc = c->dH[EFDC_CCC];
+    a.ccs = c->dH[EFDC_CCS];
+    a.ccw = c->dH[EFDC_CCW];
+    a.fptmp = c->dH[EFDC_FPTMP];
+    a.x = c->dH[EFDC_P];
+    a.ls = c->dLS;
+    a.ln = c->dLN;
+    a.nwet = c->n_wet();
+    a.lc0 = c->LC - 1;
+    for (int q = 0; q < 2; ++q) {
+      a.hr[q] = c->pcg_h32[q][0];
+      a.hw[q] = c->pcg_h32[q][1];
+      a.hs[q] = c->pcg_h32[q][2];
+    }
+    a.part = c->pcg_part;
+    a.stride = PB;
+    a.bar = bar;
+    a.st = c->st;
+    a.rsqm = c->prm.RSQM;
+    a.rel_tol = c->prm.cg_rel_tol;
+    const double rtol = c->prm.mixed_inner_rtol > 0 ? c->prm.mixed_inner_rtol : 1e-5;
+    a.rtol2 = rtol * rtol;
+    a.iterm = c->prm.ITERM;
+    a.max_outer = c->prm.mixed_max_outer;
+    const void *fn = nullptr;
+    switch (cpt) {
+      case 1: fn = (const void *)k_pcgmx<1>; break;
+      case 2: fn = (const void *)k_pcgmx<2>; break;
+      case 3: fn = (const void *)k_pcgmx<3>; break;
+      case 4: fn = (const void *)k_pcgmx<4>; break;
+      case 5: fn = (const void *)k_pcgmx<5>; break;
+      case 6: fn = (const void *)k_pcgmx<6>; break;
+      case 7: fn = (const void *)k_pcgmx<7>; break;
+      default: fn = (const void *)k_pcgmx<8>; break;
+    }
+    TIMED(c, TK_PCGMX, s = coop_launch(c, fn, nb, a));
+    if (s) return s;
+  }
+  int abort_h = 0;
+  HIP_TRY(c, hipMemcpyAsync(&abort_h, c->pcg_abort, sizeof(int), hipMemcpyDeviceToHost,
+                            c->stream));
+  HIP_TRY(c, hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (abort_h) {
+    c->err = "persistent CG: a grid barrier timed out";
+    return EFDC_ERR_HIP;
+  }
+  return EFDC_OK;
+}

@@ -97,7 +97,8 @@ const char *const efdc_timer_names[TK_COUNT] = {
     "mx_spmv_pupdate32", "mx_axpy_dots32", "mx_apply", "puv_transports",
     "puv_depths", "puv_hu_hv", "tran_upwind_vdiff", "tran_bc_con1",
     "closure_pre", "closure_post", "balance", "adac_gradients",
-    "adac_fluxes", "fct_limiters", "tran_finish_vdiff"};
+    "adac_fluxes", "fct_limiters", "tran_finish_vdiff", "cg_persistent64",
+    "mx_persistent"};
 
 hipEvent_t timer_event(efdc_ctx *c) {
   if (c->ev_pool.empty()) {
@@ -129,6 +130,7 @@ void efdc_params_default(efdc_params_t *p) {
   p->store = EFDC_STORE_FP64;
   p->mixed_inner_rtol = 1e-5;
   p->mixed_max_outer = 30;
+  p->cg_kernel = EFDC_CG_KERNEL_AUTO;
 }
 
 // Reason of the last failed efdc_ctx_create on this thread (there is no
@@ -156,6 +158,8 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
     return reject(EFDC_ERR_ARG, "unknown storage precision");
   if (prm->cg_mode != EFDC_CG_FP64 && prm->cg_mode != EFDC_CG_MIXED)
     return reject(EFDC_ERR_ARG, "unknown CG mode");
+  if (prm->cg_kernel < EFDC_CG_KERNEL_AUTO || prm->cg_kernel > EFDC_CG_KERNEL_PERSISTENT)
+    return reject(EFDC_ERR_ARG, "unknown CG kernel choice");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return reject(EFDC_ERR_NODEVICE, "no HIP device visible");
@@ -182,6 +186,7 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
   const int LA = g->LA, LC = c->LC, LCM = g->LCM;
   std::vector<int> lw(LCM), le(LCM), ls(LCM), ln(LCM), lnw(LCM), lse(LCM);
   bool rowmajor = true, mutual = true;
+  int band = 1;
   for (int L = 1; L <= LCM; ++L) {
     const int i = L - 1;
     auto chk = [&](int v) { return v >= 1 && v <= LCM; };
@@ -192,8 +197,13 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
         return fail(EFDC_ERR_ARG);
       }
       if (!(a == L - 1 || a == LC) || !(b == L + 1 || b == LC)) rowmajor = false;
+      // mutual: a wet neighbour in one direction sees L in the opposite one
       if (a != LC && a >= 2 && a <= LA && g->LEC[a - 1] != L) mutual = false;
       if (s != LC && s >= 2 && s <= LA && g->LNC[s - 1] != L) mutual = false;
+      if (b != LC && b >= 2 && b <= LA && g->LWC[b - 1] != L) mutual = false;
+      if (n != LC && n >= 2 && n <= LA && g->LSC[n - 1] != L) mutual = false;
+      for (int v : {a, b, s, n})
+        if (v >= 2 && v <= LA) band = std::max(band, std::abs(v - L));
     }
     lw[i] = chk(a) ? a - 1 : LC - 1;
     le[i] = chk(b) ? b - 1 : LC - 1;
@@ -204,6 +214,7 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
   }
   c->rowmajor = rowmajor && mutual;
   c->mutual = mutual;
+  c->band = band;
   int s;
   if ((s = upload_new(c, &c->dLW, lw.data(), LCM)) ||
       (s = upload_new(c, &c->dLE, le.data(), LCM)) ||
@@ -278,6 +289,10 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   dfree(c->dis); dfree(c->as32); dfree(c->aw32); dfree(c->r32); dfree(c->y32);
   dfree(c->p32[0]); dfree(c->p32[1]); dfree(c->ap32);
   dfree(c->part); dfree(c->st); dfree(c->tickets);
+  dfree(c->pcg_hr64); dfree(c->pcg_hp64[0]); dfree(c->pcg_hp64[1]);
+  for (auto &q : c->pcg_h32)
+    for (auto &a : q) dfree(a);
+  dfree(c->pcg_part); dfree(c->pcg_ctr); dfree(c->pcg_abort);
   if (c->st_host) (void)hipHostFree(c->st_host);
   if (c->tide_host) (void)hipHostFree(c->tide_host);
   for (auto &b : c->pbc) {
@@ -316,6 +331,10 @@ int efdc_set_params(efdc_ctx_t *c, const efdc_params_t *p) {
   }
   if (p->cg_mode == EFDC_CG_MIXED && !c->dis) {
     c->err = "EFDC_CG_MIXED needs a context created in that mode";
+    return EFDC_ERR_ARG;
+  }
+  if (p->cg_kernel < EFDC_CG_KERNEL_AUTO || p->cg_kernel > EFDC_CG_KERNEL_PERSISTENT) {
+    c->err = "unknown CG kernel choice";
     return EFDC_ERR_ARG;
   }
   c->prm = *p;

@@ -87,6 +87,7 @@ struct efdc_ctx {
   int *dLNW = nullptr, *dLSE = nullptr;
   bool rowmajor = false;  // LWC in {L-1,LC}, LEC in {L+1,LC}, maps mutual
   bool mutual = false;    // LEC(LWC(L)) = L and LNC(LSC(L)) = L for wet L
+  int band = 0;           // max |neighbour index - L| over wet neighbours
 
   double *dH[EFDC_NUM_H] = {};
   void *dK[EFDC_NK] = {};
@@ -105,6 +106,15 @@ struct efdc_ctx {
   unsigned *tickets = nullptr; // TICKET_SITES x TICKET_SITE counters
   CgState *st_host = nullptr;  // pinned mirror
   int last_iters = 0;
+  // persistent solve (efdc_cgp.hip), allocated on first use
+  bool pcg_ready = false;
+  int pcg_ncu = 0;             // CUs, 0 when cooperative launch is unsupported
+  int pcg_occ64[7] = {}, pcg_occmx[9] = {};  // workgroups per CU by CPT
+  double *pcg_hr64 = nullptr, *pcg_hp64[2] = {nullptr, nullptr};
+  float *pcg_h32[2][3] = {};   // [parity][r, w, s]
+  double *pcg_part = nullptr;  // [2][3][1024] partials
+  unsigned *pcg_ctr = nullptr; // barrier counters
+  int *pcg_abort = nullptr;
 
   // open boundaries
   PbcDev pbc[4] = {};
@@ -166,7 +176,7 @@ enum {
   TK_CG_INIT, TK_CG_A, TK_CG_B, TK_MX_PREP, TK_MX_RESID, TK_MX_A, TK_MX_B,
   TK_MX_APPLY, TK_PUV_POST1, TK_PUV_POST2, TK_PUV_POST3, TK_TRAN,
   TK_TRAN_BC, TK_CLOSURE_PRE, TK_CLOSURE_POST, TK_BALANCE, TK_ADAC_GRAD,
-  TK_ADAC_FLUX, TK_FCT, TK_COL_FINISH, TK_COUNT
+  TK_ADAC_FLUX, TK_FCT, TK_COL_FINISH, TK_PCG64, TK_PCGMX, TK_COUNT
 };
 extern const char *const efdc_timer_names[TK_COUNT];
 
@@ -365,7 +375,27 @@ __device__ __forceinline__ bool grid_min(double v, double *part,
 }
 
 // ------------------------------------------------------------ host glue
+// hipMalloc + zero fill on the context's stream
+inline int dev_alloc(efdc_ctx *c, void **p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) return EFDC_OK;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    c->err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
+    *p = nullptr;
+    return EFDC_ERR_NOMEM;
+  }
+  e = hipMemsetAsync(*p, 0, bytes, c->stream);
+  if (e != hipSuccess) {
+    c->err = hipGetErrorString(e);
+    return EFDC_ERR_HIP;
+  }
+  return EFDC_OK;
+}
 int efdc_grid_blocks(int n);
+// persistent CONGRAD (efdc_cgp.hip)
+bool pcg_applicable(efdc_ctx *c, bool mixed);
+int pcg_solve(efdc_ctx *c, bool mixed);
 int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats);
 int cg_run(efdc_ctx *c, bool check_sym, efdc_cg_stats_t *stats);
 int tran_run(efdc_ctx *c, const efdc_step_t *step);

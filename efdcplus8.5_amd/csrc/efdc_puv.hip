@@ -811,7 +811,14 @@ int cg_dispatch(efdc_ctx *c, efdc_cg_stats_t *stats) {
     if ((s = read_state(c))) return s;
     if (!c->rowmajor || c->st_host->symok != 0) mixed = false;
   }
-  s = mixed ? cg_mixed(c) : cg_fp64(c);
+  if (c->prm.cg_kernel == EFDC_CG_KERNEL_PERSISTENT && !pcg_applicable(c, mixed)) {
+    c->err = "persistent CG requested but the rows do not fit one co-resident grid";
+    return EFDC_ERR_ARG;
+  }
+  if (pcg_applicable(c, mixed))
+    s = pcg_solve(c, mixed);
+  else
+    s = mixed ? cg_mixed(c) : cg_fp64(c);
   if (s) return s;
   const CgState &h = *c->st_host;
   if (stats) {

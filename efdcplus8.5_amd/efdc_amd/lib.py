@@ -23,6 +23,7 @@ PD = C.POINTER(C.c_double)
 PI = C.POINTER(C.c_int32)
 
 CG_FP64, CG_MIXED = 0, 1
+CG_KERNEL_AUTO, CG_KERNEL_MULTI, CG_KERNEL_PERSISTENT = 0, 1, 2
 STORE_FP64, STORE_FP32 = 0, 1
 
 
@@ -44,7 +45,7 @@ class Params(C.Structure):
                 ("ITERM", C.c_int32), ("RSQM", C.c_double),
                 ("cg_rel_tol", C.c_double), ("cg_mode", C.c_int32),
                 ("store", C.c_int32), ("mixed_inner_rtol", C.c_double),
-                ("mixed_max_outer", C.c_int32)]
+                ("mixed_max_outer", C.c_int32), ("cg_kernel", C.c_int32)]
 
 
 class PbcSide(C.Structure):
@@ -179,8 +180,10 @@ class Context:
             setattr(g, m, _pi(a))
         self.params = params if params is not None else default_params(**kw)
         h = C.c_void_p()
-        self._check(self.lib.efdc_ctx_create(C.byref(g), C.byref(self.params),
-                                             C.byref(h)), ctx=None)
+        st = self.lib.efdc_ctx_create(C.byref(g), C.byref(self.params), C.byref(h))
+        if st != 0:
+            why = (self.lib.efdc_last_error(None) or b"").decode()
+            raise EfdcError(st, why)
         self.h = h
         self.nconst = 0
         self.has_manning = False

@@ -15,6 +15,9 @@ module efdc_hip
   ! efdc_cg_mode, efdc_store
   integer(c_int32_t), parameter :: EFDC_CG_FP64 = 0, EFDC_CG_MIXED = 1
   integer(c_int32_t), parameter :: EFDC_STORE_FP64 = 0, EFDC_STORE_FP32 = 1
+  ! efdc_cg_kernel
+  integer(c_int32_t), parameter :: EFDC_CG_KERNEL_AUTO = 0, EFDC_CG_KERNEL_MULTI = 1, &
+      EFDC_CG_KERNEL_PERSISTENT = 2
 
   ! efdc_array_id: (LCM) arrays
   integer(c_int32_t), parameter :: EFDC_DXYP = 0, EFDC_DXYIP = 1, EFDC_DXYU = 2, &
@@ -50,7 +53,7 @@ module efdc_hip
     real(c_double) :: RSQM, cg_rel_tol
     integer(c_int32_t) :: cg_mode, store
     real(c_double) :: mixed_inner_rtol
-    integer(c_int32_t) :: mixed_max_outer
+    integer(c_int32_t) :: mixed_max_outer, cg_kernel
   end type
 
   type, bind(C) :: efdc_pbc_side_t

@@ -59,6 +59,14 @@ typedef enum {
 } efdc_cg_mode;
 
 typedef enum {
+  EFDC_CG_KERNEL_AUTO = 0,       /* persistent solve when the rows fit one
+                                    co-resident grid, otherwise multi-kernel */
+  EFDC_CG_KERNEL_MULTI = 1,      /* two launches per iteration               */
+  EFDC_CG_KERNEL_PERSISTENT = 2  /* one cooperative launch per solve; an
+                                    error when the rows do not fit           */
+} efdc_cg_kernel;
+
+typedef enum {
   EFDC_STORE_FP64 = 0, /* constituents stored as FP64 on the device          */
   EFDC_STORE_FP32 = 1  /* constituents stored as FP32, tridiagonal in FP64   */
 } efdc_store;
@@ -89,12 +97,13 @@ typedef struct {
   int32_t store;     /* efdc_store, transport storage precision             */
   double mixed_inner_rtol; /* EFDC_CG_MIXED: inner FP32 relative reduction  */
   int32_t mixed_max_outer; /* EFDC_CG_MIXED: refinement step cap            */
+  int32_t cg_kernel;       /* efdc_cg_kernel: how the solve is launched     */
 } efdc_params_t;
 
 /* Fill *p with the values EFDC uses by default: G=9.81, RITB1=1, SNLT=0,
    HDRY=0.2, TCON_TBEGIN=0, ITERM=200, RSQM=1e-9 (the sample decks,
    SampleModels/Lake_Washington/efdc.inp:92), cg_rel_tol=0, FP64 CG, FP64
-   storage, mixed_inner_rtol=1e-5, mixed_max_outer=30. */
+   storage, mixed_inner_rtol=1e-5, mixed_max_outer=30, cg_kernel=AUTO. */
 void efdc_params_default(efdc_params_t *p);
 
 /* ------------------------------------------------------------- array ids */

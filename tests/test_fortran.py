@@ -31,7 +31,7 @@ STRUCTS = {
                     "LNWC", "LSEC"],
     "efdc_params_t": ["G", "RITB1", "SNLT", "HDRY", "TCON_TBEGIN", "ITERM",
                       "RSQM", "cg_rel_tol", "cg_mode", "store",
-                      "mixed_inner_rtol", "mixed_max_outer"],
+                      "mixed_inner_rtol", "mixed_max_outer", "cg_kernel"],
     "efdc_pbc_side_t": ["n", "L", "ispb", "pser", "pcb", "psb"],
     "efdc_open_bc_t": ["side", "mtide", "tcp"],
     "efdc_cbc_side_t": ["n", "L", "cb", "csert", "ntscr"],
@@ -62,7 +62,7 @@ def test_fortran_parameters_match_header():
     assert {k[5:]: v for k, v in arrays.items()} == abi.ARRAY_IDS
     for k, v in abi.STATUS.items():
         assert f[k] == v, k
-    for name in ("efdc_cg_mode", "efdc_store"):
+    for name in ("efdc_cg_mode", "efdc_store", "efdc_cg_kernel"):
         for k, v in abi._parse_enum(name).items():
             assert f[k] == v, k
 

@@ -1,0 +1,166 @@
+"""The two launch strategies of CONGRAD against the oracle.
+
+* multi-kernel: two launches per iteration (EFDC_CG_KERNEL_MULTI);
+* persistent: one cooperative launch per solve with the rows held in
+  registers (EFDC_CG_KERNEL_PERSISTENT).
+
+Both run the FP64 Jacobi-PCG of the reference (|d eta| <= 1e-9 m against
+the oracle at RSQM=1e-24, iteration count within one) and the FP32-refined
+solve (|d eta| <= 1e-6 m).  The persistent solve is also run on a
+column-major renumbering of the cells (neighbour maps that are not the
+row-major L+-1 pattern, so the general four-map operator is used), on a
+grid so small that one workgroup holds every row, and on a grid whose
+north/south neighbours lie beyond the workgroup's LDS window (the
+automatic choice then falls back to the multi-kernel solve, the forced
+persistent choice is refused).
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from conftest import eta_err, wet_index
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = 1e-24
+SYSTEM = ["CCC", "CCS", "CCW", "CCE", "CCN", "CCCI", "FPTMP", "P"]
+
+
+def _lake_system(oracle_mod, n, seed):
+    from efdc_amd import grids
+    wet = grids.lake_mask(n, n, seed)
+    case = grids.external_case(wet, seed=seed)
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref, rsqm=TIGHT, iterm=4000)
+    o.assemble()
+    sys0 = {k: ref.h[k].copy() for k in SYSTEM}
+    o.congrad()
+    return case, sys0, ref.h["P"].copy()
+
+
+def _solve(grid, system, **kw):
+    from efdc_amd.lib import Context
+    kw.setdefault("RSQM", TIGHT)
+    kw.setdefault("ITERM", 4000)
+    ctx = Context(grid, **kw)
+    for k, v in system.items():
+        ctx.set(k, v)
+    out = ctx.congrad()
+    p = ctx.get("P")
+    ctx.close()
+    return out, p
+
+
+@pytest.mark.parametrize("kernel", ["multi", "persistent"])
+@pytest.mark.parametrize("n,seed", [(24, 3), (64, 5), (200, 7)])
+def test_fp64_strategies_match_oracle(oracle_mod, kernel, n, seed):
+    from efdc_amd import lib
+    ck = {"multi": lib.CG_KERNEL_MULTI, "persistent": lib.CG_KERNEL_PERSISTENT}[kernel]
+    case, system, p_ref = _lake_system(oracle_mod, n, seed)
+    out, p = _solve(case.grid, system, cg_kernel=ck)
+    i = wet_index(case)
+    d = eta_err(p[i], p_ref[i])
+    print(f"{kernel} n={n} wet={case.grid.nwet} iters {out['iters']} deta {d:.3e}")
+    assert d <= 1e-9
+
+
+@pytest.mark.parametrize("kernel", ["multi", "persistent"])
+def test_mixed_strategies_within_1e6_m(oracle_mod, kernel):
+    from efdc_amd import lib
+    ck = {"multi": lib.CG_KERNEL_MULTI, "persistent": lib.CG_KERNEL_PERSISTENT}[kernel]
+    case, system, p_ref = _lake_system(oracle_mod, 160, 9)
+    out, p = _solve(case.grid, system, cg_kernel=ck, cg_mode=lib.CG_MIXED,
+                    RSQM=1e-20)
+    i = wet_index(case)
+    d = eta_err(p[i], p_ref[i])
+    print(f"mixed {kernel}: outer {out['outer']} inner {out['iters']} deta {d:.3e}")
+    assert out["outer"] >= 1
+    assert d <= 1e-6
+
+
+def test_persistent_equals_multi_iterations(oracle_mod):
+    """Same FP64 recurrences: the two launch strategies take the same
+    number of iterations (up to one, from the order of the dot sums)."""
+    from efdc_amd import lib
+    case, system, _ = _lake_system(oracle_mod, 128, 12)
+    om, pm = _solve(case.grid, system, cg_kernel=lib.CG_KERNEL_MULTI)
+    op, pp = _solve(case.grid, system, cg_kernel=lib.CG_KERNEL_PERSISTENT)
+    i = wet_index(case)
+    assert abs(om["iters"] - op["iters"]) <= 1
+    assert eta_err(pm[i], pp[i]) <= 1e-9
+
+
+def _column_major(case, system):
+    """The same cells numbered column by column (I outer, J inner)."""
+    from efdc_amd.grids import Grid
+    g = case.grid
+    old = np.arange(2, g.LA + 1)
+    key = np.lexsort((g.JL[old - 1], g.IL[old - 1]))   # I major, J minor
+    new_of_old = np.zeros(g.LCM + 1, dtype=np.int64)
+    new_of_old[old[key]] = np.arange(2, g.LA + 1)
+    LC = g.LC
+    new_of_old[1] = 1
+    new_of_old[LC] = LC
+    maps = {}
+    for name, m in g.maps.items():
+        nm = np.zeros_like(m)
+        nm[0] = m[0]
+        nm[g.LA] = m[g.LA]
+        nm[LC] = m[LC]
+        src = old[key]                       # old L in new order
+        nm[np.arange(2, g.LA + 1) - 1] = new_of_old[m[src - 1]]
+        maps[name] = nm.astype(np.int32)
+    g2 = dataclasses.replace(g, maps=maps)
+    sys2 = {}
+    for k, v in system.items():
+        w = v.copy()
+        w[np.arange(2, g.LA + 1) - 1] = v[old[key] - 1]
+        sys2[k] = w
+    return g2, sys2, old[key]
+
+
+def test_persistent_general_maps(oracle_mod):
+    from efdc_amd import lib
+    case, system, p_ref = _lake_system(oracle_mod, 96, 4)
+    g2, sys2, src = _column_major(case, system)
+    assert not np.all((g2.maps["LWC"][1:g2.LA] == np.arange(1, g2.LA)) |
+                      (g2.maps["LWC"][1:g2.LA] == g2.LC))
+    out, p = _solve(g2, sys2, cg_kernel=lib.CG_KERNEL_PERSISTENT)
+    newi = np.arange(2, g2.LA + 1) - 1
+    d = eta_err(p[newi], p_ref[src - 1])
+    print("column-major persistent: iters", out["iters"], "deta", d)
+    assert d <= 1e-9
+
+
+def test_persistent_single_workgroup(oracle_mod):
+    from efdc_amd import lib
+    case, system, p_ref = _lake_system(oracle_mod, 12, 2)
+    assert case.grid.nwet < 512
+    for mode in (lib.CG_FP64, lib.CG_MIXED):
+        out, p = _solve(case.grid, system, cg_kernel=lib.CG_KERNEL_PERSISTENT,
+                        cg_mode=mode, RSQM=TIGHT if mode == lib.CG_FP64 else 1e-22)
+        i = wet_index(case)
+        d = eta_err(p[i], p_ref[i])
+        print("single workgroup mode", mode, "iters", out["iters"], "deta", d)
+        assert d <= (1e-9 if mode == lib.CG_FP64 else 1e-6)
+
+
+def test_wide_rows_fall_back_or_refuse(oracle_mod):
+    """Two wet rows of 4200 cells: a north neighbour lies 4200 cells away,
+    beyond the persistent solve's window."""
+    from efdc_amd import grids, lib
+    wet = np.zeros((4, 4202), dtype=bool)
+    wet[1:3, 1:4201] = True
+    case = grids.external_case(wet, seed=1)
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref, rsqm=TIGHT, iterm=4000)
+    o.assemble()
+    system = {k: ref.h[k].copy() for k in SYSTEM}
+    o.congrad()
+    out, p = _solve(case.grid, system, cg_kernel=lib.CG_KERNEL_AUTO)
+    i = wet_index(case)
+    assert eta_err(p[i], ref.h["P"][i]) <= 1e-9
+    with pytest.raises(lib.EfdcError) as e:
+        _solve(case.grid, system, cg_kernel=lib.CG_KERNEL_PERSISTENT)
+    assert e.value.status == 1
