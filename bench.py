@@ -120,6 +120,96 @@ def roofline(ctx, case, n0, k, kernels, nwet):
                 cg_ms=cg_ms, kernels=per)
 
 
+def transport_bench(args):
+    """CALCONC at scale (--workload transport): the 2048x1024 all-wet grid,
+    KC=20 sigma layers, 4 constituents (salinity, temperature, dye with a
+    point source, sediment), FP32 storage with the FP64 tridiagonal solve,
+    dt = 5 s.  One step = one CALCONC call (sources, upwind advection,
+    2TL update, open boundaries, implicit vertical diffusion, dye decay) for
+    all constituents; value = column-steps/s.  The FP64 oracle runs the same
+    steps (bounded sample) for the error and the CPU baseline."""
+    import numpy as np
+    from efdc_amd import grids, lib
+
+    nx, ny, kc = args.tnx, args.tny, args.tkc
+    case = grids.transport_case(nx, ny, kc)
+    g = case.grid
+    nc = len(case.consts)
+    ref_case = case.copy() if not args.no_cpu else None
+    ctx = lib.Context(g, store=lib.STORE_FP32)
+    ctx.upload_case(case)
+    cpu = None
+    ncmp = 1
+    if ref_case is not None:
+        from oracle import oracle as orc
+        o = orc.Oracle(ref_case, nthreads=args.cpu_threads)
+        t_cpu, done = 0.0, 0
+        while done < 1 or (t_cpu < args.cpu_seconds and done < 20):
+            t0 = time.perf_counter()
+            o.calconc(niter=done + 1)
+            t_cpu += time.perf_counter() - t0
+            done += 1
+        ncmp = done
+        cpu = dict(value=g.nwet * done / t_cpu, unit="column-steps/s",
+                   cores=min(args.cpu_threads, 8), kind="port",
+                   sample=f"{done} CALCONC steps of the same {nx}x{ny}x{kc} grid, "
+                          f"{nc} constituents, by the FP64 C restatement (one "
+                          f"thread per constituent for advection, column-parallel "
+                          f"vertical diffusion), {t_cpu:.1f} s")
+    for n in range(1, ncmp + 1):
+        ctx.calconc(case.dt, n)
+    errs = None
+    if ref_case is not None:
+        errs = []
+        for c in range(nc):
+            con, _ = ctx.get_constituent(c)
+            want = ref_case.consts[c]["con"][:kc]
+            errs.append(float(np.max(np.abs(con[:kc] - want)) /
+                              max(1e-30, float(np.max(np.abs(want))))))
+    n0 = ncmp + 1
+    for n in range(n0, n0 + args.warmup):
+        ctx.calconc(case.dt, n)
+    n0 += args.warmup
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for n in range(n0, n0 + args.steps):
+        ctx.calconc(case.dt, n)
+    ctx.synchronize()
+    t = time.perf_counter() - t0
+    n0 += args.steps
+    vol, mass = ctx.balance()
+    ctx.timers(True)
+    for n in range(n0, n0 + args.steps):
+        ctx.calconc(case.dt, n)
+    tm = ctx.read_timers()
+    ctx.timers(False)
+    cnt, ms = tm["tran_upwind_vdiff"]
+    s_ = 4
+    per_col = kc * s_ * (6 + 2 * nc) + 40
+    avg_s = ms / cnt * 1e-3
+    ach = per_col * g.nwet / avg_s / 1e9
+    out = {
+        "metric": "column-steps/s (CALCONC transport), FP32 storage",
+        "value": g.nwet * args.steps / t,
+        "unit": "column-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"3D transport at scale: {nx}x{ny} all wet, KC={kc}, "
+                               f"{nc} constituents, dt={case.dt} s", "columns": g.nwet},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "tran_upwind_vdiff", "avg_us": ms / cnt * 1e3,
+                     "bytes_per_launch": per_col * g.nwet},
+        "cpu_baseline": cpu,
+        "detail": {"rel_err_vs_fp64_oracle": errs, "after_steps": ncmp,
+                   "fp64_balance": {"volume": vol, "mass": [float(m) for m in mass]},
+                   "kernels": {k: dict(launches=c_, avg_us=1e3 * m_ / c_)
+                               for k, (c_, m_) in tm.items()}},
+    }
+    print(json.dumps(out))
+
+
 def copy_probe(mib):
     """One device-to-device copy of exactly ``mib`` MiB (read once, written
     once): the known-size kernel that scales the HBM byte counters."""
@@ -142,6 +232,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("lake1024", "transport"), default="lake1024",
+                    help="lake1024: the external solve at scale (the headline); "
+                         "transport: CALCONC at scale")
+    ap.add_argument("--tnx", type=int, default=2048)
+    ap.add_argument("--tny", type=int, default=1024)
+    ap.add_argument("--tkc", type=int, default=20)
     ap.add_argument("--cg-kernel", choices=("auto", "multi", "persistent"),
                     default="auto", help="launch strategy of the CG solve")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -156,6 +252,10 @@ def main():
               "(collectives and sharding are out of scope); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "transport":
+        transport_bench(args)
+        return
 
     import numpy as np
     from efdc_amd import grids

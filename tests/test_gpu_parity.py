@@ -268,3 +268,52 @@ def test_fp32_antidiffusion_fct_within_1e5(oracle_mod):
         rel = np.max(np.abs(con[:kc] - c["con"][:kc])) / scale
         print("fp32 adac const", s, "rel", rel)
         assert rel <= 1e-5
+
+
+@pytest.mark.parametrize("rk", [2e-4, -5e-5, 1000.0])
+def test_dye_decay_modes_match_oracle(oracle_mod, rk):
+    """Decay, growth and age-of-water dye (EFDC/calconc.f90:581-616) on top
+    of the full transport step."""
+    from efdc_amd import grids
+    case = grids.transport_case(40, 20, 4, seed=23)
+    case.consts[2]["rkdye"] = rk
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref)
+    ctx = _ctx(case)
+    kc = case.grid.KC
+    for n in range(1, 6):
+        o.calconc(niter=n)
+        ctx.calconc(case.dt, n)
+    con, _ = ctx.get_constituent(2)
+    want = ref.consts[2]["con"][:kc]
+    err = np.max(np.abs(con[:kc] - want)) / max(1e-30, np.max(np.abs(want)))
+    print(f"rkdye={rk}: rel {err:.3e}")
+    assert err <= 1e-12
+
+
+def test_negative_depth_status_on_gpu(oracle_mod):
+    from efdc_amd.lib import EfdcError
+    case = _lake(24, seed=1)
+    i = wet_index(case)
+    case.h["QSUME"][i[len(i) // 2]] = -1e5   # a sink far beyond the cell volume
+    ref = case.copy()
+    st = oracle_mod.Oracle(ref, rsqm=1e-20, iterm=500).calpuv2c(n=1)
+    assert st["status"] == 4
+    ctx = _ctx(case, RSQM=1e-20)
+    with pytest.raises(EfdcError) as e:
+        ctx.calpuv2c(case.dt, 1)
+    assert e.value.status == 4     # EFDC_ERR_NEGDEPTH
+    # the step still completed: HP is the oracle's
+    np.testing.assert_allclose(ctx.get("HP")[i], ref.h["HP"][i], rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_cg_iteration_cap_status_on_gpu(kernel, mode):
+    from efdc_amd.lib import EfdcError
+    case = _lake(24, seed=1)
+    ctx = _ctx(case, RSQM=1e-30, ITERM=2, cg_kernel=kernel, cg_mode=mode,
+               mixed_max_outer=1)
+    with pytest.raises(EfdcError) as e:
+        ctx.calpuv2c(case.dt, 1)
+    assert e.value.status == 3     # EFDC_ERR_CG_MAXITER

@@ -16,6 +16,8 @@ HEAT_MODULE):
 * vertical diffusion: equals scipy's banded solve of the standard
   finite-volume implicit diffusion matrix; conserves column mass;
 * sources: the dye mass equals load x time;
+* dye decay (EFDC/calconc.f90:581-616): with no flow the dye follows the
+  closed forms C*(1/(1+k dt))^n, C*exp(-k dt)^n and C + n dt/86400 (age);
 * failure paths: ITERM exceeded and negative depth return their status.
 """
 import numpy as np
@@ -295,6 +297,39 @@ def test_dye_mass_equals_load_times_time(oracle_mod):
         o.step(n)
     _, m = o.balance()
     assert abs(m[1] - 20 * case.dt * 1.0) <= 1e-9 * 100
+
+
+@pytest.mark.parametrize("rk", [2e-4, -5e-5, 1000.0])
+def test_dye_decay_closed_forms(oracle_mod, rk):
+    """No flow, no sources, no vertical exchange: CALCONC changes the dye
+    only through the decay/age step."""
+    from efdc_amd import grids
+    case = grids.transport_case(24, 12, 3, seed=5)
+    case.sources = None
+    for name in ("UHDY2", "VHDX2", "QSUM", "AB", "U2", "V2"):
+        case.k[name][:] = 0.0
+    case.w["W2"][:] = 0.0
+    case.k["H1PK"][:] = case.k["HPK"]
+    dye = case.consts[2]
+    assert dye["is_dye"] == 1
+    rng = np.random.default_rng(4)
+    kc, L = case.grid.KC, wet_index(case)
+    dye["con"][:kc, L] = rng.uniform(0.5, 2.0, (kc, len(L)))
+    dye["con1"][:] = dye["con"]
+    dye["rkdye"] = rk
+    c0 = dye["con"][:kc, L].copy()
+    o = oracle_mod.Oracle(case)
+    n = 10
+    for it in range(1, n + 1):
+        o.calconc(niter=it)
+    got = case.consts[2]["con"][:kc, L]
+    if rk == 1000.0:
+        want = c0 + n * (case.dt / 86400.0)
+    elif rk < 0:
+        want = c0 * np.exp(-rk * case.dt) ** n
+    else:
+        want = c0 * (1.0 / (1.0 + case.dt * rk)) ** n
+    np.testing.assert_allclose(got, want, rtol=1e-13)
 
 
 # ---------------------------------------------------------- failure paths
