@@ -198,7 +198,8 @@ def transport_bench(args):
         "config": {"workload": f"3D transport at scale: {nx}x{ny} all wet, KC={kc}, "
                                f"{nc} constituents, dt={case.dt} s", "columns": g.nwet},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ach / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("tran_upwind_vdiff", f"transport{nx}x{ny}x{kc}"),
                      "kernel": "tran_upwind_vdiff", "avg_us": ms / cnt * 1e3,
                      "bytes_per_launch": per_col * g.nwet},
         "cpu_baseline": cpu,
@@ -208,6 +209,30 @@ def transport_bench(args):
                                for k, (c_, m_) in tm.items()}},
     }
     print(json.dumps(out))
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by what the counter reads for a plain copy of known size), or
+    None when no summary for this kernel and workload exists."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.startswith((" ", "#")):
+            k, v = line.split(":", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except (ValueError, IndexError):
+                pass
+    need = ("fetch_kib_per_dispatch", "write_kib_per_dispatch", "fetch_scale",
+            "write_scale")
+    if not all(k in vals for k in need):
+        return None
+    return 1024.0 * (vals["fetch_kib_per_dispatch"] * vals["fetch_scale"] +
+                     vals["write_kib_per_dispatch"] * vals["write_scale"])
 
 
 def copy_probe(mib):
@@ -336,7 +361,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": rf_m["achieved"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(rf_m["dominant"], "lake1024"),
             "kernel": rf_m["dominant"],
             "avg_us": rf_m["avg_us"],
             "bytes_per_launch": rf_m["bytes_per_launch"],

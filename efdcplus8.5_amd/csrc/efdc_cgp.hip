@@ -6,30 +6,42 @@
 // bandwidth.  Here every workgroup owns one contiguous range of wet cells
 // (L order) and keeps the rows of its cells -- coefficients, neighbour
 // indices and all CG vectors -- in registers for the whole solve.  Each
-// iteration exchanges only
-//   * the workgroup's own vector values through LDS, and
-//   * the few values that the neighbouring workgroups need (cells with a
-//     neighbour outside the range), written through to the coherent memory
-//     side and recomputed by the reader from the previous iterate,
-// and ends in a grid barrier that also carries the dot-product partials.
+// exchange between iterations moves only
+//   * the workgroup's own vector values, through an LDS window that also
+//     holds the cells just outside the range that its rows reference, and
+//   * the values that the neighbouring workgroups need (cells with a
+//     neighbour outside the range) and the dot-product partials, as
+//     tagged words in memory.
+//
+// Exchange protocol.  A workgroup writes the values its neighbours need
+// (halo values, into a slot of the exchange's parity) and its dot-product
+// partials with agent-scope stores, which are written through to the
+// coherent memory side; every wave waits for its stores to complete
+// (s_waitcnt), and then thread 0 stores the exchange number into the
+// workgroup's flag word (one 128-byte line per workgroup).  A reader polls
+// the flag words of all workgroups (thread q reads workgroup q's flag)
+// until every one carries the exchange number, and then fetches the
+// partials and the halo values it needs in a single round of loads.  Two
+// slot parities suffice: a workgroup can only write exchange T+2 after it
+// has seen every flag of T+1, and every workgroup raises T+1 only after it
+// has consumed T.  Exchange numbers grow across launches (the host numbers
+// launches), so flags never go backwards; when the numbering wraps the host
+// clears them.
 //
 // FP64 (the reference's Jacobi-PCG, EFDC/congrad.f90:52-187): the same
-// recurrences and stop test as the reference, two barriers per iteration
+// recurrences and stop test as the reference, two exchanges per iteration
 // (alpha needs p.Ap, beta needs r.z).
 //
 // Mixed: the inner FP32 CG on the Jacobi-scaled unit-diagonal operator is
-// written in the Chronopoulos-Gear form (the single reduction per
-// iteration carries both (r,r) and (Ar,r)), so it needs one barrier per
-// iteration; FP64 dot products, FP64 alpha/beta, FP64 residual refinement
-// of P inside the same launch.
+// written in the Chronopoulos-Gear form (one exchange per iteration
+// carries (r,r), (Ar,r) and the halo triples r, w = Ar, s = Ap); FP64 dot
+// products, FP64 alpha/beta, FP64 residual refinement of P inside the
+// same launch.
 //
-// Barriers: every block counts in on the counter of its XCD group, the
-// last block of a group counts in on the global counter, every block
-// polls the global counter.  Polling is bounded; a block that gives up
-// raises the abort flag and leaves, the others give up too, and the host
-// reports the failure.  All blocks take the same branch at every barrier
-// (the reductions are computed identically in every block), so the
-// barrier sequence is the same in all blocks.
+// Every wait is bounded: a workgroup that waits too long raises the abort
+// flag and leaves, the others see the flag and leave too, and the host
+// reports the failure.  All workgroups compute the same sums from the
+// same partials in the same order, so they take the same branches.
 #include "efdc_internal.h"
 
 namespace {
@@ -37,17 +49,38 @@ namespace {
 constexpr int PB = 512;           // threads per workgroup
 constexpr int PW = PB / 64;       // waves per workgroup
 constexpr unsigned SPIN_MAX = 1u << 21;
+constexpr int WIN = 2048;         // LDS window reach beyond the own range
+constexpr int HMAX = 2 * WIN;     // outside cells of one workgroup, at most
+constexpr int NVMAX = 3;          // partials per exchange, at most
+constexpr int HX_WORDS = 4;       // 4-byte words per cell and parity in the halo slots
+
+typedef unsigned long long u64;
 
 struct Bar {
   unsigned *ctr;                  // 9 counters, TICKET_PAD apart
   int *abort;
 };
 
-// Grid barrier number `gen` (1, 2, ...). Returns false if the barrier was
-// abandoned (time-out or abort raised by another block).
+__device__ __forceinline__ void put_u32(unsigned *p, unsigned v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned get_u32(const unsigned *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void put_f64(unsigned *p, double v) {
+  const u64 b = (u64)__double_as_longlong(v);
+  __hip_atomic_store((u64 *)p, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 get_u64(const unsigned *p) {
+  return __hip_atomic_load((const u64 *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier number `gen` (1, 2, ...) with counters, for the few places
+// where a plain FP64 array (P) must become visible to every workgroup.
+// Returns false if the barrier was abandoned.
 __device__ bool grid_barrier(const Bar &b, unsigned gen) {
   __shared__ int s_ok;
-  __builtin_amdgcn_s_waitcnt(0);  // this wave's published values completed
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
   __syncthreads();
   if (threadIdx.x == 0) {
     const int nb = (int)gridDim.x;
@@ -75,9 +108,23 @@ __device__ bool grid_barrier(const Bar &b, unsigned gen) {
   return s_ok != 0;
 }
 
-// Block sum of NV values, published to part[j*stride + blockIdx.x].
+// Partials: [parity][NVMAX][PB workgroups] doubles.
+__device__ __forceinline__ double *pslot(double *ps, unsigned tag, int j) {
+  return ps + ((size_t)(tag & 1u) * NVMAX + j) * PB;
+}
+
+struct Xch {
+  double *ps;          // partials
+  unsigned *hx;        // halo slots [2][ncell][HX_WORDS]
+  unsigned *flag;      // one word per workgroup, TICKET_PAD apart
+  int ncell;
+};
+
+// Block sum of NV partials into this workgroup's slots of exchange `tag`,
+// then the flag: every wave's earlier stores (halo values) have completed
+// before thread 0 raises it.
 template <int NV>
-__device__ void publish_partials(double (&v)[NV], double *part, int stride) {
+__device__ void post(double (&v)[NV], const Xch &x, unsigned tag) {
   __shared__ double sh[NV][PW];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -92,22 +139,81 @@ __device__ void publish_partials(double (&v)[NV], double *part, int stride) {
     for (int j = 0; j < NV; ++j) {
       double s = 0.0;
       for (int q = 0; q < PW; ++q) s += sh[j][q];
-      publish(&part[(size_t)j * stride + blockIdx.x], s);
+      __hip_atomic_store(&pslot(x.ps, tag, j)[blockIdx.x], s, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     }
   }
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) put_u32(&x.flag[blockIdx.x * TICKET_PAD], tag);
 }
 
-// Sum of all blocks' partials, in the same order in every block (thread q
-// takes block q's partial, then a fixed tree), so every block obtains the
-// same bits.  Requires gridDim.x <= PB.
-template <int NV>
-__device__ void gather_sums(const double *part, int stride, double (&out)[NV]) {
+// Wait for exchange `tag`, then fetch the NV partials of every workgroup
+// and NR words of each outside cell (hl[0..nhalo)) into raw[q*HMAX + h].
+// Returns the sums (same bits in every workgroup), or false if the wait was
+// abandoned.
+template <int NV, int NR>
+__device__ bool exchange(const Xch &x, unsigned tag, const int *hl, int nhalo,
+                         unsigned *raw, const Bar &b, double (&sum)[NV]) {
   __shared__ double sh[NV][PW];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const bool hasp = t < (int)gridDim.x;
+  unsigned spins = 0;
+  for (;;) {
+    const bool ready = !hasp || get_u32(&x.flag[t * TICKET_PAD]) >= tag;
+    if (__syncthreads_and(ready)) break;
+    if (++spins > SPIN_MAX ||
+        ((spins & 255u) == 0u &&
+         __hip_atomic_load(b.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+      if (t == 0)
+        __hip_atomic_store(b.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
   double v[NV];
 #pragma unroll
   for (int j = 0; j < NV; ++j)
-    v[j] = (int)threadIdx.x < (int)gridDim.x ? fetch(&part[(size_t)j * stride + threadIdx.x]) : 0.0;
+    v[j] = hasp ? __hip_atomic_load(&pslot(x.ps, tag, j)[t], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT)
+                : 0.0;
+  if (NR > 0) {
+    // all loads of a batch of HB outside cells are in flight before the
+    // first result is stored (a store per load would serialise the round
+    // trips)
+    constexpr int HB = 4;
+    const unsigned *hxp = x.hx + (size_t)(tag & 1u) * x.ncell * HX_WORDS;
+    for (int h0 = t; h0 < nhalo; h0 += HB * PB) {
+      u64 wv[HB][2];
+#pragma unroll
+      for (int u = 0; u < HB; ++u) {
+        const int h = h0 + u * PB;
+        wv[u][0] = wv[u][1] = 0;
+        if (h < nhalo) {
+          const unsigned *q = hxp + (size_t)hl[h] * HX_WORDS;
+          if (NR == 4) {
+            wv[u][0] = get_u64(q);
+            wv[u][1] = get_u64(q + 2);
+          } else if (NR >= 2) {
+            wv[u][0] = get_u64(q);
+            if (NR == 3) wv[u][1] = get_u32(q + 2);
+          } else {
+            wv[u][0] = get_u32(q);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < HB; ++u) {
+        const int h = h0 + u * PB;
+        if (h < nhalo) {
+          raw[h] = (unsigned)wv[u][0];
+          if (NR >= 2) raw[HMAX + h] = (unsigned)(wv[u][0] >> 32);
+          if (NR >= 3) raw[2 * HMAX + h] = (unsigned)wv[u][1];
+          if (NR == 4) raw[3 * HMAX + h] = (unsigned)(wv[u][1] >> 32);
+        }
+      }
+    }
+  }
 #pragma unroll
   for (int j = 0; j < NV; ++j) v[j] = wave_sum(v[j]);
   if (lane == 0) {
@@ -119,16 +225,10 @@ __device__ void gather_sums(const double *part, int stride, double (&out)[NV]) {
   for (int j = 0; j < NV; ++j) {
     double s = 0.0;
     for (int q = 0; q < PW; ++q) s += sh[j][q];
-    out[j] = s;
+    sum[j] = s;
   }
   __syncthreads();
-}
-
-__device__ __forceinline__ void publish_f(float *p, float v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ float fetch_f(const float *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
 }
 
 // neighbour slots of a row
@@ -143,15 +243,9 @@ __device__ __forceinline__ int opaque(int v) {
   return v;
 }
 
-// Every workgroup keeps a window of its range in LDS: the values of its own
-// cells plus up to WIN cells on either side (the cells just outside the
-// range that its rows reference; the host only launches when every
-// neighbour lies within WIN of its row).  Slot of cell j: j - b0 + WIN.
-constexpr int WIN = 4096;
-
-// Compact list of the outside cells a workgroup's rows reference
-// (sorted by index).  `mark` holds 2*WIN flags on entry and the list on
-// return; returns the list length.
+// Compact list of the outside cells a workgroup's rows reference (sorted
+// by index).  `mark` holds 2*WIN flags on entry (slot g: cell b0-WIN+g for
+// g < WIN, cell b1+g-WIN above) and the list on return; returns its length.
 __device__ int compact_halo(int *mark, int b0, int b1) {
   __shared__ int wtot[PW];
   __shared__ int total;
@@ -164,8 +258,7 @@ __device__ int compact_halo(int *mark, int b0, int b1) {
     f[u] = mark[t * PER + u];
     cnt += f[u] ? 1 : 0;
   }
-  // exclusive scan of cnt over the workgroup
-  int inc = cnt;
+  int inc = cnt;  // inclusive scan over the workgroup
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
     const int v = __shfl_up(inc, off, 64);
@@ -201,10 +294,8 @@ struct P64Args {
   double *x;                 // P, in/out
   const int *lw, *le, *ls, *ln;
   int nwet, lc0, rowmajor, mutual;
-  double *hr;                // published r (one buffer)
-  double *hp[2];             // published p, by iteration parity
-  double *part;              // [2 parities][3][stride]
-  int stride;
+  Xch xc;                    // exchange slots; halo words: r (2), p (2)
+  unsigned tag0;
   Bar bar;
   CgState *st;
   double rsqm, rel_tol;
@@ -215,6 +306,7 @@ template <int CPT>
 __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
   __shared__ double lds[2 * WIN + CPT * PB];
   __shared__ int hl[2 * WIN];
+  __shared__ unsigned raw[4 * HMAX];
   const int b0 = 1 + (int)blockIdx.x * CPT * PB;
   const int b1 = min(b0 + CPT * PB, a.nwet + 1);
   const bool fast = a.rowmajor && (a.st->symok == 0);
@@ -228,10 +320,15 @@ __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
   auto inside = [&](int j) { return j >= b0 && j < b1; };
   auto real = [&](int j) { return j > 0 && j != lc0; };
   auto slot = [&](int j) { return j - b0 + WIN; };
+  auto publish_rp = [&](int i, unsigned tag, double rv, double pv) {
+    unsigned *q = a.xc.hx + ((size_t)(tag & 1u) * a.xc.ncell + (size_t)opaque(i)) * HX_WORDS;
+    put_f64(q, rv);
+    put_f64(q + 2, pv);
+  };
 
   for (int g = threadIdx.x; g < 2 * WIN; g += PB) hl[g] = 0;
   __syncthreads();
-  double v3[2] = {0.0, 0.0};
+  double v2[2] = {0.0, 0.0};
 #pragma unroll
   for (int k = 0; k < CPT; ++k) {
     const int i = b0 + k * PB + (int)threadIdx.x;
@@ -275,25 +372,25 @@ __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
       const double rr = a.fptmp[i] - ccc[k] * x[k] - n_ * P[jn] - s_ * P[js] -
                         w_ * P[jw] - e_ * P[je];
       r[k] = rr;
-      v3[0] += rr * (rr * cci[k]);   // RPCG = sum RCG*PCG (:80-85)
-      v3[1] += a.fptmp[i] * a.fptmp[i];
-      if (expo & (1u << k)) publish(&a.hr[i], rr);
+      v2[0] += rr * (rr * cci[k]);   // RPCG = sum RCG*PCG (:80-85)
+      v2[1] += a.fptmp[i] * a.fptmp[i];
     }
   }
   __syncthreads();
   const int nhalo = compact_halo(hl, b0, b1);
 
-  unsigned gen = 0;
-  double rpcg, rsqm, beta = 0.0, alpha = 0.0, rsq = 0.0, papcg = 0.0, bnorm2;
-  {
-    publish_partials<2>(v3, a.part + (size_t)(gen & 1) * 3 * a.stride, a.stride);
-    if (!grid_barrier(a.bar, ++gen)) return;
-    double s2[2];
-    gather_sums<2>(a.part + (size_t)((gen - 1) & 1) * 3 * a.stride, a.stride, s2);
-    rpcg = s2[0];
-    bnorm2 = s2[1];
-  }
-  rsqm = a.rel_tol > 0.0 ? a.rel_tol * a.rel_tol * bnorm2 : a.rsqm;
+  unsigned tag = a.tag0;
+  // exchange: r and p (= 0) of the outside cells, RPCG and |FPTMP|^2
+#pragma unroll
+  for (int k = 0; k < CPT; ++k)
+    if (expo & (1u << k)) publish_rp(b0 + k * PB + (int)threadIdx.x, tag, r[k], 0.0);
+  post<2>(v2, a.xc, tag);
+  double s2[2];
+  if (!exchange<2, 4>(a.xc, tag, hl, nhalo, raw, a.bar, s2)) return;
+  double rpcg = s2[0];
+  const double bnorm2 = s2[1];
+  const double rsqm = a.rel_tol > 0.0 ? a.rel_tol * a.rel_tol * bnorm2 : a.rsqm;
+  double beta = 0.0, alpha = 0.0, rsq = 0.0, papcg = 0.0;
   int iter = 0, status = 0;
   bool done = (rpcg == 0.0);  // :88
 
@@ -305,14 +402,14 @@ __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
       p[k] = cci[k] * r[k] + beta * p[k];
       if (i < b1) lds[slot(i)] = p[k];
     }
-    // ... and on the outside cells, from their published r and old p
-    {
-      const double *pold = a.hp[(iter + 1) & 1];
-      for (int t = threadIdx.x; t < nhalo; t += PB) {
-        const int j = hl[t];
-        const double po = iter > 0 ? fetch(&pold[j]) : 0.0;
-        lds[slot(j)] = a.ccci[j] * fetch(&a.hr[j]) + beta * po;
-      }
+    // ... and on the outside cells, from their published r and p
+    for (int h = threadIdx.x; h < nhalo; h += PB) {
+      const int j = hl[h];
+      const double rj = __longlong_as_double(
+          (long long)(((u64)raw[HMAX + h] << 32) | raw[h]));
+      const double pj = __longlong_as_double(
+          (long long)(((u64)raw[3 * HMAX + h] << 32) | raw[2 * HMAX + h]));
+      lds[slot(j)] = a.ccci[j] * rj + beta * pj;
     }
     __syncthreads();
     double v1[1] = {0.0};
@@ -330,18 +427,16 @@ __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
         ap[k] = ccc[k] * p[k] + cs[k] * pq[NS] + cn[k] * pq[NN] + cw[k] * pq[NW] +
                 ce[k] * pq[NE];
         v1[0] += ap[k] * p[k];
-        if (expo & (1u << k)) publish(&a.hp[iter & 1][opaque(i)], p[k]);
       }
     }
-    publish_partials<1>(v1, a.part + (size_t)(gen & 1) * 3 * a.stride, a.stride);
-    if (!grid_barrier(a.bar, ++gen)) return;
-    {
-      double s1[1];
-      gather_sums<1>(a.part + (size_t)((gen - 1) & 1) * 3 * a.stride, a.stride, s1);
-      papcg = s1[0];
-      alpha = rpcg / papcg;  // :124
-    }
-    double v2[2] = {0.0, 0.0};
+    ++tag;
+    post<1>(v1, a.xc, tag);
+    double s1[1];
+    if (!exchange<1, 0>(a.xc, tag, hl, nhalo, raw, a.bar, s1)) return;
+    papcg = s1[0];
+    alpha = rpcg / papcg;  // :124
+    double v[2] = {0.0, 0.0};
+    ++tag;
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int i = b0 + k * PB + (int)threadIdx.x;
@@ -350,15 +445,13 @@ __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
         const double rr = r[k] - alpha * ap[k];  // :133
         r[k] = rr;
         const double z = cci[k] * rr;      // :134
-        v2[0] += rr * z;
-        v2[1] += rr * rr;
-        if (expo & (1u << k)) publish(&a.hr[opaque(i)], rr);
+        v[0] += rr * z;
+        v[1] += rr * rr;
+        if (expo & (1u << k)) publish_rp(i, tag, rr, p[k]);
       }
     }
-    publish_partials<2>(v2, a.part + (size_t)(gen & 1) * 3 * a.stride, a.stride);
-    if (!grid_barrier(a.bar, ++gen)) return;
-    double s2[2];
-    gather_sums<2>(a.part + (size_t)((gen - 1) & 1) * 3 * a.stride, a.stride, s2);
+    post<2>(v, a.xc, tag);
+    if (!exchange<2, 4>(a.xc, tag, hl, nhalo, raw, a.bar, s2)) return;
     beta = s2[0] / rpcg;  // :147-148
     rpcg = s2[0];
     rsq = s2[1];
@@ -395,9 +488,8 @@ struct PMxArgs {
   double *x;                 // P, in/out
   const int *ls, *ln;
   int nwet, lc0;
-  float *hr[2], *hw[2], *hs[2];   // published r, w = A r, s = A p
-  double *part;              // [2 parities][3][stride]
-  int stride;
+  Xch xc;                    // exchange slots; halo words: r, w = A r, s = A p
+  unsigned tag0;
   Bar bar;
   CgState *st;
   double rsqm, rel_tol, rtol2;
@@ -408,6 +500,7 @@ template <int CPT>
 __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
   __shared__ float lds[2 * WIN + CPT * PB];
   __shared__ int hl[2 * WIN];
+  __shared__ unsigned raw[3 * HMAX];
   const int b0 = 1 + (int)blockIdx.x * CPT * PB;
   const int b1 = min(b0 + CPT * PB, a.nwet + 1);
   const int lc0 = a.lc0;
@@ -422,6 +515,9 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
   auto slot = [&](int j) { return j - b0 + WIN; };
   auto dinv = [&](int j) { return 1.0 / sqrt(a.ccc[j]); };
   auto rval = [&](int j) { return real(j) ? lds[slot(j)] : 0.0f; };
+  auto hxw = [&](int i, unsigned tag) {
+    return a.xc.hx + ((size_t)(tag & 1u) * a.xc.ncell + (size_t)opaque(i)) * HX_WORDS;
+  };
 
   for (int g = threadIdx.x; g < 2 * WIN; g += PB) hl[g] = 0;
   __syncthreads();
@@ -474,7 +570,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
   __syncthreads();
   const int nhalo = compact_halo(hl, b0, b1);
 
-  unsigned gen = 0;
+  unsigned tag = a.tag0, gen = 0;
   double rsqm = a.rsqm, bnorm2 = 0.0, rsq64 = 0.0, bn_in = 0.0;
   int total = 0, outer = 0, status = 0, inner = 0;
 
@@ -500,15 +596,15 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
         y[k] = 0.0f;
         v3[0] += rr * rr;
         v3[1] += (double)rs * (double)rs;
-        if (expo & (1u << k)) publish_f(&a.hr[0][io], rs);
+        lds[slot(io)] = rs;
+        if (expo & (1u << k)) put_u32(hxw(io, tag), __float_as_uint(rs));
       }
     }
     fp2 = 0.0;
-    publish_partials<3>(v3, a.part + (size_t)(gen & 1) * 3 * a.stride, a.stride);
-    if (!grid_barrier(a.bar, ++gen)) return;
+    post<3>(v3, a.xc, tag);
     {
       double s3[3];
-      gather_sums<3>(a.part + (size_t)((gen - 1) & 1) * 3 * a.stride, a.stride, s3);
+      if (!exchange<3, 1>(a.xc, tag, hl, nhalo, raw, a.bar, s3)) return;
       if (outer == 0) {
         bnorm2 = s3[2];
         rsqm = a.rel_tol > 0.0 ? a.rel_tol * a.rel_tol * bnorm2 : a.rsqm;
@@ -520,47 +616,39 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
     if (outer >= a.max_outer || bn_in == 0.0) { status = 1; break; }
 
     // ---- w0 = A' r0; gamma0 = (r,r), delta0 = (w,r); s_-1 = p_-1 = 0
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-      const int i = b0 + k * PB + (int)threadIdx.x;
-      if (i < b1) lds[slot(i)] = r[k];
-      s[k] = 0.0f;
-      p[k] = 0.0f;
-    }
-    for (int t = threadIdx.x; t < nhalo; t += PB) {
-      const int j = hl[t];
-      lds[slot(j)] = fetch_f(&a.hr[0][j]);
-    }
+    for (int h = threadIdx.x; h < nhalo; h += PB) lds[slot(hl[h])] = __uint_as_float(raw[h]);
     __syncthreads();
+    ++tag;
     double v2[2] = {0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int i = b0 + k * PB + (int)threadIdx.x;
+      s[k] = 0.0f;
+      p[k] = 0.0f;
       if (i < b1) {
         w[k] = r[k] + as_[k] * rval(js[k]) + an_[k] * rval(jn[k]) +
                aw_[k] * rval(i - 1) + ae_[k] * rval(i + 1);
         v2[0] += (double)r[k] * (double)r[k];
         v2[1] += (double)w[k] * (double)r[k];
         if (expo & (1u << k)) {
-          const int io = opaque(i);
-          publish_f(&a.hw[0][io], w[k]);
-          publish_f(&a.hs[0][io], 0.0f);
+          unsigned *q = hxw(i, tag);
+          put_u32(q, __float_as_uint(r[k]));
+          put_u32(q + 1, __float_as_uint(w[k]));
+          put_u32(q + 2, __float_as_uint(0.0f));
         }
       }
     }
-    publish_partials<2>(v2, a.part + (size_t)(gen & 1) * 3 * a.stride, a.stride);
-    if (!grid_barrier(a.bar, ++gen)) return;
+    post<2>(v2, a.xc, tag);
     double gam, del;
     {
       double s2[2];
-      gather_sums<2>(a.part + (size_t)((gen - 1) & 1) * 3 * a.stride, a.stride, s2);
+      if (!exchange<2, 3>(a.xc, tag, hl, nhalo, raw, a.bar, s2)) return;
       gam = s2[0];
       del = s2[1];
     }
     double gam_prev = 0.0, alpha_prev = 1.0;
     inner = 0;
     for (;;) {
-      const int par = inner & 1;
       double beta, alpha;
       if (inner == 0) {
         beta = 0.0;
@@ -580,13 +668,15 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
         if (i < b1) lds[slot(i)] = r[k];
       }
       // the outside cells' update, from their published r, w, s
-      for (int t = threadIdx.x; t < nhalo; t += PB) {
-        const int j = hl[t];
-        const float sj = (float)((double)fetch_f(&a.hw[par][j]) +
-                                 beta * (double)fetch_f(&a.hs[par][j]));
-        lds[slot(j)] = (float)((double)fetch_f(&a.hr[par][j]) - alpha * (double)sj);
+      for (int h = threadIdx.x; h < nhalo; h += PB) {
+        const float rj = __uint_as_float(raw[h]);
+        const float wj = __uint_as_float(raw[HMAX + h]);
+        const float sj0 = __uint_as_float(raw[2 * HMAX + h]);
+        const float sj = (float)((double)wj + beta * (double)sj0);
+        lds[slot(hl[h])] = (float)((double)rj - alpha * (double)sj);
       }
       __syncthreads();
+      ++tag;
       double v[2] = {0.0, 0.0};
 #pragma unroll
       for (int k = 0; k < CPT; ++k) {
@@ -597,17 +687,16 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
           v[0] += (double)r[k] * (double)r[k];
           v[1] += (double)w[k] * (double)r[k];
           if (expo & (1u << k)) {
-            const int io = opaque(i);
-            publish_f(&a.hr[par ^ 1][io], r[k]);
-            publish_f(&a.hw[par ^ 1][io], w[k]);
-            publish_f(&a.hs[par ^ 1][io], s[k]);
+            unsigned *q = hxw(i, tag);
+            put_u32(q, __float_as_uint(r[k]));
+            put_u32(q + 1, __float_as_uint(w[k]));
+            put_u32(q + 2, __float_as_uint(s[k]));
           }
         }
       }
-      publish_partials<2>(v, a.part + (size_t)(gen & 1) * 3 * a.stride, a.stride);
-      if (!grid_barrier(a.bar, ++gen)) return;
+      post<2>(v, a.xc, tag);
       double s2[2];
-      gather_sums<2>(a.part + (size_t)((gen - 1) & 1) * 3 * a.stride, a.stride, s2);
+      if (!exchange<2, 3>(a.xc, tag, hl, nhalo, raw, a.bar, s2)) return;
       inner += 1;
       gam_prev = gam;
       alpha_prev = alpha;
@@ -616,7 +705,9 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
       if (gam <= a.rtol2 * bn_in || inner >= a.iterm) break;
     }
     total += inner;
-    // ---- P += D^-1/2 y (FP64), published for the next residual
+    ++tag;
+    // ---- P += D^-1/2 y (FP64), visible to every workgroup for the
+    // next residual
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int i = b0 + k * PB + (int)threadIdx.x;
@@ -663,6 +754,11 @@ int occupancy(K kern) {
 // Largest cells-per-thread instantiation of each solve.
 constexpr int PCG64_CPT_MAX = 6;
 constexpr int PCGMX_CPT_MAX = 8;
+// Exchange numbers: launch number (1 .. 4095) in the top 12 bits, exchange
+// within the launch in the low 20 bits.
+constexpr int PCG_TAG_BITS = 20;
+constexpr long PCG_TAGS_PER_LAUNCH = 1L << PCG_TAG_BITS;
+constexpr unsigned PCG_LAUNCH_CYCLE = (1u << (32 - PCG_TAG_BITS)) - 1u;
 
 int pcg_prepare(efdc_ctx *c) {
   if (c->pcg_ready) return EFDC_OK;
@@ -680,18 +776,16 @@ int pcg_prepare(efdc_ctx *c) {
       occupancy(k_pcgmx<7>), occupancy(k_pcgmx<8>)};
   for (int k = 0; k <= PCG64_CPT_MAX; ++k) c->pcg_occ64[k] = occ64[k];
   for (int k = 0; k <= PCGMX_CPT_MAX; ++k) c->pcg_occmx[k] = occmx[k];
-  const int n = c->n_wet() + 2;
+  c->pcg_ncell = c->n_wet() + 2;
   int s;
-  if ((s = dev_alloc(c, (void **)&c->pcg_hr64, sizeof(double) * n))) return s;
-  for (int q = 0; q < 2; ++q) {
-    if ((s = dev_alloc(c, (void **)&c->pcg_hp64[q], sizeof(double) * n))) return s;
-    if ((s = dev_alloc(c, (void **)&c->pcg_h32[q][0], sizeof(float) * n))) return s;
-    if ((s = dev_alloc(c, (void **)&c->pcg_h32[q][1], sizeof(float) * n))) return s;
-    if ((s = dev_alloc(c, (void **)&c->pcg_h32[q][2], sizeof(float) * n))) return s;
-  }
-  if ((s = dev_alloc(c, (void **)&c->pcg_part, sizeof(double) * 2 * 3 * PB))) return s;
+  if ((s = dev_alloc(c, &c->pcg_hx,
+                     sizeof(unsigned) * 2 * (size_t)c->pcg_ncell * HX_WORDS)))
+    return s;
+  if ((s = dev_alloc(c, &c->pcg_ps, sizeof(double) * 2 * NVMAX * PB))) return s;
+  if ((s = dev_alloc(c, (void **)&c->pcg_flag, sizeof(unsigned) * PB * TICKET_PAD))) return s;
   if ((s = dev_alloc(c, (void **)&c->pcg_ctr, sizeof(unsigned) * 9 * TICKET_PAD))) return s;
   if ((s = dev_alloc(c, (void **)&c->pcg_abort, sizeof(int)))) return s;
+  c->pcg_launch = 0;
   c->pcg_ready = true;
   return EFDC_OK;
 }
@@ -714,6 +808,11 @@ static int pcg_shape(const efdc_ctx *c, const int *occ, int cpt_max, int &nb) {
 bool pcg_applicable(efdc_ctx *c, bool mixed) {
   if (c->prm.cg_kernel == EFDC_CG_KERNEL_MULTI) return false;
   if (c->band > WIN) return false;  // a neighbour beyond the LDS window
+  // the exchanges of one solve must fit one launch's tag range
+  const long per_outer = 3L + c->prm.ITERM;
+  const long outers = mixed ? (long)c->prm.mixed_max_outer + 1 : 1;
+  if ((mixed ? per_outer * outers : 1L + 2L * c->prm.ITERM) >= PCG_TAGS_PER_LAUNCH)
+    return false;
   if (pcg_prepare(c) != EFDC_OK || c->pcg_ncu == 0) return false;
   int nb = 0;
   return mixed ? pcg_shape(c, c->pcg_occmx, PCGMX_CPT_MAX, nb) > 0
@@ -724,6 +823,19 @@ template <typename Args>
 static int coop_launch(efdc_ctx *c, const void *fn, int nb, Args &args) {
   void *kargs[] = {&args};
   HIP_TRY(c, hipLaunchCooperativeKernel(fn, dim3(nb), dim3(PB), kargs, 0, c->stream));
+  return EFDC_OK;
+}
+
+// First exchange number of the next launch.  Launches are numbered
+// 1 .. 4095, so within a cycle exchange numbers only grow and the flags never
+// go backwards; when the numbering starts a new cycle the flags are cleared.
+static int next_tag0(efdc_ctx *c, unsigned &tag0) {
+  const bool wrap = c->pcg_launch == PCG_LAUNCH_CYCLE;
+  c->pcg_launch = c->pcg_launch % PCG_LAUNCH_CYCLE + 1u;
+  if (wrap)
+    HIP_TRY(c, hipMemsetAsync(c->pcg_flag, 0, sizeof(unsigned) * PB * TICKET_PAD,
+                              c->stream));
+  tag0 = c->pcg_launch << PCG_TAG_BITS;
   return EFDC_OK;
 }
 
@@ -754,11 +866,8 @@ int pcg_solve(efdc_ctx *c, bool mixed) {
     a.lc0 = c->LC - 1;
     a.rowmajor = c->rowmajor ? 1 : 0;
     a.mutual = c->mutual ? 1 : 0;
-    a.hr = c->pcg_hr64;
-    a.hp[0] = c->pcg_hp64[0];
-    a.hp[1] = c->pcg_hp64[1];
-    a.part = c->pcg_part;
-    a.stride = PB;
+    a.xc = Xch{(double *)c->pcg_ps, (unsigned *)c->pcg_hx, c->pcg_flag, c->pcg_ncell};
+    if ((s = next_tag0(c, a.tag0))) return s;
     a.bar = bar;
     a.st = c->st;
     a.rsqm = c->prm.RSQM;
@@ -788,13 +897,8 @@ int pcg_solve(efdc_ctx *c, bool mixed) {
     a.ln = c->dLN;
     a.nwet = c->n_wet();
     a.lc0 = c->LC - 1;
-    for (int q = 0; q < 2; ++q) {
-      a.hr[q] = c->pcg_h32[q][0];
-      a.hw[q] = c->pcg_h32[q][1];
-      a.hs[q] = c->pcg_h32[q][2];
-    }
-    a.part = c->pcg_part;
-    a.stride = PB;
+    a.xc = Xch{(double *)c->pcg_ps, (unsigned *)c->pcg_hx, c->pcg_flag, c->pcg_ncell};
+    if ((s = next_tag0(c, a.tag0))) return s;
     a.bar = bar;
     a.st = c->st;
     a.rsqm = c->prm.RSQM;
@@ -824,7 +928,7 @@ int pcg_solve(efdc_ctx *c, bool mixed) {
                             c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (abort_h) {
-    c->err = "persistent CG: a grid barrier timed out";
+    c->err = "persistent CG: an exchange between workgroups timed out";
     return EFDC_ERR_HIP;
   }
   return EFDC_OK;

@@ -289,10 +289,9 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   dfree(c->dis); dfree(c->as32); dfree(c->aw32); dfree(c->r32); dfree(c->y32);
   dfree(c->p32[0]); dfree(c->p32[1]); dfree(c->ap32);
   dfree(c->part); dfree(c->st); dfree(c->tickets);
-  dfree(c->pcg_hr64); dfree(c->pcg_hp64[0]); dfree(c->pcg_hp64[1]);
-  for (auto &q : c->pcg_h32)
-    for (auto &a : q) dfree(a);
-  dfree(c->pcg_part); dfree(c->pcg_ctr); dfree(c->pcg_abort);
+  { char *q = (char *)c->pcg_hx; dfree(q); }
+  { char *q = (char *)c->pcg_ps; dfree(q); }
+  dfree(c->pcg_ctr); dfree(c->pcg_abort); dfree(c->pcg_flag);
   if (c->st_host) (void)hipHostFree(c->st_host);
   if (c->tide_host) (void)hipHostFree(c->tide_host);
   for (auto &b : c->pbc) {

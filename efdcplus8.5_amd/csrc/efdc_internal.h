@@ -110,9 +110,11 @@ struct efdc_ctx {
   bool pcg_ready = false;
   int pcg_ncu = 0;             // CUs, 0 when cooperative launch is unsupported
   int pcg_occ64[7] = {}, pcg_occmx[9] = {};  // workgroups per CU by CPT
-  double *pcg_hr64 = nullptr, *pcg_hp64[2] = {nullptr, nullptr};
-  float *pcg_h32[2][3] = {};   // [parity][r, w, s]
-  double *pcg_part = nullptr;  // [2][3][1024] partials
+  void *pcg_hx = nullptr;      // halo slots [2][ncell][4] words
+  void *pcg_ps = nullptr;      // partial slots
+  unsigned *pcg_flag = nullptr;  // exchange flag per workgroup
+  int pcg_ncell = 0;
+  unsigned pcg_launch = 0;     // launch number, gives the first tag
   unsigned *pcg_ctr = nullptr; // barrier counters
   int *pcg_abort = nullptr;
 

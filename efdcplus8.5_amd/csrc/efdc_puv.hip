@@ -15,6 +15,7 @@
 // has been met, so the host only polls every few iterations.
 #include <cmath>
 #include <cstring>
+#include <utility>
 
 #include "efdc_internal.h"
 
@@ -57,30 +58,27 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_explicit(
   if (i > nwet) return;
   const double DELT = dt, DELTD2 = 0.5 * dt, DELTI = 1. / DELT;
   const int w = lw[i], s = ls[i], e = le[i], n = ln[i];
+  // The shifts UHDY1E=UHDYE, VHDX1E=VHDXE, H1U=HU, H1V=HV, H1UI=HUI,
+  // H1VI=HVI, H2P=H1P, H1P=HP (:256-265) are buffer rotations done by the
+  // host before this launch, so the step-start values are read from the
+  // "1" arrays here and in every kernel up to the one that rewrites the
+  // current array.
   const double *P = HA(P);
   HA(FUHDYE)[i] =
-      HA(UHDYE)[i] - DELTD2 * HA(SUB)[i] * HA(HRUO)[i] * HA(HU)[i] * (P[i] - P[w]) +
+      HA(UHDY1E)[i] - DELTD2 * HA(SUB)[i] * HA(HRUO)[i] * HA(H1U)[i] * (P[i] - P[w]) +
       HA(SUB)[i] * DELT * HA(DXIU)[i] *
           (HA(DXYU)[i] * (HA(TSX)[i] - RITB1 * HA(TBX)[i]) + HA(FCAXE)[i] +
            HA(FPGXE)[i] - SNLT * HA(FXE)[i]);
   HA(FVHDXE)[i] =
-      HA(VHDXE)[i] - DELTD2 * HA(SVB)[i] * HA(HRVO)[i] * HA(HV)[i] * (P[i] - P[s]) +
+      HA(VHDX1E)[i] - DELTD2 * HA(SVB)[i] * HA(HRVO)[i] * HA(H1V)[i] * (P[i] - P[s]) +
       HA(SVB)[i] * DELT * HA(DYIV)[i] *
           (HA(DXYV)[i] * (HA(TSY)[i] - RITB1 * HA(TBY)[i]) - HA(FCAYE)[i] +
            HA(FPGYE)[i] - SNLT * HA(FYE)[i]);
-  HA(UHDY1E)[i] = HA(UHDYE)[i];
-  HA(VHDX1E)[i] = HA(VHDXE)[i];
   HA(P1)[i] = P[i];
-  HA(H1U)[i] = HA(HU)[i];
-  HA(H1V)[i] = HA(HV)[i];
-  HA(H1UI)[i] = HA(HUI)[i];
-  HA(H1VI)[i] = HA(HVI)[i];
-  HA(H2P)[i] = HA(H1P)[i];
-  HA(H1P)[i] = HA(HP)[i];
   for (int k = 0; k < KC; ++k) h1pk[i + k * pitch] = hpk[i + k * pitch];
   const double C1 = 0.5 * G;
   HA(FP1)[i] = DELTI * HA(DXYP)[i] * P[i] -
-               C1 * (HA(UHDYE)[e] - HA(UHDYE)[i] + HA(VHDXE)[n] - HA(VHDXE)[i]);
+               C1 * (HA(UHDY1E)[e] - HA(UHDY1E)[i] + HA(VHDX1E)[n] - HA(VHDX1E)[i]);
 }
 
 // ---------------------------------------------------------------------
@@ -104,10 +102,10 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_assemble(
                                    SVB[n] * HA(FVHDXE)[n] - SVB[i] * HA(FVHDXE)[i] -
                                    2.0 * HA(QSUME)[i]);
     const double C2 = -0.5 * DELTD2 * G;
-    const double cs = C2 * SVB[i] * HA(HRVO)[i] * HA(RCY)[i] * HA(HV)[i];
-    const double cw = C2 * SUB[i] * HA(HRUO)[i] * HA(RCX)[i] * HA(HU)[i];
-    const double ce = C2 * SUB[e] * HA(HRUO)[e] * HA(RCX)[e] * HA(HU)[e];
-    const double cn = C2 * SVB[n] * HA(HRVO)[n] * HA(RCY)[n] * HA(HV)[n];
+    const double cs = C2 * SVB[i] * HA(HRVO)[i] * HA(RCY)[i] * HA(H1V)[i];
+    const double cw = C2 * SUB[i] * HA(HRUO)[i] * HA(RCX)[i] * HA(H1U)[i];
+    const double ce = C2 * SUB[e] * HA(HRUO)[e] * HA(RCX)[e] * HA(H1U)[e];
+    const double cn = C2 * SVB[n] * HA(HRVO)[n] * HA(RCY)[n] * HA(H1V)[n];
     HA(CS)[i] = cs;
     HA(CW)[i] = cw;
     HA(CE)[i] = ce;
@@ -143,8 +141,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_bc(
   const double HDRY2 = 0.2 * HDRY;
   double *CC = HA(CC), *CS = HA(CS), *CW = HA(CW), *CE = HA(CE), *CN = HA(CN),
          *FP = HA(FP), *DXYP = HA(DXYP), *BELV = HA(BELV), *HRUO = HA(HRUO),
-         *HRVO = HA(HRVO), *RCX = HA(RCX), *RCY = HA(RCY), *HUT = HA(HU),
-         *HVT = HA(HV), *DXIU = HA(DXIU), *DYIU = HA(DYIU), *DXIV = HA(DXIV),
+         *HRVO = HA(HRVO), *RCX = HA(RCX), *RCY = HA(RCY), *HUT = HA(H1U),
+         *HVT = HA(H1V), *DXIU = HA(DXIU), *DYIU = HA(DYIU), *DXIV = HA(DXIV),
          *DYIV = HA(DYIV), *FUHDYE = HA(FUHDYE), *FVHDXE = HA(FVHDXE);
   for (int side = 0; side < 4; ++side) {
     const PbcDev &s = b.side[side];
@@ -335,6 +333,35 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_cg_check(
 __global__ void k_reset_flags(CgState *st) {
   st->symok = 0;
   st->negdep = 0;
+}
+
+// Buffers of the rotated time levels: pair[q] = {X, X1} after the swap,
+// tri = {HP, H1P, H2P} after the rotation.
+struct Rotated {
+  double *pair[6][2];
+  double *tri[3];
+  int lc0;
+};
+
+// Start of a CALPUV2C step: reset the flags, and give each rotated name
+// back the ghost entries (L=1 and L=LC) it had before the rotation -- the
+// reference's shifts only copy the wet cells.
+__global__ void k_step_begin(CgState *st, Rotated r) {
+  st->symok = 0;
+  st->negdep = 0;
+  const int gs[2] = {0, r.lc0};
+  for (int k = 0; k < 2; ++k) {
+    const int g = gs[k];
+    for (int q = 0; q < 6; ++q) {
+      const double a = r.pair[q][0][g], b = r.pair[q][1][g];
+      r.pair[q][0][g] = b;
+      r.pair[q][1][g] = a;
+    }
+    const double hp = r.tri[1][g], h1p = r.tri[2][g], h2p = r.tri[0][g];
+    r.tri[0][g] = hp;
+    r.tri[1][g] = h1p;
+    r.tri[2][g] = h2p;
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -660,9 +687,9 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_post1(
   const double *P = HA(P);
   const int w = lw[i], s = ls[i];
   const double u = HA(SUB)[i] * (HA(FUHDYE)[i] - DELTD2 * HA(HRUO)[i] * HA(RCX)[i] *
-                                                    HA(HU)[i] * (P[i] - P[w]));
+                                                    HA(H1U)[i] * (P[i] - P[w]));
   const double v = HA(SVB)[i] * (HA(FVHDXE)[i] - DELTD2 * HA(HRVO)[i] * HA(RCY)[i] *
-                                                    HA(HV)[i] * (P[i] - P[s]));
+                                                    HA(H1V)[i] * (P[i] - P[s]));
   HA(UHDYE)[i] = u;
   HA(VHDXE)[i] = v;
   HA(UHE)[i] = u * HA(DYIU)[i];
@@ -852,8 +879,28 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
   const int nb = efdc_grid_blocks(c->n_wet());
   const int nwet = c->n_wet(), lc0 = c->LC - 1;
   const double dt = step->dt, G = c->prm.G;
+  // time-level shifts of EFDC/calpuv2c.f90:256-265 as buffer rotations
+  static const int pairs[6][2] = {{EFDC_UHDYE, EFDC_UHDY1E}, {EFDC_VHDXE, EFDC_VHDX1E},
+                                  {EFDC_HU, EFDC_H1U},       {EFDC_HV, EFDC_H1V},
+                                  {EFDC_HUI, EFDC_H1UI},     {EFDC_HVI, EFDC_H1VI}};
+  Rotated rot;
+  for (int q = 0; q < 6; ++q) {
+    std::swap(c->dH[pairs[q][0]], c->dH[pairs[q][1]]);
+    rot.pair[q][0] = c->dH[pairs[q][0]];
+    rot.pair[q][1] = c->dH[pairs[q][1]];
+  }
+  {
+    double *hp = c->dH[EFDC_HP], *h1p = c->dH[EFDC_H1P], *h2p = c->dH[EFDC_H2P];
+    c->dH[EFDC_H2P] = h1p;
+    c->dH[EFDC_H1P] = hp;
+    c->dH[EFDC_HP] = h2p;
+  }
+  rot.tri[0] = c->dH[EFDC_HP];
+  rot.tri[1] = c->dH[EFDC_H1P];
+  rot.tri[2] = c->dH[EFDC_H2P];
+  rot.lc0 = lc0;
   HPtr h = hptr(c);
-  k_reset_flags<<<1, 1, 0, c->stream>>>(c->st);
+  k_step_begin<<<1, 1, 0, c->stream>>>(c->st, rot);
   if (c->store == EFDC_STORE_FP64)
     TIMED(c, TK_PUV_EXPLICIT, k_puv_explicit<double><<<nb, EFDC_BLOCK, 0, c->stream>>>(
         h, c->dLW, c->dLS, c->dLE, c->dLN, (const double *)c->dK[EFDC_HPK - EFDC_HPK],

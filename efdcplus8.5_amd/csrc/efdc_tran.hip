@@ -232,6 +232,177 @@ __global__ __launch_bounds__(TBLOCK) void k_tran_upwind_vdiff(TranArgs<T> a) {
   }
 }
 
+// Same step as k_tran_upwind_vdiff with one thread per (column,
+// constituent): the workgroup is 64 columns x NCG constituent lanes (a
+// wave per lane, so every wave streams one constituent's plane with
+// coalesced loads while the transport fields it shares with the other
+// lanes come from L1/L2).  VCU of the tridiagonal solve is kept in LDS per
+// lane (KC doubles per thread); the forward values are written to the new
+// concentration plane itself (in the storage type, as the LDS staging of
+// k_tran_upwind_vdiff does) and read back, still cached, by the back
+// substitution.  Arithmetic is expression for expression that of
+// k_tran_upwind_vdiff, so both kernels give identical bits.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tran_col(TranArgs<T> a) {
+  extern __shared__ double vcus[];  // [blockDim.y][KC][64]
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int i = 1 + tile * 64 + (int)threadIdx.x;
+  if (i > a.nwet) return;
+  const int KC = a.KC, nc = a.nc;
+  const int64_t pitch = a.pitch;
+  const int w = a.lw[i], s = a.ls[i];
+  const int e = a.rowmajor ? i + 1 : a.le[i];
+  const int n = a.ln[i];
+  const bool eg = a.rowmajor ? (a.le[i] == a.lc0) : (e == a.lc0);
+  const bool ng = (n == a.lc0);
+  const int w_of_e = eg ? i : (a.rowmajor ? i : a.lw[e]);
+  const int s_of_n = ng ? i : a.ls[n];
+  const double DDELT = a.dt, DELTD2 = a.dt;
+  const double dxyip = a.dxyip[i], hpi = a.hpi[i];
+  const int src0 = a.srcrec[i];
+  const int rec = a.bcrec ? a.bcrec[i] : -1;
+  int4 bi = make_int4(-1, -1, -1, -1);
+  if (rec >= 0) bi = a.bcinfo[rec];
+
+  double *vcu = vcus + (size_t)threadIdx.y * KC * 64 + threadIdx.x;  // stride 64
+  for (int c = threadIdx.y; c < nc; c += blockDim.y) {
+    const T *C1 = a.old[c];
+    T *C = a.nw[c];
+    double fwp = 0.0;  // FWUU(L,K-1), FWUU(L,0)=0
+    double vcu_prev = 0.0, yprev = 0.0;
+    for (int k = 0; k < KC; ++k) {
+      const int K = k + 1;
+      const int64_t o = (int64_t)k * pitch;
+      const double u_i = ld(a.uhdy2, i + o);
+      const double u_e = eg ? 0.0 : ld(a.uhdy2, e + o);
+      const double v_i = ld(a.vhdx2, i + o);
+      const double v_n = ng ? 0.0 : ld(a.vhdx2, n + o);
+      const double ww = (K < KC) ? ld(a.w2, i + (int64_t)K * pitch) : 0.0;
+      const double h1pk = ld(a.h1pk, i + o), hpki = ld(a.hpki, i + o);
+      double cclb = 0.0, eeb = 1.0, vcu_k = 0.0;
+      if (a.vd && KC > 1) {
+        const double ckk = a.dz_uniform ? a.cdzkk_k[k] : ld(a.cdzkk, i + o);
+        const double ckm = a.dz_uniform ? a.cdzkmk_k[k] : ld(a.cdzkmk, i + o);
+        if (K == 1) {
+          const double RCDZKK = -DELTD2 * ckk;
+          const double CCUBTMP = RCDZKK * hpi * ld(a.ab, i + o);
+          const double CCMBTMP = 1.0 - CCUBTMP;
+          eeb = 1.0 / CCMBTMP;
+          vcu_k = CCUBTMP * eeb;
+        } else if (K < KC) {
+          const double RCDZKMK = -DELTD2 * ckm;
+          const double RCDZKK = -DELTD2 * ckk;
+          cclb = RCDZKMK * hpi * ld(a.ab, i + o - pitch);
+          const double CCUBTMP = RCDZKK * hpi * ld(a.ab, i + o);
+          const double CCMBTMP = 1.0 - cclb - CCUBTMP;
+          eeb = 1.0 / (CCMBTMP - cclb * vcu_prev);
+          vcu_k = CCUBTMP * eeb;
+        } else {
+          const double RCDZKMK = -DELTD2 * ckm;
+          cclb = RCDZKMK * hpi * ld(a.ab, i + o - pitch);
+          const double CCMBTMP = 1.0 - cclb;
+          eeb = 1.0 / (CCMBTMP - cclb * vcu_prev);
+        }
+      }
+      vcu[k * 64] = vcu_k;
+      const double c1 = ld(C1, i + o);
+      const double cw = ld(C1, w + o);
+      const double cs = ld(C1, s + o);
+      // upwind fluxes (LUPU/LUPV/KUPW of EFDC/calconc.f90:95-118)
+      const double fu_i = u_i * (u_i >= 0.0 ? cw : c1);
+      const double fv_i = v_i * (v_i >= 0.0 ? cs : c1);
+      double fu_e = 0.0, fv_n = 0.0;
+      if (!eg) fu_e = u_e * (u_e >= 0.0 ? ld(C1, w_of_e + o) : ld(C1, e + o));
+      if (!ng) fv_n = v_n * (v_n >= 0.0 ? ld(C1, s_of_n + o) : ld(C1, n + o));
+      double fw = 0.0;
+      if (K < KC) fw = ww * (ww >= 0.0 ? c1 : ld(C1, i + o + pitch));
+      // sources, EFDC/calfqc.f90:211-224
+      double fqc = 0.0;
+      for (int q = src0; q >= 0; q = a.srcnext[q]) {
+        const double conq = 0.5 * (3. * c1 - c1);
+        const size_t j = (size_t)q * KC + k;
+        const double qv = a.qss[j], qs = a.qsercell[j];
+        fqc = fqc + dmx(qv, 0.) * a.cqs[j * nc + c] + dmn(qv, 0.) * conq +
+              dmx(qs, 0.) * a.csertq[j * nc + c] + dmn(qs, 0.) * conq;
+      }
+      // EFDC/caltran.f90:194-196,243
+      const double cd = c1 * h1pk + DDELT * ((fqc + fu_i - fu_e + fv_i - fv_n) * dxyip +
+                                             (fwp - fw));
+      double con = cd * hpki;
+      fwp = fw;
+      if (rec >= 0) {
+        // open boundaries S, W, E, N (EFDC/caltran.f90:319-471)
+        double cv1 = c1;
+        const int LLs[4] = {bi.x, bi.y, bi.z, bi.w};
+        for (int side = 0; side < 4; ++side) {
+          const int LL = LLs[side];
+          if (LL < 0) continue;
+          const CbcDev &b = a.cbc[side];
+          double flow, flux;
+          bool out;
+          if (side == 0) { flow = v_n; flux = fv_n; out = flow <= 0.; }
+          else if (side == 1) { flow = u_e; flux = fu_e; out = flow <= 0.; }
+          else if (side == 2) { flow = u_i; flux = fu_i; out = flow >= 0.; }
+          else { flow = v_i; flux = fv_i; out = flow >= 0.; }
+          const size_t m = ((size_t)LL * KC + k) * nc + c;
+          const double cb1 = b.cb[((size_t)LL * 2 + 0) * nc + c];
+          const double cb2 = b.cb[((size_t)LL * 2 + 1) * nc + c];
+          if (out) {
+            double CTMP;
+            if (side <= 1) CTMP = cv1 + DDELT * (flow * cv1 - flux) * dxyip * hpki;
+            else CTMP = cv1 + DDELT * (flux - flow * cv1) * dxyip * hpki;
+            con = dmx(CTMP, 0.);
+            if (a.mo[c] == 1) {
+              const double cbtmp = cb1 + b.csert[((size_t)LL * KC + 0) * nc + c];
+              if (con > cbtmp) con = cbtmp;
+            }
+            b.clo[m] = con;
+            b.nlo[m] = a.niter;
+          } else {
+            const double CBT = a.wtci[k * 2 + 0] * cb1 + a.wtci[k * 2 + 1] * cb2 + b.csert[m];
+            const long long nmnlod = a.niter - b.nlo[m];
+            if (nmnlod >= b.ntscr[LL]) {
+              con = CBT;
+            } else {
+              const double cbtmp =
+                  b.clo[m] + (CBT - b.clo[m]) * (double)nmnlod / (double)b.ntscr[LL];
+              con = dmx(cbtmp, 0.);
+            }
+          }
+          cv1 = con;
+        }
+        a.bc_con1[((size_t)rec * KC + k) * nc + c] = con;
+      }
+      if (!a.vd) {
+        a.nw[c][i + o] = (T)con;
+        continue;
+      }
+      // forward sweep of the tridiagonal solve (FP64), forward values
+      // kept in the storage type
+      double y;
+      if (KC == 1) y = con;
+      else if (K == 1) y = con * eeb;
+      else y = (con - cclb * yprev) * eeb;
+      const T yt = (T)y;
+      C[i + o] = yt;
+      yprev = (double)yt;
+      vcu_prev = vcu_k;
+    }
+    if (!a.vd) continue;
+    // back substitution and dye decay, write CON
+    const int dm = a.dye_mode[c];
+    const double f = a.dye_f[c];
+    auto fin = [&](double v) { return dm == 1 ? f * v : (dm == 2 ? v + f : v); };
+    double up = (double)C[i + (int64_t)(KC - 1) * pitch];
+    C[i + (int64_t)(KC - 1) * pitch] = (T)fin(up);
+    for (int k = KC - 2; k >= 0; --k) {
+      const double v = (double)C[i + (int64_t)k * pitch] - vcu[k * 64] * up;
+      C[i + (int64_t)k * pitch] = (T)fin(v);
+      up = v;
+    }
+  }
+}
+
 // CON1 at open-boundary columns takes the boundary value (ISUD=1,
 // EFDC/caltran.f90:356,393,430,468).
 template <typename T>
@@ -725,6 +896,18 @@ int adac_pipeline(efdc_ctx *c, TranArgs<T> &ta, const efdc_step_t *step) {
   return EFDC_OK;
 }
 
+// CON1 of the open-boundary columns, then the CON/CON1 buffer swap.
+template <typename T>
+int finish_tran(efdc_ctx *c, TranArgs<T> &a, const efdc_step_t *) {
+  if (c->nbcrec > 0) {
+    const int tot = c->nbcrec * c->KC;
+    TIMED(c, TK_TRAN_BC, k_tran_bc_con1<T><<<(tot + 255) / 256, 256, 0, c->stream>>>(a, c->bccell,
+                                                                c->nbcrec));
+  }
+  for (int j = 0; j < c->nconst; ++j) c->con[j].cur ^= 1;
+  return EFDC_OK;
+}
+
 template <typename T>
 int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
   const int nc = c->nconst, KC = c->KC;
@@ -799,6 +982,19 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
   a.bc_con1 = c->bc_con1;
   a.vd = any_adac ? 0 : 1;
 
+  if (KC <= 64) {
+    // one thread per (column, constituent)
+    const int ncg = nc < 4 ? nc : 4;
+    const dim3 blk(64, ncg);
+    const int nbc = (a.nwet + 63) / 64;
+    const size_t lds = sizeof(double) * (size_t)ncg * KC * 64;
+    TIMED(c, TK_TRAN, k_tran_col<T><<<nbc, blk, lds, c->stream>>>(a));
+    if (any_adac) {
+      int s = adac_pipeline<T>(c, a, step);
+      if (s) return s;
+    }
+    return finish_tran(c, a, step);
+  }
   // LDS per block: VCU and the FWUU carry (FP64) plus the forward values
   // (storage type); 128 columns per block when that fits in 64 KiB
   auto lds_for = [&](int tb) {
@@ -817,13 +1013,7 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
     int s = adac_pipeline<T>(c, a, step);
     if (s) return s;
   }
-  if (c->nbcrec > 0) {
-    const int tot = c->nbcrec * KC;
-    TIMED(c, TK_TRAN_BC, k_tran_bc_con1<T><<<(tot + 255) / 256, 256, 0, c->stream>>>(a, c->bccell,
-                                                                c->nbcrec));
-  }
-  for (int j = 0; j < nc; ++j) c->con[j].cur ^= 1;
-  return EFDC_OK;
+  return finish_tran(c, a, step);
 }
 
 }  // namespace

@@ -107,7 +107,9 @@ def load(path: Optional[str] = None):
     global _lib
     if _lib is not None:
         return _lib
-    path = path or LIB_PATH
+    # EFDC_HIP_LIB selects another build of the same library (for example
+    # an instrumented one); the default is the in-tree build
+    path = path or os.environ.get("EFDC_HIP_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise RuntimeError(
             f"{path} is missing: build it with __graft_entry__.build() "
