@@ -54,6 +54,11 @@ constexpr int HMAX = 2 * WIN;     // outside cells of one workgroup, at most
 constexpr int NVMAX = 3;          // partials per exchange, at most
 constexpr int HX_WORDS = 4;       // 4-byte words per cell and parity in the halo slots
 
+// Halo slots are planes (one per value and parity), so that the stores of
+// a wave and the loads of a run of outside cells are contiguous: FP32
+// value q of cell i in parity p at word (p*HX_WORDS + q)*ncell + i, FP64
+// value q (two words) at double (p*2 + q)*ncell + i.
+
 typedef unsigned long long u64;
 
 struct Bar {
@@ -115,7 +120,7 @@ __device__ __forceinline__ double *pslot(double *ps, unsigned tag, int j) {
 
 struct Xch {
   double *ps;          // partials
-  unsigned *hx;        // halo slots [2][ncell][HX_WORDS]
+  unsigned *hx;        // halo slots, [2][HX_WORDS][ncell] words
   unsigned *flag;      // one word per workgroup, TICKET_PAD apart
   int ncell;
 };
@@ -182,23 +187,26 @@ __device__ bool exchange(const Xch &x, unsigned tag, const int *hl, int nhalo,
     // first result is stored (a store per load would serialise the round
     // trips)
     constexpr int HB = 4;
-    const unsigned *hxp = x.hx + (size_t)(tag & 1u) * x.ncell * HX_WORDS;
+    const size_t par = tag & 1u, nce = (size_t)x.ncell;
+    const unsigned *w32 = x.hx + par * HX_WORDS * nce;        // FP32 planes
+    const unsigned *w64 = x.hx + par * 2 * 2 * nce;           // FP64 planes
     for (int h0 = t; h0 < nhalo; h0 += HB * PB) {
       u64 wv[HB][2];
+      unsigned wz[HB];
 #pragma unroll
       for (int u = 0; u < HB; ++u) {
         const int h = h0 + u * PB;
         wv[u][0] = wv[u][1] = 0;
+        wz[u] = 0;
         if (h < nhalo) {
-          const unsigned *q = hxp + (size_t)hl[h] * HX_WORDS;
+          const size_t j = (size_t)hl[h];
           if (NR == 4) {
-            wv[u][0] = get_u64(q);
-            wv[u][1] = get_u64(q + 2);
-          } else if (NR >= 2) {
-            wv[u][0] = get_u64(q);
-            if (NR == 3) wv[u][1] = get_u32(q + 2);
+            wv[u][0] = get_u64(w64 + 2 * j);
+            wv[u][1] = get_u64(w64 + 2 * (nce + j));
           } else {
-            wv[u][0] = get_u32(q);
+            wv[u][0] = get_u32(w32 + j);
+            if (NR >= 2) wv[u][1] = get_u32(w32 + nce + j);
+            if (NR >= 3) wz[u] = get_u32(w32 + 2 * nce + j);
           }
         }
       }
@@ -206,10 +214,16 @@ __device__ bool exchange(const Xch &x, unsigned tag, const int *hl, int nhalo,
       for (int u = 0; u < HB; ++u) {
         const int h = h0 + u * PB;
         if (h < nhalo) {
-          raw[h] = (unsigned)wv[u][0];
-          if (NR >= 2) raw[HMAX + h] = (unsigned)(wv[u][0] >> 32);
-          if (NR >= 3) raw[2 * HMAX + h] = (unsigned)wv[u][1];
-          if (NR == 4) raw[3 * HMAX + h] = (unsigned)(wv[u][1] >> 32);
+          if (NR == 4) {
+            raw[h] = (unsigned)wv[u][0];
+            raw[HMAX + h] = (unsigned)(wv[u][0] >> 32);
+            raw[2 * HMAX + h] = (unsigned)wv[u][1];
+            raw[3 * HMAX + h] = (unsigned)(wv[u][1] >> 32);
+          } else {
+            raw[h] = (unsigned)wv[u][0];
+            if (NR >= 2) raw[HMAX + h] = (unsigned)wv[u][1];
+            if (NR >= 3) raw[2 * HMAX + h] = wz[u];
+          }
         }
       }
     }
@@ -321,9 +335,10 @@ __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
   auto real = [&](int j) { return j > 0 && j != lc0; };
   auto slot = [&](int j) { return j - b0 + WIN; };
   auto publish_rp = [&](int i, unsigned tag, double rv, double pv) {
-    unsigned *q = a.xc.hx + ((size_t)(tag & 1u) * a.xc.ncell + (size_t)opaque(i)) * HX_WORDS;
-    put_f64(q, rv);
-    put_f64(q + 2, pv);
+    const size_t nce = (size_t)a.xc.ncell, j = (size_t)opaque(i);
+    unsigned *q = a.xc.hx + (size_t)(tag & 1u) * 2 * 2 * nce;
+    put_f64(q + 2 * j, rv);
+    put_f64(q + 2 * (nce + j), pv);
   };
 
   for (int g = threadIdx.x; g < 2 * WIN; g += PB) hl[g] = 0;
@@ -515,8 +530,10 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
   auto slot = [&](int j) { return j - b0 + WIN; };
   auto dinv = [&](int j) { return 1.0 / sqrt(a.ccc[j]); };
   auto rval = [&](int j) { return real(j) ? lds[slot(j)] : 0.0f; };
-  auto hxw = [&](int i, unsigned tag) {
-    return a.xc.hx + ((size_t)(tag & 1u) * a.xc.ncell + (size_t)opaque(i)) * HX_WORDS;
+  // FP32 halo value q of cell i in the slots of exchange `tag`
+  auto hxw = [&](int i, unsigned tag, int q) {
+    return a.xc.hx + ((size_t)(tag & 1u) * HX_WORDS + q) * (size_t)a.xc.ncell +
+           (size_t)opaque(i);
   };
 
   for (int g = threadIdx.x; g < 2 * WIN; g += PB) hl[g] = 0;
@@ -597,7 +614,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
         v3[0] += rr * rr;
         v3[1] += (double)rs * (double)rs;
         lds[slot(io)] = rs;
-        if (expo & (1u << k)) put_u32(hxw(io, tag), __float_as_uint(rs));
+        if (expo & (1u << k)) put_u32(hxw(io, tag, 0), __float_as_uint(rs));
       }
     }
     fp2 = 0.0;
@@ -631,10 +648,9 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
         v2[0] += (double)r[k] * (double)r[k];
         v2[1] += (double)w[k] * (double)r[k];
         if (expo & (1u << k)) {
-          unsigned *q = hxw(i, tag);
-          put_u32(q, __float_as_uint(r[k]));
-          put_u32(q + 1, __float_as_uint(w[k]));
-          put_u32(q + 2, __float_as_uint(0.0f));
+          put_u32(hxw(i, tag, 0), __float_as_uint(r[k]));
+          put_u32(hxw(i, tag, 1), __float_as_uint(w[k]));
+          put_u32(hxw(i, tag, 2), __float_as_uint(0.0f));
         }
       }
     }
@@ -687,10 +703,9 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
           v[0] += (double)r[k] * (double)r[k];
           v[1] += (double)w[k] * (double)r[k];
           if (expo & (1u << k)) {
-            unsigned *q = hxw(i, tag);
-            put_u32(q, __float_as_uint(r[k]));
-            put_u32(q + 1, __float_as_uint(w[k]));
-            put_u32(q + 2, __float_as_uint(s[k]));
+            put_u32(hxw(i, tag, 0), __float_as_uint(r[k]));
+            put_u32(hxw(i, tag, 1), __float_as_uint(w[k]));
+            put_u32(hxw(i, tag, 2), __float_as_uint(s[k]));
           }
         }
       }

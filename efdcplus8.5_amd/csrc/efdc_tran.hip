@@ -1,19 +1,25 @@
 // efdc_tran.hip -- CALCONC(ISTL=2, IS2TL=1) on gfx950.
 //
-// One thread per water column.  The kernel walks K = 1..KC once; at every
-// layer it loads the shared transport fields (UHDY2, VHDX2, W2, H1PK,
-// HPKI, AB) once for all constituents, forms the upwind update of
-// CALTRAN (EFDC/caltran.f90:95-111,188-245) with the CALFQC sources
-// (EFDC/calfqc.f90:209-225), applies the open-boundary sequence
-// (EFDC/caltran.f90:316-471) on boundary columns, and runs the forward
-// sweep of the implicit vertical diffusion (EFDC/calconc.f90:264-422) in
-// FP64.  The forward values are staged in LDS; the back substitution
-// (EFDC/calconc.f90:425-468) and the dye decay (EFDC/calconc.f90:581-616)
-// then write the new concentrations once.  CON and CON1 are two buffers
-// whose roles swap every step (CON1 = CON of EFDC/caltran.f90:59-65 costs
-// nothing).  HBM traffic per column-step is
-//   KC*s*(5 flows + 1 AB + 2*NC) + maps/metrics,
-// the roofline of this kernel.
+// The transport step of every constituent in one pass over the water
+// columns: at every layer the upwind update of CALTRAN
+// (EFDC/caltran.f90:95-111,188-245) with the CALFQC sources
+// (EFDC/calfqc.f90:209-225), the open-boundary sequence
+// (EFDC/caltran.f90:316-471) on boundary columns, and the forward sweep of
+// the implicit vertical diffusion (EFDC/calconc.f90:264-422) in FP64; the
+// back substitution (EFDC/calconc.f90:425-468) and the dye decay
+// (EFDC/calconc.f90:581-616) then write the new concentrations.  CON and
+// CON1 are two buffers whose roles swap every step (CON1 = CON of
+// EFDC/caltran.f90:59-65 costs nothing).
+//
+// k_tran_col (the default) runs one thread per (column, constituent): a
+// workgroup is 64 columns x up to 4 constituent lanes, one wave per lane,
+// so every wave streams one constituent plane with coalesced loads while
+// the shared transport fields come from L1/L2; each layer's loads are
+// issued before the previous layer is computed.  k_tran_upwind_vdiff (one
+// thread per column, all constituents, LDS staging) remains for more than
+// 64 layers.  HBM traffic per column-step is
+//   KC*s*(3 flows + H1PK + HPKI + AB + 2*NC) + maps/metrics,
+// the roofline of this step.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -242,6 +248,89 @@ __global__ __launch_bounds__(TBLOCK) void k_tran_upwind_vdiff(TranArgs<T> a) {
 // k_tran_upwind_vdiff does) and read back, still cached, by the back
 // substitution.  Arithmetic is expression for expression that of
 // k_tran_upwind_vdiff, so both kernels give identical bits.
+// Inputs of one layer of one column for one constituent.  The upwind
+// donor values are loaded for both flow signs and selected afterwards, so
+// that all loads of a layer are independent of each other and of the
+// previous layer (the kernel issues layer K+1's loads before computing
+// layer K).
+template <typename T>
+struct LayerIn {
+  T u_i, u_e, v_i, v_n, ww, h1pk, hpki, ab;  // storage type: exact
+  double ckk, ckm;
+  T c1, cw, cs, ce_w, ce, cn_s, cn, cup;
+};
+
+// Source term FQC of one layer and constituent, EFDC/calfqc.f90:211-224
+// (a separate function keeps the column loop compact).
+template <typename T>
+__device__ __forceinline__ double col_sources(const TranArgs<T> &a, int src0, int k, int c,
+                                           double c1) {
+  const int KC = a.KC, nc = a.nc;
+  double fqc = 0.0;
+  for (int q = src0; q >= 0; q = a.srcnext[q]) {
+    const double conq = 0.5 * (3. * c1 - c1);
+    const size_t j = (size_t)q * KC + k;
+    const double qv = a.qss[j], qs = a.qsercell[j];
+    fqc = fqc + dmx(qv, 0.) * a.cqs[j * nc + c] + dmn(qv, 0.) * conq +
+          dmx(qs, 0.) * a.csertq[j * nc + c] + dmn(qs, 0.) * conq;
+  }
+  return fqc;
+}
+
+// Open-boundary sequence S, W, E, N of one layer and constituent,
+// EFDC/caltran.f90:319-471 (a separate function keeps the column loop
+// compact).  Returns the new concentration.
+template <typename T>
+__device__ __forceinline__ double col_open_bc(const TranArgs<T> &a, int4 bi, int rec, int k,
+                                           int c, double c1, double con, double u_i,
+                                           double u_e, double v_i, double v_n, double fu_i,
+                                           double fu_e, double fv_i, double fv_n,
+                                           double dxyip, double hpki) {
+  const int KC = a.KC, nc = a.nc;
+  const double DDELT = a.dt;
+  double cv1 = c1;
+  const int LLs[4] = {bi.x, bi.y, bi.z, bi.w};
+  for (int side = 0; side < 4; ++side) {
+    const int LL = LLs[side];
+    if (LL < 0) continue;
+    const CbcDev &b = a.cbc[side];
+    double flow, flux;
+    bool out;
+    if (side == 0) { flow = v_n; flux = fv_n; out = flow <= 0.; }
+    else if (side == 1) { flow = u_e; flux = fu_e; out = flow <= 0.; }
+    else if (side == 2) { flow = u_i; flux = fu_i; out = flow >= 0.; }
+    else { flow = v_i; flux = fv_i; out = flow >= 0.; }
+    const size_t m = ((size_t)LL * KC + k) * nc + c;
+    const double cb1 = b.cb[((size_t)LL * 2 + 0) * nc + c];
+    const double cb2 = b.cb[((size_t)LL * 2 + 1) * nc + c];
+    if (out) {
+      double CTMP;
+      if (side <= 1) CTMP = cv1 + DDELT * (flow * cv1 - flux) * dxyip * hpki;
+      else CTMP = cv1 + DDELT * (flux - flow * cv1) * dxyip * hpki;
+      con = dmx(CTMP, 0.);
+      if (a.mo[c] == 1) {
+        const double cbtmp = cb1 + b.csert[((size_t)LL * KC + 0) * nc + c];
+        if (con > cbtmp) con = cbtmp;
+      }
+      b.clo[m] = con;
+      b.nlo[m] = a.niter;
+    } else {
+      const double CBT = a.wtci[k * 2 + 0] * cb1 + a.wtci[k * 2 + 1] * cb2 + b.csert[m];
+      const long long nmnlod = a.niter - b.nlo[m];
+      if (nmnlod >= b.ntscr[LL]) {
+        con = CBT;
+      } else {
+        const double cbtmp =
+            b.clo[m] + (CBT - b.clo[m]) * (double)nmnlod / (double)b.ntscr[LL];
+        con = dmx(cbtmp, 0.);
+      }
+    }
+    cv1 = con;
+  }
+  a.bc_con1[((size_t)rec * KC + k) * nc + c] = con;
+  return con;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_tran_col(TranArgs<T> a) {
   extern __shared__ double vcus[];  // [blockDim.y][KC][64]
@@ -249,7 +338,10 @@ __global__ __launch_bounds__(256) void k_tran_col(TranArgs<T> a) {
   const int i = 1 + tile * 64 + (int)threadIdx.x;
   if (i > a.nwet) return;
   const int KC = a.KC, nc = a.nc;
-  const int64_t pitch = a.pitch;
+  // 32-bit element offsets (the host checks that every layer array has
+  // fewer than 2^31 elements), so the loads use a scalar base plus a
+  // 32-bit vector offset instead of a 64-bit address per pointer
+  const unsigned pitch = (unsigned)a.pitch;
   const int w = a.lw[i], s = a.ls[i];
   const int e = a.rowmajor ? i + 1 : a.le[i];
   const int n = a.ln[i];
@@ -257,124 +349,101 @@ __global__ __launch_bounds__(256) void k_tran_col(TranArgs<T> a) {
   const bool ng = (n == a.lc0);
   const int w_of_e = eg ? i : (a.rowmajor ? i : a.lw[e]);
   const int s_of_n = ng ? i : a.ls[n];
+  const unsigned ui = (unsigned)i, uw = (unsigned)w, us = (unsigned)s, ue = (unsigned)e,
+                 un = (unsigned)n, uwe = (unsigned)w_of_e, usn = (unsigned)s_of_n;
   const double DDELT = a.dt, DELTD2 = a.dt;
   const double dxyip = a.dxyip[i], hpi = a.hpi[i];
   const int src0 = a.srcrec[i];
   const int rec = a.bcrec ? a.bcrec[i] : -1;
   int4 bi = make_int4(-1, -1, -1, -1);
   if (rec >= 0) bi = a.bcinfo[rec];
+  const bool vdk = a.vd && KC > 1;
 
   double *vcu = vcus + (size_t)threadIdx.y * KC * 64 + threadIdx.x;  // stride 64
   for (int c = threadIdx.y; c < nc; c += blockDim.y) {
-    const T *C1 = a.old[c];
-    T *C = a.nw[c];
+    const T *__restrict__ C1 = a.old[c];
+    T *__restrict__ C = a.nw[c];
+    auto load = [&](int k, LayerIn<T> &L) {
+      const unsigned o = (unsigned)k * pitch;
+      const int K = k + 1;
+      L.u_i = a.uhdy2[ui + o];
+      L.u_e = eg ? (T)0 : a.uhdy2[ue + o];
+      L.v_i = a.vhdx2[ui + o];
+      L.v_n = ng ? (T)0 : a.vhdx2[un + o];
+      L.ww = (K < KC) ? a.w2[ui + o + pitch] : (T)0;
+      L.h1pk = a.h1pk[ui + o];
+      L.hpki = a.hpki[ui + o];
+      L.ab = vdk && K < KC ? a.ab[ui + o] : (T)0;
+      L.ckk = vdk ? (a.dz_uniform ? a.cdzkk_k[k] : (double)a.cdzkk[ui + o]) : 0.0;
+      L.ckm = vdk ? (a.dz_uniform ? a.cdzkmk_k[k] : (double)a.cdzkmk[ui + o]) : 0.0;
+      L.c1 = C1[ui + o];
+      L.cw = C1[uw + o];
+      L.cs = C1[us + o];
+      L.ce_w = eg ? (T)0 : C1[uwe + o];
+      L.ce = eg ? (T)0 : C1[ue + o];
+      L.cn_s = ng ? (T)0 : C1[usn + o];
+      L.cn = ng ? (T)0 : C1[un + o];
+      L.cup = (K < KC) ? C1[ui + o + pitch] : (T)0;
+    };
+    LayerIn<T> cur, nxt;
+    load(0, cur);
     double fwp = 0.0;  // FWUU(L,K-1), FWUU(L,0)=0
-    double vcu_prev = 0.0, yprev = 0.0;
+    double vcu_prev = 0.0, yprev = 0.0, ab_prev = 0.0;
     for (int k = 0; k < KC; ++k) {
       const int K = k + 1;
-      const int64_t o = (int64_t)k * pitch;
-      const double u_i = ld(a.uhdy2, i + o);
-      const double u_e = eg ? 0.0 : ld(a.uhdy2, e + o);
-      const double v_i = ld(a.vhdx2, i + o);
-      const double v_n = ng ? 0.0 : ld(a.vhdx2, n + o);
-      const double ww = (K < KC) ? ld(a.w2, i + (int64_t)K * pitch) : 0.0;
-      const double h1pk = ld(a.h1pk, i + o), hpki = ld(a.hpki, i + o);
+      const unsigned o = (unsigned)k * pitch;
+      if (K < KC) load(k + 1, nxt);
+      const double u_i = (double)cur.u_i, u_e = (double)cur.u_e, v_i = (double)cur.v_i,
+                   v_n = (double)cur.v_n;
+      const double ww = (double)cur.ww, h1pk = (double)cur.h1pk, hpki = (double)cur.hpki;
       double cclb = 0.0, eeb = 1.0, vcu_k = 0.0;
-      if (a.vd && KC > 1) {
-        const double ckk = a.dz_uniform ? a.cdzkk_k[k] : ld(a.cdzkk, i + o);
-        const double ckm = a.dz_uniform ? a.cdzkmk_k[k] : ld(a.cdzkmk, i + o);
+      if (vdk) {
+        const double ckk = cur.ckk, ckm = cur.ckm;
         if (K == 1) {
           const double RCDZKK = -DELTD2 * ckk;
-          const double CCUBTMP = RCDZKK * hpi * ld(a.ab, i + o);
+          const double CCUBTMP = RCDZKK * hpi * (double)cur.ab;
           const double CCMBTMP = 1.0 - CCUBTMP;
           eeb = 1.0 / CCMBTMP;
           vcu_k = CCUBTMP * eeb;
         } else if (K < KC) {
           const double RCDZKMK = -DELTD2 * ckm;
           const double RCDZKK = -DELTD2 * ckk;
-          cclb = RCDZKMK * hpi * ld(a.ab, i + o - pitch);
-          const double CCUBTMP = RCDZKK * hpi * ld(a.ab, i + o);
+          cclb = RCDZKMK * hpi * ab_prev;
+          const double CCUBTMP = RCDZKK * hpi * (double)cur.ab;
           const double CCMBTMP = 1.0 - cclb - CCUBTMP;
           eeb = 1.0 / (CCMBTMP - cclb * vcu_prev);
           vcu_k = CCUBTMP * eeb;
         } else {
           const double RCDZKMK = -DELTD2 * ckm;
-          cclb = RCDZKMK * hpi * ld(a.ab, i + o - pitch);
+          cclb = RCDZKMK * hpi * ab_prev;
           const double CCMBTMP = 1.0 - cclb;
           eeb = 1.0 / (CCMBTMP - cclb * vcu_prev);
         }
+        vcu[k * 64] = vcu_k;
       }
-      vcu[k * 64] = vcu_k;
-      const double c1 = ld(C1, i + o);
-      const double cw = ld(C1, w + o);
-      const double cs = ld(C1, s + o);
+      const double c1 = (double)cur.c1;
       // upwind fluxes (LUPU/LUPV/KUPW of EFDC/calconc.f90:95-118)
-      const double fu_i = u_i * (u_i >= 0.0 ? cw : c1);
-      const double fv_i = v_i * (v_i >= 0.0 ? cs : c1);
+      const double fu_i = u_i * (u_i >= 0.0 ? (double)cur.cw : c1);
+      const double fv_i = v_i * (v_i >= 0.0 ? (double)cur.cs : c1);
       double fu_e = 0.0, fv_n = 0.0;
-      if (!eg) fu_e = u_e * (u_e >= 0.0 ? ld(C1, w_of_e + o) : ld(C1, e + o));
-      if (!ng) fv_n = v_n * (v_n >= 0.0 ? ld(C1, s_of_n + o) : ld(C1, n + o));
+      if (!eg) fu_e = u_e * (u_e >= 0.0 ? (double)cur.ce_w : (double)cur.ce);
+      if (!ng) fv_n = v_n * (v_n >= 0.0 ? (double)cur.cn_s : (double)cur.cn);
       double fw = 0.0;
-      if (K < KC) fw = ww * (ww >= 0.0 ? c1 : ld(C1, i + o + pitch));
+      if (K < KC) fw = ww * (ww >= 0.0 ? c1 : (double)cur.cup);
       // sources, EFDC/calfqc.f90:211-224
-      double fqc = 0.0;
-      for (int q = src0; q >= 0; q = a.srcnext[q]) {
-        const double conq = 0.5 * (3. * c1 - c1);
-        const size_t j = (size_t)q * KC + k;
-        const double qv = a.qss[j], qs = a.qsercell[j];
-        fqc = fqc + dmx(qv, 0.) * a.cqs[j * nc + c] + dmn(qv, 0.) * conq +
-              dmx(qs, 0.) * a.csertq[j * nc + c] + dmn(qs, 0.) * conq;
-      }
+      const double fqc = src0 >= 0 ? col_sources(a, src0, k, c, c1) : 0.0;
       // EFDC/caltran.f90:194-196,243
       const double cd = c1 * h1pk + DDELT * ((fqc + fu_i - fu_e + fv_i - fv_n) * dxyip +
                                              (fwp - fw));
       double con = cd * hpki;
       fwp = fw;
-      if (rec >= 0) {
-        // open boundaries S, W, E, N (EFDC/caltran.f90:319-471)
-        double cv1 = c1;
-        const int LLs[4] = {bi.x, bi.y, bi.z, bi.w};
-        for (int side = 0; side < 4; ++side) {
-          const int LL = LLs[side];
-          if (LL < 0) continue;
-          const CbcDev &b = a.cbc[side];
-          double flow, flux;
-          bool out;
-          if (side == 0) { flow = v_n; flux = fv_n; out = flow <= 0.; }
-          else if (side == 1) { flow = u_e; flux = fu_e; out = flow <= 0.; }
-          else if (side == 2) { flow = u_i; flux = fu_i; out = flow >= 0.; }
-          else { flow = v_i; flux = fv_i; out = flow >= 0.; }
-          const size_t m = ((size_t)LL * KC + k) * nc + c;
-          const double cb1 = b.cb[((size_t)LL * 2 + 0) * nc + c];
-          const double cb2 = b.cb[((size_t)LL * 2 + 1) * nc + c];
-          if (out) {
-            double CTMP;
-            if (side <= 1) CTMP = cv1 + DDELT * (flow * cv1 - flux) * dxyip * hpki;
-            else CTMP = cv1 + DDELT * (flux - flow * cv1) * dxyip * hpki;
-            con = dmx(CTMP, 0.);
-            if (a.mo[c] == 1) {
-              const double cbtmp = cb1 + b.csert[((size_t)LL * KC + 0) * nc + c];
-              if (con > cbtmp) con = cbtmp;
-            }
-            b.clo[m] = con;
-            b.nlo[m] = a.niter;
-          } else {
-            const double CBT = a.wtci[k * 2 + 0] * cb1 + a.wtci[k * 2 + 1] * cb2 + b.csert[m];
-            const long long nmnlod = a.niter - b.nlo[m];
-            if (nmnlod >= b.ntscr[LL]) {
-              con = CBT;
-            } else {
-              const double cbtmp =
-                  b.clo[m] + (CBT - b.clo[m]) * (double)nmnlod / (double)b.ntscr[LL];
-              con = dmx(cbtmp, 0.);
-            }
-          }
-          cv1 = con;
-        }
-        a.bc_con1[((size_t)rec * KC + k) * nc + c] = con;
-      }
+      if (rec >= 0)
+        con = col_open_bc(a, bi, rec, k, c, c1, con, u_i, u_e, v_i, v_n, fu_i, fu_e, fv_i,
+                          fv_n, dxyip, hpki);
+      ab_prev = (double)cur.ab;
+      cur = nxt;
       if (!a.vd) {
-        a.nw[c][i + o] = (T)con;
+        C[ui + o] = (T)con;
         continue;
       }
       // forward sweep of the tridiagonal solve (FP64), forward values
@@ -384,7 +453,7 @@ __global__ __launch_bounds__(256) void k_tran_col(TranArgs<T> a) {
       else if (K == 1) y = con * eeb;
       else y = (con - cclb * yprev) * eeb;
       const T yt = (T)y;
-      C[i + o] = yt;
+      C[ui + o] = yt;
       yprev = (double)yt;
       vcu_prev = vcu_k;
     }
@@ -393,11 +462,12 @@ __global__ __launch_bounds__(256) void k_tran_col(TranArgs<T> a) {
     const int dm = a.dye_mode[c];
     const double f = a.dye_f[c];
     auto fin = [&](double v) { return dm == 1 ? f * v : (dm == 2 ? v + f : v); };
-    double up = (double)C[i + (int64_t)(KC - 1) * pitch];
-    C[i + (int64_t)(KC - 1) * pitch] = (T)fin(up);
+    double up = (double)C[ui + (unsigned)(KC - 1) * pitch];
+    C[ui + (unsigned)(KC - 1) * pitch] = (T)fin(up);
     for (int k = KC - 2; k >= 0; --k) {
-      const double v = (double)C[i + (int64_t)k * pitch] - vcu[k * 64] * up;
-      C[i + (int64_t)k * pitch] = (T)fin(v);
+      const unsigned o = (unsigned)k * pitch;
+      const double v = (double)C[ui + o] - vcu[k * 64] * up;
+      C[ui + o] = (T)fin(v);
       up = v;
     }
   }
@@ -982,7 +1052,8 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
   a.bc_con1 = c->bc_con1;
   a.vd = any_adac ? 0 : 1;
 
-  if (KC <= 64) {
+  const bool off32 = (int64_t)c->pitch * (c->KC + 1) < ((int64_t)1 << 31);
+  if (KC <= 64 && off32) {
     // one thread per (column, constituent)
     const int ncg = nc < 4 ? nc : 4;
     const dim3 blk(64, ncg);
