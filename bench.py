@@ -211,6 +211,142 @@ def transport_bench(args):
     print(json.dumps(out))
 
 
+def ensemble_bench(args):
+    """Batched ensemble (--workload ensemble): 256 independent basins,
+    (nx, ny) log-uniform in 64x64 .. 512x256, irregular lake masks, depths
+    2-30 m, Manning n 0.015-0.04, tidal amplitude 0.1-1.0 m on a west
+    boundary segment, KC=8, salinity and a dye source each, concatenated in
+    one context with a basin table.  One step = the full hot path for every
+    basin (closure, CALPUV2C with a per-basin FP64 CG to a relative residual
+    of 1e-10, closure, CALCONC); value = column-steps/s over all basins.
+    Beside it: the same basins as separate contexts stepped one after the
+    other (each with its best single-model CG), on a bounded sample of steps,
+    and the FP64 oracle on the first basins (CPU baseline)."""
+    import numpy as np
+    from efdc_amd import grids, lib
+
+    kc = args.ekc
+    t0 = time.perf_counter()
+    case, first, members = grids.config3(args.nbasins, kc)
+    t_gen = time.perf_counter() - t0
+    g = case.grid
+    nwet = g.nwet
+    nc = len(case.consts)
+    prm = dict(cg_rel_tol=1e-10, RSQM=0.0, ITERM=2000, cg_mode=lib.CG_FP64)
+    ctx = lib.Context(g, **prm).upload_case(case)
+    ctx.set_segments(first)
+
+    # ---- sequential single-basin contexts on a bounded sample of steps
+    nseq = min(args.seq_steps, args.steps)
+    singles = [lib.Context(m.grid, **prm).upload_case(m) for m in members]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for s_, m in zip(singles, members):
+        for n in range(1, nseq + 1):
+            s_.step(m.dt, n)
+    for s_ in singles:
+        s_.synchronize()
+    t_seq = time.perf_counter() - t0
+    for n in range(1, nseq + 1):
+        ctx.step(case.dt, n)
+    p = ctx.get("P")
+    d_seq = 0.0
+    for b, (s_, m) in enumerate(zip(singles, members)):
+        sl = grids.basin_slice(case, first, b)
+        d_seq = max(d_seq, float(np.max(np.abs(p[sl] - s_.get("P")[1:1 + m.grid.nwet]))) / 9.81)
+        s_.close()
+    del singles
+
+    # ---- CPU baseline: the FP64 oracle on the first basins
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        t_cpu, cols, nb_done, st_done = 0.0, 0, 0, 0
+        for m in members:
+            o = orc.Oracle(m.copy(), rsqm=0.0, iterm=2000, nthreads=args.cpu_threads,
+                           rel_tol=1e-10)
+            for n in range(1, 4):
+                t1 = time.perf_counter()
+                o.step(n)
+                t_cpu += time.perf_counter() - t1
+                cols += m.grid.nwet
+                st_done += 1
+            nb_done += 1
+            if t_cpu >= args.cpu_seconds:
+                break
+        cpu = dict(value=cols / t_cpu, unit="column-steps/s", cores=min(args.cpu_threads, 8),
+                   kind="port",
+                   sample=f"3 full steps of each of the first {nb_done} basins by the FP64 C "
+                          f"restatement (OpenMP), {t_cpu:.1f} s")
+
+    # ---- timed region
+    n0 = nseq + 1
+    for n in range(n0, n0 + args.warmup):
+        ctx.step(case.dt, n)
+    n0 += args.warmup
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    iters = []
+    for n in range(n0, n0 + args.steps):
+        iters.append(ctx.step(case.dt, n)["iters"])
+    ctx.synchronize()
+    t = time.perf_counter() - t0
+    n0 += args.steps
+
+    # ---- per-kernel timing over a bounded number of further steps
+    nk = min(args.steps, 50)
+    ctx.timers(True)
+    cell_iters = 0
+    sizes = np.array([m.grid.nwet for m in members], dtype=np.int64)
+    for n in range(n0, n0 + nk):
+        ctx.step(case.dt, n)
+        it_b, _ = ctx.segment_stats()
+        cell_iters += int(np.sum(sizes * it_b))
+    tm = ctx.read_timers()
+    ctx.timers(False)
+    s_ = 8
+    per_col = kc * s_ * (6 + 2 * nc) + 40
+    total_bytes = {
+        "cg_spmv_pupdate": BYTES_PER_CELL["cg_spmv_pupdate"] * cell_iters,
+        "cg_axpy_dots": BYTES_PER_CELL["cg_axpy_dots"] * cell_iters,
+        "tran_upwind_vdiff": per_col * nwet * nk,
+    }
+    cand = {k: tm[k] for k in total_bytes if k in tm}
+    dom = max(cand, key=lambda k: cand[k][1])
+    cnt, ms = cand[dom]
+    ach = total_bytes[dom] / (ms * 1e-3) / 1e9
+    vol, mass = ctx.balance_segments()
+    out = {
+        "metric": "column-steps/s over all basins (batched ensemble)",
+        "value": nwet * args.steps / t,
+        "unit": "column-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batched ensemble: {args.nbasins} basins 64x64..512x256, "
+                               f"KC={kc}, {nc} constituents, {nwet} wet columns, dt={case.dt} s",
+                   "basins": args.nbasins, "columns": nwet},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "avg_us": ms / cnt * 1e3, "bytes_per_launch": total_bytes[dom] / cnt},
+        "cpu_baseline": cpu,
+        "detail": {
+            "sequential_value": int(sizes.sum()) * nseq / t_seq,
+            "sequential_sample": f"{nseq} steps of each basin in its own context, one "
+                                 f"basin after the other, {t_seq:.2f} s",
+            "eta_diff_vs_sequential_m": d_seq,
+            "cg_iters_per_step_max_basin": float(np.mean(iters)),
+            "cell_iters_per_step": cell_iters / nk,
+            "generate_s": t_gen,
+            "fp64_balance_first_basins": {"volume": [float(v) for v in vol[:4]],
+                                          "mass": [[float(x) for x in r] for r in mass[:4]]},
+            "kernels": {k: dict(launches=c_, avg_us=1e3 * m_ / c_, total_ms=m_)
+                        for k, (c_, m_) in tm.items()},
+        },
+    }
+    print(json.dumps(out))
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
@@ -257,9 +393,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("lake1024", "transport"), default="lake1024",
+    ap.add_argument("--workload", choices=("lake1024", "transport", "ensemble"),
+                    default="lake1024",
                     help="lake1024: the external solve at scale (the headline); "
-                         "transport: CALCONC at scale")
+                         "transport: CALCONC at scale; ensemble: 256 basins in one "
+                         "context")
+    ap.add_argument("--nbasins", type=int, default=256)
+    ap.add_argument("--ekc", type=int, default=8)
+    ap.add_argument("--seq-steps", type=int, default=10,
+                    help="ensemble: steps of the sequential single-basin comparison")
     ap.add_argument("--tnx", type=int, default=2048)
     ap.add_argument("--tny", type=int, default=1024)
     ap.add_argument("--tkc", type=int, default=20)
@@ -280,6 +422,9 @@ def main():
 
     if args.workload == "transport":
         transport_bench(args)
+        return
+    if args.workload == "ensemble":
+        ensemble_bench(args)
         return
 
     import numpy as np

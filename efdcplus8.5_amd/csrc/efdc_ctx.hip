@@ -292,6 +292,7 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   { char *q = (char *)c->pcg_hx; dfree(q); }
   { char *q = (char *)c->pcg_ps; dfree(q); }
   dfree(c->pcg_ctr); dfree(c->pcg_abort); dfree(c->pcg_flag);
+  seg_free(c);
   if (c->st_host) (void)hipHostFree(c->st_host);
   if (c->tide_host) (void)hipHostFree(c->tide_host);
   for (auto &b : c->pbc) {

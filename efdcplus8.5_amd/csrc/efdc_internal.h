@@ -43,6 +43,8 @@ struct CgState {
 // that no two counters share a cache line.
 #define TICKET_PAD 32
 #define TICKET_SITE (9 * TICKET_PAD)
+// cells per workgroup of the multi-kernel CG (CG_CPT per thread)
+#define CG_CPT 2
 #define TICKET_SITES 16
 enum {
   TS_PUV_MIN = 0, TS_CG_INIT, TS_CG_A, TS_CG_B, TS_MX_PREP, TS_MX_RESID,
@@ -106,6 +108,21 @@ struct efdc_ctx {
   unsigned *tickets = nullptr; // TICKET_SITES x TICKET_SITE counters
   CgState *st_host = nullptr;  // pinned mirror
   int last_iters = 0;
+  // independent basins (efdc_set_segments)
+  int nseg = 1;
+  std::vector<int> seg_first;  // first cell (0-based index i) of every basin
+  int *d_segid = nullptr;      // (LCM) basin of every cell
+  int *d_segfirst = nullptr;   // nseg+1 cell bounds
+  int4 *d_tiles = nullptr;     // CG tiles: first cell, end cell, basin, tile in basin
+  int *d_segtile0 = nullptr;   // first tile of each basin, nseg+1 entries
+  int ntiles = 0;
+  CgState *d_segst = nullptr;  // per-basin CG state
+  CgState *h_segst = nullptr;  // pinned mirror
+  unsigned *d_segtk = nullptr; // per-basin arrival counters, TICKET_PAD apart
+  unsigned *d_segdone = nullptr;  // basins whose solve has ended
+  double *d_segmin = nullptr;  // per-basin CCMNM
+  double *d_segpart = nullptr; // CG partials, 2 x ntiles
+  double *d_segbal = nullptr;  // per-basin balance sums
   // persistent solve (efdc_cgp.hip), allocated on first use
   bool pcg_ready = false;
   int pcg_ncu = 0;             // CUs, 0 when cooperative launch is unsupported
@@ -281,6 +298,31 @@ __device__ __forceinline__ bool last_arrival(unsigned *site, int nblocks) {
   return atomic_inc(gg) == (unsigned)ngroups - 1u;
 }
 
+// Sum of the partials part[j*stride + q], q < nb, j < NV, by one block in
+// a fixed order: thread t adds q = t, t+B, t+2B, ... (loads of 8 terms in
+// flight), then a fixed tree over the block.  Result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void sum_parts(const double *part, int stride, int nb,
+                                          double (&acc)[NV]) {
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    double s = 0.0;
+    const double *pj = part + (size_t)j * stride;
+    for (int q0 = threadIdx.x; q0 < nb; q0 += 8 * (int)blockDim.x) {
+      double a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int q = q0 + u * (int)blockDim.x;
+        a[u] = q < nb ? fetch(&pj[q]) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += a[u];
+    }
+    acc[j] = s;
+  }
+  block_sum<NV>(acc);
+}
+
 // Deterministic grid reduction: every block publishes its partials, the
 // last block to arrive (atomic ticket) sums the partials in block order
 // with a fixed tree, so the result does not depend on scheduling.
@@ -300,26 +342,8 @@ __device__ __forceinline__ bool grid_sum(double (&v)[NV], double *part,
   }
   __syncthreads();
   if (!am_last) return false;
-  // each thread sums part[tid], part[tid+B], part[tid+2B], ... in that
-  // order; the loads of 8 consecutive terms are issued together
   double acc[NV];
-#pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    double s = 0.0;
-    const double *pj = part + (size_t)j * nblocks;
-    for (int q0 = threadIdx.x; q0 < nblocks; q0 += 8 * (int)blockDim.x) {
-      double a[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int q = q0 + u * (int)blockDim.x;
-        a[u] = q < nblocks ? fetch(&pj[q]) : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) s += a[u];
-    }
-    acc[j] = s;
-  }
-  block_sum<NV>(acc);
+  sum_parts<NV>(part, nblocks, nblocks, acc);
   __shared__ double res[NV];
   if (threadIdx.x == 0) {
 #pragma unroll
@@ -398,6 +422,10 @@ int efdc_grid_blocks(int n);
 // persistent CONGRAD (efdc_cgp.hip)
 bool pcg_applicable(efdc_ctx *c, bool mixed);
 int pcg_solve(efdc_ctx *c, bool mixed);
+// independent basins (efdc_seg.hip)
+int seg_min_run(efdc_ctx *c);
+int seg_cg_run(efdc_ctx *c, efdc_cg_stats_t *stats);
+void seg_free(efdc_ctx *c);
 int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats);
 int cg_run(efdc_ctx *c, bool check_sym, efdc_cg_stats_t *stats);
 int tran_run(efdc_ctx *c, const efdc_step_t *step);

@@ -13,6 +13,7 @@
 // writes the scalars for the next kernel.  The next launches read a device
 // `done` flag and return immediately once the stop test (sum r^2 <= RSQM)
 // has been met, so the host only polls every few iterations.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <utility>
@@ -36,7 +37,6 @@ HPtr hptr(efdc_ctx *c) {
 // Cells per thread of the CG kernels: each workgroup owns a contiguous run
 // of CG_CPT*256 cells, which halves the number of workgroups (and of
 // reduction arrivals) per launch.
-#define CG_CPT 2
 int cg_blocks(int nwet) { return (nwet + CG_CPT * EFDC_BLOCK - 1) / (CG_CPT * EFDC_BLOCK); }
 
 __device__ __forceinline__ int cell_index(int nwet, int &tile) {
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_explicit(
 // ---------------------------------------------------------------------
 // Assembly: EFDC/calpuv2c.f90:315-342, with the min-diagonal reduction of
 // :360-376 over the cells that are not pressure open boundaries (those
-// get their final CC in k_puv_bc, which completes the minimum).
+// get their final CC in k_puv_bc_side; k_puv_bc_min completes the minimum).
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_assemble(
     HPtr h, const int *__restrict__ le, const int *__restrict__ ln,
@@ -124,7 +124,11 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_assemble(
 // ---------------------------------------------------------------------
 // SETOPENBC (EFDC/setopenbc.f90:17-206), the open-boundary part of the
 // min-diagonal reduction, and SETOPENBC2 (EFDC/setopenbc2.f90:17-45).
-// One workgroup; the sides are applied in the reference order W, E, S, N.
+// The sides are applied in the reference order W, E, S, N, one launch per
+// side (a side whose cells share a neighbour runs on one thread, in the
+// reference's cell order, because of the FP(neighbour) += updates); then
+// one launch over all boundary cells takes the minimum of their CC and
+// zeroes the couplings into the clamped cells.
 // ---------------------------------------------------------------------
 struct PbcArgs {
   PbcDev side[4];
@@ -133,10 +137,10 @@ struct PbcArgs {
   int *lopenbcdry;
 };
 
-__global__ __launch_bounds__(EFDC_BLOCK) void k_puv_bc(
-    HPtr h, PbcArgs b, const int *__restrict__ lw, const int *__restrict__ le,
+__global__ __launch_bounds__(EFDC_BLOCK) void k_puv_bc_side(
+    HPtr h, PbcArgs b, PbcDev s, int side, const int *__restrict__ lw, const int *__restrict__ le,
     const int *__restrict__ ls, const int *__restrict__ ln, double dt,
-    double G, double HDRY, CgState *st) {
+    double G, double HDRY) {
   const double D_LTD2 = 0.5 * dt, DELTI = 1. / dt, DE_TI = DELTI;
   const double HDRY2 = 0.2 * HDRY;
   double *CC = HA(CC), *CS = HA(CS), *CW = HA(CW), *CE = HA(CE), *CN = HA(CN),
@@ -144,14 +148,13 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_bc(
          *HRVO = HA(HRVO), *RCX = HA(RCX), *RCY = HA(RCY), *HUT = HA(H1U),
          *HVT = HA(H1V), *DXIU = HA(DXIU), *DYIU = HA(DYIU), *DXIV = HA(DXIV),
          *DYIV = HA(DYIV), *FUHDYE = HA(FUHDYE), *FVHDXE = HA(FVHDXE);
-  for (int side = 0; side < 4; ++side) {
-    const PbcDev &s = b.side[side];
-    const int stride = s.serial ? 1 : blockDim.x;
-    if (s.serial && threadIdx.x != 0) {
-      __syncthreads();
-      continue;
+  {
+    int LL0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    if (s.serial) {
+      if (LL0 != 0) return;
+      stride = 1;
     }
-    for (int LL = threadIdx.x; LL < s.n; LL += stride) {
+    for (int LL = LL0; LL < s.n; LL += stride) {
       const int i = s.L[LL];
       CC[i] = DELTI * DXYP[i];
       CS[i] = 0.;
@@ -251,36 +254,45 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_bc(
         }
       }
     }
-    __syncthreads();
   }
-  // minimum of CC over the open-boundary cells (:365-375)
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_puv_bc_min(
+    HPtr h, PbcArgs b, int ntot, const int *__restrict__ lw, const int *__restrict__ le,
+    const int *__restrict__ ls, const int *__restrict__ ln, double *part, CgState *st,
+    unsigned *st_tk) {
+  double *CC = HA(CC), *CS = HA(CS), *CW = HA(CW), *CE = HA(CE), *CN = HA(CN);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   double m = 1.E+18;
-  for (int side = 0; side < 4; ++side)
-    for (int LL = threadIdx.x; LL < b.side[side].n; LL += blockDim.x)
-      m = fmin(m, CC[b.side[side].L[LL]]);
-  m = wave_min(m);
-  __shared__ double sh[EFDC_BLOCK / 64];
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double mm = st->ccmnm;
-    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) mm = fmin(mm, sh[q]);
-    st->ccmnm = mm;
-  }
-  __syncthreads();
-  // SETOPENBC2
-  for (int side = 0; side < 4; ++side) {
-    const PbcDev &s = b.side[side];
-    for (int LL = threadIdx.x; LL < s.n; LL += blockDim.x) {
-      if (s.ispb[LL] != 0) continue;
-      const int i = s.L[LL];
+  if (t < ntot) {
+    // side of entry t (the sides' cells follow one another)
+    const int n0 = b.side[0].n, n01 = n0 + b.side[1].n, n012 = n01 + b.side[2].n;
+    int side, LL;
+    const int *L, *ispb;
+    if (t < n0) {
+      side = 0; LL = t; L = b.side[0].L; ispb = b.side[0].ispb;
+    } else if (t < n01) {
+      side = 1; LL = t - n0; L = b.side[1].L; ispb = b.side[1].ispb;
+    } else if (t < n012) {
+      side = 2; LL = t - n01; L = b.side[2].L; ispb = b.side[2].ispb;
+    } else {
+      side = 3; LL = t - n012; L = b.side[3].L; ispb = b.side[3].ispb;
+    }
+    const int i = L[LL];
+    // minimum of CC over the open-boundary cells (:365-375)
+    m = CC[i];
+    // SETOPENBC2: no coupling from the interior into a clamped cell
+    if (ispb[LL] == 0) {
       if (side == 0) CW[le[i]] = 0.;
       else if (side == 1) CE[lw[i]] = 0.;
       else if (side == 2) CS[ln[i]] = 0.;
       else CN[ls[i]] = 0.;
     }
-    __syncthreads();
   }
+  double mm;
+  if (grid_min(m, part, st_tk + TS_PUV_MIN * TICKET_SITE, gridDim.x, blockIdx.x, mm) &&
+      threadIdx.x == 0)
+    st->ccmnm = fmin(st->ccmnm, mm);
 }
 
 // ---------------------------------------------------------------------
@@ -290,11 +302,13 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_bc(
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_scale(
     HPtr h, const int *__restrict__ lw, const int *__restrict__ le,
-    const int *__restrict__ ln, int nwet, int lc0, CgState *st) {
+    const int *__restrict__ ln, int nwet, int lc0, CgState *st,
+    const double *segmin, const int *segid) {
   int tile;
   const int i = cell_index(nwet, tile);
   if (i > nwet) return;
-  const double CCMNMI = 1. / st->ccmnm;
+  // one CCMNM per model; per basin when the cells form several basins
+  const double CCMNMI = 1. / (segmin ? segmin[segid[i]] : st->ccmnm);
   const double cs = HA(CS)[i], cw = HA(CW)[i], ce = HA(CE)[i], cn = HA(CN)[i];
   HA(CCS)[i] = cs * CCMNMI;
   const double ccw = cw * CCMNMI;
@@ -408,15 +422,17 @@ __device__ __forceinline__ Row row_of(const CgArgs &a, int i, bool fast) {
   return q;
 }
 
-__global__ __launch_bounds__(EFDC_BLOCK) void k_cg_init(CgArgs a, double rsqm,
-                                                        double rel_tol) {
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const bool fast = a.rowmajor && (a.st->symok == 0);
-  double v[2] = {0.0, 0.0};
-  #pragma unroll
+// Per-cell work of the three CG kernels over the cells i0 .. i1-1 of one
+// tile (cell i0 + u*EFDC_BLOCK + threadIdx.x for u < CG_CPT).  Shared by
+// the single-model kernels (tile t: i0 = 1 + t*CG_CPT*EFDC_BLOCK) and the
+// per-basin kernels (tiles counted from each basin's first cell), so one
+// basin of an ensemble and the same basin alone give the same bits.
+__device__ __forceinline__ void cg_init_cells(const CgArgs &a, int i0, int i1, bool fast,
+                                              double (&v)[2]) {
+#pragma unroll
   for (int u = 0; u < CG_CPT; ++u) {
-    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
-    if (i <= a.nwet) {
+    const int i = i0 + u * EFDC_BLOCK + (int)threadIdx.x;
+    if (i < i1) {
       const Row q = row_of(a, i, fast);
       const double *P = a.x;
       // RCG = FPTMP - CCC*P - CCN*PN - CCS*PS - CCW*PW - CCE*PE (:73)
@@ -428,20 +444,31 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_cg_init(CgArgs a, double rsqm,
       v[1] += a.fptmp[i] * a.fptmp[i];
     }
   }
+}
+
+__device__ __forceinline__ void cg_init_state(CgState *st, const double (&s)[2], double rsqm,
+                                              double rel_tol) {
+  st->rpcg = s[0];
+  st->bnorm2 = s[1];
+  st->rsqm = rel_tol > 0.0 ? rel_tol * rel_tol * s[1] : rsqm;
+  st->beta = 0.0;
+  st->alpha = 0.0;
+  st->iter = 0;
+  st->rsq = 0.0;
+  st->status = 0;
+  st->done = (s[0] == 0.0) ? 1 : 0;  // :88
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_cg_init(CgArgs a, double rsqm,
+                                                        double rel_tol) {
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const bool fast = a.rowmajor && (a.st->symok == 0);
+  double v[2] = {0.0, 0.0};
+  cg_init_cells(a, 1 + tile * CG_CPT * EFDC_BLOCK, a.nwet + 1, fast, v);
   double s[2];
   if (grid_sum<2>(v, a.part, a.tk + TS_CG_INIT * TICKET_SITE, gridDim.x, tile, s) &&
-      threadIdx.x == 0) {
-    CgState *st = a.st;
-    st->rpcg = s[0];
-    st->bnorm2 = s[1];
-    st->rsqm = rel_tol > 0.0 ? rel_tol * rel_tol * s[1] : rsqm;
-    st->beta = 0.0;
-    st->alpha = 0.0;
-    st->iter = 0;
-    st->rsq = 0.0;
-    st->status = 0;
-    st->done = (s[0] == 0.0) ? 1 : 0;  // :88
-  }
+      threadIdx.x == 0)
+    cg_init_state(a.st, s, rsqm, rel_tol);
 }
 
 __device__ __forceinline__ double pnew_at(const CgArgs &a, const double *pold,
@@ -450,18 +477,13 @@ __device__ __forceinline__ double pnew_at(const CgArgs &a, const double *pold,
   return (j > 0 && j != a.lc0) ? a.ccci[j] * a.r[j] + beta * pold[j] : 0.0;
 }
 
-__global__ __launch_bounds__(EFDC_BLOCK) void k_cg_a(CgArgs a,
-                                                     const double *pold,
-                                                     double *pnew) {
-  if (a.st->done) return;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const bool fast = a.rowmajor && (a.st->symok == 0);
-  const double beta = a.st->beta;
-  double v[1] = {0.0};
-  #pragma unroll
+__device__ __forceinline__ void cg_a_cells(const CgArgs &a, int i0, int i1, bool fast,
+                                           double beta, const double *pold, double *pnew,
+                                           double (&v)[1]) {
+#pragma unroll
   for (int u = 0; u < CG_CPT; ++u) {
-    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
-    if (i <= a.nwet) {
+    const int i = i0 + u * EFDC_BLOCK + (int)threadIdx.x;
+    if (i < i1) {
       const Row q = row_of(a, i, fast);
       const double pc = a.ccci[i] * a.r[i] + beta * pold[i];
       const double ps = pnew_at(a, pold, beta, q.js);
@@ -475,6 +497,16 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_cg_a(CgArgs a,
       v[0] += apc * pc;
     }
   }
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_cg_a(CgArgs a,
+                                                     const double *pold,
+                                                     double *pnew) {
+  if (a.st->done) return;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const bool fast = a.rowmajor && (a.st->symok == 0);
+  double v[1] = {0.0};
+  cg_a_cells(a, 1 + tile * CG_CPT * EFDC_BLOCK, a.nwet + 1, fast, a.st->beta, pold, pnew, v);
   double s[1];
   if (grid_sum<1>(v, a.part, a.tk + TS_CG_A * TICKET_SITE, gridDim.x, tile, s) &&
       threadIdx.x == 0) {
@@ -483,17 +515,12 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_cg_a(CgArgs a,
   }
 }
 
-__global__ __launch_bounds__(EFDC_BLOCK) void k_cg_b(CgArgs a,
-                                                     const double *pnew,
-                                                     int iterm) {
-  if (a.st->done) return;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const double alpha = a.st->alpha;
-  double v[2] = {0.0, 0.0};
-  #pragma unroll
+__device__ __forceinline__ void cg_b_cells(const CgArgs &a, int i0, int i1, double alpha,
+                                           const double *pnew, double (&v)[2]) {
+#pragma unroll
   for (int u = 0; u < CG_CPT; ++u) {
-    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
-    if (i <= a.nwet) {
+    const int i = i0 + u * EFDC_BLOCK + (int)threadIdx.x;
+    if (i < i1) {
       a.x[i] = a.x[i] + alpha * pnew[i];       // :132
       const double r = a.r[i] - alpha * a.ap[i];  // :133
       a.r[i] = r;
@@ -502,18 +529,158 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_cg_b(CgArgs a,
       v[1] += r * r;
     }
   }
+}
+
+// beta, rsq, the stop test and the iteration cap after the r.z, r.r sums;
+// returns true when this call ended the solve.
+__device__ __forceinline__ bool cg_b_state(CgState *st, const double (&s)[2], int iterm) {
+  const double beta = s[0] / st->rpcg;  // :147-148
+  st->beta = beta;
+  st->rpcg = s[0];
+  st->rsq = s[1];
+  st->iter += 1;
+  if (s[1] <= st->rsqm) { st->done = 1; return true; }             // :165
+  if (st->iter >= iterm) { st->done = 1; st->status = 1; return true; }
+  return false;
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_cg_b(CgArgs a,
+                                                     const double *pnew,
+                                                     int iterm) {
+  if (a.st->done) return;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  double v[2] = {0.0, 0.0};
+  cg_b_cells(a, 1 + tile * CG_CPT * EFDC_BLOCK, a.nwet + 1, a.st->alpha, pnew, v);
   double s[2];
   if (grid_sum<2>(v, a.part, a.tk + TS_CG_B * TICKET_SITE, gridDim.x, tile, s) &&
-      threadIdx.x == 0) {
-    CgState *st = a.st;
-    const double beta = s[0] / st->rpcg;  // :147-148
-    st->beta = beta;
-    st->rpcg = s[0];
-    st->rsq = s[1];
-    st->iter += 1;
-    if (s[1] <= st->rsqm) st->done = 1;             // :165
-    else if (st->iter >= iterm) { st->done = 1; st->status = 1; }
+      threadIdx.x == 0)
+    (void)cg_b_state(a.st, s, iterm);
+}
+
+// ---------------------------------------------------------------------
+// Independent basins: the same three kernels over per-basin tiles.  Tile b
+// of the launch covers cells tiles[b].x .. tiles[b].y-1 of basin
+// tiles[b].z; its partials go to part[j*ntiles + b]; the last tile of a
+// basin to arrive sums the basin's partials (tiles segtile0[s] ..
+// segtile0[s+1]-1, in order) exactly as the single-model reduction sums
+// its blocks, and updates the basin's state.  Tiles of a basin whose solve
+// has ended return at once.
+// ---------------------------------------------------------------------
+struct SegArgs {
+  const int4 *tiles;
+  const int *segtile0;
+  CgState *segst;
+  unsigned *segtk;
+  unsigned *segdone;
+  double *part;
+  int ntiles;
+};
+
+// Per-basin arrival; true in the block that arrives last for its basin.
+__device__ __forceinline__ bool seg_last(const SegArgs &g, int seg, int ntile_seg) {
+  __shared__ bool am_last;
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_s_waitcnt(0);
+    unsigned *t = g.segtk + (size_t)seg * TICKET_PAD;
+    am_last = atomic_inc(t) == (unsigned)ntile_seg - 1u;
+    if (am_last) reset_ticket(t);
   }
+  __syncthreads();
+  return am_last;
+}
+
+template <int NV>
+__device__ __forceinline__ bool seg_sum(const SegArgs &g, int b, int seg, double (&v)[NV],
+                                        double (&out)[NV]) {
+  block_sum<NV>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) publish(&g.part[(size_t)j * g.ntiles + b], v[j]);
+  }
+  const int t0 = g.segtile0[seg], nt = g.segtile0[seg + 1] - t0;
+  if (!seg_last(g, seg, nt)) return false;
+  double acc[NV];
+  sum_parts<NV>(g.part + t0, g.ntiles, nt, acc);
+  __shared__ double res[NV];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) res[j] = acc[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NV; ++j) out[j] = res[j];
+  return true;
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_scg_init(CgArgs a, SegArgs g, double rsqm,
+                                                         double rel_tol) {
+  const int b = blockIdx.x;
+  const int4 t = g.tiles[b];
+  const bool fast = a.rowmajor && (a.st->symok == 0);
+  double v[2] = {0.0, 0.0};
+  cg_init_cells(a, t.x, t.y, fast, v);
+  double s[2];
+  if (seg_sum<2>(g, b, t.z, v, s) && threadIdx.x == 0) {
+    CgState *st = g.segst + t.z;
+    cg_init_state(st, s, rsqm, rel_tol);
+    if (st->done) atomic_inc(g.segdone);
+  }
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_scg_a(CgArgs a, SegArgs g, const double *pold,
+                                                      double *pnew) {
+  const int b = blockIdx.x;
+  const int4 t = g.tiles[b];
+  CgState *st = g.segst + t.z;
+  if (st->done) return;
+  const bool fast = a.rowmajor && (a.st->symok == 0);
+  double v[1] = {0.0};
+  cg_a_cells(a, t.x, t.y, fast, st->beta, pold, pnew, v);
+  double s[1];
+  if (seg_sum<1>(g, b, t.z, v, s) && threadIdx.x == 0) {
+    st->papcg = s[0];
+    st->alpha = st->rpcg / s[0];  // :124
+  }
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_scg_b(CgArgs a, SegArgs g, const double *pnew,
+                                                      int iterm) {
+  const int b = blockIdx.x;
+  const int4 t = g.tiles[b];
+  CgState *st = g.segst + t.z;
+  if (st->done) return;
+  double v[2] = {0.0, 0.0};
+  cg_b_cells(a, t.x, t.y, st->alpha, pnew, v);
+  double s[2];
+  if (seg_sum<2>(g, b, t.z, v, s) && threadIdx.x == 0) {
+    if (cg_b_state(st, s, iterm)) atomic_inc(g.segdone);
+  }
+}
+
+// Per-basin minimum of CC over all its cells after SETOPENBC: the
+// reference's minimum over the interior cells (:360-364) and over the
+// open-boundary cells (:365-375) taken together.  One workgroup per basin.
+__global__ __launch_bounds__(EFDC_BLOCK) void k_seg_min(const double *cc, const int *segfirst,
+                                                        double *segmin) {
+  const int sgi = blockIdx.x;
+  const int i0 = segfirst[sgi], i1 = segfirst[sgi + 1];
+  double m = 1.E+18;
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) m = fmin(m, cc[i]);
+  m = wave_min(m);
+  __shared__ double sh[EFDC_BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = sh[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) mm = fmin(mm, sh[q]);
+    segmin[sgi] = mm;
+  }
+}
+
+__global__ void k_seg_reset(CgState *segst, int nseg, unsigned *segdone) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nseg) segst[s].done = 0;
+  if (s == 0) *segdone = 0u;
 }
 
 // ---------------------------------------------------------------------
@@ -832,6 +999,7 @@ int cg_mixed(efdc_ctx *c) {
 
 int cg_dispatch(efdc_ctx *c, efdc_cg_stats_t *stats) {
   int s;
+  if (c->nseg > 1) return seg_cg_run(c, stats);
   bool mixed = (c->prm.cg_mode == EFDC_CG_MIXED);
   if (mixed) {
     // the mixed operator is the compact symmetric one
@@ -933,11 +1101,26 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
     b.tide = c->tide_dev;
     b.mtide = c->mtide;
     b.lopenbcdry = c->lopenbcdry;
-    TIMED(c, TK_PUV_BC, k_puv_bc<<<1, EFDC_BLOCK, 0, c->stream>>>(h, b, c->dLW, c->dLE, c->dLS, c->dLN,
-                                              dt, G, c->prm.HDRY, c->st));
+    int ntot = 0;
+    for (int q = 0; q < 4; ++q) {
+      if (c->pbc[q].n == 0) continue;
+      ntot += c->pbc[q].n;
+      const int nbs = c->pbc[q].serial ? 1 : (c->pbc[q].n + EFDC_BLOCK - 1) / EFDC_BLOCK;
+      TIMED(c, TK_PUV_BC, k_puv_bc_side<<<nbs, EFDC_BLOCK, 0, c->stream>>>(
+          h, b, c->pbc[q], q, c->dLW, c->dLE, c->dLS, c->dLN, dt, G, c->prm.HDRY));
+    }
+    TIMED(c, TK_PUV_BC, k_puv_bc_min<<<(ntot + EFDC_BLOCK - 1) / EFDC_BLOCK, EFDC_BLOCK, 0,
+                                       c->stream>>>(h, b, ntot, c->dLW, c->dLE, c->dLS,
+                                                    c->dLN, c->part, c->st, c->tickets));
   }
-  TIMED(c, TK_PUV_SCALE, k_puv_scale<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLE, c->dLN, nwet,
-                                                lc0, c->st));
+  const bool segs = c->nseg > 1;
+  if (segs) {
+    int s = seg_min_run(c);
+    if (s) return s;
+  }
+  TIMED(c, TK_PUV_SCALE, k_puv_scale<<<nb, EFDC_BLOCK, 0, c->stream>>>(
+      h, c->dLW, c->dLE, c->dLN, nwet, lc0, c->st, segs ? c->d_segmin : nullptr,
+      segs ? c->d_segid : nullptr));
   int s = cg_dispatch(c, stats);
   if (s != EFDC_OK && s != EFDC_ERR_CG_MAXITER) return s;
   const int cgs = s;
@@ -951,6 +1134,77 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
   if (c->st_host->negdep) {
     c->err = "NEGDEP: negative depth after the external solution";
     return EFDC_ERR_NEGDEPTH;
+  }
+  return EFDC_OK;
+}
+
+// ---------------------------------------------------------------------
+// Independent basins: per-basin minimum and the FP64 CG over per-basin
+// tiles (the multi-kernel schedule: two launches per iteration for all
+// basins together, the host polling every few iterations).
+// ---------------------------------------------------------------------
+int seg_min_run(efdc_ctx *c) {
+  TIMED(c, TK_PUV_SCALE, k_seg_min<<<c->nseg, EFDC_BLOCK, 0, c->stream>>>(
+      c->dH[EFDC_CC], c->d_segfirst, c->d_segmin));
+  HIP_TRY(c, hipGetLastError());
+  return EFDC_OK;
+}
+
+int seg_cg_run(efdc_ctx *c, efdc_cg_stats_t *stats) {
+  if (c->prm.cg_mode != EFDC_CG_FP64) {
+    c->err = "independent basins are solved with the FP64 CG (EFDC_CG_FP64)";
+    return EFDC_ERR_ARG;
+  }
+  CgArgs a = cg_args(c);
+  SegArgs g;
+  g.tiles = c->d_tiles;
+  g.segtile0 = c->d_segtile0;
+  g.segst = c->d_segst;
+  g.segtk = c->d_segtk;
+  g.segdone = c->d_segdone;
+  g.part = c->d_segpart;
+  g.ntiles = c->ntiles;
+  const int nt = c->ntiles, iterm = c->prm.ITERM;
+  k_seg_reset<<<(c->nseg + 255) / 256, 256, 0, c->stream>>>(c->d_segst, c->nseg, c->d_segdone);
+  TIMED(c, TK_CG_INIT, k_scg_init<<<nt, EFDC_BLOCK, 0, c->stream>>>(a, g, c->prm.RSQM,
+                                                                   c->prm.cg_rel_tol));
+  int it = 0;
+  int chunk = c->last_iters > 0 ? c->last_iters : 8;
+  unsigned done = 0;
+  for (;;) {
+    for (int j = 0; j < chunk && it < iterm; ++j, ++it) {
+      TIMED(c, TK_CG_A, k_scg_a<<<nt, EFDC_BLOCK, 0, c->stream>>>(a, g, c->pb[it & 1],
+                                                                  c->pb[(it + 1) & 1]));
+      TIMED(c, TK_CG_B, k_scg_b<<<nt, EFDC_BLOCK, 0, c->stream>>>(a, g, c->pb[(it + 1) & 1],
+                                                                  iterm));
+    }
+    HIP_TRY(c, hipMemcpyAsync(&done, c->d_segdone, sizeof(unsigned), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if ((int)done >= c->nseg || it >= iterm) break;
+    chunk = 4;
+  }
+  HIP_TRY(c, hipMemcpyAsync(c->h_segst, c->d_segst, sizeof(CgState) * c->nseg,
+                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  int imax = 0, bad = 0;
+  double rsq = 0.0;
+  for (int s = 0; s < c->nseg; ++s) {
+    imax = std::max(imax, c->h_segst[s].iter);
+    rsq = std::max(rsq, c->h_segst[s].rsq);
+    if (c->h_segst[s].status != 0) bad = 1;
+  }
+  c->last_iters = imax;
+  if (stats) {
+    stats->iters = imax;
+    stats->outer = 1;
+    stats->rsq = rsq;
+    stats->rsqm_used = c->h_segst[0].rsqm;
+    stats->ccmnm = 0.0;
+  }
+  if (bad) {
+    c->err = "CONGRAD: maximum iterations exceeded in external solution (a basin)";
+    return EFDC_ERR_CG_MAXITER;
   }
   return EFDC_OK;
 }

@@ -526,25 +526,30 @@ __global__ void k_closure_pre(double *tbx, double *tby, const double *uhe,
 }
 
 template <typename T>
-__global__ void k_closure_post(const double *uhdye, const double *uhdy1e,
-                               const double *vhdxe, const double *vhdx1e,
-                               const double *hp, const double *hu,
-                               const double *hv, const double *dyiu,
-                               const double *dxiv, const T *dzc, T *uhdy2,
-                               T *vhdx2, T *u2, T *v2, T *hpk, T *hpki,
+__global__ void k_closure_post(const double *__restrict__ uhdye, const double *__restrict__ uhdy1e,
+                               const double *__restrict__ vhdxe, const double *__restrict__ vhdx1e,
+                               const double *__restrict__ hp, const double *__restrict__ hu,
+                               const double *__restrict__ hv, const double *__restrict__ dyiu,
+                               const double *__restrict__ dxiv, const T *__restrict__ dzc,
+                               T *__restrict__ uhdy2, T *__restrict__ vhdx2, T *__restrict__ u2,
+                               T *__restrict__ v2, T *__restrict__ hpk, T *__restrict__ hpki,
                                int64_t pitch, int KC, int nwet) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   if (i > nwet) return;
   const double ub = 0.5 * (uhdye[i] + uhdy1e[i]);
   const double vb = 0.5 * (vhdxe[i] + vhdx1e[i]);
+  // the same for every layer of the column
+  const T uk = (T)(ub * dyiu[i] / hu[i]);
+  const T vk = (T)(vb * dxiv[i] / hv[i]);
+  const double hpi = hp[i];
   for (int k = 0; k < KC; ++k) {
     const int64_t j = i + k * pitch;
     const double dz = (double)dzc[j];
     uhdy2[j] = (T)(ub * dz);
     vhdx2[j] = (T)(vb * dz);
-    u2[j] = (T)(ub * dyiu[i] / hu[i]);
-    v2[j] = (T)(vb * dxiv[i] / hv[i]);
-    const double hk = hp[i] * dz;
+    u2[j] = uk;
+    v2[j] = vk;
+    const double hk = hpi * dz;
     hpk[j] = (T)hk;
     hpki[j] = (T)(1. / hk);
   }

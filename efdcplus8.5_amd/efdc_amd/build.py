@@ -16,7 +16,8 @@ PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(HERE, "libefdc_hip.so")
-SOURCES = ["efdc_ctx.hip", "efdc_puv.hip", "efdc_tran.hip", "efdc_cgp.hip"]
+SOURCES = ["efdc_ctx.hip", "efdc_puv.hip", "efdc_tran.hip", "efdc_cgp.hip",
+           "efdc_seg.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_FLAGS = [

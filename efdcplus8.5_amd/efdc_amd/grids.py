@@ -310,13 +310,23 @@ def _constituent(g: Grid, mo: int, con: np.ndarray, **kw) -> dict:
 def tidal_basin(nx=128, ny=64, kc=5, *, seed=1000, dx=100.0, dt=5.0,
                 amp=0.5, period=44712.0, manning=0.025, ab=1e-4,
                 dye_load=1.0, depth=(5.0, 20.0), depth_sigma=8.0,
-                ntscr=12, isadac=0, isfct=0, name=None) -> Case:
+                ntscr=12, isadac=0, isfct=0, name=None, wet=None) -> Case:
     """Config-0 style case: rectangular basin nx*ny, west column is a
     clamped open boundary (ISPBW=0) with an M2-like tide, other sides
     closed.  Salinity 0->30 psu W->E, a dye point source (QSS=1e-3 m^3/s
-    split over layers by DZC, CQS = dye_load/1e-3) at the centre cell."""
+    split over layers by DZC, CQS = dye_load/1e-3) at the centre cell.
+
+    With ``wet`` (a padded (ny+2, nx+2) mask such as lake_mask returns) the
+    basin takes that shape; the open boundary is then a west segment: in
+    the middle third of the wet rows, each row's westernmost wet cell whose
+    east neighbour is wet, and the dye source sits at the wet cell nearest
+    the centre."""
     rng = np.random.default_rng(seed)
-    wet = _pad_ring(np.ones((ny, nx), dtype=bool))
+    if wet is None:
+        wet = _pad_ring(np.ones((ny, nx), dtype=bool))
+    else:
+        wet = np.asarray(wet, dtype=bool)
+        ny, nx = wet.shape[0] - 2, wet.shape[1] - 2
     g = build_grid(wet, kc)
     h, k, w = _empty_arrays(g)
     fill_geometry(g, h, k, w, dx, np.ones(kc))
@@ -332,8 +342,18 @@ def tidal_basin(nx=128, ny=64, kc=5, *, seed=1000, dx=100.0, dt=5.0,
     layer_depths(g, h, k)
     k["H1PK"][:] = k["HPK"]
     k["AB"][:kc - 1, L] = ab if kc > 1 else 0.0
-    # west open boundary: first wet column (I index 1 in 0-based padded grid)
-    west = np.nonzero(ii == 1)[0]
+    if wet[1:-1, 1:-1].all():
+        # west open boundary: first wet column (I index 1 in 0-based padded grid)
+        west = np.nonzero(ii == 1)[0]
+    else:
+        rows = np.unique(jj)
+        lo, hi = rows[len(rows) // 3], rows[(2 * len(rows)) // 3]
+        west = []
+        for j in range(lo, hi + 1):
+            cols = np.nonzero(wet[j])[0]
+            if len(cols) and wet[j, cols[0] + 1]:
+                west.append(int(g.LIJ[j, cols[0]]) - 2)   # position in L
+        west = np.array(west, dtype=np.int64)
     lpbw = (L[west] + 1).astype(np.int32)
     nb = len(lpbw)
     open_bc = dict(side=[dict(L=lpbw, ispb=np.zeros(nb, np.int32),
@@ -359,6 +379,9 @@ def tidal_basin(nx=128, ny=64, kc=5, *, seed=1000, dx=100.0, dt=5.0,
                          None, None], wtci=wtci)
     # dye point source at the centre cell (I=nx/2, J=ny/2, 1-based interior)
     lsrc = int(g.LIJ[ny // 2, nx // 2])
+    if lsrc == 0:
+        d2 = (jj - ny // 2) ** 2 + (ii - nx // 2) ** 2
+        lsrc = int(L[int(np.argmin(d2))]) + 1
     q = 1e-3
     qss = np.zeros((1, kc))
     qss[0] = q * k["DZC"][:kc, lsrc - 1]
@@ -376,6 +399,30 @@ def tidal_basin(nx=128, ny=64, kc=5, *, seed=1000, dx=100.0, dt=5.0,
                 consts=consts, open_bc=open_bc, conc_bc=conc_bc,
                 sources=sources, dt=dt, manning_n=nman,
                 meta=dict(seed=seed, dx=dx))
+
+
+def ensemble_member(b: int, kc: int = 8, seed0: int = 1003) -> Case:
+    """Basin b of the batched ensemble: (nx, ny) log-uniform in
+    64x64 .. 512x256, an irregular lake mask as at scale, depths 2-30 m,
+    Manning n in U(0.015, 0.04), tidal amplitude in U(0.1, 1.0) m on a west
+    boundary segment, KC layers, salinity and a dye source; seed seed0+b."""
+    seed = seed0 + b
+    rng = np.random.default_rng(seed)
+    nx = int(round(math.exp(rng.uniform(math.log(64), math.log(512)))))
+    ny = int(round(math.exp(rng.uniform(math.log(64), math.log(256)))))
+    manning = float(rng.uniform(0.015, 0.04))
+    amp = float(rng.uniform(0.1, 1.0))
+    wet = lake_mask(nx, ny, seed)
+    return tidal_basin(nx, ny, kc, seed=seed, amp=amp, manning=manning,
+                       depth=(2.0, 30.0), wet=wet, name=f"member{b}-{nx}x{ny}")
+
+
+def config3(nbasins: int = 256, kc: int = 8):
+    """The batched ensemble: ``nbasins`` members concatenated; returns
+    (case, first_L, members)."""
+    members = [ensemble_member(b, kc) for b in range(nbasins)]
+    case, first = concat_basins(members, name=f"ensemble{nbasins}x{kc}")
+    return case, first, members
 
 
 def config0(**kw) -> Case:
@@ -482,3 +529,131 @@ def transport_case(nx=2048, ny=1024, kc=20, *, seed=1002, dx=100.0, dt=5.0,
 
 def config2(**kw) -> Case:
     return transport_case(2048, 1024, 20, name="cfg2", **kw)
+
+
+# ------------------------------------------------------------ ensembles
+
+def concat_basins(cases: List[Case], name: Optional[str] = None):
+    """Concatenate independent basins into one case.
+
+    The basins' masks are stacked north of one another (each keeps its land
+    ring, so no neighbour map joins two basins) and numbered like CELLMAP;
+    basin b's cells then form the contiguous range first_L[b] ..
+    first_L[b+1]-1 of the result, in the basin's own order.  Every array,
+    boundary table, source and constituent is carried over at the shifted
+    L.  The basins must share KC, dt, the tidal constituents, the
+    concentration-BC weights and the constituent list.  Returns the case and
+    first_L (the argument of efdc_set_segments).
+    """
+    c0 = cases[0]
+    kc = c0.grid.KC
+    for c in cases[1:]:
+        if c.grid.KC != kc or c.dt != c0.dt:
+            raise ValueError("basins must share KC and dt")
+        if [(x["mo"], x["isadac"], x["isfct"], x["rkdye"], x["is_dye"]) for x in c.consts] != \
+                [(x["mo"], x["isadac"], x["isfct"], x["rkdye"], x["is_dye"]) for x in c0.consts]:
+            raise ValueError("basins must transport the same constituents")
+    icmax = max(c.grid.IC for c in cases)
+    masks = []
+    for c in cases:
+        m = c.grid.LIJ > 0
+        masks.append(np.pad(m, ((0, 0), (0, icmax - m.shape[1]))))
+    g = build_grid(np.concatenate(masks, axis=0), kc)
+    offs = np.cumsum([0] + [c.grid.nwet for c in cases])
+    first_L = (2 + offs[:-1]).astype(np.int32)
+    LC = g.LC
+    # the stacked numbering is the concatenation of the basins' numberings
+    for b, c in enumerate(cases):
+        n, o = c.grid.nwet, int(offs[b])
+        for mname, m in c.grid.maps.items():
+            old = m[1:1 + n]
+            want = np.where(old == c.grid.LC, LC, old + o)
+            if not np.array_equal(g.maps[mname][1 + o:1 + o + n], want):
+                raise AssertionError(f"map {mname} of basin {b} does not concatenate")
+
+    def cat(arrs, sel):
+        a0 = sel(c0)
+        out = np.zeros(a0.shape[:-1] + (g.LCM,), a0.dtype)
+        out[..., 0] = a0[..., 0]
+        out[..., LC - 1] = a0[..., c0.grid.LC - 1]
+        out[..., g.LCM - 1] = a0[..., c0.grid.LCM - 1]
+        for b, c in enumerate(cases):
+            n, o = c.grid.nwet, int(offs[b])
+            out[..., 1 + o:1 + o + n] = sel(c)[..., 1:1 + n]
+        return out
+
+    h = {k: cat(cases, lambda c, k=k: c.h[k]) for k in c0.h}
+    kk = {k: cat(cases, lambda c, k=k: c.k[k]) for k in c0.k}
+    w = {k: cat(cases, lambda c, k=k: c.w[k]) for k in c0.w}
+    consts = []
+    for j, x in enumerate(c0.consts):
+        d = dict(x)
+        d["con"] = cat(cases, lambda c, j=j: c.consts[j]["con"])
+        d["con1"] = cat(cases, lambda c, j=j: c.consts[j]["con1"])
+        consts.append(d)
+
+    def shifted(b, L):
+        return (np.asarray(L, np.int64) + int(offs[b])).astype(np.int32)
+
+    open_bc = None
+    if any(c.open_bc is not None for c in cases):
+        if any(c.open_bc is None for c in cases):
+            raise ValueError("either every basin or none has open boundaries")
+        mt = c0.open_bc["mtide"]
+        for c in cases:
+            if c.open_bc["mtide"] != mt or not np.array_equal(c.open_bc["tcp"], c0.open_bc["tcp"]):
+                raise ValueError("basins must share the tidal constituents")
+        sides = []
+        for s in range(4):
+            parts = [(b, c.open_bc["side"][s]) for b, c in enumerate(cases)
+                     if c.open_bc["side"][s] is not None]
+            if not parts:
+                sides.append(None)
+                continue
+            sides.append(dict(
+                L=np.concatenate([shifted(b, d["L"]) for b, d in parts]),
+                **{f: np.concatenate([np.asarray(d[f]) for _, d in parts], axis=0)
+                   for f in ("ispb", "pser", "pcb", "psb")}))
+        open_bc = dict(side=sides, mtide=mt, tcp=np.array(c0.open_bc["tcp"]))
+    conc_bc = None
+    if any(c.conc_bc is not None for c in cases):
+        if any(c.conc_bc is None for c in cases):
+            raise ValueError("either every basin or none has concentration BCs")
+        for c in cases:
+            if not np.array_equal(c.conc_bc["wtci"], c0.conc_bc["wtci"]):
+                raise ValueError("basins must share the concentration-BC weights")
+        sides = []
+        for s in range(4):
+            parts = [(b, c.conc_bc["side"][s]) for b, c in enumerate(cases)
+                     if c.conc_bc["side"][s] is not None]
+            if not parts:
+                sides.append(None)
+                continue
+            sides.append(dict(
+                L=np.concatenate([shifted(b, d["L"]) for b, d in parts]),
+                **{f: np.concatenate([np.asarray(d[f]) for _, d in parts], axis=0)
+                   for f in ("cb", "csert", "ntscr")}))
+        conc_bc = dict(side=sides, wtci=np.array(c0.conc_bc["wtci"]))
+    sources = None
+    parts = [(b, c.sources) for b, c in enumerate(cases) if c.sources is not None]
+    if parts:
+        sources = dict(LQS=np.concatenate([shifted(b, d["LQS"]) for b, d in parts]),
+                       **{f: np.concatenate([np.asarray(d[f]) for _, d in parts], axis=0)
+                          for f in ("qss", "qsercell", "cqs", "csert")})
+    manning = None
+    if any(c.manning_n is not None for c in cases):
+        if any(c.manning_n is None for c in cases):
+            raise ValueError("either every basin or none has a Manning n")
+        manning = cat(cases, lambda c: c.manning_n)
+    case = Case(name=name or f"ensemble-{len(cases)}", grid=g, h=h, k=kk, w=w,
+                consts=consts, open_bc=open_bc, conc_bc=conc_bc, sources=sources,
+                dt=c0.dt, manning_n=manning,
+                meta=dict(first_L=first_L, basins=[c.name for c in cases]))
+    return case, first_L
+
+
+def basin_slice(case: Case, first_L, b: int) -> slice:
+    """0-based index range of basin b's cells in a concatenated array."""
+    lo = int(first_L[b]) - 1
+    hi = int(first_L[b + 1]) - 1 if b + 1 < len(first_L) else case.grid.LA
+    return slice(lo, hi)

@@ -142,6 +142,9 @@ def load(path: Optional[str] = None):
         "efdc_closure_post": ([P], C.c_int),
         "efdc_timers_enable": ([P, C.c_int32], C.c_int),
         "efdc_timers_read": ([P, C.POINTER(Timers)], C.c_int),
+        "efdc_set_segments": ([P, C.c_int32, PI], C.c_int),
+        "efdc_segment_stats": ([P, C.c_int32, PI, PI], C.c_int),
+        "efdc_balance_segments": ([P, C.c_int32, PD, PD], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -331,6 +334,29 @@ class Context:
         b = Balance()
         self._check(self.lib.efdc_balance(self.h, C.byref(b)))
         return b.volume, np.array(b.mass[:b.nconst])
+
+    # ------------------------------------------- independent basins
+    def set_segments(self, first_L):
+        """Declare the basins of a concatenated case by their first L."""
+        a = np.ascontiguousarray(first_L, np.int32)
+        self._check(self.lib.efdc_set_segments(self.h, len(a), _pi(a)))
+        self.nseg = len(a)
+
+    def segment_stats(self):
+        """Per-basin CG iterations and status of the last solve."""
+        n = self.nseg
+        it = np.zeros(n, np.int32)
+        stt = np.zeros(n, np.int32)
+        self._check(self.lib.efdc_segment_stats(self.h, n, _pi(it), _pi(stt)))
+        return it, stt
+
+    def balance_segments(self):
+        """Per-basin volume (nseg,) and constituent mass (nseg, nconst)."""
+        n = self.nseg
+        vol = np.zeros(n)
+        mass = np.zeros((n, max(self.nconst, 1)))
+        self._check(self.lib.efdc_balance_segments(self.h, n, _pd(vol), _pd(mass)))
+        return vol, mass[:, :self.nconst]
 
     def synchronize(self):
         self._check(self.lib.efdc_synchronize(self.h))

@@ -283,6 +283,31 @@ int efdc_calconc(efdc_ctx_t *ctx, const efdc_step_t *step);
 /* FP64 volume and constituent-mass totals of the device state. */
 int efdc_balance(efdc_ctx_t *ctx, efdc_balance_t *out);
 
+/* ------------------------------------------------------------ ensembles */
+/* Declare that the wet cells form nseg independent basins: basin s holds
+   the cells L = first_L[s] .. first_L[s+1]-1 (the last one up to LA),
+   first_L[0] = 2, ascending.  No neighbour map may cross a basin.  From then
+   on every entry point advances all basins at once (the pointwise kernels
+   run once over all cells), while everything the reference computes per
+   model stays per basin: the min-diagonal scaling CCMNM of CALPUV2C
+   (EFDC/calpuv2c.f90:360-376), the CG recurrences, stop test and iteration
+   cap of CONGRAD (EFDC/congrad.f90:52-187), and the balances.  With the
+   FP64 CG, each basin's results are bit for bit those of the same basin in
+   a context of its own solved with EFDC_CG_KERNEL_MULTI.  nseg = 1 returns
+   to a single model.  No reference counterpart: the batched-ensemble
+   addition, for hosts that run many basins. */
+int efdc_set_segments(efdc_ctx_t *ctx, int32_t nseg, const int32_t *first_L);
+
+/* Per-basin results of the last efdc_calpuv2c / efdc_congrad with
+   segments: iters[s] and status[s] (0 or EFDC_ERR_CG_MAXITER) for s <
+   min(nseg, cap).  The stats of the call itself report the largest
+   iteration count and the first failure. */
+int efdc_segment_stats(efdc_ctx_t *ctx, int32_t cap, int32_t *iters, int32_t *status);
+
+/* FP64 volume and constituent-mass totals per basin: volume[s] and
+   mass[s*nconst + c] for s < min(nseg, cap). */
+int efdc_balance_segments(efdc_ctx_t *ctx, int32_t cap, double *volume, double *mass);
+
 /* Wait for all work queued on the context's stream. */
 int efdc_synchronize(efdc_ctx_t *ctx);
 

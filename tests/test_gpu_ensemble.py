@@ -1,0 +1,171 @@
+"""Ensembles of independent basins in one context (efdc_set_segments).
+
+Each basin of a concatenated case must give the same bits as the same basin
+run alone with the FP64 multi-kernel CG: every pointwise kernel sees the same
+inputs, the CCMNM scaling minimum is taken per basin, and the per-basin CG
+tiles start at the basin's first cell, so the dot-product partials and their
+reduction order are those of the basin alone.  Balances are per basin (FP64
+sums in another order: equal to 1e-13 relative).
+"""
+import numpy as np
+import pytest
+
+from conftest import eta_err
+
+pytestmark = pytest.mark.gpu
+
+RSQM = 1e-20
+H_FIELDS = ["P", "P1", "HP", "H1P", "HU", "HV", "UHDYE", "VHDXE", "UHDY1E", "VHDX1E",
+            "CCC", "CCS", "CCW", "FPTMP"]
+
+
+def _ctx(case, **kw):
+    from efdc_amd.lib import CG_FP64, CG_KERNEL_MULTI, Context
+    kw.setdefault("RSQM", RSQM)
+    kw.setdefault("ITERM", 2000)
+    kw.setdefault("cg_kernel", CG_KERNEL_MULTI)
+    kw.setdefault("cg_mode", CG_FP64)
+    return Context(case.grid, **kw).upload_case(case)
+
+
+def _tidal_ensemble():
+    from efdc_amd import grids
+    basins = [grids.tidal_basin(40, 20, 3, seed=21, amp=0.5),
+              grids.tidal_basin(24, 36, 3, seed=22, amp=0.3, depth=(3.0, 12.0)),
+              grids.tidal_basin(64, 12, 3, seed=23, amp=0.8, dye_load=2.0),
+              grids.tidal_basin(30, 30, 3, seed=24, amp=0.1, manning=0.04),
+              grids.ensemble_member(0, kc=3)]
+    ens, first = grids.concat_basins(basins)
+    return basins, ens, first
+
+
+def test_tidal_ensemble_bitwise_equals_separate_basins(oracle_mod):
+    """Five basins (four rectangles, one irregular lake with a west
+    boundary segment), 30 full steps: bitwise equal to the basins run
+    alone, and within 1e-8 m / 1e-9 relative of the FP64 oracle run on
+    each basin."""
+    from efdc_amd import grids
+    basins, ens, first = _tidal_ensemble()
+    refs = [b.copy() for b in basins]
+    oracles = [oracle_mod.Oracle(r, rsqm=RSQM, iterm=2000) for r in refs]
+    singles = [_ctx(b) for b in basins]
+    ctx = _ctx(ens)
+    ctx.set_segments(first)
+    nsteps = 30
+    for n in range(1, nsteps + 1):
+        for o in oracles:
+            o.step(n)
+        its = [s.step(b.dt, n)["iters"] for s, b in zip(singles, basins)]
+        st = ctx.step(ens.dt, n)
+        it_seg, status = ctx.segment_stats()
+        assert list(it_seg) == its, (n, list(it_seg), its)
+        assert st["iters"] == max(its)
+        assert not status.any()
+    kc = ens.grid.KC
+    for b, (s, basin) in enumerate(zip(singles, basins)):
+        sl = grids.basin_slice(ens, first, b)
+        nw = basin.grid.nwet
+        for name in H_FIELDS:
+            got = ctx.get(name)[sl]
+            want = s.get(name)[1:1 + nw]
+            assert np.array_equal(got, want), (b, name, np.max(np.abs(got - want)))
+        for j in range(len(basin.consts)):
+            con_e, con1_e = ctx.get_constituent(j)
+            con_s, con1_s = s.get_constituent(j)
+            assert np.array_equal(con_e[:kc, sl], con_s[:kc, 1:1 + nw]), (b, j)
+            assert np.array_equal(con1_e[:kc, sl], con1_s[:kc, 1:1 + nw]), (b, j)
+    p = ctx.get("P")
+    for b, (r, basin) in enumerate(zip(refs, basins)):
+        sl = grids.basin_slice(ens, first, b)
+        nw = basin.grid.nwet
+        d = eta_err(p[sl], r.h["P"][1:1 + nw])
+        assert d <= 1e-8, (b, d)
+        for j, c in enumerate(r.consts):
+            con, _ = ctx.get_constituent(j)
+            want = c["con"][:kc, 1:1 + nw]
+            rel = np.max(np.abs(con[:kc, sl] - want)) / max(1e-30, np.max(np.abs(want)))
+            assert rel <= 1e-9, (b, j, rel)
+    vol, mass = ctx.balance_segments()
+    for b, s in enumerate(singles):
+        v, m = s.balance()
+        np.testing.assert_allclose(vol[b], v, rtol=1e-13)
+        np.testing.assert_allclose(mass[b], m, rtol=1e-13)
+    print("basins", len(basins), "cells", ens.grid.nwet, "last iters", list(it_seg))
+    for s in singles:
+        s.close()
+    ctx.close()
+
+
+def test_lake_ensemble_external_solve_bitwise():
+    """Irregular lake masks (ragged rows, islands) through CALPUV2C."""
+    from efdc_amd import grids
+    basins = [grids.external_case(grids.lake_mask(n, n, seed), seed=seed)
+              for n, seed in ((48, 31), (80, 32), (33, 33), (64, 34), (20, 35))]
+    ens, first = grids.concat_basins(basins)
+    singles = [_ctx(b) for b in basins]
+    ctx = _ctx(ens)
+    ctx.set_segments(first)
+    for n in (1, 2, 3):
+        its = [s.calpuv2c(b.dt, n)["iters"] for s, b in zip(singles, basins)]
+        ctx.calpuv2c(ens.dt, n)
+        assert list(ctx.segment_stats()[0]) == its
+    for b, (s, basin) in enumerate(zip(singles, basins)):
+        sl = grids.basin_slice(ens, first, b)
+        nw = basin.grid.nwet
+        for name in ("P", "HP", "UHDYE", "VHDXE"):
+            assert np.array_equal(ctx.get(name)[sl], s.get(name)[1:1 + nw]), (b, name)
+    for s in singles:
+        s.close()
+    ctx.close()
+
+
+def test_segment_iteration_cap_is_per_basin():
+    from efdc_amd.lib import EfdcError
+    _, ens, first = _tidal_ensemble()
+    ctx = _ctx(ens, RSQM=1e-30, ITERM=3)
+    ctx.set_segments(first)
+    with pytest.raises(EfdcError) as e:
+        ctx.calpuv2c(ens.dt, 1)
+    assert e.value.status == 3     # EFDC_ERR_CG_MAXITER
+    it, status = ctx.segment_stats()
+    assert list(it) == [3] * len(first)
+    assert list(status) == [3] * len(first)
+    ctx.close()
+
+
+def test_set_segments_rejects_bad_tables():
+    from efdc_amd import grids
+    from efdc_amd.lib import CG_MIXED, EfdcError
+    _, ens, first = _tidal_ensemble()
+    ctx = _ctx(ens)
+    for bad in ([3, 100], [2, int(first[2]), int(first[1])], [2, ens.grid.LA + 1]):
+        with pytest.raises(EfdcError) as e:
+            ctx.set_segments(bad)
+        assert e.value.status == 1
+    # a split inside a basin: its neighbour maps cross the split
+    with pytest.raises(EfdcError) as e:
+        ctx.set_segments([2, int(first[1]) + 40])
+    assert e.value.status == 1 and "crosses" in str(e.value)
+    ctx.close()
+    # the per-basin solve is the FP64 one
+    ctx = _ctx(ens, cg_mode=CG_MIXED)
+    ctx.set_segments(first)
+    with pytest.raises(EfdcError) as e:
+        ctx.calpuv2c(ens.dt, 1)
+    assert e.value.status == 1
+    ctx.close()
+
+
+def test_single_segment_is_the_plain_model():
+    """set_segments([2]) returns to the one-model path, bit for bit."""
+    _, ens, first = _tidal_ensemble()
+    a = _ctx(ens)
+    b = _ctx(ens)
+    b.set_segments(first)
+    b.set_segments([2])
+    for n in (1, 2):
+        a.step(ens.dt, n)
+        b.step(ens.dt, n)
+    assert eta_err(a.get("P"), b.get("P")) == 0.0
+    a.close()
+    b.close()

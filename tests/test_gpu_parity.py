@@ -317,3 +317,35 @@ def test_cg_iteration_cap_status_on_gpu(kernel, mode):
     with pytest.raises(EfdcError) as e:
         ctx.calpuv2c(case.dt, 1)
     assert e.value.status == 3     # EFDC_ERR_CG_MAXITER
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_config0_720_steps(oracle_mod, mode):
+    """The reference's CPU-runnable case end to end: 128x64 basin, KC=5,
+    M2 tide on the west boundary, salinity and a 1 kg/s dye source,
+    dt = 5 s for 720 steps (one hour), every step the full hot path
+    (closure, CALPUV2C, closure, CALCONC).  Elevation within 1e-6 m and
+    concentrations within 1e-5 relative of the FP64 oracle (the north-star
+    bars; the FP64 GPU path meets them by orders of magnitude)."""
+    from efdc_amd import grids
+    from efdc_amd.lib import CG_FP64, CG_MIXED
+    case = grids.config0()
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref, rsqm=1e-20, iterm=2000)
+    ctx = _ctx(case, RSQM=1e-20, cg_mode=CG_MIXED if mode == "mixed" else CG_FP64)
+    for n in range(1, 721):
+        o.step(n)
+        ctx.step(case.dt, n)
+    i = wet_index(case)
+    d = eta_err(ctx.get("P")[i], ref.h["P"][i])
+    kc = case.grid.KC
+    rels = []
+    for s, c in enumerate(ref.consts):
+        con, _ = ctx.get_constituent(s)
+        scale = max(1e-30, np.max(np.abs(c["con"][:kc])))
+        rels.append(float(np.max(np.abs(con[:kc] - c["con"][:kc])) / scale))
+    print(f"{mode}: 720 steps |d eta| {d:.3e} m, constituents rel {rels}")
+    assert d <= 1e-6
+    assert max(rels) <= 1e-5
+    if mode == "fp64":
+        assert d <= 1e-9 and max(rels) <= 1e-9
