@@ -347,6 +347,121 @@ def ensemble_bench(args):
     print(json.dumps(out))
 
 
+def adjoint_bench(args):
+    """Adjoint sensitivity (--workload adjoint): a 512x512 basin with KC=8
+    (built like the config-0 basin: west M2 tide, salinity, n = 0.025), 16
+    dye sources whose loads are 200-step series in U(0,5), 50 observed wet
+    cells every 10 steps, data from a seeded true forcing plus 1 % noise.
+    One gradient = forward run with a checkpoint every 20 steps, then the
+    checkpointed reverse sweep (segment recompute, step repeat, adjoint
+    step) giving dJ/dloads (200 x 16) and dJ/dn (per cell, coarsened to
+    64 x 64).  FP32 storage forward, FP64 adjoint.  value = column-steps/s
+    of forward+adjoint (wet columns x steps / gradient time).  Checked
+    against central differences of the same forward model."""
+    import numpy as np
+    from efdc_amd import adjoint, grids, lib
+
+    n, kc, nsteps = args.an, 8, args.asteps
+    case, obs = grids.adjoint_case(n, kc, 16, 50)
+    g = case.grid
+    rng = np.random.default_rng(1004)
+    store = lib.STORE_FP32 if args.astore == "fp32" else lib.STORE_FP64
+    prob = adjoint.DyeMisfit(case, nsteps=nsteps, obs_L=obs, obs_every=10, ckpt=20,
+                             store=store)
+    true = rng.uniform(0.0, 5.0, (nsteps, 16))
+    y = prob.observations(true)
+    prob.data = y * (1.0 + 0.01 * rng.standard_normal(y.shape))
+    loads0 = rng.uniform(0.0, 5.0, (nsteps, 16))
+    cidx = adjoint.coarse_index(g, 64, 64)
+    # warm-up gradient, then the timed ones
+    prob.gradient(loads0)
+    prob.ctx.synchronize()
+    t0 = time.perf_counter()
+    reps = max(1, args.steps // nsteps) if args.steps >= nsteps else 1
+    for _ in range(reps):
+        J, gl, gn, st = prob.gradient(loads0)
+    prob.ctx.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    # forward alone, for the overhead ratio
+    t0 = time.perf_counter()
+    prob.misfit(loads0)
+    prob.ctx.synchronize()
+    t_fwd = time.perf_counter() - t0
+    # per-kernel time of one gradient
+    prob.ctx.timers(True)
+    prob.gradient(loads0)
+    tm = prob.ctx.read_timers()
+    prob.ctx.timers(False)
+    # gradient checks: central differences of the same forward model
+    d = rng.standard_normal(loads0.shape)
+    fd = (prob.misfit(loads0 + d) - prob.misfit(loads0 - d)) / 2.0
+    ad = float(np.sum(gl * d))
+    gc = np.zeros(64 * 64)
+    m = cidx >= 0
+    np.add.at(gc, cidx[m], gn[m])
+    dn = rng.uniform(-1.0, 1.0, 64 * 64)
+    n0 = case.manning_n.copy()
+    h = 1e-3
+    pert = np.zeros(g.LCM)
+    pert[m] = dn[cidx[m]] * n0[m]
+    fdn = (prob.misfit(loads0, n0 + h * pert) - prob.misfit(loads0, n0 - h * pert)) / (2 * h)
+    adn = float(np.sum(gn * pert))
+    # bytes of the dominant adjoint kernel: the transport column pass reads
+    # the flows, depths, layer constants and old level (8 storage-type
+    # layer arrays), lambda of CON, writes lcd/own and 5 FP64 scratch
+    # planes, and reads the scratch planes back (KC*(8*s + 8 + 2*8 + 10*8))
+    s_ = 4 if store == lib.STORE_FP32 else 8
+    per_col = {"adj_tran_col": kc * (8 * s_ + 8 + 16 + 80) + 64,
+               "adj_tran_gather": kc * (4 * s_ + 3 * 8 + 8) + 64}
+    cand = {k: tm[k] for k in per_col if k in tm}
+    dom = max(cand, key=lambda k: cand[k][1])
+    cnt, ms = cand[dom]
+    ach = per_col[dom] * g.nwet * cnt / (ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        ref = case.copy()
+        o = orc.Oracle(ref, rsqm=0.0, iterm=4000, nthreads=args.cpu_threads, rel_tol=1e-13)
+        t_cpu, done = 0.0, 0
+        while done < 2 or (t_cpu < args.cpu_seconds and done < 10):
+            t1 = time.perf_counter()
+            o.step(done + 1)
+            t_cpu += time.perf_counter() - t1
+            done += 1
+        cpu = dict(value=g.nwet * done / t_cpu, unit="column-steps/s", cores=min(args.cpu_threads, 8),
+                   kind="port",
+                   sample=f"{done} forward steps of the same {n}x{n}x{kc} basin by the FP64 C "
+                          f"restatement, {t_cpu:.1f} s (the CPU path has no adjoint: forward only)")
+    out = {
+        "metric": "column-steps/s of forward+adjoint (dye-misfit gradient w.r.t. loads and Manning n)",
+        "value": g.nwet * nsteps / t,
+        "unit": "column-steps/s",
+        "n_gpus": 1, "steps": nsteps, "warmup": 1,
+        "ms_per_step": 1e3 * t / nsteps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if s_ == 4 else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"adjoint sensitivity: {n}x{n}x{kc} basin, 16 dye sources x "
+                               f"{nsteps} steps, 50 observed cells every 10 steps, "
+                               "checkpoint every 20 steps", "columns": g.nwet},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "avg_us": ms / cnt * 1e3, "bytes_per_launch": per_col[dom] * g.nwet},
+        "cpu_baseline": cpu,
+        "detail": {
+            "J": J, "gradient_s": t, "forward_s": t_fwd, "gradient_over_forward": t / t_fwd,
+            "load_grad_vs_central_difference": {"adjoint": ad, "fd": fd,
+                                                "rel": abs(ad - fd) / abs(fd)},
+            "manning_grad_vs_central_difference": {"adjoint": adn, "fd": fdn,
+                                                   "rel": abs(adn - fdn) / abs(fdn), "h": h},
+            "adj_cg_iters_mean": float(np.mean(st["adj_cg_iters"])),
+            "kernels": {k: dict(launches=c_, avg_us=1e3 * m_ / c_, total_ms=m_)
+                        for k, (c_, m_) in tm.items()},
+        },
+    }
+    prob.close()
+    print(json.dumps(out))
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
@@ -393,12 +508,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("lake1024", "transport", "ensemble"),
+    ap.add_argument("--workload", choices=("lake1024", "transport", "ensemble", "adjoint"),
                     default="lake1024",
                     help="lake1024: the external solve at scale (the headline); "
                          "transport: CALCONC at scale; ensemble: 256 basins in one "
                          "context")
     ap.add_argument("--nbasins", type=int, default=256)
+    ap.add_argument("--an", type=int, default=512, help="adjoint: basin size")
+    ap.add_argument("--asteps", type=int, default=200, help="adjoint: steps")
+    ap.add_argument("--astore", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--ekc", type=int, default=8)
     ap.add_argument("--seq-steps", type=int, default=10,
                     help="ensemble: steps of the sequential single-basin comparison")
@@ -425,6 +543,9 @@ def main():
         return
     if args.workload == "ensemble":
         ensemble_bench(args)
+        return
+    if args.workload == "adjoint":
+        adjoint_bench(args)
         return
 
     import numpy as np

@@ -98,7 +98,7 @@ const char *const efdc_timer_names[TK_COUNT] = {
     "puv_depths", "puv_hu_hv", "tran_upwind_vdiff", "tran_bc_con1",
     "closure_pre", "closure_post", "balance", "adac_gradients",
     "adac_fluxes", "fct_limiters", "tran_finish_vdiff", "cg_persistent64",
-    "mx_persistent"};
+    "mx_persistent", "adj_tran_col", "adj_tran_gather", "adj_hydro", "adj_cg"};
 
 hipEvent_t timer_event(efdc_ctx *c) {
   if (c->ev_pool.empty()) {
@@ -293,6 +293,7 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   { char *q = (char *)c->pcg_ps; dfree(q); }
   dfree(c->pcg_ctr); dfree(c->pcg_abort); dfree(c->pcg_flag);
   seg_free(c);
+  adj_free(c);
   if (c->st_host) (void)hipHostFree(c->st_host);
   if (c->tide_host) (void)hipHostFree(c->tide_host);
   for (auto &b : c->pbc) {
@@ -498,7 +499,9 @@ int efdc_set_open_bc(efdc_ctx_t *c, const efdc_open_bc_t *bc) {
     std::vector<unsigned char> lf(c->LCM, 0);
     for (int j : lob) lf[j] |= 1;
     for (int j : lob2) if (j >= 0 && j < c->LCM) lf[j] |= 2;
-    HIP_TRY(c, hipMemcpy(c->lobflag, lf.data(), c->LCM, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpyAsync(c->lobflag, lf.data(), c->LCM, hipMemcpyHostToDevice,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
   }
   c->mtide = bc->mtide;
   delete[] c->tcp_host;
@@ -706,6 +709,18 @@ int efdc_set_sources(efdc_ctx_t *c, const efdc_sources_t *src) {
       (st = upload_new(c, &c->csertq, src->csert, (size_t)n * KC * nc)))
     return st;
   HIP_TRY(c, hipMemcpyAsync(c->srcrec, rec.data(), sizeof(int) * c->LCM,
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
+int efdc_set_source_cqs(efdc_ctx_t *c, const double *cqs) {
+  if (!c || !cqs) return EFDC_ERR_ARG;
+  if (c->nqsij == 0) {
+    c->err = "efdc_set_source_cqs: no source table (efdc_set_sources)";
+    return EFDC_ERR_STATE;
+  }
+  HIP_TRY(c, hipMemcpyAsync(c->cqs, cqs, sizeof(double) * (size_t)c->nqsij * c->KC * c->src_nc,
                             hipMemcpyHostToDevice, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return EFDC_OK;

@@ -186,6 +186,9 @@ struct efdc_ctx {
   struct TimerRec { int id; hipEvent_t a, b; };
   std::vector<TimerRec> trec;
 
+  // discrete adjoint (efdc_adj.hip), allocated by efdc_adj_enable
+  struct AdjState *adj = nullptr;
+
   int n_wet() const { return LA - 1; }
 };
 
@@ -195,7 +198,8 @@ enum {
   TK_CG_INIT, TK_CG_A, TK_CG_B, TK_MX_PREP, TK_MX_RESID, TK_MX_A, TK_MX_B,
   TK_MX_APPLY, TK_PUV_POST1, TK_PUV_POST2, TK_PUV_POST3, TK_TRAN,
   TK_TRAN_BC, TK_CLOSURE_PRE, TK_CLOSURE_POST, TK_BALANCE, TK_ADAC_GRAD,
-  TK_ADAC_FLUX, TK_FCT, TK_COL_FINISH, TK_PCG64, TK_PCGMX, TK_COUNT
+  TK_ADAC_FLUX, TK_FCT, TK_COL_FINISH, TK_PCG64, TK_PCGMX, TK_ADJ_TCOL, TK_ADJ_TGATHER,
+  TK_ADJ_HYDRO, TK_ADJ_CG, TK_COUNT
 };
 extern const char *const efdc_timer_names[TK_COUNT];
 
@@ -426,6 +430,15 @@ int pcg_solve(efdc_ctx *c, bool mixed);
 int seg_min_run(efdc_ctx *c);
 int seg_cg_run(efdc_ctx *c, efdc_cg_stats_t *stats);
 void seg_free(efdc_ctx *c);
+// discrete adjoint (efdc_adj.hip): forward-step records and clean-up
+bool adj_recording(const efdc_ctx *c);
+int adj_record_ps(efdc_ctx *c);
+int adj_record_c0(efdc_ctx *c);
+void adj_free(efdc_ctx *c);
+// FP64 CG on the context's scaled operator for another right-hand side
+// and solution vector (x holds the initial guess), with its own state
+int cg_fp64_vec(efdc_ctx *c, double *x, const double *rhs, CgState *st, CgState *st_host,
+                double rel_tol, int itmax, int *iters, double *rsq);
 int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats);
 int cg_run(efdc_ctx *c, bool check_sym, efdc_cg_stats_t *stats);
 int tran_run(efdc_ctx *c, const efdc_step_t *step);

@@ -1032,6 +1032,39 @@ int cg_dispatch(efdc_ctx *c, efdc_cg_stats_t *stats) {
 
 }  // namespace
 
+int cg_fp64_vec(efdc_ctx *c, double *x, const double *rhs, CgState *st, CgState *st_host,
+                double rel_tol, int itmax, int *iters, double *rsq) {
+  const int nb = cg_blocks(c->n_wet());
+  CgArgs a = cg_args(c);
+  a.x = x;
+  a.fptmp = rhs;
+  a.st = st;
+  // the operator's symmetry flag (row_of's compact form) comes from the
+  // forward solve of the same coefficients
+  HIP_TRY(c, hipMemcpyAsync(st, c->st, sizeof(CgState), hipMemcpyDeviceToDevice, c->stream));
+  TIMED(c, TK_ADJ_CG, k_cg_init<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, 0.0, rel_tol));
+  int it = 0, chunk = 8;
+  for (;;) {
+    for (int j = 0; j < chunk && it < itmax; ++j, ++it) {
+      TIMED(c, TK_ADJ_CG, k_cg_a<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->pb[it & 1],
+                                                                  c->pb[(it + 1) & 1]));
+      TIMED(c, TK_ADJ_CG, k_cg_b<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->pb[(it + 1) & 1],
+                                                                  itmax));
+    }
+    HIP_TRY(c, hipMemcpyAsync(st_host, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (st_host->done || it >= itmax) break;
+    chunk = 8;
+  }
+  *iters = st_host->iter;
+  *rsq = st_host->rsq;
+  if (!st_host->done || st_host->status) {
+    c->err = "adjoint CG: no convergence within the iteration cap";
+    return EFDC_ERR_CG_MAXITER;
+  }
+  return EFDC_OK;
+}
+
 int cg_run(efdc_ctx *c, bool check_sym, efdc_cg_stats_t *stats) {
   const int nb = efdc_grid_blocks(c->n_wet());
   if (check_sym) {
@@ -1124,6 +1157,7 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
   int s = cg_dispatch(c, stats);
   if (s != EFDC_OK && s != EFDC_ERR_CG_MAXITER) return s;
   const int cgs = s;
+  if (adj_recording(c) && (s = adj_record_ps(c))) return s;
   TIMED(c, TK_PUV_POST1, k_puv_post1<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, dt));
   TIMED(c, TK_PUV_POST2, k_puv_post2<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->pbcflag,
                                                 nwet, dt, G));

@@ -110,7 +110,8 @@ int efdc_set_segments(efdc_ctx_t *c, int32_t nseg, const int32_t *first_L) {
   int st;
   auto up = [&](auto **dst, const auto *src, size_t n) -> int {
     if ((st = dev_alloc(c, (void **)dst, sizeof(*src) * n))) return st;
-    HIP_TRY(c, hipMemcpy(*dst, src, sizeof(*src) * n, hipMemcpyHostToDevice));
+    // on the context's (non-blocking) stream, after dev_alloc's zero fill
+    HIP_TRY(c, hipMemcpyAsync(*dst, src, sizeof(*src) * n, hipMemcpyHostToDevice, c->stream));
     return EFDC_OK;
   };
   if (up(&c->d_segid, segid.data(), LCM) || up(&c->d_segfirst, first.data(), nseg + 1) ||

@@ -1057,6 +1057,10 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
   a.bc_con1 = c->bc_con1;
   a.vd = any_adac ? 0 : 1;
 
+  if (adj_recording(c)) {
+    const int s = adj_record_c0(c);
+    if (s) return s;
+  }
   const bool off32 = (int64_t)c->pitch * (c->KC + 1) < ((int64_t)1 << 31);
   if (KC <= 64 && off32) {
     // one thread per (column, constituent)

@@ -425,6 +425,59 @@ def config3(nbasins: int = 256, kc: int = 8):
     return case, first, members
 
 
+def with_dye_sources(case: Case, cells, q: float = 1e-3, dye_slot: int = 1) -> Case:
+    """Replace the source table of a basin with one inflow per cell in
+    ``cells`` (L values): QSS = q split over the layers by DZC, dye CQS = 0
+    (the loads are set per step), the other constituents enter at the local
+    concentration.  QSUME/QSUM follow."""
+    c = case.copy()
+    g = c.grid
+    kc = g.KC
+    nc = len(c.consts)
+    cells = np.asarray(cells, np.int32)
+    ns = len(cells)
+    qss = np.zeros((ns, kc))
+    cqs = np.zeros((ns, kc, nc))
+    for s_, L in enumerate(cells):
+        qss[s_] = q * c.k["DZC"][:kc, L - 1]
+        for j, x in enumerate(c.consts):
+            if j != dye_slot:
+                cqs[s_, :, j] = x["con"][:kc, L - 1]
+    c.sources = dict(LQS=cells, qss=qss, qsercell=np.zeros((ns, kc)), cqs=cqs,
+                     csert=np.zeros((ns, kc, nc)))
+    c.h["QSUME"][:] = 0.0
+    c.k["QSUM"][:] = 0.0
+    for s_, L in enumerate(cells):
+        c.h["QSUME"][L - 1] += q
+        c.k["QSUM"][:kc, L - 1] += qss[s_]
+    return c
+
+
+def adjoint_case(n=512, kc=8, nsrc=16, nobs=50, seed=1004, reach=3):
+    """Adjoint workload: an n x n basin built like the config-0 basin (west
+    tidal boundary, salinity, Manning n = 0.025) with ``nsrc`` dye sources
+    at random interior wet cells and ``nobs`` observed wet cells drawn at
+    random among the cells within ``reach`` cells (in I and J) of a source,
+    where the dye of a 200-step window arrives.  Returns (case, obs_L)."""
+    rng = np.random.default_rng(seed)
+    base = tidal_basin(n, n, kc, seed=seed, name=f"adjoint-{n}x{n}x{kc}")
+    g = base.grid
+    L = np.arange(2, g.LA + 1)
+    ii, jj = g.IL[L - 1], g.JL[L - 1]
+    inner = L[(ii > 1 + reach) & (ii < n - reach) & (jj > reach) & (jj <= n - reach)]
+    src = np.sort(rng.choice(inner, size=nsrc, replace=False))
+    near = set()
+    for s_ in src:
+        i0, j0 = g.IL[s_ - 1], g.JL[s_ - 1]
+        for dj in range(-reach, reach + 1):
+            for di in range(-reach, reach + 1):
+                Lq = int(g.LIJ[j0 - 1 + dj, i0 - 1 + di])   # IL, JL are 1-based
+                if Lq > 0:
+                    near.add(Lq)
+    obs = rng.choice(np.array(sorted(near)), size=min(nobs, len(near)), replace=False)
+    return with_dye_sources(base, src), np.sort(obs).astype(np.int32)
+
+
 def config0(**kw) -> Case:
     return tidal_basin(128, 64, 5, seed=kw.pop("seed", 1000), name="cfg0", **kw)
 

@@ -99,6 +99,18 @@ class Balance(C.Structure):
                 ("nconst", C.c_int32)]
 
 
+class AdjOpts(C.Structure):
+    _fields_ = [("nadj", C.c_int32), ("slot", C.c_int32 * 16), ("cg_rel_tol", C.c_double),
+                ("cg_itmax", C.c_int32)]
+
+
+class AdjStats(C.Structure):
+    _fields_ = [("cg_iters", C.c_int32), ("cg_rsq", C.c_double)]
+
+
+ADJ_MANNING, ADJ_HP, ADJ_UHDYE, ADJ_VHDXE, ADJ_P, ADJ_HU, ADJ_HV = range(7)
+ADJ_CON = 16
+
 _lib = None
 
 
@@ -142,6 +154,17 @@ def load(path: Optional[str] = None):
         "efdc_closure_post": ([P], C.c_int),
         "efdc_timers_enable": ([P, C.c_int32], C.c_int),
         "efdc_timers_read": ([P, C.POINTER(Timers)], C.c_int),
+        "efdc_set_source_cqs": ([P, PD], C.c_int),
+        "efdc_state_bytes": ([P, C.POINTER(C.c_size_t)], C.c_int),
+        "efdc_state_save": ([P, P], C.c_int),
+        "efdc_state_load": ([P, P], C.c_int),
+        "efdc_adj_enable": ([P, C.POINTER(AdjOpts)], C.c_int),
+        "efdc_adj_disable": ([P], C.c_int),
+        "efdc_adj_reset": ([P], C.c_int),
+        "efdc_adj_seed_con": ([P, C.c_int32, C.c_int32, PI, PI, PD], C.c_int),
+        "efdc_adj_step": ([P, C.POINTER(Step), C.POINTER(AdjStats)], C.c_int),
+        "efdc_adj_source_grad": ([P, PD], C.c_int),
+        "efdc_adj_get": ([P, C.c_int32, PD], C.c_int),
         "efdc_set_segments": ([P, C.c_int32, PI], C.c_int),
         "efdc_segment_stats": ([P, C.c_int32, PI, PI], C.c_int),
         "efdc_balance_segments": ([P, C.c_int32, PD, PD], C.c_int),
@@ -334,6 +357,56 @@ class Context:
         b = Balance()
         self._check(self.lib.efdc_balance(self.h, C.byref(b)))
         return b.volume, np.array(b.mass[:b.nconst])
+
+    # ------------------------------------------------ snapshots, adjoint
+    def state_bytes(self) -> int:
+        n = C.c_size_t(0)
+        self._check(self.lib.efdc_state_bytes(self.h, C.byref(n)))
+        return n.value
+
+    def state_save(self, dev_ptr: int):
+        self._check(self.lib.efdc_state_save(self.h, C.c_void_p(dev_ptr)))
+
+    def state_load(self, dev_ptr: int):
+        self._check(self.lib.efdc_state_load(self.h, C.c_void_p(dev_ptr)))
+
+    def set_source_cqs(self, cqs):
+        a = np.ascontiguousarray(cqs, np.float64).reshape(-1)
+        self._check(self.lib.efdc_set_source_cqs(self.h, _pd(a)))
+
+    def adj_enable(self, slots, cg_rel_tol=1e-12, cg_itmax=0):
+        o = AdjOpts(nadj=len(slots), cg_rel_tol=cg_rel_tol, cg_itmax=cg_itmax)
+        for j, s_ in enumerate(slots):
+            o.slot[j] = s_
+        self._check(self.lib.efdc_adj_enable(self.h, C.byref(o)))
+
+    def adj_reset(self):
+        self._check(self.lib.efdc_adj_reset(self.h))
+
+    def adj_seed_con(self, slot, L, K, val):
+        L = np.ascontiguousarray(L, np.int32)
+        K = np.ascontiguousarray(K, np.int32)
+        v = np.ascontiguousarray(val, np.float64)
+        self._check(self.lib.efdc_adj_seed_con(self.h, slot, len(L), _pi(L), _pi(K), _pd(v)))
+
+    def adj_step(self, dt: float, n: int, niter: Optional[int] = None) -> dict:
+        st = Step(dt=dt, n=n, niter=n if niter is None else niter)
+        s = AdjStats()
+        self._check(self.lib.efdc_adj_step(self.h, C.byref(st), C.byref(s)))
+        return dict(cg_iters=s.cg_iters, cg_rsq=s.cg_rsq)
+
+    def adj_source_grad(self, nsrc: int):
+        g = self.grid
+        out = np.zeros(nsrc * g.KC * max(self.nconst, 1))
+        self._check(self.lib.efdc_adj_source_grad(self.h, _pd(out)))
+        return out.reshape(nsrc, g.KC, max(self.nconst, 1))
+
+    def adj_get(self, field: int):
+        g = self.grid
+        shape = (g.KCM, g.LCM) if field >= ADJ_CON else (g.LCM,)
+        out = np.zeros(shape)
+        self._check(self.lib.efdc_adj_get(self.h, field, _pd(out)))
+        return out
 
     # ------------------------------------------- independent basins
     def set_segments(self, first_L):

@@ -252,6 +252,10 @@ int efdc_get_array(efdc_ctx_t *ctx, int32_t id, double *host);
 int efdc_set_open_bc(efdc_ctx_t *ctx, const efdc_open_bc_t *bc);
 int efdc_set_conc_bc(efdc_ctx_t *ctx, const efdc_conc_bc_t *bc);
 int efdc_set_sources(efdc_ctx_t *ctx, const efdc_sources_t *src);
+/* Replace only the source concentrations CQS of the current table (same
+   layout as efdc_sources_t.cqs), e.g. a load series advanced every step
+   the way CALFQC interpolates CQSER (EFDC/calfqc.f90:42-90). */
+int efdc_set_source_cqs(efdc_ctx_t *ctx, const double *cqs);
 
 /* Register a transported constituent (the CALTRAN calls of
    EFDC/calconc.f90:159-228); returns its slot in *slot. */
@@ -347,6 +351,67 @@ int efdc_device_ptr(efdc_ctx_t *ctx, int32_t id, void **ptr, int64_t *pitch);
 int efdc_closure_set_manning(efdc_ctx_t *ctx, const double *n_lcm);
 int efdc_closure_pre(efdc_ctx_t *ctx);
 int efdc_closure_post(efdc_ctx_t *ctx);
+
+/* ------------------------------------------------------ state snapshots
+   No reference counterpart (checkpointing for the adjoint; the reference
+   restarts from RESTART.OUT files).  A snapshot holds every array,
+   constituent level, open-BC recovery memory and dry-boundary flag of the
+   context, so that loading it and repeating a step reproduces the step
+   bit for bit.  The buffer is device memory owned by the caller, at least
+   efdc_state_bytes() long. */
+int efdc_state_bytes(efdc_ctx_t *ctx, size_t *bytes);
+int efdc_state_save(efdc_ctx_t *ctx, void *dev_buf);
+int efdc_state_load(efdc_ctx_t *ctx, const void *dev_buf);
+
+/* --------------------------------------------------- discrete adjoint
+   No reference counterpart (gradient addition).  The reverse of one
+   hot-path step with the depth-uniform closure, in this order forward:
+     efdc_closure_pre (when a Manning field is set), efdc_calpuv2c,
+     efdc_closure_post, efdc_calconc.
+   The adjoint state is lambda = dJ/d(state after the step) for the
+   constituents named in efdc_adj_opts_t and for HP, UHDYE, VHDXE, P, HU,
+   HV; efdc_adj_step maps it to dJ/d(state before the step), accumulates
+   dJ/dn (Manning, per cell) and leaves dJ/dCQS of that step.  The solve
+   A P = FP is reversed by a second CG on the same operator (A is symmetric
+   on the cells that are not clamped open boundaries).  Requirements:
+   clamped pressure boundaries (ISPB=0), no anti-diffusion on the adjoint
+   constituents, one basin, mutual neighbour maps.
+   Use: enable; run the forward step from the state before it (the context
+   records the pieces of the step the adjoint reads: the CG solution and
+   the old levels of the adjoint constituents); add the misfit gradient of
+   that step's state with efdc_adj_seed_con; call efdc_adj_step with the
+   same step; read efdc_adj_source_grad; load the state before the
+   previous step, and so on backwards. */
+typedef struct {
+  int32_t nadj;                 /* constituents differentiated            */
+  int32_t slot[16];             /* their slots                            */
+  double cg_rel_tol;            /* stop of the adjoint CG: ||r|| <= tol*||b|| (0: 1e-12) */
+  int32_t cg_itmax;             /* 0: ITERM                               */
+} efdc_adj_opts_t;
+
+typedef struct {
+  int32_t cg_iters;             /* iterations of the adjoint solve        */
+  double cg_rsq;                /* its final squared residual (scaled)    */
+} efdc_adj_stats_t;
+
+typedef enum {
+  EFDC_ADJ_MANNING = 0,         /* (LCM) accumulated dJ/dn                */
+  EFDC_ADJ_HP, EFDC_ADJ_UHDYE, EFDC_ADJ_VHDXE, EFDC_ADJ_P, EFDC_ADJ_HU, EFDC_ADJ_HV,
+  EFDC_ADJ_CON = 16             /* + slot: (KCM,LCM) lambda of CON        */
+} efdc_adj_field;
+
+int efdc_adj_enable(efdc_ctx_t *ctx, const efdc_adj_opts_t *opts);
+int efdc_adj_disable(efdc_ctx_t *ctx);
+/* zero every lambda and the Manning gradient (tables must be set) */
+int efdc_adj_reset(efdc_ctx_t *ctx);
+/* lambda of CON(L,K) of constituent `slot` += val, for n entries */
+int efdc_adj_seed_con(efdc_ctx_t *ctx, int32_t slot, int32_t n, const int32_t *L,
+                      const int32_t *K, const double *val);
+int efdc_adj_step(efdc_ctx_t *ctx, const efdc_step_t *step, efdc_adj_stats_t *stats);
+/* dJ/dCQS(q,K,c) of the last adjoint step: out[(q*KC + K-1)*nconst + c],
+   zero for constituents that are not differentiated */
+int efdc_adj_source_grad(efdc_ctx_t *ctx, double *out);
+int efdc_adj_get(efdc_ctx_t *ctx, int32_t field, double *out);
 
 #ifdef __cplusplus
 }

@@ -189,6 +189,7 @@ class Oracle:
                         cqs=np.ascontiguousarray(src["cqs"], np.float64).reshape(-1),
                         csert=np.ascontiguousarray(src["csert"], np.float64).reshape(-1))
             self._keep.append(arrs)
+            self.src_cqs = arrs["cqs"]   # shared: edit in place to change CQS
             o.nqsij = len(arrs["LQS"])
             o.lqs = _pi(arrs["LQS"])
             o.qss, o.qsercell = _pd(arrs["qss"]), _pd(arrs["qsercell"])

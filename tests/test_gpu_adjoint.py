@@ -1,0 +1,215 @@
+"""Discrete adjoint of the hot-path step (efdc_adj_*) on a small tidal basin.
+
+The misfit J is the squared difference of layer-mean dye at observed cells
+(next to three dye sources, every 4 steps) from synthetic data.  Checks:
+
+* loads: J is quadratic in the loads (the dye is linear in them and does
+  not act on the flow), so a central difference of any size is exact up to
+  rounding; the adjoint's directional derivative must match it to 1e-8,
+  with the forward model on the GPU and with the FP64 CPU oracle (the CPU
+  tangent of J along the direction);
+* Manning n: J depends on n through the bottom drag, the external solve,
+  the transports and the advection; central differences with a relative
+  step of 1e-3 (O(h^2) truncation) must match to the 1e-4 bar, on the GPU
+  and with the CPU oracle;
+* the checkpoint interval changes nothing (same bits);
+* torch.autograd.gradcheck of the autograd.Function wrapper;
+* the FP32-storage forward gives the FP64-storage gradient to 1e-3.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NSRC = 3
+
+
+def _setup(nsteps=24, seed=7, kc=3, nx=24, ny=12):
+    from efdc_amd import grids
+    base = grids.tidal_basin(nx, ny, kc, seed=seed)
+    g = base.grid
+    rng = np.random.default_rng(seed)
+    west = set(int(x) for x in base.open_bc["side"][0]["L"])
+    inner = [L for L in range(2, g.LA + 1) if L not in west and g.IL[L - 1] > 3]
+    src = np.sort(rng.choice(inner, size=NSRC, replace=False)).astype(np.int32)
+    obs = []
+    for L in src:
+        obs += [L, g.maps["LEC"][L - 1], g.maps["LNC"][L - 1], g.maps["LWC"][L - 1]]
+    obs = np.array(sorted(set(int(x) for x in obs if x != g.LC)), np.int32)
+    case = grids.with_dye_sources(base, src)
+    true = rng.uniform(0.0, 5.0, (nsteps, NSRC))
+    loads0 = rng.uniform(0.0, 5.0, (nsteps, NSRC))
+    return case, obs, true, loads0, rng
+
+
+def _problem(case, obs, true, rng, nsteps=24, **kw):
+    from efdc_amd import adjoint
+    prob = adjoint.DyeMisfit(case, nsteps=nsteps, obs_L=obs, obs_every=4, **kw)
+    y = prob.observations(true)
+    prob.data = y * (1.0 + 0.01 * rng.standard_normal(y.shape))
+    return prob
+
+
+def _cpu_misfit(case, obs, loads, nman, data, nsteps, obs_every=4, dye=1):
+    from oracle import oracle as orc
+    c = case.copy()
+    c.manning_n = np.array(nman, np.float64)
+    o = orc.Oracle(c, rsqm=0.0, iterm=4000, rel_tol=1e-13)
+    kc = c.grid.KC
+    q = np.asarray(case.sources["qss"]).reshape(NSRC, kc).sum(axis=1)
+    cqs = o.src_cqs.reshape(NSRC, kc, -1)
+    dz = c.k["DZC"][:kc, obs - 1]
+    J = 0.0
+    for n in range(1, nsteps + 1):
+        cqs[:, :, dye] = (loads[n - 1] / q)[:, None]
+        o.step(n)
+        if n % obs_every == 0:
+            y = np.sum(dz * c.consts[dye]["con"][:kc, obs - 1], axis=0)
+            J += float(np.sum((y - data[n // obs_every - 1]) ** 2))
+    return J
+
+
+@pytest.fixture(scope="module")
+def prob_fp64():
+    case, obs, true, loads0, rng = _setup()
+    prob = _problem(case, obs, true, rng, ckpt=5)
+    J, gl, gn, st = prob.gradient(loads0)
+    yield dict(prob=prob, case=case, obs=obs, loads0=loads0, J=J, gl=gl, gn=gn, st=st,
+               rng=rng)
+    prob.close()
+
+
+def test_load_gradient_vs_central_difference(prob_fp64):
+    p = prob_fp64
+    prob, l0 = p["prob"], p["loads0"]
+    assert p["J"] > 0 and np.any(p["gl"] != 0)
+    for trial in range(2):
+        d = p["rng"].standard_normal(l0.shape)
+        fd = (prob.misfit(l0 + d) - prob.misfit(l0 - d)) / 2.0
+        ad = float(np.sum(p["gl"] * d))
+        print(f"loads: adjoint {ad:.12e} fd {fd:.12e} rel {abs(ad - fd) / abs(fd):.2e}")
+        assert abs(ad - fd) <= 1e-8 * abs(fd)
+    # single entries: the last load of a source before an observation
+    for s in range(NSRC):
+        e = np.zeros_like(l0)
+        e[19, s] = 1.0
+        fd = (prob.misfit(l0 + e) - prob.misfit(l0 - e)) / 2.0
+        assert abs(p["gl"][19, s] - fd) <= 1e-8 * max(abs(fd), 1e-30) + 1e-300
+
+
+def test_manning_gradient_vs_central_difference(prob_fp64):
+    p = prob_fp64
+    prob, case, l0 = p["prob"], p["case"], p["loads0"]
+    n0 = case.manning_n.copy()
+    g = case.grid
+    L = np.arange(2, g.LA + 1) - 1
+    d = np.zeros(g.LCM)
+    d[L] = p["rng"].uniform(-1.0, 1.0, len(L)) * n0[L]
+    h = 1e-3
+    fd = (prob.misfit(l0, n0 + h * d) - prob.misfit(l0, n0 - h * d)) / (2 * h)
+    ad = float(np.sum(p["gn"] * d))
+    print(f"manning: adjoint {ad:.10e} fd {fd:.10e} rel {abs(ad - fd) / abs(fd):.2e}")
+    assert abs(fd) > 0
+    assert abs(ad - fd) <= 1e-4 * abs(fd)
+
+
+def test_gradients_vs_cpu_oracle_tangent(prob_fp64):
+    """The FP64 oracle run of the same forward model: its central
+    difference along a load direction is the exact tangent of J (J is
+    quadratic in the loads); along a Manning direction it is the tangent up
+    to O(h^2)."""
+    p = prob_fp64
+    prob, case, l0, obs = p["prob"], p["case"], p["loads0"], p["obs"]
+    n0 = case.manning_n.copy()
+    data = prob.data
+    d = p["rng"].standard_normal(l0.shape)
+    fd = (_cpu_misfit(case, obs, l0 + d, n0, data, 24) -
+          _cpu_misfit(case, obs, l0 - d, n0, data, 24)) / 2.0
+    ad = float(np.sum(p["gl"] * d))
+    print(f"cpu loads: adjoint {ad:.12e} oracle {fd:.12e}")
+    assert abs(ad - fd) <= 1e-7 * abs(fd)
+    g = case.grid
+    L = np.arange(2, g.LA + 1) - 1
+    dn = np.zeros(g.LCM)
+    dn[L] = p["rng"].uniform(-1.0, 1.0, len(L)) * n0[L]
+    h = 1e-3
+    fdn = (_cpu_misfit(case, obs, l0, n0 + h * dn, data, 24) -
+           _cpu_misfit(case, obs, l0, n0 - h * dn, data, 24)) / (2 * h)
+    adn = float(np.sum(p["gn"] * dn))
+    print(f"cpu manning: adjoint {adn:.10e} oracle {fdn:.10e}")
+    assert abs(adn - fdn) <= 1e-4 * abs(fdn)
+
+
+def test_checkpoint_interval_changes_nothing(prob_fp64):
+    p = prob_fp64
+    prob = p["prob"]
+    prob.ckpt = 24
+    J, gl, gn, _ = prob.gradient(p["loads0"])
+    prob.ckpt = 5
+    assert J == p["J"]
+    assert np.array_equal(gl, p["gl"]) and np.array_equal(gn, p["gn"])
+
+
+def test_fp32_storage_forward_gradient():
+    from efdc_amd import lib
+    case, obs, true, loads0, rng = _setup()
+    p64 = _problem(case, obs, true, np.random.default_rng(3), ckpt=6)
+    data = p64.data
+    _, gl64, gn64, _ = p64.gradient(loads0)
+    p64.close()
+    from efdc_amd import adjoint
+    p32 = adjoint.DyeMisfit(case, nsteps=24, obs_L=obs, obs_every=4, ckpt=6,
+                            store=lib.STORE_FP32)
+    p32.data = data
+    _, gl32, gn32, _ = p32.gradient(loads0)
+    p32.close()
+    el = np.max(np.abs(gl32 - gl64)) / np.max(np.abs(gl64))
+    en = np.max(np.abs(gn32 - gn64)) / np.max(np.abs(gn64))
+    print("fp32 storage: load grad rel", el, "manning grad rel", en)
+    assert el <= 1e-3 and en <= 1e-2
+
+
+def test_torch_gradcheck_of_autograd_function():
+    import torch
+    from efdc_amd import adjoint
+    case, obs, true, loads0, rng = _setup(nsteps=8)
+    prob = _problem(case, obs, true[:8], rng, nsteps=8, ckpt=3)
+    cidx = adjoint.coarse_index(case.grid, 2, 2)
+    f = adjoint.autograd_function()
+    loads = torch.tensor(loads0[:8], dtype=torch.float64, requires_grad=True)
+    nc = torch.full((2, 2), 0.025, dtype=torch.float64, requires_grad=True)
+    J = f.apply(loads, nc, prob, cidx)
+    J.backward()
+    assert loads.grad is not None and nc.grad is not None
+    scale = float(J.detach()) + 1e-30
+    ok = torch.autograd.gradcheck(lambda a, b: f.apply(a, b, prob, cidx), (loads, nc),
+                                  eps=1e-6, atol=1e-6 * scale, rtol=1e-4,
+                                  nondet_tol=0.0, fast_mode=False)
+    assert ok
+    prob.close()
+
+
+def test_seed_is_ordered_after_pending_work():
+    """efdc_adj_seed_con called while the forward step's transport kernels
+    are still queued on the context's (non-blocking) stream: the seed list
+    must reach the device after everything queued before it, so exactly
+    the listed entries of lambda change."""
+    from efdc_amd import grids, lib
+    base = grids.tidal_basin(160, 160, 3, seed=3)
+    g = base.grid
+    case = grids.with_dye_sources(base, [int(g.LIJ[80, 80])])
+    ctx = lib.Context(g).upload_case(case)
+    ctx.adj_enable([1])
+    ctx.adj_reset()
+    L = np.array([int(g.LIJ[40, 40]), int(g.LIJ[100, 120]), int(g.LIJ[40, 40])], np.int32)
+    K = np.array([1, 3, 1], np.int32)
+    val = np.array([1.5, -2.0, 0.25])
+    for n in (1, 2):
+        ctx.step(case.dt, n)              # returns with CALCONC still queued
+        ctx.adj_seed_con(1, L, K, val)
+    lam = ctx.adj_get(lib.ADJ_CON + 1)
+    want = np.zeros_like(lam)
+    for l_, k_, v_ in zip(L, K, val):
+        want[k_ - 1, l_ - 1] += 2 * v_
+    assert np.array_equal(lam, want)
+    ctx.close()
