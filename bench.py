@@ -357,7 +357,9 @@ def adjoint_bench(args):
     step) giving dJ/dloads (200 x 16) and dJ/dn (per cell, coarsened to
     64 x 64).  FP32 storage forward, FP64 adjoint.  value = column-steps/s
     of forward+adjoint (wet columns x steps / gradient time).  Checked
-    against central differences of the same forward model."""
+    against central differences of the forward model (FP64 storage).  The
+    observed cells are drawn among the cells within 3 cells of a source:
+    in 1000 s the dye does not reach cells farther away."""
     import numpy as np
     from efdc_amd import adjoint, grids, lib
 
@@ -392,20 +394,25 @@ def adjoint_bench(args):
     prob.gradient(loads0)
     tm = prob.ctx.read_timers()
     prob.ctx.timers(False)
-    # gradient checks: central differences of the same forward model
+    # gradient checks: central differences of the forward model with FP64
+    # storage (the FP32-storage forward rounds the concentrations to ~1e-7,
+    # which a difference quotient of a 1e-3 Manning step cannot resolve)
+    ref = prob if store == lib.STORE_FP64 else adjoint.DyeMisfit(
+        case, nsteps=nsteps, obs_L=obs, obs_every=10, ckpt=20, store=lib.STORE_FP64)
+    ref.data = prob.data
     d = rng.standard_normal(loads0.shape)
-    fd = (prob.misfit(loads0 + d) - prob.misfit(loads0 - d)) / 2.0
+    fd = (ref.misfit(loads0 + d) - ref.misfit(loads0 - d)) / 2.0
     ad = float(np.sum(gl * d))
-    gc = np.zeros(64 * 64)
     m = cidx >= 0
-    np.add.at(gc, cidx[m], gn[m])
     dn = rng.uniform(-1.0, 1.0, 64 * 64)
     n0 = case.manning_n.copy()
     h = 1e-3
     pert = np.zeros(g.LCM)
     pert[m] = dn[cidx[m]] * n0[m]
-    fdn = (prob.misfit(loads0, n0 + h * pert) - prob.misfit(loads0, n0 - h * pert)) / (2 * h)
+    fdn = (ref.misfit(loads0, n0 + h * pert) - ref.misfit(loads0, n0 - h * pert)) / (2 * h)
     adn = float(np.sum(gn * pert))
+    if ref is not prob:
+        ref.close()
     # bytes of the dominant adjoint kernel: the transport column pass reads
     # the flows, depths, layer constants and old level (8 storage-type
     # layer arrays), lambda of CON, writes lcd/own and 5 FP64 scratch
