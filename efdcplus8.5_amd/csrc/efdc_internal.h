@@ -360,6 +360,61 @@ __device__ __forceinline__ bool grid_sum(double (&v)[NV], double *part,
   return true;
 }
 
+// Two-level deterministic grid reduction for launches of many blocks whose
+// hardware block b runs tile xcd_tile(b, nblocks): the last block of each
+// XCD to arrive sums that XCD's (contiguous) tiles in order and publishes
+// the XCD sum, and the last XCD to finish sums the 8 XCD sums in order.
+// The serial tail is 1/8 of grid_sum's and the 8 XCD sums run at the same
+// time.  The XCD sums are kept after the partials, at part[NV*nblocks +
+// j*8 + x].  Returns true in every thread of the block that finishes.
+template <int NV>
+__device__ __forceinline__ bool grid_sum_x(double (&v)[NV], double *part, unsigned *ticket,
+                                           int nblocks, int tile, double (&out)[NV]) {
+  __shared__ int stage;  // 0 not last, 1 last of its XCD, 2 last overall
+  block_sum<NV>(v);
+  const int x = blockIdx.x & 7;
+  const int per = nblocks >> 3, rem = nblocks & 7;
+  const int nx = per + (x < rem ? 1 : 0);
+  const int ngroups = nblocks < 8 ? nblocks : 8;
+  unsigned *gx = ticket + x * TICKET_PAD, *gg = ticket + 8 * TICKET_PAD;
+  double *xpart = part + (size_t)NV * nblocks;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) publish(&part[(size_t)j * nblocks + tile], v[j]);
+    __builtin_amdgcn_s_waitcnt(0);
+    stage = (atomic_inc(gx) == (unsigned)nx - 1u) ? 1 : 0;
+    if (stage) reset_ticket(gx);
+  }
+  __syncthreads();
+  if (!stage) return false;
+  // this XCD's tiles: [t0, t0 + nx)
+  const int t0 = x * per + (x < rem ? x : rem);
+  double acc[NV];
+  sum_parts<NV>(part + t0, nblocks, nx, acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) publish(&xpart[j * 8 + x], acc[j]);
+    __builtin_amdgcn_s_waitcnt(0);
+    stage = (atomic_inc(gg) == (unsigned)ngroups - 1u) ? 2 : 0;
+    if (stage == 2) reset_ticket(gg);
+  }
+  __syncthreads();
+  if (stage != 2) return false;
+  __shared__ double res[NV];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      double t = 0.0;
+      for (int q = 0; q < ngroups; ++q) t += fetch(&xpart[j * 8 + q]);
+      res[j] = t;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NV; ++j) out[j] = res[j];
+  return true;
+}
+
 // Same pattern for a minimum.
 __device__ __forceinline__ bool grid_min(double v, double *part,
                                          unsigned *ticket, int nblocks,

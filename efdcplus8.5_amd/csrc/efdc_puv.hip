@@ -38,6 +38,10 @@ HPtr hptr(efdc_ctx *c) {
 // of CG_CPT*256 cells, which halves the number of workgroups (and of
 // reduction arrivals) per launch.
 int cg_blocks(int nwet) { return (nwet + CG_CPT * EFDC_BLOCK - 1) / (CG_CPT * EFDC_BLOCK); }
+// The mixed solve's kernels move 24-32 bytes per cell: 4 cells per thread
+// keep more loads in flight and quarter the blocks (and reduction arrivals).
+#define MX_CPT 4
+int mx_blocks(int nwet) { return (nwet + MX_CPT * EFDC_BLOCK - 1) / (MX_CPT * EFDC_BLOCK); }
 
 __device__ __forceinline__ int cell_index(int nwet, int &tile) {
   tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -701,8 +705,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_prep(MxArgs m, double rsqm,
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   double v[1] = {0.0};
   #pragma unroll
-  for (int u = 0; u < CG_CPT; ++u) {
-    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+  for (int u = 0; u < MX_CPT; ++u) {
+    const int i = 1 + (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
     if (i <= a.nwet) {
       const double di = 1.0 / sqrt(a.ccc[i]);
       m.dis[i] = di;
@@ -716,7 +720,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_prep(MxArgs m, double rsqm,
     }
   }
   double s[1];
-  if (grid_sum<1>(v, a.part, a.tk + TS_MX_PREP * TICKET_SITE, gridDim.x, tile, s) &&
+  if (grid_sum_x<1>(v, a.part, a.tk + TS_MX_PREP * TICKET_SITE, gridDim.x, tile, s) &&
       threadIdx.x == 0) {
     a.st->bnorm2 = s[0];
     a.st->rsqm = rel_tol > 0.0 ? rel_tol * rel_tol * s[0] : rsqm;
@@ -733,8 +737,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_resid(MxArgs m) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   double v[2] = {0.0, 0.0};
   #pragma unroll
-  for (int u = 0; u < CG_CPT; ++u) {
-    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+  for (int u = 0; u < MX_CPT; ++u) {
+    const int i = 1 + (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
     if (i <= a.nwet) {
       const Row q = row_of(a, i, true);
       const double *P = a.x;
@@ -748,7 +752,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_resid(MxArgs m) {
     }
   }
   double s[2];
-  if (grid_sum<2>(v, a.part, a.tk + TS_MX_RESID * TICKET_SITE, gridDim.x, tile, s) &&
+  if (grid_sum_x<2>(v, a.part, a.tk + TS_MX_RESID * TICKET_SITE, gridDim.x, tile, s) &&
       threadIdx.x == 0) {
     CgState *st = a.st;
     st->rsq64 = s[0];
@@ -776,8 +780,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_a(MxArgs m, const float *pold
   const double beta = a.st->beta;
   double v[1] = {0.0};
   #pragma unroll
-  for (int u = 0; u < CG_CPT; ++u) {
-    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+  for (int u = 0; u < MX_CPT; ++u) {
+    const int i = 1 + (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
     if (i <= a.nwet) {
       const int js = a.ls[i], jn = a.ln[i];
       const float pc = p32_at(m, pold, beta, i);
@@ -794,7 +798,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_a(MxArgs m, const float *pold
     }
   }
   double s[1];
-  if (grid_sum<1>(v, a.part, a.tk + TS_MX_A * TICKET_SITE, gridDim.x, tile, s) &&
+  if (grid_sum_x<1>(v, a.part, a.tk + TS_MX_A * TICKET_SITE, gridDim.x, tile, s) &&
       threadIdx.x == 0)
     a.st->alpha = a.st->rz_in / s[0];
 }
@@ -807,8 +811,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_b(MxArgs m, const float *pnew
   const double alpha = a.st->alpha;
   double v[1] = {0.0};
   #pragma unroll
-  for (int u = 0; u < CG_CPT; ++u) {
-    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+  for (int u = 0; u < MX_CPT; ++u) {
+    const int i = 1 + (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
     if (i <= a.nwet) {
       m.y32[i] = (float)((double)m.y32[i] + alpha * (double)pnew[i]);
       const float r = (float)((double)m.r32[i] - alpha * (double)m.ap32[i]);
@@ -817,7 +821,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_b(MxArgs m, const float *pnew
     }
   }
   double s[1];
-  if (grid_sum<1>(v, a.part, a.tk + TS_MX_B * TICKET_SITE, gridDim.x, tile, s) &&
+  if (grid_sum_x<1>(v, a.part, a.tk + TS_MX_B * TICKET_SITE, gridDim.x, tile, s) &&
       threadIdx.x == 0) {
     CgState *st = a.st;
     st->beta = s[0] / st->rz_in;
@@ -832,8 +836,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_apply(MxArgs m) {
   const CgArgs &a = m.g;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
 #pragma unroll
-  for (int u = 0; u < CG_CPT; ++u) {
-    const int i = 1 + (tile * CG_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+  for (int u = 0; u < MX_CPT; ++u) {
+    const int i = 1 + (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
     if (i <= a.nwet) a.x[i] = a.x[i] + m.dis[i] * (double)m.y32[i];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) a.st->outer += 1;
@@ -952,7 +956,7 @@ int cg_fp64(efdc_ctx *c) {
 }
 
 int cg_mixed(efdc_ctx *c) {
-  const int nb = cg_blocks(c->n_wet());
+  const int nb = mx_blocks(c->n_wet());
   MxArgs m;
   m.g = cg_args(c);
   m.dis = c->dis;
