@@ -994,6 +994,27 @@ int efdc_state_load(efdc_ctx_t *c, const void *dev_buf) {
   return EFDC_OK;
 }
 
+int efdc_state_alloc(efdc_ctx_t *c, void **dev_buf) {
+  if (!c || !dev_buf) return EFDC_ERR_ARG;
+  size_t n = 0;
+  int s;
+  if ((s = efdc_state_bytes(c, &n))) return s;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (hipMalloc(dev_buf, n) != hipSuccess) {
+    *dev_buf = nullptr;
+    c->err = "efdc_state_alloc: hipMalloc failed";
+    return EFDC_ERR_NOMEM;
+  }
+  return EFDC_OK;
+}
+
+int efdc_state_free(efdc_ctx_t *c, void *dev_buf) {
+  if (!c) return EFDC_ERR_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (dev_buf) HIP_TRY(c, hipFree(dev_buf));
+  return EFDC_OK;
+}
+
 int efdc_adj_enable(efdc_ctx_t *c, const efdc_adj_opts_t *o) {
   if (!c || !o || o->nadj < 0 || o->nadj > c->nconst || o->nadj > 16) return EFDC_ERR_ARG;
   if (c->nseg > 1) {

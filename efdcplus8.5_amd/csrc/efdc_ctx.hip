@@ -35,6 +35,15 @@ __global__ void k_init_puv(double *rcx, double *rcy, double *cc, int lcm,
   cc[i] = 1.0;
 }
 
+// Plain streaming copy (16-byte loads and stores, grid-stride): the
+// achievable-bandwidth reference of the roofline figures.
+__global__ __launch_bounds__(256) void k_copy4(const float4 *__restrict__ x,
+                                               float4 *__restrict__ y, size_t n4) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] = x[i];
+}
+
 int blocks_for(int64_t n, int bs = EFDC_BLOCK) { return (int)((n + bs - 1) / bs); }
 
 template <typename T>
@@ -711,6 +720,42 @@ int efdc_set_sources(efdc_ctx_t *c, const efdc_sources_t *src) {
   HIP_TRY(c, hipMemcpyAsync(c->srcrec, rec.data(), sizeof(int) * c->LCM,
                             hipMemcpyHostToDevice, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
+int efdc_stream_copy_gbs(efdc_ctx_t *c, size_t bytes, int32_t reps, double *gbs) {
+  if (!c || !gbs || bytes < 4096 || reps < 1) return EFDC_ERR_ARG;
+  const size_t n4 = bytes / 16;
+  float4 *x = nullptr, *y = nullptr;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (hipMalloc((void **)&x, n4 * 16) != hipSuccess ||
+      hipMalloc((void **)&y, n4 * 16) != hipSuccess) {
+    if (x) (void)hipFree(x);
+    c->err = "efdc_stream_copy_gbs: hipMalloc failed";
+    return EFDC_ERR_NOMEM;
+  }
+  hipEvent_t a = timer_event(c), b = timer_event(c);
+  const int nb = 8 * 256;
+  float best = 1e30f;
+  int st = EFDC_OK;
+  for (int r = 0; r <= reps && st == EFDC_OK; ++r) {
+    (void)hipEventRecord(a, c->stream);
+    k_copy4<<<nb, 256, 0, c->stream>>>(x, y, n4);
+    (void)hipEventRecord(b, c->stream);
+    if (hipEventSynchronize(b) != hipSuccess) st = EFDC_ERR_HIP;
+    float ms = 0.f;
+    if (st == EFDC_OK && hipEventElapsedTime(&ms, a, b) != hipSuccess) st = EFDC_ERR_HIP;
+    if (r > 0 && ms < best) best = ms;  // the first copy is a warm-up
+  }
+  c->ev_pool.push_back(a);
+  c->ev_pool.push_back(b);
+  (void)hipFree(x);
+  (void)hipFree(y);
+  if (st) {
+    c->err = "efdc_stream_copy_gbs: copy failed";
+    return st;
+  }
+  *gbs = 2.0 * (double)(n4 * 16) / ((double)best * 1e-3) / 1e9;
   return EFDC_OK;
 }
 

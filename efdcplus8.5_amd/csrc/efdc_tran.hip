@@ -242,12 +242,12 @@ __global__ __launch_bounds__(TBLOCK) void k_tran_upwind_vdiff(TranArgs<T> a) {
 // constituent): the workgroup is 64 columns x NCG constituent lanes (a
 // wave per lane, so every wave streams one constituent's plane with
 // coalesced loads while the transport fields it shares with the other
-// lanes come from L1/L2).  VCU of the tridiagonal solve is kept in LDS per
-// lane (KC doubles per thread); the forward values are written to the new
-// concentration plane itself (in the storage type, as the LDS staging of
-// k_tran_upwind_vdiff does) and read back, still cached, by the back
-// substitution.  Arithmetic is expression for expression that of
-// k_tran_upwind_vdiff, so both kernels give identical bits.
+// lanes come from L1/L2).  VCU of the tridiagonal solve (KC doubles per
+// column, shared by the constituents) and the forward values (KC values of
+// the storage type per thread, as the LDS staging of k_tran_upwind_vdiff
+// keeps them) stay in LDS, so the new concentration plane is written once.
+// Arithmetic is expression for expression that of k_tran_upwind_vdiff, so
+// both kernels give identical bits.
 // Inputs of one layer of one column for one constituent.  The upwind
 // donor values are loaded for both flow signs and selected afterwards, so
 // that all loads of a layer are independent of each other and of the
@@ -333,7 +333,11 @@ __device__ __forceinline__ double col_open_bc(const TranArgs<T> &a, int4 bi, int
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_tran_col(TranArgs<T> a) {
-  extern __shared__ double vcus[];  // [blockDim.y][KC][64]
+  // LDS: VCU of the tridiagonal solve [KC][64] (FP64, the same for every
+  // constituent of a column: each lane computes it from the same inputs,
+  // so the waves of a block write identical values), then the forward
+  // values [blockDim.y][KC][64] in the storage type
+  extern __shared__ double vcus[];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int i = 1 + tile * 64 + (int)threadIdx.x;
   if (i > a.nwet) return;
@@ -359,7 +363,8 @@ __global__ __launch_bounds__(256) void k_tran_col(TranArgs<T> a) {
   if (rec >= 0) bi = a.bcinfo[rec];
   const bool vdk = a.vd && KC > 1;
 
-  double *vcu = vcus + (size_t)threadIdx.y * KC * 64 + threadIdx.x;  // stride 64
+  double *vcu = vcus + threadIdx.x;  // stride 64
+  T *ys = (T *)(vcus + (size_t)KC * 64) + (size_t)threadIdx.y * KC * 64 + threadIdx.x;
   for (int c = threadIdx.y; c < nc; c += blockDim.y) {
     const T *__restrict__ C1 = a.old[c];
     T *__restrict__ C = a.nw[c];
@@ -453,7 +458,7 @@ __global__ __launch_bounds__(256) void k_tran_col(TranArgs<T> a) {
       else if (K == 1) y = con * eeb;
       else y = (con - cclb * yprev) * eeb;
       const T yt = (T)y;
-      C[ui + o] = yt;
+      ys[k * 64] = yt;
       yprev = (double)yt;
       vcu_prev = vcu_k;
     }
@@ -462,11 +467,11 @@ __global__ __launch_bounds__(256) void k_tran_col(TranArgs<T> a) {
     const int dm = a.dye_mode[c];
     const double f = a.dye_f[c];
     auto fin = [&](double v) { return dm == 1 ? f * v : (dm == 2 ? v + f : v); };
-    double up = (double)C[ui + (unsigned)(KC - 1) * pitch];
+    double up = (double)ys[(KC - 1) * 64];
     C[ui + (unsigned)(KC - 1) * pitch] = (T)fin(up);
     for (int k = KC - 2; k >= 0; --k) {
       const unsigned o = (unsigned)k * pitch;
-      const double v = (double)C[ui + o] - vcu[k * 64] * up;
+      const double v = (double)ys[k * 64] - vcu[k * 64] * up;
       C[ui + o] = (T)fin(v);
       up = v;
     }
@@ -1067,7 +1072,7 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
     const int ncg = nc < 4 ? nc : 4;
     const dim3 blk(64, ncg);
     const int nbc = (a.nwet + 63) / 64;
-    const size_t lds = sizeof(double) * (size_t)ncg * KC * 64;
+    const size_t lds = (sizeof(double) + sizeof(T) * (size_t)ncg) * KC * 64;
     TIMED(c, TK_TRAN, k_tran_col<T><<<nbc, blk, lds, c->stream>>>(a));
     if (any_adac) {
       int s = adac_pipeline<T>(c, a, step);

@@ -17,8 +17,8 @@ every ``ckpt`` steps (efdc_state_save), then walks the segments backwards:
 each segment is re-run from its checkpoint with the state before every step
 saved, and every step is reversed by loading the state before it, repeating
 it (the context records what the reverse needs) and calling efdc_adj_step.
-Memory: (nsteps/ckpt + ckpt) snapshots.  The snapshot buffers are device
-memory taken from torch's allocator.
+Memory: (nsteps/ckpt + ckpt) snapshots, device buffers from
+efdc_state_alloc kept in a pool for the life of the problem.
 """
 from __future__ import annotations
 
@@ -49,8 +49,6 @@ class DyeMisfit:
     def __init__(self, case, *, nsteps: int, obs_L, obs_every: int, data=None,
                  dye_slot: int = 1, ckpt: int = 20, store: int = lib.STORE_FP64,
                  cg_rel_tol: float = 1e-12, **params):
-        import torch
-        self.torch = torch
         self.case = case
         self.nsteps = nsteps
         self.obs_L = np.asarray(obs_L, np.int32)
@@ -74,15 +72,17 @@ class DyeMisfit:
         self.obs_steps = [t for t in range(1, nsteps + 1) if t % obs_every == 0]
         self.data = None if data is None else np.asarray(data, np.float64)
         self.nman0 = None if case.manning_n is None else np.array(case.manning_n, np.float64)
-        nb = self.ctx.state_bytes()
-        self._nb = nb
-        self._init = self._buffer()
-        self.ctx.state_save(self._init.data_ptr())
+        self._pool = []          # snapshot buffers (device pointers)
+        self._init = self.ctx.state_alloc()
+        self.ctx.state_save(self._init)
         self.ctx.synchronize()
 
     # ------------------------------------------------------------ helpers
-    def _buffer(self):
-        return self.torch.empty(self._nb, dtype=self.torch.uint8, device="cuda")
+    def _buffers(self, n):
+        """n snapshot buffers from the pool (grown on demand)."""
+        while len(self._pool) < n:
+            self._pool.append(self.ctx.state_alloc())
+        return self._pool[:n]
 
     def _cqs(self, loads, n):
         c = self.cqs0.copy()
@@ -98,7 +98,7 @@ class DyeMisfit:
         self.ctx.step(self.case.dt, n)
 
     def _start(self, loads, nman):
-        self.ctx.state_load(self._init.data_ptr())
+        self.ctx.state_load(self._init)
         if nman is None:
             nman = self.nman0
         if nman is not None:
@@ -127,29 +127,29 @@ class DyeMisfit:
         ctx, K = self.ctx, self.ckpt
         ctx.adj_reset()
         self._start(loads, nman)
-        ckpts = {}
+        starts = list(range(0, self.nsteps, K))
+        bufs = self._buffers(len(starts) + min(K, self.nsteps))
+        ckpts = dict(zip(starts, bufs[:len(starts)]))
+        seg = bufs[len(starts):]
         resid = {}
         for n in range(1, self.nsteps + 1):
             if (n - 1) % K == 0:
-                b = self._buffer()
-                ctx.state_save(b.data_ptr())
-                ckpts[n - 1] = b
+                ctx.state_save(ckpts[n - 1])
             self._step(loads, n)
             if n % self.obs_every == 0:
                 resid[n] = self._observe() - self.data[n // self.obs_every - 1]
         J = float(sum(np.sum(r ** 2) for r in resid.values()))
         gl = np.zeros((self.nsteps, self.nsrc))
-        seg = [self._buffer() for _ in range(min(K, self.nsteps))]
         iters = []
         for t0 in sorted(ckpts, reverse=True):
             t1 = min(t0 + K, self.nsteps)
-            ctx.state_load(ckpts[t0].data_ptr())
+            ctx.state_load(ckpts[t0])
             for j, n in enumerate(range(t0 + 1, t1 + 1)):
-                ctx.state_save(seg[j].data_ptr())
+                ctx.state_save(seg[j])
                 if n < t1:
                     self._step(loads, n)
             for n in range(t1, t0, -1):
-                ctx.state_load(seg[n - t0 - 1].data_ptr())
+                ctx.state_load(seg[n - t0 - 1])
                 self._step(loads, n)
                 if n in resid:
                     r = resid[n]
@@ -166,6 +166,10 @@ class DyeMisfit:
         return J, gl, gn, dict(adj_cg_iters=iters)
 
     def close(self):
+        if self.ctx.h:
+            for b in self._pool + [self._init]:
+                self.ctx.state_free(b)
+            self._pool = []
         self.ctx.close()
 
 

@@ -155,9 +155,12 @@ def load(path: Optional[str] = None):
         "efdc_timers_enable": ([P, C.c_int32], C.c_int),
         "efdc_timers_read": ([P, C.POINTER(Timers)], C.c_int),
         "efdc_set_source_cqs": ([P, PD], C.c_int),
+        "efdc_stream_copy_gbs": ([P, C.c_size_t, C.c_int32, PD], C.c_int),
         "efdc_state_bytes": ([P, C.POINTER(C.c_size_t)], C.c_int),
         "efdc_state_save": ([P, P], C.c_int),
         "efdc_state_load": ([P, P], C.c_int),
+        "efdc_state_alloc": ([P, C.POINTER(P)], C.c_int),
+        "efdc_state_free": ([P, P], C.c_int),
         "efdc_adj_enable": ([P, C.POINTER(AdjOpts)], C.c_int),
         "efdc_adj_disable": ([P], C.c_int),
         "efdc_adj_reset": ([P], C.c_int),
@@ -369,6 +372,21 @@ class Context:
 
     def state_load(self, dev_ptr: int):
         self._check(self.lib.efdc_state_load(self.h, C.c_void_p(dev_ptr)))
+
+    def stream_copy_gbs(self, mib: int = 1024, reps: int = 10) -> float:
+        """Achievable HBM GB/s of a plain streaming copy of mib MiB."""
+        g = C.c_double(0.0)
+        self._check(self.lib.efdc_stream_copy_gbs(self.h, mib << 20, reps, C.byref(g)))
+        return g.value
+
+    def state_alloc(self) -> int:
+        """A device buffer for one snapshot (free it with state_free)."""
+        b = C.c_void_p()
+        self._check(self.lib.efdc_state_alloc(self.h, C.byref(b)))
+        return b.value
+
+    def state_free(self, dev_ptr: int):
+        self._check(self.lib.efdc_state_free(self.h, C.c_void_p(dev_ptr)))
 
     def set_source_cqs(self, cqs):
         a = np.ascontiguousarray(cqs, np.float64).reshape(-1)

@@ -312,6 +312,11 @@ int efdc_segment_stats(efdc_ctx_t *ctx, int32_t cap, int32_t *iters, int32_t *st
    mass[s*nconst + c] for s < min(nseg, cap). */
 int efdc_balance_segments(efdc_ctx_t *ctx, int32_t cap, double *volume, double *mass);
 
+/* Achievable HBM bandwidth on this device: best of `reps` timed runs of a
+   plain streaming copy of `bytes` (read + write bytes / time, GB/s).  No
+   reference counterpart (the roofline reference of the measurements). */
+int efdc_stream_copy_gbs(efdc_ctx_t *ctx, size_t bytes, int32_t reps, double *gbs);
+
 /* Wait for all work queued on the context's stream. */
 int efdc_synchronize(efdc_ctx_t *ctx);
 
@@ -362,6 +367,10 @@ int efdc_closure_post(efdc_ctx_t *ctx);
 int efdc_state_bytes(efdc_ctx_t *ctx, size_t *bytes);
 int efdc_state_save(efdc_ctx_t *ctx, void *dev_buf);
 int efdc_state_load(efdc_ctx_t *ctx, const void *dev_buf);
+/* One device buffer of efdc_state_bytes() bytes, for hosts without a
+   device allocator of their own (Fortran, Python without torch). */
+int efdc_state_alloc(efdc_ctx_t *ctx, void **dev_buf);
+int efdc_state_free(efdc_ctx_t *ctx, void *dev_buf);
 
 /* --------------------------------------------------- discrete adjoint
    No reference counterpart (gradient addition).  The reverse of one
