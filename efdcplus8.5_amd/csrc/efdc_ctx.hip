@@ -723,6 +723,28 @@ int efdc_set_sources(efdc_ctx_t *c, const efdc_sources_t *src) {
   return EFDC_OK;
 }
 
+int efdc_autotune(efdc_ctx_t *c, int32_t on) {
+  if (!c) return EFDC_ERR_ARG;
+  c->tune_pending = on != 0;
+  if (!on) {
+    c->tran_ncg = 0;
+    c->tran_w3 = 0;
+  }
+  return EFDC_OK;
+}
+
+int efdc_tune_report(efdc_ctx_t *c, efdc_tune_report_t *out) {
+  if (!c || !out) return EFDC_ERR_ARG;
+  std::memset(out, 0, sizeof(*out));
+  out->n = c->tune_n;
+  out->chosen = c->tune_best;
+  for (int j = 0; j < c->tune_n && j < 8; ++j) {
+    std::memcpy(out->v[j].name, c->tune_name[j], 32);
+    out->v[j].ms = c->tune_ms[j];
+  }
+  return EFDC_OK;
+}
+
 int efdc_stream_copy_gbs(efdc_ctx_t *c, size_t bytes, int32_t reps, double *gbs) {
   if (!c || !gbs || bytes < 4096 || reps < 1) return EFDC_ERR_ARG;
   const size_t n4 = bytes / 16;

@@ -186,6 +186,12 @@ struct efdc_ctx {
   struct TimerRec { int id; hipEvent_t a, b; };
   std::vector<TimerRec> trec;
 
+  // autotuner (efdc_autotune): launch shape of the column transport
+  bool tune_pending = false;
+  int tran_ncg = 0, tran_w3 = 0;  // 0: not tuned
+  int tune_n = 0, tune_best = -1;
+  char tune_name[8][32] = {};
+  float tune_ms[8] = {};
   // discrete adjoint (efdc_adj.hip), allocated by efdc_adj_enable
   struct AdjState *adj = nullptr;
 

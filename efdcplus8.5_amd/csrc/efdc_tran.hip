@@ -20,7 +20,9 @@
 // 64 layers.  HBM traffic per column-step is
 //   KC*s*(3 flows + H1PK + HPKI + AB + 2*NC) + maps/metrics,
 // the roofline of this step.
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -332,12 +334,11 @@ __device__ __forceinline__ double col_open_bc(const TranArgs<T> &a, int4 bi, int
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_tran_col(TranArgs<T> a) {
+__device__ __forceinline__ void tran_col_body(const TranArgs<T> &a, double *vcus) {
   // LDS: VCU of the tridiagonal solve [KC][64] (FP64, the same for every
   // constituent of a column: each lane computes it from the same inputs,
   // so the waves of a block write identical values), then the forward
   // values [blockDim.y][KC][64] in the storage type
-  extern __shared__ double vcus[];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int i = 1 + tile * 64 + (int)threadIdx.x;
   if (i > a.nwet) return;
@@ -476,6 +477,23 @@ __global__ __launch_bounds__(256) void k_tran_col(TranArgs<T> a) {
       up = v;
     }
   }
+}
+
+// Two register budgets of the same column kernel: the compiler's choice,
+// and at most 168 VGPRs (three waves per SIMD, a few spills for FP64).
+// Which is faster depends on KC, the constituent count and the storage
+// type; the autotuner times both.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tran_col(TranArgs<T> a) {
+  extern __shared__ double vcus[];
+  tran_col_body<T>(a, vcus);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_tran_col_w3(
+    TranArgs<T> a) {
+  extern __shared__ double vcus[];
+  tran_col_body<T>(a, vcus);
 }
 
 // CON1 at open-boundary columns takes the boundary value (ISUD=1,
@@ -1068,12 +1086,60 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
   }
   const bool off32 = (int64_t)c->pitch * (c->KC + 1) < ((int64_t)1 << 31);
   if (KC <= 64 && off32) {
-    // one thread per (column, constituent)
-    const int ncg = nc < 4 ? nc : 4;
-    const dim3 blk(64, ncg);
+    // one thread per (column, constituent); launch shape: constituent
+    // lanes per workgroup (ncg) and register budget, from the autotuner
+    // when it has run, else 4 lanes and the compiler's budget
     const int nbc = (a.nwet + 63) / 64;
-    const size_t lds = (sizeof(double) + sizeof(T) * (size_t)ncg) * KC * 64;
-    TIMED(c, TK_TRAN, k_tran_col<T><<<nbc, blk, lds, c->stream>>>(a));
+    auto launch = [&](int ncg, int w3) {
+      const dim3 blk(64, ncg);
+      const size_t lds = (sizeof(double) + sizeof(T) * (size_t)ncg) * KC * 64;
+      if (w3) k_tran_col_w3<T><<<nbc, blk, lds, c->stream>>>(a);
+      else k_tran_col<T><<<nbc, blk, lds, c->stream>>>(a);
+    };
+    if (c->tune_pending && !any_adac) {
+      // every shape computes the same bits, and a repeated launch
+      // rewrites the same outputs (CON from CON1, the recovery memory of
+      // outflow boundary layers), so the shapes are timed on the step
+      // itself; the chosen one runs last
+      c->tune_pending = false;
+      c->tune_n = 0;
+      int best = -1;
+      float best_ms = 1e30f;
+      hipEvent_t e0 = timer_event(c), e1 = timer_event(c);
+      const int lanes[3] = {nc < 4 ? nc : 4, nc < 2 ? nc : 2, 1};
+      for (int li = 0; li < 3; ++li) {
+        if (li > 0 && lanes[li] == lanes[li - 1]) continue;
+        for (int w3 = 0; w3 < 2; ++w3) {
+          float tot = 0.f;
+          for (int r = 0; r < 4; ++r) {
+            HIP_TRY(c, hipEventRecord(e0, c->stream));
+            launch(lanes[li], w3);
+            HIP_TRY(c, hipEventRecord(e1, c->stream));
+            HIP_TRY(c, hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+            if (r > 0) tot += ms;  // the first run is a warm-up
+          }
+          const float avg = tot / 3.f;
+          if (c->tune_n < 8) {
+            std::snprintf(c->tune_name[c->tune_n], 32, "tran_col lanes=%d%s", lanes[li],
+                          w3 ? " vgpr<=168" : "");
+            c->tune_ms[c->tune_n++] = avg;
+          }
+          if (avg < best_ms) {
+            best_ms = avg;
+            best = c->tune_n - 1;
+            c->tran_ncg = lanes[li];
+            c->tran_w3 = w3;
+          }
+        }
+      }
+      c->tune_best = best;
+      c->ev_pool.push_back(e0);
+      c->ev_pool.push_back(e1);
+    }
+    const int ncg = c->tran_ncg > 0 ? std::min(c->tran_ncg, nc) : (nc < 4 ? nc : 4);
+    TIMED(c, TK_TRAN, launch(ncg, c->tran_ncg > 0 ? c->tran_w3 : 0));
     if (any_adac) {
       int s = adac_pipeline<T>(c, a, step);
       if (s) return s;

@@ -99,6 +99,14 @@ class Balance(C.Structure):
                 ("nconst", C.c_int32)]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("ms", C.c_float)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("n", C.c_int32), ("chosen", C.c_int32), ("v", TuneEntry * 8)]
+
+
 class AdjOpts(C.Structure):
     _fields_ = [("nadj", C.c_int32), ("slot", C.c_int32 * 16), ("cg_rel_tol", C.c_double),
                 ("cg_itmax", C.c_int32)]
@@ -156,6 +164,8 @@ def load(path: Optional[str] = None):
         "efdc_timers_read": ([P, C.POINTER(Timers)], C.c_int),
         "efdc_set_source_cqs": ([P, PD], C.c_int),
         "efdc_stream_copy_gbs": ([P, C.c_size_t, C.c_int32, PD], C.c_int),
+        "efdc_autotune": ([P, C.c_int32], C.c_int),
+        "efdc_tune_report": ([P, C.POINTER(TuneReport)], C.c_int),
         "efdc_state_bytes": ([P, C.POINTER(C.c_size_t)], C.c_int),
         "efdc_state_save": ([P, P], C.c_int),
         "efdc_state_load": ([P, P], C.c_int),
@@ -372,6 +382,16 @@ class Context:
 
     def state_load(self, dev_ptr: int):
         self._check(self.lib.efdc_state_load(self.h, C.c_void_p(dev_ptr)))
+
+    def autotune(self, on: bool = True):
+        self._check(self.lib.efdc_autotune(self.h, 1 if on else 0))
+
+    def tune_report(self) -> dict:
+        r = TuneReport()
+        self._check(self.lib.efdc_tune_report(self.h, C.byref(r)))
+        shapes = {r.v[j].name.decode(): r.v[j].ms for j in range(r.n)}
+        chosen = r.v[r.chosen].name.decode() if 0 <= r.chosen < r.n else None
+        return dict(shapes=shapes, chosen=chosen)
 
     def stream_copy_gbs(self, mib: int = 1024, reps: int = 10) -> float:
         """Achievable HBM GB/s of a plain streaming copy of mib MiB."""

@@ -317,6 +317,22 @@ int efdc_balance_segments(efdc_ctx_t *ctx, int32_t cap, double *volume, double *
    reference counterpart (the roofline reference of the measurements). */
 int efdc_stream_copy_gbs(efdc_ctx_t *ctx, size_t bytes, int32_t reps, double *gbs);
 
+/* Autotuner (no reference counterpart).  efdc_autotune(ctx, 1) makes the
+   next efdc_calconc time every launch shape of the column transport kernel
+   (constituent lanes per workgroup 4/2/1, two register budgets) on that
+   step's own data and keep the fastest for the following steps; every shape
+   gives the same bits.  efdc_autotune(ctx, 0) returns to the default shape.
+   efdc_tune_report lists the shapes timed (ms per launch) and the choice. */
+typedef struct {
+  int32_t n, chosen;
+  struct {
+    char name[32];
+    float ms;
+  } v[8];
+} efdc_tune_report_t;
+int efdc_autotune(efdc_ctx_t *ctx, int32_t on);
+int efdc_tune_report(efdc_ctx_t *ctx, efdc_tune_report_t *out);
+
 /* Wait for all work queued on the context's stream. */
 int efdc_synchronize(efdc_ctx_t *ctx);
 
