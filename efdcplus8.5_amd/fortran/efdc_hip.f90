@@ -98,6 +98,23 @@ module efdc_hip
     real(c_double) :: rsq, rsqm_used, ccmnm
   end type
 
+  ! efdc_adj_field
+  integer(c_int32_t), parameter :: EFDC_ADJ_MANNING = 0, EFDC_ADJ_HP = 1, &
+      EFDC_ADJ_UHDYE = 2, EFDC_ADJ_VHDXE = 3, EFDC_ADJ_P = 4, EFDC_ADJ_HU = 5, &
+      EFDC_ADJ_HV = 6, EFDC_ADJ_CON = 16
+
+  type, bind(C) :: efdc_adj_opts_t
+    integer(c_int32_t) :: nadj
+    integer(c_int32_t) :: slot(16)
+    real(c_double) :: cg_rel_tol
+    integer(c_int32_t) :: cg_itmax
+  end type
+
+  type, bind(C) :: efdc_adj_stats_t
+    integer(c_int32_t) :: cg_iters
+    real(c_double) :: cg_rsq
+  end type
+
   type, bind(C) :: efdc_balance_t
     real(c_double) :: volume
     real(c_double) :: mass(16)
@@ -216,6 +233,171 @@ module efdc_hip
     end function
 
     integer(c_int) function efdc_synchronize(ctx) bind(C, name="efdc_synchronize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function efdc_set_source_cqs(ctx, cqs) bind(C, name="efdc_set_source_cqs")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: cqs(*)
+    end function
+
+    ! independent basins in one context
+    integer(c_int) function efdc_set_segments(ctx, nseg, first_L) bind(C, name="efdc_set_segments")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nseg
+      integer(c_int32_t), intent(in) :: first_L(*)
+    end function
+
+    integer(c_int) function efdc_segment_stats(ctx, cap, iters, status) &
+        bind(C, name="efdc_segment_stats")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: cap
+      integer(c_int32_t), intent(out) :: iters(*), status(*)
+    end function
+
+    integer(c_int) function efdc_balance_segments(ctx, cap, volume, mass) &
+        bind(C, name="efdc_balance_segments")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: cap
+      real(c_double), intent(out) :: volume(*), mass(*)
+    end function
+
+    ! snapshots
+    integer(c_int) function efdc_state_bytes(ctx, bytes) bind(C, name="efdc_state_bytes")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: ctx
+      integer(c_size_t), intent(out) :: bytes
+    end function
+
+    integer(c_int) function efdc_state_alloc(ctx, buf) bind(C, name="efdc_state_alloc")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), intent(out) :: buf
+    end function
+
+    integer(c_int) function efdc_state_free(ctx, buf) bind(C, name="efdc_state_free")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, buf
+    end function
+
+    integer(c_int) function efdc_state_save(ctx, buf) bind(C, name="efdc_state_save")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, buf
+    end function
+
+    integer(c_int) function efdc_state_load(ctx, buf) bind(C, name="efdc_state_load")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, buf
+    end function
+
+    ! discrete adjoint
+    integer(c_int) function efdc_adj_enable(ctx, opts) bind(C, name="efdc_adj_enable")
+      import :: c_int, c_ptr, efdc_adj_opts_t
+      type(c_ptr), value :: ctx
+      type(efdc_adj_opts_t), intent(in) :: opts
+    end function
+
+    integer(c_int) function efdc_adj_disable(ctx) bind(C, name="efdc_adj_disable")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function efdc_adj_reset(ctx) bind(C, name="efdc_adj_reset")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function efdc_adj_seed_con(ctx, slot, n, L, K, val) &
+        bind(C, name="efdc_adj_seed_con")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: slot, n
+      integer(c_int32_t), intent(in) :: L(*), K(*)
+      real(c_double), intent(in) :: val(*)
+    end function
+
+    integer(c_int) function efdc_adj_step(ctx, step, stats) bind(C, name="efdc_adj_step")
+      import :: c_int, c_ptr, efdc_step_t, efdc_adj_stats_t
+      type(c_ptr), value :: ctx
+      type(efdc_step_t), intent(in) :: step
+      type(efdc_adj_stats_t), intent(out) :: stats
+    end function
+
+    integer(c_int) function efdc_adj_source_grad(ctx, out) bind(C, name="efdc_adj_source_grad")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(*)
+    end function
+
+    integer(c_int) function efdc_adj_get(ctx, field, out) bind(C, name="efdc_adj_get")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: field
+      real(c_double), intent(out) :: out(*)
+    end function
+
+    ! autotuner; the report is an efdc_tune_report_t the caller lays out
+    integer(c_int) function efdc_autotune(ctx, on) bind(C, name="efdc_autotune")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function
+
+    integer(c_int) function efdc_tune_report(ctx, out) bind(C, name="efdc_tune_report")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, out
+    end function
+
+    ! kernel timers; the report is an efdc_timers_t the caller lays out
+    integer(c_int) function efdc_timers_enable(ctx, on) bind(C, name="efdc_timers_enable")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function
+
+    integer(c_int) function efdc_timers_read(ctx, out) bind(C, name="efdc_timers_read")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, out
+    end function
+
+    ! device pointer of an array (for hosts that launch their own kernels)
+    integer(c_int) function efdc_device_ptr(ctx, id, ptr, pitch) bind(C, name="efdc_device_ptr")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: id
+      type(c_ptr), intent(out) :: ptr
+      integer(c_int64_t), intent(out) :: pitch
+    end function
+
+    ! achievable HBM bandwidth of a plain copy
+    integer(c_int) function efdc_stream_copy_gbs(ctx, bytes, reps, gbs) &
+        bind(C, name="efdc_stream_copy_gbs")
+      import :: c_int, c_ptr, c_size_t, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_size_t), value :: bytes
+      integer(c_int32_t), value :: reps
+      real(c_double), intent(out) :: gbs
+    end function
+
+    ! depth-uniform stand-in of the internal mode (no reference counterpart)
+    integer(c_int) function efdc_closure_set_manning(ctx, n_lcm) &
+        bind(C, name="efdc_closure_set_manning")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: n_lcm(*)
+    end function
+
+    integer(c_int) function efdc_closure_pre(ctx) bind(C, name="efdc_closure_pre")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function efdc_closure_post(ctx) bind(C, name="efdc_closure_post")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function

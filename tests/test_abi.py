@@ -64,3 +64,12 @@ def test_context_without_device_fails_loudly():
     with pytest.raises(lib.EfdcError) as e:
         lib.Context(case.grid)
     assert e.value.status == STATUS["EFDC_ERR_NODEVICE"]
+
+
+def test_fortran_module_binds_every_declared_symbol():
+    """MODULE EFDC_HIP (ISO_C_BINDING) has an interface for every entry
+    point of the header, so a Fortran host can call all of them."""
+    f90 = os.path.join(ROOT, "efdcplus8.5_amd", "fortran", "efdc_hip.f90")
+    bound = set(re.findall(r'name="(efdc_\w+)"', open(f90).read()))
+    missing = [f for f in declared_functions() if f not in bound]
+    assert not missing, missing
