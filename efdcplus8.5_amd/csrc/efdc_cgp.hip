@@ -854,7 +854,7 @@ static int next_tag0(efdc_ctx *c, unsigned &tag0) {
   return EFDC_OK;
 }
 
-int pcg_solve(efdc_ctx *c, bool mixed) {
+int pcg_solve(efdc_ctx *c, bool mixed, bool defer) {
   int s;
   if ((s = pcg_prepare(c))) return s;
   k_pcg_reset<<<1, 9 * TICKET_PAD, 0, c->stream>>>(c->pcg_ctr, c->pcg_abort);
@@ -936,13 +936,23 @@ int pcg_solve(efdc_ctx *c, bool mixed) {
     TIMED(c, TK_PCGMX, s = coop_launch(c, fn, nb, a));
     if (s) return s;
   }
-  int abort_h = 0;
-  HIP_TRY(c, hipMemcpyAsync(&abort_h, c->pcg_abort, sizeof(int), hipMemcpyDeviceToHost,
-                            c->stream));
+  if (!c->pcg_abort_host &&
+      hipHostMalloc((void **)&c->pcg_abort_host, sizeof(int)) != hipSuccess) {
+    c->pcg_abort_host = nullptr;
+    c->err = "hipHostMalloc failed";
+    return EFDC_ERR_NOMEM;
+  }
+  HIP_TRY(c, hipMemcpyAsync(c->pcg_abort_host, c->pcg_abort, sizeof(int),
+                            hipMemcpyDeviceToHost, c->stream));
+  if (defer) {
+    // the caller reads the state and the abort flag after the step
+    c->pcg_deferred = true;
+    return EFDC_OK;
+  }
   HIP_TRY(c, hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost,
                             c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  if (abort_h) {
+  if (*c->pcg_abort_host) {
     c->err = "persistent CG: an exchange between workgroups timed out";
     return EFDC_ERR_HIP;
   }

@@ -304,6 +304,7 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   seg_free(c);
   adj_free(c);
   if (c->st_host) (void)hipHostFree(c->st_host);
+  if (c->pcg_abort_host) (void)hipHostFree(c->pcg_abort_host);
   if (c->tide_host) (void)hipHostFree(c->tide_host);
   for (auto &b : c->pbc) {
     dfree(b.L); dfree(b.ispb); dfree(b.pser); dfree(b.pcb); dfree(b.psb);
@@ -335,6 +336,7 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
 
 int efdc_set_params(efdc_ctx_t *c, const efdc_params_t *p) {
   if (!c || !p) return EFDC_ERR_ARG;
+  c->symok_cache = -1;
   if (p->store != c->store) {
     c->err = "the storage precision is fixed at context creation";
     return EFDC_ERR_ARG;
@@ -353,6 +355,7 @@ int efdc_set_params(efdc_ctx_t *c, const efdc_params_t *p) {
 
 int efdc_set_array(efdc_ctx_t *c, int32_t id, const double *host) {
   if (!c || !host || !valid_id(id)) return EFDC_ERR_ARG;
+  c->symok_cache = -1;
   if (id < EFDC_NUM_H) {
     HIP_TRY(c, hipMemcpyAsync(c->dH[id], host, sizeof(double) * c->LCM,
                               hipMemcpyHostToDevice, c->stream));
@@ -425,6 +428,7 @@ int efdc_device_ptr(efdc_ctx_t *c, int32_t id, void **ptr, int64_t *pitch) {
 }
 
 int efdc_set_open_bc(efdc_ctx_t *c, const efdc_open_bc_t *bc) {
+  if (c) c->symok_cache = -1;
   if (!c || !bc || bc->mtide < 0 || (bc->mtide > 0 && !bc->tcp))
     return EFDC_ERR_ARG;
   const int LA = c->LA, LC = c->LC;

@@ -186,6 +186,15 @@ struct efdc_ctx {
   struct TimerRec { int id; hipEvent_t a, b; };
   std::vector<TimerRec> trec;
 
+  // symmetry of the scaled system (k_puv_scale's flag) as last seen by the
+  // host, -1 unknown: it is fixed by the grid, the face masks and the
+  // open-boundary tables (the two coefficients of a face are the same
+  // expression), so the setters of those reset it
+  int symok_cache = -1;
+  // the persistent solve of a CALPUV2C step leaves its checks to the
+  // step's final read-back (one host round trip per step)
+  bool pcg_deferred = false;
+  int *pcg_abort_host = nullptr;  // pinned
   // autotuner (efdc_autotune): launch shape of the column transport
   bool tune_pending = false;
   int tran_ncg = 0, tran_w3 = 0;  // 0: not tuned
@@ -486,7 +495,7 @@ inline int dev_alloc(efdc_ctx *c, void **p, size_t bytes) {
 int efdc_grid_blocks(int n);
 // persistent CONGRAD (efdc_cgp.hip)
 bool pcg_applicable(efdc_ctx *c, bool mixed);
-int pcg_solve(efdc_ctx *c, bool mixed);
+int pcg_solve(efdc_ctx *c, bool mixed, bool defer = false);
 // independent basins (efdc_seg.hip)
 int seg_min_run(efdc_ctx *c);
 int seg_cg_run(efdc_ctx *c, efdc_cg_stats_t *stats);

@@ -1001,24 +1001,8 @@ int cg_mixed(efdc_ctx *c) {
   return EFDC_OK;
 }
 
-int cg_dispatch(efdc_ctx *c, efdc_cg_stats_t *stats) {
-  int s;
-  if (c->nseg > 1) return seg_cg_run(c, stats);
-  bool mixed = (c->prm.cg_mode == EFDC_CG_MIXED);
-  if (mixed) {
-    // the mixed operator is the compact symmetric one
-    if ((s = read_state(c))) return s;
-    if (!c->rowmajor || c->st_host->symok != 0) mixed = false;
-  }
-  if (c->prm.cg_kernel == EFDC_CG_KERNEL_PERSISTENT && !pcg_applicable(c, mixed)) {
-    c->err = "persistent CG requested but the rows do not fit one co-resident grid";
-    return EFDC_ERR_ARG;
-  }
-  if (pcg_applicable(c, mixed))
-    s = pcg_solve(c, mixed);
-  else
-    s = mixed ? cg_mixed(c) : cg_fp64(c);
-  if (s) return s;
+// Statistics and status of a finished solve from the host copy of the state.
+int cg_finish(efdc_ctx *c, bool mixed, efdc_cg_stats_t *stats) {
   const CgState &h = *c->st_host;
   if (stats) {
     stats->iters = h.iter;
@@ -1032,6 +1016,38 @@ int cg_dispatch(efdc_ctx *c, efdc_cg_stats_t *stats) {
     return EFDC_ERR_CG_MAXITER;
   }
   return EFDC_OK;
+}
+
+// defer: a persistent solve leaves the read-back of its state to the caller
+// (puv_run reads it with the rest of the step); *ran_mixed reports the mode.
+int cg_dispatch(efdc_ctx *c, efdc_cg_stats_t *stats, bool defer = false,
+                bool *ran_mixed = nullptr) {
+  int s;
+  c->pcg_deferred = false;
+  if (c->nseg > 1) return seg_cg_run(c, stats);
+  bool mixed = (c->prm.cg_mode == EFDC_CG_MIXED);
+  if (mixed) {
+    // the mixed operator is the compact symmetric one; the symmetry flag
+    // is read from the device once and kept (it follows from the grid,
+    // the face masks and the boundary tables)
+    if (c->symok_cache < 0) {
+      if ((s = read_state(c))) return s;
+      c->symok_cache = c->st_host->symok;
+    }
+    if (!c->rowmajor || c->symok_cache != 0) mixed = false;
+  }
+  if (ran_mixed) *ran_mixed = mixed;
+  if (c->prm.cg_kernel == EFDC_CG_KERNEL_PERSISTENT && !pcg_applicable(c, mixed)) {
+    c->err = "persistent CG requested but the rows do not fit one co-resident grid";
+    return EFDC_ERR_ARG;
+  }
+  if (pcg_applicable(c, mixed))
+    s = pcg_solve(c, mixed, defer);
+  else
+    s = mixed ? cg_mixed(c) : cg_fp64(c);
+  if (s) return s;
+  if (c->pcg_deferred) return EFDC_OK;
+  return cg_finish(c, mixed, stats);
 }
 
 }  // namespace
@@ -1158,15 +1174,34 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
   TIMED(c, TK_PUV_SCALE, k_puv_scale<<<nb, EFDC_BLOCK, 0, c->stream>>>(
       h, c->dLW, c->dLE, c->dLN, nwet, lc0, c->st, segs ? c->d_segmin : nullptr,
       segs ? c->d_segid : nullptr));
-  int s = cg_dispatch(c, stats);
+  bool ran_mixed = false;
+  int s = cg_dispatch(c, stats, /*defer=*/true, &ran_mixed);
   if (s != EFDC_OK && s != EFDC_ERR_CG_MAXITER) return s;
-  const int cgs = s;
+  int cgs = s;
   if (adj_recording(c) && (s = adj_record_ps(c))) return s;
   TIMED(c, TK_PUV_POST1, k_puv_post1<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, dt));
   TIMED(c, TK_PUV_POST2, k_puv_post2<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->pbcflag,
                                                 nwet, dt, G));
   TIMED(c, TK_PUV_POST3, k_puv_post3<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, c->st));
   if ((s = read_state(c))) return s;
+  if (c->pcg_deferred) {
+    c->pcg_deferred = false;
+    if (*c->pcg_abort_host) {
+      c->err = "persistent CG: an exchange between workgroups timed out";
+      return EFDC_ERR_HIP;
+    }
+    cgs = cg_finish(c, ran_mixed, stats);
+  }
+  if (c->nseg == 1 && c->st_host->symok != c->symok_cache && c->symok_cache >= 0) {
+    // cannot happen for a fixed grid and boundary tables (the cache is
+    // reset by their setters); refuse rather than keep a solve made with
+    // the wrong operator
+    c->symok_cache = c->st_host->symok;
+    if (ran_mixed) {
+      c->err = "the symmetry of the system changed without a setter call";
+      return EFDC_ERR_STATE;
+    }
+  }
   if (stats) stats->ccmnm = c->st_host->ccmnm;
   if (cgs != EFDC_OK) return cgs;
   if (c->st_host->negdep) {
