@@ -42,6 +42,8 @@
 // flag and leaves, the others see the flag and leave too, and the host
 // reports the failure.  All workgroups compute the same sums from the
 // same partials in the same order, so they take the same branches.
+#include <cstdio>
+
 #include "efdc_internal.h"
 
 namespace {
@@ -509,6 +511,7 @@ struct PMxArgs {
   CgState *st;
   double rsqm, rel_tol, rtol2;
   int iterm, max_outer;
+  int pdeg;                  // terms of the workgroup-local Neumann preconditioner
 };
 
 template <int CPT>
@@ -516,12 +519,18 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
   __shared__ float lds[2 * WIN + CPT * PB];
   __shared__ int hl[2 * WIN];
   __shared__ unsigned raw[3 * HMAX];
+  // y (the inner solution) and z = M^-1 w of the own rows live in LDS:
+  // with them in registers the 6..8-cells-per-thread shapes would spill
+  __shared__ float lys[CPT * PB], lzs[CPT * PB];
   const int b0 = 1 + (int)blockIdx.x * CPT * PB;
   const int b1 = min(b0 + CPT * PB, a.nwet + 1);
   const int lc0 = a.lc0;
 
   float as_[CPT], an_[CPT], aw_[CPT], ae_[CPT];
-  float r[CPT], w[CPT], s[CPT], p[CPT], y[CPT];
+  float r[CPT], w[CPT], s[CPT], p[CPT];
+  float u[CPT], q[CPT];  // M^-1 r, M^-1 s
+  auto Y = [&](int k) -> float & { return lys[k * PB + threadIdx.x]; };
+  auto Z = [&](int k) -> float & { return lzs[k * PB + threadIdx.x]; };
   int js[CPT], jn[CPT];
   unsigned expo = 0u;
 
@@ -536,6 +545,42 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
            (size_t)opaque(i);
   };
 
+  // M^-1 v: the first pdeg terms of the Neumann series of A' restricted to
+  // the workgroup's own cells, sum_j N^j v with N = I - A' (couplings to
+  // cells of other workgroups dropped), by Horner's rule t <- v + N t.  M^-1
+  // is a polynomial in the symmetric local block whose eigenvalues lie in
+  // (-1, 1), so it is symmetric positive definite and CG stays CG; it needs
+  // no exchange between workgroups.  The own slots of the LDS window carry t.
+  auto precond = [&](const float (&v)[CPT], float (&t)[CPT]) {
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) t[k] = v[k];
+    for (int d = 1; d < a.pdeg; ++d) {
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) {
+        const int i = b0 + k * PB + (int)threadIdx.x;
+        if (i < b1) lds[slot(i)] = t[k];
+      }
+      __syncthreads();
+      float tn[CPT];
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) {
+        const int i = b0 + k * PB + (int)threadIdx.x;
+        tn[k] = t[k];
+        if (i < b1) {
+          const float ts = inside(js[k]) ? lds[slot(js[k])] : 0.0f;
+          const float tnn = inside(jn[k]) ? lds[slot(jn[k])] : 0.0f;
+          const float tw = inside(i - 1) ? lds[slot(i - 1)] : 0.0f;
+          const float te = inside(i + 1) ? lds[slot(i + 1)] : 0.0f;
+          tn[k] = v[k] - (as_[k] * ts + an_[k] * tnn + aw_[k] * tw + ae_[k] * te);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) t[k] = tn[k];
+    }
+    __syncthreads();  // every read of the own slots is done
+  };
+
   for (int g = threadIdx.x; g < 2 * WIN; g += PB) hl[g] = 0;
   __syncthreads();
   double fp2 = 0.0;
@@ -543,7 +588,10 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
   for (int k = 0; k < CPT; ++k) {
     const int i = b0 + k * PB + (int)threadIdx.x;
     as_[k] = an_[k] = aw_[k] = ae_[k] = 0.0f;
-    r[k] = w[k] = s[k] = p[k] = y[k] = 0.0f;
+    r[k] = w[k] = s[k] = p[k] = 0.0f;
+    u[k] = q[k] = 0.0f;
+    Y(k) = 0.0f;
+    Z(k) = 0.0f;
     js[k] = jn[k] = lc0;
     if (i < b1) {
       const int s_ = a.ls[i], n_ = a.ln[i];
@@ -610,11 +658,19 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
                           ce * fetch(&P[io + 1]);
         const float rs = (float)(dinv(io) * rr);
         r[k] = rs;
-        y[k] = 0.0f;
+        Y(k) = 0.0f;
         v3[0] += rr * rr;
         v3[1] += (double)rs * (double)rs;
-        lds[slot(io)] = rs;
-        if (expo & (1u << k)) put_u32(hxw(io, tag, 0), __float_as_uint(rs));
+      }
+    }
+    // u0 = M^-1 r0, exported for the neighbours' A u0
+    precond(r, u);
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int i = b0 + k * PB + (int)threadIdx.x;
+      if (i < b1) {
+        lds[slot(i)] = u[k];
+        if (expo & (1u << k)) put_u32(hxw(i, tag, 0), __float_as_uint(u[k]));
       }
     }
     fp2 = 0.0;
@@ -632,35 +688,47 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
     if (rsq64 <= rsqm) break;
     if (outer >= a.max_outer || bn_in == 0.0) { status = 1; break; }
 
-    // ---- w0 = A' r0; gamma0 = (r,r), delta0 = (w,r); s_-1 = p_-1 = 0
+    // ---- w0 = A' u0, z0 = M^-1 w0; gamma0 = (r,u), delta0 = (w,u),
+    // rho0 = (r,r); s_-1 = p_-1 = q_-1 = 0
     for (int h = threadIdx.x; h < nhalo; h += PB) lds[slot(hl[h])] = __uint_as_float(raw[h]);
     __syncthreads();
     ++tag;
-    double v2[2] = {0.0, 0.0};
+    double v2[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int i = b0 + k * PB + (int)threadIdx.x;
       s[k] = 0.0f;
       p[k] = 0.0f;
+      q[k] = 0.0f;
       if (i < b1) {
-        w[k] = r[k] + as_[k] * rval(js[k]) + an_[k] * rval(jn[k]) +
+        w[k] = u[k] + as_[k] * rval(js[k]) + an_[k] * rval(jn[k]) +
                aw_[k] * rval(i - 1) + ae_[k] * rval(i + 1);
-        v2[0] += (double)r[k] * (double)r[k];
-        v2[1] += (double)w[k] * (double)r[k];
-        if (expo & (1u << k)) {
-          put_u32(hxw(i, tag, 0), __float_as_uint(r[k]));
-          put_u32(hxw(i, tag, 1), __float_as_uint(w[k]));
+        v2[0] += (double)r[k] * (double)u[k];
+        v2[1] += (double)w[k] * (double)u[k];
+        v2[2] += (double)r[k] * (double)r[k];
+      }
+    }
+    {
+      float zt[CPT];
+      precond(w, zt);
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) {
+        const int i = b0 + k * PB + (int)threadIdx.x;
+        Z(k) = zt[k];
+        if (i < b1 && (expo & (1u << k))) {
+          put_u32(hxw(i, tag, 0), __float_as_uint(u[k]));
+          put_u32(hxw(i, tag, 1), __float_as_uint(zt[k]));
           put_u32(hxw(i, tag, 2), __float_as_uint(0.0f));
         }
       }
     }
-    post<2>(v2, a.xc, tag);
+    post<3>(v2, a.xc, tag);
     double gam, del;
     {
-      double s2[2];
-      if (!exchange<2, 3>(a.xc, tag, hl, nhalo, raw, a.bar, s2)) return;
-      gam = s2[0];
-      del = s2[1];
+      double s3[3];
+      if (!exchange<3, 3>(a.xc, tag, hl, nhalo, raw, a.bar, s3)) return;
+      gam = s3[0];
+      del = s3[1];
     }
     double gam_prev = 0.0, alpha_prev = 1.0;
     inner = 0;
@@ -673,51 +741,64 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
         beta = gam / gam_prev;
         alpha = gam / (del - beta * gam / alpha_prev);
       }
-      // own rows: p = r + beta p, s = w + beta s, y += alpha p, r -= alpha s
+      // own rows: p = u + beta p, s = w + beta s, q = z + beta q,
+      // y += alpha p, r -= alpha s, u -= alpha q
 #pragma unroll
       for (int k = 0; k < CPT; ++k) {
         const int i = b0 + k * PB + (int)threadIdx.x;
-        p[k] = (float)((double)r[k] + beta * (double)p[k]);
+        p[k] = (float)((double)u[k] + beta * (double)p[k]);
         s[k] = (float)((double)w[k] + beta * (double)s[k]);
-        y[k] = (float)((double)y[k] + alpha * (double)p[k]);
+        q[k] = (float)((double)Z(k) + beta * (double)q[k]);
+        Y(k) = (float)((double)Y(k) + alpha * (double)p[k]);
         r[k] = (float)((double)r[k] - alpha * (double)s[k]);
-        if (i < b1) lds[slot(i)] = r[k];
+        u[k] = (float)((double)u[k] - alpha * (double)q[k]);
+        if (i < b1) lds[slot(i)] = u[k];
       }
-      // the outside cells' update, from their published r, w, s
+      // the outside cells' u, from their published u, z, q
       for (int h = threadIdx.x; h < nhalo; h += PB) {
-        const float rj = __uint_as_float(raw[h]);
-        const float wj = __uint_as_float(raw[HMAX + h]);
-        const float sj0 = __uint_as_float(raw[2 * HMAX + h]);
-        const float sj = (float)((double)wj + beta * (double)sj0);
-        lds[slot(hl[h])] = (float)((double)rj - alpha * (double)sj);
+        const float uj = __uint_as_float(raw[h]);
+        const float zj = __uint_as_float(raw[HMAX + h]);
+        const float qj0 = __uint_as_float(raw[2 * HMAX + h]);
+        const float qj = (float)((double)zj + beta * (double)qj0);
+        lds[slot(hl[h])] = (float)((double)uj - alpha * (double)qj);
       }
       __syncthreads();
       ++tag;
-      double v[2] = {0.0, 0.0};
+      double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
       for (int k = 0; k < CPT; ++k) {
         const int i = b0 + k * PB + (int)threadIdx.x;
         if (i < b1) {
-          w[k] = r[k] + as_[k] * rval(js[k]) + an_[k] * rval(jn[k]) +
+          w[k] = u[k] + as_[k] * rval(js[k]) + an_[k] * rval(jn[k]) +
                  aw_[k] * rval(i - 1) + ae_[k] * rval(i + 1);
-          v[0] += (double)r[k] * (double)r[k];
-          v[1] += (double)w[k] * (double)r[k];
-          if (expo & (1u << k)) {
-            put_u32(hxw(i, tag, 0), __float_as_uint(r[k]));
-            put_u32(hxw(i, tag, 1), __float_as_uint(w[k]));
-            put_u32(hxw(i, tag, 2), __float_as_uint(s[k]));
+          v[0] += (double)r[k] * (double)u[k];
+          v[1] += (double)w[k] * (double)u[k];
+          v[2] += (double)r[k] * (double)r[k];
+        }
+      }
+      {
+        float zt[CPT];
+        precond(w, zt);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+          const int i = b0 + k * PB + (int)threadIdx.x;
+          Z(k) = zt[k];
+          if (i < b1 && (expo & (1u << k))) {
+            put_u32(hxw(i, tag, 0), __float_as_uint(u[k]));
+            put_u32(hxw(i, tag, 1), __float_as_uint(zt[k]));
+            put_u32(hxw(i, tag, 2), __float_as_uint(q[k]));
           }
         }
       }
-      post<2>(v, a.xc, tag);
-      double s2[2];
-      if (!exchange<2, 3>(a.xc, tag, hl, nhalo, raw, a.bar, s2)) return;
+      post<3>(v, a.xc, tag);
+      double s3[3];
+      if (!exchange<3, 3>(a.xc, tag, hl, nhalo, raw, a.bar, s3)) return;
       inner += 1;
       gam_prev = gam;
       alpha_prev = alpha;
-      gam = s2[0];
-      del = s2[1];
-      if (gam <= a.rtol2 * bn_in || inner >= a.iterm) break;
+      gam = s3[0];
+      del = s3[1];
+      if (s3[2] <= a.rtol2 * bn_in || inner >= a.iterm) break;
     }
     total += inner;
     ++tag;
@@ -729,7 +810,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
       if (i < b1) {
         const int io = opaque(i);
         double *P = a.x;
-        publish(&P[io], fetch(&P[io]) + dinv(io) * (double)y[k]);
+        publish(&P[io], fetch(&P[io]) + dinv(io) * (double)Y(k));
       }
     }
     ++outer;
@@ -769,6 +850,9 @@ int occupancy(K kern) {
 // Largest cells-per-thread instantiation of each solve.
 constexpr int PCG64_CPT_MAX = 6;
 constexpr int PCGMX_CPT_MAX = 8;
+// Terms of the workgroup-local preconditioner of the mixed persistent solve
+// when the caller leaves the choice to the library.
+constexpr int PCG_PDEG_DEFAULT = 3;
 // Exchange numbers: launch number (1 .. 4095) in the top 12 bits, exchange
 // within the launch in the low 20 bits.
 constexpr int PCG_TAG_BITS = 20;
@@ -922,6 +1006,9 @@ int pcg_solve(efdc_ctx *c, bool mixed, bool defer) {
     a.rtol2 = rtol * rtol;
     a.iterm = c->prm.ITERM;
     a.max_outer = c->prm.mixed_max_outer;
+    a.pdeg = c->prm.mixed_precond_degree > 0 ? c->prm.mixed_precond_degree
+             : c->pcg_pdeg_tuned > 0         ? c->pcg_pdeg_tuned
+                                             : PCG_PDEG_DEFAULT;
     const void *fn = nullptr;
     switch (cpt) {
       case 1: fn = (const void *)k_pcgmx<1>; break;
@@ -956,5 +1043,54 @@ int pcg_solve(efdc_ctx *c, bool mixed, bool defer) {
     c->err = "persistent CG: an exchange between workgroups timed out";
     return EFDC_ERR_HIP;
   }
+  return EFDC_OK;
+}
+
+// Autotuner of the mixed persistent solve: the degree of the local
+// preconditioner.  The solve is repeated from the same P for every
+// candidate (timed with events), P is restored, and the fastest degree is
+// kept for the following solves; the caller then solves once more.
+int pcg_tune_degree(efdc_ctx *c) {
+  int s;
+  if (!c->tune_pbuf && (s = dev_alloc(c, (void **)&c->tune_pbuf, sizeof(double) * c->LCM)))
+    return s;
+  HIP_TRY(c, hipMemcpyAsync(c->tune_pbuf, c->dH[EFDC_P], sizeof(double) * c->LCM,
+                            hipMemcpyDeviceToDevice, c->stream));
+  hipEvent_t e0 = timer_event(c), e1 = timer_event(c);
+  const int cand[5] = {1, 2, 3, 4, 6};
+  float best_ms = 1e30f;
+  int best = PCG_PDEG_DEFAULT, best_slot = -1;
+  for (int j = 0; j < 5; ++j) {
+    c->pcg_pdeg_tuned = cand[j];
+    float tot = 0.f;
+    for (int r = 0; r < 3; ++r) {
+      HIP_TRY(c, hipMemcpyAsync(c->dH[EFDC_P], c->tune_pbuf, sizeof(double) * c->LCM,
+                                hipMemcpyDeviceToDevice, c->stream));
+      HIP_TRY(c, hipEventRecord(e0, c->stream));
+      if ((s = pcg_solve(c, true))) return s;
+      HIP_TRY(c, hipEventRecord(e1, c->stream));
+      HIP_TRY(c, hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+      if (r > 0) tot += ms;  // the first solve is a warm-up
+    }
+    const float avg = tot / 2.f;
+    if (c->tune_n < 8) {
+      std::snprintf(c->tune_name[c->tune_n], 32, "pcg_mixed degree=%d", cand[j]);
+      c->tune_ms[c->tune_n] = avg;
+      ++c->tune_n;
+    }
+    if (avg < best_ms) {
+      best_ms = avg;
+      best = cand[j];
+      best_slot = c->tune_n - 1;
+    }
+  }
+  c->pcg_pdeg_tuned = best;
+  if (best_slot >= 0 && best_slot < 8) c->tune_chosen[best_slot] = 1;
+  c->ev_pool.push_back(e0);
+  c->ev_pool.push_back(e1);
+  HIP_TRY(c, hipMemcpyAsync(c->dH[EFDC_P], c->tune_pbuf, sizeof(double) * c->LCM,
+                            hipMemcpyDeviceToDevice, c->stream));
   return EFDC_OK;
 }

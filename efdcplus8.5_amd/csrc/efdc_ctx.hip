@@ -140,6 +140,7 @@ void efdc_params_default(efdc_params_t *p) {
   p->mixed_inner_rtol = 1e-5;
   p->mixed_max_outer = 30;
   p->cg_kernel = EFDC_CG_KERNEL_AUTO;
+  p->mixed_precond_degree = 0;
 }
 
 // Reason of the last failed efdc_ctx_create on this thread (there is no
@@ -169,6 +170,8 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
     return reject(EFDC_ERR_ARG, "unknown CG mode");
   if (prm->cg_kernel < EFDC_CG_KERNEL_AUTO || prm->cg_kernel > EFDC_CG_KERNEL_PERSISTENT)
     return reject(EFDC_ERR_ARG, "unknown CG kernel choice");
+  if (prm->mixed_precond_degree < 0 || prm->mixed_precond_degree > 16)
+    return reject(EFDC_ERR_ARG, "mixed_precond_degree must be 0..16");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return reject(EFDC_ERR_NODEVICE, "no HIP device visible");
@@ -305,6 +308,7 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   adj_free(c);
   if (c->st_host) (void)hipHostFree(c->st_host);
   if (c->pcg_abort_host) (void)hipHostFree(c->pcg_abort_host);
+  dfree(c->tune_pbuf);
   if (c->tide_host) (void)hipHostFree(c->tide_host);
   for (auto &b : c->pbc) {
     dfree(b.L); dfree(b.ispb); dfree(b.pser); dfree(b.pcb); dfree(b.psb);
@@ -343,6 +347,10 @@ int efdc_set_params(efdc_ctx_t *c, const efdc_params_t *p) {
   }
   if (p->cg_mode == EFDC_CG_MIXED && !c->dis) {
     c->err = "EFDC_CG_MIXED needs a context created in that mode";
+    return EFDC_ERR_ARG;
+  }
+  if (p->mixed_precond_degree < 0 || p->mixed_precond_degree > 16) {
+    c->err = "mixed_precond_degree must be 0..16";
     return EFDC_ERR_ARG;
   }
   if (p->cg_kernel < EFDC_CG_KERNEL_AUTO || p->cg_kernel > EFDC_CG_KERNEL_PERSISTENT) {
@@ -730,9 +738,15 @@ int efdc_set_sources(efdc_ctx_t *c, const efdc_sources_t *src) {
 int efdc_autotune(efdc_ctx_t *c, int32_t on) {
   if (!c) return EFDC_ERR_ARG;
   c->tune_pending = on != 0;
-  if (!on) {
+  c->tune_cg_pending = on != 0;
+  if (on) {
+    c->tune_n = 0;
+    c->tune_best = -1;
+    for (int &f : c->tune_chosen) f = 0;
+  } else {
     c->tran_ncg = 0;
     c->tran_w3 = 0;
+    c->pcg_pdeg_tuned = 0;
   }
   return EFDC_OK;
 }
@@ -741,10 +755,10 @@ int efdc_tune_report(efdc_ctx_t *c, efdc_tune_report_t *out) {
   if (!c || !out) return EFDC_ERR_ARG;
   std::memset(out, 0, sizeof(*out));
   out->n = c->tune_n;
-  out->chosen = c->tune_best;
   for (int j = 0; j < c->tune_n && j < 8; ++j) {
     std::memcpy(out->v[j].name, c->tune_name[j], 32);
     out->v[j].ms = c->tune_ms[j];
+    out->v[j].chosen = c->tune_chosen[j];
   }
   return EFDC_OK;
 }

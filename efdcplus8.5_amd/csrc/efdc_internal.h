@@ -195,12 +195,16 @@ struct efdc_ctx {
   // step's final read-back (one host round trip per step)
   bool pcg_deferred = false;
   int *pcg_abort_host = nullptr;  // pinned
-  // autotuner (efdc_autotune): launch shape of the column transport
-  bool tune_pending = false;
+  // autotuner (efdc_autotune): launch shape of the column transport and
+  // the preconditioner degree of the mixed persistent solve
+  bool tune_pending = false, tune_cg_pending = false;
+  int pcg_pdeg_tuned = 0;         // 0: not tuned
+  double *tune_pbuf = nullptr;    // P saved while solves are timed
   int tran_ncg = 0, tran_w3 = 0;  // 0: not tuned
   int tune_n = 0, tune_best = -1;
   char tune_name[8][32] = {};
   float tune_ms[8] = {};
+  int tune_chosen[8] = {};
   // discrete adjoint (efdc_adj.hip), allocated by efdc_adj_enable
   struct AdjState *adj = nullptr;
 
@@ -496,6 +500,7 @@ int efdc_grid_blocks(int n);
 // persistent CONGRAD (efdc_cgp.hip)
 bool pcg_applicable(efdc_ctx *c, bool mixed);
 int pcg_solve(efdc_ctx *c, bool mixed, bool defer = false);
+int pcg_tune_degree(efdc_ctx *c);
 // independent basins (efdc_seg.hip)
 int seg_min_run(efdc_ctx *c);
 int seg_cg_run(efdc_ctx *c, efdc_cg_stats_t *stats);

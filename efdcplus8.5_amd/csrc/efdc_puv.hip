@@ -1041,6 +1041,11 @@ int cg_dispatch(efdc_ctx *c, efdc_cg_stats_t *stats, bool defer = false,
     c->err = "persistent CG requested but the rows do not fit one co-resident grid";
     return EFDC_ERR_ARG;
   }
+  if (pcg_applicable(c, mixed) && mixed && c->tune_cg_pending &&
+      c->prm.mixed_precond_degree == 0) {
+    c->tune_cg_pending = false;
+    if ((s = pcg_tune_degree(c))) return s;
+  }
   if (pcg_applicable(c, mixed))
     s = pcg_solve(c, mixed, defer);
   else

@@ -1102,7 +1102,6 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
       // outflow boundary layers), so the shapes are timed on the step
       // itself; the chosen one runs last
       c->tune_pending = false;
-      c->tune_n = 0;
       int best = -1;
       float best_ms = 1e30f;
       hipEvent_t e0 = timer_event(c), e1 = timer_event(c);
@@ -1134,7 +1133,7 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
           }
         }
       }
-      c->tune_best = best;
+      if (best >= 0) c->tune_chosen[best] = 1;
       c->ev_pool.push_back(e0);
       c->ev_pool.push_back(e1);
     }

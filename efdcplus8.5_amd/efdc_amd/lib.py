@@ -45,7 +45,8 @@ class Params(C.Structure):
                 ("ITERM", C.c_int32), ("RSQM", C.c_double),
                 ("cg_rel_tol", C.c_double), ("cg_mode", C.c_int32),
                 ("store", C.c_int32), ("mixed_inner_rtol", C.c_double),
-                ("mixed_max_outer", C.c_int32), ("cg_kernel", C.c_int32)]
+                ("mixed_max_outer", C.c_int32), ("cg_kernel", C.c_int32),
+                ("mixed_precond_degree", C.c_int32)]
 
 
 class PbcSide(C.Structure):
@@ -100,11 +101,11 @@ class Balance(C.Structure):
 
 
 class TuneEntry(C.Structure):
-    _fields_ = [("name", C.c_char * 32), ("ms", C.c_float)]
+    _fields_ = [("name", C.c_char * 32), ("ms", C.c_float), ("chosen", C.c_int32)]
 
 
 class TuneReport(C.Structure):
-    _fields_ = [("n", C.c_int32), ("chosen", C.c_int32), ("v", TuneEntry * 8)]
+    _fields_ = [("n", C.c_int32), ("v", TuneEntry * 8)]
 
 
 class AdjOpts(C.Structure):
@@ -390,7 +391,7 @@ class Context:
         r = TuneReport()
         self._check(self.lib.efdc_tune_report(self.h, C.byref(r)))
         shapes = {r.v[j].name.decode(): r.v[j].ms for j in range(r.n)}
-        chosen = r.v[r.chosen].name.decode() if 0 <= r.chosen < r.n else None
+        chosen = [r.v[j].name.decode() for j in range(r.n) if r.v[j].chosen]
         return dict(shapes=shapes, chosen=chosen)
 
     def stream_copy_gbs(self, mib: int = 1024, reps: int = 10) -> float:

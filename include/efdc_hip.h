@@ -98,12 +98,16 @@ typedef struct {
   double mixed_inner_rtol; /* EFDC_CG_MIXED: inner FP32 relative reduction  */
   int32_t mixed_max_outer; /* EFDC_CG_MIXED: refinement step cap            */
   int32_t cg_kernel;       /* efdc_cg_kernel: how the solve is launched     */
+  int32_t mixed_precond_degree; /* EFDC_CG_MIXED, persistent solve: terms of
+                              the workgroup-local Neumann preconditioner
+                              (1 = none; 0 = the library's default)          */
 } efdc_params_t;
 
 /* Fill *p with the values EFDC uses by default: G=9.81, RITB1=1, SNLT=0,
    HDRY=0.2, TCON_TBEGIN=0, ITERM=200, RSQM=1e-9 (the sample decks,
    SampleModels/Lake_Washington/efdc.inp:92), cg_rel_tol=0, FP64 CG, FP64
-   storage, mixed_inner_rtol=1e-5, mixed_max_outer=30, cg_kernel=AUTO. */
+   storage, mixed_inner_rtol=1e-5, mixed_max_outer=30, cg_kernel=AUTO,
+   mixed_precond_degree=0. */
 void efdc_params_default(efdc_params_t *p);
 
 /* ------------------------------------------------------------- array ids */
@@ -322,12 +326,16 @@ int efdc_stream_copy_gbs(efdc_ctx_t *ctx, size_t bytes, int32_t reps, double *gb
    (constituent lanes per workgroup 4/2/1, two register budgets) on that
    step's own data and keep the fastest for the following steps; every shape
    gives the same bits.  efdc_autotune(ctx, 0) returns to the default shape.
-   efdc_tune_report lists the shapes timed (ms per launch) and the choice. */
+   With the mixed CG and the persistent solve, the next efdc_calpuv2c also
+   times the solve with preconditioner degrees 1, 2, 3, 4, 6 from the same P
+   and keeps the fastest (mixed_precond_degree = 0 only).
+   efdc_tune_report lists the candidates timed and the ones kept. */
 typedef struct {
-  int32_t n, chosen;
+  int32_t n;
   struct {
     char name[32];
-    float ms;
+    float ms;        /* per launch (transport) or per solve (CG)            */
+    int32_t chosen;  /* 1 for the candidate kept                            */
   } v[8];
 } efdc_tune_report_t;
 int efdc_autotune(efdc_ctx_t *ctx, int32_t on);
