@@ -141,6 +141,11 @@ void efdc_params_default(efdc_params_t *p) {
   p->mixed_max_outer = 30;
   p->cg_kernel = EFDC_CG_KERNEL_AUTO;
   p->mixed_precond_degree = 0;
+  p->tran_shape = 0;
+}
+
+static bool tran_shape_valid(int q) {
+  return q == 0 || q == 1 || q == 2 || q == 4 || q == -1 || q == -2 || q == -4;
 }
 
 // Reason of the last failed efdc_ctx_create on this thread (there is no
@@ -172,6 +177,8 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
     return reject(EFDC_ERR_ARG, "unknown CG kernel choice");
   if (prm->mixed_precond_degree < 0 || prm->mixed_precond_degree > 16)
     return reject(EFDC_ERR_ARG, "mixed_precond_degree must be 0..16");
+  if (!tran_shape_valid(prm->tran_shape))
+    return reject(EFDC_ERR_ARG, "tran_shape must be 0, 1, 2, 4, -1, -2 or -4");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return reject(EFDC_ERR_NODEVICE, "no HIP device visible");
@@ -210,6 +217,8 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
       }
       if (!(a == L - 1 || a == LC) || !(b == L + 1 || b == LC)) rowmajor = false;
       // mutual: a wet neighbour in one direction sees L in the opposite one
+      // (and no neighbour is the dummy cell 1)
+      if (a == 1 || b == 1 || s == 1 || n == 1) mutual = false;
       if (a != LC && a >= 2 && a <= LA && g->LEC[a - 1] != L) mutual = false;
       if (s != LC && s >= 2 && s <= LA && g->LNC[s - 1] != L) mutual = false;
       if (b != LC && b >= 2 && b <= LA && g->LWC[b - 1] != L) mutual = false;
@@ -351,6 +360,10 @@ int efdc_set_params(efdc_ctx_t *c, const efdc_params_t *p) {
   }
   if (p->mixed_precond_degree < 0 || p->mixed_precond_degree > 16) {
     c->err = "mixed_precond_degree must be 0..16";
+    return EFDC_ERR_ARG;
+  }
+  if (!tran_shape_valid(p->tran_shape)) {
+    c->err = "tran_shape must be 0, 1, 2, 4, -1, -2 or -4";
     return EFDC_ERR_ARG;
   }
   if (p->cg_kernel < EFDC_CG_KERNEL_AUTO || p->cg_kernel > EFDC_CG_KERNEL_PERSISTENT) {

@@ -496,6 +496,249 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   tran_col_body<T>(a, vcus);
 }
 
+// ---------------------------------------------------------------------
+// k_tran_colm: the same column step with NCW constituents per thread.
+//
+// One wave owns 64 columns and NCW constituents and walks the layers once:
+// the transport fields (flows, H1PK, HPKI, AB) are loaded and converted and
+// the tridiagonal factors of the vertical solve (EEB, CCLB, VCU, with their
+// FP64 divide) are computed once per layer for all NCW constituents, where
+// the one-constituent kernel repeats them per constituent.  The kernel
+// relies on mutual neighbour maps (LWC(LEC(L)) = L and LSC(LNC(L)) = L for
+// wet neighbours), so the donor cells of the east and north faces that lie
+// in the column itself are CON1 of the column: six CON1 loads per
+// constituent and layer (the column, W, S, E, N, and the layer above).
+//
+// The loads of layer k+1 are issued before layer k is computed, and the
+// loop body issues the same loads on every path (a missing neighbour or
+// layer reads a valid address whose value is then not used), so the
+// compiler's wait counts stay exact and the prefetch is never drained
+// early.  Waves without a source or open-boundary column (all but a few)
+// run a copy of the loop without those branches (RARE = false).
+// Every value and operation order is that of tran_col_body, so both
+// kernels give the same bits.  LDS: VCU [KC][64] (FP64, shared by the
+// waves of a block, which all compute the same values) and the forward
+// values [waves][NCW][KC][64].
+// ---------------------------------------------------------------------
+template <typename T, int NCW>
+struct LayerM {
+  T u_i, u_e, v_i, v_n, ww, h1pk, hpki, ab;
+  T ckk, ckm;  // CDZKK/CDZKMK (not used when they are uniform)
+  T c1[NCW], cw[NCW], cs[NCW], ce[NCW], cn[NCW], cup[NCW];
+};
+
+template <typename T>
+__device__ __forceinline__ T ldb(const T *base, unsigned boff) {
+  return *(const T *)((const char *)base + boff);
+}
+template <typename T>
+__device__ __forceinline__ void stb(T x, T *base, unsigned boff) {
+  *(T *)((char *)base + boff) = x;
+}
+
+template <typename T, int NCW, bool RARE>
+__device__ __forceinline__ void tran_colm_run(const TranArgs<T> &a, double *vcus, int i, int c0,
+                                              int src0, int rec) {
+  const int KC = a.KC, nc = a.nc;
+  const int w = a.lw[i], s = a.ls[i];
+  const int e = a.rowmajor ? i + 1 : a.le[i];
+  const int n = a.ln[i];
+  const bool eg = a.rowmajor ? (a.le[i] == a.lc0) : (e == a.lc0);
+  const bool ng = (n == a.lc0);
+  // the east / north values of a column without that neighbour are read
+  // from the column itself and then not used (flux 0, face flow 0)
+  const unsigned bi = (unsigned)i * sizeof(T), bw = (unsigned)w * sizeof(T),
+                 bs = (unsigned)s * sizeof(T), be = eg ? bi : (unsigned)e * sizeof(T),
+                 bn = ng ? bi : (unsigned)n * sizeof(T);
+  const size_t pitch = (size_t)a.pitch;
+  const double DDELT = a.dt, DELTD2 = a.dt;
+  const double dxyip = a.dxyip[i], hpi = a.hpi[i];
+  int4 bc = make_int4(-1, -1, -1, -1);
+  if (RARE && rec >= 0) bc = a.bcinfo[rec];
+  const bool vdk = a.vd && KC > 1;
+  // CDZKK/CDZKMK planes when they vary by column, else any plane that is
+  // read anyway (the values are then taken from the per-layer table)
+  const bool colz = vdk && !a.dz_uniform;
+  const T *pkk = colz ? a.cdzkk : a.hpki, *pkm = colz ? a.cdzkmk : a.hpki;
+  bool on[NCW];
+  const T *C1[NCW];
+#pragma unroll
+  for (int j = 0; j < NCW; ++j) {
+    // the launch has ceil(nc/NCW) waves, so the first slot is always used
+    on[j] = j == 0 || c0 + j < nc;
+    C1[j] = a.old[on[j] ? c0 + j : 0];
+  }
+
+  double *vcu = vcus + threadIdx.x;  // stride 64
+  T *ys = (T *)(vcus + (size_t)KC * 64) + (size_t)threadIdx.y * NCW * KC * 64 + threadIdx.x;
+  // layer k; its upward neighbour plane is k+1 below the top, else k
+  auto load = [&](int k, LayerM<T, NCW> &L) {
+    const size_t o = (size_t)k * pitch;
+    const size_t ou = (size_t)(k + 1 < KC ? k + 1 : k) * pitch;
+    const T *U = a.uhdy2 + o, *V = a.vhdx2 + o;
+    L.u_i = ldb(U, bi);
+    L.u_e = ldb(U, be);
+    L.v_i = ldb(V, bi);
+    L.v_n = ldb(V, bn);
+    L.ww = ldb(a.w2 + o + pitch, bi);  // W2 has KC+1 planes
+    L.h1pk = ldb(a.h1pk + o, bi);
+    L.hpki = ldb(a.hpki + o, bi);
+    L.ab = ldb(a.ab + o, bi);
+    L.ckk = ldb(pkk + o, bi);
+    L.ckm = ldb(pkm + o, bi);
+#pragma unroll
+    for (int j = 0; j < NCW; ++j) {
+      if (!on[j]) continue;
+      const T *P = C1[j] + o;
+      L.c1[j] = ldb(P, bi);
+      L.cw[j] = ldb(P, bw);
+      L.cs[j] = ldb(P, bs);
+      L.ce[j] = ldb(P, be);
+      L.cn[j] = ldb(P, bn);
+      L.cup[j] = ldb(C1[j] + ou, bi);
+    }
+  };
+  LayerM<T, NCW> cur, nxt;
+  load(0, cur);
+  double fwp[NCW], yprev[NCW];
+#pragma unroll
+  for (int j = 0; j < NCW; ++j) fwp[j] = yprev[j] = 0.0;
+  double vcu_prev = 0.0, ab_prev = 0.0;
+  for (int k = 0; k < KC; ++k) {
+    const int K = k + 1;
+    const size_t o = (size_t)k * pitch;
+    // the top layer re-reads itself: the same loads on every trip
+    load(K < KC ? k + 1 : k, nxt);
+    const bool below_top = K < KC;
+    const double u_i = (double)cur.u_i, u_e = eg ? 0.0 : (double)cur.u_e,
+                 v_i = (double)cur.v_i, v_n = ng ? 0.0 : (double)cur.v_n;
+    const double ww = below_top ? (double)cur.ww : 0.0;
+    const double h1pk = (double)cur.h1pk, hpki = (double)cur.hpki;
+    const double abk = (vdk && below_top) ? (double)cur.ab : 0.0;
+    double cclb = 0.0, eeb = 1.0, vcu_k = 0.0;
+    if (vdk) {
+      const double ckk = a.dz_uniform ? a.cdzkk_k[k] : (double)cur.ckk;
+      const double ckm = a.dz_uniform ? a.cdzkmk_k[k] : (double)cur.ckm;
+      if (K == 1) {
+        const double RCDZKK = -DELTD2 * ckk;
+        const double CCUBTMP = RCDZKK * hpi * abk;
+        const double CCMBTMP = 1.0 - CCUBTMP;
+        eeb = 1.0 / CCMBTMP;
+        vcu_k = CCUBTMP * eeb;
+      } else if (K < KC) {
+        const double RCDZKMK = -DELTD2 * ckm;
+        const double RCDZKK = -DELTD2 * ckk;
+        cclb = RCDZKMK * hpi * ab_prev;
+        const double CCUBTMP = RCDZKK * hpi * abk;
+        const double CCMBTMP = 1.0 - cclb - CCUBTMP;
+        eeb = 1.0 / (CCMBTMP - cclb * vcu_prev);
+        vcu_k = CCUBTMP * eeb;
+      } else {
+        const double RCDZKMK = -DELTD2 * ckm;
+        cclb = RCDZKMK * hpi * ab_prev;
+        const double CCMBTMP = 1.0 - cclb;
+        eeb = 1.0 / (CCMBTMP - cclb * vcu_prev);
+      }
+      vcu[k * 64] = vcu_k;
+    }
+    const bool pu = cur.u_i >= (T)0, pv = cur.v_i >= (T)0, pue = cur.u_e >= (T)0,
+               pvn = cur.v_n >= (T)0, pw = cur.ww >= (T)0;
+#pragma unroll
+    for (int j = 0; j < NCW; ++j) {
+      if (!on[j]) continue;
+      const int cc = c0 + j;
+      const double c1 = (double)cur.c1[j];
+      // upwind fluxes (LUPU/LUPV/KUPW of EFDC/calconc.f90:95-118); the
+      // donor is chosen in the storage type, where the sign test and the
+      // value are those of the FP64 conversion
+      const double fu_i = u_i * (double)(pu ? cur.cw[j] : cur.c1[j]);
+      const double fv_i = v_i * (double)(pv ? cur.cs[j] : cur.c1[j]);
+      const double fu_e = eg ? 0.0 : u_e * (double)(pue ? cur.c1[j] : cur.ce[j]);
+      const double fv_n = ng ? 0.0 : v_n * (double)(pvn ? cur.c1[j] : cur.cn[j]);
+      const double fw = below_top ? ww * (double)(pw ? cur.c1[j] : cur.cup[j]) : 0.0;
+      const double fqc = (RARE && src0 >= 0) ? col_sources(a, src0, k, cc, c1) : 0.0;
+      // EFDC/caltran.f90:194-196,243
+      const double cd = c1 * h1pk + DDELT * ((fqc + fu_i - fu_e + fv_i - fv_n) * dxyip +
+                                             (fwp[j] - fw));
+      double con = cd * hpki;
+      fwp[j] = fw;
+      if (RARE && rec >= 0)
+        con = col_open_bc(a, bc, rec, k, cc, c1, con, u_i, u_e, v_i, v_n, fu_i, fu_e, fv_i,
+                          fv_n, dxyip, hpki);
+      if (!a.vd) {
+        stb((T)con, a.nw[cc] + o, bi);
+        continue;
+      }
+      double y;
+      if (KC == 1) y = con;
+      else if (K == 1) y = con * eeb;
+      else y = (con - cclb * yprev[j]) * eeb;
+      const T yt = (T)y;
+      ys[(j * KC + k) * 64] = yt;
+      yprev[j] = (double)yt;
+    }
+    ab_prev = abk;
+    vcu_prev = vcu_k;
+    cur = nxt;
+  }
+  if (!a.vd) return;
+  // back substitution and dye decay, write CON
+  double up[NCW];
+  int dm[NCW];
+  double f[NCW];
+#pragma unroll
+  for (int j = 0; j < NCW; ++j) {
+    if (!on[j]) continue;
+    dm[j] = a.dye_mode[c0 + j];
+    f[j] = a.dye_f[c0 + j];
+    up[j] = (double)ys[(j * KC + KC - 1) * 64];
+    const double v = up[j];
+    stb((T)(dm[j] == 1 ? f[j] * v : (dm[j] == 2 ? v + f[j] : v)),
+        a.nw[c0 + j] + (size_t)(KC - 1) * pitch, bi);
+  }
+  for (int kk = KC - 2; kk >= 0; --kk) {
+    const double vk = vcu[kk * 64];
+#pragma unroll
+    for (int j = 0; j < NCW; ++j) {
+      if (!on[j]) continue;
+      const double v = (double)ys[(j * KC + kk) * 64] - vk * up[j];
+      stb((T)(dm[j] == 1 ? f[j] * v : (dm[j] == 2 ? v + f[j] : v)),
+          a.nw[c0 + j] + (size_t)kk * pitch, bi);
+      up[j] = v;
+    }
+  }
+}
+
+template <typename T, int NCW>
+__device__ __forceinline__ void tran_colm_body(const TranArgs<T> &a, double *vcus) {
+  // tiles of 64 columns start at L-1 = 0 mod 64, so every plane access of
+  // a wave covers whole 256-byte segments (the dummy cell 1 is masked)
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int i = tile * 64 + (int)threadIdx.x;
+  if (i < 1 || i > a.nwet) return;
+  const int c0 = __builtin_amdgcn_readfirstlane((int)threadIdx.y) * NCW;
+  const int src0 = a.srcrec[i];
+  const int rec = a.bcrec ? a.bcrec[i] : -1;
+  if (__ballot(src0 >= 0 || rec >= 0) != 0)
+    tran_colm_run<T, NCW, true>(a, vcus, i, c0, src0, rec);
+  else
+    tran_colm_run<T, NCW, false>(a, vcus, i, c0, src0, rec);
+}
+
+template <typename T, int NCW>
+__global__ __launch_bounds__(256) void k_tran_colm(TranArgs<T> a) {
+  extern __shared__ double vcus[];
+  tran_colm_body<T, NCW>(a, vcus);
+}
+
+// at most 168 VGPRs (three waves per SIMD)
+template <typename T, int NCW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_tran_colm_w3(
+    TranArgs<T> a) {
+  extern __shared__ double vcus[];
+  tran_colm_body<T, NCW>(a, vcus);
+}
+
 // CON1 at open-boundary columns takes the boundary value (ISUD=1,
 // EFDC/caltran.f90:356,393,430,468).
 template <typename T>
@@ -1085,18 +1328,52 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
     if (s) return s;
   }
   const bool off32 = (int64_t)c->pitch * (c->KC + 1) < ((int64_t)1 << 31);
-  if (KC <= 64 && off32) {
-    // one thread per (column, constituent); launch shape: constituent
-    // lanes per workgroup (ncg) and register budget, from the autotuner
-    // when it has run, else 4 lanes and the compiler's budget
-    const int nbc = (a.nwet + 63) / 64;
-    auto launch = [&](int ncg, int w3) {
-      const dim3 blk(64, ncg);
-      const size_t lds = (sizeof(double) + sizeof(T) * (size_t)ncg) * KC * 64;
-      if (w3) k_tran_col_w3<T><<<nbc, blk, lds, c->stream>>>(a);
-      else k_tran_col<T><<<nbc, blk, lds, c->stream>>>(a);
+  // Column kernels, 64 columns per workgroup.  With mutual neighbour maps:
+  // k_tran_colm, NCW constituents per wave (the shape parameter q = NCW),
+  // ceil(nc/NCW) waves; else k_tran_col, q waves each walking its
+  // constituents one after the other.  Every shape gives the same bits.
+  const int req = c->prm.tran_shape;
+  if (req > 0 && !c->mutual) {
+    c->err = "tran_shape > 0 needs mutual neighbour maps";
+    return EFDC_ERR_ARG;
+  }
+  const bool colm = c->mutual && req >= 0;
+  auto waves_of = [&](int q) { return colm ? (nc + q - 1) / q : q; };
+  auto lds_of = [&](int q) {
+    const size_t per_wave = colm ? (size_t)q : 1;
+    return (sizeof(double) + sizeof(T) * per_wave * waves_of(q)) * KC * 64;
+  };
+  auto shape_ok = [&](int q) {
+    return q >= 1 && waves_of(q) <= 4 && lds_of(q) <= 64 * 1024 &&
+           (colm ? (q == 1 || q < 2 * nc) : q <= nc);
+  };
+  int qs[3], nq = 0;
+  for (int q : {4, 2, 1})
+    if (shape_ok(q)) qs[nq++] = q;
+  if (KC <= 64 && off32 && nq > 0) {
+    const int nbc = colm ? (a.nwet + 64) / 64 : (a.nwet + 63) / 64;
+    auto launch = [&](int q, int w3) {
+      const dim3 blk(64, waves_of(q));
+      const size_t lds = lds_of(q);
+      if (!colm) {
+        if (w3) k_tran_col_w3<T><<<nbc, blk, lds, c->stream>>>(a);
+        else k_tran_col<T><<<nbc, blk, lds, c->stream>>>(a);
+      } else if (q == 4) {
+        if (w3) k_tran_colm_w3<T, 4><<<nbc, blk, lds, c->stream>>>(a);
+        else k_tran_colm<T, 4><<<nbc, blk, lds, c->stream>>>(a);
+      } else if (q == 2) {
+        if (w3) k_tran_colm_w3<T, 2><<<nbc, blk, lds, c->stream>>>(a);
+        else k_tran_colm<T, 2><<<nbc, blk, lds, c->stream>>>(a);
+      } else {
+        if (w3) k_tran_colm_w3<T, 1><<<nbc, blk, lds, c->stream>>>(a);
+        else k_tran_colm<T, 1><<<nbc, blk, lds, c->stream>>>(a);
+      }
     };
-    if (c->tune_pending && !any_adac) {
+    if (req != 0 && !shape_ok(req < 0 ? -req : req)) {
+      c->err = "tran_shape does not fit this grid (waves per workgroup <= 4, LDS <= 64 KiB)";
+      return EFDC_ERR_ARG;
+    }
+    if (c->tune_pending && !any_adac && req == 0) {
       // every shape computes the same bits, and a repeated launch
       // rewrites the same outputs (CON from CON1, the recovery memory of
       // outflow boundary layers), so the shapes are timed on the step
@@ -1105,14 +1382,12 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
       int best = -1;
       float best_ms = 1e30f;
       hipEvent_t e0 = timer_event(c), e1 = timer_event(c);
-      const int lanes[3] = {nc < 4 ? nc : 4, nc < 2 ? nc : 2, 1};
-      for (int li = 0; li < 3; ++li) {
-        if (li > 0 && lanes[li] == lanes[li - 1]) continue;
+      for (int qi = 0; qi < nq; ++qi) {
         for (int w3 = 0; w3 < 2; ++w3) {
           float tot = 0.f;
           for (int r = 0; r < 4; ++r) {
             HIP_TRY(c, hipEventRecord(e0, c->stream));
-            launch(lanes[li], w3);
+            launch(qs[qi], w3);
             HIP_TRY(c, hipEventRecord(e1, c->stream));
             HIP_TRY(c, hipEventSynchronize(e1));
             float ms = 0.f;
@@ -1121,14 +1396,15 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
           }
           const float avg = tot / 3.f;
           if (c->tune_n < 8) {
-            std::snprintf(c->tune_name[c->tune_n], 32, "tran_col lanes=%d%s", lanes[li],
+            std::snprintf(c->tune_name[c->tune_n], 32, "%s=%d%s",
+                          colm ? "tran_colm ncw" : "tran_col lanes", qs[qi],
                           w3 ? " vgpr<=168" : "");
             c->tune_ms[c->tune_n++] = avg;
           }
           if (avg < best_ms) {
             best_ms = avg;
             best = c->tune_n - 1;
-            c->tran_ncg = lanes[li];
+            c->tran_ncg = qs[qi];
             c->tran_w3 = w3;
           }
         }
@@ -1137,13 +1413,20 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
       c->ev_pool.push_back(e0);
       c->ev_pool.push_back(e1);
     }
-    const int ncg = c->tran_ncg > 0 ? std::min(c->tran_ncg, nc) : (nc < 4 ? nc : 4);
-    TIMED(c, TK_TRAN, launch(ncg, c->tran_ncg > 0 ? c->tran_w3 : 0));
+    // untuned default: 2 constituents per wave (column kernel with mutual
+    // maps), else as many lanes as fit
+    int q = req != 0 ? (req < 0 ? -req : req) : c->tran_ncg;
+    if (q <= 0 || !shape_ok(q)) q = (colm && shape_ok(2)) ? 2 : qs[0];
+    TIMED(c, TK_TRAN, launch(q, req == 0 && c->tran_ncg > 0 ? c->tran_w3 : 0));
     if (any_adac) {
       int s = adac_pipeline<T>(c, a, step);
       if (s) return s;
     }
     return finish_tran(c, a, step);
+  }
+  if (req != 0) {
+    c->err = "tran_shape names a column kernel, which does not fit this grid";
+    return EFDC_ERR_ARG;
   }
   // LDS per block: VCU and the FWUU carry (FP64) plus the forward values
   // (storage type); 128 columns per block when that fits in 64 KiB

@@ -46,7 +46,7 @@ class Params(C.Structure):
                 ("cg_rel_tol", C.c_double), ("cg_mode", C.c_int32),
                 ("store", C.c_int32), ("mixed_inner_rtol", C.c_double),
                 ("mixed_max_outer", C.c_int32), ("cg_kernel", C.c_int32),
-                ("mixed_precond_degree", C.c_int32)]
+                ("mixed_precond_degree", C.c_int32), ("tran_shape", C.c_int32)]
 
 
 class PbcSide(C.Structure):

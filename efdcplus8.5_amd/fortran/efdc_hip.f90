@@ -53,7 +53,7 @@ module efdc_hip
     real(c_double) :: RSQM, cg_rel_tol
     integer(c_int32_t) :: cg_mode, store
     real(c_double) :: mixed_inner_rtol
-    integer(c_int32_t) :: mixed_max_outer, cg_kernel, mixed_precond_degree
+    integer(c_int32_t) :: mixed_max_outer, cg_kernel, mixed_precond_degree, tran_shape
   end type
 
   type, bind(C) :: efdc_pbc_side_t

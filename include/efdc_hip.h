@@ -101,13 +101,20 @@ typedef struct {
   int32_t mixed_precond_degree; /* EFDC_CG_MIXED, persistent solve: terms of
                               the workgroup-local Neumann preconditioner
                               (1 = none; 0 = the library's default)          */
+  int32_t tran_shape;      /* CALCONC column kernel launch shape: 0 = the
+                              library's default or the autotuner's choice;
+                              1, 2 or 4 = that many constituents per thread
+                              (needs mutual neighbour maps); -1, -2 or -4 =
+                              one constituent per thread at a time, that many
+                              waves per workgroup.  Every shape gives the
+                              same bits.                                      */
 } efdc_params_t;
 
 /* Fill *p with the values EFDC uses by default: G=9.81, RITB1=1, SNLT=0,
    HDRY=0.2, TCON_TBEGIN=0, ITERM=200, RSQM=1e-9 (the sample decks,
    SampleModels/Lake_Washington/efdc.inp:92), cg_rel_tol=0, FP64 CG, FP64
    storage, mixed_inner_rtol=1e-5, mixed_max_outer=30, cg_kernel=AUTO,
-   mixed_precond_degree=0. */
+   mixed_precond_degree=0, tran_shape=0. */
 void efdc_params_default(efdc_params_t *p);
 
 /* ------------------------------------------------------------- array ids */

@@ -41,6 +41,20 @@ def test_abi_version_and_defaults():
     assert p.cg_mode == lib.CG_FP64 and p.store == lib.STORE_FP64
 
 
+def test_invalid_params_are_refused_before_any_device_call():
+    """efdc_ctx_create checks the parameters before it looks for a device,
+    so a bad value is EFDC_ERR_ARG on any machine."""
+    from efdc_amd import grids, lib
+    from efdc_amd.abi import STATUS
+    case = grids.tidal_basin(8, 4, 2)
+    for bad in (dict(tran_shape=3), dict(tran_shape=-3), dict(tran_shape=8),
+                dict(mixed_precond_degree=17), dict(cg_kernel=7), dict(store=5)):
+        with pytest.raises(lib.EfdcError) as e:
+            lib.Context(case.grid, **bad)
+        assert e.value.status == STATUS["EFDC_ERR_ARG"], bad
+    assert lib.default_params().tran_shape == 0
+
+
 def test_enum_values_match_layout_classes():
     from efdc_amd.abi import ARRAY_IDS, shape_class
     from efdc_amd import grids
