@@ -11,13 +11,14 @@
 // CON1 are two buffers whose roles swap every step (CON1 = CON of
 // EFDC/caltran.f90:59-65 costs nothing).
 //
-// k_tran_col (the default) runs one thread per (column, constituent): a
-// workgroup is 64 columns x up to 4 constituent lanes, one wave per lane,
-// so every wave streams one constituent plane with coalesced loads while
-// the shared transport fields come from L1/L2; each layer's loads are
-// issued before the previous layer is computed.  k_tran_upwind_vdiff (one
-// thread per column, all constituents, LDS staging) remains for more than
-// 64 layers.  HBM traffic per column-step is
+// k_tran_colm (the default with mutual neighbour maps) runs one thread per
+// column and 1, 2 or 4 constituents: the transport fields and the
+// tridiagonal factors are loaded and computed once per layer for all of
+// them.  k_tran_col (any maps) runs one thread per (column, constituent):
+// a workgroup is 64 columns x up to 4 constituent waves.  In both, each
+// layer's loads are issued before the previous layer is computed.
+// k_tran_upwind_vdiff (one thread per column, all constituents, LDS
+// staging) remains for more than 64 layers.  HBM traffic per column-step is
 //   KC*s*(3 flows + H1PK + HPKI + AB + 2*NC) + maps/metrics,
 // the roofline of this step.
 #include <algorithm>
