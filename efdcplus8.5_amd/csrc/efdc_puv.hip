@@ -43,10 +43,15 @@ int cg_blocks(int nwet) { return (nwet + CG_CPT * EFDC_BLOCK - 1) / (CG_CPT * EF
 #define MX_CPT 4
 int mx_blocks(int nwet) { return (nwet + MX_CPT * EFDC_BLOCK - 1) / (MX_CPT * EFDC_BLOCK); }
 
+// Pointwise kernels: tile t covers i = 256t .. 256t+255, so every wave's
+// accesses cover whole 256-byte segments; the dummy cell i = 0 (L=1) is
+// reported as nwet+1, i.e. not wet.  Launch puv_blocks(nwet) workgroups.
 __device__ __forceinline__ int cell_index(int nwet, int &tile) {
   tile = xcd_tile(blockIdx.x, gridDim.x);
-  return 1 + tile * EFDC_BLOCK + (int)threadIdx.x;  // wet iff <= nwet
+  const int i = tile * EFDC_BLOCK + (int)threadIdx.x;
+  return i == 0 ? nwet + 1 : i;  // wet iff <= nwet
 }
+int puv_blocks(int nwet) { return efdc_grid_blocks(nwet + 1); }
 
 // ---------------------------------------------------------------------
 // Explicit stage and time-level shifts: EFDC/calpuv2c.f90:214-220,256-290
@@ -663,22 +668,37 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_scg_b(CgArgs a, SegArgs g, const
 
 // Per-basin minimum of CC over all its cells after SETOPENBC: the
 // reference's minimum over the interior cells (:360-364) and over the
-// open-boundary cells (:365-375) taken together.  One workgroup per basin.
-__global__ __launch_bounds__(EFDC_BLOCK) void k_seg_min(const double *cc, const int *segfirst,
-                                                        double *segmin) {
-  const int sgi = blockIdx.x;
-  const int i0 = segfirst[sgi], i1 = segfirst[sgi + 1];
-  double m = 1.E+18;
-  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) m = fmin(m, cc[i]);
+// open-boundary cells (:365-375) taken together.  The minimum is exact in
+// any order, so it is taken in two launches: one workgroup per CG tile of a
+// basin (the whole chip busy), then one workgroup per basin over its tiles.
+__device__ __forceinline__ double block_min(double m) {
   m = wave_min(m);
   __shared__ double sh[EFDC_BLOCK / 64];
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double mm = sh[0];
-    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) mm = fmin(mm, sh[q]);
-    segmin[sgi] = mm;
-  }
+  double mm = sh[0];
+  for (int q = 1; q < (int)(blockDim.x >> 6); ++q) mm = fmin(mm, sh[q]);
+  return mm;
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_seg_min_tiles(const double *cc,
+                                                              const int4 *tiles,
+                                                              double *tmin) {
+  const int4 t = tiles[blockIdx.x];
+  double m = 1.E+18;
+  for (int i = t.x + (int)threadIdx.x; i < t.y; i += blockDim.x) m = fmin(m, cc[i]);
+  m = block_min(m);
+  if (threadIdx.x == 0) tmin[blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_seg_min(const double *tmin, const int *segtile0,
+                                                        double *segmin) {
+  const int sgi = blockIdx.x;
+  const int t0 = segtile0[sgi], t1 = segtile0[sgi + 1];
+  double m = 1.E+18;
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) m = fmin(m, tmin[t]);
+  m = block_min(m);
+  if (threadIdx.x == 0) segmin[sgi] = m;
 }
 
 __global__ void k_seg_reset(CgState *segst, int nseg, unsigned *segdone) {
@@ -1091,7 +1111,7 @@ int cg_fp64_vec(efdc_ctx *c, double *x, const double *rhs, CgState *st, CgState 
 }
 
 int cg_run(efdc_ctx *c, bool check_sym, efdc_cg_stats_t *stats) {
-  const int nb = efdc_grid_blocks(c->n_wet());
+  const int nb = puv_blocks(c->n_wet());
   if (check_sym) {
     k_reset_flags<<<1, 1, 0, c->stream>>>(c->st);
     k_cg_check<<<nb, EFDC_BLOCK, 0, c->stream>>>(
@@ -1102,7 +1122,7 @@ int cg_run(efdc_ctx *c, bool check_sym, efdc_cg_stats_t *stats) {
 }
 
 int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
-  const int nb = efdc_grid_blocks(c->n_wet());
+  const int nb = puv_blocks(c->n_wet());
   const int nwet = c->n_wet(), lc0 = c->LC - 1;
   const double dt = step->dt, G = c->prm.G;
   // time-level shifts of EFDC/calpuv2c.f90:256-265 as buffer rotations
@@ -1222,8 +1242,11 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
 // basins together, the host polling every few iterations).
 // ---------------------------------------------------------------------
 int seg_min_run(efdc_ctx *c) {
+  // the tile minima go to the CG partials, which the solve rewrites later
+  TIMED(c, TK_PUV_SCALE, k_seg_min_tiles<<<c->ntiles, EFDC_BLOCK, 0, c->stream>>>(
+      c->dH[EFDC_CC], c->d_tiles, c->d_segpart));
   TIMED(c, TK_PUV_SCALE, k_seg_min<<<c->nseg, EFDC_BLOCK, 0, c->stream>>>(
-      c->dH[EFDC_CC], c->d_segfirst, c->d_segmin));
+      c->d_segpart, c->d_segtile0, c->d_segmin));
   HIP_TRY(c, hipGetLastError());
   return EFDC_OK;
 }

@@ -784,8 +784,10 @@ __global__ void k_closure_pre(double *tbx, double *tby, const double *uhe,
                               const double *vhe, const double *hu,
                               const double *hv, const double *nman, double G,
                               int nwet) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > nwet) return;
+  // workgroups start at L-1 = 0 mod 256 (whole 256-byte segments); the
+  // dummy cell 1 is masked
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 1 || i > nwet) return;
   const double n2 = nman[i] * nman[i];
   const double u = uhe[i] / hu[i], v = vhe[i] / hv[i];
   tbx[i] = G * n2 * u * fabs(u) / cbrt(hu[i]);
@@ -801,8 +803,8 @@ __global__ void k_closure_post(const double *__restrict__ uhdye, const double *_
                                T *__restrict__ uhdy2, T *__restrict__ vhdx2, T *__restrict__ u2,
                                T *__restrict__ v2, T *__restrict__ hpk, T *__restrict__ hpki,
                                int64_t pitch, int KC, int nwet) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > nwet) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // as k_closure_pre
+  if (i < 1 || i > nwet) return;
   const double ub = 0.5 * (uhdye[i] + uhdy1e[i]);
   const double vb = 0.5 * (vhdxe[i] + vhdx1e[i]);
   // the same for every layer of the column
@@ -1490,7 +1492,7 @@ int balance_run(efdc_ctx *c, efdc_balance_t *out) {
 
 int closure_pre_run(efdc_ctx *c, const double *nman) {
   const int nwet = c->n_wet();
-  TIMED(c, TK_CLOSURE_PRE, k_closure_pre<<<(nwet + 255) / 256, 256, 0, c->stream>>>(
+  TIMED(c, TK_CLOSURE_PRE, k_closure_pre<<<(nwet + 256) / 256, 256, 0, c->stream>>>(
       c->dH[EFDC_TBX], c->dH[EFDC_TBY], c->dH[EFDC_UHE], c->dH[EFDC_VHE],
       c->dH[EFDC_HU], c->dH[EFDC_HV], nman, c->prm.G, nwet));
   HIP_TRY(c, hipGetLastError());
@@ -1499,7 +1501,7 @@ int closure_pre_run(efdc_ctx *c, const double *nman) {
 
 int closure_post_run(efdc_ctx *c) {
   const int nwet = c->n_wet();
-  const int nb = (nwet + 255) / 256;
+  const int nb = (nwet + 256) / 256;
   if (c->store == EFDC_STORE_FP64)
     TIMED(c, TK_CLOSURE_POST, k_closure_post<double><<<nb, 256, 0, c->stream>>>(
         c->dH[EFDC_UHDYE], c->dH[EFDC_UHDY1E], c->dH[EFDC_VHDXE],
