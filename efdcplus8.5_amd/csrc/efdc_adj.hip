@@ -1032,6 +1032,10 @@ int efdc_adj_enable(efdc_ctx_t *c, const efdc_adj_opts_t *o) {
       c->err = "adjoint: anti-diffusion (ISADAC) is not differentiated";
       return EFDC_ERR_ARG;
     }
+    if (c->con[sl].desc.wset > 0.0) {
+      c->err = "adjoint: settling is not differentiated";
+      return EFDC_ERR_ARG;
+    }
   }
   if (c->KC > 64) {
     c->err = "adjoint: KC > 64";

@@ -107,7 +107,7 @@ const char *const efdc_timer_names[TK_COUNT] = {
     "puv_depths", "puv_hu_hv", "tran_upwind_vdiff", "tran_bc_con1",
     "closure_pre", "closure_post", "balance", "adac_gradients",
     "adac_fluxes", "fct_limiters", "tran_finish_vdiff", "cg_persistent64",
-    "mx_persistent", "adj_tran_col", "adj_tran_gather", "adj_hydro", "adj_cg"};
+    "mx_persistent", "adj_tran_col", "adj_tran_gather", "adj_hydro", "adj_cg", "settle"};
 
 hipEvent_t timer_event(efdc_ctx *c) {
   if (c->ev_pool.empty()) {
@@ -556,6 +556,11 @@ int efdc_add_constituent(efdc_ctx_t *c, const efdc_constituent_t *d,
   if (!c || !d || !slot) return EFDC_ERR_ARG;
   if (c->nconst >= EFDC_MAXC) {
     c->err = "too many constituents";
+    return EFDC_ERR_ARG;
+  }
+  if (!(d->wset >= 0.0 && d->wset < 1e3) || !(d->probdep >= 0.0 && d->probdep <= 1.0) ||
+      !(d->wese >= 0.0 && d->wese < 1e30)) {
+    c->err = "settling: need 0 <= wset, 0 <= probdep <= 1, 0 <= wese (all finite)";
     return EFDC_ERR_ARG;
   }
   if (d->isadac > 0 && !c->mutual) {

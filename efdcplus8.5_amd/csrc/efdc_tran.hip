@@ -49,6 +49,7 @@ struct TranArgs {
   int mo[EFDC_MAXC];
   int dye_mode[EFDC_MAXC];   // 0 none, 1 decay factor, 2 age
   double dye_f[EFDC_MAXC];   // factor or age increment
+
   // sources
   const int *srcrec, *srcnext;
   const double *qss, *qsercell, *cqs, *csertq;
@@ -740,6 +741,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   tran_colm_body<T, NCW>(a, vcus);
 }
 
+// Cohesive settling after the transport (SSEDTOX -> CALSED with DTSED = DT,
+// EFDC/calconc.f90:621-630, WSETA = WSEDO): the implicit upwind sweep from
+// the top layer down, EFDC/calsed.f90:232-256, then the bottom layer with
+// the given deposition probability and resuspension flux,
+// EFDC/calsed.f90:308-312.  One thread per column; each layer's new value
+// is stored in the storage type and the flux to the layer below is taken
+// from the stored value, as the reference reads SED back.
+template <typename T>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_settle(T *__restrict__ sed,
+                                                       const T *__restrict__ hpki,
+                                                       int64_t pitch, int KC, int nwet,
+                                                       double dtsed, double ws, double wsetmp,
+                                                       double wese) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 256-aligned tiles
+  if (i < 1 || i > nwet) return;
+  double sedf = 0.0;  // SEDF(L,KC) = 0
+  if (KC > 1) {
+    {
+      const int64_t j = i + (int64_t)(KC - 1) * pitch;
+      const double WVEL = dtsed * (double)hpki[j];
+      const double CLEFT = 1. + ws * WVEL;
+      const double CRIGHT = dmx((double)sed[j], 0.);
+      const T v = (T)(CRIGHT / CLEFT);
+      sed[j] = v;
+      sedf = -ws * (double)v;
+    }
+    for (int K = KC - 1; K >= 2; --K) {
+      const int64_t j = i + (int64_t)(K - 1) * pitch;
+      const double WVEL = dtsed * (double)hpki[j];
+      const double CLEFT = 1. + ws * WVEL;
+      const double CRIGHT = dmx((double)sed[j], 0.) - sedf * WVEL;
+      const T v = (T)(CRIGHT / CLEFT);
+      sed[j] = v;
+      sedf = -ws * (double)v;
+    }
+  }
+  const double WVEL = dtsed * (double)hpki[i];
+  const double CLEFT = 1. + wsetmp * WVEL;
+  const double CRIGHT = dmx((double)sed[i], 0.) + (wese - sedf) * WVEL;
+  sed[i] = (T)(CRIGHT / CLEFT);
+}
+
 // CON1 at open-boundary columns takes the boundary value (ISUD=1,
 // EFDC/caltran.f90:356,393,430,468).
 template <typename T>
@@ -1242,7 +1285,14 @@ int adac_pipeline(efdc_ctx *c, TranArgs<T> &ta, const efdc_step_t *step) {
 
 // CON1 of the open-boundary columns, then the CON/CON1 buffer swap.
 template <typename T>
-int finish_tran(efdc_ctx *c, TranArgs<T> &a, const efdc_step_t *) {
+int finish_tran(efdc_ctx *c, TranArgs<T> &a, const efdc_step_t *step) {
+  for (int j = 0; j < c->nconst; ++j) {
+    const efdc_constituent_t &d = c->con[j].desc;
+    if (!(d.wset > 0.0)) continue;
+    TIMED(c, TK_SETTLE, k_settle<T><<<(a.nwet + 256) / 256, 256, 0, c->stream>>>(
+        a.nw[j], a.hpki, a.pitch, a.KC, a.nwet, step->dt, d.wset, d.probdep * d.wset,
+        d.wese));
+  }
   if (c->nbcrec > 0) {
     const int tot = c->nbcrec * c->KC;
     TIMED(c, TK_TRAN_BC, k_tran_bc_con1<T><<<(tot + 255) / 256, 256, 0, c->stream>>>(a, c->bccell,

@@ -301,8 +301,8 @@ def config1(seed=1001, n=1024, wet_fraction=0.67) -> Case:
 
 
 def _constituent(g: Grid, mo: int, con: np.ndarray, **kw) -> dict:
-    d = dict(mo=mo, isadac=0, isfct=0, rkdye=0.0, is_dye=0,
-             con=con, con1=con.copy())
+    d = dict(mo=mo, isadac=0, isfct=0, rkdye=0.0, is_dye=0, wset=0.0, probdep=0.0,
+             wese=0.0, con=con, con1=con.copy())
     d.update(kw)
     return d
 
@@ -484,11 +484,15 @@ def config0(**kw) -> Case:
 
 def transport_case(nx=2048, ny=1024, kc=20, *, seed=1002, dx=100.0, dt=5.0,
                    depth=10.0, umax=1.0, isadac=0, isfct=0,
-                   name=None, with_w=True) -> Case:
+                   name=None, with_w=True, sed_ws=1e-4, sed_probdep=1.0,
+                   sed_wese=0.0) -> Case:
     """Config-2 style transport case: all cells wet, uniform depth, a
     divergence-free layer flow from a streamfunction plus a vertically
     alternating potential flow whose column sum vanishes (non-zero W2 from
-    continuity, EFDC/caluvw.f90:668-680), log-uniform AB, 4 constituents."""
+    continuity, EFDC/caluvw.f90:668-680), log-uniform AB, 4 constituents:
+    salinity, temperature, dye (one source) and a cohesive sediment class
+    of 0.1 kg/m^3 settling at sed_ws m/s (deposition probability
+    sed_probdep, resuspension flux sed_wese; sed_ws=0: no settling)."""
     from scipy.ndimage import gaussian_filter
     rng = np.random.default_rng(seed)
     wet = _pad_ring(np.ones((ny, nx), dtype=bool))
@@ -558,7 +562,8 @@ def transport_case(nx=2048, ny=1024, kc=20, *, seed=1002, dx=100.0, dt=5.0,
     consts = [_constituent(g, 1, sal, isadac=isadac, isfct=isfct),
               _constituent(g, 2, tem, isadac=isadac, isfct=isfct),
               _constituent(g, 3, dye, is_dye=1, isadac=isadac, isfct=isfct),
-              _constituent(g, 4, sed, isadac=isadac, isfct=isfct)]
+              _constituent(g, 4, sed, isadac=isadac, isfct=isfct, wset=sed_ws,
+                           probdep=sed_probdep, wese=sed_wese)]
     nc = len(consts)
     lsrc = int(g.LIJ[ny // 2 + 1, nx // 2 + 1])
     qss = np.zeros((1, kc))

@@ -74,7 +74,8 @@ class Sources(C.Structure):
 
 class ConstDesc(C.Structure):
     _fields_ = [("mo", C.c_int32), ("isadac", C.c_int32), ("isfct", C.c_int32),
-                ("rkdye", C.c_double), ("is_dye", C.c_int32)]
+                ("rkdye", C.c_double), ("is_dye", C.c_int32), ("wset", C.c_double),
+                ("probdep", C.c_double), ("wese", C.c_double)]
 
 
 class Step(C.Structure):
@@ -327,8 +328,10 @@ class Context:
         o.cqs, o.csert = _pd(arrs[3]), _pd(arrs[4])
         self._check(self.lib.efdc_set_sources(self.h, C.byref(o)))
 
-    def add_constituent(self, mo, isadac=0, isfct=0, rkdye=0.0, is_dye=0) -> int:
-        d = ConstDesc(mo=mo, isadac=isadac, isfct=isfct, rkdye=rkdye, is_dye=is_dye)
+    def add_constituent(self, mo, isadac=0, isfct=0, rkdye=0.0, is_dye=0, wset=0.0,
+                        probdep=0.0, wese=0.0) -> int:
+        d = ConstDesc(mo=mo, isadac=isadac, isfct=isfct, rkdye=rkdye, is_dye=is_dye,
+                      wset=wset, probdep=probdep, wese=wese)
         slot = C.c_int32(-1)
         self._check(self.lib.efdc_add_constituent(self.h, C.byref(d), C.byref(slot)))
         self.nconst += 1
@@ -504,7 +507,8 @@ class Context:
             self.set_open_bc(case.open_bc)
         for c in case.consts:
             slot = self.add_constituent(c["mo"], c["isadac"], c["isfct"],
-                                        c["rkdye"], c["is_dye"])
+                                        c["rkdye"], c["is_dye"], c.get("wset", 0.0),
+                                        c.get("probdep", 0.0), c.get("wese", 0.0))
             self.set_constituent(slot, c["con"], c["con1"])
         if case.conc_bc is not None and case.consts:
             self.set_conc_bc(case.conc_bc)

@@ -86,6 +86,7 @@ module efdc_hip
     integer(c_int32_t) :: mo, isadac, isfct
     real(c_double) :: rkdye
     integer(c_int32_t) :: is_dye
+    real(c_double) :: wset, probdep, wese
   end type
 
   type, bind(C) :: efdc_step_t

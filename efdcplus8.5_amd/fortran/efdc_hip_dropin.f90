@@ -94,6 +94,9 @@ contains
     integer, intent(in) :: MVAR, MO, NS_
     HXD%mo = MO; HXD%isadac = ISADAC(MVAR); HXD%isfct = ISFCT(MVAR)
     HXD%rkdye = 0.0; HXD%is_dye = 0
+    ! settling of the sediment classes stays with the host's SSEDTOX, which
+    ! also owns the bed exchange
+    HXD%wset = 0.0; HXD%probdep = 0.0; HXD%wese = 0.0
     if (MVAR == 3) then
       HXD%rkdye = RKDYE; HXD%is_dye = 1
     end if
@@ -278,6 +281,7 @@ SUBROUTINE CALCONC(ISTL_, IS2TL_)
   integer :: ISTL_, IS2TL_
   type(efdc_step_t) :: HXSTEP
   integer :: J
+  real, save :: SEDTIME = 0.0
 
   if (ISTL_ /= 2 .or. IS2TL_ /= 1) stop 'EFDC_HIP: CALCONC needs ISTL=2, IS2TL=1'
   if (.not. c_associated(HCTX)) call EFDC_HIP_INIT
@@ -314,6 +318,17 @@ SUBROUTINE CALCONC(ISTL_, IS2TL_)
                                               SED1(:,:,SLOT_NS(J))), 'SED')
     end select
   end do
+  ! sediment and toxics settling, deposition and resuspension stay on the
+  ! host with the bed model, on the reference's clock (2TL branch of
+  ! EFDC/calconc.f90:621-630)
+  if ((ISTRAN(6) >= 1 .or. ISTRAN(7) >= 1) .and. TIMEDAY >= SEDSTART) then
+    SEDTIME = SEDTIME + DELT
+    if (SEDTIME >= SEDSTEP) then
+      DTSED = SEDTIME
+      call SSEDTOX
+      SEDTIME = 0.0
+    end if
+  end if
 contains
   subroutine PUSH(ID, A)
     integer(c_int32_t), intent(in) :: ID

@@ -214,6 +214,16 @@ typedef struct {
   double rkdye;    /* dye decay RKDYE (0 none, 1000 age of water), only
                       applied when is_dye != 0 (EFDC/calconc.f90:581-616)    */
   int32_t is_dye;
+  /* Cohesive settling (CALSED water column, EFDC/calsed.f90:232-256 and the
+     bottom-layer update :308-316), applied after the transport of every
+     CALCONC with DTSED = DT (SEDSTEP <= DT) and a constant settling speed
+     (ISEDVW = 0).  wset = WSEDO(NS) in m/s (0: no settling).  The bed model
+     that sets the deposition probability and the resuspension flux from the
+     bed shear is not part of this library: probdep (0..1) and wese (mass
+     per m^2 per s, >= 0) are given, and the bed never runs out.            */
+  double wset;
+  double probdep;
+  double wese;
 } efdc_constituent_t;
 
 /* ------------------------------------------------------------------ step */

@@ -940,6 +940,48 @@ int orc_vertical_diffusion(orc_ctx *o, double dt) {
   return EFDC_OK;
 }
 
+/* Cohesive settling of the water column after the transport, SSEDTOX ->
+   CALSED with DTSED = DT (EFDC/calconc.f90:621-630 when SEDSTEP <= DT) and
+   WSETA = WSEDO (ISEDVW = 0, EFDC/calsed.f90:39-49): the implicit upwind
+   sweep from the top layer down, EFDC/calsed.f90:232-256, then the bottom
+   layer with the deposition probability PROBDEP and the resuspension flux
+   WESE given (the bed model that sets them is not restated),
+   EFDC/calsed.f90:308-312. */
+void orc_settle(orc_ctx *o, double dtsed) {
+  const int KC = o->KC;
+  const double *HPKI = A(EFDC_HPKI);
+  for (int c = 0; c < o->nconst; ++c) {
+    const double ws = o->c[c].wset;
+    if (!(ws > 0.0)) continue;
+    double *SED = o->c[c].con;
+    const double WSETMP = o->c[c].probdep * ws, WESE = o->c[c].wese;
+    for (int L = 2; L <= o->LA; ++L) {
+      double sedf = 0.0; /* SEDF(L,KC) = 0 */
+      if (KC > 1) {
+        {
+          const int K = KC;
+          const double WVEL = dtsed * HPKI[KX(L, K)];
+          const double CLEFT = 1. + ws * WVEL;
+          const double CRIGHT = dmax(SED[KX(L, K)], 0.);
+          SED[KX(L, K)] = CRIGHT / CLEFT;
+          sedf = -ws * SED[KX(L, K)];
+        }
+        for (int K = KC - 1; K >= 2; --K) {
+          const double WVEL = dtsed * HPKI[KX(L, K)];
+          const double CLEFT = 1. + ws * WVEL;
+          const double CRIGHT = dmax(SED[KX(L, K)], 0.) - sedf * WVEL;
+          SED[KX(L, K)] = CRIGHT / CLEFT;
+          sedf = -ws * SED[KX(L, K)];
+        }
+      }
+      const double WVEL = dtsed * HPKI[KX(L, 1)];
+      const double CLEFT = 1. + WSETMP * WVEL;
+      const double CRIGHT = dmax(SED[KX(L, 1)], 0.) + (WESE - sedf) * WVEL;
+      SED[KX(L, 1)] = CRIGHT / CLEFT;
+    }
+  }
+}
+
 int orc_calconc(orc_ctx *o, double dt, int64_t niter) {
   const int nc = o->nconst;
   const int nt = o->nthreads > 0 ? o->nthreads : 1;
@@ -967,6 +1009,7 @@ int orc_calconc(orc_ctx *o, double dt, int64_t niter) {
         for (int L = 2; L <= o->LA; ++L) C[KX(L, K)] = f * C[KX(L, K)];
     }
   }
+  orc_settle(o, dt);
   return EFDC_OK;
 }
 

@@ -35,6 +35,7 @@ typedef struct {
   int32_t mo, isadac, isfct, is_dye;
   double rkdye;
   double *con, *con1;    /* (LCM,KCM) */
+  double wset, probdep, wese;  /* cohesive settling (0: none) */
 } orc_const;
 
 typedef struct {
@@ -78,6 +79,9 @@ int orc_calpuv2c(orc_ctx *o, double dt, int64_t n, orc_cg_out *out);
 int orc_assemble(orc_ctx *o, double dt, int64_t n, double *ccmnm);
 /* CALCONC(2,1): EFDC/calconc.f90:65-473,581-616 with CALTRAN/CALFQC */
 int orc_calconc(orc_ctx *o, double dt, int64_t niter);
+/* cohesive settling of the water column (part of orc_calconc),
+   EFDC/calsed.f90:232-256,308-312 */
+void orc_settle(orc_ctx *o, double dtsed);
 /* only the vertical diffusion pass, EFDC/calconc.f90:239-473 */
 int orc_vertical_diffusion(orc_ctx *o, double dt);
 /* balance totals: EFDC/bal2t.f90:90,103-104 */

@@ -40,7 +40,8 @@ class CbcSide(C.Structure):
 class Const(C.Structure):
     _fields_ = [("mo", C.c_int32), ("isadac", C.c_int32), ("isfct", C.c_int32),
                 ("is_dye", C.c_int32), ("rkdye", C.c_double), ("con", PD),
-                ("con1", PD)]
+                ("con1", PD), ("wset", C.c_double), ("probdep", C.c_double),
+                ("wese", C.c_double)]
 
 
 class OrcCtx(C.Structure):
@@ -93,6 +94,7 @@ def lib():
                 orc_assemble=[C.POINTER(OrcCtx), C.c_double, C.c_int64, PD],
                 orc_calconc=[C.POINTER(OrcCtx), C.c_double, C.c_int64],
                 orc_vertical_diffusion=[C.POINTER(OrcCtx), C.c_double],
+                orc_settle=[C.POINTER(OrcCtx), C.c_double],
                 orc_balance=[C.POINTER(OrcCtx), PD, PD],
                 orc_closure_pre=[C.POINTER(OrcCtx), PD],
                 orc_closure_post=[C.POINTER(OrcCtx)],
@@ -101,7 +103,7 @@ def lib():
                                   C.c_int64]).items():
             f = getattr(_lib, fn)
             f.argtypes = args
-            f.restype = C.c_int
+            f.restype = None if fn == "orc_settle" else C.c_int
     return _lib
 
 
@@ -199,6 +201,9 @@ class Oracle:
             o.c[i].mo, o.c[i].isadac, o.c[i].isfct = c["mo"], c["isadac"], c["isfct"]
             o.c[i].is_dye, o.c[i].rkdye = c["is_dye"], c["rkdye"]
             o.c[i].con, o.c[i].con1 = _pd(c["con"]), _pd(c["con1"])
+            o.c[i].wset = c.get("wset", 0.0)
+            o.c[i].probdep = c.get("probdep", 0.0)
+            o.c[i].wese = c.get("wese", 0.0)
         self.o = o
 
     # ------------------------------------------------------------ calls
@@ -223,6 +228,10 @@ class Oracle:
 
     def vertical_diffusion(self, dt=None):
         return lib().orc_vertical_diffusion(C.byref(self.o), dt or self.case.dt)
+
+    def settle(self, dt=None):
+        """Only the settling sweep of CALCONC (EFDC/calsed.f90:232-256,308-312)."""
+        return lib().orc_settle(C.byref(self.o), dt or self.case.dt)
 
     def balance(self):
         v = C.c_double(0.0)

@@ -291,6 +291,36 @@ def test_dye_decay_modes_match_oracle(oracle_mod, rk):
     assert err <= 1e-12
 
 
+@pytest.mark.parametrize("store", ["fp64", "fp32"])
+def test_settling_matches_oracle(oracle_mod, store):
+    """CALCONC with a settling sediment class (EFDC/calsed.f90:232-256,
+    308-312; deposition probability and resuspension flux given): strong
+    settling (dt = 600 s) on top of the transport, 3 steps.  FP64 within
+    1e-12 of the oracle, FP32 storage within 1e-5."""
+    from efdc_amd import grids
+    from efdc_amd.lib import STORE_FP32, STORE_FP64
+    case = grids.transport_case(40, 24, 6, seed=33, sed_ws=2e-3, sed_probdep=0.7,
+                                sed_wese=1e-6, dt=600.0)
+    # flows scaled to the long step (CFL < 1)
+    for name in ("UHDY2", "VHDX2"):
+        case.k[name][:] *= 5.0 / 600.0
+    case.w["W2"][:] *= 5.0 / 600.0
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref)
+    ctx = _ctx(case, store=STORE_FP64 if store == "fp64" else STORE_FP32)
+    kc = case.grid.KC
+    for n in range(1, 4):
+        o.calconc(niter=n)
+        ctx.calconc(case.dt, n)
+    sed, _ = ctx.get_constituent(3)
+    want = ref.consts[3]["con"][:kc]
+    rel = np.max(np.abs(sed[:kc] - want)) / np.max(np.abs(want))
+    print(store, "settling rel", rel, "bottom/top", want[0].mean(), want[kc - 1].mean())
+    assert want[0].mean() > 1.5 * want[kc - 1].mean()   # the sediment did settle
+    assert rel <= (1e-12 if store == "fp64" else 1e-5)
+    ctx.close()
+
+
 def test_negative_depth_status_on_gpu(oracle_mod):
     from efdc_amd.lib import EfdcError
     case = _lake(24, seed=1)

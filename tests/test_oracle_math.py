@@ -184,7 +184,7 @@ def test_point_source_adds_its_volume(oracle_mod):
 
 def no_source_transport(seed=2, nx=24, ny=16, kc=5):
     from efdc_amd import grids
-    case = grids.transport_case(nx, ny, kc, seed=seed)
+    case = grids.transport_case(nx, ny, kc, seed=seed, sed_ws=0.0)
     case.sources = None
     case.k["QSUM"][:] = 0.0
     return case
@@ -333,6 +333,75 @@ def test_dye_decay_closed_forms(oracle_mod, rk):
 
 
 # ---------------------------------------------------------- failure paths
+
+# ------------------------------------------------------------- settling
+
+def _settling_case(kc=6, ws=2e-3, probdep=0.0, wese=0.0, seed=8):
+    """A transport case whose sediment class settles; random non-negative
+    and a few negative concentrations (the sweep clips them to zero)."""
+    from efdc_amd import grids
+    case = grids.transport_case(12, 8, kc, seed=seed, sed_ws=ws, sed_probdep=probdep,
+                                sed_wese=wese, dt=600.0)
+    rng = np.random.default_rng(seed)
+    L = wet_index(case)
+    sed = case.consts[3]
+    sed["con"][:kc, L] = rng.uniform(-0.02, 0.5, (kc, len(L)))
+    # layer thicknesses vary by column and layer
+    case.k["HPKI"][:kc, L] = 1.0 / rng.uniform(0.5, 3.0, (kc, len(L)))
+    return case, L
+
+
+def test_settling_conserves_mass_with_closed_bed(oracle_mod):
+    """No deposition, no resuspension: the implicit upwind fluxes only move
+    mass downwards, so sum_K SED*H_K equals sum_K max(SED,0)*H_K."""
+    case, L = _settling_case()
+    kc = case.grid.KC
+    H = 1.0 / case.k["HPKI"][:kc, L]
+    before = np.sum(np.maximum(case.consts[3]["con"][:kc, L], 0.0) * H, axis=0)
+    o = oracle_mod.Oracle(case)
+    o.settle()
+    after = case.consts[3]["con"][:kc, L]
+    assert np.all(after >= 0.0)
+    np.testing.assert_allclose(np.sum(after * H, axis=0), before, rtol=1e-13, atol=0)
+
+
+def test_settling_equals_bidiagonal_solve(oracle_mod):
+    """The sweep is the solution of the implicit upwind system
+    (H_K/dt) (c_K - c0_K) = w c_{K+1} - w_K c_K per column (w_K = ws above
+    the bed, probdep*ws at the bed, plus the resuspension flux into the
+    bottom layer), solved here as a dense linear system."""
+    ws, pd, we = 2e-3, 0.6, 1e-5
+    case, L = _settling_case(ws=ws, probdep=pd, wese=we)
+    kc, dt = case.grid.KC, case.dt
+    H = 1.0 / case.k["HPKI"][:kc, L]
+    c0 = np.maximum(case.consts[3]["con"][:kc, L], 0.0)
+    o = oracle_mod.Oracle(case)
+    o.settle()
+    got = case.consts[3]["con"][:kc, L]
+    for j in range(0, len(L), 7):
+        A = np.zeros((kc, kc))
+        b = c0[:, j] * H[:, j] / dt
+        for K in range(kc):
+            A[K, K] = H[K, j] / dt + (ws if K > 0 else pd * ws)
+            if K + 1 < kc:
+                A[K, K + 1] = -ws
+        b[0] += we
+        want = np.linalg.solve(A, b)
+        np.testing.assert_allclose(got[:, j], want, rtol=1e-12, atol=1e-15)
+
+
+def test_settling_without_deposition_collects_mass_at_the_bed(oracle_mod):
+    case, L = _settling_case(ws=5e-3)
+    kc = case.grid.KC
+    H = 1.0 / case.k["HPKI"][:kc, L]
+    total = np.sum(np.maximum(case.consts[3]["con"][:kc, L], 0.0) * H, axis=0)
+    o = oracle_mod.Oracle(case)
+    for _ in range(400):
+        o.settle()
+    c = case.consts[3]["con"][:kc, L]
+    assert np.max(c[1:]) <= 1e-6 * np.max(c[0])
+    np.testing.assert_allclose(c[0] * H[0], total, rtol=1e-9)
+
 
 def test_cg_maximum_iterations_status(oracle_mod):
     case = lake(24, seed=1)
