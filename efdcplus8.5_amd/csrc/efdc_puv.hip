@@ -41,7 +41,9 @@ int cg_blocks(int nwet) { return (nwet + CG_CPT * EFDC_BLOCK - 1) / (CG_CPT * EF
 // The mixed solve's kernels move 24-32 bytes per cell: 4 cells per thread
 // keep more loads in flight and quarter the blocks (and reduction arrivals).
 #define MX_CPT 4
-int mx_blocks(int nwet) { return (nwet + MX_CPT * EFDC_BLOCK - 1) / (MX_CPT * EFDC_BLOCK); }
+// Their tiles start at i = 0 mod 256 (whole 256-byte segments per wave);
+// the dummy cell i = 0 is skipped.
+int mx_blocks(int nwet) { return (nwet + MX_CPT * EFDC_BLOCK) / (MX_CPT * EFDC_BLOCK); }
 
 // Pointwise kernels: tile t covers i = 256t .. 256t+255, so every wave's
 // accesses cover whole 256-byte segments; the dummy cell i = 0 (L=1) is
@@ -726,8 +728,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_prep(MxArgs m, double rsqm,
   double v[1] = {0.0};
   #pragma unroll
   for (int u = 0; u < MX_CPT; ++u) {
-    const int i = 1 + (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
-    if (i <= a.nwet) {
+    const int i = (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+    if (i >= 1 && i <= a.nwet) {
       const double di = 1.0 / sqrt(a.ccc[i]);
       m.dis[i] = di;
       const int js = a.ls[i];
@@ -758,8 +760,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_resid(MxArgs m) {
   double v[2] = {0.0, 0.0};
   #pragma unroll
   for (int u = 0; u < MX_CPT; ++u) {
-    const int i = 1 + (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
-    if (i <= a.nwet) {
+    const int i = (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+    if (i >= 1 && i <= a.nwet) {
       const Row q = row_of(a, i, true);
       const double *P = a.x;
       const double r = a.fptmp[i] - a.ccc[i] * P[i] - q.cn * P[q.jn] -
@@ -801,8 +803,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_a(MxArgs m, const float *pold
   double v[1] = {0.0};
   #pragma unroll
   for (int u = 0; u < MX_CPT; ++u) {
-    const int i = 1 + (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
-    if (i <= a.nwet) {
+    const int i = (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+    if (i >= 1 && i <= a.nwet) {
       const int js = a.ls[i], jn = a.ln[i];
       const float pc = p32_at(m, pold, beta, i);
       const float ps = p32_at(m, pold, beta, js);
@@ -832,8 +834,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_b(MxArgs m, const float *pnew
   double v[1] = {0.0};
   #pragma unroll
   for (int u = 0; u < MX_CPT; ++u) {
-    const int i = 1 + (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
-    if (i <= a.nwet) {
+    const int i = (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+    if (i >= 1 && i <= a.nwet) {
       m.y32[i] = (float)((double)m.y32[i] + alpha * (double)pnew[i]);
       const float r = (float)((double)m.r32[i] - alpha * (double)m.ap32[i]);
       m.r32[i] = r;
@@ -857,8 +859,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_apply(MxArgs m) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
 #pragma unroll
   for (int u = 0; u < MX_CPT; ++u) {
-    const int i = 1 + (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
-    if (i <= a.nwet) a.x[i] = a.x[i] + m.dis[i] * (double)m.y32[i];
+    const int i = (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
+    if (i >= 1 && i <= a.nwet) a.x[i] = a.x[i] + m.dis[i] * (double)m.y32[i];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) a.st->outer += 1;
 }
