@@ -915,6 +915,38 @@ std::vector<Piece> state_pieces(efdc_ctx *c) {
   return v;
 }
 
+// Copy the pieces to (save) or from (load) consecutive 256-byte aligned
+// slots of buf on the context's stream.
+int copy_pieces(efdc_ctx *c, const std::vector<Piece> &pcs, char *buf, bool save) {
+  for (const Piece &p : pcs) {
+    if (save)
+      HIP_TRY(c, hipMemcpyAsync(buf, p.p, p.bytes, hipMemcpyDeviceToDevice, c->stream));
+    else
+      HIP_TRY(c, hipMemcpyAsync(p.p, buf, p.bytes, hipMemcpyDeviceToDevice, c->stream));
+    buf += (p.bytes + 255) & ~(size_t)255;
+  }
+  return EFDC_OK;
+}
+
+// The recording of one forward step: the CG solution, the forward CgState
+// (its CCMNM scales the adjoint right-hand side), the step's source
+// concentrations (efdc_set_source_cqs may change them every step; the
+// source term enters CD, whose derivative with respect to HPKI the reverse
+// needs) and the old levels of the adjoint constituents, in 256-byte
+// aligned pieces.
+std::vector<Piece> record_pieces(efdc_ctx *c) {
+  std::vector<Piece> v;
+  AdjState *A = c->adj;
+  v.push_back({A->ps, sizeof(double) * c->LCM});
+  v.push_back({c->st, sizeof(CgState)});
+  if (c->nqsij > 0 && c->cqs)
+    v.push_back({c->cqs, sizeof(double) * (size_t)c->nqsij * c->KC * c->src_nc});
+  const size_t kel = (size_t)c->pitch * c->KC;
+  for (int j = 0; j < A->nadj; ++j) v.push_back({A->c0[A->slot[j]], kel * c->tsz});
+  return v;
+}
+
+
 }  // namespace
 
 bool adj_recording(const efdc_ctx *c) { return c->adj && c->adj->recording; }
@@ -976,22 +1008,12 @@ int efdc_state_bytes(efdc_ctx_t *c, size_t *bytes) {
 
 int efdc_state_save(efdc_ctx_t *c, void *dev_buf) {
   if (!c || !dev_buf) return EFDC_ERR_ARG;
-  char *d = (char *)dev_buf;
-  for (const Piece &p : state_pieces(c)) {
-    HIP_TRY(c, hipMemcpyAsync(d, p.p, p.bytes, hipMemcpyDeviceToDevice, c->stream));
-    d += (p.bytes + 255) & ~(size_t)255;
-  }
-  return EFDC_OK;
+  return copy_pieces(c, state_pieces(c), (char *)dev_buf, true);
 }
 
 int efdc_state_load(efdc_ctx_t *c, const void *dev_buf) {
   if (!c || !dev_buf) return EFDC_ERR_ARG;
-  const char *d = (const char *)dev_buf;
-  for (const Piece &p : state_pieces(c)) {
-    HIP_TRY(c, hipMemcpyAsync(p.p, d, p.bytes, hipMemcpyDeviceToDevice, c->stream));
-    d += (p.bytes + 255) & ~(size_t)255;
-  }
-  return EFDC_OK;
+  return copy_pieces(c, state_pieces(c), (char *)const_cast<void *>(dev_buf), false);
 }
 
 int efdc_state_alloc(efdc_ctx_t *c, void **dev_buf) {
@@ -999,6 +1021,12 @@ int efdc_state_alloc(efdc_ctx_t *c, void **dev_buf) {
   size_t n = 0;
   int s;
   if ((s = efdc_state_bytes(c, &n))) return s;
+  if (c->adj) {
+    // room for a step's adjoint recording after the state
+    size_t r = 0;
+    for (const Piece &p : record_pieces(c)) r += (p.bytes + 255) & ~(size_t)255;
+    n += r;
+  }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (hipMalloc(dev_buf, n) != hipSuccess) {
     *dev_buf = nullptr;
@@ -1204,6 +1232,36 @@ int efdc_adj_step(efdc_ctx_t *c, const efdc_step_t *step, efdc_adj_stats_t *stat
     stats->cg_rsq = rsq;
   }
   return EFDC_OK;
+}
+
+int efdc_adj_record_bytes(efdc_ctx_t *c, size_t *bytes) {
+  if (!c || !bytes) return EFDC_ERR_ARG;
+  if (!c->adj) {
+    c->err = "adjoint not enabled (efdc_adj_enable)";
+    return EFDC_ERR_STATE;
+  }
+  size_t n = 0;
+  for (const Piece &p : record_pieces(c)) n += (p.bytes + 255) & ~(size_t)255;
+  *bytes = n;
+  return EFDC_OK;
+}
+
+int efdc_adj_record_save(efdc_ctx_t *c, void *dev_buf) {
+  if (!c || !dev_buf) return EFDC_ERR_ARG;
+  if (!c->adj) {
+    c->err = "adjoint not enabled (efdc_adj_enable)";
+    return EFDC_ERR_STATE;
+  }
+  return copy_pieces(c, record_pieces(c), (char *)dev_buf, true);
+}
+
+int efdc_adj_record_load(efdc_ctx_t *c, const void *dev_buf) {
+  if (!c || !dev_buf) return EFDC_ERR_ARG;
+  if (!c->adj) {
+    c->err = "adjoint not enabled (efdc_adj_enable)";
+    return EFDC_ERR_STATE;
+  }
+  return copy_pieces(c, record_pieces(c), (char *)const_cast<void *>(dev_buf), false);
 }
 
 int efdc_adj_source_grad(efdc_ctx_t *c, double *out) {

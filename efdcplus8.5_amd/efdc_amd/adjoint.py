@@ -14,11 +14,13 @@ layer is loads * QSS(s,K) / Q(s).
 
 ``gradient`` runs the forward model once, saving the whole context state
 every ``ckpt`` steps (efdc_state_save), then walks the segments backwards:
-each segment is re-run from its checkpoint with the state before every step
-saved, and every step is reversed by loading the state before it, repeating
-it (the context records what the reverse needs) and calling efdc_adj_step.
-Memory: (nsteps/ckpt + ckpt) snapshots, device buffers from
-efdc_state_alloc kept in a pool for the life of the problem.
+each segment is re-run from its checkpoint, keeping after every step the
+state and the step's recording (what the reverse reads: the CG solution,
+its scaling, the old constituent levels; efdc_adj_record_save) in one
+buffer, and every step is reversed by loading both and calling
+efdc_adj_step.  Two forward evaluations per step in all.  Memory:
+(nsteps/ckpt + ckpt) snapshots, device buffers from efdc_state_alloc kept
+in a pool for the life of the problem.
 """
 from __future__ import annotations
 
@@ -73,6 +75,7 @@ class DyeMisfit:
         self.data = None if data is None else np.asarray(data, np.float64)
         self.nman0 = None if case.manning_n is None else np.array(case.manning_n, np.float64)
         self._pool = []          # snapshot buffers (device pointers)
+        self._state_bytes = self.ctx.state_bytes()
         self._init = self.ctx.state_alloc()
         self.ctx.state_save(self._init)
         self.ctx.synchronize()
@@ -121,10 +124,15 @@ class DyeMisfit:
         return float(np.sum((y - self.data) ** 2))
 
     # ------------------------------------------------------------ reverse
-    def gradient(self, loads, nman=None):
-        """(J, dJ/dloads (nsteps, nsrc), dJ/dn per cell (LCM,), stats)."""
+    def gradient(self, loads, nman=None, *, rerun_steps=False):
+        """(J, dJ/dloads (nsteps, nsrc), dJ/dn per cell (LCM,), stats).
+
+        rerun_steps: instead of keeping each step's recording, reverse
+        every step by loading the state before it and running it again
+        (three forward evaluations per step; the same bits)."""
         loads = np.asarray(loads, np.float64)
         ctx, K = self.ctx, self.ckpt
+        rec_off = self._state_bytes
         ctx.adj_reset()
         self._start(loads, nman)
         starts = list(range(0, self.nsteps, K))
@@ -145,12 +153,22 @@ class DyeMisfit:
             t1 = min(t0 + K, self.nsteps)
             ctx.state_load(ckpts[t0])
             for j, n in enumerate(range(t0 + 1, t1 + 1)):
-                ctx.state_save(seg[j])
-                if n < t1:
+                if rerun_steps:
+                    # the state before step n
+                    ctx.state_save(seg[j])
+                    if n < t1:
+                        self._step(loads, n)
+                else:
+                    # the state after step n and what its reverse reads
                     self._step(loads, n)
+                    ctx.state_save(seg[j])
+                    ctx.adj_record_save(seg[j] + rec_off)
             for n in range(t1, t0, -1):
                 ctx.state_load(seg[n - t0 - 1])
-                self._step(loads, n)
+                if rerun_steps:
+                    self._step(loads, n)
+                else:
+                    ctx.adj_record_load(seg[n - t0 - 1] + rec_off)
                 if n in resid:
                     r = resid[n]
                     kk = np.repeat(np.arange(1, self.KC + 1), len(self.obs_L))

@@ -170,6 +170,9 @@ def load(path: Optional[str] = None):
         "efdc_tune_report": ([P, C.POINTER(TuneReport)], C.c_int),
         "efdc_state_bytes": ([P, C.POINTER(C.c_size_t)], C.c_int),
         "efdc_state_save": ([P, P], C.c_int),
+        "efdc_adj_record_bytes": ([P, C.POINTER(C.c_size_t)], C.c_int),
+        "efdc_adj_record_save": ([P, P], C.c_int),
+        "efdc_adj_record_load": ([P, P], C.c_int),
         "efdc_state_load": ([P, P], C.c_int),
         "efdc_state_alloc": ([P, C.POINTER(P)], C.c_int),
         "efdc_state_free": ([P, P], C.c_int),
@@ -403,8 +406,21 @@ class Context:
         self._check(self.lib.efdc_stream_copy_gbs(self.h, mib << 20, reps, C.byref(g)))
         return g.value
 
+    def adj_record_bytes(self) -> int:
+        n = C.c_size_t(0)
+        self._check(self.lib.efdc_adj_record_bytes(self.h, C.byref(n)))
+        return n.value
+
+    def adj_record_save(self, dev_ptr: int):
+        self._check(self.lib.efdc_adj_record_save(self.h, C.c_void_p(dev_ptr)))
+
+    def adj_record_load(self, dev_ptr: int):
+        self._check(self.lib.efdc_adj_record_load(self.h, C.c_void_p(dev_ptr)))
+
     def state_alloc(self) -> int:
-        """A device buffer for one snapshot (free it with state_free)."""
+        """A device buffer for one snapshot (free it with state_free); with
+        the adjoint enabled it also has room for a step's recording at
+        offset state_bytes()."""
         b = C.c_void_p()
         self._check(self.lib.efdc_state_alloc(self.h, C.byref(b)))
         return b.value

@@ -342,6 +342,23 @@ module efdc_hip
       real(c_double), intent(out) :: out(*)
     end function
 
+    integer(c_int) function efdc_adj_record_bytes(ctx, bytes) &
+        bind(C, name="efdc_adj_record_bytes")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: ctx
+      integer(c_size_t), intent(out) :: bytes
+    end function
+
+    integer(c_int) function efdc_adj_record_save(ctx, buf) bind(C, name="efdc_adj_record_save")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, buf
+    end function
+
+    integer(c_int) function efdc_adj_record_load(ctx, buf) bind(C, name="efdc_adj_record_load")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, buf
+    end function
+
     ! autotuner; the report is an efdc_tune_report_t the caller lays out
     integer(c_int) function efdc_autotune(ctx, on) bind(C, name="efdc_autotune")
       import :: c_int, c_ptr, c_int32_t

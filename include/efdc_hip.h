@@ -409,7 +409,9 @@ int efdc_state_bytes(efdc_ctx_t *ctx, size_t *bytes);
 int efdc_state_save(efdc_ctx_t *ctx, void *dev_buf);
 int efdc_state_load(efdc_ctx_t *ctx, const void *dev_buf);
 /* One device buffer of efdc_state_bytes() bytes, for hosts without a
-   device allocator of their own (Fortran, Python without torch). */
+   device allocator of their own (Fortran, Python without torch); while
+   the adjoint is enabled, efdc_adj_record_bytes() more, so that a step's
+   recording can follow its state at offset efdc_state_bytes(). */
 int efdc_state_alloc(efdc_ctx_t *ctx, void **dev_buf);
 int efdc_state_free(efdc_ctx_t *ctx, void *dev_buf);
 
@@ -431,7 +433,11 @@ int efdc_state_free(efdc_ctx_t *ctx, void *dev_buf);
    the old levels of the adjoint constituents); add the misfit gradient of
    that step's state with efdc_adj_seed_con; call efdc_adj_step with the
    same step; read efdc_adj_source_grad; load the state before the
-   previous step, and so on backwards. */
+   previous step, and so on backwards.
+   A driver that keeps the states after each step of a checkpoint segment
+   can skip the repeated forward step: save the recording of every step
+   with efdc_adj_record_save next to the state after it, and before
+   efdc_adj_step load both (efdc_state_load, efdc_adj_record_load). */
 typedef struct {
   int32_t nadj;                 /* constituents differentiated            */
   int32_t slot[16];             /* their slots                            */
@@ -462,6 +468,14 @@ int efdc_adj_step(efdc_ctx_t *ctx, const efdc_step_t *step, efdc_adj_stats_t *st
    zero for constituents that are not differentiated */
 int efdc_adj_source_grad(efdc_ctx_t *ctx, double *out);
 int efdc_adj_get(efdc_ctx_t *ctx, int32_t field, double *out);
+/* The recording of the last forward step (its CG solution, its scaling
+   minimum CCMNM, its source concentrations CQS and the old levels of the
+   adjoint constituents) to and
+   from a device buffer of efdc_adj_record_bytes bytes (no counterpart in
+   the reference: a checkpointed adjoint driver's storage). */
+int efdc_adj_record_bytes(efdc_ctx_t *ctx, size_t *bytes);
+int efdc_adj_record_save(efdc_ctx_t *ctx, void *dev_buf);
+int efdc_adj_record_load(efdc_ctx_t *ctx, const void *dev_buf);
 
 #ifdef __cplusplus
 }

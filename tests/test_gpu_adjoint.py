@@ -12,7 +12,8 @@ The misfit J is the squared difference of layer-mean dye at observed cells
   the transports and the advection; central differences with a relative
   step of 1e-3 (O(h^2) truncation) must match to the 1e-4 bar, on the GPU
   and with the CPU oracle;
-* the checkpoint interval changes nothing (same bits);
+* the checkpoint interval changes nothing (same bits), nor does keeping
+  each step's recording instead of running the step again;
 * torch.autograd.gradcheck of the autograd.Function wrapper;
 * the FP32-storage forward gives the FP64-storage gradient to 1e-3.
 """
@@ -146,6 +147,16 @@ def test_checkpoint_interval_changes_nothing(prob_fp64):
     prob.ckpt = 24
     J, gl, gn, _ = prob.gradient(p["loads0"])
     prob.ckpt = 5
+    assert J == p["J"]
+    assert np.array_equal(gl, p["gl"]) and np.array_equal(gn, p["gn"])
+
+
+def test_kept_recordings_equal_rerun_steps(prob_fp64):
+    """Reversing each step from the state after it and its saved recording
+    (efdc_adj_record_save/load) gives the bits of reversing it right after
+    running it again from the state before it."""
+    p = prob_fp64
+    J, gl, gn, _ = p["prob"].gradient(p["loads0"], rerun_steps=True)
     assert J == p["J"]
     assert np.array_equal(gl, p["gl"]) and np.array_equal(gn, p["gn"])
 
