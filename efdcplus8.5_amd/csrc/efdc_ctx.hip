@@ -47,6 +47,12 @@ __global__ __launch_bounds__(256) void k_copy4(const float4 *__restrict__ x,
 int blocks_for(int64_t n, int bs = EFDC_BLOCK) { return (int)((n + bs - 1) / bs); }
 
 template <typename T>
+__global__ void k_gather(const T *src, const int64_t *off, double *val, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) val[j] = (double)src[off[j]];
+}
+
+template <typename T>
 int dmalloc(efdc_ctx *c, T **p, size_t n) {
   *p = nullptr;
   if (n == 0) return EFDC_OK;
@@ -317,6 +323,12 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   adj_free(c);
   if (c->st_host) (void)hipHostFree(c->st_host);
   if (c->pcg_abort_host) (void)hipHostFree(c->pcg_abort_host);
+  dfree(c->gat_off);
+  dfree(c->gat_val);
+  for (int q = 0; q < efdc_ctx::CQS_RING; ++q) {
+    if (c->cqs_stage[q]) (void)hipHostFree(c->cqs_stage[q]);
+    if (c->cqs_ev[q]) (void)hipEventDestroy(c->cqs_ev[q]);
+  }
   dfree(c->tune_pbuf);
   if (c->tide_host) (void)hipHostFree(c->tide_host);
   for (auto &b : c->pbc) {
@@ -646,6 +658,46 @@ int efdc_get_constituent(efdc_ctx_t *c, int32_t slot, double *con,
   return EFDC_OK;
 }
 
+int efdc_get_constituent_at(efdc_ctx_t *c, int32_t slot, int32_t n, const int32_t *L,
+                            const int32_t *K, double *out) {
+  if (!c || slot < 0 || slot >= c->nconst || n < 0 || (n > 0 && (!L || !K || !out)))
+    return EFDC_ERR_ARG;
+  if (n == 0) return EFDC_OK;
+  std::vector<int64_t> off(n);
+  for (int j = 0; j < n; ++j) {
+    if (L[j] < 2 || L[j] > c->LA || K[j] < 1 || K[j] > c->KC) {
+      c->err = "efdc_get_constituent_at: cell outside 2..LA x 1..KC";
+      return EFDC_ERR_ARG;
+    }
+    off[j] = (int64_t)(L[j] - 1) + (int64_t)(K[j] - 1) * c->pitch;
+  }
+  if (n > c->gat_cap) {
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    dfree(c->gat_off);
+    dfree(c->gat_val);
+    c->gat_cap = 0;
+    int s;
+    if ((s = dmalloc(c, &c->gat_off, (size_t)n)) || (s = dmalloc(c, &c->gat_val, (size_t)n)))
+      return s;
+    c->gat_cap = n;
+  }
+  ConstDev &k = c->con[slot];
+  HIP_TRY(c, hipMemcpyAsync(c->gat_off, off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+                            c->stream));
+  const int nb = blocks_for(n);
+  if (c->store == EFDC_STORE_FP64)
+    k_gather<double><<<nb, EFDC_BLOCK, 0, c->stream>>>((const double *)k.buf[k.cur],
+                                                       c->gat_off, c->gat_val, n);
+  else
+    k_gather<float><<<nb, EFDC_BLOCK, 0, c->stream>>>((const float *)k.buf[k.cur],
+                                                      c->gat_off, c->gat_val, n);
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipMemcpyAsync(out, c->gat_val, sizeof(double) * n, hipMemcpyDeviceToHost,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
 int efdc_set_conc_bc(efdc_ctx_t *c, const efdc_conc_bc_t *bc) {
   if (!c || !bc) return EFDC_ERR_ARG;
   const int nc = c->nconst, KC = c->KC, LA = c->LA;
@@ -823,9 +875,31 @@ int efdc_set_source_cqs(efdc_ctx_t *c, const double *cqs) {
     c->err = "efdc_set_source_cqs: no source table (efdc_set_sources)";
     return EFDC_ERR_STATE;
   }
-  HIP_TRY(c, hipMemcpyAsync(c->cqs, cqs, sizeof(double) * (size_t)c->nqsij * c->KC * c->src_nc,
-                            hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  // staged through a pinned ring so that the call does not wait for the
+  // work already queued (the caller's array is free again on return)
+  const size_t bytes = sizeof(double) * (size_t)c->nqsij * c->KC * c->src_nc;
+  if (bytes != c->cqs_stage_bytes) {
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int q = 0; q < efdc_ctx::CQS_RING; ++q) {
+      if (c->cqs_stage[q]) (void)hipHostFree(c->cqs_stage[q]);
+      c->cqs_stage[q] = nullptr;
+      if (!c->cqs_ev[q]) HIP_TRY(c, hipEventCreateWithFlags(&c->cqs_ev[q], hipEventDisableTiming));
+    }
+    c->cqs_stage_bytes = 0;
+    for (int q = 0; q < efdc_ctx::CQS_RING; ++q)
+      if (hipHostMalloc((void **)&c->cqs_stage[q], bytes, hipHostMallocDefault) != hipSuccess) {
+        c->err = "efdc_set_source_cqs: pinned staging allocation failed";
+        return EFDC_ERR_NOMEM;
+      }
+    c->cqs_stage_bytes = bytes;
+    c->cqs_next = 0;
+  }
+  const int q = c->cqs_next;
+  c->cqs_next = (q + 1) % efdc_ctx::CQS_RING;
+  HIP_TRY(c, hipEventSynchronize(c->cqs_ev[q]));
+  std::memcpy(c->cqs_stage[q], cqs, bytes);
+  HIP_TRY(c, hipMemcpyAsync(c->cqs, c->cqs_stage[q], bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipEventRecord(c->cqs_ev[q], c->stream));
   return EFDC_OK;
 }
 

@@ -195,6 +195,17 @@ struct efdc_ctx {
   // step's final read-back (one host round trip per step)
   bool pcg_deferred = false;
   int *pcg_abort_host = nullptr;  // pinned
+  // efdc_set_source_cqs: pinned staging ring, a slot reused once the copy
+  // that last read it has completed
+  static constexpr int CQS_RING = 8;
+  double *cqs_stage[CQS_RING] = {};
+  hipEvent_t cqs_ev[CQS_RING] = {};
+  size_t cqs_stage_bytes = 0;
+  int cqs_next = 0;
+  // efdc_get_constituent_at: cell offsets and values (device), grown on demand
+  int64_t *gat_off = nullptr;
+  double *gat_val = nullptr;
+  int gat_cap = 0;
   // autotuner (efdc_autotune): launch shape of the column transport and
   // the preconditioner degree of the mixed persistent solve
   bool tune_pending = false, tune_cg_pending = false;

@@ -71,6 +71,8 @@ class DyeMisfit:
         self.cqs0 = np.array(src["cqs"], np.float64).reshape(self.nsrc, self.KC, -1)
         self.nc = self.cqs0.shape[2]
         self.dzc_obs = case.k["DZC"][:self.KC, self.obs_L - 1]      # (KC, nobs)
+        self._obs_K = np.repeat(np.arange(1, self.KC + 1), len(self.obs_L)).astype(np.int32)
+        self._obs_LL = np.tile(self.obs_L, self.KC).astype(np.int32)
         self.obs_steps = [t for t in range(1, nsteps + 1) if t % obs_every == 0]
         self.data = None if data is None else np.asarray(data, np.float64)
         self.nman0 = None if case.manning_n is None else np.array(case.manning_n, np.float64)
@@ -93,8 +95,8 @@ class DyeMisfit:
         return c
 
     def _observe(self):
-        con, _ = self.ctx.get_constituent(self.dye)
-        return np.sum(self.dzc_obs * con[:self.KC, self.obs_L - 1], axis=0)
+        c = self.ctx.get_constituent_at(self.dye, self._obs_LL, self._obs_K)
+        return np.sum(self.dzc_obs * c.reshape(self.KC, len(self.obs_L)), axis=0)
 
     def _step(self, loads, n):
         self.ctx.set_source_cqs(self._cqs(loads, n))

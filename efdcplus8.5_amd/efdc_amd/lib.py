@@ -153,6 +153,7 @@ def load(path: Optional[str] = None):
         "efdc_add_constituent": ([P, C.POINTER(ConstDesc), PI], C.c_int),
         "efdc_set_constituent": ([P, C.c_int32, PD, PD], C.c_int),
         "efdc_get_constituent": ([P, C.c_int32, PD, PD], C.c_int),
+        "efdc_get_constituent_at": ([P, C.c_int32, C.c_int32, PI, PI, PD], C.c_int),
         "efdc_congrad": ([P, C.POINTER(CgStats)], C.c_int),
         "efdc_calpuv2c": ([P, C.POINTER(Step), C.POINTER(CgStats)], C.c_int),
         "efdc_calconc": ([P, C.POINTER(Step)], C.c_int),
@@ -352,6 +353,15 @@ class Context:
         con1 = np.zeros((g.KCM, g.LCM))
         self._check(self.lib.efdc_get_constituent(self.h, slot, _pd(con), _pd(con1)))
         return con, con1
+
+    def get_constituent_at(self, slot, L, K) -> np.ndarray:
+        """CON of constituent `slot` at cells (L, K) (1-based), FP64."""
+        L = np.ascontiguousarray(L, np.int32)
+        K = np.ascontiguousarray(K, np.int32)
+        out = np.zeros(len(L))
+        self._check(self.lib.efdc_get_constituent_at(self.h, slot, len(L), _pi(L), _pi(K),
+                                                     _pd(out)))
+        return out
 
     # ------------------------------------------------------------ calls
     def congrad(self) -> dict:

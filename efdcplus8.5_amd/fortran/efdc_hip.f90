@@ -208,6 +208,15 @@ module efdc_hip
       real(c_double), intent(inout) :: con(*), con1(*)
     end function
 
+    integer(c_int) function efdc_get_constituent_at(ctx, slot, n, L, K, out) &
+        bind(C, name="efdc_get_constituent_at")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: slot, n
+      integer(c_int32_t), intent(in) :: L(*), K(*)
+      real(c_double), intent(out) :: out(*)
+    end function
+
     integer(c_int) function efdc_congrad(ctx, stats) bind(C, name="efdc_congrad")
       import :: c_int, c_ptr, efdc_cg_stats_t
       type(c_ptr), value :: ctx

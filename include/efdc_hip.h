@@ -287,6 +287,11 @@ int efdc_set_constituent(efdc_ctx_t *ctx, int32_t slot, const double *con,
                          const double *con1);
 int efdc_get_constituent(efdc_ctx_t *ctx, int32_t slot, double *con,
                          double *con1);
+/* CON of one constituent at n cells (L in 2..LA, K in 1..KC, Fortran
+   indices) into out[n], FP64: the observation operator of a host that
+   reads a few cells every few steps (no counterpart in the reference). */
+int efdc_get_constituent_at(efdc_ctx_t *ctx, int32_t slot, int32_t n, const int32_t *L,
+                            const int32_t *K, double *out);
 
 /* CONGRAD (EFDC/congrad.f90:1-191) on the scaled system already held in
    CCC, CCS, CCW, CCE, CCN, CCCI, FPTMP, starting from and updating P. */

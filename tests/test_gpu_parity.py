@@ -321,6 +321,42 @@ def test_settling_matches_oracle(oracle_mod, store):
     ctx.close()
 
 
+@pytest.mark.parametrize("store", ["fp64", "fp32"])
+def test_constituent_gather_and_cqs_updates(store):
+    """efdc_get_constituent_at returns the values of a full read at the
+    listed cells, and CQS updates between queued steps (staged without a
+    wait) reach the steps they were set for: the same loads give the same
+    bits as a context that waits after every step."""
+    from efdc_amd import grids
+    from efdc_amd.lib import EfdcError, STORE_FP32, STORE_FP64
+    st = STORE_FP64 if store == "fp64" else STORE_FP32
+    case = grids.transport_case(40, 24, 5, seed=51)
+    a = _ctx(case, store=st)
+    b = _ctx(case, store=st)
+    rng = np.random.default_rng(5)
+    cqs = np.asarray(case.sources["cqs"], np.float64)
+    for n in range(1, 13):
+        c = cqs.copy()
+        c[..., 2] = rng.uniform(0.0, 2000.0, c.shape[:-1])
+        a.set_source_cqs(c)
+        a.calconc(case.dt, n)          # queued, no wait in between
+        b.set_source_cqs(c)
+        b.calconc(case.dt, n)
+        b.synchronize()
+    g = case.grid
+    L = rng.integers(2, g.LA + 1, 300).astype(np.int32)
+    K = rng.integers(1, g.KC + 1, 300).astype(np.int32)
+    for j in range(len(case.consts)):
+        full, _ = a.get_constituent(j)
+        got = a.get_constituent_at(j, L, K)
+        assert np.array_equal(got, full[K - 1, L - 1])
+        assert np.array_equal(full, b.get_constituent(j)[0])
+    with pytest.raises(EfdcError):
+        a.get_constituent_at(0, [g.LA + 1], [1])
+    a.close()
+    b.close()
+
+
 def test_negative_depth_status_on_gpu(oracle_mod):
     from efdc_amd.lib import EfdcError
     case = _lake(24, seed=1)
