@@ -151,7 +151,9 @@ void efdc_params_default(efdc_params_t *p) {
 }
 
 static bool tran_shape_valid(int q) {
-  return q == 0 || q == 1 || q == 2 || q == 4 || q == -1 || q == -2 || q == -4;
+  if (q == 0 || q == -1 || q == -2 || q == -4) return true;
+  const int n = q & 15, v = q >> 4;
+  return q > 0 && (n == 1 || n == 2 || n == 4) && (v == 0 || v == 1 || (v == 2 && n <= 2));
 }
 
 // Reason of the last failed efdc_ctx_create on this thread (there is no
@@ -184,7 +186,7 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
   if (prm->mixed_precond_degree < 0 || prm->mixed_precond_degree > 16)
     return reject(EFDC_ERR_ARG, "mixed_precond_degree must be 0..16");
   if (!tran_shape_valid(prm->tran_shape))
-    return reject(EFDC_ERR_ARG, "tran_shape must be 0, 1, 2, 4, -1, -2 or -4");
+    return reject(EFDC_ERR_ARG, "tran_shape must be 0, -1, -2, -4, or 1, 2, 4 plus 0, 16 or 32 (32 for 1 and 2)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return reject(EFDC_ERR_NODEVICE, "no HIP device visible");
@@ -375,7 +377,7 @@ int efdc_set_params(efdc_ctx_t *c, const efdc_params_t *p) {
     return EFDC_ERR_ARG;
   }
   if (!tran_shape_valid(p->tran_shape)) {
-    c->err = "tran_shape must be 0, 1, 2, 4, -1, -2 or -4";
+    c->err = "tran_shape must be 0, -1, -2, -4, or 1, 2, 4 plus 0, 16 or 32 (32 for 1 and 2)";
     return EFDC_ERR_ARG;
   }
   if (p->cg_kernel < EFDC_CG_KERNEL_AUTO || p->cg_kernel > EFDC_CG_KERNEL_PERSISTENT) {

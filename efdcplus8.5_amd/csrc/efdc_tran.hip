@@ -522,10 +522,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // waves of a block, which all compute the same values) and the forward
 // values [waves][NCW][KC][64].
 // ---------------------------------------------------------------------
-template <typename T, int NCW>
-struct LayerM {
+// The loads of one layer: the transport fields (shared by the thread's
+// constituents) and six CON1 values per constituent.
+template <typename T>
+struct FlowL {
   T u_i, u_e, v_i, v_n, ww, h1pk, hpki, ab;
   T ckk, ckm;  // CDZKK/CDZKMK (not used when they are uniform)
+};
+template <typename T, int NCW>
+struct ConL {
   T c1[NCW], cw[NCW], cs[NCW], ce[NCW], cn[NCW], cup[NCW];
 };
 
@@ -538,7 +543,11 @@ __device__ __forceinline__ void stb(T x, T *base, unsigned boff) {
   *(T *)((char *)base + boff) = x;
 }
 
-template <typename T, int NCW, bool RARE>
+// FPF: how many layers ahead the transport fields are loaded (1 or 2; the
+// CON1 values are loaded one layer ahead).  With FPF = 2 a layer's CON1
+// loads are issued before the transport fields of the layer after next, so
+// that waiting for them does not wait for those.
+template <typename T, int NCW, bool RARE, int FPF>
 __device__ __forceinline__ void tran_colm_run(const TranArgs<T> &a, double *vcus, int i, int c0,
                                               int src0, int rec) {
   const int KC = a.KC, nc = a.nc;
@@ -573,10 +582,11 @@ __device__ __forceinline__ void tran_colm_run(const TranArgs<T> &a, double *vcus
 
   double *vcu = vcus + threadIdx.x;  // stride 64
   T *ys = (T *)(vcus + (size_t)KC * 64) + (size_t)threadIdx.y * NCW * KC * 64 + threadIdx.x;
-  // layer k; its upward neighbour plane is k+1 below the top, else k
-  auto load = [&](int k, LayerM<T, NCW> &L) {
+  // layer k (clamped to the top layer, so that every trip issues the same
+  // loads); the upward neighbour plane is k+1 below the top, else k
+  auto clampk = [&](int k) { return k < KC ? k : KC - 1; };
+  auto loadF = [&](int k, FlowL<T> &L) {
     const size_t o = (size_t)k * pitch;
-    const size_t ou = (size_t)(k + 1 < KC ? k + 1 : k) * pitch;
     const T *U = a.uhdy2 + o, *V = a.vhdx2 + o;
     L.u_i = ldb(U, bi);
     L.u_e = ldb(U, be);
@@ -588,6 +598,10 @@ __device__ __forceinline__ void tran_colm_run(const TranArgs<T> &a, double *vcus
     L.ab = ldb(a.ab + o, bi);
     L.ckk = ldb(pkk + o, bi);
     L.ckm = ldb(pkm + o, bi);
+  };
+  auto loadC = [&](int k, ConL<T, NCW> &L) {
+    const size_t o = (size_t)k * pitch;
+    const size_t ou = (size_t)(k + 1 < KC ? k + 1 : k) * pitch;
 #pragma unroll
     for (int j = 0; j < NCW; ++j) {
       if (!on[j]) continue;
@@ -600,8 +614,11 @@ __device__ __forceinline__ void tran_colm_run(const TranArgs<T> &a, double *vcus
       L.cup[j] = ldb(C1[j] + ou, bi);
     }
   };
-  LayerM<T, NCW> cur, nxt;
-  load(0, cur);
+  FlowL<T> F0, F1, F2;
+  ConL<T, NCW> Q0, Q1;
+  loadC(0, Q0);
+  loadF(0, F0);
+  if (FPF == 2) loadF(clampk(1), F1);
   double fwp[NCW], yprev[NCW];
 #pragma unroll
   for (int j = 0; j < NCW; ++j) fwp[j] = yprev[j] = 0.0;
@@ -609,18 +626,19 @@ __device__ __forceinline__ void tran_colm_run(const TranArgs<T> &a, double *vcus
   for (int k = 0; k < KC; ++k) {
     const int K = k + 1;
     const size_t o = (size_t)k * pitch;
-    // the top layer re-reads itself: the same loads on every trip
-    load(K < KC ? k + 1 : k, nxt);
+    loadC(clampk(k + 1), Q1);
+    if (FPF == 2) loadF(clampk(k + 2), F2);
+    else loadF(clampk(k + 1), F1);
     const bool below_top = K < KC;
-    const double u_i = (double)cur.u_i, u_e = eg ? 0.0 : (double)cur.u_e,
-                 v_i = (double)cur.v_i, v_n = ng ? 0.0 : (double)cur.v_n;
-    const double ww = below_top ? (double)cur.ww : 0.0;
-    const double h1pk = (double)cur.h1pk, hpki = (double)cur.hpki;
-    const double abk = (vdk && below_top) ? (double)cur.ab : 0.0;
+    const double u_i = (double)F0.u_i, u_e = eg ? 0.0 : (double)F0.u_e,
+                 v_i = (double)F0.v_i, v_n = ng ? 0.0 : (double)F0.v_n;
+    const double ww = below_top ? (double)F0.ww : 0.0;
+    const double h1pk = (double)F0.h1pk, hpki = (double)F0.hpki;
+    const double abk = (vdk && below_top) ? (double)F0.ab : 0.0;
     double cclb = 0.0, eeb = 1.0, vcu_k = 0.0;
     if (vdk) {
-      const double ckk = a.dz_uniform ? a.cdzkk_k[k] : (double)cur.ckk;
-      const double ckm = a.dz_uniform ? a.cdzkmk_k[k] : (double)cur.ckm;
+      const double ckk = a.dz_uniform ? a.cdzkk_k[k] : (double)F0.ckk;
+      const double ckm = a.dz_uniform ? a.cdzkmk_k[k] : (double)F0.ckm;
       if (K == 1) {
         const double RCDZKK = -DELTD2 * ckk;
         const double CCUBTMP = RCDZKK * hpi * abk;
@@ -643,21 +661,21 @@ __device__ __forceinline__ void tran_colm_run(const TranArgs<T> &a, double *vcus
       }
       vcu[k * 64] = vcu_k;
     }
-    const bool pu = cur.u_i >= (T)0, pv = cur.v_i >= (T)0, pue = cur.u_e >= (T)0,
-               pvn = cur.v_n >= (T)0, pw = cur.ww >= (T)0;
+    const bool pu = F0.u_i >= (T)0, pv = F0.v_i >= (T)0, pue = F0.u_e >= (T)0,
+               pvn = F0.v_n >= (T)0, pw = F0.ww >= (T)0;
 #pragma unroll
     for (int j = 0; j < NCW; ++j) {
       if (!on[j]) continue;
       const int cc = c0 + j;
-      const double c1 = (double)cur.c1[j];
+      const double c1 = (double)Q0.c1[j];
       // upwind fluxes (LUPU/LUPV/KUPW of EFDC/calconc.f90:95-118); the
       // donor is chosen in the storage type, where the sign test and the
       // value are those of the FP64 conversion
-      const double fu_i = u_i * (double)(pu ? cur.cw[j] : cur.c1[j]);
-      const double fv_i = v_i * (double)(pv ? cur.cs[j] : cur.c1[j]);
-      const double fu_e = eg ? 0.0 : u_e * (double)(pue ? cur.c1[j] : cur.ce[j]);
-      const double fv_n = ng ? 0.0 : v_n * (double)(pvn ? cur.c1[j] : cur.cn[j]);
-      const double fw = below_top ? ww * (double)(pw ? cur.c1[j] : cur.cup[j]) : 0.0;
+      const double fu_i = u_i * (double)(pu ? Q0.cw[j] : Q0.c1[j]);
+      const double fv_i = v_i * (double)(pv ? Q0.cs[j] : Q0.c1[j]);
+      const double fu_e = eg ? 0.0 : u_e * (double)(pue ? Q0.c1[j] : Q0.ce[j]);
+      const double fv_n = ng ? 0.0 : v_n * (double)(pvn ? Q0.c1[j] : Q0.cn[j]);
+      const double fw = below_top ? ww * (double)(pw ? Q0.c1[j] : Q0.cup[j]) : 0.0;
       const double fqc = (RARE && src0 >= 0) ? col_sources(a, src0, k, cc, c1) : 0.0;
       // EFDC/caltran.f90:194-196,243
       const double cd = c1 * h1pk + DDELT * ((fqc + fu_i - fu_e + fv_i - fv_n) * dxyip +
@@ -681,7 +699,9 @@ __device__ __forceinline__ void tran_colm_run(const TranArgs<T> &a, double *vcus
     }
     ab_prev = abk;
     vcu_prev = vcu_k;
-    cur = nxt;
+    Q0 = Q1;
+    F0 = F1;
+    if (FPF == 2) F1 = F2;
   }
   if (!a.vd) return;
   // back substitution and dye decay, write CON
@@ -711,7 +731,7 @@ __device__ __forceinline__ void tran_colm_run(const TranArgs<T> &a, double *vcus
   }
 }
 
-template <typename T, int NCW>
+template <typename T, int NCW, int FPF>
 __device__ __forceinline__ void tran_colm_body(const TranArgs<T> &a, double *vcus) {
   // tiles of 64 columns start at L-1 = 0 mod 64, so every plane access of
   // a wave covers whole 256-byte segments (the dummy cell 1 is masked)
@@ -722,23 +742,23 @@ __device__ __forceinline__ void tran_colm_body(const TranArgs<T> &a, double *vcu
   const int src0 = a.srcrec[i];
   const int rec = a.bcrec ? a.bcrec[i] : -1;
   if (__ballot(src0 >= 0 || rec >= 0) != 0)
-    tran_colm_run<T, NCW, true>(a, vcus, i, c0, src0, rec);
+    tran_colm_run<T, NCW, true, FPF>(a, vcus, i, c0, src0, rec);
   else
-    tran_colm_run<T, NCW, false>(a, vcus, i, c0, src0, rec);
+    tran_colm_run<T, NCW, false, FPF>(a, vcus, i, c0, src0, rec);
 }
 
 template <typename T, int NCW>
 __global__ __launch_bounds__(256) void k_tran_colm(TranArgs<T> a) {
   extern __shared__ double vcus[];
-  tran_colm_body<T, NCW>(a, vcus);
+  tran_colm_body<T, NCW, 1>(a, vcus);
 }
 
-// at most 168 VGPRs (three waves per SIMD)
-template <typename T, int NCW>
+// at most 168 VGPRs (three waves per SIMD), transport fields FPF layers ahead
+template <typename T, int NCW, int FPF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_tran_colm_w3(
     TranArgs<T> a) {
   extern __shared__ double vcus[];
-  tran_colm_body<T, NCW>(a, vcus);
+  tran_colm_body<T, NCW, FPF>(a, vcus);
 }
 
 // Cohesive settling after the transport (SSEDTOX -> CALSED with DTSED = DT,
@@ -1385,7 +1405,9 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
   // k_tran_colm, NCW constituents per wave (the shape parameter q = NCW),
   // ceil(nc/NCW) waves; else k_tran_col, q waves each walking its
   // constituents one after the other.  Every shape gives the same bits.
-  const int req = c->prm.tran_shape;
+  // requested shape: sign and count of constituents, variant in bits 4-5
+  const int req = c->prm.tran_shape > 0 ? (c->prm.tran_shape & 15) : c->prm.tran_shape;
+  const int req_var = c->prm.tran_shape > 0 ? (c->prm.tran_shape >> 4) : 0;
   if (req > 0 && !c->mutual) {
     c->err = "tran_shape > 0 needs mutual neighbour maps";
     return EFDC_ERR_ARG;
@@ -1412,13 +1434,15 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
         if (w3) k_tran_col_w3<T><<<nbc, blk, lds, c->stream>>>(a);
         else k_tran_col<T><<<nbc, blk, lds, c->stream>>>(a);
       } else if (q == 4) {
-        if (w3) k_tran_colm_w3<T, 4><<<nbc, blk, lds, c->stream>>>(a);
+        if (w3) k_tran_colm_w3<T, 4, 1><<<nbc, blk, lds, c->stream>>>(a);
         else k_tran_colm<T, 4><<<nbc, blk, lds, c->stream>>>(a);
       } else if (q == 2) {
-        if (w3) k_tran_colm_w3<T, 2><<<nbc, blk, lds, c->stream>>>(a);
+        if (w3 == 2) k_tran_colm_w3<T, 2, 2><<<nbc, blk, lds, c->stream>>>(a);
+        else if (w3) k_tran_colm_w3<T, 2, 1><<<nbc, blk, lds, c->stream>>>(a);
         else k_tran_colm<T, 2><<<nbc, blk, lds, c->stream>>>(a);
       } else {
-        if (w3) k_tran_colm_w3<T, 1><<<nbc, blk, lds, c->stream>>>(a);
+        if (w3 == 2) k_tran_colm_w3<T, 1, 2><<<nbc, blk, lds, c->stream>>>(a);
+        else if (w3) k_tran_colm_w3<T, 1, 1><<<nbc, blk, lds, c->stream>>>(a);
         else k_tran_colm<T, 1><<<nbc, blk, lds, c->stream>>>(a);
       }
     };
@@ -1436,7 +1460,11 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
       float best_ms = 1e30f;
       hipEvent_t e0 = timer_event(c), e1 = timer_event(c);
       for (int qi = 0; qi < nq; ++qi) {
-        for (int w3 = 0; w3 < 2; ++w3) {
+        // variants: 0 the compiler's register budget, 1 at most 168 VGPRs,
+        // 2 (k_tran_colm with 1 or 2 constituents per thread) at most 168
+        // VGPRs with the transport fields two layers ahead
+        const int nvar = (colm && qs[qi] <= 2) ? 3 : 2;
+        for (int w3 = 0; w3 < nvar; ++w3) {
           float tot = 0.f;
           for (int r = 0; r < 4; ++r) {
             HIP_TRY(c, hipEventRecord(e0, c->stream));
@@ -1448,15 +1476,17 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
             if (r > 0) tot += ms;  // the first run is a warm-up
           }
           const float avg = tot / 3.f;
+          int slot = -1;
           if (c->tune_n < 8) {
             std::snprintf(c->tune_name[c->tune_n], 32, "%s=%d%s",
                           colm ? "tran_colm ncw" : "tran_col lanes", qs[qi],
-                          w3 ? " vgpr<=168" : "");
+                          w3 == 2 ? " vgpr<=168 pf2" : (w3 ? " vgpr<=168" : ""));
+            slot = c->tune_n;
             c->tune_ms[c->tune_n++] = avg;
           }
           if (avg < best_ms) {
             best_ms = avg;
-            best = c->tune_n - 1;
+            best = slot;
             c->tran_ncg = qs[qi];
             c->tran_w3 = w3;
           }
@@ -1470,7 +1500,7 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
     // maps), else as many lanes as fit
     int q = req != 0 ? (req < 0 ? -req : req) : c->tran_ncg;
     if (q <= 0 || !shape_ok(q)) q = (colm && shape_ok(2)) ? 2 : qs[0];
-    TIMED(c, TK_TRAN, launch(q, req == 0 && c->tran_ncg > 0 ? c->tran_w3 : 0));
+    TIMED(c, TK_TRAN, launch(q, req != 0 ? req_var : (c->tran_ncg > 0 ? c->tran_w3 : 0)));
     if (any_adac) {
       int s = adac_pipeline<T>(c, a, step);
       if (s) return s;

@@ -104,10 +104,12 @@ typedef struct {
   int32_t tran_shape;      /* CALCONC column kernel launch shape: 0 = the
                               library's default or the autotuner's choice;
                               1, 2 or 4 = that many constituents per thread
-                              (needs mutual neighbour maps); -1, -2 or -4 =
-                              one constituent per thread at a time, that many
-                              waves per workgroup.  Every shape gives the
-                              same bits.                                      */
+                              (needs mutual neighbour maps), + 16 = at most
+                              168 VGPRs, + 32 (1 or 2 only) = that and the
+                              transport fields loaded two layers ahead;
+                              -1, -2 or -4 = one constituent per thread at a
+                              time, that many waves per workgroup.  Every
+                              shape gives the same bits.                      */
 } efdc_params_t;
 
 /* Fill *p with the values EFDC uses by default: G=9.81, RITB1=1, SNLT=0,

@@ -47,7 +47,8 @@ def test_invalid_params_are_refused_before_any_device_call():
     from efdc_amd import grids, lib
     from efdc_amd.abi import STATUS
     case = grids.tidal_basin(8, 4, 2)
-    for bad in (dict(tran_shape=3), dict(tran_shape=-3), dict(tran_shape=8),
+    for bad in (dict(tran_shape=3), dict(tran_shape=-3), dict(tran_shape=8), dict(tran_shape=36),
+                dict(tran_shape=-17), dict(tran_shape=48),
                 dict(mixed_precond_degree=17), dict(cg_kernel=7), dict(store=5)):
         with pytest.raises(lib.EfdcError) as e:
             lib.Context(case.grid, **bad)

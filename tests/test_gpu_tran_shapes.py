@@ -1,7 +1,9 @@
 """Launch shapes of the CALCONC column kernels (efdc_params_t.tran_shape).
 
 k_tran_colm (1, 2 or 4 constituents per thread, the transport fields and
-the tridiagonal factors computed once per layer for all of them) and
+the tridiagonal factors computed once per layer for all of them; with the
+compiler's register budget, at most 168 VGPRs, or that and the transport
+fields loaded two layers ahead) and
 k_tran_col (one constituent per thread at a time, 1, 2 or 4 waves) must
 give the same bits for every shape: same values, same FP64 operation order.
 The cases cover 2, 3 (a partly filled wave) and 4 constituents, FP64 and
@@ -16,7 +18,7 @@ from conftest import eta_err
 
 pytestmark = pytest.mark.gpu
 
-SHAPES4 = [1, 2, 4, -1, -2, -4]
+SHAPES4 = [1, 2, 4, 17, 18, 20, 33, 34, -1, -2, -4]
 
 
 def _ctx(case, shape, **kw):
@@ -79,7 +81,7 @@ def test_tidal_basin_steps_shapes_bitwise(oracle_mod, store):
     from efdc_amd import grids, lib
     case = grids.tidal_basin(40, 20, 5, seed=43)
     st = lib.STORE_FP64 if store == "fp64" else lib.STORE_FP32
-    shapes = [1, 2, -1, -2]
+    shapes = [1, 2, 17, 18, 33, 34, -1, -2]
     ctxs = [_ctx(case, q, store=st) for q in shapes]
     ref = case.copy()
     o = oracle_mod.Oracle(ref, rsqm=1e-24, iterm=2000)
