@@ -1049,6 +1049,10 @@ int efdc_adj_enable(efdc_ctx_t *c, const efdc_adj_opts_t *o) {
     c->err = "adjoint: one basin per context";
     return EFDC_ERR_ARG;
   }
+  if (c->prm.ISDRY > 0) {
+    c->err = "adjoint: drying (ISDRY > 0) is not differentiated";
+    return EFDC_ERR_ARG;
+  }
   if (!c->mutual) {
     c->err = "adjoint: the neighbour maps must be mutual (LEC(LWC(L)) = L, LNC(LSC(L)) = L)";
     return EFDC_ERR_ARG;

@@ -148,6 +148,7 @@ void efdc_params_default(efdc_params_t *p) {
   p->cg_kernel = EFDC_CG_KERNEL_AUTO;
   p->mixed_precond_degree = 0;
   p->tran_shape = 0;
+  p->ISDRY = 0;
 }
 
 static bool tran_shape_valid(int q) {
@@ -185,6 +186,8 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
     return reject(EFDC_ERR_ARG, "unknown CG kernel choice");
   if (prm->mixed_precond_degree < 0 || prm->mixed_precond_degree > 16)
     return reject(EFDC_ERR_ARG, "mixed_precond_degree must be 0..16");
+  if (prm->ISDRY < 0 || prm->ISDRY > 97)
+    return reject(EFDC_ERR_ARG, "ISDRY must be 0..97 (the rewetting of 98/99 is not built)");
   if (!tran_shape_valid(prm->tran_shape))
     return reject(EFDC_ERR_ARG, "tran_shape must be 0, -1, -2, -4, or 1, 2, 4 plus 0, 16 or 32 (32 for 1 and 2)");
   int ndev = 0;
@@ -288,7 +291,9 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
   if (hipHostMalloc((void **)&c->st_host, sizeof(CgState)) != hipSuccess)
     return fail(EFDC_ERR_NOMEM);
   std::memset(c->st_host, 0, sizeof(CgState));
-  if ((s = dmalloc(c, &c->lopenbcdry, LCM)) ||
+  if ((s = dmalloc(c, &c->iscdry, LCM)) || (s = dmalloc(c, &c->lmaskdry, LCM)) ||
+      (s = dmalloc(c, &c->sub1, LCM)) || (s = dmalloc(c, &c->svb1, LCM)) ||
+      (s = dmalloc(c, &c->lopenbcdry, LCM)) ||
       (s = dmalloc(c, &c->pbcflag, LCM)) || (s = dmalloc(c, &c->lobflag, LCM)) ||
       (s = dmalloc(c, &c->bcrec, LCM)) ||
       (s = dmalloc(c, &c->srcrec, LCM)) || (s = dmalloc(c, &c->cdzkk_k, 64)) ||
@@ -298,6 +303,7 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
   const int nb = blocks_for(LCM);
   k_fill<int><<<nb, EFDC_BLOCK, 0, c->stream>>>(c->bcrec, -1, LCM);
   k_fill<int><<<nb, EFDC_BLOCK, 0, c->stream>>>(c->srcrec, -1, LCM);
+  k_fill<int><<<nb, EFDC_BLOCK, 0, c->stream>>>(c->lmaskdry, 1, LCM);
   k_init_puv<<<nb, EFDC_BLOCK, 0, c->stream>>>(c->dH[EFDC_RCX], c->dH[EFDC_RCY],
                                                c->dH[EFDC_CC], LCM, LC - 1);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(EFDC_ERR_HIP);
@@ -337,6 +343,7 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
     dfree(b.L); dfree(b.ispb); dfree(b.pser); dfree(b.pcb); dfree(b.psb);
   }
   delete[] c->tcp_host;
+  dfree(c->iscdry); dfree(c->lmaskdry); dfree(c->sub1); dfree(c->svb1);
   dfree(c->tide_dev); dfree(c->lopenbcdry); dfree(c->pbcflag); dfree(c->lobflag);
   dfree(c->lobcs); dfree(c->lobcs2);
   for (auto &b : c->cbc) {
@@ -374,6 +381,10 @@ int efdc_set_params(efdc_ctx_t *c, const efdc_params_t *p) {
   }
   if (p->mixed_precond_degree < 0 || p->mixed_precond_degree > 16) {
     c->err = "mixed_precond_degree must be 0..16";
+    return EFDC_ERR_ARG;
+  }
+  if (p->ISDRY < 0 || p->ISDRY > 97) {
+    c->err = "ISDRY must be 0..97 (the rewetting of 98/99 is not built)";
     return EFDC_ERR_ARG;
   }
   if (!tran_shape_valid(p->tran_shape)) {
@@ -657,6 +668,19 @@ int efdc_get_constituent(efdc_ctx_t *c, int32_t slot, double *con,
     if (s) return s;
   }
   if (con1) return get_layers(c, k.buf[k.cur ^ 1], con1);
+  return EFDC_OK;
+}
+
+int efdc_get_dry_state(efdc_ctx_t *c, int32_t *lmaskdry, int32_t *iscdry, int32_t *ncordry) {
+  if (!c) return EFDC_ERR_ARG;
+  if (lmaskdry)
+    HIP_TRY(c, hipMemcpyAsync(lmaskdry, c->lmaskdry, sizeof(int) * c->LCM, hipMemcpyDeviceToHost,
+                              c->stream));
+  if (iscdry)
+    HIP_TRY(c, hipMemcpyAsync(iscdry, c->iscdry, sizeof(int) * c->LCM, hipMemcpyDeviceToHost,
+                              c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (ncordry) *ncordry = c->ncordry;
   return EFDC_OK;
 }
 

@@ -35,7 +35,7 @@ struct CgState {
   double rpcg, papcg, alpha, beta, rsq, rsqm, bnorm2, ccmnm;
   double rsq64, bnorm2_in, rz_in;  // mixed mode: outer residual, inner rhs
   int iter, done, status, symok;   // symok: coefficients verified symmetric
-  int negdep, outer, inner_iter, pad;
+  int negdep, outer, inner_iter, drychg;  // drychg: a cell dried in this pass
 };
 
 // Arrival counters of the grid reductions: one site per reduction kernel,
@@ -143,6 +143,10 @@ struct efdc_ctx {
   double *tide_dev = nullptr;  // [cos(M)..., sin(M)...] for the current call
   double *tide_host = nullptr; // pinned
   int *lopenbcdry = nullptr;   // (LCM)
+  // drying (ISDRY > 0): ISCDRY, LMASKDRY, SUB1/SVB1, re-solves of the step
+  int *iscdry = nullptr, *lmaskdry = nullptr;
+  double *sub1 = nullptr, *svb1 = nullptr;
+  int ncordry = 0;
   unsigned char *pbcflag = nullptr;   // (LCM) 1 on pressure-BC cells
   unsigned char *lobflag = nullptr;   // (LCM) bit 0 LOBCS, bit 1 LOBCS2
   int *lobcs = nullptr;        // LOBCS order S, W, E, N (0-based)

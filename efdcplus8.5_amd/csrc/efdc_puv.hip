@@ -63,7 +63,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_explicit(
     HPtr h, const int *__restrict__ lw, const int *__restrict__ ls,
     const int *__restrict__ le, const int *__restrict__ ln, const T *hpk,
     T *h1pk, int64_t pitch, int KC, int nwet, double dt, double G,
-    double RITB1, double SNLT) {
+    double RITB1, double SNLT, int dry) {
   int tile;
   const int i = cell_index(nwet, tile);
   if (i > nwet) return;
@@ -85,6 +85,10 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_explicit(
       HA(SVB)[i] * DELT * HA(DYIV)[i] *
           (HA(DXYV)[i] * (HA(TSY)[i] - RITB1 * HA(TBY)[i]) - HA(FCAYE)[i] +
            HA(FPGYE)[i] - SNLT * HA(FYE)[i]);
+  if (dry) {  // faces reopened after the explicit terms, :246-253
+    HA(SUB)[i] = HA(SUBO)[i];
+    HA(SVB)[i] = HA(SVBO)[i];
+  }
   HA(P1)[i] = P[i];
   for (int k = 0; k < KC; ++k) h1pk[i + k * pitch] = hpk[i + k * pitch];
   const double C1 = 0.5 * G;
@@ -924,6 +928,99 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_post3(
   if (HP[i] <= 0.) st->negdep = 1;
 }
 
+// ---------------------------------------------------------------------
+// Drying, ISDRY > 0 (0 < ISDRY < 98 for the cell drying and re-solve).
+// ---------------------------------------------------------------------
+// Step start, EFDC/calpuv2c.f90:169-177: ISCDRY = 0, SUB1/SVB1 = SUB/SVB.
+__global__ void k_dry_begin(HPtr h, int *iscdry, double *sub1, double *svb1, int nwet) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nwet) return;
+  iscdry[i] = 0;
+  sub1[i] = HA(SUB)[i];
+  svb1[i] = HA(SVB)[i];
+}
+
+// LOPENBCDRY = .FALSE. on the open-boundary cells, :153-158.
+__global__ void k_dry_obc_reset(PbcDev s, int *lopenbcdry) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < s.n) lopenbcdry[s.L[t]] = 0;
+}
+
+// Open-boundary cells that dry, :497-530: the depth is reset and the cell
+// and its faces are closed for the rest of the step.  Each thread writes
+// its own cell and zeros faces it shares with its east / north
+// neighbours (the same value from every writer).
+__global__ void k_dry_obc(HPtr h, PbcDev s, const int *le, const int *ln, int *lopenbcdry,
+                          int *iscdry, double dt, double G, double HDRY) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= s.n) return;
+  const int i = s.L[t];
+  double *HP = HA(HP);
+  const double hp = HP[i], h1p = HA(H1P)[i];
+  const bool dry = lopenbcdry[i] != 0;
+  if (!((dry || (h1p < 0.9 * HDRY && hp < h1p) || hp < 0.) && iscdry[i] == 0)) return;
+  double hn;
+  if (hp <= 0. && dry) hn = 0.2 * HDRY;
+  else hn = fmin(fmax(h1p, 0.1 * HDRY), 0.9 * HDRY);
+  HP[i] = hn;
+  iscdry[i] = 1;
+  const int e = le[i], n = ln[i];
+  HA(SUB)[i] = 0.;
+  HA(SVB)[i] = 0.;
+  HA(SUB)[e] = 0.;
+  HA(SVB)[n] = 0.;
+  HA(UHDYE)[i] = 0.;
+  HA(UHDYE)[e] = 0.;
+  HA(VHDXE)[i] = 0.;
+  HA(VHDXE)[n] = 0.;
+  lopenbcdry[i] = 1;
+  const double DELTI = 1. / dt;
+  HA(CC)[i] = DELTI * HA(DXYP)[i];
+  const double pv = (hn + HA(BELV)[i]) * G;
+  HA(P)[i] = pv;
+  HA(FP1)[i] = DELTI * HA(DXYP)[i] * pv;
+}
+
+// Cells that dry, :574-602: HP <= HDRY closes the four faces of the cell;
+// a cell that becomes dry raises st->drychg (ICORDRY = 1: solve again).
+__global__ void k_dry_cells(HPtr h, const int *le, const int *ln, int *iscdry, int nwet,
+                            double HDRY, CgState *st) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nwet) return;
+  if (!(HA(HP)[i] <= HDRY)) return;
+  if (iscdry[i] == 0) {
+    iscdry[i] = 1;
+    atomicOr(&st->drychg, 1);
+  }
+  const int e = le[i], n = ln[i];
+  HA(SUB)[i] = 0.;
+  HA(SVB)[i] = 0.;
+  HA(SUB)[e] = 0.;
+  HA(SVB)[n] = 0.;
+}
+
+__global__ void k_dry_flag_reset(CgState *st) { st->drychg = 0; }
+
+// Transport mask, :916-939: a cell is left out of the transport when
+// HP < HDRY and each of its four faces was closed at the step start and
+// is closed now.
+__global__ void k_dry_mask(HPtr h, const int *le, const int *ln, const double *sub1,
+                           const double *svb1, int *lmask, int nwet, double HDRY) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nwet) return;
+  int m = 1;
+  if (HA(HP)[i] < HDRY) {
+    const int e = le[i], n = ln[i];
+    const double *SUB = HA(SUB), *SVB = HA(SVB);
+    const int iue = sub1[i] < 0.5 && SUB[i] < 0.5;
+    const int iuw = sub1[e] < 0.5 && SUB[e] < 0.5;
+    const int ivs = svb1[i] < 0.5 && SVB[i] < 0.5;
+    const int ivn = svb1[n] < 0.5 && SVB[n] < 0.5;
+    if (iue + iuw + ivs + ivn == 4) m = 0;
+  }
+  lmask[i] = m;
+}
+
 CgArgs cg_args(efdc_ctx *c) {
   CgArgs a;
   a.ccc = c->dH[EFDC_CCC];
@@ -1148,68 +1245,118 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
   rot.tri[2] = c->dH[EFDC_H2P];
   rot.lc0 = lc0;
   HPtr h = hptr(c);
+  const bool dry = c->prm.ISDRY > 0;
+  if (dry && c->nseg > 1) {
+    c->err = "drying (ISDRY > 0) with several basins is not built";
+    return EFDC_ERR_ARG;
+  }
   k_step_begin<<<1, 1, 0, c->stream>>>(c->st, rot);
+  if (dry) {
+    k_dry_begin<<<(nwet + 255) / 256, 256, 0, c->stream>>>(h, c->iscdry, c->sub1, c->svb1,
+                                                           nwet);
+    for (int q = 0; q < 4; ++q)
+      if (c->pbc[q].n > 0)
+        k_dry_obc_reset<<<(c->pbc[q].n + 255) / 256, 256, 0, c->stream>>>(c->pbc[q],
+                                                                           c->lopenbcdry);
+  }
   if (c->store == EFDC_STORE_FP64)
     TIMED(c, TK_PUV_EXPLICIT, k_puv_explicit<double><<<nb, EFDC_BLOCK, 0, c->stream>>>(
         h, c->dLW, c->dLS, c->dLE, c->dLN, (const double *)c->dK[EFDC_HPK - EFDC_HPK],
         (double *)c->dK[EFDC_H1PK - EFDC_HPK], c->pitch, c->KC, nwet, dt, G,
-        c->prm.RITB1, c->prm.SNLT));
+        c->prm.RITB1, c->prm.SNLT, dry ? 1 : 0));
   else
     TIMED(c, TK_PUV_EXPLICIT, k_puv_explicit<float><<<nb, EFDC_BLOCK, 0, c->stream>>>(
         h, c->dLW, c->dLS, c->dLE, c->dLN, (const float *)c->dK[EFDC_HPK - EFDC_HPK],
         (float *)c->dK[EFDC_H1PK - EFDC_HPK], c->pitch, c->KC, nwet, dt, G,
-        c->prm.RITB1, c->prm.SNLT));
-  TIMED(c, TK_PUV_ASSEMBLE, k_puv_assemble<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->pbcflag,
-                                                   nwet, dt, G, c->part, c->st,
-                                                   c->tickets));
-  if (c->npbc_total > 0) {
-    // tidal phase on the host in FP64 like SETOPENBC (:22-32)
-    const double PI2 = 2. * 3.1415926535898;  // EFDC/aaefdc.f90:650-651
-    const double TN = dt * (double)step->n + c->prm.TCON_TBEGIN;
-    for (int M = 0; M < c->mtide; ++M) {
-      double TM = std::fmod(TN, c->tcp_host[M]);
-      TM = PI2 * TM / c->tcp_host[M];
-      c->tide_host[M] = std::cos(TM);
-      c->tide_host[c->mtide + M] = std::sin(TM);
-    }
-    if (c->mtide > 0)
-      HIP_TRY(c, hipMemcpyAsync(c->tide_dev, c->tide_host,
-                                sizeof(double) * 2 * c->mtide,
-                                hipMemcpyHostToDevice, c->stream));
-    PbcArgs b;
-    for (int s = 0; s < 4; ++s) b.side[s] = c->pbc[s];
-    b.tide = c->tide_dev;
-    b.mtide = c->mtide;
-    b.lopenbcdry = c->lopenbcdry;
-    int ntot = 0;
-    for (int q = 0; q < 4; ++q) {
-      if (c->pbc[q].n == 0) continue;
-      ntot += c->pbc[q].n;
-      const int nbs = c->pbc[q].serial ? 1 : (c->pbc[q].n + EFDC_BLOCK - 1) / EFDC_BLOCK;
-      TIMED(c, TK_PUV_BC, k_puv_bc_side<<<nbs, EFDC_BLOCK, 0, c->stream>>>(
-          h, b, c->pbc[q], q, c->dLW, c->dLE, c->dLS, c->dLN, dt, G, c->prm.HDRY));
-    }
-    TIMED(c, TK_PUV_BC, k_puv_bc_min<<<(ntot + EFDC_BLOCK - 1) / EFDC_BLOCK, EFDC_BLOCK, 0,
-                                       c->stream>>>(h, b, ntot, c->dLW, c->dLE, c->dLS,
-                                                    c->dLN, c->part, c->st, c->tickets));
-  }
-  const bool segs = c->nseg > 1;
-  if (segs) {
-    int s = seg_min_run(c);
-    if (s) return s;
-  }
-  TIMED(c, TK_PUV_SCALE, k_puv_scale<<<nb, EFDC_BLOCK, 0, c->stream>>>(
-      h, c->dLW, c->dLE, c->dLN, nwet, lc0, c->st, segs ? c->d_segmin : nullptr,
-      segs ? c->d_segid : nullptr));
+        c->prm.RITB1, c->prm.SNLT, dry ? 1 : 0));
+  c->ncordry = 0;
   bool ran_mixed = false;
-  int s = cg_dispatch(c, stats, /*defer=*/true, &ran_mixed);
-  if (s != EFDC_OK && s != EFDC_ERR_CG_MAXITER) return s;
-  int cgs = s;
-  if (adj_recording(c) && (s = adj_record_ps(c))) return s;
-  TIMED(c, TK_PUV_POST1, k_puv_post1<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, dt));
-  TIMED(c, TK_PUV_POST2, k_puv_post2<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->pbcflag,
-                                                nwet, dt, G));
+  int cgs = EFDC_OK;
+  for (;;) {
+    // label 1000 of EFDC/calpuv2c.f90:302: the drying loop re-enters here
+    TIMED(c, TK_PUV_ASSEMBLE, k_puv_assemble<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->pbcflag,
+                                                     nwet, dt, G, c->part, c->st,
+                                                     c->tickets));
+    if (c->npbc_total > 0) {
+      // tidal phase on the host in FP64 like SETOPENBC (:22-32)
+      const double PI2 = 2. * 3.1415926535898;  // EFDC/aaefdc.f90:650-651
+      const double TN = dt * (double)step->n + c->prm.TCON_TBEGIN;
+      for (int M = 0; M < c->mtide; ++M) {
+        double TM = std::fmod(TN, c->tcp_host[M]);
+        TM = PI2 * TM / c->tcp_host[M];
+        c->tide_host[M] = std::cos(TM);
+        c->tide_host[c->mtide + M] = std::sin(TM);
+      }
+      if (c->mtide > 0)
+        HIP_TRY(c, hipMemcpyAsync(c->tide_dev, c->tide_host,
+                                  sizeof(double) * 2 * c->mtide,
+                                  hipMemcpyHostToDevice, c->stream));
+      PbcArgs b;
+      for (int s = 0; s < 4; ++s) b.side[s] = c->pbc[s];
+      b.tide = c->tide_dev;
+      b.mtide = c->mtide;
+      b.lopenbcdry = c->lopenbcdry;
+      int ntot = 0;
+      for (int q = 0; q < 4; ++q) {
+        if (c->pbc[q].n == 0) continue;
+        ntot += c->pbc[q].n;
+        const int nbs = c->pbc[q].serial ? 1 : (c->pbc[q].n + EFDC_BLOCK - 1) / EFDC_BLOCK;
+        TIMED(c, TK_PUV_BC, k_puv_bc_side<<<nbs, EFDC_BLOCK, 0, c->stream>>>(
+            h, b, c->pbc[q], q, c->dLW, c->dLE, c->dLS, c->dLN, dt, G, c->prm.HDRY));
+      }
+      TIMED(c, TK_PUV_BC, k_puv_bc_min<<<(ntot + EFDC_BLOCK - 1) / EFDC_BLOCK, EFDC_BLOCK, 0,
+                                         c->stream>>>(h, b, ntot, c->dLW, c->dLE, c->dLS,
+                                                      c->dLN, c->part, c->st, c->tickets));
+    }
+    const bool segs = c->nseg > 1;
+    if (segs) {
+      int s = seg_min_run(c);
+      if (s) return s;
+    }
+    TIMED(c, TK_PUV_SCALE, k_puv_scale<<<nb, EFDC_BLOCK, 0, c->stream>>>(
+        h, c->dLW, c->dLE, c->dLN, nwet, lc0, c->st, segs ? c->d_segmin : nullptr,
+        segs ? c->d_segid : nullptr));
+    int s = cg_dispatch(c, stats, /*defer=*/true, &ran_mixed);
+    if (s != EFDC_OK && s != EFDC_ERR_CG_MAXITER) return s;
+    cgs = s;
+    if (adj_recording(c) && (s = adj_record_ps(c))) return s;
+    TIMED(c, TK_PUV_POST1, k_puv_post1<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, dt));
+    TIMED(c, TK_PUV_POST2, k_puv_post2<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->pbcflag,
+                                                  nwet, dt, G));
+    if (!dry) break;
+    // open-boundary drying (:497-530), then the cell drying and its flag
+    // (:574-602); P already is G (HP + BELV) (:559-569, k_puv_post2)
+    for (int q = 0; q < 4; ++q)
+      if (c->pbc[q].n > 0)
+        k_dry_obc<<<(c->pbc[q].n + 255) / 256, 256, 0, c->stream>>>(
+            h, c->pbc[q], c->dLE, c->dLN, c->lopenbcdry, c->iscdry, dt, G, c->prm.HDRY);
+    const bool cells = c->prm.ISDRY < 98;
+    if (cells) {
+      k_dry_flag_reset<<<1, 1, 0, c->stream>>>(c->st);
+      k_dry_cells<<<(nwet + 255) / 256, 256, 0, c->stream>>>(h, c->dLE, c->dLN, c->iscdry, nwet,
+                                                             c->prm.HDRY, c->st);
+    }
+    HIP_TRY(c, hipGetLastError());
+    if ((s = read_state(c))) return s;
+    if (c->pcg_deferred) {
+      c->pcg_deferred = false;
+      if (*c->pcg_abort_host) {
+        c->err = "persistent CG: an exchange between workgroups timed out";
+        return EFDC_ERR_HIP;
+      }
+      cgs = cg_finish(c, ran_mixed, stats);
+    }
+    if (!cells || !c->st_host->drychg || cgs != EFDC_OK) break;
+    if (++c->ncordry >= 500) {
+      c->err = "NCORDRY > 500: the drying loop did not settle (EFDC/calpuv2c.f90:596-601)";
+      return EFDC_ERR_STATE;
+    }
+  }  // drying loop
   TIMED(c, TK_PUV_POST3, k_puv_post3<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, c->st));
+  if (dry)
+    k_dry_mask<<<(nwet + 255) / 256, 256, 0, c->stream>>>(h, c->dLE, c->dLN, c->sub1, c->svb1,
+                                                          c->lmaskdry, nwet, c->prm.HDRY);
+  int s;
   if ((s = read_state(c))) return s;
   if (c->pcg_deferred) {
     c->pcg_deferred = false;

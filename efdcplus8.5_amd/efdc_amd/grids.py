@@ -37,7 +37,7 @@ H_KEYS = (
     "P P1 HP H1P H2P HU HV HUI HVI HPI H1U H1V H1UI H1VI "
     "UHDYE VHDXE UHDY1E VHDX1E UHE VHE "
     "TSX TSY TBX TBY FCAXE FCAYE FPGXE FPGYE FXE FYE QSUME RCX RCY "
-    "FUHDYE FVHDXE FP FP1 CC CS CW CE CN CCC CCS CCW CCE CCN CCCI FPTMP"
+    "FUHDYE FVHDXE FP FP1 CC CS CW CE CN CCC CCS CCW CCE CCN CCCI FPTMP SUBO SVBO"
 ).split()
 K_KEYS = ("HPK H1PK HPKI UHDY2 VHDX2 CDZKK CDZKMK AB QSUM U2 V2 SUB3D SVB3D "
           "DZC").split()
@@ -184,6 +184,8 @@ def fill_geometry(g: Grid, h, k, w, dx: float, dzck: np.ndarray):
     sub, svb = _face_masks(g)
     h["SUB"][:] = sub
     h["SVB"][:] = svb
+    h["SUBO"][:] = sub
+    h["SVBO"][:] = svb
     for key, val in dict(DXYP=dx * dx, DXYIP=1.0 / (dx * dx), DXYU=dx * dx,
                          DXYV=dx * dx, DXIU=1.0 / dx, DYIU=1.0 / dx,
                          DXIV=1.0 / dx, DYIV=1.0 / dx).items():
@@ -399,6 +401,37 @@ def tidal_basin(nx=128, ny=64, kc=5, *, seed=1000, dx=100.0, dt=5.0,
                 consts=consts, open_bc=open_bc, conc_bc=conc_bc,
                 sources=sources, dt=dt, manning_n=nman,
                 meta=dict(seed=seed, dx=dx))
+
+
+def drying_basin(nx=40, ny=12, kc=2, *, seed=1005, amp=0.8, period=3000.0,
+                 deep=2.0, high=0.3, hdry=0.2, dt=4.0, shoal=1.0, name=None,
+                 **kw) -> Case:
+    """Tidal basin for the drying cases (ISDRY > 0): the bottom rises
+    linearly from `deep` m below the datum at the open (west) boundary to
+    `high` m above it at the east end, so the east cells start dry
+    (HP = 0.5*hdry, their faces close on the first step), the ebb of the
+    tide (amplitude `amp`) dries a band west of them and the flood wets it
+    again.  A Gaussian shoal of height `shoal` m (radius 2 cells) at 45% of
+    the length, mid-width, dries at low water with wet cells on all four
+    sides.  Otherwise as tidal_basin."""
+    case = tidal_basin(nx, ny, kc, seed=seed, amp=amp, period=period, dt=dt,
+                       name=name or f"drying-{nx}x{ny}x{kc}", **kw)
+    g, h, k = case.grid, case.h, case.k
+    L = np.arange(2, g.LA + 1) - 1
+    x = (g.IL[L] - 2) / max(nx - 1, 1)
+    belv = -deep + (deep + high) * x
+    r2 = (g.IL[L] - 2 - 0.45 * (nx - 1)) ** 2 + (g.JL[L] - 2 - 0.5 * (ny - 1)) ** 2
+    belv = belv + shoal * np.exp(-r2 / 8.0)
+    hp = np.maximum(-belv, 0.5 * hdry)
+    h["BELV"][L] = belv
+    for key in ("HP", "H1P", "H2P"):
+        h[key][L] = hp
+    h["P"][L] = G * (hp + belv)
+    h["P1"][L] = h["P"][L]
+    update_depth_arrays(g, h)
+    layer_depths(g, h, k)
+    k["H1PK"][:] = k["HPK"]
+    return case
 
 
 def ensemble_member(b: int, kc: int = 8, seed0: int = 1003) -> Case:

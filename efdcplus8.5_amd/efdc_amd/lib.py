@@ -46,7 +46,8 @@ class Params(C.Structure):
                 ("cg_rel_tol", C.c_double), ("cg_mode", C.c_int32),
                 ("store", C.c_int32), ("mixed_inner_rtol", C.c_double),
                 ("mixed_max_outer", C.c_int32), ("cg_kernel", C.c_int32),
-                ("mixed_precond_degree", C.c_int32), ("tran_shape", C.c_int32)]
+                ("mixed_precond_degree", C.c_int32), ("tran_shape", C.c_int32),
+                ("ISDRY", C.c_int32)]
 
 
 class PbcSide(C.Structure):
@@ -154,6 +155,7 @@ def load(path: Optional[str] = None):
         "efdc_set_constituent": ([P, C.c_int32, PD, PD], C.c_int),
         "efdc_get_constituent": ([P, C.c_int32, PD, PD], C.c_int),
         "efdc_get_constituent_at": ([P, C.c_int32, C.c_int32, PI, PI, PD], C.c_int),
+        "efdc_get_dry_state": ([P, PI, PI, PI], C.c_int),
         "efdc_congrad": ([P, C.POINTER(CgStats)], C.c_int),
         "efdc_calpuv2c": ([P, C.POINTER(Step), C.POINTER(CgStats)], C.c_int),
         "efdc_calconc": ([P, C.POINTER(Step)], C.c_int),
@@ -362,6 +364,17 @@ class Context:
         self._check(self.lib.efdc_get_constituent_at(self.h, slot, len(L), _pi(L), _pi(K),
                                                      _pd(out)))
         return out
+
+    def get_dry_state(self):
+        """(lmaskdry, iscdry, ncordry) after the last calpuv2c: per-cell
+        arrays in L order (index L-1), 1 = wet in lmaskdry, 1 = dried this
+        step in iscdry; ncordry = re-solves of that step."""
+        n = self.grid.LCM
+        m = np.zeros(n, np.int32)
+        d = np.zeros(n, np.int32)
+        k = np.zeros(1, np.int32)
+        self._check(self.lib.efdc_get_dry_state(self.h, _pi(m), _pi(d), _pi(k)))
+        return m, d, int(k[0])
 
     # ------------------------------------------------------------ calls
     def congrad(self) -> dict:

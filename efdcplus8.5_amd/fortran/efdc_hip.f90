@@ -33,7 +33,8 @@ module efdc_hip
       EFDC_RCX = 44, EFDC_RCY = 45, EFDC_FUHDYE = 46, EFDC_FVHDXE = 47, &
       EFDC_FP = 48, EFDC_FP1 = 49, EFDC_CC = 50, EFDC_CS = 51, EFDC_CW = 52, &
       EFDC_CE = 53, EFDC_CN = 54, EFDC_CCC = 55, EFDC_CCS = 56, EFDC_CCW = 57, &
-      EFDC_CCE = 58, EFDC_CCN = 59, EFDC_CCCI = 60, EFDC_FPTMP = 61
+      EFDC_CCE = 58, EFDC_CCN = 59, EFDC_CCCI = 60, EFDC_FPTMP = 61, &
+      EFDC_SUBO = 62, EFDC_SVBO = 63
   ! (LCM,KCM) arrays
   integer(c_int32_t), parameter :: EFDC_HPK = 64, EFDC_H1PK = 65, EFDC_HPKI = 66, &
       EFDC_UHDY2 = 67, EFDC_VHDX2 = 68, EFDC_CDZKK = 69, EFDC_CDZKMK = 70, &
@@ -53,7 +54,7 @@ module efdc_hip
     real(c_double) :: RSQM, cg_rel_tol
     integer(c_int32_t) :: cg_mode, store
     real(c_double) :: mixed_inner_rtol
-    integer(c_int32_t) :: mixed_max_outer, cg_kernel, mixed_precond_degree, tran_shape
+    integer(c_int32_t) :: mixed_max_outer, cg_kernel, mixed_precond_degree, tran_shape, ISDRY
   end type
 
   type, bind(C) :: efdc_pbc_side_t
@@ -215,6 +216,14 @@ module efdc_hip
       integer(c_int32_t), value :: slot, n
       integer(c_int32_t), intent(in) :: L(*), K(*)
       real(c_double), intent(out) :: out(*)
+    end function
+
+    integer(c_int) function efdc_get_dry_state(ctx, lmaskdry, iscdry, ncordry) &
+        bind(C, name="efdc_get_dry_state")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: lmaskdry(*), iscdry(*)
+      integer(c_int32_t), intent(out) :: ncordry
     end function
 
     integer(c_int) function efdc_congrad(ctx, stats) bind(C, name="efdc_congrad")

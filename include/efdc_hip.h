@@ -110,13 +110,17 @@ typedef struct {
                               -1, -2 or -4 = one constituent per thread at a
                               time, that many waves per workgroup.  Every
                               shape gives the same bits.                      */
+  int32_t ISDRY;           /* wetting and drying: 0 off; 1..97 the drying of
+                              EFDC/calpuv2c.f90:497-530,574-602 with the
+                              re-solve loop and the transport mask LMASKDRY
+                              (:916-939); 98, 99 (rewetting) are refused     */
 } efdc_params_t;
 
 /* Fill *p with the values EFDC uses by default: G=9.81, RITB1=1, SNLT=0,
    HDRY=0.2, TCON_TBEGIN=0, ITERM=200, RSQM=1e-9 (the sample decks,
    SampleModels/Lake_Washington/efdc.inp:92), cg_rel_tol=0, FP64 CG, FP64
    storage, mixed_inner_rtol=1e-5, mixed_max_outer=30, cg_kernel=AUTO,
-   mixed_precond_degree=0, tran_shape=0. */
+   mixed_precond_degree=0, tran_shape=0, ISDRY=0. */
 void efdc_params_default(efdc_params_t *p);
 
 /* ------------------------------------------------------------- array ids */
@@ -137,6 +141,10 @@ typedef enum {
   EFDC_FUHDYE, EFDC_FVHDXE, EFDC_FP, EFDC_FP1,
   EFDC_CC, EFDC_CS, EFDC_CW, EFDC_CE, EFDC_CN,
   EFDC_CCC, EFDC_CCS, EFDC_CCW, EFDC_CCE, EFDC_CCN, EFDC_CCCI, EFDC_FPTMP,
+  /* (LCM) face switches of the full wet grid, SUBO/SVBO: with ISDRY > 0,
+     SUB/SVB are reset to them at each step start after FUHDYE/FVHDXE
+     (EFDC/calpuv2c.f90:246-253), which reopens the faces that dried */
+  EFDC_SUBO, EFDC_SVBO,
   EFDC_NUM_H,
   /* (LCM,KCM) arrays (EFDC/varalloc.f90:54,118-120,611,773,803,1959-1996) */
   EFDC_HPK = 64, EFDC_H1PK, EFDC_HPKI, EFDC_UHDY2, EFDC_VHDX2,
@@ -294,6 +302,12 @@ int efdc_get_constituent(efdc_ctx_t *ctx, int32_t slot, double *con,
    reads a few cells every few steps (no counterpart in the reference). */
 int efdc_get_constituent_at(efdc_ctx_t *ctx, int32_t slot, int32_t n, const int32_t *L,
                             const int32_t *K, double *out);
+
+/* Drying state after the last efdc_calpuv2c (ISDRY > 0): LMASKDRY (1 wet,
+   0 dry: left out of the transport, EFDC/caltran.f90 LKWET loops) and
+   ISCDRY per cell (L order, LCM entries; either may be NULL) and the
+   number of re-solves NCORDRY of that step. */
+int efdc_get_dry_state(efdc_ctx_t *ctx, int32_t *lmaskdry, int32_t *iscdry, int32_t *ncordry);
 
 /* CONGRAD (EFDC/congrad.f90:1-191) on the scaled system already held in
    CCC, CCS, CCW, CCE, CCN, CCCI, FPTMP, starting from and updating P. */

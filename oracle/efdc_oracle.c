@@ -287,7 +287,10 @@ static void orc_setopenbc2(orc_ctx *o) {
 /* CALPUV2C assembly: EFDC/calpuv2c.f90:128-136,164-398 (ISDRY=0,         */
 /* BSC=0, ISITB=0, ISVEG=0, MDCHH=0, ISGWIE=0)                             */
 /* ===================================================================== */
-int orc_assemble(orc_ctx *o, double dt, int64_t n, double *ccmnm_out) {
+/* The part of CALPUV2C before label 1000: the explicit stage, the
+   time-level shifts and FP1 (EFDC/calpuv2c.f90:214-290). */
+static int explicit_stage(orc_ctx *o, double dt, int64_t n) {
+  (void)n;
   const int LA = o->LA, KC = o->KC;
   const double G = o->G;
   const double DELT = dt, DELTD2 = 0.5 * dt, DELTI = 1. / DELT; /* :128-131 */
@@ -301,10 +304,7 @@ int orc_assemble(orc_ctx *o, double dt, int64_t n, double *ccmnm_out) {
          *FCAYE = A(EFDC_FCAYE), *FPGXE = A(EFDC_FPGXE),
          *FPGYE = A(EFDC_FPGYE), *FXE = A(EFDC_FXE), *FYE = A(EFDC_FYE),
          *FUHDYE = A(EFDC_FUHDYE), *FVHDXE = A(EFDC_FVHDXE),
-         *DXYP = A(EFDC_DXYP), *FP1 = A(EFDC_FP1), *FP = A(EFDC_FP),
-         *QSUME = A(EFDC_QSUME), *RCX = A(EFDC_RCX), *RCY = A(EFDC_RCY),
-         *CC = A(EFDC_CC), *CS = A(EFDC_CS), *CW = A(EFDC_CW),
-         *CE = A(EFDC_CE), *CN = A(EFDC_CN);
+         *DXYP = A(EFDC_DXYP), *FP1 = A(EFDC_FP1);
   const double RITB1 = o->RITB1, SNLT = o->SNLT;
 
   /* explicit stage, :214-220 */
@@ -319,6 +319,14 @@ int orc_assemble(orc_ctx *o, double dt, int64_t n, double *ccmnm_out) {
                 SVB[i] * DELT * DYIV[i] *
                     (DXYV[i] * (TSY[i] - RITB1 * TBY[i]) - FCAYE[i] + FPGYE[i] -
                      SNLT * FYE[i]);
+  }
+  /* faces reopened with drying on, :246-253 */
+  if (o->ISDRY > 0) {
+    const double *SUBO = A(EFDC_SUBO), *SVBO = A(EFDC_SVBO);
+    for (int L = 2; L <= LA; ++L) {
+      SUB[HX(L)] = SUBO[HX(L)];
+      SVB[HX(L)] = SVBO[HX(L)];
+    }
   }
   /* time-level shifts, :256-274 */
   {
@@ -352,6 +360,24 @@ int orc_assemble(orc_ctx *o, double dt, int64_t n, double *ccmnm_out) {
                C1 * (UHDYE[e] - UHDYE[i] + VHDXE[nn] - VHDXE[i]);
     }
   }
+  return EFDC_OK;
+}
+
+/* The part of CALPUV2C after label 1000 (EFDC/calpuv2c.f90:302): FP and
+   the coefficients (:315-342), SETOPENBC, the minimum CCMNM, SETOPENBC2 and
+   the scaling (:349-398).  The drying loop re-enters here. */
+static double system_stage(orc_ctx *o, double dt, int64_t n) {
+  const int LA = o->LA;
+  const double G = o->G;
+  const double DELT = dt, DELTD2 = 0.5 * dt, DELTI = 1. / DELT;
+  const int nopt = noptimal(o);
+  double *SUB = A(EFDC_SUB), *SVB = A(EFDC_SVB), *HRUO = A(EFDC_HRUO),
+         *HRVO = A(EFDC_HRVO), *HU = A(EFDC_HU), *HV = A(EFDC_HV),
+         *FUHDYE = A(EFDC_FUHDYE), *FVHDXE = A(EFDC_FVHDXE),
+         *DXYP = A(EFDC_DXYP), *FP1 = A(EFDC_FP1), *FP = A(EFDC_FP),
+         *QSUME = A(EFDC_QSUME), *RCX = A(EFDC_RCX), *RCY = A(EFDC_RCY),
+         *CC = A(EFDC_CC), *CS = A(EFDC_CS), *CW = A(EFDC_CW),
+         *CE = A(EFDC_CE), *CN = A(EFDC_CN);
   /* FP, CS, CW, CE, CN, CC: :315-342 */
 #pragma omp parallel for num_threads(nopt) schedule(static)
   for (int L = 2; L <= LA; ++L) {
@@ -402,6 +428,12 @@ int orc_assemble(orc_ctx *o, double dt, int64_t n, double *ccmnm_out) {
       CCCI[i] = 1. / CCC[i];
     }
   }
+  return ccmnm;
+}
+
+int orc_assemble(orc_ctx *o, double dt, int64_t n, double *ccmnm_out) {
+  explicit_stage(o, dt, n);
+  const double ccmnm = system_stage(o, dt, n);
   if (ccmnm_out) *ccmnm_out = ccmnm;
   return EFDC_OK;
 }
@@ -409,15 +441,73 @@ int orc_assemble(orc_ctx *o, double dt, int64_t n, double *ccmnm_out) {
 /* ===================================================================== */
 /* CALPUV2C: assembly + CONGRAD + post-solve updates                       */
 /* ===================================================================== */
+/* Open-boundary cells that dry, ISDRY > 0: EFDC/calpuv2c.f90:497-530 (the
+   depth is reset, the cell and its faces are closed for the rest of the
+   step).  Run after the open-boundary depths are set. */
+static void open_bc_drying(orc_ctx *o, double dt) {
+  const double G = o->G, HDRY = o->HDRY, DELTI = 1. / dt;
+  double *HP = A(EFDC_HP), *H1P = A(EFDC_H1P), *P = A(EFDC_P), *BELV = A(EFDC_BELV),
+         *SUB = A(EFDC_SUB), *SVB = A(EFDC_SVB), *UHDYE = A(EFDC_UHDYE),
+         *VHDXE = A(EFDC_VHDXE), *CC = A(EFDC_CC), *FP1 = A(EFDC_FP1),
+         *DXYP = A(EFDC_DXYP);
+  static const int order[4] = {2, 0, 1, 3};
+  for (int q = 0; q < 4; ++q) {
+    const orc_pbc_side *sd = &o->pbc[order[q]];
+    for (int LL = 0; LL < sd->n; ++LL) {
+      const size_t i = HX(sd->L[LL]);
+      const int dry = o->lopenbcdry[i] != 0;
+      if (!((dry || (H1P[i] < 0.9 * HDRY && HP[i] < H1P[i]) || HP[i] < 0.) &&
+            o->iscdry[i] == 0))
+        continue;
+      if (HP[i] <= 0. && dry) HP[i] = 0.2 * HDRY;
+      else HP[i] = dmin(dmax(H1P[i], 0.1 * HDRY), 0.9 * HDRY);
+      o->iscdry[i] = 1;
+      const size_t e = HX(o->LEC[i]), nn = HX(o->LNC[i]);
+      SUB[i] = 0.;
+      SVB[i] = 0.;
+      SUB[e] = 0.;
+      SVB[nn] = 0.;
+      UHDYE[i] = 0.;
+      UHDYE[e] = 0.;
+      VHDXE[i] = 0.;
+      VHDXE[nn] = 0.;
+      o->lopenbcdry[i] = 1;
+      CC[i] = DELTI * DXYP[i];
+      P[i] = (HP[i] + BELV[i]) * G;
+      FP1[i] = DELTI * DXYP[i] * P[i];
+    }
+  }
+}
+
+/* Cells that dry, 0 < ISDRY < 98: EFDC/calpuv2c.f90:574-602.  Returns 1
+   when a cell became dry in this pass (ICORDRY = 1: solve again). */
+static int cell_drying(orc_ctx *o) {
+  const double HDRY = o->HDRY;
+  double *HP = A(EFDC_HP), *SUB = A(EFDC_SUB), *SVB = A(EFDC_SVB);
+  int icordry = 0;
+  for (int L = 2; L <= o->LA; ++L) {
+    const size_t i = HX(L);
+    if (HP[i] <= HDRY) {
+      if (o->iscdry[i] == 0) {
+        o->iscdry[i] = 1;
+        icordry = 1;
+      }
+      const size_t e = HX(o->LEC[i]), nn = HX(o->LNC[i]);
+      SUB[i] = 0.;
+      SVB[i] = 0.;
+      SUB[e] = 0.;
+      SVB[nn] = 0.;
+    }
+  }
+  return icordry;
+}
+
 int orc_calpuv2c(orc_ctx *o, double dt, int64_t n, orc_cg_out *out) {
   const int LA = o->LA;
   const double G = o->G, GI = 1. / o->G; /* EFDC/aaefdc.f90:648 */
   const double DELTD2 = 0.5 * dt;
   const int nopt = noptimal(o);
-  double ccmnm = 0.;
-  orc_assemble(o, dt, n, &ccmnm);
-  int st = orc_congrad(o, out); /* :409 */
-  out->ccmnm = ccmnm;
+  const int isdry = o->ISDRY;
   double *UHDYE = A(EFDC_UHDYE), *VHDXE = A(EFDC_VHDXE), *P = A(EFDC_P),
          *SUB = A(EFDC_SUB), *SVB = A(EFDC_SVB), *HRUO = A(EFDC_HRUO),
          *HRVO = A(EFDC_HRVO), *HU = A(EFDC_HU), *HV = A(EFDC_HV),
@@ -429,41 +519,77 @@ int orc_calpuv2c(orc_ctx *o, double dt, int64_t n, orc_cg_out *out) {
          *BELV = A(EFDC_BELV), *DXYP = A(EFDC_DXYP), *DXYU = A(EFDC_DXYU),
          *DXYV = A(EFDC_DXYV), *HPI = A(EFDC_HPI), *HUI = A(EFDC_HUI),
          *HVI = A(EFDC_HVI);
-  /* face transports, :423-434 */
-#pragma omp parallel for num_threads(nopt) schedule(static)
-  for (int L = 2; L <= LA; ++L) {
-    const size_t i = HX(L), s = HX(o->LSC[i]), w = HX(o->LWC[i]);
-    UHDYE[i] = SUB[i] * (FUHDYE[i] - DELTD2 * HRUO[i] * RCX[i] * HU[i] * (P[i] - P[w]));
-    VHDXE[i] = SVB[i] * (FVHDXE[i] - DELTD2 * HRVO[i] * RCY[i] * HV[i] * (P[i] - P[s]));
-    UHE[i] = UHDYE[i] * DYIU[i];
-    VHE[i] = VHDXE[i] * DXIV[i];
-  }
-  /* depths, :472-477 */
-#pragma omp parallel for num_threads(nopt) schedule(static)
-  for (int L = 2; L <= LA; ++L) {
-    const size_t i = HX(L), e = HX(o->LEC[i]), nn = HX(o->LNC[i]);
-    HP[i] = H1P[i] + DELTD2 * DXYIP[i] *
-                         (2. * QSUME[i] -
-                          (UHDYE[e] + UHDY1E[e] - UHDYE[i] - UHDY1E[i] +
-                           VHDXE[nn] + VHDX1E[nn] - VHDXE[i] - VHDX1E[i]));
-  }
-  /* open boundary depths, :490-492 (LOBCS order S, W, E, N,
-     EFDC/setbcs.f90:884-938) */
-  {
-    static const int order[4] = {2, 0, 1, 3};
-    for (int q = 0; q < 4; ++q) {
-      const orc_pbc_side *s = &o->pbc[order[q]];
-      for (int LL = 0; LL < s->n; ++LL) {
-        const size_t i = HX(s->L[LL]);
-        HP[i] = GI * P[i] - BELV[i];
-      }
+  /* step start with drying on: :153-179 */
+  if (isdry > 0) {
+    for (int s = 0; s < 4; ++s)
+      for (int LL = 0; LL < o->pbc[s].n; ++LL) o->lopenbcdry[HX(o->pbc[s].L[LL])] = 0;
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L);
+      o->iscdry[i] = 0;
+      o->sub1[i] = SUB[i];
+      o->svb1[i] = SVB[i];
     }
   }
-  /* final updates, :896-914 (FSGZUDXYPI = 0.5/DXYU, :110-117) */
+  explicit_stage(o, dt, n);
+  int st = EFDC_OK;
+  o->ncordry = 0;
+  for (;;) {
+    /* label 1000, :302 */
+    const double ccmnm = system_stage(o, dt, n);
+    st = orc_congrad(o, out); /* :409 */
+    out->ccmnm = ccmnm;
+    /* face transports, :423-434 */
 #pragma omp parallel for num_threads(nopt) schedule(static)
-  for (int L = 2; L <= LA; ++L) {
-    const size_t i = HX(L);
-    P[i] = G * (HP[i] + BELV[i]);
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L), s = HX(o->LSC[i]), w = HX(o->LWC[i]);
+      UHDYE[i] = SUB[i] * (FUHDYE[i] - DELTD2 * HRUO[i] * RCX[i] * HU[i] * (P[i] - P[w]));
+      VHDXE[i] = SVB[i] * (FVHDXE[i] - DELTD2 * HRVO[i] * RCY[i] * HV[i] * (P[i] - P[s]));
+      UHE[i] = UHDYE[i] * DYIU[i];
+      VHE[i] = VHDXE[i] * DXIV[i];
+    }
+    /* depths, :472-477 */
+#pragma omp parallel for num_threads(nopt) schedule(static)
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L), e = HX(o->LEC[i]), nn = HX(o->LNC[i]);
+      HP[i] = H1P[i] + DELTD2 * DXYIP[i] *
+                           (2. * QSUME[i] -
+                            (UHDYE[e] + UHDY1E[e] - UHDYE[i] - UHDY1E[i] +
+                             VHDXE[nn] + VHDX1E[nn] - VHDXE[i] - VHDX1E[i]));
+    }
+    /* open boundary depths, :490-492 (LOBCS order S, W, E, N,
+       EFDC/setbcs.f90:884-938) */
+    {
+      static const int order[4] = {2, 0, 1, 3};
+      for (int q = 0; q < 4; ++q) {
+        const orc_pbc_side *sd = &o->pbc[order[q]];
+        for (int LL = 0; LL < sd->n; ++LL) {
+          const size_t i = HX(sd->L[LL]);
+          HP[i] = GI * P[i] - BELV[i];
+        }
+      }
+    }
+    if (isdry > 0) {
+      open_bc_drying(o, dt);
+      /* intermediate update of P, :559-569 (the next pass starts from it) */
+      for (int L = 2; L <= LA; ++L) {
+        const size_t i = HX(L);
+        P[i] = G * (HP[i] + BELV[i]);
+      }
+    }
+    if (isdry > 0 && isdry < 98 && cell_drying(o) && st == EFDC_OK) {
+      if (++o->ncordry < 500) continue; /* GOTO 1000 */
+      st = EFDC_ERR_STATE;               /* '*** NCORDRY > 500' */
+    }
+    break;
+  }
+  /* final updates, :896-914 (FSGZUDXYPI = 0.5/DXYU, :110-117); with
+     drying on, P was updated in the loop (:896-900) */
+  if (isdry == 0) {
+#pragma omp parallel for num_threads(nopt) schedule(static)
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L);
+      P[i] = G * (HP[i] + BELV[i]);
+    }
   }
 #pragma omp parallel for num_threads(nopt) schedule(static)
   for (int L = 2; L <= LA; ++L) {
@@ -477,6 +603,22 @@ int orc_calpuv2c(orc_ctx *o, double dt, int64_t n, orc_cg_out *out) {
     HPI[i] = 1. / HP[i];
     HUI[i] = 1. / HU[i];
     HVI[i] = 1. / HV[i];
+  }
+  /* transport mask of dry cells, :916-939: dry when HP < HDRY and all four
+     faces were closed at the step start and are closed now */
+  if (isdry > 0) {
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L), e = HX(o->LEC[i]), nn = HX(o->LNC[i]);
+      int m = 1;
+      if (HP[i] < o->HDRY) {
+        const int iue = o->sub1[i] < 0.5 && SUB[i] < 0.5;
+        const int iuw = o->sub1[e] < 0.5 && SUB[e] < 0.5;
+        const int ivs = o->svb1[i] < 0.5 && SVB[i] < 0.5;
+        const int ivn = o->svb1[nn] < 0.5 && SVB[nn] < 0.5;
+        if (iue + iuw + ivs + ivn == 4) m = 0;
+      }
+      o->lmaskdry[i] = m;
+    }
   }
   /* NEGDEP, EFDC/negdep.f90:34-45 */
   if (st == EFDC_OK) {

@@ -62,6 +62,12 @@ typedef struct {
   /* constituents */
   int32_t nconst;
   orc_const c[ORC_MAXC];
+  /* wetting and drying (ISDRY = 0: off; 1..97: EFDC/calpuv2c.f90:574-602) */
+  int32_t ISDRY;
+  int32_t *iscdry;       /* (LCM) ISCDRY */
+  int32_t *lmaskdry;     /* (LCM) LMASKDRY, 1 = wet */
+  double *sub1, *svb1;   /* (LCM) SUB1, SVB1: face switches at the step start */
+  int32_t ncordry;       /* re-solves of the last CALPUV2C */
 } orc_ctx;
 
 typedef struct {

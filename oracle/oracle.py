@@ -60,7 +60,9 @@ class OrcCtx(C.Structure):
                 ("cbc", CbcSide * 4), ("wtci", PD),
                 ("nqsij", C.c_int32), ("lqs", PI), ("qss", PD),
                 ("qsercell", PD), ("cqs", PD), ("csert_q", PD),
-                ("nconst", C.c_int32), ("c", Const * MAXC)]
+                ("nconst", C.c_int32), ("c", Const * MAXC),
+                ("ISDRY", C.c_int32), ("iscdry", PI), ("lmaskdry", PI),
+                ("sub1", PD), ("svb1", PD), ("ncordry", C.c_int32)]
 
 
 class CgOut(C.Structure):
@@ -119,7 +121,7 @@ class Oracle:
     """Binds a Case to the C oracle; arrays are shared, not copied."""
 
     def __init__(self, case, *, rsqm=1e-9, iterm=200, nthreads=1, g=9.81,
-                 ritb1=1.0, snlt=0.0, hdry=0.2, tcon_tbegin=0.0, rel_tol=0.0):
+                 ritb1=1.0, snlt=0.0, hdry=0.2, tcon_tbegin=0.0, rel_tol=0.0, isdry=0):
         self.case = case
         g_ = case.grid
         o = OrcCtx()
@@ -136,8 +138,15 @@ class Oracle:
         o.G, o.RITB1, o.SNLT, o.HDRY = g, ritb1, snlt, hdry
         o.TCON_TBEGIN, o.RSQM, o.ITERM, o.nthreads = tcon_tbegin, rsqm, iterm, nthreads
         o.CG_REL_TOL = rel_tol
+        o.ISDRY = isdry
         self.lopenbcdry = np.zeros(g_.LCM, np.int32)
         o.lopenbcdry = _pi(self.lopenbcdry)
+        self.iscdry = np.zeros(g_.LCM, np.int32)
+        self.lmaskdry = np.ones(g_.LCM, np.int32)
+        self.sub1 = np.zeros(g_.LCM)
+        self.svb1 = np.zeros(g_.LCM)
+        o.iscdry, o.lmaskdry = _pi(self.iscdry), _pi(self.lmaskdry)
+        o.sub1, o.svb1 = _pd(self.sub1), _pd(self.svb1)
         obc = case.open_bc
         if obc is not None:
             mt = int(obc["mtide"])

@@ -1,0 +1,175 @@
+"""Pins of the oracle's drying (ISDRY 1..97, EFDC/calpuv2c.f90:147-179,
+246-253, 302, 490-602, 916-939) by what the scheme fixes:
+
+* with nothing near HDRY, ISDRY=1 gives the same bits as ISDRY=0 (the
+  re-solve loop, the face resets and the intermediate P update change
+  nothing);
+* after every step, each cell with HP <= HDRY has its four faces closed
+  and is flagged in ISCDRY, closed faces carry no transport, and the
+  transport mask only leaves out cells with HP < HDRY;
+* a closed basin sloshing against a sloping bank keeps its volume to
+  round-off while every step opens the faces of its dry cells and closes
+  them again in the re-solve;
+* open-boundary cells that dry (shallow boundary at low water) are held
+  at a depth in [0.1, 0.9]*HDRY with their faces closed and P on that
+  depth, and return to the tide when it rises;
+* the re-solve loop ends at a fixed point: solving the step again with
+  the faces the loop ended with gives the same depths and closes no
+  further face.
+"""
+import numpy as np
+import pytest
+
+from conftest import G, wet_index
+
+HDRY = 0.2
+
+
+def _closed():
+    """The drying basin without its open boundary and source, its surface
+    tilted 0.6 m up at the west end and 0.6 m down at the east end."""
+    from efdc_amd import grids
+    c = grids.drying_basin()
+    g, h, k = c.grid, c.h, c.k
+    c.open_bc, c.conc_bc, c.sources = None, None, None
+    h["QSUME"][:] = 0.0
+    k["QSUM"][:] = 0.0
+    L = wet_index(c)
+    x = (g.IL[L] - 2) / (g.IL[L].max() - 2)
+    eta = 0.6 * (1.0 - 2.0 * x)
+    hp = np.maximum(eta - h["BELV"][L], 0.5 * HDRY)
+    for key in ("HP", "H1P", "H2P"):
+        h[key][L] = hp
+    h["P"][L] = G * (hp + h["BELV"][L])
+    h["P1"][L] = h["P"][L]
+    grids.update_depth_arrays(g, h)
+    grids.layer_depths(g, h, k)
+    k["H1PK"][:] = k["HPK"]
+    return c
+
+
+def _check_faces(case, o):
+    g, h = case.grid, case.h
+    L = wet_index(case)
+    e = g.maps["LEC"][L] - 1
+    n = g.maps["LNC"][L] - 1
+    hp = h["HP"][L]
+    low = hp <= HDRY
+    assert np.all(o.iscdry[L][low] == 1)
+    for arr, idx in (("SUB", L), ("SUB", e), ("SVB", L), ("SVB", n)):
+        assert np.all(h[arr][idx][low] == 0.0), arr
+    closed_u = h["SUB"][L] == 0.0
+    closed_v = h["SVB"][L] == 0.0
+    assert np.all(h["UHDYE"][L][closed_u] == 0.0)
+    assert np.all(h["VHDXE"][L][closed_v] == 0.0)
+    masked = o.lmaskdry[L] == 0
+    assert np.all(hp[masked] < HDRY)
+    return int(low.sum()), int(masked.sum())
+
+
+def test_no_op_when_nothing_dries(oracle_mod):
+    from efdc_amd import grids
+    case = grids.tidal_basin(24, 12, 3, seed=51)
+    a, b = case.copy(), case.copy()
+    oa = oracle_mod.Oracle(a, rsqm=1e-24, iterm=2000, isdry=0)
+    ob = oracle_mod.Oracle(b, rsqm=1e-24, iterm=2000, isdry=1, hdry=HDRY)
+    for n in range(1, 31):
+        oa.step(n)
+        ob.step(n)
+        assert ob.o.ncordry == 0
+    for key in ("P", "HP", "UHDYE", "VHDXE", "HU", "HV", "SUB", "SVB"):
+        assert np.array_equal(a.h[key], b.h[key]), key
+    for ca, cb in zip(a.consts, b.consts):
+        assert np.array_equal(ca["con"], cb["con"])
+    L = wet_index(case)
+    assert np.all(ob.iscdry[L] == 0) and np.all(ob.lmaskdry[L] == 1)
+
+
+def test_faces_close_and_reopen_over_a_tide(oracle_mod):
+    from efdc_amd import grids
+    case = grids.drying_basin()
+    o = oracle_mod.Oracle(case, rsqm=1e-24, iterm=2000, isdry=1, hdry=HDRY)
+    L = wet_index(case)
+    start_dry = case.h["HP"][L] <= HDRY
+    ever_dry = np.zeros(len(L), bool)
+    rewet = np.zeros(len(L), bool)
+    resolves = []
+    for n in range(1, 801):
+        st = o.step(n)
+        assert st["status"] == 0, n
+        resolves.append(o.o.ncordry)
+        _check_faces(case, o)
+        masked = o.lmaskdry[L] == 0
+        rewet |= ever_dry & ~masked
+        ever_dry |= masked
+        assert np.all(case.h["P"][L] == G * (case.h["HP"][L] + case.h["BELV"][L]))
+    # the ebb dries cells that started wet, the flood wets some of them again
+    assert np.any(ever_dry & ~start_dry)
+    assert np.any(rewet & ~start_dry)
+    assert max(resolves) >= 2 and min(resolves) >= 1
+
+
+def test_sloshing_basin_keeps_its_volume(oracle_mod):
+    case = _closed()
+    o = oracle_mod.Oracle(case, rsqm=1e-26, iterm=4000, isdry=1, hdry=HDRY)
+    L = wet_index(case)
+    v0 = float(np.sum(case.h["DXYP"][L] * case.h["HP"][L]))
+    for n in range(1, 401):
+        st = o.step(n)
+        assert st["status"] == 0, n
+        assert o.o.ncordry >= 1
+        ndry, _ = _check_faces(case, o)
+        assert ndry > 0
+        v = float(np.sum(case.h["DXYP"][L] * case.h["HP"][L]))
+        assert abs(v - v0) <= 1e-11 * v0, (n, v - v0)
+
+
+def test_open_boundary_cells_dry_and_return_to_the_tide(oracle_mod):
+    from efdc_amd import grids
+    case = grids.drying_basin(deep=0.6)
+    o = oracle_mod.Oracle(case, rsqm=1e-24, iterm=2000, isdry=1, hdry=HDRY)
+    g, h = case.grid, case.h
+    lb = case.open_bc["side"][0]["L"] - 1
+    e = g.maps["LEC"][lb] - 1
+    n_ = g.maps["LNC"][lb] - 1
+    dried = released = False
+    for n in range(1, 601):
+        st = o.step(n)
+        assert st["status"] == 0, n
+        _check_faces(case, o)
+        d = o.lopenbcdry[lb] == 1
+        if d.any():
+            dried = True
+            hp = h["HP"][lb][d]
+            assert np.all((hp >= 0.1 * HDRY) & (hp <= 0.9 * HDRY))
+            assert np.all(o.iscdry[lb][d] == 1)
+            for arr, idx in (("SUB", lb), ("SUB", e), ("SVB", lb), ("SVB", n_)):
+                assert np.all(h[arr][idx][d] == 0.0), arr
+            assert np.all(h["UHDYE"][e][d] == 0.0)
+        elif dried:
+            released = True
+    assert dried and released
+
+
+@pytest.mark.parametrize("nsteps", [150, 420])
+def test_resolve_loop_ends_at_a_fixed_point(oracle_mod, nsteps):
+    """Re-solving the last step from its start state with the faces the
+    drying loop ended with (as SUBO/SVBO) reproduces the depths."""
+    from efdc_amd import grids
+    case = grids.drying_basin()
+    o = oracle_mod.Oracle(case, rsqm=1e-28, iterm=4000, isdry=1, hdry=HDRY)
+    for n in range(1, nsteps):
+        o.step(n)
+    o.closure_pre()
+    pre = case.copy()
+    o.calpuv2c(n=nsteps)
+    assert o.o.ncordry >= 1
+    again = pre.copy()
+    again.h["SUBO"][:] = case.h["SUB"]
+    again.h["SVBO"][:] = case.h["SVB"]
+    o2 = oracle_mod.Oracle(again, rsqm=1e-28, iterm=4000, isdry=1, hdry=HDRY)
+    o2.calpuv2c(n=nsteps)
+    L = wet_index(case)
+    assert np.array_equal(again.h["SUB"], case.h["SUB"])
+    assert np.array_equal(again.h["SVB"], case.h["SVB"])
+    assert np.max(np.abs(again.h["HP"][L] - case.h["HP"][L])) <= 1e-10
