@@ -60,6 +60,7 @@ struct TranArgs {
   const double *wtci;
   double *bc_con1;
   int vd;  // 0: stop after the open boundaries (anti-diffusion follows)
+  const int *lmask;  // LMASKDRY (0: column left out), or null without drying
 };
 
 template <typename T>
@@ -72,6 +73,21 @@ __device__ __forceinline__ double ld(const T *p, int64_t j) {
 __device__ __forceinline__ double dmx(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double dmn(double a, double b) { return a < b ? a : b; }
 
+// A column whose cell is left out of the transport (LMASKDRY false,
+// EFDC/calpuv2c.f90:916-939: every CALCONC/CALTRAN loop runs over the wet
+// list LKWET, EFDC/caltran.f90:95-300,324-441, EFDC/calconc.f90:89-121,
+// 259-473) keeps CON for constituents c0, c0+cstep, .. < cend.  The faces
+// of such a cell were closed at both time levels, so UHDY2/VHDX2 are 0
+// there and the fluxes its wet neighbours see are 0 with or without it.
+template <typename T>
+__device__ __forceinline__ bool col_keep(const TranArgs<T> &a, int i, int c0, int cstep,
+                                         int cend) {
+  if (!a.lmask || a.lmask[i] != 0) return false;
+  for (int c = c0; c < cend; c += cstep)
+    for (int k = 0; k < a.KC; ++k) a.nw[c][i + (int64_t)k * a.pitch] = a.old[c][i + (int64_t)k * a.pitch];
+  return true;
+}
+
 template <typename T>
 __global__ __launch_bounds__(TBLOCK) void k_tran_upwind_vdiff(TranArgs<T> a) {
   extern __shared__ unsigned char smem[];
@@ -79,6 +95,7 @@ __global__ __launch_bounds__(TBLOCK) void k_tran_upwind_vdiff(TranArgs<T> a) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int i = 1 + tile * TB + (int)threadIdx.x;
   if (i > a.nwet) return;
+  if (col_keep(a, i, 0, 1, a.nc)) return;
   const int tid = threadIdx.x;
   const int KC = a.KC, nc = a.nc;
   const int64_t pitch = a.pitch;
@@ -344,6 +361,7 @@ __device__ __forceinline__ void tran_col_body(const TranArgs<T> &a, double *vcus
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int i = 1 + tile * 64 + (int)threadIdx.x;
   if (i > a.nwet) return;
+  if (col_keep(a, i, (int)threadIdx.y, (int)blockDim.y, a.nc)) return;
   const int KC = a.KC, nc = a.nc;
   // 32-bit element offsets (the host checks that every layer array has
   // fewer than 2^31 elements), so the loads use a scalar base plus a
@@ -739,6 +757,7 @@ __device__ __forceinline__ void tran_colm_body(const TranArgs<T> &a, double *vcu
   const int i = tile * 64 + (int)threadIdx.x;
   if (i < 1 || i > a.nwet) return;
   const int c0 = __builtin_amdgcn_readfirstlane((int)threadIdx.y) * NCW;
+  if (col_keep(a, i, c0, 1, c0 + NCW < a.nc ? c0 + NCW : a.nc)) return;
   const int src0 = a.srcrec[i];
   const int rec = a.bcrec ? a.bcrec[i] : -1;
   if (__ballot(src0 >= 0 || rec >= 0) != 0)
@@ -773,9 +792,10 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_settle(T *__restrict__ sed,
                                                        const T *__restrict__ hpki,
                                                        int64_t pitch, int KC, int nwet,
                                                        double dtsed, double ws, double wsetmp,
-                                                       double wese) {
+                                                       double wese, const int *lmask) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 256-aligned tiles
   if (i < 1 || i > nwet) return;
+  if (lmask && lmask[i] == 0) return;  // CALSED runs over LWET (calsed.f90:232-256)
   double sedf = 0.0;  // SEDF(L,KC) = 0
   if (KC > 1) {
     {
@@ -812,6 +832,7 @@ __global__ void k_tran_bc_con1(TranArgs<T> a, const int *bccell, int nrec) {
   if (t >= tot) return;
   const int rec = t / a.KC, k = t % a.KC;
   const int i = bccell[rec];
+  if (a.lmask && a.lmask[i] == 0) return;  // skipped, EFDC/caltran.f90:324-441
   for (int c = 0; c < a.nc; ++c)
     ((T *)a.old[c])[i + (int64_t)k * a.pitch] =
         (T)a.bc_con1[((size_t)rec * a.KC + k) * a.nc + c];
@@ -1311,7 +1332,7 @@ int finish_tran(efdc_ctx *c, TranArgs<T> &a, const efdc_step_t *step) {
     if (!(d.wset > 0.0)) continue;
     TIMED(c, TK_SETTLE, k_settle<T><<<(a.nwet + 256) / 256, 256, 0, c->stream>>>(
         a.nw[j], a.hpki, a.pitch, a.KC, a.nwet, step->dt, d.wset, d.probdep * d.wset,
-        d.wese));
+        d.wese, a.lmask));
   }
   if (c->nbcrec > 0) {
     const int tot = c->nbcrec * c->KC;
@@ -1337,6 +1358,10 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
       c->err = "FCT needs KC >= 2 (EFDC/caltran.f90:637-640 reads layer 2)";
       return EFDC_ERR_ARG;
     }
+  }
+  if (any_adac && c->prm.ISDRY > 0) {
+    c->err = "anti-diffusion (ISADAC) with drying (ISDRY > 0) is not built";
+    return EFDC_ERR_ARG;
   }
   TranArgs<T> a;
   std::memset(&a, 0, sizeof(a));
@@ -1395,6 +1420,7 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
   a.wtci = c->wtci;
   a.bc_con1 = c->bc_con1;
   a.vd = any_adac ? 0 : 1;
+  a.lmask = c->prm.ISDRY > 0 ? c->lmaskdry : nullptr;
 
   if (adj_recording(c)) {
     const int s = adj_record_c0(c);

@@ -712,10 +712,15 @@ static void orc_caltran(orc_ctx *o, int c, double dt, int64_t niter,
         s->FWUU[ZX(L, K)] = W2[WX(L, K)] * CON1[KX(L, KUPW)];
       }
 
-  /* 2TL update, :190-199 */
+  /* 2TL update, :190-199, over the wet list LKWET (:95-300): a cell left
+     out by LMASKDRY keeps CON = CON1.  Its faces were closed at both time
+     levels, so its UHDY2/VHDX2 and the fluxes above are 0 (the reference
+     zeroes them for the dry cells, EFDC/calconc.f90:127-150). */
+  const int32_t *wet = o->ISDRY > 0 ? o->lmaskdry : NULL;
   for (int K = 1; K <= KC; ++K)
     for (int L = 2; L <= LA; ++L) {
       const size_t i = HX(L);
+      if (wet && !wet[i]) continue;
       s->CD[KX(L, K)] =
           CON1[KX(L, K)] * H1PK[KX(L, K)] +
           DDELT * ((s->FQC[KX(L, K)] + s->FUHUD[KX(L, K)] -
@@ -730,7 +735,7 @@ static void orc_caltran(orc_ctx *o, int c, double dt, int64_t niter,
         s->CON2[KX(L, K)] = dmax(CON1[KX(L, K)], 0.0);
   for (int K = 1; K <= KC; ++K) /* :240-245 */
     for (int L = 2; L <= LA; ++L)
-      CON[KX(L, K)] = s->CD[KX(L, K)] * HPKI[KX(L, K)];
+      if (!wet || wet[HX(L)]) CON[KX(L, K)] = s->CD[KX(L, K)] * HPKI[KX(L, K)];
 
   /* open boundary concentrations S, W, E, N: :316-471 (ISTL=2, ISUD=1) */
   for (int side = 0; side < 4; ++side) {
@@ -739,6 +744,7 @@ static void orc_caltran(orc_ctx *o, int c, double dt, int64_t niter,
       for (int LL = 0; LL < b->n; ++LL) {
         const int L = b->L[LL];
         const size_t i = HX(L);
+        if (wet && !wet[i]) continue; /* :324,367,404,441 */
         const size_t m = ((size_t)LL * KC + K - 1) * nc + c;
         const double cb1 = b->cb[((size_t)LL * 2 + 0) * nc + c];
         const double cb2 = b->cb[((size_t)LL * 2 + 1) * nc + c];
@@ -1041,6 +1047,7 @@ int orc_vertical_diffusion(orc_ctx *o, double dt) {
 #pragma omp for schedule(dynamic, 256)
     for (int L = 2; L <= LA; ++L) {
       const size_t i = HX(L);
+      if (o->ISDRY > 0 && !o->lmaskdry[i]) continue; /* LKWET loops */
       double EEB[64], CCLB[64];
       /* bottom layer, :265-272 (KSZ=1) */
       {
@@ -1098,6 +1105,7 @@ void orc_settle(orc_ctx *o, double dtsed) {
     double *SED = o->c[c].con;
     const double WSETMP = o->c[c].probdep * ws, WESE = o->c[c].wese;
     for (int L = 2; L <= o->LA; ++L) {
+      if (o->ISDRY > 0 && !o->lmaskdry[HX(L)]) continue; /* LWET loops */
       double sedf = 0.0; /* SEDF(L,KC) = 0 */
       if (KC > 1) {
         {
@@ -1126,6 +1134,8 @@ void orc_settle(orc_ctx *o, double dtsed) {
 
 int orc_calconc(orc_ctx *o, double dt, int64_t niter) {
   const int nc = o->nconst;
+  for (int c = 0; c < nc; ++c)
+    if (o->ISDRY > 0 && o->c[c].isadac > 0) return EFDC_ERR_ARG; /* not restated */
   const int nt = o->nthreads > 0 ? o->nthreads : 1;
   /* one constituent per thread, EFDC/calconc.f90:159-229 */
 #pragma omp parallel for num_threads(nt < nc ? nt : (nc > 0 ? nc : 1)) schedule(static, 1)
@@ -1144,11 +1154,13 @@ int orc_calconc(orc_ctx *o, double dt, int64_t niter) {
     if (rk == 1000.0) {
       const double DAGE = dt / 86400.;
       for (int K = 1; K <= o->KC; ++K)
-        for (int L = 2; L <= o->LA; ++L) C[KX(L, K)] = C[KX(L, K)] + DAGE;
+        for (int L = 2; L <= o->LA; ++L)
+          if (o->ISDRY == 0 || o->lmaskdry[HX(L)]) C[KX(L, K)] = C[KX(L, K)] + DAGE;
     } else if (rk != 0.0) {
       const double f = rk < 0.0 ? exp(-rk * dt) : 1. / (1. + dt * rk);
       for (int K = 1; K <= o->KC; ++K)
-        for (int L = 2; L <= o->LA; ++L) C[KX(L, K)] = f * C[KX(L, K)];
+        for (int L = 2; L <= o->LA; ++L)
+          if (o->ISDRY == 0 || o->lmaskdry[HX(L)]) C[KX(L, K)] = f * C[KX(L, K)];
     }
   }
   orc_settle(o, dt);

@@ -1,9 +1,12 @@
 """Drying (ISDRY 1..97) on the GPU against the oracle: the re-solve loop
 (EFDC/calpuv2c.f90:302,574-602), open-boundary drying (:497-530), the
 face reopening at the step start (:246-253) and the transport mask
-LMASKDRY (:916-939).  The cell flags and the face switches must agree
-exactly and the number of re-solves NCORDRY step by step; P within the
-FP64 CG's round-off."""
+LMASKDRY (:916-939) with the transport leaving the masked columns out.
+The cell flags and the face switches must agree exactly and the number of
+re-solves NCORDRY step by step; P within the FP64 CG's round-off, the
+concentrations within 1e-9 relative (they inherit the CG's round-off
+through the flows), and every column shape of the transport kernels must
+give the same bits."""
 import numpy as np
 import pytest
 
@@ -14,12 +17,29 @@ pytestmark = pytest.mark.gpu
 HDRY = 0.2
 
 
+def _stratified(case):
+    """Layered salinity with settling, dye 1 everywhere with decay."""
+    kc = case.grid.KC
+    L = wet_index(case)
+    sal, dye = case.consts
+    for K in range(kc):
+        sal["con"][K, L] = 10.0 + 5.0 * K
+    sal["con1"][:] = sal["con"]
+    sal["wset"], sal["probdep"], sal["wese"] = 1e-4, 0.5, 1e-6
+    dye["con"][:kc, L] = 1.0
+    dye["con1"][:] = dye["con"]
+    dye["rkdye"] = 1e-4
+    return case
+
+
 @pytest.mark.parametrize("cg_kernel", [0, 1])
 @pytest.mark.parametrize("deep", [2.0, 0.6])
 def test_drying_matches_oracle(oracle_mod, deep, cg_kernel):
     from efdc_amd import grids
     from efdc_amd.lib import Context
-    case = grids.drying_basin(deep=deep)
+    case = grids.drying_basin(kc=3, deep=deep)
+    if cg_kernel == 1:
+        case = _stratified(case)
     ref = case.copy()
     o = oracle_mod.Oracle(ref, rsqm=1e-24, iterm=2000, isdry=1, hdry=HDRY)
     ctx = Context(case.grid, RSQM=1e-24, ITERM=2000, ISDRY=1, HDRY=HDRY,
@@ -45,6 +65,10 @@ def test_drying_matches_oracle(oracle_mod, deep, cg_kernel):
                 assert np.array_equal(ctx.get(key)[L], ref.h[key][L]), (n, key)
             hp = ctx.get("HP")[L]
             assert np.max(np.abs(hp - ref.h["HP"][L])) <= 1e-9, n
+            for j, c in enumerate(ref.consts):
+                con, _ = ctx.get_constituent(j)
+                scale = max(1e-30, np.max(np.abs(c["con"][:kc])))
+                assert np.max(np.abs(con[:kc] - c["con"][:kc])) / scale <= 1e-9, (n, j)
     if deep == 2.0:
         assert max(seen) >= 2
     else:
@@ -52,8 +76,30 @@ def test_drying_matches_oracle(oracle_mod, deep, cg_kernel):
     for s, c in enumerate(ref.consts):
         con, _ = ctx.get_constituent(s)
         scale = max(1e-30, np.max(np.abs(c["con"][:kc])))
-        assert np.max(np.abs(con[:kc] - c["con"][:kc])) / scale <= 1e-8, s
+        assert np.max(np.abs(con[:kc] - c["con"][:kc])) / scale <= 1e-9, s
     ctx.close()
+
+
+def test_drying_transport_shapes_bitwise():
+    from efdc_amd import grids
+    from efdc_amd.lib import Context
+    case = _stratified(grids.drying_basin(kc=3, deep=0.6))
+    shapes = [1, 2, 17, 34, -1, -2]
+    ctxs = [Context(case.grid, RSQM=1e-24, ITERM=2000, ISDRY=1, HDRY=HDRY,
+                    tran_shape=q).upload_case(case) for q in shapes]
+    for n in range(1, 301):
+        for c in ctxs:
+            c.step(case.dt, n)
+    kc = case.grid.KC
+    masked = ctxs[0].get_dry_state()[0][wet_index(case)] == 0
+    assert masked.any()
+    for j in range(len(case.consts)):
+        a, a1 = ctxs[0].get_constituent(j)
+        for c in ctxs[1:]:
+            b, b1 = c.get_constituent(j)
+            assert np.array_equal(a[:kc], b[:kc]) and np.array_equal(a1[:kc], b1[:kc]), j
+    for c in ctxs:
+        c.close()
 
 
 def test_drying_is_refused_where_not_built():

@@ -13,6 +13,12 @@
 * open-boundary cells that dry (shallow boundary at low water) are held
   at a depth in [0.1, 0.9]*HDRY with their faces closed and P on that
   depth, and return to the tide when it rises;
+* the transport leaves the masked cells out (EFDC/caltran.f90:95-300 and
+  the CALCONC/CALSED loops run over the wet list): their concentrations
+  come through CALCONC unchanged to the bit (advection, sources, open
+  boundaries, vertical diffusion, dye decay and settling all skip them),
+  while a uniform concentration stays uniform in the wet cells next to
+  them and the closed basin keeps its constituent mass;
 * the re-solve loop ends at a fixed point: solving the step again with
   the faces the loop ended with gives the same depths and closes no
   further face.
@@ -173,3 +179,53 @@ def test_resolve_loop_ends_at_a_fixed_point(oracle_mod, nsteps):
     assert np.array_equal(again.h["SUB"], case.h["SUB"])
     assert np.array_equal(again.h["SVB"], case.h["SVB"])
     assert np.max(np.abs(again.h["HP"][L] - case.h["HP"][L])) <= 1e-10
+
+
+def test_masked_columns_keep_their_concentrations(oracle_mod):
+    from efdc_amd import grids
+    case = grids.drying_basin(kc=3, deep=0.6)
+    kc = case.grid.KC
+    L = wet_index(case)
+    sal, dye = case.consts
+    for K in range(kc):
+        sal["con"][K, L] = 10.0 + 5.0 * K
+    sal["con1"][:] = sal["con"]
+    sal["wset"], sal["probdep"], sal["wese"] = 1e-4, 0.5, 1e-6
+    dye["con"][:kc, L] = 1.0
+    dye["con1"][:] = dye["con"]
+    dye["rkdye"] = 1e-4
+    o = oracle_mod.Oracle(case, rsqm=1e-24, iterm=2000, isdry=1, hdry=HDRY)
+    lb = case.open_bc["side"][0]["L"] - 1
+    masked_any = masked_bc = False
+    for n in range(1, 401):
+        before = [c["con"][:kc].copy() for c in case.consts]
+        st = o.step(n)
+        assert st["status"] == 0, n
+        m = o.lmaskdry[L] == 0
+        masked_any |= bool(m.any())
+        masked_bc |= bool(np.any(o.lmaskdry[lb] == 0))
+        for c, b in zip(case.consts, before):
+            assert np.array_equal(c["con"][:kc, L[m]], b[:, L[m]]), n
+            # the wet columns did change (decay, settling, stratification)
+            assert not np.array_equal(c["con"][:kc, L[~m]], b[:, L[~m]])
+    assert masked_any and masked_bc
+
+
+def test_uniform_concentration_stays_uniform_while_drying(oracle_mod):
+    case = _closed()
+    kc = case.grid.KC
+    L = wet_index(case)
+    for c in case.consts:
+        c["con"][:kc, L] = 7.5
+        c["con1"][:] = c["con"]
+        c["rkdye"] = 0.0
+    o = oracle_mod.Oracle(case, rsqm=1e-26, iterm=4000, isdry=1, hdry=HDRY)
+    _, m0 = o.balance()
+    for n in range(1, 301):
+        o.step(n)
+        # masked from the second step on (faces closed at both ends of a step)
+        assert n == 1 or np.any(o.lmaskdry[L] == 0)
+    for c in case.consts:
+        assert np.max(np.abs(c["con"][:kc, L] - 7.5)) <= 1e-11 * 7.5
+    _, m1 = o.balance()
+    np.testing.assert_allclose(m1, m0, rtol=1e-11)
