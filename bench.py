@@ -213,6 +213,106 @@ def transport_bench(args):
     print(json.dumps(out))
 
 
+def drying_bench(args):
+    """Wetting and drying at scale (--workload drying): the tidal drying
+    basin (bottom rising from 2 m below the datum at the open west boundary
+    to 0.3 m above it at the east end, a shoal at 45% of the length) at
+    1024x512, KC=4, salinity and a dye source, ISDRY=1, HDRY=0.2 m,
+    dt = 4 s.  One step = the full hot path (closure, CALPUV2C with its
+    drying re-solves, each an FP64 multi-kernel CG to a relative residual of
+    1e-10, closure, CALCONC with the dry columns left out); value =
+    cell-steps/s.  (Without drying the same basin runs into negative depths
+    at its dry east end, so there is no ISDRY=0 run of it to compare with;
+    the kernel statistics show what the re-solves cost.)  The FP64 oracle
+    runs a bounded sample: the CPU baseline and the check of the dry flags
+    and depths step by step."""
+    import numpy as np
+    from efdc_amd import grids, lib
+
+    nx, ny, kc = args.dnx, args.dny, 4
+    case = grids.drying_basin(nx, ny, kc)
+    g = case.grid
+    L = np.arange(2, g.LA + 1) - 1
+    prm = dict(cg_rel_tol=1e-10, RSQM=0.0, ITERM=4000, cg_mode=lib.CG_FP64, cg_kernel=1,
+               HDRY=0.2)
+    ref_case = case.copy() if not args.no_cpu else None
+    ctx = lib.Context(g, ISDRY=1, **prm).upload_case(case)
+    cpu, check = None, None
+    n0 = 1
+    if ref_case is not None:
+        from oracle import oracle as orc
+        o = orc.Oracle(ref_case, rsqm=0.0, iterm=4000, nthreads=args.cpu_threads,
+                       rel_tol=1e-10, isdry=1, hdry=0.2)
+        t_cpu, done, same_flags = 0.0, 0, True
+        while done < 1 or (t_cpu < args.cpu_seconds and done < 40):
+            t0 = time.perf_counter()
+            st = o.step(done + 1)
+            t_cpu += time.perf_counter() - t0
+            if st["status"] != 0:
+                raise RuntimeError(f"oracle status {st}")
+            ctx.step(case.dt, done + 1)
+            m, d, k = ctx.get_dry_state()
+            same_flags &= bool(np.array_equal(m[L], o.lmaskdry[L]) and
+                               np.array_equal(d[L], o.iscdry[L]) and k == o.o.ncordry)
+            done += 1
+        cpu = dict(value=g.nwet * done / t_cpu, unit="cell-steps/s",
+                   cores=min(args.cpu_threads, 8), kind="port",
+                   sample=f"{done} full steps of the same {nx}x{ny}x{kc} drying basin "
+                          f"by the FP64 C restatement, {t_cpu:.1f} s")
+        check = dict(after_steps=done, same_dry_flags_every_step=same_flags,
+                     eta_err_m=float(np.max(np.abs(ctx.get("P")[L] - ref_case.h["P"][L])) / 9.81),
+                     hp_err_m=float(np.max(np.abs(ctx.get("HP")[L] - ref_case.h["HP"][L]))))
+        n0 = done + 1
+    for n in range(n0, n0 + args.warmup):
+        ctx.step(case.dt, n)
+    n0 += args.warmup
+    ncor = []
+
+    def run(c, k, count):
+        c.synchronize()
+        t0 = time.perf_counter()
+        for n in range(n0, n0 + k):
+            st = c.step(case.dt, n)
+            if st["status"] != 0:
+                raise RuntimeError(f"step status {st}")
+            if count:
+                ncor.append(c.get_dry_state(arrays=False)[2])
+        c.synchronize()
+        return time.perf_counter() - t0
+
+    t = run(ctx, args.steps, True)
+    m, _, _ = ctx.get_dry_state()
+    n0 += args.steps
+    ctx.timers(True)
+    for n in range(n0, n0 + args.steps):
+        ctx.step(case.dt, n)
+    tm = ctx.read_timers()
+    ctx.timers(False)
+    dom = max((k_ for k_ in tm if k_ in BYTES_PER_CELL), key=lambda k_: tm[k_][1])
+    cnt, ms = tm[dom]
+    avg_s = ms / cnt * 1e-3
+    ach = BYTES_PER_CELL[dom] * g.nwet / avg_s / 1e9
+    out = {
+        "metric": "cell-steps/s (full step with wetting and drying, ISDRY=1)",
+        "value": g.nwet * args.steps / t,
+        "unit": "cell-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"tidal drying basin {nx}x{ny}, KC={kc}, 2 constituents, "
+                               f"ISDRY=1, HDRY=0.2 m, dt={case.dt} s", "cells": g.nwet},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "avg_us": ms / cnt * 1e3, "bytes_per_launch": BYTES_PER_CELL[dom] * g.nwet},
+        "cpu_baseline": cpu,
+        "detail": {"resolves_per_step": float(np.mean(ncor)), "resolves_max": int(max(ncor)),
+                   "masked_cells": int(np.sum(m[L] == 0)), "check_vs_oracle": check,
+                   "kernels": {k: dict(launches=c_, avg_us=1e3 * m_ / c_)
+                               for k, (c_, m_) in tm.items()}},
+    }
+    print(json.dumps(out))
+
+
 def ensemble_bench(args):
     """Batched ensemble (--workload ensemble): 256 independent basins,
     (nx, ny) log-uniform in 64x64 .. 512x256, irregular lake masks, depths
@@ -527,7 +627,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("lake1024", "transport", "ensemble", "adjoint"),
+    ap.add_argument("--workload", choices=("lake1024", "transport", "ensemble", "adjoint",
+                                           "drying"),
                     default="lake1024",
                     help="lake1024: the external solve at scale (the headline); "
                          "transport: CALCONC at scale; ensemble: 256 basins in one "
@@ -542,6 +643,8 @@ def main():
     ap.add_argument("--tnx", type=int, default=2048)
     ap.add_argument("--tny", type=int, default=1024)
     ap.add_argument("--tkc", type=int, default=20)
+    ap.add_argument("--dnx", type=int, default=1024, help="drying: basin length")
+    ap.add_argument("--dny", type=int, default=512, help="drying: basin width")
     ap.add_argument("--cg-kernel", choices=("auto", "multi", "persistent"),
                     default="auto", help="launch strategy of the CG solve")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -565,6 +668,9 @@ def main():
         return
     if args.workload == "adjoint":
         adjoint_bench(args)
+        return
+    if args.workload == "drying":
+        drying_bench(args)
         return
 
     import numpy as np

@@ -932,8 +932,10 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_post3(
 // Drying, ISDRY > 0 (0 < ISDRY < 98 for the cell drying and re-solve).
 // ---------------------------------------------------------------------
 // Step start, EFDC/calpuv2c.f90:169-177: ISCDRY = 0, SUB1/SVB1 = SUB/SVB.
-__global__ void k_dry_begin(HPtr h, int *iscdry, double *sub1, double *svb1, int nwet) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(EFDC_BLOCK) void k_dry_begin(HPtr h, int *iscdry, double *sub1,
+                                                          double *svb1, int nwet) {
+  int tile;
+  const int i = cell_index(nwet, tile);
   if (i > nwet) return;
   iscdry[i] = 0;
   sub1[i] = HA(SUB)[i];
@@ -982,21 +984,32 @@ __global__ void k_dry_obc(HPtr h, PbcDev s, const int *le, const int *ln, int *l
 }
 
 // Cells that dry, :574-602: HP <= HDRY closes the four faces of the cell;
-// a cell that becomes dry raises st->drychg (ICORDRY = 1: solve again).
-__global__ void k_dry_cells(HPtr h, const int *le, const int *ln, int *iscdry, int nwet,
-                            double HDRY, CgState *st) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > nwet) return;
-  if (!(HA(HP)[i] <= HDRY)) return;
-  if (iscdry[i] == 0) {
-    iscdry[i] = 1;
-    atomicOr(&st->drychg, 1);
+// a cell that becomes dry raises st->drychg (ICORDRY = 1: solve again):
+// one lane per wave stores 1 unless the flag already reads 1 (a plain
+// store of the same value; atomics on the one address serialise the pass
+// when many cells dry at once).  Faces already closed are not written again.
+__global__ __launch_bounds__(EFDC_BLOCK) void k_dry_cells(HPtr h, const int *le, const int *ln,
+                                                          int *iscdry, int nwet, double HDRY,
+                                                          CgState *st) {
+  int tile;
+  const int i = cell_index(nwet, tile);
+  bool newly = false;
+  if (i <= nwet && HA(HP)[i] <= HDRY) {
+    if (iscdry[i] == 0) {
+      iscdry[i] = 1;
+      newly = true;
+    }
+    const int e = le[i], n = ln[i];
+    double *SUB = HA(SUB), *SVB = HA(SVB);
+    if (SUB[i] != 0.) SUB[i] = 0.;
+    if (SVB[i] != 0.) SVB[i] = 0.;
+    if (SUB[e] != 0.) SUB[e] = 0.;
+    if (SVB[n] != 0.) SVB[n] = 0.;
   }
-  const int e = le[i], n = ln[i];
-  HA(SUB)[i] = 0.;
-  HA(SVB)[i] = 0.;
-  HA(SUB)[e] = 0.;
-  HA(SVB)[n] = 0.;
+  const unsigned long long b = __ballot(newly);
+  if (b != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)b) - 1 &&
+      __hip_atomic_load(&st->drychg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    __hip_atomic_store(&st->drychg, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void k_dry_flag_reset(CgState *st) { st->drychg = 0; }
@@ -1004,9 +1017,11 @@ __global__ void k_dry_flag_reset(CgState *st) { st->drychg = 0; }
 // Transport mask, :916-939: a cell is left out of the transport when
 // HP < HDRY and each of its four faces was closed at the step start and
 // is closed now.
-__global__ void k_dry_mask(HPtr h, const int *le, const int *ln, const double *sub1,
-                           const double *svb1, int *lmask, int nwet, double HDRY) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(EFDC_BLOCK) void k_dry_mask(HPtr h, const int *le, const int *ln,
+                                                         const double *sub1, const double *svb1,
+                                                         int *lmask, int nwet, double HDRY) {
+  int tile;
+  const int i = cell_index(nwet, tile);
   if (i > nwet) return;
   int m = 1;
   if (HA(HP)[i] < HDRY) {
@@ -1252,8 +1267,7 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
   }
   k_step_begin<<<1, 1, 0, c->stream>>>(c->st, rot);
   if (dry) {
-    k_dry_begin<<<(nwet + 255) / 256, 256, 0, c->stream>>>(h, c->iscdry, c->sub1, c->svb1,
-                                                           nwet);
+    k_dry_begin<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->iscdry, c->sub1, c->svb1, nwet);
     for (int q = 0; q < 4; ++q)
       if (c->pbc[q].n > 0)
         k_dry_obc_reset<<<(c->pbc[q].n + 255) / 256, 256, 0, c->stream>>>(c->pbc[q],
@@ -1333,8 +1347,8 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
     const bool cells = c->prm.ISDRY < 98;
     if (cells) {
       k_dry_flag_reset<<<1, 1, 0, c->stream>>>(c->st);
-      k_dry_cells<<<(nwet + 255) / 256, 256, 0, c->stream>>>(h, c->dLE, c->dLN, c->iscdry, nwet,
-                                                             c->prm.HDRY, c->st);
+      k_dry_cells<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->iscdry, nwet,
+                                                    c->prm.HDRY, c->st);
     }
     HIP_TRY(c, hipGetLastError());
     if ((s = read_state(c))) return s;
@@ -1354,8 +1368,8 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
   }  // drying loop
   TIMED(c, TK_PUV_POST3, k_puv_post3<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, c->st));
   if (dry)
-    k_dry_mask<<<(nwet + 255) / 256, 256, 0, c->stream>>>(h, c->dLE, c->dLN, c->sub1, c->svb1,
-                                                          c->lmaskdry, nwet, c->prm.HDRY);
+    k_dry_mask<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->sub1, c->svb1,
+                                                 c->lmaskdry, nwet, c->prm.HDRY);
   int s;
   if ((s = read_state(c))) return s;
   if (c->pcg_deferred) {

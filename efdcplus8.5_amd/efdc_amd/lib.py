@@ -365,15 +365,17 @@ class Context:
                                                      _pd(out)))
         return out
 
-    def get_dry_state(self):
+    def get_dry_state(self, arrays=True):
         """(lmaskdry, iscdry, ncordry) after the last calpuv2c: per-cell
         arrays in L order (index L-1), 1 = wet in lmaskdry, 1 = dried this
-        step in iscdry; ncordry = re-solves of that step."""
+        step in iscdry (None, None without `arrays`); ncordry = re-solves of
+        that step."""
         n = self.grid.LCM
-        m = np.zeros(n, np.int32)
-        d = np.zeros(n, np.int32)
+        m = np.zeros(n, np.int32) if arrays else None
+        d = np.zeros(n, np.int32) if arrays else None
         k = np.zeros(1, np.int32)
-        self._check(self.lib.efdc_get_dry_state(self.h, _pi(m), _pi(d), _pi(k)))
+        self._check(self.lib.efdc_get_dry_state(
+            self.h, _pi(m) if arrays else None, _pi(d) if arrays else None, _pi(k)))
         return m, d, int(k[0])
 
     # ------------------------------------------------------------ calls

@@ -5,9 +5,10 @@
 !
 ! Each routine copies the MODULE GLOBAL inputs it needs to the device,
 ! runs the hot path there and copies back every output the host reads
-! afterwards.  Scope: the 2TL scheme with ISDRY=0, IGRIDV=0, MDCHH=0,
-! ISGWIE=0, ISDYNSTP=0 and sigma layers; other settings stop at the first
-! call with a message, like the reference's own unsupported-option stops.
+! afterwards.  Scope: the 2TL scheme with ISDRY < 98 (no drying, or the
+! drying with the re-solve loop), IGRIDV=0, MDCHH=0, ISGWIE=0, ISDYNSTP=0
+! and sigma layers; other settings stop at the first call with a message,
+! like the reference's own unsupported-option stops.
 module efdc_hip_state
   use iso_c_binding
   implicit none
@@ -17,6 +18,7 @@ module efdc_hip_state
   integer, save :: SLOT_NS(16) = 0       ! sediment class of a SED slot
   integer(c_int32_t), allocatable, target, save :: MW(:), ME(:), MS(:), MN(:), MNW(:), MSE(:)
   real(c_double), allocatable, target, save :: TCPH(:)
+  integer(c_int32_t), allocatable, target, save :: MASKI(:), DRYI(:)
 end module efdc_hip_state
 
 subroutine EFDC_HIP_CHECK(HXERR, WHERE)
@@ -44,10 +46,12 @@ subroutine EFDC_HIP_INIT
   integer(c_int32_t) :: HXSLOT
   integer :: NS
 
-  if (ISDRY /= 0 .or. IGRIDV /= 0 .or. MDCHH /= 0 .or. ISGWIE /= 0 .or. ISDYNSTP /= 0) then
-    stop 'EFDC_HIP: needs ISDRY=0, IGRIDV=0, MDCHH=0, ISGWIE=0, ISDYNSTP=0'
+  if (ISDRY < 0 .or. ISDRY >= 98 .or. IGRIDV /= 0 .or. MDCHH /= 0 .or. ISGWIE /= 0 &
+      .or. ISDYNSTP /= 0) then
+    stop 'EFDC_HIP: needs ISDRY<98, IGRIDV=0, MDCHH=0, ISGWIE=0, ISDYNSTP=0'
   end if
   allocate(MW(LCM), ME(LCM), MS(LCM), MN(LCM), MNW(LCM), MSE(LCM))
+  allocate(MASKI(LCM), DRYI(LCM))
   MW = LWC(1:LCM); ME = LEC(1:LCM); MS = LSC(1:LCM); MN = LNC(1:LCM)
   MNW = LNWC(1:LCM); MSE = LSEC(1:LCM)
   HXGRID%LA = LA; HXGRID%LCM = LCM; HXGRID%KC = KC; HXGRID%KCM = KCM
@@ -56,6 +60,7 @@ subroutine EFDC_HIP_INIT
   call efdc_params_default(HXPRM)
   HXPRM%G = G; HXPRM%RITB1 = RITB1; HXPRM%SNLT = SNLT; HXPRM%HDRY = HDRY
   HXPRM%TCON_TBEGIN = TCON*TBEGIN; HXPRM%ITERM = ITERM; HXPRM%RSQM = RSQM
+  HXPRM%ISDRY = ISDRY
   HXERR = efdc_ctx_create(HXGRID, HXPRM, HCTX)
   call EFDC_HIP_CHECK(HXERR, 'efdc_ctx_create')
   ! geometry and layer constants (EFDC/aaefdc.f90:1212-1232,2172-2185)
@@ -71,6 +76,11 @@ subroutine EFDC_HIP_INIT
   HXERR = efdc_set_array(HCTX, EFDC_HRVO, HRVO);   call EFDC_HIP_CHECK(HXERR, 'HRVO')
   HXERR = efdc_set_array(HCTX, EFDC_SUB, SUB);     call EFDC_HIP_CHECK(HXERR, 'SUB')
   HXERR = efdc_set_array(HCTX, EFDC_SVB, SVB);     call EFDC_HIP_CHECK(HXERR, 'SVB')
+  if (ISDRY > 0) then
+    ! the switches the faces reopen to at each step start (calpuv2c.f90:246-253)
+    HXERR = efdc_set_array(HCTX, EFDC_SUBO, SUBO); call EFDC_HIP_CHECK(HXERR, 'SUBO')
+    HXERR = efdc_set_array(HCTX, EFDC_SVBO, SVBO); call EFDC_HIP_CHECK(HXERR, 'SVBO')
+  end if
   HXERR = efdc_set_array(HCTX, EFDC_BELV, BELV);   call EFDC_HIP_CHECK(HXERR, 'BELV')
   HXERR = efdc_set_array(HCTX, EFDC_DZC, DZC);     call EFDC_HIP_CHECK(HXERR, 'DZC')
   HXERR = efdc_set_array(HCTX, EFDC_CDZKK, CDZKK); call EFDC_HIP_CHECK(HXERR, 'CDZKK')
@@ -184,7 +194,8 @@ SUBROUTINE CALPUV2C
   type(efdc_step_t) :: HXSTEP
   type(efdc_cg_stats_t) :: HXST
   integer(c_int) :: HXERR
-  integer :: L
+  integer(c_int32_t) :: HXNCOR
+  integer :: L, LE, LN, K, ND, LF, LL, LP, NW
 
   if (.not. c_associated(HCTX)) call EFDC_HIP_INIT
   if (NBCSOP > 0) call EFDC_HIP_OPENBC
@@ -198,6 +209,16 @@ SUBROUTINE CALPUV2C
   call PUSH(EFDC_FPGXE, FPGXE); call PUSH(EFDC_FPGYE, FPGYE); call PUSH(EFDC_FXE, FXE)
   call PUSH(EFDC_FYE, FYE); call PUSH(EFDC_QSUME, QSUME); call PUSH(EFDC_RCX, RCX)
   call PUSH(EFDC_RCY, RCY); call PUSH(EFDC_HPK, HPK)
+  if (ISDRY > 0) then
+    ! step start with drying on (calpuv2c.f90:169-177,246-253): the mask
+    ! of the last step, and the buoyancy switches of the host's momentum
+    ! terms reopened like SUB/SVB (done on the device)
+    do L = 2, LA
+      OLDMASK(L) = LMASKDRY(L)
+      SBX(L) = SBXO(L)
+      SBY(L) = SBYO(L)
+    end do
+  end if
   HXSTEP%dt = DT; HXSTEP%n = N; HXSTEP%niter = NITER
   HXERR = efdc_calpuv2c(HCTX, HXSTEP, HXST)
   IDRYTBP = HXST%iters                                ! EFDC/congrad.f90:187
@@ -230,8 +251,77 @@ SUBROUTINE CALPUV2C
     write(6,'(A,F12.4)') ' EFDC_HIP: NEGATIVE DEPTH AFTER THE EXTERNAL SOLUTION AT TIMEDAY ', TIMEDAY
     stop 'NEGDEP'
   end if
-  LAWET = LA - 1                                      ! EFDC/calpuv2c.f90:1061-1063
-  LADRY = 0
+  if (ISDRY == 0) then
+    LAWET = LA - 1                                    ! EFDC/calpuv2c.f90:1061-1063
+    LADRY = 0
+  else
+    call PULL(EFDC_SUB, SUB); call PULL(EFDC_SVB, SVB)
+    call EFDC_HIP_CHECK(efdc_get_dry_state(HCTX, MASKI, DRYI, HXNCOR), 'efdc_get_dry_state')
+    do L = 2, LA
+      LMASKDRY(L) = MASKI(L) /= 0
+      ISCDRY(L) = DRYI(L)
+    end do
+    ! every cell the device flagged had its faces closed; close the
+    ! buoyancy switches of the same faces (calpuv2c.f90:519-522,589-592)
+    do L = 2, LA
+      if (ISCDRY(L) /= 0) then
+        LE = LEC(L); LN = LNC(L)
+        SBX(L) = 0.; SBY(L) = 0.; SBX(LE) = 0.; SBY(LN) = 0.
+      end if
+    end do
+    ! layer switches of the host's internal mode (calpuv2c.f90:982-990)
+    do K = 1, KC
+      do L = 2, LA
+        if (LKSZ(L,K)) cycle
+        SUB3D(L,K) = SUB(L)*SUB3DO(L,K)
+        SVB3D(L,K) = SVB(L)*SVB3DO(L,K)
+      end do
+    end do
+    ! wet list and newly dry list (calpuv2c.f90:1064-1078), then the
+    ! per-domain and whole-domain layer lists (:1096-1115,1170-1185)
+    LAWET = 0
+    LADRY = 0
+    do L = 2, LA
+      if (LMASKDRY(L)) then
+        LAWET = LAWET + 1
+        LWET(LAWET) = L
+      else if (OLDMASK(L) .or. N < NTSTBC + 1) then
+        LADRY = LADRY + 1
+        LDRY(LADRY) = L
+      end if
+    end do
+    LDMWET = INT(FLOAT(LAWET)/FLOAT(NDM)) + 1
+    LDMDRY = INT(FLOAT(LADRY)/FLOAT(NDM)) + 1
+    do ND = 1, NDM
+      LF = (ND - 1)*LDMWET + 1
+      LL = MIN(LF + LDMWET - 1, LAWET)
+      do K = 1, KC
+        NW = 0
+        do LP = LF, LL
+          L = LWET(LP)
+          if (LKSZ(L,K)) cycle
+          NW = NW + 1
+          LKWET(NW,K,ND) = L
+        end do
+        LLWET(K,ND) = NW
+      end do
+      do K = 1, KS
+        LLWETZ(K,ND) = LLWET(K,ND)
+        LKWETZ(1:LLWET(K,ND),K,ND) = LKWET(1:LLWET(K,ND),K,ND)
+      end do
+      LLWETZ(KC,ND) = LLWET(KS,ND)
+      LKWETZ(1:LLWET(KS,ND),KC,ND) = LKWET(1:LLWET(KS,ND),KS,ND)
+    end do
+    do K = 1, KC
+      NW = 0
+      do L = 2, LA
+        if (LKSZ(L,K) .or. .not. LMASKDRY(L)) cycle
+        NW = NW + 1
+        LKWET(NW,K,0) = L
+      end do
+      LLWET(K,0) = NW
+    end do
+  end if
 contains
   subroutine PUSH(ID, A)
     integer(c_int32_t), intent(in) :: ID

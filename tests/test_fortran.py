@@ -31,13 +31,15 @@ STRUCTS = {
                     "LNWC", "LSEC"],
     "efdc_params_t": ["G", "RITB1", "SNLT", "HDRY", "TCON_TBEGIN", "ITERM",
                       "RSQM", "cg_rel_tol", "cg_mode", "store",
-                      "mixed_inner_rtol", "mixed_max_outer", "cg_kernel"],
+                      "mixed_inner_rtol", "mixed_max_outer", "cg_kernel",
+                      "mixed_precond_degree", "tran_shape", "ISDRY"],
     "efdc_pbc_side_t": ["n", "L", "ispb", "pser", "pcb", "psb"],
     "efdc_open_bc_t": ["side", "mtide", "tcp"],
     "efdc_cbc_side_t": ["n", "L", "cb", "csert", "ntscr"],
     "efdc_conc_bc_t": ["side", "wtci"],
     "efdc_sources_t": ["n", "LQS", "qss", "qsercell", "cqs", "csert"],
-    "efdc_constituent_t": ["mo", "isadac", "isfct", "rkdye", "is_dye"],
+    "efdc_constituent_t": ["mo", "isadac", "isfct", "rkdye", "is_dye", "wset",
+                           "probdep", "wese"],
     "efdc_step_t": ["dt", "n", "niter"],
     "efdc_cg_stats_t": ["iters", "outer", "rsq", "rsqm_used", "ccmnm"],
     "efdc_balance_t": ["volume", "mass", "nconst"],
