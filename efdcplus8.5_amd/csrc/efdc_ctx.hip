@@ -186,8 +186,8 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
     return reject(EFDC_ERR_ARG, "unknown CG kernel choice");
   if (prm->mixed_precond_degree < 0 || prm->mixed_precond_degree > 16)
     return reject(EFDC_ERR_ARG, "mixed_precond_degree must be 0..16");
-  if (prm->ISDRY < 0 || prm->ISDRY > 97)
-    return reject(EFDC_ERR_ARG, "ISDRY must be 0..97 (the rewetting of 98/99 is not built)");
+  if (prm->ISDRY < 0 || prm->ISDRY > 99)
+    return reject(EFDC_ERR_ARG, "ISDRY must be 0..99");
   if (!tran_shape_valid(prm->tran_shape))
     return reject(EFDC_ERR_ARG, "tran_shape must be 0, -1, -2, -4, or 1, 2, 4 plus 0, 16 or 32 (32 for 1 and 2)");
   int ndev = 0;
@@ -344,6 +344,7 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   }
   delete[] c->tcp_host;
   dfree(c->iscdry); dfree(c->lmaskdry); dfree(c->sub1); dfree(c->svb1);
+  dfree(c->sub_n); dfree(c->svb_n);
   dfree(c->tide_dev); dfree(c->lopenbcdry); dfree(c->pbcflag); dfree(c->lobflag);
   dfree(c->lobcs); dfree(c->lobcs2);
   for (auto &b : c->cbc) {
@@ -383,8 +384,8 @@ int efdc_set_params(efdc_ctx_t *c, const efdc_params_t *p) {
     c->err = "mixed_precond_degree must be 0..16";
     return EFDC_ERR_ARG;
   }
-  if (p->ISDRY < 0 || p->ISDRY > 97) {
-    c->err = "ISDRY must be 0..97 (the rewetting of 98/99 is not built)";
+  if (p->ISDRY < 0 || p->ISDRY > 99) {
+    c->err = "ISDRY must be 0..99";
     return EFDC_ERR_ARG;
   }
   if (!tran_shape_valid(p->tran_shape)) {

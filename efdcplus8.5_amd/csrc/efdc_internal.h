@@ -146,6 +146,7 @@ struct efdc_ctx {
   // drying (ISDRY > 0): ISCDRY, LMASKDRY, SUB1/SVB1, re-solves of the step
   int *iscdry = nullptr, *lmaskdry = nullptr;
   double *sub1 = nullptr, *svb1 = nullptr;
+  double *sub_n = nullptr, *svb_n = nullptr;  // ISDRY = 99: new face switches of a pass
   int ncordry = 0;
   unsigned char *pbcflag = nullptr;   // (LCM) 1 on pressure-BC cells
   unsigned char *lobflag = nullptr;   // (LCM) bit 0 LOBCS, bit 1 LOBCS2

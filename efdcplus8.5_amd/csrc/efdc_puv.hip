@@ -1014,6 +1014,98 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_dry_cells(HPtr h, const int *le,
 
 __global__ void k_dry_flag_reset(CgState *st) { st->drychg = 0; }
 
+// Drying and rewetting, ISDRY = 99 (:604-716).  The reference walks the
+// cells in L order and updates the face switches in place; a cell's four
+// faces are shared with its neighbours, and a dry cell (HP <= HDRY) rewrites
+// all four.  Case of a dry cell (dry99_case): 1 rising fast enough to stay
+// wet (faces closed, then those with inflow from a neighbour deeper than
+// 2 HDRY reopened), 2 rising too slowly (faces closed), 3 falling and low
+// (faces closed), 0 no change.  Whatever a neighbour wrote on a face that
+// it shares with a dry cell is 0 (a face reopens only towards a cell deeper
+// than 2 HDRY), so the values a cell finds on its faces when its turn comes
+// are 0 where a neighbour with a smaller L had a case, else the values of
+// the pass start; and the final value of a face is the write of the cell
+// with the larger L among the two that wrote it.  Each thread owns the west
+// and south faces of its cell and writes them to sub_n / svb_n (the pass
+// start values stay readable until k_dry99_commit).
+__device__ __forceinline__ int dry99_case(const double *HP, const double *H1P, int j, int nwet,
+                                          double HDRY) {
+  if (j < 1 || j > nwet) return 0;  // L = 2..LA only
+  const double hp = HP[j], h1p = H1P[j];
+  if (!(hp <= HDRY)) return 0;
+  const double dh = hp - h1p;
+  if (dh > 0.0) return dh > (HDRY - h1p) * 0.01 ? 1 : 2;
+  return (hp < 0.9 * HDRY || h1p < HDRY) ? 3 : 0;
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_dry99(HPtr h, const int *lw, const int *le,
+                                                      const int *ls, const int *ln,
+                                                      int *iscdry, double *sub_n,
+                                                      double *svb_n, int nwet, double HDRY,
+                                                      CgState *st) {
+  int tile;
+  const int i = cell_index(nwet, tile);
+  bool chg = false;
+  if (i <= nwet) {
+    const double *HP = HA(HP), *H1P = HA(H1P), *SUB = HA(SUB), *SVB = HA(SVB),
+                 *SUBO = HA(SUBO), *SVBO = HA(SVBO), *UHDYE = HA(UHDYE), *VHDXE = HA(VHDXE);
+    const double HDRY2 = 2. * HDRY;
+    const int w = lw[i], e = le[i], s = ls[i], n = ln[i];
+    const int ci = dry99_case(HP, H1P, i, nwet, HDRY);
+    const int cw = dry99_case(HP, H1P, w, nwet, HDRY);
+    const int cs = dry99_case(HP, H1P, s, nwet, HDRY);
+    // this cell's writes on its W, E, S, N faces (case 1 may reopen)
+    double ww = 0., we = 0., wsv = 0., wn = 0.;
+    if (ci == 1) {
+      if (SUBO[i] > 0.5 && UHDYE[i] > 0.0 && HP[w] > HDRY2) ww = SUBO[i];
+      if (SUBO[e] > 0.5 && UHDYE[e] < 0.0 && HP[e] > HDRY2) we = SUBO[e];
+      if (SVBO[i] > 0.5 && VHDXE[i] > 0.0 && HP[s] > HDRY2) wsv = SVBO[i];
+      if (SVBO[n] > 0.5 && VHDXE[n] < 0.0 && HP[n] > HDRY2) wn = SVBO[n];
+    }
+    // the west / south neighbours' writes on the faces they share with i
+    double ow = 0., os = 0.;
+    if (cw == 1 && SUBO[i] > 0.5 && UHDYE[i] < 0.0 && HP[i] > HDRY2) ow = SUBO[i];
+    if (cs == 1 && SVBO[i] > 0.5 && VHDXE[i] < 0.0 && HP[i] > HDRY2) os = SVBO[i];
+    // final W and S faces: the later writer (larger L) wins
+    double fw = SUB[i], fs = SVB[i];
+    if (ci != 0 && cw != 0) fw = (i > w) ? ww : ow;
+    else if (ci != 0) fw = ww;
+    else if (cw != 0) fw = ow;
+    if (ci != 0 && cs != 0) fs = (i > s) ? wsv : os;
+    else if (ci != 0) fs = wsv;
+    else if (cs != 0) fs = os;
+    sub_n[i] = fw;
+    svb_n[i] = fs;
+    if (ci == 1) {
+      const int ce = dry99_case(HP, H1P, e, nwet, HDRY);
+      const int cn = dry99_case(HP, H1P, n, nwet, HDRY);
+      const double sw0 = (cw != 0 && w < i) ? 0. : SUB[i];
+      const double se0 = (ce != 0 && e < i) ? 0. : SUB[e];
+      const double ss0 = (cs != 0 && s < i) ? 0. : SVB[i];
+      const double sn0 = (cn != 0 && n < i) ? 0. : SVB[n];
+      iscdry[i] = (ww + we + wsv + wn) < 0.5 ? 1 : 0;
+      chg = fabs(ww - sw0) > 0.5 || fabs(we - se0) > 0.5 || fabs(wsv - ss0) > 0.5 ||
+            fabs(wn - sn0) > 0.5;
+    } else if (ci != 0 && iscdry[i] == 0) {
+      iscdry[i] = 1;
+      chg = true;
+    }
+  }
+  const unsigned long long b = __ballot(chg);
+  if (b != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)b) - 1 &&
+      __hip_atomic_load(&st->drychg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    __hip_atomic_store(&st->drychg, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(EFDC_BLOCK) void k_dry99_commit(HPtr h, const double *sub_n,
+                                                             const double *svb_n, int nwet) {
+  int tile;
+  const int i = cell_index(nwet, tile);
+  if (i > nwet) return;
+  HA(SUB)[i] = sub_n[i];
+  HA(SVB)[i] = svb_n[i];
+}
+
 // Transport mask, :916-939: a cell is left out of the transport when
 // HP < HDRY and each of its four faces was closed at the step start and
 // is closed now.
@@ -1265,6 +1357,20 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
     c->err = "drying (ISDRY > 0) with several basins is not built";
     return EFDC_ERR_ARG;
   }
+  if (c->prm.ISDRY == 99) {
+    if (!c->mutual) {
+      c->err = "ISDRY = 99 needs mutual neighbour maps (LEC(LWC(L)) = L, LNC(LSC(L)) = L)";
+      return EFDC_ERR_ARG;
+    }
+    if (!c->sub_n) {
+      const size_t n = (size_t)c->pitch;
+      if (hipMalloc((void **)&c->sub_n, n * sizeof(double)) != hipSuccess ||
+          hipMalloc((void **)&c->svb_n, n * sizeof(double)) != hipSuccess) {
+        c->err = "hipMalloc failed (ISDRY = 99 face buffers)";
+        return EFDC_ERR_NOMEM;
+      }
+    }
+  }
   k_step_begin<<<1, 1, 0, c->stream>>>(c->st, rot);
   if (dry) {
     k_dry_begin<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->iscdry, c->sub1, c->svb1, nwet);
@@ -1344,11 +1450,18 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
       if (c->pbc[q].n > 0)
         k_dry_obc<<<(c->pbc[q].n + 255) / 256, 256, 0, c->stream>>>(
             h, c->pbc[q], c->dLE, c->dLN, c->lopenbcdry, c->iscdry, dt, G, c->prm.HDRY);
-    const bool cells = c->prm.ISDRY < 98;
+    const bool cells = c->prm.ISDRY < 98 || c->prm.ISDRY == 99;
     if (cells) {
       k_dry_flag_reset<<<1, 1, 0, c->stream>>>(c->st);
-      k_dry_cells<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->iscdry, nwet,
-                                                    c->prm.HDRY, c->st);
+      if (c->prm.ISDRY < 98) {
+        k_dry_cells<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->iscdry, nwet,
+                                                      c->prm.HDRY, c->st);
+      } else {
+        k_dry99<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLE, c->dLS, c->dLN,
+                                                  c->iscdry, c->sub_n, c->svb_n, nwet,
+                                                  c->prm.HDRY, c->st);
+        k_dry99_commit<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->sub_n, c->svb_n, nwet);
+      }
     }
     HIP_TRY(c, hipGetLastError());
     if ((s = read_state(c))) return s;

@@ -5,10 +5,11 @@
 !
 ! Each routine copies the MODULE GLOBAL inputs it needs to the device,
 ! runs the hot path there and copies back every output the host reads
-! afterwards.  Scope: the 2TL scheme with ISDRY < 98 (no drying, or the
-! drying with the re-solve loop), IGRIDV=0, MDCHH=0, ISGWIE=0, ISDYNSTP=0
-! and sigma layers; other settings stop at the first call with a message,
-! like the reference's own unsupported-option stops.
+! afterwards.  Scope: the 2TL scheme with any ISDRY (no drying, the drying
+! with the re-solve loop, or drying and rewetting), NDRYSTP >= 0,
+! IGRIDV=0, MDCHH=0, ISGWIE=0, ISDYNSTP=0 and sigma layers; other settings
+! stop at the first call with a message, like the reference's own
+! unsupported-option stops.
 module efdc_hip_state
   use iso_c_binding
   implicit none
@@ -46,9 +47,9 @@ subroutine EFDC_HIP_INIT
   integer(c_int32_t) :: HXSLOT
   integer :: NS
 
-  if (ISDRY < 0 .or. ISDRY >= 98 .or. IGRIDV /= 0 .or. MDCHH /= 0 .or. ISGWIE /= 0 &
-      .or. ISDYNSTP /= 0) then
-    stop 'EFDC_HIP: needs ISDRY<98, IGRIDV=0, MDCHH=0, ISGWIE=0, ISDYNSTP=0'
+  if (ISDRY < 0 .or. (ISDRY > 0 .and. NDRYSTP < 0) .or. IGRIDV /= 0 .or. MDCHH /= 0 &
+      .or. ISGWIE /= 0 .or. ISDYNSTP /= 0) then
+    stop 'EFDC_HIP: needs NDRYSTP>=0 with drying, IGRIDV=0, MDCHH=0, ISGWIE=0, ISDYNSTP=0'
   end if
   allocate(MW(LCM), ME(LCM), MS(LCM), MN(LCM), MNW(LCM), MSE(LCM))
   allocate(MASKI(LCM), DRYI(LCM))
@@ -195,7 +196,7 @@ SUBROUTINE CALPUV2C
   type(efdc_cg_stats_t) :: HXST
   integer(c_int) :: HXERR
   integer(c_int32_t) :: HXNCOR
-  integer :: L, LE, LN, K, ND, LF, LL, LP, NW
+  integer :: L, K, ND, LF, LL, LP, NW
 
   if (.not. c_associated(HCTX)) call EFDC_HIP_INIT
   if (NBCSOP > 0) call EFDC_HIP_OPENBC
@@ -261,13 +262,11 @@ SUBROUTINE CALPUV2C
       LMASKDRY(L) = MASKI(L) /= 0
       ISCDRY(L) = DRYI(L)
     end do
-    ! every cell the device flagged had its faces closed; close the
-    ! buoyancy switches of the same faces (calpuv2c.f90:519-522,589-592)
+    ! the buoyancy switches of the host's momentum terms close and reopen
+    ! with the faces (calpuv2c.f90:519-522,589-592,633-665,703-710)
     do L = 2, LA
-      if (ISCDRY(L) /= 0) then
-        LE = LEC(L); LN = LNC(L)
-        SBX(L) = 0.; SBY(L) = 0.; SBX(LE) = 0.; SBY(LN) = 0.
-      end if
+      if (SUB(L) < 0.5) SBX(L) = 0.
+      if (SVB(L) < 0.5) SBY(L) = 0.
     end do
     ! layer switches of the host's internal mode (calpuv2c.f90:982-990)
     do K = 1, KC

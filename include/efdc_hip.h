@@ -113,7 +113,9 @@ typedef struct {
   int32_t ISDRY;           /* wetting and drying: 0 off; 1..97 the drying of
                               EFDC/calpuv2c.f90:497-530,574-602 with the
                               re-solve loop and the transport mask LMASKDRY
-                              (:916-939); 98, 99 (rewetting) are refused     */
+                              (:916-939); 98 open-boundary drying only; 99
+                              drying and rewetting (:604-716, needs mutual
+                              neighbour maps)                               */
 } efdc_params_t;
 
 /* Fill *p with the values EFDC uses by default: G=9.81, RITB1=1, SNLT=0,
@@ -313,9 +315,11 @@ int efdc_get_dry_state(efdc_ctx_t *ctx, int32_t *lmaskdry, int32_t *iscdry, int3
    CCC, CCS, CCW, CCE, CCN, CCCI, FPTMP, starting from and updating P. */
 int efdc_congrad(efdc_ctx_t *ctx, efdc_cg_stats_t *stats);
 
-/* CALPUV2C for the 2TL scheme with ISDRY=0 (EFDC/calpuv2c.f90:128-571,
-   896-914): explicit stage, assembly, open boundaries, min-diagonal
-   scaling, CONGRAD, face transports, depths, P, HU/HV and inverses. */
+/* CALPUV2C for the 2TL scheme (EFDC/calpuv2c.f90:128-571,896-939):
+   explicit stage, assembly, open boundaries, min-diagonal scaling,
+   CONGRAD, face transports, depths, P, HU/HV and inverses; with ISDRY > 0
+   the drying passes and the re-solve loop from label 1000 (:302, at most
+   500 re-solves, then EFDC_ERR_STATE) and LMASKDRY. */
 int efdc_calpuv2c(efdc_ctx_t *ctx, const efdc_step_t *step,
                   efdc_cg_stats_t *stats);
 

@@ -502,6 +502,55 @@ static int cell_drying(orc_ctx *o) {
   return icordry;
 }
 
+/* Drying and rewetting, ISDRY = 99: EFDC/calpuv2c.f90:604-716, one pass
+   over L = 2..LA in order, switches updated in place (a cell reads the
+   faces its west/south neighbours already set).  Returns ICORDRY. */
+static int rewet_99(orc_ctx *o) {
+  const double HDRY = o->HDRY, HDRY2 = 2. * HDRY, HDRY90 = 0.9 * HDRY;
+  double *HP = A(EFDC_HP), *H1P = A(EFDC_H1P), *SUB = A(EFDC_SUB), *SVB = A(EFDC_SVB),
+         *SUBO = A(EFDC_SUBO), *SVBO = A(EFDC_SVBO), *UHDYE = A(EFDC_UHDYE),
+         *VHDXE = A(EFDC_VHDXE);
+  int icordry = 0;
+  for (int L = 2; L <= o->LA; ++L) {
+    const size_t i = HX(L), w = HX(o->LWC[i]), e = HX(o->LEC[i]), s = HX(o->LSC[i]),
+                 nn = HX(o->LNC[i]);
+    if (!(HP[i] <= HDRY)) continue;
+    const double SUBW = SUB[i], SUBE = SUB[e], SVBS = SVB[i], SVBN = SVB[nn];
+    const double DHPDT = HP[i] - H1P[i];
+    if (DHPDT > 0.0) { /* re-wetting allowed, :626-700 */
+      SUB[i] = 0.;
+      SUB[e] = 0.;
+      SVB[i] = 0.;
+      SVB[nn] = 0.;
+      const double DHPDT2 = (HDRY - H1P[i]) * 0.01;
+      if (DHPDT > DHPDT2) { /* rising fast enough to stay wet */
+        if (SUBO[i] > 0.5 && UHDYE[i] > 0.0 && HP[w] > HDRY2) SUB[i] = SUBO[i];
+        if (SUBO[e] > 0.5 && UHDYE[e] < 0.0 && HP[e] > HDRY2) SUB[e] = SUBO[e];
+        if (SVBO[i] > 0.5 && VHDXE[i] > 0.0 && HP[s] > HDRY2) SVB[i] = SVBO[i];
+        if (SVBO[nn] > 0.5 && VHDXE[nn] < 0.0 && HP[nn] > HDRY2) SVB[nn] = SVBO[nn];
+        const double RDRY = SUB[i] + SUB[e] + SVB[i] + SVB[nn];
+        o->iscdry[i] = RDRY < 0.5 ? 1 : 0;
+        if (fabs(SUB[i] - SUBW) > 0.5 || fabs(SUB[e] - SUBE) > 0.5 ||
+            fabs(SVB[i] - SVBS) > 0.5 || fabs(SVB[nn] - SVBN) > 0.5)
+          icordry = 1;
+      } else if (o->iscdry[i] == 0) { /* rising, not fast enough */
+        o->iscdry[i] = 1;
+        icordry = 1;
+      }
+    } else if (HP[i] < HDRY90 || H1P[i] < HDRY) { /* :700-714 */
+      SUB[i] = 0.;
+      SUB[e] = 0.;
+      SVB[i] = 0.;
+      SVB[nn] = 0.;
+      if (o->iscdry[i] == 0) {
+        o->iscdry[i] = 1;
+        icordry = 1;
+      }
+    }
+  }
+  return icordry;
+}
+
 int orc_calpuv2c(orc_ctx *o, double dt, int64_t n, orc_cg_out *out) {
   const int LA = o->LA;
   const double G = o->G, GI = 1. / o->G; /* EFDC/aaefdc.f90:648 */
@@ -576,7 +625,10 @@ int orc_calpuv2c(orc_ctx *o, double dt, int64_t n, orc_cg_out *out) {
         P[i] = G * (HP[i] + BELV[i]);
       }
     }
-    if (isdry > 0 && isdry < 98 && cell_drying(o) && st == EFDC_OK) {
+    const int again = (isdry > 0 && isdry < 98) ? cell_drying(o)
+                      : isdry == 99            ? rewet_99(o)
+                                               : 0;
+    if (again && st == EFDC_OK) {
       if (++o->ncordry < 500) continue; /* GOTO 1000 */
       st = EFDC_ERR_STATE;               /* '*** NCORDRY > 500' */
     }

@@ -62,7 +62,8 @@ typedef struct {
   /* constituents */
   int32_t nconst;
   orc_const c[ORC_MAXC];
-  /* wetting and drying (ISDRY = 0: off; 1..97: EFDC/calpuv2c.f90:574-602) */
+  /* wetting and drying (ISDRY = 0: off; 1..97: EFDC/calpuv2c.f90:574-602;
+     98: open-boundary drying only; 99: rewetting, :604-716) */
   int32_t ISDRY;
   int32_t *iscdry;       /* (LCM) ISCDRY */
   int32_t *lmaskdry;     /* (LCM) LMASKDRY, 1 = wet */

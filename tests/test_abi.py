@@ -50,7 +50,7 @@ def test_invalid_params_are_refused_before_any_device_call():
     for bad in (dict(tran_shape=3), dict(tran_shape=-3), dict(tran_shape=8), dict(tran_shape=36),
                 dict(tran_shape=-17), dict(tran_shape=48),
                 dict(mixed_precond_degree=17), dict(cg_kernel=7), dict(store=5),
-                dict(ISDRY=98), dict(ISDRY=99), dict(ISDRY=-1)):
+                dict(ISDRY=100), dict(ISDRY=-1)):
         with pytest.raises(lib.EfdcError) as e:
             lib.Context(case.grid, **bad)
         assert e.value.status == STATUS["EFDC_ERR_ARG"], bad

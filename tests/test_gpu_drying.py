@@ -1,4 +1,6 @@
-"""Drying (ISDRY 1..97) on the GPU against the oracle: the re-solve loop
+"""Drying on the GPU against the oracle, ISDRY 1..97 and 99 (drying and
+rewetting, EFDC/calpuv2c.f90:604-716, which the GPU evaluates for all cells
+at once where the reference walks them in order): the re-solve loop
 (EFDC/calpuv2c.f90:302,574-602), open-boundary drying (:497-530), the
 face reopening at the step start (:246-253) and the transport mask
 LMASKDRY (:916-939) with the transport leaving the masked columns out.
@@ -32,17 +34,18 @@ def _stratified(case):
     return case
 
 
+@pytest.mark.parametrize("isdry", [1, 99])
 @pytest.mark.parametrize("cg_kernel", [0, 1])
 @pytest.mark.parametrize("deep", [2.0, 0.6])
-def test_drying_matches_oracle(oracle_mod, deep, cg_kernel):
+def test_drying_matches_oracle(oracle_mod, deep, cg_kernel, isdry):
     from efdc_amd import grids
     from efdc_amd.lib import Context
     case = grids.drying_basin(kc=3, deep=deep)
     if cg_kernel == 1:
         case = _stratified(case)
     ref = case.copy()
-    o = oracle_mod.Oracle(ref, rsqm=1e-24, iterm=2000, isdry=1, hdry=HDRY)
-    ctx = Context(case.grid, RSQM=1e-24, ITERM=2000, ISDRY=1, HDRY=HDRY,
+    o = oracle_mod.Oracle(ref, rsqm=1e-24, iterm=2000, isdry=isdry, hdry=HDRY)
+    ctx = Context(case.grid, RSQM=1e-24, ITERM=2000, ISDRY=isdry, HDRY=HDRY,
                   cg_kernel=cg_kernel).upload_case(case)
     L = wet_index(case)
     lb = case.open_bc["side"][0]["L"] - 1
@@ -70,7 +73,7 @@ def test_drying_matches_oracle(oracle_mod, deep, cg_kernel):
                 scale = max(1e-30, np.max(np.abs(c["con"][:kc])))
                 assert np.max(np.abs(con[:kc] - c["con"][:kc])) / scale <= 1e-9, (n, j)
     if deep == 2.0:
-        assert max(seen) >= 2
+        assert max(seen) >= (2 if isdry == 1 else 1)
     else:
         assert obc_dried
     for s, c in enumerate(ref.consts):
@@ -109,4 +112,25 @@ def test_drying_is_refused_where_not_built():
     ctx = Context(case.grid, ISDRY=1).upload_case(case)
     with pytest.raises(EfdcError):
         ctx.adj_enable([0])
+    ctx.close()
+
+
+def test_rewetting_closed_basin_matches_oracle(oracle_mod):
+    """ISDRY = 99 on the sloshing basin whose dry bank is wetted again."""
+    import test_oracle_drying as t
+    from efdc_amd.lib import Context
+    case = t._closed()
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref, rsqm=1e-26, iterm=4000, isdry=99, hdry=HDRY)
+    ctx = Context(case.grid, RSQM=1e-26, ITERM=4000, ISDRY=99, HDRY=HDRY).upload_case(case)
+    L = wet_index(case)
+    for n in range(1, 401):
+        o.step(n)
+        ctx.step(case.dt, n)
+        m, d, k = ctx.get_dry_state()
+        assert k == o.o.ncordry and np.array_equal(m[L], o.lmaskdry[L]), n
+        assert np.array_equal(d[L], o.iscdry[L]), n
+    for key in ("SUB", "SVB"):
+        assert np.array_equal(ctx.get(key)[L], ref.h[key][L]), key
+    assert np.max(np.abs(ctx.get("HP")[L] - ref.h["HP"][L])) <= 1e-9
     ctx.close()

@@ -1,12 +1,15 @@
 """Pins of the oracle's drying (ISDRY 1..97, EFDC/calpuv2c.f90:147-179,
-246-253, 302, 490-602, 916-939) by what the scheme fixes:
+246-253, 302, 490-602, 916-939), its open-boundary-only form (98) and its
+drying-and-rewetting form (99, :604-716) by what the schemes fix:
 
-* with nothing near HDRY, ISDRY=1 gives the same bits as ISDRY=0 (the
-  re-solve loop, the face resets and the intermediate P update change
-  nothing);
-* after every step, each cell with HP <= HDRY has its four faces closed
-  and is flagged in ISCDRY, closed faces carry no transport, and the
-  transport mask only leaves out cells with HP < HDRY;
+* with nothing near HDRY, ISDRY = 1, 98 or 99 gives the same bits as
+  ISDRY = 0 (the re-solve loop, the face resets and the intermediate P
+  update change nothing);
+* after every step closed faces carry no transport and the transport mask
+  only leaves out cells with HP < HDRY; with 1..97 each cell with
+  HP <= HDRY also has its four faces closed and is flagged in ISCDRY;
+* with 99, cells that start dry on a bank are wetted again by water that
+  rises fast enough (the 1..97 scheme keeps them dry for good);
 * a closed basin sloshing against a sloping bank keeps its volume to
   round-off while every step opens the faces of its dry cells and closes
   them again in the re-solve;
@@ -54,16 +57,17 @@ def _closed():
     return c
 
 
-def _check_faces(case, o):
+def _check_faces(case, o, strict=True):
     g, h = case.grid, case.h
     L = wet_index(case)
     e = g.maps["LEC"][L] - 1
     n = g.maps["LNC"][L] - 1
     hp = h["HP"][L]
     low = hp <= HDRY
-    assert np.all(o.iscdry[L][low] == 1)
-    for arr, idx in (("SUB", L), ("SUB", e), ("SVB", L), ("SVB", n)):
-        assert np.all(h[arr][idx][low] == 0.0), arr
+    if strict:
+        assert np.all(o.iscdry[L][low] == 1)
+        for arr, idx in (("SUB", L), ("SUB", e), ("SVB", L), ("SVB", n)):
+            assert np.all(h[arr][idx][low] == 0.0), arr
     closed_u = h["SUB"][L] == 0.0
     closed_v = h["SVB"][L] == 0.0
     assert np.all(h["UHDYE"][L][closed_u] == 0.0)
@@ -73,12 +77,13 @@ def _check_faces(case, o):
     return int(low.sum()), int(masked.sum())
 
 
-def test_no_op_when_nothing_dries(oracle_mod):
+@pytest.mark.parametrize("isdry", [1, 98, 99])
+def test_no_op_when_nothing_dries(oracle_mod, isdry):
     from efdc_amd import grids
     case = grids.tidal_basin(24, 12, 3, seed=51)
     a, b = case.copy(), case.copy()
     oa = oracle_mod.Oracle(a, rsqm=1e-24, iterm=2000, isdry=0)
-    ob = oracle_mod.Oracle(b, rsqm=1e-24, iterm=2000, isdry=1, hdry=HDRY)
+    ob = oracle_mod.Oracle(b, rsqm=1e-24, iterm=2000, isdry=isdry, hdry=HDRY)
     for n in range(1, 31):
         oa.step(n)
         ob.step(n)
@@ -91,10 +96,11 @@ def test_no_op_when_nothing_dries(oracle_mod):
     assert np.all(ob.iscdry[L] == 0) and np.all(ob.lmaskdry[L] == 1)
 
 
-def test_faces_close_and_reopen_over_a_tide(oracle_mod):
+@pytest.mark.parametrize("isdry", [1, 99])
+def test_faces_close_and_reopen_over_a_tide(oracle_mod, isdry):
     from efdc_amd import grids
     case = grids.drying_basin()
-    o = oracle_mod.Oracle(case, rsqm=1e-24, iterm=2000, isdry=1, hdry=HDRY)
+    o = oracle_mod.Oracle(case, rsqm=1e-24, iterm=2000, isdry=isdry, hdry=HDRY)
     L = wet_index(case)
     start_dry = case.h["HP"][L] <= HDRY
     ever_dry = np.zeros(len(L), bool)
@@ -104,7 +110,7 @@ def test_faces_close_and_reopen_over_a_tide(oracle_mod):
         st = o.step(n)
         assert st["status"] == 0, n
         resolves.append(o.o.ncordry)
-        _check_faces(case, o)
+        _check_faces(case, o, strict=isdry < 98)
         masked = o.lmaskdry[L] == 0
         rewet |= ever_dry & ~masked
         ever_dry |= masked
@@ -115,25 +121,27 @@ def test_faces_close_and_reopen_over_a_tide(oracle_mod):
     assert max(resolves) >= 2 and min(resolves) >= 1
 
 
-def test_sloshing_basin_keeps_its_volume(oracle_mod):
+@pytest.mark.parametrize("isdry", [1, 99])
+def test_sloshing_basin_keeps_its_volume(oracle_mod, isdry):
     case = _closed()
-    o = oracle_mod.Oracle(case, rsqm=1e-26, iterm=4000, isdry=1, hdry=HDRY)
+    o = oracle_mod.Oracle(case, rsqm=1e-26, iterm=4000, isdry=isdry, hdry=HDRY)
     L = wet_index(case)
     v0 = float(np.sum(case.h["DXYP"][L] * case.h["HP"][L]))
     for n in range(1, 401):
         st = o.step(n)
         assert st["status"] == 0, n
         assert o.o.ncordry >= 1
-        ndry, _ = _check_faces(case, o)
+        ndry, _ = _check_faces(case, o, strict=isdry < 98)
         assert ndry > 0
         v = float(np.sum(case.h["DXYP"][L] * case.h["HP"][L]))
         assert abs(v - v0) <= 1e-11 * v0, (n, v - v0)
 
 
-def test_open_boundary_cells_dry_and_return_to_the_tide(oracle_mod):
+@pytest.mark.parametrize("isdry", [1, 99])
+def test_open_boundary_cells_dry_and_return_to_the_tide(oracle_mod, isdry):
     from efdc_amd import grids
     case = grids.drying_basin(deep=0.6)
-    o = oracle_mod.Oracle(case, rsqm=1e-24, iterm=2000, isdry=1, hdry=HDRY)
+    o = oracle_mod.Oracle(case, rsqm=1e-24, iterm=2000, isdry=isdry, hdry=HDRY)
     g, h = case.grid, case.h
     lb = case.open_bc["side"][0]["L"] - 1
     e = g.maps["LEC"][lb] - 1
@@ -142,7 +150,7 @@ def test_open_boundary_cells_dry_and_return_to_the_tide(oracle_mod):
     for n in range(1, 601):
         st = o.step(n)
         assert st["status"] == 0, n
-        _check_faces(case, o)
+        _check_faces(case, o, strict=isdry < 98)
         d = o.lopenbcdry[lb] == 1
         if d.any():
             dried = True
@@ -181,7 +189,8 @@ def test_resolve_loop_ends_at_a_fixed_point(oracle_mod, nsteps):
     assert np.max(np.abs(again.h["HP"][L] - case.h["HP"][L])) <= 1e-10
 
 
-def test_masked_columns_keep_their_concentrations(oracle_mod):
+@pytest.mark.parametrize("isdry", [1, 99])
+def test_masked_columns_keep_their_concentrations(oracle_mod, isdry):
     from efdc_amd import grids
     case = grids.drying_basin(kc=3, deep=0.6)
     kc = case.grid.KC
@@ -194,7 +203,7 @@ def test_masked_columns_keep_their_concentrations(oracle_mod):
     dye["con"][:kc, L] = 1.0
     dye["con1"][:] = dye["con"]
     dye["rkdye"] = 1e-4
-    o = oracle_mod.Oracle(case, rsqm=1e-24, iterm=2000, isdry=1, hdry=HDRY)
+    o = oracle_mod.Oracle(case, rsqm=1e-24, iterm=2000, isdry=isdry, hdry=HDRY)
     lb = case.open_bc["side"][0]["L"] - 1
     masked_any = masked_bc = False
     for n in range(1, 401):
@@ -229,3 +238,22 @@ def test_uniform_concentration_stays_uniform_while_drying(oracle_mod):
         assert np.max(np.abs(c["con"][:kc, L] - 7.5)) <= 1e-11 * 7.5
     _, m1 = o.balance()
     np.testing.assert_allclose(m1, m0, rtol=1e-11)
+
+
+def test_rewetting_wets_cells_that_started_dry(oracle_mod):
+    """The sloshing basin floods its dry east bank: with ISDRY = 99 the
+    water rising into bank cells reopens their faces and they join the
+    wet cells; with ISDRY = 1 the faces reopened at the step start are
+    closed again by every re-solve and the bank stays dry."""
+    got = {}
+    for isdry in (1, 99):
+        case = _closed()
+        o = oracle_mod.Oracle(case, rsqm=1e-26, iterm=4000, isdry=isdry, hdry=HDRY)
+        L = wet_index(case)
+        start_dry = case.h["HP"][L] <= HDRY
+        wetted = np.zeros(len(L), bool)
+        for n in range(1, 401):
+            assert o.step(n)["status"] == 0
+            wetted |= start_dry & (o.lmaskdry[L] == 1) & (case.h["HP"][L] > HDRY)
+        got[isdry] = int(wetted.sum())
+    assert got[1] == 0 and got[99] > 0, got
