@@ -149,6 +149,7 @@ void efdc_params_default(efdc_params_t *p) {
   p->mixed_precond_degree = 0;
   p->tran_shape = 0;
   p->ISDRY = 0;
+  p->NDRYSTP = 0;
 }
 
 static bool tran_shape_valid(int q) {
@@ -293,7 +294,8 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
   std::memset(c->st_host, 0, sizeof(CgState));
   if ((s = dmalloc(c, &c->iscdry, LCM)) || (s = dmalloc(c, &c->lmaskdry, LCM)) ||
       (s = dmalloc(c, &c->sub1, LCM)) || (s = dmalloc(c, &c->svb1, LCM)) ||
-      (s = dmalloc(c, &c->lopenbcdry, LCM)) ||
+      (s = dmalloc(c, &c->lopenbcdry, LCM)) || (s = dmalloc(c, &c->natdry, LCM)) ||
+      (s = dmalloc(c, &c->qdwaste, LCM)) || (s = dmalloc(c, &c->vdwaste, LCM)) ||
       (s = dmalloc(c, &c->pbcflag, LCM)) || (s = dmalloc(c, &c->lobflag, LCM)) ||
       (s = dmalloc(c, &c->bcrec, LCM)) ||
       (s = dmalloc(c, &c->srcrec, LCM)) || (s = dmalloc(c, &c->cdzkk_k, 64)) ||
@@ -345,6 +347,7 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   delete[] c->tcp_host;
   dfree(c->iscdry); dfree(c->lmaskdry); dfree(c->sub1); dfree(c->svb1);
   dfree(c->sub_n); dfree(c->svb_n);
+  dfree(c->natdry); dfree(c->qdwaste); dfree(c->vdwaste);
   dfree(c->tide_dev); dfree(c->lopenbcdry); dfree(c->pbcflag); dfree(c->lobflag);
   dfree(c->lobcs); dfree(c->lobcs2);
   for (auto &b : c->cbc) {
@@ -682,6 +685,21 @@ int efdc_get_dry_state(efdc_ctx_t *c, int32_t *lmaskdry, int32_t *iscdry, int32_
                               c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (ncordry) *ncordry = c->ncordry;
+  return EFDC_OK;
+}
+
+int efdc_get_dry_waste(efdc_ctx_t *c, double *qdwaste, double *vdwaste, int32_t *natdry) {
+  if (!c) return EFDC_ERR_ARG;
+  if (qdwaste)
+    HIP_TRY(c, hipMemcpyAsync(qdwaste, c->qdwaste, sizeof(double) * c->LCM,
+                              hipMemcpyDeviceToHost, c->stream));
+  if (vdwaste)
+    HIP_TRY(c, hipMemcpyAsync(vdwaste, c->vdwaste, sizeof(double) * c->LCM,
+                              hipMemcpyDeviceToHost, c->stream));
+  if (natdry)
+    HIP_TRY(c, hipMemcpyAsync(natdry, c->natdry, sizeof(int) * c->LCM, hipMemcpyDeviceToHost,
+                              c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
   return EFDC_OK;
 }
 

@@ -147,6 +147,8 @@ struct efdc_ctx {
   int *iscdry = nullptr, *lmaskdry = nullptr;
   double *sub1 = nullptr, *svb1 = nullptr;
   double *sub_n = nullptr, *svb_n = nullptr;  // ISDRY = 99: new face switches of a pass
+  int *natdry = nullptr;                        // NDRYSTP < 0: isolated steps per cell
+  double *qdwaste = nullptr, *vdwaste = nullptr;  // drained rate / volume per cell
   int ncordry = 0;
   unsigned char *pbcflag = nullptr;   // (LCM) 1 on pressure-BC cells
   unsigned char *lobflag = nullptr;   // (LCM) bit 0 LOBCS, bit 1 LOBCS2

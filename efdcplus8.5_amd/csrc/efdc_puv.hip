@@ -933,10 +933,12 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_post3(
 // ---------------------------------------------------------------------
 // Step start, EFDC/calpuv2c.f90:169-177: ISCDRY = 0, SUB1/SVB1 = SUB/SVB.
 __global__ __launch_bounds__(EFDC_BLOCK) void k_dry_begin(HPtr h, int *iscdry, double *sub1,
-                                                          double *svb1, int nwet) {
+                                                          double *svb1, double *qdwaste,
+                                                          int nwet) {
   int tile;
   const int i = cell_index(nwet, tile);
   if (i > nwet) return;
+  if (qdwaste) qdwaste[i] = 0.;  // NDRYSTP < 0: the rate is per step (:181-188)
   iscdry[i] = 0;
   sub1[i] = HA(SUB)[i];
   svb1[i] = HA(SVB)[i];
@@ -1013,6 +1015,59 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_dry_cells(HPtr h, const int *le,
 }
 
 __global__ void k_dry_flag_reset(CgState *st) { st->drychg = 0; }
+
+// Draining of isolated wet cells, NDRYSTP < 0 (:838-893, 1010-1018): a wet
+// cell (HP >= HDRY) whose four faces are closed counts its isolated steps
+// in NATDRY (an open face resets it); past -NDRYSTP steps, if a face that
+// is open on the full grid leads to a lower bed, or HP < 2 HDRY, HP and
+// H1P go to 0.9 HDRY with the layer depths, and the volume taken out is
+// booked in QDWASTE (per second) and VDWASTE.  P keeps the value of the
+// last solve pass, as in the reference.  Each thread changes only its own
+// cell; the neighbours' beds and the face switches are only read.
+template <typename T>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_dry_waste(
+    HPtr h, const int *lw, const int *le, const int *ls, const int *ln, int *natdry,
+    double *qdwaste, double *vdwaste, T *hpk, T *hpki, T *h1pk, const T *dzc, int64_t pitch,
+    int KC, int nwet, double HDRY, int ntmp, double dt) {
+  int tile;
+  const int i = cell_index(nwet, tile);
+  if (i > nwet) return;
+  double *HP = HA(HP);
+  const double hp = HP[i];
+  if (!(hp >= HDRY)) return;
+  const int w = lw[i], e = le[i], s = ls[i], n = ln[i];
+  const double *SUB = HA(SUB), *SVB = HA(SVB);
+  if (SUB[i] + SUB[e] + SVB[i] + SVB[n] > 0.5) {
+    natdry[i] = 0;
+    return;
+  }
+  const int cnt = natdry[i] + 1;
+  natdry[i] = cnt;
+  if (cnt <= ntmp) return;
+  const double *SUBO = HA(SUBO), *SVBO = HA(SVBO), *BELV = HA(BELV);
+  const double b = BELV[i];
+  double RVAL = 0.;
+  if (SUBO[e] > 0.5 && BELV[e] < b) RVAL = RVAL + 1.;
+  if (SUBO[i] > 0.5 && BELV[w] < b) RVAL = RVAL + 1.;
+  if (SVBO[n] > 0.5 && BELV[n] < b) RVAL = RVAL + 1.;
+  if (SVBO[i] > 0.5 && BELV[s] < b) RVAL = RVAL + 1.;
+  if (!(RVAL > 0. || hp < HDRY * 2.)) return;
+  const double hn = 0.90 * HDRY;
+  HP[i] = hn;
+  HA(H1P)[i] = hn;
+  for (int k = 0; k < KC; ++k) {
+    const int64_t j = i + (int64_t)k * pitch;
+    const double dz = (double)dzc[j];
+    const double hk = hn * dz;
+    hpk[j] = (T)hk;
+    hpki[j] = (T)(1. / hk);
+    h1pk[j] = (T)(hn * dz);
+  }
+  natdry[i] = 0;
+  const double dxyp = HA(DXYP)[i];
+  qdwaste[i] = (1. / dt) * dxyp * (hp - hn);
+  vdwaste[i] = vdwaste[i] + dxyp * (hp - hn);
+}
 
 // Drying and rewetting, ISDRY = 99 (:604-716).  The reference walks the
 // cells in L order and updates the face switches in place; a cell's four
@@ -1373,7 +1428,9 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
   }
   k_step_begin<<<1, 1, 0, c->stream>>>(c->st, rot);
   if (dry) {
-    k_dry_begin<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->iscdry, c->sub1, c->svb1, nwet);
+    k_dry_begin<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->iscdry, c->sub1, c->svb1,
+                                                  c->prm.NDRYSTP < 0 ? c->qdwaste : nullptr,
+                                                  nwet);
     for (int q = 0; q < 4; ++q)
       if (c->pbc[q].n > 0)
         k_dry_obc_reset<<<(c->pbc[q].n + 255) / 256, 256, 0, c->stream>>>(c->pbc[q],
@@ -1479,6 +1536,20 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
       return EFDC_ERR_STATE;
     }
   }  // drying loop
+  if (dry && c->prm.NDRYSTP < 0) {
+    if (c->store == EFDC_STORE_FP64)
+      k_dry_waste<double><<<nb, EFDC_BLOCK, 0, c->stream>>>(
+          h, c->dLW, c->dLE, c->dLS, c->dLN, c->natdry, c->qdwaste, c->vdwaste,
+          (double *)c->dK[0], (double *)c->dK[EFDC_HPKI - EFDC_HPK],
+          (double *)c->dK[EFDC_H1PK - EFDC_HPK], (const double *)c->dK[EFDC_DZC - EFDC_HPK],
+          c->pitch, c->KC, nwet, c->prm.HDRY, -c->prm.NDRYSTP, dt);
+    else
+      k_dry_waste<float><<<nb, EFDC_BLOCK, 0, c->stream>>>(
+          h, c->dLW, c->dLE, c->dLS, c->dLN, c->natdry, c->qdwaste, c->vdwaste,
+          (float *)c->dK[0], (float *)c->dK[EFDC_HPKI - EFDC_HPK],
+          (float *)c->dK[EFDC_H1PK - EFDC_HPK], (const float *)c->dK[EFDC_DZC - EFDC_HPK],
+          c->pitch, c->KC, nwet, c->prm.HDRY, -c->prm.NDRYSTP, dt);
+  }
   TIMED(c, TK_PUV_POST3, k_puv_post3<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, c->st));
   if (dry)
     k_dry_mask<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->sub1, c->svb1,

@@ -47,7 +47,7 @@ class Params(C.Structure):
                 ("store", C.c_int32), ("mixed_inner_rtol", C.c_double),
                 ("mixed_max_outer", C.c_int32), ("cg_kernel", C.c_int32),
                 ("mixed_precond_degree", C.c_int32), ("tran_shape", C.c_int32),
-                ("ISDRY", C.c_int32)]
+                ("ISDRY", C.c_int32), ("NDRYSTP", C.c_int32)]
 
 
 class PbcSide(C.Structure):
@@ -156,6 +156,7 @@ def load(path: Optional[str] = None):
         "efdc_get_constituent": ([P, C.c_int32, PD, PD], C.c_int),
         "efdc_get_constituent_at": ([P, C.c_int32, C.c_int32, PI, PI, PD], C.c_int),
         "efdc_get_dry_state": ([P, PI, PI, PI], C.c_int),
+        "efdc_get_dry_waste": ([P, PD, PD, PI], C.c_int),
         "efdc_congrad": ([P, C.POINTER(CgStats)], C.c_int),
         "efdc_calpuv2c": ([P, C.POINTER(Step), C.POINTER(CgStats)], C.c_int),
         "efdc_calconc": ([P, C.POINTER(Step)], C.c_int),
@@ -377,6 +378,15 @@ class Context:
         self._check(self.lib.efdc_get_dry_state(
             self.h, _pi(m) if arrays else None, _pi(d) if arrays else None, _pi(k)))
         return m, d, int(k[0])
+
+    def get_dry_waste(self):
+        """(qdwaste, vdwaste, natdry) per cell in L order (index L-1): the
+        drained rate of the last step (m^3/s), the drained volume so far
+        (m^3) and the isolated-step counters (NDRYSTP < 0)."""
+        n = self.grid.LCM
+        q, v, k = np.zeros(n), np.zeros(n), np.zeros(n, np.int32)
+        self._check(self.lib.efdc_get_dry_waste(self.h, _pd(q), _pd(v), _pi(k)))
+        return q, v, k
 
     # ------------------------------------------------------------ calls
     def congrad(self) -> dict:

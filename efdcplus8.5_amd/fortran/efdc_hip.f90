@@ -54,7 +54,8 @@ module efdc_hip
     real(c_double) :: RSQM, cg_rel_tol
     integer(c_int32_t) :: cg_mode, store
     real(c_double) :: mixed_inner_rtol
-    integer(c_int32_t) :: mixed_max_outer, cg_kernel, mixed_precond_degree, tran_shape, ISDRY
+    integer(c_int32_t) :: mixed_max_outer, cg_kernel, mixed_precond_degree, tran_shape, ISDRY, &
+                          NDRYSTP
   end type
 
   type, bind(C) :: efdc_pbc_side_t
@@ -224,6 +225,14 @@ module efdc_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), intent(out) :: lmaskdry(*), iscdry(*)
       integer(c_int32_t), intent(out) :: ncordry
+    end function
+
+    integer(c_int) function efdc_get_dry_waste(ctx, qdwaste, vdwaste, natdry) &
+        bind(C, name="efdc_get_dry_waste")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: qdwaste(*), vdwaste(*)
+      integer(c_int32_t), intent(out) :: natdry(*)
     end function
 
     integer(c_int) function efdc_congrad(ctx, stats) bind(C, name="efdc_congrad")

@@ -6,8 +6,9 @@
 ! Each routine copies the MODULE GLOBAL inputs it needs to the device,
 ! runs the hot path there and copies back every output the host reads
 ! afterwards.  Scope: the 2TL scheme with any ISDRY (no drying, the drying
-! with the re-solve loop, or drying and rewetting), NDRYSTP >= 0,
-! IGRIDV=0, MDCHH=0, ISGWIE=0, ISDYNSTP=0 and sigma layers; other settings
+! with the re-solve loop, or drying and rewetting, with or without the
+! draining of isolated cells of NDRYSTP < 0), IGRIDV=0, MDCHH=0, ISGWIE=0,
+! ISDYNSTP=0 and sigma layers; other settings
 ! stop at the first call with a message, like the reference's own
 ! unsupported-option stops.
 module efdc_hip_state
@@ -47,9 +48,8 @@ subroutine EFDC_HIP_INIT
   integer(c_int32_t) :: HXSLOT
   integer :: NS
 
-  if (ISDRY < 0 .or. (ISDRY > 0 .and. NDRYSTP < 0) .or. IGRIDV /= 0 .or. MDCHH /= 0 &
-      .or. ISGWIE /= 0 .or. ISDYNSTP /= 0) then
-    stop 'EFDC_HIP: needs NDRYSTP>=0 with drying, IGRIDV=0, MDCHH=0, ISGWIE=0, ISDYNSTP=0'
+  if (ISDRY < 0 .or. IGRIDV /= 0 .or. MDCHH /= 0 .or. ISGWIE /= 0 .or. ISDYNSTP /= 0) then
+    stop 'EFDC_HIP: needs IGRIDV=0, MDCHH=0, ISGWIE=0, ISDYNSTP=0'
   end if
   allocate(MW(LCM), ME(LCM), MS(LCM), MN(LCM), MNW(LCM), MSE(LCM))
   allocate(MASKI(LCM), DRYI(LCM))
@@ -61,7 +61,7 @@ subroutine EFDC_HIP_INIT
   call efdc_params_default(HXPRM)
   HXPRM%G = G; HXPRM%RITB1 = RITB1; HXPRM%SNLT = SNLT; HXPRM%HDRY = HDRY
   HXPRM%TCON_TBEGIN = TCON*TBEGIN; HXPRM%ITERM = ITERM; HXPRM%RSQM = RSQM
-  HXPRM%ISDRY = ISDRY
+  HXPRM%ISDRY = ISDRY; HXPRM%NDRYSTP = NDRYSTP
   HXERR = efdc_ctx_create(HXGRID, HXPRM, HCTX)
   call EFDC_HIP_CHECK(HXERR, 'efdc_ctx_create')
   ! geometry and layer constants (EFDC/aaefdc.f90:1212-1232,2172-2185)
@@ -257,6 +257,16 @@ SUBROUTINE CALPUV2C
     LADRY = 0
   else
     call PULL(EFDC_SUB, SUB); call PULL(EFDC_SVB, SVB)
+    if (NDRYSTP < 0) then
+      ! drained cells (calpuv2c.f90:838-893,1010-1021): depths of all
+      ! levels, the drained rate and volume, the isolation counters
+      call PULL(EFDC_HPK, HPK); call PULL(EFDC_HPKI, HPKI)
+      call EFDC_HIP_CHECK(efdc_get_dry_waste(HCTX, QDWASTE, VDWASTE, DRYI), 'efdc_get_dry_waste')
+      do L = 2, LA
+        NATDRY(L) = DRYI(L)
+        if (ISBAL == 0) QDWASTE(L) = 0.0
+      end do
+    end if
     call EFDC_HIP_CHECK(efdc_get_dry_state(HCTX, MASKI, DRYI, HXNCOR), 'efdc_get_dry_state')
     do L = 2, LA
       LMASKDRY(L) = MASKI(L) /= 0

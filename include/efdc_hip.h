@@ -116,13 +116,19 @@ typedef struct {
                               (:916-939); 98 open-boundary drying only; 99
                               drying and rewetting (:604-716, needs mutual
                               neighbour maps)                               */
+  int32_t NDRYSTP;         /* with ISDRY > 0 and NDRYSTP < 0: a wet cell
+                              isolated by dry neighbours for more than
+                              -NDRYSTP steps is drained to 0.9 HDRY when
+                              it is shallow (< 2 HDRY) or above a
+                              neighbour's bed (:838-893, 1010-1018); >= 0:
+                              no draining                                   */
 } efdc_params_t;
 
 /* Fill *p with the values EFDC uses by default: G=9.81, RITB1=1, SNLT=0,
    HDRY=0.2, TCON_TBEGIN=0, ITERM=200, RSQM=1e-9 (the sample decks,
    SampleModels/Lake_Washington/efdc.inp:92), cg_rel_tol=0, FP64 CG, FP64
    storage, mixed_inner_rtol=1e-5, mixed_max_outer=30, cg_kernel=AUTO,
-   mixed_precond_degree=0, tran_shape=0, ISDRY=0. */
+   mixed_precond_degree=0, tran_shape=0, ISDRY=0, NDRYSTP=0. */
 void efdc_params_default(efdc_params_t *p);
 
 /* ------------------------------------------------------------- array ids */
@@ -310,6 +316,13 @@ int efdc_get_constituent_at(efdc_ctx_t *ctx, int32_t slot, int32_t n, const int3
    ISCDRY per cell (L order, LCM entries; either may be NULL) and the
    number of re-solves NCORDRY of that step. */
 int efdc_get_dry_state(efdc_ctx_t *ctx, int32_t *lmaskdry, int32_t *iscdry, int32_t *ncordry);
+
+/* Drained volume with NDRYSTP < 0 (L order, LCM entries, any may be NULL):
+   QDWASTE, the rate of the last step (m^3/s, 0 where nothing was drained;
+   the reference keeps it only with ISBAL > 0), VDWASTE, the volume drained
+   since the context was created (m^3), and NATDRY, the steps each wet cell
+   has been isolated (EFDC/calpuv2c.f90:838-893). */
+int efdc_get_dry_waste(efdc_ctx_t *ctx, double *qdwaste, double *vdwaste, int32_t *natdry);
 
 /* CONGRAD (EFDC/congrad.f90:1-191) on the scaled system already held in
    CCC, CCS, CCW, CCE, CCN, CCCI, FPTMP, starting from and updating P. */

@@ -577,6 +577,7 @@ int orc_calpuv2c(orc_ctx *o, double dt, int64_t n, orc_cg_out *out) {
       o->iscdry[i] = 0;
       o->sub1[i] = SUB[i];
       o->svb1[i] = SVB[i];
+      if (o->NDRYSTP < 0) o->qdwaste[i] = 0.; /* :181-188 (ISBAL > 0) */
     }
   }
   explicit_stage(o, dt, n);
@@ -633,6 +634,47 @@ int orc_calpuv2c(orc_ctx *o, double dt, int64_t n, orc_cg_out *out) {
       st = EFDC_ERR_STATE;               /* '*** NCORDRY > 500' */
     }
     break;
+  }
+  /* volume wasting, NDRYSTP < 0 (:838-893, then :1010-1018): a wet cell
+     whose four faces are closed counts its isolated steps; past |NDRYSTP|,
+     if an originally open face leads to a lower bed or HP < 2 HDRY, its
+     depth goes to 0.9 HDRY (H1P and the layer depths too; P stays) and the
+     volume taken is booked in QDWASTE (rate) and VDWASTE (total) */
+  if (isdry > 0 && o->NDRYSTP < 0) {
+    const int ntmp = -o->NDRYSTP;
+    const double HDRY = o->HDRY, DELTI = 1. / dt;
+    const double *SUBO = A(EFDC_SUBO), *SVBO = A(EFDC_SVBO), *DZC = A(EFDC_DZC);
+    double *H1PK = A(EFDC_H1PK), *HPK = A(EFDC_HPK), *HPKI = A(EFDC_HPKI);
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L);
+      if (!(HP[i] >= HDRY)) continue;
+      const size_t w = HX(o->LWC[i]), e = HX(o->LEC[i]), s = HX(o->LSC[i]),
+                   nn = HX(o->LNC[i]);
+      const double RDRY = SUB[i] + SUB[e] + SVB[i] + SVB[nn];
+      if (RDRY > 0.5) {
+        o->natdry[i] = 0;
+        continue;
+      }
+      o->natdry[i] = o->natdry[i] + 1;
+      if (o->natdry[i] <= ntmp) continue;
+      double RVAL = 0.;
+      if (SUBO[e] > 0.5 && BELV[e] < BELV[i]) RVAL = RVAL + 1.;
+      if (SUBO[i] > 0.5 && BELV[w] < BELV[i]) RVAL = RVAL + 1.;
+      if (SVBO[nn] > 0.5 && BELV[nn] < BELV[i]) RVAL = RVAL + 1.;
+      if (SVBO[i] > 0.5 && BELV[s] < BELV[i]) RVAL = RVAL + 1.;
+      if (!(RVAL > 0. || HP[i] < HDRY * 2.)) continue;
+      const double HOLDTMP = HP[i];
+      HP[i] = 0.90 * HDRY;
+      H1P[i] = 0.90 * HDRY; /* :1012-1016 */
+      for (int K = 1; K <= o->KC; ++K) {
+        HPK[KX(L, K)] = HP[i] * DZC[KX(L, K)];
+        HPKI[KX(L, K)] = 1. / HPK[KX(L, K)];
+        H1PK[KX(L, K)] = H1P[i] * DZC[KX(L, K)];
+      }
+      o->natdry[i] = 0;
+      o->qdwaste[i] = DELTI * DXYP[i] * (HOLDTMP - HP[i]);
+      o->vdwaste[i] = o->vdwaste[i] + DXYP[i] * (HOLDTMP - HP[i]);
+    }
   }
   /* final updates, :896-914 (FSGZUDXYPI = 0.5/DXYU, :110-117); with
      drying on, P was updated in the loop (:896-900) */

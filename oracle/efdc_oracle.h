@@ -69,6 +69,11 @@ typedef struct {
   int32_t *lmaskdry;     /* (LCM) LMASKDRY, 1 = wet */
   double *sub1, *svb1;   /* (LCM) SUB1, SVB1: face switches at the step start */
   int32_t ncordry;       /* re-solves of the last CALPUV2C */
+  /* volume wasting of isolated wet cells, NDRYSTP < 0 (:838-893) */
+  int32_t NDRYSTP;
+  int32_t *natdry;       /* (LCM) NATDRY: steps a wet cell has been isolated */
+  double *qdwaste;       /* (LCM) QDWASTE of the last step (m^3/s) */
+  double *vdwaste;       /* (LCM) VDWASTE accumulated (m^3) */
 } orc_ctx;
 
 typedef struct {

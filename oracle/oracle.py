@@ -62,7 +62,8 @@ class OrcCtx(C.Structure):
                 ("qsercell", PD), ("cqs", PD), ("csert_q", PD),
                 ("nconst", C.c_int32), ("c", Const * MAXC),
                 ("ISDRY", C.c_int32), ("iscdry", PI), ("lmaskdry", PI),
-                ("sub1", PD), ("svb1", PD), ("ncordry", C.c_int32)]
+                ("sub1", PD), ("svb1", PD), ("ncordry", C.c_int32),
+                ("NDRYSTP", C.c_int32), ("natdry", PI), ("qdwaste", PD), ("vdwaste", PD)]
 
 
 class CgOut(C.Structure):
@@ -121,7 +122,8 @@ class Oracle:
     """Binds a Case to the C oracle; arrays are shared, not copied."""
 
     def __init__(self, case, *, rsqm=1e-9, iterm=200, nthreads=1, g=9.81,
-                 ritb1=1.0, snlt=0.0, hdry=0.2, tcon_tbegin=0.0, rel_tol=0.0, isdry=0):
+                 ritb1=1.0, snlt=0.0, hdry=0.2, tcon_tbegin=0.0, rel_tol=0.0, isdry=0,
+                 ndrystp=0):
         self.case = case
         g_ = case.grid
         o = OrcCtx()
@@ -147,6 +149,11 @@ class Oracle:
         self.svb1 = np.zeros(g_.LCM)
         o.iscdry, o.lmaskdry = _pi(self.iscdry), _pi(self.lmaskdry)
         o.sub1, o.svb1 = _pd(self.sub1), _pd(self.svb1)
+        o.NDRYSTP = ndrystp
+        self.natdry = np.zeros(g_.LCM, np.int32)
+        self.qdwaste = np.zeros(g_.LCM)
+        self.vdwaste = np.zeros(g_.LCM)
+        o.natdry, o.qdwaste, o.vdwaste = _pi(self.natdry), _pd(self.qdwaste), _pd(self.vdwaste)
         obc = case.open_bc
         if obc is not None:
             mt = int(obc["mtide"])

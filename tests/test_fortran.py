@@ -32,7 +32,7 @@ STRUCTS = {
     "efdc_params_t": ["G", "RITB1", "SNLT", "HDRY", "TCON_TBEGIN", "ITERM",
                       "RSQM", "cg_rel_tol", "cg_mode", "store",
                       "mixed_inner_rtol", "mixed_max_outer", "cg_kernel",
-                      "mixed_precond_degree", "tran_shape", "ISDRY"],
+                      "mixed_precond_degree", "tran_shape", "ISDRY", "NDRYSTP"],
     "efdc_pbc_side_t": ["n", "L", "ispb", "pser", "pcb", "psb"],
     "efdc_open_bc_t": ["side", "mtide", "tcp"],
     "efdc_cbc_side_t": ["n", "L", "cb", "csert", "ntscr"],

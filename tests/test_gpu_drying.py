@@ -134,3 +134,31 @@ def test_rewetting_closed_basin_matches_oracle(oracle_mod):
         assert np.array_equal(ctx.get(key)[L], ref.h[key][L]), key
     assert np.max(np.abs(ctx.get("HP")[L] - ref.h["HP"][L])) <= 1e-9
     ctx.close()
+
+
+@pytest.mark.parametrize("isdry", [1, 99])
+def test_draining_matches_oracle(oracle_mod, isdry):
+    """NDRYSTP < 0: the isolated shallow pit is drained on the same step,
+    with the same rate, volume, counters and depths as in the oracle."""
+    import test_oracle_drying as t
+    from efdc_amd.lib import Context
+    case, (shallow, deep) = t._puddles()
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref, rsqm=1e-26, iterm=4000, isdry=isdry, hdry=HDRY, ndrystp=-5)
+    ctx = Context(case.grid, RSQM=1e-26, ITERM=4000, ISDRY=isdry, HDRY=HDRY,
+                  NDRYSTP=-5).upload_case(case)
+    L = wet_index(case)
+    kc = case.grid.KC
+    for n in range(1, 21):
+        o.step(n)
+        ctx.step(case.dt, n)
+        q, v, k = ctx.get_dry_waste()
+        assert np.array_equal(k[L], o.natdry[L]), n
+        assert np.max(np.abs(q[L] - o.qdwaste[L])) <= 1e-9 * max(1.0, o.qdwaste.max()), n
+        assert np.max(np.abs(v[L] - o.vdwaste[L])) <= 1e-9 * max(1.0, o.vdwaste.max()), n
+        assert np.max(np.abs(ctx.get("HP")[L] - ref.h["HP"][L])) <= 1e-9, n
+    assert v[shallow] > 0 and v[deep] == 0
+    h1pk = ctx.get("H1PK")
+    assert np.array_equal(h1pk[:kc, shallow], ref.k["H1PK"][:kc, shallow])
+    assert np.max(np.abs(h1pk[:kc, L] - ref.k["H1PK"][:kc, L])) <= 1e-9
+    ctx.close()

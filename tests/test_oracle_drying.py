@@ -257,3 +257,67 @@ def test_rewetting_wets_cells_that_started_dry(oracle_mod):
             wetted |= start_dry & (o.lmaskdry[L] == 1) & (case.h["HP"][L] > HDRY)
         got[isdry] = int(wetted.sum())
     assert got[1] == 0 and got[99] > 0, got
+
+
+def _puddles():
+    """The closed basin with two wet cells inside its dry east bank: a pit
+    0.3 m deep (less than 2 HDRY) and a pit 0.6 m deep, each with four dry
+    neighbours.  Returns the case and the two cells (0-based L-1)."""
+    from efdc_amd import grids
+    c = _closed()
+    g, h, k = c.grid, c.h, c.k
+    L = wet_index(c)
+    jj, ii = g.JL[L], g.IL[L]
+    mid = jj == jj[len(L) // 2]
+    cells = []
+    for col, depth in ((ii.max() - 2, 0.3), (ii.max() - 5, 0.6)):
+        i = int(L[mid & (ii == col)][0])
+        h["BELV"][i] -= 0.2
+        h["HP"][i] = h["H1P"][i] = h["H2P"][i] = depth
+        h["P"][i] = h["P1"][i] = G * (depth + h["BELV"][i])
+        cells.append(i)
+    grids.update_depth_arrays(g, h)
+    grids.layer_depths(g, h, k)
+    k["H1PK"][:] = k["HPK"]
+    return c, cells
+
+
+@pytest.mark.parametrize("isdry", [1, 99])
+def test_isolated_shallow_cells_are_wasted(oracle_mod, isdry):
+    """NDRYSTP < 0 (EFDC/calpuv2c.f90:838-893): a wet cell cut off by dry
+    neighbours counts its isolated steps; after |NDRYSTP| of them the
+    shallow pit (HP < 2 HDRY, no lower neighbour) is set to 0.9 HDRY and
+    its volume booked as wasted, the deep pit stays; water in the basin
+    plus wasted volume is conserved."""
+    case, (shallow, deep) = _puddles()
+    nd = 5
+    o = oracle_mod.Oracle(case, rsqm=1e-26, iterm=4000, isdry=isdry, hdry=HDRY,
+                          ndrystp=-nd)
+    L = wet_index(case)
+    dxyp = case.h["DXYP"]
+    v0 = float(np.sum(dxyp[L] * case.h["HP"][L]))
+    wasted_at = None
+    for n in range(1, 21):
+        hold = case.h["HP"][shallow]
+        assert o.step(n)["status"] == 0
+        if o.qdwaste[shallow] > 0 and wasted_at is None:
+            wasted_at = n
+            assert case.h["HP"][shallow] == 0.9 * HDRY
+            assert case.h["H1P"][shallow] == 0.9 * HDRY
+            vol = dxyp[shallow] * (hold - 0.9 * HDRY)
+            assert np.isclose(o.vdwaste[shallow], vol, rtol=1e-12)
+            assert np.isclose(o.qdwaste[shallow], vol / case.dt, rtol=1e-12)
+        v = float(np.sum(dxyp[L] * case.h["HP"][L])) + float(o.vdwaste[L].sum())
+        assert abs(v - v0) <= 1e-11 * v0, n
+    # isolated from step 1 on, counted to nd, wasted on the next step
+    assert wasted_at == nd + 1
+    assert o.vdwaste[deep] == 0.0 and case.h["HP"][deep] > 2 * HDRY
+    assert o.natdry[deep] > nd
+
+
+def test_no_wasting_without_negative_ndrystp(oracle_mod):
+    case, (shallow, _) = _puddles()
+    o = oracle_mod.Oracle(case, rsqm=1e-26, iterm=4000, isdry=1, hdry=HDRY, ndrystp=5)
+    for n in range(1, 21):
+        assert o.step(n)["status"] == 0
+    assert case.h["HP"][shallow] > HDRY and not o.vdwaste.any()
