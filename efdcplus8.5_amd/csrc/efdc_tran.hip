@@ -936,6 +936,10 @@ struct AdacArgs {
   int64_t pitch;
   double dt;
   T *uuuu, *vvvv, *wwww, *fu, *fv, *fw, *duu, *dvv;
+  // drying: SUB/SVB scale SUB3D/SVB3D (which hold SUB3DO/SVB3DO,
+  // EFDC/calpuv2c.f90:982-990), LMASKDRY leaves columns out; null without
+  const double *sub, *svb;
+  const int *lmask;
 };
 
 template <typename T>
@@ -1082,8 +1086,14 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_fct_beta(AdacArgs<T> a) {
   cont_mx_mn(a, LE, o, mxE, mnE);
   cont_mx_mn(a, LS, o, mxS, mnS);
   cont_mx_mn(a, LN, o, mxN, mnN);
-  const double sbi = (double)a.sub3d[i + o], sbe = (double)a.sub3d[LE + o];
-  const double svi = (double)a.svb3d[i + o], svn = (double)a.svb3d[LN + o];
+  double sbi = (double)a.sub3d[i + o], sbe = (double)a.sub3d[LE + o];
+  double svi = (double)a.svb3d[i + o], svn = (double)a.svb3d[LN + o];
+  if (a.sub) {
+    sbi = a.sub[i] * sbi;
+    sbe = a.sub[LE] * sbe;
+    svi = a.svb[i] * svi;
+    svn = a.svb[LN] * svn;
+  }
   const double CWMAX = sbi * mxW, CEMAX = sbe * mxE, CSMAX = svi * mxS, CNMAX = svn * mxN;
   double CMAXT = dmx(CNMAX, CEMAX);
   CMAXT = dmx(CMAXT, CSMAX);
@@ -1165,6 +1175,7 @@ __global__ __launch_bounds__(TBLOCK) void k_col_finish(FinishArgs<T> f) {
   const int TB = blockDim.x;
   const int i = 1 + xcd_tile(blockIdx.x, gridDim.x) * TB + (int)threadIdx.x;
   if (i > a.nwet) return;
+  if (a.lmask && a.lmask[i] == 0) return;  // CON = CON1 from the column kernel
   const int tid = threadIdx.x, KC = a.KC;
   const int64_t p = a.pitch;
   double *vcu = (double *)smem;
@@ -1261,6 +1272,10 @@ int adac_pipeline(efdc_ctx *c, TranArgs<T> &ta, const efdc_step_t *step) {
   a.qsum = kp<T>(c, EFDC_QSUM);
   a.sub3d = kp<T>(c, EFDC_SUB3D);
   a.svb3d = kp<T>(c, EFDC_SVB3D);
+  const bool dry = c->prm.ISDRY > 0;
+  a.sub = dry ? c->dH[EFDC_SUB] : nullptr;
+  a.svb = dry ? c->dH[EFDC_SVB] : nullptr;
+  a.lmask = dry ? c->lmaskdry : nullptr;
   a.hpk = kp<T>(c, EFDC_HPK);
   a.hpki = ta.hpki;
   a.dxiu = c->dH[EFDC_DXIU];
@@ -1358,10 +1373,6 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
       c->err = "FCT needs KC >= 2 (EFDC/caltran.f90:637-640 reads layer 2)";
       return EFDC_ERR_ARG;
     }
-  }
-  if (any_adac && c->prm.ISDRY > 0) {
-    c->err = "anti-diffusion (ISADAC) with drying (ISDRY > 0) is not built";
-    return EFDC_ERR_ARG;
   }
   TranArgs<T> a;
   std::memset(&a, 0, sizeof(a));

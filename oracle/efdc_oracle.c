@@ -1032,27 +1032,36 @@ static void orc_caltran(orc_ctx *o, int c, double dt, int64_t niter,
         const double CMINT = dmin(s->CONTMN[KX(L, K - 1)], s->CONTMN[KX(L, K + 1)]);
         s->CMIN[KX(L, K)] = dmin(s->CONTMN[KX(L, K)], CMINT);
       }
+    /* with drying, SUB3D = SUB*SUB3DO (EFDC/calpuv2c.f90:982-990): the
+       SUB3D/SVB3D arrays hold SUB3DO/SVB3DO and are scaled here */
+    const double *SUBh = A(EFDC_SUB), *SVBh = A(EFDC_SVB);
+    const int dry3d = o->ISDRY > 0;
+#define SB3(L_, K_) (dry3d ? SUBh[HX(L_)] * SUB3D[KX(L_, K_)] : SUB3D[KX(L_, K_)])
+#define SV3(L_, K_) (dry3d ? SVBh[HX(L_)] * SVB3D[KX(L_, K_)] : SVB3D[KX(L_, K_)])
     for (int K = 1; K <= KC; ++K)
       for (int L = 2; L <= LA; ++L) {
         const size_t i = HX(L);
         const int LE = LEC[i], LS = LSC[i], LN = LNC[i], LW = LWC[i];
-        const double CWMAX = SUB3D[KX(L, K)] * s->CONTMX[KX(LW, K)];
-        const double CEMAX = SUB3D[KX(LE, K)] * s->CONTMX[KX(LE, K)];
-        const double CSMAX = SVB3D[KX(L, K)] * s->CONTMX[KX(LS, K)];
-        const double CNMAX = SVB3D[KX(LN, K)] * s->CONTMX[KX(LN, K)];
+        const double sbi = SB3(L, K), sbe = SB3(LE, K), svi = SV3(L, K), svn = SV3(LN, K);
+        const double CWMAX = sbi * s->CONTMX[KX(LW, K)];
+        const double CEMAX = sbe * s->CONTMX[KX(LE, K)];
+        const double CSMAX = svi * s->CONTMX[KX(LS, K)];
+        const double CNMAX = svn * s->CONTMX[KX(LN, K)];
         double CMAXT = dmax(CNMAX, CEMAX);
         CMAXT = dmax(CMAXT, CSMAX);
         CMAXT = dmax(CMAXT, CWMAX);
         s->CMAX[KX(L, K)] = dmax(s->CMAX[KX(L, K)], CMAXT);
-        const double CWMIN = SUB3D[KX(L, K)] * s->CONTMN[KX(LW, K)] + 1.E+6 * (1. - SUB3D[KX(L, K)]);
-        const double CEMIN = SUB3D[KX(LE, K)] * s->CONTMN[KX(LE, K)] + 1.E+6 * (1. - SUB3D[KX(LE, K)]);
-        const double CSMIN = SVB3D[KX(L, K)] * s->CONTMN[KX(LS, K)] + 1.E+6 * (1. - SVB3D[KX(L, K)]);
-        const double CNMIN = SVB3D[KX(LN, K)] * s->CONTMN[KX(LN, K)] + 1.E+6 * (1. - SVB3D[KX(LN, K)]);
+        const double CWMIN = sbi * s->CONTMN[KX(LW, K)] + 1.E+6 * (1. - sbi);
+        const double CEMIN = sbe * s->CONTMN[KX(LE, K)] + 1.E+6 * (1. - sbe);
+        const double CSMIN = svi * s->CONTMN[KX(LS, K)] + 1.E+6 * (1. - svi);
+        const double CNMIN = svn * s->CONTMN[KX(LN, K)] + 1.E+6 * (1. - svn);
         double CMINT = dmin(CNMIN, CEMIN);
         CMINT = dmin(CMINT, CSMIN);
         CMINT = dmin(CMINT, CWMIN);
         s->CMIN[KX(L, K)] = dmin(s->CMIN[KX(L, K)], CMINT);
       }
+#undef SB3
+#undef SV3
     for (int K = 1; K <= KC; ++K) /* :683-691 */
       for (int L = 2; L <= LA; ++L) {
         s->FUHVD[KX(L, K)] = dmin(s->FUHUD[KX(L, K)], 0.);
@@ -1108,9 +1117,10 @@ static void orc_caltran(orc_ctx *o, int c, double dt, int64_t niter,
     for (int q = 0; q < nob2; ++q) /* :761-766 */
       for (int K = 1; K <= KS; ++K) s->FWUU[ZX(lob2[q], K)] = 0.0;
   }
-  for (int K = 1; K <= KC; ++K) /* :771-778 */
+  for (int K = 1; K <= KC; ++K) /* :771-778, wet list */
     for (int L = 2; L <= LA; ++L) {
       const size_t i = HX(L);
+      if (wet && !wet[i]) continue;
       s->CD[KX(L, K)] = CON[KX(L, K)] * HPK[KX(L, K)] +
           DDELT * ((s->FUHUD[KX(L, K)] - s->FUHUD[KX(LEC[i], K)] + s->FVHUD[KX(L, K)] -
                     s->FVHUD[KX(LNC[i], K)]) * DXYIP[i] +
@@ -1228,8 +1238,6 @@ void orc_settle(orc_ctx *o, double dtsed) {
 
 int orc_calconc(orc_ctx *o, double dt, int64_t niter) {
   const int nc = o->nconst;
-  for (int c = 0; c < nc; ++c)
-    if (o->ISDRY > 0 && o->c[c].isadac > 0) return EFDC_ERR_ARG; /* not restated */
   const int nt = o->nthreads > 0 ? o->nthreads : 1;
   /* one constituent per thread, EFDC/calconc.f90:159-229 */
 #pragma omp parallel for num_threads(nt < nc ? nt : (nc > 0 ? nc : 1)) schedule(static, 1)

@@ -162,3 +162,29 @@ def test_draining_matches_oracle(oracle_mod, isdry):
     assert np.array_equal(h1pk[:kc, shallow], ref.k["H1PK"][:kc, shallow])
     assert np.max(np.abs(h1pk[:kc, L] - ref.k["H1PK"][:kc, L])) <= 1e-9
     ctx.close()
+
+
+@pytest.mark.parametrize("isdry", [1, 99])
+def test_antidiffusion_with_drying_matches_oracle(oracle_mod, isdry):
+    """ISADAC = 1 with FCT on the drying basin: the FCT limiter sees the
+    closed faces through SUB3D = SUB*SUB3DO, masked columns are left out."""
+    from efdc_amd import grids
+    from efdc_amd.lib import Context
+    case = _stratified(grids.drying_basin(kc=3, deep=0.6, isadac=1, isfct=1))
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref, rsqm=1e-24, iterm=2000, isdry=isdry, hdry=HDRY)
+    ctx = Context(case.grid, RSQM=1e-24, ITERM=2000, ISDRY=isdry, HDRY=HDRY).upload_case(case)
+    L = wet_index(case)
+    kc = case.grid.KC
+    for n in range(1, 301):
+        o.step(n)
+        ctx.step(case.dt, n)
+        m, d, k = ctx.get_dry_state()
+        assert k == o.o.ncordry and np.array_equal(m[L], o.lmaskdry[L]), n
+        if n % 50 == 0:
+            for j, c in enumerate(ref.consts):
+                con, _ = ctx.get_constituent(j)
+                scale = max(1e-30, np.max(np.abs(c["con"][:kc])))
+                assert np.max(np.abs(con[:kc] - c["con"][:kc])) / scale <= 1e-9, (n, j)
+    assert np.any(m[L] == 0)
+    ctx.close()

@@ -189,10 +189,12 @@ def test_resolve_loop_ends_at_a_fixed_point(oracle_mod, nsteps):
     assert np.max(np.abs(again.h["HP"][L] - case.h["HP"][L])) <= 1e-10
 
 
-@pytest.mark.parametrize("isdry", [1, 99])
-def test_masked_columns_keep_their_concentrations(oracle_mod, isdry):
+@pytest.mark.parametrize("isdry,adac", [(1, 0), (99, 0), (1, 1), (99, 1)])
+def test_masked_columns_keep_their_concentrations(oracle_mod, isdry, adac):
+    """adac = 1: anti-diffusion with FCT, whose limiter sees the closed
+    faces through SUB3D = SUB*SUB3DO."""
     from efdc_amd import grids
-    case = grids.drying_basin(kc=3, deep=0.6)
+    case = grids.drying_basin(kc=3, deep=0.6, isadac=adac, isfct=adac)
     kc = case.grid.KC
     L = wet_index(case)
     sal, dye = case.consts
