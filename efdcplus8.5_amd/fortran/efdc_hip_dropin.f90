@@ -87,8 +87,14 @@ subroutine EFDC_HIP_INIT
   HXERR = efdc_set_array(HCTX, EFDC_CDZKK, CDZKK); call EFDC_HIP_CHECK(HXERR, 'CDZKK')
   HXERR = efdc_set_array(HCTX, EFDC_CDZKMK, CDZKMK); call EFDC_HIP_CHECK(HXERR, 'CDZKMK')
   HXERR = efdc_set_array(HCTX, EFDC_DZIG, DZIG);   call EFDC_HIP_CHECK(HXERR, 'DZIG')
-  HXERR = efdc_set_array(HCTX, EFDC_SUB3D, SUB3D); call EFDC_HIP_CHECK(HXERR, 'SUB3D')
-  HXERR = efdc_set_array(HCTX, EFDC_SVB3D, SVB3D); call EFDC_HIP_CHECK(HXERR, 'SVB3D')
+  if (ISDRY > 0) then
+    ! with drying the library scales the full-grid layer switches by SUB/SVB
+    HXERR = efdc_set_array(HCTX, EFDC_SUB3D, SUB3DO); call EFDC_HIP_CHECK(HXERR, 'SUB3DO')
+    HXERR = efdc_set_array(HCTX, EFDC_SVB3D, SVB3DO); call EFDC_HIP_CHECK(HXERR, 'SVB3DO')
+  else
+    HXERR = efdc_set_array(HCTX, EFDC_SUB3D, SUB3D); call EFDC_HIP_CHECK(HXERR, 'SUB3D')
+    HXERR = efdc_set_array(HCTX, EFDC_SVB3D, SVB3D); call EFDC_HIP_CHECK(HXERR, 'SVB3D')
+  end if
   ! transported constituents in the order CALCONC calls CALTRAN
   ! (EFDC/calconc.f90:159-228)
   NSLOT = 0
