@@ -1,4 +1,4 @@
-// efdc_puv.hip -- CALPUV2C (2TL, ISDRY=0) and CONGRAD on gfx950.
+// efdc_puv.hip -- CALPUV2C (2TL, with the drying passes of ISDRY > 0) and CONGRAD on gfx950.
 //
 // One thread per wet cell, 256-thread workgroups, XCD-aware tile order.
 // Every kernel here is a streaming pass over L-ordered (LCM) FP64 arrays;
@@ -929,7 +929,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_post3(
 }
 
 // ---------------------------------------------------------------------
-// Drying, ISDRY > 0 (0 < ISDRY < 98 for the cell drying and re-solve).
+// Drying, ISDRY > 0: open boundaries (all), cells (1..97), cells with
+// rewetting (99), draining of isolated cells (NDRYSTP < 0), LMASKDRY.
 // ---------------------------------------------------------------------
 // Step start, EFDC/calpuv2c.f90:169-177: ISCDRY = 0, SUB1/SVB1 = SUB/SVB.
 __global__ __launch_bounds__(EFDC_BLOCK) void k_dry_begin(HPtr h, int *iscdry, double *sub1,

@@ -284,8 +284,8 @@ static void orc_setopenbc2(orc_ctx *o) {
 }
 
 /* ===================================================================== */
-/* CALPUV2C assembly: EFDC/calpuv2c.f90:128-136,164-398 (ISDRY=0,         */
-/* BSC=0, ISITB=0, ISVEG=0, MDCHH=0, ISGWIE=0)                             */
+/* CALPUV2C assembly: EFDC/calpuv2c.f90:128-136,164-398 (BSC=0, ISITB=0,  */
+/* ISVEG=0, MDCHH=0, ISGWIE=0; the drying passes follow further down)      */
 /* ===================================================================== */
 /* The part of CALPUV2C before label 1000: the explicit stage, the
    time-level shifts and FP1 (EFDC/calpuv2c.f90:214-290). */

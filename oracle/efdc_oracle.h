@@ -85,7 +85,7 @@ typedef struct {
 
 /* CONGRAD on the scaled system held in a[]: EFDC/congrad.f90:54-187 */
 int orc_congrad(orc_ctx *o, orc_cg_out *out);
-/* CALPUV2C, 2TL, ISDRY=0: EFDC/calpuv2c.f90:128-571,896-914 */
+/* CALPUV2C, 2TL: EFDC/calpuv2c.f90:128-571,574-939 (drying with ISDRY > 0) */
 int orc_calpuv2c(orc_ctx *o, double dt, int64_t n, orc_cg_out *out);
 /* Only the assembly part (explicit stage .. scaling), for unit tests */
 int orc_assemble(orc_ctx *o, double dt, int64_t n, double *ccmnm);
