@@ -113,7 +113,7 @@ const char *const efdc_timer_names[TK_COUNT] = {
     "puv_depths", "puv_hu_hv", "tran_upwind_vdiff", "tran_bc_con1",
     "closure_pre", "closure_post", "balance", "adac_gradients",
     "adac_fluxes", "fct_limiters", "tran_finish_vdiff", "cg_persistent64",
-    "mx_persistent", "adj_tran_col", "adj_tran_gather", "adj_hydro", "adj_cg", "settle"};
+    "mx_persistent", "adj_tran_col", "adj_tran_gather", "adj_hydro", "adj_cg", "settle", "cg_iterations"};
 
 hipEvent_t timer_event(efdc_ctx *c) {
   if (c->ev_pool.empty()) {
@@ -976,6 +976,8 @@ int efdc_timers_read(efdc_ctx_t *c, efdc_timers_t *out) {
     c->ev_pool.push_back(r.b);
   }
   c->trec.clear();
+  out->k[TK_CG_ITERS].count = c->cg_iter_count;  // iterations, no duration
+  c->cg_iter_count = 0;
   return EFDC_OK;
 }
 

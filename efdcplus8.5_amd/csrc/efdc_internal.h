@@ -189,6 +189,7 @@ struct efdc_ctx {
 
   // kernel timers (efdc_timers_*)
   bool timing = false;
+  int64_t cg_iter_count = 0;  // TK_CG_ITERS while timing
   std::vector<hipEvent_t> ev_pool;
   struct TimerRec { int id; hipEvent_t a, b; };
   std::vector<TimerRec> trec;
@@ -236,7 +237,9 @@ enum {
   TK_MX_APPLY, TK_PUV_POST1, TK_PUV_POST2, TK_PUV_POST3, TK_TRAN,
   TK_TRAN_BC, TK_CLOSURE_PRE, TK_CLOSURE_POST, TK_BALANCE, TK_ADAC_GRAD,
   TK_ADAC_FLUX, TK_FCT, TK_COL_FINISH, TK_PCG64, TK_PCGMX, TK_ADJ_TCOL, TK_ADJ_TGATHER,
-  TK_ADJ_HYDRO, TK_ADJ_CG, TK_SETTLE, TK_COUNT
+  TK_ADJ_HYDRO, TK_ADJ_CG, TK_SETTLE,
+  TK_CG_ITERS,  // not a kernel: FP64 multi-kernel CG iterations performed
+  TK_COUNT
 };
 extern const char *const efdc_timer_names[TK_COUNT];
 

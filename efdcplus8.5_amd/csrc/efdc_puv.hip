@@ -1234,6 +1234,7 @@ int cg_fp64(efdc_ctx *c) {
     chunk = 4;
   }
   c->last_iters = c->st_host->iter;
+  if (c->timing) c->cg_iter_count += c->st_host->iter;
   return EFDC_OK;
 }
 
