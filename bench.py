@@ -109,7 +109,14 @@ def roofline(ctx, case, n0, k, kernels, nwet):
             tot = sum(persistent_bytes(name, st["iters"], st["outer"], nwet)
                       for st in stats)
             return tot / max(len(stats), 1), tot
-        return BYTES_PER_CELL[name] * nwet, BYTES_PER_CELL[name] * nwet * per[name]["launches"]
+        # launches after a solve has converged return at once: count the
+        # iterations performed (the timers' cg_iterations) where known
+        n_it = per.get("cg_iterations", {}).get("launches", 0)
+        iter_kernels = ("cg_spmv_pupdate", "cg_axpy_dots", "mx_spmv_pupdate32",
+                        "mx_axpy_dots32")
+        n = n_it if (name in iter_kernels and n_it > 0) else per[name]["launches"]
+        return (BYTES_PER_CELL[name] * nwet * n / per[name]["launches"],
+                BYTES_PER_CELL[name] * nwet * n)
 
     bytes_launch = launch_bytes(dom)[0]
     ach = bytes_launch / avg_s / 1e9

@@ -238,7 +238,7 @@ enum {
   TK_TRAN_BC, TK_CLOSURE_PRE, TK_CLOSURE_POST, TK_BALANCE, TK_ADAC_GRAD,
   TK_ADAC_FLUX, TK_FCT, TK_COL_FINISH, TK_PCG64, TK_PCGMX, TK_ADJ_TCOL, TK_ADJ_TGATHER,
   TK_ADJ_HYDRO, TK_ADJ_CG, TK_SETTLE,
-  TK_CG_ITERS,  // not a kernel: FP64 multi-kernel CG iterations performed
+  TK_CG_ITERS,  // not a kernel: multi-kernel CG iterations performed
   TK_COUNT
 };
 extern const char *const efdc_timer_names[TK_COUNT];

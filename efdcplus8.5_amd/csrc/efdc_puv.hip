@@ -1280,6 +1280,7 @@ int cg_mixed(efdc_ctx *c) {
     ++outer;
   }
   c->st_host->iter = total;
+  if (c->timing) c->cg_iter_count += total;
   c->st_host->outer = outer;
   return EFDC_OK;
 }
