@@ -986,11 +986,25 @@ __global__ void k_dry_obc(HPtr h, PbcDev s, const int *le, const int *ln, int *l
   HA(FP1)[i] = DELTI * HA(DXYP)[i] * pv;
 }
 
+// st->drychg = 1 if any thread of the workgroup passes `mine`: one LDS
+// flag per workgroup, then one plain store by thread 0 unless the flag
+// already reads 1 (atomics or one store per wave on the one address
+// serialise a pass in which many cells change).  Every thread of the
+// workgroup must call it (two barriers).
+__device__ __forceinline__ void raise_flag_once(bool mine, CgState *st) {
+  __shared__ int any;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  if (mine) any = 1;
+  __syncthreads();
+  if (threadIdx.x == 0 && any &&
+      __hip_atomic_load(&st->drychg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    __hip_atomic_store(&st->drychg, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Cells that dry, :574-602: HP <= HDRY closes the four faces of the cell;
-// a cell that becomes dry raises st->drychg (ICORDRY = 1: solve again):
-// one lane per wave stores 1 unless the flag already reads 1 (a plain
-// store of the same value; atomics on the one address serialise the pass
-// when many cells dry at once).  Faces already closed are not written again.
+// a cell that becomes dry raises st->drychg (ICORDRY = 1: solve again)
+// through raise_flag_once.  Faces already closed are not written again.
 __global__ __launch_bounds__(EFDC_BLOCK) void k_dry_cells(HPtr h, const int *le, const int *ln,
                                                           int *iscdry, int nwet, double HDRY,
                                                           CgState *st) {
@@ -1009,10 +1023,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_dry_cells(HPtr h, const int *le,
     if (SUB[e] != 0.) SUB[e] = 0.;
     if (SVB[n] != 0.) SVB[n] = 0.;
   }
-  const unsigned long long b = __ballot(newly);
-  if (b != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)b) - 1 &&
-      __hip_atomic_load(&st->drychg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-    __hip_atomic_store(&st->drychg, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  raise_flag_once(newly, st);
 }
 
 __global__ void k_dry_flag_reset(CgState *st) { st->drychg = 0; }
@@ -1147,10 +1158,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_dry99(HPtr h, const int *lw, con
       chg = true;
     }
   }
-  const unsigned long long b = __ballot(chg);
-  if (b != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)b) - 1 &&
-      __hip_atomic_load(&st->drychg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-    __hip_atomic_store(&st->drychg, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  raise_flag_once(chg, st);
 }
 
 __global__ __launch_bounds__(EFDC_BLOCK) void k_dry99_commit(HPtr h, const double *sub_n,
