@@ -91,40 +91,92 @@ def timed(ctx, case, n0, k):
     return time.perf_counter() - t0, iters
 
 
-def roofline(ctx, case, n0, k, kernels, nwet):
+# Algorithmic bytes of the reference method per wet cell (the plain 3-pass
+# Jacobi-PCG on the 5-point operator with separate coefficients): FP64
+# 160 B per iteration; the FP32 inner solve 88 B per iteration, an FP64
+# residual with the FP64 operator 80 B per refinement step (one more than
+# the inner solves), the FP32 down-conversion of the coefficients 60 B per
+# solve.  The library's compact symmetric operator moves fewer bytes
+# (BYTES_PER_CELL); the roofline is quoted on the method's bytes and the
+# compact figure is reported beside it.
+ALG_FP64_ITER, ALG_FP32_ITER, ALG_RESID64, ALG_DOWNCONV = 160, 88, 80, 60
+
+
+def alg_solve_bytes(st, nwet, mixed):
+    if mixed:
+        return nwet * (ALG_FP32_ITER * st["iters"] + ALG_RESID64 * (st["outer"] + 1) +
+                       ALG_DOWNCONV)
+    return nwet * ALG_FP64_ITER * st["iters"]
+
+
+def solve_roofline(ctx, case, n0, k, mixed, nwet, workload):
+    """Roofline of the CG solve over k further steps: the method's bytes
+    per solve (alg_solve_bytes) over the time of the solve's kernels per
+    solve (HIP events on the library's stream).  One launch per solve for
+    the persistent solve; the sum of the multi-kernel schedule's launches
+    otherwise.  ``traffic`` = the HBM bytes per solve from the committed
+    rocprofv3 counter summaries of those kernels on this workload (None
+    where a summary is missing); ``hbm_gbs`` = traffic / time."""
+    kernels = CG_KERNELS_MIXED if mixed else CG_KERNELS_FP64
     ctx.timers(True)
     stats = []
     run_steps(ctx, case, n0, k, stats)
     t = ctx.read_timers()
     ctx.timers(False)
-    per = {}
-    for name, (cnt, ms) in t.items():
-        per[name] = dict(launches=cnt, avg_us=1e3 * ms / cnt, total_ms=ms)
+    per = {name: dict(launches=cnt, avg_us=1e3 * ms / cnt, total_ms=ms)
+           for name, (cnt, ms) in t.items()}
     cg = {k_: per[k_] for k_ in kernels if k_ in per}
-    dom = max(cg, key=lambda k_: cg[k_]["total_ms"])
-    avg_s = cg[dom]["avg_us"] * 1e-6
-
-    def launch_bytes(name):
+    nsolve = len(stats)
+    cg_s = sum(v["total_ms"] for v in cg.values()) * 1e-3
+    alg = sum(alg_solve_bytes(st, nwet, mixed) for st in stats)
+    # the compact operator's bytes, iterations performed only
+    n_it = per.get("cg_iterations", {}).get("launches", 0)
+    compact = 0.0
+    for name, v in cg.items():
         if name in ("cg_persistent64", "mx_persistent"):
-            tot = sum(persistent_bytes(name, st["iters"], st["outer"], nwet)
-                      for st in stats)
-            return tot / max(len(stats), 1), tot
-        # launches after a solve has converged return at once: count the
-        # iterations performed (the timers' cg_iterations) where known
-        n_it = per.get("cg_iterations", {}).get("launches", 0)
-        iter_kernels = ("cg_spmv_pupdate", "cg_axpy_dots", "mx_spmv_pupdate32",
-                        "mx_axpy_dots32")
-        n = n_it if (name in iter_kernels and n_it > 0) else per[name]["launches"]
-        return (BYTES_PER_CELL[name] * nwet * n / per[name]["launches"],
-                BYTES_PER_CELL[name] * nwet * n)
+            compact += sum(persistent_bytes(name, st["iters"], st["outer"], nwet) for st in stats)
+        else:
+            it_k = name in ("cg_spmv_pupdate", "cg_axpy_dots", "mx_spmv_pupdate32",
+                            "mx_axpy_dots32")
+            compact += BYTES_PER_CELL[name] * nwet * (n_it if (it_k and n_it) else v["launches"])
+    traffic = 0.0
+    for name, v in cg.items():
+        tr = measured_traffic(name, workload)
+        if tr is None:
+            traffic = None
+            break
+        traffic += tr * v["launches"] / nsolve
+    solve_us = cg_s * 1e6 / nsolve
+    names = list(cg)
+    return dict(kernel=names[0] if len(names) == 1 else "CG solve: " + "+".join(names),
+                achieved=alg / cg_s / 1e9, avg_us=solve_us, bytes_per_launch=alg / nsolve,
+                compact_gbs=compact / cg_s / 1e9, compact_bytes_per_solve=compact / nsolve,
+                traffic=traffic,
+                hbm_gbs=None if traffic is None else traffic / (solve_us * 1e-6) / 1e9,
+                iters_per_solve=sum(st["iters"] for st in stats) / nsolve,
+                outer_per_solve=sum(st["outer"] for st in stats) / nsolve,
+                kernels=per)
 
-    bytes_launch = launch_bytes(dom)[0]
-    ach = bytes_launch / avg_s / 1e9
-    cg_bytes = sum(launch_bytes(k_)[1] for k_ in cg)
-    cg_ms = sum(v["total_ms"] for v in cg.values())
-    return dict(dominant=dom, achieved=ach, bytes_per_launch=bytes_launch,
-                avg_us=cg[dom]["avg_us"], cg_gbs=cg_bytes / (cg_ms * 1e-3) / 1e9,
-                cg_ms=cg_ms, kernels=per)
+
+def host_cpus():
+    """CPUs of this host as lscpu counts them (logical CPUs, cores)."""
+    import subprocess
+    out = {}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        txt = ""
+    for line in txt.splitlines():
+        if ":" in line:
+            k_, v = line.split(":", 1)
+            out[k_.strip()] = v.strip()
+    try:
+        cores = int(out["Core(s) per socket"]) * int(out["Socket(s)"])
+    except (KeyError, ValueError):
+        cores = None
+    return dict(logical=int(out.get("CPU(s)", os.cpu_count() or 0)), cores=cores,
+                model=out.get("Model name"),
+                usable=len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None)
 
 
 def transport_bench(args):
@@ -158,7 +210,7 @@ def transport_bench(args):
             done += 1
         ncmp = done
         cpu = dict(value=g.nwet * done / t_cpu, unit="column-steps/s",
-                   cores=min(args.cpu_threads, 8), kind="port",
+                   cores=min(args.cpu_threads, 8), host_cpus=host_cpus(), kind="port",
                    sample=f"{done} CALCONC steps of the same {nx}x{ny}x{kc} grid, "
                           f"{nc} constituents, by the FP64 C restatement (one "
                           f"thread per constituent for advection, column-parallel "
@@ -218,6 +270,7 @@ def transport_bench(args):
                                for k, (c_, m_) in tm.items()}},
     }
     print(json.dumps(out))
+    ctx.close()
 
 
 def drying_bench(args):
@@ -266,7 +319,7 @@ def drying_bench(args):
                                np.array_equal(d[L], o.iscdry[L]) and k == o.o.ncordry)
             done += 1
         cpu = dict(value=g.nwet * done / t_cpu, unit="cell-steps/s",
-                   cores=min(args.cpu_threads, 8), kind="port",
+                   cores=min(args.cpu_threads, 8), host_cpus=host_cpus(), kind="port",
                    sample=f"{done} full steps of the same {nx}x{ny}x{kc} drying basin "
                           f"by the FP64 C restatement, {t_cpu:.1f} s")
         check = dict(after_steps=done, same_dry_flags_every_step=same_flags,
@@ -326,6 +379,7 @@ def drying_bench(args):
                                for k, (c_, m_) in tm.items()}},
     }
     print(json.dumps(out))
+    ctx.close()
 
 
 def ensemble_bench(args):
@@ -391,7 +445,7 @@ def ensemble_bench(args):
             nb_done += 1
             if t_cpu >= args.cpu_seconds:
                 break
-        cpu = dict(value=cols / t_cpu, unit="column-steps/s", cores=min(args.cpu_threads, 8),
+        cpu = dict(value=cols / t_cpu, unit="column-steps/s", cores=min(args.cpu_threads, 8), host_cpus=host_cpus(),
                    kind="port",
                    sample=f"3 full steps of each of the first {nb_done} basins by the FP64 C "
                           f"restatement (OpenMP), {t_cpu:.1f} s")
@@ -446,7 +500,7 @@ def ensemble_bench(args):
                    "basins": args.nbasins, "columns": nwet},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(dom, f"drying{nx}x{ny}"), "kernel": dom,
+                     "traffic": measured_traffic(dom, f"ensemble{args.nbasins}"), "kernel": dom,
                      "avg_us": ms / cnt * 1e3, "bytes_per_launch": total_bytes[dom] / cnt},
         "cpu_baseline": cpu,
         "detail": {
@@ -466,6 +520,7 @@ def ensemble_bench(args):
         },
     }
     print(json.dumps(out))
+    ctx.close()
 
 
 def adjoint_bench(args):
@@ -556,7 +611,7 @@ def adjoint_bench(args):
             o.step(done + 1)
             t_cpu += time.perf_counter() - t1
             done += 1
-        cpu = dict(value=g.nwet * done / t_cpu, unit="column-steps/s", cores=min(args.cpu_threads, 8),
+        cpu = dict(value=g.nwet * done / t_cpu, unit="column-steps/s", cores=min(args.cpu_threads, 8), host_cpus=host_cpus(),
                    kind="port",
                    sample=f"{done} forward steps of the same {n}x{n}x{kc} basin by the FP64 C "
                           f"restatement, {t_cpu:.1f} s (the CPU path has no adjoint: forward only)")
@@ -573,7 +628,7 @@ def adjoint_bench(args):
                                "checkpoint every 20 steps", "columns": g.nwet},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(dom, f"drying{nx}x{ny}"), "kernel": dom,
+                     "traffic": measured_traffic(dom, f"adjoint{n}x{n}x{kc}"), "kernel": dom,
                      "avg_us": ms / cnt * 1e3, "bytes_per_launch": per_col[dom] * g.nwet},
         "cpu_baseline": cpu,
         "detail": {
@@ -644,12 +699,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("lake1024", "transport", "ensemble", "adjoint",
-                                           "drying"),
+    ap.add_argument("--workload", choices=("lake1024", "lake4096", "transport", "ensemble",
+                                           "adjoint", "drying"),
                     default="lake1024",
-                    help="lake1024: the external solve at scale (the headline); "
-                         "transport: CALCONC at scale; ensemble: 256 basins in one "
-                         "context")
+                    help="lake1024: the external solve at scale (the headline; --n sets "
+                         "the lattice); lake4096: the same on a 4096x4096 lattice (~11 M "
+                         "cells, past the MALL); transport: CALCONC at scale; ensemble: "
+                         "256 basins in one context")
     ap.add_argument("--nbasins", type=int, default=256)
     ap.add_argument("--an", type=int, default=512, help="adjoint: basin size")
     ap.add_argument("--asteps", type=int, default=200, help="adjoint: steps")
@@ -692,10 +748,18 @@ def main():
         drying_bench(args)
         return
 
+    lake_bench(args)
+
+
+def lake_bench(args):
+    """The headline: CALPUV2C steps on the irregular lake at scale,
+    FP32-refined and FP64 solves, inputs resident in HBM."""
     import numpy as np
     from efdc_amd import grids
 
-    case = grids.config1(n=args.n)
+    n = 4096 if args.workload == "lake4096" else args.n
+    workload = f"lake{n}"
+    case = grids.config1(n=n)
     nwet = case.grid.nwet
     ref_case = case.copy() if not args.no_cpu else None
     ck = {"auto": 0, "multi": 1, "persistent": 2}[args.cg_kernel]
@@ -708,7 +772,7 @@ def main():
     ncmp = 2
     if not args.no_cpu:
         from oracle import oracle as orc
-        o = orc.Oracle(ref_case, rsqm=0.0, iterm=1000, nthreads=args.cpu_threads,
+        o = orc.Oracle(ref_case, rsqm=0.0, iterm=4000, nthreads=args.cpu_threads,
                        rel_tol=1e-10)
         t_cpu = 0.0
         done = 0
@@ -721,10 +785,11 @@ def main():
                 raise RuntimeError(f"oracle status {st}")
         ncmp = done
         cpu = dict(value=nwet * done / t_cpu, unit="cell-steps/s",
-                   cores=min(args.cpu_threads, 8), kind="port",
-                   sample=f"{done} CALPUV2C steps of the same {args.n}x{args.n} "
+                   cores=min(args.cpu_threads, 8), host_cpus=host_cpus(), kind="port",
+                   sample=f"{done} CALPUV2C steps of the same {n}x{n} "
                           f"lake by the FP64 C restatement (OpenMP, <=8-way "
                           f"L-domains, serial dot products), {t_cpu:.1f} s")
+        del o
     run_steps(ctx_m, case, 1, ncmp)
     run_steps(ctx_d, case, 1, ncmp)
     err_m = err_d = None
@@ -732,6 +797,7 @@ def main():
         pr = ref_case.h["P"][i]
         err_m = float(np.max(np.abs(ctx_m.get("P")[i] - pr))) / 9.81
         err_d = float(np.max(np.abs(ctx_d.get("P")[i] - pr))) / 9.81
+        del ref_case, pr
 
     # ---- timed region: FP32-refined, then FP64 (the first warm-up step
     # of the FP32-refined solve runs the autotuner)
@@ -743,11 +809,14 @@ def main():
     run_steps(ctx_d, case, n0, wu)
     t_d, it_d = timed(ctx_d, case, n0 + wu, args.steps)
     n1 = n0 + wu + args.steps
-    rf_m = roofline(ctx_m, case, n1, args.steps, CG_KERNELS_MIXED, nwet)
-    rf_d = roofline(ctx_d, case, n1, args.steps, CG_KERNELS_FP64, nwet)
+    rf_m = solve_roofline(ctx_m, case, n1, args.steps, True, nwet, workload)
+    rf_d = solve_roofline(ctx_d, case, n1, args.steps, False, nwet, workload)
 
     value = nwet * args.steps / t_m
     copy_gbs = stream_copy_gbs(ctx_m)
+    tune = ctx_m.tune_report()
+    ctx_m.close()
+    ctx_d.close()
     out = {
         "metric": "cell-steps/s + CG HBM GB/s (% peak), FP32-refined vs FP64 at <=1e-6 m error",
         "value": value,
@@ -762,7 +831,7 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"external-mode solve at scale: {args.n}x{args.n} irregular "
+            "workload": f"external-mode solve at scale: {n}x{n} irregular "
                         f"lake, {nwet} wet cells, dt=10 s, CALPUV2C per step, "
                         "CONGRAD to relative residual 1e-10",
             "wet_cells": nwet,
@@ -775,22 +844,29 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": rf_m["achieved"] / HBM_PEAK_GBS,
-            "traffic": measured_traffic(rf_m["dominant"], "lake1024"),
-            "kernel": rf_m["dominant"],
+            "traffic": rf_m["traffic"],
+            "kernel": rf_m["kernel"],
             "avg_us": rf_m["avg_us"],
             "bytes_per_launch": rf_m["bytes_per_launch"],
+            "basis": f"method bytes per solve: {ALG_FP32_ITER} B x inner iterations + "
+                     f"{ALG_RESID64} B x (refinement steps + 1) + {ALG_DOWNCONV} B, per wet cell",
+            "hbm_gbs": rf_m["hbm_gbs"],
+            "compact_operator_gbs": rf_m["compact_gbs"],
+            "compact_bytes_per_launch": rf_m["compact_bytes_per_solve"],
         },
         "cpu_baseline": cpu,
         "detail": {
-            "autotune": ctx_m.tune_report(),
+            "autotune": tune,
             "stream_copy_gbs": copy_gbs,
-            "dominant_frac_of_copy": rf_m["achieved"] / copy_gbs,
-            "cg_gbs_mixed": rf_m["cg_gbs"],
-            "cg_gbs_fp64": rf_d["cg_gbs"],
+            "solve_frac_of_copy": rf_m["achieved"] / copy_gbs,
+            "inner_iters_per_solve_mixed": rf_m["iters_per_solve"],
+            "refinements_per_solve_mixed": rf_m["outer_per_solve"],
             "fp64_value": nwet * args.steps / t_d,
             "fp64_ms_per_step": 1e3 * t_d / args.steps,
-            "fp64_dominant": rf_d["dominant"],
-            "fp64_dominant_gbs": rf_d["achieved"],
+            "fp64_roofline": {k_: rf_d[k_] for k_ in ("kernel", "achieved", "avg_us",
+                                                     "bytes_per_launch", "traffic", "hbm_gbs",
+                                                     "compact_gbs", "iters_per_solve")},
+            "fp64_frac": rf_d["achieved"] / HBM_PEAK_GBS,
             "cg_iters_per_step_mixed": float(np.mean(it_m)),
             "cg_iters_per_step_fp64": float(np.mean(it_d)),
             "eta_err_vs_oracle_m": {"mixed": err_m, "fp64": err_d,

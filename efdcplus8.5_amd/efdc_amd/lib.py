@@ -8,8 +8,11 @@ calls raise.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -123,6 +126,18 @@ ADJ_MANNING, ADJ_HP, ADJ_UHDYE, ADJ_VHDXE, ADJ_P, ADJ_HU, ADJ_HV = range(7)
 ADJ_CON = 16
 
 _lib = None
+# contexts not closed yet: closed by close_all() at interpreter exit, while
+# the HIP runtime is still whole (a context destroyed from __del__ during
+# interpreter teardown would free device memory and destroy its stream
+# after the runtime's, or a profiler's, own exit handlers may have run)
+_live = weakref.WeakSet()
+
+
+def close_all():
+    """Destroy every context that is still open (registered with atexit
+    when the library is loaded)."""
+    for ctx in list(_live):
+        ctx.close()
 
 
 def load(path: Optional[str] = None):
@@ -196,6 +211,7 @@ def load(path: Optional[str] = None):
         f.argtypes = args
         f.restype = res
     _lib = lib
+    atexit.register(close_all)
     return lib
 
 
@@ -237,6 +253,7 @@ class Context:
         self.h = h
         self.nconst = 0
         self.has_manning = False
+        _live.add(self)
 
     # ----------------------------------------------------------- errors
     def _check(self, st: int, ctx="self"):
@@ -250,8 +267,13 @@ class Context:
         if getattr(self, "h", None):
             self.lib.efdc_ctx_destroy(self.h)
             self.h = None
+        _live.discard(self)
 
     def __del__(self):
+        # during interpreter teardown the atexit hook has already closed
+        # every context; no HIP call is made from a finaliser then
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -88,7 +88,13 @@ typedef struct {
   double G;        /* gravity (EFDC GLOBAL G)                               */
   double RITB1;    /* explicit bottom-stress weight (EFDC/caltbxy.f90:94-106)*/
   double SNLT;     /* nonlinear-term switch multiplying FXE/FYE              */
-  double HDRY;     /* drying depth (only the open-BC inactivation test)     */
+  double HDRY;     /* drying depth: the open-boundary inactivation test
+                      (EFDC/setopenbc.f90:62-74) and, with ISDRY > 0, the
+                      drying of open-boundary cells (EFDC/calpuv2c.f90:
+                      497-530), cell drying (ISDRY 1..97, :574-602),
+                      drying and rewetting (99, :604-716), the draining of
+                      isolated cells (NDRYSTP < 0, :838-893) and the
+                      transport mask LMASKDRY (:916-939)                   */
   double TCON_TBEGIN; /* TCON*TBEGIN, offset of the tidal clock             */
   int32_t ITERM;   /* CG iteration cap                                      */
   double RSQM;     /* CG stop: sum r^2 <= RSQM on the scaled system          */

@@ -33,6 +33,25 @@
 
 static const double PI_EFDC = 3.1415926535898; /* EFDC/aaefdc.f90:650-651 */
 
+/* Planted mistakes.  Built with -DORC_PLANT=n (tests/test_oracle_plants.py
+   compiles such variants into a temporary directory; the checker that the
+   tests use is always built with ORC_PLANT = 0), the oracle makes mistake n,
+   so that a test can show that one of its checks fails on it:
+     1  coefficient constant -G*DELT/2 instead of -G*DELT/4
+        (EFDC/calpuv2c.f90:329)
+     2  FP1 with G instead of G/2 (:289)
+     3  FP with G instead of G/2 (:320)
+     4  face transports with DELT instead of DELT/2 (:426)
+     5  HU averaged with the east neighbour instead of the west (:905)
+     6  HV without the factor 1/2 (:906)
+     7  bottom stress added instead of subtracted (:217)
+     8  QSUME counted once in FP instead of twice (:320)
+     9  continuity with DELT instead of DELT/2 (:475)                     */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+#define PLANT(n, mistake, right) (ORC_PLANT == (n) ? (mistake) : (right))
+
 static inline double dmax(double a, double b) { return a > b ? a : b; }
 static inline double dmin(double a, double b) { return a < b ? a : b; }
 
@@ -313,12 +332,12 @@ static int explicit_stage(orc_ctx *o, double dt, int64_t n) {
     const size_t i = HX(L), s = HX(o->LSC[i]), w = HX(o->LWC[i]);
     FUHDYE[i] = UHDYE[i] - DELTD2 * SUB[i] * HRUO[i] * HU[i] * (P[i] - P[w]) +
                 SUB[i] * DELT * DXIU[i] *
-                    (DXYU[i] * (TSX[i] - RITB1 * TBX[i]) + FCAXE[i] + FPGXE[i] -
-                     SNLT * FXE[i]);
+                    (DXYU[i] * PLANT(7, TSX[i] + RITB1 * TBX[i], TSX[i] - RITB1 * TBX[i]) +
+                     FCAXE[i] + FPGXE[i] - SNLT * FXE[i]);
     FVHDXE[i] = VHDXE[i] - DELTD2 * SVB[i] * HRVO[i] * HV[i] * (P[i] - P[s]) +
                 SVB[i] * DELT * DYIV[i] *
-                    (DXYV[i] * (TSY[i] - RITB1 * TBY[i]) - FCAYE[i] + FPGYE[i] -
-                     SNLT * FYE[i]);
+                    (DXYV[i] * PLANT(7, TSY[i] + RITB1 * TBY[i], TSY[i] - RITB1 * TBY[i]) -
+                     FCAYE[i] + FPGYE[i] - SNLT * FYE[i]);
   }
   /* faces reopened with drying on, :246-253 */
   if (o->ISDRY > 0) {
@@ -352,7 +371,7 @@ static int explicit_stage(orc_ctx *o, double dt, int64_t n) {
   }
   /* FP1, :285-290 */
   {
-    const double C1 = 0.5 * G;
+    const double C1 = PLANT(2, G, 0.5 * G);
 #pragma omp parallel for num_threads(nopt) schedule(static)
     for (int L = 2; L <= LA; ++L) {
       const size_t i = HX(L), e = HX(o->LEC[i]), nn = HX(o->LNC[i]);
@@ -382,15 +401,15 @@ static double system_stage(orc_ctx *o, double dt, int64_t n) {
 #pragma omp parallel for num_threads(nopt) schedule(static)
   for (int L = 2; L <= LA; ++L) {
     const size_t i = HX(L), e = HX(o->LEC[i]), nn = HX(o->LNC[i]);
-    const double C1 = 0.5 * G;
+    const double C1 = PLANT(3, G, 0.5 * G);
     FP[i] = FP1[i] - C1 * (SUB[e] * FUHDYE[e] - SUB[i] * FUHDYE[i] +
                            SVB[nn] * FVHDXE[nn] - SVB[i] * FVHDXE[i] -
-                           2.0 * QSUME[i]);
+                           PLANT(8, 1.0, 2.0) * QSUME[i]);
   }
 #pragma omp parallel for num_threads(nopt) schedule(static)
   for (int L = 2; L <= LA; ++L) {
     const size_t i = HX(L), e = HX(o->LEC[i]), nn = HX(o->LNC[i]);
-    const double C1 = -0.5 * DELTD2 * G;
+    const double C1 = PLANT(1, -DELTD2 * G, -0.5 * DELTD2 * G);
     CS[i] = C1 * SVB[i] * HRVO[i] * RCY[i] * HV[i];
     CW[i] = C1 * SUB[i] * HRUO[i] * RCX[i] * HU[i];
     CE[i] = C1 * SUB[e] * HRUO[e] * RCX[e] * HU[e];
@@ -592,8 +611,9 @@ int orc_calpuv2c(orc_ctx *o, double dt, int64_t n, orc_cg_out *out) {
 #pragma omp parallel for num_threads(nopt) schedule(static)
     for (int L = 2; L <= LA; ++L) {
       const size_t i = HX(L), s = HX(o->LSC[i]), w = HX(o->LWC[i]);
-      UHDYE[i] = SUB[i] * (FUHDYE[i] - DELTD2 * HRUO[i] * RCX[i] * HU[i] * (P[i] - P[w]));
-      VHDXE[i] = SVB[i] * (FVHDXE[i] - DELTD2 * HRVO[i] * RCY[i] * HV[i] * (P[i] - P[s]));
+      const double DTU = PLANT(4, 2.0 * DELTD2, DELTD2);
+      UHDYE[i] = SUB[i] * (FUHDYE[i] - DTU * HRUO[i] * RCX[i] * HU[i] * (P[i] - P[w]));
+      VHDXE[i] = SVB[i] * (FVHDXE[i] - DTU * HRVO[i] * RCY[i] * HV[i] * (P[i] - P[s]));
       UHE[i] = UHDYE[i] * DYIU[i];
       VHE[i] = VHDXE[i] * DXIV[i];
     }
@@ -601,7 +621,7 @@ int orc_calpuv2c(orc_ctx *o, double dt, int64_t n, orc_cg_out *out) {
 #pragma omp parallel for num_threads(nopt) schedule(static)
     for (int L = 2; L <= LA; ++L) {
       const size_t i = HX(L), e = HX(o->LEC[i]), nn = HX(o->LNC[i]);
-      HP[i] = H1P[i] + DELTD2 * DXYIP[i] *
+      HP[i] = H1P[i] + PLANT(9, 2.0 * DELTD2, DELTD2) * DXYIP[i] *
                            (2. * QSUME[i] -
                             (UHDYE[e] + UHDY1E[e] - UHDYE[i] - UHDY1E[i] +
                              VHDXE[nn] + VHDX1E[nn] - VHDXE[i] - VHDX1E[i]));
@@ -687,9 +707,9 @@ int orc_calpuv2c(orc_ctx *o, double dt, int64_t n, orc_cg_out *out) {
   }
 #pragma omp parallel for num_threads(nopt) schedule(static)
   for (int L = 2; L <= LA; ++L) {
-    const size_t i = HX(L), s = HX(o->LSC[i]), w = HX(o->LWC[i]);
+    const size_t i = HX(L), s = HX(o->LSC[i]), w = HX(PLANT(5, o->LEC[i], o->LWC[i]));
     HU[i] = (DXYP[i] * HP[i] + DXYP[w] * HP[w]) * (0.5 / DXYU[i]);
-    HV[i] = (DXYP[i] * HP[i] + DXYP[s] * HP[s]) * (0.5 / DXYV[i]);
+    HV[i] = (DXYP[i] * HP[i] + DXYP[s] * HP[s]) * (PLANT(6, 1.0, 0.5) / DXYV[i]);
   }
 #pragma omp parallel for num_threads(nopt) schedule(static)
   for (int L = 2; L <= LA; ++L) {

@@ -71,25 +71,35 @@ class CgOut(C.Structure):
                 ("ccmnm", C.c_double), ("status", C.c_int32)]
 
 
-def build(force: bool = False) -> str:
+def build(force: bool = False, plant: int = 0, out: str = None) -> str:
+    """Compile the oracle into ``out`` (default oracle/liboracle.so, the
+    checker).  ``plant`` > 0 builds a variant with that planted mistake
+    (the ORC_PLANT list in efdc_oracle.c), for the tests that show a check
+    fails on it; such a variant never replaces the checker."""
     src = os.path.join(HERE, "efdc_oracle.c")
-    if force or not os.path.exists(LIB) or \
-            os.path.getmtime(LIB) < os.path.getmtime(src):
+    dst = out or LIB
+    if plant and dst == LIB:
+        raise ValueError("a planted oracle needs its own output path")
+    if force or plant or not os.path.exists(dst) or \
+            os.path.getmtime(dst) < os.path.getmtime(src):
         subprocess.check_call(
             ["gcc", "-O2", "-fopenmp", "-fPIC", "-shared",
-             "-ffp-contract=off", "-Wall", src, "-o", LIB, "-lm"])
-    return LIB
+             "-ffp-contract=off", "-Wall", f"-DORC_PLANT={int(plant)}", src,
+             "-o", dst, "-lm"])
+    return dst
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(path: str = None):
+    """The oracle library at ``path`` (default: the checker, built on
+    first use)."""
+    path = path or LIB
+    if path not in _libs:
+        if path == LIB and not os.path.exists(LIB):
             build()
-        _lib = C.CDLL(LIB)
+        _lib = C.CDLL(path)
         for fn, args in dict(
                 orc_congrad=[C.POINTER(OrcCtx), C.POINTER(CgOut)],
                 orc_calpuv2c=[C.POINTER(OrcCtx), C.c_double, C.c_int64,
@@ -107,7 +117,8 @@ def lib():
             f = getattr(_lib, fn)
             f.argtypes = args
             f.restype = None if fn == "orc_settle" else C.c_int
-    return _lib
+        _libs[path] = _lib
+    return _libs[path]
 
 
 def _pd(a):
@@ -123,8 +134,9 @@ class Oracle:
 
     def __init__(self, case, *, rsqm=1e-9, iterm=200, nthreads=1, g=9.81,
                  ritb1=1.0, snlt=0.0, hdry=0.2, tcon_tbegin=0.0, rel_tol=0.0, isdry=0,
-                 ndrystp=0):
+                 ndrystp=0, lib_path=None):
         self.case = case
+        self.L = lib(lib_path)
         g_ = case.grid
         o = OrcCtx()
         self._keep = []
@@ -225,48 +237,48 @@ class Oracle:
     # ------------------------------------------------------------ calls
     def congrad(self):
         out = CgOut()
-        lib().orc_congrad(C.byref(self.o), C.byref(out))
+        self.L.orc_congrad(C.byref(self.o), C.byref(out))
         return dict(iters=out.iters, rsq=out.rsq, status=out.status)
 
     def assemble(self, n=1, dt=None):
         cm = C.c_double(0.0)
-        lib().orc_assemble(C.byref(self.o), dt or self.case.dt, n, C.byref(cm))
+        self.L.orc_assemble(C.byref(self.o), dt or self.case.dt, n, C.byref(cm))
         return cm.value
 
     def calpuv2c(self, n=1, dt=None):
         out = CgOut()
-        lib().orc_calpuv2c(C.byref(self.o), dt or self.case.dt, n, C.byref(out))
+        self.L.orc_calpuv2c(C.byref(self.o), dt or self.case.dt, n, C.byref(out))
         return dict(iters=out.iters, rsq=out.rsq, ccmnm=out.ccmnm,
                     status=out.status)
 
     def calconc(self, niter=1, dt=None):
-        return lib().orc_calconc(C.byref(self.o), dt or self.case.dt, niter)
+        return self.L.orc_calconc(C.byref(self.o), dt or self.case.dt, niter)
 
     def vertical_diffusion(self, dt=None):
-        return lib().orc_vertical_diffusion(C.byref(self.o), dt or self.case.dt)
+        return self.L.orc_vertical_diffusion(C.byref(self.o), dt or self.case.dt)
 
     def settle(self, dt=None):
         """Only the settling sweep of CALCONC (EFDC/calsed.f90:232-256,308-312)."""
-        return lib().orc_settle(C.byref(self.o), dt or self.case.dt)
+        return self.L.orc_settle(C.byref(self.o), dt or self.case.dt)
 
     def balance(self):
         v = C.c_double(0.0)
         m = np.zeros(MAXC)
-        lib().orc_balance(C.byref(self.o), C.byref(v), _pd(m))
+        self.L.orc_balance(C.byref(self.o), C.byref(v), _pd(m))
         return v.value, m[:len(self.case.consts)].copy()
 
     def closure_pre(self):
         nman = np.ascontiguousarray(self.case.manning_n, np.float64)
-        lib().orc_closure_pre(C.byref(self.o), _pd(nman))
+        self.L.orc_closure_pre(C.byref(self.o), _pd(nman))
 
     def setopenbc_apply(self, n=1, dt=None):
-        lib().orc_setopenbc_apply(C.byref(self.o), dt or self.case.dt, n)
+        self.L.orc_setopenbc_apply(C.byref(self.o), dt or self.case.dt, n)
 
     def caltran_only(self, c, niter=1, dt=None):
-        lib().orc_caltran_only(C.byref(self.o), c, dt or self.case.dt, niter)
+        self.L.orc_caltran_only(C.byref(self.o), c, dt or self.case.dt, niter)
 
     def closure_post(self):
-        lib().orc_closure_post(C.byref(self.o))
+        self.L.orc_closure_post(C.byref(self.o))
 
     def step(self, n):
         """One 2TL hot-path step of the harness sequence (HDMT2T order,

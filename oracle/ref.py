@@ -35,6 +35,7 @@ def lib():
             getattr(_lib, f).argtypes = [C.c_char_p, p, i64]
         _lib.ref_caltran.argtypes = [i, i, p, p, d, i64]
         _lib.ref_setopenbc.argtypes = [d, i64]
+        _lib.ref_wetlist.argtypes = [p]
     return _lib
 
 

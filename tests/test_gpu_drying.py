@@ -83,6 +83,65 @@ def test_drying_matches_oracle(oracle_mod, deep, cg_kernel, isdry):
     ctx.close()
 
 
+def _sill_basin(kc=3, sill=0.6, deep=3.0):
+    """A 3 m deep basin behind a 0.6 m sill at its open (west) boundary:
+    with the 0.8 m tide of drying_basin the boundary cells dry at low
+    water and come back, the interior never gets shallower than ~0.85 m."""
+    from efdc_amd import grids
+    case = grids.drying_basin(kc=kc, deep=deep, high=-deep, shoal=0.0)
+    g, h, k = case.grid, case.h, case.k
+    L = wet_index(case)
+    belv = np.where(g.IL[L] == 2, -sill, -deep)
+    for key in ("HP", "H1P", "H2P"):
+        h[key][L] = -belv
+    h["BELV"][L] = belv
+    h["P"][L] = 0.0
+    h["P1"][L] = 0.0
+    grids.update_depth_arrays(g, h)
+    grids.layer_depths(g, h, k)
+    k["H1PK"][:] = k["HPK"]
+    return case
+
+
+@pytest.mark.parametrize("cg_kernel", [0, 1])
+def test_open_boundary_drying_98_matches_oracle(oracle_mod, cg_kernel):
+    """ISDRY = 98: only open-boundary cells dry (EFDC/calpuv2c.f90:497-530),
+    the cell pass is skipped, so no step solves twice."""
+    from efdc_amd.lib import Context
+    case = _sill_basin()
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref, rsqm=1e-24, iterm=2000, isdry=98, hdry=HDRY)
+    ctx = Context(case.grid, RSQM=1e-24, ITERM=2000, ISDRY=98, HDRY=HDRY,
+                  cg_kernel=cg_kernel).upload_case(case)
+    L = wet_index(case)
+    lb = case.open_bc["side"][0]["L"] - 1
+    kc = case.grid.KC
+    dried = returned = 0
+    prev = False
+    for n in range(1, 601):
+        so = o.step(n)
+        sg = ctx.step(case.dt, n)
+        assert so["status"] == 0 and sg["status"] == 0, n
+        m, d, k = ctx.get_dry_state()
+        assert k == 0 and o.o.ncordry == 0, n
+        assert np.array_equal(m[L], o.lmaskdry[L]), n
+        assert np.array_equal(d[L], o.iscdry[L]), n
+        now = bool(np.any(d[lb] == 1))
+        dried += now
+        returned += prev and not now
+        prev = now
+        if n % 50 == 0 or n < 4:
+            assert eta_err(ctx.get("P")[L], ref.h["P"][L]) <= 1e-9, n
+            for key in ("SUB", "SVB"):
+                assert np.array_equal(ctx.get(key)[L], ref.h[key][L]), (n, key)
+            for j, c in enumerate(ref.consts):
+                con, _ = ctx.get_constituent(j)
+                scale = max(1e-30, np.max(np.abs(c["con"][:kc])))
+                assert np.max(np.abs(con[:kc] - c["con"][:kc])) / scale <= 1e-9, (n, j)
+    assert dried > 100 and returned >= 1
+    ctx.close()
+
+
 def test_drying_transport_shapes_bitwise():
     from efdc_amd import grids
     from efdc_amd.lib import Context

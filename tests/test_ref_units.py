@@ -7,81 +7,10 @@ build is absent (it is built only where /root/reference exists)."""
 import numpy as np
 import pytest
 
+from edge_cases import random_case
 from oracle import ref as R
 
 pytestmark = pytest.mark.skipif(not R.available(), reason="oracle/_ref not built")
-
-
-def boundary_cells(g, mapname, rng, nmax):
-    L = np.arange(2, g.LA + 1)
-    cand = L[g.maps[mapname][L - 1] == g.LC]
-    rng.shuffle(cand)
-    return np.sort(cand[:nmax]).astype(np.int32)
-
-
-def random_case(seed, kc=4, nx=16, ny=12, isadac=1, isfct=1, nconst=4):
-    from efdc_amd import grids
-    rng = np.random.default_rng(seed)
-    wet = grids.lake_mask(nx, ny, seed, wet_fraction=0.75, sigma=2.0)
-    g = grids.build_grid(wet, kc)
-    h, k, w = grids._empty_arrays(g)
-    grids.fill_geometry(g, h, k, w, 100.0, rng.uniform(0.5, 1.5, kc))
-    L = np.arange(2, g.LA + 1) - 1
-    n = len(L)
-    for K in range(kc):
-        k["UHDY2"][K, L] = rng.normal(0, 40, n) * h["SUB"][L]
-        k["VHDX2"][K, L] = rng.normal(0, 40, n) * h["SVB"][L]
-        k["H1PK"][K, L] = rng.uniform(1, 3, n)
-        k["HPK"][K, L] = rng.uniform(1, 3, n)
-        k["HPKI"][K, L] = 1.0 / k["HPK"][K, L]
-        k["U2"][K, L] = rng.normal(0, 0.3, n)
-        k["V2"][K, L] = rng.normal(0, 0.3, n)
-        k["QSUM"][K, L] = np.where(rng.uniform(size=n) < 0.1, rng.normal(0, 1e-3, n), 0.0)
-    for K in range(1, kc):
-        w["W2"][K, L] = rng.normal(0, 2e-3, n)
-    h["HPI"][L] = rng.uniform(0.1, 0.5, n)
-    consts = []
-    mos = [1, 2, 3, 4][:nconst]
-    for mo in mos:
-        con = np.zeros((g.KCM, g.LCM))
-        con[:kc, L] = rng.uniform(-1, 30, (kc, n))
-        consts.append(dict(mo=mo, isadac=isadac, isfct=isfct, rkdye=0.0,
-                           is_dye=int(mo == 3), con=con, con1=con.copy()))
-    nc = len(consts)
-    sides_c = [boundary_cells(g, m, rng, 5) for m in ("LSC", "LWC", "LEC", "LNC")]
-    # make sure one corner cell sits in two concentration-BC sides
-    corner = [l for l in range(2, g.LA + 1)
-              if g.maps["LSC"][l - 1] == g.LC and g.maps["LWC"][l - 1] == g.LC]
-    if corner:
-        c0 = corner[0]
-        sides_c[0] = np.unique(np.append(sides_c[0], c0)).astype(np.int32)
-        sides_c[1] = np.unique(np.append(sides_c[1], c0)).astype(np.int32)
-    wtci = rng.uniform(0, 1, (kc, 2))
-    cside = []
-    for cells in sides_c:
-        m = len(cells)
-        cside.append(dict(L=cells, cb=rng.uniform(0, 30, (m, 2, nc)),
-                          csert=rng.uniform(0, 5, (m, kc, nc)),
-                          ntscr=rng.integers(0, 4, m).astype(np.int32)))
-    conc_bc = dict(side=cside, wtci=wtci)
-    # pressure BC lists (W, E, S, N) on the same cells: they define LOBCS
-    order = {0: 1, 1: 2, 2: 0, 3: 3}  # pbc side -> conc side
-    pside = []
-    for s in range(4):
-        cells = sides_c[order[s]]
-        m = len(cells)
-        pside.append(dict(L=cells, ispb=rng.integers(0, 3, m).astype(np.int32),
-                          pser=rng.normal(0, 5, m), pcb=rng.normal(0, 3, (m, 2)),
-                          psb=rng.normal(0, 3, (m, 2))))
-    open_bc = dict(side=pside, mtide=2, tcp=np.array([44712.0, 43200.0]))
-    nq = 3
-    lqs = rng.choice(L + 1, nq, replace=False).astype(np.int32)
-    sources = dict(LQS=lqs, qss=rng.normal(0, 1e-2, (nq, kc)),
-                   qsercell=rng.normal(0, 1e-2, (nq, kc)),
-                   cqs=rng.uniform(0, 50, (nq, kc, nc)),
-                   csert=rng.uniform(0, 50, (nq, kc, nc)))
-    return grids.Case(name="rand", grid=g, h=h, k=k, w=w, consts=consts,
-                      open_bc=open_bc, conc_bc=conc_bc, sources=sources, dt=5.0)
 
 
 def load_reference(case, clo_init=None, nlo_init=None):
@@ -217,6 +146,63 @@ def test_caltran_bitexact_vs_reference(oracle_mod, seed, isadac, isfct):
             mine_n = mem["nlo"].reshape(m, kc, nc)[:, :, ci]
             np.testing.assert_array_equal(mine_c, clo[:m, :kc, M])
             np.testing.assert_array_equal(mine_n, nlo[:m, :kc, M])
+
+
+@pytest.mark.parametrize("isadac,isfct", [(0, 0), (1, 1)])
+@pytest.mark.parametrize("seed", [6, 7])
+def test_caltran_with_dry_cells_bitexact_vs_reference(oracle_mod, seed, isadac, isfct):
+    """CALTRAN over the wet lists (ISDRY > 0): about a fifth of the cells
+    are left out by LMASKDRY (the reference's LKWET/LWET loops and the
+    open-boundary CYCLEs, EFDC/caltran.f90:95-300,324-441,515), and, as
+    after a drying step, the faces of a masked cell carry no flow."""
+    case = random_case(seed, isadac=isadac, isfct=isfct)
+    g = case.grid
+    kc = g.KC
+    rng = np.random.default_rng(50 + seed)
+    mask = np.ones(g.LCM, np.int32)
+    L = np.arange(2, g.LA + 1)
+    dry = L[rng.uniform(size=len(L)) < 0.2]
+    mask[dry - 1] = 0
+    for l in dry:
+        i, e, n_ = l - 1, g.maps["LEC"][l - 1] - 1, g.maps["LNC"][l - 1] - 1
+        for key, faces in (("UHDY2", (i, e)), ("VHDX2", (i, n_)), ("U2", (i, e)),
+                           ("V2", (i, n_))):
+            case.k[key][:, list(faces)] = 0.0
+        case.w["W2"][:, i] = 0.0
+        case.h["SUB"][[i, e]] = 0.0
+        case.h["SVB"][[i, n_]] = 0.0
+    ref_case = case.copy()
+    # the library and the oracle keep SUB3DO in SUB3D and scale it by SUB
+    # when drying is on; the reference's CALPUV2C has already set
+    # SUB3D = SUB*SUB3DO (EFDC/calpuv2c.f90:982-990)
+    ref_case.k["SUB3D"][:] = ref_case.k["SUB3D"] * ref_case.h["SUB"][None, :]
+    ref_case.k["SVB3D"][:] = ref_case.k["SVB3D"] * ref_case.h["SVB"][None, :]
+    o = oracle_mod.Oracle(case, isdry=1)
+    o.lmaskdry[:] = mask
+    load_reference(ref_case)
+    R.lib().ref_wetlist(mask.ctypes.data)
+    for c in ref_case.consts:
+        R.lib().ref_flags(R.mvar_of(c["mo"]), c["isadac"], c["isfct"])
+    moved = False
+    for niter in (1, 2, 3, 7):
+        for ci, c in enumerate(case.consts):
+            o.caltran_only(ci, niter=niter)
+            rc = ref_case.consts[ci]
+            con = np.ascontiguousarray(rc["con"])
+            con1 = np.ascontiguousarray(rc["con1"])
+            R.lib().ref_caltran(R.mvar_of(rc["mo"]), rc["mo"], con.ctypes.data,
+                                con1.ctypes.data, case.dt, niter)
+            rc["con"][:] = con
+            rc["con1"][:] = con1
+            np.testing.assert_array_equal(c["con"][:kc], rc["con"][:kc],
+                                          err_msg=f"CON mo={c['mo']} niter={niter}")
+            np.testing.assert_array_equal(c["con1"][:kc], rc["con1"][:kc],
+                                          err_msg=f"CON1 mo={c['mo']} niter={niter}")
+            # a masked column keeps its old level
+            np.testing.assert_array_equal(c["con"][:kc, dry - 1], c["con1"][:kc, dry - 1])
+            wet = L[mask[L - 1] == 1] - 1
+            moved |= bool(np.any(c["con"][:kc, wet] != c["con1"][:kc, wet]))
+    assert moved
 
 
 @pytest.mark.parametrize("seed", [3, 4, 5])
