@@ -162,6 +162,10 @@ static bool tran_shape_valid(int q) {
 // context to hold it); efdc_last_error(NULL) returns it.
 static thread_local std::string g_create_err = "no error";
 
+}  // extern "C"
+void efdc_thread_error(const char *why) { g_create_err = why; }
+extern "C" {
+
 const char *efdc_last_error(const efdc_ctx_t *ctx) {
   return ctx ? ctx->err.c_str() : g_create_err.c_str();
 }

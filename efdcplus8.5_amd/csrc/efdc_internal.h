@@ -518,6 +518,8 @@ inline int dev_alloc(efdc_ctx *c, void **p, size_t bytes) {
   return EFDC_OK;
 }
 int efdc_grid_blocks(int n);
+// the reason of a failed context-free call (efdc_last_error(NULL))
+void efdc_thread_error(const char *why);
 // persistent CONGRAD (efdc_cgp.hip)
 bool pcg_applicable(efdc_ctx *c, bool mixed);
 int pcg_solve(efdc_ctx *c, bool mixed, bool defer = false);

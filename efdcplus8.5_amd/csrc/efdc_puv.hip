@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <utility>
 
 #include "efdc_internal.h"
@@ -1665,5 +1666,173 @@ int seg_cg_run(efdc_ctx *c, efdc_cg_stats_t *stats) {
     c->err = "CONGRAD: maximum iterations exceeded in external solution (a basin)";
     return EFDC_ERR_CG_MAXITER;
   }
+  return EFDC_OK;
+}
+
+// ---------------------------------------------------------------------
+// CONGRAD without device allocation (efdc_congrad_noalloc): the FP64
+// multi-kernel schedule above (k_cg_init, k_cg_a, k_cg_b) on the caller's
+// device arrays, with every scratch vector, partial sum, arrival counter
+// and the solve state in one static device array per GPU.
+// Layout (bytes): CgState at 0, arrival counters at NA_OFF_TK, partials at
+// NA_OFF_PART, then r, Ap, p (two buffers) as FP64 vectors of
+// EFDC_NOALLOC_MAX_LCM entries from NA_OFF_VEC, then the 0-based
+// LWC/LEC/LSC/LNC as int32 vectors of the same length.
+// ---------------------------------------------------------------------
+namespace {
+constexpr size_t na_up(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t NA_MAXC = EFDC_NOALLOC_MAX_LCM;
+constexpr int NA_MAXB = (int)((NA_MAXC + CG_CPT * EFDC_BLOCK - 1) / (CG_CPT * EFDC_BLOCK));
+constexpr size_t NA_OFF_TK = na_up(sizeof(CgState));
+constexpr size_t NA_TK_BYTES = sizeof(unsigned) * TICKET_SITES * TICKET_SITE;
+constexpr size_t NA_OFF_PART = NA_OFF_TK + na_up(NA_TK_BYTES);
+constexpr size_t NA_OFF_VEC = 65536;
+constexpr size_t NA_OFF_MAP = NA_OFF_VEC + 4 * NA_MAXC * sizeof(double);
+constexpr size_t NA_BYTES = NA_OFF_MAP + 4 * NA_MAXC * sizeof(int);
+static_assert(NA_OFF_PART + na_up(sizeof(double) * (2 * NA_MAXB + 16)) <= NA_OFF_VEC,
+              "no-alloc header overflows its 64 KiB");
+static_assert(NA_BYTES == EFDC_NOALLOC_SCRATCH_BYTES, "scratch size differs from the header");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+// Device address of `scratch` on each GPU, looked up once per device (each
+// GPU holds its own copy of the array at its own address).
+std::mutex na_mu;
+void *na_ptr[64] = {};
+
+// Zero the solve state, the arrival counters and the first p buffer; turn
+// the 1-based maps into 0-based indices; any wet row whose neighbour lies
+// outside 2..LC raises st->status = 2 and is pointed at the ghost LC so
+// that nothing reads out of bounds.
+__global__ __launch_bounds__(EFDC_BLOCK) void k_na_prepare(
+    unsigned char *base, const int *__restrict__ lwc, const int *__restrict__ lec,
+    const int *__restrict__ lsc, const int *__restrict__ lnc, int la) {
+  const int i = blockIdx.x * EFDC_BLOCK + (int)threadIdx.x;  // 0 .. la (L-1)
+  if (blockIdx.x == 0) {
+    unsigned *w = reinterpret_cast<unsigned *>(base);
+    for (int j = threadIdx.x; j < (int)(NA_OFF_PART / sizeof(unsigned)); j += EFDC_BLOCK)
+      w[j] = 0u;
+  }
+  if (i > la) return;
+  int *m = reinterpret_cast<int *>(base + NA_OFF_MAP);
+  double *p0 = reinterpret_cast<double *>(base + NA_OFF_VEC) + 2 * NA_MAXC;
+  p0[i] = 0.0;
+  const int *src[4] = {lwc, lec, lsc, lnc};
+  const bool wet = (i >= 1 && i < la);
+  bool bad = false;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    int v = src[q][i] - 1;
+    if (v < 1 || v > la) {
+      if (wet) bad = true;
+      v = la;
+    }
+    m[q * NA_MAXC + i] = v;
+  }
+  if (bad) {
+    CgState *st = reinterpret_cast<CgState *>(base);
+    __hip_atomic_store(&st->status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+}  // namespace
+
+int efdc_congrad_noalloc(int32_t LA, const int32_t *LWC, const int32_t *LEC,
+                         const int32_t *LSC, const int32_t *LNC, const double *CCC,
+                         const double *CCS, const double *CCW, const double *CCE,
+                         const double *CCN, const double *CCCI, const double *FPTMP,
+                         double *P, double RSQM, int32_t ITERM, void *stream,
+                         efdc_cg_stats_t *stats) {
+  auto fail = [](int st, const std::string &why) {
+    efdc_thread_error(why.c_str());
+    return st;
+  };
+  if (LA < 2 || (size_t)LA + 1 > NA_MAXC)
+    return fail(EFDC_ERR_ARG, "efdc_congrad_noalloc: LA+1 must lie in 3.." +
+                                  std::to_string(NA_MAXC));
+  if (!LWC || !LEC || !LSC || !LNC || !CCC || !CCS || !CCW || !CCE || !CCN || !CCCI ||
+      !FPTMP || !P)
+    return fail(EFDC_ERR_ARG, "efdc_congrad_noalloc: null array");
+  if (ITERM < 1 || !(RSQM >= 0.0))
+    return fail(EFDC_ERR_ARG, "efdc_congrad_noalloc: ITERM < 1 or RSQM not >= 0");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(EFDC_ERR_NODEVICE, hipGetErrorString(e));
+  if (dev < 0 || dev >= 64) return fail(EFDC_ERR_ARG, "efdc_congrad_noalloc: device id >= 64");
+  unsigned char *base;
+  {
+    std::lock_guard<std::mutex> g(na_mu);
+    if (!na_ptr[dev]) {
+      void *p = nullptr;
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+        return fail(EFDC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                      hipGetErrorString(e));
+      na_ptr[dev] = p;
+    }
+    base = static_cast<unsigned char *>(na_ptr[dev]);
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  CgState *st = reinterpret_cast<CgState *>(base);
+  double *vec = reinterpret_cast<double *>(base + NA_OFF_VEC);
+  int *map = reinterpret_cast<int *>(base + NA_OFF_MAP);
+  CgArgs a;
+  a.ccc = CCC;
+  a.ccs = CCS;
+  a.ccw = CCW;
+  a.ccn = CCN;
+  a.cce = CCE;
+  a.ccci = CCCI;
+  a.fptmp = FPTMP;
+  a.x = P;
+  a.r = vec;
+  a.ap = vec + NA_MAXC;
+  double *pb[2] = {vec + 2 * NA_MAXC, vec + 3 * NA_MAXC};
+  a.lw = map;
+  a.le = map + NA_MAXC;
+  a.ls = map + 2 * NA_MAXC;
+  a.ln = map + 3 * NA_MAXC;
+  a.nwet = LA - 1;
+  a.lc0 = LA;
+  a.rowmajor = 0;  // the general four-map operator: no layout is assumed
+  a.st = st;
+  a.part = reinterpret_cast<double *>(base + NA_OFF_PART);
+  a.tk = reinterpret_cast<unsigned *>(base + NA_OFF_TK);
+  CgState h;
+  auto sync_state = [&]() -> hipError_t {
+    hipError_t q = hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, s);
+    if (q != hipSuccess) return q;
+    return hipStreamSynchronize(s);
+  };
+  k_na_prepare<<<(LA + EFDC_BLOCK) / EFDC_BLOCK, EFDC_BLOCK, 0, s>>>(base, LWC, LEC, LSC, LNC,
+                                                                    LA);
+  if ((e = hipGetLastError()) != hipSuccess || (e = sync_state()) != hipSuccess)
+    return fail(EFDC_ERR_HIP, hipGetErrorString(e));
+  if (h.status == 2)
+    return fail(EFDC_ERR_ARG, "efdc_congrad_noalloc: a wet row's neighbour map lies outside 2..LA+1");
+  const int nb = cg_blocks(LA - 1);
+  k_cg_init<<<nb, EFDC_BLOCK, 0, s>>>(a, RSQM, 0.0);
+  int it = 0, chunk = 16;
+  for (;;) {
+    for (int j = 0; j < chunk && it < ITERM; ++j, ++it) {
+      k_cg_a<<<nb, EFDC_BLOCK, 0, s>>>(a, pb[it & 1], pb[(it + 1) & 1]);
+      k_cg_b<<<nb, EFDC_BLOCK, 0, s>>>(a, pb[(it + 1) & 1], ITERM);
+    }
+    if ((e = hipGetLastError()) != hipSuccess || (e = sync_state()) != hipSuccess)
+      return fail(EFDC_ERR_HIP, hipGetErrorString(e));
+    if (h.done || it >= ITERM) break;
+    chunk = 8;
+  }
+  if (stats) {
+    stats->iters = h.iter;
+    stats->outer = 1;
+    stats->rsq = h.rsq;
+    stats->rsqm_used = h.rsqm;
+    stats->ccmnm = 0.0;
+  }
+  if (h.status != 0)
+    return fail(EFDC_ERR_CG_MAXITER,
+                "CONGRAD: maximum iterations exceeded in external solution");
   return EFDC_OK;
 }

@@ -205,6 +205,9 @@ def load(path: Optional[str] = None):
         "efdc_set_segments": ([P, C.c_int32, PI], C.c_int),
         "efdc_segment_stats": ([P, C.c_int32, PI, PI], C.c_int),
         "efdc_balance_segments": ([P, C.c_int32, PD, PD], C.c_int),
+        # device pointers (maps, 7 coefficient arrays, P), RSQM, ITERM, stream, stats
+        "efdc_congrad_noalloc": ([C.c_int32] + [P] * 12 + [C.c_double, C.c_int32, P,
+                                                          C.POINTER(CgStats)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

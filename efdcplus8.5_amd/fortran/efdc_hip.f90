@@ -241,6 +241,19 @@ module efdc_hip
       type(efdc_cg_stats_t), intent(out) :: stats
     end function
 
+    ! CONGRAD on device arrays the caller holds (C_DEVLOC of a device
+    ! allocation), with no device allocation of its own
+    integer(c_int) function efdc_congrad_noalloc(la, lwc, lec, lsc, lnc, ccc, ccs, ccw, &
+        cce, ccn, ccci, fptmp, p, rsqm, iterm, stream, stats) &
+        bind(C, name="efdc_congrad_noalloc")
+      import :: c_int, c_int32_t, c_ptr, c_double, efdc_cg_stats_t
+      integer(c_int32_t), value :: la, iterm
+      type(c_ptr), value :: lwc, lec, lsc, lnc, ccc, ccs, ccw, cce, ccn, ccci, fptmp, p
+      real(c_double), value :: rsqm
+      type(c_ptr), value :: stream
+      type(efdc_cg_stats_t), intent(out) :: stats
+    end function
+
     integer(c_int) function efdc_calpuv2c(ctx, step, stats) bind(C, name="efdc_calpuv2c")
       import :: c_int, c_ptr, efdc_step_t, efdc_cg_stats_t
       type(c_ptr), value :: ctx

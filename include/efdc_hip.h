@@ -285,7 +285,8 @@ int efdc_ctx_create(const efdc_grid_t *grid, const efdc_params_t *params,
                     efdc_ctx_t **out);
 int efdc_ctx_destroy(efdc_ctx_t *ctx);
 /* Text of the last error of a context; with ctx == NULL, the reason the
-   last efdc_ctx_create on the calling thread failed. */
+   last efdc_ctx_create or efdc_congrad_noalloc on the calling thread
+   failed. */
 const char *efdc_last_error(const efdc_ctx_t *ctx);
 int efdc_set_params(efdc_ctx_t *ctx, const efdc_params_t *params);
 
@@ -333,6 +334,33 @@ int efdc_get_dry_waste(efdc_ctx_t *ctx, double *qdwaste, double *vdwaste, int32_
 /* CONGRAD (EFDC/congrad.f90:1-191) on the scaled system already held in
    CCC, CCS, CCW, CCE, CCN, CCCI, FPTMP, starting from and updating P. */
 int efdc_congrad(efdc_ctx_t *ctx, efdc_cg_stats_t *stats);
+
+/* CONGRAD without any device allocation.  No counterpart in the reference:
+   it is for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   The same FP64 Jacobi-PCG as efdc_congrad with EFDC_CG_KERNEL_MULTI (the
+   same kernels, so the same bits), on arrays the caller already holds in
+   device memory: the neighbour maps LWC/LEC/LSC/LNC (1-based L values as
+   the host keeps them, LC for land) and CCC, CCS, CCW, CCE, CCN, CCCI,
+   FPTMP, P, each with at least LA+1 entries in the (LCM) layout; P is the
+   initial guess and is overwritten by the solution.  RSQM and ITERM are
+   the stop test and the cap (sum r^2 <= RSQM, EFDC/congrad.f90:165).
+   Work goes to `stream` (a hipStream_t; NULL: the null stream) and the
+   call returns after the solve has finished.
+   Scratch: one static device array of EFDC_NOALLOC_SCRATCH_BYTES per GPU
+   (held until the process ends), which limits LA+1 to EFDC_NOALLOC_MAX_LCM
+   (EFDC_ERR_ARG beyond).  Every call on a GPU shares that GPU's array:
+   calls that overlap on one GPU race.  A wet row whose map points outside
+   2..LA+1 is EFDC_ERR_ARG; efdc_last_error(NULL) gives the reason of a
+   failed call on the calling thread. */
+#define EFDC_NOALLOC_MAX_LCM 1048576
+#define EFDC_NOALLOC_SCRATCH_BYTES (48u * 1048576u + 65536u)
+int efdc_congrad_noalloc(int32_t LA, const int32_t *LWC, const int32_t *LEC,
+                         const int32_t *LSC, const int32_t *LNC, const double *CCC,
+                         const double *CCS, const double *CCW, const double *CCE,
+                         const double *CCN, const double *CCCI, const double *FPTMP,
+                         double *P, double RSQM, int32_t ITERM, void *stream,
+                         efdc_cg_stats_t *stats);
 
 /* CALPUV2C for the 2TL scheme (EFDC/calpuv2c.f90:128-571,896-939):
    explicit stage, assembly, open boundaries, min-diagonal scaling,

@@ -89,3 +89,18 @@ def test_fortran_module_binds_every_declared_symbol():
     bound = set(re.findall(r'name="(efdc_\w+)"', open(f90).read()))
     missing = [f for f in declared_functions() if f not in bound]
     assert not missing, missing
+
+
+def test_noalloc_congrad_refuses_sizes_beyond_its_scratch():
+    """efdc_congrad_noalloc checks LA against its static scratch before any
+    device call: too large (or too small) is EFDC_ERR_ARG with a reason."""
+    import ctypes as C
+    from efdc_amd import lib
+    from efdc_amd.abi import STATUS
+    L = lib.load()
+    st = lib.CgStats()
+    dummy = [1] * 12  # never dereferenced: the size check comes first
+    for la in (L.efdc_congrad_noalloc and 1048576, 5_000_000, 1):
+        rc = L.efdc_congrad_noalloc(la, *dummy, 1e-9, 100, None, C.byref(st))
+        assert rc == STATUS["EFDC_ERR_ARG"]
+        assert b"LA+1" in L.efdc_last_error(None)
