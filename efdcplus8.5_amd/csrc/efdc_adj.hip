@@ -912,6 +912,9 @@ std::vector<Piece> state_pieces(efdc_ctx *c) {
     v.push_back({c->cbc[s].nlo, sizeof(long long) * n});
   }
   v.push_back({c->lopenbcdry, sizeof(int) * c->LCM});
+  std::vector<std::pair<void *, size_t>> iv;
+  uvw_state_pieces(c, iv);
+  for (auto &q : iv) v.push_back({q.first, q.second});
   return v;
 }
 

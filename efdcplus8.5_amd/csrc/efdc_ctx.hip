@@ -99,7 +99,7 @@ int upload_new(efdc_ctx *c, T **dst, const T *src, size_t n) {
 bool valid_id(int id) {
   return (id >= 0 && id < EFDC_NUM_H) ||
          (id >= EFDC_HPK && id < EFDC_NUM_K_END) ||
-         (id >= EFDC_W2 && id < EFDC_NUM_W_END);
+         (id >= EFDC_W2 && id < EFDC_NUM_W_END) || uvw_id(id);
 }
 
 }  // namespace
@@ -113,7 +113,8 @@ const char *const efdc_timer_names[TK_COUNT] = {
     "puv_depths", "puv_hu_hv", "tran_upwind_vdiff", "tran_bc_con1",
     "closure_pre", "closure_post", "balance", "adac_gradients",
     "adac_fluxes", "fct_limiters", "tran_finish_vdiff", "cg_persistent64",
-    "mx_persistent", "adj_tran_col", "adj_tran_gather", "adj_hydro", "adj_cg", "settle", "cg_iterations"};
+    "mx_persistent", "adj_tran_col", "adj_tran_gather", "adj_hydro", "adj_cg", "settle",
+    "uvw_columns", "uvw_w_averages", "uvw_openbc", "uvw_depths", "uvw_uv", "cg_iterations"};
 
 hipEvent_t timer_event(efdc_ctx *c) {
   if (c->ev_pool.empty()) {
@@ -251,6 +252,11 @@ int efdc_ctx_create(const efdc_grid_t *g, const efdc_params_t *prm,
   }
   c->rowmajor = rowmajor && mutual;
   c->mutual = mutual;
+  c->has_diag = g->LNWC != nullptr && g->LSEC != nullptr;
+  c->host_lw = lw;
+  c->host_le = le;
+  c->host_ls = ls;
+  c->host_ln = ln;
   c->band = band;
   int s;
   if ((s = upload_new(c, &c->dLW, lw.data(), LCM)) ||
@@ -335,6 +341,7 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   dfree(c->pcg_ctr); dfree(c->pcg_abort); dfree(c->pcg_flag);
   seg_free(c);
   adj_free(c);
+  uvw_free(c);
   if (c->st_host) (void)hipHostFree(c->st_host);
   if (c->pcg_abort_host) (void)hipHostFree(c->pcg_abort_host);
   dfree(c->gat_off);
@@ -410,6 +417,35 @@ int efdc_set_params(efdc_ctx_t *c, const efdc_params_t *p) {
 int efdc_set_array(efdc_ctx_t *c, int32_t id, const double *host) {
   if (!c || !host || !valid_id(id)) return EFDC_ERR_ARG;
   c->symok_cache = -1;
+  if (uvw_id(id)) {
+    double *d = nullptr;
+    int planes = 0, s;
+    if ((s = uvw_ptr(c, id, &d, &planes))) return s;
+    if (planes == 0)
+      HIP_TRY(c, hipMemcpyAsync(d, host, sizeof(double) * c->LCM, hipMemcpyHostToDevice,
+                                c->stream));
+    else
+      HIP_TRY(c, hipMemcpy2DAsync(d, sizeof(double) * c->pitch, host, sizeof(double) * c->LCM,
+                                  sizeof(double) * c->LCM, planes, hipMemcpyHostToDevice,
+                                  c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return EFDC_OK;
+  }
+  if (id == EFDC_DZC) {
+    // sigma layers: one thickness per layer in every wet cell (the internal
+    // mode's face constants, EFDC/aaefdc.f90:2001-2126, are then per layer)
+    bool uni = true;
+    std::vector<double> perk(64, 0.0);
+    for (int k = 0; k < c->KC && uni; ++k) {
+      const double *row = host + (size_t)k * c->LCM;
+      perk[k] = row[1];
+      for (int L = 2; L <= c->LA; ++L)
+        if (row[L - 1] != perk[k]) { uni = false; break; }
+    }
+    c->dzc_uni = uni;
+    c->dzc_k = perk;
+    c->dzc_changed = true;
+  }
   if (id < EFDC_NUM_H) {
     HIP_TRY(c, hipMemcpyAsync(c->dH[id], host, sizeof(double) * c->LCM,
                               hipMemcpyHostToDevice, c->stream));
@@ -450,6 +486,20 @@ int efdc_set_array(efdc_ctx_t *c, int32_t id, const double *host) {
 
 int efdc_get_array(efdc_ctx_t *c, int32_t id, double *host) {
   if (!c || !host || !valid_id(id)) return EFDC_ERR_ARG;
+  if (uvw_id(id)) {
+    double *d = nullptr;
+    int planes = 0, s;
+    if ((s = uvw_ptr(c, id, &d, &planes))) return s;
+    if (planes == 0)
+      HIP_TRY(c, hipMemcpyAsync(host, d, sizeof(double) * c->LCM, hipMemcpyDeviceToHost,
+                                c->stream));
+    else
+      HIP_TRY(c, hipMemcpy2DAsync(host, sizeof(double) * c->LCM, d, sizeof(double) * c->pitch,
+                                  sizeof(double) * c->LCM, planes, hipMemcpyDeviceToHost,
+                                  c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return EFDC_OK;
+  }
   if (id < EFDC_NUM_H) {
     HIP_TRY(c, hipMemcpyAsync(host, c->dH[id], sizeof(double) * c->LCM,
                               hipMemcpyDeviceToHost, c->stream));
@@ -474,7 +524,12 @@ int efdc_get_array(efdc_ctx_t *c, int32_t id, double *host) {
 
 int efdc_device_ptr(efdc_ctx_t *c, int32_t id, void **ptr, int64_t *pitch) {
   if (!c || !ptr || !valid_id(id)) return EFDC_ERR_ARG;
-  if (id < EFDC_NUM_H) *ptr = c->dH[id];
+  if (uvw_id(id)) {
+    double *d = nullptr;
+    int planes = 0, s;
+    if ((s = uvw_ptr(c, id, &d, &planes))) return s;
+    *ptr = d;
+  } else if (id < EFDC_NUM_H) *ptr = c->dH[id];
   else if (id >= EFDC_W2) *ptr = c->dW[id - EFDC_W2];
   else *ptr = c->dK[id - EFDC_HPK];
   if (pitch) *pitch = c->pitch;
@@ -531,6 +586,7 @@ int efdc_set_open_bc(efdc_ctx_t *c, const efdc_open_bc_t *bc) {
       std::memcpy(pc.data(), s.pcb, sizeof(double) * s.n * bc->mtide);
       std::memcpy(ps.data(), s.psb, sizeof(double) * s.n * bc->mtide);
     }
+    c->pbc_ispb_host[side].assign(s.ispb, s.ispb + s.n);
     if ((st = upload_new(c, &d.L, L0.data(), s.n)) ||
         (st = upload_new(c, &d.ispb, s.ispb, s.n)) ||
         (st = upload_new(c, &d.pser, s.pser, s.n)) ||
@@ -570,6 +626,7 @@ int efdc_set_open_bc(efdc_ctx_t *c, const efdc_open_bc_t *bc) {
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
   }
+  uvw_obc_changed(c);
   c->mtide = bc->mtide;
   delete[] c->tcp_host;
   c->tcp_host = new double[std::max(bc->mtide, 1)];
@@ -783,6 +840,7 @@ int efdc_set_conc_bc(efdc_ctx_t *c, const efdc_conc_bc_t *bc) {
       else if (side == 2) q.z = LL;
       else q.w = LL;
     }
+    c->cbc_cells_host[side] = L0;
     int st;
     if ((st = upload_new(c, &d.L, L0.data(), s.n)) ||
         (st = upload_new(c, &d.cb, s.cb, (size_t)s.n * 2 * nc)) ||
@@ -798,6 +856,7 @@ int efdc_set_conc_bc(efdc_ctx_t *c, const efdc_conc_bc_t *bc) {
     }
     d.n = s.n;
   }
+  uvw_obc_changed(c);
   int st;
   c->nbcrec = (int)info.size();
   c->cbc_nc = nc;
@@ -1018,6 +1077,11 @@ int efdc_calpuv2c(efdc_ctx_t *c, const efdc_step_t *step,
                   efdc_cg_stats_t *stats) {
   if (!c || !step || !(step->dt > 0.0)) return EFDC_ERR_ARG;
   return puv_run(c, step, stats);
+}
+
+int efdc_caluvw(efdc_ctx_t *c, const efdc_step_t *step, const efdc_uvw_opts_t *o) {
+  if (!c || !step || !o || !(step->dt > 0.0)) return EFDC_ERR_ARG;
+  return uvw_run(c, step, o);
 }
 
 int efdc_calconc(efdc_ctx_t *c, const efdc_step_t *step) {

@@ -9,6 +9,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/efdc_hip.h"
@@ -226,6 +227,15 @@ struct efdc_ctx {
   int tune_chosen[8] = {};
   // discrete adjoint (efdc_adj.hip), allocated by efdc_adj_enable
   struct AdjState *adj = nullptr;
+  // internal mode (efdc_uvw.hip), allocated on first use
+  struct UvwState *uvw = nullptr;
+  bool dzc_uni = false;           // DZC set and equal in every wet cell of a layer
+  bool dzc_changed = false;
+  std::vector<double> dzc_k;      // DZC per layer when dzc_uni
+  bool has_diag = false;          // LNWC and LSEC were given
+  std::vector<int> host_lw, host_le, host_ls, host_ln;  // 0-based maps
+  std::vector<int> cbc_cells_host[4];  // concentration-BC cells S, W, E, N (0-based)
+  std::vector<int> pbc_ispb_host[4];   // ISPB of the pressure sides W, E, S, N
 
   int n_wet() const { return LA - 1; }
 };
@@ -238,6 +248,7 @@ enum {
   TK_TRAN_BC, TK_CLOSURE_PRE, TK_CLOSURE_POST, TK_BALANCE, TK_ADAC_GRAD,
   TK_ADAC_FLUX, TK_FCT, TK_COL_FINISH, TK_PCG64, TK_PCGMX, TK_ADJ_TCOL, TK_ADJ_TGATHER,
   TK_ADJ_HYDRO, TK_ADJ_CG, TK_SETTLE,
+  TK_UVW_COL, TK_UVW_W, TK_UVW_OBC, TK_UVW_HP, TK_UVW_UV,
   TK_CG_ITERS,  // not a kernel: multi-kernel CG iterations performed
   TK_COUNT
 };
@@ -538,6 +549,14 @@ void adj_free(efdc_ctx *c);
 int cg_fp64_vec(efdc_ctx *c, double *x, const double *rhs, CgState *st, CgState *st_host,
                 double rel_tol, int itmax, int *iters, double *rsq);
 int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats);
+// internal mode (efdc_uvw.hip)
+bool uvw_id(int id);
+int uvw_ptr(efdc_ctx *c, int id, double **p, int *planes);  // planes 0: (LCM)
+void uvw_step_begin(efdc_ctx *c);
+void uvw_state_pieces(efdc_ctx *c, std::vector<std::pair<void *, size_t>> &v);
+void uvw_obc_changed(efdc_ctx *c);
+void uvw_free(efdc_ctx *c);
+int uvw_run(efdc_ctx *c, const efdc_step_t *step, const efdc_uvw_opts_t *o);
 int cg_run(efdc_ctx *c, bool check_sym, efdc_cg_stats_t *stats);
 int tran_run(efdc_ctx *c, const efdc_step_t *step);
 int balance_run(efdc_ctx *c, efdc_balance_t *out);

@@ -1438,6 +1438,7 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
       }
     }
   }
+  uvw_step_begin(c);
   k_step_begin<<<1, 1, 0, c->stream>>>(c->st, rot);
   if (dry) {
     k_dry_begin<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->iscdry, c->sub1, c->svb1,

@@ -45,8 +45,8 @@ STATUS_NAMES = {v: k for k, v in STATUS.items()}
 def shape_class(name: str) -> str:
     """'h' for (LCM), 'k' for (LCM,KCM), 'w' for (LCM,0:KCM)."""
     v = ARRAY_IDS[name]
-    if v < 64:
+    if v < 64 or 128 <= v < 160:
         return "h"
-    if v < 96:
+    if v < 96 or 160 <= v < 192:
         return "k"
     return "w"

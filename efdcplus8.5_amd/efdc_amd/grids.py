@@ -42,6 +42,11 @@ H_KEYS = (
 K_KEYS = ("HPK H1PK HPKI UHDY2 VHDX2 CDZKK CDZKMK AB QSUM U2 V2 SUB3D SVB3D "
           "DZC").split()
 W_KEYS = ("W2", "DZIG")
+# internal mode (CALUVW): bound by add_internal_mode
+IH_KEYS = "STBX STBY VU UV V1U U1V DYU DXV".split()
+IK_KEYS = ("UHDYF VHDXF UHDY VHDX UHDYF1 VHDXF1 UHDY1 VHDX1 U V U1 V1 UHDYF2 VHDXF2 "
+           "UHDYEK VHDXEK UHDY1EK VHDX1EK DU DV AVUI AVVI").split()
+IW_KEYS = ("W", "W1")
 
 
 @dataclasses.dataclass
@@ -312,7 +317,8 @@ def _constituent(g: Grid, mo: int, con: np.ndarray, **kw) -> dict:
 def tidal_basin(nx=128, ny=64, kc=5, *, seed=1000, dx=100.0, dt=5.0,
                 amp=0.5, period=44712.0, manning=0.025, ab=1e-4,
                 dye_load=1.0, depth=(5.0, 20.0), depth_sigma=8.0,
-                ntscr=12, isadac=0, isfct=0, name=None, wet=None) -> Case:
+                ntscr=12, isadac=0, isfct=0, name=None, wet=None,
+                dzck=None) -> Case:
     """Config-0 style case: rectangular basin nx*ny, west column is a
     clamped open boundary (ISPBW=0) with an M2-like tide, other sides
     closed.  Salinity 0->30 psu W->E, a dye point source (QSS=1e-3 m^3/s
@@ -331,7 +337,7 @@ def tidal_basin(nx=128, ny=64, kc=5, *, seed=1000, dx=100.0, dt=5.0,
         ny, nx = wet.shape[0] - 2, wet.shape[1] - 2
     g = build_grid(wet, kc)
     h, k, w = _empty_arrays(g)
-    fill_geometry(g, h, k, w, dx, np.ones(kc))
+    fill_geometry(g, h, k, w, dx, np.ones(kc) if dzck is None else np.asarray(dzck, float))
     dep_i = _rescale(_smooth_noise(rng, (ny, nx), depth_sigma), *depth)
     dep = _pad_ring(dep_i, 0.0)
     L = np.arange(2, g.LA + 1) - 1
@@ -748,3 +754,49 @@ def basin_slice(case: Case, first_L, b: int) -> slice:
     lo = int(first_L[b]) - 1
     hi = int(first_L[b + 1]) - 1 if b + 1 < len(first_L) else case.grid.LA
     return slice(lo, hi)
+
+
+def add_internal_mode(case: Case, *, av=1e-3, drag=None) -> Case:
+    """Bind the arrays of the internal mode (CALUVW, EFDC/caluvw.f90) to a
+    case, in place: the layer flows of the depth-uniform state that the
+    external flow UHDYE/VHDXE gives (UHDYF = UHDYE, UHDY = UHDYE*DZC,
+    U = UHDYE*HUI*DYIU per layer, W = 0) at both levels, the inverse face
+    viscosities AVUI = AVVI = 1/av (EFDC/ainit.f90:211; `av` a scalar or a
+    (KC-1,) vector per interface), the face widths DYU = DXV = dx, and the
+    bottom-drag coefficients STBX = STBY: `drag` if given, else
+    G*n^2/HU^(1/3) from the case's Manning n at the initial depth (the
+    mapping of a Manning coefficient onto EFDC's quadratic drag)."""
+    g = case.grid
+    LCM, KCM, KC = g.LCM, g.KCM, g.KC
+    L = np.arange(2, g.LA + 1) - 1
+    h, k, w = case.h, case.k, case.w
+    for key in IH_KEYS:
+        h[key] = np.zeros(LCM)
+    for key in IK_KEYS:
+        k[key] = np.zeros((KCM, LCM))
+    for key in IW_KEYS:
+        w[key] = np.zeros((KCM + 1, LCM))
+    dx = float(case.meta.get("dx", 1.0 / h["DYIU"][L[0]]))
+    h["DYU"][L] = dx
+    h["DXV"][L] = dx
+    if drag is not None:
+        h["STBX"][L] = drag
+        h["STBY"][L] = drag
+    elif case.manning_n is not None:
+        n2 = case.manning_n[L] ** 2
+        h["STBX"][L] = G * n2 / np.cbrt(h["HU"][L])
+        h["STBY"][L] = G * n2 / np.cbrt(h["HV"][L])
+    avk = np.broadcast_to(np.asarray(av, float), (max(KC - 1, 1),))
+    for K in range(KC - 1):
+        k["AVUI"][K, L] = 1.0 / avk[K]
+        k["AVVI"][K, L] = 1.0 / avk[K]
+    for K in range(KC):
+        for lvl in ("", "1"):
+            k["UHDYF" + lvl][K, L] = h["UHDYE"][L]
+            k["VHDXF" + lvl][K, L] = h["VHDXE"][L]
+            k["UHDY" + lvl][K, L] = h["UHDYE"][L] * k["DZC"][K, L]
+            k["VHDX" + lvl][K, L] = h["VHDXE"][L] * k["DZC"][K, L]
+            k["U" + lvl][K, L] = h["UHDYE"][L] * h["HUI"][L] * h["DYIU"][L]
+            k["V" + lvl][K, L] = h["VHDXE"][L] * h["HVI"][L] * h["DXIV"][L]
+    case.meta["internal"] = True
+    return case

@@ -113,6 +113,11 @@ class TuneReport(C.Structure):
     _fields_ = [("n", C.c_int32), ("v", TuneEntry * 8)]
 
 
+class UvwOpts(C.Structure):
+    _fields_ = [("AVCON1", C.c_double), ("shear", C.c_int32), ("wind", C.c_int32),
+                ("uv_after", C.c_int32)]
+
+
 class AdjOpts(C.Structure):
     _fields_ = [("nadj", C.c_int32), ("slot", C.c_int32 * 16), ("cg_rel_tol", C.c_double),
                 ("cg_itmax", C.c_int32)]
@@ -175,6 +180,7 @@ def load(path: Optional[str] = None):
         "efdc_congrad": ([P, C.POINTER(CgStats)], C.c_int),
         "efdc_calpuv2c": ([P, C.POINTER(Step), C.POINTER(CgStats)], C.c_int),
         "efdc_calconc": ([P, C.POINTER(Step)], C.c_int),
+        "efdc_caluvw": ([P, C.POINTER(Step), C.POINTER(UvwOpts)], C.c_int),
         "efdc_balance": ([P, C.POINTER(Balance)], C.c_int),
         "efdc_synchronize": ([P], C.c_int),
         "efdc_device_ptr": ([P, C.c_int32, C.POINTER(P), C.POINTER(C.c_int64)], C.c_int),
@@ -594,13 +600,25 @@ class Context:
             self.set_manning(case.manning_n)
         return self
 
-    def step(self, dt: float, n: int) -> dict:
-        """One hot-path step in the order of EFDC/hdmt2t.f90:494-562 with
-        the depth-uniform stand-in for the internal mode."""
+    def caluvw(self, dt: float, n: int = 1, avcon1: float = 0.0, shear: int = 1,
+               wind: int = 0, uv_after: int = 1):
+        """CALUVW (EFDC/caluvw.f90) with the HDMT2T steps around it
+        (EFDC/hdmt2t.f90:513-527,638-655)."""
+        st = Step(dt, n, n)
+        op = UvwOpts(avcon1, shear, wind, uv_after)
+        self._check(self.lib.efdc_caluvw(self.h, C.byref(st), C.byref(op)))
+
+    def step(self, dt: float, n: int, internal: bool = False, uvw: Optional[dict] = None) -> dict:
+        """One hot-path step in the order of EFDC/hdmt2t.f90:494-562: the
+        external solve, then the internal mode (CALUVW) when `internal`,
+        else the depth-uniform stand-in for it, then the transport."""
         if self.has_manning:
             self.closure_pre()
         st = self.calpuv2c(dt, n)
-        self.closure_post()
+        if internal:
+            self.caluvw(dt, n, **(uvw or {}))
+        else:
+            self.closure_post()
         if self.nconst:
             self.calconc(dt, n)
         return st

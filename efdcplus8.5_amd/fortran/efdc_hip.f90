@@ -42,6 +42,16 @@ module efdc_hip
       EFDC_SVB3D = 76, EFDC_DZC = 77
   ! (LCM,0:KCM) arrays
   integer(c_int32_t), parameter :: EFDC_W2 = 96, EFDC_DZIG = 97
+  ! internal mode (CALUVW): (LCM), (LCM,KCM) and (LCM,0:KCM) arrays
+  integer(c_int32_t), parameter :: EFDC_STBX = 128, EFDC_STBY = 129, EFDC_VU = 130, &
+      EFDC_UV = 131, EFDC_V1U = 132, EFDC_U1V = 133, EFDC_DYU = 134, EFDC_DXV = 135
+  integer(c_int32_t), parameter :: EFDC_UHDYF = 160, EFDC_VHDXF = 161, EFDC_UHDY = 162, &
+      EFDC_VHDX = 163, EFDC_UHDYF1 = 164, EFDC_VHDXF1 = 165, EFDC_UHDY1 = 166, &
+      EFDC_VHDX1 = 167, EFDC_U = 168, EFDC_V = 169, EFDC_U1 = 170, EFDC_V1 = 171, &
+      EFDC_UHDYF2 = 172, EFDC_VHDXF2 = 173, EFDC_UHDYEK = 174, EFDC_VHDXEK = 175, &
+      EFDC_UHDY1EK = 176, EFDC_VHDX1EK = 177, EFDC_DU = 178, EFDC_DV = 179, &
+      EFDC_AVUI = 180, EFDC_AVVI = 181
+  integer(c_int32_t), parameter :: EFDC_W = 192, EFDC_W1 = 193
 
   type, bind(C) :: efdc_grid_t
     integer(c_int32_t) :: LA, LCM, KC, KCM
@@ -94,6 +104,11 @@ module efdc_hip
   type, bind(C) :: efdc_step_t
     real(c_double) :: dt
     integer(c_int64_t) :: n, niter
+  end type
+
+  type, bind(C) :: efdc_uvw_opts_t
+    real(c_double) :: AVCON1
+    integer(c_int32_t) :: shear, wind, uv_after
   end type
 
   type, bind(C) :: efdc_cg_stats_t
@@ -265,6 +280,13 @@ module efdc_hip
       import :: c_int, c_ptr, efdc_step_t
       type(c_ptr), value :: ctx
       type(efdc_step_t), intent(in) :: step
+    end function
+
+    integer(c_int) function efdc_caluvw(ctx, step, opts) bind(C, name="efdc_caluvw")
+      import :: c_int, c_ptr, efdc_step_t, efdc_uvw_opts_t
+      type(c_ptr), value :: ctx
+      type(efdc_step_t), intent(in) :: step
+      type(efdc_uvw_opts_t), intent(in) :: opts
     end function
 
     integer(c_int) function efdc_balance(ctx, out) bind(C, name="efdc_balance")

@@ -167,7 +167,28 @@ typedef enum {
   EFDC_NUM_K_END,
   /* (LCM,0:KCM) arrays (EFDC/varalloc.f90:199,839) */
   EFDC_W2 = 96, EFDC_DZIG,
-  EFDC_NUM_W_END
+  EFDC_NUM_W_END,
+  /* Internal mode (CALUVW, EFDC/caluvw.f90, with the steps of HDMT2T around
+     it, EFDC/hdmt2t.f90:513-527,638-655).  FP64 on the device in every
+     storage mode; allocated by the first efdc_set_array/efdc_caluvw that
+     needs them (zero until set).  (LCM) arrays (EFDC/varalloc.f90:196,
+     693-694,834-835,...): bottom-drag coefficients, V at U points and U at
+     V points at the two time levels, face widths DYU and DXV: */
+  EFDC_STBX = 128, EFDC_STBY, EFDC_VU, EFDC_UV, EFDC_V1U, EFDC_U1V,
+  EFDC_DYU, EFDC_DXV,
+  EFDC_NUM_IH_END,
+  /* (LCM,KCM) arrays (EFDC/varalloc.f90:76,170,769-790,2001-2003; KSM =
+     KCM): layer flows, velocities, their previous level and averages,
+     the external flow split per layer, the shear forcing DU/DV of
+     CALEXP2T and the inverse face viscosities AVUI/AVVI of CALAVB: */
+  EFDC_UHDYF = 160, EFDC_VHDXF, EFDC_UHDY, EFDC_VHDX, EFDC_UHDYF1, EFDC_VHDXF1,
+  EFDC_UHDY1, EFDC_VHDX1, EFDC_U, EFDC_V, EFDC_U1, EFDC_V1, EFDC_UHDYF2,
+  EFDC_VHDXF2, EFDC_UHDYEK, EFDC_VHDXEK, EFDC_UHDY1EK, EFDC_VHDX1EK,
+  EFDC_DU, EFDC_DV, EFDC_AVUI, EFDC_AVVI,
+  EFDC_NUM_IK_END,
+  /* (LCM,0:KCM) arrays (EFDC/varalloc.f90:837-838): vertical velocity */
+  EFDC_W = 192, EFDC_W1,
+  EFDC_NUM_IW_END
 } efdc_array_id;
 
 /* ------------------------------------------------------------ open BCs  */
@@ -379,6 +400,52 @@ int efdc_calconc(efdc_ctx_t *ctx, const efdc_step_t *step);
 
 /* FP64 volume and constituent-mass totals of the device state. */
 int efdc_balance(efdc_ctx_t *ctx, efdc_balance_t *out);
+
+/* ---------------------------------------------------------- internal mode
+   CALUVW(ISTL=2) on sigma layers (IGRIDV = 0, KSZ = 1), 2TL, no waves and
+   no subgrid channels (EFDC/caluvw.f90:176-1172), with the two steps HDMT2T
+   takes around it: the advance of the internal variables UHDYF1 = UHDYF,
+   UHDY1 = UHDY, U1 = U, W1 = W (and V's) before (EFDC/hdmt2t.f90:513-527)
+   and, when uv_after is set, the update of U1V, V1U, UV, VU after
+   (:638-655).  Per wet cell: the bottom-drag coefficients RCXX/RCYY
+   (caluvw.f90:285-330), the split of the external flow per layer
+   UHDYEK/VHDXEK (:332-345), the implicit shear solve (tridiagonal sweep
+   plus the Sherman-Morrison correction for the bottom drag, :415-524),
+   the layer flows UHDYF/VHDXF and their correction to the external
+   solution with CERRU/CERRV (:527-611), U and V (:613-647), W from the
+   layer continuity (:649-685) zeroed on open-boundary cells (:687-694),
+   the layer flows of the open-boundary faces (:700-767), the averages
+   UHDY2/VHDX2/UHDYF2/VHDXF2/U2/V2/W2 (:770-811), the depth from the layer
+   flows (:813-1040) with the negative-depth branches, HP of open-boundary
+   cells from P (:1130-1134) and HPK/HPKI (:1161-1172).  The face constants
+   SGZU, CDZFU/CDZUU/CDZLU/CDZRU/CDZDU/CDZMU, DZGU, CERRU (and V's) follow
+   from DZC and SUBO/SVBO as EFDC/aaefdc.f90:2001-2126 and
+   EFDC/caluvw.f90:82-113 set them.  The open-boundary faces use the
+   concentration-BC cell lists (LCBS/LCBW/LCBE/LCBN) with the radiation type
+   ISPB* of the pressure-BC cell of the same index (0 past the end of the
+   pressure side), as the reference does.  The next efdc_calconc reads the
+   averages and HPK/HPKI this leaves.  Requirements: DZC the same in every
+   wet cell of a layer; with ISDRY > 0 the transport mask of the last
+   efdc_calpuv2c decides the wet cells.  A non-open-boundary cell whose
+   depth from the layer flows is <= 0 while HP > 0 is EFDC_ERR_NEGDEPTH
+   (the reference logs it, :1010-1035), unless ISDRY > 0 and HP <= HDRY,
+   where the cell keeps HP and its flows are zeroed (:957-1009). */
+typedef struct {
+  double AVCON1;     /* 2*(1-AVCON)*DZI*AVO (EFDC/aaefdc.f90:645), the
+                        molecular part of RCXX/RCYY; below 1e-5 it is left
+                        out as the reference does (caluvw.f90:299-330)      */
+  int32_t shear;     /* 1: form DU/DV on the device as CALEXP2T does before
+                        CALPUV2C when its advective, Coriolis, baroclinic and
+                        horizontal-diffusion terms FCAX/FBBX/FX are zero
+                        (EFDC/calexp2t.f90:1332-1333): DU(L,K) = CDZFU(L,K)*
+                        (H1U(L)*(U1(L,K+1)-U1(L,K))/DT) from the velocities
+                        of the step start; 0: DU/DV as the host set them    */
+  int32_t wind;      /* with shear = 1: the wind-stress term of NWSER > 0,
+                        DU(L,KS) -= CDZUU(L,KS)*TSX(L) (:1337-1343)          */
+  int32_t uv_after;  /* 1: update U1V, V1U, UV, VU after the step
+                        (EFDC/hdmt2t.f90:638-655; needs LNWC and LSEC)     */
+} efdc_uvw_opts_t;
+int efdc_caluvw(efdc_ctx_t *ctx, const efdc_step_t *step, const efdc_uvw_opts_t *opts);
 
 /* ------------------------------------------------------------ ensembles */
 /* Declare that the wet cells form nseg independent basins: basin s holds

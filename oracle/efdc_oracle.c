@@ -46,7 +46,13 @@ static const double PI_EFDC = 3.1415926535898; /* EFDC/aaefdc.f90:650-651 */
      6  HV without the factor 1/2 (:906)
      7  bottom stress added instead of subtracted (:217)
      8  QSUME counted once in FP instead of twice (:320)
-     9  continuity with DELT instead of DELT/2 (:475)                     */
+     9  continuity with DELT instead of DELT/2 (:475)
+   and in CALUVW (EFDC/caluvw.f90):
+    20  layer flows not corrected to the external solution (:592-600)
+    21  Sherman-Morrison correction for the bottom drag left out (:491-520)
+    22  W without the external-flow split UHDYEK (:676-679)
+    23  W without the factor 1/2 of the two-level flow average (:676)
+    24  depth from the layer flows with the east sum added (:953)    */
 #ifndef ORC_PLANT
 #define ORC_PLANT 0
 #endif
@@ -368,6 +374,16 @@ static int explicit_stage(orc_ctx *o, double dt, int64_t n) {
       H1P[i] = HP[i];
       for (int K = 1; K <= KC; ++K) H1PK[KX(L, K)] = HPK[KX(L, K)];
     }
+    /* UHDY1EK = UHDYEK, VHDX1EK = VHDXEK (:270-272), kept only when the
+       internal-mode arrays are bound */
+    double *UHDYEK = A(EFDC_UHDYEK), *UHDY1EK = A(EFDC_UHDY1EK),
+           *VHDXEK = A(EFDC_VHDXEK), *VHDX1EK = A(EFDC_VHDX1EK);
+    if (UHDYEK && UHDY1EK && VHDXEK && VHDX1EK)
+      for (int K = 1; K <= KC; ++K)
+        for (int L = 2; L <= LA; ++L) {
+          UHDY1EK[KX(L, K)] = UHDYEK[KX(L, K)];
+          VHDX1EK[KX(L, K)] = VHDXEK[KX(L, K)];
+        }
   }
   /* FP1, :285-290 */
   {
@@ -1360,4 +1376,554 @@ int orc_closure_post(orc_ctx *o) {
       HPKI[j] = 1. / HPK[j];
     }
   return EFDC_OK;
+}
+
+/* ===================================================================== */
+/* CALUVW(ISTL = 2), sigma layers (IGRIDV = 0, KSZ = 1), with the steps of   */
+/* HDMT2T around it: EFDC/hdmt2t.f90:513-527 (advance the internal            */
+/* variables), EFDC/caluvw.f90:176-1172, EFDC/hdmt2t.f90:638-655 (U at V and  */
+/* V at U after the step, when uv_after is set).  No waves, no subgrid        */
+/* channels, no groundwater, 2TL with ISDYNSTP = 0 (DELT = DT).               */
+/* ===================================================================== */
+
+/* Face constants of the U faces (v = 0) or V faces (v = 1) of cell L, as
+   EFDC/aaefdc.f90:2001-2126 sets SGZU, DZGU, CDZFU, CDZUU, CDZLU, CDZRU,
+   CDZDU, CDZMU and EFDC/caluvw.f90:82-113 sets CERRU (each array KC long,
+   index K-1). */
+typedef struct {
+  double sg[64], dzg[64], cdzf[64], cdzu[64], cdzl[64], cdzr[64], cdzd[64],
+      cdzm[64], cerr[64];
+} face_consts;
+
+static void face_constants(const orc_ctx *o, int L, int v, face_consts *f) {
+  const int KC = o->KC, KS = KC - 1;
+  const size_t i = HX(L);
+  const double *DZC = A(EFDC_DZC);
+  const double open = v ? A(EFDC_SVBO)[i] : A(EFDC_SUBO)[i];
+  const int Lb = v ? o->LSC[i] : o->LWC[i];
+  memset(f, 0, sizeof(*f));
+  for (int K = 1; K <= KC; ++K)
+    if (open > 0.) f->sg[K - 1] = dmax(DZC[KX(Lb, K)], DZC[KX(L, K)]);
+  for (int K = 1; K <= KS; ++K) {
+    const double a = f->sg[K - 1], b = f->sg[K];
+    f->dzg[K - 1] = 0.5 * (a + b);
+    if (open > 0.) {
+      f->cdzf[K - 1] = a * b / (a + b);
+      f->cdzu[K - 1] = -a / (a + b);
+      f->cdzl[K - 1] = -b / (a + b);
+    }
+  }
+  if (open > 0.) {
+    f->cdzr[0] = f->sg[0] - 1.;
+    f->cdzd[0] = f->sg[0];
+    for (int K = 2; K <= KS; ++K) {
+      f->cdzr[K - 1] = f->cdzr[K - 2] + f->sg[K - 1];
+      f->cdzd[K - 1] = f->cdzd[K - 2] + f->sg[K - 1];
+    }
+  }
+  for (int K = 1; K <= KS; ++K)
+    if (open > 0.) {
+      f->cdzr[K - 1] = f->cdzr[K - 1] * f->dzg[K - 1] * f->cdzl[0];
+      f->cdzm[K - 1] = 0.5 * f->sg[K - 1] * f->sg[K];
+    }
+  if (KC > 1) {
+    double q1 = 0.;
+    for (int K = 1; K <= KC; ++K) q1 = q1 + 1. / (1. - f->sg[K - 1]);
+    if (q1 > 0.)
+      for (int K = 1; K <= KC; ++K) {
+        f->cerr[K - 1] = (1. / (1. - f->sg[K - 1])) / q1;
+        f->cerr[K - 1] = f->cerr[K - 1] * (double)KC;
+      }
+  } else {
+    f->cerr[KC - 1] = 1.;
+  }
+}
+
+/* is L one of the pressure open-boundary cells (LOBCS) */
+static int is_lobcs(const orc_ctx *o, int L) {
+  for (int s = 0; s < 4; ++s)
+    for (int q = 0; q < o->pbc[s].n; ++q)
+      if (o->pbc[s].L[q] == L) return 1;
+  return 0;
+}
+
+/* ISPB of index LL of a pressure side (0 past its end) */
+static int ispb_at(const orc_ctx *o, int side, int LL) {
+  return LL < o->pbc[side].n ? o->pbc[side].ispb[LL] : 0;
+}
+
+int orc_caluvw(orc_ctx *o, double dt, const efdc_uvw_opts_t *opt) {
+  const int LA = o->LA, LCM = o->LCM, KC = o->KC, KS = KC - 1;
+  const size_t LC = (size_t)LA + 1;
+  const double DELT = dt, DELTI = 1. / DELT;
+  double *UHDYF = A(EFDC_UHDYF), *VHDXF = A(EFDC_VHDXF), *UHDY = A(EFDC_UHDY),
+         *VHDX = A(EFDC_VHDX), *UHDYF1 = A(EFDC_UHDYF1), *VHDXF1 = A(EFDC_VHDXF1),
+         *UHDY1 = A(EFDC_UHDY1), *VHDX1 = A(EFDC_VHDX1), *U = A(EFDC_U), *V = A(EFDC_V),
+         *U1 = A(EFDC_U1), *V1 = A(EFDC_V1), *UHDYF2 = A(EFDC_UHDYF2),
+         *VHDXF2 = A(EFDC_VHDXF2), *UHDYEK = A(EFDC_UHDYEK), *VHDXEK = A(EFDC_VHDXEK),
+         *UHDY1EK = A(EFDC_UHDY1EK), *VHDX1EK = A(EFDC_VHDX1EK), *DU = A(EFDC_DU),
+         *DV = A(EFDC_DV), *AVUI = A(EFDC_AVUI), *AVVI = A(EFDC_AVVI), *W = A(EFDC_W),
+         *W1 = A(EFDC_W1), *STBX = A(EFDC_STBX), *STBY = A(EFDC_STBY), *VU = A(EFDC_VU),
+         *UV = A(EFDC_UV), *V1U = A(EFDC_V1U), *U1V = A(EFDC_U1V), *DYU = A(EFDC_DYU),
+         *DXV = A(EFDC_DXV);
+  double *UHDYE = A(EFDC_UHDYE), *VHDXE = A(EFDC_VHDXE), *UHDY1E = A(EFDC_UHDY1E),
+         *VHDX1E = A(EFDC_VHDX1E), *UHE = A(EFDC_UHE), *VHE = A(EFDC_VHE),
+         *HU = A(EFDC_HU), *HV = A(EFDC_HV), *HUI = A(EFDC_HUI), *HVI = A(EFDC_HVI),
+         *H1U = A(EFDC_H1U), *H1V = A(EFDC_H1V), *SUB = A(EFDC_SUB), *SVB = A(EFDC_SVB),
+         *DYIU = A(EFDC_DYIU), *DXIV = A(EFDC_DXIV), *DXYIP = A(EFDC_DXYIP),
+         *QSUME = A(EFDC_QSUME), *HP = A(EFDC_HP), *H1P = A(EFDC_H1P), *HPI = A(EFDC_HPI),
+         *P = A(EFDC_P), *BELV = A(EFDC_BELV), *TSX = A(EFDC_TSX), *TSY = A(EFDC_TSY);
+  double *QSUM = A(EFDC_QSUM), *DZC = A(EFDC_DZC), *SUB3D = A(EFDC_SUB3D),
+         *SVB3D = A(EFDC_SVB3D), *UHDY2 = A(EFDC_UHDY2), *VHDX2 = A(EFDC_VHDX2),
+         *U2 = A(EFDC_U2), *V2 = A(EFDC_V2), *HPK = A(EFDC_HPK), *HPKI = A(EFDC_HPKI),
+         *W2 = A(EFDC_W2);
+  const int32_t *LWC = o->LWC, *LEC = o->LEC, *LSC = o->LSC, *LNC = o->LNC;
+  int status = EFDC_OK;
+  if (KC > 64) return EFDC_ERR_ARG;
+  /* wet cells: LKWET (all L = 2..LA with ISDRY = 0; LMASKDRY otherwise,
+     EFDC/calpuv2c.f90:1061-1079) */
+#define WETC(L) (o->ISDRY == 0 || o->lmaskdry[HX(L)])
+  /* SUB3D after the drying of CALPUV2C: SUB*SUB3DO (:982-990) */
+#define S3U(L, K) (o->ISDRY > 0 ? SUB[HX(L)] * SUB3D[KX(L, K)] : SUB3D[KX(L, K)])
+#define S3V(L, K) (o->ISDRY > 0 ? SVB[HX(L)] * SVB3D[KX(L, K)] : SVB3D[KX(L, K)])
+
+  /* the internal variables advance, EFDC/hdmt2t.f90:513-527 */
+  for (int K = 1; K <= KC; ++K)
+    for (int L = 2; L <= LA; ++L) {
+      if (!WETC(L)) continue;
+      const size_t j = KX(L, K);
+      UHDYF1[j] = UHDYF[j];
+      VHDXF1[j] = VHDXF[j];
+      UHDY1[j] = UHDY[j];
+      VHDX1[j] = VHDX[j];
+      U1[j] = U[j];
+      V1[j] = V[j];
+      W1[WX(L, K)] = W[WX(L, K)];
+    }
+  /* CALEXP2T's internal shears with FCAX = FBBX = FX = 0, from the levels
+     of the step start (EFDC/calexp2t.f90:1332-1343) */
+  if (opt->shear && KC > 1) {
+    for (int L = 2; L <= LA; ++L) {
+      if (!WETC(L)) continue;
+      face_consts fu, fv;
+      face_constants(o, L, 0, &fu);
+      face_constants(o, L, 1, &fv);
+      const size_t i = HX(L);
+      for (int K = 1; K <= KS; ++K) {
+        DU[KX(L, K)] = fu.cdzf[K - 1] * (H1U[i] * (U1[KX(L, K + 1)] - U1[KX(L, K)]) * DELTI +
+                                         0.0);
+        DV[KX(L, K)] = fv.cdzf[K - 1] * (H1V[i] * (V1[KX(L, K + 1)] - V1[KX(L, K)]) * DELTI +
+                                         0.0);
+      }
+      if (opt->wind) {
+        DU[KX(L, KS)] = DU[KX(L, KS)] - fu.cdzu[KS - 1] * TSX[i];
+        DV[KX(L, KS)] = DV[KX(L, KS)] - fv.cdzu[KS - 1] * TSY[i];
+      }
+    }
+  }
+  /* KC = 1: HDMT2T sets the single layer from the external flow
+     (EFDC/hdmt2t.f90:537-553) */
+  if (KC == 1)
+    for (int L = 2; L <= LA; ++L) {
+      const size_t i = HX(L), j = KX(L, 1);
+      UHDYF[j] = UHDYE[i];
+      UHDY[j] = UHDYE[i];
+      U[j] = UHDYE[i] * HUI[i] * DYIU[i];
+      VHDXF[j] = VHDXE[i];
+      VHDX[j] = VHDXE[i];
+      V[j] = VHDXE[i] * HVI[i] * DXIV[i];
+      W[WX(L, 1)] = 0.;
+    }
+  /* zero the dry cells (:186-263).  The reference zeroes the newly dry
+     ones; a cell that stays dry keeps the zeros (no loop writes it), so
+     zeroing every dry cell gives the same state. */
+  if (o->ISDRY > 0)
+    for (int L = 2; L <= LA; ++L) {
+      if (WETC(L)) continue;
+      const size_t i = HX(L), e = HX(LEC[i]), n = HX(LNC[i]);
+      for (int K = 1; K <= KC; ++K) {
+        const size_t j = KX(L, K), je = e + (size_t)(K - 1) * LCM,
+                     jn = n + (size_t)(K - 1) * LCM;
+        UHDY[j] = VHDX[j] = UHDYF[j] = VHDXF[j] = U[j] = V[j] = 0.;
+        W[WX(L, K)] = 0.;
+        UHDY1[j] = VHDX1[j] = UHDYF1[j] = VHDXF1[j] = U1[j] = V1[j] = 0.;
+        W1[WX(L, K)] = 0.;
+        UHDY2[j] = VHDX2[j] = UHDYF2[j] = VHDXF2[j] = U2[j] = V2[j] = 0.;
+        W2[WX(L, K)] = 0.;
+        UHDY[je] = 0.;
+        VHDX[jn] = 0.;
+        UHDYF[je] = 0.;
+        VHDXF[jn] = 0.;
+        V[jn] = 0.;
+        U[je] = 0.;
+        UHDY1[je] = 0.;
+        VHDX1[jn] = 0.;
+        UHDYF1[je] = 0.;
+        VHDXF1[jn] = 0.;
+        U1[je] = 0.;
+        V1[jn] = 0.;
+        UHDY2[je] = 0.;
+        VHDX2[jn] = 0.;
+        UHDYF2[je] = 0.;
+        VHDXF2[jn] = 0.;
+        U2[je] = 0.;
+        V2[jn] = 0.;
+      }
+    }
+  double *TVARE = calloc(LCM, sizeof(double)), *TVARN = calloc(LCM, sizeof(double));
+  if (KC > 1) {
+    double *RCXX = calloc(LCM, sizeof(double)), *RCYY = calloc(LCM, sizeof(double));
+    double *CU1 = calloc((size_t)LCM * KC, sizeof(double)),
+           *CU2 = calloc((size_t)LCM * KC, sizeof(double)),
+           *UUU = calloc((size_t)LCM * KC, sizeof(double)),
+           *VVV = calloc((size_t)LCM * KC, sizeof(double));
+    double *AAU = calloc(LCM, sizeof(double)), *AAV = calloc(LCM, sizeof(double)),
+           *BBU = calloc(LCM, sizeof(double)), *BBV = calloc(LCM, sizeof(double));
+    const double AVCON1 = opt->AVCON1;
+    /* bottom drag coefficients, ISTL = 2 (:296-330) */
+    for (int L = 2; L <= LA; ++L) {
+      if (!WETC(L)) continue;
+      const size_t i = HX(L), j1 = KX(L, 1);
+      double Q1 = sqrt(U1[j1] * U1[j1] + V1U[i] * V1U[i]);
+      double Q2 = sqrt(U[j1] * U[j1] + VU[i] * VU[i]);
+      double qq = Q1 * Q2;
+      if (AVCON1 < 0.00001)
+        RCXX[i] = STBX[i] * sqrt(qq);
+      else
+        RCXX[i] = AVCON1 / sqrt(H1U[i] * HU[i]) + STBX[i] * sqrt(qq);
+      Q1 = sqrt(V1[j1] * V1[j1] + U1V[i] * U1V[i]);
+      Q2 = sqrt(V[j1] * V[j1] + UV[i] * UV[i]);
+      qq = Q1 * Q2;
+      if (AVCON1 < 0.00001)
+        RCYY[i] = STBY[i] * sqrt(qq);
+      else
+        RCYY[i] = AVCON1 / sqrt(H1V[i] * HV[i]) + STBY[i] * sqrt(qq);
+    }
+    /* the external flows split by layer (:332-345) */
+    for (int K = 1; K <= KS; ++K)
+      for (int L = 2; L <= LA; ++L) {
+        if (!WETC(L)) continue;
+        const size_t i = HX(L), j = KX(L, K);
+        UHDYEK[j] = DZC[j] * (UHDYE[i] - UHDYE[HX(LEC[i])]);
+        VHDXEK[j] = DZC[j] * (VHDXE[i] - VHDXE[HX(LNC[i])]);
+      }
+    /* the U and V shears (:415-524), then the layer flows (:527-567),
+       their correction to the external solution (:571-611) and the
+       velocities (:613-647), one wet cell at a time */
+    for (int L = 2; L <= LA; ++L) {
+      if (!WETC(L)) continue;
+      const size_t i = HX(L);
+      face_consts fu, fv;
+      face_constants(o, L, 0, &fu);
+      face_constants(o, L, 1, &fv);
+      for (int K = 1; K <= KS; ++K) {
+        const size_t j = KX(L, K), jm = KX(L, K - 1 > 0 ? K - 1 : 1);
+        const double cl_u = fu.cdzl[K - 1], cu_u = fu.cdzu[K - 1];
+        DU[j] = S3U(L, K) * DU[j];
+        if (K > 1) {
+          const double RCDZL = cl_u, RCDZM = fu.cdzm[K - 1] * DELTI, RCDZU = cu_u;
+          const double CMU = 1. + RCDZM * HU[i] * AVUI[j];
+          const double EU = 1. / (CMU - RCDZL * CU1[jm]);
+          CU1[j] = RCDZU * EU;
+          DU[j] = (DU[j] - RCDZL * DU[jm]) * EU;
+          UUU[j] = -RCDZL * UUU[jm] * EU;
+        } else {
+          const double RCDZL = cl_u, RCDZM = fu.cdzm[K - 1] * DELTI, RCDZU = cu_u;
+          const double CMU = 1. + RCDZM * HU[i] * AVUI[j];
+          const double EU = 1. / CMU;
+          CU1[j] = RCDZU * EU;
+          DU[j] = (DU[j] - RCDZL * RCXX[i] * UHE[i] * HUI[i]) * EU;
+          UUU[j] = EU;
+        }
+        DV[j] = S3V(L, K) * DV[j];
+        if (K > 1) {
+          const double RCDZL = fv.cdzl[K - 1], RCDZM = fv.cdzm[K - 1] * DELTI,
+                       RCDZU = fv.cdzu[K - 1];
+          const double CMV = 1. + RCDZM * HV[i] * AVVI[j];
+          const double EV = 1. / (CMV - RCDZL * CU2[jm]);
+          CU2[j] = RCDZU * EV;
+          DV[j] = (DV[j] - RCDZL * DV[jm]) * EV;
+          VVV[j] = -RCDZL * VVV[jm] * EV;
+        } else {
+          const double RCDZL = fv.cdzl[K - 1], RCDZM = fv.cdzm[K - 1] * DELTI,
+                       RCDZU = fv.cdzu[K - 1];
+          const double CMV = 1. + RCDZM * HV[i] * AVVI[j];
+          const double EV = 1. / CMV;
+          CU2[j] = RCDZU * EV;
+          DV[j] = (DV[j] - RCDZL * RCYY[i] * VHE[i] * HVI[i]) * EV;
+          VVV[j] = EV;
+        }
+      }
+      for (int K = KS - 1; K >= 1; --K) {
+        const size_t j = KX(L, K), jp = KX(L, K + 1);
+        DU[j] = DU[j] - CU1[j] * DU[jp];
+        DV[j] = DV[j] - CU2[j] * DV[jp];
+        UUU[j] = UUU[j] - CU1[j] * UUU[jp];
+        VVV[j] = VVV[j] - CU2[j] * VVV[jp];
+      }
+      AAU[i] = 0.;
+      AAV[i] = 0.;
+      BBU[i] = 1.;
+      BBV[i] = 1.;
+      for (int K = 1; K <= KS; ++K) {
+        const size_t j = KX(L, K);
+        const double CRU = fu.cdzr[K - 1] * RCXX[i] * AVUI[j];
+        AAU[i] = AAU[i] + CRU * DU[j];
+        BBU[i] = BBU[i] + CRU * UUU[j];
+        const double CRV = fv.cdzr[K - 1] * RCYY[i] * AVVI[j];
+        AAV[i] = AAV[i] + CRV * DV[j];
+        BBV[i] = BBV[i] + CRV * VVV[j];
+      }
+      AAU[i] = PLANT(21, 0., AAU[i] / BBU[i]);
+      AAV[i] = PLANT(21, 0., AAV[i] / BBV[i]);
+      for (int K = 1; K <= KS; ++K) {
+        const size_t j = KX(L, K);
+        DU[j] = S3U(L, K) * fu.dzg[K - 1] * HU[i] * AVUI[j] * (DU[j] - AAU[i] * UUU[j]);
+        DV[j] = S3V(L, K) * fv.dzg[K - 1] * HV[i] * AVVI[j] * (DV[j] - AAV[i] * VVV[j]);
+      }
+      /* layer flows (:527-567) */
+      for (int K = 1; K <= KS; ++K) {
+        UHE[i] = UHE[i] + fu.cdzd[K - 1] * DU[KX(L, K)];
+        VHE[i] = VHE[i] + fv.cdzd[K - 1] * DV[KX(L, K)];
+      }
+      UHDYF[KX(L, KC)] = UHE[i] * SUB[i];
+      VHDXF[KX(L, KC)] = VHE[i] * SVB[i];
+      for (int K = KS; K >= 1; --K) {
+        UHDYF[KX(L, K)] = S3U(L, K) * (UHDYF[KX(L, K + 1)] - DU[KX(L, K)]);
+        VHDXF[KX(L, K)] = S3V(L, K) * (VHDXF[KX(L, K + 1)] - DV[KX(L, K)]);
+      }
+      for (int K = 1; K <= KC; ++K) {
+        UHDYF[KX(L, K)] = UHDYF[KX(L, K)] * DYU[i];
+        VHDXF[KX(L, K)] = VHDXF[KX(L, K)] * DXV[i];
+      }
+      /* correction to the external solution (:571-611) */
+      TVARE[i] = 0.;
+      TVARN[i] = 0.;
+      for (int K = 1; K <= KC; ++K) {
+        TVARE[i] = TVARE[i] + UHDYF[KX(L, K)] * fu.sg[K - 1];
+        TVARN[i] = TVARN[i] + VHDXF[KX(L, K)] * fv.sg[K - 1];
+      }
+      TVARE[i] = TVARE[i] - UHDYE[i];
+      TVARN[i] = TVARN[i] - VHDXE[i];
+      for (int K = 1; K <= KC; ++K) {
+        const size_t j = KX(L, K);
+        UHDYF[j] = PLANT(20, UHDYF[j], UHDYF[j] - TVARE[i] * fu.cerr[K - 1]);
+        UHDY[j] = UHDYF[j] * fu.sg[K - 1];
+        VHDXF[j] = PLANT(20, VHDXF[j], VHDXF[j] - TVARN[i] * fv.cerr[K - 1]);
+        VHDX[j] = VHDXF[j] * fv.sg[K - 1];
+      }
+      /* velocities (:613-647) */
+      UHE[i] = 0.;
+      VHE[i] = 0.;
+      for (int K = 1; K <= KC; ++K) {
+        const size_t j = KX(L, K);
+        UHE[i] = UHE[i] + UHDYF[j] * fu.sg[K - 1];
+        VHE[i] = VHE[i] + VHDXF[j] * fv.sg[K - 1];
+        U[j] = UHDYF[j] * HUI[i];
+        V[j] = VHDXF[j] * HVI[i];
+      }
+      for (int K = 1; K <= KC; ++K) {
+        U[KX(L, K)] = U[KX(L, K)] * DYIU[i];
+        V[KX(L, K)] = V[KX(L, K)] * DXIV[i];
+      }
+      UHE[i] = UHE[i] * DYIU[i];
+      VHE[i] = VHE[i] * DXIV[i];
+    }
+    /* W from the layer continuity, ISTL = 2 (:665-683) */
+    for (int K = 1; K <= KS; ++K)
+      for (int L = 2; L <= LA; ++L) {
+        if (!WETC(L)) continue;
+        const size_t i = HX(L), j = KX(L, K), jw = WX(L, K), jwm = WX(L, K - 1);
+        const size_t je = HX(LEC[i]) + (size_t)(K - 1) * LCM,
+                     jn = HX(LNC[i]) + (size_t)(K - 1) * LCM;
+        const double ek = PLANT(22, 0., UHDYEK[j]), e1k = UHDY1EK[j];
+        const double vk = PLANT(22, 0., VHDXEK[j]), v1k = VHDX1EK[j];
+        W[jw] = W[jwm] -
+                PLANT(23, DXYIP[i], 0.5 * DXYIP[i]) *
+                    (UHDY[je] - UHDY[j] + ek + UHDY1[je] - UHDY1[j] + e1k + VHDX[jn] -
+                     VHDX[j] + vk + VHDX1[jn] - VHDX1[j] + v1k) +
+                (QSUM[j] - DZC[j] * QSUME[i]) * DXYIP[i];
+      }
+    /* W on the open-boundary cells (:687-694) */
+    for (int s = 0; s < 4; ++s)
+      for (int q = 0; q < o->pbc[s].n; ++q)
+        for (int K = 1; K <= KS; ++K) W[WX(o->pbc[s].L[q], K)] = 0.;
+    free(RCXX);
+    free(RCYY);
+    free(CU1);
+    free(CU2);
+    free(UUU);
+    free(VVV);
+    free(AAU);
+    free(AAV);
+    free(BBU);
+    free(BBV);
+  }
+  /* label 30: U and V on the open boundaries (:700-767), sides S, W, E, N
+     with the concentration-BC cell lists */
+  {
+    face_consts f;
+    for (int K = 1; K <= KC; ++K) {
+      const orc_cbc_side *cs = &o->cbc[0];   /* south */
+      for (int LL = 0; LL < cs->n; ++LL) {
+        if (ispb_at(o, 2, LL) == 2) continue;
+        const int L = cs->L[LL], LN = LNC[HX(L)], LNN = LNC[HX(LN)];
+        if ((size_t)LN != LC) {
+          face_constants(o, LN, 1, &f);
+          const size_t jn = KX(LN, K);
+          VHDXF[jn] = VHDXF[KX(LNN, K)] - VHDXE[HX(LNN)] + VHDXE[HX(LN)];
+          VHDX[jn] = VHDXF[jn] * f.sg[K - 1];
+          V[jn] = VHDXF[jn] / (HV[HX(LN)] * DXV[HX(LN)]);
+        }
+        W[WX(LN, K)] = 0.;
+      }
+    }
+    for (int K = 1; K <= KC; ++K) {
+      const orc_cbc_side *cs = &o->cbc[1];   /* west */
+      for (int LL = 0; LL < cs->n; ++LL) {
+        if (ispb_at(o, 0, LL) == 2) continue;
+        const int L = cs->L[LL], LE = LEC[HX(L)], L1P = LEC[HX(LE)];
+        if ((size_t)LE != LC) {
+          face_constants(o, LE, 0, &f);
+          const size_t je = KX(LE, K);
+          UHDYF[je] = UHDYF[KX(L1P, K)] - UHDYE[HX(L1P)] + UHDYE[HX(LE)];
+          UHDY[je] = UHDYF[je] * f.sg[K - 1];
+          U[je] = UHDYF[je] / (HU[HX(LE)] * DYU[HX(LE)]);
+        }
+        W[WX(LE, K)] = 0.;
+      }
+    }
+    for (int K = 1; K <= KC; ++K) {
+      const orc_cbc_side *cs = &o->cbc[2];   /* east */
+      for (int LL = 0; LL < cs->n; ++LL) {
+        const int L = cs->L[LL], LW = LWC[HX(L)];
+        if (ispb_at(o, 1, LL) != 2) {
+          face_constants(o, L, 0, &f);
+          const size_t j = KX(L, K);
+          UHDYF[j] = UHDYF[KX(LW, K)] - UHDYE[HX(LW)] + UHDYE[HX(L)];
+          UHDY[j] = UHDYF[j] * f.sg[K - 1];
+          U[j] = UHDYF[j] / (HU[HX(L)] * DYU[HX(L)]);
+        }
+        W[WX(L, K)] = 0.;
+      }
+    }
+    for (int K = 1; K <= KC; ++K) {
+      const orc_cbc_side *cs = &o->cbc[3];   /* north */
+      for (int LL = 0; LL < cs->n; ++LL) {
+        const int L = cs->L[LL];
+        if (ispb_at(o, 3, LL) != 2) {
+          const int LS = LSC[HX(L)];
+          face_constants(o, L, 1, &f);
+          const size_t j = KX(L, K);
+          VHDXF[j] = VHDXF[KX(LS, K)] - VHDXE[HX(LS)] + VHDXE[HX(L)];
+          VHDX[j] = VHDXF[j] * f.sg[K - 1];
+          V[j] = VHDXF[j] / (HV[HX(L)] * DXV[HX(L)]);
+        }
+        W[WX(L, K)] = 0.;
+      }
+    }
+  }
+  /* averages of the two levels, ISTL = 2 (:770-791) */
+  for (int K = 1; K <= KC; ++K)
+    for (int L = 2; L <= LA; ++L) {
+      if (!WETC(L)) continue;
+      const size_t j = KX(L, K);
+      UHDYF2[j] = 0.5 * (UHDYF[j] + UHDYF1[j]);
+      VHDXF2[j] = 0.5 * (VHDXF[j] + VHDXF1[j]);
+      UHDY2[j] = 0.5 * (UHDY[j] + UHDY1[j]);
+      VHDX2[j] = 0.5 * (VHDX[j] + VHDX1[j]);
+      U2[j] = 0.5 * (U[j] + U1[j]);
+      V2[j] = 0.5 * (V[j] + V1[j]);
+      W2[WX(L, K)] = 0.5 * (W[WX(L, K)] + W1[WX(L, K)]);
+    }
+  /* the depth from the layer flows, 2TL (:813-1040) */
+  if (KC > 1) {
+    for (int L = 2; L <= LA; ++L) {
+      if (!WETC(L)) continue;
+      const size_t i = HX(L);
+      TVARE[i] = 0.;
+      TVARN[i] = 0.;
+      for (int K = 1; K <= KC; ++K) {
+        TVARE[i] = TVARE[i] + UHDY2[KX(L, K)];
+        TVARN[i] = TVARN[i] + VHDX2[KX(L, K)];
+      }
+    }
+    for (int L = 2; L <= LA; ++L) {
+      if (!WETC(L)) continue;
+      const size_t i = HX(L), e = HX(LEC[i]), n = HX(LNC[i]);
+      double HPPTMP = H1P[i] + DELT * DXYIP[i] *
+                                   (QSUME[i] + TVARE[i] + PLANT(24, TVARE[e], -TVARE[e]) +
+                                    TVARN[i] - TVARN[n]);
+      if (HPPTMP <= 0. && HP[i] > 0.) {
+        if (is_lobcs(o, L)) continue;
+        if (HP[i] <= o->HDRY && o->ISDRY > 0) {
+          HPPTMP = HP[i];
+          UHDYE[i] = 0.;
+          VHDXE[i] = 0.;
+          for (int K = 1; K <= KC; ++K) {
+            const size_t j = KX(L, K), je = e + (size_t)(K - 1) * LCM,
+                         jn = n + (size_t)(K - 1) * LCM;
+            UHDYF[j] = VHDXF[j] = UHDY[j] = VHDX[j] = U[j] = V[j] = 0.;
+            W[WX(L, K)] = 0.;
+            UHDYF[je] = 0.;
+            VHDXF[jn] = 0.;
+            UHDY[je] = 0.;
+            VHDX[jn] = 0.;
+            U[je] = 0.;
+            V[jn] = 0.;
+            UHDY2[j] = VHDX2[j] = U2[j] = V2[j] = 0.;
+            W2[WX(L, K)] = 0.;
+            UHDY2[je] = 0.;
+            VHDX2[jn] = 0.;
+            U2[je] = 0.;
+            V2[jn] = 0.;
+          }
+        } else {
+          status = EFDC_ERR_NEGDEPTH;
+        }
+      }
+      HP[i] = HPPTMP;
+      HPI[i] = 1. / HP[i];
+    }
+  }
+  /* open-boundary cells take HP from P (:1130-1134) */
+  for (int s = 0; s < 4; ++s)
+    for (int q = 0; q < o->pbc[s].n; ++q) {
+      const size_t i = HX(o->pbc[s].L[q]);
+      HP[i] = (1. / o->G) * P[i] - BELV[i];
+      HPI[i] = 1. / HP[i];
+    }
+  /* layer thicknesses (:1161-1172) */
+  for (int K = 1; K <= KC; ++K)
+    for (int L = 2; L <= LA; ++L) {
+      if (!WETC(L)) continue;
+      const size_t j = KX(L, K);
+      HPK[j] = HP[HX(L)] * DZC[j];
+      HPKI[j] = 1. / HPK[j];
+    }
+  /* U at V and V at U at the new level, EFDC/hdmt2t.f90:638-655 */
+  if (opt->uv_after) {
+    for (int L = 2; L <= LA; ++L) {
+      if (!WETC(L)) continue;
+      const size_t i = HX(L);
+      const int LSE = o->LSEC[i], LS = LSC[i], LE = LEC[i], LNW = o->LNWC[i], LW = LWC[i],
+                LN = LNC[i];
+      U1V[i] = UV[i];
+      V1U[i] = VU[i];
+      UV[i] = ((HP[HX(LSE)] * U[KX(LSE, 1)] + HP[HX(LS)] * U[KX(LS, 1)]) +
+               (HP[HX(LE)] * U[KX(LE, 1)] + HP[i] * U[KX(L, 1)])) /
+              (1.0 + SUB[HX(LE)] + SUB[HX(LS)] + SUB[HX(LSE)]) * HVI[i];
+      VU[i] = ((HP[HX(LNW)] * V[KX(LNW, 1)] + HP[HX(LW)] * V[KX(LW, 1)]) +
+               (HP[HX(LN)] * V[KX(LN, 1)] + HP[i] * V[KX(L, 1)])) /
+              (1.0 + SVB[HX(LN)] + SVB[HX(LW)] + SVB[HX(LNW)]) * HUI[i];
+    }
+  }
+  free(TVARE);
+  free(TVARN);
+#undef WETC
+#undef S3U
+#undef S3V
+  (void)UHDY1E;
+  (void)VHDX1E;
+  return status;
 }

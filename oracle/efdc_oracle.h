@@ -12,7 +12,7 @@
 #include <stdint.h>
 #include "../include/efdc_hip.h"
 
-#define ORC_NARR 128
+#define ORC_NARR 256
 #define ORC_MAXC 16
 
 typedef struct {
@@ -102,6 +102,10 @@ int orc_balance(orc_ctx *o, double *volume, double *mass);
    alone: the units compared with the reference build in oracle/_ref */
 int orc_setopenbc_apply(orc_ctx *o, double dt, int64_t n);
 int orc_caltran_only(orc_ctx *o, int c, double dt, int64_t niter);
+/* CALUVW(ISTL=2) on sigma layers with the HDMT2T steps around it
+   (EFDC/hdmt2t.f90:513-527,638-655, EFDC/caluvw.f90:176-1172); the
+   internal-mode arrays must be bound */
+int orc_caluvw(orc_ctx *o, double dt, const efdc_uvw_opts_t *opt);
 /* depth-uniform closure standing in for the internal mode */
 int orc_closure_pre(orc_ctx *o, const double *manning_n);
 int orc_closure_post(orc_ctx *o);

@@ -19,7 +19,7 @@ sys.path.insert(0, os.path.join(ROOT, "efdcplus8.5_amd"))
 from efdc_amd.abi import ARRAY_IDS  # noqa: E402
 
 LIB = os.path.join(HERE, "liboracle.so")
-NARR = 128
+NARR = 256
 MAXC = 16
 
 PD = C.POINTER(C.c_double)
@@ -64,6 +64,11 @@ class OrcCtx(C.Structure):
                 ("ISDRY", C.c_int32), ("iscdry", PI), ("lmaskdry", PI),
                 ("sub1", PD), ("svb1", PD), ("ncordry", C.c_int32),
                 ("NDRYSTP", C.c_int32), ("natdry", PI), ("qdwaste", PD), ("vdwaste", PD)]
+
+
+class UvwOpts(C.Structure):
+    _fields_ = [("AVCON1", C.c_double), ("shear", C.c_int32), ("wind", C.c_int32),
+                ("uv_after", C.c_int32)]
 
 
 class CgOut(C.Structure):
@@ -111,6 +116,7 @@ def lib(path: str = None):
                 orc_balance=[C.POINTER(OrcCtx), PD, PD],
                 orc_closure_pre=[C.POINTER(OrcCtx), PD],
                 orc_closure_post=[C.POINTER(OrcCtx)],
+                orc_caluvw=[C.POINTER(OrcCtx), C.c_double, C.POINTER(UvwOpts)],
                 orc_setopenbc_apply=[C.POINTER(OrcCtx), C.c_double, C.c_int64],
                 orc_caltran_only=[C.POINTER(OrcCtx), C.c_int, C.c_double,
                                   C.c_int64]).items():
@@ -280,14 +286,24 @@ class Oracle:
     def closure_post(self):
         self.L.orc_closure_post(C.byref(self.o))
 
-    def step(self, n):
+    def caluvw(self, dt=None, avcon1=0.0, shear=1, wind=0, uv_after=1):
+        """CALUVW with the HDMT2T steps around it (EFDC/hdmt2t.f90:513-527,
+        638-655); returns the status (4 = negative depth)."""
+        op = UvwOpts(avcon1, shear, wind, uv_after)
+        return self.L.orc_caluvw(C.byref(self.o), dt or self.case.dt, C.byref(op))
+
+    def step(self, n, uvw=None):
         """One 2TL hot-path step of the harness sequence (HDMT2T order,
-        EFDC/hdmt2t.f90:494-562): closure -> CALPUV2C -> closure ->
-        CALCONC."""
+        EFDC/hdmt2t.f90:494-562): closure -> CALPUV2C -> the internal mode
+        (CALUVW, when the case carries its arrays; else the depth-uniform
+        closure) -> CALCONC."""
         if self.case.manning_n is not None:
             self.closure_pre()
         st = self.calpuv2c(n=n)
-        self.closure_post()
+        if self.case.meta.get("internal"):
+            st["uvw"] = self.caluvw(**(uvw or {}))
+        else:
+            self.closure_post()
         if self.case.consts:
             self.calconc(niter=n)
         return st
