@@ -273,6 +273,150 @@ def transport_bench(args):
     ctx.close()
 
 
+def uvw_bytes(kc, s):
+    """Algorithmic bytes per water column of the internal-mode kernels
+    (every array element a kernel must read or write counted once, FP64
+    internal arrays, s bytes per stored transport value, the internal
+    shears formed on the device): k_uvw_col reads U1/V1 (KC planes),
+    AVUI/AVVI (KS), SUB3D/SVB3D and DZC (KS, stored) and 26 (LCM) values,
+    writes UHDYF/VHDXF/UHDY/VHDX/U/V (KC) and UHDYEK/VHDXEK (KS); k_uvw_w
+    reads the current and old layer flows, transports and velocities (12 KC
+    planes), the four split planes (KS), QSUM and DZC (KS, stored), W1 (KC),
+    writes W (KS), UHDYF2/VHDXF2 (KC), the five stored averages (KC) and
+    the column sums; k_uvw_hp reads DZC, writes HPK/HPKI (KC, stored) plus
+    11 (LCM) values; k_uvw_uv 13 (LCM) values."""
+    ks = kc - 1
+    return {"uvw_columns": 64 * kc + 32 * ks + 3 * s * ks + 224,
+            "uvw_w_averages": 120 * kc + 40 * ks + 2 * s * ks + 5 * s * kc + 41,
+            "uvw_depths": 3 * s * kc + 88,
+            "uvw_uv": 104}
+
+
+def hydro3d_bench(args):
+    """The full 3-D hot-path step at scale (--workload hydro3d): a tidal
+    basin of 2048x1024 columns (all wet, west open boundary with an M2
+    tide, depths 5-20 m, Manning drag), KC=20 sigma layers, salinity and a
+    dye source, FP32 storage.  One step = the external solve (CALPUV2C with
+    the FP32-refined CG to relative residual 1e-10), the internal mode
+    (CALUVW with the internal shears formed on the device, then U/V
+    interpolation) and CALCONC; value = column-steps/s.  The FP64 oracle
+    runs the same sequence on a 256x128 tile of the same generator for the
+    error and the CPU baseline."""
+    import numpy as np
+    from efdc_amd import grids, lib
+
+    nx, ny, kc = args.tnx, args.tny, args.tkc
+
+    def make(nx_, ny_):
+        c = grids.tidal_basin(nx_, ny_, kc, seed=1002, dt=5.0)
+        return grids.add_internal_mode(c, av=1e-3)
+
+    def context(case):
+        ctx = lib.Context(case.grid, store=lib.STORE_FP32, cg_mode=lib.CG_MIXED,
+                          cg_rel_tol=1e-10, RSQM=0.0, ITERM=1000)
+        return ctx.upload_case(case)
+
+    # accuracy and CPU baseline on the bounded sample
+    cpu = errs = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        small = make(256, 128)
+        ref = small.copy()
+        o = orc.Oracle(ref, rsqm=0.0, iterm=3000, rel_tol=1e-12, nthreads=args.cpu_threads)
+        sctx = context(small)
+        t_cpu, done = 0.0, 0
+        while done < 3 or (t_cpu < args.cpu_seconds and done < 400):
+            t0 = time.perf_counter()
+            o.step(done + 1)
+            t_cpu += time.perf_counter() - t0
+            done += 1
+            sctx.step(small.dt, done, internal=True)
+        L = np.arange(2, small.grid.LA + 1) - 1
+        eta = float(np.max(np.abs(sctx.get("P")[L] - ref.h["P"][L]))) / 9.81
+        errs = dict(eta_m=eta, after_steps=done)
+        for name in ("UHDY2", "VHDX2"):
+            b = ref.k[name][:kc, L]
+            errs[name] = float(np.max(np.abs(sctx.get(name)[:kc, L] - b)) /
+                               max(1e-30, float(np.max(np.abs(b)))))
+        errs["W2"] = float(np.max(np.abs(sctx.get("W2")[:kc + 1, L] - ref.w["W2"][:kc + 1, L])))
+        errs["con"] = []
+        for j, c in enumerate(ref.consts):
+            con, _ = sctx.get_constituent(j)
+            b = c["con"][:kc, L]
+            errs["con"].append(float(np.max(np.abs(con[:kc, L] - b)) /
+                                     max(1e-30, float(np.max(np.abs(b))))))
+        sctx.close()
+        cpu = dict(value=small.grid.nwet * done / t_cpu, unit="column-steps/s",
+                   cores=min(args.cpu_threads, 8), host_cpus=host_cpus(), kind="port",
+                   sample=f"{done} full steps (CALPUV2C with the FP64 CG, CALUVW, CALCONC) "
+                          f"of a 256x128x{kc} tile of the same generator by the FP64 C "
+                          f"restatement, {t_cpu:.1f} s (the CALUVW restatement runs on "
+                          f"one thread)")
+    case = make(nx, ny)
+    g = case.grid
+    nc = len(case.consts)
+    ctx = context(case)
+    n0 = 1
+    ctx.autotune(True)
+    for n in range(n0, n0 + max(args.warmup, 1)):
+        ctx.step(case.dt, n, internal=True)
+    n0 += max(args.warmup, 1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for n in range(n0, n0 + args.steps):
+        ctx.step(case.dt, n, internal=True)
+    ctx.synchronize()
+    t = time.perf_counter() - t0
+    n0 += args.steps
+    ctx.timers(True)
+    stats = []
+    for n in range(n0, n0 + args.steps):
+        stats.append(ctx.step(case.dt, n, internal=True))
+    tm = ctx.read_timers()
+    ctx.timers(False)
+    nstep = args.steps
+    per = {k: dict(launches=c_, avg_us=1e3 * m_ / c_, total_ms=m_) for k, (c_, m_) in tm.items()}
+    ub = uvw_bytes(kc, 4)
+    cand = {}
+    for k_, b in ub.items():
+        if k_ in per:
+            cand[k_] = (b * g.nwet, per[k_]["avg_us"] * 1e-6, per[k_]["launches"])
+    if "tran_upwind_vdiff" in per:
+        b = (kc * 4 * (6 + 2 * nc) + 40) * g.nwet
+        cand["tran_upwind_vdiff"] = (b, per["tran_upwind_vdiff"]["avg_us"] * 1e-6,
+                                     per["tran_upwind_vdiff"]["launches"])
+    cg_names = [k_ for k_ in CG_KERNELS_MIXED if k_ in per]
+    if cg_names:
+        cg_s = sum(per[k_]["total_ms"] for k_ in cg_names) * 1e-3 / nstep
+        alg = sum(alg_solve_bytes(st, g.nwet, True) for st in stats) / nstep
+        cand["CG solve: " + "+".join(cg_names)] = (alg, cg_s, nstep)
+    dom = max(cand, key=lambda k_: cand[k_][1] * cand[k_][2])
+    b, s_, _ = cand[dom]
+    ach = b / s_ / 1e9
+    out = {
+        "metric": "column-steps/s (CALPUV2C + CALUVW + CALCONC), FP32-refined CG, FP32 storage",
+        "value": g.nwet * args.steps / t,
+        "unit": "column-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"full 3-D step at scale: {nx}x{ny} tidal basin, KC={kc}, "
+                               f"internal mode, {nc} constituents, dt={case.dt} s",
+                   "columns": g.nwet},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom, f"hydro3d{nx}x{ny}x{kc}"),
+                     "kernel": dom, "avg_us": s_ * 1e6, "bytes_per_launch": b},
+        "cpu_baseline": cpu,
+        "detail": {"stream_copy_gbs": stream_copy_gbs(ctx), "errors_vs_fp64_oracle": errs,
+                   "per_kernel_gbs": {k_: v[0] / v[1] / 1e9 for k_, v in cand.items()},
+                   "cg_iters_per_step": sum(st["iters"] for st in stats) / nstep,
+                   "kernels": per},
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
 def drying_bench(args):
     """Wetting and drying at scale (--workload drying): the tidal drying
     basin (bottom rising from 2 m below the datum at the open west boundary
@@ -700,12 +844,13 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("lake1024", "lake4096", "transport", "ensemble",
-                                           "adjoint", "drying"),
+                                           "adjoint", "drying", "hydro3d"),
                     default="lake1024",
                     help="lake1024: the external solve at scale (the headline; --n sets "
                          "the lattice); lake4096: the same on a 4096x4096 lattice (~11 M "
                          "cells, past the MALL); transport: CALCONC at scale; ensemble: "
-                         "256 basins in one context")
+                         "256 basins in one context; hydro3d: the full 3-D step "
+                         "with the internal mode")
     ap.add_argument("--nbasins", type=int, default=256)
     ap.add_argument("--an", type=int, default=512, help="adjoint: basin size")
     ap.add_argument("--asteps", type=int, default=200, help="adjoint: steps")
@@ -746,6 +891,9 @@ def main():
         return
     if args.workload == "drying":
         drying_bench(args)
+        return
+    if args.workload == "hydro3d":
+        hydro3d_bench(args)
         return
 
     lake_bench(args)

@@ -1023,6 +1023,8 @@ int efdc_timers_enable(efdc_ctx_t *c, int32_t on) {
   return EFDC_OK;
 }
 
+static_assert(TK_COUNT <= EFDC_TIMER_MAX, "efdc_timers_t holds fewer entries than the timers");
+
 int efdc_timers_read(efdc_ctx_t *c, efdc_timers_t *out) {
   if (!c || !out) return EFDC_ERR_ARG;
   HIP_TRY(c, hipStreamSynchronize(c->stream));

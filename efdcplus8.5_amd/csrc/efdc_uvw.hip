@@ -11,9 +11,9 @@
 //   k_uvw_w    W from the layer continuity (it reads the east and north
 //              neighbours' new layer flows), the two-level averages and the
 //              column sums of the averaged layer flows;
-//   k_uvw_obc  one workgroup: the layer flows of the open-boundary faces,
-//              side by side in the reference's order, then the averages and
-//              sums of the faces it changed;
+//   k_uvw_obc_side, k_uvw_obc_avg  the layer flows of the open-boundary
+//              faces, one launch per side in the reference's order, then the
+//              averages and sums of the faces they changed;
 //   k_uvw_hp   the depth from the layer flows, HP of open-boundary cells,
 //              HPK and HPKI;
 //   k_uvw_uv   (optional) U at V points and V at U points of the new level.
@@ -140,10 +140,12 @@ __device__ __forceinline__ bool wet_at(const UvwArgs<T> &a, int i) {
 // 1332-1343), the layer flows (:527-567), their correction to the external
 // solution (:571-611) and the velocities (:613-647).  Returns the new
 // UHE (VHE).  Arrays hold only the first KS (shears) or KC (flows) entries.
-template <typename T, int KM>
+template <typename T, int KM, bool EXACT>
 __device__ __forceinline__ double face_column(const UvwArgs<T> &a, int i, bool vface,
                                               double rc) {
-  const int KC = a.KC, KS = KC - 1;
+  // with EXACT the layer count is the template's, so every layer loop
+  // below is straight-line code without predicates
+  const int KC = EXACT ? KM : a.KC, KS = KC - 1;
   const int64_t P = a.P;
   const LayerConsts &C = *a.lc;
   const bool open = (vface ? a.svbo[i] : a.subo[i]) > 0.0;
@@ -165,27 +167,38 @@ __device__ __forceinline__ double face_column(const UvwArgs<T> &a, int i, bool v
   double *vel = vface ? a.v : a.u;
   const bool dscale = a.isdry > 0;
   const double dti = a.dti;
+  // Loads are issued for every unrolled layer at a valid address (layers
+  // past the column's read its last shear layer and are not used), so the
+  // compiler can issue them ahead of the dependent FP64 chain.
+  auto lay = [&](int k) -> int64_t { return i + (int64_t)(k < KS ? k : KS - 1) * P; };
 
-  double du[KM], uu[KM], cu[KM];
+  double du[KM], uu[KM], cu[KM], s3[KM];
   // forward sweep (:419-486)
 #pragma unroll
   for (int k = 0; k < KM; ++k) {
-    if (k < KS) {
-      const int64_t j = i + k * P;
-      double d;
+    // EXACT: loads at a valid address on every layer (issued early); else
+    // only on the column's layers (fewer registers live at once)
+    double av = 0.0, s3k = 0.0, d = 0.0;
+    if (EXACT || k < KS) {
+      const int64_t j = lay(k);
+      av = avi[j];
+      s3k = dscale ? sb * (double)s3d[j] : (double)s3d[j];
       if (a.shear) {
+        const double v0 = vel1[j], v1 = vel1[j + P];
         const double cf = open ? C.cdzf[k] : 0.0;
-        d = cf * (h1 * (vel1[j + P] - vel1[j]) * dti + 0.0);
+        d = cf * (h1 * (v1 - v0) * dti + 0.0);
         if (a.wind && k == KS - 1) d = d - (open ? C.cdzu[k] : 0.0) * ts;
       } else {
         d = din[j];
       }
-      const double s3 = dscale ? sb * (double)s3d[j] : (double)s3d[j];
-      d = s3 * d;
+    }
+    if (k < KS) {
+      s3[k] = s3k;
+      d = s3k * d;
       const double cl = open ? C.cdzl[k] : 0.0;
       const double cm = (open ? C.cdzm[k] : 0.0) * dti;
       const double cuu = open ? C.cdzu[k] : 0.0;
-      const double cmu = 1. + cm * hh * avi[j];
+      const double cmu = 1. + cm * hh * av;
       if (k > 0) {
         const double eu = 1. / (cmu - cl * cu[k - 1]);
         cu[k] = cuu * eu;
@@ -211,8 +224,9 @@ __device__ __forceinline__ double face_column(const UvwArgs<T> &a, int i, bool v
   double aa = 0., bb = 1.;
 #pragma unroll
   for (int k = 0; k < KM; ++k) {
+    const double av = (EXACT || k < KS) ? avi[lay(k)] : 0.0;
     if (k < KS) {
-      const double cr = (open ? C.cdzr[k] : 0.0) * rc * avi[i + k * P];
+      const double cr = (open ? C.cdzr[k] : 0.0) * rc * av;
       aa = aa + cr * du[k];
       bb = bb + cr * uu[k];
     }
@@ -220,48 +234,41 @@ __device__ __forceinline__ double face_column(const UvwArgs<T> &a, int i, bool v
   aa = aa / bb;
 #pragma unroll
   for (int k = 0; k < KM; ++k) {
-    if (k < KS) {
-      const int64_t j = i + k * P;
-      const double s3 = dscale ? sb * (double)s3d[j] : (double)s3d[j];
-      du[k] = s3 * (open ? C.dzg[k] : 0.0) * hh * avi[j] * (du[k] - aa * uu[k]);
-    }
+    const double av = (EXACT || k < KS) ? avi[lay(k)] : 0.0;
+    if (k < KS) du[k] = s3[k] * (open ? C.dzg[k] : 0.0) * hh * av * (du[k] - aa * uu[k]);
   }
-  // layer flows (:527-567); uu holds them from here on, KC entries
+  // layer flows (:527-567), kept in uu from here on (KC entries)
 #pragma unroll
   for (int k = 0; k < KM; ++k)
     if (k < KS) uhe = uhe + (open ? C.cdzd[k] : 0.0) * du[k];
-  double f[KM];
 #pragma unroll
   for (int k = KM - 1; k >= 0; --k) {
-    if (k == KC - 1) {
-      f[k] = uhe * sb;
-    } else if (k < KC - 1) {
-      const int64_t j = i + k * P;
-      const double s3 = dscale ? sb * (double)s3d[j] : (double)s3d[j];
-      f[k] = s3 * (f[k + 1] - du[k]);
-    }
+    if (k == KC - 1)
+      uu[k] = uhe * sb;
+    else if (k < KC - 1)
+      uu[k] = s3[k] * (uu[k + 1] - du[k]);
   }
 #pragma unroll
   for (int k = 0; k < KM; ++k)
-    if (k < KC) f[k] = f[k] * width;
+    if (k < KC) uu[k] = uu[k] * width;
   // correction to the external solution (:571-611)
   double tv = 0.;
 #pragma unroll
   for (int k = 0; k < KM; ++k)
-    if (k < KC) tv = tv + f[k] * (open ? C.sg[k] : 0.0);
+    if (k < KC) tv = tv + uu[k] * (open ? C.sg[k] : 0.0);
   tv = tv - ext;
   double un = 0.;
 #pragma unroll
   for (int k = 0; k < KM; ++k) {
     if (k < KC) {
-      const int64_t j = i + k * P;
+      const int64_t j = i + (int64_t)k * P;
       const double sg = open ? C.sg[k] : 0.0;
-      f[k] = f[k] - tv * (open ? C.cerr[k] : C.cerr_closed[k]);
-      flowf[j] = f[k];
-      flow[j] = f[k] * sg;
-      un = un + f[k] * sg;
+      const double f = uu[k] - tv * (open ? C.cerr[k] : C.cerr_closed[k]);
+      flowf[j] = f;
+      flow[j] = f * sg;
+      un = un + f * sg;
       // velocities (:613-647): two roundings as in the reference
-      const double ve = f[k] * hhi;
+      const double ve = f * hhi;
       vel[j] = ve * widthi;
     }
   }
@@ -291,7 +298,7 @@ __device__ void zero_dry_column(const UvwArgs<T> &a, int i) {
   a.tvarn[i] = 0.0;
 }
 
-template <typename T, int KM>
+template <typename T, int KM, bool EXACT>
 __global__ __launch_bounds__(EFDC_BLOCK) void k_uvw_col(UvwArgs<T> a) {
   const int i = uvw_cell(a.nwet);
   if (i > a.nwet) return;
@@ -321,9 +328,9 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_uvw_col(UvwArgs<T> a) {
     const double ue = a.uhdye[i], ve = a.vhdxe[i];
     a.uhdyf[i] = ue;
     a.uhdy[i] = ue;
-    a.u[i] = ue * a.hui[i] * a.dyiu[i];
     a.vhdxf[i] = ve;
     a.vhdx[i] = ve;
+    a.u[i] = ue * a.hui[i] * a.dyiu[i];
     a.v[i] = ve * a.hvi[i] * a.dxiv[i];
     a.w[i + P] = 0.0;
     return;
@@ -356,8 +363,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_uvw_col(UvwArgs<T> a) {
       a.vhdxek[j] = dz * ve;
     }
   }
-  a.uhe[i] = face_column<T, KM>(a, i, false, rcx);
-  a.vhe[i] = face_column<T, KM>(a, i, true, rcy);
+  a.uhe[i] = face_column<T, KM, EXACT>(a, i, false, rcx);
+  a.vhe[i] = face_column<T, KM, EXACT>(a, i, true, rcy);
 }
 
 // W from the layer continuity, ISTL = 2 (:665-683), zero on LOBCS cells
@@ -455,31 +462,35 @@ __device__ void obc_face(const UvwArgs<T> &a, const UvwObcDev &b, int side, int 
   }
 }
 
-// One workgroup: the four sides in the order S, W, E, N, each side's cells
-// in parallel (one thread per cell, its layers in order) unless a cell of
-// the side reads a face another cell of it writes (then one thread walks
-// the side layer by layer as the reference does); then the averages, W2
-// and column sums of the cells whose faces or W changed.
+// One side of the open-boundary layer flows: one thread per (layer, cell)
+// of the side (the sides run one launch after the other in the order S, W,
+// E, N), or, when a cell of the side reads a face another cell of it
+// writes, one thread walking the side layer by layer as the reference does.
 template <typename T>
-__global__ __launch_bounds__(EFDC_BLOCK) void k_uvw_obc(UvwArgs<T> a, UvwObcDev b) {
+__global__ __launch_bounds__(EFDC_BLOCK) void k_uvw_obc_side(UvwArgs<T> a, UvwObcDev b,
+                                                             int side) {
+  const int n = b.n[side], KC = a.KC;
+  if (b.serial[side]) {
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      for (int k = 0; k < KC; ++k)
+        for (int LL = 0; LL < n; ++LL) obc_face(a, b, side, LL, k);
+    return;
+  }
+  const int t = blockIdx.x * EFDC_BLOCK + (int)threadIdx.x;
+  if (t >= n * KC) return;
+  obc_face(a, b, side, t % n, t / n);
+}
+
+// The averages, W2 and column sums of the cells whose faces or W the sides
+// changed: one thread per listed cell.
+template <typename T>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_uvw_obc_avg(UvwArgs<T> a, UvwObcDev b) {
   const int KC = a.KC;
   const int64_t P = a.P;
-  for (int side = 0; side < 4; ++side) {
-    const int n = b.n[side];
-    if (b.serial[side]) {
-      if (threadIdx.x == 0)
-        for (int k = 0; k < KC; ++k)
-          for (int LL = 0; LL < n; ++LL) obc_face(a, b, side, LL, k);
-    } else {
-      for (int LL = threadIdx.x; LL < n; LL += EFDC_BLOCK)
-        for (int k = 0; k < KC; ++k) obc_face(a, b, side, LL, k);
-    }
-    __threadfence_block();
-    __syncthreads();
-  }
-  for (int q = threadIdx.x; q < b.navg_u; q += EFDC_BLOCK) {
+  int q = blockIdx.x * EFDC_BLOCK + (int)threadIdx.x;
+  if (q < b.navg_u) {
     const int x = b.avg_u[q];
-    if (!wet_at(a, x)) continue;
+    if (!wet_at(a, x)) return;
     double te = 0.0;
     for (int k = 0; k < KC; ++k) {
       const int64_t j = x + k * P;
@@ -490,10 +501,12 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_uvw_obc(UvwArgs<T> a, UvwObcDev 
       te = te + uh2;
     }
     a.tvare[x] = te;
+    return;
   }
-  for (int q = threadIdx.x; q < b.navg_v; q += EFDC_BLOCK) {
+  q -= b.navg_u;
+  if (q < b.navg_v) {
     const int x = b.avg_v[q];
-    if (!wet_at(a, x)) continue;
+    if (!wet_at(a, x)) return;
     double tn = 0.0;
     for (int k = 0; k < KC; ++k) {
       const int64_t j = x + k * P;
@@ -504,10 +517,12 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_uvw_obc(UvwArgs<T> a, UvwObcDev 
       tn = tn + vh2;
     }
     a.tvarn[x] = tn;
+    return;
   }
-  for (int q = threadIdx.x; q < b.nwz; q += EFDC_BLOCK) {
+  q -= b.navg_v;
+  if (q < b.nwz) {
     const int x = b.wz[q];
-    if (!wet_at(a, x)) continue;
+    if (!wet_at(a, x)) return;
     for (int k = 0; k < KC; ++k) {
       const int64_t j = x + k * P;
       a.w2[j + P] = (T)(0.5 * (a.w[j + P] + a.w1[j + P]));
@@ -851,20 +866,40 @@ int uvw_launch(efdc_ctx *c, const efdc_step_t *st, const efdc_uvw_opts_t *o) {
   const UvwArgs<T> a = uvw_args<T>(c, st, o);
   const int nb = efdc_grid_blocks(c->n_wet() + 1);
   const int KC = c->KC;
-  if (KC <= 8)
-    TIMED(c, TK_UVW_COL, k_uvw_col<T, 8><<<nb, EFDC_BLOCK, 0, c->stream>>>(a));
-  else if (KC <= 16)
-    TIMED(c, TK_UVW_COL, k_uvw_col<T, 16><<<nb, EFDC_BLOCK, 0, c->stream>>>(a));
-  else if (KC <= 24)
-    TIMED(c, TK_UVW_COL, k_uvw_col<T, 24><<<nb, EFDC_BLOCK, 0, c->stream>>>(a));
-  else if (KC <= 32)
-    TIMED(c, TK_UVW_COL, k_uvw_col<T, 32><<<nb, EFDC_BLOCK, 0, c->stream>>>(a));
-  else
-    TIMED(c, TK_UVW_COL, k_uvw_col<T, 64><<<nb, EFDC_BLOCK, 0, c->stream>>>(a));
+  switch (KC) {  // the common layer counts get a kernel of their own
+#define UVW_EXACT(n)                                                                     \
+  case n:                                                                                \
+    TIMED(c, TK_UVW_COL, k_uvw_col<T, n, true><<<nb, EFDC_BLOCK, 0, c->stream>>>(a)); \
+    break;
+    UVW_EXACT(2) UVW_EXACT(3) UVW_EXACT(4) UVW_EXACT(5) UVW_EXACT(6) UVW_EXACT(8)
+    UVW_EXACT(10) UVW_EXACT(12) UVW_EXACT(16) UVW_EXACT(20)
+#undef UVW_EXACT
+    default:
+      if (KC <= 8)
+        TIMED(c, TK_UVW_COL, k_uvw_col<T, 8, false><<<nb, EFDC_BLOCK, 0, c->stream>>>(a));
+      else if (KC <= 16)
+        TIMED(c, TK_UVW_COL, k_uvw_col<T, 16, false><<<nb, EFDC_BLOCK, 0, c->stream>>>(a));
+      else if (KC <= 24)
+        TIMED(c, TK_UVW_COL, k_uvw_col<T, 24, false><<<nb, EFDC_BLOCK, 0, c->stream>>>(a));
+      else if (KC <= 32)
+        TIMED(c, TK_UVW_COL, k_uvw_col<T, 32, false><<<nb, EFDC_BLOCK, 0, c->stream>>>(a));
+      else
+        TIMED(c, TK_UVW_COL, k_uvw_col<T, 64, false><<<nb, EFDC_BLOCK, 0, c->stream>>>(a));
+  }
   TIMED(c, TK_UVW_W, k_uvw_w<T><<<nb, EFDC_BLOCK, 0, c->stream>>>(a));
   const UvwObcDev &b = u->obc;
-  if (b.n[0] + b.n[1] + b.n[2] + b.n[3] > 0)
-    TIMED(c, TK_UVW_OBC, k_uvw_obc<T><<<1, EFDC_BLOCK, 0, c->stream>>>(a, b));
+  if (b.n[0] + b.n[1] + b.n[2] + b.n[3] > 0) {
+    for (int side = 0; side < 4; ++side) {
+      if (b.n[side] == 0) continue;
+      const int nt = b.serial[side] ? 1 : (b.n[side] * KC + EFDC_BLOCK - 1) / EFDC_BLOCK;
+      TIMED(c, TK_UVW_OBC,
+            k_uvw_obc_side<T><<<nt, EFDC_BLOCK, 0, c->stream>>>(a, b, side));
+    }
+    const int na = b.navg_u + b.navg_v + b.nwz;
+    if (na > 0)
+      TIMED(c, TK_UVW_OBC, k_uvw_obc_avg<T><<<(na + EFDC_BLOCK - 1) / EFDC_BLOCK, EFDC_BLOCK,
+                                               0, c->stream>>>(a, b));
+  }
   TIMED(c, TK_UVW_HP, k_uvw_hp<T><<<nb, EFDC_BLOCK, 0, c->stream>>>(a));
   if (o->uv_after) TIMED(c, TK_UVW_UV, k_uvw_uv<T><<<nb, EFDC_BLOCK, 0, c->stream>>>(a));
   HIP_TRY(c, hipGetLastError());

@@ -97,7 +97,7 @@ class TimerEntry(C.Structure):
 
 
 class Timers(C.Structure):
-    _fields_ = [("n", C.c_int32), ("k", TimerEntry * 32)]
+    _fields_ = [("n", C.c_int32), ("k", TimerEntry * 64)]
 
 
 class Balance(C.Structure):

@@ -506,7 +506,7 @@ int efdc_synchronize(efdc_ctx_t *ctx);
    enabled, every kernel launch of the context is bracketed by HIP events
    on the context's stream; efdc_timers_read() waits for the stream,
    returns the count and total duration per kernel and clears the record. */
-#define EFDC_TIMER_MAX 32
+#define EFDC_TIMER_MAX 64
 typedef struct {
   int32_t n;
   struct {
