@@ -342,6 +342,7 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   seg_free(c);
   adj_free(c);
   uvw_free(c);
+  bal_free(c);
   if (c->st_host) (void)hipHostFree(c->st_host);
   if (c->pcg_abort_host) (void)hipHostFree(c->pcg_abort_host);
   dfree(c->gat_off);

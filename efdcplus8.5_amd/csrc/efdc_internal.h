@@ -229,6 +229,8 @@ struct efdc_ctx {
   struct AdjState *adj = nullptr;
   // internal mode (efdc_uvw.hip), allocated on first use
   struct UvwState *uvw = nullptr;
+  // two-level balances (efdc_bal.hip), allocated by efdc_bal2t_begin
+  struct BalState *bal = nullptr;
   bool dzc_uni = false;           // DZC set and equal in every wet cell of a layer
   bool dzc_changed = false;
   std::vector<double> dzc_k;      // DZC per layer when dzc_uni
@@ -557,6 +559,11 @@ void uvw_state_pieces(efdc_ctx *c, std::vector<std::pair<void *, size_t>> &v);
 void uvw_obc_changed(efdc_ctx *c);
 void uvw_free(efdc_ctx *c);
 int uvw_run(efdc_ctx *c, const efdc_step_t *step, const efdc_uvw_opts_t *o);
+// two-level balances (efdc_bal.hip): the terms CALCONC adds while a
+// period is open
+int bal_before_tran(efdc_ctx *c, const efdc_step_t *step);
+int bal_after_tran(efdc_ctx *c, const efdc_step_t *step);
+void bal_free(efdc_ctx *c);
 int cg_run(efdc_ctx *c, bool check_sym, efdc_cg_stats_t *stats);
 int tran_run(efdc_ctx *c, const efdc_step_t *step);
 int balance_run(efdc_ctx *c, efdc_balance_t *out);

@@ -1572,9 +1572,11 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
 }  // namespace
 
 int tran_run(efdc_ctx *c, const efdc_step_t *step) {
-  int s = c->store == EFDC_STORE_FP64 ? tran_impl<double>(c, step)
-                                      : tran_impl<float>(c, step);
+  int s;
+  if ((s = bal_before_tran(c, step))) return s;  // BAL2T3A, EFDC/calconc.f90:76-80
+  s = c->store == EFDC_STORE_FP64 ? tran_impl<double>(c, step) : tran_impl<float>(c, step);
   if (s) return s;
+  if ((s = bal_after_tran(c, step))) return s;   // BAL2T2, EFDC/calconc.f90:659-664
   HIP_TRY(c, hipGetLastError());
   return EFDC_OK;
 }

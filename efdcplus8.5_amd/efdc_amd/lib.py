@@ -118,6 +118,13 @@ class UvwOpts(C.Structure):
                 ("uv_after", C.c_int32)]
 
 
+class Bal2t(C.Structure):
+    _fields_ = [("nconst", C.c_int32), ("nsteps", C.c_int64), ("volbeg", C.c_double),
+                ("volend", C.c_double), ("volout", C.c_double), ("volerr", C.c_double),
+                ("massbeg", C.c_double * 16), ("massend", C.c_double * 16),
+                ("massout", C.c_double * 16), ("masserr", C.c_double * 16)]
+
+
 class AdjOpts(C.Structure):
     _fields_ = [("nadj", C.c_int32), ("slot", C.c_int32 * 16), ("cg_rel_tol", C.c_double),
                 ("cg_itmax", C.c_int32)]
@@ -181,6 +188,8 @@ def load(path: Optional[str] = None):
         "efdc_calpuv2c": ([P, C.POINTER(Step), C.POINTER(CgStats)], C.c_int),
         "efdc_calconc": ([P, C.POINTER(Step)], C.c_int),
         "efdc_caluvw": ([P, C.POINTER(Step), C.POINTER(UvwOpts)], C.c_int),
+        "efdc_bal2t_begin": ([P], C.c_int),
+        "efdc_bal2t_end": ([P, C.POINTER(Bal2t)], C.c_int),
         "efdc_balance": ([P, C.POINTER(Balance)], C.c_int),
         "efdc_synchronize": ([P], C.c_int),
         "efdc_device_ptr": ([P, C.c_int32, C.POINTER(P), C.POINTER(C.c_int64)], C.c_int),
@@ -438,6 +447,20 @@ class Context:
     def calconc(self, dt: float, n: int, niter: Optional[int] = None):
         st = Step(dt=dt, n=n, niter=n if niter is None else niter)
         self._check(self.lib.efdc_calconc(self.h, C.byref(st)))
+
+    def bal2t_begin(self):
+        """BAL2T1: open a balance period (EFDC/bal2t.f90:8-176)."""
+        self._check(self.lib.efdc_bal2t_begin(self.h))
+
+    def bal2t_end(self) -> dict:
+        """BAL2T5: close it and return the totals, outflows and errors."""
+        b = Bal2t()
+        self._check(self.lib.efdc_bal2t_end(self.h, C.byref(b)))
+        nc = b.nconst
+        return dict(nsteps=b.nsteps, volbeg=b.volbeg, volend=b.volend, volout=b.volout,
+                    volerr=b.volerr, massbeg=np.array(b.massbeg[:nc]),
+                    massend=np.array(b.massend[:nc]), massout=np.array(b.massout[:nc]),
+                    masserr=np.array(b.masserr[:nc]))
 
     def balance(self):
         b = Balance()

@@ -133,6 +133,13 @@ module efdc_hip
     real(c_double) :: cg_rsq
   end type
 
+  type, bind(C) :: efdc_bal2t_t
+    integer(c_int32_t) :: nconst
+    integer(c_int64_t) :: nsteps
+    real(c_double) :: volbeg, volend, volout, volerr
+    real(c_double) :: massbeg(16), massend(16), massout(16), masserr(16)
+  end type
+
   type, bind(C) :: efdc_balance_t
     real(c_double) :: volume
     real(c_double) :: mass(16)
@@ -287,6 +294,17 @@ module efdc_hip
       type(c_ptr), value :: ctx
       type(efdc_step_t), intent(in) :: step
       type(efdc_uvw_opts_t), intent(in) :: opts
+    end function
+
+    integer(c_int) function efdc_bal2t_begin(ctx) bind(C, name="efdc_bal2t_begin")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function efdc_bal2t_end(ctx, out) bind(C, name="efdc_bal2t_end")
+      import :: c_int, c_ptr, efdc_bal2t_t
+      type(c_ptr), value :: ctx
+      type(efdc_bal2t_t), intent(out) :: out
     end function
 
     integer(c_int) function efdc_balance(ctx, out) bind(C, name="efdc_balance")

@@ -401,6 +401,33 @@ int efdc_calconc(efdc_ctx_t *ctx, const efdc_step_t *step);
 /* FP64 volume and constituent-mass totals of the device state. */
 int efdc_balance(efdc_ctx_t *ctx, efdc_balance_t *out);
 
+/* ------------------------------------------------- two-level balances
+   The balances of a period (ISBAL >= 1, EFDC/bal2t.f90) for the volume and
+   every registered constituent, in FP64 on the device, over the balance
+   region the reference uses (the open-boundary cells left out: SPB = 0 on
+   pressure-BC cells, SCB = 0 on concentration-BC cells,
+   EFDC/setbcs.f90:180-239,952).  efdc_bal2t_begin is BAL2T1 (the totals at
+   the start, EFDC/bal2t.f90:8-176); while the period is open every
+   efdc_calconc adds BAL2T3A before its transport (the volume of the sources,
+   QSUME - QDWASTE, and the mass they bring in or take out, :316-692) and
+   BAL2T2 after it (the volume and mass across the faces of the
+   concentration-BC cells, at the upwind concentration after the transport,
+   :179-315); efdc_bal2t_end is BAL2T5's error accounting (:866-1040):
+   the totals at the end, the outflow totals and END - (BEG - OUT).  Not
+   accumulated: the momentum and energy terms (UMO, UUE, PPE, BBE), the
+   sediment-bed, toxics and hydraulic-structure terms, which need state
+   outside this library. */
+typedef struct {
+  int32_t nconst;         /* constituents reported                           */
+  int64_t nsteps;         /* CALCONC calls in the period                     */
+  double volbeg, volend;  /* VOLBEG2T, VOLEND2T (m^3)                       */
+  double volout;          /* VOLOUT: net volume out of the region            */
+  double volerr;          /* VOLERR2T = VOLEND2T - (VOLBEG2T - VOLOUT)       */
+  double massbeg[16], massend[16], massout[16], masserr[16]; /* per constituent */
+} efdc_bal2t_t;
+int efdc_bal2t_begin(efdc_ctx_t *ctx);
+int efdc_bal2t_end(efdc_ctx_t *ctx, efdc_bal2t_t *out);
+
 /* ---------------------------------------------------------- internal mode
    CALUVW(ISTL=2) on sigma layers (IGRIDV = 0, KSZ = 1), 2TL, no waves and
    no subgrid channels (EFDC/caluvw.f90:176-1172), with the two steps HDMT2T

@@ -52,7 +52,11 @@ static const double PI_EFDC = 3.1415926535898; /* EFDC/aaefdc.f90:650-651 */
     21  Sherman-Morrison correction for the bottom drag left out (:491-520)
     22  W without the external-flow split UHDYEK (:676-679)
     23  W without the factor 1/2 of the two-level flow average (:676)
-    24  depth from the layer flows with the east sum added (:953)    */
+    24  depth from the layer flows with the east sum added (:953)
+   and in the two-level balances (EFDC/bal2t.f90):
+    30  west open-boundary volume flux with the wrong sign (:224)
+    31  source mass without the withdrawal term MIN(QSS,0)*C (:374)
+    32  volume totals without the SPB mask of the open-boundary cells (:86) */
 #ifndef ORC_PLANT
 #define ORC_PLANT 0
 #endif
@@ -1272,9 +1276,13 @@ void orc_settle(orc_ctx *o, double dtsed) {
   }
 }
 
+static void bal2t_sources(orc_ctx *o, double dt);
+static void bal2t_open(orc_ctx *o, double dt);
+
 int orc_calconc(orc_ctx *o, double dt, int64_t niter) {
   const int nc = o->nconst;
   const int nt = o->nthreads > 0 ? o->nthreads : 1;
+  if (o->isbal) bal2t_sources(o, dt);  /* EFDC/calconc.f90:76-80 */
   /* one constituent per thread, EFDC/calconc.f90:159-229 */
 #pragma omp parallel for num_threads(nt < nc ? nt : (nc > 0 ? nc : 1)) schedule(static, 1)
   for (int c = 0; c < nc; ++c) {
@@ -1302,6 +1310,7 @@ int orc_calconc(orc_ctx *o, double dt, int64_t niter) {
     }
   }
   orc_settle(o, dt);
+  if (o->isbal) bal2t_open(o, dt);  /* EFDC/calconc.f90:659-664 */
   return EFDC_OK;
 }
 
@@ -1926,4 +1935,129 @@ int orc_caluvw(orc_ctx *o, double dt, const efdc_uvw_opts_t *opt) {
   (void)UHDY1E;
   (void)VHDX1E;
   return status;
+}
+
+/* ===================================================================== */
+/* Two-level balances (ISBAL >= 1): BAL2T1 totals, the source terms of      */
+/* BAL2T3A and the open-boundary fluxes of BAL2T2 for the volume and every  */
+/* constituent, the totals of BAL2T5 (EFDC/bal2t.f90).  The balance region  */
+/* leaves out the open-boundary cells: SPB = 0 on pressure-BC cells and     */
+/* SCB = 0 on concentration-BC cells (EFDC/setbcs.f90:180-239,952).         */
+/* ===================================================================== */
+static void bal2t_masks(const orc_ctx *o, double *spb, double *scb) {
+  for (int L = 1; L <= o->LCM; ++L) spb[HX(L)] = scb[HX(L)] = 1.0;
+  for (int s = 0; s < 4; ++s)
+    for (int q = 0; q < o->pbc[s].n; ++q) spb[HX(o->pbc[s].L[q])] = PLANT(32, 1.0, 0.0);
+  for (int s = 0; s < 4; ++s)
+    for (int q = 0; q < o->cbc[s].n; ++q) scb[HX(o->cbc[s].L[q])] = 0.0;
+}
+
+/* VOLBEG2T / VOLEND2T (:86,939) and the constituent masses (:93-96,
+   945-948): K outer, L inner, in the reference's order */
+int orc_bal2t_totals(orc_ctx *o, double *volume, double *mass) {
+  double *spb = malloc(sizeof(double) * o->LCM), *scb = malloc(sizeof(double) * o->LCM);
+  bal2t_masks(o, spb, scb);
+  const double *DXYP = A(EFDC_DXYP), *HP = A(EFDC_HP), *DZC = A(EFDC_DZC);
+  double v = 0.;
+  for (int L = 2; L <= o->LA; ++L) v = v + spb[HX(L)] * DXYP[HX(L)] * HP[HX(L)];
+  *volume = v;
+  for (int c = 0; c < o->nconst; ++c) {
+    const double *C = o->c[c].con;
+    double m = 0.;
+    for (int K = 1; K <= o->KC; ++K)
+      for (int L = 2; L <= o->LA; ++L)
+        m = m + scb[HX(L)] * DXYP[HX(L)] * HP[HX(L)] * C[KX(L, K)] * DZC[KX(L, K)];
+    mass[c] = m;
+  }
+  free(spb);
+  free(scb);
+  return EFDC_OK;
+}
+
+/* BAL2T1 (:8-176): the totals, and the outflow totals zeroed */
+int orc_bal2t_begin(orc_ctx *o, double *volume, double *mass) {
+  o->bal_volout = 0.;
+  for (int c = 0; c < ORC_MAXC; ++c) o->bal_massout[c] = 0.;
+  return orc_bal2t_totals(o, volume, mass);
+}
+
+/* BAL2T3A (:316-692): the volume the sources add, QSUME - QDWASTE over
+   every cell (:332-336), and the mass of each constituent they add, the
+   inflow at CQS / CSERT and the withdrawal at the cell's concentration
+   (:360-366 for salinity, the same for every constituent); called before
+   the transport, so C is the level at the step start */
+static void bal2t_sources(orc_ctx *o, double dt) {
+  const double DELT = dt;
+  const double *QSUME = A(EFDC_QSUME);
+  for (int L = 2; L <= o->LA; ++L) {
+    const double qd = (o->NDRYSTP < 0 && o->qdwaste) ? o->qdwaste[HX(L)] : 0.;
+    o->bal_volout = o->bal_volout - DELT * (QSUME[HX(L)] - qd);
+  }
+  const int KC = o->KC, nc = o->nconst;
+  for (int c = 0; c < nc; ++c) {
+    const double *C = o->c[c].con;
+    double out = o->bal_massout[c];
+    for (int q = 0; q < o->nqsij; ++q) {
+      const int L = o->lqs[q];
+      for (int K = 1; K <= KC; ++K) {
+        const double qss = o->qss[q * KC + K - 1], qser = o->qsercell[q * KC + K - 1];
+        const double cqs = o->cqs[(size_t)(q * KC + K - 1) * nc + c];
+        const double cser = o->csert_q[(size_t)(q * KC + K - 1) * nc + c];
+        const double ck = C[KX(L, K)];
+        out = out - DELT * dmax(qss, 0.) * cqs - PLANT(31, 0., DELT * dmin(qss, 0.) * ck) -
+              DELT * dmax(qser, 0.) * cser - DELT * dmin(qser, 0.) * ck;
+      }
+    }
+    o->bal_massout[c] = out;
+  }
+}
+
+/* BAL2T2 (:179-315): the volume and constituent mass across the faces of
+   the concentration-BC cells, sides S, W, E, N, layer by layer; the mass
+   is carried at the upwind cell's concentration after the transport */
+static void bal2t_open(orc_ctx *o, double dt) {
+  const double DELT = dt;
+  const int KC = o->KC, nc = o->nconst;
+  const double *UHDY2 = A(EFDC_UHDY2), *VHDX2 = A(EFDC_VHDX2);
+  double vol = o->bal_volout;
+  double mo[ORC_MAXC];
+  for (int c = 0; c < nc; ++c) mo[c] = o->bal_massout[c];
+  for (int K = 1; K <= KC; ++K)       /* south: the face of LN (:193-214) */
+    for (int q = 0; q < o->cbc[0].n; ++q) {
+      const int L = o->cbc[0].L[q], LN = o->LNC[HX(L)];
+      const double f = VHDX2[KX(LN, K)];
+      vol = vol - DELT * f;
+      for (int c = 0; c < nc; ++c)
+        mo[c] = mo[c] - DELT * dmin(f, 0.) * o->c[c].con[KX(LN, K)] -
+                DELT * dmax(f, 0.) * o->c[c].con[KX(L, K)];
+    }
+  for (int K = 1; K <= KC; ++K)       /* west: the face of LEC(L) (:217-238) */
+    for (int q = 0; q < o->cbc[1].n; ++q) {
+      const int L = o->cbc[1].L[q], LE = o->LEC[HX(L)];
+      const double f = UHDY2[KX(LE, K)];
+      vol = PLANT(30, vol + DELT * f, vol - DELT * f);
+      for (int c = 0; c < nc; ++c)
+        mo[c] = mo[c] - DELT * dmin(f, 0.) * o->c[c].con[KX(LE, K)] -
+                DELT * dmax(f, 0.) * o->c[c].con[KX(L, K)];
+    }
+  for (int K = 1; K <= KC; ++K)       /* east: the face of L (:241-262) */
+    for (int q = 0; q < o->cbc[2].n; ++q) {
+      const int L = o->cbc[2].L[q], LW = o->LWC[HX(L)];
+      const double f = UHDY2[KX(L, K)];
+      vol = vol + DELT * f;
+      for (int c = 0; c < nc; ++c)
+        mo[c] = mo[c] + DELT * dmin(f, 0.) * o->c[c].con[KX(L, K)] +
+                DELT * dmax(f, 0.) * o->c[c].con[KX(LW, K)];
+    }
+  for (int K = 1; K <= KC; ++K)       /* north: the face of L (:265-287) */
+    for (int q = 0; q < o->cbc[3].n; ++q) {
+      const int L = o->cbc[3].L[q], LS = o->LSC[HX(L)];
+      const double f = VHDX2[KX(L, K)];
+      vol = vol + DELT * f;
+      for (int c = 0; c < nc; ++c)
+        mo[c] = mo[c] + DELT * dmin(f, 0.) * o->c[c].con[KX(L, K)] +
+                DELT * dmax(f, 0.) * o->c[c].con[KX(LS, K)];
+    }
+  o->bal_volout = vol;
+  for (int c = 0; c < nc; ++c) o->bal_massout[c] = mo[c];
 }

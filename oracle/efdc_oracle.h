@@ -74,6 +74,12 @@ typedef struct {
   int32_t *natdry;       /* (LCM) NATDRY: steps a wet cell has been isolated */
   double *qdwaste;       /* (LCM) QDWASTE of the last step (m^3/s) */
   double *vdwaste;       /* (LCM) VDWASTE accumulated (m^3) */
+  /* two-level balances of a period (ISBAL >= 1, BAL2T1..BAL2T5): while
+     isbal is set CALCONC adds BAL2T3A before and BAL2T2 after its
+     transport to the outflow totals */
+  int32_t isbal;
+  double bal_volout;
+  double bal_massout[ORC_MAXC];
 } orc_ctx;
 
 typedef struct {
@@ -98,6 +104,11 @@ void orc_settle(orc_ctx *o, double dtsed);
 int orc_vertical_diffusion(orc_ctx *o, double dt);
 /* balance totals: EFDC/bal2t.f90:90,103-104 */
 int orc_balance(orc_ctx *o, double *volume, double *mass);
+/* BAL2T1 totals with the SPB/SCB masks (EFDC/bal2t.f90:82-105), used at the
+   start (BAL2T1, which also zeroes the outflow totals) and the end
+   (BAL2T5, :935-958) of a balance period */
+int orc_bal2t_totals(orc_ctx *o, double *volume, double *mass);
+int orc_bal2t_begin(orc_ctx *o, double *volume, double *mass);
 /* SETOPENBC + SETOPENBC2 alone, and CALTRAN (+CALFQC) of one constituent
    alone: the units compared with the reference build in oracle/_ref */
 int orc_setopenbc_apply(orc_ctx *o, double dt, int64_t n);

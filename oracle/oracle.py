@@ -63,7 +63,9 @@ class OrcCtx(C.Structure):
                 ("nconst", C.c_int32), ("c", Const * MAXC),
                 ("ISDRY", C.c_int32), ("iscdry", PI), ("lmaskdry", PI),
                 ("sub1", PD), ("svb1", PD), ("ncordry", C.c_int32),
-                ("NDRYSTP", C.c_int32), ("natdry", PI), ("qdwaste", PD), ("vdwaste", PD)]
+                ("NDRYSTP", C.c_int32), ("natdry", PI), ("qdwaste", PD), ("vdwaste", PD),
+                ("isbal", C.c_int32), ("bal_volout", C.c_double),
+                ("bal_massout", C.c_double * MAXC)]
 
 
 class UvwOpts(C.Structure):
@@ -117,6 +119,8 @@ def lib(path: str = None):
                 orc_closure_pre=[C.POINTER(OrcCtx), PD],
                 orc_closure_post=[C.POINTER(OrcCtx)],
                 orc_caluvw=[C.POINTER(OrcCtx), C.c_double, C.POINTER(UvwOpts)],
+                orc_bal2t_totals=[C.POINTER(OrcCtx), PD, PD],
+                orc_bal2t_begin=[C.POINTER(OrcCtx), PD, PD],
                 orc_setopenbc_apply=[C.POINTER(OrcCtx), C.c_double, C.c_int64],
                 orc_caltran_only=[C.POINTER(OrcCtx), C.c_int, C.c_double,
                                   C.c_int64]).items():
@@ -272,6 +276,31 @@ class Oracle:
         m = np.zeros(MAXC)
         self.L.orc_balance(C.byref(self.o), C.byref(v), _pd(m))
         return v.value, m[:len(self.case.consts)].copy()
+
+    def bal2t_begin(self):
+        """BAL2T1: the masked volume and mass totals at the start of a
+        balance period; from now on every CALCONC adds its source
+        (BAL2T3A) and open-boundary (BAL2T2) terms to the outflow totals."""
+        v = C.c_double(0.0)
+        m = np.zeros(MAXC)
+        self.L.orc_bal2t_begin(C.byref(self.o), C.byref(v), _pd(m))
+        self.o.isbal = 1
+        self.bal_beg = (v.value, m[:len(self.case.consts)].copy())
+
+    def bal2t_end(self):
+        """BAL2T5: totals at the end of the period, the outflow totals and
+        the errors END - (BEG - OUT) for the volume and every constituent."""
+        v = C.c_double(0.0)
+        m = np.zeros(MAXC)
+        self.L.orc_bal2t_totals(C.byref(self.o), C.byref(v), _pd(m))
+        nc = len(self.case.consts)
+        vb, mb = self.bal_beg
+        vo = self.o.bal_volout
+        mo = np.array(self.o.bal_massout[:nc])
+        self.o.isbal = 0
+        return dict(volbeg=vb, volend=v.value, volout=vo, volerr=v.value - (vb - vo),
+                    massbeg=mb, massend=m[:nc].copy(), massout=mo,
+                    masserr=m[:nc] - (mb - mo))
 
     def closure_pre(self):
         nman = np.ascontiguousarray(self.case.manning_n, np.float64)
