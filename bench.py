@@ -417,6 +417,116 @@ def hydro3d_bench(args):
     ctx.close()
 
 
+def restart_bench(args):
+    """RESTART.OUT from the device (--workload restart): a tidal basin of
+    --rnx x --rny columns, KC = --tkc, internal mode, salinity and a dye,
+    FP64 storage, after a few full steps; one "step" = one RESTOUT of the
+    whole state (efdc_restart_text: every value formatted E17.8 on the
+    device into the file image, then copied to host memory).  value = bytes
+    of restart text the device formats per second; the roofline counts
+    the FP64 value read and the text written.  The CPU baseline is the
+    oracle's writer (the C library's correctly rounded conversion, one
+    thread) on a 128x64 tile."""
+    import ctypes as C
+    import numpy as np
+    from efdc_amd import formats, grids, lib
+
+    nx, ny, kc = args.rnx, args.rny, args.tkc
+
+    def make(nx_, ny_):
+        c = grids.tidal_basin(nx_, ny_, kc, seed=77, dt=5.0)
+        return grids.add_internal_mode(c, av=1e-3)
+
+    def host_arrays(case):
+        rng = np.random.default_rng(3)
+        lcm = case.grid.LCM
+        hp = case.h["HP"]
+        return dict(hwq=hp.copy(), h2wq=hp.copy(),
+                    qq=rng.uniform(1e-8, 1e-3, (kc + 1, lcm)),
+                    qq1=rng.uniform(1e-8, 1e-3, (kc + 1, lcm)),
+                    qql=rng.uniform(1e-9, 1e-4, (kc + 1, lcm)),
+                    qql1=rng.uniform(1e-9, 1e-4, (kc + 1, lcm)),
+                    dml=rng.uniform(0.0, 1.0, (kc + 1, lcm)))
+
+    def opts(case):
+        mo = [c["mo"] for c in case.consts]
+        return dict(isco=[0, 1, 0, 1, 0, 0, 0, 0, 0], istran=[0, 1, 0, 1, 0, 0, 0, 0, 0],
+                    slots=(mo.index(1), -1, mo.index(3)), **host_arrays(case))
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        small = make(128, 64)
+        o = orc.Oracle(small, rsqm=1e-20, iterm=3000)
+        for n in range(1, 4):
+            o.step(n)
+        kw = opts(small)
+        t_cpu, done, nbytes = 0.0, 0, 0
+        while done < 2 or (t_cpu < args.cpu_seconds and done < 50):
+            t0 = time.perf_counter()
+            nbytes += len(o.restart_text(3, 0.5, keep_nlo=True, **kw))
+            t_cpu += time.perf_counter() - t0
+            done += 1
+        cpu = dict(value=nbytes / t_cpu, unit="restart bytes/s", cores=1, host_cpus=host_cpus(),
+                   kind="port",
+                   sample=f"{done} RESTOUTs of a 128x64x{kc} tile by the C restatement "
+                          f"(snprintf's exact conversion), {t_cpu:.1f} s")
+    case = make(nx, ny)
+    g = case.grid
+    ctx = lib.Context(g, RSQM=1e-20, ITERM=3000).upload_case(case)
+    for n in range(1, 4):
+        ctx.step(case.dt, n, internal=True)
+    kw = opts(case)
+    r, keep = formats.restart_opts(3, 0.5, keep_nlo=True, **kw)
+    nb = C.c_int64(0)
+    ctx._check(ctx.lib.efdc_restart_text(ctx.h, C.byref(r), None, 0, C.byref(nb)))
+    buf = C.create_string_buffer(nb.value)
+    for _ in range(max(args.warmup, 1)):
+        ctx._check(ctx.lib.efdc_restart_text(ctx.h, C.byref(r), buf, nb.value, C.byref(nb)))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx._check(ctx.lib.efdc_restart_text(ctx.h, C.byref(r), buf, nb.value, C.byref(nb)))
+    t = time.perf_counter() - t0
+    ctx.timers(True)
+    for _ in range(args.steps):
+        ctx._check(ctx.lib.efdc_restart_text(ctx.h, C.byref(r), buf, nb.value, C.byref(nb)))
+    tm = ctx.read_timers()
+    ctx.timers(False)
+    launches, ms = tm["restart_format"]
+    avg = ms * 1e-3 / launches
+    text = nb.value
+    nc = len(case.consts)
+    nbc = sum(len(s["L"]) for s in case.conc_bc["side"] if s is not None)
+    nvals = g.nwet * (5 + 4 + 4 * kc + 5 * (kc + 1) + 2 * 2 * kc + 1 + kc) + nbc * 2 * 2 * kc
+    alg = text + 8 * nvals
+    ach = alg / avg / 1e9
+    head = bytes(buf.raw[:200]).split(b"\n")[0]
+    out = {
+        "metric": "RESTART.OUT bytes formatted per second on the device (E17.8, exact)",
+        "value": text / avg,
+        "unit": "bytes/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * avg, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64->text", "data": "synthetic",
+        "config": {"workload": f"RESTOUT of a {nx}x{ny} tidal basin, KC={kc}, internal mode, "
+                               f"{nc} constituents", "columns": g.nwet, "text_bytes": text,
+                   "values": nvals},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("restart_format", f"restart{nx}x{ny}x{kc}"),
+                     "kernel": "restart_format", "avg_us": avg * 1e6,
+                     "bytes_per_launch": alg},
+        "cpu_baseline": cpu,
+        "detail": {"stream_copy_gbs": stream_copy_gbs(ctx),
+                   "end_to_end_host_bytes_per_s": text * args.steps / t,
+                   "end_to_end_ms": 1e3 * t / args.steps, "first_line": head.decode()},
+    }
+    print(json.dumps(out))
+    del keep
+    ctx.close()
+
+
 def drying_bench(args):
     """Wetting and drying at scale (--workload drying): the tidal drying
     basin (bottom rising from 2 m below the datum at the open west boundary
@@ -844,7 +954,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("lake1024", "lake4096", "transport", "ensemble",
-                                           "adjoint", "drying", "hydro3d"),
+                                           "adjoint", "drying", "hydro3d", "restart"),
                     default="lake1024",
                     help="lake1024: the external solve at scale (the headline; --n sets "
                          "the lattice); lake4096: the same on a 4096x4096 lattice (~11 M "
@@ -861,6 +971,8 @@ def main():
     ap.add_argument("--tnx", type=int, default=2048)
     ap.add_argument("--tny", type=int, default=1024)
     ap.add_argument("--tkc", type=int, default=20)
+    ap.add_argument("--rnx", type=int, default=1024, help="restart: basin length")
+    ap.add_argument("--rny", type=int, default=512, help="restart: basin width")
     ap.add_argument("--dnx", type=int, default=1024, help="drying: basin length")
     ap.add_argument("--dny", type=int, default=512, help="drying: basin width")
     ap.add_argument("--isdry", type=int, default=99, choices=(1, 99),
@@ -894,6 +1006,9 @@ def main():
         return
     if args.workload == "hydro3d":
         hydro3d_bench(args)
+        return
+    if args.workload == "restart":
+        restart_bench(args)
         return
 
     lake_bench(args)

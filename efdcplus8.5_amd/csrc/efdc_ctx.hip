@@ -114,7 +114,8 @@ const char *const efdc_timer_names[TK_COUNT] = {
     "closure_pre", "closure_post", "balance", "adac_gradients",
     "adac_fluxes", "fct_limiters", "tran_finish_vdiff", "cg_persistent64",
     "mx_persistent", "adj_tran_col", "adj_tran_gather", "adj_hydro", "adj_cg", "settle",
-    "uvw_columns", "uvw_w_averages", "uvw_openbc", "uvw_depths", "uvw_uv", "cg_iterations"};
+    "uvw_columns", "uvw_w_averages", "uvw_openbc", "uvw_depths", "uvw_uv",
+    "restart_format", "ee_pack", "cg_iterations"};
 
 hipEvent_t timer_event(efdc_ctx *c) {
   if (c->ev_pool.empty()) {
@@ -343,6 +344,7 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   adj_free(c);
   uvw_free(c);
   bal_free(c);
+  io_free(c);
   if (c->st_host) (void)hipHostFree(c->st_host);
   if (c->pcg_abort_host) (void)hipHostFree(c->pcg_abort_host);
   dfree(c->gat_off);

@@ -231,6 +231,8 @@ struct efdc_ctx {
   struct UvwState *uvw = nullptr;
   // two-level balances (efdc_bal.hip), allocated by efdc_bal2t_begin
   struct BalState *bal = nullptr;
+  // restart and EE output (efdc_io.hip), allocated on first use
+  struct IoState *io = nullptr;
   bool dzc_uni = false;           // DZC set and equal in every wet cell of a layer
   bool dzc_changed = false;
   std::vector<double> dzc_k;      // DZC per layer when dzc_uni
@@ -250,7 +252,7 @@ enum {
   TK_TRAN_BC, TK_CLOSURE_PRE, TK_CLOSURE_POST, TK_BALANCE, TK_ADAC_GRAD,
   TK_ADAC_FLUX, TK_FCT, TK_COL_FINISH, TK_PCG64, TK_PCGMX, TK_ADJ_TCOL, TK_ADJ_TGATHER,
   TK_ADJ_HYDRO, TK_ADJ_CG, TK_SETTLE,
-  TK_UVW_COL, TK_UVW_W, TK_UVW_OBC, TK_UVW_HP, TK_UVW_UV,
+  TK_UVW_COL, TK_UVW_W, TK_UVW_OBC, TK_UVW_HP, TK_UVW_UV, TK_IO_FMT, TK_IO_PACK,
   TK_CG_ITERS,  // not a kernel: multi-kernel CG iterations performed
   TK_COUNT
 };
@@ -564,6 +566,8 @@ int uvw_run(efdc_ctx *c, const efdc_step_t *step, const efdc_uvw_opts_t *o);
 int bal_before_tran(efdc_ctx *c, const efdc_step_t *step);
 int bal_after_tran(efdc_ctx *c, const efdc_step_t *step);
 void bal_free(efdc_ctx *c);
+// restart and EE output (efdc_io.hip)
+void io_free(efdc_ctx *c);
 int cg_run(efdc_ctx *c, bool check_sym, efdc_cg_stats_t *stats);
 int tran_run(efdc_ctx *c, const efdc_step_t *step);
 int balance_run(efdc_ctx *c, efdc_balance_t *out);

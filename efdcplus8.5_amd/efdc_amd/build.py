@@ -17,7 +17,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(HERE, "libefdc_hip.so")
 SOURCES = ["efdc_ctx.hip", "efdc_puv.hip", "efdc_tran.hip", "efdc_cgp.hip",
-           "efdc_seg.hip", "efdc_adj.hip", "efdc_uvw.hip", "efdc_bal.hip"]
+           "efdc_seg.hip", "efdc_adj.hip", "efdc_uvw.hip", "efdc_bal.hip",
+           "efdc_io.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_FLAGS = [

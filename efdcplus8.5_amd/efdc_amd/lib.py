@@ -17,6 +17,7 @@ from typing import Optional
 
 import numpy as np
 
+from . import formats as _fmt
 from .abi import ARRAY_IDS, STATUS_NAMES, shape_class
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -189,6 +190,12 @@ def load(path: Optional[str] = None):
         "efdc_calconc": ([P, C.POINTER(Step)], C.c_int),
         "efdc_caluvw": ([P, C.POINTER(Step), C.POINTER(UvwOpts)], C.c_int),
         "efdc_bal2t_begin": ([P], C.c_int),
+        "efdc_restart_text": ([P, C.POINTER(_fmt.Restart), C.c_char_p, C.c_int64,
+                               C.POINTER(C.c_int64)], C.c_int),
+        "efdc_restart_write": ([P, C.POINTER(_fmt.Restart), C.c_char_p], C.c_int),
+        "efdc_ee_open": ([P, C.c_char_p, C.POINTER(_fmt.EeGrid)], C.c_int),
+        "efdc_ee_append": ([P, C.c_char_p, C.c_int32, C.c_double, C.c_double, C.c_int32],
+                           C.c_int),
         "efdc_bal2t_end": ([P, C.POINTER(Bal2t)], C.c_int),
         "efdc_balance": ([P, C.POINTER(Balance)], C.c_int),
         "efdc_synchronize": ([P], C.c_int),
@@ -461,6 +468,32 @@ class Context:
                     volerr=b.volerr, massbeg=np.array(b.massbeg[:nc]),
                     massend=np.array(b.massend[:nc]), massout=np.array(b.massout[:nc]),
                     masserr=np.array(b.masserr[:nc]))
+
+    def restart_text(self, n: int, timeday: float, **kw) -> bytes:
+        """RESTOUT's RESTART.OUT text (EFDC/restartmod.f90:17-272) from the
+        device state; keywords as efdc_amd.formats.restart_opts."""
+        r, keep = _fmt.restart_opts(n, timeday, **kw)
+        nb = C.c_int64(0)
+        self._check(self.lib.efdc_restart_text(self.h, C.byref(r), None, 0, C.byref(nb)))
+        buf = C.create_string_buffer(max(nb.value, 1))
+        self._check(self.lib.efdc_restart_text(self.h, C.byref(r), buf, nb.value, C.byref(nb)))
+        del keep
+        return buf.raw[:nb.value]
+
+    def restart_write(self, path: str, n: int, timeday: float, **kw):
+        r, keep = _fmt.restart_opts(n, timeday, **kw)
+        self._check(self.lib.efdc_restart_write(self.h, C.byref(r), path.encode()))
+        del keep
+
+    def ee_open(self, dirname: str, ic: int, jc: int, rssbc, igridv: int = 0, ksz=None):
+        """HEADEROUT for EE_WS.OUT and EE_VEL.OUT (EFDC/efdcout.f90:73-113)."""
+        g, keep = _fmt.ee_grid(ic, jc, rssbc, igridv, ksz)
+        self._check(self.lib.efdc_ee_open(self.h, dirname.encode(), C.byref(g)))
+        del keep
+
+    def ee_append(self, dirname: str, n: int, eetime: float, delt: float, lminstep: int = 0):
+        """WSOUT and VELOUT (EFDC/efdcout.f90:371-497)."""
+        self._check(self.lib.efdc_ee_append(self.h, dirname.encode(), n, eetime, delt, lminstep))
 
     def balance(self):
         b = Balance()

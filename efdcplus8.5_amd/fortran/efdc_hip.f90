@@ -11,7 +11,7 @@ module efdc_hip
   ! efdc_status
   integer(c_int), parameter :: EFDC_OK = 0, EFDC_ERR_ARG = 1, EFDC_ERR_HIP = 2, &
       EFDC_ERR_CG_MAXITER = 3, EFDC_ERR_NEGDEPTH = 4, EFDC_ERR_STATE = 5, &
-      EFDC_ERR_NOMEM = 6, EFDC_ERR_NODEVICE = 7
+      EFDC_ERR_NOMEM = 6, EFDC_ERR_NODEVICE = 7, EFDC_ERR_IO = 8
   ! efdc_cg_mode, efdc_store
   integer(c_int32_t), parameter :: EFDC_CG_FP64 = 0, EFDC_CG_MIXED = 1
   integer(c_int32_t), parameter :: EFDC_STORE_FP64 = 0, EFDC_STORE_FP32 = 1
@@ -131,6 +131,23 @@ module efdc_hip
   type, bind(C) :: efdc_adj_stats_t
     integer(c_int32_t) :: cg_iters
     real(c_double) :: cg_rsq
+  end type
+
+  ! RESTOUT's records (EFDC/restartmod.f90:17-297); host arrays by c_loc
+  type, bind(C) :: efdc_restart_t
+    integer(c_int64_t) :: n
+    real(c_double) :: timeday
+    integer(c_int32_t) :: isco(0:8), istran(0:8), isice, slot(3), nsed
+    type(c_ptr) :: sed_slot
+    integer(c_int32_t) :: kb, kbm, keep_nlo
+    type(c_ptr) :: hwq, h2wq, qq, qq1, qql, qql1, dml, ice_thick, ice_temp, temb
+    type(c_ptr) :: sedb, sedb1, hbed, hbed1, vdrbed, vdrbed1
+  end type
+
+  ! HEADEROUT's grid (EFDC/efdcout.f90:73-113)
+  type, bind(C) :: efdc_ee_grid_t
+    integer(c_int32_t) :: ic, jc, igridv
+    type(c_ptr) :: ksz, rssbce, rssbcw, rssbcs, rssbcn
   end type
 
   type, bind(C) :: efdc_bal2t_t
@@ -305,6 +322,39 @@ module efdc_hip
       import :: c_int, c_ptr, efdc_bal2t_t
       type(c_ptr), value :: ctx
       type(efdc_bal2t_t), intent(out) :: out
+    end function
+
+    integer(c_int) function efdc_restart_text(ctx, r, buf, cap, bytes) &
+        bind(C, name="efdc_restart_text")
+      import :: c_int, c_ptr, c_int64_t, efdc_restart_t
+      type(c_ptr), value :: ctx
+      type(efdc_restart_t), intent(in) :: r
+      type(c_ptr), value :: buf
+      integer(c_int64_t), value :: cap
+      integer(c_int64_t), intent(out) :: bytes
+    end function
+
+    integer(c_int) function efdc_restart_write(ctx, r, path) bind(C, name="efdc_restart_write")
+      import :: c_int, c_ptr, c_char, efdc_restart_t
+      type(c_ptr), value :: ctx
+      type(efdc_restart_t), intent(in) :: r
+      character(kind=c_char), dimension(*), intent(in) :: path
+    end function
+
+    integer(c_int) function efdc_ee_open(ctx, dir, grid) bind(C, name="efdc_ee_open")
+      import :: c_int, c_ptr, c_char, efdc_ee_grid_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: dir
+      type(efdc_ee_grid_t), intent(in) :: grid
+    end function
+
+    integer(c_int) function efdc_ee_append(ctx, dir, n, eetime, delt, lminstep) &
+        bind(C, name="efdc_ee_append")
+      import :: c_int, c_ptr, c_char, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: dir
+      integer(c_int32_t), value :: n, lminstep
+      real(c_double), value :: eetime, delt
     end function
 
     integer(c_int) function efdc_balance(ctx, out) bind(C, name="efdc_balance")

@@ -44,7 +44,8 @@ typedef enum {
                               (EFDC/negdep.f90:34-45)                          */
   EFDC_ERR_STATE = 5,      /* call out of order (e.g. no open-BC table yet)    */
   EFDC_ERR_NOMEM = 6,      /* device allocation failed                         */
-  EFDC_ERR_NODEVICE = 7    /* no HIP device visible                            */
+  EFDC_ERR_NODEVICE = 7,   /* no HIP device visible                            */
+  EFDC_ERR_IO = 8          /* a file could not be opened or written            */
 } efdc_status;
 
 /* ------------------------------------------------------------- precision */
@@ -427,6 +428,76 @@ typedef struct {
 } efdc_bal2t_t;
 int efdc_bal2t_begin(efdc_ctx_t *ctx);
 int efdc_bal2t_end(efdc_ctx_t *ctx, efdc_bal2t_t *out);
+
+/* ------------------------------------------------------ restart output
+   RESTART.OUT as RESTOUT writes it (EFDC/restartmod.f90:17-297, VER 720):
+   the line N, TIMEDAY, VER (I20,4X,F12.4,I10), ISCO(0:8) (9I6), then per
+   cell L = 2..LA the records of FORMAT 906 (50E17.8): HP H1P HWQ H2WQ BELV;
+   UHDYE UHDY1E VHDXE VHDX1E; U, U1, V, V1 (K = 1..KC); QQ, QQ1, QQL, QQL1,
+   DML (K = 0..KC); the ice pair when ISTRAN(2) >= 1 and ISICE > 2; SAL and
+   SAL1; TEMB with TEM, and TEM1; DYE and DYE1 (each when ISTRAN(M) >= 1 and
+   ISCO(M) == 1); per cohesive class SEDB, SED, SEDB1, SED1, then HBED,
+   HBED1, VDRBED, VDRBED1; then for SAL, TEM, DYE and every sediment class
+   the concentration-BC cells of sides S, W, E, N: NLO (908 = 50I10) after
+   NLO = MAX(NLO - N, -NTSCR), as RESTOUT leaves it in memory, and CLO; then
+   QSUME with QSUM(K = 1..KC) per cell.  The records the host owns that
+   follow (MDCHH channels, groundwater AGWELV, rating-curve controls, :273-
+   295) are appended by the host after this call.  The values the library
+   holds come from the device (HP, H1P, BELV, the external flows, the
+   internal-mode U/U1/V/V1, the constituents CON/CON1, NLO/CLO, QSUME/QSUM);
+   the others are host arrays in the var_global layout (L = 1..LCM fastest,
+   then K).  Shellfish larvae, toxics and non-cohesive sediment
+   (ISTRAN(4), (5), (7) with ISCO == 1) are refused: this library does not
+   transport them.  Numbers are formatted as the Fortran runtime writes
+   them (correctly rounded, ties to even; exponents of three digits drop
+   the E; NaN and Inf right-justified). */
+typedef struct {
+  int64_t n;                 /* N                                              */
+  double timeday;            /* TIMEDAY                                        */
+  int32_t isco[9];           /* ISCO(0:8)                                      */
+  int32_t istran[9];         /* ISTRAN(0:8), 1..7 read                         */
+  int32_t isice;             /* ISICE                                          */
+  int32_t slot[3];           /* constituent slots of SAL, TEM, DYE (when written) */
+  int32_t nsed;              /* NSCM cohesive classes written (ISTRAN(6))      */
+  const int32_t *sed_slot;   /* their constituent slots                        */
+  int32_t kb, kbm;           /* KB and the declared bed dimension KBM          */
+  int32_t keep_nlo;          /* 1: leave NLO on the device unchanged           */
+  const double *hwq, *h2wq;                     /* (LCM)                       */
+  const double *qq, *qq1, *qql, *qql1, *dml;    /* (LCM, 0:KC)                 */
+  const double *ice_thick;   /* (LCM) the first ice value, ICETHICK + ICEVOL*999.8426/RHOI*DXYIP
+                                (+ the frazil ice with ISICE == 4)           */
+  const double *ice_temp;    /* (LCM) ICETEMP                                  */
+  const double *temb;        /* (LCM) TEMB                                     */
+  const double *sedb, *sedb1;                   /* (LCM, KBM, NSCM)            */
+  const double *hbed, *hbed1, *vdrbed, *vdrbed1; /* (LCM, KBM)                 */
+} efdc_restart_t;
+/* The restart text into buf (cap bytes); *bytes = its length (buf == NULL:
+   only the length, nothing changes on the device). */
+int efdc_restart_text(efdc_ctx_t *ctx, const efdc_restart_t *r, char *buf, int64_t cap,
+                      int64_t *bytes);
+/* RESTOUT(0): the restart text written to path. */
+int efdc_restart_write(efdc_ctx_t *ctx, const efdc_restart_t *r, const char *path);
+
+/* ---------------------------------------------------------- EE linkage
+   EE_WS.OUT and EE_VEL.OUT, unformatted stream files (FORM='BINARY').
+   efdc_ee_open is HEADEROUT's part for them (EFDC/efdcout.f90:73-113):
+   EE_WS.OUT gets VER = 8400, HSIZE, BSIZE, IC, JC, NACTIVE = LA-1 (INT*4);
+   EE_VEL.OUT gets VER, HSIZE, BSIZE, IGRIDV, CELL3D, IC, JC, KC, NACTIVE
+   and RSSBCE, RSSBCW, RSSBCS, RSSBCN as REAL*4 over L = 1..LCM.
+   efdc_ee_append is WSOUT and VELOUT (:371-497) for a run that does not
+   continue an earlier file: EE_WS.OUT gets N, EETIME (REAL*8), DELT
+   (REAL*4), LMINSTEP and HP(L = 2..LA) as REAL*4; EE_VEL.OUT gets N, EETIME,
+   DELT and U, V, W (REAL*4) over K = KSZ(L)..KC for L = 2..LA, each array
+   in turn.  N is the caller's N+NRESTART; EETIME = (DT*N + TCON*TBEGIN)/86400
+   (:44-45). */
+typedef struct {
+  int32_t ic, jc, igridv;
+  const int32_t *ksz;        /* (LCM) KSZ(L); NULL: 1 in every cell            */
+  const double *rssbce, *rssbcw, *rssbcs, *rssbcn;  /* (LCM)                   */
+} efdc_ee_grid_t;
+int efdc_ee_open(efdc_ctx_t *ctx, const char *dir, const efdc_ee_grid_t *grid);
+int efdc_ee_append(efdc_ctx_t *ctx, const char *dir, int32_t n, double eetime, double delt,
+                   int32_t lminstep);
 
 /* ---------------------------------------------------------- internal mode
    CALUVW(ISTL=2) on sigma layers (IGRIDV = 0, KSZ = 1), 2TL, no waves and

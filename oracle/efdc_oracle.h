@@ -118,6 +118,15 @@ int orc_caltran_only(orc_ctx *o, int c, double dt, int64_t niter);
    internal-mode arrays must be bound */
 int orc_caluvw(orc_ctx *o, double dt, const efdc_uvw_opts_t *opt);
 /* depth-uniform closure standing in for the internal mode */
+/* On-disk formats (efdc_oracle_io.c): RESTOUT's text into a malloc'ed
+   buffer (released with orc_free), HEADEROUT / WSOUT / VELOUT's EE_WS.OUT
+   and EE_VEL.OUT, and the E17.8 edit descriptor */
+int orc_restart_text(orc_ctx *o, const efdc_restart_t *r, char **text, int64_t *bytes);
+void orc_free(void *p);
+int orc_ee_open(orc_ctx *o, const char *dir, const efdc_ee_grid_t *g);
+int orc_ee_append(orc_ctx *o, const char *dir, const efdc_ee_grid_t *g, int32_t n,
+                  double eetime, double delt, int32_t lminstep);
+void orc_fmt_e17_8(double v, char out[17]);
 int orc_closure_pre(orc_ctx *o, const double *manning_n);
 int orc_closure_post(orc_ctx *o);
 

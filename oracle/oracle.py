@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 sys.path.insert(0, os.path.join(ROOT, "efdcplus8.5_amd"))
 from efdc_amd.abi import ARRAY_IDS  # noqa: E402
+from efdc_amd import formats as _fmt  # noqa: E402
 
 LIB = os.path.join(HERE, "liboracle.so")
 NARR = 256
@@ -83,15 +84,17 @@ def build(force: bool = False, plant: int = 0, out: str = None) -> str:
     checker).  ``plant`` > 0 builds a variant with that planted mistake
     (the ORC_PLANT list in efdc_oracle.c), for the tests that show a check
     fails on it; such a variant never replaces the checker."""
-    src = os.path.join(HERE, "efdc_oracle.c")
+    srcs = [os.path.join(HERE, f) for f in ("efdc_oracle.c", "efdc_oracle_io.c")]
+    deps = srcs + [os.path.join(HERE, "efdc_oracle.h"),
+                   os.path.join(ROOT, "include", "efdc_hip.h")]
     dst = out or LIB
     if plant and dst == LIB:
         raise ValueError("a planted oracle needs its own output path")
     if force or plant or not os.path.exists(dst) or \
-            os.path.getmtime(dst) < os.path.getmtime(src):
+            os.path.getmtime(dst) < max(os.path.getmtime(f) for f in deps):
         subprocess.check_call(
             ["gcc", "-O2", "-fopenmp", "-fPIC", "-shared",
-             "-ffp-contract=off", "-Wall", f"-DORC_PLANT={int(plant)}", src,
+             "-ffp-contract=off", "-Wall", f"-DORC_PLANT={int(plant)}", *srcs,
              "-o", dst, "-lm"])
     return dst
 
@@ -123,10 +126,17 @@ def lib(path: str = None):
                 orc_bal2t_begin=[C.POINTER(OrcCtx), PD, PD],
                 orc_setopenbc_apply=[C.POINTER(OrcCtx), C.c_double, C.c_int64],
                 orc_caltran_only=[C.POINTER(OrcCtx), C.c_int, C.c_double,
-                                  C.c_int64]).items():
+                                  C.c_int64],
+                orc_restart_text=[C.POINTER(OrcCtx), C.POINTER(_fmt.Restart),
+                                  C.POINTER(C.c_void_p), C.POINTER(C.c_int64)],
+                orc_free=[C.c_void_p],
+                orc_ee_open=[C.POINTER(OrcCtx), C.c_char_p, C.POINTER(_fmt.EeGrid)],
+                orc_ee_append=[C.POINTER(OrcCtx), C.c_char_p, C.POINTER(_fmt.EeGrid),
+                               C.c_int32, C.c_double, C.c_double, C.c_int32],
+                orc_fmt_e17_8=[C.c_double, C.c_char_p]).items():
             f = getattr(_lib, fn)
             f.argtypes = args
-            f.restype = None if fn == "orc_settle" else C.c_int
+            f.restype = None if fn in ("orc_settle", "orc_fmt_e17_8", "orc_free") else C.c_int
         _libs[path] = _lib
     return _libs[path]
 
@@ -301,6 +311,28 @@ class Oracle:
         return dict(volbeg=vb, volend=v.value, volout=vo, volerr=v.value - (vb - vo),
                     massbeg=mb, massend=m[:nc].copy(), massout=mo,
                     masserr=m[:nc] - (mb - mo))
+
+    def restart_text(self, n, timeday, **kw) -> bytes:
+        """RESTOUT's RESTART.OUT text (EFDC/restartmod.f90:17-272)."""
+        r, keep = _fmt.restart_opts(n, timeday, **kw)
+        out = C.c_void_p()
+        nb = C.c_int64(0)
+        st = self.L.orc_restart_text(C.byref(self.o), C.byref(r), C.byref(out), C.byref(nb))
+        assert st == 0, st
+        text = C.string_at(out, nb.value)
+        self.L.orc_free(out)
+        del keep
+        return text
+
+    def ee_open(self, dirname, ic, jc, rssbc, igridv=0, ksz=None):
+        g, keep = _fmt.ee_grid(ic, jc, rssbc, igridv, ksz)
+        self._ee = (g, keep)
+        assert self.L.orc_ee_open(C.byref(self.o), dirname.encode(), C.byref(g)) == 0
+
+    def ee_append(self, dirname, n, eetime, delt, lminstep=0):
+        g, _ = self._ee
+        assert self.L.orc_ee_append(C.byref(self.o), dirname.encode(), C.byref(g), n, eetime,
+                                    delt, lminstep) == 0
 
     def closure_pre(self):
         nman = np.ascontiguousarray(self.case.manning_n, np.float64)

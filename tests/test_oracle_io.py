@@ -1,0 +1,375 @@
+"""The on-disk formats on the CPU (RESTOUT, EFDC/restartmod.f90:17-297;
+HEADEROUT / WSOUT / VELOUT, EFDC/efdcout.f90:73-117,371-497):
+
+* the oracle's edit descriptors against the Fortran runtime: flang writes
+  the same values with FORMAT(50E17.8), (50I10), (I20,4X,F12.4,2I10) and
+  (9I6) and the text must be identical, over specials (signed zeros,
+  subnormals, NaN, infinities), exact ties, values next to every power of
+  ten and random bit patterns over the whole exponent range;
+* the device formatter (efdcplus8.5_amd/csrc/efdc_fmt.h, compiled for the
+  host by tests/fmt_harness.hip) against the oracle over two million
+  values: identical wherever it formats, and it hands over only ties,
+  near-ties and magnitudes outside [1e-280, 1e280];
+* RESTART.OUT's structure: every record of every cell in the reference's
+  statement order, each value read back to eight digits from the array
+  it belongs to, records of at most 50 values, and NLO left at
+  MAX(NLO - N, -NTSCR);
+* EE_WS.OUT / EE_VEL.OUT as GetEFDC reads them: header sizes, the record
+  size BSIZE times the snapshots filling the file exactly, and the REAL*4
+  values of HP, U, V, W.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FLANG = os.environ.get("EFDC_FC", "/opt/rocm/llvm/bin/flang")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+
+def _specials():
+    v = [0.0, -0.0, 1.0, -1.0, 0.5, 0.1, 1e-300, 1e300, 1e100, -1e-100, 5e-324, 2.2250738585072014e-308,
+         1.7976931348623157e308, 12345678.5, 12345677.5, 123456785.0, 1234567850.0, 0.99999999,
+         0.999999995, 0.9999999949999999, 9.9999999949e-5, 1e280, 1e-280, 9.999999995e279,
+         1.0000000049999999e-280, np.inf, -np.inf, np.nan, 1e7, 1e8, 99999999.5, 9999999.95]
+    for j in range(-300, 301):
+        p = float(f"1e{j}")
+        v += [p, np.nextafter(p, np.inf), np.nextafter(p, -np.inf)]
+    return np.array(v)
+
+
+def _ties(rng, n):
+    """Values within an ulp of N + 1/2 times a power of ten."""
+    N = rng.integers(10_000_000, 100_000_000, n)
+    k = rng.integers(-40, 40, n)
+    base = (N + 0.5) * np.power(10.0, -k.astype(float))
+    out = [base, np.nextafter(base, np.inf), np.nextafter(base, -np.inf)]
+    return np.concatenate(out)
+
+
+def _random_bits(rng, n):
+    b = rng.integers(0, 2 ** 63, n, dtype=np.int64).view(np.float64)
+    b = b[np.isfinite(b)]
+    sign = rng.integers(0, 2, len(b)) * 2 - 1
+    return b * sign
+
+
+def _values(n=20000, seed=7):
+    rng = np.random.default_rng(seed)
+    lg = 10.0 ** rng.uniform(-30, 30, n) * rng.choice([-1, 1], n)
+    return np.concatenate([_specials(), _ties(rng, n // 4), _random_bits(rng, n), lg,
+                           rng.normal(0, 10, n), np.round(rng.normal(0, 1e4, n))])
+
+
+def _oracle_e17(oracle_mod, v):
+    L = oracle_mod.lib()
+    out = C.create_string_buffer(17)
+    lines = []
+    for i in range(0, len(v), 50):
+        s = b""
+        for x in v[i:i + 50]:
+            L.orc_fmt_e17_8(float(x), out)
+            s += out.raw[:17]
+        lines.append(s)
+    return b"\n".join(lines) + b"\n"
+
+
+@pytest.fixture(scope="module")
+def flang_dir(tmp_path_factory):
+    if not os.path.exists(FLANG):
+        pytest.skip("no Fortran compiler")
+    return tmp_path_factory.mktemp("flang")
+
+
+def _flang_run(d, src, inputs=None):
+    f = os.path.join(d, "p.f90")
+    with open(f, "w") as fh:
+        fh.write(src)
+    exe = os.path.join(d, "p")
+    subprocess.check_call([FLANG, "-O0", f, "-o", exe], cwd=d)
+    if inputs is not None:
+        inputs.tofile(os.path.join(d, "in.bin"))
+    subprocess.check_call([exe], cwd=d)
+    return open(os.path.join(d, "out.txt"), "rb").read()
+
+
+def test_e17_8_matches_fortran_runtime(oracle_mod, flang_dir):
+    v = _values()
+    src = f"""
+program p
+  implicit none
+  integer, parameter :: n = {len(v)}
+  real(8) :: v(n)
+  integer :: i
+  open(11, file='in.bin', access='stream', form='unformatted', status='old')
+  read(11) v
+  close(11)
+  open(12, file='out.txt', status='replace')
+  write(12, '(50E17.8)') (v(i), i = 1, n)
+  close(12)
+end program
+"""
+    ref = _flang_run(str(flang_dir), src, v.astype("<f8"))
+    mine = _oracle_e17(oracle_mod, v)
+    if ref != mine:
+        a, b = ref.split(b"\n"), mine.split(b"\n")
+        for i, (x, y) in enumerate(zip(a, b)):
+            if x != y:
+                for j in range(0, len(x), 17):
+                    if x[j:j + 17] != y[j:j + 17]:
+                        raise AssertionError((i, j // 17, v[i * 50 + j // 17], x[j:j + 17],
+                                              y[j:j + 17]))
+    assert ref == mine
+
+
+def test_integer_and_header_formats_match_fortran_runtime(oracle_mod, flang_dir):
+    """The restart's first two lines (I20,4X,F12.4,2I10 and 9I6) and the
+    I10 records, against flang."""
+    from efdc_amd import grids
+    cases = [(0, 0.0), (123, 1.23456), (7, 0.00005), (7, 0.00015), (99999, 12345678.5),
+             (2 ** 40, 9999999.99995), (5, -0.00004), (1, 365.25)]
+    case = grids.tidal_basin(6, 4, 2, seed=1)
+    o = oracle_mod.Oracle(case)
+    lcm, kc = case.grid.LCM, case.grid.KC
+    z1, zk = np.zeros(lcm), np.zeros((kc + 1, lcm))
+    host = dict(hwq=z1, h2wq=z1, qq=zk, qq1=zk, qql=zk, qql1=zk, dml=zk)
+    lines = []
+    isco = [-99999, 1234567, 0, 1, 2, 3, 40, 500, 6000]
+    for n, t in cases:
+        txt = o.restart_text(n, t, isco=isco, **host)
+        lines += txt.split(b"\n")[:2]
+    prog = "\n".join(
+        f"  write(12, '(I20,4X,F12.4,2I10)') {n}_8, {('%.17e' % t).replace('e', 'd')}, 720\n"
+        f"  write(12, '(9I6)') {', '.join(str(i) for i in isco)}"
+        for n, t in cases)
+    ints = [-999999999, -1000000000, 9999999999, 10000000000, 0, 7, -7, 123456789]
+    prog += "\n  write(12, '(50I10)') " + ", ".join(f"{i}_8" for i in ints)
+    ref = _flang_run(str(flang_dir), f"""
+program p
+  implicit none
+  open(12, file='out.txt', status='replace')
+{prog}
+  close(12)
+end program
+""")
+    ref_lines = ref.split(b"\n")
+    assert ref_lines[:len(lines)] == lines
+    # I10 of the library's formatter (the BC records), via the harness
+    h = _harness_lib()
+    arr = np.array(ints, dtype=np.int64)
+    out = C.create_string_buffer(10 * len(ints))
+    h.fmt_i10_many(arr.ctypes.data_as(C.c_void_p), len(ints), out)
+    assert out.raw == ref_lines[len(lines)]
+
+
+_HARNESS = {}
+
+
+def _harness_lib():
+    if "lib" not in _HARNESS:
+        if not os.path.exists(HIPCC):
+            pytest.skip("no hipcc")
+        d = os.path.join(ROOT, "tests", "_build")
+        os.makedirs(d, exist_ok=True)
+        so = os.path.join(d, "fmt_harness.so")
+        src = os.path.join(ROOT, "tests", "fmt_harness.hip")
+        hdr = [os.path.join(ROOT, "efdcplus8.5_amd", "csrc", f) for f in ("efdc_fmt.h",
+                                                                          "efdc_pow10.h")]
+        if not os.path.exists(so) or os.path.getmtime(so) < max(
+                os.path.getmtime(f) for f in [src] + hdr):
+            subprocess.check_call(
+                [HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-fPIC", "-shared",
+                 "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",
+                 "-packed-fp32-ops", "-I", os.path.join(ROOT, "efdcplus8.5_amd", "csrc"), src,
+                 "-o", so + ".tmp"], stderr=subprocess.DEVNULL)
+            os.replace(so + ".tmp", so)
+        lib = C.CDLL(so)
+        lib.fmt_e17_many.restype = C.c_int64
+        lib.fmt_e17_many.argtypes = [C.c_void_p, C.c_int64, C.c_char_p, C.c_void_p]
+        lib.fmt_i10_many.argtypes = [C.c_void_p, C.c_int64, C.c_char_p]
+        _HARNESS["lib"] = lib
+    return _HARNESS["lib"]
+
+
+def test_device_formatter_matches_oracle(oracle_mod):
+    h = _harness_lib()
+    rng = np.random.default_rng(11)
+    v = np.concatenate([_values(200000, seed=3), _ties(rng, 300000),
+                        10.0 ** rng.uniform(-280, 280, 1000000)])
+    out = C.create_string_buffer(17 * len(v))
+    handed = np.zeros(len(v), np.uint8)
+    nh = h.fmt_e17_many(v.ctypes.data_as(C.c_void_p), len(v), out, handed.ctypes.data_as(C.c_void_p))
+    got = np.frombuffer(out.raw, "S17")
+    L = oracle_mod.lib()
+    buf = C.create_string_buffer(17)
+    # compare every formatted value (vectorised through the oracle, one call each)
+    idx = np.flatnonzero(handed == 0)
+    bad = []
+    for i in idx:
+        L.orc_fmt_e17_8(float(v[i]), buf)
+        if buf.raw[:17] != got[i]:
+            bad.append((v[i], got[i], buf.raw[:17]))
+            if len(bad) > 5:
+                break
+    assert not bad, bad
+    # what it hands over: magnitudes outside [1e-280, 1e280], or values
+    # whose scaled decimal lies within 2^-59 of a half-integer (checked
+    # with exact decimal arithmetic)
+    from decimal import Decimal, getcontext
+    getcontext().prec = 800
+    a = np.abs(v[handed == 1])
+    inr = a[(a >= 1e-280) & (a <= 1e280)]
+    # of the million log-uniform values at the end, next to none
+    assert handed[-1000000:].sum() < 100
+    for x in inr:
+        d = Decimal(float(x))
+        e = d.adjusted() + 1           # 10^(e-1) <= d < 10^e
+        sc = d.scaleb(8 - e)
+        frac = sc - int(sc)
+        assert abs(frac - Decimal("0.5")) <= Decimal(2) ** -59, x
+
+
+def _restart_case(oracle_mod, nsteps=6):
+    """A tidal basin with the internal mode, salinity and a dye on a west
+    concentration boundary, after a few steps."""
+    from efdc_amd import grids
+    case = grids.add_internal_mode(grids.tidal_basin(12, 7, 3, seed=5, dt=10.0, ntscr=3), av=1e-3)
+    o = oracle_mod.Oracle(case, rsqm=1e-24, iterm=3000)
+    for n in range(1, nsteps + 1):
+        assert o.step(n, uvw={})["status"] == 0
+    return case, o
+
+
+def _host_arrays(case, rng, kb=2):
+    lcm, kc = case.grid.LCM, case.grid.KC
+    return dict(hwq=rng.uniform(1, 5, lcm), h2wq=rng.uniform(1, 5, lcm),
+                qq=rng.uniform(0, 1e-3, (kc + 1, lcm)), qq1=rng.uniform(0, 1e-3, (kc + 1, lcm)),
+                qql=rng.uniform(0, 1e-4, (kc + 1, lcm)), qql1=rng.uniform(0, 1e-4, (kc + 1, lcm)),
+                dml=rng.uniform(0, 1, (kc + 1, lcm)), temb=rng.uniform(4, 20, lcm))
+
+
+def _fields(line, w=17):
+    out = []
+    for j in range(0, len(line), w):
+        s = line[j:j + w].decode().strip()
+        if w == 17 and "E" not in s and s not in ("NaN", "Inf", "-Inf"):
+            s = s[:-4] + "E" + s[-4:]
+        out.append(float(s) if w == 17 else int(s))
+    return out
+
+
+def test_restart_records_follow_restout(oracle_mod):
+    case, o = _restart_case(oracle_mod)
+    rng = np.random.default_rng(2)
+    host = _host_arrays(case, rng)
+    g = case.grid
+    LA, kc, nc = g.LA, g.KC, len(case.consts)
+    sal = [j for j, c in enumerate(case.consts) if c["mo"] == 1]
+    dye = [j for j, c in enumerate(case.consts) if c["mo"] == 3]
+    assert sal and dye
+    istran = [0, 1, 0, 1, 0, 0, 0, 0, 0]
+    isco = [0, 1, 0, 1, 0, 0, 0, 0, 0]
+    sides = [s for s in range(4) if case.conc_bc["side"][s] is not None]
+    mem = dict(zip(sides, o.cbc_mem))
+    # a history of outflow steps in the NLO memory
+    for s in sides:
+        mem[s]["nlo"][:] = rng.integers(-5, 9, mem[s]["nlo"].shape)
+    nlo_before = {s: mem[s]["nlo"].copy() for s in sides}
+    N = 6
+    txt = o.restart_text(N, 1.25, isco=isco, istran=istran, slots=(sal[0], -1, dye[0]), **host)
+    lines = txt.split(b"\n")
+    assert lines[-1] == b""
+    lines = lines[:-1]
+    assert lines[0] == b"%20d    %12.4f%10d" % (N, 1.25, 720)
+    it = iter(lines[2:])
+
+    def rec(expect):
+        got = _fields(next(it))
+        assert len(got) == len(expect)
+        np.testing.assert_allclose(got, expect, rtol=5.1e-8, atol=0)
+
+    h, k, w = case.h, case.k, case.w
+    for L in range(2, LA + 1):
+        i = L - 1
+        rec([h["HP"][i], h["H1P"][i], host["hwq"][i], host["h2wq"][i], h["BELV"][i]])
+        rec([h["UHDYE"][i], h["UHDY1E"][i], h["VHDXE"][i], h["VHDX1E"][i]])
+        for name in ("U", "U1", "V", "V1"):
+            rec(k[name][:kc, i])
+        for name in ("qq", "qq1", "qql", "qql1", "dml"):
+            rec(host[name][:kc + 1, i])
+        rec(case.consts[sal[0]]["con"][:kc, i])
+        rec(case.consts[sal[0]]["con1"][:kc, i])
+        rec(case.consts[dye[0]]["con"][:kc, i])
+        rec(case.consts[dye[0]]["con1"][:kc, i])
+    for slot in (sal[0], dye[0]):
+        for s in sides:
+            nb = len(mem[s]["L"])
+            ntscr = mem[s]["ntscr"]
+            nlo0 = nlo_before[s].reshape(nb, kc, nc)
+            clo = mem[s]["clo"].reshape(nb, kc, nc)
+            for LL in range(nb):
+                want = np.maximum(nlo0[LL, :, slot] - N, -ntscr[LL])
+                assert _fields(next(it), 10) == list(want)
+                rec(clo[LL, :, slot])
+            # RESTOUT leaves the shifted NLO in memory
+            now = mem[s]["nlo"].reshape(nb, kc, nc)[:, :, slot]
+            np.testing.assert_array_equal(
+                now, np.maximum(nlo0[:, :, slot] - N, -ntscr[:, None]))
+    for L in range(2, LA + 1):
+        rec([h["QSUME"][L - 1]] + list(k["QSUM"][:kc, L - 1]))
+    assert next(it, None) is None
+
+
+def test_restart_records_of_50_values(oracle_mod):
+    """KC = 60: layer records wrap after 50 values, as FORMAT(50E17.8)."""
+    from efdc_amd import grids
+    case = grids.tidal_basin(5, 4, 60, seed=2)
+    o = oracle_mod.Oracle(case)
+    rng = np.random.default_rng(4)
+    host = _host_arrays(case, rng)
+    txt = o.restart_text(1, 0.0, **host)
+    lines = txt.split(b"\n")[2:-1]
+    per_cell = 2 + 4 * 2 + 5 * 2
+    assert set(len(x) for x in lines[2:10]) <= {50 * 17, 10 * 17, 11 * 17, 4 * 17, 5 * 17}
+    assert len(lines) == (case.grid.LA - 1) * (per_cell + 2)
+
+
+def test_ee_files_as_getefdc_reads_them(oracle_mod, tmp_path):
+    from efdc_amd import formats
+    case, o = _restart_case(oracle_mod)
+    g = case.grid
+    rng = np.random.default_rng(8)
+    rs = [rng.uniform(0, 1, g.LCM) for _ in range(4)]
+    ksz = np.ones(g.LCM, np.int32)
+    ksz[3:9] = 2
+    o.ee_open(str(tmp_path), 12, 7, rs, igridv=1, ksz=ksz)
+    for n in (10, 20):
+        o.ee_append(str(tmp_path), n, n * 10.0 / 86400, 10.0, 3)
+    ws = formats.read_ee_ws(str(tmp_path / "EE_WS.OUT"))
+    vel = formats.read_ee_vel(str(tmp_path / "EE_VEL.OUT"))
+    nact = g.LA - 1
+    cell3d = sum(g.KC - ksz[L - 1] + 1 for L in range(2, g.LA + 1))
+    assert (ws["ver"], ws["hsize"], ws["bsize"], ws["nactive"]) == (8400, 24, (5 + nact) * 4, nact)
+    assert os.path.getsize(tmp_path / "EE_WS.OUT") == ws["hsize"] + 2 * ws["bsize"]
+    assert (vel["hsize"], vel["bsize"], vel["cell3d"], vel["lcm"]) == (
+        (9 + 4 * g.LCM) * 4, (4 + 3 * cell3d) * 4, cell3d, g.LCM)
+    assert os.path.getsize(tmp_path / "EE_VEL.OUT") == vel["hsize"] + 2 * vel["bsize"]
+    np.testing.assert_array_equal(vel["rssbc"], np.array(rs, dtype=np.float32))
+    r = ws["records"][1]
+    assert (r["n"], r["delt"], r["lminstep"]) == (20, 10.0, 3)
+    np.testing.assert_array_equal(r["hp"], case.h["HP"][1:g.LA].astype(np.float32))
+    u = np.concatenate([case.k["U"][ksz[L - 1] - 1:g.KC, L - 1] for L in range(2, g.LA + 1)])
+    wv = np.concatenate([case.w["W"][ksz[L - 1]:g.KC + 1, L - 1] for L in range(2, g.LA + 1)])
+    np.testing.assert_array_equal(vel["records"][0]["u"], u.astype(np.float32))
+    np.testing.assert_array_equal(vel["records"][0]["w"], wv.astype(np.float32))
+
+
+def test_pow10_table_matches_its_generator():
+    """efdc_pow10.h is what gen_pow10.py writes from exact rationals (the
+    generator also checks every pair orders the doubles as 10^j does)."""
+    import sys
+    gen = os.path.join(ROOT, "efdcplus8.5_amd", "csrc", "gen_pow10.py")
+    assert subprocess.run([sys.executable, gen, "--check"]).returncode == 0
