@@ -704,9 +704,11 @@ def ensemble_bench(args):
                    sample=f"3 full steps of each of the first {nb_done} basins by the FP64 C "
                           f"restatement (OpenMP), {t_cpu:.1f} s")
 
-    # ---- timed region (the first warm-up step runs the autotuner)
+    # ---- timed region (the first warm-up step runs the autotuner, the CG
+    # tile trials among them)
     n0 = nseq + 1
     ctx.autotune(True)
+    ctx.set_segment_tile(0)
     for n in range(n0, n0 + max(args.warmup, 1)):
         ctx.step(case.dt, n)
     n0 += max(args.warmup, 1)
@@ -758,7 +760,7 @@ def ensemble_bench(args):
                      "avg_us": ms / cnt * 1e3, "bytes_per_launch": total_bytes[dom] / cnt},
         "cpu_baseline": cpu,
         "detail": {
-            "autotune": ctx.tune_report(),
+            "autotune": ctx.tune_report(), "cg_tile_cells": ctx.segment_tile(),
             "stream_copy_gbs": stream_copy_gbs(ctx),
             "sequential_value": int(sizes.sum()) * nseq / t_seq,
             "sequential_sample": f"{nseq} steps of each basin in its own context, one "

@@ -345,6 +345,8 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   uvw_free(c);
   bal_free(c);
   io_free(c);
+  cg_graphs_free(c);
+  dfree(c->tune_psave);
   if (c->st_host) (void)hipHostFree(c->st_host);
   if (c->pcg_abort_host) (void)hipHostFree(c->pcg_abort_host);
   dfree(c->gat_off);
@@ -916,10 +918,25 @@ int efdc_set_sources(efdc_ctx_t *c, const efdc_sources_t *src) {
   return EFDC_OK;
 }
 
+int efdc_set_cg_replay(efdc_ctx_t *c, int32_t mode) {
+  if (!c || mode < 0 || mode > 2) return EFDC_ERR_ARG;
+  c->cg_graph = mode == 1;
+  c->cg_graph_tune = mode == 2;
+  return EFDC_OK;
+}
+
+int efdc_get_cg_replay(efdc_ctx_t *c, int32_t *mode) {
+  if (!c || !mode) return EFDC_ERR_ARG;
+  *mode = c->cg_graph_tune ? 2 : (c->cg_graph ? 1 : 0);
+  return EFDC_OK;
+}
+
 int efdc_autotune(efdc_ctx_t *c, int32_t on) {
   if (!c) return EFDC_ERR_ARG;
   c->tune_pending = on != 0;
   c->tune_cg_pending = on != 0;
+  c->cg_graph_tune = on != 0;
+  if (!on) c->cg_graph = false;
   if (on) {
     c->tune_n = 0;
     c->tune_best = -1;

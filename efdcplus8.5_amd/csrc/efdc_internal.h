@@ -117,6 +117,26 @@ struct efdc_ctx {
   int4 *d_tiles = nullptr;     // CG tiles: first cell, end cell, basin, tile in basin
   int *d_segtile0 = nullptr;   // first tile of each basin, nseg+1 entries
   int ntiles = 0;
+  // the tile sets of 256, 512, 1024 and 2048 cells (d_tiles, d_segtile0
+  // and ntiles are those of set seg_tset); 1 (512 cells) is the set whose
+  // reduction order is that of a basin alone
+  int4 *d_tiles_s[4] = {};
+  int *d_segtile0_s[4] = {};
+  int ntiles_s[4] = {};
+  int seg_tset = 1;
+  bool seg_tune_pending = false;
+  double *tune_psave = nullptr;  // P saved for the CG trials (LCM)
+  // replay mode of the FP64 multi-kernel solves: captured graphs or
+  // launch by launch (the default); with autotune on, the next solve times
+  // both from the same P and keeps the faster
+  bool cg_graph = false;
+  bool cg_graph_tune = false;
+  // captured runs of CG iterations replayed by the FP64 multi-kernel
+  // solves (0, 1, 2: one model, 8 iterations and one from the even and
+  // the odd direction buffer; 3, 4, 5: the same for basins); key = the
+  // launch arguments
+  hipGraphExec_t cg_gexec[6] = {};
+  std::vector<unsigned char> cg_gkey[6];
   CgState *d_segst = nullptr;  // per-basin CG state
   CgState *h_segst = nullptr;  // pinned mirror
   unsigned *d_segtk = nullptr; // per-basin arrival counters, TICKET_PAD apart
@@ -566,6 +586,10 @@ int uvw_run(efdc_ctx *c, const efdc_step_t *step, const efdc_uvw_opts_t *o);
 int bal_before_tran(efdc_ctx *c, const efdc_step_t *step);
 int bal_after_tran(efdc_ctx *c, const efdc_step_t *step);
 void bal_free(efdc_ctx *c);
+// basins (efdc_seg.hip): make tile set q (256 << q cells) the current one
+void seg_select_tiles(efdc_ctx *c, int q);
+// captured CG iterations (efdc_puv.hip)
+void cg_graphs_free(efdc_ctx *c);
 // restart and EE output (efdc_io.hip)
 void io_free(efdc_ctx *c);
 int cg_run(efdc_ctx *c, bool check_sym, efdc_cg_stats_t *stats);

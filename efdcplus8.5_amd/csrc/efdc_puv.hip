@@ -15,6 +15,8 @@
 // has been met, so the host only polls every few iterations.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <utility>
@@ -443,10 +445,11 @@ __device__ __forceinline__ Row row_of(const CgArgs &a, int i, bool fast) {
 // the single-model kernels (tile t: i0 = 1 + t*CG_CPT*EFDC_BLOCK) and the
 // per-basin kernels (tiles counted from each basin's first cell), so one
 // basin of an ensemble and the same basin alone give the same bits.
+template <int CPT = CG_CPT>
 __device__ __forceinline__ void cg_init_cells(const CgArgs &a, int i0, int i1, bool fast,
                                               double (&v)[2]) {
 #pragma unroll
-  for (int u = 0; u < CG_CPT; ++u) {
+  for (int u = 0; u < CPT; ++u) {
     const int i = i0 + u * EFDC_BLOCK + (int)threadIdx.x;
     if (i < i1) {
       const Row q = row_of(a, i, fast);
@@ -493,11 +496,12 @@ __device__ __forceinline__ double pnew_at(const CgArgs &a, const double *pold,
   return (j > 0 && j != a.lc0) ? a.ccci[j] * a.r[j] + beta * pold[j] : 0.0;
 }
 
+template <int CPT = CG_CPT>
 __device__ __forceinline__ void cg_a_cells(const CgArgs &a, int i0, int i1, bool fast,
                                            double beta, const double *pold, double *pnew,
                                            double (&v)[1]) {
 #pragma unroll
-  for (int u = 0; u < CG_CPT; ++u) {
+  for (int u = 0; u < CPT; ++u) {
     const int i = i0 + u * EFDC_BLOCK + (int)threadIdx.x;
     if (i < i1) {
       const Row q = row_of(a, i, fast);
@@ -531,10 +535,11 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_cg_a(CgArgs a,
   }
 }
 
+template <int CPT = CG_CPT>
 __device__ __forceinline__ void cg_b_cells(const CgArgs &a, int i0, int i1, double alpha,
                                            const double *pnew, double (&v)[2]) {
 #pragma unroll
-  for (int u = 0; u < CG_CPT; ++u) {
+  for (int u = 0; u < CPT; ++u) {
     const int i = i0 + u * EFDC_BLOCK + (int)threadIdx.x;
     if (i < i1) {
       a.x[i] = a.x[i] + alpha * pnew[i];       // :132
@@ -628,13 +633,14 @@ __device__ __forceinline__ bool seg_sum(const SegArgs &g, int b, int seg, double
   return true;
 }
 
+template <int CPT>
 __global__ __launch_bounds__(EFDC_BLOCK) void k_scg_init(CgArgs a, SegArgs g, double rsqm,
                                                          double rel_tol) {
   const int b = blockIdx.x;
   const int4 t = g.tiles[b];
   const bool fast = a.rowmajor && (a.st->symok == 0);
   double v[2] = {0.0, 0.0};
-  cg_init_cells(a, t.x, t.y, fast, v);
+  cg_init_cells<CPT>(a, t.x, t.y, fast, v);
   double s[2];
   if (seg_sum<2>(g, b, t.z, v, s) && threadIdx.x == 0) {
     CgState *st = g.segst + t.z;
@@ -643,6 +649,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_scg_init(CgArgs a, SegArgs g, do
   }
 }
 
+template <int CPT>
 __global__ __launch_bounds__(EFDC_BLOCK) void k_scg_a(CgArgs a, SegArgs g, const double *pold,
                                                       double *pnew) {
   const int b = blockIdx.x;
@@ -651,7 +658,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_scg_a(CgArgs a, SegArgs g, const
   if (st->done) return;
   const bool fast = a.rowmajor && (a.st->symok == 0);
   double v[1] = {0.0};
-  cg_a_cells(a, t.x, t.y, fast, st->beta, pold, pnew, v);
+  cg_a_cells<CPT>(a, t.x, t.y, fast, st->beta, pold, pnew, v);
   double s[1];
   if (seg_sum<1>(g, b, t.z, v, s) && threadIdx.x == 0) {
     st->papcg = s[0];
@@ -659,6 +666,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_scg_a(CgArgs a, SegArgs g, const
   }
 }
 
+template <int CPT>
 __global__ __launch_bounds__(EFDC_BLOCK) void k_scg_b(CgArgs a, SegArgs g, const double *pnew,
                                                       int iterm) {
   const int b = blockIdx.x;
@@ -666,7 +674,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_scg_b(CgArgs a, SegArgs g, const
   CgState *st = g.segst + t.z;
   if (st->done) return;
   double v[2] = {0.0, 0.0};
-  cg_b_cells(a, t.x, t.y, st->alpha, pnew, v);
+  cg_b_cells<CPT>(a, t.x, t.y, st->alpha, pnew, v);
   double s[2];
   if (seg_sum<2>(g, b, t.z, v, s) && threadIdx.x == 0) {
     if (cg_b_state(st, s, iterm)) atomic_inc(g.segdone);
@@ -1195,6 +1203,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_dry_mask(HPtr h, const int *le, 
 
 CgArgs cg_args(efdc_ctx *c) {
   CgArgs a;
+  std::memset(&a, 0, sizeof a);  // the bytes are a graph key
   a.ccc = c->dH[EFDC_CCC];
   a.ccs = c->dH[EFDC_CCS];
   a.ccw = c->dH[EFDC_CCW];
@@ -1225,26 +1234,174 @@ int read_state(efdc_ctx *c) {
   return EFDC_OK;
 }
 
-int cg_fp64(efdc_ctx *c) {
+// ---------------------------------------------------------------------
+// Captured CG iterations: the FP64 multi-kernel solves replay hipGraphs of
+// 8 and of 1 iterations (two launches each) instead of launching every
+// kernel, the same number of iterations between polls of the solve state
+// as the launch-by-launch loop (the last solve's count first, then 4 at a
+// time).  The 8-iteration graph starts on the even direction buffer and
+// ends on it; the 1-iteration graph is captured for the buffer parity it
+// is replayed at (two graphs).  Iterations after the stop test are
+// launches that return at once (the kernels read the state's done flag),
+// so the results are those of the launch-by-launch loop bit for bit.  A
+// graph is captured again when its launch arguments (the key) change.
+// With the library's timers on, the kernels are launched one by one so
+// that each is timed.
+// ---------------------------------------------------------------------
+constexpr int CG_GRAPH_ITERS = 8;
+
+// replay `n` iterations starting at iteration `it0` with graph slots g8
+// (8 iterations, even start) and g1[2] (one iteration from parity 0 / 1)
+inline int cg_graph_replay(efdc_ctx *c, hipGraphExec_t g8, hipGraphExec_t g1e,
+                           hipGraphExec_t g1o, int it0, int n) {
+  int it = it0, left = n;
+  while (left > 0) {
+    if ((it & 1) == 0 && left >= CG_GRAPH_ITERS) {
+      HIP_TRY(c, hipGraphLaunch(g8, c->stream));
+      it += CG_GRAPH_ITERS;
+      left -= CG_GRAPH_ITERS;
+    } else {
+      HIP_TRY(c, hipGraphLaunch((it & 1) ? g1o : g1e, c->stream));
+      ++it;
+      --left;
+    }
+  }
+  return EFDC_OK;
+}
+
+template <typename F>
+int cg_graph_get(efdc_ctx *c, int slot, const std::vector<unsigned char> &key, F enqueue) {
+  if (c->cg_gexec[slot] && c->cg_gkey[slot] == key) return EFDC_OK;
+  if (c->cg_gexec[slot]) (void)hipGraphExecDestroy(c->cg_gexec[slot]);
+  c->cg_gexec[slot] = nullptr;
+  c->cg_gkey[slot].clear();
+  HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  enqueue();
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (e == hipSuccess) e = hipGraphInstantiate(&c->cg_gexec[slot], g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    c->cg_gexec[slot] = nullptr;
+    c->err = std::string("CG graph capture: ") + hipGetErrorString(e);
+    return EFDC_ERR_HIP;
+  }
+  c->cg_gkey[slot] = key;
+  return EFDC_OK;
+}
+
+template <typename... A>
+void key_of(std::vector<unsigned char> &k, const A &...a) {
+  k.clear();
+  (k.insert(k.end(), (const unsigned char *)&a, (const unsigned char *)&a + sizeof(a)), ...);
+}
+
+// A timed trial of the two replay modes of a solve (`solve(graph)` runs
+// it whole), each from the P the solve starts from; the faster becomes
+// the context's mode and P is restored.
+template <typename F>
+int cg_graph_trial(efdc_ctx *c, F solve) {
+  int s;
+  const size_t bytes = sizeof(double) * c->LCM;
+  if (!c->tune_psave && (s = dev_alloc(c, (void **)&c->tune_psave, bytes))) return s;
+  HIP_TRY(c, hipMemcpyAsync(c->tune_psave, c->dH[EFDC_P], bytes, hipMemcpyDeviceToDevice,
+                            c->stream));
+  hipEvent_t e0, e1;
+  HIP_TRY(c, hipEventCreate(&e0));
+  HIP_TRY(c, hipEventCreate(&e1));
+  float ms[2] = {0.f, 0.f};
+  s = EFDC_OK;
+  for (int g = 0; g < 2 && !s; ++g) {
+    hipError_t e = hipMemcpyAsync(c->dH[EFDC_P], c->tune_psave, bytes, hipMemcpyDeviceToDevice,
+                                  c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+    if (e != hipSuccess) {
+      c->err = hipGetErrorString(e);
+      s = EFDC_ERR_HIP;
+      break;
+    }
+    if ((s = solve(g == 1))) break;
+    e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[g], e0, e1);
+    if (e != hipSuccess) {
+      c->err = hipGetErrorString(e);
+      s = EFDC_ERR_HIP;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (s) return s;
+  c->cg_graph = ms[1] < ms[0];
+  c->cg_graph_tune = false;
+  const char *names[2] = {"cg_fp64 launches", "cg_fp64 graphs"};
+  for (int g = 0; g < 2; ++g)
+    if (c->tune_n < 8) {
+      std::snprintf(c->tune_name[c->tune_n], 32, "%s", names[g]);
+      c->tune_ms[c->tune_n] = ms[g];
+      c->tune_chosen[c->tune_n] = (g == 1) == c->cg_graph ? 1 : 0;
+      ++c->tune_n;
+    }
+  HIP_TRY(c, hipMemcpyAsync(c->dH[EFDC_P], c->tune_psave, bytes, hipMemcpyDeviceToDevice,
+                            c->stream));
+  return EFDC_OK;
+}
+
+int cg_fp64_solve(efdc_ctx *c, bool graph) {
   const int nb = cg_blocks(c->n_wet());
   CgArgs a = cg_args(c);
   const int iterm = c->prm.ITERM;
   TIMED(c, TK_CG_INIT, k_cg_init<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->prm.RSQM, c->prm.cg_rel_tol));
   int it = 0;
-  int chunk = c->last_iters > 0 ? c->last_iters : 8;
-  for (;;) {
-    for (int j = 0; j < chunk && it < iterm; ++j, ++it) {
-      TIMED(c, TK_CG_A, k_cg_a<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->pb[it & 1], c->pb[(it + 1) & 1]));
-      TIMED(c, TK_CG_B, k_cg_b<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->pb[(it + 1) & 1], iterm));
+  if (graph) {
+    std::vector<unsigned char> key;
+    double *p0 = c->pb[0], *p1 = c->pb[1];
+    int s;
+    for (int q = 0; q < 3; ++q) {
+      const int nit = q == 0 ? CG_GRAPH_ITERS : 1, par = q == 2 ? 1 : 0;
+      key_of(key, a, p0, p1, iterm, nb, q);
+      if ((s = cg_graph_get(c, q, key, [&] {
+             for (int j = par; j < par + nit; ++j) {
+               k_cg_a<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->pb[j & 1], c->pb[(j + 1) & 1]);
+               k_cg_b<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->pb[(j + 1) & 1], iterm);
+             }
+           })))
+        return s;
     }
-    int s = read_state(c);
-    if (s) return s;
-    if (c->st_host->done || it >= iterm) break;
-    chunk = 4;
+    int chunk = c->last_iters > 0 ? c->last_iters : 8;
+    for (;;) {
+      const int n = std::min(chunk, iterm - it);
+      if ((s = cg_graph_replay(c, c->cg_gexec[0], c->cg_gexec[1], c->cg_gexec[2], it, n)))
+        return s;
+      it += n;
+      if ((s = read_state(c))) return s;
+      if (c->st_host->done || it >= iterm) break;
+      chunk = 4;
+    }
+  } else {
+    int chunk = c->last_iters > 0 ? c->last_iters : 8;
+    for (;;) {
+      for (int j = 0; j < chunk && it < iterm; ++j, ++it) {
+        TIMED(c, TK_CG_A, k_cg_a<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->pb[it & 1], c->pb[(it + 1) & 1]));
+        TIMED(c, TK_CG_B, k_cg_b<<<nb, EFDC_BLOCK, 0, c->stream>>>(a, c->pb[(it + 1) & 1], iterm));
+      }
+      int s = read_state(c);
+      if (s) return s;
+      if (c->st_host->done || it >= iterm) break;
+      chunk = 4;
+    }
   }
   c->last_iters = c->st_host->iter;
   if (c->timing) c->cg_iter_count += c->st_host->iter;
   return EFDC_OK;
+}
+
+int cg_fp64(efdc_ctx *c) {
+  int s;
+  if (c->cg_graph_tune && !c->timing &&
+      (s = cg_graph_trial(c, [&](bool g) { return cg_fp64_solve(c, g); })))
+    return s;
+  return cg_fp64_solve(c, c->cg_graph && !c->timing);
 }
 
 int cg_mixed(efdc_ctx *c) {
@@ -1349,6 +1506,15 @@ int cg_dispatch(efdc_ctx *c, efdc_cg_stats_t *stats, bool defer = false,
 }
 
 }  // namespace
+
+void cg_graphs_free(efdc_ctx *c) {
+  for (int q = 0; q < 6; ++q) {
+    if (c->cg_gexec[q]) (void)hipGraphExecDestroy(c->cg_gexec[q]);
+    c->cg_gexec[q] = nullptr;
+    c->cg_gkey[q].clear();
+  }
+}
+
 
 int cg_fp64_vec(efdc_ctx *c, double *x, const double *rhs, CgState *st, CgState *st_host,
                 double rel_tol, int itmax, int *iters, double *rsq) {
@@ -1611,13 +1777,11 @@ int seg_min_run(efdc_ctx *c) {
   return EFDC_OK;
 }
 
-int seg_cg_run(efdc_ctx *c, efdc_cg_stats_t *stats) {
-  if (c->prm.cg_mode != EFDC_CG_FP64) {
-    c->err = "independent basins are solved with the FP64 CG (EFDC_CG_FP64)";
-    return EFDC_ERR_ARG;
-  }
-  CgArgs a = cg_args(c);
+namespace {
+
+SegArgs seg_args(efdc_ctx *c) {
   SegArgs g;
+  std::memset(&g, 0, sizeof g);  // the bytes are a graph key
   g.tiles = c->d_tiles;
   g.segtile0 = c->d_segtile0;
   g.segst = c->d_segst;
@@ -1625,37 +1789,166 @@ int seg_cg_run(efdc_ctx *c, efdc_cg_stats_t *stats) {
   g.segdone = c->d_segdone;
   g.part = c->d_segpart;
   g.ntiles = c->ntiles;
-  const int nt = c->ntiles, iterm = c->prm.ITERM;
-  k_seg_reset<<<(c->nseg + 255) / 256, 256, 0, c->stream>>>(c->d_segst, c->nseg, c->d_segdone);
-  TIMED(c, TK_CG_INIT, k_scg_init<<<nt, EFDC_BLOCK, 0, c->stream>>>(a, g, c->prm.RSQM,
-                                                                   c->prm.cg_rel_tol));
-  int it = 0;
-  int chunk = c->last_iters > 0 ? c->last_iters : 8;
-  unsigned done = 0;
-  for (;;) {
-    for (int j = 0; j < chunk && it < iterm; ++j, ++it) {
-      TIMED(c, TK_CG_A, k_scg_a<<<nt, EFDC_BLOCK, 0, c->stream>>>(a, g, c->pb[it & 1],
-                                                                  c->pb[(it + 1) & 1]));
-      TIMED(c, TK_CG_B, k_scg_b<<<nt, EFDC_BLOCK, 0, c->stream>>>(a, g, c->pb[(it + 1) & 1],
-                                                                  iterm));
-    }
-    HIP_TRY(c, hipMemcpyAsync(&done, c->d_segdone, sizeof(unsigned), hipMemcpyDeviceToHost,
-                              c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if ((int)done >= c->nseg || it >= iterm) break;
-    chunk = 4;
+  return g;
+}
+
+template <int CPT>
+void seg_iter(efdc_ctx *c, const CgArgs &a, const SegArgs &g, int j, int iterm, bool timed) {
+  const int nt = g.ntiles;
+  if (timed) {
+    TIMED(c, TK_CG_A, k_scg_a<CPT><<<nt, EFDC_BLOCK, 0, c->stream>>>(a, g, c->pb[j & 1],
+                                                                     c->pb[(j + 1) & 1]));
+    TIMED(c, TK_CG_B, k_scg_b<CPT><<<nt, EFDC_BLOCK, 0, c->stream>>>(a, g, c->pb[(j + 1) & 1],
+                                                                     iterm));
+  } else {
+    k_scg_a<CPT><<<nt, EFDC_BLOCK, 0, c->stream>>>(a, g, c->pb[j & 1], c->pb[(j + 1) & 1]);
+    k_scg_b<CPT><<<nt, EFDC_BLOCK, 0, c->stream>>>(a, g, c->pb[(j + 1) & 1], iterm);
   }
+}
+
+template <int CPT>
+void seg_init(efdc_ctx *c, const CgArgs &a, const SegArgs &g) {
+  k_seg_reset<<<(c->nseg + 255) / 256, 256, 0, c->stream>>>(c->d_segst, c->nseg, c->d_segdone);
+  TIMED(c, TK_CG_INIT, k_scg_init<CPT><<<g.ntiles, EFDC_BLOCK, 0, c->stream>>>(
+      a, g, c->prm.RSQM, c->prm.cg_rel_tol));
+}
+
+// One solve of every basin with tile set q: graph replays (or, with the
+// timers on or during the tile trials, launch by launch) until every
+// basin has stopped or ITERM iterations ran.
+template <int CPT>
+int seg_solve(efdc_ctx *c, bool graph) {
+  const CgArgs a = cg_args(c);
+  const SegArgs g = seg_args(c);
+  const int iterm = c->prm.ITERM;
+  seg_init<CPT>(c, a, g);
+  int it = 0;
+  unsigned done = 0;
+  if (graph) {
+    std::vector<unsigned char> key;
+    double *p0 = c->pb[0], *p1 = c->pb[1];
+    const int cpt = CPT;
+    int s;
+    for (int q = 0; q < 3; ++q) {
+      const int nit = q == 0 ? CG_GRAPH_ITERS : 1, par = q == 2 ? 1 : 0;
+      key_of(key, a, g, p0, p1, iterm, cpt, q);
+      if ((s = cg_graph_get(c, 3 + q, key, [&] {
+             for (int j = par; j < par + nit; ++j) seg_iter<CPT>(c, a, g, j, iterm, false);
+           })))
+        return s;
+    }
+    int chunk = c->last_iters > 0 ? c->last_iters : 8;
+    for (;;) {
+      const int n = std::min(chunk, iterm - it);
+      if ((s = cg_graph_replay(c, c->cg_gexec[3], c->cg_gexec[4], c->cg_gexec[5], it, n)))
+        return s;
+      it += n;
+      HIP_TRY(c, hipMemcpyAsync(&done, c->d_segdone, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                c->stream));
+      HIP_TRY(c, hipStreamSynchronize(c->stream));
+      if ((int)done >= c->nseg || it >= iterm) break;
+      chunk = 4;
+    }
+  } else {
+    int chunk = c->last_iters > 0 ? c->last_iters : 8;
+    for (;;) {
+      for (int j = 0; j < chunk && it < iterm; ++j, ++it) seg_iter<CPT>(c, a, g, it, iterm, true);
+      HIP_TRY(c, hipMemcpyAsync(&done, c->d_segdone, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                c->stream));
+      HIP_TRY(c, hipStreamSynchronize(c->stream));
+      if ((int)done >= c->nseg || it >= iterm) break;
+      chunk = 4;
+    }
+  }
+  HIP_TRY(c, hipGetLastError());
+  return EFDC_OK;
+}
+
+int seg_solve_set(efdc_ctx *c, int q, bool graph) {
+  switch (q) {
+    case 0: return seg_solve<1>(c, graph);
+    case 1: return seg_solve<2>(c, graph);
+    case 2: return seg_solve<4>(c, graph);
+    default: return seg_solve<8>(c, graph);
+  }
+}
+
+// The tile trials: each set solves the same system from the same P, timed
+// with events on the context stream; the fastest becomes the current set.
+int seg_tune_tiles(efdc_ctx *c) {
+  int s;
+  if (!c->tune_psave && (s = dev_alloc(c, (void **)&c->tune_psave, sizeof(double) * c->LCM)))
+    return s;
+  const size_t bytes = sizeof(double) * c->LCM;
+  HIP_TRY(c, hipMemcpyAsync(c->tune_psave, c->dH[EFDC_P], bytes, hipMemcpyDeviceToDevice,
+                            c->stream));
+  hipEvent_t e0, e1;
+  HIP_TRY(c, hipEventCreate(&e0));
+  HIP_TRY(c, hipEventCreate(&e1));
+  float best = 1e30f;
+  int bq = c->seg_tset;
+  const bool timing = c->timing;
+  c->timing = false;
+  for (int q = 0; q < 4; ++q) {
+    HIP_TRY(c, hipMemcpyAsync(c->dH[EFDC_P], c->tune_psave, bytes, hipMemcpyDeviceToDevice,
+                              c->stream));
+    seg_select_tiles(c, q);
+    HIP_TRY(c, hipEventRecord(e0, c->stream));
+    if ((s = seg_solve_set(c, q, false))) break;
+    HIP_TRY(c, hipEventRecord(e1, c->stream));
+    HIP_TRY(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+    if (c->tune_n < 8) {
+      std::snprintf(c->tune_name[c->tune_n], 32, "seg_cg_tile_%d", EFDC_BLOCK << q);
+      c->tune_ms[c->tune_n] = ms;
+      c->tune_chosen[c->tune_n] = 0;
+      ++c->tune_n;
+    }
+    if (ms < best) {
+      best = ms;
+      bq = q;
+    }
+  }
+  c->timing = timing;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (s) return s;
+  for (int j = 0; j < c->tune_n; ++j)
+    if (std::strncmp(c->tune_name[j], "seg_cg_tile_", 12) == 0)
+      c->tune_chosen[j] = std::atoi(c->tune_name[j] + 12) == (EFDC_BLOCK << bq) ? 1 : 0;
+  seg_select_tiles(c, bq);
+  HIP_TRY(c, hipMemcpyAsync(c->dH[EFDC_P], c->tune_psave, bytes, hipMemcpyDeviceToDevice,
+                            c->stream));
+  c->seg_tune_pending = false;
+  return EFDC_OK;
+}
+
+}  // namespace
+
+int seg_cg_run(efdc_ctx *c, efdc_cg_stats_t *stats) {
+  if (c->prm.cg_mode != EFDC_CG_FP64) {
+    c->err = "independent basins are solved with the FP64 CG (EFDC_CG_FP64)";
+    return EFDC_ERR_ARG;
+  }
+  int s;
+  if (c->seg_tune_pending && (s = seg_tune_tiles(c))) return s;
+  if (c->cg_graph_tune && !c->timing &&
+      (s = cg_graph_trial(c, [&](bool g) { return seg_solve_set(c, c->seg_tset, g); })))
+    return s;
+  if ((s = seg_solve_set(c, c->seg_tset, c->cg_graph && !c->timing))) return s;
   HIP_TRY(c, hipMemcpyAsync(c->h_segst, c->d_segst, sizeof(CgState) * c->nseg,
                             hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   int imax = 0, bad = 0;
   double rsq = 0.0;
-  for (int s = 0; s < c->nseg; ++s) {
-    imax = std::max(imax, c->h_segst[s].iter);
-    rsq = std::max(rsq, c->h_segst[s].rsq);
-    if (c->h_segst[s].status != 0) bad = 1;
+  for (int q = 0; q < c->nseg; ++q) {
+    imax = std::max(imax, c->h_segst[q].iter);
+    rsq = std::max(rsq, c->h_segst[q].rsq);
+    if (c->h_segst[q].status != 0) bad = 1;
   }
   c->last_iters = imax;
+  if (c->timing) c->cg_iter_count += imax;
   if (stats) {
     stats->iters = imax;
     stats->outer = 1;

@@ -16,8 +16,6 @@
 
 namespace {
 
-// CG tile of the multi-kernel schedule
-constexpr int SEG_TILE = CG_CPT * EFDC_BLOCK;
 
 // FP64 volume and mass of one basin: thread t sums the basin's cells
 // t, t+B, ... in order, then a fixed tree.  mass: con == nullptr -> volume.
@@ -45,12 +43,26 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_seg_balance(
 
 }  // namespace
 
+void seg_select_tiles(efdc_ctx *c, int q) {
+  c->seg_tset = q;
+  c->d_tiles = c->d_tiles_s[q];
+  c->d_segtile0 = c->d_segtile0_s[q];
+  c->ntiles = c->ntiles_s[q];
+}
+
 void seg_free(efdc_ctx *c) {
   auto f = [](auto *&p) {
     if (p) (void)hipFree((void *)p);
     p = nullptr;
   };
-  f(c->d_segid); f(c->d_segfirst); f(c->d_tiles); f(c->d_segtile0); f(c->d_segst);
+  for (int q = 0; q < 4; ++q) {
+    f(c->d_tiles_s[q]);
+    f(c->d_segtile0_s[q]);
+    c->ntiles_s[q] = 0;
+  }
+  c->d_tiles = nullptr;
+  c->d_segtile0 = nullptr;
+  f(c->d_segid); f(c->d_segfirst); f(c->d_segst);
   f(c->d_segtk); f(c->d_segdone); f(c->d_segmin); f(c->d_segpart); f(c->d_segbal);
   if (c->h_segst) (void)hipHostFree(c->h_segst);
   c->h_segst = nullptr;
@@ -94,37 +106,46 @@ int efdc_set_segments(efdc_ctx_t *c, int32_t nseg, const int32_t *first_L) {
                  std::to_string(i + 1);
         return EFDC_ERR_ARG;
       }
-  // CG tiles counted from each basin's first cell
-  std::vector<int4> tiles;
-  std::vector<int> tile0(nseg + 1);
-  for (int s = 0; s < nseg; ++s) {
-    tile0[s] = (int)tiles.size();
-    int t = 0;
-    for (int i0 = first[s]; i0 < first[s + 1]; i0 += SEG_TILE, ++t)
-      tiles.push_back(make_int4(i0, std::min(i0 + SEG_TILE, first[s + 1]), s, t));
-  }
-  tile0[nseg] = (int)tiles.size();
-  c->ntiles = (int)tiles.size();
   c->nseg = nseg;
   c->seg_first = first;
-  int st;
+  int st = EFDC_OK;
   auto up = [&](auto **dst, const auto *src, size_t n) -> int {
     if ((st = dev_alloc(c, (void **)dst, sizeof(*src) * n))) return st;
     // on the context's (non-blocking) stream, after dev_alloc's zero fill
     HIP_TRY(c, hipMemcpyAsync(*dst, src, sizeof(*src) * n, hipMemcpyHostToDevice, c->stream));
     return EFDC_OK;
   };
-  if (up(&c->d_segid, segid.data(), LCM) || up(&c->d_segfirst, first.data(), nseg + 1) ||
-      up(&c->d_tiles, tiles.data(), tiles.size()) ||
-      up(&c->d_segtile0, tile0.data(), nseg + 1)) {
+  if (up(&c->d_segid, segid.data(), LCM) || up(&c->d_segfirst, first.data(), nseg + 1)) {
     seg_free(c);
     return st ? st : EFDC_ERR_HIP;
   }
+  // CG tiles of 256 << q cells counted from each basin's first cell
+  int maxtiles = 0;
+  for (int q = 0; q < 4; ++q) {
+    const int tile = EFDC_BLOCK << q;
+    std::vector<int4> tiles;
+    std::vector<int> tile0(nseg + 1);
+    for (int s = 0; s < nseg; ++s) {
+      tile0[s] = (int)tiles.size();
+      int t = 0;
+      for (int i0 = first[s]; i0 < first[s + 1]; i0 += tile, ++t)
+        tiles.push_back(make_int4(i0, std::min(i0 + tile, first[s + 1]), s, t));
+    }
+    tile0[nseg] = (int)tiles.size();
+    c->ntiles_s[q] = (int)tiles.size();
+    maxtiles = std::max(maxtiles, c->ntiles_s[q]);
+    if (up(&c->d_tiles_s[q], tiles.data(), tiles.size()) ||
+        up(&c->d_segtile0_s[q], tile0.data(), nseg + 1)) {
+      seg_free(c);
+      return st ? st : EFDC_ERR_HIP;
+    }
+  }
+  seg_select_tiles(c, c->seg_tset);
   if ((st = dev_alloc(c, (void **)&c->d_segst, sizeof(CgState) * nseg)) ||
       (st = dev_alloc(c, (void **)&c->d_segtk, sizeof(unsigned) * TICKET_PAD * nseg)) ||
       (st = dev_alloc(c, (void **)&c->d_segdone, sizeof(unsigned))) ||
       (st = dev_alloc(c, (void **)&c->d_segmin, sizeof(double) * nseg)) ||
-      (st = dev_alloc(c, (void **)&c->d_segpart, sizeof(double) * 2 * c->ntiles)) ||
+      (st = dev_alloc(c, (void **)&c->d_segpart, sizeof(double) * 2 * maxtiles)) ||
       (st = dev_alloc(c, (void **)&c->d_segbal, sizeof(double) * nseg * (EFDC_MAXC + 1)))) {
     seg_free(c);
     return st;
@@ -137,6 +158,29 @@ int efdc_set_segments(efdc_ctx_t *c, int32_t nseg, const int32_t *first_L) {
   std::memset(c->h_segst, 0, sizeof(CgState) * nseg);
   c->last_iters = 0;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
+int efdc_set_segment_tile(efdc_ctx_t *c, int32_t cells) {
+  if (!c) return EFDC_ERR_ARG;
+  if (cells == 0) {
+    c->seg_tune_pending = true;
+    return EFDC_OK;
+  }
+  for (int q = 0; q < 4; ++q)
+    if (cells == (EFDC_BLOCK << q)) {
+      c->seg_tune_pending = false;
+      if (c->nseg > 1) seg_select_tiles(c, q);
+      else c->seg_tset = q;
+      return EFDC_OK;
+    }
+  c->err = "efdc_set_segment_tile: cells must be 0, 256, 512, 1024 or 2048";
+  return EFDC_ERR_ARG;
+}
+
+int efdc_get_segment_tile(efdc_ctx_t *c, int32_t *cells) {
+  if (!c || !cells) return EFDC_ERR_ARG;
+  *cells = EFDC_BLOCK << c->seg_tset;
   return EFDC_OK;
 }
 

@@ -190,6 +190,10 @@ def load(path: Optional[str] = None):
         "efdc_calconc": ([P, C.POINTER(Step)], C.c_int),
         "efdc_caluvw": ([P, C.POINTER(Step), C.POINTER(UvwOpts)], C.c_int),
         "efdc_bal2t_begin": ([P], C.c_int),
+        "efdc_set_segment_tile": ([P, C.c_int32], C.c_int),
+        "efdc_set_cg_replay": ([P, C.c_int32], C.c_int),
+        "efdc_get_cg_replay": ([P, C.POINTER(C.c_int32)], C.c_int),
+        "efdc_get_segment_tile": ([P, C.POINTER(C.c_int32)], C.c_int),
         "efdc_restart_text": ([P, C.POINTER(_fmt.Restart), C.c_char_p, C.c_int64,
                                C.POINTER(C.c_int64)], C.c_int),
         "efdc_restart_write": ([P, C.POINTER(_fmt.Restart), C.c_char_p], C.c_int),
@@ -594,6 +598,26 @@ class Context:
         a = np.ascontiguousarray(first_L, np.int32)
         self._check(self.lib.efdc_set_segments(self.h, len(a), _pi(a)))
         self.nseg = len(a)
+
+    def set_segment_tile(self, cells: int):
+        """CG tile of the basins: 256, 512 (bitwise, the default), 1024,
+        2048, or 0 to time the four on the next solve."""
+        self._check(self.lib.efdc_set_segment_tile(self.h, cells))
+
+    def set_cg_replay(self, mode: int):
+        """0 launch by launch, 1 captured graphs, 2 time both on the next
+        solve (FP64 multi-kernel CG)."""
+        self._check(self.lib.efdc_set_cg_replay(self.h, mode))
+
+    def cg_replay(self) -> int:
+        v = C.c_int32(0)
+        self._check(self.lib.efdc_get_cg_replay(self.h, C.byref(v)))
+        return v.value
+
+    def segment_tile(self) -> int:
+        v = C.c_int32(0)
+        self._check(self.lib.efdc_get_segment_tile(self.h, C.byref(v)))
+        return v.value
 
     def segment_stats(self):
         """Per-basin CG iterations and status of the last solve."""

@@ -324,6 +324,30 @@ module efdc_hip
       type(efdc_bal2t_t), intent(out) :: out
     end function
 
+    integer(c_int) function efdc_set_cg_replay(ctx, mode) bind(C, name="efdc_set_cg_replay")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: mode
+    end function
+
+    integer(c_int) function efdc_get_cg_replay(ctx, mode) bind(C, name="efdc_get_cg_replay")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: mode
+    end function
+
+    integer(c_int) function efdc_set_segment_tile(ctx, cells) bind(C, name="efdc_set_segment_tile")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: cells
+    end function
+
+    integer(c_int) function efdc_get_segment_tile(ctx, cells) bind(C, name="efdc_get_segment_tile")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: cells
+    end function
+
     integer(c_int) function efdc_restart_text(ctx, r, buf, cap, bytes) &
         bind(C, name="efdc_restart_text")
       import :: c_int, c_ptr, c_int64_t, efdc_restart_t

@@ -560,6 +560,26 @@ int efdc_caluvw(efdc_ctx_t *ctx, const efdc_step_t *step, const efdc_uvw_opts_t 
    addition, for hosts that run many basins. */
 int efdc_set_segments(efdc_ctx_t *ctx, int32_t nseg, const int32_t *first_L);
 
+/* CG tile of the basins: each basin's cells are cut into tiles of `cells`
+   (256, 512, 1024 or 2048) whose partial sums the basin's reduction adds in
+   order.  512, the default, is the tile of the single-model solve, so each
+   basin's results are bit for bit those of the basin alone; other sizes
+   sum in another order (results within round-off).  cells = 0 times the
+   four sizes on the next solve (each from the same P) and keeps the
+   fastest; efdc_get_segment_tile gives the size in use and
+   efdc_tune_report lists the trial times. */
+int efdc_set_segment_tile(efdc_ctx_t *ctx, int32_t cells);
+int efdc_get_segment_tile(efdc_ctx_t *ctx, int32_t *cells);
+
+/* How the FP64 multi-kernel solves (one model and basins) issue their
+   iterations: 0 launch by launch (the default), 1 by replaying captured
+   hipGraphs of 8 and 1 iterations, 2 time both on the next solve from the
+   same P and keep the faster (efdc_autotune(ctx, 1) does the same).  The
+   two give the same bits; with the library's timers on the solves launch
+   kernel by kernel. */
+int efdc_set_cg_replay(efdc_ctx_t *ctx, int32_t mode);
+int efdc_get_cg_replay(efdc_ctx_t *ctx, int32_t *mode);
+
 /* Per-basin results of the last efdc_calpuv2c / efdc_congrad with
    segments: iters[s] and status[s] (0 or EFDC_ERR_CG_MAXITER) for s <
    min(nseg, cap).  The stats of the call itself report the largest

@@ -169,3 +169,93 @@ def test_single_segment_is_the_plain_model():
     assert eta_err(a.get("P"), b.get("P")) == 0.0
     a.close()
     b.close()
+
+
+def test_captured_cg_iterations_equal_launch_by_launch():
+    """The FP64 multi-kernel solves replaying captured graphs of 8 and 1
+    CG iterations give the bits of the launch-by-launch loop, for one model
+    and for basins; the timed trial of the two keeps one of them and
+    reports both."""
+    _, ens, first = _tidal_ensemble()
+    for segs in (False, True):
+        a, b, t = _ctx(ens), _ctx(ens), _ctx(ens)
+        if segs:
+            for x in (a, b, t):
+                x.set_segments(first)
+        b.set_cg_replay(1)
+        t.set_cg_replay(2)
+        assert (a.cg_replay(), b.cg_replay(), t.cg_replay()) == (0, 1, 2)
+        for n in range(1, 6):
+            ia = a.step(ens.dt, n)["iters"]
+            ib = b.step(ens.dt, n)["iters"]
+            it = t.step(ens.dt, n)["iters"]
+            assert ia == ib == it, (segs, n)
+        assert t.cg_replay() in (0, 1)
+        rep = t.tune_report()
+        assert "cg_fp64 launches" in rep["shapes"] and "cg_fp64 graphs" in rep["shapes"]
+        for name in ("P", "HP", "UHDYE", "VHDXE"):
+            assert np.array_equal(a.get(name), b.get(name)), (segs, name)
+            assert np.array_equal(a.get(name), t.get(name)), (segs, name)
+        for x in (a, b, t):
+            x.close()
+
+
+def test_segment_tiles_and_their_trials():
+    """Tiles of 256, 1024 and 2048 cells sum the basins' dot products in
+    another order: within round-off of the 512-cell tile (which is the
+    basins-alone order); cells = 0 times the four sizes on the next solve,
+    keeps the fastest and reports the trials."""
+    _, ens, first = _tidal_ensemble()
+    ref = _ctx(ens)
+    ref.set_segments(first)
+    assert ref.segment_tile() == 512
+    for n in range(1, 9):
+        ref.step(ens.dt, n)
+    p_ref = ref.get("P")
+    L = np.arange(1, ens.grid.LA)
+    for cells in (256, 1024, 2048, 0):
+        ctx = _ctx(ens)
+        ctx.set_segments(first)
+        ctx.set_segment_tile(cells)
+        for n in range(1, 9):
+            ctx.step(ens.dt, n)
+        if cells:
+            assert ctx.segment_tile() == cells
+        else:
+            assert ctx.segment_tile() in (256, 512, 1024, 2048)
+            rep = ctx.tune_report()
+            trials = [k for k in rep["shapes"] if k.startswith("seg_cg_tile_")]
+            assert len(trials) == 4
+            chosen = [k for k in rep["chosen"] if k.startswith("seg_cg_tile_")]
+            assert chosen == [f"seg_cg_tile_{ctx.segment_tile()}"]
+        assert eta_err(ctx.get("P")[L], p_ref[L]) <= 1e-9, cells
+        ctx.close()
+    with pytest.raises(Exception):
+        ref.set_segment_tile(300)
+    ref.close()
+
+
+def test_256_basins_each_equal_to_the_basin_alone():
+    """256 basins of 8 shapes in one context, 3 full steps: a sample of
+    them, run alone, give the same bits (the 512-cell tile)."""
+    from efdc_amd import grids
+    basins = [grids.tidal_basin(12 + 4 * (i % 4), 10 + 3 * (i % 2), 2, seed=300 + i,
+                                amp=0.05 + 0.003 * (i % 50), dye_load=0.5 + (i % 7))
+              for i in range(256)]
+    ens, first = grids.concat_basins(basins)
+    ctx = _ctx(ens)
+    ctx.set_segments(first)
+    for n in range(1, 4):
+        ctx.step(ens.dt, n)
+    p = ctx.get("P")
+    hp = ctx.get("HP")
+    for b in (0, 1, 77, 128, 200, 254, 255):
+        s = _ctx(basins[b])
+        for n in range(1, 4):
+            s.step(basins[b].dt, n)
+        sl = grids.basin_slice(ens, first, b)
+        nw = basins[b].grid.nwet
+        assert np.array_equal(p[sl], s.get("P")[1:1 + nw]), b
+        assert np.array_equal(hp[sl], s.get("HP")[1:1 + nw]), b
+        s.close()
+    ctx.close()

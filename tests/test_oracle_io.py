@@ -184,8 +184,8 @@ def _harness_lib():
                 [HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-fPIC", "-shared",
                  "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",
                  "-packed-fp32-ops", "-I", os.path.join(ROOT, "efdcplus8.5_amd", "csrc"), src,
-                 "-o", so + ".tmp"], stderr=subprocess.DEVNULL)
-            os.replace(so + ".tmp", so)
+                 "-o", f"{so}.{os.getpid()}.tmp"], stderr=subprocess.DEVNULL)
+            os.replace(f"{so}.{os.getpid()}.tmp", so)
         lib = C.CDLL(so)
         lib.fmt_e17_many.restype = C.c_int64
         lib.fmt_e17_many.argtypes = [C.c_void_p, C.c_int64, C.c_char_p, C.c_void_p]
