@@ -37,6 +37,7 @@ struct CgState {
   double rsq64, bnorm2_in, rz_in;  // mixed mode: outer residual, inner rhs
   int iter, done, status, symok;   // symok: coefficients verified symmetric
   int negdep, outer, inner_iter, drychg;  // drychg: a cell dried in this pass
+  int fin, pad_;                   // basins' mixed solve: outer rounds ended
 };
 
 // Arrival counters of the grid reductions: one site per reduction kernel,
@@ -141,6 +142,7 @@ struct efdc_ctx {
   CgState *h_segst = nullptr;  // pinned mirror
   unsigned *d_segtk = nullptr; // per-basin arrival counters, TICKET_PAD apart
   unsigned *d_segdone = nullptr;  // basins whose solve has ended
+  unsigned *d_segfin = nullptr;   // basins whose FP32-refined solve has ended
   double *d_segmin = nullptr;  // per-basin CCMNM
   double *d_segpart = nullptr; // CG partials, 2 x ntiles
   double *d_segbal = nullptr;  // per-basin balance sums
@@ -561,7 +563,7 @@ int pcg_solve(efdc_ctx *c, bool mixed, bool defer = false);
 int pcg_tune_degree(efdc_ctx *c);
 // independent basins (efdc_seg.hip)
 int seg_min_run(efdc_ctx *c);
-int seg_cg_run(efdc_ctx *c, efdc_cg_stats_t *stats);
+int seg_cg_run(efdc_ctx *c, efdc_cg_stats_t *stats, bool *ran_mixed = nullptr);
 void seg_free(efdc_ctx *c);
 // discrete adjoint (efdc_adj.hip): forward-step records and clean-up
 bool adj_recording(const efdc_ctx *c);

@@ -734,6 +734,67 @@ struct MxArgs {
   float *r32, *y32, *ap32;
 };
 
+// Per-cell work of the mixed solve, shared by the single-model kernels and
+// the per-basin ones.
+__device__ __forceinline__ void mx_prep_cell(const MxArgs &m, int i, double (&v)[1]) {
+  const CgArgs &a = m.g;
+  const double di = 1.0 / sqrt(a.ccc[i]);
+  m.dis[i] = di;
+  const int js = a.ls[i];
+  const double ds = (js != a.lc0) ? 1.0 / sqrt(a.ccc[js]) : 0.0;
+  const double dw = (i - 1 > 0) ? 1.0 / sqrt(a.ccc[i - 1]) : 0.0;
+  const double cs = (js != a.lc0) ? a.ccs[i] : 0.0;
+  m.as[i] = (float)(cs * di * ds);
+  m.aw[i] = (float)(a.ccw[i] * di * dw);
+  v[0] += a.fptmp[i] * a.fptmp[i];
+}
+
+// r64 = FPTMP - A P (FP64, compact operator), r' = D^-1/2 r64 in FP32, y = 0
+__device__ __forceinline__ void mx_resid_cell(const MxArgs &m, int i, double (&v)[2]) {
+  const CgArgs &a = m.g;
+  const Row q = row_of(a, i, true);
+  const double *P = a.x;
+  const double r = a.fptmp[i] - a.ccc[i] * P[i] - q.cn * P[q.jn] -
+                   q.cs * P[q.js] - q.cw * P[q.jw] - q.ce * P[q.je];
+  const float rs = (float)(m.dis[i] * r);
+  m.r32[i] = rs;
+  m.y32[i] = 0.0f;
+  v[0] += r * r;
+  v[1] += (double)rs * (double)rs;
+}
+
+__device__ __forceinline__ float p32_at(const MxArgs &m, const float *pold,
+                                        double beta, int j) {
+  return (j > 0 && j != m.g.lc0)
+             ? (float)((double)m.r32[j] + beta * (double)pold[j])
+             : 0.0f;
+}
+
+__device__ __forceinline__ void mx_a_cell(const MxArgs &m, int i, double beta, const float *pold,
+                                          float *pnew, double (&v)[1]) {
+  const CgArgs &a = m.g;
+  const int js = a.ls[i], jn = a.ln[i];
+  const float pc = p32_at(m, pold, beta, i);
+  const float ps = p32_at(m, pold, beta, js);
+  const float pn = p32_at(m, pold, beta, jn);
+  const float pw = p32_at(m, pold, beta, i - 1);
+  const float pe = p32_at(m, pold, beta, i + 1);
+  const float an = (jn != a.lc0) ? m.as[jn] : 0.0f;
+  const float ae = (i + 1 != a.lc0) ? m.aw[i + 1] : 0.0f;
+  const float apc = pc + m.as[i] * ps + an * pn + m.aw[i] * pw + ae * pe;
+  pnew[i] = pc;
+  m.ap32[i] = apc;
+  v[0] += (double)apc * (double)pc;
+}
+
+__device__ __forceinline__ void mx_b_cell(const MxArgs &m, int i, double alpha, const float *pnew,
+                                          double (&v)[1]) {
+  m.y32[i] = (float)((double)m.y32[i] + alpha * (double)pnew[i]);
+  const float r = (float)((double)m.r32[i] - alpha * (double)m.ap32[i]);
+  m.r32[i] = r;
+  v[0] += (double)r * (double)r;
+}
+
 __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_prep(MxArgs m, double rsqm,
                                                         double rel_tol) {
   const CgArgs &a = m.g;
@@ -742,17 +803,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_prep(MxArgs m, double rsqm,
   #pragma unroll
   for (int u = 0; u < MX_CPT; ++u) {
     const int i = (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
-    if (i >= 1 && i <= a.nwet) {
-      const double di = 1.0 / sqrt(a.ccc[i]);
-      m.dis[i] = di;
-      const int js = a.ls[i];
-      const double ds = (js != a.lc0) ? 1.0 / sqrt(a.ccc[js]) : 0.0;
-      const double dw = (i - 1 > 0) ? 1.0 / sqrt(a.ccc[i - 1]) : 0.0;
-      const double cs = (js != a.lc0) ? a.ccs[i] : 0.0;
-      m.as[i] = (float)(cs * di * ds);
-      m.aw[i] = (float)(a.ccw[i] * di * dw);
-      v[0] += a.fptmp[i] * a.fptmp[i];
-    }
+    if (i >= 1 && i <= a.nwet) mx_prep_cell(m, i, v);
   }
   double s[1];
   if (grid_sum_x<1>(v, a.part, a.tk + TS_MX_PREP * TICKET_SITE, gridDim.x, tile, s) &&
@@ -765,8 +816,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_prep(MxArgs m, double rsqm,
   }
 }
 
-// r64 = FPTMP - A P (FP64, compact operator), r' = D^-1/2 r64 in FP32,
-// y = 0; starts one inner solve.
+// starts one inner solve
 __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_resid(MxArgs m) {
   const CgArgs &a = m.g;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -774,17 +824,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_resid(MxArgs m) {
   #pragma unroll
   for (int u = 0; u < MX_CPT; ++u) {
     const int i = (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
-    if (i >= 1 && i <= a.nwet) {
-      const Row q = row_of(a, i, true);
-      const double *P = a.x;
-      const double r = a.fptmp[i] - a.ccc[i] * P[i] - q.cn * P[q.jn] -
-                       q.cs * P[q.js] - q.cw * P[q.jw] - q.ce * P[q.je];
-      const float rs = (float)(m.dis[i] * r);
-      m.r32[i] = rs;
-      m.y32[i] = 0.0f;
-      v[0] += r * r;
-      v[1] += (double)rs * (double)rs;
-    }
+    if (i >= 1 && i <= a.nwet) mx_resid_cell(m, i, v);
   }
   double s[2];
   if (grid_sum_x<2>(v, a.part, a.tk + TS_MX_RESID * TICKET_SITE, gridDim.x, tile, s) &&
@@ -800,13 +840,6 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_resid(MxArgs m) {
   }
 }
 
-__device__ __forceinline__ float p32_at(const MxArgs &m, const float *pold,
-                                        double beta, int j) {
-  return (j > 0 && j != m.g.lc0)
-             ? (float)((double)m.r32[j] + beta * (double)pold[j])
-             : 0.0f;
-}
-
 __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_a(MxArgs m, const float *pold,
                                                      float *pnew) {
   const CgArgs &a = m.g;
@@ -817,20 +850,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_a(MxArgs m, const float *pold
   #pragma unroll
   for (int u = 0; u < MX_CPT; ++u) {
     const int i = (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
-    if (i >= 1 && i <= a.nwet) {
-      const int js = a.ls[i], jn = a.ln[i];
-      const float pc = p32_at(m, pold, beta, i);
-      const float ps = p32_at(m, pold, beta, js);
-      const float pn = p32_at(m, pold, beta, jn);
-      const float pw = p32_at(m, pold, beta, i - 1);
-      const float pe = p32_at(m, pold, beta, i + 1);
-      const float an = (jn != a.lc0) ? m.as[jn] : 0.0f;
-      const float ae = (i + 1 != a.lc0) ? m.aw[i + 1] : 0.0f;
-      const float apc = pc + m.as[i] * ps + an * pn + m.aw[i] * pw + ae * pe;
-      pnew[i] = pc;
-      m.ap32[i] = apc;
-      v[0] += (double)apc * (double)pc;
-    }
+    if (i >= 1 && i <= a.nwet) mx_a_cell(m, i, beta, pold, pnew, v);
   }
   double s[1];
   if (grid_sum_x<1>(v, a.part, a.tk + TS_MX_A * TICKET_SITE, gridDim.x, tile, s) &&
@@ -848,12 +868,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_b(MxArgs m, const float *pnew
   #pragma unroll
   for (int u = 0; u < MX_CPT; ++u) {
     const int i = (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
-    if (i >= 1 && i <= a.nwet) {
-      m.y32[i] = (float)((double)m.y32[i] + alpha * (double)pnew[i]);
-      const float r = (float)((double)m.r32[i] - alpha * (double)m.ap32[i]);
-      m.r32[i] = r;
-      v[0] += (double)r * (double)r;
-    }
+    if (i >= 1 && i <= a.nwet) mx_b_cell(m, i, alpha, pnew, v);
   }
   double s[1];
   if (grid_sum_x<1>(v, a.part, a.tk + TS_MX_B * TICKET_SITE, gridDim.x, tile, s) &&
@@ -876,6 +891,134 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_apply(MxArgs m) {
     if (i >= 1 && i <= a.nwet) a.x[i] = a.x[i] + m.dis[i] * (double)m.y32[i];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) a.st->outer += 1;
+}
+
+// ---------------------------------------------------------------------
+// The mixed solve of independent basins: the per-cell work above over the
+// per-basin tiles of 1024 cells, every reduction, state, stop test and
+// outer correction per basin.  A basin whose FP64 residual meets its stop
+// test is finished (fin) and skips the later rounds; the host runs outer
+// rounds until every basin is finished or the round cap is reached, and
+// inner iterations until every running basin's inner solve has stopped
+// (counters: segfin of finished basins, segdone of stopped inner solves).
+// ---------------------------------------------------------------------
+struct SMxArgs {
+  MxArgs m;
+  SegArgs g;
+  unsigned *segfin;
+};
+
+template <int CPT, typename F>
+__device__ __forceinline__ void tile_cells(const int4 &t, F f) {
+#pragma unroll
+  for (int u = 0; u < CPT; ++u) {
+    const int i = t.x + u * EFDC_BLOCK + (int)threadIdx.x;
+    if (i < t.y) f(i);
+  }
+}
+
+template <int CPT>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_smx_prep(SMxArgs x, double rsqm,
+                                                         double rel_tol) {
+  const int b = blockIdx.x;
+  const int4 t = x.g.tiles[b];
+  double v[1] = {0.0};
+  tile_cells<CPT>(t, [&](int i) { mx_prep_cell(x.m, i, v); });
+  double s[1];
+  if (seg_sum<1>(x.g, b, t.z, v, s) && threadIdx.x == 0) {
+    CgState *st = x.g.segst + t.z;
+    st->bnorm2 = s[0];
+    st->rsqm = rel_tol > 0.0 ? rel_tol * rel_tol * s[0] : rsqm;
+    st->outer = 0;
+    st->iter = 0;
+    st->status = 0;
+    st->fin = 0;
+    st->done = 0;
+  }
+}
+
+template <int CPT>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_smx_resid(SMxArgs x, int max_outer) {
+  const int b = blockIdx.x;
+  const int4 t = x.g.tiles[b];
+  CgState *st = x.g.segst + t.z;
+  if (st->fin) return;
+  double v[2] = {0.0, 0.0};
+  tile_cells<CPT>(t, [&](int i) { mx_resid_cell(x.m, i, v); });
+  double s[2];
+  if (seg_sum<2>(x.g, b, t.z, v, s) && threadIdx.x == 0) {
+    st->rsq64 = s[0];
+    st->rsq = s[0];
+    st->bnorm2_in = s[1];
+    st->rz_in = s[1];
+    st->beta = 0.0;
+    st->inner_iter = 0;
+    const bool conv = s[0] <= st->rsqm;
+    const bool cap = !conv && st->outer >= max_outer;
+    if (conv || cap) {
+      st->fin = 1;
+      if (cap) st->status = 1;
+      atomic_inc(x.segfin);
+    }
+    st->done = (conv || cap || s[1] == 0.0) ? 1 : 0;
+    if (st->done) atomic_inc(x.g.segdone);
+  }
+}
+
+template <int CPT>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_smx_a(SMxArgs x, const float *pold, float *pnew) {
+  const int b = blockIdx.x;
+  const int4 t = x.g.tiles[b];
+  CgState *st = x.g.segst + t.z;
+  if (st->done) return;
+  const double beta = st->beta;
+  double v[1] = {0.0};
+  tile_cells<CPT>(t, [&](int i) { mx_a_cell(x.m, i, beta, pold, pnew, v); });
+  double s[1];
+  if (seg_sum<1>(x.g, b, t.z, v, s) && threadIdx.x == 0) st->alpha = st->rz_in / s[0];
+}
+
+template <int CPT>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_smx_b(SMxArgs x, const float *pnew, double rtol2,
+                                                      int cap) {
+  const int b = blockIdx.x;
+  const int4 t = x.g.tiles[b];
+  CgState *st = x.g.segst + t.z;
+  if (st->done) return;
+  const double alpha = st->alpha;
+  double v[1] = {0.0};
+  tile_cells<CPT>(t, [&](int i) { mx_b_cell(x.m, i, alpha, pnew, v); });
+  double s[1];
+  if (seg_sum<1>(x.g, b, t.z, v, s) && threadIdx.x == 0) {
+    st->beta = s[0] / st->rz_in;
+    st->rz_in = s[0];
+    st->inner_iter += 1;
+    st->iter += 1;
+    if (s[0] <= rtol2 * st->bnorm2_in || st->inner_iter >= cap) {
+      st->done = 1;
+      atomic_inc(x.g.segdone);
+    }
+  }
+}
+
+// P += D^-1/2 y for the basins that ran an inner solve in this round
+template <int CPT>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_smx_apply(SMxArgs x) {
+  const int b = blockIdx.x;
+  const int4 t = x.g.tiles[b];
+  CgState *st = x.g.segst + t.z;
+  if (st->fin) return;
+  tile_cells<CPT>(t, [&](int i) {
+    x.m.g.x[i] = x.m.g.x[i] + x.m.dis[i] * (double)x.m.y32[i];
+  });
+  if (t.w == 0 && threadIdx.x == 0) st->outer += 1;
+}
+
+__global__ void k_fill_u32x2(unsigned *a, unsigned *b) {
+  if (threadIdx.x == 0) {
+    *a = 0u;
+    if (b) *b = 0u;
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -1474,7 +1617,7 @@ int cg_dispatch(efdc_ctx *c, efdc_cg_stats_t *stats, bool defer = false,
                 bool *ran_mixed = nullptr) {
   int s;
   c->pcg_deferred = false;
-  if (c->nseg > 1) return seg_cg_run(c, stats);
+  if (c->nseg > 1) return seg_cg_run(c, stats, ran_mixed);
   bool mixed = (c->prm.cg_mode == EFDC_CG_MIXED);
   if (mixed) {
     // the mixed operator is the compact symmetric one; the symmetry flag
@@ -1743,7 +1886,7 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
     }
     cgs = cg_finish(c, ran_mixed, stats);
   }
-  if (c->nseg == 1 && c->st_host->symok != c->symok_cache && c->symok_cache >= 0) {
+  if (c->st_host->symok != c->symok_cache && c->symok_cache >= 0) {
     // cannot happen for a fixed grid and boundary tables (the cache is
     // reset by their setters); refuse rather than keep a solve made with
     // the wrong operator
@@ -1926,12 +2069,107 @@ int seg_tune_tiles(efdc_ctx *c) {
 
 }  // namespace
 
-int seg_cg_run(efdc_ctx *c, efdc_cg_stats_t *stats) {
-  if (c->prm.cg_mode != EFDC_CG_FP64) {
-    c->err = "independent basins are solved with the FP64 CG (EFDC_CG_FP64)";
-    return EFDC_ERR_ARG;
+namespace {
+
+// The FP32-refined solve of every basin (tiles of 1024 cells, the
+// single-model mixed kernels' per-cell work).
+int seg_mixed_run(efdc_ctx *c, efdc_cg_stats_t *stats) {
+  if (!c->dis) {
+    c->err = "the FP32-refined solve needs a context created with EFDC_CG_MIXED";
+    return EFDC_ERR_STATE;
   }
+  constexpr int Q = 2, CPT = 4;  // 1024-cell tiles
+  SMxArgs x;
+  std::memset(&x, 0, sizeof x);
+  x.m.g = cg_args(c);
+  x.m.dis = c->dis;
+  x.m.as = c->as32;
+  x.m.aw = c->aw32;
+  x.m.r32 = c->r32;
+  x.m.y32 = c->y32;
+  x.m.ap32 = c->ap32;
+  x.g = seg_args(c);
+  x.g.tiles = c->d_tiles_s[Q];
+  x.g.segtile0 = c->d_segtile0_s[Q];
+  x.g.ntiles = c->ntiles_s[Q];
+  x.segfin = c->d_segfin;
+  const int nt = c->ntiles_s[Q], cap = c->prm.ITERM;
+  const double rtol = c->prm.mixed_inner_rtol > 0 ? c->prm.mixed_inner_rtol : 1e-5;
+  k_fill_u32x2<<<1, 64, 0, c->stream>>>(c->d_segfin, c->d_segdone);
+  TIMED(c, TK_MX_PREP, k_smx_prep<CPT><<<nt, EFDC_BLOCK, 0, c->stream>>>(x, c->prm.RSQM,
+                                                                        c->prm.cg_rel_tol));
+  unsigned fin = 0, done = 0;
+  for (;;) {
+    const unsigned fin_before = fin;
+    k_fill_u32x2<<<1, 64, 0, c->stream>>>(c->d_segdone, nullptr);
+    TIMED(c, TK_MX_RESID, k_smx_resid<CPT><<<nt, EFDC_BLOCK, 0, c->stream>>>(
+        x, c->prm.mixed_max_outer));
+    HIP_TRY(c, hipMemcpyAsync(&fin, c->d_segfin, sizeof(unsigned), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if ((int)fin >= c->nseg) break;
+    const unsigned target = (unsigned)c->nseg - fin_before;
+    int it = 0, chunk = c->last_iters > 0 ? c->last_iters : 8;
+    for (;;) {
+      for (int j = 0; j < chunk && it < cap; ++j, ++it) {
+        TIMED(c, TK_MX_A, k_smx_a<CPT><<<nt, EFDC_BLOCK, 0, c->stream>>>(
+            x, c->p32[it & 1], c->p32[(it + 1) & 1]));
+        TIMED(c, TK_MX_B, k_smx_b<CPT><<<nt, EFDC_BLOCK, 0, c->stream>>>(
+            x, c->p32[(it + 1) & 1], rtol * rtol, cap));
+      }
+      HIP_TRY(c, hipMemcpyAsync(&done, c->d_segdone, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                c->stream));
+      HIP_TRY(c, hipStreamSynchronize(c->stream));
+      if (done >= target || it >= cap) break;
+      chunk = 4;
+    }
+    c->last_iters = it;
+    TIMED(c, TK_MX_APPLY, k_smx_apply<CPT><<<nt, EFDC_BLOCK, 0, c->stream>>>(x));
+  }
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipMemcpyAsync(c->h_segst, c->d_segst, sizeof(CgState) * c->nseg,
+                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  int imax = 0, omax = 0, bad = 0;
+  double rsq = 0.0;
+  for (int q = 0; q < c->nseg; ++q) {
+    imax = std::max(imax, c->h_segst[q].iter);
+    omax = std::max(omax, c->h_segst[q].outer);
+    rsq = std::max(rsq, c->h_segst[q].rsq64);
+    if (c->h_segst[q].status != 0) bad = 1;
+  }
+  if (c->timing) c->cg_iter_count += imax;
+  if (stats) {
+    stats->iters = imax;
+    stats->outer = omax;
+    stats->rsq = rsq;
+    stats->rsqm_used = c->h_segst[0].rsqm;
+    stats->ccmnm = 0.0;
+  }
+  if (bad) {
+    c->err = "CONGRAD: the FP32-refined solve of a basin did not converge within the outer "
+             "rounds";
+    return EFDC_ERR_CG_MAXITER;
+  }
+  return EFDC_OK;
+}
+
+}  // namespace
+
+int seg_cg_run(efdc_ctx *c, efdc_cg_stats_t *stats, bool *ran_mixed) {
   int s;
+  if (ran_mixed) *ran_mixed = false;
+  if (c->prm.cg_mode == EFDC_CG_MIXED) {
+    // the compact symmetric operator, as for one model (else FP64)
+    if (c->symok_cache < 0) {
+      if ((s = read_state(c))) return s;
+      c->symok_cache = c->st_host->symok;
+    }
+    if (c->rowmajor && c->symok_cache == 0) {
+      if (ran_mixed) *ran_mixed = true;
+      return seg_mixed_run(c, stats);
+    }
+  }
   if (c->seg_tune_pending && (s = seg_tune_tiles(c))) return s;
   if (c->cg_graph_tune && !c->timing &&
       (s = cg_graph_trial(c, [&](bool g) { return seg_solve_set(c, c->seg_tset, g); })))

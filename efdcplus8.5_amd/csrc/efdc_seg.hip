@@ -64,6 +64,7 @@ void seg_free(efdc_ctx *c) {
   c->d_segtile0 = nullptr;
   f(c->d_segid); f(c->d_segfirst); f(c->d_segst);
   f(c->d_segtk); f(c->d_segdone); f(c->d_segmin); f(c->d_segpart); f(c->d_segbal);
+  f(c->d_segfin);
   if (c->h_segst) (void)hipHostFree(c->h_segst);
   c->h_segst = nullptr;
   c->nseg = 1;
@@ -144,6 +145,7 @@ int efdc_set_segments(efdc_ctx_t *c, int32_t nseg, const int32_t *first_L) {
   if ((st = dev_alloc(c, (void **)&c->d_segst, sizeof(CgState) * nseg)) ||
       (st = dev_alloc(c, (void **)&c->d_segtk, sizeof(unsigned) * TICKET_PAD * nseg)) ||
       (st = dev_alloc(c, (void **)&c->d_segdone, sizeof(unsigned))) ||
+      (st = dev_alloc(c, (void **)&c->d_segfin, sizeof(unsigned))) ||
       (st = dev_alloc(c, (void **)&c->d_segmin, sizeof(double) * nseg)) ||
       (st = dev_alloc(c, (void **)&c->d_segpart, sizeof(double) * 2 * maxtiles)) ||
       (st = dev_alloc(c, (void **)&c->d_segbal, sizeof(double) * nseg * (EFDC_MAXC + 1)))) {
