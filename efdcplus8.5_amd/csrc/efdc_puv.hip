@@ -66,7 +66,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_explicit(
     HPtr h, const int *__restrict__ lw, const int *__restrict__ ls,
     const int *__restrict__ le, const int *__restrict__ ln, const T *hpk,
     T *h1pk, int64_t pitch, int KC, int nwet, double dt, double G,
-    double RITB1, double SNLT, int dry) {
+    double RITB1, double SNLT, int dry, int drag) {
   int tile;
   const int i = cell_index(nwet, tile);
   if (i > nwet) return;
@@ -88,6 +88,10 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_explicit(
       HA(SVB)[i] * DELT * HA(DYIV)[i] *
           (HA(DXYV)[i] * (HA(TSY)[i] - RITB1 * HA(TBY)[i]) - HA(FCAYE)[i] +
            HA(FPGYE)[i] - SNLT * HA(FYE)[i]);
+  if (drag) {  // implicit bottom / vegetation drag, :236-241
+    HA(FUHDYE)[i] = HA(FUHDYE)[i] * HA(RCX)[i];
+    HA(FVHDXE)[i] = HA(FVHDXE)[i] * HA(RCY)[i];
+  }
   if (dry) {  // faces reopened after the explicit terms, :246-253
     HA(SUB)[i] = HA(SUBO)[i];
     HA(SVB)[i] = HA(SVBO)[i];
@@ -1762,12 +1766,12 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
     TIMED(c, TK_PUV_EXPLICIT, k_puv_explicit<double><<<nb, EFDC_BLOCK, 0, c->stream>>>(
         h, c->dLW, c->dLS, c->dLE, c->dLN, (const double *)c->dK[EFDC_HPK - EFDC_HPK],
         (double *)c->dK[EFDC_H1PK - EFDC_HPK], c->pitch, c->KC, nwet, dt, G,
-        c->prm.RITB1, c->prm.SNLT, dry ? 1 : 0));
+        c->prm.RITB1, c->prm.SNLT, dry ? 1 : 0, c->prm.implicit_drag ? 1 : 0));
   else
     TIMED(c, TK_PUV_EXPLICIT, k_puv_explicit<float><<<nb, EFDC_BLOCK, 0, c->stream>>>(
         h, c->dLW, c->dLS, c->dLE, c->dLN, (const float *)c->dK[EFDC_HPK - EFDC_HPK],
         (float *)c->dK[EFDC_H1PK - EFDC_HPK], c->pitch, c->KC, nwet, dt, G,
-        c->prm.RITB1, c->prm.SNLT, dry ? 1 : 0));
+        c->prm.RITB1, c->prm.SNLT, dry ? 1 : 0, c->prm.implicit_drag ? 1 : 0));
   c->ncordry = 0;
   bool ran_mixed = false;
   int cgs = EFDC_OK;

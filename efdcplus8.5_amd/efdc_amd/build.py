@@ -81,13 +81,16 @@ def build_fortran(global_mod_dir: str | None = None) -> str:
     mod = os.path.join(FBIN, "mod")
     os.makedirs(mod, exist_ok=True)
     iface = os.path.join(FORTRAN, "efdc_hip.f90")
-    subprocess.check_call([FC, "-O2", "-c", "-module-dir", mod, iface,
+    subprocess.check_call([FC, "-O2", "-fPIC", "-c", "-module-dir", mod, iface,
                            "-o", os.path.join(mod, "efdc_hip.o")])
     if global_mod_dir:
-        subprocess.check_call([FC, "-O2", "-c", "-fdefault-real-8", "-cpp",
+        subprocess.check_call([FC, "-O2", "-fPIC", "-c", "-fdefault-real-8", "-cpp",
                                "-module-dir", mod, "-I", global_mod_dir,
                                os.path.join(FORTRAN, "efdc_hip_dropin.f90"),
                                "-o", os.path.join(mod, "efdc_hip_dropin.o")])
+        gobj = os.path.join(global_mod_dir, "var_global_mod.o")
+        if os.path.exists(gobj):
+            build_dropin_harness(global_mod_dir, mod, gobj)
     exe = os.path.join(FBIN, "congrad_check")
     subprocess.check_call([FC, "-O2", "-I", mod,
                            os.path.join(ROOT, "tests", "fortran", "congrad_check.f90"),
@@ -95,6 +98,32 @@ def build_fortran(global_mod_dir: str | None = None) -> str:
                            "-L" + HERE, "-lefdc_hip",
                            "-Wl,-rpath,$ORIGIN/../../efdc_amd", "-o", exe])
     return exe
+
+
+HARNESS = os.path.join(FBIN, "libdropin_harness.so")
+
+
+def build_dropin_harness(global_mod_dir: str, mod: str, gobj: str) -> str:
+    """The test harness that runs the drop-ins through the host's MODULE
+    GLOBAL (tests/fortran/gen_dropin_harness.py) as a shared library."""
+    import importlib.util
+    gen_path = os.path.join(ROOT, "tests", "fortran", "gen_dropin_harness.py")
+    spec = importlib.util.spec_from_file_location("gen_dropin_harness", gen_path)
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    src = os.path.join(FBIN, "dropin_harness.f90")
+    open(src, "w").write(gen.gen())
+    hobj = os.path.join(FBIN, "dropin_harness.o")
+    subprocess.check_call([FC, "-O2", "-fPIC", "-c", "-fdefault-real-8", "-cpp",
+                           "-module-dir", mod, "-I", global_mod_dir, "-I", mod, src,
+                           "-o", hobj])
+    tmp = HARNESS + ".tmp"
+    subprocess.check_call([FC, "-shared", "-fPIC", gobj, os.path.join(mod, "efdc_hip.o"),
+                           os.path.join(mod, "efdc_hip_dropin.o"), hobj,
+                           "-L" + HERE, "-lefdc_hip",
+                           "-Wl,-rpath,$ORIGIN/../../efdc_amd", "-o", tmp])
+    os.replace(tmp, HARNESS)
+    return HARNESS
 
 
 if __name__ == "__main__":

@@ -51,7 +51,7 @@ class Params(C.Structure):
                 ("store", C.c_int32), ("mixed_inner_rtol", C.c_double),
                 ("mixed_max_outer", C.c_int32), ("cg_kernel", C.c_int32),
                 ("mixed_precond_degree", C.c_int32), ("tran_shape", C.c_int32),
-                ("ISDRY", C.c_int32), ("NDRYSTP", C.c_int32)]
+                ("ISDRY", C.c_int32), ("NDRYSTP", C.c_int32), ("implicit_drag", C.c_int32)]
 
 
 class PbcSide(C.Structure):

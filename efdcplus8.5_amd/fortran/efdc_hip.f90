@@ -65,7 +65,7 @@ module efdc_hip
     integer(c_int32_t) :: cg_mode, store
     real(c_double) :: mixed_inner_rtol
     integer(c_int32_t) :: mixed_max_outer, cg_kernel, mixed_precond_degree, tran_shape, ISDRY, &
-                          NDRYSTP
+                          NDRYSTP, implicit_drag
   end type
 
   type, bind(C) :: efdc_pbc_side_t

@@ -8,7 +8,13 @@
 ! afterwards.  Scope: the 2TL scheme with any ISDRY (no drying, the drying
 ! with the re-solve loop, or drying and rewetting, with or without the
 ! draining of isolated cells of NDRYSTP < 0), IGRIDV=0, MDCHH=0, ISGWIE=0,
-! ISDYNSTP=0 and sigma layers; other settings
+! ISDYNSTP=0 and sigma layers; salinity, temperature, dye, toxics and
+! both sediment kinds transported with the standard scheme (ISCDCA < 4),
+! their open-boundary concentrations and point sources; the baroclinic
+! term (BSC > 0, with the host's CALEBI) and the implicit bottom and
+! vegetation drag (ISITB >= 1, ISVEG > 0) formed on the host; CALHEAT,
+! the sediment totals and bed model (SSEDTOX) and the two-level balances
+! (ISBAL >= 1) called on the host around the transport.  Other settings
 ! stop at the first call with a message, like the reference's own
 ! unsupported-option stops.
 module efdc_hip_state
@@ -16,12 +22,43 @@ module efdc_hip_state
   implicit none
   type(c_ptr), save :: HCTX = c_null_ptr
   integer, save :: NSLOT = 0
-  integer, save :: SLOT_KIND(16) = 0     ! 1 SAL, 2 TEM, 3 DYE, 6 SED
-  integer, save :: SLOT_NS(16) = 0       ! sediment class of a SED slot
+  integer, save :: SLOT_KIND(16) = 0     ! MVAR: 1 SAL, 2 TEM, 3 DYE, 5 TOX, 6 SED, 7 SND
+  integer, save :: SLOT_NS(16) = 0       ! class of a TOX, SED or SND slot
+  integer, save :: SLOT_MO(16) = 0       ! its MO index (the CB, CQS, CSERT column)
   integer(c_int32_t), allocatable, target, save :: MW(:), ME(:), MS(:), MN(:), MNW(:), MSE(:)
   real(c_double), allocatable, target, save :: TCPH(:)
   integer(c_int32_t), allocatable, target, save :: MASKI(:), DRYI(:)
+  ! the pressure open-boundary tables (EFDC_HIP_OPENBC)
+  integer(c_int32_t), allocatable, target, save :: LW_(:), LE_(:), LS_(:), LN_(:)
+  integer(c_int32_t), allocatable, target, save :: IW_(:), IE_(:), IS_(:), IN_(:)
+  real(c_double), allocatable, target, save :: SW_(:), SE_(:), SS_(:), SN_(:)
+  real(c_double), allocatable, target, save :: CW_(:,:), CE_(:,:), CS_(:,:), CN_(:,:)
+  real(c_double), allocatable, target, save :: QW_(:,:), QE_(:,:), QS_(:,:), QN_(:,:)
+  ! the concentration-BC and source tables (EFDC_HIP_TABLES)
+  integer(c_int32_t), allocatable, target, save :: HXLB(:,:), HXNTB(:,:), HXLQ(:)
+  real(c_double), allocatable, target, save :: HXCB(:,:,:,:), HXCS(:,:,:,:), HXWT(:,:)
+  real(c_double), allocatable, target, save :: HXQS(:,:), HXQC(:,:), HXCQ(:,:,:), HXCT(:,:,:)
 end module efdc_hip_state
+
+! End of the run: the device context and every table of the drop-ins are
+! released (the next call of a drop-in starts over from MODULE GLOBAL).
+subroutine EFDC_HIP_FINISH
+  use efdc_hip
+  use efdc_hip_state
+  implicit none
+  integer(c_int) :: HXERR
+  if (c_associated(HCTX)) then
+    HXERR = efdc_ctx_destroy(HCTX)
+    HCTX = c_null_ptr
+  end if
+  NSLOT = 0
+  if (allocated(MW)) deallocate(MW, ME, MS, MN, MNW, MSE)
+  if (allocated(MASKI)) deallocate(MASKI, DRYI)
+  if (allocated(TCPH)) deallocate(TCPH)
+  if (allocated(LW_)) deallocate(LW_, LE_, LS_, LN_, IW_, IE_, IS_, IN_, SW_, SE_, SS_, SN_, &
+                                 CW_, CE_, CS_, CN_, QW_, QE_, QS_, QN_)
+  if (allocated(HXLB)) deallocate(HXLB, HXNTB, HXCB, HXCS, HXWT, HXLQ, HXQS, HXQC, HXCQ, HXCT)
+end subroutine EFDC_HIP_FINISH
 
 subroutine EFDC_HIP_CHECK(HXERR, WHERE)
   use efdc_hip
@@ -51,6 +88,13 @@ subroutine EFDC_HIP_INIT
   if (ISDRY < 0 .or. IGRIDV /= 0 .or. MDCHH /= 0 .or. ISGWIE /= 0 .or. ISDYNSTP /= 0) then
     stop 'EFDC_HIP: needs IGRIDV=0, MDCHH=0, ISGWIE=0, ISDYNSTP=0'
   end if
+  do NS = 1, 7
+    if (ISTRAN(NS) >= 1 .and. NS /= 4 .and. ISCDCA(NS) >= 4) &
+      stop 'EFDC_HIP: only the standard transport scheme (ISCDCA < 4) is built'
+  end do
+  if (ISTRAN(4) >= 1) stop 'EFDC_HIP: shellfish larvae (ISTRAN(4)) are not transported here'
+  if (ISTRAN(2) >= 1 .and. ISICE == 4) stop 'EFDC_HIP: frazil ice transport (ISICE=4) is not built'
+  if (NTOX + NSED + NSND + 3 > 16) stop 'EFDC_HIP: at most 16 transported constituents'
   allocate(MW(LCM), ME(LCM), MS(LCM), MN(LCM), MNW(LCM), MSE(LCM))
   allocate(MASKI(LCM), DRYI(LCM))
   MW = LWC(1:LCM); ME = LEC(1:LCM); MS = LSC(1:LCM); MN = LNC(1:LCM)
@@ -62,6 +106,10 @@ subroutine EFDC_HIP_INIT
   HXPRM%G = G; HXPRM%RITB1 = RITB1; HXPRM%SNLT = SNLT; HXPRM%HDRY = HDRY
   HXPRM%TCON_TBEGIN = TCON*TBEGIN; HXPRM%ITERM = ITERM; HXPRM%RSQM = RSQM
   HXPRM%ISDRY = ISDRY; HXPRM%NDRYSTP = NDRYSTP
+  ! implicit bottom / vegetation drag: RCX/RCY formed here each step
+  ! (EFDC/calpuv2c.f90:223-243) scale FUHDYE/FVHDXE on the device
+  HXPRM%implicit_drag = 0
+  if (ISITB >= 1 .or. ISVEG > 0) HXPRM%implicit_drag = 1
   HXERR = efdc_ctx_create(HXGRID, HXPRM, HCTX)
   call EFDC_HIP_CHECK(HXERR, 'efdc_ctx_create')
   ! geometry and layer constants (EFDC/aaefdc.f90:1212-1232,2172-2185)
@@ -101,9 +149,19 @@ subroutine EFDC_HIP_INIT
   if (ISTRAN(1) == 1) call ADD(1, 1, 0)
   if (ISTRAN(2) == 1) call ADD(2, 2, 0)
   if (ISTRAN(3) == 1) call ADD(3, 3, 0)
+  if (ISTRAN(5) == 1) then
+    do NS = 1, NTOX
+      call ADD(5, MSVTOX(NS), NS)
+    end do
+  end if
   if (ISTRAN(6) == 1) then
     do NS = 1, NSED
       call ADD(6, MSVSED(NS), NS)
+    end do
+  end if
+  if (ISTRAN(7) == 1) then
+    do NS = 1, NSND
+      call ADD(7, MSVSND(NS), NS)
     end do
   end if
 contains
@@ -122,8 +180,95 @@ contains
     NSLOT = NSLOT + 1
     SLOT_KIND(NSLOT) = MVAR
     SLOT_NS(NSLOT) = NS_
+    SLOT_MO(NSLOT) = MO
   end subroutine
 end subroutine EFDC_HIP_INIT
+
+! The concentration open boundaries (CALTRAN, EFDC/caltran.f90:302-471)
+! and the point sources (CALFQC) of this step, gathered from MODULE GLOBAL
+! for the registered constituents: the inflow concentrations CB*, the
+! series values CSERT of each boundary cell and source, the ramp lengths
+! NTSCR*, the layer weights WTCI, and QSS, QSERCELL, CQS.  The recovery
+! memory CLO*/NLO* stays on the device.
+subroutine EFDC_HIP_TABLES
+  use GLOBAL
+  use efdc_hip
+  use efdc_hip_state
+  implicit none
+  type(efdc_conc_bc_t) :: HXC
+  type(efdc_sources_t) :: HXS
+  integer :: NB(4), NBM, LL, K, J, M, S, NQ
+  integer(c_int) :: HXERR
+
+  if (NSLOT == 0) return
+  NB = [NCBS, NCBW, NCBE, NCBN]
+  NBM = max(maxval(NB), 1)
+  NQ = NQSIJ
+  if (.not. allocated(HXLB)) then
+    allocate(HXLB(NBM,4), HXNTB(NBM,4), HXCB(NSLOT,2,NBM,4), HXCS(NSLOT,KC,NBM,4), HXWT(2,KC))
+    allocate(HXLQ(max(NQ,1)), HXQS(KC,max(NQ,1)), HXQC(KC,max(NQ,1)))
+    allocate(HXCQ(NSLOT,KC,max(NQ,1)), HXCT(NSLOT,KC,max(NQ,1)))
+  end if
+  if (sum(NB) > 0) then
+    do S = 1, 4
+      do LL = 1, NB(S)
+        select case (S)
+        case (1)
+          HXLB(LL,S) = LCBS(LL); HXNTB(LL,S) = NTSCRS(LL)
+        case (2)
+          HXLB(LL,S) = LCBW(LL); HXNTB(LL,S) = NTSCRW(LL)
+        case (3)
+          HXLB(LL,S) = LCBE(LL); HXNTB(LL,S) = NTSCRE(LL)
+        case (4)
+          HXLB(LL,S) = LCBN(LL); HXNTB(LL,S) = NTSCRN(LL)
+        end select
+        do J = 1, NSLOT
+          M = SLOT_MO(J)
+          select case (S)
+          case (1)
+            HXCB(J,1:2,LL,S) = CBS(LL,1:2,M)
+            HXCS(J,1:KC,LL,S) = CSERT(1:KC,NCSERS(LL,M),M)
+          case (2)
+            HXCB(J,1:2,LL,S) = CBW(LL,1:2,M)
+            HXCS(J,1:KC,LL,S) = CSERT(1:KC,NCSERW(LL,M),M)
+          case (3)
+            HXCB(J,1:2,LL,S) = CBE(LL,1:2,M)
+            HXCS(J,1:KC,LL,S) = CSERT(1:KC,NCSERE(LL,M),M)
+          case (4)
+            HXCB(J,1:2,LL,S) = CBN(LL,1:2,M)
+            HXCS(J,1:KC,LL,S) = CSERT(1:KC,NCSERN(LL,M),M)
+          end select
+        end do
+      end do
+      HXC%side(S) = efdc_cbc_side_t(NB(S), c_loc(HXLB(1,S)), c_loc(HXCB(1,1,1,S)), &
+                                    c_loc(HXCS(1,1,1,S)), c_loc(HXNTB(1,S)))
+    end do
+    do K = 1, KC
+      HXWT(1,K) = WTCI(K,1)
+      HXWT(2,K) = WTCI(K,2)
+    end do
+    HXC%wtci = c_loc(HXWT)
+    HXERR = efdc_set_conc_bc(HCTX, HXC)
+    call EFDC_HIP_CHECK(HXERR, 'efdc_set_conc_bc')
+  end if
+  if (NQ > 0) then
+    do LL = 1, NQ
+      HXLQ(LL) = LQS(LL)
+      do K = 1, KC
+        HXQS(K,LL) = QSS(K,LL)
+        HXQC(K,LL) = QSERCELL(K,LL)
+        do J = 1, NSLOT
+          M = SLOT_MO(J)
+          HXCQ(J,K,LL) = CQS(K,LL,M)
+          HXCT(J,K,LL) = CSERT(K,NCSERQ(LL,M),M)
+        end do
+      end do
+    end do
+    HXS = efdc_sources_t(NQ, c_loc(HXLQ), c_loc(HXQS), c_loc(HXQC), c_loc(HXCQ), c_loc(HXCT))
+    HXERR = efdc_set_sources(HCTX, HXS)
+    call EFDC_HIP_CHECK(HXERR, 'efdc_set_sources')
+  end if
+end subroutine EFDC_HIP_TABLES
 
 ! Pressure open boundaries of this step (the time-series part of
 ! EFDC/setopenbc.f90:44-48 is formed on the host; the tide is added on the
@@ -134,11 +279,6 @@ subroutine EFDC_HIP_OPENBC
   use efdc_hip_state
   implicit none
   type(efdc_open_bc_t) :: HXBC
-  integer(c_int32_t), allocatable, target, save :: LW_(:), LE_(:), LS_(:), LN_(:)
-  integer(c_int32_t), allocatable, target, save :: IW_(:), IE_(:), IS_(:), IN_(:)
-  real(c_double), allocatable, target, save :: SW_(:), SE_(:), SS_(:), SN_(:)
-  real(c_double), allocatable, target, save :: CW_(:,:), CE_(:,:), CS_(:,:), CN_(:,:)
-  real(c_double), allocatable, target, save :: QW_(:,:), QE_(:,:), QS_(:,:), QN_(:,:)
   integer(c_int) :: HXERR
   integer :: LL, M
 
@@ -202,11 +342,45 @@ SUBROUTINE CALPUV2C
   type(efdc_cg_stats_t) :: HXST
   integer(c_int) :: HXERR
   integer(c_int32_t) :: HXNCOR
-  integer :: L, K, ND, LF, LL, LP, NW
+  integer :: L, K, ND, LF, LL, LP, NW, LW, LSX
+  real(8) :: TMPX, TMPY
 
   if (.not. c_associated(HCTX)) call EFDC_HIP_INIT
   if (NBCSOP > 0) call EFDC_HIP_OPENBC
   DELT = DT; DELTI = 1./DELT; ISTL = 2                ! EFDC/calpuv2c.f90:128-137
+  if (BSC > 1.E-6) then
+    ! external buoyancy integrals at level N and the explicit external
+    ! density gradients (EFDC/calpuv2c.f90:161,192-201, IGRIDV = 0)
+    call CALEBI
+    do L = 2, LA
+      LW = LWC(L)
+      FPGXE(L) = -SBX(L)*HU(L)*GP*((BI2W(L)+BI2W(LW))*(HP(L)-HP(LW)) + &
+                 2.0*HU(L)*(BI1W(L)-BI1W(LW)) + (BEW(L)+BEW(LW))*(BELV(L)-BELV(LW)))
+      LSX = LSC(L)
+      FPGYE(L) = -SBY(L)*HV(L)*GP*((BI2S(L)+BI2S(LSX))*(HP(L)-HP(LSX)) + &
+                 2.0*HV(L)*(BI1S(L)-BI1S(LSX)) + (BES(L)+BES(LSX))*(BELV(L)-BELV(LSX)))
+    end do
+  end if
+  if (ISITB >= 1) then
+    ! implicit bottom drag with vegetation (EFDC/calpuv2c.f90:223-233); the
+    ! device scales FUHDYE/FVHDXE by them after the explicit terms
+    do L = 2, LA
+      TMPX = 1.0
+      TMPY = 1.0
+      if (UHE(L) /= 0.0) TMPX = U(L,KSZU(L))*HU(L)/UHE(L)
+      if (VHE(L) /= 0.0) TMPY = V(L,KSZV(L))*HV(L)/VHE(L)
+      RCX(L) = 1./(1. + TMPX*RITB*DT*HUI(L)*STBX(L)*SQRT(VU(L)*VU(L) + &
+               U(L,KSZU(L))*U(L,KSZU(L))) + DT*FXVEGE(L))
+      RCY(L) = 1./(1. + TMPY*RITB*DT*HVI(L)*STBY(L)*SQRT(UV(L)*UV(L) + &
+               V(L,KSZV(L))*V(L,KSZV(L))) + DT*FYVEGE(L))
+    end do
+  else if (ISVEG > 0) then
+    ! vegetation drag only (:234-241)
+    do L = 2, LA
+      RCX(L) = 1./(1. + DT*FXVEGE(L))
+      RCY(L) = 1./(1. + DT*FYVEGE(L))
+    end do
+  end if
   ! inputs produced by the host since the last call
   call PUSH(EFDC_UHDYE, UHDYE); call PUSH(EFDC_VHDXE, VHDXE); call PUSH(EFDC_P, P)
   call PUSH(EFDC_HU, HU); call PUSH(EFDC_HV, HV); call PUSH(EFDC_HUI, HUI)
@@ -385,11 +559,15 @@ SUBROUTINE CALCONC(ISTL_, IS2TL_)
   implicit none
   integer :: ISTL_, IS2TL_
   type(efdc_step_t) :: HXSTEP
-  integer :: J
+  integer :: J, K, L, LP, NS
   real, save :: SEDTIME = 0.0
 
   if (ISTL_ /= 2 .or. IS2TL_ /= 1) stop 'EFDC_HIP: CALCONC needs ISTL=2, IS2TL=1'
   if (.not. c_associated(HCTX)) call EFDC_HIP_INIT
+  DELT = DT
+  ! mass balance terms of the sources at the step start (EFDC/calconc.f90:76-80)
+  if (ISBAL >= 1) call BAL2T3A
+  call EFDC_HIP_TABLES
   call PUSH(EFDC_UHDY2, UHDY2); call PUSH(EFDC_VHDX2, VHDX2); call PUSH(EFDC_W2, W2)
   call PUSH(EFDC_H1PK, H1PK); call PUSH(EFDC_HPK, HPK); call PUSH(EFDC_HPKI, HPKI)
   call PUSH(EFDC_AB, AB); call PUSH(EFDC_HPI, HPI); call PUSH(EFDC_QSUM, QSUM)
@@ -402,12 +580,17 @@ SUBROUTINE CALCONC(ISTL_, IS2TL_)
       call EFDC_HIP_CHECK(efdc_set_constituent(HCTX, J-1, TEM, TEM1), 'TEM')
     case (3)
       call EFDC_HIP_CHECK(efdc_set_constituent(HCTX, J-1, DYE, DYE1), 'DYE')
+    case (5)
+      call EFDC_HIP_CHECK(efdc_set_constituent(HCTX, J-1, TOX(:,:,SLOT_NS(J)), &
+                                              TOX1(:,:,SLOT_NS(J))), 'TOX')
     case (6)
       call EFDC_HIP_CHECK(efdc_set_constituent(HCTX, J-1, SED(:,:,SLOT_NS(J)), &
                                               SED1(:,:,SLOT_NS(J))), 'SED')
+    case (7)
+      call EFDC_HIP_CHECK(efdc_set_constituent(HCTX, J-1, SND(:,:,SLOT_NS(J)), &
+                                              SND1(:,:,SLOT_NS(J))), 'SND')
     end select
   end do
-  DELT = DT
   HXSTEP%dt = DT; HXSTEP%n = N; HXSTEP%niter = NITER
   call EFDC_HIP_CHECK(efdc_calconc(HCTX, HXSTEP), 'efdc_calconc')
   do J = 1, NSLOT
@@ -418,11 +601,42 @@ SUBROUTINE CALCONC(ISTL_, IS2TL_)
       call EFDC_HIP_CHECK(efdc_get_constituent(HCTX, J-1, TEM, TEM1), 'TEM')
     case (3)
       call EFDC_HIP_CHECK(efdc_get_constituent(HCTX, J-1, DYE, DYE1), 'DYE')
+    case (5)
+      call EFDC_HIP_CHECK(efdc_get_constituent(HCTX, J-1, TOX(:,:,SLOT_NS(J)), &
+                                              TOX1(:,:,SLOT_NS(J))), 'TOX')
     case (6)
       call EFDC_HIP_CHECK(efdc_get_constituent(HCTX, J-1, SED(:,:,SLOT_NS(J)), &
                                               SED1(:,:,SLOT_NS(J))), 'SED')
+    case (7)
+      call EFDC_HIP_CHECK(efdc_get_constituent(HCTX, J-1, SND(:,:,SLOT_NS(J)), &
+                                              SND1(:,:,SLOT_NS(J))), 'SND')
     end select
   end do
+  ! sediment totals after the transport (EFDC/calconc.f90:479-570)
+  if (ISTRAN(6) > 0) then
+    do K = 1, KC
+      do LP = 1, LLWET(K,0)
+        L = LKWET(LP,K,0)
+        SEDT(L,K) = 0.
+        do NS = 1, NSED
+          SEDT(L,K) = SEDT(L,K) + SED(L,K,NS)
+        end do
+      end do
+    end do
+  end if
+  if (ISTRAN(7) > 0) then
+    do K = 1, KC
+      do LP = 1, LLWET(K,0)
+        L = LKWET(LP,K,0)
+        SNDT(L,K) = 0.
+        do NS = 1, NSND
+          SNDT(L,K) = SNDT(L,K) + SND(L,K,NS)
+        end do
+      end do
+    end do
+  end if
+  ! surface and internal heat sources and sinks (:574-578)
+  if (ISTRAN(2) >= 1) call CALHEAT(ISTL_)
   ! sediment and toxics settling, deposition and resuspension stay on the
   ! host with the bed model, on the reference's clock (2TL branch of
   ! EFDC/calconc.f90:621-630)
@@ -433,6 +647,11 @@ SUBROUTINE CALCONC(ISTL_, IS2TL_)
       call SSEDTOX
       SEDTIME = 0.0
     end if
+  end if
+  ! two-level balances: boundary fluxes and bed exchange (:659-664)
+  if (ISBAL >= 1) then
+    call BAL2T2
+    call BAL2T3B(1)
   end if
 contains
   subroutine PUSH(ID, A)

@@ -129,6 +129,12 @@ typedef struct {
                               it is shallow (< 2 HDRY) or above a
                               neighbour's bed (:838-893, 1010-1018); >= 0:
                               no draining                                   */
+  int32_t implicit_drag;   /* 1: FUHDYE and FVHDXE are scaled by the host's
+                              RCX and RCY after the explicit terms, the
+                              implicit bottom and vegetation drag of ISITB
+                              >= 1 or ISVEG > 0 (EFDC/calpuv2c.f90:223-243);
+                              0: RCX/RCY enter only the coupling and the
+                              transport update (:329-336, :426)             */
 } efdc_params_t;
 
 /* Fill *p with the values EFDC uses by default: G=9.81, RITB1=1, SNLT=0,

@@ -349,6 +349,15 @@ static int explicit_stage(orc_ctx *o, double dt, int64_t n) {
                     (DXYV[i] * PLANT(7, TSY[i] + RITB1 * TBY[i], TSY[i] - RITB1 * TBY[i]) -
                      FCAYE[i] + FPGYE[i] - SNLT * FYE[i]);
   }
+  /* implicit bottom and vegetation drag, :223-243 (RCX/RCY formed by the
+     host) */
+  if (o->IMPLICIT_DRAG) {
+    const double *RCX = A(EFDC_RCX), *RCY = A(EFDC_RCY);
+    for (int L = 2; L <= LA; ++L) {
+      FUHDYE[HX(L)] = FUHDYE[HX(L)] * RCX[HX(L)];
+      FVHDXE[HX(L)] = FVHDXE[HX(L)] * RCY[HX(L)];
+    }
+  }
   /* faces reopened with drying on, :246-253 */
   if (o->ISDRY > 0) {
     const double *SUBO = A(EFDC_SUBO), *SVBO = A(EFDC_SVBO);

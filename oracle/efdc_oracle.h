@@ -80,6 +80,9 @@ typedef struct {
   int32_t isbal;
   double bal_volout;
   double bal_massout[ORC_MAXC];
+  /* ISITB >= 1 or ISVEG > 0: FUHDYE/FVHDXE scaled by RCX/RCY after the
+     explicit terms (EFDC/calpuv2c.f90:223-243) */
+  int32_t IMPLICIT_DRAG;
 } orc_ctx;
 
 typedef struct {

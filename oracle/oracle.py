@@ -66,7 +66,8 @@ class OrcCtx(C.Structure):
                 ("sub1", PD), ("svb1", PD), ("ncordry", C.c_int32),
                 ("NDRYSTP", C.c_int32), ("natdry", PI), ("qdwaste", PD), ("vdwaste", PD),
                 ("isbal", C.c_int32), ("bal_volout", C.c_double),
-                ("bal_massout", C.c_double * MAXC)]
+                ("bal_massout", C.c_double * MAXC),
+                ("IMPLICIT_DRAG", C.c_int32)]
 
 
 class UvwOpts(C.Structure):
@@ -154,7 +155,7 @@ class Oracle:
 
     def __init__(self, case, *, rsqm=1e-9, iterm=200, nthreads=1, g=9.81,
                  ritb1=1.0, snlt=0.0, hdry=0.2, tcon_tbegin=0.0, rel_tol=0.0, isdry=0,
-                 ndrystp=0, lib_path=None):
+                 ndrystp=0, implicit_drag=0, lib_path=None):
         self.case = case
         self.L = lib(lib_path)
         g_ = case.grid
@@ -182,6 +183,7 @@ class Oracle:
         o.iscdry, o.lmaskdry = _pi(self.iscdry), _pi(self.lmaskdry)
         o.sub1, o.svb1 = _pd(self.sub1), _pd(self.svb1)
         o.NDRYSTP = ndrystp
+        o.IMPLICIT_DRAG = implicit_drag
         self.natdry = np.zeros(g_.LCM, np.int32)
         self.qdwaste = np.zeros(g_.LCM)
         self.vdwaste = np.zeros(g_.LCM)

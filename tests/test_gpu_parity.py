@@ -96,6 +96,44 @@ def test_calpuv2c_assembly_bitexact_and_solution(oracle_mod):
         ctx.closure_post()
 
 
+def test_implicit_drag_scaling_bitexact(oracle_mod):
+    """implicit_drag = 1 (ISITB >= 1 or ISVEG > 0, EFDC/calpuv2c.f90:
+    223-243): the host's RCX/RCY scale FUHDYE/FVHDXE after the explicit
+    terms and enter the coupling and the transport update; the assembly is
+    bit for bit the oracle's and the solution within the CG's round-off."""
+    case = _basin(seed=17)
+    i = wet_index(case)
+    rng = np.random.default_rng(4)
+    for name in ("RCX", "RCY"):
+        case.h[name][i] = rng.uniform(0.55, 1.0, len(i))
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref, rsqm=TIGHT, iterm=2000, implicit_drag=1)
+    ctx = _ctx(case, implicit_drag=1)
+    for n in range(1, 4):
+        o.closure_pre()
+        ctx.closure_pre()
+        o.calpuv2c(n=n)
+        ctx.calpuv2c(case.dt, n)
+        if n == 1:
+            for name in ASSEMBLY:
+                np.testing.assert_array_equal(ctx.get(name)[i], ref.h[name][i], err_msg=name)
+        assert eta_err(ctx.get("P")[i], ref.h["P"][i]) <= 1e-9
+        for name in ["P", "HP", "UHDYE", "VHDXE", "HU", "HV", "HUI", "HVI",
+                     "HPI", "UHE", "VHE", "UHDY1E", "VHDX1E", "H1P"]:
+            ctx.set(name, ref.h[name])
+        o.closure_post()
+        ctx.closure_post()
+    # and the scaling is really applied: without it FUHDYE differs
+    plain = _ctx(ref.copy())
+    plain.closure_pre()
+    plain.calpuv2c(case.dt, 4)
+    o.closure_pre()
+    o.calpuv2c(n=4)
+    assert not np.array_equal(plain.get("FUHDYE")[i], ref.h["FUHDYE"][i])
+    plain.close()
+    ctx.close()
+
+
 def test_calconc_fp64_matches_oracle(oracle_mod):
     from efdc_amd import grids
     case = grids.transport_case(48, 24, 6, seed=21)
