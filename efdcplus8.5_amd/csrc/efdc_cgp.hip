@@ -127,15 +127,14 @@ struct Xch {
   int ncell;
 };
 
-// Block sum of NV partials into this workgroup's slots of exchange `tag`,
-// then the flag: every wave's earlier stores (halo values) have completed
-// before thread 0 raises it.
+// Block sum of NV partials into this workgroup's slots of exchange `tag`
+// (stored by thread 0, not yet announced).
 template <int NV>
-__device__ void post(double (&v)[NV], const Xch &x, unsigned tag) {
+__device__ void post_partials(double (&v)[NV], const Xch &x, unsigned tag) {
   __shared__ double sh[NV][PW];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-  for (int j = 0; j < NV; ++j) v[j] = wave_sum(v[j]);
+  for (int j = 0; j < NV; ++j) v[j] = wave_sum_dpp(v[j]);
   if (lane == 0) {
 #pragma unroll
     for (int j = 0; j < NV; ++j) sh[j][w] = v[j];
@@ -150,40 +149,97 @@ __device__ void post(double (&v)[NV], const Xch &x, unsigned tag) {
                          __HIP_MEMORY_SCOPE_AGENT);
     }
   }
+}
+
+// The flag of exchange `tag`: every wave's earlier stores (partials, halo
+// values) have completed before thread 0 raises it.
+__device__ void post_flag(const Xch &x, unsigned tag) {
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) put_u32(&x.flag[blockIdx.x * TICKET_PAD], tag);
 }
 
+template <int NV>
+__device__ void post(double (&v)[NV], const Xch &x, unsigned tag) {
+  post_partials<NV>(v, x, tag);
+  post_flag(x, tag);
+}
+
 // Wait for exchange `tag`, then fetch the NV partials of every workgroup
 // and NR words of each outside cell (hl[0..nhalo)) into raw[q*HMAX + h].
-// Returns the sums (same bits in every workgroup), or false if the wait was
-// abandoned.
+// Wave 0 alone polls the flags (lane l reads the flags of workgroups l,
+// l+64, ...; a wave vote ends the loop), so the waiting costs no workgroup
+// barrier per round; after one barrier every wave fetches the halo words
+// while wave 0 adds the partials (lane l adds those of workgroups l, l+64,
+// ... in that order, then a DPP wave sum).  Returns the sums (same bits in
+// every workgroup), or false if the wait was abandoned.
 template <int NV, int NR>
 __device__ bool exchange(const Xch &x, unsigned tag, const int *hl, int nhalo,
                          unsigned *raw, const Bar &b, double (&sum)[NV]) {
-  __shared__ double sh[NV][PW];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const bool hasp = t < (int)gridDim.x;
-  unsigned spins = 0;
-  for (;;) {
-    const bool ready = !hasp || get_u32(&x.flag[t * TICKET_PAD]) >= tag;
-    if (__syncthreads_and(ready)) break;
-    if (++spins > SPIN_MAX ||
-        ((spins & 255u) == 0u &&
-         __hip_atomic_load(b.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-      if (t == 0)
-        __hip_atomic_store(b.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return false;
-    }
-    __builtin_amdgcn_s_sleep(1);
-  }
-  double v[NV];
+  __shared__ double shs[NV];
+  __shared__ int s_ok;
+  const int t = threadIdx.x, lane = t & 63;
+  const int nb = (int)gridDim.x;
+  if (t < 64) {
+    unsigned spins = 0;
+    int ok = 1;
+    for (;;) {
+      unsigned f[PB / 64];
 #pragma unroll
-  for (int j = 0; j < NV; ++j)
-    v[j] = hasp ? __hip_atomic_load(&pslot(x.ps, tag, j)[t], __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT)
-                : 0.0;
+      for (int u = 0; u < PB / 64; ++u) {
+        const int q = lane + 64 * u;
+        f[u] = q < nb ? get_u32(&x.flag[q * TICKET_PAD]) : tag;
+      }
+      bool ready = true;
+#pragma unroll
+      for (int u = 0; u < PB / 64; ++u) ready = ready && f[u] >= tag;
+      if (__all(ready)) break;
+      ++spins;
+      const int ab = (spins & 255u) == 0u
+                         ? __builtin_amdgcn_readfirstlane(
+                               __hip_atomic_load(b.abort, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+                         : 0;
+      if (spins > SPIN_MAX || ab) {
+        if (lane == 0)
+          __hip_atomic_store(b.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (lane == 0) s_ok = ok;
+  }
+  __syncthreads();
+  if (!s_ok) return false;
+  if (t < 64) {
+    double v[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[j] = 0.0;
+    // four workgroups' partials per lane in flight at a time
+    for (int u0 = 0; 64 * u0 < nb; u0 += 4) {
+      double a[4][NV];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int q = lane + 64 * (u0 + u);
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+          a[u][j] = q < nb ? __hip_atomic_load(&pslot(x.ps, tag, j)[q], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)
+                           : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) v[j] += a[u][j];
+    }
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[j] = wave_sum_dpp(v[j]);
+    if (lane == 0) {
+#pragma unroll
+      for (int j = 0; j < NV; ++j) shs[j] = v[j];
+    }
+  }
   if (NR > 0) {
     // all loads of a batch of HB outside cells are in flight before the
     // first result is stored (a store per load would serialise the round
@@ -230,19 +286,9 @@ __device__ bool exchange(const Xch &x, unsigned tag, const int *hl, int nhalo,
       }
     }
   }
-#pragma unroll
-  for (int j = 0; j < NV; ++j) v[j] = wave_sum(v[j]);
-  if (lane == 0) {
-#pragma unroll
-    for (int j = 0; j < NV; ++j) sh[j][w] = v[j];
-  }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    double s = 0.0;
-    for (int q = 0; q < PW; ++q) s += sh[j][q];
-    sum[j] = s;
-  }
+  for (int j = 0; j < NV; ++j) sum[j] = shs[j];
   __syncthreads();
   return true;
 }
@@ -502,6 +548,8 @@ __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
 // ---------------------------------------------------------------------
 struct PMxArgs {
   const double *ccc, *ccs, *ccw, *fptmp;
+  const double *dis;         // 1/sqrt(CCC)
+  const float *as, *aw;      // A'(i,S), A'(i,W) in FP32
   double *x;                 // P, in/out
   const int *ls, *ln;
   int nwet, lc0;
@@ -537,7 +585,6 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
   auto inside = [&](int j) { return j >= b0 && j < b1; };
   auto real = [&](int j) { return j > 0 && j != lc0; };
   auto slot = [&](int j) { return j - b0 + WIN; };
-  auto dinv = [&](int j) { return 1.0 / sqrt(a.ccc[j]); };
   auto rval = [&](int j) { return real(j) ? lds[slot(j)] : 0.0f; };
   // FP32 halo value q of cell i in the slots of exchange `tag`
   auto hxw = [&](int i, unsigned tag, int q) {
@@ -597,27 +644,13 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
       const int s_ = a.ls[i], n_ = a.ln[i];
       js[k] = s_;
       jn[k] = n_;
-      const double di = dinv(i);
-      // A'(i,S) = CCS(i) d(i) d(S), A'(i,W) = CCW(i) d(i) d(i-1); the north
-      // and east entries are the south / west entries of the neighbour,
-      // evaluated with the same expression as the neighbour's own
-      const double ds = (s_ != lc0) ? dinv(s_) : 0.0;
-      const double dw = (i - 1 > 0) ? dinv(i - 1) : 0.0;
-      const double cs = (s_ != lc0) ? a.ccs[i] : 0.0;
-      as_[k] = (float)(cs * di * ds);
-      aw_[k] = (float)(a.ccw[i] * di * dw);
-      if (n_ != lc0) {
-        const double dn = dinv(n_);
-        const int ss = a.ls[n_];
-        const double dss = (ss != lc0) ? dinv(ss) : 0.0;
-        const double csn = (ss != lc0) ? a.ccs[n_] : 0.0;
-        an_[k] = (float)(csn * dn * dss);
-      }
-      if (i + 1 != lc0) {
-        const double de = dinv(i + 1);
-        const double dwe = (i > 0) ? di : 0.0;
-        ae_[k] = (float)(a.ccw[i + 1] * de * dwe);
-      }
+      // A'(i,S) = CCS(i) d(i) d(S), A'(i,W) = CCW(i) d(i) d(i-1), formed
+      // once per solve by the scaling pass; the north and east entries are
+      // the south / west entries of the neighbour
+      as_[k] = a.as[i];
+      aw_[k] = a.aw[i];
+      if (n_ != lc0) an_[k] = a.as[n_];
+      if (i + 1 != lc0) ae_[k] = a.aw[i + 1];
       const int nbq[4] = {s_, n_, i - 1, i + 1};
       bool out = false;
 #pragma unroll
@@ -656,13 +689,15 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
         const double rr = a.fptmp[io] - a.ccc[io] * fetch(&P[io]) - cn * fetch(&P[n_]) -
                           cs * fetch(&P[s_]) - cw * fetch(&P[io - 1]) -
                           ce * fetch(&P[io + 1]);
-        const float rs = (float)(dinv(io) * rr);
+        const float rs = (float)(a.dis[io] * rr);
         r[k] = rs;
         Y(k) = 0.0f;
         v3[0] += rr * rr;
         v3[1] += (double)rs * (double)rs;
       }
     }
+    // the partials go out while the preconditioner runs
+    post_partials<3>(v3, a.xc, tag);
     // u0 = M^-1 r0, exported for the neighbours' A u0
     precond(r, u);
 #pragma unroll
@@ -674,7 +709,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
       }
     }
     fp2 = 0.0;
-    post<3>(v3, a.xc, tag);
+    post_flag(a.xc, tag);
     {
       double s3[3];
       if (!exchange<3, 1>(a.xc, tag, hl, nhalo, raw, a.bar, s3)) return;
@@ -708,6 +743,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
         v2[2] += (double)r[k] * (double)r[k];
       }
     }
+    post_partials<3>(v2, a.xc, tag);
     {
       float zt[CPT];
       precond(w, zt);
@@ -722,7 +758,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
         }
       }
     }
-    post<3>(v2, a.xc, tag);
+    post_flag(a.xc, tag);
     double gam, del;
     {
       double s3[3];
@@ -776,6 +812,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
           v[2] += (double)r[k] * (double)r[k];
         }
       }
+      post_partials<3>(v, a.xc, tag);
       {
         float zt[CPT];
         precond(w, zt);
@@ -790,7 +827,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
           }
         }
       }
-      post<3>(v, a.xc, tag);
+      post_flag(a.xc, tag);
       double s3[3];
       if (!exchange<3, 3>(a.xc, tag, hl, nhalo, raw, a.bar, s3)) return;
       inner += 1;
@@ -810,7 +847,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
       if (i < b1) {
         const int io = opaque(i);
         double *P = a.x;
-        publish(&P[io], fetch(&P[io]) + dinv(io) * (double)Y(k));
+        publish(&P[io], fetch(&P[io]) + a.dis[io] * (double)Y(k));
       }
     }
     ++outer;
@@ -829,6 +866,23 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
     st->done = 1;
     st->status = status;
   }
+}
+
+// Jacobi scaling of the mixed solve from a scaled system (CCC, CCS, CCW)
+// given directly: dis = 1/sqrt(CCC), A'(i,S), A'(i,W) in FP32.
+__global__ __launch_bounds__(256) void k_mx_coef(const double *ccc, const double *ccs,
+                                                 const double *ccw, const int *ls, int nwet,
+                                                 int lc0, double *dis, float *as, float *aw) {
+  const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i > nwet) return;
+  const int js = ls[i];
+  const double di = 1.0 / sqrt(ccc[i]);
+  const double ds = (js != lc0) ? 1.0 / sqrt(ccc[js]) : 0.0;
+  const double dw = (i - 1 > 0) ? 1.0 / sqrt(ccc[i - 1]) : 0.0;
+  const double cs = (js != lc0) ? ccs[i] : 0.0;
+  dis[i] = di;
+  as[i] = (float)(cs * di * ds);
+  aw[i] = (float)(ccw[i] * di * dw);
 }
 
 __global__ void k_pcg_reset(unsigned *ctr, int *abort) {
@@ -986,11 +1040,21 @@ int pcg_solve(efdc_ctx *c, bool mixed, bool defer) {
   } else {
     const int cpt = pcg_shape(c, c->pcg_occmx, PCGMX_CPT_MAX, nb);
     if (!cpt) { c->err = "persistent CG: problem too large"; return EFDC_ERR_ARG; }
+    if (!c->mx_coef_ok) {
+      const int n = c->n_wet();
+      k_mx_coef<<<(n + 255) / 256, 256, 0, c->stream>>>(
+          c->dH[EFDC_CCC], c->dH[EFDC_CCS], c->dH[EFDC_CCW], c->dLS, n, c->LC - 1, c->dis,
+          c->as32, c->aw32);
+      HIP_TRY(c, hipGetLastError());
+    }
     PMxArgs a;
     a.ccc = c->dH[EFDC_CCC];
     a.ccs = c->dH[EFDC_CCS];
     a.ccw = c->dH[EFDC_CCW];
     a.fptmp = c->dH[EFDC_FPTMP];
+    a.dis = c->dis;
+    a.as = c->as32;
+    a.aw = c->aw32;
     a.x = c->dH[EFDC_P];
     a.ls = c->dLS;
     a.ln = c->dLN;

@@ -102,6 +102,7 @@ struct efdc_ctx {
   // CG scratch
   double *r = nullptr, *pb[2] = {nullptr, nullptr}, *ap = nullptr;
   double *dis = nullptr;       // 1/sqrt(CCC) (mixed mode)
+  bool mx_coef_ok = false;     // dis/as32/aw32 formed for the current CCC
   float *as32 = nullptr, *aw32 = nullptr, *r32 = nullptr, *y32 = nullptr;
   float *p32[2] = {nullptr, nullptr}, *ap32 = nullptr;
   double *part = nullptr;      // reduction partials, 4 x max_blocks
@@ -310,6 +311,32 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// The same sum through DPP lane moves, which stay in the VALU (a shuffle is
+// an LDS permute per 32-bit half and step): pairs, quads, half rows and
+// rows (row_shr with bound_ctrl reads 0 from lanes before the row start),
+// then row 0 into row 1 and rows 0-1 into rows 2-3 (row_bcast:15/31);
+// lane 63 then holds the total, which is broadcast.  Fixed order, so every
+// wave that sums the same values gets the same bits.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_move<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_move<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 __device__ __forceinline__ double wave_min(double v) {

@@ -322,10 +322,14 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_bc_min(
 // check that the system is symmetric and E/W neighbours are contiguous,
 // which selects the compact CG operator (CCN(L)=CCS(LN), CCE(L)=CCW(L+1)).
 // ---------------------------------------------------------------------
+// With dis != nullptr (one model, FP32-refined solve) the pass also forms
+// the Jacobi scaling of the mixed solve: dis = 1/sqrt(CCC) and the FP32
+// entries A'(i,S) = CCS d(i) d(S), A'(i,W) = CCW d(i) d(i-1), the
+// neighbours' CCC recomputed as CC*CCMNMI (the same bits as theirs).
 __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_scale(
     HPtr h, const int *__restrict__ lw, const int *__restrict__ le,
-    const int *__restrict__ ln, int nwet, int lc0, CgState *st,
-    const double *segmin, const int *segid) {
+    const int *__restrict__ ln, const int *__restrict__ ls, int nwet, int lc0, CgState *st,
+    const double *segmin, const int *segid, double *dis, float *as32, float *aw32) {
   int tile;
   const int i = cell_index(nwet, tile);
   if (i > nwet) return;
@@ -347,6 +351,16 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_scale(
   if (e != lc0 && ce != HA(CW)[e]) bad = true;
   if (w == lc0 && ccw != 0.0) bad = true;
   if (bad) atomicOr(&st->symok, 1);
+  if (dis) {
+    const int js = ls[i];
+    const double di = 1.0 / sqrt(ccc);
+    const double ds = (js != lc0) ? 1.0 / sqrt(HA(CC)[js] * CCMNMI) : 0.0;
+    const double dw = (i - 1 > 0) ? 1.0 / sqrt(HA(CC)[i - 1] * CCMNMI) : 0.0;
+    const double cs2 = (js != lc0) ? cs * CCMNMI : 0.0;
+    dis[i] = di;
+    as32[i] = (float)(cs2 * di * ds);
+    aw32[i] = (float)(ccw * di * dw);
+  }
 }
 
 // Same check on an already scaled system (standalone CONGRAD calls).
@@ -1816,10 +1830,13 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
       int s = seg_min_run(c);
       if (s) return s;
     }
+    const bool coef = !segs && c->dis && c->prm.cg_mode == EFDC_CG_MIXED;
     TIMED(c, TK_PUV_SCALE, k_puv_scale<<<nb, EFDC_BLOCK, 0, c->stream>>>(
-        h, c->dLW, c->dLE, c->dLN, nwet, lc0, c->st, segs ? c->d_segmin : nullptr,
-        segs ? c->d_segid : nullptr));
+        h, c->dLW, c->dLE, c->dLN, c->dLS, nwet, lc0, c->st, segs ? c->d_segmin : nullptr,
+        segs ? c->d_segid : nullptr, coef ? c->dis : nullptr, c->as32, c->aw32));
+    c->mx_coef_ok = coef;
     int s = cg_dispatch(c, stats, /*defer=*/true, &ran_mixed);
+    c->mx_coef_ok = false;
     if (s != EFDC_OK && s != EFDC_ERR_CG_MAXITER) return s;
     cgs = s;
     if (adj_recording(c) && (s = adj_record_ps(c))) return s;
