@@ -20,10 +20,16 @@
 // max(.,0) clips and the boundary inflow/outflow switches are taken as
 // the forward step took them (the derivative of the branch taken).
 //
-// The context records two things during a forward step while the adjoint
-// is enabled: the CG solution (the forward step replaces P by
-// G (HP + BELV) afterwards) and the old level of every differentiated
-// constituent (the forward step overwrites CON1 on boundary columns).
+// The context records during a forward step while the adjoint is enabled:
+// the CG solution (the forward step replaces P by G (HP + BELV)
+// afterwards), the old level of every differentiated constituent (the
+// forward step overwrites CON1 on boundary columns) and, for a settling
+// one, its column after the transport (the settling overwrites it).
+//
+// Settling (SSEDTOX -> CALSED, EFDC/calsed.f90:232-256,308-312) is an
+// implicit upwind sweep from the top layer down; its reverse runs from the
+// bottom layer up and passes each layer's lambda to the layer above (whose
+// settled value fed the flux into it) and to HPKI of the layer.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -48,6 +54,7 @@ struct AdjState {
   double *nman = nullptr;  // dJ/dn
   double *lam[EFDC_MAXC] = {};  // lambda of CON, KC planes (FP64)
   void *c0[EFDC_MAXC] = {};     // recorded CON before the step (storage type)
+  void *cset[EFDC_MAXC] = {};   // recorded CON before the settling (storage type)
   double *lcd = nullptr, *own = nullptr;
   double *wk[5] = {};
   double *bcx = nullptr;
@@ -511,6 +518,46 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_adj_tran_gather(TaArgs<T> a) {
   a.av_prev[j] += 0.5 * avb;
 }
 
+// Reverse of the settling sweep of one column.  Forward (top K = KC down):
+//   v_KC = m_KC / (1 + ws W_KC),
+//   v_K  = (m_K + ws v_(K+1) W_K) / (1 + ws W_K),             1 < K < KC,
+//   v_1  = (m_1 + (wese + ws v_2) W_1) / (1 + wsetmp W_1),
+// with m_K = max(c_K, 0), W_K = DTSED HPKI_K (v_2 absent when KC = 1).
+// lambda of v becomes lambda of c in place; lambda of HPKI goes to HP
+// (HPKI = 1 / (HP DZC)).  c are the recorded values before the sweep, v
+// the values it stored.
+template <typename T>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_adj_settle(
+    double *lam, const T *cpre, const T *cpost, const T *hpki, const T *dzc, int64_t pitch,
+    int KC, int nwet, double dtsed, double ws, double wsetmp, double wese, double *ahp) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nwet) return;
+  double carry = 0.0, lam_hp = 0.0;
+  for (int k = 0; k < KC; ++k) {
+    const int64_t o = i + (int64_t)k * pitch;
+    const double lv = lam[o] + carry;
+    const double h = ld(hpki, o), W = dtsed * h;
+    const double v = ld(cpost, o), c = ld(cpre, o);
+    const double vup = (k + 1 < KC) ? ld(cpost, o + pitch) : 0.0;
+    double CL, dW;
+    if (k == 0) {
+      CL = 1. + wsetmp * W;
+      dW = (wese + ws * vup) - wsetmp * v;
+    } else if (k + 1 < KC) {
+      CL = 1. + ws * W;
+      dW = ws * vup - ws * v;
+    } else {
+      CL = 1. + ws * W;
+      dW = -ws * v;
+    }
+    carry = (k + 1 < KC) ? lv * ws * W / CL : 0.0;
+    lam[o] = c > 0.0 ? lv / CL : 0.0;
+    const double lam_hpki = lv * dW / CL * dtsed;
+    lam_hp += lam_hpki * (-h * h * ld(dzc, o));
+  }
+  ahp[i] += lam_hp;
+}
+
 // ---------------------------------------------------------- hydrodynamics
 struct HPtrA {
   double *a[EFDC_NUM_H];
@@ -879,6 +926,20 @@ int adj_tran(efdc_ctx *c, AdjState *A, int slot, const efdc_step_t *step) {
   return EFDC_OK;
 }
 
+template <typename T>
+int adj_settle(efdc_ctx *c, AdjState *A, int slot, const efdc_step_t *step) {
+  const efdc_constituent_t &d = c->con[slot].desc;
+  if (!(d.wset > 0.0) || !A->cset[slot]) return EFDC_OK;
+  const int nb = (c->n_wet() + EFDC_BLOCK - 1) / EFDC_BLOCK;
+  TIMED(c, TK_ADJ_TCOL, k_adj_settle<T><<<nb, EFDC_BLOCK, 0, c->stream>>>(
+      A->lam[slot], (const T *)A->cset[slot], (const T *)c->con[slot].buf[c->con[slot].cur],
+      (const T *)c->dK[EFDC_HPKI - EFDC_HPK], (const T *)c->dK[EFDC_DZC - EFDC_HPK],
+      c->pitch, c->KC, c->n_wet(), step->dt, d.wset, d.probdep * d.wset, d.wese,
+      A->hp[A->cur]));
+  HIP_TRY(c, hipGetLastError());
+  return EFDC_OK;
+}
+
 // KC planes of pitch doubles to a host (KCM, LCM) array
 int get_planes(efdc_ctx *c, const double *d, double *out) {
   std::memset(out, 0, sizeof(double) * (size_t)c->KCM * c->LCM);
@@ -946,6 +1007,8 @@ std::vector<Piece> record_pieces(efdc_ctx *c) {
     v.push_back({c->cqs, sizeof(double) * (size_t)c->nqsij * c->KC * c->src_nc});
   const size_t kel = (size_t)c->pitch * c->KC;
   for (int j = 0; j < A->nadj; ++j) v.push_back({A->c0[A->slot[j]], kel * c->tsz});
+  for (int j = 0; j < A->nadj; ++j)
+    if (A->cset[A->slot[j]]) v.push_back({A->cset[A->slot[j]], kel * c->tsz});
   return v;
 }
 
@@ -971,6 +1034,14 @@ int adj_record_c0(efdc_ctx *c) {
   return EFDC_OK;
 }
 
+int adj_record_settle(efdc_ctx *c, int slot, const void *con) {
+  AdjState *A = c->adj;
+  if (!A || !A->cset[slot]) return EFDC_OK;
+  HIP_TRY(c, hipMemcpyAsync(A->cset[slot], con, (size_t)c->pitch * c->KC * c->tsz,
+                            hipMemcpyDeviceToDevice, c->stream));
+  return EFDC_OK;
+}
+
 void adj_free(efdc_ctx *c) {
   AdjState *A = c->adj;
   if (!A) return;
@@ -986,6 +1057,7 @@ void adj_free(efdc_ctx *c) {
   for (int j = 0; j < EFDC_MAXC; ++j) {
     dfree_d((void *&)A->lam[j]);
     dfree_d(A->c0[j]);
+    dfree_d(A->cset[j]);
   }
   dfree_d((void *&)A->bcx);
   for (auto &p : A->lamclo) dfree_d((void *&)p);
@@ -1067,10 +1139,6 @@ int efdc_adj_enable(efdc_ctx_t *c, const efdc_adj_opts_t *o) {
       c->err = "adjoint: anti-diffusion (ISADAC) is not differentiated";
       return EFDC_ERR_ARG;
     }
-    if (c->con[sl].desc.wset > 0.0) {
-      c->err = "adjoint: settling is not differentiated";
-      return EFDC_ERR_ARG;
-    }
   }
   if (c->KC > 64) {
     c->err = "adjoint: KC > 64";
@@ -1105,6 +1173,8 @@ int efdc_adj_enable(efdc_ctx_t *c, const efdc_adj_opts_t *o) {
   for (int j = 0; j < o->nadj; ++j) {
     const int sl = o->slot[j];
     if ((s = alloc_d(c, &A->lam[sl], KP)) || (s = dev_alloc(c, &A->c0[sl], KP * c->tsz)))
+      return fail(s);
+    if (c->con[sl].desc.wset > 0.0 && (s = dev_alloc(c, &A->cset[sl], KP * c->tsz)))
       return fail(s);
   }
   if ((s = dev_alloc(c, (void **)&A->z2, L)) ||
@@ -1200,10 +1270,14 @@ int efdc_adj_step(efdc_ctx_t *c, const efdc_step_t *step, efdc_adj_stats_t *stat
   }
   int s;
   if ((s = zero(c, A->srcg, A->srcg_n))) return s;
-  // transport, one differentiated constituent after the other
+  // settling, then transport, one differentiated constituent after the other
   for (int j = 0; j < A->nadj; ++j) {
-    s = c->store == EFDC_STORE_FP64 ? adj_tran<double>(c, A, A->slot[j], step)
-                                    : adj_tran<float>(c, A, A->slot[j], step);
+    const int sl = A->slot[j];
+    s = c->store == EFDC_STORE_FP64 ? adj_settle<double>(c, A, sl, step)
+                                    : adj_settle<float>(c, A, sl, step);
+    if (s) return s;
+    s = c->store == EFDC_STORE_FP64 ? adj_tran<double>(c, A, sl, step)
+                                    : adj_tran<float>(c, A, sl, step);
     if (s) return s;
   }
   // hydrodynamics

@@ -596,6 +596,7 @@ void seg_free(efdc_ctx *c);
 bool adj_recording(const efdc_ctx *c);
 int adj_record_ps(efdc_ctx *c);
 int adj_record_c0(efdc_ctx *c);
+int adj_record_settle(efdc_ctx *c, int slot, const void *con);
 void adj_free(efdc_ctx *c);
 // FP64 CG on the context's scaled operator for another right-hand side
 // and solution vector (x holds the initial guess), with its own state

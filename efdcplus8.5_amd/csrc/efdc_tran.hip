@@ -1345,6 +1345,10 @@ int finish_tran(efdc_ctx *c, TranArgs<T> &a, const efdc_step_t *step) {
   for (int j = 0; j < c->nconst; ++j) {
     const efdc_constituent_t &d = c->con[j].desc;
     if (!(d.wset > 0.0)) continue;
+    if (adj_recording(c)) {
+      const int s = adj_record_settle(c, j, a.nw[j]);
+      if (s) return s;
+    }
     TIMED(c, TK_SETTLE, k_settle<T><<<(a.nwet + 256) / 256, 256, 0, c->stream>>>(
         a.nw[j], a.hpki, a.pitch, a.KC, a.nwet, step->dt, d.wset, d.probdep * d.wset,
         d.wese, a.lmask));

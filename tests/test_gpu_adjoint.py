@@ -15,7 +15,12 @@ The misfit J is the squared difference of layer-mean dye at observed cells
 * the checkpoint interval changes nothing (same bits), nor does keeping
   each step's recording instead of running the step again;
 * torch.autograd.gradcheck of the autograd.Function wrapper;
-* the FP32-storage forward gives the FP64-storage gradient to 1e-3.
+* the FP32-storage forward gives the FP64-storage gradient to 1e-3;
+* with settling on the observed constituent (the implicit sweep of
+  EFDC/calsed.f90:232-256 with deposition and resuspension): the load
+  gradient against central differences of the GPU model and of the CPU
+  oracle, and the Manning gradient (through HPKI of the sweep) against
+  central differences.
 """
 import numpy as np
 import pytest
@@ -224,3 +229,67 @@ def test_seed_is_ordered_after_pending_work():
         want[k_ - 1, l_ - 1] += 2 * v_
     assert np.array_equal(lam, want)
     ctx.close()
+
+
+def _settling_case():
+    case, obs, true, loads0, rng = _setup(seed=11)
+    d = case.consts[1]
+    d["wset"], d["probdep"], d["wese"] = 1e-3, 0.5, 1e-5
+    return case, obs, true, loads0, rng
+
+
+@pytest.fixture(scope="module")
+def prob_settle():
+    case, obs, true, loads0, rng = _settling_case()
+    prob = _problem(case, obs, true, rng, ckpt=6)
+    J, gl, gn, st = prob.gradient(loads0)
+    yield dict(prob=prob, case=case, obs=obs, loads0=loads0, J=J, gl=gl, gn=gn, rng=rng)
+    prob.close()
+
+
+def test_settling_changes_the_misfit(prob_settle):
+    """The settling acts: the same loads without it give another J."""
+    p = prob_settle
+    case = p["case"].copy()
+    case.consts[1]["wset"] = 0.0
+    from efdc_amd import adjoint
+    q = adjoint.DyeMisfit(case, nsteps=24, obs_L=p["obs"], obs_every=4, ckpt=6)
+    q.data = p["prob"].data
+    J0 = q.misfit(p["loads0"])
+    q.close()
+    print("J with settling", p["J"], "without", J0)
+    assert abs(J0 - p["J"]) > 1e-3 * abs(p["J"])
+
+
+def test_settling_load_gradient_vs_central_differences(prob_settle):
+    """The settled dye is affine in the loads (no clip is active where the
+    dye is positive), so J is quadratic and a central difference is exact
+    up to rounding, on the GPU model and on the FP64 CPU oracle."""
+    p = prob_settle
+    prob, l0 = p["prob"], p["loads0"]
+    d = p["rng"].standard_normal(l0.shape)
+    fd = (prob.misfit(l0 + d) - prob.misfit(l0 - d)) / 2.0
+    ad = float(np.sum(p["gl"] * d))
+    print(f"settling loads: adjoint {ad:.12e} fd {fd:.12e}")
+    assert abs(ad - fd) <= 1e-8 * abs(fd)
+    n0 = p["case"].manning_n.copy()
+    fdc = (_cpu_misfit(p["case"], p["obs"], l0 + d, n0, prob.data, 24) -
+           _cpu_misfit(p["case"], p["obs"], l0 - d, n0, prob.data, 24)) / 2.0
+    print(f"settling loads: oracle {fdc:.12e}")
+    assert abs(ad - fdc) <= 1e-7 * abs(fdc)
+
+
+def test_settling_manning_gradient_vs_central_difference(prob_settle):
+    p = prob_settle
+    prob, case, l0 = p["prob"], p["case"], p["loads0"]
+    n0 = case.manning_n.copy()
+    g = case.grid
+    L = np.arange(2, g.LA + 1) - 1
+    d = np.zeros(g.LCM)
+    d[L] = p["rng"].uniform(-1.0, 1.0, len(L)) * n0[L]
+    h = 1e-3
+    fd = (prob.misfit(l0, n0 + h * d) - prob.misfit(l0, n0 - h * d)) / (2 * h)
+    ad = float(np.sum(p["gn"] * d))
+    print(f"settling manning: adjoint {ad:.10e} fd {fd:.10e}")
+    assert abs(fd) > 0
+    assert abs(ad - fd) <= 1e-4 * abs(fd)
