@@ -55,6 +55,10 @@ struct AdjState {
   double *lam[EFDC_MAXC] = {};  // lambda of CON, KC planes (FP64)
   void *c0[EFDC_MAXC] = {};     // recorded CON before the step (storage type)
   void *cset[EFDC_MAXC] = {};   // recorded CON before the settling (storage type)
+  void *cadac[EFDC_MAXC] = {};  // recorded CON before the anti-diffusion (storage type)
+  // anti-diffusion reverse: lambda of CON after the upwind step and of
+  // CON1, and the work planes of the reverse passes (FP64)
+  double *ad[11] = {};
   double *lcd = nullptr, *own = nullptr;
   double *wk[5] = {};
   double *bcx = nullptr;
@@ -85,6 +89,11 @@ __global__ void k_fill0(double *p, size_t n) {
   if (t < n) p[t] = 0.0;
 }
 
+__global__ void k_add(double *p, const double *q, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] += q[t];
+}
+
 int zero(efdc_ctx *c, double *p, size_t n) {
   if (!p || n == 0) return EFDC_OK;
   k_fill0<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(p, n);
@@ -111,6 +120,7 @@ struct TaArgs {
   const double *wtci;
   double *lamclo[4];
   double *lam, *lcd, *own, *wcd, *wv, *wvcu, *weeb, *ww;
+  const double *zin;  // lambda of the column's update given (anti-diffusion), or null
   double *bcx;
   double *srcg;
   double *ahp_cur, *ahp_prev, *au_cur, *au_prev, *av_cur, *av_prev;
@@ -297,9 +307,11 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_adj_tran_col(TaArgs<T> a) {
   const int rec = a.bcrec ? a.bcrec[i] : -1;
   int4 bi = make_int4(-1, -1, -1, -1);
   if (rec >= 0) bi = a.bcinfo[rec];
-  // pass 1: forward values, tridiagonal forward elimination
+  // pass 1: forward values, tridiagonal forward elimination (with
+  // anti-diffusion the column finish's reverse has given lambda of the
+  // update, zin, and passes 1-3 are skipped)
   double vcu_prev = 0.0, y_prev = 0.0;
-  for (int k = 0; k < KC; ++k) {
+  for (int k = 0; k < (a.zin ? 0 : KC); ++k) {
     LayerVals L;
     layer_fwd(a, i, k, q, L);
     double con = L.cd * L.hpki;
@@ -328,7 +340,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_adj_tran_col(TaArgs<T> a) {
     y_prev = y;
   }
   // pass 2: back substitution -> solution v (before the decay factor)
-  if (KC > 1) {
+  if (KC > 1 && !a.zin) {
     double up = a.wv[i + (int64_t)(KC - 1) * P];
     for (int k = KC - 2; k >= 0; --k) {
       const int64_t o = i + (int64_t)k * P;
@@ -339,7 +351,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_adj_tran_col(TaArgs<T> a) {
   }
   // pass 3: lambda of v, then U^T w = g
   const double fdec = a.dye_mode == 1 ? a.dye_f : 1.0;
-  if (KC > 1) {
+  if (KC > 1 && !a.zin) {
     double wprev = 0.0;
     for (int k = 0; k < KC; ++k) {
       const int64_t o = i + (int64_t)k * P;
@@ -356,7 +368,9 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_adj_tran_col(TaArgs<T> a) {
     const int64_t o = i + (int64_t)k * P;
     double z;
     double l = 0.0, u = 0.0, dl = 0.0, du = 0.0;
-    if (KC == 1) {
+    if (a.zin) {
+      z = a.zin[o];
+    } else if (KC == 1) {
       z = fdec * a.lam[o];
     } else {
       vd_coef(a, i, k, hpi, l, u, dl, du);
@@ -915,8 +929,10 @@ int adj_tables(efdc_ctx *c, AdjState *A) {
 }
 
 template <typename T>
-int adj_tran(efdc_ctx *c, AdjState *A, int slot, const efdc_step_t *step) {
+int adj_tran(efdc_ctx *c, AdjState *A, int slot, const efdc_step_t *step,
+             const double *zin = nullptr) {
   TaArgs<T> a = ta_args<T>(c, A, slot, step);
+  a.zin = zin;
   const int nb = (c->n_wet() + EFDC_BLOCK - 1) / EFDC_BLOCK;
   int s;
   if ((s = zero(c, A->bcx, A->bcx_n))) return s;
@@ -938,6 +954,67 @@ int adj_settle(efdc_ctx *c, AdjState *A, int slot, const efdc_step_t *step) {
       A->hp[A->cur]));
   HIP_TRY(c, hipGetLastError());
   return EFDC_OK;
+}
+
+// Reverse of the anti-diffusion of one constituent: lambda of its CON
+// after the upwind step into ad[0], of CON1 into ad[1].
+int adj_adac(efdc_ctx *c, AdjState *A, int slot, const efdc_step_t *step) {
+  const size_t n = (size_t)c->pitch * (c->KC + 1);
+  int s;
+  for (double *p : A->ad)
+    if (!p) {
+      c->err = "adjoint: anti-diffusion work planes missing";
+      return EFDC_ERR_STATE;
+    }
+  if (!A->cadac[slot] || !A->c0[slot] || !A->lam[slot]) {
+    c->err = "adjoint: anti-diffusion recording missing";
+    return EFDC_ERR_STATE;
+  }
+  for (double *p : A->ad)
+    if ((s = zero(c, p, n))) return s;
+  AdacAdj io;
+  std::memset(&io, 0, sizeof(io));
+  io.conu = A->cadac[slot];
+  io.con1 = A->c0[slot];
+  io.lam = A->lam[slot];
+  io.lconu = A->ad[0];
+  io.lcon1 = A->ad[1];
+  io.lcd = A->ad[2];
+  io.lfu = A->ad[3];
+  io.lfv = A->ad[4];
+  io.lfw = A->ad[5];
+  io.lduu = A->ad[6];
+  io.ldvv = A->ad[7];
+  io.lu = A->ad[8];
+  io.lv = A->ad[9];
+  io.lww = A->ad[10];
+  io.wv = A->wk[1];
+  io.wvcu = A->wk[2];
+  io.weeb = A->wk[3];
+  io.ww = A->wk[4];
+  const int cu = A->cur, pv = A->cur ^ 1;
+  io.ahp = A->hp[cu];
+  io.au_c = A->u[cu];
+  io.au_p = A->u[pv];
+  io.av_c = A->v[cu];
+  io.av_p = A->v[pv];
+  io.ahu = A->hu[cu];
+  io.ahv = A->hv[cu];
+  io.hu = c->dH[EFDC_HU];
+  io.hv = c->dH[EFDC_HV];
+  io.uhdye = c->dH[EFDC_UHDYE];
+  io.uhdy1e = c->dH[EFDC_UHDY1E];
+  io.vhdxe = c->dH[EFDC_VHDXE];
+  io.vhdx1e = c->dH[EFDC_VHDX1E];
+  io.dyiu = c->dH[EFDC_DYIU];
+  io.dxiv = c->dH[EFDC_DXIV];
+  {
+    // the decay of the column finish, as in ta_args
+    const TaArgs<double> ta = ta_args<double>(c, A, slot, step);
+    io.dye_mode = ta.dye_mode;
+    io.dye_f = ta.dye_f;
+  }
+  return adj_adac_run(c, slot, io, step);
 }
 
 // KC planes of pitch doubles to a host (KCM, LCM) array
@@ -1009,6 +1086,8 @@ std::vector<Piece> record_pieces(efdc_ctx *c) {
   for (int j = 0; j < A->nadj; ++j) v.push_back({A->c0[A->slot[j]], kel * c->tsz});
   for (int j = 0; j < A->nadj; ++j)
     if (A->cset[A->slot[j]]) v.push_back({A->cset[A->slot[j]], kel * c->tsz});
+  for (int j = 0; j < A->nadj; ++j)
+    if (A->cadac[A->slot[j]]) v.push_back({A->cadac[A->slot[j]], kel * c->tsz});
   return v;
 }
 
@@ -1042,6 +1121,14 @@ int adj_record_settle(efdc_ctx *c, int slot, const void *con) {
   return EFDC_OK;
 }
 
+int adj_record_adac(efdc_ctx *c, int slot, const void *con) {
+  AdjState *A = c->adj;
+  if (!A || !A->cadac[slot]) return EFDC_OK;
+  HIP_TRY(c, hipMemcpyAsync(A->cadac[slot], con, (size_t)c->pitch * c->KC * c->tsz,
+                            hipMemcpyDeviceToDevice, c->stream));
+  return EFDC_OK;
+}
+
 void adj_free(efdc_ctx *c) {
   AdjState *A = c->adj;
   if (!A) return;
@@ -1058,7 +1145,9 @@ void adj_free(efdc_ctx *c) {
     dfree_d((void *&)A->lam[j]);
     dfree_d(A->c0[j]);
     dfree_d(A->cset[j]);
+    dfree_d(A->cadac[j]);
   }
+  for (auto &p : A->ad) dfree_d((void *&)p);
   dfree_d((void *&)A->bcx);
   for (auto &p : A->lamclo) dfree_d((void *&)p);
   dfree_d((void *&)A->srcg);
@@ -1135,10 +1224,6 @@ int efdc_adj_enable(efdc_ctx_t *c, const efdc_adj_opts_t *o) {
   for (int j = 0; j < o->nadj; ++j) {
     const int sl = o->slot[j];
     if (sl < 0 || sl >= c->nconst) return EFDC_ERR_ARG;
-    if (c->con[sl].desc.isadac > 0) {
-      c->err = "adjoint: anti-diffusion (ISADAC) is not differentiated";
-      return EFDC_ERR_ARG;
-    }
   }
   if (c->KC > 64) {
     c->err = "adjoint: KC > 64";
@@ -1176,6 +1261,12 @@ int efdc_adj_enable(efdc_ctx_t *c, const efdc_adj_opts_t *o) {
       return fail(s);
     if (c->con[sl].desc.wset > 0.0 && (s = dev_alloc(c, &A->cset[sl], KP * c->tsz)))
       return fail(s);
+    if (c->con[sl].desc.isadac > 0) {
+      if ((s = dev_alloc(c, &A->cadac[sl], KP * c->tsz))) return fail(s);
+      if (!A->ad[0])
+        for (auto &p : A->ad)
+          if ((s = alloc_d(c, &p, KP + (size_t)c->pitch))) return fail(s);
+    }
   }
   if ((s = dev_alloc(c, (void **)&A->z2, L)) ||
       (s = dev_alloc(c, (void **)&A->st, sizeof(CgState))))
@@ -1276,6 +1367,17 @@ int efdc_adj_step(efdc_ctx_t *c, const efdc_step_t *step, efdc_adj_stats_t *stat
     s = c->store == EFDC_STORE_FP64 ? adj_settle<double>(c, A, sl, step)
                                     : adj_settle<float>(c, A, sl, step);
     if (s) return s;
+    if (c->con[sl].desc.isadac > 0) {
+      if ((s = adj_adac(c, A, sl, step))) return s;
+      s = c->store == EFDC_STORE_FP64 ? adj_tran<double>(c, A, sl, step, A->ad[0])
+                                      : adj_tran<float>(c, A, sl, step, A->ad[0]);
+      if (s) return s;
+      // lambda of CON1 through the FCT bounds joins that of the old level
+      const size_t n = (size_t)c->pitch * c->KC;
+      k_add<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(A->lam[sl], A->ad[1], n);
+      HIP_TRY(c, hipGetLastError());
+      continue;
+    }
     s = c->store == EFDC_STORE_FP64 ? adj_tran<double>(c, A, sl, step)
                                     : adj_tran<float>(c, A, sl, step);
     if (s) return s;

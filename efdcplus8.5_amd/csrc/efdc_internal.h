@@ -597,6 +597,23 @@ bool adj_recording(const efdc_ctx *c);
 int adj_record_ps(efdc_ctx *c);
 int adj_record_c0(efdc_ctx *c);
 int adj_record_settle(efdc_ctx *c, int slot, const void *con);
+int adj_record_adac(efdc_ctx *c, int slot, const void *con);
+// Reverse of the anti-diffusion / FCT passes of one constituent
+// (efdc_tran.hip); the planes are FP64 with the layer pitch (KC+1 planes
+// for the vertical-face fields), the hydrodynamic lambdas those of the
+// step's state.
+struct AdacAdj {
+  const void *conu, *con1;   // recorded CON after the upwind step, CON1
+  const double *lam;         // lambda of CON after the column finish
+  double *lconu, *lcon1;     // out: lambda of CON after the upwind step, of CON1
+  double *lcd, *lfu, *lfv, *lfw, *lduu, *ldvv, *lu, *lv, *lww;  // work
+  double *wv, *wvcu, *weeb, *ww;                                 // work (KC planes)
+  double *ahp, *au_c, *au_p, *av_c, *av_p, *ahu, *ahv;
+  const double *hu, *hv, *uhdye, *uhdy1e, *vhdxe, *vhdx1e, *dyiu, *dxiv;
+  int dye_mode;
+  double dye_f;
+};
+int adj_adac_run(efdc_ctx *c, int slot, const AdacAdj &io, const efdc_step_t *step);
 void adj_free(efdc_ctx *c);
 // FP64 CG on the context's scaled operator for another right-hand side
 // and solution vector (x holds the initial guess), with its own state

@@ -1246,6 +1246,57 @@ T *kp(efdc_ctx *c, int id) {
 }
 
 
+// The fields, maps and scratch planes of the anti-diffusion passes (the
+// constituent's CON/CON1 and switches are set per constituent).
+template <typename T>
+AdacArgs<T> adac_args(efdc_ctx *c, const T *uhdy2, const T *vhdx2, const T *w2,
+                      const T *hpki, const int *bcrec, double dt) {
+  AdacArgs<T> a;
+  std::memset(&a, 0, sizeof(a));
+  a.uhdy2 = uhdy2;
+  a.vhdx2 = vhdx2;
+  a.w2 = w2;
+  a.u2 = kp<T>(c, EFDC_U2);
+  a.v2 = kp<T>(c, EFDC_V2);
+  a.dzig = (const T *)c->dW[EFDC_DZIG - EFDC_W2];
+  a.qsum = kp<T>(c, EFDC_QSUM);
+  a.sub3d = kp<T>(c, EFDC_SUB3D);
+  a.svb3d = kp<T>(c, EFDC_SVB3D);
+  const bool dry = c->prm.ISDRY > 0;
+  a.sub = dry ? c->dH[EFDC_SUB] : nullptr;
+  a.svb = dry ? c->dH[EFDC_SVB] : nullptr;
+  a.lmask = dry ? c->lmaskdry : nullptr;
+  a.hpk = kp<T>(c, EFDC_HPK);
+  a.hpki = hpki;
+  a.dxiu = c->dH[EFDC_DXIU];
+  a.dyiv = c->dH[EFDC_DYIV];
+  a.hpi = c->dH[EFDC_HPI];
+  a.dxyip = c->dH[EFDC_DXYIP];
+  a.lw = c->dLW;
+  a.le = c->dLE;
+  a.ls = c->dLS;
+  a.ln = c->dLN;
+  a.lnw = c->dLNW;
+  a.lse = c->dLSE;
+  a.bcrec = bcrec;
+  a.bcinfo = c->bcinfo;
+  a.lobflag = c->lobflag;
+  a.nwet = c->n_wet();
+  a.lc0 = c->LC - 1;
+  a.KC = c->KC;
+  a.pitch = c->pitch;
+  a.dt = dt;
+  a.uuuu = (T *)c->adac[0];
+  a.vvvv = (T *)c->adac[1];
+  a.wwww = (T *)c->adac[2];
+  a.fu = (T *)c->adac[3];
+  a.fv = (T *)c->adac[4];
+  a.fw = (T *)c->adac[5];
+  a.duu = (T *)c->adac[6];
+  a.dvv = (T *)c->adac[7];
+  return a;
+}
+
 // Anti-diffusion/FCT for each constituent that asks for it, then the
 // vertical diffusion and decay of every constituent.
 template <typename T>
@@ -1261,49 +1312,7 @@ int adac_pipeline(efdc_ctx *c, TranArgs<T> &ta, const efdc_step_t *step) {
     }
     c->adac_alloc = true;
   }
-  AdacArgs<T> a;
-  std::memset(&a, 0, sizeof(a));
-  a.uhdy2 = ta.uhdy2;
-  a.vhdx2 = ta.vhdx2;
-  a.w2 = ta.w2;
-  a.u2 = kp<T>(c, EFDC_U2);
-  a.v2 = kp<T>(c, EFDC_V2);
-  a.dzig = (const T *)c->dW[EFDC_DZIG - EFDC_W2];
-  a.qsum = kp<T>(c, EFDC_QSUM);
-  a.sub3d = kp<T>(c, EFDC_SUB3D);
-  a.svb3d = kp<T>(c, EFDC_SVB3D);
-  const bool dry = c->prm.ISDRY > 0;
-  a.sub = dry ? c->dH[EFDC_SUB] : nullptr;
-  a.svb = dry ? c->dH[EFDC_SVB] : nullptr;
-  a.lmask = dry ? c->lmaskdry : nullptr;
-  a.hpk = kp<T>(c, EFDC_HPK);
-  a.hpki = ta.hpki;
-  a.dxiu = c->dH[EFDC_DXIU];
-  a.dyiv = c->dH[EFDC_DYIV];
-  a.hpi = c->dH[EFDC_HPI];
-  a.dxyip = c->dH[EFDC_DXYIP];
-  a.lw = c->dLW;
-  a.le = c->dLE;
-  a.ls = c->dLS;
-  a.ln = c->dLN;
-  a.lnw = c->dLNW;
-  a.lse = c->dLSE;
-  a.bcrec = ta.bcrec;
-  a.bcinfo = c->bcinfo;
-  a.lobflag = c->lobflag;
-  a.nwet = nwet;
-  a.lc0 = c->LC - 1;
-  a.KC = KC;
-  a.pitch = c->pitch;
-  a.dt = step->dt;
-  a.uuuu = (T *)c->adac[0];
-  a.vvvv = (T *)c->adac[1];
-  a.wwww = (T *)c->adac[2];
-  a.fu = (T *)c->adac[3];
-  a.fv = (T *)c->adac[4];
-  a.fw = (T *)c->adac[5];
-  a.duu = (T *)c->adac[6];
-  a.dvv = (T *)c->adac[7];
+  AdacArgs<T> a = adac_args<T>(c, ta.uhdy2, ta.vhdx2, ta.w2, ta.hpki, ta.bcrec, step->dt);
   const dim3 grid((nwet + EFDC_BLOCK - 1) / EFDC_BLOCK, KC);
   auto lds_for = [&](int tb) { return (sizeof(double) + sizeof(T)) * (size_t)KC * tb; };
   int tb = TBLOCK;
@@ -1316,6 +1325,10 @@ int adac_pipeline(efdc_ctx *c, TranArgs<T> &ta, const efdc_step_t *step) {
     a.isadac = d.isadac;
     a.isfct = d.isfct;
     if (d.isadac > 0) {
+      if (adj_recording(c)) {
+        const int s = adj_record_adac(c, j, a.con);
+        if (s) return s;
+      }
       TIMED(c, TK_ADAC_GRAD, k_adac_grad<T><<<grid, EFDC_BLOCK, 0, c->stream>>>(a));
       TIMED(c, TK_ADAC_FLUX, k_adac_flux<T><<<grid, EFDC_BLOCK, 0, c->stream>>>(a));
       if (d.isfct != 0)
@@ -1573,6 +1586,561 @@ int tran_impl(efdc_ctx *c, const efdc_step_t *step) {
   return finish_tran(c, a, step);
 }
 
+// ---------------------------------------------------------------------
+// Reverse of the anti-diffusion / FCT passes of one differentiated
+// constituent (EFDC/caltran.f90:473-786) and of the column finish that
+// applies them.  The forward fields (UUUU..DVV) are recomputed from the
+// recorded CON after the upwind step and CON1 by the forward kernels; the
+// passes below run in the reverse order of the forward ones:
+//   k_adj_adac_col  : per column, the vertical diffusion and decay (the
+//                     transposed Thomas solve) and the update
+//                     CON = (CON*HPK + DT*(flux divergence))*HPKI, leaving
+//                     lambda of that update per cell (lcd)
+//   k_adj_adac_lim  : per face, lambda of the limited flux from the two
+//                     cells it enters, through the limiter to the raw flux
+//                     and to DUU/DVV of the cells whose factors it took
+//   k_adj_fct       : DUU/DVV -> CMAX/CMIN (to the CON/CON1 value the
+//                     MAX/MIN chain selected), the raw fluxes and HPKI
+//   k_adj_adac_flux : raw fluxes -> POS, UUUU/VVVV/WWWW, UHDY2/VHDX2
+//   k_adj_adac_grad : UUUU/VVVV/WWWW -> POS, U2/V2, HPI
+// Every MAX/MIN and sign switch takes the branch the forward step took.
+// A value read by several cells' formulas receives their terms by FP64
+// atomic adds (the sums' order, and so their last bits, can vary from run
+// to run); the upwind step's reverse then starts from lambda of CON after
+// the upwind step.
+// ---------------------------------------------------------------------
+template <typename T>
+struct AdjAdacK {
+  AdacArgs<T> a;
+  const T *conu, *con1, *ab, *cdzkk, *cdzkmk, *dzc;
+  const double *cdzkk_k, *cdzkmk_k;
+  int dz_uniform;
+  const unsigned char *pbcflag;
+  AdacAdj io;
+};
+
+__device__ __forceinline__ void aadd(double *p, double v) {
+  if (v != 0.0) atomicAdd(p, v);
+}
+
+// lambda of POS(j) = max(CON(j), 0) (zero at the ghost cells)
+template <typename T>
+__device__ __forceinline__ void lpos(const AdjAdacK<T> &q, int j, int64_t o, double l) {
+  const AdacArgs<T> &a = q.a;
+  if (j == 0 || j == a.lc0) return;
+  if ((double)q.conu[j + o] > 0.0) aadd(q.io.lconu + j + o, l);
+}
+
+// lambda of the limited flux across a horizontal face (d: the cell whose
+// W/S face it is, up: its W/S neighbour): to the raw flux (returned) and
+// to the DUU/DVV factors the limiter took
+template <typename T>
+__device__ __forceinline__ double lim_h_rev(const AdjAdacK<T> &q, const T *f, int d, int up,
+                                           int64_t o, double lF) {
+  const AdacArgs<T> &a = q.a;
+  if (!a.isfct) return lF;
+  const double fl = (double)f[d + o];
+  const double duu_d = (double)a.duu[d + o], dvv_d = (double)a.dvv[d + o];
+  const double duu_u = (double)a.duu[up + o], dvv_u = (double)a.dvv[up + o];
+  const double m1 = dmn(dvv_u, duu_d), m2 = dmn(duu_u, dvv_d);
+  const double l1 = lF * dmx(fl, 0.), l2 = lF * dmn(fl, 0.);
+  if (dvv_u < duu_d) aadd(q.io.ldvv + up + o, l1); else aadd(q.io.lduu + d + o, l1);
+  if (duu_u < dvv_d) aadd(q.io.lduu + up + o, l2); else aadd(q.io.ldvv + d + o, l2);
+  return lF * ((fl > 0. ? m1 : 0.) + (fl < 0. ? m2 : 0.));
+}
+
+template <typename T>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_adj_adac_col(AdjAdacK<T> q) {
+  const AdacArgs<T> &a = q.a;
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > a.nwet) return;
+  const int KC = a.KC;
+  const int64_t P = a.pitch;
+  const bool adac = !q.pbcflag[i];
+  const double DDELT = a.dt, DELTD2 = a.dt;
+  const double D = a.dxyip[i], hpi = a.hpi[i];
+  const int LE = a.le[i], LN = a.ln[i], LW = a.lw[i], LS = a.ls[i];
+  const bool eg = (LE == a.lc0), ng = (LN == a.lc0);
+  double *wv = q.io.wv, *wvcu = q.io.wvcu, *weeb = q.io.weeb, *ww = q.io.ww;
+  auto cd_of = [&](int k, double &cd) {
+    const int64_t o = (int64_t)k * P;
+    const double con = (double)q.conu[i + o];
+    if (!adac) {
+      cd = con;
+      return con;
+    }
+    const double fui = lim_h(a, a.fu, i, LW, o);
+    const double fue = eg ? 0.0 : lim_h(a, a.fu, LE, a.lw[LE], o);
+    const double fvi = lim_h(a, a.fv, i, LS, o);
+    const double fvn = ng ? 0.0 : lim_h(a, a.fv, LN, a.ls[LN], o);
+    cd = con * (double)a.hpk[i + o] +
+         DDELT * ((fui - fue + fvi - fvn) * D + (lim_w(a, i, k) - lim_w(a, i, k + 1)));
+    return cd * (double)a.hpki[i + o];
+  };
+  auto coef = [&](int k, double &l, double &u, double &dl, double &du) {
+    const int64_t o = (int64_t)k * P;
+    const double ckk = q.dz_uniform ? q.cdzkk_k[k] : (double)q.cdzkk[i + o];
+    const double ckm = q.dz_uniform ? q.cdzkmk_k[k] : (double)q.cdzkmk[i + o];
+    l = u = dl = du = 0.0;
+    if (k > 0) {
+      dl = (-DELTD2 * ckm) * (double)q.ab[i + o - P];
+      l = (-DELTD2 * ckm) * hpi * (double)q.ab[i + o - P];
+    }
+    if (k < KC - 1) {
+      du = (-DELTD2 * ckk) * (double)q.ab[i + o];
+      u = (-DELTD2 * ckk) * hpi * (double)q.ab[i + o];
+    }
+  };
+  // forward: the updated CON of every layer and the Thomas solve
+  double vcu_prev = 0.0, y_prev = 0.0;
+  for (int k = 0; k < KC; ++k) {
+    const int64_t o = i + (int64_t)k * P;
+    double cd;
+    const double con = cd_of(k, cd);
+    if (KC == 1) {
+      wv[o] = con;
+      continue;
+    }
+    double l, u, dl, du;
+    coef(k, l, u, dl, du);
+    double eeb;
+    if (k == 0) eeb = 1.0 / (1.0 - u);
+    else if (k < KC - 1) eeb = 1.0 / ((1.0 - l - u) - l * vcu_prev);
+    else eeb = 1.0 / ((1.0 - l) - l * vcu_prev);
+    const double y = (k == 0) ? con * eeb : (con - l * y_prev) * eeb;
+    wv[o] = y;
+    wvcu[o] = u * eeb;
+    weeb[o] = eeb;
+    vcu_prev = u * eeb;
+    y_prev = (double)(T)y;
+  }
+  if (KC > 1) {
+    double up = (double)(T)wv[i + (int64_t)(KC - 1) * P];
+    wv[i + (int64_t)(KC - 1) * P] = up;
+    for (int k = KC - 2; k >= 0; --k) {
+      const int64_t o = i + (int64_t)k * P;
+      const double v = (double)(T)wv[o] - wvcu[o] * up;
+      wv[o] = v;
+      up = v;
+    }
+  }
+  // reverse: decay, U^T w = g, L^T z = w
+  const double fdec = q.io.dye_mode == 1 ? q.io.dye_f : 1.0;
+  if (KC > 1) {
+    double wprev = 0.0;
+    for (int k = 0; k < KC; ++k) {
+      const int64_t o = i + (int64_t)k * P;
+      const double g = fdec * q.io.lam[o];
+      const double wk = (k == 0) ? g : g - wvcu[o - P] * wprev;
+      ww[o] = wk;
+      wprev = wk;
+    }
+  }
+  double lam_hpi = 0.0, lam_hp = 0.0, z_next = 0.0, l_next = 0.0;
+  for (int k = KC - 1; k >= 0; --k) {
+    const int64_t o = i + (int64_t)k * P;
+    double z, l = 0.0;
+    if (KC == 1) {
+      z = fdec * q.io.lam[o];
+    } else {
+      double u, dl, du;
+      coef(k, l, u, dl, du);
+      z = (ww[o] - l_next * z_next) * weeb[o];
+      const double lam_d = -z * wv[o];
+      if (k > 0) lam_hpi += (-z * wv[o - P] - lam_d) * dl;
+      if (k < KC - 1) lam_hpi += (-z * wv[o + P] - lam_d) * du;
+    }
+    z_next = z;
+    l_next = l;
+    if (!adac) {
+      q.io.lconu[o] = z;
+      q.io.lcd[o] = 0.0;
+      continue;
+    }
+    double cd;
+    (void)cd_of(k, cd);
+    const double hpk = (double)a.hpk[o], hpki = (double)a.hpki[o], dz = (double)q.dzc[o];
+    const double lcd = z * hpki;
+    q.io.lcd[o] = lcd;
+    q.io.lconu[o] = lcd * hpk;
+    // HPK = HP DZC, HPKI = 1 / (HP DZC)
+    lam_hp += lcd * (double)q.conu[o] * dz + z * cd * (-hpki * hpki * dz);
+  }
+  lam_hp += -lam_hpi * hpi * hpi;
+  q.io.ahp[i] += lam_hp;
+}
+
+// Faces owned by cell i at layer k: its W and S faces and the face above
+// layer k (K = k+1 < KC).
+template <typename T>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_adj_adac_lim(AdjAdacK<T> q) {
+  const AdacArgs<T> &a = q.a;
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > a.nwet) return;
+  const int k = blockIdx.y, KC = a.KC;
+  const int64_t P = a.pitch, o = (int64_t)k * P;
+  const double dt = a.dt;
+  const double *lcd = q.io.lcd;
+  auto lcdD = [&](int j) { return (j == 0 || j == a.lc0) ? 0.0 : lcd[j + o] * a.dxyip[j]; };
+  {
+    const int LW = a.lw[i];
+    const double lF = dt * (lcdD(i) - lcdD(LW));
+    q.io.lfu[i + o] = lim_h_rev(q, a.fu, i, LW, o, lF);
+  }
+  {
+    const int LS = a.ls[i];
+    const double lF = dt * (lcdD(i) - lcdD(LS));
+    q.io.lfv[i + o] = lim_h_rev(q, a.fv, i, LS, o, lF);
+  }
+  const int K = k + 1;  // face between layers k and k+1 (FWUU plane K)
+  if (K < KC) {
+    const double lF = dt * (lcd[i + (int64_t)K * P] - lcd[i + o]);
+    double lfl = lF;
+    if (a.isfct) {
+      if (a.lobflag[i] & 3) {
+        lfl = 0.0;
+      } else {
+        const double fl = (double)a.fw[i + (int64_t)K * P];
+        const int64_t oa = o, ob = o + P;  // layers K-1 and K (1-based K, K+1)
+        const double duuK = (double)a.duu[i + oa], dvvK = (double)a.dvv[i + oa];
+        const double duuK1 = (double)a.duu[i + ob], dvvK1 = (double)a.dvv[i + ob];
+        const double m1 = dmn(dvvK, duuK1), m2 = dmn(duuK, dvvK1);
+        const double l1 = lF * dmx(fl, 0.), l2 = lF * dmn(fl, 0.);
+        if (dvvK < duuK1) aadd(q.io.ldvv + i + oa, l1); else aadd(q.io.lduu + i + ob, l1);
+        if (duuK < dvvK1) aadd(q.io.lduu + i + oa, l2); else aadd(q.io.ldvv + i + ob, l2);
+        lfl = lF * ((fl > 0. ? m1 : 0.) + (fl < 0. ? m2 : 0.));
+      }
+    }
+    aadd(q.io.lfw + i + (int64_t)K * P, lfl);
+  }
+}
+
+// One candidate of the CMAX/CMIN chains: its value and where its lambda
+// goes (kind 1: CON after the upwind step, 2: CON1, 0: a constant).
+struct Cand {
+  double v, s;   // value, factor of the lambda
+  int kind;
+  int64_t at;
+};
+
+template <typename T>
+__device__ __forceinline__ void cand_mx_mn(const AdjAdacK<T> &q, int j, int64_t o, Cand &mx,
+                                           Cand &mn) {
+  const AdacArgs<T> &a = q.a;
+  if (j == 0 || j == a.lc0) {
+    mx = Cand{0.0, 0.0, 0, 0};
+    mn = mx;
+    return;
+  }
+  const double cu = (double)q.conu[j + o], c1 = (double)q.con1[j + o];
+  const Cand pp = cu > 0. ? Cand{cu, 1.0, 1, j + o} : Cand{0.0, 0.0, 0, 0};
+  const Cand c2 = c1 > 0. ? Cand{c1, 1.0, 2, j + o} : Cand{0.0, 0.0, 0, 0};
+  mx = pp.v > c2.v ? pp : c2;   // dmx(pp, c2)
+  mn = pp.v < c2.v ? pp : c2;   // dmn(pp, c2)
+}
+__device__ __forceinline__ Cand cmx(const Cand &x, const Cand &y) { return x.v > y.v ? x : y; }
+__device__ __forceinline__ Cand cmn(const Cand &x, const Cand &y) { return x.v < y.v ? x : y; }
+__device__ __forceinline__ Cand cscale(const Cand &x, double s, double add) {
+  return Cand{s * x.v + add, s * x.s, x.kind, x.at};
+}
+
+template <typename T>
+__device__ __forceinline__ void lcand(const AdjAdacK<T> &q, const Cand &c, double l) {
+  if (c.kind == 1) aadd(q.io.lconu + c.at, l * c.s);
+  else if (c.kind == 2) aadd(q.io.lcon1 + c.at, l * c.s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_adj_fct(AdjAdacK<T> q) {
+  const AdacArgs<T> &a = q.a;
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > a.nwet) return;
+  const int k = blockIdx.y, K = k + 1, KC = a.KC;
+  const int64_t P = a.pitch, o = (int64_t)k * P;
+  if (a.lobflag[i] & 1) return;  // DUU = DVV = 0
+  const double lduu = q.io.lduu[i + o], ldvv = q.io.ldvv[i + o];
+  if (lduu == 0.0 && ldvv == 0.0) return;
+  const double DDELT = a.dt, BSMALL = 1.0E-6;
+  const int LE = a.le[i], LW = a.lw[i], LN = a.ln[i], LS = a.ls[i];
+  // CMAX / CMIN with the candidates they took
+  Cand mx0, mn0, mxa, mna, mxb, mnb, cmax, cmin;
+  cand_mx_mn(q, i, o, mx0, mn0);
+  if (K == 1) {
+    cand_mx_mn(q, i, o + P, mxa, mna);
+    cmax = cmx(mx0, mxa);
+    cmin = cmn(mn0, mna);
+  } else if (K == KC) {
+    cand_mx_mn(q, i, o - P, mxa, mna);
+    cmax = cmx(mxa, mx0);
+    cmin = cmn(mna, mn0);
+  } else {
+    cand_mx_mn(q, i, o - P, mxa, mna);
+    cand_mx_mn(q, i, o + P, mxb, mnb);
+    cmax = cmx(mx0, cmx(mxa, mxb));
+    cmin = cmn(mn0, cmn(mna, mnb));
+  }
+  Cand mxW, mnW, mxE, mnE, mxS, mnS, mxN, mnN;
+  cand_mx_mn(q, LW, o, mxW, mnW);
+  cand_mx_mn(q, LE, o, mxE, mnE);
+  cand_mx_mn(q, LS, o, mxS, mnS);
+  cand_mx_mn(q, LN, o, mxN, mnN);
+  const double sbi = (double)a.sub3d[i + o], sbe = (double)a.sub3d[LE + o];
+  const double svi = (double)a.svb3d[i + o], svn = (double)a.svb3d[LN + o];
+  Cand CMAXT = cmx(cscale(mxN, svn, 0.0), cscale(mxE, sbe, 0.0));
+  CMAXT = cmx(CMAXT, cscale(mxS, svi, 0.0));
+  CMAXT = cmx(CMAXT, cscale(mxW, sbi, 0.0));
+  cmax = cmx(cmax, CMAXT);
+  Cand CMINT = cmn(cscale(mnN, svn, 1.E+6 * (1. - svn)), cscale(mnE, sbe, 1.E+6 * (1. - sbe)));
+  CMINT = cmn(CMINT, cscale(mnS, svi, 1.E+6 * (1. - svi)));
+  CMINT = cmn(CMINT, cscale(mnW, sbi, 1.E+6 * (1. - sbi)));
+  cmin = cmn(cmin, CMINT);
+  const double fui = (double)a.fu[i + o], fue = (double)a.fu[LE + o];
+  const double fvi = (double)a.fv[i + o], fvn = (double)a.fv[LN + o];
+  const double fwb = (double)a.fw[i + (int64_t)(K - 1) * P], fwt = (double)a.fw[i + (int64_t)K * P];
+  const double dxyip = a.dxyip[i], hpki = (double)a.hpki[i + o];
+  const double XU = DDELT * (dxyip * (dmx(fui, 0.) - dmn(fue, 0.) + dmx(fvi, 0.) - dmn(fvn, 0.)) +
+                             (dmx(fwb, 0.) - dmn(fwt, 0.)));
+  const double XV = DDELT * (dxyip * (dmx(fue, 0.) - dmn(fui, 0.) + dmx(fvn, 0.) - dmn(fvi, 0.)) +
+                             (dmx(fwt, 0.) - dmn(fwb, 0.)));
+  const double duu_raw = XU * hpki, dvv_raw = XV * hpki;
+  const double cu = (double)q.conu[i + o], pc = dmx(cu, 0.);
+  double l_pc = 0.0, l_cmax = 0.0, l_cmin = 0.0, l_duu = 0.0, l_dvv = 0.0;
+  if (duu_raw > 0.) {
+    const double den = duu_raw + BSMALL, r = (cmax.v - pc) / den;
+    if (r < 1.) {
+      l_cmax += lduu / den;
+      l_pc -= lduu / den;
+      l_duu -= lduu * r / den;
+    }
+  } else {
+    l_duu += lduu;  // DUU <= 0 < 1
+  }
+  if (dvv_raw > 0.) {
+    const double den = dvv_raw + BSMALL, r = (pc - cmin.v) / den;
+    if (r < 1.) {
+      l_pc += ldvv / den;
+      l_cmin -= ldvv / den;
+      l_dvv -= ldvv * r / den;
+    }
+  } else {
+    l_dvv += ldvv;
+  }
+  lcand(q, cmax, l_cmax);
+  lcand(q, cmin, l_cmin);
+  if (cu > 0.) aadd(q.io.lconu + i + o, l_pc);
+  // the raw fluxes and HPKI
+  const double lxu = l_duu * hpki, lxv = l_dvv * hpki;
+  const double gu = lxu * DDELT, gv = lxv * DDELT;
+  aadd(q.io.lfu + i + o, gu * dxyip * (fui > 0.) - gv * dxyip * (fui < 0.));
+  aadd(q.io.lfu + LE + o, -gu * dxyip * (fue < 0.) + gv * dxyip * (fue > 0.));
+  aadd(q.io.lfv + i + o, gu * dxyip * (fvi > 0.) - gv * dxyip * (fvi < 0.));
+  aadd(q.io.lfv + LN + o, -gu * dxyip * (fvn < 0.) + gv * dxyip * (fvn > 0.));
+  aadd(q.io.lfw + i + (int64_t)(K - 1) * P, gu * (fwb > 0.) - gv * (fwb < 0.));
+  aadd(q.io.lfw + i + (int64_t)K * P, -gu * (fwt < 0.) + gv * (fwt > 0.));
+  const double lhpki = l_duu * XU + l_dvv * XV;
+  aadd(q.io.ahp + i, lhpki * (-hpki * hpki * (double)q.dzc[i + o]));
+}
+
+template <typename T>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_adj_adac_flux(AdjAdacK<T> q) {
+  const AdacArgs<T> &a = q.a;
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > a.nwet) return;
+  const int k = blockIdx.y, K = k + 1;
+  const int64_t o = (int64_t)k * a.pitch, p = a.pitch, oK = (int64_t)K * p;
+  const double DDELTA = a.dt, BSMALL = 1.0E-6;
+  const int LE = a.le[i], LW = a.lw[i], LN = a.ln[i], LS = a.ls[i];
+  const int LNW = a.lnw[i], LSE = a.lse[i];
+  const T *U = a.uuuu, *V = a.vvvv, *Wz = a.wwww;
+  auto u = [&](int j, int64_t off) { return (double)U[j + off]; };
+  auto v = [&](int j, int64_t off) { return (double)V[j + off]; };
+  auto w = [&](int j, int KK) { return (double)Wz[j + (int64_t)KK * p]; };
+  auto lu = [&](int j, int64_t off, double l) { aadd(q.io.lu + j + off, l); };
+  auto lv = [&](int j, int64_t off, double l) { aadd(q.io.lv + j + off, l); };
+  auto lw = [&](int j, int KK, double l) { aadd(q.io.lww + j + (int64_t)KK * p, l); };
+  const double pc = posv(a, i, o), pw = posv(a, LW, o), ps = posv(a, LS, o);
+  // the zeroing of the forward flux (sources, open boundaries)
+  bool zu = false, zv = false, zw = false;
+  if (a.isadac == 1) {
+    if (src_at(a, i, o) || src_at(a, LW, o)) zu = true;
+    if (src_at(a, i, o) || src_at(a, LS, o)) zv = true;
+    if (src_at(a, i, o)) zw = true;
+  }
+  if (bc_side(a, LS, 0) || bc_side(a, i, 3)) zv = true;
+  if (bc_side(a, LW, 1) || bc_side(a, i, 2)) zu = true;
+  const double dzc = (double)q.dzc[i + o];
+  // U component
+  const double lfu = zu ? 0.0 : q.io.lfu[i + o];
+  if (lfu != 0.0) {
+    const double uh = (double)a.uhdy2[i + o];
+    double S;
+    if (uh >= 0.0) S = v(LNW, o) + v(LW, o) + w(LW, K) + w(LW, K - 1) + u(i, o) + u(LW, o);
+    else S = v(LN, o) + v(i, o) + w(i, K) + w(i, K - 1) + u(i, o) + u(LE, o);
+    const double UTERM = fabs(uh) * (pc - pw) - 0.5 * DDELTA * uh * S;
+    const double den = pc + pw + BSMALL, UHU = UTERM / den;
+    const double lUHU = lfu * ((UHU > 0. ? pw : 0.) + (UHU < 0. ? pc : 0.));
+    double l_pw = lfu * dmx(UHU, 0.), l_pc = lfu * dmn(UHU, 0.);
+    const double lT = lUHU / den;
+    l_pc += -lUHU * UHU / den + lT * fabs(uh);
+    l_pw += -lUHU * UHU / den - lT * fabs(uh);
+    const double luh = lT * ((uh >= 0.0 ? 1.0 : -1.0) * (pc - pw) - 0.5 * DDELTA * S);
+    const double lS = -0.5 * DDELTA * uh * lT;
+    if (uh >= 0.0) {
+      lv(LNW, o, lS); lv(LW, o, lS); lw(LW, K, lS); lw(LW, K - 1, lS); lu(i, o, lS); lu(LW, o, lS);
+    } else {
+      lv(LN, o, lS); lv(i, o, lS); lw(i, K, lS); lw(i, K - 1, lS); lu(i, o, lS); lu(LE, o, lS);
+    }
+    lpos(q, i, o, l_pc);
+    lpos(q, LW, o, l_pw);
+    // UHDY2 = (UHDYE + UHDY1E) DZC / 2
+    aadd(q.io.au_c + i, 0.5 * dzc * luh);
+    aadd(q.io.au_p + i, 0.5 * dzc * luh);
+  }
+  // V component
+  const double lfv = zv ? 0.0 : q.io.lfv[i + o];
+  if (lfv != 0.0) {
+    const double vh = (double)a.vhdx2[i + o];
+    double S;
+    if (vh >= 0.0) S = u(LS, o) + u(LSE, o) + w(LS, K) + w(LS, K - 1) + v(LS, o) + v(i, o);
+    else S = u(i, o) + u(LE, o) + w(i, K) + w(i, K - 1) + v(LN, o) + v(i, o);
+    const double VTERM = fabs(vh) * (pc - ps) - 0.5 * DDELTA * vh * S;
+    const double den = pc + ps + BSMALL, VHV = VTERM / den;
+    const double lVHV = lfv * ((VHV > 0. ? ps : 0.) + (VHV < 0. ? pc : 0.));
+    double l_ps = lfv * dmx(VHV, 0.), l_pc = lfv * dmn(VHV, 0.);
+    const double lT = lVHV / den;
+    l_pc += -lVHV * VHV / den + lT * fabs(vh);
+    l_ps += -lVHV * VHV / den - lT * fabs(vh);
+    const double lvh = lT * ((vh >= 0.0 ? 1.0 : -1.0) * (pc - ps) - 0.5 * DDELTA * S);
+    const double lS = -0.5 * DDELTA * vh * lT;
+    if (vh >= 0.0) {
+      lu(LS, o, lS); lu(LSE, o, lS); lw(LS, K, lS); lw(LS, K - 1, lS); lv(LS, o, lS); lv(i, o, lS);
+    } else {
+      lu(i, o, lS); lu(LE, o, lS); lw(i, K, lS); lw(i, K - 1, lS); lv(LN, o, lS); lv(i, o, lS);
+    }
+    lpos(q, i, o, l_pc);
+    lpos(q, LS, o, l_ps);
+    aadd(q.io.av_c + i, 0.5 * dzc * lvh);
+    aadd(q.io.av_p + i, 0.5 * dzc * lvh);
+  }
+  // W component (FWUU plane K, K < KC); W2 is an input field here
+  if (K < a.KC && !zw) {
+    const double lfw = q.io.lfw[i + oK];
+    if (lfw != 0.0) {
+      const double pu = posv(a, i, o + p);
+      const double ww = (double)a.w2[i + oK];
+      double S;
+      if (ww >= 0.0) S = u(i, o) + u(LE, o) + v(i, o) + v(LN, o) + w(i, K) + w(i, K - 1);
+      else S = u(i, o + p) + u(LE, o + p) + v(i, o + p) + v(LN, o + p) + w(i, K) + w(i, K + 1);
+      const double WTERM = fabs(ww) * (pu - pc) - 0.5 * DDELTA * ww * S;
+      const double den = pu + pc + BSMALL, WW = WTERM / den;
+      const double lWW = lfw * ((WW > 0. ? pc : 0.) + (WW < 0. ? pu : 0.));
+      double l_pc = lfw * dmx(WW, 0.), l_pu = lfw * dmn(WW, 0.);
+      const double lT = lWW / den;
+      l_pu += -lWW * WW / den + lT * fabs(ww);
+      l_pc += -lWW * WW / den - lT * fabs(ww);
+      const double lS = -0.5 * DDELTA * ww * lT;
+      if (ww >= 0.0) {
+        lu(i, o, lS); lu(LE, o, lS); lv(i, o, lS); lv(LN, o, lS); lw(i, K, lS); lw(i, K - 1, lS);
+      } else {
+        lu(i, o + p, lS); lu(LE, o + p, lS); lv(i, o + p, lS); lv(LN, o + p, lS);
+        lw(i, K, lS); lw(i, K + 1, lS);
+      }
+      lpos(q, i, o, l_pc);
+      lpos(q, i, o + p, l_pu);
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(EFDC_BLOCK) void k_adj_adac_grad(AdjAdacK<T> q) {
+  const AdacArgs<T> &a = q.a;
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > a.nwet) return;
+  const int k = blockIdx.y;
+  const int64_t o = (int64_t)k * a.pitch, p = a.pitch;
+  const double pc = posv(a, i, o);
+  const int LW = a.lw[i], LS = a.ls[i];
+  double l_pc = 0.0;
+  const double lU = q.io.lu[i + o], lV = q.io.lv[i + o];
+  if (lU != 0.0) {
+    const double u2 = (double)a.u2[i + o], pw = posv(a, LW, o);
+    const double g = lU * u2 * a.dxiu[i];
+    l_pc += g;
+    lpos(q, LW, o, -g);
+    // U2 = (UHDYE + UHDY1E)/2 * DYIU / HU, the same in every layer
+    const double lu2 = lU * (pc - pw) * a.dxiu[i];
+    const double hu = q.io.hu[i], ub = q.io.uhdye[i] * 0.5 + q.io.uhdy1e[i] * 0.5;
+    aadd(q.io.au_c + i, 0.5 * lu2 * q.io.dyiu[i] / hu);
+    aadd(q.io.au_p + i, 0.5 * lu2 * q.io.dyiu[i] / hu);
+    aadd(q.io.ahu + i, -lu2 * ub * q.io.dyiu[i] / (hu * hu));
+  }
+  if (lV != 0.0) {
+    const double v2 = (double)a.v2[i + o], ps = posv(a, LS, o);
+    const double g = lV * v2 * a.dyiv[i];
+    l_pc += g;
+    lpos(q, LS, o, -g);
+    const double lv2 = lV * (pc - ps) * a.dyiv[i];
+    const double hv = q.io.hv[i], vb = q.io.vhdxe[i] * 0.5 + q.io.vhdx1e[i] * 0.5;
+    aadd(q.io.av_c + i, 0.5 * lv2 * q.io.dxiv[i] / hv);
+    aadd(q.io.av_p + i, 0.5 * lv2 * q.io.dxiv[i] / hv);
+    aadd(q.io.ahv + i, -lv2 * vb * q.io.dxiv[i] / (hv * hv));
+  }
+  if (k < a.KC - 1) {  // WWWW plane K = k+1
+    const int64_t oK = (int64_t)(k + 1) * p;
+    const double lW = q.io.lww[i + oK];
+    if (lW != 0.0) {
+      const double w2 = (double)a.w2[i + oK], dzig = (double)a.dzig[i + oK];
+      const double pu = posv(a, i, o + p), hpi = a.hpi[i];
+      const double g = lW * w2 * hpi * dzig;
+      lpos(q, i, o + p, g);
+      l_pc -= g;
+      // HPI = 1/HP
+      aadd(q.io.ahp + i, -lW * w2 * (pu - pc) * dzig * hpi * hpi);
+    }
+  }
+  lpos(q, i, o, l_pc);
+}
+
+
+template <typename T>
+int adj_adac_impl(efdc_ctx *c, int slot, const AdacAdj &io, const efdc_step_t *step) {
+  const efdc_constituent_t &d = c->con[slot].desc;
+  const int KC = c->KC, nwet = c->n_wet();
+  const int *bcrec = c->nbcrec > 0 ? c->bcrec : nullptr;
+  AdjAdacK<T> q;
+  std::memset(&q, 0, sizeof(q));
+  q.a = adac_args<T>(c, kp<T>(c, EFDC_UHDY2), kp<T>(c, EFDC_VHDX2), (const T *)c->dW[0],
+                     kp<T>(c, EFDC_HPKI), bcrec, step->dt);
+  q.a.con = (const T *)io.conu;
+  q.a.con1 = (const T *)io.con1;
+  q.a.isadac = d.isadac;
+  q.a.isfct = d.isfct;
+  q.conu = (const T *)io.conu;
+  q.con1 = (const T *)io.con1;
+  q.ab = kp<T>(c, EFDC_AB);
+  q.cdzkk = kp<T>(c, EFDC_CDZKK);
+  q.cdzkmk = kp<T>(c, EFDC_CDZKMK);
+  q.dzc = kp<T>(c, EFDC_DZC);
+  q.cdzkk_k = c->cdzkk_k;
+  q.cdzkmk_k = c->cdzkmk_k;
+  q.dz_uniform = c->dz_uniform() ? 1 : 0;
+  q.pbcflag = c->pbcflag;
+  q.io = io;
+  const dim3 grid((nwet + EFDC_BLOCK - 1) / EFDC_BLOCK, KC);
+  const int nb = (nwet + EFDC_BLOCK - 1) / EFDC_BLOCK;
+  // the forward fields from the recorded levels
+  k_adac_grad<T><<<grid, EFDC_BLOCK, 0, c->stream>>>(q.a);
+  k_adac_flux<T><<<grid, EFDC_BLOCK, 0, c->stream>>>(q.a);
+  if (d.isfct != 0) k_fct_beta<T><<<grid, EFDC_BLOCK, 0, c->stream>>>(q.a);
+  TIMED(c, TK_ADJ_TCOL, k_adj_adac_col<T><<<nb, EFDC_BLOCK, 0, c->stream>>>(q));
+  TIMED(c, TK_ADJ_TCOL, k_adj_adac_lim<T><<<grid, EFDC_BLOCK, 0, c->stream>>>(q));
+  if (d.isfct != 0) TIMED(c, TK_ADJ_TCOL, k_adj_fct<T><<<grid, EFDC_BLOCK, 0, c->stream>>>(q));
+  TIMED(c, TK_ADJ_TCOL, k_adj_adac_flux<T><<<grid, EFDC_BLOCK, 0, c->stream>>>(q));
+  TIMED(c, TK_ADJ_TCOL, k_adj_adac_grad<T><<<grid, EFDC_BLOCK, 0, c->stream>>>(q));
+  HIP_TRY(c, hipGetLastError());
+  return EFDC_OK;
+}
+
 }  // namespace
 
 int tran_run(efdc_ctx *c, const efdc_step_t *step) {
@@ -1643,4 +2211,13 @@ int closure_post_run(efdc_ctx *c) {
         c->pitch, c->KC, nwet));
   HIP_TRY(c, hipGetLastError());
   return EFDC_OK;
+}
+
+int adj_adac_run(efdc_ctx *c, int slot, const AdacAdj &io, const efdc_step_t *step) {
+  if (!c->adac_alloc) {
+    c->err = "adjoint: anti-diffusion fields were never formed (no forward step with ISADAC)";
+    return EFDC_ERR_STATE;
+  }
+  return c->store == EFDC_STORE_FP64 ? adj_adac_impl<double>(c, slot, io, step)
+                                     : adj_adac_impl<float>(c, slot, io, step);
 }

@@ -689,9 +689,11 @@ int efdc_state_free(efdc_ctx_t *ctx, void *dev_buf);
    dJ/dn (Manning, per cell) and leaves dJ/dCQS of that step.  The solve
    A P = FP is reversed by a second CG on the same operator (A is symmetric
    on the cells that are not clamped open boundaries); the settling sweep
-   of a constituent with wset > 0 is reversed before its transport.
-   Requirements: clamped pressure boundaries (ISPB=0), no anti-diffusion on
-   the adjoint constituents, one basin, mutual neighbour maps.
+   of a constituent with wset > 0 is reversed before its transport, and
+   the anti-diffusion / FCT passes (ISADAC, ISFCT) of one before its upwind
+   step (their atomic sums can vary in the last bits from run to run).
+   Requirements: clamped pressure boundaries (ISPB=0), one basin, mutual
+   neighbour maps.
    Use: enable; run the forward step from the state before it (the context
    records the pieces of the step the adjoint reads: the CG solution and
    the old levels of the adjoint constituents); add the misfit gradient of

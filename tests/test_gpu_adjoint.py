@@ -20,7 +20,11 @@ The misfit J is the squared difference of layer-mean dye at observed cells
   EFDC/calsed.f90:232-256 with deposition and resuspension): the load
   gradient against central differences of the GPU model and of the CPU
   oracle, and the Manning gradient (through HPKI of the sweep) against
-  central differences.
+  central differences;
+* with anti-diffusion, with and without FCT (EFDC/caltran.f90:473-786), on
+  a dye block whose edges the limiter has to clip (Courant numbers ~0.3; J
+  changes 20-fold with the limiter): the same checks, the Manning one with a
+  step small enough for the strongly nonlinear J.
 """
 import numpy as np
 import pytest
@@ -30,9 +34,9 @@ pytestmark = pytest.mark.gpu
 NSRC = 3
 
 
-def _setup(nsteps=24, seed=7, kc=3, nx=24, ny=12):
+def _setup(nsteps=24, seed=7, kc=3, nx=24, ny=12, dt=5.0):
     from efdc_amd import grids
-    base = grids.tidal_basin(nx, ny, kc, seed=seed)
+    base = grids.tidal_basin(nx, ny, kc, seed=seed, dt=dt)
     g = base.grid
     rng = np.random.default_rng(seed)
     west = set(int(x) for x in base.open_bc["side"][0]["L"])
@@ -293,3 +297,93 @@ def test_settling_manning_gradient_vs_central_difference(prob_settle):
     print(f"settling manning: adjoint {ad:.10e} fd {fd:.10e}")
     assert abs(fd) > 0
     assert abs(ad - fd) <= 1e-4 * abs(fd)
+
+
+def _adac_case(isfct):
+    """Anti-diffusion on the observed dye, which starts as a block of 4
+    units in the middle of the basin, so that the antidiffusive fluxes
+    overshoot at its edges and the FCT limiter acts."""
+    case, obs, true, loads0, rng = _setup(seed=5, dt=60.0)
+    g = case.grid
+    d = case.consts[1]
+    d["isadac"], d["isfct"] = 1, isfct
+    L = np.arange(2, g.LA + 1)
+    I, J = g.IL[L - 1], g.JL[L - 1]
+    inb = (I >= 8) & (I <= 15) & (J >= 4) & (J <= 8)
+    blk = L[inb]
+    d["con"][:g.KC, blk - 1] = 4.0
+    d["con1"][:g.KC, blk - 1] = 4.0
+    # the cells around the block are observed too
+    ring = L[(I >= 7) & (I <= 16) & (J >= 3) & (J <= 9) & ~inb]
+    obs = np.array(sorted(set(obs.tolist()) | set(int(x) for x in ring)), np.int32)
+    return case, obs, true, loads0, rng
+
+
+@pytest.fixture(scope="module", params=[0, 1], ids=["adac", "adac_fct"])
+def prob_adac(request):
+    case, obs, true, loads0, rng = _adac_case(request.param)
+    prob = _problem(case, obs, true, rng, ckpt=6)
+    J, gl, gn, st = prob.gradient(loads0)
+    yield dict(prob=prob, case=case, obs=obs, loads0=loads0, J=J, gl=gl, gn=gn, rng=rng,
+               isfct=request.param)
+    prob.close()
+
+
+def test_adac_load_gradient_vs_central_differences(prob_adac):
+    """Anti-diffusion (and FCT) make the dye nonlinear in the loads: the
+    central difference of the GPU model with steps of 1e-3 of the loads
+    (O(h^2) truncation, no MAX/MIN branch changes within the step) and the
+    same difference of the FP64 CPU oracle must match the adjoint's
+    directional derivative."""
+    p = prob_adac
+    prob, l0 = p["prob"], p["loads0"]
+    assert np.any(p["gl"] != 0)
+    h = 1e-3
+    for trial in range(2):
+        d = p["rng"].standard_normal(l0.shape)
+        fd = (prob.misfit(l0 + h * d) - prob.misfit(l0 - h * d)) / (2 * h)
+        ad = float(np.sum(p["gl"] * d))
+        print(f"adac isfct={p['isfct']} loads: adjoint {ad:.12e} fd {fd:.12e} "
+              f"rel {abs(ad - fd) / abs(fd):.2e}")
+        assert abs(ad - fd) <= 1e-5 * abs(fd)
+    n0 = p["case"].manning_n.copy()
+    fdc = (_cpu_misfit(p["case"], p["obs"], l0 + h * d, n0, prob.data, 24) -
+           _cpu_misfit(p["case"], p["obs"], l0 - h * d, n0, prob.data, 24)) / (2 * h)
+    print(f"adac isfct={p['isfct']} loads: oracle {fdc:.12e}")
+    assert abs(ad - fdc) <= 1e-5 * abs(fdc)
+
+
+def test_fct_limiter_acts(prob_adac):
+    """The limiter changes the misfit (so the FCT terms of the reverse are
+    exercised by the gradient checks)."""
+    p = prob_adac
+    if not p["isfct"]:
+        pytest.skip("the FCT case compares itself with the plain one")
+    from efdc_amd import adjoint
+    case, obs, true, loads0, rng = _adac_case(0)
+    q = adjoint.DyeMisfit(case, nsteps=24, obs_L=obs, obs_every=4, ckpt=6)
+    q.data = p["prob"].data
+    J0 = q.misfit(p["loads0"])
+    q.close()
+    print("J with FCT", p["J"], "without", J0)
+    assert abs(J0 - p["J"]) > 1e-3 * abs(p["J"])
+
+
+def test_adac_manning_gradient_vs_central_difference(prob_adac):
+    p = prob_adac
+    prob, case, l0 = p["prob"], p["case"], p["loads0"]
+    n0 = case.manning_n.copy()
+    g = case.grid
+    L = np.arange(2, g.LA + 1) - 1
+    d = np.zeros(g.LCM)
+    d[L] = p["rng"].uniform(-1.0, 1.0, len(L)) * n0[L]
+    # the central difference converges as h^2 to the adjoint (1e-5 of it
+    # at h = 1e-3, 1e-7 at 1e-4, 1e-8 at 1e-5, with and without FCT): the
+    # anti-diffusion makes J strongly nonlinear in n, so the step is small
+    h = 1e-5
+    fd = (prob.misfit(l0, n0 + h * d) - prob.misfit(l0, n0 - h * d)) / (2 * h)
+    ad = float(np.sum(p["gn"] * d))
+    print(f"adac isfct={p['isfct']} manning: adjoint {ad:.10e} fd {fd:.10e} "
+          f"rel {abs(ad - fd) / abs(fd):.2e}")
+    assert abs(fd) > 0
+    assert abs(ad - fd) <= 1e-6 * abs(fd)
