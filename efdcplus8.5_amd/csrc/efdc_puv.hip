@@ -396,17 +396,21 @@ struct Rotated {
 // Start of a CALPUV2C step: reset the flags, and give each rotated name
 // back the ghost entries (L=1 and L=LC) it had before the rotation -- the
 // reference's shifts only copy the wet cells.
+// One thread per swap (6 pairs and the triple, at each of the two ghost
+// cells), so that the 14 load/store round trips run side by side.
 __global__ void k_step_begin(CgState *st, Rotated r) {
-  st->symok = 0;
-  st->negdep = 0;
-  const int gs[2] = {0, r.lc0};
-  for (int k = 0; k < 2; ++k) {
-    const int g = gs[k];
-    for (int q = 0; q < 6; ++q) {
-      const double a = r.pair[q][0][g], b = r.pair[q][1][g];
-      r.pair[q][0][g] = b;
-      r.pair[q][1][g] = a;
-    }
+  const int t = threadIdx.x;
+  if (t == 0) {
+    st->symok = 0;
+    st->negdep = 0;
+  }
+  if (t >= 14) return;
+  const int g = (t & 1) ? r.lc0 : 0, q = t >> 1;
+  if (q < 6) {
+    const double a = r.pair[q][0][g], b = r.pair[q][1][g];
+    r.pair[q][0][g] = b;
+    r.pair[q][1][g] = a;
+  } else {
     const double hp = r.tri[1][g], h1p = r.tri[2][g], h2p = r.tri[0][g];
     r.tri[0][g] = hp;
     r.tri[1][g] = h1p;
@@ -1766,7 +1770,7 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
     }
   }
   uvw_step_begin(c);
-  k_step_begin<<<1, 1, 0, c->stream>>>(c->st, rot);
+  k_step_begin<<<1, 64, 0, c->stream>>>(c->st, rot);
   if (dry) {
     k_dry_begin<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->iscdry, c->sub1, c->svb1,
                                                   c->prm.NDRYSTP < 0 ? c->qdwaste : nullptr,
