@@ -42,6 +42,7 @@
 // flag and leaves, the others see the flag and leave too, and the host
 // reports the failure.  All workgroups compute the same sums from the
 // same partials in the same order, so they take the same branches.
+#include <algorithm>
 #include <cstdio>
 
 #include "efdc_internal.h"
@@ -868,6 +869,334 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
   }
 }
 
+// Wait until the workgroups lo..hi (the owners of this workgroup's outside
+// cells) have raised the flag of exchange `tag`, then fetch one word of each
+// outside cell into raw[h].  Wave 0 polls (lane l: workgroup lo+l); false if
+// the wait was abandoned.
+__device__ bool halo_wait(const Xch &x, unsigned tag, int lo, int hi, const int *hl, int nhalo,
+                          unsigned *raw, const Bar &b) {
+  __shared__ int s_ok2;
+  const int t = threadIdx.x, lane = t & 63;
+  if (t < 64) {
+    unsigned spins = 0;
+    int ok = 1;
+    for (;;) {
+      bool ready = true;
+      for (int q = lo + lane; q <= hi; q += 64) ready = ready && get_u32(&x.flag[q * TICKET_PAD]) >= tag;
+      if (__all(ready)) break;
+      ++spins;
+      const int ab = (spins & 255u) == 0u
+                         ? __builtin_amdgcn_readfirstlane(
+                               __hip_atomic_load(b.abort, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+                         : 0;
+      if (spins > SPIN_MAX || ab) {
+        if (lane == 0)
+          __hip_atomic_store(b.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (lane == 0) s_ok2 = ok;
+  }
+  __syncthreads();
+  if (!s_ok2) return false;
+  const unsigned *w32 = x.hx + (size_t)(tag & 1u) * HX_WORDS * (size_t)x.ncell;
+  constexpr int HB = 4;
+  for (int h0 = t; h0 < nhalo; h0 += HB * PB) {
+    unsigned v[HB];
+#pragma unroll
+    for (int u = 0; u < HB; ++u) {
+      const int h = h0 + u * PB;
+      v[u] = h < nhalo ? get_u32(w32 + (size_t)hl[h]) : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < HB; ++u) {
+      const int h = h0 + u * PB;
+      if (h < nhalo) raw[h] = v[u];
+    }
+  }
+  __syncthreads();
+  return true;
+}
+
+// ---------------------------------------------------------------------
+// Mixed, pipelined: the same FP32 inner solve in the pipelined form of the
+// preconditioned Chronopoulos-Gear CG (Ghysels and Vanroose, 2014): each
+// iteration posts the partial sums of (r,u), (w,u), (r,r) and one halo word
+// (m = M^-1 w), waits only for the workgroups that own its outside cells,
+// forms n = A' m, and then waits for the sums of every workgroup -- which
+// were posted one SpMV earlier -- to take alpha and beta:
+//   z = n + beta z, q = m + beta q, s = w + beta s, p = u + beta p,
+//   y += alpha p, r -= alpha s, u -= alpha q, w -= alpha z.
+// The iterates are those of k_pcgmx in exact arithmetic; the outer FP64
+// refinement is the same.
+// ---------------------------------------------------------------------
+template <int CPT>
+__global__ __launch_bounds__(PB) void k_pcgmx_pl(PMxArgs a) {
+  __shared__ float lds[2 * WIN + CPT * PB];
+  __shared__ int hl[2 * WIN];
+  __shared__ unsigned raw[HMAX];
+  // y, q = M^-1 s and z = A' q of the own rows live in LDS
+  __shared__ float lys[CPT * PB], lqs[CPT * PB], lzs[CPT * PB];
+  const int b0 = 1 + (int)blockIdx.x * CPT * PB;
+  const int b1 = min(b0 + CPT * PB, a.nwet + 1);
+  const int lc0 = a.lc0;
+
+  float as_[CPT], an_[CPT], aw_[CPT], ae_[CPT];
+  float r[CPT], u[CPT], w[CPT], s[CPT], p[CPT];
+  auto Y = [&](int k) -> float & { return lys[k * PB + threadIdx.x]; };
+  auto Q = [&](int k) -> float & { return lqs[k * PB + threadIdx.x]; };
+  auto Z = [&](int k) -> float & { return lzs[k * PB + threadIdx.x]; };
+  int js[CPT], jn[CPT];
+  unsigned expo = 0u;
+
+  auto inside = [&](int j) { return j >= b0 && j < b1; };
+  auto real = [&](int j) { return j > 0 && j != lc0; };
+  auto slot = [&](int j) { return j - b0 + WIN; };
+  auto rval = [&](int j) { return real(j) ? lds[slot(j)] : 0.0f; };
+  auto hxw = [&](int i, unsigned tag) {
+    return a.xc.hx + (size_t)(tag & 1u) * HX_WORDS * (size_t)a.xc.ncell + (size_t)opaque(i);
+  };
+  // M^-1 v by the local Neumann polynomial, as in k_pcgmx
+  auto precond = [&](const float (&v)[CPT], float (&t)[CPT]) {
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) t[k] = v[k];
+    for (int d = 1; d < a.pdeg; ++d) {
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) {
+        const int i = b0 + k * PB + (int)threadIdx.x;
+        if (i < b1) lds[slot(i)] = t[k];
+      }
+      __syncthreads();
+      float tn[CPT];
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) {
+        const int i = b0 + k * PB + (int)threadIdx.x;
+        tn[k] = t[k];
+        if (i < b1) {
+          const float ts = inside(js[k]) ? lds[slot(js[k])] : 0.0f;
+          const float tnn = inside(jn[k]) ? lds[slot(jn[k])] : 0.0f;
+          const float tw = inside(i - 1) ? lds[slot(i - 1)] : 0.0f;
+          const float te = inside(i + 1) ? lds[slot(i + 1)] : 0.0f;
+          tn[k] = v[k] - (as_[k] * ts + an_[k] * tnn + aw_[k] * tw + ae_[k] * te);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) t[k] = tn[k];
+    }
+    __syncthreads();
+  };
+  // y = A' x on the own rows from the LDS window (own slots and outside cells)
+  auto spmv = [&](const float (&x)[CPT], float (&y)[CPT]) {
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int i = b0 + k * PB + (int)threadIdx.x;
+      y[k] = 0.0f;
+      if (i < b1)
+        y[k] = x[k] + as_[k] * rval(js[k]) + an_[k] * rval(jn[k]) + aw_[k] * rval(i - 1) +
+               ae_[k] * rval(i + 1);
+    }
+  };
+
+  for (int g = threadIdx.x; g < 2 * WIN; g += PB) hl[g] = 0;
+  __syncthreads();
+  double fp2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int i = b0 + k * PB + (int)threadIdx.x;
+    as_[k] = an_[k] = aw_[k] = ae_[k] = 0.0f;
+    r[k] = u[k] = w[k] = s[k] = p[k] = 0.0f;
+    Y(k) = Q(k) = Z(k) = 0.0f;
+    js[k] = jn[k] = lc0;
+    if (i < b1) {
+      const int s_ = a.ls[i], n_ = a.ln[i];
+      js[k] = s_;
+      jn[k] = n_;
+      as_[k] = a.as[i];
+      aw_[k] = a.aw[i];
+      if (n_ != lc0) an_[k] = a.as[n_];
+      if (i + 1 != lc0) ae_[k] = a.aw[i + 1];
+      const int nbq[4] = {s_, n_, i - 1, i + 1};
+      bool out = false;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = nbq[q];
+        if (real(j) && !inside(j)) {
+          out = true;
+          hl[ghost_of(j, b0, b1)] = 1;
+        }
+      }
+      if (out) expo |= 1u << k;
+      fp2 += a.fptmp[i] * a.fptmp[i];
+    }
+  }
+  __syncthreads();
+  const int nhalo = compact_halo(hl, b0, b1);
+  // the workgroups that own the outside cells (hl is sorted)
+  const int span = CPT * PB;
+  const int nlo = nhalo > 0 ? (hl[0] - 1) / span : 1;
+  const int nhi = nhalo > 0 ? (hl[nhalo - 1] - 1) / span : 0;
+
+  unsigned tag = a.tag0, gen = 0;
+  double rsqm = a.rsqm, bnorm2 = 0.0, rsq64 = 0.0, bn_in = 0.0;
+  int total = 0, outer = 0, status = 0, inner = 0;
+
+  for (;;) {
+    // ---- FP64 residual r = FPTMP - A P, scaled to r' = D^-1/2 r (FP32)
+    double v3[3] = {0.0, 0.0, fp2};
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int i = b0 + k * PB + (int)threadIdx.x;
+      if (i < b1) {
+        const int s_ = opaque(js[k]), n_ = opaque(jn[k]);
+        const int io = opaque(i);
+        const double cs = (s_ != lc0) ? a.ccs[io] : 0.0;
+        const double cn = (n_ != lc0) ? a.ccs[n_] : 0.0;
+        const double cw = a.ccw[io];
+        const double ce = (io + 1 != lc0) ? a.ccw[io + 1] : 0.0;
+        const double *P = a.x;
+        const double rr = a.fptmp[io] - a.ccc[io] * fetch(&P[io]) - cn * fetch(&P[n_]) -
+                          cs * fetch(&P[s_]) - cw * fetch(&P[io - 1]) -
+                          ce * fetch(&P[io + 1]);
+        const float rs = (float)(a.dis[io] * rr);
+        r[k] = rs;
+        Y(k) = 0.0f;
+        v3[0] += rr * rr;
+        v3[1] += (double)rs * (double)rs;
+      }
+    }
+    post_partials<3>(v3, a.xc, tag);
+    precond(r, u);
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int i = b0 + k * PB + (int)threadIdx.x;
+      if (i < b1) {
+        lds[slot(i)] = u[k];
+        if (expo & (1u << k)) put_u32(hxw(i, tag), __float_as_uint(u[k]));
+      }
+    }
+    fp2 = 0.0;
+    post_flag(a.xc, tag);
+    {
+      double s3[3];
+      if (!exchange<3, 1>(a.xc, tag, hl, nhalo, raw, a.bar, s3)) return;
+      if (outer == 0) {
+        bnorm2 = s3[2];
+        rsqm = a.rel_tol > 0.0 ? a.rel_tol * a.rel_tol * bnorm2 : a.rsqm;
+      }
+      rsq64 = s3[0];
+      bn_in = s3[1];
+    }
+    if (rsq64 <= rsqm) break;
+    if (outer >= a.max_outer || bn_in == 0.0) { status = 1; break; }
+
+    // ---- w0 = A' u0; s, p, q, z start at 0
+    for (int h = threadIdx.x; h < nhalo; h += PB) lds[slot(hl[h])] = __uint_as_float(raw[h]);
+    __syncthreads();
+    spmv(u, w);
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      s[k] = p[k] = 0.0f;
+      Q(k) = Z(k) = 0.0f;
+    }
+    double gam_prev = 0.0, alpha_prev = 1.0;
+    inner = 0;
+    for (;;) {
+      ++tag;
+      // the sums of this iteration go out first ...
+      double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) {
+        const int i = b0 + k * PB + (int)threadIdx.x;
+        if (i < b1) {
+          v[0] += (double)r[k] * (double)u[k];
+          v[1] += (double)w[k] * (double)u[k];
+          v[2] += (double)r[k] * (double)r[k];
+        }
+      }
+      post_partials<3>(v, a.xc, tag);
+      // ... then m = M^-1 w, its outside values, the flag
+      float m[CPT];
+      precond(w, m);
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) {
+        const int i = b0 + k * PB + (int)threadIdx.x;
+        if (i < b1) {
+          lds[slot(i)] = m[k];
+          if (expo & (1u << k)) put_u32(hxw(i, tag), __float_as_uint(m[k]));
+        }
+      }
+      post_flag(a.xc, tag);
+      // n = A' m once the neighbouring workgroups' m has arrived
+      if (nhalo > 0 && !halo_wait(a.xc, tag, nlo, nhi, hl, nhalo, raw, a.bar)) return;
+      for (int h = threadIdx.x; h < nhalo; h += PB) lds[slot(hl[h])] = __uint_as_float(raw[h]);
+      __syncthreads();
+      float n[CPT];
+      spmv(m, n);
+      // the sums of every workgroup
+      double s3[3];
+      if (!exchange<3, 0>(a.xc, tag, hl, 0, raw, a.bar, s3)) return;
+      const double gam = s3[0], del = s3[1];
+      if (s3[2] <= a.rtol2 * bn_in || inner >= a.iterm) break;
+      double beta, alpha;
+      if (inner == 0) {
+        beta = 0.0;
+        alpha = gam / del;
+      } else {
+        beta = gam / gam_prev;
+        alpha = gam / (del - beta * gam / alpha_prev);
+      }
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) {
+        const float zk = (float)((double)n[k] + beta * (double)Z(k));
+        const float qk = (float)((double)m[k] + beta * (double)Q(k));
+        s[k] = (float)((double)w[k] + beta * (double)s[k]);
+        p[k] = (float)((double)u[k] + beta * (double)p[k]);
+        Y(k) = (float)((double)Y(k) + alpha * (double)p[k]);
+        r[k] = (float)((double)r[k] - alpha * (double)s[k]);
+        u[k] = (float)((double)u[k] - alpha * (double)qk);
+        w[k] = (float)((double)w[k] - alpha * (double)zk);
+        Z(k) = zk;
+        Q(k) = qk;
+      }
+      inner += 1;
+      gam_prev = gam;
+      alpha_prev = alpha;
+    }
+    total += inner;
+    ++tag;
+    // ---- P += D^-1/2 y (FP64), visible to every workgroup for the next
+    // residual
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int i = b0 + k * PB + (int)threadIdx.x;
+      if (i < b1) {
+        const int io = opaque(i);
+        double *P = a.x;
+        publish(&P[io], fetch(&P[io]) + a.dis[io] * (double)Y(k));
+      }
+    }
+    ++outer;
+    if (!grid_barrier(a.bar, ++gen)) return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CgState *st = a.st;
+    st->rsq64 = rsq64;
+    st->rsq = rsq64;
+    st->rsqm = rsqm;
+    st->bnorm2 = bnorm2;
+    st->bnorm2_in = bn_in;
+    st->iter = total;
+    st->inner_iter = inner;
+    st->outer = outer;
+    st->done = 1;
+    st->status = status;
+  }
+}
+
 // Jacobi scaling of the mixed solve from a scaled system (CCC, CCS, CCW)
 // given directly: dis = 1/sqrt(CCC), A'(i,S), A'(i,W) in FP32.
 __global__ __launch_bounds__(256) void k_mx_coef(const double *ccc, const double *ccs,
@@ -927,8 +1256,12 @@ int pcg_prepare(efdc_ctx *c) {
       0, occupancy(k_pcgmx<1>), occupancy(k_pcgmx<2>), occupancy(k_pcgmx<3>),
       occupancy(k_pcgmx<4>), occupancy(k_pcgmx<5>), occupancy(k_pcgmx<6>),
       occupancy(k_pcgmx<7>), occupancy(k_pcgmx<8>)};
+  const int occpl[PCGMX_CPT_MAX + 1] = {
+      0, occupancy(k_pcgmx_pl<1>), occupancy(k_pcgmx_pl<2>), occupancy(k_pcgmx_pl<3>),
+      occupancy(k_pcgmx_pl<4>), occupancy(k_pcgmx_pl<5>), occupancy(k_pcgmx_pl<6>),
+      occupancy(k_pcgmx_pl<7>), occupancy(k_pcgmx_pl<8>)};
   for (int k = 0; k <= PCG64_CPT_MAX; ++k) c->pcg_occ64[k] = occ64[k];
-  for (int k = 0; k <= PCGMX_CPT_MAX; ++k) c->pcg_occmx[k] = occmx[k];
+  for (int k = 0; k <= PCGMX_CPT_MAX; ++k) c->pcg_occmx[k] = std::min(occmx[k], occpl[k]);
   c->pcg_ncell = c->n_wet() + 2;
   int s;
   if ((s = dev_alloc(c, &c->pcg_hx,
@@ -1073,16 +1406,30 @@ int pcg_solve(efdc_ctx *c, bool mixed, bool defer) {
     a.pdeg = c->prm.mixed_precond_degree > 0 ? c->prm.mixed_precond_degree
              : c->pcg_pdeg_tuned > 0         ? c->pcg_pdeg_tuned
                                              : PCG_PDEG_DEFAULT;
+    const bool pl = c->pcg_pipe_mode == 2 || (c->pcg_pipe_mode == 0 && c->pcg_pipe_tuned);
     const void *fn = nullptr;
-    switch (cpt) {
-      case 1: fn = (const void *)k_pcgmx<1>; break;
-      case 2: fn = (const void *)k_pcgmx<2>; break;
-      case 3: fn = (const void *)k_pcgmx<3>; break;
-      case 4: fn = (const void *)k_pcgmx<4>; break;
-      case 5: fn = (const void *)k_pcgmx<5>; break;
-      case 6: fn = (const void *)k_pcgmx<6>; break;
-      case 7: fn = (const void *)k_pcgmx<7>; break;
-      default: fn = (const void *)k_pcgmx<8>; break;
+    if (pl) {
+      switch (cpt) {
+        case 1: fn = (const void *)k_pcgmx_pl<1>; break;
+        case 2: fn = (const void *)k_pcgmx_pl<2>; break;
+        case 3: fn = (const void *)k_pcgmx_pl<3>; break;
+        case 4: fn = (const void *)k_pcgmx_pl<4>; break;
+        case 5: fn = (const void *)k_pcgmx_pl<5>; break;
+        case 6: fn = (const void *)k_pcgmx_pl<6>; break;
+        case 7: fn = (const void *)k_pcgmx_pl<7>; break;
+        default: fn = (const void *)k_pcgmx_pl<8>; break;
+      }
+    } else {
+      switch (cpt) {
+        case 1: fn = (const void *)k_pcgmx<1>; break;
+        case 2: fn = (const void *)k_pcgmx<2>; break;
+        case 3: fn = (const void *)k_pcgmx<3>; break;
+        case 4: fn = (const void *)k_pcgmx<4>; break;
+        case 5: fn = (const void *)k_pcgmx<5>; break;
+        case 6: fn = (const void *)k_pcgmx<6>; break;
+        case 7: fn = (const void *)k_pcgmx<7>; break;
+        default: fn = (const void *)k_pcgmx<8>; break;
+      }
     }
     TIMED(c, TK_PCGMX, s = coop_launch(c, fn, nb, a));
     if (s) return s;
@@ -1151,6 +1498,30 @@ int pcg_tune_degree(efdc_ctx *c) {
     }
   }
   c->pcg_pdeg_tuned = best;
+  // the pipelined form at that degree, when the caller leaves the choice
+  if (c->pcg_pipe_mode == 0) {
+    c->pcg_pipe_tuned = true;
+    float tot = 0.f;
+    for (int r = 0; r < 3; ++r) {
+      HIP_TRY(c, hipMemcpyAsync(c->dH[EFDC_P], c->tune_pbuf, sizeof(double) * c->LCM,
+                                hipMemcpyDeviceToDevice, c->stream));
+      HIP_TRY(c, hipEventRecord(e0, c->stream));
+      if ((s = pcg_solve(c, true))) return s;
+      HIP_TRY(c, hipEventRecord(e1, c->stream));
+      HIP_TRY(c, hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+      if (r > 0) tot += ms;
+    }
+    const float avg = tot / 2.f;
+    c->pcg_pipe_tuned = avg < best_ms;
+    if (c->tune_n < 8) {
+      std::snprintf(c->tune_name[c->tune_n], 32, "pcg_mixed pipelined d=%d", best);
+      c->tune_ms[c->tune_n] = avg;
+      ++c->tune_n;
+      if (c->pcg_pipe_tuned) best_slot = c->tune_n - 1;
+    }
+  }
   if (best_slot >= 0 && best_slot < 8) c->tune_chosen[best_slot] = 1;
   c->ev_pool.push_back(e0);
   c->ev_pool.push_back(e1);

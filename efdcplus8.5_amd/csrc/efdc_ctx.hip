@@ -931,6 +931,18 @@ int efdc_get_cg_replay(efdc_ctx_t *c, int32_t *mode) {
   return EFDC_OK;
 }
 
+int efdc_set_cg_pipeline(efdc_ctx_t *c, int32_t mode) {
+  if (!c || mode < 0 || mode > 2) return EFDC_ERR_ARG;
+  c->pcg_pipe_mode = mode;
+  return EFDC_OK;
+}
+
+int efdc_get_cg_pipeline(efdc_ctx_t *c, int32_t *mode) {
+  if (!c || !mode) return EFDC_ERR_ARG;
+  *mode = c->pcg_pipe_mode != 0 ? c->pcg_pipe_mode : (c->pcg_pipe_tuned ? 2 : 1);
+  return EFDC_OK;
+}
+
 int efdc_autotune(efdc_ctx_t *c, int32_t on) {
   if (!c) return EFDC_ERR_ARG;
   c->tune_pending = on != 0;
@@ -945,6 +957,7 @@ int efdc_autotune(efdc_ctx_t *c, int32_t on) {
     c->tran_ncg = 0;
     c->tran_w3 = 0;
     c->pcg_pdeg_tuned = 0;
+    c->pcg_pipe_tuned = false;
   }
   return EFDC_OK;
 }

@@ -193,6 +193,8 @@ def load(path: Optional[str] = None):
         "efdc_set_segment_tile": ([P, C.c_int32], C.c_int),
         "efdc_set_cg_replay": ([P, C.c_int32], C.c_int),
         "efdc_get_cg_replay": ([P, C.POINTER(C.c_int32)], C.c_int),
+        "efdc_set_cg_pipeline": ([P, C.c_int32], C.c_int),
+        "efdc_get_cg_pipeline": ([P, C.POINTER(C.c_int32)], C.c_int),
         "efdc_get_segment_tile": ([P, C.POINTER(C.c_int32)], C.c_int),
         "efdc_restart_text": ([P, C.POINTER(_fmt.Restart), C.c_char_p, C.c_int64,
                                C.POINTER(C.c_int64)], C.c_int),
@@ -612,6 +614,16 @@ class Context:
     def cg_replay(self) -> int:
         v = C.c_int32(0)
         self._check(self.lib.efdc_get_cg_replay(self.h, C.byref(v)))
+        return v.value
+
+    def set_cg_pipeline(self, mode: int):
+        """Form of the mixed persistent inner solve: 0 tuned, 1 the
+        Chronopoulos-Gear form, 2 its pipelined form."""
+        self._check(self.lib.efdc_set_cg_pipeline(self.h, mode))
+
+    def cg_pipeline(self) -> int:
+        v = C.c_int32(0)
+        self._check(self.lib.efdc_get_cg_pipeline(self.h, C.byref(v)))
         return v.value
 
     def segment_tile(self) -> int:

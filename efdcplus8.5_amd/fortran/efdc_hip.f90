@@ -336,6 +336,18 @@ module efdc_hip
       integer(c_int32_t), intent(out) :: mode
     end function
 
+    integer(c_int) function efdc_set_cg_pipeline(ctx, mode) bind(C, name="efdc_set_cg_pipeline")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: mode
+    end function
+
+    integer(c_int) function efdc_get_cg_pipeline(ctx, mode) bind(C, name="efdc_get_cg_pipeline")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: mode
+    end function
+
     integer(c_int) function efdc_set_segment_tile(ctx, cells) bind(C, name="efdc_set_segment_tile")
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx

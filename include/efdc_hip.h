@@ -585,6 +585,17 @@ int efdc_get_segment_tile(efdc_ctx_t *ctx, int32_t *cells);
    kernel by kernel. */
 int efdc_set_cg_replay(efdc_ctx_t *ctx, int32_t mode);
 int efdc_get_cg_replay(efdc_ctx_t *ctx, int32_t *mode);
+/* Form of the mixed persistent inner solve (EFDC_CG_MIXED on one
+   co-resident grid; no counterpart in the reference, whose CONGRAD is the
+   FP64 Jacobi-PCG): 0 (default) the autotuner chooses, the
+   Chronopoulos-Gear form until it has run; 1 that form, one exchange
+   between workgroups per iteration that carries the dot products and three
+   halo words; 2 its pipelined form, which waits for the dot products one
+   SpMV later and for one halo word from the neighbouring workgroups only.
+   Both give the same iterates in exact arithmetic.  get returns 1 or 2,
+   the form the next solve runs. */
+int efdc_set_cg_pipeline(efdc_ctx_t *ctx, int32_t mode);
+int efdc_get_cg_pipeline(efdc_ctx_t *ctx, int32_t *mode);
 
 /* Per-basin results of the last efdc_calpuv2c / efdc_congrad with
    segments: iters[s] and status[s] (0 or EFDC_ERR_CG_MAXITER) for s <

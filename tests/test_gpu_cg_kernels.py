@@ -12,7 +12,8 @@ row-major L+-1 pattern, so the general four-map operator is used), on a
 grid so small that one workgroup holds every row, and on a grid whose
 north/south neighbours lie beyond the workgroup's LDS window (the
 automatic choice then falls back to the multi-kernel solve, the forced
-persistent choice is refused).
+persistent choice is refused).  The persistent mixed solve runs in both its
+forms (Chronopoulos-Gear and pipelined).
 """
 import dataclasses
 
@@ -39,11 +40,14 @@ def _lake_system(oracle_mod, n, seed):
     return case, sys0, ref.h["P"].copy()
 
 
-def _solve(grid, system, **kw):
+def _solve(grid, system, pipeline=None, **kw):
     from efdc_amd.lib import Context
     kw.setdefault("RSQM", TIGHT)
     kw.setdefault("ITERM", 4000)
     ctx = Context(grid, **kw)
+    if pipeline is not None:
+        ctx.set_cg_pipeline(pipeline)
+        assert ctx.cg_pipeline() == pipeline
     for k, v in system.items():
         ctx.set(k, v)
     out = ctx.congrad()
@@ -131,6 +135,28 @@ def test_persistent_general_maps(oracle_mod):
     d = eta_err(p[newi], p_ref[src - 1])
     print("column-major persistent: iters", out["iters"], "deta", d)
     assert d <= 1e-9
+
+
+@pytest.mark.parametrize("n,seed,deg", [(12, 2, 3), (60, 4, 1), (160, 9, 3), (300, 11, 2)])
+def test_mixed_persistent_forms(oracle_mod, n, seed, deg):
+    """The Chronopoulos-Gear form (1) and its pipelined form (2) of the
+    persistent mixed solve: both within 1e-6 m of the FP64 oracle, with
+    inner iteration counts that differ by rounding only (the iterates are
+    the same in exact arithmetic); one workgroup, several, and different
+    preconditioner degrees."""
+    from efdc_amd import lib
+    case, system, p_ref = _lake_system(oracle_mod, n, seed)
+    i = wet_index(case)
+    res = {}
+    for form in (1, 2):
+        out, p = _solve(case.grid, system, pipeline=form, cg_kernel=lib.CG_KERNEL_PERSISTENT,
+                        cg_mode=lib.CG_MIXED, RSQM=1e-20, mixed_precond_degree=deg)
+        d = eta_err(p[i], p_ref[i])
+        print(f"n={n} wet={case.grid.nwet} form {form}: outer {out['outer']} "
+              f"inner {out['iters']} deta {d:.3e}")
+        assert d <= 1e-6
+        res[form] = out["iters"]
+    assert abs(res[1] - res[2]) <= max(2, res[1] // 10)
 
 
 def test_persistent_single_workgroup(oracle_mod):
