@@ -32,8 +32,9 @@ def test_autotuned_transport_is_bitwise_the_default(store):
 
 def test_autotuned_mixed_solve_degree():
     """With the mixed CG on the persistent solve, the tuner times the local
-    preconditioner degrees from the same P, keeps one, and the step's
-    result stays within the mixed-precision bar of the FP64 solve."""
+    preconditioner degrees from the same P, then the pipelined form at the
+    fastest degree, keeps one, and the step's result stays within the
+    mixed-precision bar of the FP64 solve."""
     from efdc_amd import grids, lib
     from conftest import eta_err, wet_index
     wet = grids.lake_mask(160, 160, 5)
@@ -46,8 +47,11 @@ def test_autotuned_mixed_solve_degree():
         d.calpuv2c(case.dt, n)
     rep = m.tune_report()
     print(rep)
-    degs = [k for k in rep["shapes"] if k.startswith("pcg_mixed")]
-    assert len(degs) == 5 and len([k for k in rep["chosen"] if k.startswith("pcg_mixed")]) == 1
+    degs = [k for k in rep["shapes"] if k.startswith("pcg_mixed degree=")]
+    forms = [k for k in rep["shapes"] if k.startswith("pcg_mixed pipelined")]
+    assert len(degs) == 5 and len(forms) == 1
+    assert len([k for k in rep["chosen"] if k.startswith("pcg_mixed")]) == 1
+    assert m.cg_pipeline() == (2 if forms[0] in rep["chosen"] else 1)
     i = wet_index(case)
     assert eta_err(m.get("P")[i], d.get("P")[i]) <= 1e-6
     m.close()
