@@ -1270,7 +1270,7 @@ int pcg_prepare(efdc_ctx *c) {
   if ((s = dev_alloc(c, &c->pcg_ps, sizeof(double) * 2 * NVMAX * PB))) return s;
   if ((s = dev_alloc(c, (void **)&c->pcg_flag, sizeof(unsigned) * PB * TICKET_PAD))) return s;
   if ((s = dev_alloc(c, (void **)&c->pcg_ctr, sizeof(unsigned) * 9 * TICKET_PAD))) return s;
-  if ((s = dev_alloc(c, (void **)&c->pcg_abort, sizeof(int)))) return s;
+  c->pcg_abort = &c->st->pabort;
   c->pcg_launch = 0;
   c->pcg_ready = true;
   return EFDC_OK;
@@ -1434,14 +1434,6 @@ int pcg_solve(efdc_ctx *c, bool mixed, bool defer) {
     TIMED(c, TK_PCGMX, s = coop_launch(c, fn, nb, a));
     if (s) return s;
   }
-  if (!c->pcg_abort_host &&
-      hipHostMalloc((void **)&c->pcg_abort_host, sizeof(int)) != hipSuccess) {
-    c->pcg_abort_host = nullptr;
-    c->err = "hipHostMalloc failed";
-    return EFDC_ERR_NOMEM;
-  }
-  HIP_TRY(c, hipMemcpyAsync(c->pcg_abort_host, c->pcg_abort, sizeof(int),
-                            hipMemcpyDeviceToHost, c->stream));
   if (defer) {
     // the caller reads the state and the abort flag after the step
     c->pcg_deferred = true;
@@ -1450,7 +1442,7 @@ int pcg_solve(efdc_ctx *c, bool mixed, bool defer) {
   HIP_TRY(c, hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost,
                             c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  if (*c->pcg_abort_host) {
+  if (c->st_host->pabort) {
     c->err = "persistent CG: an exchange between workgroups timed out";
     return EFDC_ERR_HIP;
   }

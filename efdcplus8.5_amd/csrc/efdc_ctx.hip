@@ -339,7 +339,7 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   dfree(c->part); dfree(c->st); dfree(c->tickets);
   { char *q = (char *)c->pcg_hx; dfree(q); }
   { char *q = (char *)c->pcg_ps; dfree(q); }
-  dfree(c->pcg_ctr); dfree(c->pcg_abort); dfree(c->pcg_flag);
+  dfree(c->pcg_ctr); dfree(c->pcg_flag);
   seg_free(c);
   adj_free(c);
   uvw_free(c);
@@ -348,7 +348,6 @@ int efdc_ctx_destroy(efdc_ctx_t *c) {
   cg_graphs_free(c);
   dfree(c->tune_psave);
   if (c->st_host) (void)hipHostFree(c->st_host);
-  if (c->pcg_abort_host) (void)hipHostFree(c->pcg_abort_host);
   dfree(c->gat_off);
   dfree(c->gat_val);
   for (int q = 0; q < efdc_ctx::CQS_RING; ++q) {

@@ -37,7 +37,8 @@ struct CgState {
   double rsq64, bnorm2_in, rz_in;  // mixed mode: outer residual, inner rhs
   int iter, done, status, symok;   // symok: coefficients verified symmetric
   int negdep, outer, inner_iter, drychg;  // drychg: a cell dried in this pass
-  int fin, pad_;                   // basins' mixed solve: outer rounds ended
+  int fin;                         // basins' mixed solve: outer rounds ended
+  int pabort;                      // persistent solve: a wait was abandoned
 };
 
 // Arrival counters of the grid reductions: one site per reduction kernel,
@@ -157,7 +158,7 @@ struct efdc_ctx {
   int pcg_ncell = 0;
   unsigned pcg_launch = 0;     // launch number, gives the first tag
   unsigned *pcg_ctr = nullptr; // barrier counters
-  int *pcg_abort = nullptr;
+  int *pcg_abort = nullptr;     // &st->pabort (read back with the state)
 
   // open boundaries
   PbcDev pbc[4] = {};
@@ -226,7 +227,6 @@ struct efdc_ctx {
   // the persistent solve of a CALPUV2C step leaves its checks to the
   // step's final read-back (one host round trip per step)
   bool pcg_deferred = false;
-  int *pcg_abort_host = nullptr;  // pinned
   // efdc_set_source_cqs: pinned staging ring, a slot reused once the copy
   // that last read it has completed
   static constexpr int CQS_RING = 8;

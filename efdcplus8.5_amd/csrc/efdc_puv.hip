@@ -1871,7 +1871,7 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
     if ((s = read_state(c))) return s;
     if (c->pcg_deferred) {
       c->pcg_deferred = false;
-      if (*c->pcg_abort_host) {
+      if (c->st_host->pabort) {
         c->err = "persistent CG: an exchange between workgroups timed out";
         return EFDC_ERR_HIP;
       }
@@ -1905,7 +1905,7 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
   if ((s = read_state(c))) return s;
   if (c->pcg_deferred) {
     c->pcg_deferred = false;
-    if (*c->pcg_abort_host) {
+    if (c->st_host->pabort) {
       c->err = "persistent CG: an exchange between workgroups timed out";
       return EFDC_ERR_HIP;
     }
