@@ -1463,6 +1463,9 @@ int pcg_tune_degree(efdc_ctx *c) {
   const int cand[5] = {1, 2, 3, 4, 6};
   float best_ms = 1e30f;
   int best = PCG_PDEG_DEFAULT, best_slot = -1;
+  // the degrees are timed in the Chronopoulos-Gear form when the caller
+  // leaves the form to the tuner
+  if (c->pcg_pipe_mode == 0) c->pcg_pipe_tuned = false;
   for (int j = 0; j < 5; ++j) {
     c->pcg_pdeg_tuned = cand[j];
     float tot = 0.f;

@@ -956,7 +956,7 @@ int efdc_autotune(efdc_ctx_t *c, int32_t on) {
     c->tran_ncg = 0;
     c->tran_w3 = 0;
     c->pcg_pdeg_tuned = 0;
-    c->pcg_pipe_tuned = false;
+    c->pcg_pipe_tuned = true;
   }
   return EFDC_OK;
 }

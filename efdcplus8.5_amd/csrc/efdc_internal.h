@@ -242,10 +242,10 @@ struct efdc_ctx {
   // the preconditioner degree of the mixed persistent solve
   bool tune_pending = false, tune_cg_pending = false;
   int pcg_pdeg_tuned = 0;         // 0: not tuned
-  // mixed persistent solve: 0 the autotuner picks (classic until tuned),
-  // 1 the Chronopoulos-Gear form, 2 its pipelined form
+  // mixed persistent solve: 0 the autotuner picks (pipelined until it has
+  // run), 1 the Chronopoulos-Gear form, 2 its pipelined form
   int pcg_pipe_mode = 0;
-  bool pcg_pipe_tuned = false;    // the autotuner chose the pipelined form
+  bool pcg_pipe_tuned = true;     // the pipelined form (the autotuner's choice)
   double *tune_pbuf = nullptr;    // P saved while solves are timed
   int tran_ncg = 0, tran_w3 = 0;  // 0: not tuned
   int tune_n = 0, tune_best = -1;

@@ -587,8 +587,8 @@ int efdc_set_cg_replay(efdc_ctx_t *ctx, int32_t mode);
 int efdc_get_cg_replay(efdc_ctx_t *ctx, int32_t *mode);
 /* Form of the mixed persistent inner solve (EFDC_CG_MIXED on one
    co-resident grid; no counterpart in the reference, whose CONGRAD is the
-   FP64 Jacobi-PCG): 0 (default) the autotuner chooses, the
-   Chronopoulos-Gear form until it has run; 1 that form, one exchange
+   FP64 Jacobi-PCG): 0 (default) the autotuner chooses, the pipelined
+   form until it has run; 1 the Chronopoulos-Gear form, one exchange
    between workgroups per iteration that carries the dot products and three
    halo words; 2 its pipelined form, which waits for the dot products one
    SpMV later and for one halo word from the neighbouring workgroups only.
