@@ -142,10 +142,18 @@ __device__ void post_partials(double (&v)[NV], const Xch &x, unsigned tag) {
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    // all the loads first, so that they are in flight together; the sums
+    // keep the wave order
+    double pv[NV][PW];
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+#pragma unroll
+      for (int q = 0; q < PW; ++q) pv[j][q] = sh[j][q];
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
       double s = 0.0;
-      for (int q = 0; q < PW; ++q) s += sh[j][q];
+#pragma unroll
+      for (int q = 0; q < PW; ++q) s += pv[j][q];
       __hip_atomic_store(&pslot(x.ps, tag, j)[blockIdx.x], s, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -935,9 +943,12 @@ __device__ bool halo_wait(const Xch &x, unsigned tag, int lo, int hi, const int 
 // ---------------------------------------------------------------------
 template <int CPT>
 __global__ __launch_bounds__(PB) void k_pcgmx_pl(PMxArgs a) {
-  __shared__ float lds[2 * WIN + CPT * PB];
+  // the window, then one word that stays zero: a row reads it in place of
+  // a neighbour it does not couple to, so that every lane loads
+  __shared__ float lds[2 * WIN + CPT * PB + 1];
   __shared__ int hl[2 * WIN];
   __shared__ unsigned raw[HMAX];
+  constexpr int ZS = 2 * WIN + CPT * PB;
   // y, q = M^-1 s and z = A' q of the own rows live in LDS
   __shared__ float lys[CPT * PB], lqs[CPT * PB], lzs[CPT * PB];
   const int b0 = 1 + (int)blockIdx.x * CPT * PB;
@@ -959,49 +970,51 @@ __global__ __launch_bounds__(PB) void k_pcgmx_pl(PMxArgs a) {
   auto hxw = [&](int i, unsigned tag) {
     return a.xc.hx + (size_t)(tag & 1u) * HX_WORDS * (size_t)a.xc.ncell + (size_t)opaque(i);
   };
-  // M^-1 v by the local Neumann polynomial, as in k_pcgmx
+  // M^-1 v by the local Neumann polynomial, as in k_pcgmx.  No branch per
+  // row: the lanes past b1 (the last workgroup's) hold zero coefficients
+  // and zero vectors, so their rows stay zero, and their slots, which no
+  // row reads, hold those zeros.
   auto precond = [&](const float (&v)[CPT], float (&t)[CPT]) {
 #pragma unroll
     for (int k = 0; k < CPT; ++k) t[k] = v[k];
     for (int d = 1; d < a.pdeg; ++d) {
       __syncthreads();
 #pragma unroll
-      for (int k = 0; k < CPT; ++k) {
-        const int i = b0 + k * PB + (int)threadIdx.x;
-        if (i < b1) lds[slot(i)] = t[k];
-      }
+      for (int k = 0; k < CPT; ++k) lds[slot(b0 + k * PB + (int)threadIdx.x)] = t[k];
       __syncthreads();
       float tn[CPT];
 #pragma unroll
       for (int k = 0; k < CPT; ++k) {
         const int i = b0 + k * PB + (int)threadIdx.x;
-        tn[k] = t[k];
-        if (i < b1) {
-          const float ts = inside(js[k]) ? lds[slot(js[k])] : 0.0f;
-          const float tnn = inside(jn[k]) ? lds[slot(jn[k])] : 0.0f;
-          const float tw = inside(i - 1) ? lds[slot(i - 1)] : 0.0f;
-          const float te = inside(i + 1) ? lds[slot(i + 1)] : 0.0f;
-          tn[k] = v[k] - (as_[k] * ts + an_[k] * tnn + aw_[k] * tw + ae_[k] * te);
-        }
+        const float ts = lds[inside(js[k]) ? slot(js[k]) : ZS];
+        const float tnn = lds[inside(jn[k]) ? slot(jn[k]) : ZS];
+        const float tw = lds[i - 1 >= b0 ? slot(i - 1) : ZS];
+        const float te = lds[i + 1 < b1 ? slot(i + 1) : ZS];
+        tn[k] = v[k] - (as_[k] * ts + an_[k] * tnn + aw_[k] * tw + ae_[k] * te);
       }
 #pragma unroll
       for (int k = 0; k < CPT; ++k) t[k] = tn[k];
     }
     __syncthreads();
   };
-  // y = A' x on the own rows from the LDS window (own slots and outside cells)
+  // y = A' x on the own rows from the LDS window (own slots and outside
+  // cells); a lane past b1 may read a slot no one wrote, so its row is
+  // replaced by zero
   auto spmv = [&](const float (&x)[CPT], float (&y)[CPT]) {
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int i = b0 + k * PB + (int)threadIdx.x;
-      y[k] = 0.0f;
-      if (i < b1)
-        y[k] = x[k] + as_[k] * rval(js[k]) + an_[k] * rval(jn[k]) + aw_[k] * rval(i - 1) +
-               ae_[k] * rval(i + 1);
+      const float vs = lds[real(js[k]) ? slot(js[k]) : ZS];
+      const float vn = lds[real(jn[k]) ? slot(jn[k]) : ZS];
+      const float vw = lds[real(i - 1) ? slot(i - 1) : ZS];
+      const float ve = lds[real(i + 1) ? slot(i + 1) : ZS];
+      const float yk = x[k] + as_[k] * vs + an_[k] * vn + aw_[k] * vw + ae_[k] * ve;
+      y[k] = i < b1 ? yk : 0.0f;
     }
   };
 
   for (int g = threadIdx.x; g < 2 * WIN; g += PB) hl[g] = 0;
+  if (threadIdx.x == 0) lds[ZS] = 0.0f;
   __syncthreads();
   double fp2 = 0.0;
 #pragma unroll
@@ -1047,36 +1060,39 @@ __global__ __launch_bounds__(PB) void k_pcgmx_pl(PMxArgs a) {
   for (;;) {
     // ---- FP64 residual r = FPTMP - A P, scaled to r' = D^-1/2 r (FP32)
     double v3[3] = {0.0, 0.0, fp2};
+    // every lane loads (a lane past b1 from the last own cell; its js, jn
+    // are lc0) and the own rows are selected, so the loads of all the
+    // thread's rows are in flight together
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int i = b0 + k * PB + (int)threadIdx.x;
-      if (i < b1) {
-        const int s_ = opaque(js[k]), n_ = opaque(jn[k]);
-        const int io = opaque(i);
-        const double cs = (s_ != lc0) ? a.ccs[io] : 0.0;
-        const double cn = (n_ != lc0) ? a.ccs[n_] : 0.0;
-        const double cw = a.ccw[io];
-        const double ce = (io + 1 != lc0) ? a.ccw[io + 1] : 0.0;
-        const double *P = a.x;
-        const double rr = a.fptmp[io] - a.ccc[io] * fetch(&P[io]) - cn * fetch(&P[n_]) -
-                          cs * fetch(&P[s_]) - cw * fetch(&P[io - 1]) -
-                          ce * fetch(&P[io + 1]);
-        const float rs = (float)(a.dis[io] * rr);
-        r[k] = rs;
-        Y(k) = 0.0f;
-        v3[0] += rr * rr;
-        v3[1] += (double)rs * (double)rs;
-      }
+      const bool own = i < b1;
+      const int s_ = opaque(js[k]), n_ = opaque(jn[k]);
+      const int io = opaque(own ? i : b1 - 1);
+      const int ie = io + 1;  // <= lc0
+      const double *P = a.x;
+      const double ccs_i = a.ccs[io], ccs_n = a.ccs[n_], ccw_i = a.ccw[io];
+      const double ccw_e = a.ccw[ie], fpt = a.fptmp[io], cci = a.ccc[io], di = a.dis[io];
+      const double p_i = fetch(&P[io]), p_n = fetch(&P[n_]), p_s = fetch(&P[s_]);
+      const double p_w = fetch(&P[io - 1]), p_e = fetch(&P[ie]);
+      const double cs = (s_ != lc0) ? ccs_i : 0.0;
+      const double cn = (n_ != lc0) ? ccs_n : 0.0;
+      const double cw = ccw_i;
+      const double ce = (ie != lc0) ? ccw_e : 0.0;
+      const double rr = fpt - cci * p_i - cn * p_n - cs * p_s - cw * p_w - ce * p_e;
+      const float rs = (float)(di * rr);
+      r[k] = own ? rs : 0.0f;
+      Y(k) = 0.0f;
+      v3[0] += own ? rr * rr : 0.0;
+      v3[1] += own ? (double)rs * (double)rs : 0.0;
     }
     post_partials<3>(v3, a.xc, tag);
     precond(r, u);
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int i = b0 + k * PB + (int)threadIdx.x;
-      if (i < b1) {
-        lds[slot(i)] = u[k];
-        if (expo & (1u << k)) put_u32(hxw(i, tag), __float_as_uint(u[k]));
-      }
+      lds[slot(i)] = u[k];
+      if (expo & (1u << k)) put_u32(hxw(i, tag), __float_as_uint(u[k]));
     }
     fp2 = 0.0;
     post_flag(a.xc, tag);
@@ -1107,15 +1123,13 @@ __global__ __launch_bounds__(PB) void k_pcgmx_pl(PMxArgs a) {
     for (;;) {
       ++tag;
       // the sums of this iteration go out first ...
+      // (the lanes past b1 add zeros)
       double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
       for (int k = 0; k < CPT; ++k) {
-        const int i = b0 + k * PB + (int)threadIdx.x;
-        if (i < b1) {
-          v[0] += (double)r[k] * (double)u[k];
-          v[1] += (double)w[k] * (double)u[k];
-          v[2] += (double)r[k] * (double)r[k];
-        }
+        v[0] += (double)r[k] * (double)u[k];
+        v[1] += (double)w[k] * (double)u[k];
+        v[2] += (double)r[k] * (double)r[k];
       }
       post_partials<3>(v, a.xc, tag);
       // ... then m = M^-1 w, its outside values, the flag
@@ -1124,10 +1138,8 @@ __global__ __launch_bounds__(PB) void k_pcgmx_pl(PMxArgs a) {
 #pragma unroll
       for (int k = 0; k < CPT; ++k) {
         const int i = b0 + k * PB + (int)threadIdx.x;
-        if (i < b1) {
-          lds[slot(i)] = m[k];
-          if (expo & (1u << k)) put_u32(hxw(i, tag), __float_as_uint(m[k]));
-        }
+        lds[slot(i)] = m[k];
+        if (expo & (1u << k)) put_u32(hxw(i, tag), __float_as_uint(m[k]));
       }
       post_flag(a.xc, tag);
       // n = A' m once the neighbouring workgroups' m has arrived
