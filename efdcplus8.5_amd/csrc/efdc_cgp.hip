@@ -375,9 +375,12 @@ struct P64Args {
 
 template <int CPT>
 __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
-  __shared__ double lds[2 * WIN + CPT * PB];
+  // the window, then one word that stays zero (read in place of a
+  // neighbour that is not a cell, so that every lane loads)
+  __shared__ double lds[2 * WIN + CPT * PB + 1];
   __shared__ int hl[2 * WIN];
   __shared__ unsigned raw[4 * HMAX];
+  constexpr int ZS = 2 * WIN + CPT * PB;
   const int b0 = 1 + (int)blockIdx.x * CPT * PB;
   const int b1 = min(b0 + CPT * PB, a.nwet + 1);
   const bool fast = a.rowmajor && (a.st->symok == 0);
@@ -399,6 +402,7 @@ __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
   };
 
   for (int g = threadIdx.x; g < 2 * WIN; g += PB) hl[g] = 0;
+  if (threadIdx.x == 0) lds[ZS] = 0.0;
   __syncthreads();
   double v2[2] = {0.0, 0.0};
 #pragma unroll
@@ -467,12 +471,14 @@ __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
   bool done = (rpcg == 0.0);  // :88
 
   while (!done) {
-    // p = CCCI*r + beta*p (:134,158) on the own rows ...
+    // p = CCCI*r + beta*p (:134,158) on the own rows ...  (no branch per
+    // row: the lanes past b1, the last workgroup's, hold zero vectors and
+    // zero couplings to lc0, so their rows stay zero; their slots are read
+    // by no row)
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
-      const int i = b0 + k * PB + (int)threadIdx.x;
       p[k] = cci[k] * r[k] + beta * p[k];
-      if (i < b1) lds[slot(i)] = p[k];
+      lds[slot(b0 + k * PB + (int)threadIdx.x)] = p[k];
     }
     // ... and on the outside cells, from their published r and p
     for (int h = threadIdx.x; h < nhalo; h += PB) {
@@ -487,19 +493,16 @@ __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
     double v1[1] = {0.0};
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
-      const int i = b0 + k * PB + (int)threadIdx.x;
-      if (i < b1) {
-        double pq[4];
+      double pq[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int j = nb[k][q];
-          pq[q] = real(j) ? lds[slot(j)] : 0.0;
-        }
-        // APCG = CCC*PCG + CCS*PS + CCN*PN + CCW*PW + CCE*PE (:113)
-        ap[k] = ccc[k] * p[k] + cs[k] * pq[NS] + cn[k] * pq[NN] + cw[k] * pq[NW] +
-                ce[k] * pq[NE];
-        v1[0] += ap[k] * p[k];
+      for (int q = 0; q < 4; ++q) {
+        const int j = nb[k][q];
+        pq[q] = lds[real(j) ? slot(j) : ZS];
       }
+      // APCG = CCC*PCG + CCS*PS + CCN*PN + CCW*PW + CCE*PE (:113)
+      ap[k] = ccc[k] * p[k] + cs[k] * pq[NS] + cn[k] * pq[NN] + cw[k] * pq[NW] +
+              ce[k] * pq[NE];
+      v1[0] += ap[k] * p[k];
     }
     ++tag;
     post<1>(v1, a.xc, tag);
@@ -512,15 +515,13 @@ __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int i = b0 + k * PB + (int)threadIdx.x;
-      if (i < b1) {
-        x[k] = x[k] + alpha * p[k];        // :132
-        const double rr = r[k] - alpha * ap[k];  // :133
-        r[k] = rr;
-        const double z = cci[k] * rr;      // :134
-        v[0] += rr * z;
-        v[1] += rr * rr;
-        if (expo & (1u << k)) publish_rp(i, tag, rr, p[k]);
-      }
+      x[k] = x[k] + alpha * p[k];        // :132
+      const double rr = r[k] - alpha * ap[k];  // :133
+      r[k] = rr;
+      const double z = cci[k] * rr;      // :134
+      v[0] += rr * z;
+      v[1] += rr * rr;
+      if (expo & (1u << k)) publish_rp(i, tag, rr, p[k]);
     }
     post<2>(v, a.xc, tag);
     if (!exchange<2, 4>(a.xc, tag, hl, nhalo, raw, a.bar, s2)) return;
@@ -573,9 +574,12 @@ struct PMxArgs {
 
 template <int CPT>
 __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
-  __shared__ float lds[2 * WIN + CPT * PB];
+  // the window, then one word that stays zero: a row reads it in place of
+  // a neighbour it does not couple to, so that every lane loads
+  __shared__ float lds[2 * WIN + CPT * PB + 1];
   __shared__ int hl[2 * WIN];
   __shared__ unsigned raw[3 * HMAX];
+  constexpr int ZS = 2 * WIN + CPT * PB;
   // y (the inner solution) and z = M^-1 w of the own rows live in LDS:
   // with them in registers the 6..8-cells-per-thread shapes would spill
   __shared__ float lys[CPT * PB], lzs[CPT * PB];
@@ -594,7 +598,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
   auto inside = [&](int j) { return j >= b0 && j < b1; };
   auto real = [&](int j) { return j > 0 && j != lc0; };
   auto slot = [&](int j) { return j - b0 + WIN; };
-  auto rval = [&](int j) { return real(j) ? lds[slot(j)] : 0.0f; };
+  auto rval = [&](int j) { return lds[real(j) ? slot(j) : ZS]; };
   // FP32 halo value q of cell i in the slots of exchange `tag`
   auto hxw = [&](int i, unsigned tag, int q) {
     return a.xc.hx + ((size_t)(tag & 1u) * HX_WORDS + q) * (size_t)a.xc.ncell +
@@ -607,29 +611,29 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
   // is a polynomial in the symmetric local block whose eigenvalues lie in
   // (-1, 1), so it is symmetric positive definite and CG stays CG; it needs
   // no exchange between workgroups.  The own slots of the LDS window carry t.
+  // No branch per row: the lanes past b1 (the last workgroup's) hold zero
+  // coefficients and zero vectors, so their rows stay zero, and their
+  // slots, which no row reads, hold those zeros.
   auto precond = [&](const float (&v)[CPT], float (&t)[CPT]) {
 #pragma unroll
     for (int k = 0; k < CPT; ++k) t[k] = v[k];
     for (int d = 1; d < a.pdeg; ++d) {
       __syncthreads();
 #pragma unroll
-      for (int k = 0; k < CPT; ++k) {
-        const int i = b0 + k * PB + (int)threadIdx.x;
-        if (i < b1) lds[slot(i)] = t[k];
-      }
+      for (int k = 0; k < CPT; ++k) lds[slot(b0 + k * PB + (int)threadIdx.x)] = t[k];
       __syncthreads();
       float tn[CPT];
 #pragma unroll
       for (int k = 0; k < CPT; ++k) {
-        const int i = b0 + k * PB + (int)threadIdx.x;
-        tn[k] = t[k];
-        if (i < b1) {
-          const float ts = inside(js[k]) ? lds[slot(js[k])] : 0.0f;
-          const float tnn = inside(jn[k]) ? lds[slot(jn[k])] : 0.0f;
-          const float tw = inside(i - 1) ? lds[slot(i - 1)] : 0.0f;
-          const float te = inside(i + 1) ? lds[slot(i + 1)] : 0.0f;
-          tn[k] = v[k] - (as_[k] * ts + an_[k] * tnn + aw_[k] * tw + ae_[k] * te);
-        }
+        // indices made opaque: the slot addresses are formed at each use
+        // rather than held in registers across the whole solve
+        const int i = opaque(b0 + k * PB + (int)threadIdx.x);
+        const int jsk = opaque(js[k]), jnk = opaque(jn[k]);
+        const float ts = lds[inside(jsk) ? slot(jsk) : ZS];
+        const float tnn = lds[inside(jnk) ? slot(jnk) : ZS];
+        const float tw = lds[i - 1 >= b0 ? slot(i - 1) : ZS];
+        const float te = lds[i + 1 < b1 ? slot(i + 1) : ZS];
+        tn[k] = v[k] - (as_[k] * ts + an_[k] * tnn + aw_[k] * tw + ae_[k] * te);
       }
 #pragma unroll
       for (int k = 0; k < CPT; ++k) t[k] = tn[k];
@@ -638,6 +642,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
   };
 
   for (int g = threadIdx.x; g < 2 * WIN; g += PB) hl[g] = 0;
+  if (threadIdx.x == 0) lds[ZS] = 0.0f;
   __syncthreads();
   double fp2 = 0.0;
 #pragma unroll
@@ -684,26 +689,30 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
   for (;;) {
     // ---- FP64 residual r = FPTMP - A P, scaled to r' = D^-1/2 r (FP32)
     double v3[3] = {0.0, 0.0, fp2};
+    // every lane loads (a lane past b1 from the last own cell; its js, jn
+    // are lc0) and the own rows are selected
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int i = b0 + k * PB + (int)threadIdx.x;
-      if (i < b1) {
-        const int s_ = opaque(js[k]), n_ = opaque(jn[k]);
-        const int io = opaque(i);
-        const double cs = (s_ != lc0) ? a.ccs[io] : 0.0;
-        const double cn = (n_ != lc0) ? a.ccs[n_] : 0.0;
-        const double cw = a.ccw[io];
-        const double ce = (io + 1 != lc0) ? a.ccw[io + 1] : 0.0;
-        const double *P = a.x;
-        const double rr = a.fptmp[io] - a.ccc[io] * fetch(&P[io]) - cn * fetch(&P[n_]) -
-                          cs * fetch(&P[s_]) - cw * fetch(&P[io - 1]) -
-                          ce * fetch(&P[io + 1]);
-        const float rs = (float)(a.dis[io] * rr);
-        r[k] = rs;
-        Y(k) = 0.0f;
-        v3[0] += rr * rr;
-        v3[1] += (double)rs * (double)rs;
-      }
+      const bool own = i < b1;
+      const int s_ = opaque(js[k]), n_ = opaque(jn[k]);
+      const int io = opaque(own ? i : b1 - 1);
+      const int ie = io + 1;  // <= lc0
+      const double *P = a.x;
+      const double ccs_i = a.ccs[io], ccs_n = a.ccs[n_], ccw_i = a.ccw[io];
+      const double ccw_e = a.ccw[ie], fpt = a.fptmp[io], cci = a.ccc[io], di = a.dis[io];
+      const double p_i = fetch(&P[io]), p_n = fetch(&P[n_]), p_s = fetch(&P[s_]);
+      const double p_w = fetch(&P[io - 1]), p_e = fetch(&P[ie]);
+      const double cs = (s_ != lc0) ? ccs_i : 0.0;
+      const double cn = (n_ != lc0) ? ccs_n : 0.0;
+      const double cw = ccw_i;
+      const double ce = (ie != lc0) ? ccw_e : 0.0;
+      const double rr = fpt - cci * p_i - cn * p_n - cs * p_s - cw * p_w - ce * p_e;
+      const float rs = (float)(di * rr);
+      r[k] = own ? rs : 0.0f;
+      Y(k) = 0.0f;
+      v3[0] += own ? rr * rr : 0.0;
+      v3[1] += own ? (double)rs * (double)rs : 0.0;
     }
     // the partials go out while the preconditioner runs
     post_partials<3>(v3, a.xc, tag);
@@ -712,10 +721,8 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int i = b0 + k * PB + (int)threadIdx.x;
-      if (i < b1) {
-        lds[slot(i)] = u[k];
-        if (expo & (1u << k)) put_u32(hxw(i, tag, 0), __float_as_uint(u[k]));
-      }
+      lds[slot(i)] = u[k];
+      if (expo & (1u << k)) put_u32(hxw(i, tag, 0), __float_as_uint(u[k]));
     }
     fp2 = 0.0;
     post_flag(a.xc, tag);
@@ -744,13 +751,13 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
       s[k] = 0.0f;
       p[k] = 0.0f;
       q[k] = 0.0f;
-      if (i < b1) {
-        w[k] = u[k] + as_[k] * rval(js[k]) + an_[k] * rval(jn[k]) +
-               aw_[k] * rval(i - 1) + ae_[k] * rval(i + 1);
-        v2[0] += (double)r[k] * (double)u[k];
-        v2[1] += (double)w[k] * (double)u[k];
-        v2[2] += (double)r[k] * (double)r[k];
-      }
+      // a lane past b1 may read a slot no one wrote: its row is zero
+      const float wk = u[k] + as_[k] * rval(js[k]) + an_[k] * rval(jn[k]) +
+                       aw_[k] * rval(i - 1) + ae_[k] * rval(i + 1);
+      w[k] = i < b1 ? wk : 0.0f;
+      v2[0] += (double)r[k] * (double)u[k];
+      v2[1] += (double)w[k] * (double)u[k];
+      v2[2] += (double)r[k] * (double)r[k];
     }
     post_partials<3>(v2, a.xc, tag);
     {
@@ -797,7 +804,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
         Y(k) = (float)((double)Y(k) + alpha * (double)p[k]);
         r[k] = (float)((double)r[k] - alpha * (double)s[k]);
         u[k] = (float)((double)u[k] - alpha * (double)q[k]);
-        if (i < b1) lds[slot(i)] = u[k];
+        lds[slot(i)] = u[k];
       }
       // the outside cells' u, from their published u, z, q
       for (int h = threadIdx.x; h < nhalo; h += PB) {
@@ -813,13 +820,12 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
 #pragma unroll
       for (int k = 0; k < CPT; ++k) {
         const int i = b0 + k * PB + (int)threadIdx.x;
-        if (i < b1) {
-          w[k] = u[k] + as_[k] * rval(js[k]) + an_[k] * rval(jn[k]) +
-                 aw_[k] * rval(i - 1) + ae_[k] * rval(i + 1);
-          v[0] += (double)r[k] * (double)u[k];
-          v[1] += (double)w[k] * (double)u[k];
-          v[2] += (double)r[k] * (double)r[k];
-        }
+        const float wk = u[k] + as_[k] * rval(js[k]) + an_[k] * rval(jn[k]) +
+                         aw_[k] * rval(i - 1) + ae_[k] * rval(i + 1);
+        w[k] = i < b1 ? wk : 0.0f;
+        v[0] += (double)r[k] * (double)u[k];
+        v[1] += (double)w[k] * (double)u[k];
+        v[2] += (double)r[k] * (double)r[k];
       }
       post_partials<3>(v, a.xc, tag);
       {
@@ -966,7 +972,6 @@ __global__ __launch_bounds__(PB) void k_pcgmx_pl(PMxArgs a) {
   auto inside = [&](int j) { return j >= b0 && j < b1; };
   auto real = [&](int j) { return j > 0 && j != lc0; };
   auto slot = [&](int j) { return j - b0 + WIN; };
-  auto rval = [&](int j) { return real(j) ? lds[slot(j)] : 0.0f; };
   auto hxw = [&](int i, unsigned tag) {
     return a.xc.hx + (size_t)(tag & 1u) * HX_WORDS * (size_t)a.xc.ncell + (size_t)opaque(i);
   };
