@@ -69,6 +69,15 @@ struct Bar {
   int *abort;
 };
 
+// An index the compiler must treat as unknown at this point: keeps it from
+// hoisting the 64-bit addresses of every neighbour access out of the
+// iteration loop (which would pin two VGPRs per address for the whole
+// solve and spill); the address is recomputed where it is used.
+__device__ __forceinline__ int opaque(int v) {
+  __asm__ volatile("" : "+v"(v));
+  return v;
+}
+
 __device__ __forceinline__ void put_u32(unsigned *p, unsigned v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -193,15 +202,19 @@ __device__ bool exchange(const Xch &x, unsigned tag, const int *hl, int nhalo,
     unsigned spins = 0;
     int ok = 1;
     for (;;) {
+      // every lane loads (past the last workgroup, that one's flag again),
+      // so that the loads of a round are in flight together: loads under a
+      // condition would each wait for its own result
+      // (the lane number made opaque: the flag addresses are formed here,
+      // not held in registers across the caller's loops)
+      const int ln = opaque(lane);
       unsigned f[PB / 64];
 #pragma unroll
-      for (int u = 0; u < PB / 64; ++u) {
-        const int q = lane + 64 * u;
-        f[u] = q < nb ? get_u32(&x.flag[q * TICKET_PAD]) : tag;
-      }
+      for (int u = 0; u < PB / 64; ++u)
+        f[u] = get_u32(&x.flag[min(ln + 64 * u, nb - 1) * TICKET_PAD]);
       bool ready = true;
 #pragma unroll
-      for (int u = 0; u < PB / 64; ++u) ready = ready && f[u] >= tag;
+      for (int u = 0; u < PB / 64; ++u) ready = ready && (ln + 64 * u >= nb || f[u] >= tag);
       if (__all(ready)) break;
       ++spins;
       const int ab = (spins & 255u) == 0u
@@ -230,17 +243,16 @@ __device__ bool exchange(const Xch &x, unsigned tag, const int *hl, int nhalo,
       double a[4][NV];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int q = lane + 64 * (u0 + u);
+        const int q = min(opaque(lane) + 64 * (u0 + u), nb - 1);  // every lane loads
 #pragma unroll
         for (int j = 0; j < NV; ++j)
-          a[u][j] = q < nb ? __hip_atomic_load(&pslot(x.ps, tag, j)[q], __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT)
-                           : 0.0;
+          a[u][j] = __hip_atomic_load(&pslot(x.ps, tag, j)[q], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int j = 0; j < NV; ++j) v[j] += a[u][j];
+        for (int j = 0; j < NV; ++j) v[j] += lane + 64 * (u0 + u) < nb ? a[u][j] : 0.0;
     }
 #pragma unroll
     for (int j = 0; j < NV; ++j) v[j] = wave_sum_dpp(v[j]);
@@ -262,19 +274,17 @@ __device__ bool exchange(const Xch &x, unsigned tag, const int *hl, int nhalo,
       unsigned wz[HB];
 #pragma unroll
       for (int u = 0; u < HB; ++u) {
-        const int h = h0 + u * PB;
+        // every lane loads (past the list's end, its last cell again)
+        const size_t j = (size_t)hl[min(h0 + u * PB, nhalo - 1)];
         wv[u][0] = wv[u][1] = 0;
         wz[u] = 0;
-        if (h < nhalo) {
-          const size_t j = (size_t)hl[h];
-          if (NR == 4) {
-            wv[u][0] = get_u64(w64 + 2 * j);
-            wv[u][1] = get_u64(w64 + 2 * (nce + j));
-          } else {
-            wv[u][0] = get_u32(w32 + j);
-            if (NR >= 2) wv[u][1] = get_u32(w32 + nce + j);
-            if (NR >= 3) wz[u] = get_u32(w32 + 2 * nce + j);
-          }
+        if (NR == 4) {
+          wv[u][0] = get_u64(w64 + 2 * j);
+          wv[u][1] = get_u64(w64 + 2 * (nce + j));
+        } else {
+          wv[u][0] = get_u32(w32 + j);
+          if (NR >= 2) wv[u][1] = get_u32(w32 + nce + j);
+          if (NR >= 3) wz[u] = get_u32(w32 + 2 * nce + j);
         }
       }
 #pragma unroll
@@ -305,14 +315,6 @@ __device__ bool exchange(const Xch &x, unsigned tag, const int *hl, int nhalo,
 // neighbour slots of a row
 enum { NS = 0, NN = 1, NW = 2, NE = 3 };
 
-// An index the compiler must treat as unknown at this point: keeps it from
-// hoisting the 64-bit addresses of every neighbour access out of the
-// iteration loop (which would pin two VGPRs per address for the whole
-// solve and spill); the address is recomputed where it is used.
-__device__ __forceinline__ int opaque(int v) {
-  __asm__ volatile("" : "+v"(v));
-  return v;
-}
 
 // Compact list of the outside cells a workgroup's rows reference (sorted
 // by index).  `mark` holds 2*WIN flags on entry (slot g: cell b0-WIN+g for
@@ -921,10 +923,8 @@ __device__ bool halo_wait(const Xch &x, unsigned tag, int lo, int hi, const int 
   for (int h0 = t; h0 < nhalo; h0 += HB * PB) {
     unsigned v[HB];
 #pragma unroll
-    for (int u = 0; u < HB; ++u) {
-      const int h = h0 + u * PB;
-      v[u] = h < nhalo ? get_u32(w32 + (size_t)hl[h]) : 0u;
-    }
+    for (int u = 0; u < HB; ++u)  // every lane loads
+      v[u] = get_u32(w32 + (size_t)hl[min(h0 + u * PB, nhalo - 1)]);
 #pragma unroll
     for (int u = 0; u < HB; ++u) {
       const int h = h0 + u * PB;
