@@ -861,11 +861,11 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int i = b0 + k * PB + (int)threadIdx.x;
-      if (i < b1) {
-        const int io = opaque(i);
-        double *P = a.x;
-        publish(&P[io], fetch(&P[io]) + a.dis[io] * (double)Y(k));
-      }
+      const bool own = i < b1;
+      const int io = opaque(own ? i : b1 - 1);  // every lane loads
+      double *P = a.x;
+      const double pn = fetch(&P[io]) + a.dis[io] * (double)Y(k);
+      if (own) publish(&P[io], pn);
     }
     ++outer;
     if (!grid_barrier(a.bar, ++gen)) return;
@@ -1022,21 +1022,25 @@ __global__ __launch_bounds__(PB) void k_pcgmx_pl(PMxArgs a) {
   if (threadIdx.x == 0) lds[ZS] = 0.0f;
   __syncthreads();
   double fp2 = 0.0;
+  // every lane loads (a lane past b1 from the last own cell) and keeps the
+  // values of an own row only
 #pragma unroll
   for (int k = 0; k < CPT; ++k) {
     const int i = b0 + k * PB + (int)threadIdx.x;
-    as_[k] = an_[k] = aw_[k] = ae_[k] = 0.0f;
+    const bool own = i < b1;
+    const int ic = own ? i : b1 - 1;
     r[k] = u[k] = w[k] = s[k] = p[k] = 0.0f;
     Y(k) = Q(k) = Z(k) = 0.0f;
-    js[k] = jn[k] = lc0;
-    if (i < b1) {
-      const int s_ = a.ls[i], n_ = a.ln[i];
-      js[k] = s_;
-      jn[k] = n_;
-      as_[k] = a.as[i];
-      aw_[k] = a.aw[i];
-      if (n_ != lc0) an_[k] = a.as[n_];
-      if (i + 1 != lc0) ae_[k] = a.aw[i + 1];
+    const int s_ = a.ls[ic], n_ = a.ln[ic];
+    const float as_i = a.as[ic], aw_i = a.aw[ic], as_n = a.as[n_], aw_e = a.aw[ic + 1];
+    const double fpt = a.fptmp[ic];
+    js[k] = own ? s_ : lc0;
+    jn[k] = own ? n_ : lc0;
+    as_[k] = own ? as_i : 0.0f;
+    aw_[k] = own ? aw_i : 0.0f;
+    an_[k] = (own && n_ != lc0) ? as_n : 0.0f;
+    ae_[k] = (own && ic + 1 != lc0) ? aw_e : 0.0f;
+    if (own) {
       const int nbq[4] = {s_, n_, i - 1, i + 1};
       bool out = false;
 #pragma unroll
@@ -1048,8 +1052,8 @@ __global__ __launch_bounds__(PB) void k_pcgmx_pl(PMxArgs a) {
         }
       }
       if (out) expo |= 1u << k;
-      fp2 += a.fptmp[i] * a.fptmp[i];
     }
+    fp2 += own ? fpt * fpt : 0.0;
   }
   __syncthreads();
   const int nhalo = compact_halo(hl, b0, b1);
@@ -1190,11 +1194,11 @@ __global__ __launch_bounds__(PB) void k_pcgmx_pl(PMxArgs a) {
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const int i = b0 + k * PB + (int)threadIdx.x;
-      if (i < b1) {
-        const int io = opaque(i);
-        double *P = a.x;
-        publish(&P[io], fetch(&P[io]) + a.dis[io] * (double)Y(k));
-      }
+      const bool own = i < b1;
+      const int io = opaque(own ? i : b1 - 1);  // every lane loads
+      double *P = a.x;
+      const double pn = fetch(&P[io]) + a.dis[io] * (double)Y(k);
+      if (own) publish(&P[io], pn);
     }
     ++outer;
     if (!grid_barrier(a.bar, ++gen)) return;

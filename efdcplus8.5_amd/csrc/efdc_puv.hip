@@ -346,16 +346,20 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_scale(
   HA(FPTMP)[i] = HA(FP)[i] * CCMNMI;
   HA(CCCI)[i] = 1. / ccc;
   const int n = ln[i], e = le[i], w = lw[i];
+  // the neighbours' values are loaded whatever the tests below find (every
+  // index is a valid cell, lc0 included), so the loads are in flight together
+  const double cs_n = HA(CS)[n], cw_e = HA(CW)[e];
   bool bad = false;
-  if (n != lc0 && cn != HA(CS)[n]) bad = true;
-  if (e != lc0 && ce != HA(CW)[e]) bad = true;
+  if (n != lc0 && cn != cs_n) bad = true;
+  if (e != lc0 && ce != cw_e) bad = true;
   if (w == lc0 && ccw != 0.0) bad = true;
   if (bad) atomicOr(&st->symok, 1);
   if (dis) {
     const int js = ls[i];
+    const double cc_s = HA(CC)[js], cc_w = HA(CC)[i - 1];
     const double di = 1.0 / sqrt(ccc);
-    const double ds = (js != lc0) ? 1.0 / sqrt(HA(CC)[js] * CCMNMI) : 0.0;
-    const double dw = (i - 1 > 0) ? 1.0 / sqrt(HA(CC)[i - 1] * CCMNMI) : 0.0;
+    const double ds = (js != lc0) ? 1.0 / sqrt(cc_s * CCMNMI) : 0.0;
+    const double dw = (i - 1 > 0) ? 1.0 / sqrt(cc_w * CCMNMI) : 0.0;
     const double cs2 = (js != lc0) ? cs * CCMNMI : 0.0;
     dis[i] = di;
     as32[i] = (float)(cs2 * di * ds);
@@ -446,16 +450,21 @@ __device__ __forceinline__ Row row_of(const CgArgs &a, int i, bool fast) {
   q.jn = a.ln[i];
   q.cs = a.ccs[i];
   q.cw = a.ccw[i];
+  // the coefficients are loaded whatever the neighbour (every index is a
+  // valid cell, lc0 included) and selected after, so that the loads are in
+  // flight together
   if (fast) {
     q.jw = i - 1;
     q.je = i + 1;
-    q.cn = (q.jn != a.lc0) ? a.ccs[q.jn] : 0.0;
-    q.ce = (q.je != a.lc0) ? a.ccw[q.je] : 0.0;
+    const double cn = a.ccs[q.jn], ce = a.ccw[q.je];
+    q.cn = (q.jn != a.lc0) ? cn : 0.0;
+    q.ce = (q.je != a.lc0) ? ce : 0.0;
   } else {
     q.jw = a.lw[i];
     q.je = a.le[i];
-    q.cn = (q.jn != a.lc0) ? a.ccn[i] : 0.0;
-    q.ce = (q.je != a.lc0) ? a.cce[i] : 0.0;
+    const double cn = a.ccn[i], ce = a.cce[i];
+    q.cn = (q.jn != a.lc0) ? cn : 0.0;
+    q.ce = (q.je != a.lc0) ? ce : 0.0;
     if (q.jw == a.lc0) q.cw = 0.0;
   }
   if (q.js == a.lc0) q.cs = 0.0;
@@ -470,20 +479,24 @@ __device__ __forceinline__ Row row_of(const CgArgs &a, int i, bool fast) {
 template <int CPT = CG_CPT>
 __device__ __forceinline__ void cg_init_cells(const CgArgs &a, int i0, int i1, bool fast,
                                               double (&v)[2]) {
+  // every lane computes a row (past i1, the tile's last cell again: i0 <
+  // i1 for every tile launched) and only the own rows are kept, so the loads
+  // of all the thread's rows are in flight together
 #pragma unroll
   for (int u = 0; u < CPT; ++u) {
     const int i = i0 + u * EFDC_BLOCK + (int)threadIdx.x;
-    if (i < i1) {
-      const Row q = row_of(a, i, fast);
-      const double *P = a.x;
-      // RCG = FPTMP - CCC*P - CCN*PN - CCS*PS - CCW*PW - CCE*PE (:73)
-      const double r = a.fptmp[i] - a.ccc[i] * P[i] - q.cn * P[q.jn] -
-                       q.cs * P[q.js] - q.cw * P[q.jw] - q.ce * P[q.je];
-      a.r[i] = r;
-      const double z = r * a.ccci[i];  // PCG = RCG*CCCI (:80)
-      v[0] += r * z;
-      v[1] += a.fptmp[i] * a.fptmp[i];
-    }
+    const bool own = i < i1;
+    const int ic = own ? i : i1 - 1;
+    const Row q = row_of(a, ic, fast);
+    const double *P = a.x;
+    // RCG = FPTMP - CCC*P - CCN*PN - CCS*PS - CCW*PW - CCE*PE (:73)
+    const double fpt = a.fptmp[ic];
+    const double r = fpt - a.ccc[ic] * P[ic] - q.cn * P[q.jn] - q.cs * P[q.js] -
+                     q.cw * P[q.jw] - q.ce * P[q.je];
+    const double z = r * a.ccci[ic];  // PCG = RCG*CCCI (:80)
+    if (own) a.r[i] = r;
+    v[0] += own ? r * z : 0.0;
+    v[1] += own ? fpt * fpt : 0.0;
   }
 }
 
@@ -515,7 +528,9 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_cg_init(CgArgs a, double rsqm,
 __device__ __forceinline__ double pnew_at(const CgArgs &a, const double *pold,
                                           double beta, int j) {
   // PCG = TMPCG + BETA*PCG with TMPCG = CCCI*RCG (:134,158); ghosts hold 0
-  return (j > 0 && j != a.lc0) ? a.ccci[j] * a.r[j] + beta * pold[j] : 0.0;
+  // (loaded whatever j, a valid cell, and selected)
+  const double v = a.ccci[j] * a.r[j] + beta * pold[j];
+  return (j > 0 && j != a.lc0) ? v : 0.0;
 }
 
 template <int CPT = CG_CPT>
@@ -525,19 +540,21 @@ __device__ __forceinline__ void cg_a_cells(const CgArgs &a, int i0, int i1, bool
 #pragma unroll
   for (int u = 0; u < CPT; ++u) {
     const int i = i0 + u * EFDC_BLOCK + (int)threadIdx.x;
-    if (i < i1) {
-      const Row q = row_of(a, i, fast);
-      const double pc = a.ccci[i] * a.r[i] + beta * pold[i];
-      const double ps = pnew_at(a, pold, beta, q.js);
-      const double pn = pnew_at(a, pold, beta, q.jn);
-      const double pw = pnew_at(a, pold, beta, q.jw);
-      const double pe = pnew_at(a, pold, beta, q.je);
-      // APCG = CCC*PCG + CCS*PS + CCN*PN + CCW*PW + CCE*PE (:113)
-      const double apc = a.ccc[i] * pc + q.cs * ps + q.cn * pn + q.cw * pw + q.ce * pe;
+    const bool own = i < i1;
+    const int ic = own ? i : i1 - 1;  // as in cg_init_cells
+    const Row q = row_of(a, ic, fast);
+    const double pc = a.ccci[ic] * a.r[ic] + beta * pold[ic];
+    const double ps = pnew_at(a, pold, beta, q.js);
+    const double pn = pnew_at(a, pold, beta, q.jn);
+    const double pw = pnew_at(a, pold, beta, q.jw);
+    const double pe = pnew_at(a, pold, beta, q.je);
+    // APCG = CCC*PCG + CCS*PS + CCN*PN + CCW*PW + CCE*PE (:113)
+    const double apc = a.ccc[ic] * pc + q.cs * ps + q.cn * pn + q.cw * pw + q.ce * pe;
+    if (own) {
       pnew[i] = pc;
       a.ap[i] = apc;
-      v[0] += apc * pc;
     }
+    v[0] += own ? apc * pc : 0.0;
   }
 }
 
@@ -563,14 +580,17 @@ __device__ __forceinline__ void cg_b_cells(const CgArgs &a, int i0, int i1, doub
 #pragma unroll
   for (int u = 0; u < CPT; ++u) {
     const int i = i0 + u * EFDC_BLOCK + (int)threadIdx.x;
-    if (i < i1) {
-      a.x[i] = a.x[i] + alpha * pnew[i];       // :132
-      const double r = a.r[i] - alpha * a.ap[i];  // :133
+    const bool own = i < i1;
+    const int ic = own ? i : i1 - 1;  // as in cg_init_cells
+    const double x = a.x[ic] + alpha * pnew[ic];  // :132
+    const double r = a.r[ic] - alpha * a.ap[ic];  // :133
+    const double z = a.ccci[ic] * r;              // :134
+    if (own) {
+      a.x[i] = x;
       a.r[i] = r;
-      const double z = a.ccci[i] * r;          // :134
-      v[0] += r * z;
-      v[1] += r * r;
     }
+    v[0] += own ? r * z : 0.0;
+    v[1] += own ? r * r : 0.0;
   }
 }
 
