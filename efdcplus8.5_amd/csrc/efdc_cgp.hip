@@ -1237,7 +1237,7 @@ __global__ __launch_bounds__(256) void k_mx_coef(const double *ccc, const double
 
 __global__ void k_pcg_reset(unsigned *ctr, int *abort) {
   const int t = threadIdx.x;
-  if (t < 9 * TICKET_PAD) ctr[t] = 0u;
+  if (t < PCG_NCTR) ctr[t] = 0u;
   if (t == 0) *abort = 0;
 }
 
@@ -1290,7 +1290,7 @@ int pcg_prepare(efdc_ctx *c) {
     return s;
   if ((s = dev_alloc(c, &c->pcg_ps, sizeof(double) * 2 * NVMAX * PB))) return s;
   if ((s = dev_alloc(c, (void **)&c->pcg_flag, sizeof(unsigned) * PB * TICKET_PAD))) return s;
-  if ((s = dev_alloc(c, (void **)&c->pcg_ctr, sizeof(unsigned) * 9 * TICKET_PAD))) return s;
+  if ((s = dev_alloc(c, (void **)&c->pcg_ctr, sizeof(unsigned) * PCG_NCTR))) return s;
   c->pcg_abort = &c->st->pabort;
   c->pcg_launch = 0;
   c->pcg_ready = true;
@@ -1349,8 +1349,11 @@ static int next_tag0(efdc_ctx *c, unsigned &tag0) {
 int pcg_solve(efdc_ctx *c, bool mixed, bool defer) {
   int s;
   if ((s = pcg_prepare(c))) return s;
-  k_pcg_reset<<<1, 9 * TICKET_PAD, 0, c->stream>>>(c->pcg_ctr, c->pcg_abort);
-  HIP_TRY(c, hipGetLastError());
+  if (!c->pcg_reset_done) {
+    k_pcg_reset<<<1, PCG_NCTR, 0, c->stream>>>(c->pcg_ctr, c->pcg_abort);
+    HIP_TRY(c, hipGetLastError());
+  }
+  c->pcg_reset_done = false;  // this launch uses them
   int nb = 0;
   Bar bar{c->pcg_ctr, c->pcg_abort};
   if (!mixed) {

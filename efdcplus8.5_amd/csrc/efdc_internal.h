@@ -46,6 +46,8 @@ struct CgState {
 // that no two counters share a cache line.
 #define TICKET_PAD 32
 #define TICKET_SITE (9 * TICKET_PAD)
+// words of the persistent solve's barrier counters (9 counters, TICKET_PAD apart)
+#define PCG_NCTR (9 * TICKET_PAD)
 // cells per workgroup of the multi-kernel CG (CG_CPT per thread)
 #define CG_CPT 2
 #define TICKET_SITES 16
@@ -159,6 +161,10 @@ struct efdc_ctx {
   unsigned pcg_launch = 0;     // launch number, gives the first tag
   unsigned *pcg_ctr = nullptr; // barrier counters
   int *pcg_abort = nullptr;     // &st->pabort (read back with the state)
+  // the persistent solve's counters and abort flag are already zero (the
+  // scale pass before it cleared them); the next persistent launch skips
+  // its own clearing
+  bool pcg_reset_done = false;
 
   // open boundaries
   PbcDev pbc[4] = {};

@@ -326,10 +326,18 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_bc_min(
 // the Jacobi scaling of the mixed solve: dis = 1/sqrt(CCC) and the FP32
 // entries A'(i,S) = CCS d(i) d(S), A'(i,W) = CCW d(i) d(i-1), the
 // neighbours' CCC recomputed as CC*CCMNMI (the same bits as theirs).
+// With rctr != nullptr the first workgroup also clears the persistent
+// solve's barrier counters (nctr words) and abort flag, which the solve
+// launched next would otherwise clear in a launch of its own.
 __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_scale(
     HPtr h, const int *__restrict__ lw, const int *__restrict__ le,
     const int *__restrict__ ln, const int *__restrict__ ls, int nwet, int lc0, CgState *st,
-    const double *segmin, const int *segid, double *dis, float *as32, float *aw32) {
+    const double *segmin, const int *segid, double *dis, float *as32, float *aw32,
+    unsigned *rctr, int *rabort, int nctr) {
+  if (rctr && blockIdx.x == 0) {
+    for (int q = threadIdx.x; q < nctr; q += blockDim.x) rctr[q] = 0u;
+    if (threadIdx.x == 0) *rabort = 0;
+  }
   int tile;
   const int i = cell_index(nwet, tile);
   if (i > nwet) return;
@@ -1855,9 +1863,13 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
       if (s) return s;
     }
     const bool coef = !segs && c->dis && c->prm.cg_mode == EFDC_CG_MIXED;
+    // the persistent solve's counters, once they exist, are cleared here
+    const bool rst = !segs && c->pcg_ctr != nullptr && c->pcg_abort != nullptr;
     TIMED(c, TK_PUV_SCALE, k_puv_scale<<<nb, EFDC_BLOCK, 0, c->stream>>>(
         h, c->dLW, c->dLE, c->dLN, c->dLS, nwet, lc0, c->st, segs ? c->d_segmin : nullptr,
-        segs ? c->d_segid : nullptr, coef ? c->dis : nullptr, c->as32, c->aw32));
+        segs ? c->d_segid : nullptr, coef ? c->dis : nullptr, c->as32, c->aw32,
+        rst ? c->pcg_ctr : nullptr, rst ? c->pcg_abort : nullptr, PCG_NCTR));
+    if (rst) c->pcg_reset_done = true;
     c->mx_coef_ok = coef;
     int s = cg_dispatch(c, stats, /*defer=*/true, &ran_mixed);
     c->mx_coef_ok = false;
