@@ -67,6 +67,7 @@ typedef unsigned long long u64;
 struct Bar {
   unsigned *ctr;                  // 9 counters, TICKET_PAD apart
   int *abort;
+  unsigned spin_max;              // polling rounds before a wait gives up
 };
 
 // An index the compiler must treat as unknown at this point: keeps it from
@@ -110,7 +111,7 @@ __device__ bool grid_barrier(const Bar &b, unsigned gen) {
     int ok = 1;
     unsigned spins = 0;
     while (__hip_atomic_load(gg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (++spins > SPIN_MAX ||
+      if (++spins > b.spin_max ||
           ((spins & 255u) == 0u &&
            __hip_atomic_load(b.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
         __hip_atomic_store(b.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -222,7 +223,7 @@ __device__ bool exchange(const Xch &x, unsigned tag, const int *hl, int nhalo,
                                __hip_atomic_load(b.abort, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT))
                          : 0;
-      if (spins > SPIN_MAX || ab) {
+      if (spins > b.spin_max || ab) {
         if (lane == 0)
           __hip_atomic_store(b.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ok = 0;
@@ -906,7 +907,7 @@ __device__ bool halo_wait(const Xch &x, unsigned tag, int lo, int hi, const int 
                                __hip_atomic_load(b.abort, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT))
                          : 0;
-      if (spins > SPIN_MAX || ab) {
+      if (spins > b.spin_max || ab) {
         if (lane == 0)
           __hip_atomic_store(b.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ok = 0;
@@ -1326,10 +1327,24 @@ bool pcg_applicable(efdc_ctx *c, bool mixed) {
                : pcg_shape(c, c->pcg_occ64, PCG64_CPT_MAX, nb) > 0;
 }
 
+// The persistent grid is sized to be co-resident (the occupancy table times
+// the CUs).  A cooperative launch guarantees it, at the price of the
+// runtime's cooperative dispatch (about 15 us per solve on MI355X); a plain
+// launch of the same grid is co-resident whenever no other work holds the
+// CUs, and otherwise every bounded wait gives up and the solve reports an
+// abort, after which it is solved again with the cooperative launch
+// (pcg_solve, pcg_retry_deferred).  pcg_launch_mode 1 always uses the
+// cooperative launch.
 template <typename Args>
 static int coop_launch(efdc_ctx *c, const void *fn, int nb, Args &args) {
   void *kargs[] = {&args};
-  HIP_TRY(c, hipLaunchCooperativeKernel(fn, dim3(nb), dim3(PB), kargs, 0, c->stream));
+  if (c->pcg_launch_mode == 0 && !c->pcg_force_coop) {
+    HIP_TRY(c, hipLaunchKernel(fn, dim3(nb), dim3(PB), kargs, 0, c->stream));
+    c->pcg_last_plain = true;
+  } else {
+    HIP_TRY(c, hipLaunchCooperativeKernel(fn, dim3(nb), dim3(PB), kargs, 0, c->stream));
+    c->pcg_last_plain = false;
+  }
   return EFDC_OK;
 }
 
@@ -1355,7 +1370,11 @@ int pcg_solve(efdc_ctx *c, bool mixed, bool defer) {
   }
   c->pcg_reset_done = false;  // this launch uses them
   int nb = 0;
-  Bar bar{c->pcg_ctr, c->pcg_abort};
+  // a plain launch may be given a small spin limit (to exercise the retry
+  // from the host); the cooperative one always has the full limit
+  const bool plain = c->pcg_launch_mode == 0 && !c->pcg_force_coop;
+  Bar bar{c->pcg_ctr, c->pcg_abort,
+          plain && c->pcg_spin_test > 0 ? c->pcg_spin_test : SPIN_MAX};
   if (!mixed) {
     const int cpt = pcg_shape(c, c->pcg_occ64, PCG64_CPT_MAX, nb);
     if (!cpt) { c->err = "persistent CG: problem too large"; return EFDC_ERR_ARG; }
@@ -1467,6 +1486,15 @@ int pcg_solve(efdc_ctx *c, bool mixed, bool defer) {
                             c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (c->st_host->pabort) {
+    if (c->pcg_last_plain) {
+      // the plain launch was not co-resident: solve again, cooperatively,
+      // from the P it reached (an iterate of the same system)
+      ++c->pcg_retries;
+      c->pcg_force_coop = true;
+      s = pcg_solve(c, mixed, false);
+      c->pcg_force_coop = false;
+      return s;
+    }
     c->err = "persistent CG: an exchange between workgroups timed out";
     return EFDC_ERR_HIP;
   }

@@ -936,6 +936,20 @@ int efdc_set_cg_pipeline(efdc_ctx_t *c, int32_t mode) {
   return EFDC_OK;
 }
 
+int efdc_set_cg_launch(efdc_ctx_t *c, int32_t mode, uint32_t spin_limit) {
+  if (!c || mode < 0 || mode > 1) return EFDC_ERR_ARG;
+  c->pcg_launch_mode = mode;
+  c->pcg_spin_test = spin_limit;
+  return EFDC_OK;
+}
+
+int efdc_get_cg_launch(efdc_ctx_t *c, int32_t *mode, int32_t *retries) {
+  if (!c || !mode || !retries) return EFDC_ERR_ARG;
+  *mode = c->pcg_launch_mode;
+  *retries = c->pcg_retries;
+  return EFDC_OK;
+}
+
 int efdc_get_cg_pipeline(efdc_ctx_t *c, int32_t *mode) {
   if (!c || !mode) return EFDC_ERR_ARG;
   *mode = c->pcg_pipe_mode != 0 ? c->pcg_pipe_mode : (c->pcg_pipe_tuned ? 2 : 1);

@@ -165,6 +165,11 @@ struct efdc_ctx {
   // scale pass before it cleared them); the next persistent launch skips
   // its own clearing
   bool pcg_reset_done = false;
+  int pcg_launch_mode = 0;      // 0 plain launch with a cooperative retry, 1 cooperative
+  bool pcg_force_coop = false;  // the retry (and drying steps) launch cooperatively
+  bool pcg_last_plain = false;  // the last persistent launch was a plain one
+  unsigned pcg_spin_test = 0;   // > 0: spin limit of the plain launches (tests)
+  int pcg_retries = 0;          // plain launches solved again cooperatively
 
   // open boundaries
   PbcDev pbc[4] = {};

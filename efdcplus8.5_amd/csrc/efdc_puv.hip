@@ -392,6 +392,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_cg_check(
   if (bad) atomicOr(&st->symok, 1);
 }
 
+__global__ void k_clear_negdep(CgState *st) { st->negdep = 0; }
+
 __global__ void k_reset_flags(CgState *st) {
   st->symok = 0;
   st->negdep = 0;
@@ -1819,6 +1821,10 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
         (float *)c->dK[EFDC_H1PK - EFDC_HPK], c->pitch, c->KC, nwet, dt, G,
         c->prm.RITB1, c->prm.SNLT, dry ? 1 : 0, c->prm.implicit_drag ? 1 : 0));
   c->ncordry = 0;
+  // a step that may solve more than once (drying) or records the solution
+  // (adjoint) launches the persistent solve cooperatively, so that it never
+  // has to be solved again after the passes that follow it
+  c->pcg_force_coop = dry || adj_recording(c);
   bool ran_mixed = false;
   int cgs = EFDC_OK;
   for (;;) {
@@ -1937,11 +1943,29 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
   if ((s = read_state(c))) return s;
   if (c->pcg_deferred) {
     c->pcg_deferred = false;
-    if (c->st_host->pabort) {
+    if (c->st_host->pabort && c->pcg_last_plain && !c->pcg_force_coop) {
+      // the plain launch was not co-resident: solve again, cooperatively,
+      // from the P the step reached (any P is a starting iterate of the same
+      // system), then redo the passes that followed the solve
+      ++c->pcg_retries;
+      c->pcg_force_coop = true;
+      s = cg_dispatch(c, stats, /*defer=*/false, &ran_mixed);
+      c->pcg_force_coop = false;
+      if (s != EFDC_OK && s != EFDC_ERR_CG_MAXITER) return s;
+      cgs = s;
+      k_clear_negdep<<<1, 1, 0, c->stream>>>(c->st);
+      TIMED(c, TK_PUV_POST1, k_puv_post1<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, dt));
+      TIMED(c, TK_PUV_POST2, k_puv_post2<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->pbcflag,
+                                                    nwet, dt, G));
+      TIMED(c, TK_PUV_POST3, k_puv_post3<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLW, c->dLS, nwet, c->st));
+      HIP_TRY(c, hipGetLastError());
+      if ((s = read_state(c))) return s;
+    } else if (c->st_host->pabort) {
       c->err = "persistent CG: an exchange between workgroups timed out";
       return EFDC_ERR_HIP;
+    } else {
+      cgs = cg_finish(c, ran_mixed, stats);
     }
-    cgs = cg_finish(c, ran_mixed, stats);
   }
   if (c->st_host->symok != c->symok_cache && c->symok_cache >= 0) {
     // cannot happen for a fixed grid and boundary tables (the cache is

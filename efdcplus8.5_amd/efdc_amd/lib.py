@@ -195,6 +195,8 @@ def load(path: Optional[str] = None):
         "efdc_get_cg_replay": ([P, C.POINTER(C.c_int32)], C.c_int),
         "efdc_set_cg_pipeline": ([P, C.c_int32], C.c_int),
         "efdc_get_cg_pipeline": ([P, C.POINTER(C.c_int32)], C.c_int),
+        "efdc_set_cg_launch": ([P, C.c_int32, C.c_uint32], C.c_int),
+        "efdc_get_cg_launch": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
         "efdc_get_segment_tile": ([P, C.POINTER(C.c_int32)], C.c_int),
         "efdc_restart_text": ([P, C.POINTER(_fmt.Restart), C.c_char_p, C.c_int64,
                                C.POINTER(C.c_int64)], C.c_int),
@@ -625,6 +627,18 @@ class Context:
         v = C.c_int32(0)
         self._check(self.lib.efdc_get_cg_pipeline(self.h, C.byref(v)))
         return v.value
+
+    def set_cg_launch(self, mode: int, spin_limit: int = 0):
+        """Launch of the persistent solves: 0 plain (repeated cooperatively
+        if it was not co-resident), 1 cooperative; spin_limit > 0 shortens the
+        plain launches' waits (tests of the repetition)."""
+        self._check(self.lib.efdc_set_cg_launch(self.h, mode, spin_limit))
+
+    def cg_launch(self) -> tuple:
+        """(mode, solves repeated cooperatively so far)."""
+        m, r = C.c_int32(0), C.c_int32(0)
+        self._check(self.lib.efdc_get_cg_launch(self.h, C.byref(m), C.byref(r)))
+        return m.value, r.value
 
     def segment_tile(self) -> int:
         v = C.c_int32(0)

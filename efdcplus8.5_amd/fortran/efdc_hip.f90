@@ -348,6 +348,18 @@ module efdc_hip
       integer(c_int32_t), intent(out) :: mode
     end function
 
+    integer(c_int) function efdc_set_cg_launch(ctx, mode, spin_limit) bind(C, name="efdc_set_cg_launch")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: mode, spin_limit
+    end function
+
+    integer(c_int) function efdc_get_cg_launch(ctx, mode, retries) bind(C, name="efdc_get_cg_launch")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: mode, retries
+    end function
+
     integer(c_int) function efdc_set_segment_tile(ctx, cells) bind(C, name="efdc_set_segment_tile")
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx

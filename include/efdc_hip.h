@@ -596,6 +596,21 @@ int efdc_get_cg_replay(efdc_ctx_t *ctx, int32_t *mode);
    the form the next solve runs. */
 int efdc_set_cg_pipeline(efdc_ctx_t *ctx, int32_t mode);
 int efdc_get_cg_pipeline(efdc_ctx_t *ctx, int32_t *mode);
+/* Launch of the persistent solves (no counterpart in the reference).  The
+   grid is sized to be co-resident on the GPU.  mode 0 (default): a plain
+   launch, which is co-resident whenever no other work holds the CUs; if it
+   was not, its bounded waits give up and the solve is repeated with a
+   cooperative launch from the P the first one reached (in a CALPUV2C step,
+   together with the passes after the solve), so the result is a converged
+   solve either way; steps with drying or adjoint recording launch
+   cooperatively.  mode 1: always the cooperative launch (guaranteed
+   co-residency, about 15 us more per solve).  spin_limit > 0 gives the
+   plain launches' waits that many polling rounds instead of the full limit,
+   which makes them give up whenever a wait is not met at once (to exercise
+   the repetition); 0 restores the full limit.  get returns the mode and
+   the number of solves repeated so far. */
+int efdc_set_cg_launch(efdc_ctx_t *ctx, int32_t mode, uint32_t spin_limit);
+int efdc_get_cg_launch(efdc_ctx_t *ctx, int32_t *mode, int32_t *retries);
 
 /* Per-basin results of the last efdc_calpuv2c / efdc_congrad with
    segments: iters[s] and status[s] (0 or EFDC_ERR_CG_MAXITER) for s <

@@ -190,3 +190,84 @@ def test_wide_rows_fall_back_or_refuse(oracle_mod):
     with pytest.raises(lib.EfdcError) as e:
         _solve(case.grid, system, cg_kernel=lib.CG_KERNEL_PERSISTENT)
     assert e.value.status == 1
+
+
+def _solve_launch(grid, system, mode, spin_limit=0, **kw):
+    from efdc_amd.lib import Context
+    kw.setdefault("RSQM", TIGHT)
+    kw.setdefault("ITERM", 4000)
+    ctx = Context(grid, **kw)
+    ctx.set_cg_launch(mode, spin_limit)
+    for k, v in system.items():
+        ctx.set(k, v)
+    out = ctx.congrad()
+    p = ctx.get("P")
+    launch = ctx.cg_launch()
+    ctx.close()
+    return out, p, launch
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_plain_and_cooperative_launch_same_bits(oracle_mod, mixed):
+    """The persistent solve launched plainly (the default) and cooperatively
+    runs the same kernel on the same grid: the same P, bit for bit."""
+    from efdc_amd import lib
+    case, system, p_ref = _lake_system(oracle_mod, 200, 7)
+    kw = dict(cg_kernel=lib.CG_KERNEL_PERSISTENT)
+    if mixed:
+        kw.update(cg_mode=lib.CG_MIXED, RSQM=1e-20)
+    o0, p0, l0 = _solve_launch(case.grid, system, 0, **kw)
+    o1, p1, l1 = _solve_launch(case.grid, system, 1, **kw)
+    print("plain", o0["iters"], l0, "cooperative", o1["iters"], l1)
+    assert l0[0] == 0 and l1 == (1, 0)
+    assert o0["iters"] == o1["iters"]
+    i = wet_index(case)
+    np.testing.assert_array_equal(p0[i], p1[i])
+    assert eta_err(p0[i], p_ref[i]) <= (1e-6 if mixed else 1e-9)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_plain_launch_repeated_after_its_waits_give_up(oracle_mod, mixed):
+    """A spin limit of one polling round makes the plain launch's waits give
+    up as soon as a workgroup is not there yet (what a grid that is not
+    co-resident would see); the solve is then repeated cooperatively from
+    the P the first launch reached and meets the same bar against the
+    oracle."""
+    from efdc_amd import lib
+    case, system, p_ref = _lake_system(oracle_mod, 200, 7)
+    kw = dict(cg_kernel=lib.CG_KERNEL_PERSISTENT)
+    if mixed:
+        kw.update(cg_mode=lib.CG_MIXED, RSQM=1e-20)
+    out, p, launch = _solve_launch(case.grid, system, 0, spin_limit=1, **kw)
+    i = wet_index(case)
+    d = eta_err(p[i], p_ref[i])
+    print(f"mixed={mixed}: launch {launch} iters {out['iters']} deta {d:.3e}")
+    assert launch[1] >= 1
+    assert d <= (1e-6 if mixed else 1e-9)
+
+
+def test_step_repeats_the_solve_and_the_passes_after_it(oracle_mod):
+    """The same inside CALPUV2C steps, where the abort is read after the
+    step: the solve and the passes after it are run again, and P, HP and the
+    transports meet the oracle's bars of the parity tests."""
+    from efdc_amd import grids, lib
+    # enough cells for about a hundred workgroups, whose arrivals at an
+    # exchange are spread more than one polling round apart
+    case = grids.tidal_basin(320, 160, 1, seed=11)
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref, rsqm=TIGHT, iterm=2000)
+    ctx = lib.Context(case.grid, RSQM=TIGHT, ITERM=2000,
+                      cg_kernel=lib.CG_KERNEL_PERSISTENT).upload_case(case)
+    ctx.set_cg_launch(0, 1)
+    i = wet_index(case)
+    for n in (1, 2, 3):
+        o.step(n)
+        ctx.step(case.dt, n)
+        d = eta_err(ctx.get("P")[i], ref.h["P"][i])
+        print(f"step {n}: launch {ctx.cg_launch()} deta {d:.3e}")
+        assert d <= 1e-9
+        for name in ("HP", "UHDYE", "VHDXE", "HU", "HV"):
+            a, b = ctx.get(name)[i], ref.h[name][i]
+            assert np.max(np.abs(a - b)) <= 1e-9 * max(1.0, np.max(np.abs(b))), name
+    assert ctx.cg_launch()[1] >= 1
+    ctx.close()
