@@ -52,6 +52,12 @@ namespace {
 constexpr int PB = 512;           // threads per workgroup
 constexpr int PW = PB / 64;       // waves per workgroup
 constexpr unsigned SPIN_MAX = 1u << 21;
+// An inner (FP32) solve stops at the smaller of two reductions of its
+// residual: the set one (rtol), or what the outer FP64 test still needs,
+// with a margin of 10 in the norm (squared here), so that a round that only
+// has to finish the solve is not carried past the target.  The outer test
+// is always made on the FP64 residual.
+constexpr double INNER_MARGIN2 = 1e-2;
 constexpr int WIN = 2048;         // LDS window reach beyond the own range
 constexpr int HMAX = 2 * WIN;     // outside cells of one workgroup, at most
 constexpr int NVMAX = 3;          // partials per exchange, at most
@@ -741,6 +747,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
     }
     if (rsq64 <= rsqm) break;
     if (outer >= a.max_outer || bn_in == 0.0) { status = 1; break; }
+    const double rt2 = fmax(a.rtol2, INNER_MARGIN2 * rsqm / rsq64);
 
     // ---- w0 = A' u0, z0 = M^-1 w0; gamma0 = (r,u), delta0 = (w,u),
     // rho0 = (r,r); s_-1 = p_-1 = q_-1 = 0
@@ -853,7 +860,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
       alpha_prev = alpha;
       gam = s3[0];
       del = s3[1];
-      if (s3[2] <= a.rtol2 * bn_in || inner >= a.iterm) break;
+      if (s3[2] <= rt2 * bn_in || inner >= a.iterm) break;
     }
     total += inner;
     ++tag;
@@ -1118,6 +1125,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx_pl(PMxArgs a) {
     }
     if (rsq64 <= rsqm) break;
     if (outer >= a.max_outer || bn_in == 0.0) { status = 1; break; }
+    const double rt2 = fmax(a.rtol2, INNER_MARGIN2 * rsqm / rsq64);
 
     // ---- w0 = A' u0; s, p, q, z start at 0
     for (int h = threadIdx.x; h < nhalo; h += PB) lds[slot(hl[h])] = __uint_as_float(raw[h]);
@@ -1162,7 +1170,7 @@ __global__ __launch_bounds__(PB) void k_pcgmx_pl(PMxArgs a) {
       double s3[3];
       if (!exchange<3, 0>(a.xc, tag, hl, 0, raw, a.bar, s3)) return;
       const double gam = s3[0], del = s3[1];
-      if (s3[2] <= a.rtol2 * bn_in || inner >= a.iterm) break;
+      if (s3[2] <= rt2 * bn_in || inner >= a.iterm) break;
       double beta, alpha;
       if (inner == 0) {
         beta = 0.0;
