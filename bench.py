@@ -1030,6 +1030,10 @@ def lake_bench(args):
     ck = {"auto": 0, "multi": 1, "persistent": 2}[args.cg_kernel]
     ctx_m = make_ctx(case, mixed=True, cg_kernel=ck)
     ctx_d = make_ctx(case, mixed=False, cg_kernel=ck)
+    # the FP64 solve in the pipelined form (one exchange per iteration), a
+    # side line: the reference's recurrences are the FP64 default
+    ctx_p = make_ctx(case, mixed=False, cg_kernel=ck)
+    ctx_p.set_cg64_pipeline(True)
     i = np.arange(2, case.grid.LA + 1) - 1
 
     # ---- accuracy + CPU baseline on a bounded sample of the same steps
@@ -1057,11 +1061,13 @@ def lake_bench(args):
         del o
     run_steps(ctx_m, case, 1, ncmp)
     run_steps(ctx_d, case, 1, ncmp)
-    err_m = err_d = None
+    run_steps(ctx_p, case, 1, ncmp)
+    err_m = err_d = err_p = None
     if ref_case is not None:
         pr = ref_case.h["P"][i]
         err_m = float(np.max(np.abs(ctx_m.get("P")[i] - pr))) / 9.81
         err_d = float(np.max(np.abs(ctx_d.get("P")[i] - pr))) / 9.81
+        err_p = float(np.max(np.abs(ctx_p.get("P")[i] - pr))) / 9.81
         del ref_case, pr
 
     # ---- timed region: FP32-refined, then FP64 (the first warm-up step
@@ -1073,9 +1079,13 @@ def lake_bench(args):
     t_m, it_m = timed(ctx_m, case, n0 + wu, args.steps)
     run_steps(ctx_d, case, n0, wu)
     t_d, it_d = timed(ctx_d, case, n0 + wu, args.steps)
+    run_steps(ctx_p, case, n0, wu)
+    t_p, it_p = timed(ctx_p, case, n0 + wu, args.steps)
     n1 = n0 + wu + args.steps
     rf_m = solve_roofline(ctx_m, case, n1, args.steps, True, nwet, workload)
     rf_d = solve_roofline(ctx_d, case, n1, args.steps, False, nwet, workload)
+    rf_p = solve_roofline(ctx_p, case, n1, args.steps, False, nwet, workload)
+    ctx_p.close()
 
     value = nwet * args.steps / t_m
     copy_gbs = stream_copy_gbs(ctx_m)
@@ -1132,6 +1142,13 @@ def lake_bench(args):
                                                      "bytes_per_launch", "traffic", "hbm_gbs",
                                                      "compact_gbs", "iters_per_solve")},
             "fp64_frac": rf_d["achieved"] / HBM_PEAK_GBS,
+            "fp64_pipelined": {
+                "ms_per_step": 1e3 * t_p / args.steps,
+                "value": nwet * args.steps / t_p,
+                "solve_avg_us": rf_p["avg_us"],
+                "iters_per_solve": rf_p["iters_per_solve"],
+                "eta_err_vs_oracle_m": err_p,
+            },
             "cg_iters_per_step_mixed": float(np.mean(it_m)),
             "cg_iters_per_step_fp64": float(np.mean(it_d)),
             "eta_err_vs_oracle_m": {"mixed": err_m, "fp64": err_d,

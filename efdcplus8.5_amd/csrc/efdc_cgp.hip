@@ -894,11 +894,9 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
 }
 
 // Wait until the workgroups lo..hi (the owners of this workgroup's outside
-// cells) have raised the flag of exchange `tag`, then fetch one word of each
-// outside cell into raw[h].  Wave 0 polls (lane l: workgroup lo+l); false if
-// the wait was abandoned.
-__device__ bool halo_wait(const Xch &x, unsigned tag, int lo, int hi, const int *hl, int nhalo,
-                          unsigned *raw, const Bar &b) {
+// cells) have raised the flag of exchange `tag`.  Wave 0 polls (lane l:
+// workgroup lo+l); false if the wait was abandoned.
+__device__ bool halo_poll(const Xch &x, unsigned tag, int lo, int hi, const Bar &b) {
   __shared__ int s_ok2;
   const int t = threadIdx.x, lane = t & 63;
   if (t < 64) {
@@ -925,7 +923,14 @@ __device__ bool halo_wait(const Xch &x, unsigned tag, int lo, int hi, const int 
     if (lane == 0) s_ok2 = ok;
   }
   __syncthreads();
-  if (!s_ok2) return false;
+  return s_ok2 != 0;
+}
+
+// The same, then one FP32 word of each outside cell into raw[h].
+__device__ bool halo_wait(const Xch &x, unsigned tag, int lo, int hi, const int *hl, int nhalo,
+                          unsigned *raw, const Bar &b) {
+  if (!halo_poll(x, tag, lo, hi, b)) return false;
+  const int t = threadIdx.x;
   const unsigned *w32 = x.hx + (size_t)(tag & 1u) * HX_WORDS * (size_t)x.ncell;
   constexpr int HB = 4;
   for (int h0 = t; h0 < nhalo; h0 += HB * PB) {
@@ -941,6 +946,31 @@ __device__ bool halo_wait(const Xch &x, unsigned tag, int lo, int hi, const int 
   }
   __syncthreads();
   return true;
+}
+
+// FP64 value 0 of each outside cell in the slots of exchange `tag` (posted
+// with put_f64 at word 2*i of that parity's first FP64 plane) into raw[h]
+// (low word) and raw[HMAX + h] (high word); the flags must already be up.
+__device__ void fetch_halo64(const Xch &x, unsigned tag, const int *hl, int nhalo,
+                             unsigned *raw) {
+  const int t = threadIdx.x;
+  const unsigned *w64 = x.hx + (size_t)(tag & 1u) * 2 * 2 * (size_t)x.ncell;
+  constexpr int HB = 4;
+  for (int h0 = t; h0 < nhalo; h0 += HB * PB) {
+    u64 v[HB];
+#pragma unroll
+    for (int u = 0; u < HB; ++u)  // every lane loads
+      v[u] = get_u64(w64 + 2 * (size_t)hl[min(h0 + u * PB, nhalo - 1)]);
+#pragma unroll
+    for (int u = 0; u < HB; ++u) {
+      const int h = h0 + u * PB;
+      if (h < nhalo) {
+        raw[h] = (unsigned)v[u];
+        raw[HMAX + h] = (unsigned)(v[u] >> 32);
+      }
+    }
+  }
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------
@@ -1227,6 +1257,239 @@ __global__ __launch_bounds__(PB) void k_pcgmx_pl(PMxArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------
+// FP64 Jacobi-PCG, pipelined (opt-in: efdc_set_cg64_pipeline).  The
+// reference's PCG (EFDC/congrad.f90:91-169) rewritten in the pipelined order
+// of Ghysels and Vanroose (2014), as k_pcgmx_pl does for the mixed inner
+// solve: an iteration posts its sums (r,u), (w,u), (r,r), the m = CCCI w
+// of its outside-read cells and its flag; it waits only for the workgroups
+// that own its outside cells, forms n = A m, and only then waits for every
+// workgroup's sums to take alpha and beta.  The Jacobi preconditioner is
+// diagonal, so u = CCCI r, m = CCCI w and q = CCCI s are formed where they
+// are used; the state is x, r, w, s, p (registers) and z = A q (LDS):
+//   z = n + beta z, s = w + beta s, p = u + beta p,
+//   x += alpha p, r -= alpha s, w -= alpha z.
+// Same iterates as the reference's recurrences in exact arithmetic, one
+// exchange per iteration instead of two.  The stop test is the
+// reference's, (r,r) <= RSQM (:165), made on the sums of the iteration
+// that would update from r.
+// ---------------------------------------------------------------------
+template <int CPT>
+__global__ __launch_bounds__(PB) void k_pcg64_pl(P64Args a) {
+  // the window, then one word that stays zero (read in place of a
+  // neighbour that is not a cell, so that every lane loads)
+  __shared__ double lds[2 * WIN + CPT * PB + 1];
+  __shared__ int hl[2 * WIN];
+  __shared__ unsigned raw[2 * HMAX];
+  __shared__ double lzs[CPT * PB];
+  constexpr int ZS = 2 * WIN + CPT * PB;
+  const int b0 = 1 + (int)blockIdx.x * CPT * PB;
+  const int b1 = min(b0 + CPT * PB, a.nwet + 1);
+  const bool fast = a.rowmajor && (a.st->symok == 0);
+  const int lc0 = a.lc0;
+
+  double ccc[CPT], cs[CPT], cn[CPT], cw[CPT], ce[CPT], cci[CPT];
+  double x[CPT], r[CPT], w[CPT], s[CPT], p[CPT];
+  int nb[CPT][4];
+  unsigned expo = 0u;        // bit k: another workgroup reads cell k
+  auto Z = [&](int k) -> double & { return lzs[k * PB + threadIdx.x]; };
+
+  auto inside = [&](int j) { return j >= b0 && j < b1; };
+  auto real = [&](int j) { return j > 0 && j != lc0; };
+  auto slot = [&](int j) { return j - b0 + WIN; };
+  auto w64 = [&](int i, unsigned tag) {
+    return a.xc.hx + (size_t)(tag & 1u) * 2 * 2 * (size_t)a.xc.ncell +
+           2 * (size_t)opaque(i);
+  };
+  // y = A v on the own rows from the window (own slots and outside cells);
+  // the lanes past b1 have ccc = 1, zero couplings and v = 0
+  auto spmv = [&](const double (&v)[CPT], double (&y)[CPT]) {
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      double pq[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = nb[k][q];
+        pq[q] = lds[real(j) ? slot(j) : ZS];
+      }
+      // APCG = CCC*PCG + CCS*PS + CCN*PN + CCW*PW + CCE*PE (:113)
+      y[k] = ccc[k] * v[k] + cs[k] * pq[NS] + cn[k] * pq[NN] + cw[k] * pq[NW] +
+             ce[k] * pq[NE];
+    }
+  };
+
+  for (int g = threadIdx.x; g < 2 * WIN; g += PB) hl[g] = 0;
+  if (threadIdx.x == 0) lds[ZS] = 0.0;
+  __syncthreads();
+  double fp2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int i = b0 + k * PB + (int)threadIdx.x;
+    x[k] = r[k] = w[k] = s[k] = p[k] = 0.0;
+    Z(k) = 0.0;
+    ccc[k] = 1.0; cs[k] = cn[k] = cw[k] = ce[k] = cci[k] = 0.0;
+    nb[k][NS] = nb[k][NN] = nb[k][NW] = nb[k][NE] = lc0;
+    if (i < b1) {
+      // the row as in the multi-kernel solve (row_of)
+      const int js = a.ls[i], jn = a.ln[i];
+      int jw, je;
+      double s_ = a.ccs[i], w_ = a.ccw[i], n_, e_;
+      if (fast) {
+        jw = i - 1;
+        je = i + 1;
+        n_ = (jn != lc0) ? a.ccs[jn] : 0.0;
+        e_ = (je != lc0) ? a.ccw[je] : 0.0;
+      } else {
+        jw = a.lw[i];
+        je = a.le[i];
+        n_ = (jn != lc0) ? a.ccn[i] : 0.0;
+        e_ = (je != lc0) ? a.cce[i] : 0.0;
+        if (jw == lc0) w_ = 0.0;
+      }
+      if (js == lc0) s_ = 0.0;
+      nb[k][NS] = js; nb[k][NN] = jn; nb[k][NW] = jw; nb[k][NE] = je;
+      ccc[k] = a.ccc[i]; cs[k] = s_; cn[k] = n_; cw[k] = w_; ce[k] = e_;
+      cci[k] = a.ccci[i];
+      bool out = false;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = nb[k][q];
+        if (real(j) && !inside(j)) {
+          out = true;
+          hl[ghost_of(j, b0, b1)] = 1;
+        }
+      }
+      if (out || !a.mutual) expo |= 1u << k;
+      const double *P = a.x;
+      x[k] = P[i];
+      // RCG = FPTMP - CCC*P - CCN*PN - CCS*PS - CCW*PW - CCE*PE (:73)
+      r[k] = a.fptmp[i] - ccc[k] * x[k] - n_ * P[jn] - s_ * P[js] - w_ * P[jw] - e_ * P[je];
+      fp2 += a.fptmp[i] * a.fptmp[i];
+    }
+  }
+  __syncthreads();
+  const int nhalo = compact_halo(hl, b0, b1);
+  const int span = CPT * PB;
+  const int nlo = nhalo > 0 ? (hl[0] - 1) / span : 1;
+  const int nhi = nhalo > 0 ? (hl[nhalo - 1] - 1) / span : 0;
+
+  // set-up exchange: u = CCCI r of the outside-read cells and |FPTMP|^2;
+  // then w = A u (the owners publish u, so no CCCI of an outside cell is
+  // loaded)
+  unsigned tag = a.tag0;
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int i = b0 + k * PB + (int)threadIdx.x;
+    const double uk = cci[k] * r[k];
+    lds[slot(i)] = uk;
+    if (expo & (1u << k)) put_f64(w64(i, tag), uk);
+  }
+  {
+    double v1[1] = {fp2};
+    post<1>(v1, a.xc, tag);
+  }
+  double s1[1];
+  if (!exchange<1, 0>(a.xc, tag, hl, 0, raw, a.bar, s1)) return;
+  const double bnorm2 = s1[0];
+  const double rsqm = a.rel_tol > 0.0 ? a.rel_tol * a.rel_tol * bnorm2 : a.rsqm;
+  fetch_halo64(a.xc, tag, hl, nhalo, raw);
+  for (int h = threadIdx.x; h < nhalo; h += PB)
+    lds[slot(hl[h])] = __longlong_as_double((long long)(((u64)raw[HMAX + h] << 32) | raw[h]));
+  __syncthreads();
+  {
+    double u0[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) u0[k] = cci[k] * r[k];
+    spmv(u0, w);
+  }
+
+  double gam_prev = 0.0, alpha_prev = 1.0, gam = 0.0, alpha = 0.0, beta = 0.0, rsq = 0.0;
+  int iter = 0, status = 0;
+  for (;;) {
+    ++tag;
+    // the sums of this iteration go out first (the lanes past b1 add zeros) ...
+    double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const double uk = cci[k] * r[k];
+      v[0] += r[k] * uk;
+      v[1] += w[k] * uk;
+      v[2] += r[k] * r[k];
+    }
+    post_partials<3>(v, a.xc, tag);
+    // ... then m = CCCI w on the window (after every read of the last SpMV),
+    // m of the outside-read cells, the flag
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const int i = b0 + k * PB + (int)threadIdx.x;
+      const double mk = cci[k] * w[k];
+      lds[slot(i)] = mk;
+      if (expo & (1u << k)) put_f64(w64(i, tag), mk);
+    }
+    post_flag(a.xc, tag);
+    // n = A m once the neighbouring workgroups' m has arrived
+    if (nhalo > 0) {
+      if (!halo_poll(a.xc, tag, nlo, nhi, a.bar)) return;
+      fetch_halo64(a.xc, tag, hl, nhalo, raw);
+      for (int h = threadIdx.x; h < nhalo; h += PB)
+        lds[slot(hl[h])] =
+            __longlong_as_double((long long)(((u64)raw[HMAX + h] << 32) | raw[h]));
+    }
+    __syncthreads();
+    double m[CPT], n[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) m[k] = cci[k] * w[k];
+    spmv(m, n);
+    // the sums of every workgroup
+    double s3[3];
+    if (!exchange<3, 0>(a.xc, tag, hl, 0, raw, a.bar, s3)) return;
+    gam = s3[0];
+    const double del = s3[1];
+    rsq = s3[2];
+    if (rsq <= rsqm || gam == 0.0) break;  // :165 (and RPCG = 0, :88)
+    if (iter >= a.iterm) { status = 1; break; }
+    if (iter == 0) {
+      beta = 0.0;
+      alpha = gam / del;
+    } else {
+      beta = gam / gam_prev;
+      alpha = gam / (del - beta * gam / alpha_prev);
+    }
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const double zk = n[k] + beta * Z(k);
+      s[k] = w[k] + beta * s[k];
+      p[k] = cci[k] * r[k] + beta * p[k];
+      x[k] = x[k] + alpha * p[k];
+      r[k] = r[k] - alpha * s[k];
+      w[k] = w[k] - alpha * zk;
+      Z(k) = zk;
+    }
+    iter += 1;
+    gam_prev = gam;
+    alpha_prev = alpha;
+  }
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int i = b0 + k * PB + (int)threadIdx.x;
+    if (i < b1) a.x[opaque(i)] = x[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CgState *st = a.st;
+    st->rpcg = gam;
+    st->papcg = 0.0;
+    st->alpha = alpha;
+    st->beta = beta;
+    st->rsq = rsq;
+    st->rsqm = rsqm;
+    st->bnorm2 = bnorm2;
+    st->iter = iter;
+    st->done = 1;
+    st->status = status;
+  }
+}
+
 // Jacobi scaling of the mixed solve from a scaled system (CCC, CCS, CCW)
 // given directly: dis = 1/sqrt(CCC), A'(i,S), A'(i,W) in FP32.
 __global__ __launch_bounds__(256) void k_mx_coef(const double *ccc, const double *ccs,
@@ -1290,7 +1553,10 @@ int pcg_prepare(efdc_ctx *c) {
       0, occupancy(k_pcgmx_pl<1>), occupancy(k_pcgmx_pl<2>), occupancy(k_pcgmx_pl<3>),
       occupancy(k_pcgmx_pl<4>), occupancy(k_pcgmx_pl<5>), occupancy(k_pcgmx_pl<6>),
       occupancy(k_pcgmx_pl<7>), occupancy(k_pcgmx_pl<8>)};
-  for (int k = 0; k <= PCG64_CPT_MAX; ++k) c->pcg_occ64[k] = occ64[k];
+  const int occ64p[PCG64_CPT_MAX + 1] = {
+      0, occupancy(k_pcg64_pl<1>), occupancy(k_pcg64_pl<2>), occupancy(k_pcg64_pl<3>),
+      occupancy(k_pcg64_pl<4>), occupancy(k_pcg64_pl<5>), occupancy(k_pcg64_pl<6>)};
+  for (int k = 0; k <= PCG64_CPT_MAX; ++k) c->pcg_occ64[k] = std::min(occ64[k], occ64p[k]);
   for (int k = 0; k <= PCGMX_CPT_MAX; ++k) c->pcg_occmx[k] = std::min(occmx[k], occpl[k]);
   c->pcg_ncell = c->n_wet() + 2;
   int s;
@@ -1411,13 +1677,24 @@ int pcg_solve(efdc_ctx *c, bool mixed, bool defer) {
     a.rel_tol = c->prm.cg_rel_tol;
     a.iterm = c->prm.ITERM;
     const void *fn = nullptr;
-    switch (cpt) {
-      case 1: fn = (const void *)k_pcg64<1>; break;
-      case 2: fn = (const void *)k_pcg64<2>; break;
-      case 3: fn = (const void *)k_pcg64<3>; break;
-      case 4: fn = (const void *)k_pcg64<4>; break;
-      case 5: fn = (const void *)k_pcg64<5>; break;
-      default: fn = (const void *)k_pcg64<6>; break;
+    if (c->pcg64_pipe) {
+      switch (cpt) {
+        case 1: fn = (const void *)k_pcg64_pl<1>; break;
+        case 2: fn = (const void *)k_pcg64_pl<2>; break;
+        case 3: fn = (const void *)k_pcg64_pl<3>; break;
+        case 4: fn = (const void *)k_pcg64_pl<4>; break;
+        case 5: fn = (const void *)k_pcg64_pl<5>; break;
+        default: fn = (const void *)k_pcg64_pl<6>; break;
+      }
+    } else {
+      switch (cpt) {
+        case 1: fn = (const void *)k_pcg64<1>; break;
+        case 2: fn = (const void *)k_pcg64<2>; break;
+        case 3: fn = (const void *)k_pcg64<3>; break;
+        case 4: fn = (const void *)k_pcg64<4>; break;
+        case 5: fn = (const void *)k_pcg64<5>; break;
+        default: fn = (const void *)k_pcg64<6>; break;
+      }
     }
     TIMED(c, TK_PCG64, s = coop_launch(c, fn, nb, a));
     if (s) return s;

@@ -936,6 +936,18 @@ int efdc_set_cg_pipeline(efdc_ctx_t *c, int32_t mode) {
   return EFDC_OK;
 }
 
+int efdc_set_cg64_pipeline(efdc_ctx_t *c, int32_t on) {
+  if (!c || on < 0 || on > 1) return EFDC_ERR_ARG;
+  c->pcg64_pipe = on != 0;
+  return EFDC_OK;
+}
+
+int efdc_get_cg64_pipeline(efdc_ctx_t *c, int32_t *on) {
+  if (!c || !on) return EFDC_ERR_ARG;
+  *on = c->pcg64_pipe ? 1 : 0;
+  return EFDC_OK;
+}
+
 int efdc_set_cg_launch(efdc_ctx_t *c, int32_t mode, uint32_t spin_limit) {
   if (!c || mode < 0 || mode > 1) return EFDC_ERR_ARG;
   c->pcg_launch_mode = mode;

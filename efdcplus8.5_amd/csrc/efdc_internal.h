@@ -166,6 +166,7 @@ struct efdc_ctx {
   // its own clearing
   bool pcg_reset_done = false;
   int pcg_launch_mode = 0;      // 0 plain launch with a cooperative retry, 1 cooperative
+  bool pcg64_pipe = false;      // the FP64 persistent solve in the pipelined form
   bool pcg_force_coop = false;  // the retry (and drying steps) launch cooperatively
   bool pcg_last_plain = false;  // the last persistent launch was a plain one
   unsigned pcg_spin_test = 0;   // > 0: spin limit of the plain launches (tests)

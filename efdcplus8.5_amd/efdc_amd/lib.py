@@ -196,6 +196,8 @@ def load(path: Optional[str] = None):
         "efdc_set_cg_pipeline": ([P, C.c_int32], C.c_int),
         "efdc_get_cg_pipeline": ([P, C.POINTER(C.c_int32)], C.c_int),
         "efdc_set_cg_launch": ([P, C.c_int32, C.c_uint32], C.c_int),
+        "efdc_set_cg64_pipeline": ([P, C.c_int32], C.c_int),
+        "efdc_get_cg64_pipeline": ([P, C.POINTER(C.c_int32)], C.c_int),
         "efdc_get_cg_launch": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
         "efdc_get_segment_tile": ([P, C.POINTER(C.c_int32)], C.c_int),
         "efdc_restart_text": ([P, C.POINTER(_fmt.Restart), C.c_char_p, C.c_int64,
@@ -633,6 +635,16 @@ class Context:
         if it was not co-resident), 1 cooperative; spin_limit > 0 shortens the
         plain launches' waits (tests of the repetition)."""
         self._check(self.lib.efdc_set_cg_launch(self.h, mode, spin_limit))
+
+    def set_cg64_pipeline(self, on: bool):
+        """FP64 persistent solve in the pipelined form (one exchange per
+        iteration) instead of the reference's recurrences."""
+        self._check(self.lib.efdc_set_cg64_pipeline(self.h, 1 if on else 0))
+
+    def cg64_pipeline(self) -> bool:
+        v = C.c_int32(0)
+        self._check(self.lib.efdc_get_cg64_pipeline(self.h, C.byref(v)))
+        return bool(v.value)
 
     def cg_launch(self) -> tuple:
         """(mode, solves repeated cooperatively so far)."""

@@ -348,6 +348,18 @@ module efdc_hip
       integer(c_int32_t), intent(out) :: mode
     end function
 
+    integer(c_int) function efdc_set_cg64_pipeline(ctx, on) bind(C, name="efdc_set_cg64_pipeline")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function
+
+    integer(c_int) function efdc_get_cg64_pipeline(ctx, on) bind(C, name="efdc_get_cg64_pipeline")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: on
+    end function
+
     integer(c_int) function efdc_set_cg_launch(ctx, mode, spin_limit) bind(C, name="efdc_set_cg_launch")
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx

@@ -610,6 +610,16 @@ int efdc_get_cg_pipeline(efdc_ctx_t *ctx, int32_t *mode);
    the repetition); 0 restores the full limit.  get returns the mode and
    the number of solves repeated so far. */
 int efdc_set_cg_launch(efdc_ctx_t *ctx, int32_t mode, uint32_t spin_limit);
+/* Form of the FP64 persistent solve (EFDC_CG_FP64 on one co-resident grid):
+   0 (default) the reference's recurrences (EFDC/congrad.f90:91-169), two
+   exchanges between workgroups per iteration; 1 the same Jacobi-PCG in the
+   pipelined order of Ghysels and Vanroose, one exchange per iteration whose
+   wait comes one SpMV later, and one halo value from the neighbouring
+   workgroups only.  Same iterates in exact arithmetic and the same stop
+   test ((r,r) <= RSQM); the rounding differs, so form 1 is not the
+   reference's bits.  No counterpart in the reference. */
+int efdc_set_cg64_pipeline(efdc_ctx_t *ctx, int32_t on);
+int efdc_get_cg64_pipeline(efdc_ctx_t *ctx, int32_t *on);
 int efdc_get_cg_launch(efdc_ctx_t *ctx, int32_t *mode, int32_t *retries);
 
 /* Per-basin results of the last efdc_calpuv2c / efdc_congrad with

@@ -271,3 +271,73 @@ def test_step_repeats_the_solve_and_the_passes_after_it(oracle_mod):
             assert np.max(np.abs(a - b)) <= 1e-9 * max(1.0, np.max(np.abs(b))), name
     assert ctx.cg_launch()[1] >= 1
     ctx.close()
+
+
+def _solve64(grid, system, pipelined, **kw):
+    from efdc_amd.lib import Context
+    from efdc_amd import lib
+    kw.setdefault("RSQM", TIGHT)
+    kw.setdefault("ITERM", 4000)
+    ctx = Context(grid, cg_kernel=lib.CG_KERNEL_PERSISTENT, **kw)
+    ctx.set_cg64_pipeline(pipelined)
+    assert ctx.cg64_pipeline() == pipelined
+    for k, v in system.items():
+        ctx.set(k, v)
+    out = ctx.congrad()
+    p = ctx.get("P")
+    ctx.close()
+    return out, p
+
+
+@pytest.mark.parametrize("n,seed", [(12, 2), (24, 3), (64, 5), (200, 7)])
+def test_fp64_pipelined_form_matches_oracle(oracle_mod, n, seed):
+    """The FP64 persistent solve in the pipelined order (one exchange per
+    iteration) against the oracle, |d eta| <= 1e-9 m, and its iteration
+    count within one of the reference's recurrences (same iterates in exact
+    arithmetic, same stop test)."""
+    case, system, p_ref = _lake_system(oracle_mod, n, seed)
+    o_ref, p_rec = _solve64(case.grid, system, False)
+    o_pl, p_pl = _solve64(case.grid, system, True)
+    i = wet_index(case)
+    d = eta_err(p_pl[i], p_ref[i])
+    print(f"n={n} wet={case.grid.nwet}: iters {o_ref['iters']} (recurrences) "
+          f"{o_pl['iters']} (pipelined), deta {d:.3e}, "
+          f"vs recurrences {eta_err(p_pl[i], p_rec[i]):.3e}")
+    assert abs(o_pl["iters"] - o_ref["iters"]) <= 1
+    assert d <= 1e-9
+
+
+def test_fp64_pipelined_form_general_maps(oracle_mod):
+    """The same on the column-major renumbering (the general four-map
+    operator, maps that are not the row-major pattern)."""
+    case, system, p_ref = _lake_system(oracle_mod, 96, 4)
+    g2, sys2, src = _column_major(case, system)
+    out, p = _solve64(g2, sys2, True)
+    newi = np.arange(2, g2.LA + 1) - 1
+    d = eta_err(p[newi], p_ref[src - 1])
+    print("column-major, pipelined FP64: iters", out["iters"], "deta", d)
+    assert d <= 1e-9
+
+
+def test_fp64_pipelined_form_in_steps(oracle_mod):
+    """CALPUV2C steps of a tidal basin (a clamped open boundary: rows that
+    are not symmetric) with the pipelined FP64 solve: P, HP and the face
+    transports against the oracle at the parity tests' bars."""
+    from efdc_amd import grids, lib
+    case = grids.tidal_basin(160, 80, 1, seed=12)
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref, rsqm=TIGHT, iterm=2000)
+    ctx = lib.Context(case.grid, RSQM=TIGHT, ITERM=2000,
+                      cg_kernel=lib.CG_KERNEL_PERSISTENT).upload_case(case)
+    ctx.set_cg64_pipeline(True)
+    i = wet_index(case)
+    for n in (1, 2, 3):
+        o.step(n)
+        st = ctx.step(case.dt, n)
+        d = eta_err(ctx.get("P")[i], ref.h["P"][i])
+        print(f"step {n}: deta {d:.3e}")
+        assert d <= 1e-9
+        for name in ("HP", "UHDYE", "VHDXE", "HU", "HV"):
+            a, b = ctx.get(name)[i], ref.h[name][i]
+            assert np.max(np.abs(a - b)) <= 1e-9 * max(1.0, np.max(np.abs(b))), name
+    ctx.close()
