@@ -465,10 +465,13 @@ __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
   const int nhalo = compact_halo(hl, b0, b1);
 
   unsigned tag = a.tag0;
-  // exchange: r and p (= 0) of the outside cells, RPCG and |FPTMP|^2
+  // exchange: z = CCCI*r and p (= 0) of the outside cells, RPCG and
+  // |FPTMP|^2 (the owner forms z, the product its neighbours would form, so
+  // they load no CCCI of an outside cell)
 #pragma unroll
   for (int k = 0; k < CPT; ++k)
-    if (expo & (1u << k)) publish_rp(b0 + k * PB + (int)threadIdx.x, tag, r[k], 0.0);
+    if (expo & (1u << k))
+      publish_rp(b0 + k * PB + (int)threadIdx.x, tag, cci[k] * r[k], 0.0);
   post<2>(v2, a.xc, tag);
   double s2[2];
   if (!exchange<2, 4>(a.xc, tag, hl, nhalo, raw, a.bar, s2)) return;
@@ -489,14 +492,14 @@ __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
       p[k] = cci[k] * r[k] + beta * p[k];
       lds[slot(b0 + k * PB + (int)threadIdx.x)] = p[k];
     }
-    // ... and on the outside cells, from their published r and p
+    // ... and on the outside cells, from their published z = CCCI*r and p
     for (int h = threadIdx.x; h < nhalo; h += PB) {
       const int j = hl[h];
-      const double rj = __longlong_as_double(
+      const double zj = __longlong_as_double(
           (long long)(((u64)raw[HMAX + h] << 32) | raw[h]));
       const double pj = __longlong_as_double(
           (long long)(((u64)raw[3 * HMAX + h] << 32) | raw[2 * HMAX + h]));
-      lds[slot(j)] = a.ccci[j] * rj + beta * pj;
+      lds[slot(j)] = zj + beta * pj;
     }
     __syncthreads();
     double v1[1] = {0.0};
@@ -530,7 +533,7 @@ __global__ __launch_bounds__(PB) void k_pcg64(P64Args a) {
       const double z = cci[k] * rr;      // :134
       v[0] += rr * z;
       v[1] += rr * rr;
-      if (expo & (1u << k)) publish_rp(i, tag, rr, p[k]);
+      if (expo & (1u << k)) publish_rp(i, tag, z, p[k]);
     }
     post<2>(v, a.xc, tag);
     if (!exchange<2, 4>(a.xc, tag, hl, nhalo, raw, a.bar, s2)) return;
