@@ -34,6 +34,53 @@ def _stratified(case):
     return case
 
 
+@pytest.mark.parametrize("cg_kernel", [0, 1])
+def test_open_boundary_drying_only_isdry98(oracle_mod, cg_kernel):
+    """ISDRY = 98: only the open-boundary cells dry (EFDC/calpuv2c.f90:
+    497-530); the cell pass and its re-solve are skipped, so NCORDRY stays
+    0.  Flags, eta, HP and the face masks against the oracle every step of
+    a basin whose open boundary dries."""
+    from efdc_amd import grids
+    from efdc_amd.lib import Context
+    # a basin 3 m deep everywhere but a 0.6 m sill on the open-boundary
+    # column: the tide (0.8 m) dries the sill, no other cell comes near HDRY
+    case = grids.drying_basin(kc=3, deep=3.0, high=-3.0, shoal=0.0)
+    g, h = case.grid, case.h
+    lb = case.open_bc["side"][0]["L"] - 1
+    h["BELV"][lb] = -0.6
+    for key in ("HP", "H1P", "H2P"):
+        h[key][lb] = 0.6
+    h["P"][lb] = grids.G * (h["HP"][lb] + h["BELV"][lb])
+    h["P1"][lb] = h["P"][lb]
+    grids.update_depth_arrays(g, h)
+    grids.layer_depths(g, h, case.k)
+    case.k["H1PK"][:] = case.k["HPK"]
+    if cg_kernel == 1:
+        case = _stratified(case)
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref, rsqm=1e-24, iterm=2000, isdry=98, hdry=HDRY)
+    ctx = Context(case.grid, RSQM=1e-24, ITERM=2000, ISDRY=98, HDRY=HDRY,
+                  cg_kernel=cg_kernel).upload_case(case)
+    L = wet_index(case)
+    obc_dried = False
+    for n in range(1, 301):
+        so = o.step(n)
+        sg = ctx.step(case.dt, n)
+        assert so["status"] == 0 and sg["status"] == 0, n
+        m, d, k = ctx.get_dry_state()
+        assert k == o.o.ncordry == 0, (n, k, o.o.ncordry)
+        assert np.array_equal(m[L], o.lmaskdry[L]), n
+        assert np.array_equal(d[L], o.iscdry[L]), n
+        obc_dried |= bool(np.any(d[lb] == 1))
+        if n % 50 == 0 or n < 4:
+            assert eta_err(ctx.get("P")[L], ref.h["P"][L]) <= 1e-9, n
+            for key in ("SUB", "SVB"):
+                assert np.array_equal(ctx.get(key)[L], ref.h[key][L]), (n, key)
+            assert np.max(np.abs(ctx.get("HP")[L] - ref.h["HP"][L])) <= 1e-9, n
+    assert obc_dried
+    ctx.close()
+
+
 @pytest.mark.parametrize("isdry", [1, 99])
 @pytest.mark.parametrize("cg_kernel", [0, 1])
 @pytest.mark.parametrize("deep", [2.0, 0.6])
