@@ -753,6 +753,19 @@ int efdc_get_dry_state(efdc_ctx_t *c, int32_t *lmaskdry, int32_t *iscdry, int32_
   return EFDC_OK;
 }
 
+int efdc_set_dry_counters(efdc_ctx_t *c, const double *vdwaste, const int32_t *natdry) {
+  if (!c) return EFDC_ERR_ARG;
+  if (vdwaste)
+    HIP_TRY(c, hipMemcpyAsync(c->vdwaste, vdwaste, sizeof(double) * c->LCM,
+                              hipMemcpyHostToDevice, c->stream));
+  if (natdry)
+    HIP_TRY(c, hipMemcpyAsync(c->natdry, natdry, sizeof(int) * c->LCM, hipMemcpyHostToDevice,
+                              c->stream));
+  // the host arrays may be reused as soon as the call returns
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return EFDC_OK;
+}
+
 int efdc_get_dry_waste(efdc_ctx_t *c, double *qdwaste, double *vdwaste, int32_t *natdry) {
   if (!c) return EFDC_ERR_ARG;
   if (qdwaste)

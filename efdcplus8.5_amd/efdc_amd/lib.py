@@ -196,6 +196,7 @@ def load(path: Optional[str] = None):
         "efdc_set_cg_pipeline": ([P, C.c_int32], C.c_int),
         "efdc_get_cg_pipeline": ([P, C.POINTER(C.c_int32)], C.c_int),
         "efdc_set_cg_launch": ([P, C.c_int32, C.c_uint32], C.c_int),
+        "efdc_set_dry_counters": ([P, PD, PI], C.c_int),
         "efdc_set_cg64_pipeline": ([P, C.c_int32], C.c_int),
         "efdc_get_cg64_pipeline": ([P, C.POINTER(C.c_int32)], C.c_int),
         "efdc_get_cg_launch": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
@@ -444,6 +445,17 @@ class Context:
         q, v, k = np.zeros(n), np.zeros(n), np.zeros(n, np.int32)
         self._check(self.lib.efdc_get_dry_waste(self.h, _pd(q), _pd(v), _pi(k)))
         return q, v, k
+
+    def set_dry_counters(self, vdwaste=None, natdry=None):
+        """Drained volume (m^3) and isolated-step counters per cell in L
+        order (index L-1), e.g. from a restart; either may be None."""
+        v = None if vdwaste is None else np.ascontiguousarray(vdwaste, np.float64)
+        k = None if natdry is None else np.ascontiguousarray(natdry, np.int32)
+        for a in (v, k):
+            if a is not None and a.shape != (self.grid.LCM,):
+                raise ValueError("set_dry_counters: arrays of LCM entries")
+        self._check(self.lib.efdc_set_dry_counters(
+            self.h, None if v is None else _pd(v), None if k is None else _pi(k)))
 
     # ------------------------------------------------------------ calls
     def congrad(self) -> dict:

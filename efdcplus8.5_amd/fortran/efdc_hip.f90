@@ -266,6 +266,14 @@ module efdc_hip
       integer(c_int32_t), intent(out) :: ncordry
     end function
 
+    integer(c_int) function efdc_set_dry_counters(ctx, vdwaste, natdry) &
+        bind(C, name="efdc_set_dry_counters")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: vdwaste(*)
+      integer(c_int32_t), intent(in) :: natdry(*)
+    end function
+
     integer(c_int) function efdc_get_dry_waste(ctx, qdwaste, vdwaste, natdry) &
         bind(C, name="efdc_get_dry_waste")
       import :: c_int, c_ptr, c_int32_t, c_double

@@ -130,6 +130,14 @@ subroutine EFDC_HIP_INIT
     HXERR = efdc_set_array(HCTX, EFDC_SUBO, SUBO); call EFDC_HIP_CHECK(HXERR, 'SUBO')
     HXERR = efdc_set_array(HCTX, EFDC_SVBO, SVBO); call EFDC_HIP_CHECK(HXERR, 'SVBO')
   end if
+  if (ISDRY > 0 .and. NDRYSTP < 0) then
+    ! drained volume and isolation counters continue from the host's
+    ! (a restart's) values (calpuv2c.f90:838-893)
+    DRYI = 0
+    DRYI(2:LA) = NATDRY(2:LA)
+    HXERR = efdc_set_dry_counters(HCTX, VDWASTE, DRYI)
+    call EFDC_HIP_CHECK(HXERR, 'efdc_set_dry_counters')
+  end if
   HXERR = efdc_set_array(HCTX, EFDC_BELV, BELV);   call EFDC_HIP_CHECK(HXERR, 'BELV')
   HXERR = efdc_set_array(HCTX, EFDC_DZC, DZC);     call EFDC_HIP_CHECK(HXERR, 'DZC')
   HXERR = efdc_set_array(HCTX, EFDC_CDZKK, CDZKK); call EFDC_HIP_CHECK(HXERR, 'CDZKK')

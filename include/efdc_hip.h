@@ -358,6 +358,11 @@ int efdc_get_dry_state(efdc_ctx_t *ctx, int32_t *lmaskdry, int32_t *iscdry, int3
    since the context was created (m^3), and NATDRY, the steps each wet cell
    has been isolated (EFDC/calpuv2c.f90:838-893). */
 int efdc_get_dry_waste(efdc_ctx_t *ctx, double *qdwaste, double *vdwaste, int32_t *natdry);
+/* Sets VDWASTE and NATDRY (L order, LCM entries; either may be NULL), e.g.
+   from a restart, so that the drained volume and the isolation counters
+   (EFDC/calpuv2c.f90:838-893) continue from the host's values instead of
+   starting at 0 when the context is created. */
+int efdc_set_dry_counters(efdc_ctx_t *ctx, const double *vdwaste, const int32_t *natdry);
 
 /* CONGRAD (EFDC/congrad.f90:1-191) on the scaled system already held in
    CCC, CCS, CCW, CCE, CCN, CCCI, FPTMP, starting from and updating P. */

@@ -270,6 +270,35 @@ def test_draining_matches_oracle(oracle_mod, isdry):
     ctx.close()
 
 
+def test_drain_counters_continue_after_restart(oracle_mod):
+    """A context created mid-run (a restart after step 3) and given the
+    oracle's VDWASTE and NATDRY continues like the oracle: the shallow pit,
+    isolated since step 1, drains on step 6 as in the uninterrupted run
+    (with its counter back at 0 it would drain on step 9)."""
+    import test_oracle_drying as t
+    from efdc_amd.lib import Context
+    case, (shallow, deep) = t._puddles()
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref, rsqm=1e-26, iterm=4000, isdry=1, hdry=HDRY, ndrystp=-5)
+    for n in (1, 2, 3):
+        o.step(n)
+    assert o.natdry[shallow] == 3 and o.vdwaste[shallow] == 0.0
+    ctx = Context(case.grid, RSQM=1e-26, ITERM=4000, ISDRY=1, HDRY=HDRY,
+                  NDRYSTP=-5).upload_case(ref.copy())
+    ctx.set_dry_counters(o.vdwaste.copy(), o.natdry.copy())
+    L = wet_index(case)
+    for n in range(4, 11):
+        o.step(n)
+        ctx.step(case.dt, n)
+        q, v, k = ctx.get_dry_waste()
+        assert np.array_equal(k[L], o.natdry[L]), n
+        assert np.max(np.abs(v[L] - o.vdwaste[L])) <= 1e-9 * max(1.0, o.vdwaste.max()), n
+        assert np.max(np.abs(ctx.get("HP")[L] - ref.h["HP"][L])) <= 1e-9, n
+        if n == 6:
+            assert v[shallow] > 0
+    ctx.close()
+
+
 @pytest.mark.parametrize("isdry", [1, 99])
 def test_antidiffusion_with_drying_matches_oracle(oracle_mod, isdry):
     """ISADAC = 1 with FCT on the drying basin: the FCT limiter sees the
