@@ -815,15 +815,18 @@ __device__ __forceinline__ void mx_resid_cell(const MxArgs &m, int i, double (&v
   v[1] += (double)rs * (double)rs;
 }
 
+// (loaded whatever j, a valid cell lc0 included, and selected, so that the
+// loads of a thread's cells are in flight together)
 __device__ __forceinline__ float p32_at(const MxArgs &m, const float *pold,
                                         double beta, int j) {
-  return (j > 0 && j != m.g.lc0)
-             ? (float)((double)m.r32[j] + beta * (double)pold[j])
-             : 0.0f;
+  const float pj = (float)((double)m.r32[j] + beta * (double)pold[j]);
+  return (j > 0 && j != m.g.lc0) ? pj : 0.0f;
 }
 
-__device__ __forceinline__ void mx_a_cell(const MxArgs &m, int i, double beta, const float *pold,
-                                          float *pnew, double (&v)[1]) {
+// own = false: a lane past the cells (i clamped to a wet cell by the
+// caller) computes the row and keeps nothing
+__device__ __forceinline__ void mx_a_cell(const MxArgs &m, int i, bool own, double beta,
+                                          const float *pold, float *pnew, double (&v)[1]) {
   const CgArgs &a = m.g;
   const int js = a.ls[i], jn = a.ln[i];
   const float pc = p32_at(m, pold, beta, i);
@@ -831,20 +834,26 @@ __device__ __forceinline__ void mx_a_cell(const MxArgs &m, int i, double beta, c
   const float pn = p32_at(m, pold, beta, jn);
   const float pw = p32_at(m, pold, beta, i - 1);
   const float pe = p32_at(m, pold, beta, i + 1);
-  const float an = (jn != a.lc0) ? m.as[jn] : 0.0f;
-  const float ae = (i + 1 != a.lc0) ? m.aw[i + 1] : 0.0f;
+  const float as_n = m.as[jn], aw_e = m.aw[i + 1];
+  const float an = (jn != a.lc0) ? as_n : 0.0f;
+  const float ae = (i + 1 != a.lc0) ? aw_e : 0.0f;
   const float apc = pc + m.as[i] * ps + an * pn + m.aw[i] * pw + ae * pe;
-  pnew[i] = pc;
-  m.ap32[i] = apc;
-  v[0] += (double)apc * (double)pc;
+  if (own) {
+    pnew[i] = pc;
+    m.ap32[i] = apc;
+  }
+  v[0] += own ? (double)apc * (double)pc : 0.0;
 }
 
-__device__ __forceinline__ void mx_b_cell(const MxArgs &m, int i, double alpha, const float *pnew,
-                                          double (&v)[1]) {
-  m.y32[i] = (float)((double)m.y32[i] + alpha * (double)pnew[i]);
+__device__ __forceinline__ void mx_b_cell(const MxArgs &m, int i, bool own, double alpha,
+                                          const float *pnew, double (&v)[1]) {
+  const float y = (float)((double)m.y32[i] + alpha * (double)pnew[i]);
   const float r = (float)((double)m.r32[i] - alpha * (double)m.ap32[i]);
-  m.r32[i] = r;
-  v[0] += (double)r * (double)r;
+  if (own) {
+    m.y32[i] = y;
+    m.r32[i] = r;
+  }
+  v[0] += own ? (double)r * (double)r : 0.0;
 }
 
 __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_prep(MxArgs m, double rsqm,
@@ -902,7 +911,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_a(MxArgs m, const float *pold
   #pragma unroll
   for (int u = 0; u < MX_CPT; ++u) {
     const int i = (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
-    if (i >= 1 && i <= a.nwet) mx_a_cell(m, i, beta, pold, pnew, v);
+    const bool own = i >= 1 && i <= a.nwet;
+    mx_a_cell(m, own ? i : (i < 1 ? 1 : a.nwet), own, beta, pold, pnew, v);
   }
   double s[1];
   if (grid_sum_x<1>(v, a.part, a.tk + TS_MX_A * TICKET_SITE, gridDim.x, tile, s) &&
@@ -920,7 +930,8 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_b(MxArgs m, const float *pnew
   #pragma unroll
   for (int u = 0; u < MX_CPT; ++u) {
     const int i = (tile * MX_CPT + u) * EFDC_BLOCK + (int)threadIdx.x;
-    if (i >= 1 && i <= a.nwet) mx_b_cell(m, i, alpha, pnew, v);
+    const bool own = i >= 1 && i <= a.nwet;
+    mx_b_cell(m, own ? i : (i < 1 ? 1 : a.nwet), own, alpha, pnew, v);
   }
   double s[1];
   if (grid_sum_x<1>(v, a.part, a.tk + TS_MX_B * TICKET_SITE, gridDim.x, tile, s) &&
@@ -1025,7 +1036,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_smx_a(SMxArgs x, const float *po
   if (st->done) return;
   const double beta = st->beta;
   double v[1] = {0.0};
-  tile_cells<CPT>(t, [&](int i) { mx_a_cell(x.m, i, beta, pold, pnew, v); });
+  tile_cells<CPT>(t, [&](int i) { mx_a_cell(x.m, i, true, beta, pold, pnew, v); });
   double s[1];
   if (seg_sum<1>(x.g, b, t.z, v, s) && threadIdx.x == 0) st->alpha = st->rz_in / s[0];
 }
@@ -1039,7 +1050,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_smx_b(SMxArgs x, const float *pn
   if (st->done) return;
   const double alpha = st->alpha;
   double v[1] = {0.0};
-  tile_cells<CPT>(t, [&](int i) { mx_b_cell(x.m, i, alpha, pnew, v); });
+  tile_cells<CPT>(t, [&](int i) { mx_b_cell(x.m, i, true, alpha, pnew, v); });
   double s[1];
   if (seg_sum<1>(x.g, b, t.z, v, s) && threadIdx.x == 0) {
     st->beta = s[0] / st->rz_in;
