@@ -63,8 +63,9 @@ typedef enum {
   EFDC_CG_KERNEL_AUTO = 0,       /* persistent solve when the rows fit one
                                     co-resident grid, otherwise multi-kernel */
   EFDC_CG_KERNEL_MULTI = 1,      /* two launches per iteration               */
-  EFDC_CG_KERNEL_PERSISTENT = 2  /* one cooperative launch per solve; an
-                                    error when the rows do not fit           */
+  EFDC_CG_KERNEL_PERSISTENT = 2  /* one launch per solve of a co-resident
+                                    grid (efdc_set_cg_launch); an error
+                                    when the rows do not fit                 */
 } efdc_cg_kernel;
 
 typedef enum {
