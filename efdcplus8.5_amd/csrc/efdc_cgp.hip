@@ -808,15 +808,18 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
       }
       // own rows: p = u + beta p, s = w + beta s, q = z + beta q,
       // y += alpha p, r -= alpha s, u -= alpha q
+      // the vector updates in FP32 (one rounding each, fused multiply-add),
+      // the same operations as a neighbour's recomputation below
+      const float bf = (float)beta, af = (float)alpha;
 #pragma unroll
       for (int k = 0; k < CPT; ++k) {
         const int i = b0 + k * PB + (int)threadIdx.x;
-        p[k] = (float)((double)u[k] + beta * (double)p[k]);
-        s[k] = (float)((double)w[k] + beta * (double)s[k]);
-        q[k] = (float)((double)Z(k) + beta * (double)q[k]);
-        Y(k) = (float)((double)Y(k) + alpha * (double)p[k]);
-        r[k] = (float)((double)r[k] - alpha * (double)s[k]);
-        u[k] = (float)((double)u[k] - alpha * (double)q[k]);
+        p[k] = fmaf(bf, p[k], u[k]);
+        s[k] = fmaf(bf, s[k], w[k]);
+        q[k] = fmaf(bf, q[k], Z(k));
+        Y(k) = fmaf(af, p[k], Y(k));
+        r[k] = fmaf(-af, s[k], r[k]);
+        u[k] = fmaf(-af, q[k], u[k]);
         lds[slot(i)] = u[k];
       }
       // the outside cells' u, from their published u, z, q
@@ -824,8 +827,8 @@ __global__ __launch_bounds__(PB) void k_pcgmx(PMxArgs a) {
         const float uj = __uint_as_float(raw[h]);
         const float zj = __uint_as_float(raw[HMAX + h]);
         const float qj0 = __uint_as_float(raw[2 * HMAX + h]);
-        const float qj = (float)((double)zj + beta * (double)qj0);
-        lds[slot(hl[h])] = (float)((double)uj - alpha * (double)qj);
+        const float qj = fmaf(bf, qj0, zj);
+        lds[slot(hl[h])] = fmaf(-af, qj, uj);
       }
       __syncthreads();
       ++tag;
@@ -1212,16 +1215,20 @@ __global__ __launch_bounds__(PB) void k_pcgmx_pl(PMxArgs a) {
         beta = gam / gam_prev;
         alpha = gam / (del - beta * gam / alpha_prev);
       }
+      // the vector updates in FP32 (one rounding each, fused multiply-add);
+      // the sums, alpha and beta stay FP64 and the outer FP64 residual
+      // decides convergence
+      const float bf = (float)beta, af = (float)alpha;
 #pragma unroll
       for (int k = 0; k < CPT; ++k) {
-        const float zk = (float)((double)n[k] + beta * (double)Z(k));
-        const float qk = (float)((double)m[k] + beta * (double)Q(k));
-        s[k] = (float)((double)w[k] + beta * (double)s[k]);
-        p[k] = (float)((double)u[k] + beta * (double)p[k]);
-        Y(k) = (float)((double)Y(k) + alpha * (double)p[k]);
-        r[k] = (float)((double)r[k] - alpha * (double)s[k]);
-        u[k] = (float)((double)u[k] - alpha * (double)qk);
-        w[k] = (float)((double)w[k] - alpha * (double)zk);
+        const float zk = fmaf(bf, Z(k), n[k]);
+        const float qk = fmaf(bf, Q(k), m[k]);
+        s[k] = fmaf(bf, s[k], w[k]);
+        p[k] = fmaf(bf, p[k], u[k]);
+        Y(k) = fmaf(af, p[k], Y(k));
+        r[k] = fmaf(-af, s[k], r[k]);
+        u[k] = fmaf(-af, qk, u[k]);
+        w[k] = fmaf(-af, zk, w[k]);
         Z(k) = zk;
         Q(k) = qk;
       }
