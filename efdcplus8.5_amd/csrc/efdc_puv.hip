@@ -817,15 +817,17 @@ __device__ __forceinline__ void mx_resid_cell(const MxArgs &m, int i, double (&v
 
 // (loaded whatever j, a valid cell lc0 included, and selected, so that the
 // loads of a thread's cells are in flight together)
+// (p = r + beta p in FP32, one fused multiply-add, for a cell's own p and
+// for its neighbours' alike)
 __device__ __forceinline__ float p32_at(const MxArgs &m, const float *pold,
-                                        double beta, int j) {
-  const float pj = (float)((double)m.r32[j] + beta * (double)pold[j]);
+                                        float beta, int j) {
+  const float pj = fmaf(beta, pold[j], m.r32[j]);
   return (j > 0 && j != m.g.lc0) ? pj : 0.0f;
 }
 
 // own = false: a lane past the cells (i clamped to a wet cell by the
 // caller) computes the row and keeps nothing
-__device__ __forceinline__ void mx_a_cell(const MxArgs &m, int i, bool own, double beta,
+__device__ __forceinline__ void mx_a_cell(const MxArgs &m, int i, bool own, float beta,
                                           const float *pold, float *pnew, double (&v)[1]) {
   const CgArgs &a = m.g;
   const int js = a.ls[i], jn = a.ln[i];
@@ -845,10 +847,11 @@ __device__ __forceinline__ void mx_a_cell(const MxArgs &m, int i, bool own, doub
   v[0] += own ? (double)apc * (double)pc : 0.0;
 }
 
-__device__ __forceinline__ void mx_b_cell(const MxArgs &m, int i, bool own, double alpha,
+__device__ __forceinline__ void mx_b_cell(const MxArgs &m, int i, bool own, float alpha,
                                           const float *pnew, double (&v)[1]) {
-  const float y = (float)((double)m.y32[i] + alpha * (double)pnew[i]);
-  const float r = (float)((double)m.r32[i] - alpha * (double)m.ap32[i]);
+  // the vector updates in FP32 (fused multiply-add); the sums stay FP64
+  const float y = fmaf(alpha, pnew[i], m.y32[i]);
+  const float r = fmaf(-alpha, m.ap32[i], m.r32[i]);
   if (own) {
     m.y32[i] = y;
     m.r32[i] = r;
@@ -906,7 +909,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_a(MxArgs m, const float *pold
   const CgArgs &a = m.g;
   if (a.st->done) return;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const double beta = a.st->beta;
+  const float beta = (float)a.st->beta;
   double v[1] = {0.0};
   #pragma unroll
   for (int u = 0; u < MX_CPT; ++u) {
@@ -925,7 +928,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_mx_b(MxArgs m, const float *pnew
   const CgArgs &a = m.g;
   if (a.st->done) return;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const double alpha = a.st->alpha;
+  const float alpha = (float)a.st->alpha;
   double v[1] = {0.0};
   #pragma unroll
   for (int u = 0; u < MX_CPT; ++u) {
@@ -1034,7 +1037,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_smx_a(SMxArgs x, const float *po
   const int4 t = x.g.tiles[b];
   CgState *st = x.g.segst + t.z;
   if (st->done) return;
-  const double beta = st->beta;
+  const float beta = (float)st->beta;
   double v[1] = {0.0};
   tile_cells<CPT>(t, [&](int i) { mx_a_cell(x.m, i, true, beta, pold, pnew, v); });
   double s[1];
@@ -1048,7 +1051,7 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_smx_b(SMxArgs x, const float *pn
   const int4 t = x.g.tiles[b];
   CgState *st = x.g.segst + t.z;
   if (st->done) return;
-  const double alpha = st->alpha;
+  const float alpha = (float)st->alpha;
   double v[1] = {0.0};
   tile_cells<CPT>(t, [&](int i) { mx_b_cell(x.m, i, true, alpha, pnew, v); });
   double s[1];
