@@ -99,8 +99,14 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_explicit(
   HA(P1)[i] = P[i];
   for (int k = 0; k < KC; ++k) h1pk[i + k * pitch] = hpk[i + k * pitch];
   const double C1 = 0.5 * G;
+  // FP1 reads UHDYE(LE), VHDXE(LN) (:284-289): the step-start values, in the
+  // "1" arrays at wet cells; a land neighbour's entry is UHDYE's own ghost
+  // (the shift copies wet cells only), which the current array holds
+  const bool ge = e < 1 || e > nwet, gn = n < 1 || n > nwet;
+  const double ue = ge ? HA(UHDYE)[e] : HA(UHDY1E)[e];
+  const double vn = gn ? HA(VHDXE)[n] : HA(VHDX1E)[n];
   HA(FP1)[i] = DELTI * HA(DXYP)[i] * P[i] -
-               C1 * (HA(UHDY1E)[e] - HA(UHDY1E)[i] + HA(VHDX1E)[n] - HA(VHDX1E)[i]);
+               C1 * (ue - HA(UHDY1E)[i] + vn - HA(VHDX1E)[i]);
 }
 
 // ---------------------------------------------------------------------
@@ -124,10 +130,15 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_assemble(
                                    SVB[n] * HA(FVHDXE)[n] - SVB[i] * HA(FVHDXE)[i] -
                                    2.0 * HA(QSUME)[i]);
     const double C2 = -0.5 * DELTD2 * G;
+    // HU(LE), HV(LN) (:329-342): the step-start values in the "1" arrays at
+    // wet cells, the current arrays' own ghost entries at a land neighbour
+    const bool ge = e < 1 || e > nwet, gn = n < 1 || n > nwet;
+    const double hue = ge ? HA(HU)[e] : HA(H1U)[e];
+    const double hvn = gn ? HA(HV)[n] : HA(H1V)[n];
     const double cs = C2 * SVB[i] * HA(HRVO)[i] * HA(RCY)[i] * HA(H1V)[i];
     const double cw = C2 * SUB[i] * HA(HRUO)[i] * HA(RCX)[i] * HA(H1U)[i];
-    const double ce = C2 * SUB[e] * HA(HRUO)[e] * HA(RCX)[e] * HA(H1U)[e];
-    const double cn = C2 * SVB[n] * HA(HRVO)[n] * HA(RCY)[n] * HA(H1V)[n];
+    const double ce = C2 * SUB[e] * HA(HRUO)[e] * HA(RCX)[e] * hue;
+    const double cn = C2 * SVB[n] * HA(HRVO)[n] * HA(RCY)[n] * hvn;
     HA(CS)[i] = cs;
     HA(CW)[i] = cw;
     HA(CE)[i] = ce;

@@ -453,3 +453,47 @@ def test_config0_720_steps(oracle_mod, mode):
     assert max(rels) <= 1e-5
     if mode == "fp64":
         assert d <= 1e-9 and max(rels) <= 1e-9
+
+
+ROTATED = ["UHDYE", "UHDY1E", "VHDXE", "VHDX1E", "HU", "H1U", "HV", "H1V", "HUI",
+           "H1UI", "HVI", "H1VI", "HP", "H1P", "H2P"]
+
+
+def test_ghost_entries_of_rotated_levels(oracle_mod):
+    """The time levels shifted by CALPUV2C (EFDC/calpuv2c.f90:256-265) only
+    shift the wet cells, so each name keeps the ghost entries (L=1 and L=LC)
+    it had.  With distinct nonzero ghost entries on all fifteen names (read
+    by the explicit stage, the assembly and the depth pass at cells next to
+    land), the assembly of every step is the oracle's bit for bit, P, HP and
+    the transports meet the parity bars, and every name still holds its own
+    ghost entries after each step."""
+    case = _basin(seed=23)
+    lc0 = case.grid.LC - 1
+    for j, name in enumerate(ROTATED):
+        case.h[name][0] = 0.25 + 0.01 * j
+        case.h[name][lc0] = 0.5 + 0.02 * j
+    ref = case.copy()
+    o = oracle_mod.Oracle(ref, rsqm=TIGHT, iterm=2000)
+    ctx = _ctx(case)
+    i = wet_index(case)
+    g = np.array([0, lc0])
+    for n in range(1, 5):
+        o.closure_pre()
+        ctx.closure_pre()
+        o.calpuv2c(n=n)
+        ctx.calpuv2c(case.dt, n)
+        for name in ASSEMBLY:
+            np.testing.assert_array_equal(ctx.get(name)[i], ref.h[name][i], err_msg=f"{n} {name}")
+        assert eta_err(ctx.get("P")[i], ref.h["P"][i]) <= 1e-9, n
+        for name in ["HP", "UHDYE", "VHDXE", "HU", "HV"]:
+            a, b = ctx.get(name)[i], ref.h[name][i]
+            assert np.max(np.abs(a - b)) <= 1e-9 * max(1.0, np.max(np.abs(b))), (n, name)
+        for name in ROTATED:
+            np.testing.assert_array_equal(ctx.get(name)[g], ref.h[name][g], err_msg=f"{n} {name}")
+        # keep the two runs on identical wet-cell inputs for the next step
+        for name in ["P", "HP", "UHDYE", "VHDXE", "HU", "HV", "HUI", "HVI",
+                     "HPI", "UHE", "VHE", "UHDY1E", "VHDX1E", "H1P"]:
+            ctx.set(name, ref.h[name])
+        o.closure_post()
+        ctx.closure_post()
+    ctx.close()
