@@ -58,15 +58,66 @@ __device__ __forceinline__ int cell_index(int nwet, int &tile) {
 }
 int puv_blocks(int nwet) { return efdc_grid_blocks(nwet + 1); }
 
+// Buffers of the rotated time levels: pair[q] = {X, X1} after the swap,
+// tri = {HP, H1P, H2P} after the rotation.
+struct Rotated {
+  double *pair[6][2];
+  double *tri[3];
+  int lc0;
+};
+
+// Part t (0..13) of giving each rotated name back the ghost entries (L=1
+// and L=LC) it had before the rotation -- the reference's shifts only copy
+// the wet cells: pair q = t/2 (t < 12) or the triple (t = 12, 13), at ghost
+// L=1 (t even) or L=LC (t odd).
+__device__ __forceinline__ void rot_ghost(const Rotated &r, int t) {
+  const int g = (t & 1) ? r.lc0 : 0, q = t >> 1;
+  if (q < 6) {
+    const double a = r.pair[q][0][g], b = r.pair[q][1][g];
+    r.pair[q][0][g] = b;
+    r.pair[q][1][g] = a;
+  } else {
+    const double hp = r.tri[1][g], h1p = r.tri[2][g], h2p = r.tri[0][g];
+    r.tri[0][g] = hp;
+    r.tri[1][g] = h1p;
+    r.tri[2][g] = h2p;
+  }
+}
+
+// Start of a CALPUV2C step in its own launch (steps with drying or adjoint
+// recording; the others do this work in the explicit and assembly passes):
+// reset the flags and restore the ghost entries, one thread per part so
+// that the 14 load/store round trips run side by side.
+__global__ void k_step_begin(CgState *st, Rotated r) {
+  const int t = threadIdx.x;
+  if (t == 0) {
+    st->symok = 0;
+    st->negdep = 0;
+  }
+  if (t < 14) rot_ghost(r, t);
+}
+
 // ---------------------------------------------------------------------
 // Explicit stage and time-level shifts: EFDC/calpuv2c.f90:214-220,256-290
 // ---------------------------------------------------------------------
+// fold = 1: this pass also does the step start of k_step_begin for the
+// names it does not read at a ghost (the flags, the HU, HV, HUI, HVI pairs
+// and the HP triple); the assembly then does the UHDYE and VHDXE pairs, so
+// here their ghost entries still sit where the rotation left them.
 template <typename T>
 __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_explicit(
     HPtr h, const int *__restrict__ lw, const int *__restrict__ ls,
     const int *__restrict__ le, const int *__restrict__ ln, const T *hpk,
     T *h1pk, int64_t pitch, int KC, int nwet, double dt, double G,
-    double RITB1, double SNLT, int dry, int drag) {
+    double RITB1, double SNLT, int dry, int drag, Rotated rot, int fold, CgState *st) {
+  if (fold && blockIdx.x == 0) {
+    const int t = threadIdx.x;
+    if (t == 0) {
+      st->symok = 0;
+      st->negdep = 0;
+    }
+    if (t >= 4 && t < 14) rot_ghost(rot, t);
+  }
   int tile;
   const int i = cell_index(nwet, tile);
   if (i > nwet) return;
@@ -101,8 +152,9 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_explicit(
   const double C1 = 0.5 * G;
   // FP1 reads UHDYE(LE), VHDXE(LN) (:284-289): the step-start values, in the
   // "1" arrays at wet cells; a land neighbour's entry is UHDYE's own ghost
-  // (the shift copies wet cells only), which the current array holds
-  const bool ge = e < 1 || e > nwet, gn = n < 1 || n > nwet;
+  // (the shift copies wet cells only), which the current array holds once
+  // the ghosts are restored -- and the "1" array while they are not (fold)
+  const bool ge = !fold && (e < 1 || e > nwet), gn = !fold && (n < 1 || n > nwet);
   const double ue = ge ? HA(UHDYE)[e] : HA(UHDY1E)[e];
   const double vn = gn ? HA(VHDXE)[n] : HA(VHDX1E)[n];
   HA(FP1)[i] = DELTI * HA(DXYP)[i] * P[i] -
@@ -114,10 +166,13 @@ __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_explicit(
 // :360-376 over the cells that are not pressure open boundaries (those
 // get their final CC in k_puv_bc_side; k_puv_bc_min completes the minimum).
 // ---------------------------------------------------------------------
+// fold = 1: the UHDYE and VHDXE pairs' ghost entries are restored here (the
+// explicit pass did the other names; this pass reads neither pair).
 __global__ __launch_bounds__(EFDC_BLOCK) void k_puv_assemble(
     HPtr h, const int *__restrict__ le, const int *__restrict__ ln,
     const unsigned char *__restrict__ pbcflag, int nwet, double dt, double G,
-    double *part, CgState *st, unsigned *st_tk) {
+    double *part, CgState *st, unsigned *st_tk, Rotated rot, int fold) {
+  if (fold && blockIdx.x == 0 && threadIdx.x < 4) rot_ghost(rot, threadIdx.x);
   int tile;
   const int i = cell_index(nwet, tile);
   double ccv = 1.E+18;
@@ -410,38 +465,6 @@ __global__ void k_reset_flags(CgState *st) {
   st->negdep = 0;
 }
 
-// Buffers of the rotated time levels: pair[q] = {X, X1} after the swap,
-// tri = {HP, H1P, H2P} after the rotation.
-struct Rotated {
-  double *pair[6][2];
-  double *tri[3];
-  int lc0;
-};
-
-// Start of a CALPUV2C step: reset the flags, and give each rotated name
-// back the ghost entries (L=1 and L=LC) it had before the rotation -- the
-// reference's shifts only copy the wet cells.
-// One thread per swap (6 pairs and the triple, at each of the two ghost
-// cells), so that the 14 load/store round trips run side by side.
-__global__ void k_step_begin(CgState *st, Rotated r) {
-  const int t = threadIdx.x;
-  if (t == 0) {
-    st->symok = 0;
-    st->negdep = 0;
-  }
-  if (t >= 14) return;
-  const int g = (t & 1) ? r.lc0 : 0, q = t >> 1;
-  if (q < 6) {
-    const double a = r.pair[q][0][g], b = r.pair[q][1][g];
-    r.pair[q][0][g] = b;
-    r.pair[q][1][g] = a;
-  } else {
-    const double hp = r.tri[1][g], h1p = r.tri[2][g], h2p = r.tri[0][g];
-    r.tri[0][g] = hp;
-    r.tri[1][g] = h1p;
-    r.tri[2][g] = h2p;
-  }
-}
 
 // ---------------------------------------------------------------------
 // CONGRAD, FP64 (EFDC/congrad.f90)
@@ -1825,7 +1848,10 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
     }
   }
   uvw_step_begin(c);
-  k_step_begin<<<1, 64, 0, c->stream>>>(c->st, rot);
+  // without drying or adjoint recording the explicit and assembly passes
+  // do the step start themselves
+  const int fold = (!dry && !adj_recording(c)) ? 1 : 0;
+  if (!fold) k_step_begin<<<1, 64, 0, c->stream>>>(c->st, rot);
   if (dry) {
     k_dry_begin<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->iscdry, c->sub1, c->svb1,
                                                   c->prm.NDRYSTP < 0 ? c->qdwaste : nullptr,
@@ -1839,12 +1865,14 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
     TIMED(c, TK_PUV_EXPLICIT, k_puv_explicit<double><<<nb, EFDC_BLOCK, 0, c->stream>>>(
         h, c->dLW, c->dLS, c->dLE, c->dLN, (const double *)c->dK[EFDC_HPK - EFDC_HPK],
         (double *)c->dK[EFDC_H1PK - EFDC_HPK], c->pitch, c->KC, nwet, dt, G,
-        c->prm.RITB1, c->prm.SNLT, dry ? 1 : 0, c->prm.implicit_drag ? 1 : 0));
+        c->prm.RITB1, c->prm.SNLT, dry ? 1 : 0, c->prm.implicit_drag ? 1 : 0, rot, fold,
+        c->st));
   else
     TIMED(c, TK_PUV_EXPLICIT, k_puv_explicit<float><<<nb, EFDC_BLOCK, 0, c->stream>>>(
         h, c->dLW, c->dLS, c->dLE, c->dLN, (const float *)c->dK[EFDC_HPK - EFDC_HPK],
         (float *)c->dK[EFDC_H1PK - EFDC_HPK], c->pitch, c->KC, nwet, dt, G,
-        c->prm.RITB1, c->prm.SNLT, dry ? 1 : 0, c->prm.implicit_drag ? 1 : 0));
+        c->prm.RITB1, c->prm.SNLT, dry ? 1 : 0, c->prm.implicit_drag ? 1 : 0, rot, fold,
+        c->st));
   c->ncordry = 0;
   // a step that may solve more than once (drying) or records the solution
   // (adjoint) launches the persistent solve cooperatively, so that it never
@@ -1854,9 +1882,11 @@ int puv_run(efdc_ctx *c, const efdc_step_t *step, efdc_cg_stats_t *stats) {
   int cgs = EFDC_OK;
   for (;;) {
     // label 1000 of EFDC/calpuv2c.f90:302: the drying loop re-enters here
+    // (fold is 0 on the re-solves of the drying loop: only steps without
+    // drying fold, and they assemble once)
     TIMED(c, TK_PUV_ASSEMBLE, k_puv_assemble<<<nb, EFDC_BLOCK, 0, c->stream>>>(h, c->dLE, c->dLN, c->pbcflag,
                                                      nwet, dt, G, c->part, c->st,
-                                                     c->tickets));
+                                                     c->tickets, rot, fold));
     if (c->npbc_total > 0) {
       // tidal phase on the host in FP64 like SETOPENBC (:22-32)
       const double PI2 = 2. * 3.1415926535898;  // EFDC/aaefdc.f90:650-651

@@ -459,22 +459,25 @@ ROTATED = ["UHDYE", "UHDY1E", "VHDXE", "VHDX1E", "HU", "H1U", "HV", "H1V", "HUI"
            "H1UI", "HVI", "H1VI", "HP", "H1P", "H2P"]
 
 
-def test_ghost_entries_of_rotated_levels(oracle_mod):
+@pytest.mark.parametrize("isdry", [0, 1])
+def test_ghost_entries_of_rotated_levels(oracle_mod, isdry):
     """The time levels shifted by CALPUV2C (EFDC/calpuv2c.f90:256-265) only
     shift the wet cells, so each name keeps the ghost entries (L=1 and L=LC)
     it had.  With distinct nonzero ghost entries on all fifteen names (read
     by the explicit stage, the assembly and the depth pass at cells next to
     land), the assembly of every step is the oracle's bit for bit, P, HP and
     the transports meet the parity bars, and every name still holds its own
-    ghost entries after each step."""
+    ghost entries after each step.  Without drying the explicit and assembly
+    passes restore the ghost entries themselves; with ISDRY = 1 (on a basin
+    where nothing comes near HDRY) a launch of its own does."""
     case = _basin(seed=23)
     lc0 = case.grid.LC - 1
     for j, name in enumerate(ROTATED):
         case.h[name][0] = 0.25 + 0.01 * j
         case.h[name][lc0] = 0.5 + 0.02 * j
     ref = case.copy()
-    o = oracle_mod.Oracle(ref, rsqm=TIGHT, iterm=2000)
-    ctx = _ctx(case)
+    o = oracle_mod.Oracle(ref, rsqm=TIGHT, iterm=2000, isdry=isdry, hdry=0.01)
+    ctx = _ctx(case, ISDRY=isdry, HDRY=0.01)
     i = wet_index(case)
     g = np.array([0, lc0])
     for n in range(1, 5):
