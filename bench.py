@@ -1090,6 +1090,7 @@ def lake_bench(args):
     value = nwet * args.steps / t_m
     copy_gbs = stream_copy_gbs(ctx_m)
     tune = ctx_m.tune_report()
+    launch_m = ctx_m.cg_launch()
     ctx_m.close()
     ctx_d.close()
     out = {
@@ -1132,6 +1133,9 @@ def lake_bench(args):
         "cpu_baseline": cpu,
         "detail": {
             "autotune": tune,
+            # persistent solves: 0 = plain launch of the co-resident grid;
+            # solves repeated cooperatively because a wait gave up
+            "cg_launch": {"mode": launch_m[0], "repeated_solves": launch_m[1]},
             "stream_copy_gbs": copy_gbs,
             "solve_frac_of_copy": rf_m["achieved"] / copy_gbs,
             "inner_iters_per_solve_mixed": rf_m["iters_per_solve"],
